@@ -1,0 +1,209 @@
+// Internal declarations shared by the HIP translation units of
+// libatomistica_amd: handle layouts, device helpers and error plumbing.
+// Nothing here is part of the C-ABI (see include/atomistica_amd.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/atomistica_amd.h"
+
+namespace agpu {
+
+// ---------------------------------------------------------------------------
+// errors: the last message per host thread, returned by agpu_get_error()
+// ---------------------------------------------------------------------------
+int set_error(int code, const std::string &msg);
+int hip_fail(hipError_t e, const char *what, const char *file, int line);
+
+#define AGPU_HIP(call)                                                       \
+    do {                                                                     \
+        hipError_t e_ = (call);                                              \
+        if (e_ != hipSuccess) return agpu::hip_fail(e_, #call, __FILE__, __LINE__); \
+    } while (0)
+
+#define AGPU_CHECK(expr)                                                     \
+    do {                                                                     \
+        int rc_ = (expr);                                                    \
+        if (rc_ != AGPU_OK) return rc_;                                      \
+    } while (0)
+
+// device-side error bits, OR-ed into a per-call flag word
+enum DevErr : int {
+    DERR_NL_OVERFLOW = 1,
+    DERR_IMAGE_RANGE = 2,
+    DERR_SPLINE_RANGE = 4,
+    DERR_BOND_OVERFLOW = 8,
+};
+
+hipStream_t stream();
+
+// Grow-only device buffer owned by a handle.
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    int reserve(size_t nbytes);
+    void release();
+    template <typename T> T *as() const { return static_cast<T *>(ptr); }
+    ~DevBuf() { release(); }
+};
+
+// Pinned host staging word for flags and small statistics.
+struct HostStats {
+    int *h = nullptr;
+    int init();
+    ~HostStats();
+};
+
+// ---------------------------------------------------------------------------
+// handles
+// ---------------------------------------------------------------------------
+struct Cell {
+    double A[9];     // A[3*k + x] = component x of lattice vector k
+    double B[9];     // B[3*k + x] = Bbox(x, k): column-major image of A^-1
+    int pbc[3];
+};
+
+}  // namespace agpu
+
+struct agpu_particles {
+    int nat = 0;
+    int nalloc = 0;
+    agpu::Cell cell{};
+    int cell_set = 0;
+    double *d_r = nullptr;        // (nat, 3) device positions
+    int *d_Z = nullptr;           // (nat) device atomic numbers
+    bool own_r = true, own_Z = true;
+    agpu::DevBuf r_buf, Z_buf;
+    uint64_t pos_rev = 0;         // bumped on every position change
+    uint64_t cell_rev = 0;
+    uint64_t Z_rev = 0;
+    std::vector<int> h_Z;         // host copy of atomic numbers
+};
+
+struct agpu_neighbors {
+    int avgn = 100;
+    double interaction_range = 0.0;
+    double cutoff = 0.0;
+    double bin_size = 0.0;        // <= 0: equal to the cutoff
+    // geometry of the last build
+    int nc[3] = {0, 0, 0};
+    int nd[3] = {0, 0, 0};
+    double rec[9];                // rec_cell_size, column-major image
+    // provenance of the last build
+    const agpu_particles *p = nullptr;
+    uint64_t pos_rev = ~0ull, cell_rev = ~0ull;
+    int nat = 0;
+    int64_t cap = 0;              // nat * avgn entries
+    int64_t total = 0;            // neighbours in the list
+    int maxrow = 0;
+    // device storage
+    agpu::DevBuf key, key_sorted, idx, idx_sorted, cell_start, atoms_sorted,
+        counts, offsets, nbr, sort_tmp, rowmax, rank;
+    agpu::HostStats stats;
+    int64_t n_updates = 0;
+};
+
+namespace agpu {
+
+int nl_update(agpu_neighbors *nl, agpu_particles *p, int force);
+int nl_export(agpu_neighbors *nl, int64_t *offsets, int *nbr, int *dc, int on_device);
+int compute_inverse(const double *A, double *B);
+
+// ---- device helpers ----
+struct CellArg {
+    double A[9];
+};
+
+__device__ __forceinline__ int dc_pack_ok(int a) { return a >= -512 && a <= 511; }
+__device__ __forceinline__ int dc_pack(int a, int b, int c)
+{
+    return ((a + 512) << 20) | ((b + 512) << 10) | (c + 512);
+}
+__device__ __forceinline__ void dc_unpack(int code, int &a, int &b, int &c)
+{
+    a = ((code >> 20) & 1023) - 512;
+    b = ((code >> 10) & 1023) - 512;
+    c = (code & 1023) - 512;
+}
+
+// A.s summed k = 0..2 in the order the reference's matmul writes it
+__device__ __forceinline__ double image_x(const CellArg &C, int x, int s0, int s1, int s2)
+{
+#pragma clang fp contract(off)
+    double as = 0.0;
+    as = as + C.A[0 + x] * (double)s0;
+    as = as + C.A[3 + x] * (double)s1;
+    as = as + C.A[6 + x] * (double)s2;
+    return as;
+}
+
+// dr = r_i - r_j + A.dc (delta_r of python_neighbors.f90:708, GET_DRJ of
+// macros.inc:76), non-contracted
+__device__ __forceinline__ void delta_r(const CellArg &C, const double *ri, const double *rj,
+                                        int s0, int s1, int s2, double *d)
+{
+#pragma clang fp contract(off)
+    for (int x = 0; x < 3; ++x) d[x] = (ri[x] - rj[x]) + image_x(C, x, s0, s1, s2);
+}
+
+__device__ __forceinline__ double dot3_nc(const double *d)
+{
+#pragma clang fp contract(off)
+    double s = 0.0;
+    s = s + d[0] * d[0];
+    s = s + d[1] * d[1];
+    s = s + d[2] * d[2];
+    return s;
+}
+
+// warp-level (wave64) helpers
+template <typename T> __device__ __forceinline__ T shfl_xor(T v, int m)
+{
+    return __shfl_xor(v, m, 64);
+}
+
+template <int G, typename T> __device__ __forceinline__ T group_sum(T v)
+{
+#pragma unroll
+    for (int m = G / 2; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+}  // namespace agpu
+
+// potential handle (one struct for every class; kind selects the kernels)
+struct agpu_potential {
+    int kind = 0;                 // AGPU_KIND_*
+    int precision = AGPU_PREC_FP64;
+    int bound = 0;
+    double cutoff = 0.0;          // interaction range requested at bind_to
+    agpu_particles *p = nullptr;
+    agpu_neighbors *nl = nullptr;
+    void *impl = nullptr;         // class-specific state
+    void (*free_impl)(void *) = nullptr;
+    agpu::DevBuf d_el, flags, partial, partial_out;
+    agpu::HostStats stats;
+};
+
+namespace agpu {
+// class hooks
+int tersoff_bind(agpu_potential *pot);
+int tersoff_compute(agpu_potential *pot, const agpu_eval_args *a);
+int eam_bind(agpu_potential *pot);
+int eam_compute(agpu_potential *pot, const agpu_eval_args *a);
+int ewald_bind(agpu_potential *pot);
+int ewald_compute(agpu_potential *pot, const agpu_eval_args *a);
+int ewald_potential(agpu_potential *pot, const double *q, double *phi, int flags);
+
+// Deterministic fixed-order reduction of nblk per-block records of width w
+// (double): column 0 -> *epot (+=), columns 1..9 -> wpot[0..8] (+=),
+// columns 10.. -> extra[0..] (+=).  Runs on the library stream.
+int reduce_ev(const double *partials, int nblk, int w, double *epot, double *wpot, double *extra);
+
+// map atomic numbers to database element indices (-1 = not handled)
+int map_elements(agpu_potential *pot, const int *z2db, int nz);
+}  // namespace agpu

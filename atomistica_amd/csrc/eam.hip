@@ -1,0 +1,601 @@
+// Tabulated alloy EAM (setfl) on gfx950.
+//
+// Semantics: tabulated_alloy_eam.f90 -- reader :147-259, bind_to :297-350,
+// kernel :423-627 -- with the natural cubic spline tables of
+// simple_spline.f90:127-615.  Work decomposition (this library's own):
+//
+//   k_eam_density  one wavefront per atom, lanes stride over its neighbour
+//                  row: rho_i = sum rho_{el j}(r), wave reduction, rho clamped
+//                  at 0, then F(rho_i) and F'(rho_i) (extrapolating spline).
+//   k_eam_force    one wavefront per atom again, owner-computes:
+//                  f_i = -sum_j [m_i (F'_i rho'_{el j}(r) + p(r))
+//                               + m_j (F'_j rho'_{el i}(r) + p(r))] dr/r
+//                  with p = (phi' - phi/r)/r of the half-scaled r*phi table;
+//                  the reference's scatter to j becomes the second term, so no
+//                  atomics are needed and every sum has a fixed order.
+// Spline records are 8-wide AoS per interval {y, c1, c2, c3, dc1, dc2, dc3, 0}
+// so one f-and-derivative evaluation reads one 64-byte (FP64) or 32-byte
+// (FP32) record.
+#include <cctype>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "agpu_internal.hpp"
+#include "profile.hpp"
+
+namespace agpu {
+
+static const char *kElementNames[] = {
+    "xx", "H",  "He", "Li", "Be", "B",  "C",  "N",  "O",  "F",  "Ne", "Na", "Mg", "Al", "Si", "P",
+    "S",  "Cl", "Ar", "K",  "Ca", "Sc", "Ti", "V",  "Cr", "Mn", "Fe", "Co", "Ni", "Cu", "Zn", "Ga",
+    "Ge", "As", "Se", "Br", "Kr", "Rb", "Sr", "Y",  "Zr", "Nb", "Mo", "Tc", "Ru", "Rh", "Pd", "Ag",
+    "Cd", "In", "Sn", "Sb", "Te", "I",  "Xe", "Cs", "Ba", "La", "Ce", "Pr", "Nd", "Pm", "Sm", "Eu",
+    "Gd", "Tb", "Dy", "Ho", "Er", "Tm", "Yb", "Lu", "Hf", "Ta", "W",  "Re", "Os", "Ir", "Pt", "Au",
+    "Hg", "Tl", "Pb", "Bi", "Po", "At", "Rn", "Fr", "Ra", "Ac", "Th", "Pa", "U",  "Np", "Pu", "Am",
+    "Cm", "Bk", "Cf", "Es", "Fm", "Md", "No", "Lr", "Rf", "Db", "Sg", "Bh", "Hs", "Mt", "Ds", "Rg",
+    "Cn", "Nh", "Fl", "Mc", "Lv", "Ts", "Og"};
+static const int kNumElements = (int)(sizeof(kElementNames) / sizeof(kElementNames[0]));
+
+static int z_of_symbol(const std::string &s)
+{
+    for (int z = 1; z < kNumElements; ++z)
+        if (s == kElementNames[z]) return z;
+    return -1;
+}
+
+// One natural cubic spline on a uniform grid from x0 = 0 (simple_spline_init).
+struct HostSpline {
+    int n = 0;
+    double dx = 0.0;
+    std::vector<double> rec;   // 8 doubles per interval
+};
+
+static void spline_build(HostSpline &s, const std::vector<double> &y, double dx, double scale)
+{
+    const int n = (int)y.size();
+    const double sig = 0.5;
+    std::vector<double> d2y(n), u(n);
+    d2y[0] = 0.0;
+    u[0] = 0.0;
+    for (int i = 1; i < n - 1; ++i) {
+        double p = sig * d2y[i - 1] + 2;
+        d2y[i] = (sig - 1) / p;
+        u[i] = (6.0 * ((y[i + 1] - y[i]) / dx - (y[i] - y[i - 1]) / dx) / (2 * dx) - sig * u[i - 1]) / p;
+    }
+    d2y[n - 1] = 0.0;   // natural end: (un - qn*u)/(qn*d2y + 1) with qn = un = 0
+    for (int k = n - 2; k >= 0; --k) d2y[k] = d2y[k] * d2y[k + 1] + u[k];
+    const double dx2 = dx * dx;
+    s.n = n;
+    s.dx = dx;
+    s.rec.assign(8 * (size_t)(n - 1), 0.0);
+    for (int k = 0; k < n - 1; ++k) {
+        double c1 = y[k + 1] - y[k] - (2 * d2y[k] + d2y[k + 1]) * dx2 / 6;
+        double c2 = d2y[k] * dx2 / 2;
+        double c3 = (d2y[k + 1] - d2y[k]) * dx2 / 6;
+        double *r = &s.rec[8 * (size_t)k];
+        // scale_y_axis multiplies every coefficient (simple_spline.f90:756-773)
+        r[0] = scale * y[k];
+        r[1] = scale * c1;
+        r[2] = scale * c2;
+        r[3] = scale * c3;
+        r[4] = scale * (c1 / dx);
+        r[5] = scale * (2 * c2 / dx);
+        r[6] = scale * (3 * c3 / dx);
+        r[7] = 0.0;
+    }
+}
+
+struct SplHdr {
+    int n;        // grid points
+    int off;      // first record
+    double rdx;   // 1/dx
+    double cut;   // x0 + dx*(n-1)
+    double dx;
+};
+
+constexpr int EAM_MAX_EL = 8;
+
+struct EamDev {
+    int nel;
+    int tF[EAM_MAX_EL], tR[EAM_MAX_EL], tP[EAM_MAX_EL * EAM_MAX_EL];
+    double cutoff_sq;
+};
+
+struct EamState {
+    std::string fn, elements;
+    int nel = 0;
+    std::vector<std::string> names;
+    std::vector<int> Zs;
+    double cutoff = 0.0;
+    std::vector<HostSpline> F, rho, phi;   // phi: nel*nel (symmetric)
+    std::vector<SplHdr> hdr;
+    EamDev dev{};
+    int z2db[128];
+    uint64_t Z_rev = ~0ull;
+    DevBuf d_hdr, d_rec64, d_rec32, dF, partial, stats;
+};
+
+static void eam_free(void *p) { delete static_cast<EamState *>(p); }
+
+// list-directed number stream over the whole file (Fortran read(f, *))
+struct Tok {
+    std::vector<std::string> lines;
+    size_t line = 0;
+    std::vector<std::string> pending;
+    bool next_line(std::string &out)
+    {
+        if (line >= lines.size()) return false;
+        out = lines[line++];
+        pending.clear();
+        return true;
+    }
+    bool number(double &v)
+    {
+        while (pending.empty()) {
+            if (line >= lines.size()) return false;
+            std::string l = lines[line++];
+            for (char &c : l)
+                if (c == ',') c = ' ';
+            std::istringstream is(l);
+            std::string t;
+            std::vector<std::string> toks;
+            while (is >> t) toks.push_back(t);
+            pending.assign(toks.rbegin(), toks.rend());
+        }
+        std::string t = pending.back();
+        pending.pop_back();
+        for (char &c : t)
+            if (c == 'd' || c == 'D') c = 'e';
+        char *end = nullptr;
+        v = std::strtod(t.c_str(), &end);
+        return end && *end == '\0';
+    }
+};
+
+static int read_setfl(EamState &st)
+{
+    std::ifstream in(st.fn);
+    if (!in) return set_error(AGPU_ERR_IO, "TabulatedAlloyEAM: cannot open '" + st.fn + "'");
+    Tok tk;
+    std::string l;
+    while (std::getline(in, l)) tk.lines.push_back(l);
+    if (tk.lines.size() < 6) return set_error(AGPU_ERR_IO, "TabulatedAlloyEAM: file too short");
+    std::string dummy;
+    for (int k = 0; k < 3; ++k) tk.next_line(dummy);
+    tk.next_line(l);
+    {
+        std::istringstream is(l);
+        is >> st.nel;
+        if (!is || st.nel < 1 || st.nel > EAM_MAX_EL)
+            return set_error(AGPU_ERR_IO, "TabulatedAlloyEAM: bad element count line");
+        st.names.resize(st.nel);
+        for (int e = 0; e < st.nel; ++e) is >> st.names[e];
+    }
+    tk.next_line(l);
+    int nF = 0, nr = 0;
+    double dF = 0, dr = 0, cutoff = 0;
+    {
+        for (char &c : l)
+            if (c == ',') c = ' ';
+        std::istringstream is(l);
+        is >> nF >> dF >> nr >> dr >> cutoff;
+        if (!is || nF < 3 || nr < 3) return set_error(AGPU_ERR_IO, "TabulatedAlloyEAM: bad grid line");
+    }
+    st.cutoff = cutoff;
+    st.Zs.resize(st.nel);
+    st.F.resize(st.nel);
+    st.rho.resize(st.nel);
+    st.phi.resize((size_t)st.nel * st.nel);
+    for (int e = 0; e < st.nel; ++e) {
+        tk.next_line(l);
+        std::istringstream is(l);
+        int Z = 0;
+        is >> Z;
+        if (Z <= 0 || Z >= kNumElements || st.names[e] != kElementNames[Z])
+            return set_error(AGPU_ERR_IO, "TabulatedAlloyEAM: atomic number " + std::to_string(Z) +
+                                              " does not match element name " + st.names[e]);
+        st.Zs[e] = Z;
+        std::vector<double> y(nF);
+        for (int k = 0; k < nF; ++k)
+            if (!tk.number(y[k])) return set_error(AGPU_ERR_IO, "TabulatedAlloyEAM: truncated F table");
+        tk.pending.clear();   // one list-directed read per table: rest of its last line is skipped
+        spline_build(st.F[e], y, dF, 1.0);
+        std::vector<double> yr(nr + 2, 0.0);   // padded with two zeros (:222)
+        for (int k = 0; k < nr; ++k)
+            if (!tk.number(yr[k])) return set_error(AGPU_ERR_IO, "TabulatedAlloyEAM: truncated rho table");
+        spline_build(st.rho[e], yr, dr, 1.0);
+        tk.pending.clear();
+    }
+    for (int a = 0; a < st.nel; ++a)
+        for (int b = 0; b <= a; ++b) {
+            std::vector<double> yp(nr + 2, 0.0);
+            for (int k = 0; k < nr; ++k)
+                if (!tk.number(yp[k])) return set_error(AGPU_ERR_IO, "TabulatedAlloyEAM: truncated phi table");
+            tk.pending.clear();
+            spline_build(st.phi[a * st.nel + b], yp, dr, 0.5);
+            st.phi[b * st.nel + a] = st.phi[a * st.nel + b];
+        }
+    return AGPU_OK;
+}
+
+static int upload_tables(EamState &st)
+{
+    std::vector<double> rec64;
+    st.hdr.clear();
+    auto add = [&](const HostSpline &s) {
+        SplHdr h;
+        h.n = s.n;
+        h.off = (int)(rec64.size() / 8);
+        h.dx = s.dx;
+        h.rdx = 1.0 / s.dx;
+        h.cut = s.dx * (s.n - 1);
+        rec64.insert(rec64.end(), s.rec.begin(), s.rec.end());
+        st.hdr.push_back(h);
+        return (int)st.hdr.size() - 1;
+    };
+    for (int e = 0; e < st.nel; ++e) st.dev.tF[e] = add(st.F[e]);
+    for (int e = 0; e < st.nel; ++e) st.dev.tR[e] = add(st.rho[e]);
+    for (int a = 0; a < st.nel; ++a)
+        for (int b = 0; b < st.nel; ++b) st.dev.tP[a * EAM_MAX_EL + b] = add(st.phi[a * st.nel + b]);
+    st.dev.nel = st.nel;
+    st.dev.cutoff_sq = st.cutoff * st.cutoff;
+    std::vector<float> rec32(rec64.begin(), rec64.end());
+    AGPU_CHECK(st.d_hdr.reserve(sizeof(SplHdr) * st.hdr.size()));
+    AGPU_CHECK(st.d_rec64.reserve(sizeof(double) * rec64.size()));
+    AGPU_CHECK(st.d_rec32.reserve(sizeof(float) * rec32.size()));
+    hipStream_t s = stream();
+    AGPU_HIP(hipMemcpyAsync(st.d_hdr.ptr, st.hdr.data(), sizeof(SplHdr) * st.hdr.size(), hipMemcpyHostToDevice, s));
+    AGPU_HIP(hipMemcpyAsync(st.d_rec64.ptr, rec64.data(), sizeof(double) * rec64.size(), hipMemcpyHostToDevice, s));
+    AGPU_HIP(hipMemcpyAsync(st.d_rec32.ptr, rec32.data(), sizeof(float) * rec32.size(), hipMemcpyHostToDevice, s));
+    AGPU_HIP(hipStreamSynchronize(s));
+    return AGPU_OK;
+}
+
+// --------------------------------------------------------------------------
+// device spline evaluation
+// --------------------------------------------------------------------------
+template <typename R> struct Rec8 { R v[8]; };
+
+// simple_spline_f / dfunc: no extrapolation, error outside [0, cut]
+// interval index of xf, or 0 when outside [1, n) (NaN included)
+__device__ __forceinline__ int spl_index(const SplHdr &h, double x, double &xf)
+{
+    if (x == h.cut) {
+        xf = h.n;
+        return h.n - 1;
+    }
+    xf = x * h.rdx + 1.0;
+    double fl = floor(xf);
+    if (!(fl >= 1.0 && fl < (double)h.n)) return 0;
+    return (int)fl;
+}
+
+template <typename R>
+__device__ __forceinline__ R spl_f(const SplHdr &h, const R *__restrict__ rec, double x, int &err)
+{
+    double xf;
+    int i = spl_index(h, x, xf);
+    if (i < 1) {
+        err |= DERR_SPLINE_RANGE;
+        return (R)1;
+    }
+    R B = (R)(xf - i);
+    const R *c = rec + 8 * (size_t)(h.off + i - 1);
+    return c[0] + B * (c[1] + B * (c[2] + B * c[3]));
+}
+
+template <typename R>
+__device__ __forceinline__ R spl_df(const SplHdr &h, const R *__restrict__ rec, double x, int &err)
+{
+    double xf;
+    int i = spl_index(h, x, xf);
+    if (i < 1) {
+        err |= DERR_SPLINE_RANGE;
+        return (R)1;
+    }
+    R B = (R)(xf - i);
+    const R *c = rec + 8 * (size_t)(h.off + i - 1);
+    return c[4] + B * (c[5] + B * c[6]);
+}
+
+// f_and_df(extrapolate=.true.)
+template <typename R>
+__device__ __forceinline__ void spl_fdf_x(const SplHdr &h, const R *__restrict__ rec, double x, R &f, R &df)
+{
+    double xf = x * h.rdx + 1.0;
+    double fl = floor(xf);
+    // clamp to the first / last interval; NaN goes to the first
+    int i = !(fl >= 1.0) ? 1 : (!(fl < (double)h.n) ? h.n - 1 : (int)fl);
+    R B = (R)(xf - i);
+    const R *c = rec + 8 * (size_t)(h.off + i - 1);
+    f = c[0] + B * (c[1] + B * (c[2] + B * c[3]));
+    df = c[4] + B * (c[5] + B * c[6]);
+}
+
+template <typename T> __device__ __forceinline__ T wave_sum(T v)
+{
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256)
+k_eam_density(EamDev P, const SplHdr *__restrict__ hdr, const R *__restrict__ rec, int nat,
+              const int *__restrict__ order, const double *__restrict__ r, const int *__restrict__ el,
+              CellArg C, const int *__restrict__ offsets, const int2 *__restrict__ nbr,
+              const int *__restrict__ mask, double *__restrict__ dF, double *__restrict__ epa,
+              double *__restrict__ partial, int *__restrict__ flags)
+{
+    __shared__ double s_e[4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int ai = blockIdx.x * 4 + wave;
+    double Fi = 0.0;
+    int err = 0;
+    if (ai < nat) {
+        const int i = order[ai];
+        const int eli = el[i];
+        const bool on = eli >= 0 && !(mask && mask[i] == 0);
+        double rho = 0.0;
+        if (on) {
+            const double ri[3] = {r[3 * i], r[3 * i + 1], r[3 * i + 2]};
+            const int beg = offsets[i], end = offsets[i + 1];
+            for (int s = beg + lane; s < end; s += 64) {
+                int2 e = nbr[s];
+                int elj = el[e.x];
+                if (elj < 0) continue;
+                int a, b, c;
+                dc_unpack(e.y, a, b, c);
+                const double rj[3] = {r[3 * e.x], r[3 * e.x + 1], r[3 * e.x + 2]};
+                double d[3];
+                delta_r(C, ri, rj, a, b, c, d);
+                double d2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+                if (d2 < P.cutoff_sq) rho += (double)spl_f<R>(hdr[P.tR[elj]], rec, sqrt(d2), err);
+            }
+        }
+        rho = wave_sum(rho);
+        double dFi = 0.0;
+        if (on) {
+            if (rho < 0.0) rho = 0.0;
+            R f, df;
+            spl_fdf_x<R>(hdr[P.tF[eli]], rec, rho, f, df);
+            Fi = (double)f;
+            dFi = (double)df;
+        }
+        if (lane == 0) {
+            dF[i] = dFi;
+            if (epa && on) epa[i] += Fi;
+        }
+    }
+    if (err) atomicOr(flags, err);
+    if (lane == 0) s_e[wave] = Fi;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[blockIdx.x] = ((s_e[0] + s_e[1]) + s_e[2]) + s_e[3];
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256)
+k_eam_force(EamDev P, const SplHdr *__restrict__ hdr, const R *__restrict__ rec, int nat,
+            const int *__restrict__ order, const double *__restrict__ r, const int *__restrict__ el,
+            CellArg C, const int *__restrict__ offsets, const int2 *__restrict__ nbr,
+            const int *__restrict__ mask, const double *__restrict__ dF, double *__restrict__ f,
+            double *__restrict__ epa, double *__restrict__ wpa, double *__restrict__ partial,
+            int *__restrict__ flags)
+{
+    __shared__ double s_red[4][7];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int ai = blockIdx.x * 4 + wave;
+    double acc[3] = {0.0, 0.0, 0.0}, e = 0.0;
+    double w[6] = {0, 0, 0, 0, 0, 0};      // own-loop virial (xx yy zz yz xz xy)
+    double wp[6] = {0, 0, 0, 0, 0, 0};     // per-atom virial
+    int err = 0;
+    int i = -1;
+    if (ai < nat) {
+        i = order[ai];
+        const int eli = el[i];
+        if (eli >= 0) {
+            const double mi = (mask && mask[i] == 0) ? 0.0 : 1.0;
+            const double dFi = dF[i];
+            const double ri[3] = {r[3 * i], r[3 * i + 1], r[3 * i + 2]};
+            const int beg = offsets[i], end = offsets[i + 1];
+            const SplHdr hri = hdr[P.tR[eli]];
+            for (int s = beg + lane; s < end; s += 64) {
+                int2 en = nbr[s];
+                const int j = en.x;
+                int elj = el[j];
+                if (elj < 0) continue;
+                int a, b, c;
+                dc_unpack(en.y, a, b, c);
+                const double rj[3] = {r[3 * j], r[3 * j + 1], r[3 * j + 2]};
+                double d[3];
+                delta_r(C, ri, rj, a, b, c, d);
+                double d2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+                if (!(d2 < P.cutoff_sq)) continue;
+                const double mj = (mask && mask[j] == 0) ? 0.0 : 1.0;
+                double ad = sqrt(d2);
+                R phi, dphi;
+                spl_fdf_x<R>(hdr[P.tP[eli * EAM_MAX_EL + elj]], rec, ad, phi, dphi);
+                R rr = (R)(1.0 / ad);
+                R pp = (dphi - phi * rr) * rr;
+                R drho_j = spl_df<R>(hdr[P.tR[elj]], rec, ad, err);
+                R drho_i = spl_df<R>(hri, rec, ad, err);
+                double ci = mi * ((double)((R)dFi * drho_j + pp));
+                double cj = mj * ((double)((R)dF[j] * drho_i + pp));
+                e += mi * (double)(phi * rr);
+                double rrd = 1.0 / ad;
+                double fs = -(ci + cj) * rrd;
+                acc[0] += fs * d[0];
+                acc[1] += fs * d[1];
+                acc[2] += fs * d[2];
+                double wo = ci * rrd, wq = 0.5 * (ci + cj) * rrd;
+                double dd[6] = {d[0] * d[0], d[1] * d[1], d[2] * d[2], d[1] * d[2], d[0] * d[2], d[0] * d[1]};
+                for (int q = 0; q < 6; ++q) {
+                    w[q] += wo * dd[q];
+                    wp[q] += wq * dd[q];
+                }
+            }
+        }
+    }
+    for (int x = 0; x < 3; ++x) acc[x] = wave_sum(acc[x]);
+    e = wave_sum(e);
+    for (int q = 0; q < 6; ++q) w[q] = wave_sum(w[q]);
+    if (wpa)
+        for (int q = 0; q < 6; ++q) wp[q] = wave_sum(wp[q]);
+    if (i >= 0 && lane == 0) {
+        f[3 * (size_t)i] += acc[0];
+        f[3 * (size_t)i + 1] += acc[1];
+        f[3 * (size_t)i + 2] += acc[2];
+        if (epa) epa[i] += e;
+        if (wpa) {
+            double *o = wpa + 9 * (size_t)i;
+            // column-major 3x3: [a + 3b]
+            o[0] += wp[0];
+            o[4] += wp[1];
+            o[8] += wp[2];
+            o[5] += wp[3];
+            o[7] += wp[3];
+            o[2] += wp[4];
+            o[6] += wp[4];
+            o[1] += wp[5];
+            o[3] += wp[5];
+        }
+    }
+    if (err) atomicOr(flags, err);
+    if (lane == 0) {
+        s_red[wave][0] = e;
+        for (int q = 0; q < 6; ++q) s_red[wave][1 + q] = w[q];
+    }
+    __syncthreads();
+    if (threadIdx.x < 7) {
+        int q = threadIdx.x;
+        double v = ((s_red[0][q] + s_red[1][q]) + s_red[2][q]) + s_red[3][q];
+        // record layout [e, W column-major 3x3]
+        double *rec10 = partial + (size_t)blockIdx.x * 10;
+        if (q == 0) rec10[0] = v;
+        else if (q == 1) rec10[1 + 0] = v;
+        else if (q == 2) rec10[1 + 4] = v;
+        else if (q == 3) rec10[1 + 8] = v;
+        else if (q == 4) { rec10[1 + 5] = v; rec10[1 + 7] = v; }
+        else if (q == 5) { rec10[1 + 2] = v; rec10[1 + 6] = v; }
+        else { rec10[1 + 1] = v; rec10[1 + 3] = v; }
+    }
+}
+
+int eam_bind(agpu_potential *pot)
+{
+    EamState *st = static_cast<EamState *>(pot->impl);
+    agpu_particles *p = pot->p;
+    for (int z = 0; z < 128; ++z) st->z2db[z] = -1;
+    // element filter (filter_from_string): "*" or empty selects everything
+    std::vector<int> allowed;
+    bool all = st->elements.empty() || st->elements == "*";
+    if (!all) {
+        std::string tok;
+        std::string s = st->elements + ",";
+        for (char c : s) {
+            if (c == ',' || c == ' ') {
+                if (!tok.empty()) {
+                    int z = z_of_symbol(tok);
+                    if (z < 0) return set_error(AGPU_ERR_ELEMENT, "TabulatedAlloyEAM: unknown element '" + tok + "'");
+                    allowed.push_back(z);
+                }
+                tok.clear();
+            } else {
+                tok += c;
+            }
+        }
+    }
+    for (int e = 0; e < st->nel; ++e) {
+        int z = st->Zs[e];
+        bool ok = all;
+        for (int a : allowed) ok = ok || a == z;
+        if (ok) st->z2db[z] = e;
+    }
+    AGPU_CHECK(agpu_neighbors_request_interaction_range(pot->nl, st->cutoff));
+    AGPU_CHECK(map_elements(pot, st->z2db, 128));
+    st->Z_rev = p->Z_rev;
+    return AGPU_OK;
+}
+
+template <typename R>
+static int eam_launch(EamState *st, agpu_potential *pot, const agpu_eval_args *a, const R *rec, int nblk)
+{
+    agpu_particles *p = pot->p;
+    agpu_neighbors *nl = pot->nl;
+    hipStream_t s = stream();
+    CellArg C;
+    std::memcpy(C.A, p->cell.A, sizeof C.A);
+    const int *el = pot->d_el.as<int>();
+    const int *order = nl->idx_sorted.as<int>();
+    double *part = st->partial.as<double>();
+    int *flags = st->stats.as<int>();
+    const bool mixed = sizeof(R) == 4;
+    AGPU_LAUNCH(mixed ? "eam_density_mixed" : "eam_density_fp64", k_eam_density<R>, dim3(nblk), dim3(256), 0, s,
+                st->dev, st->d_hdr.as<SplHdr>(), rec, p->nat, order, p->d_r, el, C, nl->offsets.as<int>(),
+                nl->nbr.as<int2>(), a->mask, st->dF.as<double>(), a->epot_per_at, part, flags);
+    AGPU_LAUNCH(mixed ? "eam_force_mixed" : "eam_force_fp64", k_eam_force<R>, dim3(nblk), dim3(256), 0, s,
+                st->dev, st->d_hdr.as<SplHdr>(), rec, p->nat, order, p->d_r, el, C, nl->offsets.as<int>(),
+                nl->nbr.as<int2>(), a->mask, st->dF.as<double>(), a->f, a->epot_per_at, a->wpot_per_at,
+                part + nblk, flags);
+    return AGPU_OK;
+}
+
+int eam_compute(agpu_potential *pot, const agpu_eval_args *a)
+{
+    EamState *st = static_cast<EamState *>(pot->impl);
+    agpu_particles *p = pot->p;
+    if (st->Z_rev != p->Z_rev) AGPU_CHECK(eam_bind(pot));
+    AGPU_CHECK(nl_update(pot->nl, p, 0));
+    const int nat = p->nat;
+    hipStream_t s = stream();
+    AGPU_HIP(hipMemsetAsync(a->epot, 0, sizeof(double), s));
+    if (nat == 0) return AGPU_OK;
+    const int nblk = (nat + 3) / 4;
+    AGPU_CHECK(st->dF.reserve(sizeof(double) * (nat + 1)));
+    // layout: nblk energy scalars of the density pass, then nblk 10-records
+    AGPU_CHECK(st->partial.reserve(sizeof(double) * 11 * (size_t)nblk));
+    AGPU_CHECK(st->stats.reserve(sizeof(int) * 4));
+    AGPU_CHECK(pot->stats.init());
+    AGPU_HIP(hipMemsetAsync(st->stats.ptr, 0, sizeof(int) * 4, s));
+    if (pot->precision == AGPU_PREC_MIXED)
+        AGPU_CHECK(eam_launch<float>(st, pot, a, st->d_rec32.as<float>(), nblk));
+    else
+        AGPU_CHECK(eam_launch<double>(st, pot, a, st->d_rec64.as<double>(), nblk));
+    AGPU_CHECK(reduce_ev(st->partial.as<double>(), nblk, 1, a->epot, nullptr, nullptr));
+    AGPU_CHECK(reduce_ev(st->partial.as<double>() + nblk, nblk, 10, a->epot, a->wpot, nullptr));
+    AGPU_HIP(hipMemcpyAsync(pot->stats.h, st->stats.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
+    AGPU_HIP(hipStreamSynchronize(s));
+    if (pot->stats.h[0] & DERR_SPLINE_RANGE)
+        return set_error(AGPU_ERR_SPLINE, "TabulatedAlloyEAM: spline argument outside the defined interval");
+    return AGPU_OK;
+}
+
+}  // namespace agpu
+
+using namespace agpu;
+
+extern "C" int agpu_eam_alloy_create(agpu_potential **pot, const char *fn, const char *elements)
+{
+    if (!pot || !fn) return set_error(AGPU_ERR_ARG, "eam_alloy_create: NULL argument");
+    EamState *st = new EamState();
+    st->fn = fn;
+    st->elements = elements ? elements : "*";
+    int rc = read_setfl(*st);
+    if (rc == AGPU_OK) rc = upload_tables(*st);
+    if (rc != AGPU_OK) {
+        delete st;
+        return rc;
+    }
+    agpu_potential *p = new agpu_potential();
+    p->kind = AGPU_KIND_EAM_ALLOY;
+    p->cutoff = st->cutoff;
+    p->impl = st;
+    p->free_impl = eam_free;
+    *pot = p;
+    return AGPU_OK;
+}

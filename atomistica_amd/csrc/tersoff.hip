@@ -1,0 +1,663 @@
+// Tersoff bond-order potential: energy, forces and virial on gfx950.
+//
+// Semantics follow the reference's generic BOP kernel with the Tersoff forms
+// (src/potentials/bop/bop_kernel.f90:563-1529, tersoff_func.f90:32-211);
+// the work decomposition is this library's own:
+//
+//   k_bop_bonds   compact the neighbour row of every atom to its bonds
+//                 (r < r2 of the element pair, loop1 of bop_kernel.f90:997-1058)
+//   k_bop_rev     for each bond i->j (image n) the slot of j->i (image -n)
+//   k_tersoff     a group of G lanes per atom, lane = bond ij: zeta_ij over the
+//                 group's bonds through LDS, b_ij, then every triplet
+//                 derivative once.  Forces on k are reduced across the group
+//                 into the lane that owns bond ik (a fixed butterfly, no
+//                 atomics); the result per bond slot is "force on the
+//                 neighbour from the terms centred at i".
+//   k_bop_gather  owner-computes assembly: f_a = f_own(a) + sum over a's bonds
+//                 of the slot force its neighbour computed for a.
+// Energy and virial are summed per block and reduced in a fixed order, so
+// every result is bitwise reproducible run to run.
+//
+// Precision: R = double (FP64) or float (mixed: FP32 pair/triplet arithmetic,
+// FP64 position differences and FP64 accumulation of forces, energy, virial).
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "agpu_internal.hpp"
+#include "profile.hpp"
+
+namespace agpu {
+
+struct TersoffDev {
+    int nel;
+    double A[6], B[6], xi[6], lambda[6], mu[6], omega[6], mubo[6];
+    int m[6];
+    double beta[3], n[3], c[3], d[3], h[3];
+    double r1[6], r2[6];
+    // derived
+    double c2[3], d2[3], bn[3], e[3], fac[6];
+};
+
+struct TersoffState {
+    agpu_tersoff_params prm;
+    TersoffDev dev;
+    double cutoff = 0.0;
+    int z2db[128];
+    uint64_t Z_rev = ~0ull;
+    DevBuf bslot, bcnt, rev, fslot, fown, eslot, wslot, partial, stats;
+};
+
+__host__ __device__ __forceinline__ int pair_index(int i, int j, int nel)
+{
+    int a = i + j * nel, b = j + i * nel;
+    int c = i * (i + 1) / 2, d = j * (j + 1) / 2;
+    return (a < b ? a : b) - (c < d ? c : d);
+}
+
+static void derive(TersoffDev &D, const agpu_tersoff_params &p)
+{
+    D.nel = p.nel;
+    for (int k = 0; k < 6; ++k) {
+        D.A[k] = p.A[k];
+        D.B[k] = p.B[k];
+        D.xi[k] = p.xi[k];
+        D.lambda[k] = p.lambda[k];
+        D.mu[k] = p.mu[k];
+        D.omega[k] = p.omega[k];
+        D.mubo[k] = p.mubo[k];
+        D.m[k] = p.m[k];
+        D.r1[k] = p.r1[k];
+        D.r2[k] = p.r2[k];
+        D.fac[k] = M_PI / (p.r2[k] - p.r1[k]);
+    }
+    for (int k = 0; k < 3; ++k) {
+        D.beta[k] = p.beta[k];
+        D.n[k] = p.n[k];
+        D.c[k] = p.c[k];
+        D.d[k] = p.d[k];
+        D.h[k] = p.h[k];
+        D.c2[k] = p.c[k] * p.c[k];
+        D.d2[k] = p.d[k] * p.d[k];
+        D.bn[k] = std::pow(p.beta[k], p.n[k]);
+        D.e[k] = -0.5 / p.n[k];
+    }
+}
+
+// -------------------------------------------------------------------------
+// bonds: compact each row to the pairs inside r2 of their element pair
+// -------------------------------------------------------------------------
+__global__ void k_bop_bonds(int nat, const double *__restrict__ r, const int *__restrict__ el,
+                            CellArg C, TersoffDev P, const int *__restrict__ offsets,
+                            const int2 *__restrict__ nbr, int *__restrict__ bslot,
+                            int *__restrict__ bcnt, int *__restrict__ stats)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nat) return;
+    int eli = el[i];
+    int nb = 0;
+    if (eli >= 0) {
+        const double ri[3] = {r[3 * i], r[3 * i + 1], r[3 * i + 2]};
+        int beg = offsets[i], end = offsets[i + 1];
+        for (int s = beg; s < end; ++s) {
+            int2 e = nbr[s];
+            int elj = el[e.x];
+            if (elj < 0) continue;
+            int a, b, c;
+            dc_unpack(e.y, a, b, c);
+            const double rj[3] = {r[3 * e.x], r[3 * e.x + 1], r[3 * e.x + 2]};
+            double d[3];
+            delta_r(C, ri, rj, a, b, c, d);
+            double rl = dot3_nc(d);
+            int t = pair_index(eli, elj, P.nel);
+            if (rl < P.r2[t] * P.r2[t]) bslot[beg + nb++] = s;
+        }
+    }
+    bcnt[i] = nb;
+    atomicMax(stats, nb);
+}
+
+__global__ void k_bop_rev(int nat, const int *__restrict__ offsets, const int2 *__restrict__ nbr,
+                          const int *__restrict__ bslot, const int *__restrict__ bcnt,
+                          int *__restrict__ rev, int *__restrict__ stats)
+{
+    int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= nat) return;
+    int beg = offsets[a], nb = bcnt[a];
+    for (int k = 0; k < nb; ++k) {
+        int s = bslot[beg + k];
+        int2 e = nbr[s];
+        int x, y, z;
+        dc_unpack(e.y, x, y, z);
+        int want = dc_pack(-x, -y, -z);
+        int j = e.x, jb = offsets[j], jn = bcnt[j], found = -1;
+        for (int t = 0; t < jn; ++t) {
+            int sj = bslot[jb + t];
+            int2 ej = nbr[sj];
+            if (ej.x == a && ej.y == want) {
+                found = sj;
+                break;
+            }
+        }
+        rev[s] = found;
+        if (found < 0) atomicOr(stats + 1, DERR_BOND_OVERFLOW);
+    }
+}
+
+// -------------------------------------------------------------------------
+// the bond-order kernel
+// -------------------------------------------------------------------------
+template <typename R> __device__ __forceinline__ R rexp(R x);
+template <> __device__ __forceinline__ double rexp<double>(double x) { return exp(x); }
+template <> __device__ __forceinline__ float rexp<float>(float x) { return expf(x); }
+template <typename R> __device__ __forceinline__ R rpow(R x, R y);
+template <> __device__ __forceinline__ double rpow<double>(double x, double y) { return pow(x, y); }
+template <> __device__ __forceinline__ float rpow<float>(float x, float y) { return powf(x, y); }
+template <typename R> __device__ __forceinline__ R rsqrt_(R x);
+template <> __device__ __forceinline__ double rsqrt_<double>(double x) { return sqrt(x); }
+template <> __device__ __forceinline__ float rsqrt_<float>(float x) { return sqrtf(x); }
+template <typename R> __device__ __forceinline__ void rsincos(R x, R *s, R *c);
+template <> __device__ __forceinline__ void rsincos<double>(double x, double *s, double *c) { sincos(x, s, c); }
+template <> __device__ __forceinline__ void rsincos<float>(float x, float *s, float *c) { sincosf(x, s, c); }
+
+template <typename R>
+__device__ __forceinline__ void h_fn(const TersoffDev &P, int ikpot, R dr, R &val, R &dval)
+{
+    R mu = (R)P.mubo[ikpot];
+    if (mu == (R)0) {
+        val = (R)1;
+        dval = (R)0;
+        return;
+    }
+    int m = P.m[ikpot];
+    if (m == 1) {
+        val = rexp<R>((R)2 * mu * dr);
+        dval = (R)2 * mu * val;
+    } else if (m == 3) {
+        R arg = (R)2 * mu * dr;
+        val = rexp<R>(arg * arg * arg);
+        dval = (R)2 * mu * (R)m * arg * arg * val;
+    } else {
+        val = rexp<R>(rpow<R>((R)2 * mu * dr, (R)m));
+        dval = (R)2 * mu * (R)m * rpow<R>((R)2 * mu * dr, (R)(m - 1)) * val;
+    }
+}
+
+template <typename R, int G>
+__global__ void __launch_bounds__(256)
+k_tersoff(TersoffDev P, int nat, const int *__restrict__ order, const double *__restrict__ r,
+          const int *__restrict__ el, CellArg C, const int *__restrict__ offsets,
+          const int2 *__restrict__ nbr, const int *__restrict__ bslot, const int *__restrict__ bcnt,
+          const int *__restrict__ mask, double *__restrict__ fslot, double *__restrict__ fown,
+          double *__restrict__ eslot, double *__restrict__ wslot, double *__restrict__ partial)
+{
+    constexpr int GPB = 256 / G;
+    __shared__ R s_rn[3][256];
+    __shared__ R s_rl[256], s_fc[256], s_dfc[256];
+    __shared__ int s_typ[256];
+    __shared__ double s_red[4][10];
+
+    const int tid = threadIdx.x;
+    const int lane = tid % G;
+    const int grp = tid / G;
+    const int gbase = grp * G;
+    const int ai = blockIdx.x * GPB + grp;
+    const bool valid = ai < nat;
+    const int i = valid ? order[ai] : 0;
+    const int eli = valid ? el[i] : -1;
+    // G is chosen >= every bond count; the clamp only keeps lanes in range
+    const int nb = (eli >= 0) ? min(bcnt[i], G) : 0;
+    const bool act = lane < nb;
+
+    int s = -1, j = 0, typ = 0;
+    R rn[3] = {(R)0, (R)0, (R)0}, rl = (R)1, fc = (R)0, dfc = (R)0;
+    bool ok = false;
+    if (act) {
+        s = bslot[offsets[i] + lane];
+        int2 e = nbr[s];
+        j = e.x;
+        int a, b, c;
+        dc_unpack(e.y, a, b, c);
+        // rij = r_j - r_i - A.dc   (bop_kernel.f90:611)
+        double rij[3];
+        for (int x = 0; x < 3; ++x)
+            rij[x] = (r[3 * j + x] - r[3 * i + x]) - image_x(C, x, a, b, c);
+        double rl2 = rij[0] * rij[0] + rij[1] * rij[1] + rij[2] * rij[2];
+        double rld = sqrt(rl2);
+        typ = pair_index(eli, el[j], P.nel);
+        for (int x = 0; x < 3; ++x) rn[x] = (R)(rij[x] / rld);
+        rl = (R)rld;
+        ok = rld < P.r2[typ];
+        if (rl2 < P.r1[typ] * P.r1[typ]) {
+            fc = (R)1;
+            dfc = (R)0;
+        } else {
+            R fac = (R)P.fac[typ];
+            R x = fac * (rl - (R)P.r1[typ]);
+            R sn, cs;
+            rsincos<R>(x, &sn, &cs);
+            fc = (R)0.5 * ((R)1 + cs);
+            dfc = (R)-0.5 * fac * sn;
+            if (!(rld > P.r1[typ])) { fc = (R)1; dfc = (R)0; }
+            if (!(rld < P.r2[typ])) { fc = (R)0; dfc = (R)0; }
+        }
+    }
+    s_rn[0][tid] = rn[0];
+    s_rn[1][tid] = rn[1];
+    s_rn[2][tid] = rn[2];
+    s_rl[tid] = rl;
+    s_fc[tid] = fc;
+    s_dfc[tid] = dfc;
+    s_typ[tid] = ok ? typ : -1;
+    __syncthreads();
+
+    int maskfac = 2;
+    if (mask && act) {
+        int mi = mask[i], mj = mask[j];
+        if (mi == 0 && mj == 0) maskfac = 0;
+        else if (mi == 0 || mj == 0) maskfac = 1;
+    }
+    const bool on = act && ok && maskfac > 0;
+
+    // ---- pass 1: zeta_ij ----
+    const R hi = (R)P.h[eli < 0 ? 0 : eli];
+    const R c2 = (R)P.c2[eli < 0 ? 0 : eli];
+    const R d2 = (R)P.d2[eli < 0 ? 0 : eli];
+    const R cd = c2 / d2;
+    R zij = (R)0;
+#pragma unroll
+    for (int k = 0; k < G; ++k) {
+        const int tk = gbase + k;
+        const int tpk = s_typ[tk];
+        if (on && k != lane && k < nb && tpk >= 0) {
+            R rnk[3] = {s_rn[0][tk], s_rn[1][tk], s_rn[2][tk]};
+            R costh = rnk[0] * rn[0] + rnk[1] * rn[1] + rnk[2] * rn[2];
+            R h_c = hi - costh;
+            R hh = d2 + h_c * h_c;
+            R g = (R)P.omega[tpk] * ((R)1 + cd - c2 / hh);
+            R hv, dh;
+            h_fn<R>(P, tpk, rl - s_rl[tk], hv, dh);
+            zij += s_fc[tk] * g * hv;
+        }
+    }
+
+    // ---- bond order ----
+    R VA = (R)0, dVA = (R)0, VR = (R)0, dVR = (R)0, bij = (R)1, Pz = (R)0;
+    if (on) {
+        R ea = rexp<R>(-(R)P.mu[typ] * rl);
+        VA = -(R)P.B[typ] * ea;
+        dVA = (R)P.B[typ] * (R)P.mu[typ] * ea;
+        R er = rexp<R>(-(R)P.lambda[typ] * rl);
+        VR = (R)P.A[typ] * er;
+        dVR = -(R)P.A[typ] * (R)P.lambda[typ] * er;
+        R mf = (R)0.5 * (R)maskfac;
+        VA *= mf;
+        dVA *= mf;
+        VR *= mf;
+        dVR *= mf;
+        if (zij > (R)0) {
+            R nn = (R)P.n[eli];
+            R e = (R)P.e[eli];
+            R b = (R)P.bn[eli];
+            R zn1 = rpow<R>(zij, nn - (R)1);
+            R arg = (R)1 + b * zn1 * zij;
+            R ae1 = rpow<R>(arg, e - (R)1);
+            bij = (R)P.xi[typ] * ae1 * arg;
+            Pz = (R)-0.25 * fc * VA * (R)P.xi[typ] * b * zn1 * ae1;
+        }
+    }
+
+    // ---- pass 2: derivatives; forces on k reduced into lane k ----
+    R dbidi[3] = {(R)0, (R)0, (R)0}, dbidj[3] = {(R)0, (R)0, (R)0};
+    R wijb[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) wijb[q] = (R)0;
+    double recv[3] = {0.0, 0.0, 0.0};
+    const R rlijr = (R)1 / rl;
+    const R rij[3] = {rl * rn[0], rl * rn[1], rl * rn[2]};
+#pragma unroll
+    for (int k = 0; k < G; ++k) {
+        const int tk = gbase + k;
+        const int tpk = s_typ[tk];
+        R dk[3] = {(R)0, (R)0, (R)0};
+        if (on && k != lane && k < nb && tpk >= 0) {
+            R rnk[3] = {s_rn[0][tk], s_rn[1][tk], s_rn[2][tk]};
+            R rlik = s_rl[tk], fcik = s_fc[tk], dfcik = s_dfc[tk];
+            R hv, dh;
+            h_fn<R>(P, tpk, rl - rlik, hv, dh);
+            R costh = rnk[0] * rn[0] + rnk[1] * rn[1] + rnk[2] * rn[2];
+            R h_c = hi - costh;
+            R hh = d2 + h_c * h_c;
+            R om = (R)P.omega[tpk];
+            R g = om * ((R)1 + cd - c2 / hh);
+            R dg = (R)-2 * om * c2 * h_c / (hh * hh);
+            R dkc[3];
+            for (int x = 0; x < 3; ++x) dkc[x] = rnk[x] * rlik - rn[x] * rl;
+            R disjk = rsqrt_<R>(dkc[0] * dkc[0] + dkc[1] * dkc[1] + dkc[2] * dkc[2]);
+            for (int x = 0; x < 3; ++x) dkc[x] = dkc[x] / disjk;
+            R rlikr = (R)1 / rlik;
+            R dcsdij = rlikr - costh * rlijr;
+            R dcsdik = rlijr - costh * rlikr;
+            R dcsdjk = -disjk * rlijr * rlikr;
+            R dzfac = fcik * dg * hv;
+            R dzdrij = g * fcik * dh;
+            R dzdrik = g * (dfcik * hv - fcik * dh);
+            R df[3];
+            for (int x = 0; x < 3; ++x) {
+                R dcsdi = -dcsdij * rn[x] - dcsdik * rnk[x];
+                R dcsdj = dcsdij * rn[x] - dcsdjk * dkc[x];
+                R dcsdk = dcsdik * rnk[x] + dcsdjk * dkc[x];
+                dbidi[x] += -dzdrij * rn[x] - dzdrik * rnk[x] + dzfac * dcsdi;
+                df[x] = dzdrij * rn[x] + dzfac * dcsdj;
+                dbidj[x] += df[x];
+                dk[x] = dzdrik * rnk[x] + dzfac * dcsdk;
+            }
+            R rik[3] = {rlik * rnk[0], rlik * rnk[1], rlik * rnk[2]};
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) wijb[a + 3 * b] -= rij[a] * df[b] + rik[a] * dk[b];
+        }
+        // force on k from bond ij: -P_ij * dzeta_ij/dr_k, summed over the group
+        double c0 = -(double)Pz * (double)dk[0];
+        double c1 = -(double)Pz * (double)dk[1];
+        double c2v = -(double)Pz * (double)dk[2];
+        c0 = group_sum<G>(c0);
+        c1 = group_sum<G>(c1);
+        c2v = group_sum<G>(c2v);
+        if (lane == k) {
+            recv[0] = c0;
+            recv[1] = c1;
+            recv[2] = c2v;
+        }
+    }
+
+    double fi[3] = {0.0, 0.0, 0.0}, fj[3] = {0.0, 0.0, 0.0};
+    double e_ij = 0.0;
+    double w[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) w[q] = 0.0;
+    if (on) {
+        e_ij = (double)((R)0.5 * fc * (VR + bij * VA));
+        R dffac = (R)0.5 * (dVR * fc + bij * dVA * fc + VR * dfc + bij * VA * dfc);
+        for (int x = 0; x < 3; ++x) {
+            double df = (double)(dffac * rn[x]);
+            fi[x] = df - (double)Pz * (double)dbidi[x];
+            fj[x] = -df - (double)Pz * (double)dbidj[x];
+        }
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b)
+                w[a + 3 * b] = (double)rij[a] * (double)(dffac * rn[b]) - (double)Pz * (double)wijb[a + 3 * b];
+    }
+    if (act) {
+        fslot[3 * (size_t)s + 0] = fj[0] + recv[0];
+        fslot[3 * (size_t)s + 1] = fj[1] + recv[1];
+        fslot[3 * (size_t)s + 2] = fj[2] + recv[2];
+        if (eslot) eslot[s] = e_ij;
+        if (wslot)
+            for (int q = 0; q < 9; ++q) wslot[9 * (size_t)s + q] = w[q];
+    }
+    // own force of i: sum over the group
+    double f0 = group_sum<G>(fi[0]);
+    double f1 = group_sum<G>(fi[1]);
+    double f2 = group_sum<G>(fi[2]);
+    if (valid && lane == 0) {
+        fown[3 * (size_t)i + 0] = f0;
+        fown[3 * (size_t)i + 1] = f1;
+        fown[3 * (size_t)i + 2] = f2;
+    }
+    // energy + virial: wave butterfly, then the four waves in order
+    double red[10];
+    red[0] = e_ij;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) red[1 + q] = w[q];
+#pragma unroll
+    for (int q = 0; q < 10; ++q) red[q] = group_sum<64>(red[q]);
+    const int wave = tid / 64;
+    if ((tid & 63) == 0)
+        for (int q = 0; q < 10; ++q) s_red[wave][q] = red[q];
+    __syncthreads();
+    if (tid < 10)
+        partial[(size_t)blockIdx.x * 10 + tid] =
+            ((s_red[0][tid] + s_red[1][tid]) + s_red[2][tid]) + s_red[3][tid];
+}
+
+__global__ void k_bop_gather(int nat, const int *__restrict__ offsets, const int *__restrict__ bslot,
+                             const int *__restrict__ bcnt, const int *__restrict__ rev,
+                             const double *__restrict__ fslot, const double *__restrict__ fown,
+                             const double *__restrict__ eslot, const double *__restrict__ wslot,
+                             double *__restrict__ f, double *__restrict__ epa, double *__restrict__ wpa)
+{
+    int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= nat) return;
+    double acc[3] = {fown[3 * (size_t)a], fown[3 * (size_t)a + 1], fown[3 * (size_t)a + 2]};
+    double e = 0.0, w[9];
+    for (int q = 0; q < 9; ++q) w[q] = 0.0;
+    int beg = offsets[a], nb = bcnt[a];
+    for (int k = 0; k < nb; ++k) {
+        int s = bslot[beg + k];
+        int t = rev[s];
+        if (t < 0) continue;
+        acc[0] += fslot[3 * (size_t)t];
+        acc[1] += fslot[3 * (size_t)t + 1];
+        acc[2] += fslot[3 * (size_t)t + 2];
+        if (epa) e += eslot[s] + eslot[t];
+        if (wpa)
+            for (int q = 0; q < 9; ++q) w[q] += wslot[9 * (size_t)s + q] + wslot[9 * (size_t)t + q];
+    }
+    f[3 * (size_t)a] += acc[0];
+    f[3 * (size_t)a + 1] += acc[1];
+    f[3 * (size_t)a + 2] += acc[2];
+    if (epa) epa[a] += 0.5 * e;
+    if (wpa)
+        for (int q = 0; q < 9; ++q) wpa[9 * (size_t)a + q] += 0.5 * w[q];
+}
+
+// -------------------------------------------------------------------------
+// host side
+// -------------------------------------------------------------------------
+static void tersoff_free(void *p) { delete static_cast<TersoffState *>(p); }
+
+static agpu_tersoff_params default_params()
+{
+    // Tersoff_PRB_39_5566_SiC (tersoff_params.f90:99-138): elements C, Si
+    agpu_tersoff_params p;
+    std::memset(&p, 0, sizeof p);
+    p.nel = 2;
+    p.Z[0] = 6;
+    p.Z[1] = 14;
+    double A[3] = {1.3936e3, std::sqrt(1.3936e3 * 1.8308e3), 1.8308e3};
+    double B[3] = {3.4674e2, std::sqrt(3.4674e2 * 4.7118e2), 4.7118e2};
+    double xi[3] = {1.0, 0.9776e0, 1.0};
+    double la[3] = {3.4879e0, (3.4879e0 + 2.4799e0) / 2, 2.4799e0};
+    double mu[3] = {2.2119e0, (2.2119e0 + 1.7322e0) / 2, 1.7322e0};
+    double r1[3] = {1.80, std::sqrt(1.80 * 2.70), 2.70};
+    double r2[3] = {2.10, std::sqrt(2.10 * 3.00), 3.00};
+    for (int k = 0; k < 6; ++k) {
+        p.A[k] = k < 3 ? A[k] : 1.0;
+        p.B[k] = k < 3 ? B[k] : 1.0;
+        p.xi[k] = k < 3 ? xi[k] : 1.0;
+        p.lambda[k] = k < 3 ? la[k] : 1.0;
+        p.mu[k] = k < 3 ? mu[k] : 1.0;
+        p.omega[k] = 1.0;
+        p.mubo[k] = 0.0;
+        p.m[k] = 1;
+        p.r1[k] = k < 3 ? r1[k] : 1.0;
+        p.r2[k] = k < 3 ? r2[k] : 2.0;
+    }
+    double beta[2] = {1.5724e-7, 1.1000e-6}, n[2] = {7.2751e-1, 7.8734e-1},
+           c[2] = {3.8049e4, 1.0039e5}, d[2] = {4.3484e0, 1.6217e1}, h[2] = {-5.7058e-1, -5.9825e-1};
+    for (int k = 0; k < 3; ++k) {
+        p.beta[k] = k < 2 ? beta[k] : 1.0;
+        p.n[k] = k < 2 ? n[k] : 1.0;
+        p.c[k] = k < 2 ? c[k] : 1.0;
+        p.d[k] = k < 2 ? d[k] : 1.0;
+        p.h[k] = k < 2 ? h[k] : 1.0;
+    }
+    return p;
+}
+
+int tersoff_bind(agpu_potential *pot)
+{
+    TersoffState *st = static_cast<TersoffState *>(pot->impl);
+    agpu_particles *p = pot->p;
+    const auto &prm = st->prm;
+    for (int z = 0; z < 128; ++z) st->z2db[z] = -1;
+    for (int e = 0; e < prm.nel; ++e) {
+        if (prm.Z[e] <= 0 || prm.Z[e] >= 128)
+            return set_error(AGPU_ERR_ELEMENT, "Tersoff: unknown element in the parameter set");
+        st->z2db[prm.Z[e]] = e;
+    }
+    // request_interaction_range(nl, r2(pair)) for every element pair present
+    // (default_bind_to_func.f90:123-144)
+    bool present[3] = {false, false, false};
+    for (int z : p->h_Z)
+        if (z > 0 && z < 128 && st->z2db[z] >= 0) present[st->z2db[z]] = true;
+    double cut = 0.0;
+    for (int a = 0; a < prm.nel; ++a)
+        for (int b = 0; b < prm.nel; ++b)
+            if (present[a] && present[b]) {
+                double r2 = prm.r2[pair_index(a, b, prm.nel)];
+                if (r2 > cut) cut = r2;
+                AGPU_CHECK(agpu_neighbors_request_interaction_range(pot->nl, r2));
+            }
+    st->cutoff = cut;
+    AGPU_CHECK(map_elements(pot, st->z2db, 128));
+    st->Z_rev = p->Z_rev;
+    return AGPU_OK;
+}
+
+template <typename R, int G>
+static int launch_tersoff(TersoffState *st, int nat, const int *order, const double *r, const int *el,
+                          const CellArg &C, agpu_neighbors *nl, const int *mask, double *eslot,
+                          double *wslot, int nblk, hipStream_t s)
+{
+    AGPU_LAUNCH(sizeof(R) == 8 ? "tersoff_fp64" : "tersoff_mixed", (k_tersoff<R, G>), dim3(nblk), dim3(256), 0,
+                s, st->dev, nat, order, r, el, C, nl->offsets.as<int>(), nl->nbr.as<int2>(),
+                st->bslot.as<int>(), st->bcnt.as<int>(), mask, st->fslot.as<double>(),
+                st->fown.as<double>(), eslot, wslot, st->partial.as<double>());
+    return AGPU_OK;
+}
+
+template <typename R>
+static int dispatch_tersoff(int G, TersoffState *st, int nat, const int *order, const double *r,
+                            const int *el, const CellArg &C, agpu_neighbors *nl, const int *mask,
+                            double *eslot, double *wslot, hipStream_t s, int *nblk_out)
+{
+    int gpb = 256 / G;
+    int nblk = (nat + gpb - 1) / gpb;
+    if (nblk < 1) nblk = 1;
+    *nblk_out = nblk;
+    AGPU_CHECK(st->partial.reserve(sizeof(double) * 10 * nblk));
+    switch (G) {
+    case 2: return launch_tersoff<R, 2>(st, nat, order, r, el, C, nl, mask, eslot, wslot, nblk, s);
+    case 4: return launch_tersoff<R, 4>(st, nat, order, r, el, C, nl, mask, eslot, wslot, nblk, s);
+    case 8: return launch_tersoff<R, 8>(st, nat, order, r, el, C, nl, mask, eslot, wslot, nblk, s);
+    case 16: return launch_tersoff<R, 16>(st, nat, order, r, el, C, nl, mask, eslot, wslot, nblk, s);
+    case 32: return launch_tersoff<R, 32>(st, nat, order, r, el, C, nl, mask, eslot, wslot, nblk, s);
+    default: return launch_tersoff<R, 64>(st, nat, order, r, el, C, nl, mask, eslot, wslot, nblk, s);
+    }
+}
+
+int tersoff_compute(agpu_potential *pot, const agpu_eval_args *a)
+{
+    TersoffState *st = static_cast<TersoffState *>(pot->impl);
+    agpu_particles *p = pot->p;
+    agpu_neighbors *nl = pot->nl;
+    if (st->Z_rev != p->Z_rev) AGPU_CHECK(tersoff_bind(pot));
+    AGPU_CHECK(nl_update(nl, p, 0));
+    const int nat = p->nat;
+    hipStream_t s = stream();
+    AGPU_HIP(hipMemsetAsync(a->epot, 0, sizeof(double), s));
+    if (nat == 0) return AGPU_OK;
+    const int64_t total = nl->total;
+    AGPU_CHECK(st->bslot.reserve(sizeof(int) * (total + 1)));
+    AGPU_CHECK(st->rev.reserve(sizeof(int) * (total + 1)));
+    AGPU_CHECK(st->fslot.reserve(sizeof(double) * 3 * (total + 1)));
+    AGPU_CHECK(st->bcnt.reserve(sizeof(int) * (nat + 1)));
+    AGPU_CHECK(st->fown.reserve(sizeof(double) * 3 * (nat + 1)));
+    AGPU_CHECK(st->stats.reserve(sizeof(int) * 4));
+    AGPU_CHECK(pot->stats.init());
+    double *eslot = nullptr, *wslot = nullptr;
+    if (a->epot_per_at) {
+        AGPU_CHECK(st->eslot.reserve(sizeof(double) * (total + 1)));
+        eslot = st->eslot.as<double>();
+    }
+    if (a->wpot_per_at) {
+        AGPU_CHECK(st->wslot.reserve(sizeof(double) * 9 * (total + 1)));
+        wslot = st->wslot.as<double>();
+    }
+    int *dstats = st->stats.as<int>();
+    AGPU_HIP(hipMemsetAsync(dstats, 0, sizeof(int) * 4, s));
+    CellArg C;
+    std::memcpy(C.A, p->cell.A, sizeof C.A);
+    const int *el = pot->d_el.as<int>();
+    const int nb = (nat + 255) / 256;
+    AGPU_LAUNCH("bop_bonds", k_bop_bonds, dim3(nb), dim3(256), 0, s, nat, p->d_r, el, C, st->dev,
+                nl->offsets.as<int>(), nl->nbr.as<int2>(), st->bslot.as<int>(), st->bcnt.as<int>(), dstats);
+    AGPU_LAUNCH("bop_rev", k_bop_rev, dim3(nb), dim3(256), 0, s, nat, nl->offsets.as<int>(),
+                nl->nbr.as<int2>(), st->bslot.as<int>(), st->bcnt.as<int>(), st->rev.as<int>(), dstats);
+    // group width: the row maximum bounds the bond count without a host
+    // round trip when it is small; otherwise read the exact bond maximum
+    int maxb = nl->maxrow;
+    if (maxb > 64) {
+        AGPU_HIP(hipMemcpyAsync(pot->stats.h, dstats, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
+        AGPU_HIP(hipStreamSynchronize(s));
+        maxb = pot->stats.h[0];
+        if (maxb > 64)
+            return set_error(AGPU_ERR_NEB, "Tersoff: more than 64 bonds on one atom (nebmax too small)");
+    }
+    int G = 2;
+    while (G < maxb) G <<= 1;
+    int nblk = 0;
+    const int *order = nl->idx_sorted.as<int>();
+    if (pot->precision == AGPU_PREC_MIXED)
+        AGPU_CHECK(dispatch_tersoff<float>(G, st, nat, order, p->d_r, el, C, nl, a->mask, eslot, wslot, s, &nblk));
+    else
+        AGPU_CHECK(dispatch_tersoff<double>(G, st, nat, order, p->d_r, el, C, nl, a->mask, eslot, wslot, s, &nblk));
+    AGPU_LAUNCH("bop_gather", k_bop_gather, dim3(nb), dim3(256), 0, s, nat, nl->offsets.as<int>(),
+                st->bslot.as<int>(), st->bcnt.as<int>(), st->rev.as<int>(), st->fslot.as<double>(),
+                st->fown.as<double>(), eslot, wslot, a->f, a->epot_per_at, a->wpot_per_at);
+    // per-block records [e, W(9)] -> epot, wpot
+    AGPU_CHECK(reduce_ev(st->partial.as<double>(), nblk, 10, a->epot, a->wpot, nullptr));
+    AGPU_HIP(hipMemcpyAsync(pot->stats.h, dstats, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
+    AGPU_HIP(hipStreamSynchronize(s));
+    if (pot->stats.h[1] & DERR_BOND_OVERFLOW)
+        return set_error(AGPU_ERR_NEB, "Tersoff: bond without its reverse bond (inconsistent list)");
+    return AGPU_OK;
+}
+
+}  // namespace agpu
+
+using namespace agpu;
+
+extern "C" int agpu_tersoff_create(agpu_potential **pot, const agpu_tersoff_params *params)
+{
+    if (!pot) return set_error(AGPU_ERR_ARG, "pot is NULL");
+    TersoffState *st = new TersoffState();
+    st->prm = params ? *params : default_params();
+    if (st->prm.nel < 1 || st->prm.nel > 3) {
+        delete st;
+        return set_error(AGPU_ERR_ARG, "Tersoff: 1 to 3 elements supported");
+    }
+    derive(st->dev, st->prm);
+    agpu_potential *p = new agpu_potential();
+    p->kind = AGPU_KIND_TERSOFF;
+    for (int k = 0; k < st->prm.nel * (st->prm.nel + 1) / 2; ++k)
+        if (st->prm.r2[k] > p->cutoff) p->cutoff = st->prm.r2[k];
+    p->impl = st;
+    p->free_impl = tersoff_free;
+    *pot = p;
+    return AGPU_OK;
+}
+
+extern "C" int agpu_tersoff_set_params(agpu_potential *pot, const agpu_tersoff_params *params)
+{
+    if (!pot || pot->kind != AGPU_KIND_TERSOFF || !params) return set_error(AGPU_ERR_ARG, "bad arguments");
+    TersoffState *st = static_cast<TersoffState *>(pot->impl);
+    st->prm = *params;
+    derive(st->dev, st->prm);
+    pot->cutoff = 0.0;
+    for (int k = 0; k < st->prm.nel * (st->prm.nel + 1) / 2; ++k)
+        if (st->prm.r2[k] > pot->cutoff) pot->cutoff = st->prm.r2[k];
+    if (pot->bound) return tersoff_bind(pot);
+    return AGPU_OK;
+}

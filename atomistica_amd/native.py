@@ -1,0 +1,385 @@
+"""Host-side mirror of the reference's ``_atomistica`` extension types.
+
+``Particles``, ``Neighbors`` and the potential classes keep the reference's
+method names, argument meanings and error behaviour
+(src/python/c/particles.c, neighbors.c, potential.c:347-666, coulomb.c),
+and drive the MI355X kernels through the C-ABI in ``libatomistica_amd.so``.
+
+Host numpy arrays are staged through HBM by the library (``AGPU_HOST``);
+torch CUDA tensors (device pointers) are passed through without copies by
+the ``*_device`` variants.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import AgpuError, call
+from .parameters import ATOMIC_NUMBERS, pair_index
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+def _dev(t):
+    """Device pointer of a torch tensor (or None)."""
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise ValueError("expected a device tensor")
+    if not t.is_contiguous():
+        raise ValueError("expected a contiguous device tensor")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class Particles:
+    """particles_t (src/python/f90/python_particles.f90:84-170)."""
+
+    def __init__(self):
+        h = ctypes.c_void_p()
+        call("agpu_particles_create", ctypes.byref(h))
+        self._h = h
+        self.nat = 0
+        self.coordinates = np.zeros((0, 3))
+        self.Z = np.zeros(0, np.int32)
+        self.cell = np.zeros((3, 3))
+        self.pbc = np.array([True, True, True])
+        self._dev_positions = None
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib._lib is not None:
+            _lib.load().agpu_particles_destroy(h)
+            self._h = None
+
+    def __len__(self):
+        return self.nat
+
+    def register_data(self, *args):
+        pass
+
+    def allocate(self, nat):
+        call("agpu_particles_allocate", self._h, int(nat))
+        self.nat = int(nat)
+        self.coordinates = np.zeros((self.nat, 3))
+        self.Z = np.zeros(self.nat, np.int32)
+
+    def set_cell(self, cell, pbc=None):
+        cell = np.ascontiguousarray(np.asarray(cell, dtype=np.float64).reshape(3, 3))
+        if pbc is None:
+            pbc = [True, True, True]
+        pbc_ = np.ascontiguousarray(np.asarray(pbc, dtype=bool).astype(np.int32).reshape(3))
+        call("agpu_particles_set_cell", self._h, cell.ctypes.data_as(_lib.dp),
+             pbc_.ctypes.data_as(_lib.ip))
+        self.cell = cell.copy()
+        self.pbc = pbc_.astype(bool)
+
+    def update_elements(self):
+        Z = np.ascontiguousarray(self.Z, dtype=np.int32)
+        call("agpu_particles_set_atomic_numbers", self._h, _ptr(Z), _lib.HOST)
+
+    def I_changed_positions(self):
+        """Push host coordinates to HBM and mark positions changed."""
+        r = np.ascontiguousarray(self.coordinates, dtype=np.float64)
+        call("agpu_particles_set_positions", self._h, _ptr(r), _lib.HOST)
+        self._dev_positions = None
+
+    def set_positions_device(self, r):
+        """Alias a (nat, 3) float64 device tensor as the positions."""
+        if tuple(r.shape) != (self.nat, 3):
+            raise ValueError("positions must have shape (nat, 3)")
+        call("agpu_particles_set_position_pointer", self._h, self.nat, _dev(r))
+        self._dev_positions = r
+
+    def changed_positions(self):
+        call("agpu_particles_changed_positions", self._h)
+
+    def get_cell(self):
+        return self.cell.copy()
+
+    def get_volume(self):
+        return abs(np.linalg.det(self.cell))
+
+
+class Neighbors:
+    """neighbors_t (src/python/f90/python_neighbors.f90:51-139)."""
+
+    def __init__(self, avgn=100):
+        h = ctypes.c_void_p()
+        call("agpu_neighbors_create", ctypes.byref(h), int(avgn))
+        self._h = h
+        self.avgn = int(avgn)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib._lib is not None:
+            _lib.load().agpu_neighbors_destroy(h)
+            self._h = None
+
+    def request_interaction_range(self, cutoff):
+        call("agpu_neighbors_request_interaction_range", self._h, float(cutoff))
+
+    def set_bin_size(self, bin_size):
+        call("agpu_neighbors_set_bin_size", self._h, float(bin_size))
+
+    def update(self, p):
+        call("agpu_neighbors_update", self._h, p._h)
+
+    def get_size(self):
+        total = ctypes.c_int64()
+        maxrow = ctypes.c_int()
+        call("agpu_neighbors_get_size", self._h, ctypes.byref(total), ctypes.byref(maxrow))
+        return total.value, maxrow.value
+
+    def get_list(self, p):
+        """CSR list in the reference's fill order: (offsets, j, dc)."""
+        self.update(p)
+        total, _ = self.get_size()
+        offsets = np.zeros(p.nat + 1, np.int64)
+        j = np.zeros(max(total, 1), np.int32)
+        dc = np.zeros((max(total, 1), 3), np.int32)
+        call("agpu_neighbors_get", self._h, _ptr(offsets), _ptr(j), _ptr(dc), _lib.HOST)
+        return offsets, j[:total], dc[:total]
+
+    def get_neighbors(self, p, vec=False):
+        """(i, j, abs_dr) or (i, j, dr, abs_dr) -- neighbors.c:143-232, with
+        dr = r_i - r_j + cell.dc (neighbors_wrap.f90:461-492)."""
+        offsets, j, dc = self.get_list(p)
+        i = np.repeat(np.arange(p.nat, dtype=np.int32), np.diff(offsets))
+        r = p.coordinates
+        dr = r[i] - r[j] + dc.astype(np.float64) @ p.cell
+        absdr = np.sqrt(np.sum(dr * dr, axis=1))
+        if vec:
+            return i, j, dr, absdr
+        return i, j, absdr
+
+    def get_coordination_numbers(self, p, cutoff):
+        i, j, absdr = self.get_neighbors(p)
+        return np.bincount(i[absdr < cutoff], minlength=p.nat)
+
+    def coordination(self, p, cutoff):
+        return self.get_coordination_numbers(p, cutoff)
+
+
+class _Potential:
+    kind = 0
+
+    def __init__(self):
+        self._h = None
+        self.precision = "fp64"
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib._lib is not None:
+            _lib.load().agpu_potential_destroy(h)
+            self._h = None
+
+    def set_precision(self, precision):
+        """'fp64' (default) or 'mixed' (FP32 arithmetic, FP64 accumulation)."""
+        mode = {"fp64": _lib.PREC_FP64, "mixed": _lib.PREC_MIXED}[precision]
+        call("agpu_potential_set_precision", self._h, mode)
+        self.precision = precision
+
+    def get_cutoff(self):
+        c = ctypes.c_double()
+        call("agpu_potential_get_cutoff", self._h, ctypes.byref(c))
+        return c.value
+
+    def register_data(self, p):
+        pass
+
+    def bind_to(self, p, nl):
+        call("agpu_potential_bind_to", self._h, p._h, nl._h)
+
+    def set_Coulomb(self, coul):
+        raise AgpuError(_lib.ERR_UNSUPPORTED, "set_Coulomb: this class has no charge coupling")
+
+    def energy_and_forces(self, p, nl, forces=None, charges=None, epot_per_at=False,
+                          epot_per_bond=False, f_per_bond=False, wpot_per_at=False,
+                          wpot_per_bond=False, mask=None):
+        """Returns (epot, forces, wpot, epot_per_at, epot_per_bond, f_per_bond,
+        wpot_per_at, wpot_per_bond) as potential.c:622-657 does."""
+        nat = p.nat
+        if forces is None:
+            forces = np.zeros((nat, 3))
+        elif forces.shape != (nat, 3):
+            raise RuntimeError("Length of force array (= %d) does not equal number of atoms (= %d)."
+                               % (len(forces), nat))
+        if not (forces.flags.c_contiguous and forces.dtype == np.float64):
+            raise ValueError("forces must be a C-contiguous float64 array")
+        if mask is not None:
+            mask = np.ascontiguousarray(mask, dtype=np.int32)
+            if len(mask) != nat:
+                raise RuntimeError("Length of mask array (= %d) does not equal number of atoms (= %d)."
+                                   % (len(mask), nat))
+        if epot_per_bond or f_per_bond or wpot_per_bond:
+            raise AgpuError(_lib.ERR_UNSUPPORTED, "per-bond properties are not supported")
+        epot = ctypes.c_double(0.0)
+        wpot = np.zeros(9)
+        epa = np.zeros(nat) if epot_per_at else None
+        wpa = np.zeros(9 * nat) if wpot_per_at else None
+        q = np.ascontiguousarray(charges, dtype=np.float64) if charges is not None else None
+        call("agpu_potential_energy_and_forces", self._h, p._h, nl._h, _ptr(q),
+             ctypes.byref(epot), _ptr(forces), _ptr(wpot), _ptr(mask), _ptr(epa), None, None,
+             _ptr(wpa), None, _lib.HOST)
+        w = wpot.reshape(3, 3, order="F").copy()
+        wpa_out = wpa.reshape(nat, 3, 3).transpose(0, 2, 1).copy() if wpot_per_at else None
+        return epot.value, forces, w, epa, None, None, wpa_out, None
+
+    def energy_and_forces_device(self, p, nl, f, epot, wpot, charges=None, mask=None,
+                                 epot_per_at=None, wpot_per_at=None):
+        """Device variant: every array is a torch CUDA tensor (f (nat,3) and
+        wpot (9,) float64 accumulated, epot (1,) float64 set)."""
+        call("agpu_potential_energy_and_forces", self._h, p._h, nl._h, _dev(charges), _dev(epot),
+             _dev(f), _dev(wpot), _dev(mask), _dev(epot_per_at), None, None, _dev(wpot_per_at),
+             None, 0)
+
+
+def _tersoff_struct(params):
+    s = _lib.TersoffParams()
+    el = params["el"]
+    nel = len(el)
+    if not 1 <= nel <= 3:
+        raise AgpuError(_lib.ERR_ARG, "Tersoff: 1 to 3 elements supported")
+    s.nel = nel
+    for k, sym in enumerate(el):
+        if sym not in ATOMIC_NUMBERS:
+            raise AgpuError(_lib.ERR_ELEMENT, "Unknown element '%s'." % sym)
+        s.Z[k] = ATOMIC_NUMBERS[sym]
+    npairs = nel * (nel + 1) // 2
+    defaults = {"A": 1.0, "B": 1.0, "xi": 1.0, "lambda": 1.0, "mu": 1.0, "omega": 1.0,
+                "mubo": 0.0, "r1": 1.0, "r2": 2.0}
+    for k, dflt in defaults.items():
+        arr = getattr(s, "lambda_" if k == "lambda" else k)
+        vals = list(params.get(k, [dflt] * npairs))
+        for q in range(6):
+            arr[q] = float(vals[q]) if q < len(vals) else dflt
+    mv = list(params.get("m", [1] * npairs))
+    for q in range(6):
+        s.m[q] = int(mv[q]) if q < len(mv) else 1
+    for k in ("beta", "n", "c", "d", "h"):
+        arr = getattr(s, k)
+        vals = list(params.get(k, [1.0] * nel))
+        for q in range(3):
+            arr[q] = float(vals[q]) if q < len(vals) else 1.0
+    return s
+
+
+class Tersoff(_Potential):
+    """Tersoff (tersoff.f90, classname Tersoff).  Keyword arguments are the
+    ptrdict names of tersoff_registry.f90:22-115 (el, A, B, xi, lambda, mu,
+    omega, mubo, m, beta, n, c, d, h, r1, r2); without arguments the
+    Tersoff_PRB_39_5566_SiC database is used."""
+    kind = _lib.KIND_TERSOFF
+
+    def __init__(self, **kwargs):
+        super().__init__()
+        h = ctypes.c_void_p()
+        if kwargs:
+            from .parameters import Tersoff_PRB_39_5566_Si_C
+            params = dict(Tersoff_PRB_39_5566_Si_C)
+            params.update(kwargs)
+            self._params = params
+            s = _tersoff_struct(params)
+            call("agpu_tersoff_create", ctypes.byref(h), ctypes.byref(s))
+        else:
+            self._params = None
+            call("agpu_tersoff_create", ctypes.byref(h), None)
+        self._h = h
+
+    def set_params(self, **kwargs):
+        from .parameters import Tersoff_PRB_39_5566_Si_C
+        params = dict(self._params or Tersoff_PRB_39_5566_Si_C)
+        params.update(kwargs)
+        self._params = params
+        s = _tersoff_struct(params)
+        call("agpu_tersoff_set_params", self._h, ctypes.byref(s))
+
+
+class TabulatedAlloyEAM(_Potential):
+    """TabulatedAlloyEAM (tabulated_alloy_eam.f90): fn = setfl file,
+    elements = filter string."""
+    kind = _lib.KIND_EAM_ALLOY
+
+    def __init__(self, fn, elements="*"):
+        super().__init__()
+        h = ctypes.c_void_p()
+        call("agpu_eam_alloy_create", ctypes.byref(h), str(fn).encode(), str(elements).encode())
+        self._h = h
+
+
+class Ewald(_Potential):
+    """Ewald Coulomb solver (Coulomb flavour: energy_and_forces(p, nl, q, f),
+    potential(p, nl, q, phi)).  beta <= 0: beta^2 = ln(10)*12/cutoff^2 as
+    pme.f90:192; kmax <= 0: exp(-kmax^2/4beta^2) = 1e-13."""
+    kind = _lib.KIND_EWALD
+
+    def __init__(self, cutoff=10.0, beta=0.0, kmax=0.0):
+        super().__init__()
+        h = ctypes.c_void_p()
+        call("agpu_ewald_create", ctypes.byref(h), float(cutoff), float(beta), float(kmax))
+        self._h = h
+        self.cutoff = cutoff
+
+    def energy_and_forces(self, p, nl, q, f=None):
+        nat = p.nat
+        if f is None:
+            f = np.zeros((nat, 3))
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        epot = ctypes.c_double(0.0)
+        wpot = np.zeros(9)
+        call("agpu_potential_energy_and_forces", self._h, p._h, nl._h, _ptr(q), ctypes.byref(epot),
+             _ptr(f), _ptr(wpot), None, None, None, None, None, None, _lib.HOST)
+        return epot.value, f, wpot.reshape(3, 3, order="F").copy()
+
+    def potential(self, p, nl, q, phi=None):
+        nat = p.nat
+        if phi is None:
+            phi = np.zeros(nat)
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        call("agpu_coulomb_potential", self._h, p._h, nl._h, _ptr(q), _ptr(phi), _lib.HOST)
+        return phi
+
+
+def use_torch_stream():
+    """Launch every kernel on torch's current stream of the current device,
+    so library work is ordered with torch ops on the same tensors."""
+    import torch
+    call("agpu_set_stream", ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+
+
+def profile_enable(on=True):
+    call("agpu_profile_enable", 1 if on else 0)
+
+
+def profile_reset():
+    call("agpu_profile_reset")
+
+
+def profile_query(kernel):
+    """(total ms, launches) of one kernel, from HIP events on its launch stream."""
+    t = ctypes.c_double()
+    n = ctypes.c_int()
+    call("agpu_profile_query", kernel.encode(), ctypes.byref(t), ctypes.byref(n))
+    return t.value, n.value
+
+
+def from_atoms(positions, cell, pbc, numbers):
+    """Build a Particles object from plain arrays (native.from_atoms analogue)."""
+    p = Particles()
+    p.allocate(len(positions))
+    p.set_cell(cell, pbc)
+    p.Z[:] = numbers
+    p.coordinates[:, :] = positions
+    p.I_changed_positions()
+    p.update_elements()
+    return p
+
+
+def tersoff_struct(params):
+    return _tersoff_struct(params)
+
+
+__all__ = ["Particles", "Neighbors", "Tersoff", "TabulatedAlloyEAM", "Ewald", "from_atoms",
+           "pair_index"]

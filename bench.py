@@ -1,0 +1,282 @@
+#!/usr/bin/env python
+"""Throughput of the potential-evaluation path on one MI355X.
+
+A step is one pass of the hot path over one batch of synthetic input: the
+neighbour list is rebuilt (positions are flagged as changed, as the ASE
+calculator does on every call) and energy, forces and virial are evaluated,
+with positions, forces and virial resident in HBM.  The default workload is
+the tabulated-EAM Cu system (40^3 fcc cells, 256 000 atoms, rc = 5.5 A,
+Finnis-Sinclair-form functions on 10^4-point splines) in FP64.
+
+Prints one JSON line.  Per-kernel durations come from HIP events recorded
+around every launch on the library's launch stream during a profiled pass of
+the same steps right after the timed region.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "atom-evals/s (E+F+virial), FP64 plain vs FP32/FP64-acc batched at force err≤1e-5"
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def build_workload(name):
+    from atomistica_amd import workloads as W
+    if name == "eam_cu_256k":
+        pos, cell, pbc, Z = W.fcc_cu(40, sigma=0.08, seed=2)
+        d = tempfile.mkdtemp()
+        fn = os.path.join(d, "fs_cu.eam.alloy")
+        with open(fn, "w") as f:
+            f.write(W.fs_cu_setfl_text())
+        desc = ("EAM Cu fcc 40x40x40 cells = 256000 atoms, a=3.615, sigma=0.08 (rng 2), "
+                "periodic; Finnis-Sinclair-form rho/phi/F on 10000-point splines, rc=5.5")
+        return dict(pos=pos, cell=cell, pbc=pbc, Z=Z, kind="eam", fn=fn, avgn=100, desc=desc)
+    if name in ("tersoff_si_1m", "tersoff_si_4096"):
+        n = 50 if name == "tersoff_si_1m" else 8
+        pos, cell, pbc, Z = W.diamond_si(n, sigma=0.05, seed=1)
+        desc = ("Tersoff Si (PRB 39, 5566) diamond %dx%dx%d cells = %d atoms, a=5.431, "
+                "sigma=0.05 (rng 1), periodic" % (n, n, n, len(pos)))
+        return dict(pos=pos, cell=cell, pbc=pbc, Z=Z, kind="tersoff", avgn=16, desc=desc)
+    if name == "ewald_nacl_216k":
+        pos, cell, pbc, Z, q = W.rocksalt(30, sigma=0.1, seed=5)
+        beta = W.ewald_beta(10.0, 1e-8)
+        desc = ("Ewald rocksalt +-1e 30^3 cells = 216000 ions, a=5.64, sigma=0.1 (rng 5), "
+                "rc=10, erfc(beta rc)=1e-8, exp(-kmax^2/4beta^2)=1e-8")
+        return dict(pos=pos, cell=cell, pbc=pbc, Z=Z, q=q, kind="ewald", beta=beta,
+                    kmax=W.ewald_kmax(beta, 1e-8), avgn=220, desc=desc)
+    raise SystemExit("unknown workload " + name)
+
+
+def make_potential(N, wl, precision):
+    if wl["kind"] == "eam":
+        pot = N.TabulatedAlloyEAM(fn=wl["fn"])
+    elif wl["kind"] == "tersoff":
+        pot = N.Tersoff()
+    else:
+        pot = N.Ewald(cutoff=10.0, beta=wl["beta"], kmax=wl["kmax"])
+    pot.set_precision(precision)
+    return pot
+
+
+# algorithmic HBM bytes per atom of one launch: positions r (24), species (4),
+# row offset (4), neighbour row (8 per entry: index + packed image), forces (24)
+def algorithmic_bytes(kernel, nn):
+    if kernel.startswith("eam_force"):
+        return 64 + 8 * nn            # + F'(rho) of the density pass (8)
+    if kernel.startswith("eam_density"):
+        return 40 + 8 * nn            # writes F'(rho) (8)
+    if kernel.startswith("tersoff"):
+        return 56 + 8 * nn
+    if kernel == "nl_fill":
+        return 40 + 8 * nn
+    if kernel == "nl_count":
+        return 36
+    if kernel.startswith("ewald_real"):
+        return 64 + 8 * nn
+    return None
+
+
+class Runner:
+    def __init__(self, wl, precision):
+        import torch
+        from atomistica_amd import native as N
+        self.torch = torch
+        self.N = N
+        N.use_torch_stream()
+        dev = torch.device("cuda:0")
+        self.wl = wl
+        nat = len(wl["pos"])
+        self.nat = nat
+        self.p = N.from_atoms(wl["pos"], wl["cell"], wl["pbc"], wl["Z"])
+        self.r = torch.tensor(wl["pos"], dtype=torch.float64, device=dev).contiguous()
+        self.p.set_positions_device(self.r)
+        self.nl = N.Neighbors(wl["avgn"])
+        self.pot = make_potential(N, wl, precision)
+        self.pot.bind_to(self.p, self.nl)
+        self.f = torch.zeros((nat, 3), dtype=torch.float64, device=dev)
+        self.epot = torch.zeros(1, dtype=torch.float64, device=dev)
+        self.wpot = torch.zeros(9, dtype=torch.float64, device=dev)
+        self.q = (torch.tensor(wl["q"], dtype=torch.float64, device=dev) if "q" in wl else None)
+
+    def step(self, rebuild=True):
+        if rebuild:
+            self.p.changed_positions()
+        self.f.zero_()
+        self.wpot.zero_()
+        self.pot.energy_and_forces_device(self.p, self.nl, self.f, self.epot, self.wpot, charges=self.q)
+
+    def timed(self, steps, rebuild=True):
+        torch = self.torch
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            self.step(rebuild)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+
+def cpu_baseline(wl, budget_s=12.0):
+    """The CPU oracle (a restatement of the reference algorithm) on the host
+    cores: neighbour-list rebuild + energy/forces/virial per step."""
+    import numpy as np
+    from oracle import oracle as O
+    from atomistica_amd import parameters as P
+    threads = max(1, min(16, os.cpu_count() or 1))
+    pos, cell, pbc = wl["pos"], wl["cell"], wl["pbc"]
+    nat = len(pos)
+    if wl["kind"] == "eam":
+        tables = O.read_setfl_tables(wl["fn"])
+        el = np.zeros(nat, np.int32)
+
+        def one():
+            nl = O.nl_build(pos, cell, pbc, tables["cutoff"], nthreads=threads)
+            O.eam(tables, pos, el, cell, nl, nthreads=threads)
+        sample_atoms = nat
+    elif wl["kind"] == "tersoff":
+        el = np.ones(nat, np.int32)
+
+        def one():
+            nl = O.nl_build(pos, cell, pbc, 3.0, nthreads=threads)
+            O.tersoff(P.Tersoff_PRB_39_5566_Si_C, pos, el, cell, nl, nthreads=threads)
+        sample_atoms = nat
+    else:
+        return None
+    t0 = time.perf_counter()
+    one()                      # warm-up (page faults, thread pool)
+    warm = time.perf_counter() - t0
+    n = max(1, min(10, int(budget_s / max(warm, 1e-3))))
+    t0 = time.perf_counter()
+    for _ in range(n):
+        one()
+    dt = time.perf_counter() - t0
+    return {"value": sample_atoms * n / dt, "unit": "atom-evals/s", "cores": threads,
+            "kind": "port",
+            "sample": "%d full steps (NL rebuild + E+F+virial) of the same %d-atom system, "
+                      "OpenMP %d threads, C restatement of the reference loops" % (n, sample_atoms, threads)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="eam_cu_256k")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only (single-GPU boxes; no collective path "
+              "is built); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from atomistica_amd import native as N
+
+    wl = build_workload(args.workload)
+    run = Runner(wl, args.precision)
+    nat = run.nat
+    for _ in range(args.warmup):
+        run.step()
+    dt = run.timed(args.steps)
+    value = nat * args.steps / dt
+    total, maxrow = run.nl.get_size()
+    nn = total / nat
+
+    # force pass with the list reused
+    dt_force = run.timed(args.steps, rebuild=False)
+    f64 = run.f.cpu().numpy().copy()
+    e64 = float(run.epot.cpu()[0])
+    w64 = run.wpot.cpu().numpy().copy()
+
+    # profiled pass: per-kernel HIP-event durations on the launch stream
+    N.profile_reset()
+    N.profile_enable(True)
+    for _ in range(args.steps):
+        run.step()
+    torch.cuda.synchronize()
+    N.profile_enable(False)
+    names = ["nl_bin", "nl_cell_start", "nl_gather", "nl_count", "nl_fill", "nl_offsets_tail",
+             "eam_density_fp64", "eam_force_fp64", "eam_density_mixed", "eam_force_mixed",
+             "bop_bonds", "bop_rev", "tersoff_fp64", "tersoff_mixed", "bop_gather",
+             "ewald_real_fp64", "ewald_real_mixed", "ewald_frac", "ewald_sk", "ewald_kenergy",
+             "ewald_kforce", "reduce_ev", "map_elements"]
+    kern = {}
+    for k in names:
+        ms, cnt = N.profile_query(k)
+        if cnt:
+            kern[k] = {"avg_us": 1e3 * ms / cnt, "launches": cnt}
+    dom = max(kern, key=lambda k: kern[k]["avg_us"] * kern[k]["launches"]) if kern else None
+    roof = None
+    if dom:
+        bpa = algorithmic_bytes(dom, nn)
+        traffic = None
+        tf = os.path.join(ROOT, "profiles", "traffic.json")
+        if os.path.exists(tf):
+            try:
+                tj = json.load(open(tf))
+                ent = tj.get(args.workload, {}).get(dom)
+                if ent:
+                    traffic = ent["bytes_per_launch"]
+            except Exception:
+                traffic = None
+        if bpa is not None:
+            achieved = bpa * nat / (kern[dom]["avg_us"] * 1e-6) / 1e9
+            roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
+                    "algorithmic_bytes_per_atom": bpa, "atoms_per_launch": nat}
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "atom-evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * dt / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32-arith/f64-acc",
+        "data": "synthetic",
+        "config": {"workload": wl["desc"], "atoms": nat, "neighbors_per_atom": nn,
+                   "step": "neighbour-list rebuild + energy, forces, virial (FP64)",
+                   "inputs": "positions resident in HBM"},
+        "force_only": {"value": nat * args.steps / dt_force, "ms_per_step": 1e3 * dt_force / args.steps},
+        "kernels": kern,
+        "roofline": roof,
+    }
+
+    if not args.no_mixed and args.precision == "fp64" and wl["kind"] in ("eam", "tersoff", "ewald"):
+        run.pot.set_precision("mixed")
+        for _ in range(args.warmup):
+            run.step()
+        dtm = run.timed(args.steps)
+        fm = run.f.cpu().numpy()
+        em = float(run.epot.cpu()[0])
+        out["mixed"] = {
+            "value": nat * args.steps / dtm, "ms_per_step": 1e3 * dtm / args.steps,
+            "dtype": "f32-arith/f64-acc",
+            "speedup_vs_fp64": (nat * args.steps / dtm) / value,
+            "force_rms_err": float(np.sqrt(np.mean((fm - f64) ** 2))),
+            "force_max_err": float(np.abs(fm - f64).max()),
+            "energy_err_per_atom": abs(em - e64) / nat,
+        }
+        run.pot.set_precision("fp64")
+
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(wl)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,208 @@
+/*
+ * atomistica_amd -- C-ABI of the MI355X (gfx950) potential-evaluation path.
+ *
+ * The ABI mirrors the per-class function table that the reference generates
+ * for its Python and LAMMPS front ends, so a host that drives Atomistica
+ * through that table can drive this library the same way:
+ *
+ *   reference potential_class_t        src/python/c/factory.template.h:82-98
+ *   reference LAMMPS class table       src/lammps/factory.template.h:37-49
+ *   reference Coulomb class table      src/python/gen_factory.py:570-615
+ *
+ * Conventions (identical to the reference's Fortran memory images):
+ *   - positions r[3*i + x], forces f[3*i + x]                    (r(3,nat), f(3,nat))
+ *   - cell[3*k + x] = component x of lattice vector k            (Abox(3,3))
+ *   - wpot[a + 3*b] = W(a,b); wpot_per_at[9*i + a + 3*b]         (wpot(3,3), wpot_per_at(3,3,nat))
+ *   - neighbour images: dr_ij = r_i - r_j + Abox.dc              (src/macros.inc:76)
+ *   - atom and neighbour indices are 0-based in every array here.
+ *   - f, wpot and per-atom outputs are accumulated (+=); *epot is set to
+ *     the energy of the call (the generated wrapper zeroes it first,
+ *     src/python/gen_factory.py:283).
+ *   - every call returns AGPU_OK (0) or an error code; the message of the
+ *     last error on the calling thread is returned by agpu_get_error().
+ *   - array arguments are device (HBM) pointers unless the call's flags
+ *     contain AGPU_HOST; then they are host pointers staged through HBM.
+ *   - calls are synchronous on the library stream (agpu_set_stream) and not
+ *     re-entrant on one handle, like the reference (src/support/tls.f90:45-49).
+ */
+#ifndef ATOMISTICA_AMD_H
+#define ATOMISTICA_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- return codes (reference error cases: error.inc:58-69) ---- */
+#define AGPU_OK 0
+#define AGPU_ERR_ARG 1            /* invalid argument / shape */
+#define AGPU_ERR_HIP 2            /* HIP runtime failure */
+#define AGPU_ERR_NL_OVERFLOW 3    /* python_neighbors.f90:713-718 */
+#define AGPU_ERR_DOMAIN 4         /* python_neighbors.f90:944-948; image shift out of range */
+#define AGPU_ERR_SPLINE 5         /* simple_spline.f90:426-429 */
+#define AGPU_ERR_UNSUPPORTED 6    /* optional argument not supported: gen_factory.py:303-304 */
+#define AGPU_ERR_ELEMENT 7        /* default_bind_to_func.f90:83 */
+#define AGPU_ERR_PBC 8            /* pme.f90:185-187 */
+#define AGPU_ERR_IO 9             /* table file unreadable */
+#define AGPU_ERR_STATE 10         /* handle not initialised / not bound */
+#define AGPU_ERR_NEB 11           /* bond buffer exhausted: bop_kernel.f90:672-678 */
+
+/* ---- flags ---- */
+#define AGPU_HOST 1               /* array arguments are host pointers */
+
+/* ---- precision modes ---- */
+#define AGPU_PREC_FP64 0          /* FP64 arithmetic and accumulation */
+#define AGPU_PREC_MIXED 1         /* FP32 pair/triplet arithmetic, FP64 accumulation */
+
+/* ---- potential kinds ---- */
+#define AGPU_KIND_TERSOFF 1
+#define AGPU_KIND_EAM_ALLOY 2
+#define AGPU_KIND_EWALD 3
+
+typedef struct agpu_particles agpu_particles;
+typedef struct agpu_neighbors agpu_neighbors;
+typedef struct agpu_potential agpu_potential;
+
+/* Internal argument pack (not used across the ABI). */
+typedef struct agpu_eval_args {
+    const double *q;
+    double *epot;
+    double *f;
+    double *wpot;
+    const int *mask;
+    double *epot_per_at;
+    double *wpot_per_at;
+    int flags;
+} agpu_eval_args;
+
+/* ======================================================================
+ * library / device
+ * ====================================================================== */
+const char *agpu_version(void);
+/* Copies the last error message of this thread into buf (NUL-terminated).
+ * Replaces get_full_error_string (src/lammps/lammps_particles.f90:404-425)
+ * and error_to_py (src/python/c/py_f.c:36). */
+int agpu_get_error(char *buf, int len);
+int agpu_device_count(int *n);
+int agpu_set_device(int device);
+/* Use an existing hipStream_t (passed as void*) for every launch; NULL
+ * selects the library's own stream. */
+int agpu_set_stream(void *stream);
+int agpu_get_stream(void **stream);
+int agpu_synchronize(void);
+int agpu_malloc(void **ptr, size_t bytes);
+int agpu_free(void *ptr);
+/* kind: 0 host->device, 1 device->host, 2 device->device */
+int agpu_memcpy(void *dst, const void *src, size_t bytes, int kind);
+/* Per-kernel HIP-event timing on the launch stream (0 = off). */
+int agpu_profile_enable(int on);
+int agpu_profile_reset(void);
+/* Sum of event-measured durations (ms) and launch count of one kernel. */
+int agpu_profile_query(const char *kernel, double *total_ms, int *count);
+
+/* ======================================================================
+ * particles -- python_particles.f90:84-170 (particles_t),
+ *              lammps_particles.f90:273-298 (set_pointers)
+ * ====================================================================== */
+int agpu_particles_create(agpu_particles **p);
+int agpu_particles_destroy(agpu_particles *p);
+/* particles_allocate: device storage for nat atoms */
+int agpu_particles_allocate(agpu_particles *p, int nat);
+/* particles_set_cell (python_particles.f90:286-346): cell rows = lattice
+ * vectors, pbc[3] nonzero = periodic.  Bbox = cell^-1 by partial-pivot LU. */
+int agpu_particles_set_cell(agpu_particles *p, const double *cell, const int *pbc);
+/* Z[nat] atomic numbers (host or device per flags) -- update_elements
+ * (python_particles.f90:617-658) */
+int agpu_particles_set_atomic_numbers(agpu_particles *p, const int *Z, int flags);
+/* copies r(3,nat) and marks positions changed (I_changed_positions) */
+int agpu_particles_set_positions(agpu_particles *p, const double *r, int flags);
+/* aliases caller-owned device positions r(3,nat) (LAMMPS-style
+ * particles_set_pointers); the caller calls agpu_particles_changed_positions
+ * after modifying them in place */
+int agpu_particles_set_position_pointer(agpu_particles *p, int nat, double *r_device);
+int agpu_particles_changed_positions(agpu_particles *p);
+int agpu_particles_get_nat(agpu_particles *p, int *nat);
+
+/* ======================================================================
+ * neighbours -- python_neighbors.f90:51-139 (neighbors_t)
+ * ====================================================================== */
+/* Neighbors(avgn): capacity nat*avgn entries (python_neighbors.f90:499) */
+int agpu_neighbors_create(agpu_neighbors **nl, int avgn);
+int agpu_neighbors_destroy(agpu_neighbors *nl);
+/* request_interaction_range (python_neighbors.f90:381-403): keeps the max */
+int agpu_neighbors_request_interaction_range(agpu_neighbors *nl, double cutoff);
+/* bin_size > 0 overrides the bin edge (default: the cutoff, :518) */
+int agpu_neighbors_set_bin_size(agpu_neighbors *nl, double bin_size);
+/* neighbors_update (python_neighbors.f90:430-456): rebuild when positions
+ * or cell changed since the last build (verlet shell 0) */
+int agpu_neighbors_update(agpu_neighbors *nl, agpu_particles *p);
+int agpu_neighbors_get_size(agpu_neighbors *nl, int64_t *total, int *maxrow);
+/* CSR export: offsets[nat+1] (int64), neighbours[total] (0-based j),
+ * dc[3*total] image shifts; row order is the reference's fill order
+ * (stencil x-major, then ascending j within a bin). */
+int agpu_neighbors_get(agpu_neighbors *nl, int64_t *offsets, int *neighbors, int *dc,
+                       int flags);
+
+/* ======================================================================
+ * potentials -- init / bind_to / energy_and_forces
+ * ====================================================================== */
+
+/* Tersoff parameter database (BOP_DB_TYPE, tersoff_params.f90:32-78).
+ * Pair-indexed arrays use the symmetric pair index of macros.inc:123
+ * (0-based): for nel elements, pair(i,j) = min(i+j*nel, j+i*nel)
+ * - min(i(i+1)/2, j(j+1)/2). */
+typedef struct agpu_tersoff_params {
+    int nel;
+    int Z[3];                    /* atomic numbers of the database elements */
+    double A[6], B[6], xi[6], lambda[6], mu[6], omega[6], mubo[6];
+    int m[6];
+    double beta[3], n[3], c[3], d[3], h[3];
+    double r1[6], r2[6];
+} agpu_tersoff_params;
+
+/* Tersoff: tersoff_init (tersoff_module.f90:30-78).  params == NULL selects
+ * Tersoff_PRB_39_5566_SiC (tersoff_params.f90:99-138). */
+int agpu_tersoff_create(agpu_potential **pot, const agpu_tersoff_params *params);
+int agpu_tersoff_set_params(agpu_potential *pot, const agpu_tersoff_params *params);
+
+/* TabulatedAlloyEAM: tabulated_alloy_eam_init (tabulated_alloy_eam.f90:147-259)
+ * reading a setfl file; elements = filter string ("*" or NULL = all). */
+int agpu_eam_alloy_create(agpu_potential **pot, const char *fn, const char *elements);
+
+/* Ewald Coulomb: real-space erfc sum and self term as pme.f90:332-484, with
+ * the reciprocal space as a direct FP64 structure-factor sum over
+ * |k| <= kmax.  beta <= 0 selects the reference rule
+ * beta^2 = ln(10)*12/cutoff^2 (pme.f90:192); kmax <= 0 selects
+ * exp(-kmax^2/(4 beta^2)) = 1e-13. */
+int agpu_ewald_create(agpu_potential **pot, double cutoff, double beta, double kmax);
+
+int agpu_potential_destroy(agpu_potential *pot);
+int agpu_potential_kind(agpu_potential *pot, int *kind);
+int agpu_potential_set_precision(agpu_potential *pot, int precision);
+int agpu_potential_get_cutoff(agpu_potential *pot, double *cutoff);
+
+/* bind_to (default_bind_to_func.f90:25-146, tabulated_alloy_eam.f90:297-350,
+ * pme.f90:174-207): element map, interaction range request on nl. */
+int agpu_potential_bind_to(agpu_potential *pot, agpu_particles *p, agpu_neighbors *nl);
+
+/* energy_and_forces: the argument list of potential_class_t.energy_and_forces
+ * (factory.template.h:92-94).  q is used by Coulomb classes only; per-bond
+ * arrays must be NULL (AGPU_ERR_UNSUPPORTED otherwise). */
+int agpu_potential_energy_and_forces(agpu_potential *pot, agpu_particles *p,
+                                     agpu_neighbors *nl, const double *q, double *epot,
+                                     double *f, double *wpot, const int *mask,
+                                     double *epot_per_at, double *epot_per_bond,
+                                     double *f_per_bond, double *wpot_per_at,
+                                     double *wpot_per_bond, int flags);
+
+/* Coulomb potential phi_i = dE/dq_i (pme_potential, pme.f90:215-321);
+ * phi is accumulated (+=). */
+int agpu_coulomb_potential(agpu_potential *pot, agpu_particles *p, agpu_neighbors *nl,
+                           const double *q, double *phi, int flags);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

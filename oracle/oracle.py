@@ -1,0 +1,342 @@
+"""ctypes front end of the CPU oracle.
+
+TEST INFRASTRUCTURE ONLY: imported by ``tests/``, ``__graft_entry__.smoke()``
+and the ``cpu_baseline`` leg of ``bench.py`` -- never by the product package
+``atomistica_amd``.  Every routine is a restatement of the reference Fortran
+(file:line citations in the C sources next to this file).
+
+Array conventions
+-----------------
+``r``       (nat, 3) float64 unwrapped positions
+``cell``    (3, 3) float64, row k = lattice vector k (ASE order)
+``pbc``     (3,) bool
+neighbour lists are returned as ``(offsets[nat+1] int64, nbr[int32], dc[(n,3) int32])``
+``wpot``    (3, 3) with ``wpot[a, b] = W(a, b)`` of the reference
+``wpot_per_at`` (nat, 3, 3) with ``[i, a, b] = W_i(a, b)``
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIBPATH = os.path.join(_HERE, "_build", "liboracle.so")
+_lib = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_lp = ctypes.POINTER(ctypes.c_int64)
+
+
+class OracleError(RuntimeError):
+    pass
+
+
+def build():
+    """Compile the oracle with its Makefile (gcc, -ffp-contract=off)."""
+    import subprocess
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIBPATH):
+            raise OracleError("oracle not built: run `make -C oracle` (or __graft_entry__.build())")
+        _lib = ctypes.CDLL(_LIBPATH)
+    return _lib
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+def _l(a):
+    return a.ctypes.data_as(_lp)
+
+
+def _c(a, dtype):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+def _check(err, what):
+    if err != 0:
+        raise OracleError("%s failed with code %d" % (what, err))
+
+
+def inverse_cell(cell):
+    cell = _c(cell, np.float64)
+    out = np.zeros(9)
+    _check(lib().orc_inverse_cell(_d(cell), _d(out)), "inverse_cell")
+    # out holds the column-major image of Bbox = Abox^-1
+    return out.reshape(3, 3).T.copy()
+
+
+def nl_geometry(cell, pbc, cutoff, bin_size=0.0):
+    cell = _c(cell, np.float64)
+    pbc_ = _c(np.asarray(pbc, dtype=bool).astype(np.int32), np.int32)
+    nc = np.zeros(3, np.int32)
+    nd = np.zeros(3, np.int32)
+    lib().orc_nl_geometry(_d(cell), _i(pbc_), ctypes.c_double(cutoff),
+                          ctypes.c_double(bin_size), _i(nc), _i(nd))
+    return nc, nd
+
+
+def nl_build(r, cell, pbc, cutoff, bin_size=0.0, avgn=100, nthreads=1):
+    r = _c(r, np.float64)
+    nat = len(r)
+    cell = _c(cell, np.float64)
+    pbc_ = _c(np.asarray(pbc, dtype=bool).astype(np.int32), np.int32)
+    cap = max(nat * avgn, 1)
+    offsets = np.zeros(nat + 1, np.int64)
+    nbr = np.zeros(cap, np.int32)
+    dc = np.zeros((cap, 3), np.int32)
+    if nthreads > 1:
+        err = lib().orc_nl_build_mt(nat, _d(r), _d(cell), _i(pbc_), ctypes.c_double(cutoff),
+                                    ctypes.c_double(bin_size), ctypes.c_int64(cap),
+                                    _l(offsets), _i(nbr), _i(dc), int(nthreads))
+    else:
+        err = lib().orc_nl_build(nat, _d(r), _d(cell), _i(pbc_), ctypes.c_double(cutoff),
+                                 ctypes.c_double(bin_size), ctypes.c_int64(cap),
+                                 _l(offsets), _i(nbr), _i(dc))
+    _check(err, "nl_build (neighbour list overflow?)")
+    n = offsets[-1]
+    return offsets, nbr[:n].copy(), dc[:n].copy()
+
+
+def nl_brute(r, cell, pbc, cutoff, cap=None):
+    r = _c(r, np.float64)
+    nat = len(r)
+    cell = _c(cell, np.float64)
+    pbc_ = _c(np.asarray(pbc, dtype=bool).astype(np.int32), np.int32)
+    cap = cap or max(nat * 400, 1000)
+    offsets = np.zeros(nat + 1, np.int64)
+    nbr = np.zeros(cap, np.int32)
+    dc = np.zeros((cap, 3), np.int32)
+    err = lib().orc_nl_brute(nat, _d(r), _d(cell), _i(pbc_), ctypes.c_double(cutoff),
+                             ctypes.c_int64(cap), _l(offsets), _i(nbr), _i(dc))
+    _check(err, "nl_brute")
+    n = offsets[-1]
+    return offsets, nbr[:n].copy(), dc[:n].copy()
+
+
+# --------------------------------------------------------------------------
+# Tersoff
+# --------------------------------------------------------------------------
+
+class _TersoffParams(ctypes.Structure):
+    _fields_ = [("nel", ctypes.c_int), ("Z", ctypes.c_int * 3)] + \
+        [(k, ctypes.c_double * 6) for k in ("A", "B", "xi", "lambda_", "mu", "omega", "mubo")] + \
+        [("m", ctypes.c_int * 6)] + \
+        [(k, ctypes.c_double * 3) for k in ("beta", "n", "c", "d", "h")] + \
+        [(k, ctypes.c_double * 6) for k in ("r1", "r2")]
+
+
+def _tersoff_struct(params, Zs):
+    s = _TersoffParams()
+    nel = len(params["el"])
+    s.nel = nel
+    for i, z in enumerate(Zs):
+        s.Z[i] = z
+    npairs = nel * (nel + 1) // 2
+    defaults = {"A": 1.0, "B": 1.0, "xi": 1.0, "lambda": 1.0, "mu": 1.0,
+                "omega": 1.0, "mubo": 0.0, "r1": 1.0, "r2": 2.0}
+    for k, dflt in defaults.items():
+        arr = getattr(s, "lambda_" if k == "lambda" else k)
+        vals = params.get(k, [dflt] * npairs)
+        for i in range(6):
+            arr[i] = float(vals[i]) if i < len(vals) else dflt
+    mvals = params.get("m", [1] * npairs)
+    for i in range(6):
+        s.m[i] = int(mvals[i]) if i < len(mvals) else 1
+    for k in ("beta", "n", "c", "d", "h"):
+        arr = getattr(s, k)
+        vals = params[k]
+        for i in range(3):
+            arr[i] = float(vals[i]) if i < len(vals) else 1.0
+    return s
+
+
+def tersoff(params, r, el, cell, nl, mask=None, per_at=False, nthreads=1):
+    """Energy, forces and virial; ``el`` holds database element indices (-1 = none)."""
+    r = _c(r, np.float64)
+    nat = len(r)
+    el = _c(el, np.int32)
+    cell = _c(cell, np.float64)
+    offsets, nbr, dc = nl
+    offsets = _c(offsets, np.int64)
+    nbr = _c(nbr, np.int32)
+    dc = _c(dc, np.int32)
+    s = _tersoff_struct(params, [0, 0, 0])
+    epot = ctypes.c_double(0.0)
+    f = np.zeros((nat, 3))
+    w = np.zeros(9)
+    epa = np.zeros(nat) if per_at else None
+    wpa = np.zeros(9 * nat) if per_at else None
+    m = _c(mask, np.int32) if mask is not None else None
+    err = lib().orc_tersoff(ctypes.byref(s), nat, _d(r), _i(el), _d(cell), _l(offsets),
+                            _i(nbr), _i(dc), _i(m) if m is not None else None,
+                            ctypes.byref(epot), _d(f), _d(w),
+                            _d(epa) if per_at else None, _d(wpa) if per_at else None,
+                            int(nthreads))
+    _check(err, "tersoff")
+    wpot = w.reshape(3, 3).T.copy()
+    out = [epot.value, f, wpot]
+    if per_at:
+        out += [epa, wpa.reshape(nat, 3, 3).transpose(0, 2, 1).copy()]
+    return tuple(out)
+
+
+# --------------------------------------------------------------------------
+# EAM
+# --------------------------------------------------------------------------
+
+def spline_init(y, dx, x0=0.0):
+    y = _c(y, np.float64)
+    n = len(y)
+    coef = np.zeros(8 * n)
+    _check(lib().orc_spline_init(n, ctypes.c_double(x0), ctypes.c_double(dx), _d(y), _d(coef)),
+           "spline_init")
+    return coef
+
+
+def eam(tables, r, el, cell, nl, mask=None, per_at=False, nthreads=1):
+    """``tables``: dict from :func:`atomistica_amd.eam_tables` style data --
+    keys nel, cutoff, F (list of (y, dF)), rho (list of (y, dr)), phi (nel x nel
+    list of (y, dr)) with y already padded / scaled as the reader does."""
+    nel = tables["nel"]
+    r = _c(r, np.float64)
+    nat = len(r)
+    el = _c(el, np.int32)
+    cell = _c(cell, np.float64)
+    offsets, nbr, dc = nl
+    offsets = _c(offsets, np.int64)
+    nbr = _c(nbr, np.int32)
+    dc = _c(dc, np.int32)
+    keep = []
+
+    def pack(items):
+        ns = np.array([len(y) for y, _ in items], np.int32)
+        ds = np.array([d for _, d in items], np.float64)
+        coefs = [spline_init(y, d) for y, d in items]
+        keep.extend(coefs)
+        ptrs = (_dp * len(items))(*[_d(c) for c in coefs])
+        keep.append(ptrs)
+        keep.extend([ns, ds])
+        return ns, ds, ptrs
+
+    nF, dF, cF = pack(tables["F"])
+    nR, dR, cR = pack(tables["rho"])
+    phi_items = []
+    for a in range(nel):
+        for b in range(nel):
+            phi_items.append(tables["phi"][a][b])
+    nP, dP, cP = pack(phi_items)
+    epot = ctypes.c_double(0.0)
+    f = np.zeros((nat, 3))
+    w = np.zeros(9)
+    epa = np.zeros(nat) if per_at else None
+    wpa = np.zeros(9 * nat) if per_at else None
+    m = _c(mask, np.int32) if mask is not None else None
+    err = lib().orc_eam(nel, nat, _d(r), _i(el), _d(cell), _l(offsets), _i(nbr), _i(dc),
+                        _i(m) if m is not None else None, ctypes.c_double(tables["cutoff"]),
+                        _i(nF), _d(dF), cF, _i(nR), _d(dR), cR, _i(nP), _d(dP), cP,
+                        ctypes.byref(epot), _d(f), _d(w),
+                        _d(epa) if per_at else None, _d(wpa) if per_at else None,
+                        int(nthreads))
+    _check(err, "eam (spline argument outside its table?)")
+    out = [epot.value, f, w.reshape(3, 3).T.copy()]
+    if per_at:
+        out += [epa, wpa.reshape(nat, 3, 3).transpose(0, 2, 1).copy()]
+    return tuple(out)
+
+
+# --------------------------------------------------------------------------
+# Ewald
+# --------------------------------------------------------------------------
+
+def ewald_real(r, q, cell, nl, cutoff, beta, nthreads=1):
+    """Real-space + self term. Returns (epot, f, wpot, phi) in e^2/A units."""
+    r = _c(r, np.float64)
+    nat = len(r)
+    q = _c(q, np.float64)
+    cell = _c(cell, np.float64)
+    offsets, nbr, dc = nl
+    epot = ctypes.c_double(0.0)
+    f = np.zeros((nat, 3))
+    w = np.zeros(9)
+    phi = np.zeros(nat)
+    err = lib().orc_ewald_real(nat, _d(r), _d(q), _d(cell), _l(_c(offsets, np.int64)),
+                               _i(_c(nbr, np.int32)), _i(_c(dc, np.int32)),
+                               ctypes.c_double(cutoff), ctypes.c_double(beta),
+                               ctypes.byref(epot), _d(f), _d(w), _d(phi), int(nthreads))
+    _check(err, "ewald_real")
+    return epot.value, f, w.reshape(3, 3).T.copy(), phi
+
+
+def ewald_kspace(r, q, cell, beta, kmax, nthreads=1):
+    r = _c(r, np.float64)
+    nat = len(r)
+    q = _c(q, np.float64)
+    cell = _c(cell, np.float64)
+    epot = ctypes.c_double(0.0)
+    f = np.zeros((nat, 3))
+    w = np.zeros(9)
+    phi = np.zeros(nat)
+    err = lib().orc_ewald_kspace(nat, _d(r), _d(q), _d(cell), ctypes.c_double(beta),
+                                 ctypes.c_double(kmax), ctypes.byref(epot), _d(f), _d(w),
+                                 _d(phi), int(nthreads))
+    _check(err, "ewald_kspace")
+    return epot.value, f, w.reshape(3, 3).T.copy(), phi
+
+
+# --------------------------------------------------------------------------
+# setfl tables for eam()
+# --------------------------------------------------------------------------
+
+def read_setfl_tables(path):
+    """Tables of a setfl file in the form the oracle's eam() takes, read as
+    tabulated_alloy_eam_init does (:147-259): F from rho=0, rho(r) and r*phi(r)
+    padded with two zeros, pair tables scaled by 1/2."""
+    with open(path) as f:
+        lines = f.read().splitlines()
+    head = lines[3].split()
+    nel = int(head[0])
+    names = head[1:1 + nel]
+    nF, dF, nr, dr, cutoff = lines[4].replace(",", " ").split()[:5]
+    nF, nr = int(nF), int(nr)
+    dF, dr, cutoff = float(dF), float(dr), float(cutoff)
+    toks = []
+    pos = 5
+    F, rho, Zs = [], [], []
+
+    def take(n):
+        nonlocal pos, toks
+        out = []
+        while len(out) < n:
+            if not toks:
+                toks = lines[pos].replace(",", " ").split()
+                pos += 1
+                continue
+            out.append(float(toks.pop(0).replace("D", "e").replace("d", "e")))
+        return np.array(out)
+
+    for e in range(nel):
+        Zs.append(int(lines[pos].split()[0]))
+        pos += 1
+        toks = []
+        F.append((take(nF), dF))
+        toks = []      # one list-directed read per table: the rest of its last line is skipped
+        rho.append((np.concatenate([take(nr), [0.0, 0.0]]), dr))
+        toks = []
+    phi = [[None] * nel for _ in range(nel)]
+    for a in range(nel):
+        for b in range(a + 1):
+            y = 0.5 * np.concatenate([take(nr), [0.0, 0.0]])
+            toks = []
+            phi[a][b] = phi[b][a] = (y, dr)
+    return {"nel": nel, "cutoff": cutoff, "F": F, "rho": rho, "phi": phi, "names": names, "Z": Zs}

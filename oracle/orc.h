@@ -1,0 +1,90 @@
+/*
+ * CPU oracle for the potential-evaluation path: a plain-C FP64 restatement
+ * of the reference algorithms.  TEST INFRASTRUCTURE ONLY -- the product path
+ * (atomistica_amd) never links or calls anything here; tests, the smoke check
+ * and the bench's cpu_baseline leg use it as the checker.
+ *
+ * Conventions shared by every routine:
+ *   - cell[9] is row-major with cell[3*k + x] = component x of lattice vector
+ *     k (ASE order).  This is the memory image of the reference's Fortran
+ *     Abox(3,3) with Abox(:,k) = lattice vector k
+ *     (src/python/f90/particles_wrap.f90:147-166).
+ *   - r[3*i + x] are unwrapped positions (the reference's r_non_cyc).
+ *   - neighbour lists are CSR: offsets[nat+1], nbr[j], dc[3*slot]; the image
+ *     convention follows src/macros.inc:76: dr_ij = r_i - r_j + Abox.dc.
+ *
+ * Compiled with -ffp-contract=off so that sums and products round exactly as
+ * written, in the order the reference's Fortran writes them.
+ */
+#ifndef ORC_H
+#define ORC_H
+#include <stdint.h>
+
+#define ORC_OK 0
+#define ORC_ERR_OVERFLOW 1
+#define ORC_ERR_DOMAIN 2
+#define ORC_ERR_SPLINE 3
+#define ORC_ERR_ARG 4
+#define ORC_ERR_SINGULAR 5
+#define ORC_ERR_NEB 6
+
+/* ---- cell / neighbour list ---- */
+int orc_inverse_cell(const double *cell, double *bbox);
+int orc_nl_geometry(const double *cell, const int *pbc, double cutoff,
+                    double bin_size, int *n_cells, int *n_stencil);
+int orc_nl_build(int nat, const double *r, const double *cell, const int *pbc,
+                 double cutoff, double bin_size, int64_t cap,
+                 int64_t *offsets, int *nbr, int *dc);
+int orc_nl_build_mt(int nat, const double *r, const double *cell, const int *pbc,
+                    double cutoff, double bin_size, int64_t cap,
+                    int64_t *offsets, int *nbr, int *dc, int nthreads);
+int orc_nl_brute(int nat, const double *r, const double *cell, const int *pbc,
+                 double cutoff, int64_t cap, int64_t *offsets, int *nbr,
+                 int *dc);
+
+/* ---- Tersoff ---- */
+typedef struct {
+    int nel;
+    int Z[3];
+    double A[6], B[6], xi[6], lambda[6], mu[6], omega[6], mubo[6];
+    int m[6];
+    double beta[3], n[3], c[3], d[3], h[3];
+    double r1[6], r2[6];
+} orc_tersoff_params;
+
+int orc_tersoff(const orc_tersoff_params *p, int nat, const double *r,
+                const int *el, const double *cell, const int64_t *offsets,
+                const int *nbr, const int *dc, const int *mask,
+                double *epot, double *f, double *wpot, double *epot_per_at,
+                double *wpot_per_at, int nthreads);
+
+/* ---- splines / EAM ---- */
+typedef struct {
+    int n;
+    double x0, dx, cut;
+    double *y, *d2y, *c1, *c2, *c3, *dc1, *dc2, *dc3;
+} orc_spline;
+
+int orc_spline_init(int n, double x0, double dx, const double *y,
+                    double *coef /* 8*n doubles: y,d2y,c1,c2,c3,dc1,dc2,dc3 */);
+int orc_eam(int nel, int nat, const double *r, const int *el,
+            const double *cell, const int64_t *offsets, const int *nbr,
+            const int *dc, const int *mask, double cutoff,
+            const int *n_F, const double *dF, const double *const *coef_F,
+            const int *n_rho, const double *drho,
+            const double *const *coef_rho, const int *n_phi,
+            const double *dphi, const double *const *coef_phi,
+            double *epot, double *f, double *wpot, double *epot_per_at,
+            double *wpot_per_at, int nthreads);
+
+/* ---- Ewald ---- */
+int orc_ewald_real(int nat, const double *r, const double *q,
+                   const double *cell, const int64_t *offsets, const int *nbr,
+                   const int *dc, double cutoff, double beta, double *epot,
+                   double *f, double *wpot, double *phi, int nthreads);
+int orc_ewald_kspace(int nat, const double *r, const double *q,
+                     const double *cell, double beta, double kmax,
+                     double *epot, double *f, double *wpot, double *phi,
+                     int nthreads);
+
+#endif

@@ -1,0 +1,237 @@
+"""The oracle's Tersoff / EAM / Ewald restatements against what the
+mathematics and the reference's tests fix.
+
+Tersoff: perfect-diamond closed form (4 bonds at a*sqrt(3)/4, cos = -1/3),
+Si(100) unrelaxed surface energy 2.309 J/m^2 (tests/test_pbc.py:55-59),
+central finite differences of forces and strain (atomistica/tests.py:44-120),
+mask decomposition (tests/test_mask.py:38-63), per-atom sums.
+EAM: perfect-fcc closed form of the analytic functions (through a 10^4-point
+spline), finite differences, a spline that must reproduce sin(x).
+Ewald: the rocksalt Madelung constant, beta independence, finite
+differences of forces, strain and charges (phi = dE/dq, tests.py:122-146).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from atomistica_amd import parameters as P
+from atomistica_amd import workloads as W
+from helpers import fd_forces, fd_virial, read_cfg
+
+SI = P.Tersoff_Si
+
+
+def tersoff_E(pos, cell, pbc, params=SI, el=None, mask=None):
+    nl = O.nl_build(pos, cell, pbc, max(params["r2"]))
+    el = np.zeros(len(pos), np.int32) if el is None else el
+    return O.tersoff(params, pos, el, cell, nl, mask=mask)
+
+
+def test_tersoff_diamond_closed_form():
+    # independent closed form of a perfect diamond crystal
+    a = 5.431
+    r = a * math.sqrt(3) / 4
+    A, B, lam, mu = 1830.8, 471.18, 2.4799, 1.7322
+    beta, n, c, d, h = 1.1e-6, 0.78734, 1.0039e5, 16.217, -0.59825
+    g = 1 + c * c / (d * d) - c * c / (d * d + (h + 1.0 / 3.0) ** 2)
+    zeta = 3 * g
+    b = (1 + beta ** n * zeta ** n) ** (-1 / (2 * n))
+    e_atom = 2 * (A * math.exp(-lam * r) - b * B * math.exp(-mu * r))
+    assert abs(e_atom - (-4.629593127667)) < 1e-11      # value quoted for the build
+    pos, cell, pbc, Z = W.diamond_si(3, sigma=0.0)
+    e, f, w = tersoff_E(pos, cell, pbc)
+    assert abs(e / len(pos) - e_atom) < 1e-12
+    assert np.abs(f).max() < 1e-10
+
+
+def test_tersoff_si100_surface_energy():
+    # tests/test_pbc.py:41-59 (Diamond Si a=5.432, 2x2x2, surface along z)
+    pos, cell, pbc, Z = W.diamond_si(2, a=5.432, sigma=0.0)
+    e1 = tersoff_E(pos, cell, pbc)[0]
+    e2 = tersoff_E(pos, cell, np.array([True, True, False]))[0]
+    cell3 = cell.copy()
+    cell3[2, 2] *= 2
+    e3 = tersoff_E(pos, cell3, pbc)[0]
+    assert e2 == e3
+    Jm2 = 1.602176634e-19 * 1e20
+    esurf = (e2 - e1) / (2 * cell[0, 0] * cell[1, 1]) * Jm2
+    assert abs(esurf - 2.309) < 1e-3
+
+
+def test_tersoff_fd_forces_and_virial_aC():
+    pos, cell, pbc, Z = read_cfg("aC_small.cfg")
+    prm = P.Tersoff_PRB_39_5566_Si_C
+    el = np.zeros(len(pos), np.int32)          # all carbon
+    pos = pos + np.random.default_rng(7).normal(0, 0.05, pos.shape)
+    e, f, w = tersoff_E(pos, cell, pbc, prm, el)
+    fd = fd_forces(lambda p: tersoff_E(p, cell, pbc, prm, el)[0], pos, range(0, 128, 9))
+    assert np.max(np.abs(fd - f[::9][: len(fd)])) < 1e-6
+    Wfd = fd_virial(lambda p, c: tersoff_E(p, c, pbc, prm, el)[0], pos, cell)
+    assert np.max(np.abs(Wfd - w)) < 1e-5 * max(1.0, np.abs(w).max())
+    assert np.abs(f.sum(axis=0)).max() < 1e-9
+
+
+def test_tersoff_sic_mixed_species():
+    rng = np.random.default_rng(11)
+    pos, cell, pbc, Z = W.diamond_si(2, a=4.36, sigma=0.06, seed=11)   # B3-like SiC density
+    el = (np.arange(len(pos)) % 8 >= 4).astype(np.int32)              # C on fcc, Si on the shifted sites
+    prm = P.Tersoff_PRB_39_5566_Si_C
+    e, f, w = tersoff_E(pos, cell, pbc, prm, el)
+    fd = fd_forces(lambda p: tersoff_E(p, cell, pbc, prm, el)[0], pos, [0, 5, 17, 40])
+    assert np.max(np.abs(fd - f[[0, 5, 17, 40]])) < 1e-6
+    assert rng is not None
+
+
+def test_tersoff_mask_decomposition_and_per_atom():
+    pos, cell, pbc, Z = read_cfg("aC_small.cfg")
+    prm = P.Tersoff_PRB_39_5566_Si_C
+    el = np.zeros(len(pos), np.int32)
+    nl = O.nl_build(pos, cell, pbc, 2.1)
+    e, f, w, epa, wpa = O.tersoff(prm, pos, el, cell, nl, per_at=True)
+    m = (np.random.default_rng(5).integers(0, len(pos), len(pos)) < len(pos) / 2).astype(np.int32)
+    e1, f1, w1 = O.tersoff(prm, pos, el, cell, nl, mask=m)
+    e2, f2, w2 = O.tersoff(prm, pos, el, cell, nl, mask=1 - m)
+    assert abs(e - e1 - e2) < 1e-9
+    assert np.abs(f - f1 - f2).max() < 1e-9
+    assert np.abs(w - w1 - w2).max() < 1e-9
+    assert abs(epa.sum() - e) < 1e-9
+    assert np.abs(wpa.sum(axis=0) - w).max() < 1e-9
+
+
+def test_tersoff_thread_count_invariance():
+    pos, cell, pbc, Z = W.diamond_si(3, sigma=0.08, seed=3)
+    nl = O.nl_build(pos, cell, pbc, 3.0)
+    el = np.zeros(len(pos), np.int32)
+    e1, f1, w1 = O.tersoff(SI, pos, el, cell, nl, nthreads=1)
+    e4, f4, w4 = O.tersoff(SI, pos, el, cell, nl, nthreads=4)
+    assert abs(e1 - e4) < 1e-10 and np.abs(f1 - f4).max() < 1e-12
+
+
+# ---------------------------------------------------------------------------
+# EAM
+# ---------------------------------------------------------------------------
+
+def fs_tables(nr=10000, nrho=10000, cutoff=5.5, rho_max=40.0):
+    dr = cutoff / (nr - 1)
+    drho = rho_max / (nrho - 1)
+    rg = np.arange(nr) * dr
+    rhog = np.arange(nrho) * drho
+    pad = lambda y: np.concatenate([y, [0.0, 0.0]])
+    return {"nel": 1, "cutoff": cutoff,
+            "F": [(W.fs_embed(rhog), drho)],
+            "rho": [(pad(W.fs_rho(rg)), dr)],
+            # r*phi table scaled by 1/2 (scale_y_axis) and padded
+            "phi": [[(0.5 * pad(rg * W.fs_phi(rg)), dr)]]}
+
+
+def eam_E(pos, cell, pbc, tables, mask=None, per_at=False):
+    nl = O.nl_build(pos, cell, pbc, tables["cutoff"])
+    el = np.zeros(len(pos), np.int32)
+    return O.eam(tables, pos, el, cell, nl, mask=mask, per_at=per_at)
+
+
+def test_spline_reproduces_smooth_function():
+    x = np.linspace(0, 3, 3001)
+    coef = O.spline_init(np.sin(x), x[1] - x[0])
+    n = len(x)
+    # evaluate through the EAM F-table path: single atom embedding = F(0)
+    assert abs(coef[0] - 0.0) < 1e-15
+    # interval midpoints: y + B(c1 + B(c2 + B c3)) at B = 0.5
+    k = np.arange(100, 2900, 97)
+    B = 0.5
+    val = coef[k] + B * (coef[2 * n + k] + B * (coef[3 * n + k] + B * coef[4 * n + k]))
+    assert np.max(np.abs(val - np.sin(x[k] + 0.0005))) < 1e-12
+    der = coef[5 * n + k] + B * (coef[6 * n + k] + B * coef[7 * n + k])
+    assert np.max(np.abs(der - np.cos(x[k] + 0.0005))) < 1e-8
+
+
+def test_eam_fcc_closed_form():
+    e_ref, nn, rho = W.fs_cu_perfect_energy()
+    assert nn == 54
+    assert abs(rho - 16.994155157944) < 1e-9
+    assert abs(e_ref - (-4.155850776094)) < 1e-9
+    pos, cell, pbc, Z = W.fcc_cu(4, sigma=0.0)
+    e, f, w = eam_E(pos, cell, pbc, fs_tables())
+    assert abs(e / len(pos) - e_ref) < 1e-6      # 10^4-point spline interpolation error
+    assert np.abs(f).max() < 1e-9
+
+
+def test_eam_fd_forces_virial_mask():
+    tables = fs_tables(nr=2000, nrho=2000)
+    pos, cell, pbc, Z = W.fcc_cu(3, sigma=0.08, seed=2)
+    e, f, w, epa, wpa = eam_E(pos, cell, pbc, tables, per_at=True)
+    fd = fd_forces(lambda p: eam_E(p, cell, pbc, tables)[0], pos, [0, 13, 50, 101])
+    assert np.max(np.abs(fd - f[[0, 13, 50, 101]])) < 1e-6
+    Wfd = fd_virial(lambda p, c: eam_E(p, c, pbc, tables)[0], pos, cell)
+    assert np.max(np.abs(Wfd - w)) < 1e-5 * max(1.0, np.abs(w).max())
+    assert abs(epa.sum() - e) < 1e-9 and np.abs(wpa.sum(axis=0) - w).max() < 1e-8
+    m = (np.arange(len(pos)) % 3 == 0).astype(np.int32)
+    e1 = eam_E(pos, cell, pbc, tables, mask=m)
+    e2 = eam_E(pos, cell, pbc, tables, mask=1 - m)
+    assert abs(e - e1[0] - e2[0]) < 1e-9
+    assert np.abs(f - e1[1] - e2[1]).max() < 1e-9
+
+
+# ---------------------------------------------------------------------------
+# Ewald
+# ---------------------------------------------------------------------------
+
+MADELUNG_NACL = 1.747564594633182
+
+
+def ewald_E(pos, q, cell, rc, beta, kmax, with_phi=False):
+    pbc = np.ones(3, bool)
+    nl = O.nl_build(pos, cell, pbc, rc, avgn=600)
+    er, fr, wr, pr = O.ewald_real(pos, q, cell, nl, rc, beta)
+    ek, fk, wk, pk = O.ewald_kspace(pos, q, cell, beta, kmax)
+    out = (er + ek, fr + fk, wr + wk)
+    return out + ((pr + pk),) if with_phi else out
+
+
+def test_ewald_madelung():
+    pos, cell, pbc, Z, q = W.rocksalt(2, sigma=0.0)
+    rc = 8.0
+    beta = W.ewald_beta(rc, 1e-10)
+    kmax = W.ewald_kmax(beta, 1e-12)
+    e, f, w = ewald_E(pos, q, cell, rc, beta, kmax)
+    r0 = 5.64 / 2
+    assert abs(e / len(pos) - (-MADELUNG_NACL / (2 * r0))) < 1e-9
+    assert np.abs(f).max() < 1e-9
+    # a second splitting gives the same total
+    b2 = W.ewald_beta(6.0, 1e-10)
+    e2 = ewald_E(pos, q, cell, 6.0, b2, W.ewald_kmax(b2, 1e-12))[0]
+    assert abs(e2 - e) < 1e-8
+
+
+def test_ewald_fd_forces_virial_phi():
+    pos, cell, pbc, Z, q = W.rocksalt(1, sigma=0.15, seed=5)
+    rc = 6.0
+    beta = W.ewald_beta(rc, 1e-10)
+    kmax = W.ewald_kmax(beta, 1e-12)
+    e, f, w, phi = ewald_E(pos, q, cell, rc, beta, kmax, with_phi=True)
+    fd = fd_forces(lambda p: ewald_E(p, q, cell, rc, beta, kmax)[0], pos, [0, 3, 6])
+    assert np.max(np.abs(fd - f[[0, 3, 6]])) < 1e-6
+    Wfd = fd_virial(lambda p, c: ewald_E(p, q, c, rc, beta, kmax)[0], pos, cell)
+    assert np.max(np.abs(Wfd - w)) < 1e-6
+    h = 1e-6
+    for i in [0, 5]:
+        qp = q.copy(); qp[i] += h
+        qm = q.copy(); qm[i] -= h
+        d = (ewald_E(pos, qp, cell, rc, beta, kmax)[0] - ewald_E(pos, qm, cell, rc, beta, kmax)[0]) / (2 * h)
+        assert abs(d - phi[i]) < 1e-6
+
+
+def test_ewald_nacl_dimer_reference_value():
+    # tests/test_coulomb.py:52-56: dimer at 2 A in a large box, E = -1/2 e^2/A
+    pos = np.array([[10.0, 10.0, 10.0], [12.0, 10.0, 10.0]])
+    q = np.array([1.0, -1.0])
+    cell = np.eye(3) * 60.0
+    rc = 14.0
+    beta = W.ewald_beta(rc, 1e-12)
+    e, f, w, phi = ewald_E(pos, q, cell, rc, beta, W.ewald_kmax(beta, 1e-14), with_phi=True)
+    # periodic images at 60 A change the dimer energy by a dipole-lattice term
+    # of order (2/60)^3 ~ 4e-5 of it
+    assert abs(e - (-0.5)) < 5e-4
+    assert abs(phi[0] - (-0.5)) < 5e-4 and abs(phi[1] - 0.5) < 5e-4
