@@ -37,6 +37,7 @@ enum DevErr : int {
     DERR_IMAGE_RANGE = 2,
     DERR_SPLINE_RANGE = 4,
     DERR_BOND_OVERFLOW = 8,
+    DERR_STAGE_OVERFLOW = 16,
 };
 
 hipStream_t stream();
@@ -102,7 +103,8 @@ struct agpu_neighbors {
     int maxrow = 0;
     // device storage
     agpu::DevBuf key, key_sorted, idx, idx_sorted, cell_start, atoms_sorted,
-        counts, offsets, nbr, sort_tmp, rowmax, rank;
+        counts, offsets, nbr, sort_tmp, rowmax, rank, stage;
+    int stage_w = 0;              // staging stride (entries per atom)
     agpu::HostStats stats;
     int64_t n_updates = 0;
 };

@@ -132,37 +132,67 @@ static void prof_collect()
 // ---------------------------------------------------------------------------
 // reductions and element maps
 // ---------------------------------------------------------------------------
-__global__ void k_reduce_ev(const double *__restrict__ part, int nblk, int w,
-                            double *__restrict__ epot, double *__restrict__ wpot,
-                            double *__restrict__ extra)
+// Stage 1: block b sums records [b*chunk, min(nblk, (b+1)*chunk)) column by
+// column; every thread walks a fixed record stride, then a fixed LDS tree.
+// Stage 2 (one block) sums the stage-1 records the same way and adds them to
+// the outputs.  The grouping depends only on nblk, so results are bitwise
+// reproducible.
+constexpr int RED_W_MAX = 16;
+
+__global__ void __launch_bounds__(256)
+k_reduce_rec(const double *__restrict__ part, int nblk, int w, int chunk,
+             double *__restrict__ out, double *__restrict__ epot, double *__restrict__ wpot,
+             double *__restrict__ extra)
 {
-    // one block of 256 threads per record column; fixed-order tree
     __shared__ double sh[256];
-    int c = blockIdx.x;
+    const int lanes = 256 / w;         // records read per sweep
+    const int c = threadIdx.x % w;
+    const int rr = threadIdx.x / w;
+    const int b0 = blockIdx.x * chunk;
+    const int b1 = min(nblk, b0 + chunk);
     double s = 0.0;
-    for (int b = threadIdx.x; b < nblk; b += 256) s += part[(size_t)b * w + c];
+    if (rr < lanes)
+        for (int b = b0 + rr; b < b1; b += lanes) s += part[(size_t)b * w + c];
     sh[threadIdx.x] = s;
     __syncthreads();
-    for (int m = 128; m > 0; m >>= 1) {
-        if ((int)threadIdx.x < m) sh[threadIdx.x] += sh[threadIdx.x + m];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        if (c == 0) {
-            if (epot) *epot += sh[0];
-        } else if (c < 10) {
-            if (wpot) wpot[c - 1] += sh[0];
-        } else if (extra) {
-            extra[c - 10] += sh[0];
+    if ((int)threadIdx.x < w) {
+        double t = 0.0;
+        for (int k = 0; k < lanes; ++k) t += sh[k * w + threadIdx.x];
+        if (out) {
+            out[(size_t)blockIdx.x * w + threadIdx.x] = t;
+        } else {
+            int col = threadIdx.x;
+            if (col == 0) {
+                if (epot) *epot += t;
+            } else if (col < 10) {
+                if (wpot) wpot[col - 1] += t;
+            } else if (extra) {
+                extra[col - 10] += t;
+            }
         }
     }
 }
 
+// process-lifetime scratch (never freed: no device frees from static destructors)
+static DevBuf *g_red_tmp = new DevBuf();
+
 int reduce_ev(const double *partials, int nblk, int w, double *epot, double *wpot, double *extra)
 {
     if (nblk <= 0 || w <= 0) return AGPU_OK;
-    AGPU_LAUNCH("reduce_ev", k_reduce_ev, dim3(w), dim3(256), 0, stream(), partials, nblk, w,
-                epot, wpot, extra);
+    if (w > RED_W_MAX) return set_error(AGPU_ERR_ARG, "reduce_ev: record too wide");
+    hipStream_t s = stream();
+    const int chunk = 256;
+    int n1 = (nblk + chunk - 1) / chunk;
+    if (n1 <= 1) {
+        AGPU_LAUNCH("reduce_ev", k_reduce_rec, dim3(1), dim3(256), 0, s, partials, nblk, w, nblk,
+                    (double *)nullptr, epot, wpot, extra);
+        return AGPU_OK;
+    }
+    AGPU_CHECK(g_red_tmp->reserve(sizeof(double) * (size_t)n1 * w));
+    AGPU_LAUNCH("reduce_ev", k_reduce_rec, dim3(n1), dim3(256), 0, s, partials, nblk, w, chunk,
+                g_red_tmp->as<double>(), (double *)nullptr, (double *)nullptr, (double *)nullptr);
+    AGPU_LAUNCH("reduce_ev", k_reduce_rec, dim3(1), dim3(256), 0, s, g_red_tmp->as<double>(), n1, w,
+                n1, (double *)nullptr, epot, wpot, extra);
     return AGPU_OK;
 }
 

@@ -104,6 +104,7 @@ struct EamDev {
     int nel;
     int tF[EAM_MAX_EL], tR[EAM_MAX_EL], tP[EAM_MAX_EL * EAM_MAX_EL];
     double cutoff_sq;
+    SplHdr h1F, h1R, h1P;     // the three tables of a single-element potential
 };
 
 struct EamState {
@@ -244,6 +245,9 @@ static int upload_tables(EamState &st)
         for (int b = 0; b < st.nel; ++b) st.dev.tP[a * EAM_MAX_EL + b] = add(st.phi[a * st.nel + b]);
     st.dev.nel = st.nel;
     st.dev.cutoff_sq = st.cutoff * st.cutoff;
+    st.dev.h1F = st.hdr[st.dev.tF[0]];
+    st.dev.h1R = st.hdr[st.dev.tR[0]];
+    st.dev.h1P = st.hdr[st.dev.tP[0]];
     std::vector<float> rec32(rec64.begin(), rec64.end());
     AGPU_CHECK(st.d_hdr.reserve(sizeof(SplHdr) * st.hdr.size()));
     AGPU_CHECK(st.d_rec64.reserve(sizeof(double) * rec64.size()));
@@ -317,14 +321,21 @@ __device__ __forceinline__ void spl_fdf_x(const SplHdr &h, const R *__restrict__
     df = c[4] + B * (c[5] + B * c[6]);
 }
 
-template <typename T> __device__ __forceinline__ T wave_sum(T v)
+// Lanes per atom: rows of ~50 neighbours are split over G lanes that each
+// walk a strided share, so several independent gathers are in flight per lane.
+constexpr int EAM_G = 16;
+
+// Table header for (table kind, element) -- from the kernel arguments when the
+// potential has a single element (ONE), else from the device header array.
+template <bool ONE>
+__device__ __forceinline__ SplHdr hdr_of(const EamDev &P, const SplHdr *__restrict__ hdr, int kind, int a, int b)
 {
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
-    return v;
+    if (ONE) return kind == 0 ? P.h1F : (kind == 1 ? P.h1R : P.h1P);
+    int t = kind == 0 ? P.tF[a] : (kind == 1 ? P.tR[a] : P.tP[a * EAM_MAX_EL + b]);
+    return hdr[t];
 }
 
-template <typename R>
+template <typename R, int G, bool ONE>
 __global__ void __launch_bounds__(256)
 k_eam_density(EamDev P, const SplHdr *__restrict__ hdr, const R *__restrict__ rec, int nat,
               const int *__restrict__ order, const double *__restrict__ r, const int *__restrict__ el,
@@ -332,9 +343,10 @@ k_eam_density(EamDev P, const SplHdr *__restrict__ hdr, const R *__restrict__ re
               const int *__restrict__ mask, double *__restrict__ dF, double *__restrict__ epa,
               double *__restrict__ partial, int *__restrict__ flags)
 {
-    __shared__ double s_e[4];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int ai = blockIdx.x * 4 + wave;
+    constexpr int APB = 256 / G;
+    __shared__ double s_e[APB];
+    const int lane = threadIdx.x % G, grp = threadIdx.x / G;
+    const int ai = blockIdx.x * APB + grp;
     double Fi = 0.0;
     int err = 0;
     if (ai < nat) {
@@ -345,25 +357,27 @@ k_eam_density(EamDev P, const SplHdr *__restrict__ hdr, const R *__restrict__ re
         if (on) {
             const double ri[3] = {r[3 * i], r[3 * i + 1], r[3 * i + 2]};
             const int beg = offsets[i], end = offsets[i + 1];
-            for (int s = beg + lane; s < end; s += 64) {
-                int2 e = nbr[s];
-                int elj = el[e.x];
+#pragma unroll 4
+            for (int s = beg + lane; s < end; s += G) {
+                const int2 e = nbr[s];
+                const int elj = el[e.x];
                 if (elj < 0) continue;
                 int a, b, c;
                 dc_unpack(e.y, a, b, c);
                 const double rj[3] = {r[3 * e.x], r[3 * e.x + 1], r[3 * e.x + 2]};
                 double d[3];
                 delta_r(C, ri, rj, a, b, c, d);
-                double d2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-                if (d2 < P.cutoff_sq) rho += (double)spl_f<R>(hdr[P.tR[elj]], rec, sqrt(d2), err);
+                const double d2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+                if (d2 < P.cutoff_sq)
+                    rho += (double)spl_f<R>(hdr_of<ONE>(P, hdr, 1, elj, 0), rec, sqrt(d2), err);
             }
         }
-        rho = wave_sum(rho);
+        rho = group_sum<G>(rho);
         double dFi = 0.0;
         if (on) {
             if (rho < 0.0) rho = 0.0;
             R f, df;
-            spl_fdf_x<R>(hdr[P.tF[eli]], rec, rho, f, df);
+            spl_fdf_x<R>(hdr_of<ONE>(P, hdr, 0, eli, 0), rec, rho, f, df);
             Fi = (double)f;
             dFi = (double)df;
         }
@@ -373,12 +387,16 @@ k_eam_density(EamDev P, const SplHdr *__restrict__ hdr, const R *__restrict__ re
         }
     }
     if (err) atomicOr(flags, err);
-    if (lane == 0) s_e[wave] = Fi;
+    if (lane == 0) s_e[grp] = Fi;
     __syncthreads();
-    if (threadIdx.x == 0) partial[blockIdx.x] = ((s_e[0] + s_e[1]) + s_e[2]) + s_e[3];
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int k = 0; k < APB; ++k) t += s_e[k];
+        partial[blockIdx.x] = t;
+    }
 }
 
-template <typename R>
+template <typename R, int G, bool ONE>
 __global__ void __launch_bounds__(256)
 k_eam_force(EamDev P, const SplHdr *__restrict__ hdr, const R *__restrict__ rec, int nat,
             const int *__restrict__ order, const double *__restrict__ r, const int *__restrict__ el,
@@ -387,9 +405,10 @@ k_eam_force(EamDev P, const SplHdr *__restrict__ hdr, const R *__restrict__ rec,
             double *__restrict__ epa, double *__restrict__ wpa, double *__restrict__ partial,
             int *__restrict__ flags)
 {
-    __shared__ double s_red[4][7];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int ai = blockIdx.x * 4 + wave;
+    constexpr int APB = 256 / G;
+    __shared__ double s_red[APB][7];
+    const int lane = threadIdx.x % G, grp = threadIdx.x / G;
+    const int ai = blockIdx.x * APB + grp;
     double acc[3] = {0.0, 0.0, 0.0}, e = 0.0;
     double w[6] = {0, 0, 0, 0, 0, 0};      // own-loop virial (xx yy zz yz xz xy)
     double wp[6] = {0, 0, 0, 0, 0, 0};     // per-atom virial
@@ -400,40 +419,43 @@ k_eam_force(EamDev P, const SplHdr *__restrict__ hdr, const R *__restrict__ rec,
         const int eli = el[i];
         if (eli >= 0) {
             const double mi = (mask && mask[i] == 0) ? 0.0 : 1.0;
-            const double dFi = dF[i];
+            const R dFi = (R)dF[i];
             const double ri[3] = {r[3 * i], r[3 * i + 1], r[3 * i + 2]};
             const int beg = offsets[i], end = offsets[i + 1];
-            const SplHdr hri = hdr[P.tR[eli]];
-            for (int s = beg + lane; s < end; s += 64) {
-                int2 en = nbr[s];
+            const SplHdr hri = hdr_of<ONE>(P, hdr, 1, eli, 0);
+#pragma unroll 4
+            for (int s = beg + lane; s < end; s += G) {
+                const int2 en = nbr[s];
                 const int j = en.x;
-                int elj = el[j];
+                const int elj = el[j];
                 if (elj < 0) continue;
                 int a, b, c;
                 dc_unpack(en.y, a, b, c);
                 const double rj[3] = {r[3 * j], r[3 * j + 1], r[3 * j + 2]};
                 double d[3];
                 delta_r(C, ri, rj, a, b, c, d);
-                double d2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+                const double d2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
                 if (!(d2 < P.cutoff_sq)) continue;
                 const double mj = (mask && mask[j] == 0) ? 0.0 : 1.0;
-                double ad = sqrt(d2);
+                const double dFj = dF[j];
+                const double ad = sqrt(d2);
+                const double rrd = 1.0 / ad;
                 R phi, dphi;
-                spl_fdf_x<R>(hdr[P.tP[eli * EAM_MAX_EL + elj]], rec, ad, phi, dphi);
-                R rr = (R)(1.0 / ad);
-                R pp = (dphi - phi * rr) * rr;
-                R drho_j = spl_df<R>(hdr[P.tR[elj]], rec, ad, err);
-                R drho_i = spl_df<R>(hri, rec, ad, err);
-                double ci = mi * ((double)((R)dFi * drho_j + pp));
-                double cj = mj * ((double)((R)dF[j] * drho_i + pp));
+                spl_fdf_x<R>(hdr_of<ONE>(P, hdr, 2, eli, elj), rec, ad, phi, dphi);
+                const R rr = (R)rrd;
+                const R pp = (dphi - phi * rr) * rr;
+                const R drho_j = spl_df<R>(hdr_of<ONE>(P, hdr, 1, elj, 0), rec, ad, err);
+                const R drho_i = spl_df<R>(hri, rec, ad, err);
+                const double ci = mi * (double)(dFi * drho_j + pp);
+                const double cj = mj * (double)((R)dFj * drho_i + pp);
                 e += mi * (double)(phi * rr);
-                double rrd = 1.0 / ad;
-                double fs = -(ci + cj) * rrd;
+                const double fs = -(ci + cj) * rrd;
                 acc[0] += fs * d[0];
                 acc[1] += fs * d[1];
                 acc[2] += fs * d[2];
-                double wo = ci * rrd, wq = 0.5 * (ci + cj) * rrd;
-                double dd[6] = {d[0] * d[0], d[1] * d[1], d[2] * d[2], d[1] * d[2], d[0] * d[2], d[0] * d[1]};
+                const double wo = ci * rrd, wq = 0.5 * (ci + cj) * rrd;
+                const double dd[6] = {d[0] * d[0], d[1] * d[1], d[2] * d[2], d[1] * d[2], d[0] * d[2], d[0] * d[1]};
+#pragma unroll
                 for (int q = 0; q < 6; ++q) {
                     w[q] += wo * dd[q];
                     wp[q] += wq * dd[q];
@@ -441,11 +463,13 @@ k_eam_force(EamDev P, const SplHdr *__restrict__ hdr, const R *__restrict__ rec,
             }
         }
     }
-    for (int x = 0; x < 3; ++x) acc[x] = wave_sum(acc[x]);
-    e = wave_sum(e);
-    for (int q = 0; q < 6; ++q) w[q] = wave_sum(w[q]);
+#pragma unroll
+    for (int x = 0; x < 3; ++x) acc[x] = group_sum<G>(acc[x]);
+    e = group_sum<G>(e);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) w[q] = group_sum<G>(w[q]);
     if (wpa)
-        for (int q = 0; q < 6; ++q) wp[q] = wave_sum(wp[q]);
+        for (int q = 0; q < 6; ++q) wp[q] = group_sum<G>(wp[q]);
     if (i >= 0 && lane == 0) {
         f[3 * (size_t)i] += acc[0];
         f[3 * (size_t)i + 1] += acc[1];
@@ -467,13 +491,14 @@ k_eam_force(EamDev P, const SplHdr *__restrict__ hdr, const R *__restrict__ rec,
     }
     if (err) atomicOr(flags, err);
     if (lane == 0) {
-        s_red[wave][0] = e;
-        for (int q = 0; q < 6; ++q) s_red[wave][1 + q] = w[q];
+        s_red[grp][0] = e;
+        for (int q = 0; q < 6; ++q) s_red[grp][1 + q] = w[q];
     }
     __syncthreads();
     if (threadIdx.x < 7) {
-        int q = threadIdx.x;
-        double v = ((s_red[0][q] + s_red[1][q]) + s_red[2][q]) + s_red[3][q];
+        const int q = threadIdx.x;
+        double v = 0.0;
+        for (int k = 0; k < APB; ++k) v += s_red[k][q];
         // record layout [e, W column-major 3x3]
         double *rec10 = partial + (size_t)blockIdx.x * 10;
         if (q == 0) rec10[0] = v;
@@ -522,7 +547,7 @@ int eam_bind(agpu_potential *pot)
     return AGPU_OK;
 }
 
-template <typename R>
+template <typename R, bool ONE>
 static int eam_launch(EamState *st, agpu_potential *pot, const agpu_eval_args *a, const R *rec, int nblk)
 {
     agpu_particles *p = pot->p;
@@ -535,11 +560,12 @@ static int eam_launch(EamState *st, agpu_potential *pot, const agpu_eval_args *a
     double *part = st->partial.as<double>();
     int *flags = st->stats.as<int>();
     const bool mixed = sizeof(R) == 4;
-    AGPU_LAUNCH(mixed ? "eam_density_mixed" : "eam_density_fp64", k_eam_density<R>, dim3(nblk), dim3(256), 0, s,
-                st->dev, st->d_hdr.as<SplHdr>(), rec, p->nat, order, p->d_r, el, C, nl->offsets.as<int>(),
-                nl->nbr.as<int2>(), a->mask, st->dF.as<double>(), a->epot_per_at, part, flags);
-    AGPU_LAUNCH(mixed ? "eam_force_mixed" : "eam_force_fp64", k_eam_force<R>, dim3(nblk), dim3(256), 0, s,
-                st->dev, st->d_hdr.as<SplHdr>(), rec, p->nat, order, p->d_r, el, C, nl->offsets.as<int>(),
+    AGPU_LAUNCH(mixed ? "eam_density_mixed" : "eam_density_fp64", (k_eam_density<R, EAM_G, ONE>), dim3(nblk),
+                dim3(256), 0, s, st->dev, st->d_hdr.as<SplHdr>(), rec, p->nat, order, p->d_r, el, C,
+                nl->offsets.as<int>(), nl->nbr.as<int2>(), a->mask, st->dF.as<double>(), a->epot_per_at, part,
+                flags);
+    AGPU_LAUNCH(mixed ? "eam_force_mixed" : "eam_force_fp64", (k_eam_force<R, EAM_G, ONE>), dim3(nblk), dim3(256),
+                0, s, st->dev, st->d_hdr.as<SplHdr>(), rec, p->nat, order, p->d_r, el, C, nl->offsets.as<int>(),
                 nl->nbr.as<int2>(), a->mask, st->dF.as<double>(), a->f, a->epot_per_at, a->wpot_per_at,
                 part + nblk, flags);
     return AGPU_OK;
@@ -555,17 +581,21 @@ int eam_compute(agpu_potential *pot, const agpu_eval_args *a)
     hipStream_t s = stream();
     AGPU_HIP(hipMemsetAsync(a->epot, 0, sizeof(double), s));
     if (nat == 0) return AGPU_OK;
-    const int nblk = (nat + 3) / 4;
+    const int nblk = (nat + 256 / EAM_G - 1) / (256 / EAM_G);
     AGPU_CHECK(st->dF.reserve(sizeof(double) * (nat + 1)));
     // layout: nblk energy scalars of the density pass, then nblk 10-records
     AGPU_CHECK(st->partial.reserve(sizeof(double) * 11 * (size_t)nblk));
     AGPU_CHECK(st->stats.reserve(sizeof(int) * 4));
     AGPU_CHECK(pot->stats.init());
     AGPU_HIP(hipMemsetAsync(st->stats.ptr, 0, sizeof(int) * 4, s));
-    if (pot->precision == AGPU_PREC_MIXED)
-        AGPU_CHECK(eam_launch<float>(st, pot, a, st->d_rec32.as<float>(), nblk));
-    else
-        AGPU_CHECK(eam_launch<double>(st, pot, a, st->d_rec64.as<double>(), nblk));
+    const bool one = st->nel == 1;
+    if (pot->precision == AGPU_PREC_MIXED) {
+        if (one) AGPU_CHECK((eam_launch<float, true>(st, pot, a, st->d_rec32.as<float>(), nblk)));
+        else AGPU_CHECK((eam_launch<float, false>(st, pot, a, st->d_rec32.as<float>(), nblk)));
+    } else {
+        if (one) AGPU_CHECK((eam_launch<double, true>(st, pot, a, st->d_rec64.as<double>(), nblk)));
+        else AGPU_CHECK((eam_launch<double, false>(st, pot, a, st->d_rec64.as<double>(), nblk)));
+    }
     AGPU_CHECK(reduce_ev(st->partial.as<double>(), nblk, 1, a->epot, nullptr, nullptr));
     AGPU_CHECK(reduce_ev(st->partial.as<double>() + nblk, nblk, 10, a->epot, a->wpot, nullptr));
     AGPU_HIP(hipMemcpyAsync(pot->stats.h, st->stats.ptr, sizeof(int), hipMemcpyDeviceToHost, s));

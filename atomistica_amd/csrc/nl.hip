@@ -11,13 +11,14 @@
 //   3. k_cell_start: first sorted slot of each bin.
 //   4. k_gather:   bin-sorted copy of (r, atom, wrap) so that candidate scans
 //                  read contiguous 32-byte records.
-//   5. k_scan_bins<false>: per atom, count accepted (j, image) pairs over
-//                  the stencil (x-major, then y, then z) -> exclusive scan.
-//   6. k_scan_bins<true>:  same walk, writes (j, packed image) in order.
-// Threads walk atoms in bin order, so a wavefront scans the same candidate
-// bins and its loads broadcast from L1/L2.  The distance test is the
-// reference's FP64 expression without contraction, so the accepted set is
-// bit-identical.
+//   5. k_bin_pairs<false>: one wavefront per home bin counts, for every atom
+//                  of the bin, the accepted (j, image) pairs over the stencil
+//                  (x-major, then y, then z) -> exclusive scan -> row offsets.
+//   6. k_bin_pairs<true>:  same walk, writes (j, packed image) in row order.
+// A home bin's atoms share one candidate list staged in LDS, so candidate
+// loads are shared by the whole wavefront and the pair tests run without
+// divergence.  The distance test is the reference's FP64 expression without
+// contraction, so the accepted set is bit-identical.
 #include <cmath>
 #include <cstring>
 
@@ -116,90 +117,286 @@ __global__ void k_gather(int nat, const double *__restrict__ r, BinGeom g,
     rank[j] = p;
 }
 
-// FILL = false: count rows; FILL = true: write them.
-template <bool FILL>
-__global__ void __launch_bounds__(256)
-k_scan_bins(int nat, BinGeom g, CellArg C, double cutoff_sq, const SortedAtom *__restrict__ atoms,
-            const int *__restrict__ cell_start, int *__restrict__ counts,
-            const int *__restrict__ offsets, int2 *__restrict__ nbr, long long cap,
-            int *__restrict__ flags, int *__restrict__ rowmax)
-{
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= nat) return;
-    const SortedAtom ai = atoms[p];
-    const int i = ai.j;
-    const double ri[3] = {ai.x, ai.y, ai.z};
-    int wi0, wi1, wi2;
-    dc_unpack(ai.w, wi0, wi1, wi2);
-    // bin of i, recomputed exactly as the reference does per atom
-    int ci[3], wtmp[3];
-    bin_of(g, ri, ci, wtmp);
+constexpr int NL_KM = 128;      // candidate records staged in LDS per wavefront
+constexpr int NL_WAVES = 4;     // wavefronts per block
 
-    int out = 0, cnt = 0;
-    long long base = 0;
-    bool ok = true;
-    if (FILL) {
-        if (*flags & DERR_NL_OVERFLOW) return;
-        base = offsets[i];
-        cnt = counts[i];
-        // serial overflow rule of python_neighbors.f90:713-718: the last
-        // 1-based slot written, counting one separator per earlier atom
-        if (cnt > 0 && base + cnt + (long long)i >= cap) {
-            atomicOr(flags, DERR_NL_OVERFLOW);
-            ok = false;
+__device__ __forceinline__ unsigned long long lanes_below(int l)
+{
+    return l >= 64 ? ~0ull : ((1ull << l) - 1ull);
+}
+
+// smallest b in [0, 64) with pre[b] > e (pre = inclusive prefix, pre[63] > e)
+__device__ __forceinline__ int find_bin(const int *pre, int e)
+{
+    int lo = 0;
+#pragma unroll
+    for (int step = 32; step >= 1; step >>= 1)
+        if (pre[lo + step - 1] <= e) lo += step;
+    return lo;
+}
+
+__device__ __forceinline__ int off_pack(int a, int b, int c) { return ((a + 128) << 16) | ((b + 128) << 8) | (c + 128); }
+
+struct NlShared {
+    double cx[NL_KM], cy[NL_KM], cz[NL_KM];
+    int cj[NL_KM], cu[NL_KM], cw[NL_KM], co[NL_KM];   // atom, bin wrap, atom wrap, block offset
+    double hx[64], hy[64], hz[64];
+    int hi[64], hw[64], ho[64], hp[64], hcnt[64];       // atom, wrap, block offset, sorted slot, count
+    int pre[64], start[64], bu[64], boff[64];
+};
+
+// One wavefront per block of B x B x B home bins (B = 1 for dense bins).
+// The candidate list of the block -- the atoms of its (B + 2 nd)^3 stencil
+// bins in x-major bin order, ascending atom order inside each bin -- is
+// staged in LDS, 128 records at a time.  Pairs (home atom a, candidate g) are
+// tested 64 at a time with g fastest; a candidate counts for a only if its
+// bin lies in a's own (2 nd + 1)^3 stencil, so every atom sees exactly the
+// reference's candidates in the reference's order (python_neighbors.f90:
+// 664-735).  The ballot of accepted lanes ranks each accepted pair inside its
+// atom's row; rows go to a padded staging area (stride S per sorted slot)
+// and are compacted afterwards.
+__global__ void __launch_bounds__(256)
+k_bin_pairs(int B, BinGeom g, CellArg C, double cutoff_sq, const SortedAtom *__restrict__ atoms,
+            const int *__restrict__ cell_start, int *__restrict__ counts, int2 *__restrict__ stage,
+            int S, int *__restrict__ flags, int *__restrict__ rowmax)
+{
+    __shared__ NlShared sh[NL_WAVES];
+    NlShared &L = sh[threadIdx.x >> 6];
+    const int lane = threadIdx.x & 63;
+    const int nbx = (g.nc[0] + B - 1) / B, nby = (g.nc[1] + B - 1) / B, nbz = (g.nc[2] + B - 1) / B;
+    const int blk = blockIdx.x * NL_WAVES + (threadIdx.x >> 6);
+    if (blk >= nbx * nby * nbz) return;
+    const int bz = blk % nbz, by = (blk / nbz) % nby, bx = blk / (nbz * nby);
+    // 1-based first home bin of the block
+    const int h0[3] = {bx * B + 1, by * B + 1, bz * B + 1};
+    const int nh = B * B * B;
+
+    // ---- home bins: lane L owns home bin L (x-major inside the block) ----
+    {
+        int kL = 0, sL = 0;
+        if (lane < nh) {
+            const int oz = lane % B, oy = (lane / B) % B, ox = lane / (B * B);
+            const int cx = h0[0] + ox, cy = h0[1] + oy, cz = h0[2] + oz;
+            if (cx <= g.nc[0] && cy <= g.nc[1] && cz <= g.nc[2]) {
+                const int b = ((cx - 1) * g.nc[1] + (cy - 1)) * g.nc[2] + (cz - 1);
+                sL = cell_start[b];
+                kL = cell_start[b + 1] - sL;
+            }
+            L.boff[lane] = off_pack(ox, oy, oz);
         }
-        if (!ok) return;
+        int incl = kL;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            int v = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += v;
+        }
+        L.pre[lane] = incl;
+        L.start[lane] = sL;
     }
-    for (int dx = -g.nd[0]; dx <= g.nd[0]; ++dx) {
-        int cx = ci[0] + dx, s0 = wi0;
-        if (g.pbc[0]) {
-            int t = floordiv(cx - 1, g.nc[0]);
-            cx -= t * g.nc[0];
-            s0 -= t;
-        } else if (cx < 1 || cx > g.nc[0]) continue;
-        for (int dy = -g.nd[1]; dy <= g.nd[1]; ++dy) {
-            int cy = ci[1] + dy, s1 = wi1;
-            if (g.pbc[1]) {
-                int t = floordiv(cy - 1, g.nc[1]);
-                cy -= t * g.nc[1];
-                s1 -= t;
-            } else if (cy < 1 || cy > g.nc[1]) continue;
-            for (int dz = -g.nd[2]; dz <= g.nd[2]; ++dz) {
-                int cz = ci[2] + dz, s2 = wi2;
-                if (g.pbc[2]) {
-                    int t = floordiv(cz - 1, g.nc[2]);
-                    cz -= t * g.nc[2];
-                    s2 -= t;
-                } else if (cz < 1 || cz > g.nc[2]) continue;
-                int cell = ((cx - 1) * g.nc[1] + (cy - 1)) * g.nc[2] + (cz - 1);
-                int pb = cell_start[cell], pe = cell_start[cell + 1];
-                for (int q = pb; q < pe; ++q) {
-                    const SortedAtom aj = atoms[q];
-                    int wj0, wj1, wj2;
-                    dc_unpack(aj.w, wj0, wj1, wj2);
-                    int t0 = s0 - wj0, t1 = s1 - wj1, t2 = s2 - wj2;
-                    if (aj.j == i && t0 == 0 && t1 == 0 && t2 == 0) continue;
-                    const double rj[3] = {aj.x, aj.y, aj.z};
-                    double d[3];
-                    delta_r(C, ri, rj, t0, t1, t2, d);
-                    double d2 = dot3_nc(d);
-                    if (d2 < cutoff_sq) {
-                        if (FILL) {
-                            if (out >= cnt) continue;   // never taken: same walk as the count
-                            if (!dc_pack_ok(t0) || !dc_pack_ok(t1) || !dc_pack_ok(t2))
-                                atomicOr(flags, DERR_IMAGE_RANGE);
-                            nbr[base + out] = make_int2(aj.j, dc_pack(t0, t1, t2));
+    __builtin_amdgcn_wave_barrier();
+    const int MH = L.pre[63];
+    if (MH == 0) return;
+    // home atoms beyond 64 are processed in further chunks; keep the home-bin
+    // table in registers of the lanes (pre/start are reused for candidates)
+    const int hpre = L.pre[lane], hstart = L.start[lane], hboff = lane < nh ? L.boff[lane] : 0;
+    const int sx = B + 2 * g.nd[0], sy = B + 2 * g.nd[1], sz = B + 2 * g.nd[2];
+    const int nst = sx * sy * sz;
+    int wmax = 0;
+    bool stage_over = false;
+
+    for (int a0 = 0; a0 < MH; a0 += 64) {
+        const int ma = min(64, MH - a0);
+        __builtin_amdgcn_wave_barrier();
+        L.pre[lane] = hpre;
+        L.start[lane] = hstart;
+        L.boff[lane] = hboff;
+        __builtin_amdgcn_wave_barrier();
+        if (lane < ma) {
+            const int e = a0 + lane;
+            const int hb = find_bin(L.pre, e);
+            const int p = L.start[hb] + e - (hb ? L.pre[hb - 1] : 0);
+            const SortedAtom A = atoms[p];
+            L.hx[lane] = A.x;
+            L.hy[lane] = A.y;
+            L.hz[lane] = A.z;
+            L.hi[lane] = A.j;
+            L.hw[lane] = A.w;
+            L.ho[lane] = L.boff[hb];
+            L.hp[lane] = p;
+            L.hcnt[lane] = 0;
+        }
+        __builtin_amdgcn_wave_barrier();
+        int my_cnt = 0;
+        int kfill = 0;
+
+        auto flush = [&](int K) {
+            if (K <= 0) return;
+            int a = 0, gi = lane;
+            while (gi >= K) {
+                gi -= K;
+                ++a;
+            }
+            const int npairs = ma * K;
+            for (int t0 = 0; t0 < npairs; t0 += 64) {
+                bool acc = false;
+                int t0i = 0, t1i = 0, t2i = 0, jj = 0;
+                if (a < ma) {
+                    const int ho = L.ho[a], co = L.co[gi];
+                    const int d0 = ((co >> 16) & 255) - ((ho >> 16) & 255);
+                    const int d1 = ((co >> 8) & 255) - ((ho >> 8) & 255);
+                    const int d2 = (co & 255) - (ho & 255);
+                    if (d0 >= -g.nd[0] && d0 <= g.nd[0] && d1 >= -g.nd[1] && d1 <= g.nd[1] &&
+                        d2 >= -g.nd[2] && d2 <= g.nd[2]) {
+                        int wa0, wa1, wa2, u0, u1, u2, wb0, wb1, wb2;
+                        dc_unpack(L.hw[a], wa0, wa1, wa2);
+                        dc_unpack(L.cu[gi], u0, u1, u2);
+                        dc_unpack(L.cw[gi], wb0, wb1, wb2);
+                        t0i = wa0 + u0 - wb0;
+                        t1i = wa1 + u1 - wb1;
+                        t2i = wa2 + u2 - wb2;
+                        jj = L.cj[gi];
+                        if (!(jj == L.hi[a] && t0i == 0 && t1i == 0 && t2i == 0)) {
+                            const double ri[3] = {L.hx[a], L.hy[a], L.hz[a]};
+                            const double rj[3] = {L.cx[gi], L.cy[gi], L.cz[gi]};
+                            double d[3];
+                            delta_r(C, ri, rj, t0i, t1i, t2i, d);
+                            acc = dot3_nc(d) < cutoff_sq;
                         }
-                        ++out;
                     }
+                }
+                const unsigned long long m = __ballot(acc);
+                if (acc) {
+                    const int ast = max(0, a * K - t0);
+                    const int pos = L.hcnt[a] + __popcll(m & lanes_below(lane) & ~lanes_below(ast));
+                    if (!dc_pack_ok(t0i) || !dc_pack_ok(t1i) || !dc_pack_ok(t2i))
+                        atomicOr(flags, DERR_IMAGE_RANGE);
+                    if (pos < S) stage[(size_t)L.hp[a] * S + pos] = make_int2(jj, dc_pack(t0i, t1i, t2i));
+                }
+                __builtin_amdgcn_wave_barrier();
+                if (lane < ma) {
+                    const int lo = lane * K - t0, hi = (lane + 1) * K - t0;
+                    if (hi > 0 && lo < 64) {
+                        const unsigned long long rng = lanes_below(min(64, hi)) & ~lanes_below(max(0, lo));
+                        my_cnt += __popcll(m & rng);
+                        L.hcnt[lane] = my_cnt;
+                    }
+                }
+                __builtin_amdgcn_wave_barrier();
+                gi += 64;
+                while (gi >= K && a < ma) {
+                    gi -= K;
+                    ++a;
+                }
+            }
+        };
+
+        for (int st0 = 0; st0 < nst; st0 += 64) {
+            // lane L owns stencil bin st0 + L of the block (x-major order)
+            const int st = st0 + lane;
+            int kL = 0, startL = 0, uL = 0, oL = 0;
+            if (st < nst) {
+                const int oz = st % sz - g.nd[2];
+                const int oy = (st / sz) % sy - g.nd[1];
+                const int ox = st / (sz * sy) - g.nd[0];
+                int cc[3] = {h0[0] + ox, h0[1] + oy, h0[2] + oz};
+                int uu[3] = {0, 0, 0};
+                bool ok = true;
+                for (int k = 0; k < 3; ++k) {
+                    if (g.pbc[k]) {
+                        int t = floordiv(cc[k] - 1, g.nc[k]);
+                        cc[k] -= t * g.nc[k];
+                        uu[k] = -t;
+                    } else if (cc[k] < 1 || cc[k] > g.nc[k]) {
+                        ok = false;
+                    }
+                }
+                if (ok) {
+                    const int b = ((cc[0] - 1) * g.nc[1] + (cc[1] - 1)) * g.nc[2] + (cc[2] - 1);
+                    startL = cell_start[b];
+                    kL = cell_start[b + 1] - startL;
+                    uL = dc_pack(uu[0], uu[1], uu[2]);
+                    oL = off_pack(ox, oy, oz);
+                }
+            }
+            int incl = kL;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                int v = __shfl_up(incl, o, 64);
+                if (lane >= o) incl += v;
+            }
+            __builtin_amdgcn_wave_barrier();
+            L.pre[lane] = incl;
+            L.start[lane] = startL;
+            L.bu[lane] = uL;
+            L.boff[lane] = oL;
+            __builtin_amdgcn_wave_barrier();
+            const int T = L.pre[63];
+            int q = 0;
+            while (q < T) {
+                const int take = min(NL_KM - kfill, T - q);
+                for (int e = q + lane; e < q + take; e += 64) {
+                    const int b = find_bin(L.pre, e);
+                    const int src = L.start[b] + e - (b ? L.pre[b - 1] : 0);
+                    const SortedAtom A = atoms[src];
+                    const int dst = kfill + e - q;
+                    L.cx[dst] = A.x;
+                    L.cy[dst] = A.y;
+                    L.cz[dst] = A.z;
+                    L.cj[dst] = A.j;
+                    L.cw[dst] = A.w;
+                    L.cu[dst] = L.bu[b];
+                    L.co[dst] = L.boff[b];
+                }
+                kfill += take;
+                q += take;
+                __builtin_amdgcn_wave_barrier();
+                if (kfill == NL_KM) {
+                    flush(kfill);
+                    kfill = 0;
+                    __builtin_amdgcn_wave_barrier();
                 }
             }
         }
+        flush(kfill);
+        __builtin_amdgcn_wave_barrier();
+        if (lane < ma) {
+            counts[L.hi[lane]] = my_cnt;
+            wmax = max(wmax, my_cnt);
+            if (my_cnt > S) stage_over = true;
+        }
     }
-    if (!FILL) {
-        counts[i] = out;
-        atomicMax(rowmax, out);
+    // one atomic per wavefront
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) wmax = max(wmax, __shfl_xor(wmax, o, 64));
+    const bool any_over = __ballot(stage_over) != 0ull;
+    if (lane == 0) {
+        atomicMax(rowmax, wmax);
+        if (any_over) atomicOr(flags, DERR_STAGE_OVERFLOW);
     }
+}
+
+// Copy staged rows (sorted slot p, stride S) into the CSR rows of the caller
+// atoms; applies the reference's capacity rule (python_neighbors.f90:713-718).
+__global__ void __launch_bounds__(256)
+k_nl_compact(int nat, const SortedAtom *__restrict__ atoms, const int *__restrict__ counts,
+             const int *__restrict__ offsets, const int2 *__restrict__ stage, int S, int2 *__restrict__ nbr,
+             long long cap, int *__restrict__ flags)
+{
+    constexpr int G = 16;
+    const int lane = threadIdx.x % G;
+    const int p = blockIdx.x * (256 / G) + threadIdx.x / G;
+    if (p >= nat) return;
+    if (*flags & (DERR_NL_OVERFLOW | DERR_STAGE_OVERFLOW)) return;
+    const int i = atoms[p].j;
+    const int cnt = counts[i];
+    const long long base = offsets[i];
+    // the last 1-based slot written, counting one separator per earlier atom
+    if (cnt > 0 && base + cnt + (long long)i >= cap) {
+        if (lane == 0) atomicOr(flags, DERR_NL_OVERFLOW);
+        return;
+    }
+    if (cnt > S) return;
+    for (int o = lane; o < cnt; o += G) nbr[base + o] = stage[(size_t)p * S + o];
 }
 
 __global__ void k_offsets_tail(int nat, const int *__restrict__ counts, int *__restrict__ offsets,
@@ -325,9 +522,29 @@ int nl_update(agpu_neighbors *nl, agpu_particles *p, int force)
         AGPU_LAUNCH("nl_gather", k_gather, dim3(nb), dim3(TB), 0, s, nat, p->d_r, g,
                     nl->idx_sorted.as<int>(), nl->atoms_sorted.as<SortedAtom>(), nl->rank.as<int>());
         double cutoff_sq = nl->cutoff * nl->cutoff;
-        AGPU_LAUNCH("nl_count", k_scan_bins<false>, dim3(nb), dim3(TB), 0, s, nat, g, C, cutoff_sq,
+        // bins per wavefront edge: one bin when bins are dense, a 2^3 or 3^3
+        // block of bins when they hold few atoms (stencil of +-1 bin only)
+        const double per_bin = (double)nat / (double)ncell;
+        int B = 1;
+        if (g.nd[0] == 1 && g.nd[1] == 1 && g.nd[2] == 1) {
+            if (per_bin < 0.8) B = 3;
+            else if (per_bin < 4.0) B = 2;
+        }
+        const long long nblk_b = (long long)((g.nc[0] + B - 1) / B) * ((g.nc[1] + B - 1) / B) * ((g.nc[2] + B - 1) / B);
+        const int nbin_blk = (int)((nblk_b + NL_WAVES - 1) / NL_WAVES);
+        // staging stride: expected neighbours from the density, grown on demand
+        if (nl->stage_w <= 0) {
+            const double *A = p->cell.A;
+            double V = std::fabs(A[0] * (A[4] * A[8] - A[5] * A[7]) - A[1] * (A[3] * A[8] - A[5] * A[6]) +
+                                 A[2] * (A[3] * A[7] - A[4] * A[6]));
+            double est = (V > 0 ? nat / V : 0.0) * 4.18879 * nl->cutoff * nl->cutoff * nl->cutoff;
+            nl->stage_w = ((int)(1.4 * est) + 16 + 7) & ~7;
+        }
+        const int S = nl->stage_w;
+        AGPU_CHECK(nl->stage.reserve(sizeof(int2) * (size_t)nat * S));
+        AGPU_LAUNCH("nl_pairs", k_bin_pairs, dim3(nbin_blk), dim3(TB), 0, s, B, g, C, cutoff_sq,
                     nl->atoms_sorted.as<SortedAtom>(), nl->cell_start.as<int>(), nl->counts.as<int>(),
-                    (const int *)nullptr, (int2 *)nullptr, (long long)nl->cap, flags + 3, flags);
+                    nl->stage.as<int2>(), S, flags + 3, flags);
         size_t tmp2 = 0;
         AGPU_HIP(rocprim::exclusive_scan(nullptr, tmp2, nl->counts.as<int>(), nl->offsets.as<int>(), 0,
                                          (size_t)nat, rocprim::plus<int>(), s));
@@ -336,10 +553,9 @@ int nl_update(agpu_neighbors *nl, agpu_particles *p, int force)
                                          nl->offsets.as<int>(), 0, (size_t)nat, rocprim::plus<int>(), s));
         AGPU_LAUNCH("nl_offsets_tail", k_offsets_tail, dim3(1), dim3(64), 0, s, nat,
                     nl->counts.as<int>(), nl->offsets.as<int>(), flags);
-        AGPU_LAUNCH("nl_fill", k_scan_bins<true>, dim3(nb), dim3(TB), 0, s, nat, g, C, cutoff_sq,
-                    nl->atoms_sorted.as<SortedAtom>(), nl->cell_start.as<int>(), nl->counts.as<int>(),
-                    nl->offsets.as<int>(), nl->nbr.as<int2>(), (long long)nl->cap, flags + 3,
-                    (int *)nullptr);
+        AGPU_LAUNCH("nl_compact", k_nl_compact, dim3((nat + 15) / 16), dim3(TB), 0, s, nat,
+                    nl->atoms_sorted.as<SortedAtom>(), nl->counts.as<int>(), nl->offsets.as<int>(),
+                    nl->stage.as<int2>(), S, nl->nbr.as<int2>(), (long long)nl->cap, flags + 3);
     } else {
         AGPU_HIP(hipMemsetAsync(nl->offsets.ptr, 0, sizeof(int), s));
     }
@@ -347,6 +563,13 @@ int nl_update(agpu_neighbors *nl, agpu_particles *p, int force)
     AGPU_HIP(hipStreamSynchronize(s));
     int fl = nl->stats.h[3];
     nl->maxrow = nl->stats.h[0];
+    if (fl & DERR_STAGE_OVERFLOW) {
+        // a row outgrew the staging stride: widen it and rebuild once
+        nl->stage_w = ((int)(1.25 * nl->maxrow) + 16 + 7) & ~7;
+        nl->p = nullptr;
+        if (force > 1) return set_error(AGPU_ERR_NL_OVERFLOW, "neighbour list: staging stride overflow");
+        return nl_update(nl, p, 2);
+    }
     nl->total = (int64_t)nl->stats.h[1] | ((int64_t)nl->stats.h[2] << 31);
     nl->p = p;
     nl->nat = nat;

@@ -93,8 +93,7 @@ __global__ void k_bop_bonds(int nat, const double *__restrict__ r, const int *__
                             int *__restrict__ bcnt, int *__restrict__ stats)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nat) return;
-    int eli = el[i];
+    int eli = i < nat ? el[i] : -1;
     int nb = 0;
     if (eli >= 0) {
         const double ri[3] = {r[3 * i], r[3 * i + 1], r[3 * i + 2]};
@@ -113,8 +112,10 @@ __global__ void k_bop_bonds(int nat, const double *__restrict__ r, const int *__
             if (rl < P.r2[t] * P.r2[t]) bslot[beg + nb++] = s;
         }
     }
-    bcnt[i] = nb;
-    atomicMax(stats, nb);
+    if (i < nat) bcnt[i] = nb;
+    // one atomic per wavefront
+    for (int o = 32; o >= 1; o >>= 1) nb = max(nb, __shfl_xor(nb, o, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(stats, nb);
 }
 
 __global__ void k_bop_rev(int nat, const int *__restrict__ offsets, const int2 *__restrict__ nbr,
@@ -263,10 +264,15 @@ k_tersoff(TersoffDev P, int nat, const int *__restrict__ order, const double *__
     const R hi = (R)P.h[eli < 0 ? 0 : eli];
     const R c2 = (R)P.c2[eli < 0 ? 0 : eli];
     const R d2 = (R)P.d2[eli < 0 ? 0 : eli];
-    const R cd = c2 / d2;
     R zij = (R)0;
-#pragma unroll
+    // the k loops stop at the wavefront's largest bond count (uniform, so the
+    // group shuffles below always see their whole group)
+    int nbw = nb;
+    for (int o = 32; o >= 1; o >>= 1) nbw = max(nbw, __shfl_xor(nbw, o, 64));
+    constexpr int UNR = G <= 8 ? G : 1;
+#pragma unroll UNR
     for (int k = 0; k < G; ++k) {
+        if (k >= nbw) break;
         const int tk = gbase + k;
         const int tpk = s_typ[tk];
         if (on && k != lane && k < nb && tpk >= 0) {
@@ -274,7 +280,8 @@ k_tersoff(TersoffDev P, int nat, const int *__restrict__ order, const double *__
             R costh = rnk[0] * rn[0] + rnk[1] * rn[1] + rnk[2] * rn[2];
             R h_c = hi - costh;
             R hh = d2 + h_c * h_c;
-            R g = (R)P.omega[tpk] * ((R)1 + cd - c2 / hh);
+            // c^2/d^2 - c^2/(d^2 + t^2) = c^2 t^2 / (d^2 (d^2 + t^2)): no cancellation
+            R g = (R)P.omega[tpk] * ((R)1 + c2 * h_c * h_c / (d2 * hh));
             R hv, dh;
             h_fn<R>(P, tpk, rl - s_rl[tk], hv, dh);
             zij += s_fc[tk] * g * hv;
@@ -315,8 +322,9 @@ k_tersoff(TersoffDev P, int nat, const int *__restrict__ order, const double *__
     double recv[3] = {0.0, 0.0, 0.0};
     const R rlijr = (R)1 / rl;
     const R rij[3] = {rl * rn[0], rl * rn[1], rl * rn[2]};
-#pragma unroll
+#pragma unroll UNR
     for (int k = 0; k < G; ++k) {
+        if (k >= nbw) break;
         const int tk = gbase + k;
         const int tpk = s_typ[tk];
         R dk[3] = {(R)0, (R)0, (R)0};
@@ -329,7 +337,7 @@ k_tersoff(TersoffDev P, int nat, const int *__restrict__ order, const double *__
             R h_c = hi - costh;
             R hh = d2 + h_c * h_c;
             R om = (R)P.omega[tpk];
-            R g = om * ((R)1 + cd - c2 / hh);
+            R g = om * ((R)1 + c2 * h_c * h_c / (d2 * hh));
             R dg = (R)-2 * om * c2 * h_c / (hh * hh);
             R dkc[3];
             for (int x = 0; x < 3; ++x) dkc[x] = rnk[x] * rlik - rn[x] * rl;
