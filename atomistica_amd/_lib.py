@@ -73,6 +73,7 @@ PROTOTYPES = {
     "agpu_profile_enable": [i],
     "agpu_profile_reset": [],
     "agpu_profile_query": [ctypes.c_char_p, dp, ip],
+    "agpu_profile_list": [ctypes.c_char_p, i],
     "agpu_particles_create": [ctypes.POINTER(vp)],
     "agpu_particles_destroy": [vp],
     "agpu_particles_allocate": [vp, i],

@@ -357,6 +357,19 @@ int agpu_profile_reset(void)
     return AGPU_OK;
 }
 
+int agpu_profile_list(char *buf, int len)
+{
+    if (!buf || len <= 0) return set_error(AGPU_ERR_ARG, "profile_list: bad buffer");
+    prof_collect();
+    std::string s;
+    for (auto &kv : g_prof_done) {
+        if (!s.empty()) s += ",";
+        s += kv.first;
+    }
+    std::snprintf(buf, (size_t)len, "%s", s.c_str());
+    return AGPU_OK;
+}
+
 int agpu_profile_query(const char *kernel, double *total_ms, int *count)
 {
     prof_collect();

@@ -357,6 +357,14 @@ def profile_reset():
     call("agpu_profile_reset")
 
 
+def profile_list():
+    """Names of every kernel timed since the last profile_reset()."""
+    buf = ctypes.create_string_buffer(8192)
+    call("agpu_profile_list", buf, 8192)
+    v = buf.value.decode()
+    return v.split(",") if v else []
+
+
 def profile_query(kernel):
     """(total ms, launches) of one kernel, from HIP events on its launch stream."""
     t = ctypes.c_double()

@@ -205,13 +205,8 @@ def main():
         run.step()
     torch.cuda.synchronize()
     N.profile_enable(False)
-    names = ["nl_bin", "nl_cell_start", "nl_gather", "nl_count", "nl_fill", "nl_offsets_tail",
-             "eam_density_fp64", "eam_force_fp64", "eam_density_mixed", "eam_force_mixed",
-             "bop_bonds", "bop_rev", "tersoff_fp64", "tersoff_mixed", "bop_gather",
-             "ewald_real_fp64", "ewald_real_mixed", "ewald_frac", "ewald_sk", "ewald_kenergy",
-             "ewald_kforce", "reduce_ev", "map_elements"]
     kern = {}
-    for k in names:
+    for k in N.profile_list():
         ms, cnt = N.profile_query(k)
         if cnt:
             kern[k] = {"avg_us": 1e3 * ms / cnt, "launches": cnt}

@@ -101,6 +101,8 @@ int agpu_profile_enable(int on);
 int agpu_profile_reset(void);
 /* Sum of event-measured durations (ms) and launch count of one kernel. */
 int agpu_profile_query(const char *kernel, double *total_ms, int *count);
+/* Comma-separated names of every kernel timed since the last reset. */
+int agpu_profile_list(char *buf, int len);
 
 /* ======================================================================
  * particles -- python_particles.f90:84-170 (particles_t),
