@@ -335,6 +335,11 @@ __device__ __forceinline__ SplHdr hdr_of(const EamDev &P, const SplHdr *__restri
     return hdr[t];
 }
 
+// Entries per lane and batch: every lane first issues the row loads of its
+// EAM_U entries, then their position gathers, then the table gathers, so one
+// latency is paid per batch instead of per neighbour.
+constexpr int EAM_U = 4;
+
 template <typename R, int G, bool ONE>
 __global__ void __launch_bounds__(256)
 k_eam_density(EamDev P, const SplHdr *__restrict__ hdr, const R *__restrict__ rec, int nat,
@@ -357,19 +362,34 @@ k_eam_density(EamDev P, const SplHdr *__restrict__ hdr, const R *__restrict__ re
         if (on) {
             const double ri[3] = {r[3 * i], r[3 * i + 1], r[3 * i + 2]};
             const int beg = offsets[i], end = offsets[i + 1];
-#pragma unroll 4
-            for (int s = beg + lane; s < end; s += G) {
-                const int2 e = nbr[s];
-                const int elj = el[e.x];
-                if (elj < 0) continue;
-                int a, b, c;
-                dc_unpack(e.y, a, b, c);
-                const double rj[3] = {r[3 * e.x], r[3 * e.x + 1], r[3 * e.x + 2]};
-                double d[3];
-                delta_r(C, ri, rj, a, b, c, d);
-                const double d2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-                if (d2 < P.cutoff_sq)
-                    rho += (double)spl_f<R>(hdr_of<ONE>(P, hdr, 1, elj, 0), rec, sqrt(d2), err);
+            for (int s0 = beg + lane; s0 < end; s0 += EAM_U * G) {
+                int2 e[EAM_U];
+#pragma unroll
+                for (int u = 0; u < EAM_U; ++u) {
+                    const int s = s0 + u * G;
+                    e[u] = s < end ? nbr[s] : make_int2(-1, 0);
+                }
+                double rj[EAM_U][3];
+                int elj[EAM_U];
+#pragma unroll
+                for (int u = 0; u < EAM_U; ++u) {
+                    const int j = e[u].x < 0 ? i : e[u].x;
+                    elj[u] = e[u].x < 0 ? -1 : el[j];
+                    rj[u][0] = r[3 * j];
+                    rj[u][1] = r[3 * j + 1];
+                    rj[u][2] = r[3 * j + 2];
+                }
+#pragma unroll
+                for (int u = 0; u < EAM_U; ++u) {
+                    if (elj[u] < 0) continue;
+                    int a, b, c;
+                    dc_unpack(e[u].y, a, b, c);
+                    double d[3];
+                    delta_r(C, ri, rj[u], a, b, c, d);
+                    const double d2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+                    if (d2 < P.cutoff_sq)
+                        rho += (double)spl_f<R>(hdr_of<ONE>(P, hdr, 1, elj[u], 0), rec, sqrt(d2), err);
+                }
             }
         }
         rho = group_sum<G>(rho);
@@ -423,42 +443,56 @@ k_eam_force(EamDev P, const SplHdr *__restrict__ hdr, const R *__restrict__ rec,
             const double ri[3] = {r[3 * i], r[3 * i + 1], r[3 * i + 2]};
             const int beg = offsets[i], end = offsets[i + 1];
             const SplHdr hri = hdr_of<ONE>(P, hdr, 1, eli, 0);
-#pragma unroll 4
-            for (int s = beg + lane; s < end; s += G) {
-                const int2 en = nbr[s];
-                const int j = en.x;
-                const int elj = el[j];
-                if (elj < 0) continue;
-                int a, b, c;
-                dc_unpack(en.y, a, b, c);
-                const double rj[3] = {r[3 * j], r[3 * j + 1], r[3 * j + 2]};
-                double d[3];
-                delta_r(C, ri, rj, a, b, c, d);
-                const double d2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-                if (!(d2 < P.cutoff_sq)) continue;
-                const double mj = (mask && mask[j] == 0) ? 0.0 : 1.0;
-                const double dFj = dF[j];
-                const double ad = sqrt(d2);
-                const double rrd = 1.0 / ad;
-                R phi, dphi;
-                spl_fdf_x<R>(hdr_of<ONE>(P, hdr, 2, eli, elj), rec, ad, phi, dphi);
-                const R rr = (R)rrd;
-                const R pp = (dphi - phi * rr) * rr;
-                const R drho_j = spl_df<R>(hdr_of<ONE>(P, hdr, 1, elj, 0), rec, ad, err);
-                const R drho_i = spl_df<R>(hri, rec, ad, err);
-                const double ci = mi * (double)(dFi * drho_j + pp);
-                const double cj = mj * (double)((R)dFj * drho_i + pp);
-                e += mi * (double)(phi * rr);
-                const double fs = -(ci + cj) * rrd;
-                acc[0] += fs * d[0];
-                acc[1] += fs * d[1];
-                acc[2] += fs * d[2];
-                const double wo = ci * rrd, wq = 0.5 * (ci + cj) * rrd;
-                const double dd[6] = {d[0] * d[0], d[1] * d[1], d[2] * d[2], d[1] * d[2], d[0] * d[2], d[0] * d[1]};
+            for (int s0 = beg + lane; s0 < end; s0 += EAM_U * G) {
+                int2 en[EAM_U];
 #pragma unroll
-                for (int q = 0; q < 6; ++q) {
-                    w[q] += wo * dd[q];
-                    wp[q] += wq * dd[q];
+                for (int u = 0; u < EAM_U; ++u) {
+                    const int s = s0 + u * G;
+                    en[u] = s < end ? nbr[s] : make_int2(-1, 0);
+                }
+                double rj[EAM_U][3], dFj[EAM_U], mj[EAM_U];
+                int elj[EAM_U];
+#pragma unroll
+                for (int u = 0; u < EAM_U; ++u) {
+                    const int j = en[u].x < 0 ? i : en[u].x;
+                    elj[u] = en[u].x < 0 ? -1 : el[j];
+                    rj[u][0] = r[3 * j];
+                    rj[u][1] = r[3 * j + 1];
+                    rj[u][2] = r[3 * j + 2];
+                    dFj[u] = dF[j];
+                    mj[u] = (mask && mask[j] == 0) ? 0.0 : 1.0;
+                }
+#pragma unroll
+                for (int u = 0; u < EAM_U; ++u) {
+                    if (elj[u] < 0) continue;
+                    int a, b, c;
+                    dc_unpack(en[u].y, a, b, c);
+                    double d[3];
+                    delta_r(C, ri, rj[u], a, b, c, d);
+                    const double d2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+                    if (!(d2 < P.cutoff_sq)) continue;
+                    const double rrd = rsqrt(d2);
+                    const double ad = d2 * rrd;
+                    R phi, dphi;
+                    spl_fdf_x<R>(hdr_of<ONE>(P, hdr, 2, eli, elj[u]), rec, ad, phi, dphi);
+                    const R rr = (R)rrd;
+                    const R pp = (dphi - phi * rr) * rr;
+                    const R drho_j = spl_df<R>(hdr_of<ONE>(P, hdr, 1, elj[u], 0), rec, ad, err);
+                    const R drho_i = spl_df<R>(hri, rec, ad, err);
+                    const double ci = mi * (double)(dFi * drho_j + pp);
+                    const double cj = mj[u] * (double)((R)dFj[u] * drho_i + pp);
+                    e += mi * (double)(phi * rr);
+                    const double fs = -(ci + cj) * rrd;
+                    acc[0] += fs * d[0];
+                    acc[1] += fs * d[1];
+                    acc[2] += fs * d[2];
+                    const double wo = ci * rrd, wq = 0.5 * (ci + cj) * rrd;
+                    const double dd[6] = {d[0] * d[0], d[1] * d[1], d[2] * d[2], d[1] * d[2], d[0] * d[2], d[0] * d[1]};
+#pragma unroll
+                    for (int q = 0; q < 6; ++q) {
+                        w[q] += wo * dd[q];
+                        wp[q] += wq * dd[q];
+                    }
                 }
             }
         }

@@ -117,8 +117,9 @@ __global__ void k_gather(int nat, const double *__restrict__ r, BinGeom g,
     rank[j] = p;
 }
 
-constexpr int NL_KM = 128;      // candidate records staged in LDS per wavefront
+constexpr int NL_R = 4;         // candidate slots per lane (registers): 256 per pass
 constexpr int NL_WAVES = 4;     // wavefronts per block
+constexpr int NL_HMAX = 64;     // home atoms per pass
 
 __device__ __forceinline__ unsigned long long lanes_below(int l)
 {
@@ -138,23 +139,43 @@ __device__ __forceinline__ int find_bin(const int *pre, int e)
 __device__ __forceinline__ int off_pack(int a, int b, int c) { return ((a + 128) << 16) | ((b + 128) << 8) | (c + 128); }
 
 struct NlShared {
-    double cx[NL_KM], cy[NL_KM], cz[NL_KM];
-    int cj[NL_KM], cu[NL_KM], cw[NL_KM], co[NL_KM];   // atom, bin wrap, atom wrap, block offset
-    double hx[64], hy[64], hz[64];
-    int hi[64], hw[64], ho[64], hp[64], hcnt[64];       // atom, wrap, block offset, sorted slot, count
+    double hx[NL_HMAX], hy[NL_HMAX], hz[NL_HMAX];
+    int hi[NL_HMAX], hw0[NL_HMAX], hw1[NL_HMAX], hw2[NL_HMAX], ho[NL_HMAX], hp[NL_HMAX], hcnt[NL_HMAX];
     int pre[64], start[64], bu[64], boff[64];
 };
 
+// pair shift s = w_home + u_bin - w_cand; distance as the reference writes it
+template <bool ORTHO>
+__device__ __forceinline__ double pair_d2(const CellArg &C, double xi, double yi, double zi, double xj, double yj,
+                                          double zj, int s0, int s1, int s2)
+{
+#pragma clang fp contract(off)
+    double d[3];
+    if (ORTHO) {
+        // off-diagonal cell entries are exactly zero, so A.s reduces to the
+        // diagonal products bit for bit
+        d[0] = (xi - xj) + C.A[0] * (double)s0;
+        d[1] = (yi - yj) + C.A[4] * (double)s1;
+        d[2] = (zi - zj) + C.A[8] * (double)s2;
+    } else {
+        const double ri[3] = {xi, yi, zi}, rj[3] = {xj, yj, zj};
+        delta_r(C, ri, rj, s0, s1, s2, d);
+    }
+    return dot3_nc(d);
+}
+
 // One wavefront per block of B x B x B home bins (B = 1 for dense bins).
-// The candidate list of the block -- the atoms of its (B + 2 nd)^3 stencil
-// bins in x-major bin order, ascending atom order inside each bin -- is
-// staged in LDS, 128 records at a time.  Pairs (home atom a, candidate g) are
-// tested 64 at a time with g fastest; a candidate counts for a only if its
-// bin lies in a's own (2 nd + 1)^3 stencil, so every atom sees exactly the
-// reference's candidates in the reference's order (python_neighbors.f90:
-// 664-735).  The ballot of accepted lanes ranks each accepted pair inside its
-// atom's row; rows go to a padded staging area (stride S per sorted slot)
-// and are compacted afterwards.
+// The block's candidates -- the atoms of its (B + 2 nd)^3 stencil bins in
+// x-major bin order, ascending atom order inside each bin -- are held in
+// registers, 256 per pass (lane L holds candidates L, L+64, L+128, L+192).
+// The wavefront walks its home atoms one at a time (wave-uniform data) and
+// tests 64 candidates per instruction; a candidate counts for a home atom
+// only if its bin lies in the atom's own (2 nd + 1)^3 stencil, so every atom
+// sees exactly the reference's candidates in the reference's order
+// (python_neighbors.f90:664-735).  The ballot of accepted lanes ranks each
+// accepted pair inside the row; rows go to a padded staging area (stride S
+// per sorted slot) and are compacted afterwards.
+template <bool ORTHO>
 __global__ void __launch_bounds__(256)
 k_bin_pairs(int B, BinGeom g, CellArg C, double cutoff_sq, const SortedAtom *__restrict__ atoms,
             const int *__restrict__ cell_start, int *__restrict__ counts, int2 *__restrict__ stage,
@@ -167,128 +188,60 @@ k_bin_pairs(int B, BinGeom g, CellArg C, double cutoff_sq, const SortedAtom *__r
     const int blk = blockIdx.x * NL_WAVES + (threadIdx.x >> 6);
     if (blk >= nbx * nby * nbz) return;
     const int bz = blk % nbz, by = (blk / nbz) % nby, bx = blk / (nbz * nby);
-    // 1-based first home bin of the block
     const int h0[3] = {bx * B + 1, by * B + 1, bz * B + 1};
     const int nh = B * B * B;
 
     // ---- home bins: lane L owns home bin L (x-major inside the block) ----
-    {
-        int kL = 0, sL = 0;
-        if (lane < nh) {
-            const int oz = lane % B, oy = (lane / B) % B, ox = lane / (B * B);
-            const int cx = h0[0] + ox, cy = h0[1] + oy, cz = h0[2] + oz;
-            if (cx <= g.nc[0] && cy <= g.nc[1] && cz <= g.nc[2]) {
-                const int b = ((cx - 1) * g.nc[1] + (cy - 1)) * g.nc[2] + (cz - 1);
-                sL = cell_start[b];
-                kL = cell_start[b + 1] - sL;
-            }
-            L.boff[lane] = off_pack(ox, oy, oz);
+    int hk = 0, hs = 0, hb_off = 0;
+    if (lane < nh) {
+        const int oz = lane % B, oy = (lane / B) % B, ox = lane / (B * B);
+        const int cx = h0[0] + ox, cy = h0[1] + oy, cz = h0[2] + oz;
+        if (cx <= g.nc[0] && cy <= g.nc[1] && cz <= g.nc[2]) {
+            const int b = ((cx - 1) * g.nc[1] + (cy - 1)) * g.nc[2] + (cz - 1);
+            hs = cell_start[b];
+            hk = cell_start[b + 1] - hs;
         }
-        int incl = kL;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            int v = __shfl_up(incl, o, 64);
-            if (lane >= o) incl += v;
-        }
-        L.pre[lane] = incl;
-        L.start[lane] = sL;
+        hb_off = off_pack(ox, oy, oz);
     }
-    __builtin_amdgcn_wave_barrier();
-    const int MH = L.pre[63];
+    int hpre = hk;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int v = __shfl_up(hpre, o, 64);
+        if (lane >= o) hpre += v;
+    }
+    const int MH = __shfl(hpre, 63, 64);
     if (MH == 0) return;
-    // home atoms beyond 64 are processed in further chunks; keep the home-bin
-    // table in registers of the lanes (pre/start are reused for candidates)
-    const int hpre = L.pre[lane], hstart = L.start[lane], hboff = lane < nh ? L.boff[lane] : 0;
     const int sx = B + 2 * g.nd[0], sy = B + 2 * g.nd[1], sz = B + 2 * g.nd[2];
     const int nst = sx * sy * sz;
     int wmax = 0;
     bool stage_over = false;
 
-    for (int a0 = 0; a0 < MH; a0 += 64) {
-        const int ma = min(64, MH - a0);
+    for (int a0 = 0; a0 < MH; a0 += NL_HMAX) {
+        const int ma = min(NL_HMAX, MH - a0);
         __builtin_amdgcn_wave_barrier();
         L.pre[lane] = hpre;
-        L.start[lane] = hstart;
-        L.boff[lane] = hboff;
+        L.start[lane] = hs;
+        L.boff[lane] = hb_off;
         __builtin_amdgcn_wave_barrier();
         if (lane < ma) {
             const int e = a0 + lane;
             const int hb = find_bin(L.pre, e);
             const int p = L.start[hb] + e - (hb ? L.pre[hb - 1] : 0);
             const SortedAtom A = atoms[p];
+            int w0, w1, w2;
+            dc_unpack(A.w, w0, w1, w2);
             L.hx[lane] = A.x;
             L.hy[lane] = A.y;
             L.hz[lane] = A.z;
             L.hi[lane] = A.j;
-            L.hw[lane] = A.w;
+            L.hw0[lane] = w0;
+            L.hw1[lane] = w1;
+            L.hw2[lane] = w2;
             L.ho[lane] = L.boff[hb];
             L.hp[lane] = p;
             L.hcnt[lane] = 0;
         }
         __builtin_amdgcn_wave_barrier();
-        int my_cnt = 0;
-        int kfill = 0;
-
-        auto flush = [&](int K) {
-            if (K <= 0) return;
-            int a = 0, gi = lane;
-            while (gi >= K) {
-                gi -= K;
-                ++a;
-            }
-            const int npairs = ma * K;
-            for (int t0 = 0; t0 < npairs; t0 += 64) {
-                bool acc = false;
-                int t0i = 0, t1i = 0, t2i = 0, jj = 0;
-                if (a < ma) {
-                    const int ho = L.ho[a], co = L.co[gi];
-                    const int d0 = ((co >> 16) & 255) - ((ho >> 16) & 255);
-                    const int d1 = ((co >> 8) & 255) - ((ho >> 8) & 255);
-                    const int d2 = (co & 255) - (ho & 255);
-                    if (d0 >= -g.nd[0] && d0 <= g.nd[0] && d1 >= -g.nd[1] && d1 <= g.nd[1] &&
-                        d2 >= -g.nd[2] && d2 <= g.nd[2]) {
-                        int wa0, wa1, wa2, u0, u1, u2, wb0, wb1, wb2;
-                        dc_unpack(L.hw[a], wa0, wa1, wa2);
-                        dc_unpack(L.cu[gi], u0, u1, u2);
-                        dc_unpack(L.cw[gi], wb0, wb1, wb2);
-                        t0i = wa0 + u0 - wb0;
-                        t1i = wa1 + u1 - wb1;
-                        t2i = wa2 + u2 - wb2;
-                        jj = L.cj[gi];
-                        if (!(jj == L.hi[a] && t0i == 0 && t1i == 0 && t2i == 0)) {
-                            const double ri[3] = {L.hx[a], L.hy[a], L.hz[a]};
-                            const double rj[3] = {L.cx[gi], L.cy[gi], L.cz[gi]};
-                            double d[3];
-                            delta_r(C, ri, rj, t0i, t1i, t2i, d);
-                            acc = dot3_nc(d) < cutoff_sq;
-                        }
-                    }
-                }
-                const unsigned long long m = __ballot(acc);
-                if (acc) {
-                    const int ast = max(0, a * K - t0);
-                    const int pos = L.hcnt[a] + __popcll(m & lanes_below(lane) & ~lanes_below(ast));
-                    if (!dc_pack_ok(t0i) || !dc_pack_ok(t1i) || !dc_pack_ok(t2i))
-                        atomicOr(flags, DERR_IMAGE_RANGE);
-                    if (pos < S) stage[(size_t)L.hp[a] * S + pos] = make_int2(jj, dc_pack(t0i, t1i, t2i));
-                }
-                __builtin_amdgcn_wave_barrier();
-                if (lane < ma) {
-                    const int lo = lane * K - t0, hi = (lane + 1) * K - t0;
-                    if (hi > 0 && lo < 64) {
-                        const unsigned long long rng = lanes_below(min(64, hi)) & ~lanes_below(max(0, lo));
-                        my_cnt += __popcll(m & rng);
-                        L.hcnt[lane] = my_cnt;
-                    }
-                }
-                __builtin_amdgcn_wave_barrier();
-                gi += 64;
-                while (gi >= K && a < ma) {
-                    gi -= K;
-                    ++a;
-                }
-            }
-        };
 
         for (int st0 = 0; st0 < nst; st0 += 64) {
             // lane L owns stencil bin st0 + L of the block (x-major order)
@@ -331,38 +284,79 @@ k_bin_pairs(int B, BinGeom g, CellArg C, double cutoff_sq, const SortedAtom *__r
             L.boff[lane] = oL;
             __builtin_amdgcn_wave_barrier();
             const int T = L.pre[63];
-            int q = 0;
-            while (q < T) {
-                const int take = min(NL_KM - kfill, T - q);
-                for (int e = q + lane; e < q + take; e += 64) {
-                    const int b = find_bin(L.pre, e);
-                    const int src = L.start[b] + e - (b ? L.pre[b - 1] : 0);
-                    const SortedAtom A = atoms[src];
-                    const int dst = kfill + e - q;
-                    L.cx[dst] = A.x;
-                    L.cy[dst] = A.y;
-                    L.cz[dst] = A.z;
-                    L.cj[dst] = A.j;
-                    L.cw[dst] = A.w;
-                    L.cu[dst] = L.bu[b];
-                    L.co[dst] = L.boff[b];
+            for (int c0 = 0; c0 < T; c0 += 64 * NL_R) {
+                // ---- load this pass's candidates into registers ----
+                double cx[NL_R], cy[NL_R], cz[NL_R];
+                int cj[NL_R], ct0[NL_R], ct1[NL_R], ct2[NL_R], co[NL_R];
+                bool cv[NL_R];
+#pragma unroll
+                for (int r = 0; r < NL_R; ++r) {
+                    const int e = c0 + 64 * r + lane;
+                    cv[r] = e < T;
+                    cx[r] = cy[r] = cz[r] = 0.0;
+                    cj[r] = -1;
+                    ct0[r] = ct1[r] = ct2[r] = 0;
+                    co[r] = 0;
+                    if (cv[r]) {
+                        const int b = find_bin(L.pre, e);
+                        const int src = L.start[b] + e - (b ? L.pre[b - 1] : 0);
+                        const SortedAtom A = atoms[src];
+                        int u0, u1, u2, w0, w1, w2;
+                        dc_unpack(L.bu[b], u0, u1, u2);
+                        dc_unpack(A.w, w0, w1, w2);
+                        cx[r] = A.x;
+                        cy[r] = A.y;
+                        cz[r] = A.z;
+                        cj[r] = A.j;
+                        ct0[r] = u0 - w0;
+                        ct1[r] = u1 - w1;
+                        ct2[r] = u2 - w2;
+                        co[r] = L.boff[b];
+                    }
                 }
-                kfill += take;
-                q += take;
-                __builtin_amdgcn_wave_barrier();
-                if (kfill == NL_KM) {
-                    flush(kfill);
-                    kfill = 0;
+                // ---- home atoms one at a time (wave-uniform) ----
+                for (int a = 0; a < ma; ++a) {
+                    const double xi = L.hx[a], yi = L.hy[a], zi = L.hz[a];
+                    const int ia = L.hi[a], w0 = L.hw0[a], w1 = L.hw1[a], w2 = L.hw2[a];
+                    const int ho = L.ho[a];
+                    const size_t srow = (size_t)L.hp[a] * S;
+                    int cnt = L.hcnt[a];
+#pragma unroll
+                    for (int r = 0; r < NL_R; ++r) {
+                        bool acc = false;
+                        const int s0 = w0 + ct0[r], s1 = w1 + ct1[r], s2 = w2 + ct2[r];
+                        if (cv[r]) {
+                            bool member = true;
+                            if (B > 1) {
+                                const int d0 = ((co[r] >> 16) & 255) - ((ho >> 16) & 255);
+                                const int d1 = ((co[r] >> 8) & 255) - ((ho >> 8) & 255);
+                                const int d2 = (co[r] & 255) - (ho & 255);
+                                member = d0 >= -g.nd[0] && d0 <= g.nd[0] && d1 >= -g.nd[1] && d1 <= g.nd[1] &&
+                                         d2 >= -g.nd[2] && d2 <= g.nd[2];
+                            }
+                            if (member && !(cj[r] == ia && s0 == 0 && s1 == 0 && s2 == 0))
+                                acc = pair_d2<ORTHO>(C, xi, yi, zi, cx[r], cy[r], cz[r], s0, s1, s2) < cutoff_sq;
+                        }
+                        const unsigned long long m = __ballot(acc);
+                        if (acc) {
+                            const int pos = cnt + __popcll(m & lanes_below(lane));
+                            if (!dc_pack_ok(s0) || !dc_pack_ok(s1) || !dc_pack_ok(s2))
+                                atomicOr(flags, DERR_IMAGE_RANGE);
+                            if (pos < S) stage[srow + pos] = make_int2(cj[r], dc_pack(s0, s1, s2));
+                        }
+                        cnt += __popcll(m);
+                    }
+                    if (lane == 0) L.hcnt[a] = cnt;
                     __builtin_amdgcn_wave_barrier();
                 }
             }
         }
-        flush(kfill);
         __builtin_amdgcn_wave_barrier();
         if (lane < ma) {
-            counts[L.hi[lane]] = my_cnt;
-            wmax = max(wmax, my_cnt);
-            if (my_cnt > S) stage_over = true;
+            const int c = L.hcnt[lane];
+            counts[L.hi[lane]] = c;
+            wmax = max(wmax, c);
+            if (c > S) stage_over = true;
         }
     }
     // one atomic per wavefront
@@ -526,10 +520,9 @@ int nl_update(agpu_neighbors *nl, agpu_particles *p, int force)
         // block of bins when they hold few atoms (stencil of +-1 bin only)
         const double per_bin = (double)nat / (double)ncell;
         int B = 1;
-        if (g.nd[0] == 1 && g.nd[1] == 1 && g.nd[2] == 1) {
-            if (per_bin < 0.8) B = 3;
-            else if (per_bin < 4.0) B = 2;
-        }
+        if (g.nd[0] == 1 && g.nd[1] == 1 && g.nd[2] == 1 && per_bin < 4.0) B = 2;
+        const bool ortho = C.A[1] == 0.0 && C.A[2] == 0.0 && C.A[3] == 0.0 && C.A[5] == 0.0 &&
+                           C.A[6] == 0.0 && C.A[7] == 0.0;
         const long long nblk_b = (long long)((g.nc[0] + B - 1) / B) * ((g.nc[1] + B - 1) / B) * ((g.nc[2] + B - 1) / B);
         const int nbin_blk = (int)((nblk_b + NL_WAVES - 1) / NL_WAVES);
         // staging stride: expected neighbours from the density, grown on demand
@@ -542,9 +535,14 @@ int nl_update(agpu_neighbors *nl, agpu_particles *p, int force)
         }
         const int S = nl->stage_w;
         AGPU_CHECK(nl->stage.reserve(sizeof(int2) * (size_t)nat * S));
-        AGPU_LAUNCH("nl_pairs", k_bin_pairs, dim3(nbin_blk), dim3(TB), 0, s, B, g, C, cutoff_sq,
-                    nl->atoms_sorted.as<SortedAtom>(), nl->cell_start.as<int>(), nl->counts.as<int>(),
-                    nl->stage.as<int2>(), S, flags + 3, flags);
+        if (ortho)
+            AGPU_LAUNCH("nl_pairs", k_bin_pairs<true>, dim3(nbin_blk), dim3(TB), 0, s, B, g, C, cutoff_sq,
+                        nl->atoms_sorted.as<SortedAtom>(), nl->cell_start.as<int>(), nl->counts.as<int>(),
+                        nl->stage.as<int2>(), S, flags + 3, flags);
+        else
+            AGPU_LAUNCH("nl_pairs", k_bin_pairs<false>, dim3(nbin_blk), dim3(TB), 0, s, B, g, C, cutoff_sq,
+                        nl->atoms_sorted.as<SortedAtom>(), nl->cell_start.as<int>(), nl->counts.as<int>(),
+                        nl->stage.as<int2>(), S, flags + 3, flags);
         size_t tmp2 = 0;
         AGPU_HIP(rocprim::exclusive_scan(nullptr, tmp2, nl->counts.as<int>(), nl->offsets.as<int>(), 0,
                                          (size_t)nat, rocprim::plus<int>(), s));
