@@ -206,6 +206,9 @@ int ewald_potential(agpu_potential *pot, const double *q, double *phi, int flags
 // columns 10.. -> extra[0..] (+=).  Runs on the library stream.
 int reduce_ev(const double *partials, int nblk, int w, double *epot, double *wpot, double *extra);
 
+// *out = max(*out, max(v[0..n))) on the library stream (device pointers)
+int int_max(const int *v, int n, int *out);
+
 // map atomic numbers to database element indices (-1 = not handled)
 int map_elements(agpu_potential *pot, const int *z2db, int nz);
 }  // namespace agpu

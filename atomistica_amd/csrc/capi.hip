@@ -196,6 +196,27 @@ int reduce_ev(const double *partials, int nblk, int w, double *epot, double *wpo
     return AGPU_OK;
 }
 
+// max over n ints into *out (atomicMax, one atomic per block of 4096 values)
+__global__ void __launch_bounds__(256) k_int_max(const int *__restrict__ v, int n, int *__restrict__ out)
+{
+    __shared__ int sm[4];
+    int m = 0;
+    const int base = blockIdx.x * 4096;
+    for (int k = threadIdx.x; k < 4096; k += 256)
+        if (base + k < n) m = max(m, v[base + k]);
+    for (int o = 32; o >= 1; o >>= 1) m = max(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicMax(out, max(max(sm[0], sm[1]), max(sm[2], sm[3])));
+}
+
+int int_max(const int *v, int n, int *out)
+{
+    if (n <= 0) return AGPU_OK;
+    AGPU_LAUNCH("int_max", k_int_max, dim3((n + 4095) / 4096), dim3(256), 0, stream(), v, n, out);
+    return AGPU_OK;
+}
+
 __global__ void k_map_el(int nat, const int *__restrict__ Z, const int *__restrict__ z2db, int nz,
                          int *__restrict__ el)
 {

@@ -29,7 +29,6 @@ namespace agpu {
 
 constexpr int EW_L = 16;       // k per run in k_ewald_sk
 constexpr int EW_CHUNK = 256;  // ions per LDS chunk
-constexpr int EW_KCHUNK = 512; // k per LDS chunk in k_ewald_kforce
 
 struct EwItem {
     int n1, n2, n3s, cnt, kofs, pad0, pad1, pad2;
@@ -273,13 +272,17 @@ k_ewald_kenergy(int K, const int *__restrict__ kpack, CellArg Bt, double inv4b2,
             ((s_red[0][threadIdx.x] + s_red[1][threadIdx.x]) + s_red[2][threadIdx.x]) + s_red[3][threadIdx.x];
 }
 
+constexpr int EW_ICHUNK = 64;   // runs per LDS chunk in k_ewald_kforce
+
+// Thread per ion: sum over the same runs of k as k_ewald_sk -- one sincospi
+// per run, complex rotation by exp(2 pi i s3) along it -- of T(k) exp(-i k.r).
 __global__ void __launch_bounds__(256)
-k_ewald_kforce(int nat, int K, const double *__restrict__ s4, const double *__restrict__ q,
-               const int *__restrict__ kpack, const double *__restrict__ T, CellArg Bt, double pref2,
+k_ewald_kforce(int nat, int nitems, const double *__restrict__ s4, const double *__restrict__ q,
+               const EwItem *__restrict__ items, const double *__restrict__ T, CellArg Bt, double pref2,
                double *__restrict__ f, double *__restrict__ phi)
 {
-    __shared__ int sh_k[EW_KCHUNK];
-    __shared__ double sh_t[EW_KCHUNK][2];
+    __shared__ int sh_i[EW_ICHUNK][5];
+    __shared__ double sh_t[EW_ICHUNK * EW_L][2];
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     double s1 = 0, s2 = 0, s3 = 0, ec = 1, es = 0;
     if (j < nat) {
@@ -289,38 +292,44 @@ k_ewald_kforce(int nat, int K, const double *__restrict__ s4, const double *__re
         sincospi(2.0 * frac_turn(s3), &es, &ec);
     }
     double a1 = 0, a2 = 0, a3 = 0, ap = 0;
-    int pn1 = 1 << 30, pn2 = 0, pn3 = 0;
-    double c = 1.0, s = 0.0;
-    for (int base = 0; base < K; base += EW_KCHUNK) {
+    for (int ib = 0; ib < nitems; ib += EW_ICHUNK) {
+        const int mi = min(EW_ICHUNK, nitems - ib);
+        const int k0 = items[ib].kofs;
+        const int nk = items[ib + mi - 1].kofs + items[ib + mi - 1].cnt - k0;
         __syncthreads();
-        for (int t = threadIdx.x; t < EW_KCHUNK; t += blockDim.x) {
-            int kk = base + t;
-            sh_k[t] = kk < K ? kpack[kk] : 0;
-            sh_t[t][0] = kk < K ? T[2 * (size_t)kk] : 0.0;
-            sh_t[t][1] = kk < K ? T[2 * (size_t)kk + 1] : 0.0;
+        if ((int)threadIdx.x < mi) {
+            const EwItem I = items[ib + threadIdx.x];
+            sh_i[threadIdx.x][0] = I.n1;
+            sh_i[threadIdx.x][1] = I.n2;
+            sh_i[threadIdx.x][2] = I.n3s;
+            sh_i[threadIdx.x][3] = I.cnt;
+            sh_i[threadIdx.x][4] = I.kofs - k0;
+        }
+        for (int t = threadIdx.x; t < nk; t += blockDim.x) {
+            sh_t[t][0] = T[2 * (size_t)(k0 + t)];
+            sh_t[t][1] = T[2 * (size_t)(k0 + t) + 1];
         }
         __syncthreads();
-        const int m = min(EW_KCHUNK, K - base);
-        for (int t = 0; t < m; ++t) {
-            int n1, n2, n3;
-            dc_unpack(sh_k[t], n1, n2, n3);
-            if (n1 == pn1 && n2 == pn2 && n3 == pn3 + 1) {
-                double c2 = c * ec - s * es;
+        for (int it = 0; it < mi; ++it) {
+            const int n1 = sh_i[it][0], n2 = sh_i[it][1], n3s = sh_i[it][2], cnt = sh_i[it][3];
+            const int ko = sh_i[it][4];
+            const double ph = frac_turn((double)n1 * s1 + (double)n2 * s2) + frac_turn((double)n3s * s3);
+            double c, s;
+            sincospi(2.0 * frac_turn(ph), &s, &c);
+            double sim = 0.0, s3im = 0.0;
+            for (int l = 0; l < cnt; ++l) {
+                const double tc = sh_t[ko + l][0], ts = sh_t[ko + l][1];
+                const double im = s * tc - c * ts;
+                ap += c * tc + s * ts;
+                sim += im;
+                s3im += (double)(n3s + l) * im;
+                const double c2 = c * ec - s * es;
                 s = c * es + s * ec;
                 c = c2;
-            } else {
-                double ph = frac_turn((double)n1 * s1 + (double)n2 * s2) + frac_turn((double)n3 * s3);
-                sincospi(2.0 * frac_turn(ph), &s, &c);
             }
-            pn1 = n1;
-            pn2 = n2;
-            pn3 = n3;
-            const double tc = sh_t[t][0], ts = sh_t[t][1];
-            const double im = s * tc - c * ts;
-            ap += c * tc + s * ts;
-            a1 += (double)n1 * im;
-            a2 += (double)n2 * im;
-            a3 += (double)n3 * im;
+            a1 += (double)n1 * sim;
+            a2 += (double)n2 * sim;
+            a3 += s3im;
         }
     }
     if (j < nat) {
@@ -467,8 +476,8 @@ static int ewald_run(agpu_potential *pot, const double *q, double *epot, double 
         AGPU_LAUNCH("ewald_kenergy", k_ewald_kenergy, dim3(nblk_k), dim3(256), 0, s, st->K, st->d_kpack.as<int>(),
                     Bt, 1.0 / (4.0 * st->beta * st->beta), 4.0 * M_PI / V, st->d_S.as<double>(),
                     st->d_T.as<double>(), part + 10 * (size_t)nblk_r);
-        AGPU_LAUNCH("ewald_kforce", k_ewald_kforce, dim3((nat + 255) / 256), dim3(256), 0, s, nat, st->K,
-                    st->d_frac.as<double>(), q, st->d_kpack.as<int>(), st->d_T.as<double>(), Bt, 8.0 * M_PI / V,
+        AGPU_LAUNCH("ewald_kforce", k_ewald_kforce, dim3((nat + 255) / 256), dim3(256), 0, s, nat, st->nitems,
+                    st->d_frac.as<double>(), q, st->d_items.as<EwItem>(), st->d_T.as<double>(), Bt, 8.0 * M_PI / V,
                     f, phi);
     }
     AGPU_CHECK(reduce_ev(part, nblk_r + (st->K > 0 ? nblk_k : 0), 10, epot, wpot, nullptr));

@@ -359,14 +359,12 @@ k_bin_pairs(int B, BinGeom g, CellArg C, double cutoff_sq, const SortedAtom *__r
             if (c > S) stage_over = true;
         }
     }
-    // one atomic per wavefront
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) wmax = max(wmax, __shfl_xor(wmax, o, 64));
+    // the row maximum is reduced afterwards from counts (k_int_max); only the
+    // rare staging overflow raises a flag here
+    (void)wmax;
+    (void)rowmax;
     const bool any_over = __ballot(stage_over) != 0ull;
-    if (lane == 0) {
-        atomicMax(rowmax, wmax);
-        if (any_over) atomicOr(flags, DERR_STAGE_OVERFLOW);
-    }
+    if (lane == 0 && any_over) atomicOr(flags, DERR_STAGE_OVERFLOW);
 }
 
 // Copy staged rows (sorted slot p, stride S) into the CSR rows of the caller
@@ -549,6 +547,7 @@ int nl_update(agpu_neighbors *nl, agpu_particles *p, int force)
         AGPU_CHECK(nl->sort_tmp.reserve(tmp2 + 16));
         AGPU_HIP(rocprim::exclusive_scan(nl->sort_tmp.ptr, tmp2, nl->counts.as<int>(),
                                          nl->offsets.as<int>(), 0, (size_t)nat, rocprim::plus<int>(), s));
+        AGPU_CHECK(int_max(nl->counts.as<int>(), nat, flags));
         AGPU_LAUNCH("nl_offsets_tail", k_offsets_tail, dim3(1), dim3(64), 0, s, nat,
                     nl->counts.as<int>(), nl->offsets.as<int>(), flags);
         AGPU_LAUNCH("nl_compact", k_nl_compact, dim3((nat + 15) / 16), dim3(TB), 0, s, nat,

@@ -113,36 +113,35 @@ __global__ void k_bop_bonds(int nat, const double *__restrict__ r, const int *__
         }
     }
     if (i < nat) bcnt[i] = nb;
-    // one atomic per wavefront
-    for (int o = 32; o >= 1; o >>= 1) nb = max(nb, __shfl_xor(nb, o, 64));
-    if ((threadIdx.x & 63) == 0) atomicMax(stats, nb);
+    (void)stats;   // the bond maximum is reduced from bcnt afterwards
 }
 
-__global__ void k_bop_rev(int nat, const int *__restrict__ offsets, const int2 *__restrict__ nbr,
+// Thread per (atom a, bond k): position of the reverse bond j->a (image -n)
+// in j's compacted bond list.
+__global__ void k_bop_rev(int nat, int G, const int *__restrict__ offsets, const int2 *__restrict__ nbr,
                           const int *__restrict__ bslot, const int *__restrict__ bcnt,
                           int *__restrict__ rev, int *__restrict__ stats)
 {
-    int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= nat) return;
-    int beg = offsets[a], nb = bcnt[a];
-    for (int k = 0; k < nb; ++k) {
-        int s = bslot[beg + k];
-        int2 e = nbr[s];
-        int x, y, z;
-        dc_unpack(e.y, x, y, z);
-        int want = dc_pack(-x, -y, -z);
-        int j = e.x, jb = offsets[j], jn = bcnt[j], found = -1;
-        for (int t = 0; t < jn; ++t) {
-            int sj = bslot[jb + t];
-            int2 ej = nbr[sj];
-            if (ej.x == a && ej.y == want) {
-                found = sj;
-                break;
-            }
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int a = (int)(t / G), k = (int)(t % G);
+    if (a >= nat || k >= bcnt[a]) return;
+    const int s = bslot[offsets[a] + k];
+    const int2 e = nbr[s];
+    int x, y, z;
+    dc_unpack(e.y, x, y, z);
+    const int want = dc_pack(-x, -y, -z);
+    const int j = e.x, jb = offsets[j], jn = bcnt[j];
+    int found = -1;
+    for (int q = 0; q < jn; ++q) {
+        const int sj = bslot[jb + q];
+        const int2 ej = nbr[sj];
+        if (ej.x == a && ej.y == want) {
+            found = sj;
+            break;
         }
-        rev[s] = found;
-        if (found < 0) atomicOr(stats + 1, DERR_BOND_OVERFLOW);
     }
+    rev[s] = found;
+    if (found < 0) atomicOr(stats + 1, DERR_BOND_OVERFLOW);
 }
 
 // -------------------------------------------------------------------------
@@ -428,35 +427,48 @@ k_tersoff(TersoffDev P, int nat, const int *__restrict__ order, const double *__
             ((s_red[0][tid] + s_red[1][tid]) + s_red[2][tid]) + s_red[3][tid];
 }
 
-__global__ void k_bop_gather(int nat, const int *__restrict__ offsets, const int *__restrict__ bslot,
-                             const int *__restrict__ bcnt, const int *__restrict__ rev,
-                             const double *__restrict__ fslot, const double *__restrict__ fown,
-                             const double *__restrict__ eslot, const double *__restrict__ wslot,
-                             double *__restrict__ f, double *__restrict__ epa, double *__restrict__ wpa)
+// Owner-computes assembly, G lanes per atom (lane = bond): the slot forces
+// the neighbours computed for a, summed over the group in a fixed butterfly.
+template <int G>
+__global__ void __launch_bounds__(256)
+k_bop_gather(int nat, const int *__restrict__ offsets, const int *__restrict__ bslot,
+             const int *__restrict__ bcnt, const int *__restrict__ rev,
+             const double *__restrict__ fslot, const double *__restrict__ fown,
+             const double *__restrict__ eslot, const double *__restrict__ wslot,
+             double *__restrict__ f, double *__restrict__ epa, double *__restrict__ wpa)
 {
-    int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= nat) return;
-    double acc[3] = {fown[3 * (size_t)a], fown[3 * (size_t)a + 1], fown[3 * (size_t)a + 2]};
+    const int lane = threadIdx.x % G;
+    const int a = blockIdx.x * (256 / G) + threadIdx.x / G;
+    const bool ok = a < nat;
+    double acc[3] = {0.0, 0.0, 0.0};
     double e = 0.0, w[9];
+#pragma unroll
     for (int q = 0; q < 9; ++q) w[q] = 0.0;
-    int beg = offsets[a], nb = bcnt[a];
-    for (int k = 0; k < nb; ++k) {
-        int s = bslot[beg + k];
-        int t = rev[s];
-        if (t < 0) continue;
-        acc[0] += fslot[3 * (size_t)t];
-        acc[1] += fslot[3 * (size_t)t + 1];
-        acc[2] += fslot[3 * (size_t)t + 2];
-        if (epa) e += eslot[s] + eslot[t];
-        if (wpa)
-            for (int q = 0; q < 9; ++q) w[q] += wslot[9 * (size_t)s + q] + wslot[9 * (size_t)t + q];
+    if (ok && lane < bcnt[a]) {
+        const int s = bslot[offsets[a] + lane];
+        const int t = rev[s];
+        if (t >= 0) {
+            acc[0] = fslot[3 * (size_t)t];
+            acc[1] = fslot[3 * (size_t)t + 1];
+            acc[2] = fslot[3 * (size_t)t + 2];
+            if (epa) e = eslot[s] + eslot[t];
+            if (wpa)
+                for (int q = 0; q < 9; ++q) w[q] = wslot[9 * (size_t)s + q] + wslot[9 * (size_t)t + q];
+        }
     }
-    f[3 * (size_t)a] += acc[0];
-    f[3 * (size_t)a + 1] += acc[1];
-    f[3 * (size_t)a + 2] += acc[2];
-    if (epa) epa[a] += 0.5 * e;
+#pragma unroll
+    for (int x = 0; x < 3; ++x) acc[x] = group_sum<G>(acc[x]);
+    if (epa) e = group_sum<G>(e);
     if (wpa)
-        for (int q = 0; q < 9; ++q) wpa[9 * (size_t)a + q] += 0.5 * w[q];
+        for (int q = 0; q < 9; ++q) w[q] = group_sum<G>(w[q]);
+    if (ok && lane == 0) {
+        f[3 * (size_t)a] += fown[3 * (size_t)a] + acc[0];
+        f[3 * (size_t)a + 1] += fown[3 * (size_t)a + 1] + acc[1];
+        f[3 * (size_t)a + 2] += fown[3 * (size_t)a + 2] + acc[2];
+        if (epa) epa[a] += 0.5 * e;
+        if (wpa)
+            for (int q = 0; q < 9; ++q) wpa[9 * (size_t)a + q] += 0.5 * w[q];
+    }
 }
 
 // -------------------------------------------------------------------------
@@ -565,6 +577,30 @@ static int dispatch_tersoff(int G, TersoffState *st, int nat, const int *order, 
     }
 }
 
+template <int G>
+static int launch_gather_g(int nat, agpu_neighbors *nl, TersoffState *st, double *eslot, double *wslot,
+                           const agpu_eval_args *a, hipStream_t s)
+{
+    const int nblk = (nat + 256 / G - 1) / (256 / G);
+    AGPU_LAUNCH("bop_gather", k_bop_gather<G>, dim3(nblk), dim3(256), 0, s, nat, nl->offsets.as<int>(),
+                st->bslot.as<int>(), st->bcnt.as<int>(), st->rev.as<int>(), st->fslot.as<double>(),
+                st->fown.as<double>(), eslot, wslot, a->f, a->epot_per_at, a->wpot_per_at);
+    return AGPU_OK;
+}
+
+static int launch_gather(int G, int nat, agpu_neighbors *nl, TersoffState *st, double *eslot, double *wslot,
+                         const agpu_eval_args *a, hipStream_t s)
+{
+    switch (G) {
+    case 2: return launch_gather_g<2>(nat, nl, st, eslot, wslot, a, s);
+    case 4: return launch_gather_g<4>(nat, nl, st, eslot, wslot, a, s);
+    case 8: return launch_gather_g<8>(nat, nl, st, eslot, wslot, a, s);
+    case 16: return launch_gather_g<16>(nat, nl, st, eslot, wslot, a, s);
+    case 32: return launch_gather_g<32>(nat, nl, st, eslot, wslot, a, s);
+    default: return launch_gather_g<64>(nat, nl, st, eslot, wslot, a, s);
+    }
+}
+
 int tersoff_compute(agpu_potential *pot, const agpu_eval_args *a)
 {
     TersoffState *st = static_cast<TersoffState *>(pot->impl);
@@ -601,12 +637,11 @@ int tersoff_compute(agpu_potential *pot, const agpu_eval_args *a)
     const int nb = (nat + 255) / 256;
     AGPU_LAUNCH("bop_bonds", k_bop_bonds, dim3(nb), dim3(256), 0, s, nat, p->d_r, el, C, st->dev,
                 nl->offsets.as<int>(), nl->nbr.as<int2>(), st->bslot.as<int>(), st->bcnt.as<int>(), dstats);
-    AGPU_LAUNCH("bop_rev", k_bop_rev, dim3(nb), dim3(256), 0, s, nat, nl->offsets.as<int>(),
-                nl->nbr.as<int2>(), st->bslot.as<int>(), st->bcnt.as<int>(), st->rev.as<int>(), dstats);
     // group width: the row maximum bounds the bond count without a host
     // round trip when it is small; otherwise read the exact bond maximum
     int maxb = nl->maxrow;
     if (maxb > 64) {
+        AGPU_CHECK(int_max(st->bcnt.as<int>(), nat, dstats));
         AGPU_HIP(hipMemcpyAsync(pot->stats.h, dstats, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
         AGPU_HIP(hipStreamSynchronize(s));
         maxb = pot->stats.h[0];
@@ -615,15 +650,16 @@ int tersoff_compute(agpu_potential *pot, const agpu_eval_args *a)
     }
     int G = 2;
     while (G < maxb) G <<= 1;
+    AGPU_LAUNCH("bop_rev", k_bop_rev, dim3((unsigned)(((long long)nat * G + 255) / 256)), dim3(256), 0, s, nat, G,
+                nl->offsets.as<int>(), nl->nbr.as<int2>(), st->bslot.as<int>(), st->bcnt.as<int>(),
+                st->rev.as<int>(), dstats);
     int nblk = 0;
     const int *order = nl->idx_sorted.as<int>();
     if (pot->precision == AGPU_PREC_MIXED)
         AGPU_CHECK(dispatch_tersoff<float>(G, st, nat, order, p->d_r, el, C, nl, a->mask, eslot, wslot, s, &nblk));
     else
         AGPU_CHECK(dispatch_tersoff<double>(G, st, nat, order, p->d_r, el, C, nl, a->mask, eslot, wslot, s, &nblk));
-    AGPU_LAUNCH("bop_gather", k_bop_gather, dim3(nb), dim3(256), 0, s, nat, nl->offsets.as<int>(),
-                st->bslot.as<int>(), st->bcnt.as<int>(), st->rev.as<int>(), st->fslot.as<double>(),
-                st->fown.as<double>(), eslot, wslot, a->f, a->epot_per_at, a->wpot_per_at);
+    AGPU_CHECK(launch_gather(G, nat, nl, st, eslot, wslot, a, s));
     // per-block records [e, W(9)] -> epot, wpot
     AGPU_CHECK(reduce_ev(st->partial.as<double>(), nblk, 10, a->epot, a->wpot, nullptr));
     AGPU_HIP(hipMemcpyAsync(pot->stats.h, dstats, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
