@@ -194,3 +194,34 @@ def ewald_beta(cutoff, tol=1e-8):
 def ewald_kmax(beta, tol=1e-8):
     """kmax with exp(-kmax^2 / 4 beta^2) = tol."""
     return 2.0 * beta * math.sqrt(-math.log(tol))
+
+
+# --------------------------------------------------------------------------
+# hydrocarbons (REBO2)
+# --------------------------------------------------------------------------
+
+C_MASS = 12.011
+H_MASS = 1.008
+
+
+def diamond_c(n=50, a=3.566, sigma=0.05, seed=6):
+    """REBO2 carbon workload: n^3 conventional diamond cells (8 n^3 atoms)."""
+    pos, cell = lattice(DIAMOND_BASIS, n, a)
+    if sigma > 0:
+        pos = pos + np.random.default_rng(seed).normal(0.0, sigma, pos.shape)
+    Z = np.full(len(pos), 6, np.int32)
+    return pos, cell, np.array([True] * 3), Z
+
+
+def random_ch(nC=50, nH=10, density=3.0, seed=7, dmin=0.9, strain=0.0):
+    """Random periodic C/H solid at the given mass density (g/cm^3) with a
+    minimum interatomic distance dmin over every periodic image; the cell is
+    cubic, or triclinic with random entries of +/-strain."""
+    rng = np.random.default_rng(seed)
+    V = (nC * C_MASS + nH * H_MASS) * AMU_G / density * 1e24
+    cell = np.eye(3) + rng.uniform(-strain, strain, (3, 3))
+    cell *= (V / abs(np.linalg.det(cell))) ** (1.0 / 3.0)
+    pos = _random_positions(nC + nH, cell, dmin, rng, 200)
+    Z = np.array([6] * nC + [1] * nH, np.int32)
+    perm = rng.permutation(nC + nH)
+    return pos[perm], cell, np.array([True] * 3), Z[perm]

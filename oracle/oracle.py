@@ -340,3 +340,86 @@ def read_setfl_tables(path):
             toks = []
             phi[a][b] = phi[b][a] = (y, dr)
     return {"nel": nel, "cutoff": cutoff, "F": F, "rho": rho, "phi": phi, "names": names, "Z": Zs}
+
+
+# --------------------------------------------------------------------------
+# REBO2 (C, H)
+# --------------------------------------------------------------------------
+
+class Rebo2:
+    """Handle on the oracle's REBO2 parameter set and tables."""
+
+    def __init__(self, zero_tables=False):
+        L = lib()
+        L.orc_rebo2_new.restype = ctypes.c_void_p
+        L.orc_rebo2_new.argtypes = [ctypes.c_int]
+        L.orc_rebo2_free.argtypes = [ctypes.c_void_p]
+        L.orc_rebo2_eval_g.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
+                                       ctypes.c_double, _dp]
+        L.orc_rebo2_eval_table.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
+                                           ctypes.c_double, ctypes.c_double, _dp]
+        L.orc_rebo2_default_table.argtypes = [ctypes.c_int, _dp]
+        self._h = L.orc_rebo2_new(int(bool(zero_tables)))
+        if not self._h:
+            raise OracleError("orc_rebo2_new failed")
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().orc_rebo2_free(self._h)
+            self._h = None
+
+    def g(self, ktyp, costh, n):
+        """(g, dg/dcos, dg/dN) for centre type ktyp (1 = C, 3 = H)."""
+        out = np.zeros(3)
+        lib().orc_rebo2_eval_g(self._h, int(ktyp), float(costh), float(n), _d(out))
+        return out
+
+    TABLES = {"Fcc": 0, "Fch": 1, "Fhh": 2, "Pcc": 3, "Pch": 4}
+
+    def table(self, name, x, y, z=0.0):
+        """Interpolated table value and its partial derivatives."""
+        out = np.zeros(4)
+        _check(lib().orc_rebo2_eval_table(self._h, self.TABLES[name], float(x), float(y),
+                                          float(z), _d(out)), "rebo2 table")
+        return out
+
+    @staticmethod
+    def default_table(name):
+        """Tabulated node data: F tables (5, 5, 10) (Fcc also returns dF/di,j,k), P (6, 6)."""
+        w = Rebo2.TABLES[name]
+        if w == 0:
+            out = np.zeros(1000)
+            _check(lib().orc_rebo2_default_table(w, _d(out)), "rebo2 default table")
+            return tuple(out[250 * q:250 * (q + 1)].reshape(5, 5, 10) for q in range(4))
+        n = 250 if w < 3 else 36
+        out = np.zeros(n)
+        _check(lib().orc_rebo2_default_table(w, _d(out)), "rebo2 default table")
+        return out.reshape((5, 5, 10) if w < 3 else (6, 6))
+
+    def __call__(self, r, Z, cell, nl, per_at=False, nthreads=1):
+        """Energy, forces and virial (plus per-atom energies/virials)."""
+        L = lib()
+        L.orc_rebo2.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _ip, _dp, _lp, _ip, _ip,
+                                _dp, _dp, _dp, _dp, _dp, ctypes.c_int]
+        r = _c(r, np.float64)
+        nat = len(r)
+        Z = _c(Z, np.int32)
+        cell = _c(cell, np.float64)
+        offsets, nbr, dc = nl
+        offsets = _c(offsets, np.int64)
+        nbr = _c(nbr, np.int32)
+        dc = _c(dc, np.int32)
+        epot = ctypes.c_double(0.0)
+        f = np.zeros((nat, 3))
+        w = np.zeros(9)
+        epa = np.zeros(nat) if per_at else None
+        wpa = np.zeros(9 * nat) if per_at else None
+        err = L.orc_rebo2(self._h, nat, _d(r), _i(Z), _d(cell), _l(offsets), _i(nbr), _i(dc),
+                          ctypes.byref(epot), _d(f), _d(w), _d(epa) if per_at else None,
+                          _d(wpa) if per_at else None, int(nthreads))
+        _check(err, "rebo2")
+        wpot = w.reshape(3, 3).T.copy()
+        out = [epot.value, f, wpot]
+        if per_at:
+            out += [epa, wpa.reshape(nat, 3, 3).transpose(0, 2, 1).copy()]
+        return tuple(out)

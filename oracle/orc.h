@@ -87,4 +87,17 @@ int orc_ewald_kspace(int nat, const double *r, const double *q,
                      double *epot, double *f, double *wpot, double *phi,
                      int nthreads);
 
+/* ---- REBO2 (C, H; no screening, no dihedral) ---- */
+typedef struct orc_rebo2_s orc_rebo2_t;
+orc_rebo2_t *orc_rebo2_new(int zero_tables);
+void orc_rebo2_free(orc_rebo2_t *p);
+int orc_rebo2(const orc_rebo2_t *p, int nat, const double *r, const int *Z,
+              const double *cell, const int64_t *offsets, const int *nbr, const int *dc,
+              double *epot, double *f, double *wpot, double *epot_per_at,
+              double *wpot_per_at, int nthreads);
+int orc_rebo2_eval_g(const orc_rebo2_t *p, int ktyp, double costh, double n, double *out3);
+int orc_rebo2_eval_table(const orc_rebo2_t *p, int which, double x, double y, double z,
+                         double *out4);
+int orc_rebo2_default_table(int which, double *out);
+
 #endif

@@ -81,3 +81,57 @@ def fd_virial(energy, pos, cell, h=1e-6):
 
 
 from oracle.oracle import read_setfl_tables  # noqa: E402,F401
+
+
+def read_xyz(name):
+    """Plain xyz file under tests/golden (symbol x y z per line)."""
+    with open(os.path.join(GOLDEN, name)) as f:
+        lines = f.read().splitlines()
+    n = int(lines[0])
+    pos, Z = [], []
+    for line in lines[2:2 + n]:
+        t = line.split()
+        Z.append(ATOMIC_NUMBERS[t[0]])
+        pos.append([float(x) for x in t[1:4]])
+    return np.array(pos), np.array(Z, np.int32)
+
+
+def small_hydrocarbon(name):
+    """Idealised starting geometries of the molecules the reference builds
+    with ase.build.molecule (relaxation brings them to the REBO2 minimum)."""
+    s3 = 1.0 / np.sqrt(3.0)
+    if name == "CH4":
+        P = [[0, 0, 0]] + [[1.09 * a * s3, 1.09 * b * s3, 1.09 * a * b * s3]
+                           for a in (1, -1) for b in (1, -1)]
+        Z = [6, 1, 1, 1, 1]
+    elif name == "C2H2":
+        P = [[0, 0, -0.60], [0, 0, 0.60], [0, 0, -1.66], [0, 0, 1.66]]
+        Z = [6, 6, 1, 1]
+    elif name == "C2H4":
+        P = [[-0.667, 0, 0], [0.667, 0, 0], [-1.23, 0.92, 0], [-1.23, -0.92, 0],
+             [1.23, 0.92, 0], [1.23, -0.92, 0]]
+        Z = [6, 6, 1, 1, 1, 1]
+    elif name == "C2H6":
+        P, Z = [[0, 0, -0.765], [0, 0, 0.765]], [6, 6]
+        for k in range(3):
+            t = 2 * np.pi * k / 3
+            P.append([1.02 * np.cos(t), 1.02 * np.sin(t), -1.16])
+            P.append([1.02 * np.cos(t + np.pi / 3), 1.02 * np.sin(t + np.pi / 3), 1.16])
+            Z += [1, 1]
+    elif name == "C6H6":
+        P, Z = [], []
+        for k in range(6):
+            t = 2 * np.pi * k / 6
+            P.append([1.39 * np.cos(t), 1.39 * np.sin(t), 0])
+            P.append([2.48 * np.cos(t), 2.48 * np.sin(t), 0])
+            Z += [6, 1]
+    else:
+        raise KeyError(name)
+    return np.array(P, float), np.array(Z, np.int32)
+
+
+def in_vacuum_box(pos, vacuum=5.0):
+    """Shift a molecule into a non-periodic orthorhombic box with `vacuum`
+    on every side."""
+    pos = pos - pos.min(0) + vacuum
+    return pos, np.diag(pos.max(0) + vacuum), np.array([False] * 3)
