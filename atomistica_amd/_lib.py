@@ -29,6 +29,9 @@ PREC_MIXED = 1
 KIND_TERSOFF = 1
 KIND_EAM_ALLOY = 2
 KIND_EWALD = 3
+KIND_REBO2 = 4
+REBO2_ZERO_TABLES = 1
+REBO2_DIHEDRAL = 2
 
 _ERRNAMES = {1: "ARG", 2: "HIP", 3: "NL_OVERFLOW", 4: "DOMAIN", 5: "SPLINE",
              6: "UNSUPPORTED", 7: "ELEMENT", 8: "PBC", 9: "IO", 10: "STATE", 11: "NEB"}
@@ -94,6 +97,7 @@ PROTOTYPES = {
     "agpu_tersoff_set_params": [vp, ctypes.POINTER(TersoffParams)],
     "agpu_eam_alloy_create": [ctypes.POINTER(vp), ctypes.c_char_p, ctypes.c_char_p],
     "agpu_ewald_create": [ctypes.POINTER(vp), d, d, d],
+    "agpu_rebo2_create": [ctypes.POINTER(vp), i],
     "agpu_potential_destroy": [vp],
     "agpu_potential_kind": [vp, ip],
     "agpu_potential_set_precision": [vp, i],

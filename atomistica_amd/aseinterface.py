@@ -256,3 +256,4 @@ def _make(cls, name):
 Tersoff = _make(native.Tersoff, "Tersoff")
 TabulatedAlloyEAM = _make(native.TabulatedAlloyEAM, "TabulatedAlloyEAM")
 Ewald = _make(native.Ewald, "Ewald")
+Rebo2 = _make(native.Rebo2, "Rebo2")

@@ -38,6 +38,7 @@ enum DevErr : int {
     DERR_SPLINE_RANGE = 4,
     DERR_BOND_OVERFLOW = 8,
     DERR_STAGE_OVERFLOW = 16,
+    DERR_FIXED_RANGE = 32,
 };
 
 hipStream_t stream();
@@ -200,6 +201,8 @@ int eam_compute(agpu_potential *pot, const agpu_eval_args *a);
 int ewald_bind(agpu_potential *pot);
 int ewald_compute(agpu_potential *pot, const agpu_eval_args *a);
 int ewald_potential(agpu_potential *pot, const double *q, double *phi, int flags);
+int rebo2_bind(agpu_potential *pot);
+int rebo2_compute(agpu_potential *pot, const agpu_eval_args *a);
 
 // Deterministic fixed-order reduction of nblk per-block records of width w
 // (double): column 0 -> *epot (+=), columns 1..9 -> wpot[0..8] (+=),

@@ -612,6 +612,7 @@ int agpu_potential_bind_to(agpu_potential *pot, agpu_particles *p, agpu_neighbor
     case AGPU_KIND_TERSOFF: rc = tersoff_bind(pot); break;
     case AGPU_KIND_EAM_ALLOY: rc = eam_bind(pot); break;
     case AGPU_KIND_EWALD: rc = ewald_bind(pot); break;
+    case AGPU_KIND_REBO2: rc = rebo2_bind(pot); break;
     default: return set_error(AGPU_ERR_STATE, "bind_to: unknown potential kind");
     }
     if (rc == AGPU_OK) pot->bound = 1;
@@ -630,6 +631,7 @@ static int stage_and_call(agpu_potential *pot, agpu_particles *p, agpu_neighbors
         case AGPU_KIND_TERSOFF: rc = tersoff_compute(pot, &a); break;
         case AGPU_KIND_EAM_ALLOY: rc = eam_compute(pot, &a); break;
         case AGPU_KIND_EWALD: rc = ewald_compute(pot, &a); break;
+        case AGPU_KIND_REBO2: rc = rebo2_compute(pot, &a); break;
         default: rc = set_error(AGPU_ERR_STATE, "unknown potential kind");
         }
         return rc;
@@ -673,6 +675,7 @@ static int stage_and_call(agpu_potential *pot, agpu_particles *p, agpu_neighbors
     case AGPU_KIND_TERSOFF: rc = tersoff_compute(pot, &d); break;
     case AGPU_KIND_EAM_ALLOY: rc = eam_compute(pot, &d); break;
     case AGPU_KIND_EWALD: rc = ewald_compute(pot, &d); break;
+    case AGPU_KIND_REBO2: rc = rebo2_compute(pot, &d); break;
     default: rc = set_error(AGPU_ERR_STATE, "unknown potential kind");
     }
     if (rc != AGPU_OK) return rc;

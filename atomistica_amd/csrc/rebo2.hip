@@ -1,0 +1,1098 @@
+// Second-generation REBO (Brenner et al. 2002) for carbon and hydrogen: the
+// non-screened rebo2 class of the reference (src/potentials/bop/rebo2/),
+// without the dihedral term (the reference default, with_dihedral = false).
+//
+// Kernels (one stream, no host round trip inside a call):
+//   k_rb_zero     clear the 64-bit fixed-point accumulators
+//   k_rb_bonds    thread per atom: compact the neighbour row to its bonds
+//                 (r^2 < r2^2 of the pair type), bond record = unit vector,
+//                 length, cut-off and derivative; coordination N_C, N_H
+//                 (loop1 and the nn sum of bop_kernel_rebo2.f90:700-1205)
+//   k_rb_conj     thread per atom: F(x_ik) of every carbon neighbour k and
+//                 the sum N^conj_i (ik_loop1, bop_kernel_rebo2.f90:1231-1325)
+//   k_rb_pair     thread per bond i->j with (dc = 0 and j > i) or dc > 0:
+//                 b_ij, b_ji with P, F_ij(N_i, N_j, N_conj), the pair energy
+//                 and every force on i, j, k, l, m, n (ij_loop,
+//                 bop_kernel_rebo2.f90:1326-2779).  Forces, per-atom energies
+//                 and virials go to 64-bit fixed-point accumulators with
+//                 integer atomics (exact and order independent, so results
+//                 are bitwise reproducible); total energy and virial are
+//                 reduced per block and then in a fixed order.
+//   k_rb_finish   fixed point -> FP64, added to the caller's arrays.
+//
+// Tables (F_CC, F_CH, F_HH tricubic, P_CC, P_CH bicubic) are built on the
+// host from their node data in closed form: the tensor product of cubic
+// Hermite polynomials with the tabulated values and first derivatives and
+// zero cross derivatives, which is the unique solution of the corner system
+// the reference solves with gaussn (src/special/table3d.f90:40-196).
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "agpu_internal.hpp"
+#include "profile.hpp"
+
+namespace agpu {
+
+// pair codes of the reference (Z2pair): C = 1, H = 3; CC = 1, CH = 3, HH = 6
+enum { RB_C = 1, RB_H = 3, RB_CC = 1, RB_CH = 3, RB_HH = 6 };
+
+struct RbDev {
+    double ccB[3], ccBeta[3];        // VA of C-C: three exponentials
+    double B1[7], beta1[7];          // VA of C-H and H-H
+    double A[7], Q[7], alpha[7];     // VR
+    double cut_l[7], cut_h[7], cut_h2[7], cut_fac[7];
+    double conear[7][7];
+    double conalp, conpe, conan, conpf;
+    double th1, th2, n_hi;           // g interval boundaries; upper N switch
+    double g1c[3][6], g2c[3][6];     // carbon g(cos): high / low coordination
+    double spgh[3][6];               // hydrogen g(cos) polynomials
+    int igh[25];
+};
+
+// table layout in one device block: 3 tricubic tables of 4*4*9 boxes x 64
+// coefficients, then 2 bicubic tables of 5*5 boxes x 16 coefficients
+constexpr int RB_T3_BOXES = 4 * 4 * 9;
+constexpr int RB_T3 = RB_T3_BOXES * 64;
+constexpr int RB_T2 = 25 * 16;
+constexpr int RB_TAB_FCC = 0, RB_TAB_FCH = RB_T3, RB_TAB_FHH = 2 * RB_T3, RB_TAB_PCC = 3 * RB_T3,
+              RB_TAB_PCH = 3 * RB_T3 + RB_T2, RB_TAB_N = 3 * RB_T3 + 2 * RB_T2;
+
+// fixed point: 2^-40 eV/A (or eV) per unit
+constexpr double RB_FX = 1099511627776.0;
+constexpr double RB_FX_INV = 1.0 / 1099511627776.0;
+constexpr double RB_FX_MAX = 2097152.0;   // 2^21: contributions beyond flag an error
+
+struct RbBond {
+    double n[3];
+    double len;
+    double fc, dfc, fx, dfx;
+};
+
+struct Rebo2State {
+    int zero_tables = 0;
+    RbDev hdev{};
+    std::vector<double> htab;
+    DevBuf dev, tab, bond, bij, bown, bcnt, nn, nconj, facc, eacc, wacc, partial, stats;
+    int z2db[128];
+    uint64_t Z_rev = ~0ull;
+    double cutoff = 0.0;
+};
+
+// ---------------------------------------------------------------------------
+// host set-up
+// ---------------------------------------------------------------------------
+static int solve6(double A[6][6], double b[6])
+{
+    for (int c = 0; c < 6; ++c) {
+        int p = c;
+        for (int r = c + 1; r < 6; ++r)
+            if (std::fabs(A[r][c]) > std::fabs(A[p][c])) p = r;
+        if (A[p][c] == 0.0) return -1;
+        if (p != c) {
+            for (int k = 0; k < 6; ++k) std::swap(A[c][k], A[p][k]);
+            std::swap(b[c], b[p]);
+        }
+        for (int r = c + 1; r < 6; ++r) {
+            double m = A[r][c] / A[c][c];
+            for (int k = c; k < 6; ++k) A[r][k] -= m * A[c][k];
+            b[r] -= m * b[c];
+        }
+    }
+    for (int c = 5; c >= 0; --c) {
+        double s = b[c];
+        for (int k = c + 1; k < 6; ++k) s -= A[c][k] * b[k];
+        b[c] = s / A[c][c];
+    }
+    return 0;
+}
+
+static double hpow(double x, int p)
+{
+    double r = 1.0;
+    for (int i = 0; i < p; ++i) r *= x;
+    return r;
+}
+
+// Quintic pieces of g_C on [-1,-1/2], [-1/2,-1/3] (value, slope, curvature
+// at both ends) and [-1/3, 1] (values at -1/3, 0, 1/2, 1 plus slope and
+// curvature at -1/3), rebo2_db.f90:405-524.
+static int g_spline(const double th[6], const double *g, const double *dg, const double *d2g,
+                    const double *gtail, double c[3][6])
+{
+    double A[6][6], b[6];
+    std::memset(A, 0, sizeof A);
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 6; ++j) A[i][j] = hpow(th[2 + i], j);
+    for (int j = 1; j < 6; ++j) A[4][j] = j * hpow(th[2], j - 1);
+    for (int j = 2; j < 6; ++j) A[5][j] = j * (j - 1) * hpow(th[2], j - 2);
+    for (int i = 0; i < 4; ++i) b[i] = gtail[2 + i];
+    b[4] = dg[2];
+    b[5] = d2g[2];
+    if (solve6(A, b)) return -1;
+    std::memcpy(c[2], b, sizeof b);
+    for (int k = 0; k < 2; ++k) {
+        std::memset(A, 0, sizeof A);
+        for (int e = 0; e < 2; ++e) {
+            double z = th[k + e];
+            for (int j = 0; j < 6; ++j) A[3 * e][j] = hpow(z, j);
+            for (int j = 1; j < 6; ++j) A[3 * e + 1][j] = j * hpow(z, j - 1);
+            for (int j = 2; j < 6; ++j) A[3 * e + 2][j] = j * (j - 1) * hpow(z, j - 2);
+        }
+        b[0] = g[k]; b[1] = dg[k]; b[2] = d2g[k];
+        b[3] = g[k + 1]; b[4] = dg[k + 1]; b[5] = d2g[k + 1];
+        if (solve6(A, b)) return -1;
+        std::memcpy(c[k], b, sizeof b);
+    }
+    return 0;
+}
+
+// monomial coefficients of the cubic Hermite basis on [0,1]:
+// value at 0, value at 1, slope at 0, slope at 1
+static const double HV[2][4] = {{1, 0, -3, 2}, {0, 0, 3, -2}};
+static const double HD[2][4] = {{0, 1, -2, 1}, {0, 0, -1, 1}};
+
+// V, Dx, Dy, Dz indexed [(ix*(ny+1)+iy)*(nz+1)+iz]; box (bx,by,bz) stored at
+// ((bx*ny)+by)*nz+bz with coefficient [16 p1 + 4 p2 + p3]
+static void hermite3(int nx, int ny, int nz, const double *V, const double *Dx, const double *Dy,
+                     const double *Dz, double *out)
+{
+    for (int bx = 0; bx < nx; ++bx)
+        for (int by = 0; by < ny; ++by)
+            for (int bz = 0; bz < nz; ++bz) {
+                double *c = out + (size_t)(((bx * ny) + by) * nz + bz) * 64;
+                for (int q = 0; q < 64; ++q) c[q] = 0.0;
+                for (int a = 0; a < 2; ++a)
+                    for (int b = 0; b < 2; ++b)
+                        for (int e = 0; e < 2; ++e) {
+                            int g = ((bx + a) * (ny + 1) + by + b) * (nz + 1) + bz + e;
+                            double f = V[g], fx = Dx ? Dx[g] : 0.0, fy = Dy ? Dy[g] : 0.0,
+                                   fz = Dz ? Dz[g] : 0.0;
+                            for (int p1 = 0; p1 < 4; ++p1)
+                                for (int p2 = 0; p2 < 4; ++p2)
+                                    for (int p3 = 0; p3 < 4; ++p3) {
+                                        double v = f * HV[a][p1] * HV[b][p2] * HV[e][p3] +
+                                                   fx * HD[a][p1] * HV[b][p2] * HV[e][p3] +
+                                                   fy * HV[a][p1] * HD[b][p2] * HV[e][p3] +
+                                                   fz * HV[a][p1] * HV[b][p2] * HD[e][p3];
+                                        c[16 * p1 + 4 * p2 + p3] += v;
+                                    }
+                        }
+            }
+}
+
+static void hermite2(int nx, int ny, const double *V, double *out)
+{
+    for (int bx = 0; bx < nx; ++bx)
+        for (int by = 0; by < ny; ++by) {
+            double *c = out + (size_t)(bx * ny + by) * 16;
+            for (int q = 0; q < 16; ++q) c[q] = 0.0;
+            for (int a = 0; a < 2; ++a)
+                for (int b = 0; b < 2; ++b) {
+                    double f = V[(bx + a) * (ny + 1) + by + b];
+                    for (int p1 = 0; p1 < 4; ++p1)
+                        for (int p2 = 0; p2 < 4; ++p2) c[4 * p1 + p2] += f * HV[a][p1] * HV[b][p2];
+                }
+        }
+}
+
+// Node data of rebo2_default_tables.f90 (Brenner et al. 2002, Tables 4-9):
+// F(0:4, 0:4, 0:9) at [(i*5+j)*10+k], P(0:5, 0:5) at [i*6+j].
+#define T3(i, j, k) (((i) * 5 + (j)) * 10 + (k))
+static void default_tables(double *Fcc, double *Fi, double *Fj, double *Fk, double *Fch,
+                           double *Fhh, double *Pcc, double *Pch)
+{
+    for (int q = 0; q < 250; ++q) Fcc[q] = Fi[q] = Fj[q] = Fk[q] = Fch[q] = Fhh[q] = 0.0;
+    for (int q = 0; q < 36; ++q) Pcc[q] = Pch[q] = 0.0;
+    // F_CC values (Table 4) with the linear ramps in N_conj of the reference
+    Fcc[T3(1, 1, 0)] = 0.105000;
+    Fcc[T3(1, 1, 1)] = -0.0041775;
+    for (int k = 2; k <= 8; ++k) Fcc[T3(1, 1, k)] = -0.0160856;
+    Fcc[T3(2, 2, 0)] = 0.09444957;
+    Fcc[T3(2, 2, 1)] = 0.02200000;
+    Fcc[T3(2, 2, 2)] = 0.03970587;
+    Fcc[T3(0, 1, 0)] = 0.04338699;
+    for (int k = 1; k <= 8; ++k) Fcc[T3(0, 1, k)] = 0.0099172158;
+    Fcc[T3(0, 2, 0)] = 0.0493976637;
+    Fcc[T3(0, 2, 1)] = -0.011942669;
+    for (int k = 2; k <= 8; ++k) Fcc[T3(0, 2, k)] = Fcc[T3(0, 1, 1)];
+    Fcc[T3(0, 3, 0)] = Fcc[T3(0, 3, 1)] = -0.119798935;
+    for (int k = 2; k <= 8; ++k) Fcc[T3(0, 3, k)] = Fcc[T3(0, 1, 1)];
+    Fcc[T3(1, 2, 0)] = 0.0096495698;
+    Fcc[T3(1, 2, 1)] = 0.030;
+    Fcc[T3(1, 2, 2)] = -0.0200;
+    for (int k = 5; k <= 8; ++k) Fcc[T3(1, 2, k)] = -0.030133632;
+    for (int k = 1; k <= 8; ++k) Fcc[T3(1, 3, k)] = -0.124836752;
+    for (int k = 0; k <= 8; ++k) Fcc[T3(2, 3, k)] = -0.044709383;
+    for (int k = 3; k <= 7; ++k)
+        Fcc[T3(2, 2, k)] = Fcc[T3(2, 2, 2)] + (k - 2) * (Fcc[T3(2, 2, 8)] - Fcc[T3(2, 2, 2)]) / 6;
+    for (int k = 3; k <= 4; ++k)
+        Fcc[T3(1, 2, k)] = Fcc[T3(1, 2, 2)] + (k - 2) * (Fcc[T3(1, 2, 5)] - Fcc[T3(1, 2, 2)]) / 3;
+    Fi[T3(2, 1, 0)] = -0.052500;
+    for (int k = 4; k <= 8; ++k) Fi[T3(2, 1, k)] = -0.054376;
+    for (int k = 1; k <= 8; ++k) Fi[T3(2, 3, k)] = 0.062418;
+    for (int k = 3; k <= 7; ++k) Fk[T3(2, 2, k)] = -0.006618;
+    Fk[T3(1, 1, 1)] = -0.060543;
+    Fk[T3(1, 2, 3)] = Fk[T3(1, 2, 4)] = -0.020044;
+    // symmetrisation F(i,j) = F(j,i), dF/dNi(i,j) = dF/dNj(j,i)
+    for (int k = 0; k <= 9; ++k)
+        for (int i = 0; i <= 3; ++i)
+            for (int j = i + 1; j <= 3; ++j) {
+                double x = Fcc[T3(i, j, k)] + Fcc[T3(j, i, k)];
+                Fcc[T3(i, j, k)] = Fcc[T3(j, i, k)] = x;
+                x = Fi[T3(i, j, k)] + Fj[T3(j, i, k)];
+                Fi[T3(i, j, k)] = Fj[T3(j, i, k)] = x;
+                x = Fi[T3(j, i, k)] + Fj[T3(i, j, k)];
+                Fi[T3(j, i, k)] = Fj[T3(i, j, k)] = x;
+                x = Fk[T3(i, j, k)] + Fk[T3(j, i, k)];
+                Fk[T3(i, j, k)] = Fk[T3(j, i, k)] = x;
+            }
+    // F_CH (Table 9)
+    for (int k = 4; k <= 8; ++k) Fch[T3(0, 2, k)] = -0.0090477875161288110;
+    for (int k = 0; k <= 8; ++k) {
+        Fch[T3(1, 3, k)] = -0.213;
+        Fch[T3(1, 2, k)] = -0.25;
+        Fch[T3(1, 1, k)] = -0.5;
+    }
+    for (int k = 0; k <= 9; ++k)
+        for (int i = 0; i <= 2; ++i)
+            for (int j = i + 1; j <= 3; ++j) {
+                double x = Fch[T3(i, j, k)] + Fch[T3(j, i, k)];
+                Fch[T3(i, j, k)] = Fch[T3(j, i, k)] = x;
+            }
+    // F_HH (Table 6)
+    Fhh[T3(1, 1, 0)] = 0.249831916;
+    // P_CC, P_CH (Table 8), indexed (N_H, N_C)
+    Pcc[1 * 6 + 1] = 0.003026697473481;
+    Pcc[2 * 6 + 0] = 0.007860700254745;
+    Pcc[3 * 6 + 0] = 0.016125364564267;
+    Pcc[1 * 6 + 2] = 0.003179530830731;
+    Pcc[2 * 6 + 1] = 0.006326248241119;
+    Pch[1 * 6 + 0] = 0.2093367328250380;
+    Pch[2 * 6 + 0] = -0.064449615432525;
+    Pch[3 * 6 + 0] = -0.303927546346162;
+    Pch[0 * 6 + 1] = 0.01;
+    Pch[0 * 6 + 2] = -0.1220421462782555;
+    Pch[1 * 6 + 1] = -0.1251234006287090;
+    Pch[2 * 6 + 1] = -0.298905245783;
+    Pch[0 * 6 + 3] = -0.307584705066;
+    Pch[1 * 6 + 2] = -0.3005291724067579;
+}
+#undef T3
+
+static int rebo2_setup(Rebo2State *st)
+{
+    RbDev &D = st->hdev;
+    std::memset(&D, 0, sizeof D);
+    // rebo2_type.f90 defaults (Brenner et al. 2002, Tables 2, 6, 7)
+    D.ccB[0] = 12388.79197798; D.ccB[1] = 17.56740646509; D.ccB[2] = 30.71493208065;
+    D.ccBeta[0] = 4.7204523127; D.ccBeta[1] = 1.4332132499; D.ccBeta[2] = 1.3826912506;
+    D.Q[RB_CC] = 0.3134602960833; D.A[RB_CC] = 10953.544162170; D.alpha[RB_CC] = 4.7465390606595;
+    D.B1[RB_CH] = 32.3551866587; D.beta1[RB_CH] = 1.43445805925;
+    D.Q[RB_CH] = 0.340775728; D.A[RB_CH] = 149.94098723; D.alpha[RB_CH] = 4.10254983;
+    D.B1[RB_HH] = 29.632593; D.beta1[RB_HH] = 1.71589217;
+    D.Q[RB_HH] = 0.370471487045; D.A[RB_HH] = 32.817355747; D.alpha[RB_HH] = 3.536298648;
+    // the g knots and node values are single-precision literals there
+    const double th[6] = {-1.0, -1.0 / 2, -1.0 / 3, 0.0, 1.0 / 2, 1.0};
+    const double g1[6] = {(double)-0.01f, (double)0.05280f, (double)0.09733f,
+                          (double)0.37545f, (double)2.0014f, (double)8.0f};
+    const double dg1[6] = {(double)0.10400f, (double)0.17000f, (double)0.40000f, 0, 0, 0};
+    const double d2g1[6] = {0.0, (double)0.37000f, (double)1.98000f, 0, 0, 0};
+    const double g2[6] = {0.0, 0.0, (double)0.09733f, (double)0.271856f, (double)0.416335f, 1.0};
+    if (g_spline(th, g1, dg1, d2g1, g1, D.g1c) || g_spline(th, g1, dg1, d2g1, g2, D.g2c))
+        return set_error(AGPU_ERR_ARG, "REBO2: singular g spline system");
+    D.th1 = th[1];
+    D.th2 = th[2];
+    D.n_hi = (double)3.7f;
+    static const double spgh[3][6] = {
+        {270.467795364007301, 1549.701314596994564, 3781.927258631323866,
+         4582.337619544424228, 2721.538161662818368, 630.658598136730774},
+        {16.956325544514659, -21.059084522755980, -102.394184748124742,
+         -210.527926707779059, -229.759473570467513, -94.968528666251945},
+        {19.065031149937783, 2.017732531534021, -2.566444502991983,
+         3.291353893907436, -2.653536801884563, 0.837650930130006}};
+    std::memcpy(D.spgh, spgh, sizeof spgh);
+    for (int i = 0; i < 25; ++i) D.igh[i] = i < 18 ? 3 : (i < 22 ? 2 : 1);
+    // rebo2_db.f90:150-200
+    const double lam = 4.0, cc_re = 1.4, ch_re = 1.09, hh_re = 0.7415886997;
+    D.conpe = -0.5;
+    D.conan = 0.5 * D.conpe;
+    D.conpf = D.conpe - 1.0;
+    D.conalp = lam;
+    D.conear[RB_CC][RB_CC] = 1.0;
+    D.conear[RB_CC][RB_CH] = std::exp(lam * (ch_re - cc_re));
+    D.conear[RB_CC][RB_HH] = std::exp(lam * (hh_re - cc_re));
+    D.conear[RB_CH][RB_CC] = 1.0 / D.conear[RB_CC][RB_CH];
+    D.conear[RB_CH][RB_CH] = 1.0;
+    D.conear[RB_CH][RB_HH] = std::exp(lam * (hh_re - ch_re));
+    D.conear[RB_HH][RB_CC] = 1.0 / D.conear[RB_CC][RB_HH];
+    D.conear[RB_HH][RB_CH] = 1.0 / D.conear[RB_CH][RB_HH];
+    D.conear[RB_HH][RB_HH] = 1.0;
+    const int t3[3] = {RB_CC, RB_CH, RB_HH};
+    const double r1[3] = {1.70, 1.30, 1.10}, r2[3] = {2.00, 1.80, 1.70};
+    for (int q = 0; q < 3; ++q) {
+        D.cut_l[t3[q]] = r1[q];
+        D.cut_h[t3[q]] = r2[q];
+        D.cut_h2[t3[q]] = r2[q] * r2[q];
+        D.cut_fac[t3[q]] = M_PI / (r2[q] - r1[q]);
+    }
+    // tables
+    std::vector<double> F(250 * 6), P(72);
+    default_tables(&F[0], &F[250], &F[500], &F[750], &F[1000], &F[1250], &P[0], &P[36]);
+    if (st->zero_tables) {
+        std::fill(F.begin(), F.end(), 0.0);
+        std::fill(P.begin(), P.end(), 0.0);
+    }
+    st->htab.assign(RB_TAB_N, 0.0);
+    hermite3(4, 4, 9, &F[0], &F[250], &F[500], &F[750], &st->htab[RB_TAB_FCC]);
+    hermite3(4, 4, 9, &F[1000], nullptr, nullptr, nullptr, &st->htab[RB_TAB_FCH]);
+    hermite3(4, 4, 9, &F[1250], nullptr, nullptr, nullptr, &st->htab[RB_TAB_FHH]);
+    hermite2(5, 5, &P[0], &st->htab[RB_TAB_PCC]);
+    hermite2(5, 5, &P[36], &st->htab[RB_TAB_PCH]);
+    AGPU_CHECK(st->dev.reserve(sizeof(RbDev)));
+    AGPU_CHECK(st->tab.reserve(sizeof(double) * RB_TAB_N));
+    AGPU_HIP(hipMemcpy(st->dev.ptr, &D, sizeof(RbDev), hipMemcpyHostToDevice));
+    AGPU_HIP(hipMemcpy(st->tab.ptr, st->htab.data(), sizeof(double) * RB_TAB_N, hipMemcpyHostToDevice));
+    return AGPU_OK;
+}
+
+// ---------------------------------------------------------------------------
+// device functions (rebo2_func.f90)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int rb_pair(int ki, int kj)
+{
+    return ki == RB_C ? kj : (kj == RB_C ? ki : ki + kj);
+}
+
+__device__ __forceinline__ void rb_fconj(double x, double &fx, double &dfx)
+{
+    if (x <= 2.0) { fx = 1.0; dfx = 0.0; }
+    else if (x >= 3.0) { fx = 0.0; dfx = 0.0; }
+    else {
+        double s, c;
+        sincos(M_PI * (x - 2.0), &s, &c);
+        fx = 0.5 * (1.0 + c);
+        dfx = -0.5 * M_PI * s;
+    }
+}
+
+__device__ __forceinline__ void rb_fcin(const RbDev &D, int t, double r, double &v, double &dv)
+{
+    const double l = D.cut_l[t], h = D.cut_h[t];
+    if (r >= h) { v = 0.0; dv = 0.0; }
+    else if (r <= l) { v = 1.0; dv = 0.0; }
+    else {
+        const double fac = D.cut_fac[t];
+        double s, c;
+        sincos(fac * (r - l), &s, &c);
+        v = 0.5 * (1.0 + c);
+        dv = -0.5 * fac * s;
+    }
+}
+
+__device__ __forceinline__ void rb_va(const RbDev &D, int t, double r, double &v, double &dv)
+{
+    if (t == RB_CC) {
+        const double e1 = D.ccB[0] * exp(-D.ccBeta[0] * r);
+        const double e2 = D.ccB[1] * exp(-D.ccBeta[1] * r);
+        const double e3 = D.ccB[2] * exp(-D.ccBeta[2] * r);
+        v = -(e1 + e2 + e3);
+        dv = -(-D.ccBeta[0] * e1 - D.ccBeta[1] * e2 - D.ccBeta[2] * e3);
+    } else {
+        const double e1 = D.B1[t] * exp(-D.beta1[t] * r);
+        v = -e1;
+        dv = D.beta1[t] * e1;
+    }
+}
+
+__device__ __forceinline__ void rb_vr(const RbDev &D, int t, double r, double &v, double &dv)
+{
+    const double e1 = D.A[t] * exp(-D.alpha[t] * r);
+    const double hl = 1 + D.Q[t] / r;
+    v = hl * e1;
+    dv = (-D.Q[t] / (r * r) - hl * D.alpha[t]) * e1;
+}
+
+__device__ __forceinline__ void rb_poly5(const double *c, double x, double &v, double &dv)
+{
+    const double x2 = x * x, x3 = x2 * x, x4 = x3 * x, x5 = x4 * x;
+    v = c[0] + c[1] * x + c[2] * x2 + c[3] * x3 + c[4] * x4 + c[5] * x5;
+    dv = c[1] + 2 * c[2] * x + 3 * c[3] * x2 + 4 * c[4] * x3 + 5 * c[5] * x4;
+}
+
+__device__ __forceinline__ void rb_g(const RbDev &D, int ktyp, double costh, double n, double &val,
+                                     double &dcos, double &dn)
+{
+    if (ktyp == RB_C) {
+        const int j = costh < D.th1 ? 0 : (costh < D.th2 ? 1 : 2);
+        if (n < 3.2) {
+            rb_poly5(D.g2c[j], costh, val, dcos);
+            dn = 0.0;
+        } else if (n > D.n_hi) {
+            rb_poly5(D.g1c[j], costh, val, dcos);
+            dn = 0.0;
+        } else {
+            double v1, d1, v2, d2, s, c;
+            rb_poly5(D.g1c[j], costh, v1, d1);
+            rb_poly5(D.g2c[j], costh, v2, d2);
+            sincos(2 * M_PI * (n - 3.2), &s, &c);
+            const double sw = (1 + c) / 2;
+            val = v1 * (1 - sw) + v2 * sw;
+            dcos = d1 * (1 - sw) + d2 * sw;
+            dn = (v2 - v1) * (-M_PI * s);
+        }
+    } else {
+        int q = (int)(-costh * 12.0) + 12;
+        q = q < 0 ? 0 : (q > 24 ? 24 : q);
+        rb_poly5(D.spgh[D.igh[q] - 1], costh, val, dcos);
+        dn = 0.0;
+    }
+}
+
+__device__ __forceinline__ void rb_h(const RbDev &D, int ijpot, int ikpot, double dr, double &v,
+                                     double &dv)
+{
+    if (ijpot + ikpot <= 4) { v = 1.0; dv = 0.0; }
+    else {
+        v = D.conear[ijpot][ikpot] * exp(D.conalp * dr);
+        dv = D.conalp * v;
+    }
+}
+
+__device__ __forceinline__ int rb_box(double v, int n)
+{
+    int b = (int)v;
+    return b < 0 ? 0 : (b >= n ? n - 1 : b);
+}
+
+__device__ void rb_table3(const double *__restrict__ T, double x, double y, double z, double &f,
+                          double &fx, double &fy, double &fz)
+{
+    const int bx = rb_box(x, 4), by = rb_box(y, 4), bz = rb_box(z, 9);
+    const double *c = T + (size_t)((bx * 4 + by) * 9 + bz) * 64;
+    const double x1 = x - bx, x2 = y - by, x3 = z - bz;
+    double F = 0.0, Fi = 0.0, Fj = 0.0, Fk = 0.0;
+    for (int i = 3; i >= 0; --i) {
+        double s = 0.0, sj = 0.0, sk = 0.0;
+        for (int j = 3; j >= 0; --j) {
+            double t = 0.0, tk = 0.0;
+            for (int k = 3; k >= 0; --k) {
+                const double cijk = c[16 * i + 4 * j + k];
+                t = t * x3 + cijk;
+                if (k > 0) tk = tk * x3 + k * cijk;
+            }
+            s = s * x2 + t;
+            if (j > 0) sj = sj * x2 + j * t;
+            sk = sk * x2 + tk;
+        }
+        F = F * x1 + s;
+        if (i > 0) Fi = Fi * x1 + i * s;
+        Fj = Fj * x1 + sj;
+        Fk = Fk * x1 + sk;
+    }
+    f = F; fx = Fi; fy = Fj; fz = Fk;
+}
+
+__device__ void rb_table2(const double *__restrict__ T, double x, double y, double &f, double &fx,
+                          double &fy)
+{
+    const int bx = rb_box(x, 5), by = rb_box(y, 5);
+    const double *c = T + (size_t)(bx * 5 + by) * 16;
+    const double x1 = x - bx, x2 = y - by;
+    double h = 0.0, dh = 0.0, dc = 0.0;
+    for (int i = 3; i >= 0; --i) {
+        double s = 0.0, sd = 0.0;
+        for (int j = 3; j >= 0; --j) {
+            const double cij = c[4 * i + j];
+            s = s * x2 + cij;
+            if (j > 0) sd = sd * x2 + j * cij;
+        }
+        h = h * x1 + s;
+        if (i > 0) dh = dh * x1 + i * s;
+        dc = dc * x1 + sd;
+    }
+    f = h; fx = dh; fy = dc;
+}
+
+// fixed-point accumulation
+__device__ __forceinline__ void fx_add(unsigned long long *a, double v, int &bad)
+{
+    if (!(fabs(v) < RB_FX_MAX)) bad |= 1;
+    const long long q = __double2ll_rn(v * RB_FX);
+    atomicAdd(a, (unsigned long long)q);
+}
+
+__device__ __forceinline__ void fx_add3(unsigned long long *a, int i, const double *v, int &bad)
+{
+    fx_add(a + 3 * (size_t)i + 0, v[0], bad);
+    fx_add(a + 3 * (size_t)i + 1, v[1], bad);
+    fx_add(a + 3 * (size_t)i + 2, v[2], bad);
+}
+
+__device__ __forceinline__ bool dc_zero_sum(int c1, int c2)
+{
+    int a1, b1, e1, a2, b2, e2;
+    dc_unpack(c1, a1, b1, e1);
+    dc_unpack(c2, a2, b2, e2);
+    return a1 + a2 == 0 && b1 + b2 == 0 && e1 + e2 == 0;
+}
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+__global__ void k_rb_zero(unsigned long long *__restrict__ a, size_t n)
+{
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+        a[q] = 0ull;
+}
+
+__global__ void __launch_bounds__(256)
+k_rb_bonds(int nat, const double *__restrict__ r, const int *__restrict__ el, CellArg C,
+           const RbDev *__restrict__ Dp, const int *__restrict__ offsets,
+           const int2 *__restrict__ nbr, RbBond *__restrict__ bond, int2 *__restrict__ bij,
+           int *__restrict__ bown, int *__restrict__ bcnt, double2 *__restrict__ nn)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nat) return;
+    const RbDev &D = *Dp;
+    const int ki = el[i];
+    const int beg = offsets[i], end = offsets[i + 1];
+    int nb = 0;
+    double nC = 0.0, nH = 0.0;
+    if (ki > 0) {
+        const double ri[3] = {r[3 * i], r[3 * i + 1], r[3 * i + 2]};
+        for (int s = beg; s < end; ++s) {
+            const int2 e = nbr[s];
+            const int kj = el[e.x];
+            if (kj <= 0) continue;
+            int a, b, c;
+            dc_unpack(e.y, a, b, c);
+            const double rj[3] = {r[3 * e.x], r[3 * e.x + 1], r[3 * e.x + 2]};
+            double d[3];
+            delta_r(C, ri, rj, a, b, c, d);
+            const double rl = dot3_nc(d);
+            const int t = rb_pair(ki, kj);
+            RbBond B;
+            if (rl < D.cut_l[t] * D.cut_l[t]) {
+                B.len = sqrt(rl);
+                B.fc = 1.0;
+                B.dfc = 0.0;
+            } else if (rl < D.cut_h2[t]) {
+                B.len = sqrt(rl);
+                rb_fcin(D, t, B.len, B.fc, B.dfc);
+            } else {
+                continue;
+            }
+            // r_ij = r_j - r_i - A.dc = -d, normalised by division
+            B.n[0] = -d[0] / B.len;
+            B.n[1] = -d[1] / B.len;
+            B.n[2] = -d[2] / B.len;
+            B.fx = 0.0;
+            B.dfx = 0.0;
+            if (kj == RB_C) nC += B.fc;
+            else nH += B.fc;
+            bond[beg + nb] = B;
+            bij[beg + nb] = e;
+            bown[beg + nb] = i;
+            ++nb;
+        }
+    }
+    for (int s = beg + nb; s < end; ++s) bown[s] = -1;
+    bcnt[i] = nb;
+    nn[i] = make_double2(nC, nH);
+}
+
+__global__ void __launch_bounds__(256)
+k_rb_conj(int nat, const int *__restrict__ el, const int *__restrict__ offsets,
+          const int *__restrict__ bcnt, const int2 *__restrict__ bij, const double2 *__restrict__ nn,
+          RbBond *__restrict__ bond, double *__restrict__ nconjit)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nat) return;
+    const int beg = offsets[i], nb = bcnt[i];
+    double s = 0.0;
+    for (int b = beg; b < beg + nb; ++b) {
+        const int k = bij[b].x;
+        if (el[k] != RB_C) continue;
+        const double2 nk = nn[k];
+        const double fc = bond[b].fc;
+        const double x = nk.x + nk.y - fc;
+        double fx, dfx;
+        rb_fconj(x, fx, dfx);
+        bond[b].fx = fx;
+        bond[b].dfx = dfx;
+        s = s + fc * fx;
+    }
+    nconjit[i] = s;
+}
+
+__device__ __forceinline__ void outer_sub(double *w, const double *a, const double *b)
+{
+#pragma unroll
+    for (int y = 0; y < 3; ++y)
+#pragma unroll
+        for (int x = 0; x < 3; ++x) w[x + 3 * y] = w[x + 3 * y] - a[x] * b[y];
+}
+
+// bond-order pass over the neighbours of centre c for the bond c->o:
+// zeta, dzeta/dN (pass 1) or the bond-order force terms (pass 2)
+struct RbSide {
+    double z, dzdn;
+    double dbc[3], dbo[3];      // d b / d r_c, d b / d r_o (pass 2 sums)
+    double wb[9];               // bond-order virial sum (pass 2)
+};
+
+template <bool FORCES>
+__device__ __forceinline__ void rb_zeta(const RbDev &D, int kc, int ijpot, double rlij, double rlijr,
+                                        const double *rnij, double sgn, int beg, int nb, int skip,
+                                        const RbBond *__restrict__ bond, const int2 *__restrict__ bij,
+                                        const int *__restrict__ el, double nt, double dfb,
+                                        RbSide &S, unsigned long long *__restrict__ facc, int &bad)
+{
+    // sgn = +1: centre i, bond vector rnij towards j;  sgn = -1: centre j,
+    // the bond j->i is -rnij (costh = -rnjl.rnij, bop_kernel_rebo2.f90:1780)
+    for (int b = beg; b < beg + nb; ++b) {
+        if (b == skip) continue;
+        const RbBond B = bond[b];
+        const int ko = el[bij[b].x];
+        const int ikpot = rb_pair(kc, ko);
+        const double rlik = B.len;
+        if (!(rlik < D.cut_h[ikpot])) continue;
+        double qf, qd, gf, gd, dgdn;
+        rb_h(D, ijpot, ikpot, rlij - rlik, qf, qd);
+        double costh = 0.0;
+        costh = costh + B.n[0] * rnij[0];
+        costh = costh + B.n[1] * rnij[1];
+        costh = costh + B.n[2] * rnij[2];
+        if (sgn < 0) costh = -costh;
+        rb_g(D, kc, costh, nt, gf, gd, dgdn);
+        const double fcik = B.fc;
+        S.dzdn = S.dzdn + fcik * dgdn * qf;
+        S.z = S.z + fcik * gf * qf;
+        if (!FORCES) continue;
+        double dk[3], dd = 0.0;
+#pragma unroll
+        for (int x = 0; x < 3; ++x) dk[x] = B.n[x] * rlik - sgn * rnij[x] * rlij;
+        dd = dd + dk[0] * dk[0];
+        dd = dd + dk[1] * dk[1];
+        dd = dd + dk[2] * dk[2];
+        const double dis = sqrt(dd);
+#pragma unroll
+        for (int x = 0; x < 3; ++x) dk[x] = dk[x] / dis;
+        const double dcs_io = 1.0 / rlik - costh * rlijr;
+        const double dcs_ik = rlijr - costh / rlik;
+        const double dcs_ok = -dis * rlijr / rlik;
+        const double dzfac = fcik * gd * qf;
+        const double dzdro = gf * fcik * qd;
+        const double dzdrk = gf * (B.dfc * qf - fcik * qd);
+        double dfo[3], dbk[3];
+        const double rik[3] = {rlik * B.n[0], rlik * B.n[1], rlik * B.n[2]};
+#pragma unroll
+        for (int x = 0; x < 3; ++x) {
+            const double u = sgn * rnij[x];      // unit vector centre -> other
+            const double dcs_c = -dcs_io * u - dcs_ik * B.n[x];
+            const double dcs_o = dcs_io * u - dcs_ok * dk[x];
+            const double dcs_k = dcs_ik * B.n[x] + dcs_ok * dk[x];
+            S.dbc[x] = S.dbc[x] - dzdro * u - dzdrk * B.n[x] + dzfac * dcs_c;
+            dfo[x] = dzdro * u + dzfac * dcs_o;
+            S.dbo[x] = S.dbo[x] + dfo[x];
+            dbk[x] = dzdrk * B.n[x] + dzfac * dcs_k;
+        }
+        // virial: wijb -= r_ij (x) df_o + r_ik (x) db_k, with r_ij the
+        // centre -> other vector
+        const double rco[3] = {sgn * rlij * rnij[0], sgn * rlij * rnij[1], sgn * rlij * rnij[2]};
+        outer_sub(S.wb, rco, dfo);
+        outer_sub(S.wb, rik, dbk);
+        double fk[3] = {-dfb * dbk[0], -dfb * dbk[1], -dfb * dbk[2]};
+        fx_add3(facc, bij[b].x, fk, bad);
+    }
+}
+
+// forces from dF/dN and dF/dN_conj (and the P correction) on the
+// neighbours k of a centre and on their neighbours m (bop_kernel_rebo2.f90:2400-2520)
+__device__ __forceinline__ void rb_nforces(int centre, int beg, int nb, int skip,
+                                           const RbBond *__restrict__ bond, const int2 *__restrict__ bij,
+                                           const int *__restrict__ offsets, const int *__restrict__ bcnt,
+                                           const int *__restrict__ el, double dfdn, double dfdncn,
+                                           double dfb, double dpdnc, double dpdnh, double *fcen,
+                                           double *w, unsigned long long *__restrict__ facc, int &bad)
+{
+    for (int b = beg; b < beg + nb; ++b) {
+        if (b == skip) continue;
+        const RbBond B = bond[b];
+        const int2 e = bij[b];
+        const int k = e.x;
+        const bool kC = el[k] == RB_C;
+        double df[3], dr[3];
+        const double dpn = kC ? dpdnc : dpdnh;
+#pragma unroll
+        for (int x = 0; x < 3; ++x) {
+            const double dn = B.n[x] * B.dfc;
+            const double dncn = kC ? (B.fx * B.dfc) * B.n[x] : 0.0;
+            df[x] = -(dfdn * dn + dfdncn * dncn) - dfb * (dpn * dn);
+            fcen[x] -= df[x];
+            dr[x] = B.len * B.n[x];
+        }
+        outer_sub(w, dr, df);
+        if (kC && dfdncn != 0.0) {
+            const double nconjdx = B.fc * B.dfx;
+            const int kb = offsets[k], kn = bcnt[k];
+            for (int m = kb; m < kb + kn; ++m) {
+                const int2 em = bij[m];
+                // the centre itself (m == centre at the centre's image) is skipped
+                if (em.x == centre && dc_zero_sum(e.y, em.y)) continue;
+                const RbBond M = bond[m];
+                double d2[3], drm[3];
+#pragma unroll
+                for (int x = 0; x < 3; ++x) {
+                    d2[x] = -dfdncn * (nconjdx * (M.dfc * M.n[x]));
+                    df[x] -= d2[x];
+                    drm[x] = M.len * M.n[x];
+                }
+                outer_sub(w, drm, d2);
+                fx_add3(facc, em.x, d2, bad);
+            }
+        }
+        fx_add3(facc, k, df, bad);
+    }
+}
+
+__global__ void __launch_bounds__(256)
+k_rb_pair(int ntot, const RbDev *__restrict__ Dp, const double *__restrict__ tab,
+          const int *__restrict__ el, const int *__restrict__ offsets, const int *__restrict__ bcnt,
+          const int *__restrict__ bown, const RbBond *__restrict__ bond, const int2 *__restrict__ bij,
+          const double2 *__restrict__ nn, const double *__restrict__ nconjit,
+          unsigned long long *__restrict__ facc, unsigned long long *__restrict__ eacc,
+          unsigned long long *__restrict__ wacc, double *__restrict__ partial, int *__restrict__ flags)
+{
+    __shared__ double s_red[4][10];
+    const RbDev &D = *Dp;
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    int bad = 0;
+    double epair = 0.0;
+    double wij[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) wij[q] = 0.0;
+    const int i = s < ntot ? bown[s] : -1;
+    bool act = i >= 0;
+    int2 eij = make_int2(0, 0);
+    if (act) {
+        eij = bij[s];
+        int a, b, c;
+        dc_unpack(eij.y, a, b, c);
+        const bool pos = a != 0 ? a > 0 : (b != 0 ? b > 0 : c > 0);
+        const bool zero = a == 0 && b == 0 && c == 0;
+        act = (zero && eij.x > i) || pos;
+    }
+    if (act) {
+        const int j = eij.x;
+        const int ki = el[i], kj = el[j];
+        const RbBond Bij = bond[s];
+        const int ijpot = rb_pair(ki, kj);
+        const double rlij = Bij.len;
+        if (rlij < D.cut_h[ijpot]) {
+            const double rlijr = 1.0 / rlij;
+            const double *rnij = Bij.n;
+            const double rij[3] = {rlij * rnij[0], rlij * rnij[1], rlij * rnij[2]};
+            const double fcar = Bij.fc, dfcar = Bij.dfc;
+            const double2 nni = nn[i], nnj = nn[j];
+            double niC = nni.x, niH = nni.y, njC = nnj.x, njH = nnj.y;
+            if (kj == RB_C) niC = niC - Bij.fc; else niH = niH - Bij.fc;
+            if (ki == RB_C) njC = njC - Bij.fc; else njH = njH - Bij.fc;
+            niC = niC > 4.0 ? 4.0 : niC;
+            niH = niH > 4.0 ? 4.0 : niH;
+            njC = njC > 4.0 ? 4.0 : njC;
+            njH = njH > 4.0 ? 4.0 : njH;
+            // N_i, N_j as g sees them; F sees them clamped to 3 below
+            const double nti_g = niC + niH, ntj_g = njC + njH;
+            double nti = nti_g, ntj = ntj_g;
+            double fa, dfa, fr, dfr;
+            rb_va(D, ijpot, rlij, fa, dfa);
+            rb_vr(D, ijpot, rlij, fr, dfr);
+            const int ib = offsets[i], inb = bcnt[i];
+            const int jb = offsets[j], jnb = bcnt[j];
+            // the bond j->i (image -dc) in j's list
+            int sji = -1;
+            for (int b = jb; b < jb + jnb; ++b) {
+                const int2 e = bij[b];
+                if (e.x == i && dc_zero_sum(e.y, eij.y)) { sji = b; break; }
+            }
+            if (sji < 0) bad |= 2;
+            // pass 1: zeta_ij, zeta_ji
+            RbSide Si{}, Sj{};
+            int dummy = 0;
+            rb_zeta<false>(D, ki, ijpot, rlij, rlijr, rnij, 1.0, ib, inb, s, bond, bij, el, nti_g, 0.0,
+                           Si, facc, dummy);
+            rb_zeta<false>(D, kj, ijpot, rlij, rlijr, rnij, -1.0, jb, jnb, sji, bond, bij, el, ntj_g, 0.0,
+                           Sj, facc, dummy);
+            const double nconji = nconjit[i] - Bij.fc * Bij.fx;
+            double nconjj = 0.0;
+            for (int b = jb; b < jb + jnb; ++b)
+                if (b != sji && el[bij[b].x] == RB_C) nconjj = nconjj + bond[b].fc * bond[b].fx;
+            // P and the bond orders
+            double pij = 0.0, dpdnci = 0.0, dpdnhi = 0.0, pji = 0.0, dpdncj = 0.0, dpdnhj = 0.0;
+            const double *P = tab + (ijpot == RB_CC ? RB_TAB_PCC : RB_TAB_PCH);
+            if (ki == RB_C) {
+                rb_table2(P, niH, niC, pij, dpdnhi, dpdnci);
+                Si.z = Si.z + pij;
+                dpdnci = dpdnci + Si.dzdn;
+                dpdnhi = dpdnhi + Si.dzdn;
+            }
+            if (kj == RB_C) {
+                rb_table2(P, njH, njC, pji, dpdnhj, dpdncj);
+                Sj.z = Sj.z + pji;
+                dpdncj = dpdncj + Sj.dzdn;
+                dpdnhj = dpdnhj + Sj.dzdn;
+            }
+            const double bi = pow(1.0 + Si.z, D.conpe);
+            const double dfbi = D.conan * fcar * fa * pow(1.0 + Si.z, D.conpf);
+            const double bj = pow(1.0 + Sj.z, D.conpe);
+            const double dfbj = D.conan * fcar * fa * pow(1.0 + Sj.z, D.conpf);
+            // F(N_i, N_j, N_conj)
+            double nconj = nconji * nconji + nconjj * nconjj;
+            nconj = nconj > 8.0 ? 8.0 : nconj;
+            nti = nti > 3.0 ? 3.0 : nti;
+            ntj = ntj > 3.0 ? 3.0 : ntj;
+            double fij = 0.0, dfdni = 0.0, dfdnj = 0.0, dfdncn = 0.0;
+            if (ijpot == RB_CC)
+                rb_table3(tab + RB_TAB_FCC, nti, ntj, nconj, fij, dfdni, dfdnj, dfdncn);
+            else if (ijpot == RB_HH)
+                rb_table3(tab + RB_TAB_FHH, nti, ntj, nconj, fij, dfdni, dfdnj, dfdncn);
+            else if (ki == RB_C)
+                rb_table3(tab + RB_TAB_FCH, ntj, nti, nconj, fij, dfdnj, dfdni, dfdncn);
+            else
+                rb_table3(tab + RB_TAB_FCH, nti, ntj, nconj, fij, dfdni, dfdnj, dfdncn);
+            dfdni = 0.5 * fcar * fa * dfdni;
+            dfdnj = 0.5 * fcar * fa * dfdnj;
+            dfdncn = 0.5 * fcar * fa * dfdncn;
+            const double dfdncni = 2 * dfdncn * nconji;
+            const double dfdncnj = 2 * dfdncn * nconjj;
+            double fi[3] = {0, 0, 0}, fj[3] = {0, 0, 0};
+            // coordination and conjugation forces
+            rb_nforces(i, ib, inb, s, bond, bij, offsets, bcnt, el, dfdni, dfdncni, dfbi, dpdnci,
+                       dpdnhi, fi, wij, facc, bad);
+            rb_nforces(j, jb, jnb, sji, bond, bij, offsets, bcnt, el, dfdnj, dfdncnj, dfbj, dpdncj,
+                       dpdnhj, fj, wij, facc, bad);
+            // pass 2: bond-order forces on k and l, sums for i and j
+            Si.z = Sj.z = Si.dzdn = Sj.dzdn = 0.0;
+            rb_zeta<true>(D, ki, ijpot, rlij, rlijr, rnij, 1.0, ib, inb, s, bond, bij, el, nti_g, dfbi, Si,
+                          facc, bad);
+            rb_zeta<true>(D, kj, ijpot, rlij, rlijr, rnij, -1.0, jb, jnb, sji, bond, bij, el, ntj_g, dfbj, Sj,
+                          facc, bad);
+            // pair energy and forces
+            const double bave = 0.5 * (bi + bj + fij);
+            const double hlf = fcar * (fr + bave * fa) / 2;
+            epair = hlf + hlf;
+            const double dffac = dfr * fcar + bave * dfa * fcar + fr * dfcar + bave * fa * dfcar;
+            double df[3];
+#pragma unroll
+            for (int x = 0; x < 3; ++x) {
+                df[x] = dffac * rnij[x];
+                fi[x] += df[x];
+                fj[x] -= df[x];
+            }
+            // the j-side sums are w.r.t. (centre j, other i): map them back
+#pragma unroll
+            for (int y = 0; y < 3; ++y)
+#pragma unroll
+                for (int x = 0; x < 3; ++x)
+                    wij[x + 3 * y] = wij[x + 3 * y] + rij[x] * df[y] - dfbi * Si.wb[x + 3 * y] -
+                                     dfbj * Sj.wb[x + 3 * y];
+#pragma unroll
+            for (int x = 0; x < 3; ++x) {
+                fi[x] += -(dfbi * Si.dbc[x] + dfbj * Sj.dbo[x]);
+                fj[x] += -(dfbi * Si.dbo[x] + dfbj * Sj.dbc[x]);
+            }
+            fx_add3(facc, i, fi, bad);
+            fx_add3(facc, j, fj, bad);
+            if (eacc) {
+                fx_add(eacc + i, hlf, bad);
+                fx_add(eacc + j, hlf, bad);
+            }
+            if (wacc) {
+                for (int q = 0; q < 9; ++q) {
+                    fx_add(wacc + 9 * (size_t)i + q, wij[q] / 2, bad);
+                    fx_add(wacc + 9 * (size_t)j + q, wij[q] / 2, bad);
+                }
+            }
+        }
+    }
+    if (bad) atomicOr(flags, ((bad & 1) ? DERR_FIXED_RANGE : 0) | ((bad & 2) ? DERR_BOND_OVERFLOW : 0));
+    // block reduction of [e, W(9)]
+    double red[10];
+    red[0] = epair;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) red[1 + q] = wij[q];
+#pragma unroll
+    for (int q = 0; q < 10; ++q) red[q] = group_sum<64>(red[q]);
+    const int tid = threadIdx.x, wave = tid / 64;
+    if ((tid & 63) == 0)
+        for (int q = 0; q < 10; ++q) s_red[wave][q] = red[q];
+    __syncthreads();
+    if (tid < 10)
+        partial[(size_t)blockIdx.x * 10 + tid] =
+            ((s_red[0][tid] + s_red[1][tid]) + s_red[2][tid]) + s_red[3][tid];
+}
+
+__global__ void k_rb_finish(int nat, unsigned long long *__restrict__ facc,
+                            unsigned long long *__restrict__ eacc, unsigned long long *__restrict__ wacc,
+                            double *__restrict__ f, double *__restrict__ epa, double *__restrict__ wpa)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nat) return;
+    for (int x = 0; x < 3; ++x) {
+        f[3 * i + x] += (double)(long long)facc[3 * (size_t)i + x] * RB_FX_INV;
+        facc[3 * (size_t)i + x] = 0ull;
+    }
+    if (epa) {
+        epa[i] += (double)(long long)eacc[i] * RB_FX_INV;
+        eacc[i] = 0ull;
+    }
+    if (wpa)
+        for (int q = 0; q < 9; ++q) {
+            wpa[9 * (size_t)i + q] += (double)(long long)wacc[9 * (size_t)i + q] * RB_FX_INV;
+            wacc[9 * (size_t)i + q] = 0ull;
+        }
+}
+
+}  // namespace agpu
+
+// ---------------------------------------------------------------------------
+// host driver
+// ---------------------------------------------------------------------------
+namespace agpu {
+
+static void rebo2_free(void *p) { delete static_cast<Rebo2State *>(p); }
+
+// rebo2_bind_to (rebo2_module.f90:57-110): element map C, H and the
+// interaction ranges r2(CC), r2(CH), r2(HH) for the pairs present
+int rebo2_bind(agpu_potential *pot)
+{
+    Rebo2State *st = static_cast<Rebo2State *>(pot->impl);
+    agpu_particles *p = pot->p;
+    for (int z = 0; z < 128; ++z) st->z2db[z] = -1;
+    st->z2db[6] = RB_C;
+    st->z2db[1] = RB_H;
+    bool hasC = false, hasH = false;
+    for (int z : p->h_Z) {
+        hasC = hasC || z == 6;
+        hasH = hasH || z == 1;
+    }
+    double cut = 0.0;
+    const RbDev &D = st->hdev;
+    if (hasC) {
+        AGPU_CHECK(agpu_neighbors_request_interaction_range(pot->nl, D.cut_h[RB_CC]));
+        cut = D.cut_h[RB_CC];
+    }
+    if (hasC && hasH) {
+        AGPU_CHECK(agpu_neighbors_request_interaction_range(pot->nl, D.cut_h[RB_CH]));
+        cut = cut > D.cut_h[RB_CH] ? cut : D.cut_h[RB_CH];
+    }
+    if (hasH) {
+        AGPU_CHECK(agpu_neighbors_request_interaction_range(pot->nl, D.cut_h[RB_HH]));
+        cut = cut > D.cut_h[RB_HH] ? cut : D.cut_h[RB_HH];
+    }
+    st->cutoff = cut;
+    AGPU_CHECK(map_elements(pot, st->z2db, 128));
+    st->Z_rev = p->Z_rev;
+    return AGPU_OK;
+}
+
+int rebo2_compute(agpu_potential *pot, const agpu_eval_args *a)
+{
+    Rebo2State *st = static_cast<Rebo2State *>(pot->impl);
+    agpu_particles *p = pot->p;
+    agpu_neighbors *nl = pot->nl;
+    if (pot->precision != AGPU_PREC_FP64)
+        return set_error(AGPU_ERR_UNSUPPORTED, "REBO2: only the FP64 path is implemented");
+    if (a->mask)
+        return set_error(AGPU_ERR_UNSUPPORTED, "REBO2: the reference kernel takes no mask");
+    if (st->Z_rev != p->Z_rev) AGPU_CHECK(rebo2_bind(pot));
+    AGPU_CHECK(nl_update(nl, p, 0));
+    const int nat = p->nat;
+    hipStream_t s = stream();
+    AGPU_HIP(hipMemsetAsync(a->epot, 0, sizeof(double), s));
+    if (nat == 0) return AGPU_OK;
+    const int64_t total = nl->total;
+    AGPU_CHECK(st->bond.reserve(sizeof(RbBond) * (total + 1)));
+    AGPU_CHECK(st->bij.reserve(sizeof(int2) * (total + 1)));
+    AGPU_CHECK(st->bown.reserve(sizeof(int) * (total + 1)));
+    AGPU_CHECK(st->bcnt.reserve(sizeof(int) * (nat + 1)));
+    AGPU_CHECK(st->nn.reserve(sizeof(double2) * (nat + 1)));
+    AGPU_CHECK(st->nconj.reserve(sizeof(double) * (nat + 1)));
+    AGPU_CHECK(st->facc.reserve(sizeof(unsigned long long) * 3 * (nat + 1)));
+    AGPU_CHECK(st->stats.reserve(sizeof(int) * 4));
+    AGPU_CHECK(pot->stats.init());
+    unsigned long long *eacc = nullptr, *wacc = nullptr;
+    if (a->epot_per_at) {
+        AGPU_CHECK(st->eacc.reserve(sizeof(unsigned long long) * (nat + 1)));
+        eacc = st->eacc.as<unsigned long long>();
+    }
+    if (a->wpot_per_at) {
+        AGPU_CHECK(st->wacc.reserve(sizeof(unsigned long long) * 9 * (nat + 1)));
+        wacc = st->wacc.as<unsigned long long>();
+    }
+    // accumulators start at zero (k_rb_finish leaves them zero for the next call)
+    AGPU_LAUNCH("rebo2_zero", k_rb_zero, dim3(512), dim3(256), 0, s, st->facc.as<unsigned long long>(),
+                (size_t)3 * nat);
+    if (eacc) AGPU_LAUNCH("rebo2_zero", k_rb_zero, dim3(256), dim3(256), 0, s, eacc, (size_t)nat);
+    if (wacc) AGPU_LAUNCH("rebo2_zero", k_rb_zero, dim3(512), dim3(256), 0, s, wacc, (size_t)9 * nat);
+    int *dstats = st->stats.as<int>();
+    AGPU_HIP(hipMemsetAsync(dstats, 0, sizeof(int) * 4, s));
+    CellArg C;
+    std::memcpy(C.A, p->cell.A, sizeof C.A);
+    const int *el = pot->d_el.as<int>();
+    const RbDev *D = st->dev.as<RbDev>();
+    const int nb = (nat + 255) / 256;
+    AGPU_LAUNCH("rebo2_bonds", k_rb_bonds, dim3(nb), dim3(256), 0, s, nat, p->d_r, el, C, D,
+                nl->offsets.as<int>(), nl->nbr.as<int2>(), st->bond.as<RbBond>(), st->bij.as<int2>(),
+                st->bown.as<int>(), st->bcnt.as<int>(), st->nn.as<double2>());
+    AGPU_LAUNCH("rebo2_conj", k_rb_conj, dim3(nb), dim3(256), 0, s, nat, el, nl->offsets.as<int>(),
+                st->bcnt.as<int>(), st->bij.as<int2>(), st->nn.as<double2>(), st->bond.as<RbBond>(),
+                st->nconj.as<double>());
+    const int ntot = (int)total;
+    int nblk = (ntot + 255) / 256;
+    if (nblk < 1) nblk = 1;
+    AGPU_CHECK(st->partial.reserve(sizeof(double) * 10 * nblk));
+    AGPU_LAUNCH("rebo2_pair", k_rb_pair, dim3(nblk), dim3(256), 0, s, ntot, D, st->tab.as<double>(), el,
+                nl->offsets.as<int>(), st->bcnt.as<int>(), st->bown.as<int>(), st->bond.as<RbBond>(),
+                st->bij.as<int2>(), st->nn.as<double2>(), st->nconj.as<double>(),
+                st->facc.as<unsigned long long>(), eacc, wacc, st->partial.as<double>(), dstats);
+    AGPU_LAUNCH("rebo2_finish", k_rb_finish, dim3(nb), dim3(256), 0, s, nat,
+                st->facc.as<unsigned long long>(), eacc, wacc, a->f, a->epot_per_at, a->wpot_per_at);
+    AGPU_CHECK(reduce_ev(st->partial.as<double>(), nblk, 10, a->epot, a->wpot, nullptr));
+    AGPU_HIP(hipMemcpyAsync(pot->stats.h, dstats, sizeof(int), hipMemcpyDeviceToHost, s));
+    AGPU_HIP(hipStreamSynchronize(s));
+    if (pot->stats.h[0] & DERR_FIXED_RANGE)
+        return set_error(AGPU_ERR_DOMAIN, "REBO2: a force contribution exceeds the fixed-point range (2^21 eV/A)");
+    if (pot->stats.h[0] & DERR_BOND_OVERFLOW)
+        return set_error(AGPU_ERR_NEB, "REBO2: bond without its reverse bond (inconsistent list)");
+    return AGPU_OK;
+}
+
+}  // namespace agpu
+
+using namespace agpu;
+
+extern "C" int agpu_rebo2_create(agpu_potential **pot, int flags)
+{
+    if (!pot) return set_error(AGPU_ERR_ARG, "pot is NULL");
+    if (flags & AGPU_REBO2_DIHEDRAL)
+        return set_error(AGPU_ERR_UNSUPPORTED, "REBO2: the dihedral term is not implemented");
+    if (flags & ~(AGPU_REBO2_ZERO_TABLES | AGPU_REBO2_DIHEDRAL))
+        return set_error(AGPU_ERR_ARG, "REBO2: unknown flag");
+    Rebo2State *st = new Rebo2State();
+    st->zero_tables = (flags & AGPU_REBO2_ZERO_TABLES) != 0;
+    int rc = rebo2_setup(st);
+    if (rc != AGPU_OK) {
+        delete st;
+        return rc;
+    }
+    agpu_potential *P = new agpu_potential();
+    P->kind = AGPU_KIND_REBO2;
+    P->impl = st;
+    P->free_impl = rebo2_free;
+    P->cutoff = st->hdev.cut_h[RB_CC];
+    *pot = P;
+    return AGPU_OK;
+}

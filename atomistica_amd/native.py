@@ -309,6 +309,23 @@ class TabulatedAlloyEAM(_Potential):
         self._h = h
 
 
+class Rebo2(_Potential):
+    """Second-generation REBO for C and H (rebo2.f90, classname Rebo2) with
+    the default tables of rebo2_default_tables.f90.  Keyword arguments follow
+    rebo2_registry.f90: zero_tables (bool) and dihedral (only False is
+    implemented)."""
+    kind = _lib.KIND_REBO2
+
+    def __init__(self, zero_tables=False, dihedral=False, elements="C,H"):
+        super().__init__()
+        if str(elements).replace(" ", "") not in ("C,H", "H,C", "*"):
+            raise AgpuError(_lib.ERR_UNSUPPORTED, "Rebo2: only the elements C and H are implemented")
+        flags = (_lib.REBO2_ZERO_TABLES if zero_tables else 0) | (_lib.REBO2_DIHEDRAL if dihedral else 0)
+        h = ctypes.c_void_p()
+        call("agpu_rebo2_create", ctypes.byref(h), int(flags))
+        self._h = h
+
+
 class Ewald(_Potential):
     """Ewald Coulomb solver (Coulomb flavour: energy_and_forces(p, nl, q, f),
     potential(p, nl, q, phi)).  beta <= 0: beta^2 = ln(10)*12/cutoff^2 as

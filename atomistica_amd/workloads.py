@@ -225,3 +225,60 @@ def random_ch(nC=50, nH=10, density=3.0, seed=7, dmin=0.9, strain=0.0):
     Z = np.array([6] * nC + [1] * nH, np.int32)
     perm = rng.permutation(nC + nH)
     return pos[perm], cell, np.array([True] * 3), Z[perm]
+
+
+def amorphous_ch(nat=100000, x_h=0.2, density=2.0, seed=10,
+                 dmin=(("CC", 1.2), ("CH", 1.0), ("HH", 0.8)), batch=20000):
+    """Amorphous hydrocarbon: nat atoms (fraction x_h hydrogen) uniformly at
+    random in a cubic periodic box of the given mass density (g/cm^3), with
+    rejection at the pair minimum distances dmin (random sequential
+    addition, candidates proposed in batches and accepted in order).
+    Returns (pos, cell, pbc, Z) with the atoms in insertion order."""
+    from scipy.spatial import cKDTree
+    rng = np.random.default_rng(seed)
+    nH = int(round(nat * x_h))
+    nC = nat - nH
+    V = (nC * C_MASS + nH * H_MASS) * AMU_G / density * 1e24
+    L = V ** (1.0 / 3.0)
+    d = dict(dmin)
+    dmax = max(d.values())
+    dij = np.array([[d["CC"], d["CH"]], [d["CH"], d["HH"]]])    # species 0 = C, 1 = H
+    species = np.array([0] * nC + [1] * nH)
+    rng.shuffle(species)
+    pos = np.zeros((nat, 3))
+    n = 0
+    tree = None
+    while n < nat:
+        m = min(batch, nat - n)
+        cand = rng.random((m, 3)) * L
+        sp = species[n:n + m]
+        ok = np.ones(m, bool)
+        if tree is not None:
+            hits = tree.query_ball_point(cand, dmax)
+            for c, h in enumerate(hits):
+                if h:
+                    dd = pos[h] - cand[c]
+                    dd -= L * np.round(dd / L)
+                    r = np.sqrt(np.einsum("kx,kx->k", dd, dd))
+                    if np.any(r < dij[sp[c], species[h]]):
+                        ok[c] = False
+        # conflicts inside the batch: keep the earlier candidate
+        bt = cKDTree(cand, boxsize=L)
+        for a, b in sorted(bt.query_pairs(dmax)):
+            if ok[a] and ok[b]:
+                dd = cand[b] - cand[a]
+                dd -= L * np.round(dd / L)
+                if np.sqrt(dd @ dd) < dij[sp[a], sp[b]]:
+                    ok[b] = False
+        acc = cand[ok]
+        # keep the species sequence: accepted candidates take the next slots
+        k = len(acc)
+        spk = sp[ok]
+        # reorder the remaining species so that the accepted ones come first
+        rest = np.concatenate([spk, sp[~ok], species[n + m:]])
+        species[n:] = rest
+        pos[n:n + k] = acc
+        n += k
+        tree = cKDTree(np.mod(pos[:n], L), boxsize=L)
+    Z = np.where(species == 0, 6, 1).astype(np.int32)
+    return pos, np.diag([L, L, L]).astype(np.float64), np.array([True] * 3), Z

@@ -52,6 +52,11 @@ def build_workload(name):
                 "rc=10, erfc(beta rc)=1e-8, exp(-kmax^2/4beta^2)=1e-8")
         return dict(pos=pos, cell=cell, pbc=pbc, Z=Z, q=q, kind="ewald", beta=beta,
                     kmax=W.ewald_kmax(beta, 1e-8), avgn=220, desc=desc)
+    if name == "rebo2_ch_100k":
+        pos, cell, pbc, Z = W.amorphous_ch(100000, x_h=0.2, density=2.0, seed=10)
+        desc = ("REBO2 amorphous hydrocarbon 100000 atoms (80% C, 20% H), cubic periodic box at "
+                "2.0 g/cm^3, uniform random with rejection at C-C 1.2, C-H 1.0, H-H 0.8 A (rng 10)")
+        return dict(pos=pos, cell=cell, pbc=pbc, Z=Z, kind="rebo2", avgn=24, desc=desc)
     raise SystemExit("unknown workload " + name)
 
 
@@ -60,6 +65,8 @@ def make_potential(N, wl, precision):
         pot = N.TabulatedAlloyEAM(fn=wl["fn"])
     elif wl["kind"] == "tersoff":
         pot = N.Tersoff()
+    elif wl["kind"] == "rebo2":
+        pot = N.Rebo2(dihedral=wl.get("dihedral", False))
     else:
         pot = N.Ewald(cutoff=10.0, beta=wl["beta"], kmax=wl["kmax"])
     pot.set_precision(precision)
@@ -81,6 +88,8 @@ def algorithmic_bytes(kernel, nn):
         return 36
     if kernel.startswith("ewald_real"):
         return 64 + 8 * nn
+    if kernel == "rebo2_pair":
+        return 64 + 8 * nn            # bond records stream once per bond-pair owner
     return None
 
 
@@ -146,6 +155,13 @@ def cpu_baseline(wl, budget_s=12.0):
         def one():
             nl = O.nl_build(pos, cell, pbc, 3.0, nthreads=threads)
             O.tersoff(P.Tersoff_PRB_39_5566_Si_C, pos, el, cell, nl, nthreads=threads)
+        sample_atoms = nat
+    elif wl["kind"] == "rebo2":
+        R = O.Rebo2(dihedral=wl.get("dihedral", False))
+
+        def one():
+            nl = O.nl_build(pos, cell, pbc, 2.0, nthreads=threads)
+            R(pos, wl["Z"], cell, nl, nthreads=threads)
         sample_atoms = nat
     else:
         return None

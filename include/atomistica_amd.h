@@ -60,6 +60,11 @@ extern "C" {
 #define AGPU_KIND_TERSOFF 1
 #define AGPU_KIND_EAM_ALLOY 2
 #define AGPU_KIND_EWALD 3
+#define AGPU_KIND_REBO2 4
+
+/* agpu_rebo2_create flags */
+#define AGPU_REBO2_ZERO_TABLES 1  /* rebo2 zero_tables = .true. (rebo2_module.f90:30-42) */
+#define AGPU_REBO2_DIHEDRAL 2     /* with_dihedral = .true.: AGPU_ERR_UNSUPPORTED */
 
 typedef struct agpu_particles agpu_particles;
 typedef struct agpu_neighbors agpu_neighbors;
@@ -179,6 +184,13 @@ int agpu_eam_alloy_create(agpu_potential **pot, const char *fn, const char *elem
  * beta^2 = ln(10)*12/cutoff^2 (pme.f90:192); kmax <= 0 selects
  * exp(-kmax^2/(4 beta^2)) = 1e-13. */
 int agpu_ewald_create(agpu_potential **pot, double cutoff, double beta, double kmax);
+
+/* REBO2 (Brenner et al. 2002) for C and H without screening: rebo2_init and
+ * rebo2_db_init_with_parameters (rebo2_module.f90:30-42, rebo2_db.f90:81-303)
+ * with the default tables of rebo2_default_tables.f90; energy_and_forces is
+ * rebo2_kernel (bop_kernel_rebo2.f90).  FP64 only; no mask (the reference
+ * kernel takes none). */
+int agpu_rebo2_create(agpu_potential **pot, int flags);
 
 int agpu_potential_destroy(agpu_potential *pot);
 int agpu_potential_kind(agpu_potential *pot, int *kind);

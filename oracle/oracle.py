@@ -349,17 +349,17 @@ def read_setfl_tables(path):
 class Rebo2:
     """Handle on the oracle's REBO2 parameter set and tables."""
 
-    def __init__(self, zero_tables=False):
+    def __init__(self, zero_tables=False, dihedral=False):
         L = lib()
         L.orc_rebo2_new.restype = ctypes.c_void_p
-        L.orc_rebo2_new.argtypes = [ctypes.c_int]
+        L.orc_rebo2_new.argtypes = [ctypes.c_int, ctypes.c_int]
         L.orc_rebo2_free.argtypes = [ctypes.c_void_p]
         L.orc_rebo2_eval_g.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
                                        ctypes.c_double, _dp]
         L.orc_rebo2_eval_table.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
                                            ctypes.c_double, ctypes.c_double, _dp]
         L.orc_rebo2_default_table.argtypes = [ctypes.c_int, _dp]
-        self._h = L.orc_rebo2_new(int(bool(zero_tables)))
+        self._h = L.orc_rebo2_new(int(bool(zero_tables)), int(bool(dihedral)))
         if not self._h:
             raise OracleError("orc_rebo2_new failed")
 
@@ -374,7 +374,7 @@ class Rebo2:
         lib().orc_rebo2_eval_g(self._h, int(ktyp), float(costh), float(n), _d(out))
         return out
 
-    TABLES = {"Fcc": 0, "Fch": 1, "Fhh": 2, "Pcc": 3, "Pch": 4}
+    TABLES = {"Fcc": 0, "Fch": 1, "Fhh": 2, "Pcc": 3, "Pch": 4, "Tcc": 5}
 
     def table(self, name, x, y, z=0.0):
         """Interpolated table value and its partial derivatives."""
@@ -391,10 +391,10 @@ class Rebo2:
             out = np.zeros(1000)
             _check(lib().orc_rebo2_default_table(w, _d(out)), "rebo2 default table")
             return tuple(out[250 * q:250 * (q + 1)].reshape(5, 5, 10) for q in range(4))
-        n = 250 if w < 3 else 36
-        out = np.zeros(n)
+        tri = w in (1, 2, 5)
+        out = np.zeros(250 if tri else 36)
         _check(lib().orc_rebo2_default_table(w, _d(out)), "rebo2 default table")
-        return out.reshape((5, 5, 10) if w < 3 else (6, 6))
+        return out.reshape((5, 5, 10) if tri else (6, 6))
 
     def __call__(self, r, Z, cell, nl, per_at=False, nthreads=1):
         """Energy, forces and virial (plus per-atom energies/virials)."""

@@ -87,9 +87,9 @@ int orc_ewald_kspace(int nat, const double *r, const double *q,
                      double *epot, double *f, double *wpot, double *phi,
                      int nthreads);
 
-/* ---- REBO2 (C, H; no screening, no dihedral) ---- */
+/* ---- REBO2 (C, H; no screening; optional dihedral term) ---- */
 typedef struct orc_rebo2_s orc_rebo2_t;
-orc_rebo2_t *orc_rebo2_new(int zero_tables);
+orc_rebo2_t *orc_rebo2_new(int zero_tables, int with_dihedral);
 void orc_rebo2_free(orc_rebo2_t *p);
 int orc_rebo2(const orc_rebo2_t *p, int nat, const double *r, const int *Z,
               const double *cell, const int64_t *offsets, const int *nbr, const int *dc,

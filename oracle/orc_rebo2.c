@@ -1,7 +1,7 @@
 /*
  * Oracle: second-generation REBO (Brenner et al. 2002) for C and H, without
- * screening and without the dihedral term (the reference's default,
- * with_dihedral = .false.).  TEST INFRASTRUCTURE ONLY (see orc.h).
+ * screening; the dihedral (torsion) term is optional as in the reference
+ * (with_dihedral, default .false.).  TEST INFRASTRUCTURE ONLY (see orc.h).
  *
  * Restates:
  *   - parameters and constants      src/potentials/bop/rebo2/rebo2_type.f90:73-200
@@ -60,6 +60,8 @@ struct orc_rebo2_s {
     /* tables: coefficients c[box][p1][p2][p3] (3d) and c[box][p1][p2] (2d) */
     double Fcc[4 * 4 * 9][64], Fch[4 * 4 * 9][64], Fhh[4 * 4 * 9][64];
     double Pcc[5 * 5][16], Pch[5 * 5][16];
+    double Tcc[4 * 4 * 9][64];
+    int with_dihedral;
 };
 
 /* ------------------------------------------------------------------------ */
@@ -328,6 +330,13 @@ static void default_Fhh(double *F)
     F[I3(1, 1, 0)] = 0.249831916;
 }
 
+static void default_Tcc(double *T)
+{
+    memset(T, 0, sizeof(double) * 250);
+    T[I3(2, 2, 0)] = -0.070280085;
+    for (int k = 1; k <= 8; ++k) T[I3(2, 2, k)] = -0.00809675;
+}
+
 static void default_Pcc(double *P)
 {
     memset(P, 0, sizeof(double) * 36);
@@ -407,7 +416,7 @@ static int make_cc_g_spline(struct orc_rebo2_s *p)
 /* ------------------------------------------------------------------------ */
 /* construction                                                               */
 /* ------------------------------------------------------------------------ */
-orc_rebo2_t *orc_rebo2_new(int zero_tables)
+orc_rebo2_t *orc_rebo2_new(int zero_tables, int with_dihedral)
 {
     struct orc_rebo2_s *p = calloc(1, sizeof(struct orc_rebo2_s));
     if (!p) return NULL;
@@ -441,6 +450,7 @@ orc_rebo2_t *orc_rebo2_new(int zero_tables)
     p->cc_in_r1 = 1.70; p->cc_in_r2 = 2.00;
     p->ch_r1 = 1.30; p->ch_r2 = 1.80;
     p->hh_r1 = 1.10; p->hh_r2 = 1.70;
+    p->with_dihedral = with_dihedral != 0;
 
     /* rebo2_db_init_with_parameters */
     p->conpe[KC] = -0.5; p->conpe[KH] = -0.5;
@@ -477,6 +487,11 @@ orc_rebo2_t *orc_rebo2_new(int zero_tables)
         default_Fhh(F);
         if (zero_tables) memset(F, 0, sizeof(double) * 250);
         err = table3d_init(4, 4, 9, F, NULL, NULL, NULL, p->Fhh);
+    }
+    if (!err) {
+        default_Tcc(F);
+        if (zero_tables) memset(F, 0, sizeof(double) * 250);
+        err = table3d_init(4, 4, 9, F, NULL, NULL, NULL, p->Tcc);
     }
     if (!err) {
         default_Pcc(P);
@@ -643,6 +658,7 @@ int orc_rebo2_eval_table(const orc_rebo2_t *p, int which, double x, double y, do
     case 2: table3d_eval(p->Fhh, 4, 4, 9, x, y, z, &out4[0], &out4[1], &out4[2], &out4[3]); break;
     case 3: table2d_eval(p->Pcc, 5, 5, x, y, &out4[0], &out4[1], &out4[2]); break;
     case 4: table2d_eval(p->Pch, 5, 5, x, y, &out4[0], &out4[1], &out4[2]); break;
+    case 5: table3d_eval(p->Tcc, 4, 4, 9, x, y, z, &out4[0], &out4[1], &out4[2], &out4[3]); break;
     default: return ORC_ERR_ARG;
     }
     return ORC_OK;
@@ -657,6 +673,7 @@ int orc_rebo2_default_table(int which, double *out)
     case 2: default_Fhh(out); break;
     case 3: default_Pcc(out); break;
     case 4: default_Pch(out); break;
+    case 5: default_Tcc(out); break;
     default: return ORC_ERR_ARG;
     }
     return ORC_OK;
@@ -983,6 +1000,91 @@ int orc_rebo2(const orc_rebo2_t *p, int nat, const double *r, const int *Z,
                 if (nconj > 8.0) nconj = 8.0;
                 if (nti > 3.0) nti = 3.0;
                 if (ntj > 3.0) ntj = 3.0;
+                /* torsion (bop_kernel_rebo2.f90:1939-2120, non-ALT_DIHEDRAL form) */
+                double bdh = 0.0, tij = 0.0, dtdni = 0.0, dtdnj = 0.0, dtdncn = 0.0;
+                if (p->with_dihedral && ijpot == PCC) {
+                    table3d_eval(p->Tcc, 4, 4, 9, nti, ntj, nconj, &tij, &dtdni, &dtdnj, &dtdncn);
+                    double tije = tij * faij * fcarij;
+                    if (tij != 0) {
+                        for (int64_t ik = is; ik < ie; ++ik) {
+                            if (ik == ij) continue;
+                            const rbond *bk = &bd[ik];
+                            int k = bk->j;
+                            double rlik = bk->len;
+                            const double *rnik = bk->nm;
+                            double fcik = bk->fc, dfcikr = bk->dfc;
+                            double dot_ij_ik = 0.0;
+                            dot_ij_ik = dot_ij_ik + rnij[0] * rnik[0];
+                            dot_ij_ik = dot_ij_ik + rnij[1] * rnik[1];
+                            dot_ij_ik = dot_ij_ik + rnij[2] * rnik[2];
+                            double dcik = 1.0 - dot_ij_ik * dot_ij_ik;
+                            for (int64_t jl = js; jl < je; ++jl) {
+                                const rbond *bl = &bd[jl];
+                                int l = bl->j;
+                                int ldc[3] = {bij_->dc[0] + bl->dc[0], bij_->dc[1] + bl->dc[1],
+                                              bij_->dc[2] + bl->dc[2]};
+                                if (l == i && ldc[0] == 0 && ldc[1] == 0 && ldc[2] == 0) continue;
+                                if (l == k && ldc[0] == bk->dc[0] && ldc[1] == bk->dc[1] &&
+                                    ldc[2] == bk->dc[2])
+                                    continue;
+                                double rljl = bl->len;
+                                const double *rnjl = bl->nm;
+                                double fcjl = bl->fc, dfcjlr = bl->dfc;
+                                double dot_ij_jl = 0.0, dot_ik_jl = 0.0;
+                                dot_ij_jl = dot_ij_jl + rnij[0] * rnjl[0];
+                                dot_ij_jl = dot_ij_jl + rnij[1] * rnjl[1];
+                                dot_ij_jl = dot_ij_jl + rnij[2] * rnjl[2];
+                                dot_ik_jl = dot_ik_jl + rnik[0] * rnjl[0];
+                                dot_ik_jl = dot_ik_jl + rnik[1] * rnjl[1];
+                                dot_ik_jl = dot_ik_jl + rnik[2] * rnjl[2];
+                                double dcjl = 1.0 - dot_ij_jl * dot_ij_jl;
+                                double abs_dc = sqrt(dcik * dcjl);
+                                double cost = (dot_ij_ik * dot_ij_jl - dot_ik_jl) / abs_dc;
+                                double bdhij = 1 - cost * cost;
+                                bdh = bdh + bdhij * fcik * fcjl;
+                                bdhij = bdhij * tij * faij * fcarij / 2;
+                                double dbdhij = -2 * cost * tije * fcik * fcjl / 2;
+                                double df[3], v[3];
+                                for (int x = 0; x < 3; ++x)
+                                    df[x] = dbdhij *
+                                            ((dot_ij_jl / abs_dc + cost * dot_ij_ik / dcik) * rnik[x] +
+                                             (dot_ij_ik / abs_dc + cost * dot_ij_jl / dcjl) * rnjl[x] -
+                                             (2 * dot_ik_jl / abs_dc + cost * (1.0 / dcik + 1.0 / dcjl)) * rnij[x]) /
+                                            rlij;
+                                for (int x = 0; x < 3; ++x) {
+                                    fi[x] += df[x];
+                                    fj[x] -= df[x];
+                                    v[x] = rlij * rnij[x];
+                                }
+                                outer_add(wij, 1.0, v, df);
+                                for (int x = 0; x < 3; ++x)
+                                    df[x] = dbdhij *
+                                                (-1.0 / dcik * cost * rnik[x] - 1.0 / abs_dc * rnjl[x] +
+                                                 (dot_ij_jl / abs_dc + cost * dot_ij_ik / dcik) * rnij[x]) /
+                                                rlik +
+                                            bdhij * dfcikr * fcjl * rnik[x];
+                                for (int x = 0; x < 3; ++x) {
+                                    fi[x] += df[x];
+                                    fl[3 * k + x] += -df[x];
+                                    v[x] = rlik * rnik[x];
+                                }
+                                outer_add(wij, 1.0, v, df);
+                                for (int x = 0; x < 3; ++x)
+                                    df[x] = dbdhij *
+                                                (-1.0 / dcjl * cost * rnjl[x] - 1.0 / abs_dc * rnik[x] +
+                                                 (dot_ij_ik / abs_dc + cost * dot_ij_jl / dcjl) * rnij[x]) /
+                                                rljl +
+                                            bdhij * fcik * dfcjlr * rnjl[x];
+                                for (int x = 0; x < 3; ++x) {
+                                    fj[x] += df[x];
+                                    fl[3 * l + x] += -df[x];
+                                    v[x] = rljl * rnjl[x];
+                                }
+                                outer_add(wij, 1.0, v, df);
+                            }
+                        }
+                    }
+                }
                 double fij = 0.0, dfdni = 0.0, dfdnj = 0.0, dfdncn = 0.0;
                 if (ijpot == PCC)
                     table3d_eval(p->Fcc, 4, 4, 9, nti, ntj, nconj, &fij, &dfdni, &dfdnj, &dfdncn);
@@ -992,6 +1094,9 @@ int orc_rebo2(const orc_rebo2_t *p, int nat, const double *r, const int *Z,
                     table3d_eval(p->Fch, 4, 4, 9, ntj, nti, nconj, &fij, &dfdnj, &dfdni, &dfdncn);
                 else if (kj == KC)
                     table3d_eval(p->Fch, 4, 4, 9, nti, ntj, nconj, &fij, &dfdni, &dfdnj, &dfdncn);
+                dfdni = dfdni + dtdni * bdh;
+                dfdnj = dfdnj + dtdnj * bdh;
+                dfdncn = dfdncn + dtdncn * bdh;
                 dfdni = 0.5 * fcarij * faij * dfdni;
                 dfdnj = 0.5 * fcarij * faij * dfdnj;
                 dfdncn = 0.5 * fcarij * faij * dfdncn;
@@ -1083,7 +1188,7 @@ int orc_rebo2(const orc_rebo2_t *p, int nat, const double *r, const int *Z,
                 }
 
                 /* pair energy and the remaining forces */
-                double baveij = 0.5 * (bij + bji + fij);
+                double baveij = 0.5 * (bij + bji + fij + tij * bdh);
                 double hlfvij = fcarij * (frij + baveij * faij) / 2;
                 pe[i] += hlfvij;
                 pe[j] += hlfvij;

@@ -1,0 +1,118 @@
+"""REBO2 HIP path against the CPU oracle through the C-ABI (MI355X only).
+
+Bars: energy, forces, virial and per-atom energies/virials within 1e-10
+relative (forces and virials scaled by their largest entry, as in the other
+parity tests); results bitwise reproducible run to run (fixed-point force
+accumulation).  At the full bench size: perfect diamond at the REBO2 lattice
+constant gives the cohesive energy of the small cell per atom and zero net
+force.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from atomistica_amd import native as N
+from atomistica_amd import workloads as W
+from atomistica_amd._lib import AgpuError
+from helpers import in_vacuum_box, read_cfg, read_xyz
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-10
+
+
+def close(a, b, rel=REL):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    scale = max(1.0, float(np.abs(b).max()) if b.size else 1.0)
+    return float(np.abs(a - b).max()) <= rel * scale
+
+
+def run_gpu(pos, cell, pbc, Z, per_at=False, precision="fp64", mask=None):
+    p = N.from_atoms(pos, cell, pbc, Z)
+    nl = N.Neighbors(100)
+    pot = N.Rebo2()
+    pot.set_precision(precision)
+    pot.bind_to(p, nl)
+    return pot.energy_and_forces(p, nl, epot_per_at=per_at, wpot_per_at=per_at, mask=mask)
+
+
+_R = None
+
+
+def run_oracle(pos, cell, pbc, Z, per_at=False):
+    global _R
+    if _R is None:
+        _R = O.Rebo2()
+    nl = O.nl_build(pos, cell, pbc, 2.0)
+    return _R(pos, Z, cell, nl, per_at=per_at)
+
+
+def _molecule(name):
+    pos, Z = read_xyz("molecules/%s.xyz" % name)
+    pos, cell, pbc = in_vacuum_box(pos)
+    return pos, cell, pbc, Z
+
+
+CASES = [
+    ("diamond_rattled", lambda: W.diamond_c(3, sigma=0.08, seed=3)),
+    ("aC_small", lambda: read_cfg("aC_small.cfg")),
+    ("aC", lambda: read_cfg("aC.cfg")),
+    ("random_CH", lambda: W.random_ch(50, 10, 3.0, seed=7, strain=0.1)),
+    ("random_CH_large", lambda: W.random_ch(1500, 500, 2.2, seed=8, strain=0.15)),
+    ("naphthalene", lambda: _molecule("naphthalene")),
+    ("neopentane", lambda: _molecule("neopentane")),
+    ("small_cell", lambda: W.random_ch(10, 4, 2.5, seed=5, strain=0.2)),
+]
+
+
+@pytest.mark.parametrize("name,make", CASES, ids=[c[0] for c in CASES])
+def test_rebo2_fp64_parity(name, make):
+    pos, cell, pbc, Z = make()
+    print("rebo2 case", name, len(pos), flush=True)
+    e, f, w, epa, _, _, wpa, _ = run_gpu(pos, cell, pbc, Z, per_at=True)
+    re_, rf, rw, repa, rwpa = run_oracle(pos, cell, pbc, Z, per_at=True)
+    assert abs(e - re_) <= REL * max(1.0, abs(re_))
+    assert close(f, rf) and close(w, rw)
+    assert close(epa, repa) and close(wpa, rwpa)
+
+
+def test_rebo2_ignores_other_elements():
+    pos, cell, pbc, Z = W.random_ch(60, 20, 2.5, seed=12, strain=0.1)
+    Z = Z.copy()
+    Z[::9] = 14       # atoms outside C/H are not part of any bond
+    e, f, w, *_ = run_gpu(pos, cell, pbc, Z)
+    re_, rf, rw = run_oracle(pos, cell, pbc, Z)
+    assert abs(e - re_) <= REL * abs(re_) and close(f, rf) and close(w, rw)
+
+
+def test_rebo2_bitwise_reproducible():
+    pos, cell, pbc, Z = W.random_ch(2000, 600, 2.4, seed=13, strain=0.1)
+    e1, f1, w1, *_ = run_gpu(pos, cell, pbc, Z)
+    e2, f2, w2, *_ = run_gpu(pos, cell, pbc, Z)
+    assert e1 == e2 and np.array_equal(f1, f2) and np.array_equal(w1, w2)
+
+
+def test_rebo2_unsupported_modes():
+    pos, cell, pbc, Z = W.diamond_c(2, sigma=0.02, seed=1)
+    with pytest.raises(AgpuError):
+        run_gpu(pos, cell, pbc, Z, precision="mixed")
+    with pytest.raises(AgpuError):
+        run_gpu(pos, cell, pbc, Z, mask=np.ones(len(pos), np.int32))
+    with pytest.raises(AgpuError):
+        N.Rebo2(dihedral=True)
+
+
+def test_rebo2_full_size_diamond():
+    # 50^3 cells = 1,000,000 atoms at the lattice constant of the small-cell
+    # energy minimum: every atom sees the same environment
+    a0 = 3.5657500628
+    pos, cell, pbc, Z = W.diamond_c(50, a=a0, sigma=0.0)
+    e, f, w, *_ = run_gpu(pos, cell, pbc, Z)
+    sp, sc, _, sZ = W.diamond_c(2, a=a0, sigma=0.0)
+    re_, rf, rw = run_oracle(sp, sc, pbc, sZ)
+    print("rebo2 1M diamond E/atom %.12f  small cell %.12f  max|f| %.3e" %
+          (e / len(pos), re_ / len(sp), np.abs(f).max()), flush=True)
+    assert abs(e / len(pos) - re_ / len(sp)) <= 1e-10 * abs(re_ / len(sp))
+    assert np.abs(f).max() < 1e-9
+    assert close(w / len(pos), rw / len(sp), rel=1e-9)

@@ -234,3 +234,47 @@ def test_dimer_curves_smooth(R, pair, dmin, dmax, de, df):
         fs.append(f[0, 0])
     assert np.max(np.abs(np.diff(es))) < de
     assert np.max(np.abs(np.diff(fs))) < df
+
+
+# ---------------------------------------------------------------------------
+# torsion term (with_dihedral = .true.)
+# ---------------------------------------------------------------------------
+
+@pytest.fixture(scope="module")
+def Rd():
+    return O.Rebo2(dihedral=True)
+
+
+def test_torsion_table_nodes(Rd):
+    T = O.Rebo2.default_table("Tcc")
+    assert T[2, 2, 0] == -0.070280085 and T[2, 2, 5] == -0.00809675
+    for k in range(10):
+        assert abs(Rd.table("Tcc", 2, 2, k)[0] - T[2, 2, k]) < 1e-14
+
+
+def test_torsion_ethylene_closed_form(R, Rd):
+    # Rotating one CH2 of ethylene about the C-C axis leaves every bond
+    # length and bond angle unchanged, so only the torsion term changes:
+    # planar -> sin^2(omega) = 0 for all four H-C-C-H; twisted by 90 deg ->
+    # sin^2 = 1, and E_tors = 1/2 T_CC(2, 2, 0) V_A(r_CC) * 4.
+    planar = np.array([[-0.667, 0, 0], [0.667, 0, 0], [-1.23, 0.92, 0], [-1.23, -0.92, 0],
+                       [1.23, 0.92, 0], [1.23, -0.92, 0]])
+    twisted = planar.copy()
+    twisted[4] = [1.23, 0, 0.92]
+    twisted[5] = [1.23, 0, -0.92]
+    Z = np.array([6, 6, 1, 1, 1, 1], np.int32)
+    r = 1.334
+    VA = -(12388.79197798 * np.exp(-4.7204523127 * r) + 17.56740646509 * np.exp(-1.4332132499 * r)
+           + 30.71493208065 * np.exp(-1.3826912506 * r))
+    for X, n_sin2 in ((planar, 0.0), (twisted, 4.0)):
+        pos, cell, pbc = in_vacuum_box(X)
+        e0 = energy(R, pos, cell, pbc, Z)[0]
+        e1 = energy(Rd, pos, cell, pbc, Z)[0]
+        assert abs((e1 - e0) - 0.5 * (-0.070280085) * VA * n_sin2) < 1e-12
+
+
+def test_torsion_fd_forces_and_virial(Rd):
+    pos, cell, pbc, Z = read_cfg("aC_small.cfg")
+    _fd_case(Rd, pos, cell, pbc, Z, range(0, len(pos), 17), 1e-6, 1e-6)
+    pos, cell, pbc, Z = W.random_ch(50, 10, 3.0, seed=7, strain=0.1)
+    _fd_case(Rd, pos, cell, pbc, Z, range(0, 60, 5), 1e-6, 1e-6)
