@@ -1,6 +1,6 @@
 // Second-generation REBO (Brenner et al. 2002) for carbon and hydrogen: the
 // non-screened rebo2 class of the reference (src/potentials/bop/rebo2/),
-// without the dihedral term (the reference default, with_dihedral = false).
+// with the optional dihedral (torsion) term of with_dihedral = .true.
 //
 // Kernels (one stream, no host round trip inside a call):
 //   k_rb_zero     clear the 64-bit fixed-point accumulators
@@ -11,7 +11,8 @@
 //   k_rb_conj     thread per atom: F(x_ik) of every carbon neighbour k and
 //                 the sum N^conj_i (ik_loop1, bop_kernel_rebo2.f90:1231-1325)
 //   k_rb_pair     thread per bond i->j with (dc = 0 and j > i) or dc > 0:
-//                 b_ij, b_ji with P, F_ij(N_i, N_j, N_conj), the pair energy
+//                 b_ij, b_ji with P, F_ij(N_i, N_j, N_conj), the torsion sum
+//                 T_ij(N_i, N_j, N_conj) sum_kl (1 - cos^2 w_kijl), the pair energy
 //                 and every force on i, j, k, l, m, n (ij_loop,
 //                 bop_kernel_rebo2.f90:1326-2779).  Forces, per-atom energies
 //                 and virials go to 64-bit fixed-point accumulators with
@@ -20,7 +21,7 @@
 //                 reduced per block and then in a fixed order.
 //   k_rb_finish   fixed point -> FP64, added to the caller's arrays.
 //
-// Tables (F_CC, F_CH, F_HH tricubic, P_CC, P_CH bicubic) are built on the
+// Tables (F_CC, F_CH, F_HH, T_CC tricubic, P_CC, P_CH bicubic) are built on the
 // host from their node data in closed form: the tensor product of cubic
 // Hermite polynomials with the tabulated values and first derivatives and
 // zero cross derivatives, which is the unique solution of the corner system
@@ -48,6 +49,7 @@ struct RbDev {
     double g1c[3][6], g2c[3][6];     // carbon g(cos): high / low coordination
     double spgh[3][6];               // hydrogen g(cos) polynomials
     int igh[25];
+    int with_dihedral;
 };
 
 // table layout in one device block: 3 tricubic tables of 4*4*9 boxes x 64
@@ -56,7 +58,8 @@ constexpr int RB_T3_BOXES = 4 * 4 * 9;
 constexpr int RB_T3 = RB_T3_BOXES * 64;
 constexpr int RB_T2 = 25 * 16;
 constexpr int RB_TAB_FCC = 0, RB_TAB_FCH = RB_T3, RB_TAB_FHH = 2 * RB_T3, RB_TAB_PCC = 3 * RB_T3,
-              RB_TAB_PCH = 3 * RB_T3 + RB_T2, RB_TAB_N = 3 * RB_T3 + 2 * RB_T2;
+              RB_TAB_PCH = 3 * RB_T3 + RB_T2, RB_TAB_TCC = 3 * RB_T3 + 2 * RB_T2,
+              RB_TAB_N = 4 * RB_T3 + 2 * RB_T2;
 
 // fixed point: 2^-40 eV/A (or eV) per unit
 constexpr double RB_FX = 1099511627776.0;
@@ -71,6 +74,7 @@ struct RbBond {
 
 struct Rebo2State {
     int zero_tables = 0;
+    int with_dihedral = 0;
     RbDev hdev{};
     std::vector<double> htab;
     DevBuf dev, tab, bond, bij, bown, bcnt, nn, nconj, facc, eacc, wacc, partial, stats;
@@ -200,9 +204,9 @@ static void hermite2(int nx, int ny, const double *V, double *out)
 // F(0:4, 0:4, 0:9) at [(i*5+j)*10+k], P(0:5, 0:5) at [i*6+j].
 #define T3(i, j, k) (((i) * 5 + (j)) * 10 + (k))
 static void default_tables(double *Fcc, double *Fi, double *Fj, double *Fk, double *Fch,
-                           double *Fhh, double *Pcc, double *Pch)
+                           double *Fhh, double *Tcc, double *Pcc, double *Pch)
 {
-    for (int q = 0; q < 250; ++q) Fcc[q] = Fi[q] = Fj[q] = Fk[q] = Fch[q] = Fhh[q] = 0.0;
+    for (int q = 0; q < 250; ++q) Fcc[q] = Fi[q] = Fj[q] = Fk[q] = Fch[q] = Fhh[q] = Tcc[q] = 0.0;
     for (int q = 0; q < 36; ++q) Pcc[q] = Pch[q] = 0.0;
     // F_CC values (Table 4) with the linear ramps in N_conj of the reference
     Fcc[T3(1, 1, 0)] = 0.105000;
@@ -262,6 +266,9 @@ static void default_tables(double *Fcc, double *Fi, double *Fj, double *Fk, doub
             }
     // F_HH (Table 6)
     Fhh[T3(1, 1, 0)] = 0.249831916;
+    // T_CC (Table 5)
+    Tcc[T3(2, 2, 0)] = -0.070280085;
+    for (int k = 1; k <= 8; ++k) Tcc[T3(2, 2, k)] = -0.00809675;
     // P_CC, P_CH (Table 8), indexed (N_H, N_C)
     Pcc[1 * 6 + 1] = 0.003026697473481;
     Pcc[2 * 6 + 0] = 0.007860700254745;
@@ -337,8 +344,9 @@ static int rebo2_setup(Rebo2State *st)
         D.cut_fac[t3[q]] = M_PI / (r2[q] - r1[q]);
     }
     // tables
-    std::vector<double> F(250 * 6), P(72);
-    default_tables(&F[0], &F[250], &F[500], &F[750], &F[1000], &F[1250], &P[0], &P[36]);
+    D.with_dihedral = st->with_dihedral;
+    std::vector<double> F(250 * 7), P(72);
+    default_tables(&F[0], &F[250], &F[500], &F[750], &F[1000], &F[1250], &F[1500], &P[0], &P[36]);
     if (st->zero_tables) {
         std::fill(F.begin(), F.end(), 0.0);
         std::fill(P.begin(), P.end(), 0.0);
@@ -347,6 +355,7 @@ static int rebo2_setup(Rebo2State *st)
     hermite3(4, 4, 9, &F[0], &F[250], &F[500], &F[750], &st->htab[RB_TAB_FCC]);
     hermite3(4, 4, 9, &F[1000], nullptr, nullptr, nullptr, &st->htab[RB_TAB_FCH]);
     hermite3(4, 4, 9, &F[1250], nullptr, nullptr, nullptr, &st->htab[RB_TAB_FHH]);
+    hermite3(4, 4, 9, &F[1500], nullptr, nullptr, nullptr, &st->htab[RB_TAB_TCC]);
     hermite2(5, 5, &P[0], &st->htab[RB_TAB_PCC]);
     hermite2(5, 5, &P[36], &st->htab[RB_TAB_PCH]);
     AGPU_CHECK(st->dev.reserve(sizeof(RbDev)));
@@ -757,6 +766,117 @@ __device__ __forceinline__ void rb_nforces(int centre, int beg, int nb, int skip
     }
 }
 
+__device__ __forceinline__ void outer_add(double *w, const double *a, const double *b)
+{
+#pragma unroll
+    for (int y = 0; y < 3; ++y)
+#pragma unroll
+        for (int x = 0; x < 3; ++x) w[x + 3 * y] = w[x + 3 * y] + a[x] * b[y];
+}
+
+// torsion sum over k (neighbours of i but j) and l (neighbours of j but i,
+// and not k): returns bdh = sum (1 - cos^2 w) fc_ik fc_jl; forces on i, j
+// into fi, fj, on k and l through the accumulators; virial into w
+// (bop_kernel_rebo2.f90:1939-2120)
+__device__ double rb_torsion(int ib, int inb, int sij, int jb, int jnb, int sji, int dcij,
+                             const RbBond *__restrict__ bond, const int2 *__restrict__ bij,
+                             const double *rnij, double rlij, double tij, double fa, double fcar,
+                             double *fi, double *fj, double *w, unsigned long long *__restrict__ facc,
+                             int &bad)
+{
+    const double tije = tij * fa * fcar;
+    double bdh = 0.0;
+    int a0, b0, c0;
+    dc_unpack(dcij, a0, b0, c0);
+    for (int kb = ib; kb < ib + inb; ++kb) {
+        if (kb == sij) continue;
+        const RbBond K = bond[kb];
+        const int2 ek = bij[kb];
+        int ak, bk, ck;
+        dc_unpack(ek.y, ak, bk, ck);
+        const double *rnik = K.n;
+        const double rlik = K.len;
+        double dot_ij_ik = 0.0;
+        dot_ij_ik = dot_ij_ik + rnij[0] * rnik[0];
+        dot_ij_ik = dot_ij_ik + rnij[1] * rnik[1];
+        dot_ij_ik = dot_ij_ik + rnij[2] * rnik[2];
+        const double dcik = 1.0 - dot_ij_ik * dot_ij_ik;
+        double fk[3] = {0, 0, 0};
+        for (int lb = jb; lb < jb + jnb; ++lb) {
+            if (lb == sji) continue;
+            const int2 e = bij[lb];
+            if (e.x == ek.x) {
+                int al, bl, cl;
+                dc_unpack(e.y, al, bl, cl);
+                if (a0 + al == ak && b0 + bl == bk && c0 + cl == ck) continue;
+            }
+            const RbBond L = bond[lb];
+            const double *rnjl = L.n;
+            const double rljl = L.len;
+            double dot_ij_jl = 0.0, dot_ik_jl = 0.0;
+            dot_ij_jl = dot_ij_jl + rnij[0] * rnjl[0];
+            dot_ij_jl = dot_ij_jl + rnij[1] * rnjl[1];
+            dot_ij_jl = dot_ij_jl + rnij[2] * rnjl[2];
+            dot_ik_jl = dot_ik_jl + rnik[0] * rnjl[0];
+            dot_ik_jl = dot_ik_jl + rnik[1] * rnjl[1];
+            dot_ik_jl = dot_ik_jl + rnik[2] * rnjl[2];
+            const double dcjl = 1.0 - dot_ij_jl * dot_ij_jl;
+            const double abs_dc = sqrt(dcik * dcjl);
+            const double cost = (dot_ij_ik * dot_ij_jl - dot_ik_jl) / abs_dc;
+            double bdhij = 1 - cost * cost;
+            bdh = bdh + bdhij * K.fc * L.fc;
+            bdhij = bdhij * tij * fa * fcar / 2;
+            const double dbdhij = -2 * cost * tije * K.fc * L.fc / 2;
+            double df[3], v[3], fl[3];
+#pragma unroll
+            for (int x = 0; x < 3; ++x)
+                df[x] = dbdhij *
+                        ((dot_ij_jl / abs_dc + cost * dot_ij_ik / dcik) * rnik[x] +
+                         (dot_ij_ik / abs_dc + cost * dot_ij_jl / dcjl) * rnjl[x] -
+                         (2 * dot_ik_jl / abs_dc + cost * (1.0 / dcik + 1.0 / dcjl)) * rnij[x]) /
+                        rlij;
+#pragma unroll
+            for (int x = 0; x < 3; ++x) {
+                fi[x] += df[x];
+                fj[x] -= df[x];
+                v[x] = rlij * rnij[x];
+            }
+            outer_add(w, v, df);
+#pragma unroll
+            for (int x = 0; x < 3; ++x)
+                df[x] = dbdhij *
+                            (-1.0 / dcik * cost * rnik[x] - 1.0 / abs_dc * rnjl[x] +
+                             (dot_ij_jl / abs_dc + cost * dot_ij_ik / dcik) * rnij[x]) /
+                            rlik +
+                        bdhij * K.dfc * L.fc * rnik[x];
+#pragma unroll
+            for (int x = 0; x < 3; ++x) {
+                fi[x] += df[x];
+                fk[x] -= df[x];
+                v[x] = rlik * rnik[x];
+            }
+            outer_add(w, v, df);
+#pragma unroll
+            for (int x = 0; x < 3; ++x)
+                df[x] = dbdhij *
+                            (-1.0 / dcjl * cost * rnjl[x] - 1.0 / abs_dc * rnik[x] +
+                             (dot_ij_ik / abs_dc + cost * dot_ij_jl / dcjl) * rnij[x]) /
+                            rljl +
+                        bdhij * K.fc * L.dfc * rnjl[x];
+#pragma unroll
+            for (int x = 0; x < 3; ++x) {
+                fj[x] += df[x];
+                fl[x] = -df[x];
+                v[x] = rljl * rnjl[x];
+            }
+            outer_add(w, v, df);
+            fx_add3(facc, e.x, fl, bad);
+        }
+        fx_add3(facc, ek.x, fk, bad);
+    }
+    return bdh;
+}
+
 __global__ void __launch_bounds__(256)
 k_rb_pair(int ntot, const RbDev *__restrict__ Dp, const double *__restrict__ tab,
           const int *__restrict__ el, const int *__restrict__ offsets, const int *__restrict__ bcnt,
@@ -853,6 +973,14 @@ k_rb_pair(int ntot, const RbDev *__restrict__ Dp, const double *__restrict__ tab
             nconj = nconj > 8.0 ? 8.0 : nconj;
             nti = nti > 3.0 ? 3.0 : nti;
             ntj = ntj > 3.0 ? 3.0 : ntj;
+            double fi[3] = {0, 0, 0}, fj[3] = {0, 0, 0};
+            double bdh = 0.0, tij = 0.0, dtdni = 0.0, dtdnj = 0.0, dtdncn = 0.0;
+            if (D.with_dihedral && ijpot == RB_CC) {
+                rb_table3(tab + RB_TAB_TCC, nti, ntj, nconj, tij, dtdni, dtdnj, dtdncn);
+                if (tij != 0)
+                    bdh = rb_torsion(ib, inb, s, jb, jnb, sji, eij.y, bond, bij, rnij, rlij, tij, fa, fcar,
+                                     fi, fj, wij, facc, bad);
+            }
             double fij = 0.0, dfdni = 0.0, dfdnj = 0.0, dfdncn = 0.0;
             if (ijpot == RB_CC)
                 rb_table3(tab + RB_TAB_FCC, nti, ntj, nconj, fij, dfdni, dfdnj, dfdncn);
@@ -862,12 +990,14 @@ k_rb_pair(int ntot, const RbDev *__restrict__ Dp, const double *__restrict__ tab
                 rb_table3(tab + RB_TAB_FCH, ntj, nti, nconj, fij, dfdnj, dfdni, dfdncn);
             else
                 rb_table3(tab + RB_TAB_FCH, nti, ntj, nconj, fij, dfdni, dfdnj, dfdncn);
+            dfdni = dfdni + dtdni * bdh;
+            dfdnj = dfdnj + dtdnj * bdh;
+            dfdncn = dfdncn + dtdncn * bdh;
             dfdni = 0.5 * fcar * fa * dfdni;
             dfdnj = 0.5 * fcar * fa * dfdnj;
             dfdncn = 0.5 * fcar * fa * dfdncn;
             const double dfdncni = 2 * dfdncn * nconji;
             const double dfdncnj = 2 * dfdncn * nconjj;
-            double fi[3] = {0, 0, 0}, fj[3] = {0, 0, 0};
             // coordination and conjugation forces
             rb_nforces(i, ib, inb, s, bond, bij, offsets, bcnt, el, dfdni, dfdncni, dfbi, dpdnci,
                        dpdnhi, fi, wij, facc, bad);
@@ -880,7 +1010,7 @@ k_rb_pair(int ntot, const RbDev *__restrict__ Dp, const double *__restrict__ tab
             rb_zeta<true>(D, kj, ijpot, rlij, rlijr, rnij, -1.0, jb, jnb, sji, bond, bij, el, ntj_g, dfbj, Sj,
                           facc, bad);
             // pair energy and forces
-            const double bave = 0.5 * (bi + bj + fij);
+            const double bave = 0.5 * (bi + bj + fij + tij * bdh);
             const double hlf = fcar * (fr + bave * fa) / 2;
             epair = hlf + hlf;
             const double dffac = dfr * fcar + bave * dfa * fcar + fr * dfcar + bave * fa * dfcar;
@@ -1077,12 +1207,11 @@ using namespace agpu;
 extern "C" int agpu_rebo2_create(agpu_potential **pot, int flags)
 {
     if (!pot) return set_error(AGPU_ERR_ARG, "pot is NULL");
-    if (flags & AGPU_REBO2_DIHEDRAL)
-        return set_error(AGPU_ERR_UNSUPPORTED, "REBO2: the dihedral term is not implemented");
     if (flags & ~(AGPU_REBO2_ZERO_TABLES | AGPU_REBO2_DIHEDRAL))
         return set_error(AGPU_ERR_ARG, "REBO2: unknown flag");
     Rebo2State *st = new Rebo2State();
     st->zero_tables = (flags & AGPU_REBO2_ZERO_TABLES) != 0;
+    st->with_dihedral = (flags & AGPU_REBO2_DIHEDRAL) != 0;
     int rc = rebo2_setup(st);
     if (rc != AGPU_OK) {
         delete st;

@@ -41,7 +41,7 @@ module atomistica_gpu
   end type tersoff_params_t
 
   public :: particles_init, particles_set_cell, particles_set_Z, particles_set_positions
-  public :: neighbors_init, tersoff_init, eam_alloy_init, ewald_init
+  public :: neighbors_init, tersoff_init, eam_alloy_init, ewald_init, rebo2_init
   public :: set_precision, bind_to, energy_and_forces, coulomb_energy_and_forces
   public :: last_error, del
 
@@ -111,6 +111,11 @@ module atomistica_gpu
        import :: c_int, c_ptr, c_double
        type(c_ptr) :: pot
        real(c_double), value :: cutoff, beta, kmax
+     end function
+     integer(c_int) function agpu_rebo2_create(pot, flags) bind(C, name="agpu_rebo2_create")
+       import :: c_int, c_ptr
+       type(c_ptr) :: pot
+       integer(c_int), value :: flags
      end function
      integer(c_int) function agpu_potential_destroy(pot) bind(C, name="agpu_potential_destroy")
        import :: c_int, c_ptr
@@ -222,6 +227,23 @@ contains
     integer, intent(out) :: ierror
     ierror = agpu_ewald_create(this%h, cutoff, beta, kmax)
   end subroutine ewald_init
+
+  !> REBO2 (rebo2_init, rebo2_module.f90:30-42): dihedral = with_dihedral,
+  !> zero_tables as the reference's switch of the same name
+  subroutine rebo2_init(this, ierror, dihedral, zero_tables)
+    type(potential_gpu_t), intent(inout) :: this
+    integer, intent(out) :: ierror
+    logical, intent(in), optional :: dihedral, zero_tables
+    integer(c_int) :: flags
+    flags = 0
+    if (present(zero_tables)) then
+       if (zero_tables) flags = ior(flags, 1_c_int)
+    end if
+    if (present(dihedral)) then
+       if (dihedral) flags = ior(flags, 2_c_int)
+    end if
+    ierror = agpu_rebo2_create(this%h, flags)
+  end subroutine rebo2_init
 
   subroutine set_precision(this, prec, ierror)
     type(potential_gpu_t), intent(inout) :: this

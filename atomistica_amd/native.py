@@ -312,8 +312,8 @@ class TabulatedAlloyEAM(_Potential):
 class Rebo2(_Potential):
     """Second-generation REBO for C and H (rebo2.f90, classname Rebo2) with
     the default tables of rebo2_default_tables.f90.  Keyword arguments follow
-    rebo2_registry.f90: zero_tables (bool) and dihedral (only False is
-    implemented)."""
+    rebo2_registry.f90: zero_tables (bool) and dihedral (bool, the torsion
+    term; default False as with_dihedral in rebo2_type.f90)."""
     kind = _lib.KIND_REBO2
 
     def __init__(self, zero_tables=False, dihedral=False, elements="C,H"):

@@ -56,7 +56,8 @@ def build_workload(name):
         pos, cell, pbc, Z = W.amorphous_ch(100000, x_h=0.2, density=2.0, seed=10)
         desc = ("REBO2 amorphous hydrocarbon 100000 atoms (80% C, 20% H), cubic periodic box at "
                 "2.0 g/cm^3, uniform random with rejection at C-C 1.2, C-H 1.0, H-H 0.8 A (rng 10)")
-        return dict(pos=pos, cell=cell, pbc=pbc, Z=Z, kind="rebo2", avgn=24, desc=desc)
+        return dict(pos=pos, cell=cell, pbc=pbc, Z=Z, kind="rebo2", dihedral=True, avgn=24,
+                    desc=desc + "; P bicubic, F and T tricubic (torsion term on)")
     raise SystemExit("unknown workload " + name)
 
 

@@ -64,7 +64,7 @@ extern "C" {
 
 /* agpu_rebo2_create flags */
 #define AGPU_REBO2_ZERO_TABLES 1  /* rebo2 zero_tables = .true. (rebo2_module.f90:30-42) */
-#define AGPU_REBO2_DIHEDRAL 2     /* with_dihedral = .true.: AGPU_ERR_UNSUPPORTED */
+#define AGPU_REBO2_DIHEDRAL 2     /* with_dihedral = .true. (rebo2_type.f90, default .false.) */
 
 typedef struct agpu_particles agpu_particles;
 typedef struct agpu_neighbors agpu_neighbors;
@@ -188,8 +188,9 @@ int agpu_ewald_create(agpu_potential **pot, double cutoff, double beta, double k
 /* REBO2 (Brenner et al. 2002) for C and H without screening: rebo2_init and
  * rebo2_db_init_with_parameters (rebo2_module.f90:30-42, rebo2_db.f90:81-303)
  * with the default tables of rebo2_default_tables.f90; energy_and_forces is
- * rebo2_kernel (bop_kernel_rebo2.f90).  FP64 only; no mask (the reference
- * kernel takes none). */
+ * rebo2_kernel (bop_kernel_rebo2.f90), including the torsion term when
+ * AGPU_REBO2_DIHEDRAL is set.  FP64 only; no mask (the reference kernel
+ * takes none). */
 int agpu_rebo2_create(agpu_potential **pot, int flags);
 
 int agpu_potential_destroy(agpu_potential *pot);

@@ -28,24 +28,23 @@ def close(a, b, rel=REL):
     return float(np.abs(a - b).max()) <= rel * scale
 
 
-def run_gpu(pos, cell, pbc, Z, per_at=False, precision="fp64", mask=None):
+def run_gpu(pos, cell, pbc, Z, per_at=False, precision="fp64", mask=None, dihedral=False):
     p = N.from_atoms(pos, cell, pbc, Z)
     nl = N.Neighbors(100)
-    pot = N.Rebo2()
+    pot = N.Rebo2(dihedral=dihedral)
     pot.set_precision(precision)
     pot.bind_to(p, nl)
     return pot.energy_and_forces(p, nl, epot_per_at=per_at, wpot_per_at=per_at, mask=mask)
 
 
-_R = None
+_R = {}
 
 
-def run_oracle(pos, cell, pbc, Z, per_at=False):
-    global _R
-    if _R is None:
-        _R = O.Rebo2()
+def run_oracle(pos, cell, pbc, Z, per_at=False, dihedral=False):
+    if dihedral not in _R:
+        _R[dihedral] = O.Rebo2(dihedral=dihedral)
     nl = O.nl_build(pos, cell, pbc, 2.0)
-    return _R(pos, Z, cell, nl, per_at=per_at)
+    return _R[dihedral](pos, Z, cell, nl, per_at=per_at)
 
 
 def _molecule(name):
@@ -66,12 +65,13 @@ CASES = [
 ]
 
 
+@pytest.mark.parametrize("dihedral", [False, True], ids=["plain", "torsion"])
 @pytest.mark.parametrize("name,make", CASES, ids=[c[0] for c in CASES])
-def test_rebo2_fp64_parity(name, make):
+def test_rebo2_fp64_parity(name, make, dihedral):
     pos, cell, pbc, Z = make()
-    print("rebo2 case", name, len(pos), flush=True)
-    e, f, w, epa, _, _, wpa, _ = run_gpu(pos, cell, pbc, Z, per_at=True)
-    re_, rf, rw, repa, rwpa = run_oracle(pos, cell, pbc, Z, per_at=True)
+    print("rebo2 case", name, len(pos), "dihedral", dihedral, flush=True)
+    e, f, w, epa, _, _, wpa, _ = run_gpu(pos, cell, pbc, Z, per_at=True, dihedral=dihedral)
+    re_, rf, rw, repa, rwpa = run_oracle(pos, cell, pbc, Z, per_at=True, dihedral=dihedral)
     assert abs(e - re_) <= REL * max(1.0, abs(re_))
     assert close(f, rf) and close(w, rw)
     assert close(epa, repa) and close(wpa, rwpa)
@@ -88,8 +88,8 @@ def test_rebo2_ignores_other_elements():
 
 def test_rebo2_bitwise_reproducible():
     pos, cell, pbc, Z = W.random_ch(2000, 600, 2.4, seed=13, strain=0.1)
-    e1, f1, w1, *_ = run_gpu(pos, cell, pbc, Z)
-    e2, f2, w2, *_ = run_gpu(pos, cell, pbc, Z)
+    e1, f1, w1, *_ = run_gpu(pos, cell, pbc, Z, dihedral=True)
+    e2, f2, w2, *_ = run_gpu(pos, cell, pbc, Z, dihedral=True)
     assert e1 == e2 and np.array_equal(f1, f2) and np.array_equal(w1, w2)
 
 
@@ -99,8 +99,6 @@ def test_rebo2_unsupported_modes():
         run_gpu(pos, cell, pbc, Z, precision="mixed")
     with pytest.raises(AgpuError):
         run_gpu(pos, cell, pbc, Z, mask=np.ones(len(pos), np.int32))
-    with pytest.raises(AgpuError):
-        N.Rebo2(dihedral=True)
 
 
 def test_rebo2_full_size_diamond():
@@ -116,3 +114,11 @@ def test_rebo2_full_size_diamond():
     assert abs(e / len(pos) - re_ / len(sp)) <= 1e-10 * abs(re_ / len(sp))
     assert np.abs(f).max() < 1e-9
     assert close(w / len(pos), rw / len(sp), rel=1e-9)
+
+
+def test_rebo2_amorphous_hydrocarbon_sample():
+    # the bench system's generator at 1/20 of its size, with torsion
+    pos, cell, pbc, Z = W.amorphous_ch(5000, x_h=0.2, density=2.0, seed=10)
+    e, f, w, *_ = run_gpu(pos, cell, pbc, Z, dihedral=True)
+    re_, rf, rw = run_oracle(pos, cell, pbc, Z, dihedral=True)
+    assert abs(e - re_) <= REL * abs(re_) and close(f, rf) and close(w, rw)
