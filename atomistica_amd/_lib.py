@@ -105,6 +105,14 @@ PROTOTYPES = {
     "agpu_potential_bind_to": [vp, vp, vp],
     "agpu_potential_energy_and_forces": [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, i],
     "agpu_coulomb_potential": [vp, vp, vp, vp, vp, i],
+    "agpu_batch_create": [ctypes.POINTER(vp)],
+    "agpu_batch_destroy": [vp],
+    "agpu_batch_set_structures": [vp, i, vp, vp],
+    "agpu_batch_set_positions": [vp, vp, i],
+    "agpu_batch_get_size": [vp, ip, ip, lp],
+    "agpu_tersoff_batch_forward": [vp, vp, i, vp, vp, i],
+    "agpu_tersoff_batch_backward": [vp, vp, vp, vp, vp, i],
+    "agpu_random_positions": [i, vp, d, ctypes.c_uint64, i, vp],
 }
 _RESTYPE = {"agpu_version": ctypes.c_char_p}
 

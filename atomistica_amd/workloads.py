@@ -137,39 +137,71 @@ def random_si_structures(nstruct=4096, seed=4, nmin=8, nmax=512, rho0=2.33,
         E = rng.uniform(-strain, strain, (3, 3))
         cell = np.eye(3) + E
         cell *= (V / abs(np.linalg.det(cell))) ** (1.0 / 3.0)
-        out.append((_random_positions(n, cell, dmin, rng, max_tries), cell))
+        seed_s = int(rng.integers(0, 2 ** 63))
+        out.append((random_positions(n, cell, dmin, seed_s, max_tries), cell))
     return out
 
 
-def _random_positions(n, cell, dmin, rng, max_tries):
+def random_positions(n, cell, dmin, seed, max_tries=200):
+    """n random points in a periodic cell with minimum distance dmin over all
+    images (random sequential addition in the native library, splitmix64
+    stream seeded by ``seed``)."""
+    import ctypes
+    from ._lib import call
+    cell = np.ascontiguousarray(cell, dtype=np.float64)
+    pos = np.zeros((n, 3))
+    call("agpu_random_positions", int(n), cell.ctypes.data_as(ctypes.c_void_p), float(dmin),
+         ctypes.c_uint64(seed), int(max_tries), pos.ctypes.data_as(ctypes.c_void_p))
+    return pos
+
+
+def _random_positions(n, cell, dmin, rng, max_tries, block=32):
+    """Random sequential addition of n points in a periodic (triclinic) cell
+    with a minimum distance dmin over every periodic image.  Candidates are
+    proposed in blocks: each block is tested against the accepted points in
+    one vectorised step, then accepted in order against each other.  If
+    max_tries blocks in a row add nothing the cell is jammed and dmin is
+    relaxed by 3 %."""
     inv = np.linalg.inv(cell)
-    frac = np.zeros((0, 3))
     # lattice translations covering dmin around the cell
     h = dmin * np.linalg.norm(inv, axis=0)
     rngs = [np.arange(-int(math.ceil(x)) - 1, int(math.ceil(x)) + 2) for x in h]
     shifts = np.stack(np.meshgrid(*rngs, indexing="ij"), -1).reshape(-1, 3)
-    dmin2 = dmin * dmin
+    vs = shifts @ cell
+    vs2 = np.einsum("kx,kx->k", vs, vs)
+    frac = np.zeros((n, 3))
+    m = 0
     scale = 1.0
-    while len(frac) < n:
-        ok = False
-        for _ in range(max_tries):
-            s = rng.random(3)
-            d = (s[None, :] - frac)           # (m, 3)
-            d = d - np.round(d)
-            if len(frac):
-                v = ((d[:, None, :] + shifts[None]) @ cell)
-                if np.min(np.einsum("mkx,mkx->mk", v, v)) < dmin2 * scale:
+    stall = 0
+    while m < n:
+        d2min = dmin * dmin * scale
+        if np.any((vs2 > 0) & (vs2 < d2min)):
+            scale *= 0.97             # the cell itself is shorter than dmin
+            continue
+        cand = rng.random((block, 3))
+        ok = np.ones(block, bool)
+        if m:
+            d = cand[:, None, :] - frac[None, :m, :]
+            d -= np.round(d)
+            v = (d[:, :, None, :] + shifts[None, None]) @ cell          # (B, m, S, 3)
+            ok &= np.einsum("bmsx,bmsx->bms", v, v).reshape(block, -1).min(1) >= d2min
+        added = 0
+        for c in np.nonzero(ok)[0]:
+            if m >= n:
+                break
+            if added:
+                d = cand[c] - frac[m - added:m]
+                d -= np.round(d)
+                v = (d[:, None, :] + shifts[None]) @ cell
+                if np.einsum("msx,msx->ms", v, v).min() < d2min:
                     continue
-            # self images
-            vs = shifts @ cell
-            vs2 = np.einsum("kx,kx->k", vs, vs)
-            if np.any((vs2 > 0) & (vs2 < dmin2 * scale)):
-                continue
-            frac = np.vstack([frac, s])
-            ok = True
-            break
-        if not ok:
-            scale *= 0.97      # relax the minimum distance if the cell is jammed
+            frac[m] = cand[c]
+            m += 1
+            added += 1
+        stall = 0 if added else stall + 1
+        if stall >= max_tries:
+            scale *= 0.97             # relax the minimum distance if the cell is jammed
+            stall = 0
     return frac @ cell
 
 

@@ -217,6 +217,44 @@ int agpu_potential_energy_and_forces(agpu_potential *pot, agpu_particles *p,
 int agpu_coulomb_potential(agpu_potential *pot, agpu_particles *p, agpu_neighbors *nl,
                            const double *q, double *phi, int flags);
 
+/* ======================================================================
+ * batched Tersoff for parameter fitting (no counterpart in the reference,
+ * which evaluates one structure per call): many small fully periodic
+ * structures of one element evaluated together, with the gradient of
+ * energies and forces with respect to the 11 parameters
+ * theta = (A, B, lambda, mu, beta, n, c, d, h, R, S) of the single-element
+ * Tersoff form of bop_kernel.f90 / tersoff_func.f90 (xi = omega = 1, no
+ * h-term).  The batch builds its own neighbour list (every periodic image
+ * within S, all structures in one pass).
+ * ====================================================================== */
+typedef struct agpu_batch agpu_batch;
+int agpu_batch_create(agpu_batch **b);
+int agpu_batch_destroy(agpu_batch *b);
+/* atom_offsets[nstruct+1] (host, atom_offsets[0] = 0), cells[9*nstruct]
+ * (host, cell[9*s + 3*k + x] = component x of lattice vector k) */
+int agpu_batch_set_structures(agpu_batch *b, int nstruct, const int *atom_offsets,
+                              const double *cells);
+/* r[3*ntot] concatenated positions (device, or host with AGPU_HOST) */
+int agpu_batch_set_positions(agpu_batch *b, const double *r, int flags);
+int agpu_batch_get_size(agpu_batch *b, int *nstruct, int *ntot, int64_t *npairs);
+/* e_struct[nstruct] is set, f[3*ntot] accumulated; theta[11] on the host;
+ * precision AGPU_PREC_FP64 or AGPU_PREC_MIXED (FP32 pair/triplet
+ * arithmetic, FP64 displacements and accumulation) */
+int agpu_tersoff_batch_forward(agpu_batch *b, const double *theta, int precision,
+                               double *e_struct, double *f, int flags);
+/* grad_theta[11] = sum_s grad_e[s] dE_s/dtheta + sum_i grad_f[i] . dF_i/dtheta
+ * (set, FP64); grad_e or grad_f may be NULL (zero) */
+int agpu_tersoff_batch_backward(agpu_batch *b, const double *theta, const double *grad_e,
+                                const double *grad_f, double *grad_theta, int flags);
+
+/* Synthetic-workload utility (no counterpart in the reference): n random
+ * points in the periodic cell (cell[3k+x] = component x of lattice vector k)
+ * by random sequential addition with minimum distance dmin over every
+ * periodic image; dmin is relaxed by 3 % after max_tries rejections in a row.
+ * Deterministic for a given seed (splitmix64).  pos[3*n], host memory. */
+int agpu_random_positions(int n, const double *cell, double dmin, uint64_t seed, int max_tries,
+                          double *pos);
+
 #ifdef __cplusplus
 }
 #endif
