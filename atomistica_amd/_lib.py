@@ -8,7 +8,7 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIBPATH = os.path.join(HERE, "libatomistica_amd.so")
+LIBPATH = os.environ.get("ATOMISTICA_AMD_LIB") or os.path.join(HERE, "libatomistica_amd.so")
 
 OK = 0
 ERR_ARG = 1

@@ -39,6 +39,7 @@ enum DevErr : int {
     DERR_BOND_OVERFLOW = 8,
     DERR_STAGE_OVERFLOW = 16,
     DERR_FIXED_RANGE = 32,
+    DERR_SORT_FALLBACK = 64,
 };
 
 hipStream_t stream();
@@ -106,6 +107,8 @@ struct agpu_neighbors {
     agpu::DevBuf key, key_sorted, idx, idx_sorted, cell_start, atoms_sorted,
         counts, offsets, nbr, sort_tmp, rowmax, rank, stage;
     int stage_w = 0;              // staging stride (entries per atom)
+    bool radix = false;           // bin sort by radix sort (after a very full bin)
+    agpu::DevBuf cell_cnt;
     agpu::HostStats stats;
     int64_t n_updates = 0;
 };

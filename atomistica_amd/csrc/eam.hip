@@ -323,7 +323,10 @@ __device__ __forceinline__ void spl_fdf_x(const SplHdr &h, const R *__restrict__
 
 // Lanes per atom: rows of ~50 neighbours are split over G lanes that each
 // walk a strided share, so several independent gathers are in flight per lane.
-constexpr int EAM_G = 16;
+#ifndef AGPU_EAM_G
+#define AGPU_EAM_G 16
+#endif
+constexpr int EAM_G = AGPU_EAM_G;
 
 // Table header for (table kind, element) -- from the kernel arguments when the
 // potential has a single element (ONE), else from the device header array.
@@ -338,7 +341,10 @@ __device__ __forceinline__ SplHdr hdr_of(const EamDev &P, const SplHdr *__restri
 // Entries per lane and batch: every lane first issues the row loads of its
 // EAM_U entries, then their position gathers, then the table gathers, so one
 // latency is paid per batch instead of per neighbour.
-constexpr int EAM_U = 4;
+#ifndef AGPU_EAM_U
+#define AGPU_EAM_U 4
+#endif
+constexpr int EAM_U = AGPU_EAM_U;
 
 template <typename R, int G, bool ONE>
 __global__ void __launch_bounds__(256)
@@ -416,7 +422,9 @@ k_eam_density(EamDev P, const SplHdr *__restrict__ hdr, const R *__restrict__ re
     }
 }
 
-template <typename R, int G, bool ONE>
+// GEN = false: no mask and no per-atom virial requested (the common call),
+// so neither is loaded nor accumulated.
+template <typename R, int G, bool ONE, bool GEN>
 __global__ void __launch_bounds__(256)
 k_eam_force(EamDev P, const SplHdr *__restrict__ hdr, const R *__restrict__ rec, int nat,
             const int *__restrict__ order, const double *__restrict__ r, const int *__restrict__ el,
@@ -438,7 +446,7 @@ k_eam_force(EamDev P, const SplHdr *__restrict__ hdr, const R *__restrict__ rec,
         i = order[ai];
         const int eli = el[i];
         if (eli >= 0) {
-            const double mi = (mask && mask[i] == 0) ? 0.0 : 1.0;
+            const double mi = (GEN && mask && mask[i] == 0) ? 0.0 : 1.0;
             const R dFi = (R)dF[i];
             const double ri[3] = {r[3 * i], r[3 * i + 1], r[3 * i + 2]};
             const int beg = offsets[i], end = offsets[i + 1];
@@ -460,7 +468,7 @@ k_eam_force(EamDev P, const SplHdr *__restrict__ hdr, const R *__restrict__ rec,
                     rj[u][1] = r[3 * j + 1];
                     rj[u][2] = r[3 * j + 2];
                     dFj[u] = dF[j];
-                    mj[u] = (mask && mask[j] == 0) ? 0.0 : 1.0;
+                    mj[u] = (GEN && mask && mask[j] == 0) ? 0.0 : 1.0;
                 }
 #pragma unroll
                 for (int u = 0; u < EAM_U; ++u) {
@@ -478,7 +486,8 @@ k_eam_force(EamDev P, const SplHdr *__restrict__ hdr, const R *__restrict__ rec,
                     const R rr = (R)rrd;
                     const R pp = (dphi - phi * rr) * rr;
                     const R drho_j = spl_df<R>(hdr_of<ONE>(P, hdr, 1, elj[u], 0), rec, ad, err);
-                    const R drho_i = spl_df<R>(hri, rec, ad, err);
+                    // one element: rho'_i(r) is the same table at the same r
+                    const R drho_i = ONE ? drho_j : spl_df<R>(hri, rec, ad, err);
                     const double ci = mi * (double)(dFi * drho_j + pp);
                     const double cj = mj[u] * (double)((R)dFj[u] * drho_i + pp);
                     e += mi * (double)(phi * rr);
@@ -491,7 +500,7 @@ k_eam_force(EamDev P, const SplHdr *__restrict__ hdr, const R *__restrict__ rec,
 #pragma unroll
                     for (int q = 0; q < 6; ++q) {
                         w[q] += wo * dd[q];
-                        wp[q] += wq * dd[q];
+                        if (GEN) wp[q] += wq * dd[q];
                     }
                 }
             }
@@ -502,14 +511,14 @@ k_eam_force(EamDev P, const SplHdr *__restrict__ hdr, const R *__restrict__ rec,
     e = group_sum<G>(e);
 #pragma unroll
     for (int q = 0; q < 6; ++q) w[q] = group_sum<G>(w[q]);
-    if (wpa)
+    if (GEN && wpa)
         for (int q = 0; q < 6; ++q) wp[q] = group_sum<G>(wp[q]);
     if (i >= 0 && lane == 0) {
         f[3 * (size_t)i] += acc[0];
         f[3 * (size_t)i + 1] += acc[1];
         f[3 * (size_t)i + 2] += acc[2];
         if (epa) epa[i] += e;
-        if (wpa) {
+        if (GEN && wpa) {
             double *o = wpa + 9 * (size_t)i;
             // column-major 3x3: [a + 3b]
             o[0] += wp[0];
@@ -598,10 +607,16 @@ static int eam_launch(EamState *st, agpu_potential *pot, const agpu_eval_args *a
                 dim3(256), 0, s, st->dev, st->d_hdr.as<SplHdr>(), rec, p->nat, order, p->d_r, el, C,
                 nl->offsets.as<int>(), nl->nbr.as<int2>(), a->mask, st->dF.as<double>(), a->epot_per_at, part,
                 flags);
-    AGPU_LAUNCH(mixed ? "eam_force_mixed" : "eam_force_fp64", (k_eam_force<R, EAM_G, ONE>), dim3(nblk), dim3(256),
-                0, s, st->dev, st->d_hdr.as<SplHdr>(), rec, p->nat, order, p->d_r, el, C, nl->offsets.as<int>(),
-                nl->nbr.as<int2>(), a->mask, st->dF.as<double>(), a->f, a->epot_per_at, a->wpot_per_at,
-                part + nblk, flags);
+    if (a->mask || a->wpot_per_at)
+        AGPU_LAUNCH(mixed ? "eam_force_mixed" : "eam_force_fp64", (k_eam_force<R, EAM_G, ONE, true>), dim3(nblk),
+                    dim3(256), 0, s, st->dev, st->d_hdr.as<SplHdr>(), rec, p->nat, order, p->d_r, el, C,
+                    nl->offsets.as<int>(), nl->nbr.as<int2>(), a->mask, st->dF.as<double>(), a->f,
+                    a->epot_per_at, a->wpot_per_at, part + nblk, flags);
+    else
+        AGPU_LAUNCH(mixed ? "eam_force_mixed" : "eam_force_fp64", (k_eam_force<R, EAM_G, ONE, false>), dim3(nblk),
+                    dim3(256), 0, s, st->dev, st->d_hdr.as<SplHdr>(), rec, p->nat, order, p->d_r, el, C,
+                    nl->offsets.as<int>(), nl->nbr.as<int2>(), a->mask, st->dF.as<double>(), a->f,
+                    a->epot_per_at, a->wpot_per_at, part + nblk, flags);
     return AGPU_OK;
 }
 

@@ -97,6 +97,69 @@ __global__ void k_cell_start(int nat, int ncell, const int *__restrict__ key_sor
     for (int c = kprev + 1; c <= kcur; ++c) cell_start[c] = p;
 }
 
+// Counting sort of the atoms by bin: counts, exclusive scan, scatter with
+// atomic slots, then each bin's atoms are put back into ascending index
+// order, which is the order a stable sort gives.  The reordering takes one
+// wavefront per bin: every element's rank is the number of smaller indices
+// in the bin (indices are distinct), counted through lane broadcasts.  A bin
+// with more than NL_SORT_MAX atoms flags a fallback to the radix sort.
+constexpr int NL_SORT_MAX = 256;
+
+__global__ void k_zero_i(int *__restrict__ a, long long n)
+{
+    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long long)gridDim.x * blockDim.x)
+        a[q] = 0;
+}
+
+__global__ void k_cell_count(int nat, const int *__restrict__ key, int *__restrict__ cnt)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nat) atomicAdd(&cnt[key[i]], 1);
+}
+
+__global__ void k_cell_fill(int nat, int ncell, const int *__restrict__ key, int *__restrict__ cell_start,
+                            int *__restrict__ cnt, int *__restrict__ idx_sorted)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) cell_start[ncell] = nat;
+    if (i >= nat) return;
+    const int c = key[i];
+    const int slot = atomicSub(&cnt[c], 1) - 1;     // leaves cnt at zero
+    idx_sorted[cell_start[c] + slot] = i;
+}
+
+__global__ void __launch_bounds__(256)
+k_cell_order(int ncell, const int *__restrict__ cell_start, int *__restrict__ idx_sorted, int *__restrict__ flags)
+{
+    const int lane = threadIdx.x & 63;
+    const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (c >= ncell) return;                      // uniform per wavefront
+    const int b = cell_start[c], n = cell_start[c + 1] - b;
+    if (n <= 1) return;
+    if (n > NL_SORT_MAX) {
+        if (lane == 0) atomicOr(flags, DERR_SORT_FALLBACK);
+        return;
+    }
+    int v[4], r[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int k = lane + 64 * q;
+        v[q] = k < n ? idx_sorted[b + k] : 0x7fffffff;
+    }
+    for (int k = 0; k < n; ++k) {
+        const int src = k >> 6;
+        const int w = src == 0 ? v[0] : (src == 1 ? v[1] : (src == 2 ? v[2] : v[3]));
+        const int u = __shfl(w, k & 63, 64);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) r[q] += u < v[q];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int k = lane + 64 * q;
+        if (k < n) idx_sorted[b + r[q]] = v[q];
+    }
+}
+
 __global__ void k_gather(int nat, const double *__restrict__ r, BinGeom g,
                          const int *__restrict__ idx_sorted, SortedAtom *__restrict__ out,
                          int *__restrict__ rank)
@@ -499,18 +562,36 @@ int nl_update(agpu_neighbors *nl, agpu_particles *p, int force)
         int nb = (nat + TB - 1) / TB;
         AGPU_LAUNCH("nl_bin", k_bin, dim3(nb), dim3(TB), 0, s, nat, p->d_r, g,
                     nl->key.as<int>(), nl->idx.as<int>(), flags + 3);
-        size_t tmp = 0;
-        int end_bit = 1;
-        while ((1ll << end_bit) <= ncell) ++end_bit;
-        AGPU_HIP(rocprim::radix_sort_pairs(nullptr, tmp, nl->key.as<int>(), nl->key_sorted.as<int>(),
-                                           nl->idx.as<int>(), nl->idx_sorted.as<int>(),
-                                           (size_t)nat, 0, end_bit, s));
-        AGPU_CHECK(nl->sort_tmp.reserve(tmp + 16));
-        AGPU_HIP(rocprim::radix_sort_pairs(nl->sort_tmp.ptr, tmp, nl->key.as<int>(),
-                                           nl->key_sorted.as<int>(), nl->idx.as<int>(),
-                                           nl->idx_sorted.as<int>(), (size_t)nat, 0, end_bit, s));
-        AGPU_LAUNCH("nl_cell_start", k_cell_start, dim3((nat + 1 + TB - 1) / TB), dim3(TB), 0, s,
-                    nat, (int)ncell, nl->key_sorted.as<int>(), nl->cell_start.as<int>());
+        if (!nl->radix) {
+            AGPU_CHECK(nl->cell_cnt.reserve(sizeof(int) * (ncell + 1)));
+            int *cnt = nl->cell_cnt.as<int>();
+            const int nzb = (int)std::min<long long>(1024, (ncell + TB - 1) / TB);
+            AGPU_LAUNCH("nl_sort_zero", k_zero_i, dim3(nzb), dim3(TB), 0, s, cnt, ncell);
+            AGPU_LAUNCH("nl_sort_count", k_cell_count, dim3(nb), dim3(TB), 0, s, nat, nl->key.as<int>(), cnt);
+            size_t tmp = 0;
+            AGPU_HIP(rocprim::exclusive_scan(nullptr, tmp, cnt, nl->cell_start.as<int>(), 0, (size_t)ncell,
+                                             rocprim::plus<int>(), s));
+            AGPU_CHECK(nl->sort_tmp.reserve(tmp + 16));
+            AGPU_HIP(rocprim::exclusive_scan(nl->sort_tmp.ptr, tmp, cnt, nl->cell_start.as<int>(), 0,
+                                             (size_t)ncell, rocprim::plus<int>(), s));
+            AGPU_LAUNCH("nl_sort_fill", k_cell_fill, dim3(nb), dim3(TB), 0, s, nat, (int)ncell, nl->key.as<int>(),
+                        nl->cell_start.as<int>(), cnt, nl->idx_sorted.as<int>());
+            AGPU_LAUNCH("nl_sort_order", k_cell_order, dim3((unsigned)((ncell + 3) / 4)), dim3(TB), 0, s,
+                        (int)ncell, nl->cell_start.as<int>(), nl->idx_sorted.as<int>(), flags + 3);
+        } else {
+            size_t tmp = 0;
+            int end_bit = 1;
+            while ((1ll << end_bit) <= ncell) ++end_bit;
+            AGPU_HIP(rocprim::radix_sort_pairs(nullptr, tmp, nl->key.as<int>(), nl->key_sorted.as<int>(),
+                                               nl->idx.as<int>(), nl->idx_sorted.as<int>(),
+                                               (size_t)nat, 0, end_bit, s));
+            AGPU_CHECK(nl->sort_tmp.reserve(tmp + 16));
+            AGPU_HIP(rocprim::radix_sort_pairs(nl->sort_tmp.ptr, tmp, nl->key.as<int>(),
+                                               nl->key_sorted.as<int>(), nl->idx.as<int>(),
+                                               nl->idx_sorted.as<int>(), (size_t)nat, 0, end_bit, s));
+            AGPU_LAUNCH("nl_cell_start", k_cell_start, dim3((nat + 1 + TB - 1) / TB), dim3(TB), 0, s,
+                        nat, (int)ncell, nl->key_sorted.as<int>(), nl->cell_start.as<int>());
+        }
         AGPU_LAUNCH("nl_gather", k_gather, dim3(nb), dim3(TB), 0, s, nat, p->d_r, g,
                     nl->idx_sorted.as<int>(), nl->atoms_sorted.as<SortedAtom>(), nl->rank.as<int>());
         double cutoff_sq = nl->cutoff * nl->cutoff;
@@ -560,6 +641,13 @@ int nl_update(agpu_neighbors *nl, agpu_particles *p, int force)
     AGPU_HIP(hipStreamSynchronize(s));
     int fl = nl->stats.h[3];
     nl->maxrow = nl->stats.h[0];
+    if (fl & DERR_SORT_FALLBACK) {
+        // a bin holds more atoms than the per-bin ordering handles: redo the
+        // binning with the radix sort from now on
+        nl->radix = true;
+        nl->p = nullptr;
+        return nl_update(nl, p, force > 0 ? force : 1);
+    }
     if (fl & DERR_STAGE_OVERFLOW) {
         // a row outgrew the staging stride: widen it and rebuild once
         nl->stage_w = ((int)(1.25 * nl->maxrow) + 16 + 7) & ~7;

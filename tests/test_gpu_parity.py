@@ -77,6 +77,22 @@ def test_neighbor_list_bit_identical(name, make):
     assert np.array_equal(dc, rdc)
 
 
+def test_neighbor_list_crowded_bin():
+    # 1500 atoms inside one bin: the per-bin ordering of the counting sort
+    # hands over to the radix sort, and the list stays bit-identical
+    rng = np.random.default_rng(21)
+    pos = 40.0 + rng.random((1500, 3)) * 2.0
+    cell = np.diag([100.0, 100.0, 100.0])
+    pbc = np.array([True, False, True])
+    Z = np.full(len(pos), 14, np.int32)
+    p = particles(pos, cell, pbc, Z)
+    nl = N.Neighbors(1600)
+    nl.request_interaction_range(4.0)
+    off, j, dc = nl.get_list(p)
+    ro, rj, rdc = O.nl_build(pos, cell, pbc, 4.0, avgn=1600)
+    assert np.array_equal(off, ro) and np.array_equal(j, rj) and np.array_equal(dc, rdc)
+
+
 def test_neighbor_list_overflow_error():
     pos, cell, pbc, Z = W.diamond_si(2, sigma=0.0)
     p = particles(pos, cell, pbc, Z)
