@@ -118,7 +118,8 @@ struct EamState {
     EamDev dev{};
     int z2db[128];
     uint64_t Z_rev = ~0ull;
-    DevBuf d_hdr, d_rec64, d_rec32, dF, partial, stats;
+    DevBuf d_hdr, d_rec64, d_rec32, dF, partial, stats, pk;
+    bool all_known = false;       // every atom is of a tabulated element
 };
 
 static void eam_free(void *p) { delete static_cast<EamState *>(p); }
@@ -346,12 +347,24 @@ __device__ __forceinline__ SplHdr hdr_of(const EamDev &P, const SplHdr *__restri
 #endif
 constexpr int EAM_U = AGPU_EAM_U;
 
-template <typename R, int G, bool ONE>
+// Packed per-atom records of 32 bytes, so that one neighbour gather reads
+// one 32-byte sector: q1 = {x, y, z, element index} for the density pass,
+// q2 = {x, y, z, F'(rho)} written by the density pass for the force pass.
+__global__ void k_eam_pack(int nat, const double *__restrict__ r, const int *__restrict__ el,
+                           double4 *__restrict__ q1)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nat) return;
+    q1[i] = make_double4(r[3 * i], r[3 * i + 1], r[3 * i + 2], (double)el[i]);
+}
+
+// KN: one element and every atom tabulated, so no element is read per pair.
+template <typename R, int G, bool ONE, bool KN>
 __global__ void __launch_bounds__(256)
 k_eam_density(EamDev P, const SplHdr *__restrict__ hdr, const R *__restrict__ rec, int nat,
-              const int *__restrict__ order, const double *__restrict__ r, const int *__restrict__ el,
-              CellArg C, const int *__restrict__ offsets, const int2 *__restrict__ nbr,
-              const int *__restrict__ mask, double *__restrict__ dF, double *__restrict__ epa,
+              const int *__restrict__ order, const double4 *__restrict__ q1, CellArg C,
+              const int *__restrict__ offsets, const int2 *__restrict__ nbr, const int *__restrict__ mask,
+              double *__restrict__ dF, double4 *__restrict__ q2, double *__restrict__ epa,
               double *__restrict__ partial, int *__restrict__ flags)
 {
     constexpr int APB = 256 / G;
@@ -362,11 +375,12 @@ k_eam_density(EamDev P, const SplHdr *__restrict__ hdr, const R *__restrict__ re
     int err = 0;
     if (ai < nat) {
         const int i = order[ai];
-        const int eli = el[i];
+        const double4 qi = q1[i];
+        const int eli = KN ? 0 : (int)qi.w;
         const bool on = eli >= 0 && !(mask && mask[i] == 0);
         double rho = 0.0;
         if (on) {
-            const double ri[3] = {r[3 * i], r[3 * i + 1], r[3 * i + 2]};
+            const double ri[3] = {qi.x, qi.y, qi.z};
             const int beg = offsets[i], end = offsets[i + 1];
             for (int s0 = beg + lane; s0 < end; s0 += EAM_U * G) {
                 int2 e[EAM_U];
@@ -375,26 +389,21 @@ k_eam_density(EamDev P, const SplHdr *__restrict__ hdr, const R *__restrict__ re
                     const int s = s0 + u * G;
                     e[u] = s < end ? nbr[s] : make_int2(-1, 0);
                 }
-                double rj[EAM_U][3];
-                int elj[EAM_U];
+                double4 qj[EAM_U];
+#pragma unroll
+                for (int u = 0; u < EAM_U; ++u) qj[u] = q1[e[u].x < 0 ? i : e[u].x];
 #pragma unroll
                 for (int u = 0; u < EAM_U; ++u) {
-                    const int j = e[u].x < 0 ? i : e[u].x;
-                    elj[u] = e[u].x < 0 ? -1 : el[j];
-                    rj[u][0] = r[3 * j];
-                    rj[u][1] = r[3 * j + 1];
-                    rj[u][2] = r[3 * j + 2];
-                }
-#pragma unroll
-                for (int u = 0; u < EAM_U; ++u) {
-                    if (elj[u] < 0) continue;
+                    const int elj = e[u].x < 0 ? -1 : (KN ? 0 : (int)qj[u].w);
+                    if (elj < 0) continue;
                     int a, b, c;
                     dc_unpack(e[u].y, a, b, c);
+                    const double rj[3] = {qj[u].x, qj[u].y, qj[u].z};
                     double d[3];
-                    delta_r(C, ri, rj[u], a, b, c, d);
+                    delta_r(C, ri, rj, a, b, c, d);
                     const double d2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
                     if (d2 < P.cutoff_sq)
-                        rho += (double)spl_f<R>(hdr_of<ONE>(P, hdr, 1, elj[u], 0), rec, sqrt(d2), err);
+                        rho += (double)spl_f<R>(hdr_of<ONE>(P, hdr, 1, elj, 0), rec, sqrt(d2), err);
                 }
             }
         }
@@ -409,6 +418,7 @@ k_eam_density(EamDev P, const SplHdr *__restrict__ hdr, const R *__restrict__ re
         }
         if (lane == 0) {
             dF[i] = dFi;
+            q2[i] = make_double4(qi.x, qi.y, qi.z, dFi);
             if (epa && on) epa[i] += Fi;
         }
     }
@@ -424,14 +434,13 @@ k_eam_density(EamDev P, const SplHdr *__restrict__ hdr, const R *__restrict__ re
 
 // GEN = false: no mask and no per-atom virial requested (the common call),
 // so neither is loaded nor accumulated.
-template <typename R, int G, bool ONE, bool GEN>
+template <typename R, int G, bool ONE, bool GEN, bool KN>
 __global__ void __launch_bounds__(256)
 k_eam_force(EamDev P, const SplHdr *__restrict__ hdr, const R *__restrict__ rec, int nat,
-            const int *__restrict__ order, const double *__restrict__ r, const int *__restrict__ el,
+            const int *__restrict__ order, const double4 *__restrict__ q1, const double4 *__restrict__ q2,
             CellArg C, const int *__restrict__ offsets, const int2 *__restrict__ nbr,
-            const int *__restrict__ mask, const double *__restrict__ dF, double *__restrict__ f,
-            double *__restrict__ epa, double *__restrict__ wpa, double *__restrict__ partial,
-            int *__restrict__ flags)
+            const int *__restrict__ mask, double *__restrict__ f, double *__restrict__ epa,
+            double *__restrict__ wpa, double *__restrict__ partial, int *__restrict__ flags)
 {
     constexpr int APB = 256 / G;
     __shared__ double s_red[APB][7];
@@ -444,11 +453,12 @@ k_eam_force(EamDev P, const SplHdr *__restrict__ hdr, const R *__restrict__ rec,
     int i = -1;
     if (ai < nat) {
         i = order[ai];
-        const int eli = el[i];
+        const int eli = KN ? 0 : (int)q1[i].w;
         if (eli >= 0) {
             const double mi = (GEN && mask && mask[i] == 0) ? 0.0 : 1.0;
-            const R dFi = (R)dF[i];
-            const double ri[3] = {r[3 * i], r[3 * i + 1], r[3 * i + 2]};
+            const double4 qi = q2[i];
+            const R dFi = (R)qi.w;
+            const double ri[3] = {qi.x, qi.y, qi.z};
             const int beg = offsets[i], end = offsets[i + 1];
             const SplHdr hri = hdr_of<ONE>(P, hdr, 1, eli, 0);
             for (int s0 = beg + lane; s0 < end; s0 += EAM_U * G) {
@@ -458,16 +468,14 @@ k_eam_force(EamDev P, const SplHdr *__restrict__ hdr, const R *__restrict__ rec,
                     const int s = s0 + u * G;
                     en[u] = s < end ? nbr[s] : make_int2(-1, 0);
                 }
-                double rj[EAM_U][3], dFj[EAM_U], mj[EAM_U];
+                double4 qj[EAM_U];
                 int elj[EAM_U];
+                double mj[EAM_U];
 #pragma unroll
                 for (int u = 0; u < EAM_U; ++u) {
                     const int j = en[u].x < 0 ? i : en[u].x;
-                    elj[u] = en[u].x < 0 ? -1 : el[j];
-                    rj[u][0] = r[3 * j];
-                    rj[u][1] = r[3 * j + 1];
-                    rj[u][2] = r[3 * j + 2];
-                    dFj[u] = dF[j];
+                    qj[u] = q2[j];
+                    elj[u] = en[u].x < 0 ? -1 : (KN ? 0 : (int)q1[j].w);
                     mj[u] = (GEN && mask && mask[j] == 0) ? 0.0 : 1.0;
                 }
 #pragma unroll
@@ -475,8 +483,9 @@ k_eam_force(EamDev P, const SplHdr *__restrict__ hdr, const R *__restrict__ rec,
                     if (elj[u] < 0) continue;
                     int a, b, c;
                     dc_unpack(en[u].y, a, b, c);
+                    const double rj[3] = {qj[u].x, qj[u].y, qj[u].z};
                     double d[3];
-                    delta_r(C, ri, rj[u], a, b, c, d);
+                    delta_r(C, ri, rj, a, b, c, d);
                     const double d2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
                     if (!(d2 < P.cutoff_sq)) continue;
                     const double rrd = rsqrt(d2);
@@ -489,7 +498,7 @@ k_eam_force(EamDev P, const SplHdr *__restrict__ hdr, const R *__restrict__ rec,
                     // one element: rho'_i(r) is the same table at the same r
                     const R drho_i = ONE ? drho_j : spl_df<R>(hri, rec, ad, err);
                     const double ci = mi * (double)(dFi * drho_j + pp);
-                    const double cj = mj[u] * (double)((R)dFj[u] * drho_i + pp);
+                    const double cj = mj[u] * (double)((R)qj[u].w * drho_i + pp);
                     e += mi * (double)(phi * rr);
                     const double fs = -(ci + cj) * rrd;
                     acc[0] += fs * d[0];
@@ -586,11 +595,14 @@ int eam_bind(agpu_potential *pot)
     }
     AGPU_CHECK(agpu_neighbors_request_interaction_range(pot->nl, st->cutoff));
     AGPU_CHECK(map_elements(pot, st->z2db, 128));
+    st->all_known = true;
+    for (int z : p->h_Z)
+        if (z <= 0 || z >= 128 || st->z2db[z] < 0) st->all_known = false;
     st->Z_rev = p->Z_rev;
     return AGPU_OK;
 }
 
-template <typename R, bool ONE>
+template <typename R, bool ONE, bool KN>
 static int eam_launch(EamState *st, agpu_potential *pot, const agpu_eval_args *a, const R *rec, int nblk)
 {
     agpu_particles *p = pot->p;
@@ -603,21 +615,32 @@ static int eam_launch(EamState *st, agpu_potential *pot, const agpu_eval_args *a
     double *part = st->partial.as<double>();
     int *flags = st->stats.as<int>();
     const bool mixed = sizeof(R) == 4;
-    AGPU_LAUNCH(mixed ? "eam_density_mixed" : "eam_density_fp64", (k_eam_density<R, EAM_G, ONE>), dim3(nblk),
-                dim3(256), 0, s, st->dev, st->d_hdr.as<SplHdr>(), rec, p->nat, order, p->d_r, el, C,
-                nl->offsets.as<int>(), nl->nbr.as<int2>(), a->mask, st->dF.as<double>(), a->epot_per_at, part,
+    const int nat = p->nat;
+    double4 *q1 = st->pk.as<double4>(), *q2 = q1 + nat;
+    AGPU_LAUNCH("eam_pack", k_eam_pack, dim3((nat + 255) / 256), dim3(256), 0, s, nat, p->d_r, el, q1);
+    AGPU_LAUNCH(mixed ? "eam_density_mixed" : "eam_density_fp64", (k_eam_density<R, EAM_G, ONE, KN>), dim3(nblk),
+                dim3(256), 0, s, st->dev, st->d_hdr.as<SplHdr>(), rec, nat, order, q1, C,
+                nl->offsets.as<int>(), nl->nbr.as<int2>(), a->mask, st->dF.as<double>(), q2, a->epot_per_at, part,
                 flags);
     if (a->mask || a->wpot_per_at)
-        AGPU_LAUNCH(mixed ? "eam_force_mixed" : "eam_force_fp64", (k_eam_force<R, EAM_G, ONE, true>), dim3(nblk),
-                    dim3(256), 0, s, st->dev, st->d_hdr.as<SplHdr>(), rec, p->nat, order, p->d_r, el, C,
-                    nl->offsets.as<int>(), nl->nbr.as<int2>(), a->mask, st->dF.as<double>(), a->f,
-                    a->epot_per_at, a->wpot_per_at, part + nblk, flags);
+        AGPU_LAUNCH(mixed ? "eam_force_mixed" : "eam_force_fp64", (k_eam_force<R, EAM_G, ONE, true, KN>),
+                    dim3(nblk), dim3(256), 0, s, st->dev, st->d_hdr.as<SplHdr>(), rec, nat, order, q1, q2, C,
+                    nl->offsets.as<int>(), nl->nbr.as<int2>(), a->mask, a->f, a->epot_per_at, a->wpot_per_at,
+                    part + nblk, flags);
     else
-        AGPU_LAUNCH(mixed ? "eam_force_mixed" : "eam_force_fp64", (k_eam_force<R, EAM_G, ONE, false>), dim3(nblk),
-                    dim3(256), 0, s, st->dev, st->d_hdr.as<SplHdr>(), rec, p->nat, order, p->d_r, el, C,
-                    nl->offsets.as<int>(), nl->nbr.as<int2>(), a->mask, st->dF.as<double>(), a->f,
-                    a->epot_per_at, a->wpot_per_at, part + nblk, flags);
+        AGPU_LAUNCH(mixed ? "eam_force_mixed" : "eam_force_fp64", (k_eam_force<R, EAM_G, ONE, false, KN>),
+                    dim3(nblk), dim3(256), 0, s, st->dev, st->d_hdr.as<SplHdr>(), rec, nat, order, q1, q2, C,
+                    nl->offsets.as<int>(), nl->nbr.as<int2>(), a->mask, a->f, a->epot_per_at, a->wpot_per_at,
+                    part + nblk, flags);
     return AGPU_OK;
+}
+
+template <typename R>
+static int eam_dispatch(EamState *st, agpu_potential *pot, const agpu_eval_args *a, const R *rec, int nblk)
+{
+    if (st->nel != 1) return eam_launch<R, false, false>(st, pot, a, rec, nblk);
+    if (st->all_known) return eam_launch<R, true, true>(st, pot, a, rec, nblk);
+    return eam_launch<R, true, false>(st, pot, a, rec, nblk);
 }
 
 int eam_compute(agpu_potential *pot, const agpu_eval_args *a)
@@ -637,14 +660,11 @@ int eam_compute(agpu_potential *pot, const agpu_eval_args *a)
     AGPU_CHECK(st->stats.reserve(sizeof(int) * 4));
     AGPU_CHECK(pot->stats.init());
     AGPU_HIP(hipMemsetAsync(st->stats.ptr, 0, sizeof(int) * 4, s));
-    const bool one = st->nel == 1;
-    if (pot->precision == AGPU_PREC_MIXED) {
-        if (one) AGPU_CHECK((eam_launch<float, true>(st, pot, a, st->d_rec32.as<float>(), nblk)));
-        else AGPU_CHECK((eam_launch<float, false>(st, pot, a, st->d_rec32.as<float>(), nblk)));
-    } else {
-        if (one) AGPU_CHECK((eam_launch<double, true>(st, pot, a, st->d_rec64.as<double>(), nblk)));
-        else AGPU_CHECK((eam_launch<double, false>(st, pot, a, st->d_rec64.as<double>(), nblk)));
-    }
+    AGPU_CHECK(st->pk.reserve(sizeof(double4) * 2 * (size_t)(nat + 1)));
+    if (pot->precision == AGPU_PREC_MIXED)
+        AGPU_CHECK(eam_dispatch<float>(st, pot, a, st->d_rec32.as<float>(), nblk));
+    else
+        AGPU_CHECK(eam_dispatch<double>(st, pot, a, st->d_rec64.as<double>(), nblk));
     AGPU_CHECK(reduce_ev(st->partial.as<double>(), nblk, 1, a->epot, nullptr, nullptr));
     AGPU_CHECK(reduce_ev(st->partial.as<double>() + nblk, nblk, 10, a->epot, a->wpot, nullptr));
     AGPU_HIP(hipMemcpyAsync(pot->stats.h, st->stats.ptr, sizeof(int), hipMemcpyDeviceToHost, s));

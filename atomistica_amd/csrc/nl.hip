@@ -100,11 +100,10 @@ __global__ void k_cell_start(int nat, int ncell, const int *__restrict__ key_sor
 // Counting sort of the atoms by bin: counts, exclusive scan, scatter with
 // atomic slots, then each bin's atoms are put back into ascending index
 // order, which is the order a stable sort gives.  The reordering takes one
-// wavefront per bin: every element's rank is the number of smaller indices
-// in the bin (indices are distinct), counted through lane broadcasts.  A bin
-// with more than NL_SORT_MAX atoms flags a fallback to the radix sort.
-constexpr int NL_SORT_MAX = 256;
-
+// group of lanes per bin: every element's rank is the number of smaller
+// indices in the bin (indices are distinct), counted through lane
+// broadcasts.  A bin too full for its group flags a fallback to the radix
+// sort.
 __global__ void k_zero_i(int *__restrict__ a, long long n)
 {
     for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long long)gridDim.x * blockDim.x)
@@ -128,34 +127,37 @@ __global__ void k_cell_fill(int nat, int ncell, const int *__restrict__ key, int
     idx_sorted[cell_start[c] + slot] = i;
 }
 
+// G lanes per bin, up to 4 atoms per lane: bins of up to 4G atoms are
+// ordered in registers; a fuller bin flags the radix-sort fallback.
+template <int G>
 __global__ void __launch_bounds__(256)
 k_cell_order(int ncell, const int *__restrict__ cell_start, int *__restrict__ idx_sorted, int *__restrict__ flags)
 {
-    const int lane = threadIdx.x & 63;
-    const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (c >= ncell) return;                      // uniform per wavefront
+    const int lane = threadIdx.x % G;
+    const int c = blockIdx.x * (256 / G) + threadIdx.x / G;
+    if (c >= ncell) return;                      // uniform per group
     const int b = cell_start[c], n = cell_start[c + 1] - b;
     if (n <= 1) return;
-    if (n > NL_SORT_MAX) {
+    if (n > 4 * G) {
         if (lane == 0) atomicOr(flags, DERR_SORT_FALLBACK);
         return;
     }
     int v[4], r[4] = {0, 0, 0, 0};
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        const int k = lane + 64 * q;
+        const int k = lane + G * q;
         v[q] = k < n ? idx_sorted[b + k] : 0x7fffffff;
     }
     for (int k = 0; k < n; ++k) {
-        const int src = k >> 6;
+        const int src = k / G;
         const int w = src == 0 ? v[0] : (src == 1 ? v[1] : (src == 2 ? v[2] : v[3]));
-        const int u = __shfl(w, k & 63, 64);
+        const int u = __shfl(w, k % G, G);
 #pragma unroll
         for (int q = 0; q < 4; ++q) r[q] += u < v[q];
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        const int k = lane + 64 * q;
+        const int k = lane + G * q;
         if (k < n) idx_sorted[b + r[q]] = v[q];
     }
 }
@@ -576,8 +578,17 @@ int nl_update(agpu_neighbors *nl, agpu_particles *p, int force)
                                              (size_t)ncell, rocprim::plus<int>(), s));
             AGPU_LAUNCH("nl_sort_fill", k_cell_fill, dim3(nb), dim3(TB), 0, s, nat, (int)ncell, nl->key.as<int>(),
                         nl->cell_start.as<int>(), cnt, nl->idx_sorted.as<int>());
-            AGPU_LAUNCH("nl_sort_order", k_cell_order, dim3((unsigned)((ncell + 3) / 4)), dim3(TB), 0, s,
-                        (int)ncell, nl->cell_start.as<int>(), nl->idx_sorted.as<int>(), flags + 3);
+            // lanes per bin from the mean occupancy (capacity 4 atoms per lane)
+            const double occ = (double)nat / (double)ncell;
+            if (occ > 8.0)
+                AGPU_LAUNCH("nl_sort_order", k_cell_order<64>, dim3((unsigned)((ncell + 3) / 4)), dim3(TB), 0, s,
+                            (int)ncell, nl->cell_start.as<int>(), nl->idx_sorted.as<int>(), flags + 3);
+            else if (occ > 2.0)
+                AGPU_LAUNCH("nl_sort_order", k_cell_order<16>, dim3((unsigned)((ncell + 15) / 16)), dim3(TB), 0, s,
+                            (int)ncell, nl->cell_start.as<int>(), nl->idx_sorted.as<int>(), flags + 3);
+            else
+                AGPU_LAUNCH("nl_sort_order", k_cell_order<4>, dim3((unsigned)((ncell + 63) / 64)), dim3(TB), 0, s,
+                            (int)ncell, nl->cell_start.as<int>(), nl->idx_sorted.as<int>(), flags + 3);
         } else {
             size_t tmp = 0;
             int end_bit = 1;

@@ -83,8 +83,8 @@ def algorithmic_bytes(kernel, nn):
         return 40 + 8 * nn            # writes F'(rho) (8)
     if kernel.startswith("tersoff"):
         return 56 + 8 * nn
-    if kernel == "nl_fill":
-        return 40 + 8 * nn
+    if kernel in ("nl_fill", "nl_pairs"):
+        return 40 + 8 * nn            # list build: r, species, bin key, offset + the row written
     if kernel == "nl_count":
         return 36
     if kernel.startswith("ewald_real"):
