@@ -113,6 +113,7 @@ PROTOTYPES = {
     "agpu_tersoff_batch_forward": [vp, vp, i, vp, vp, i],
     "agpu_tersoff_batch_backward": [vp, vp, vp, vp, vp, i],
     "agpu_random_positions": [i, vp, d, ctypes.c_uint64, i, vp],
+    "agpu_lammps_compute": [vp, vp, vp, i, i, vp, vp, vp, vp, vp, vp],
 }
 _RESTYPE = {"agpu_version": ctypes.c_char_p}
 

@@ -4,6 +4,7 @@
 // potential dispatch of init / bind_to / energy_and_forces.
 #include <cmath>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -731,6 +732,56 @@ int agpu_coulomb_potential(agpu_potential *pot, agpu_particles *p, agpu_neighbor
     AGPU_CHECK(ewald_potential(pot, qd, pd, 0));
     AGPU_HIP(hipMemcpyAsync(phi, pd, sizeof(double) * nat, hipMemcpyDeviceToHost, s));
     AGPU_HIP(hipStreamSynchronize(s));
+    return AGPU_OK;
+}
+
+// LAMMPS pair-style step (PairAtomistica::FAtomistica, pair_atomistica.cpp:
+// 499-561): local atoms plus ghosts in an open box, the mask marking the
+// local atoms so that only terms centred on them carry energy, forces on all
+// nall atoms (LAMMPS returns the ghost forces to their owners), energy added
+// to *eng and the virial subtracted in Voigt order as the reference does.
+int agpu_lammps_compute(agpu_potential *pot, agpu_particles *p, agpu_neighbors *nl, int nlocal, int nall,
+                        const double *x, const int *Z, double *f, double *eng, double *virial, double *eatom)
+{
+    if (!pot || !p || !nl || nlocal < 0 || nall < nlocal || (nall > 0 && (!x || !Z || !f)) || !eng || !virial)
+        return set_error(AGPU_ERR_ARG, "lammps_compute: bad arguments");
+    if (pot->kind == AGPU_KIND_REBO2)
+        return set_error(AGPU_ERR_UNSUPPORTED, "lammps_compute: REBO2 takes no mask (needs the tag half rule)");
+    if (pot->kind == AGPU_KIND_EWALD)
+        return set_error(AGPU_ERR_UNSUPPORTED, "lammps_compute: Ewald needs a periodic cell");
+    if (nall == 0) return AGPU_OK;
+    double lo[3], hi[3];
+    for (int k = 0; k < 3; ++k) lo[k] = hi[k] = x[k];
+    for (int i = 1; i < nall; ++i)
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = std::min(lo[k], x[3 * (size_t)i + k]);
+            hi[k] = std::max(hi[k], x[3 * (size_t)i + k]);
+        }
+    const double margin = 1.0 + pot->cutoff;
+    double cell[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int k = 0; k < 3; ++k) cell[4 * k] = hi[k] - lo[k] + 2 * margin;
+    std::vector<double> xs(3 * (size_t)nall);
+    for (int i = 0; i < nall; ++i)
+        for (int k = 0; k < 3; ++k) xs[3 * (size_t)i + k] = x[3 * (size_t)i + k] - lo[k] + margin;
+    const int pbc[3] = {0, 0, 0};
+    if (p->nat != nall) AGPU_CHECK(agpu_particles_allocate(p, nall));
+    AGPU_CHECK(agpu_particles_set_cell(p, cell, pbc));
+    if (p->h_Z.size() != (size_t)nall || !std::equal(Z, Z + nall, p->h_Z.begin()))
+        AGPU_CHECK(agpu_particles_set_atomic_numbers(p, Z, AGPU_HOST));
+    AGPU_CHECK(agpu_particles_set_positions(p, xs.data(), AGPU_HOST));
+    if (!pot->bound || pot->p != p || pot->nl != nl) AGPU_CHECK(agpu_potential_bind_to(pot, p, nl));
+    std::vector<int> mask(nall);
+    for (int i = 0; i < nall; ++i) mask[i] = i < nlocal;
+    double epot = 0.0, wpot[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    AGPU_CHECK(agpu_potential_energy_and_forces(pot, p, nl, nullptr, &epot, f, wpot, mask.data(), eatom,
+                                                nullptr, nullptr, nullptr, nullptr, AGPU_HOST));
+    *eng += epot;
+    virial[0] -= wpot[0];
+    virial[1] -= wpot[4];
+    virial[2] -= wpot[8];
+    virial[3] -= 0.5 * (wpot[1] + wpot[3]);
+    virial[4] -= 0.5 * (wpot[2] + wpot[6]);
+    virial[5] -= 0.5 * (wpot[5] + wpot[7]);
     return AGPU_OK;
 }
 

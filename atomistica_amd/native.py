@@ -359,6 +359,22 @@ class Ewald(_Potential):
         return phi
 
 
+def lammps_compute(pot, p, nl, nlocal, x, Z, f=None, eatom=False):
+    """One LAMMPS pair-style step: local atoms then ghosts in x (nall, 3).
+    Returns (eng, f (nall, 3), virial[6] in LAMMPS order and sign, eatom)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    Z = np.ascontiguousarray(Z, dtype=np.int32)
+    nall = len(x)
+    if f is None:
+        f = np.zeros((nall, 3))
+    eng = ctypes.c_double(0.0)
+    vir = np.zeros(6)
+    ea = np.zeros(nall) if eatom else None
+    call("agpu_lammps_compute", pot._h, p._h, nl._h, int(nlocal), int(nall), _ptr(x), _ptr(Z), _ptr(f),
+         ctypes.byref(eng), _ptr(vir), _ptr(ea))
+    return eng.value, f, vir, ea
+
+
 def use_torch_stream():
     """Launch every kernel on torch's current stream of the current device,
     so library work is ordered with torch ops on the same tensors."""

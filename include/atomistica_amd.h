@@ -247,6 +247,20 @@ int agpu_tersoff_batch_forward(agpu_batch *b, const double *theta, int precision
 int agpu_tersoff_batch_backward(agpu_batch *b, const double *theta, const double *grad_e,
                                 const double *grad_f, double *grad_theta, int flags);
 
+/* LAMMPS pair-style step (PairAtomistica::FAtomistica,
+ * src/lammps/pair_style/pair_atomistica.cpp:499-561): x[3*nall] holds the
+ * nlocal local atoms followed by the ghosts (host memory, LAMMPS order), Z
+ * their atomic numbers.  The atoms are placed in an open box and the local
+ * atoms are masked in, so only terms centred on local atoms carry energy;
+ * f[3*nall] is accumulated on local atoms and ghosts (LAMMPS' reverse
+ * communication returns the ghost forces), *eng += energy, virial[6] -=
+ * wpot in LAMMPS Voigt order (xx yy zz xy xz yz) as the reference does.
+ * eatom (nall) is optional.  Tersoff and EAM; REBO2 and Ewald return
+ * AGPU_ERR_UNSUPPORTED. */
+int agpu_lammps_compute(agpu_potential *pot, agpu_particles *p, agpu_neighbors *nl, int nlocal, int nall,
+                        const double *x, const int *Z, double *f, double *eng, double *virial,
+                        double *eatom);
+
 /* Synthetic-workload utility (no counterpart in the reference): n random
  * points in the periodic cell (cell[3k+x] = component x of lattice vector k)
  * by random sequential addition with minimum distance dmin over every
