@@ -10,8 +10,11 @@
 //                 (loop1 and the nn sum of bop_kernel_rebo2.f90:700-1205)
 //   k_rb_conj     thread per atom: F(x_ik) of every carbon neighbour k and
 //                 the sum N^conj_i (ik_loop1, bop_kernel_rebo2.f90:1231-1325)
-//   k_rb_pair     thread per bond i->j with (dc = 0 and j > i) or dc > 0:
-//                 b_ij, b_ji with P, F_ij(N_i, N_j, N_conj), the torsion sum
+//   k_rb_bo       thread per directed bond: zeta with P, b and dV_A b', the
+//                 reverse bond's slot, the half-pair flag
+//   k_rb_list     ordered list of half pairs (exclusive scan of the flags)
+//   k_rb_pair     thread per half pair i->j ((dc = 0 and j > i) or dc > 0):
+//                 F_ij(N_i, N_j, N_conj), the torsion sum
 //                 T_ij(N_i, N_j, N_conj) sum_kl (1 - cos^2 w_kijl), the pair energy
 //                 and every force on i, j, k, l, m, n (ij_loop,
 //                 bop_kernel_rebo2.f90:1326-2779).  Forces, per-atom energies
@@ -29,6 +32,8 @@
 #include <cmath>
 #include <cstring>
 #include <vector>
+
+#include <rocprim/device/device_scan.hpp>
 
 #include "agpu_internal.hpp"
 #include "profile.hpp"
@@ -77,7 +82,8 @@ struct Rebo2State {
     int with_dihedral = 0;
     RbDev hdev{};
     std::vector<double> htab;
-    DevBuf dev, tab, bond, bij, bown, bcnt, nn, nconj, facc, eacc, wacc, partial, stats;
+    DevBuf dev, tab, bond, bij, bown, bcnt, nn, nconj, facc, eacc, wacc, partial, stats, bo, rev, half, hpos,
+        list, scan_tmp;
     int z2db[128];
     uint64_t Z_rev = ~0ull;
     double cutoff = 0.0;
@@ -643,106 +649,69 @@ __device__ __forceinline__ void outer_sub(double *w, const double *a, const doub
         for (int x = 0; x < 3; ++x) w[x + 3 * y] = w[x + 3 * y] - a[x] * b[y];
 }
 
-// bond-order pass over the neighbours of centre c for the bond c->o:
-// zeta, dzeta/dN (pass 1) or the bond-order force terms (pass 2)
-struct RbSide {
-    double z, dzdn;
-    double dbc[3], dbo[3];      // d b / d r_c, d b / d r_o (pass 2 sums)
-    double wb[9];               // bond-order virial sum (pass 2)
-};
-
-template <bool FORCES>
-__device__ __forceinline__ void rb_zeta(const RbDev &D, int kc, int ijpot, double rlij, double rlijr,
-                                        const double *rnij, double sgn, int beg, int nb, int skip,
-                                        const RbBond *__restrict__ bond, const int2 *__restrict__ bij,
-                                        const int *__restrict__ el, double nt, double dfb,
-                                        RbSide &S, unsigned long long *__restrict__ facc, int &bad)
+// zeta of the bond c->o over the other bonds of centre c, and dzeta/dN
+// (ik_loop2 of bop_kernel_rebo2.f90:1407-1594 without the derivatives)
+__device__ __forceinline__ void rb_zeta(const RbDev &D, int kc, int ijpot, double rlij, const double *rn,
+                                        int beg, int nb, int skip, const RbBond *__restrict__ bond,
+                                        const int2 *__restrict__ bij, const int *__restrict__ el, double nt,
+                                        double &z, double &dzdn)
 {
-    // sgn = +1: centre i, bond vector rnij towards j;  sgn = -1: centre j,
-    // the bond j->i is -rnij (costh = -rnjl.rnij, bop_kernel_rebo2.f90:1780)
+    z = 0.0;
+    dzdn = 0.0;
     for (int b = beg; b < beg + nb; ++b) {
         if (b == skip) continue;
         const RbBond B = bond[b];
-        const int ko = el[bij[b].x];
-        const int ikpot = rb_pair(kc, ko);
+        const int ikpot = rb_pair(kc, el[bij[b].x]);
         const double rlik = B.len;
         if (!(rlik < D.cut_h[ikpot])) continue;
         double qf, qd, gf, gd, dgdn;
         rb_h(D, ijpot, ikpot, rlij - rlik, qf, qd);
         double costh = 0.0;
-        costh = costh + B.n[0] * rnij[0];
-        costh = costh + B.n[1] * rnij[1];
-        costh = costh + B.n[2] * rnij[2];
-        if (sgn < 0) costh = -costh;
+        costh = costh + B.n[0] * rn[0];
+        costh = costh + B.n[1] * rn[1];
+        costh = costh + B.n[2] * rn[2];
         rb_g(D, kc, costh, nt, gf, gd, dgdn);
-        const double fcik = B.fc;
-        S.dzdn = S.dzdn + fcik * dgdn * qf;
-        S.z = S.z + fcik * gf * qf;
-        if (!FORCES) continue;
-        double dk[3], dd = 0.0;
-#pragma unroll
-        for (int x = 0; x < 3; ++x) dk[x] = B.n[x] * rlik - sgn * rnij[x] * rlij;
-        dd = dd + dk[0] * dk[0];
-        dd = dd + dk[1] * dk[1];
-        dd = dd + dk[2] * dk[2];
-        const double dis = sqrt(dd);
-#pragma unroll
-        for (int x = 0; x < 3; ++x) dk[x] = dk[x] / dis;
-        const double dcs_io = 1.0 / rlik - costh * rlijr;
-        const double dcs_ik = rlijr - costh / rlik;
-        const double dcs_ok = -dis * rlijr / rlik;
-        const double dzfac = fcik * gd * qf;
-        const double dzdro = gf * fcik * qd;
-        const double dzdrk = gf * (B.dfc * qf - fcik * qd);
-        double dfo[3], dbk[3];
-        const double rik[3] = {rlik * B.n[0], rlik * B.n[1], rlik * B.n[2]};
-#pragma unroll
-        for (int x = 0; x < 3; ++x) {
-            const double u = sgn * rnij[x];      // unit vector centre -> other
-            const double dcs_c = -dcs_io * u - dcs_ik * B.n[x];
-            const double dcs_o = dcs_io * u - dcs_ok * dk[x];
-            const double dcs_k = dcs_ik * B.n[x] + dcs_ok * dk[x];
-            S.dbc[x] = S.dbc[x] - dzdro * u - dzdrk * B.n[x] + dzfac * dcs_c;
-            dfo[x] = dzdro * u + dzfac * dcs_o;
-            S.dbo[x] = S.dbo[x] + dfo[x];
-            dbk[x] = dzdrk * B.n[x] + dzfac * dcs_k;
-        }
-        // virial: wijb -= r_ij (x) df_o + r_ik (x) db_k, with r_ij the
-        // centre -> other vector
-        const double rco[3] = {sgn * rlij * rnij[0], sgn * rlij * rnij[1], sgn * rlij * rnij[2]};
-        outer_sub(S.wb, rco, dfo);
-        outer_sub(S.wb, rik, dbk);
-        double fk[3] = {-dfb * dbk[0], -dfb * dbk[1], -dfb * dbk[2]};
-        fx_add3(facc, bij[b].x, fk, bad);
+        dzdn = dzdn + B.fc * dgdn * qf;
+        z = z + B.fc * gf * qf;
     }
 }
 
-// forces from dF/dN and dF/dN_conj (and the P correction) on the
-// neighbours k of a centre and on their neighbours m (bop_kernel_rebo2.f90:2400-2520)
-__device__ __forceinline__ void rb_nforces(int centre, int beg, int nb, int skip,
-                                           const RbBond *__restrict__ bond, const int2 *__restrict__ bij,
-                                           const int *__restrict__ offsets, const int *__restrict__ bcnt,
-                                           const int *__restrict__ el, double dfdn, double dfdncn,
-                                           double dfb, double dpdnc, double dpdnh, double *fcen,
-                                           double *w, unsigned long long *__restrict__ facc, int &bad)
+// Forces on the neighbours k of a centre c (bond c->o): the coordination
+// and conjugation terms (dF/dN, dF/dN_conj, P) with their second neighbours
+// m, and the bond-order terms of b_co, one accumulator update per k
+// (bop_kernel_rebo2.f90:2405-2520 and the dbidk/dbjdl parts of 1407-1920).
+// sgn = +1 for the centre i (bond towards j), -1 for the centre j; the
+// bond-order forces on the centre and on the other atom of the bond go to
+// fcen and foth, the virial to w, already scaled by -dV_A b'.
+__device__ __forceinline__ void rb_neighbours(const RbDev &D, int centre, int kc, int ijpot, double rlij,
+                                              double rlijr, const double *rnij, double sgn, int beg, int nb,
+                                              int skip, const RbBond *__restrict__ bond,
+                                              const int2 *__restrict__ bij, const int *__restrict__ offsets,
+                                              const int *__restrict__ bcnt, const int *__restrict__ el, double nt,
+                                              double dfdn, double dfdncn, double dfb, double dpdnc, double dpdnh,
+                                              double *fcen, double *foth, double *w,
+                                              unsigned long long *__restrict__ facc, int &bad)
 {
+    const double rco[3] = {sgn * rlij * rnij[0], sgn * rlij * rnij[1], sgn * rlij * rnij[2]};
     for (int b = beg; b < beg + nb; ++b) {
         if (b == skip) continue;
         const RbBond B = bond[b];
         const int2 e = bij[b];
         const int k = e.x;
-        const bool kC = el[k] == RB_C;
-        double df[3], dr[3];
+        const int ko = el[k];
+        const bool kC = ko == RB_C;
+        double fk[3], dr[3];
+        // coordination / conjugation
         const double dpn = kC ? dpdnc : dpdnh;
 #pragma unroll
         for (int x = 0; x < 3; ++x) {
             const double dn = B.n[x] * B.dfc;
             const double dncn = kC ? (B.fx * B.dfc) * B.n[x] : 0.0;
-            df[x] = -(dfdn * dn + dfdncn * dncn) - dfb * (dpn * dn);
-            fcen[x] -= df[x];
+            fk[x] = -(dfdn * dn + dfdncn * dncn) - dfb * (dpn * dn);
+            fcen[x] -= fk[x];
             dr[x] = B.len * B.n[x];
         }
-        outer_sub(w, dr, df);
+        outer_sub(w, dr, fk);
         if (kC && dfdncn != 0.0) {
             const double nconjdx = B.fc * B.dfx;
             const int kb = offsets[k], kn = bcnt[k];
@@ -755,14 +724,63 @@ __device__ __forceinline__ void rb_nforces(int centre, int beg, int nb, int skip
 #pragma unroll
                 for (int x = 0; x < 3; ++x) {
                     d2[x] = -dfdncn * (nconjdx * (M.dfc * M.n[x]));
-                    df[x] -= d2[x];
+                    fk[x] -= d2[x];
                     drm[x] = M.len * M.n[x];
                 }
                 outer_sub(w, drm, d2);
                 fx_add3(facc, em.x, d2, bad);
             }
         }
-        fx_add3(facc, k, df, bad);
+        // bond order: angular terms of zeta_co with the final dV_A b'
+        const int ikpot = rb_pair(kc, ko);
+        const double rlik = B.len;
+        if (rlik < D.cut_h[ikpot]) {
+            double qf, qd, gf, gd, dgdn;
+            rb_h(D, ijpot, ikpot, rlij - rlik, qf, qd);
+            double costh = 0.0;
+            costh = costh + B.n[0] * rnij[0];
+            costh = costh + B.n[1] * rnij[1];
+            costh = costh + B.n[2] * rnij[2];
+            if (sgn < 0) costh = -costh;
+            rb_g(D, kc, costh, nt, gf, gd, dgdn);
+            const double fcik = B.fc;
+            double dk[3], dd = 0.0;
+#pragma unroll
+            for (int x = 0; x < 3; ++x) dk[x] = B.n[x] * rlik - sgn * rnij[x] * rlij;
+            dd = dd + dk[0] * dk[0];
+            dd = dd + dk[1] * dk[1];
+            dd = dd + dk[2] * dk[2];
+            const double dis = sqrt(dd);
+#pragma unroll
+            for (int x = 0; x < 3; ++x) dk[x] = dk[x] / dis;
+            const double dcs_io = 1.0 / rlik - costh * rlijr;
+            const double dcs_ik = rlijr - costh / rlik;
+            const double dcs_ok = -dis * rlijr / rlik;
+            const double dzfac = fcik * gd * qf;
+            const double dzdro = gf * fcik * qd;
+            const double dzdrk = gf * (B.dfc * qf - fcik * qd);
+            double dfo[3], dbk[3];
+            const double rik[3] = {rlik * B.n[0], rlik * B.n[1], rlik * B.n[2]};
+#pragma unroll
+            for (int x = 0; x < 3; ++x) {
+                const double u = sgn * rnij[x];      // unit vector centre -> other
+                const double dcs_c = -dcs_io * u - dcs_ik * B.n[x];
+                const double dcs_o = dcs_io * u - dcs_ok * dk[x];
+                const double dcs_k = dcs_ik * B.n[x] + dcs_ok * dk[x];
+                fcen[x] += -dfb * (-dzdro * u - dzdrk * B.n[x] + dzfac * dcs_c);
+                dfo[x] = dzdro * u + dzfac * dcs_o;
+                foth[x] += -dfb * dfo[x];
+                dbk[x] = dzdrk * B.n[x] + dzfac * dcs_k;
+                fk[x] += -dfb * dbk[x];
+            }
+            // virial -dV_A b' (-(r_co x dfo) - (r_ck x dbk))
+#pragma unroll
+            for (int y = 0; y < 3; ++y)
+#pragma unroll
+                for (int x = 0; x < 3; ++x)
+                    w[x + 3 * y] = w[x + 3 * y] + dfb * (rco[x] * dfo[y] + rik[x] * dbk[y]);
+        }
+        fx_add3(facc, k, fk, bad);
     }
 }
 
@@ -877,173 +895,191 @@ __device__ double rb_torsion(int ib, int inb, int sij, int jb, int jnb, int sji,
     return bdh;
 }
 
+// Bond order of every directed bond c->o (thread per bond slot): zeta with
+// P(N_H, N_C) for a carbon centre, b = (1 + zeta)^-1/2 and dV_A b'/dzeta,
+// dP/dN + dzeta/dN; the slot of the reverse bond o->c; and whether the slot
+// carries a half pair ((dc = 0 and o > c) or dc > 0).
 __global__ void __launch_bounds__(256)
-k_rb_pair(int ntot, const RbDev *__restrict__ Dp, const double *__restrict__ tab,
-          const int *__restrict__ el, const int *__restrict__ offsets, const int *__restrict__ bcnt,
-          const int *__restrict__ bown, const RbBond *__restrict__ bond, const int2 *__restrict__ bij,
-          const double2 *__restrict__ nn, const double *__restrict__ nconjit,
+k_rb_bo(int ntot, const RbDev *__restrict__ Dp, const double *__restrict__ tab, const int *__restrict__ el,
+        const int *__restrict__ offsets, const int *__restrict__ bcnt, const int *__restrict__ bown,
+        const RbBond *__restrict__ bond, const int2 *__restrict__ bij, const double2 *__restrict__ nn,
+        double4 *__restrict__ bo, int *__restrict__ rev, int *__restrict__ half, int *__restrict__ flags)
+{
+    const RbDev &D = *Dp;
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ntot) return;
+    const int i = bown[s];
+    if (i < 0) {
+        half[s] = 0;
+        return;
+    }
+    const int2 e = bij[s];
+    const int j = e.x;
+    const int ki = el[i], kj = el[j];
+    const RbBond B = bond[s];
+    const int ijpot = rb_pair(ki, kj);
+    int a, b, c;
+    dc_unpack(e.y, a, b, c);
+    const bool pos = a != 0 ? a > 0 : (b != 0 ? b > 0 : c > 0);
+    const bool zero = a == 0 && b == 0 && c == 0;
+    const bool act = B.len < D.cut_h[ijpot];
+    half[s] = (act && ((zero && j > i) || pos)) ? 1 : 0;
+    int sr = -1;
+    const int jb = offsets[j], jn = bcnt[j];
+    for (int q = jb; q < jb + jn; ++q) {
+        const int2 eq = bij[q];
+        if (eq.x == i && dc_zero_sum(eq.y, e.y)) { sr = q; break; }
+    }
+    rev[s] = sr;
+    if (sr < 0) atomicOr(flags, DERR_BOND_OVERFLOW);
+    if (!act) return;
+    const double2 nni = nn[i];
+    double niC = nni.x, niH = nni.y;
+    if (kj == RB_C) niC = niC - B.fc; else niH = niH - B.fc;
+    niC = niC > 4.0 ? 4.0 : niC;
+    niH = niH > 4.0 ? 4.0 : niH;
+    double z, dzdn;
+    rb_zeta(D, ki, ijpot, B.len, B.n, offsets[i], bcnt[i], s, bond, bij, el, niC + niH, z, dzdn);
+    double dpc = 0.0, dph = 0.0;
+    if (ki == RB_C) {
+        double pij;
+        rb_table2(tab + (ijpot == RB_CC ? RB_TAB_PCC : RB_TAB_PCH), niH, niC, pij, dph, dpc);
+        z = z + pij;
+        dpc = dpc + dzdn;
+        dph = dph + dzdn;
+    }
+    double fa, dfa;
+    rb_va(D, ijpot, B.len, fa, dfa);
+    const double bb = pow(1.0 + z, D.conpe);
+    const double dfb = D.conan * B.fc * fa * pow(1.0 + z, D.conpf);
+    bo[s] = make_double4(bb, dfb, dpc, dph);
+}
+
+// ordered list of the half-pair slots (exclusive scan of the flags)
+__global__ void k_rb_list(int ntot, const int *__restrict__ half, const int *__restrict__ pos,
+                          int *__restrict__ list, int *__restrict__ count)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ntot) return;
+    if (half[s]) list[pos[s]] = s;
+    if (s == ntot - 1) *count = pos[s] + half[s];
+}
+
+__global__ void __launch_bounds__(256)
+k_rb_pair(int ntot, const int *__restrict__ list, const int *__restrict__ count, const RbDev *__restrict__ Dp,
+          const double *__restrict__ tab, const int *__restrict__ el, const int *__restrict__ offsets,
+          const int *__restrict__ bcnt, const int *__restrict__ bown, const RbBond *__restrict__ bond,
+          const int2 *__restrict__ bij, const double2 *__restrict__ nn, const double *__restrict__ nconjit,
+          const double4 *__restrict__ bo, const int *__restrict__ rev,
           unsigned long long *__restrict__ facc, unsigned long long *__restrict__ eacc,
           unsigned long long *__restrict__ wacc, double *__restrict__ partial, int *__restrict__ flags)
 {
     __shared__ double s_red[4][10];
     const RbDev &D = *Dp;
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
     int bad = 0;
     double epair = 0.0;
     double wij[9];
 #pragma unroll
     for (int q = 0; q < 9; ++q) wij[q] = 0.0;
-    const int i = s < ntot ? bown[s] : -1;
-    bool act = i >= 0;
-    int2 eij = make_int2(0, 0);
-    if (act) {
-        eij = bij[s];
-        int a, b, c;
-        dc_unpack(eij.y, a, b, c);
-        const bool pos = a != 0 ? a > 0 : (b != 0 ? b > 0 : c > 0);
-        const bool zero = a == 0 && b == 0 && c == 0;
-        act = (zero && eij.x > i) || pos;
-    }
-    if (act) {
+    if (t < *count) {
+        const int s = list[t];
+        const int i = bown[s];
+        const int2 eij = bij[s];
         const int j = eij.x;
+        const int sji = rev[s];
         const int ki = el[i], kj = el[j];
         const RbBond Bij = bond[s];
         const int ijpot = rb_pair(ki, kj);
         const double rlij = Bij.len;
-        if (rlij < D.cut_h[ijpot]) {
-            const double rlijr = 1.0 / rlij;
-            const double *rnij = Bij.n;
-            const double rij[3] = {rlij * rnij[0], rlij * rnij[1], rlij * rnij[2]};
-            const double fcar = Bij.fc, dfcar = Bij.dfc;
-            const double2 nni = nn[i], nnj = nn[j];
-            double niC = nni.x, niH = nni.y, njC = nnj.x, njH = nnj.y;
-            if (kj == RB_C) niC = niC - Bij.fc; else niH = niH - Bij.fc;
-            if (ki == RB_C) njC = njC - Bij.fc; else njH = njH - Bij.fc;
-            niC = niC > 4.0 ? 4.0 : niC;
-            niH = niH > 4.0 ? 4.0 : niH;
-            njC = njC > 4.0 ? 4.0 : njC;
-            njH = njH > 4.0 ? 4.0 : njH;
-            // N_i, N_j as g sees them; F sees them clamped to 3 below
-            const double nti_g = niC + niH, ntj_g = njC + njH;
-            double nti = nti_g, ntj = ntj_g;
-            double fa, dfa, fr, dfr;
-            rb_va(D, ijpot, rlij, fa, dfa);
-            rb_vr(D, ijpot, rlij, fr, dfr);
-            const int ib = offsets[i], inb = bcnt[i];
-            const int jb = offsets[j], jnb = bcnt[j];
-            // the bond j->i (image -dc) in j's list
-            int sji = -1;
-            for (int b = jb; b < jb + jnb; ++b) {
-                const int2 e = bij[b];
-                if (e.x == i && dc_zero_sum(e.y, eij.y)) { sji = b; break; }
-            }
-            if (sji < 0) bad |= 2;
-            // pass 1: zeta_ij, zeta_ji
-            RbSide Si{}, Sj{};
-            int dummy = 0;
-            rb_zeta<false>(D, ki, ijpot, rlij, rlijr, rnij, 1.0, ib, inb, s, bond, bij, el, nti_g, 0.0,
-                           Si, facc, dummy);
-            rb_zeta<false>(D, kj, ijpot, rlij, rlijr, rnij, -1.0, jb, jnb, sji, bond, bij, el, ntj_g, 0.0,
-                           Sj, facc, dummy);
-            const double nconji = nconjit[i] - Bij.fc * Bij.fx;
-            double nconjj = 0.0;
-            for (int b = jb; b < jb + jnb; ++b)
-                if (b != sji && el[bij[b].x] == RB_C) nconjj = nconjj + bond[b].fc * bond[b].fx;
-            // P and the bond orders
-            double pij = 0.0, dpdnci = 0.0, dpdnhi = 0.0, pji = 0.0, dpdncj = 0.0, dpdnhj = 0.0;
-            const double *P = tab + (ijpot == RB_CC ? RB_TAB_PCC : RB_TAB_PCH);
-            if (ki == RB_C) {
-                rb_table2(P, niH, niC, pij, dpdnhi, dpdnci);
-                Si.z = Si.z + pij;
-                dpdnci = dpdnci + Si.dzdn;
-                dpdnhi = dpdnhi + Si.dzdn;
-            }
-            if (kj == RB_C) {
-                rb_table2(P, njH, njC, pji, dpdnhj, dpdncj);
-                Sj.z = Sj.z + pji;
-                dpdncj = dpdncj + Sj.dzdn;
-                dpdnhj = dpdnhj + Sj.dzdn;
-            }
-            const double bi = pow(1.0 + Si.z, D.conpe);
-            const double dfbi = D.conan * fcar * fa * pow(1.0 + Si.z, D.conpf);
-            const double bj = pow(1.0 + Sj.z, D.conpe);
-            const double dfbj = D.conan * fcar * fa * pow(1.0 + Sj.z, D.conpf);
-            // F(N_i, N_j, N_conj)
-            double nconj = nconji * nconji + nconjj * nconjj;
-            nconj = nconj > 8.0 ? 8.0 : nconj;
-            nti = nti > 3.0 ? 3.0 : nti;
-            ntj = ntj > 3.0 ? 3.0 : ntj;
-            double fi[3] = {0, 0, 0}, fj[3] = {0, 0, 0};
-            double bdh = 0.0, tij = 0.0, dtdni = 0.0, dtdnj = 0.0, dtdncn = 0.0;
-            if (D.with_dihedral && ijpot == RB_CC) {
-                rb_table3(tab + RB_TAB_TCC, nti, ntj, nconj, tij, dtdni, dtdnj, dtdncn);
-                if (tij != 0)
-                    bdh = rb_torsion(ib, inb, s, jb, jnb, sji, eij.y, bond, bij, rnij, rlij, tij, fa, fcar,
-                                     fi, fj, wij, facc, bad);
-            }
-            double fij = 0.0, dfdni = 0.0, dfdnj = 0.0, dfdncn = 0.0;
-            if (ijpot == RB_CC)
-                rb_table3(tab + RB_TAB_FCC, nti, ntj, nconj, fij, dfdni, dfdnj, dfdncn);
-            else if (ijpot == RB_HH)
-                rb_table3(tab + RB_TAB_FHH, nti, ntj, nconj, fij, dfdni, dfdnj, dfdncn);
-            else if (ki == RB_C)
-                rb_table3(tab + RB_TAB_FCH, ntj, nti, nconj, fij, dfdnj, dfdni, dfdncn);
-            else
-                rb_table3(tab + RB_TAB_FCH, nti, ntj, nconj, fij, dfdni, dfdnj, dfdncn);
-            dfdni = dfdni + dtdni * bdh;
-            dfdnj = dfdnj + dtdnj * bdh;
-            dfdncn = dfdncn + dtdncn * bdh;
-            dfdni = 0.5 * fcar * fa * dfdni;
-            dfdnj = 0.5 * fcar * fa * dfdnj;
-            dfdncn = 0.5 * fcar * fa * dfdncn;
-            const double dfdncni = 2 * dfdncn * nconji;
-            const double dfdncnj = 2 * dfdncn * nconjj;
-            // coordination and conjugation forces
-            rb_nforces(i, ib, inb, s, bond, bij, offsets, bcnt, el, dfdni, dfdncni, dfbi, dpdnci,
-                       dpdnhi, fi, wij, facc, bad);
-            rb_nforces(j, jb, jnb, sji, bond, bij, offsets, bcnt, el, dfdnj, dfdncnj, dfbj, dpdncj,
-                       dpdnhj, fj, wij, facc, bad);
-            // pass 2: bond-order forces on k and l, sums for i and j
-            Si.z = Sj.z = Si.dzdn = Sj.dzdn = 0.0;
-            rb_zeta<true>(D, ki, ijpot, rlij, rlijr, rnij, 1.0, ib, inb, s, bond, bij, el, nti_g, dfbi, Si,
-                          facc, bad);
-            rb_zeta<true>(D, kj, ijpot, rlij, rlijr, rnij, -1.0, jb, jnb, sji, bond, bij, el, ntj_g, dfbj, Sj,
-                          facc, bad);
-            // pair energy and forces
-            const double bave = 0.5 * (bi + bj + fij + tij * bdh);
-            const double hlf = fcar * (fr + bave * fa) / 2;
-            epair = hlf + hlf;
-            const double dffac = dfr * fcar + bave * dfa * fcar + fr * dfcar + bave * fa * dfcar;
-            double df[3];
+        const double rlijr = 1.0 / rlij;
+        const double *rnij = Bij.n;
+        const double rij[3] = {rlij * rnij[0], rlij * rnij[1], rlij * rnij[2]};
+        const double fcar = Bij.fc, dfcar = Bij.dfc;
+        const double2 nni = nn[i], nnj = nn[j];
+        double niC = nni.x, niH = nni.y, njC = nnj.x, njH = nnj.y;
+        if (kj == RB_C) niC = niC - Bij.fc; else niH = niH - Bij.fc;
+        if (ki == RB_C) njC = njC - Bij.fc; else njH = njH - Bij.fc;
+        niC = niC > 4.0 ? 4.0 : niC;
+        niH = niH > 4.0 ? 4.0 : niH;
+        njC = njC > 4.0 ? 4.0 : njC;
+        njH = njH > 4.0 ? 4.0 : njH;
+        // N_i, N_j as g sees them; F sees them clamped to 3 below
+        const double nti_g = niC + niH, ntj_g = njC + njH;
+        double nti = nti_g, ntj = ntj_g;
+        double fa, dfa, fr, dfr;
+        rb_va(D, ijpot, rlij, fa, dfa);
+        rb_vr(D, ijpot, rlij, fr, dfr);
+        const int ib = offsets[i], inb = bcnt[i];
+        const int jb = offsets[j], jnb = bcnt[j];
+        const double4 boi = bo[s], boj = bo[sji];
+        const double bi = boi.x, dfbi = boi.y, dpdnci = boi.z, dpdnhi = boi.w;
+        const double bj = boj.x, dfbj = boj.y, dpdncj = boj.z, dpdnhj = boj.w;
+        const double nconji = nconjit[i] - Bij.fc * Bij.fx;
+        double nconjj = 0.0;
+        for (int b = jb; b < jb + jnb; ++b)
+            if (b != sji && el[bij[b].x] == RB_C) nconjj = nconjj + bond[b].fc * bond[b].fx;
+        // F(N_i, N_j, N_conj) and the torsion sum
+        double nconj = nconji * nconji + nconjj * nconjj;
+        nconj = nconj > 8.0 ? 8.0 : nconj;
+        nti = nti > 3.0 ? 3.0 : nti;
+        ntj = ntj > 3.0 ? 3.0 : ntj;
+        double fi[3] = {0, 0, 0}, fj[3] = {0, 0, 0};
+        double bdh = 0.0, tij = 0.0, dtdni = 0.0, dtdnj = 0.0, dtdncn = 0.0;
+        if (D.with_dihedral && ijpot == RB_CC) {
+            rb_table3(tab + RB_TAB_TCC, nti, ntj, nconj, tij, dtdni, dtdnj, dtdncn);
+            if (tij != 0)
+                bdh = rb_torsion(ib, inb, s, jb, jnb, sji, eij.y, bond, bij, rnij, rlij, tij, fa, fcar, fi, fj,
+                                 wij, facc, bad);
+        }
+        double fij = 0.0, dfdni = 0.0, dfdnj = 0.0, dfdncn = 0.0;
+        if (ijpot == RB_CC)
+            rb_table3(tab + RB_TAB_FCC, nti, ntj, nconj, fij, dfdni, dfdnj, dfdncn);
+        else if (ijpot == RB_HH)
+            rb_table3(tab + RB_TAB_FHH, nti, ntj, nconj, fij, dfdni, dfdnj, dfdncn);
+        else if (ki == RB_C)
+            rb_table3(tab + RB_TAB_FCH, ntj, nti, nconj, fij, dfdnj, dfdni, dfdncn);
+        else
+            rb_table3(tab + RB_TAB_FCH, nti, ntj, nconj, fij, dfdni, dfdnj, dfdncn);
+        dfdni = dfdni + dtdni * bdh;
+        dfdnj = dfdnj + dtdnj * bdh;
+        dfdncn = dfdncn + dtdncn * bdh;
+        dfdni = 0.5 * fcar * fa * dfdni;
+        dfdnj = 0.5 * fcar * fa * dfdnj;
+        dfdncn = 0.5 * fcar * fa * dfdncn;
+        const double dfdncni = 2 * dfdncn * nconji;
+        const double dfdncnj = 2 * dfdncn * nconjj;
+        // neighbours of i and of j: coordination, conjugation and bond-order forces
+        rb_neighbours(D, i, ki, ijpot, rlij, rlijr, rnij, 1.0, ib, inb, s, bond, bij, offsets, bcnt, el, nti_g,
+                      dfdni, dfdncni, dfbi, dpdnci, dpdnhi, fi, fj, wij, facc, bad);
+        rb_neighbours(D, j, kj, ijpot, rlij, rlijr, rnij, -1.0, jb, jnb, sji, bond, bij, offsets, bcnt, el, ntj_g,
+                      dfdnj, dfdncnj, dfbj, dpdncj, dpdnhj, fj, fi, wij, facc, bad);
+        // pair energy and forces
+        const double bave = 0.5 * (bi + bj + fij + tij * bdh);
+        const double hlf = fcar * (fr + bave * fa) / 2;
+        epair = hlf + hlf;
+        const double dffac = dfr * fcar + bave * dfa * fcar + fr * dfcar + bave * fa * dfcar;
+        double df[3];
 #pragma unroll
-            for (int x = 0; x < 3; ++x) {
-                df[x] = dffac * rnij[x];
-                fi[x] += df[x];
-                fj[x] -= df[x];
-            }
-            // the j-side sums are w.r.t. (centre j, other i): map them back
+        for (int x = 0; x < 3; ++x) {
+            df[x] = dffac * rnij[x];
+            fi[x] += df[x];
+            fj[x] -= df[x];
+        }
 #pragma unroll
-            for (int y = 0; y < 3; ++y)
+        for (int y = 0; y < 3; ++y)
 #pragma unroll
-                for (int x = 0; x < 3; ++x)
-                    wij[x + 3 * y] = wij[x + 3 * y] + rij[x] * df[y] - dfbi * Si.wb[x + 3 * y] -
-                                     dfbj * Sj.wb[x + 3 * y];
-#pragma unroll
-            for (int x = 0; x < 3; ++x) {
-                fi[x] += -(dfbi * Si.dbc[x] + dfbj * Sj.dbo[x]);
-                fj[x] += -(dfbi * Si.dbo[x] + dfbj * Sj.dbc[x]);
-            }
-            fx_add3(facc, i, fi, bad);
-            fx_add3(facc, j, fj, bad);
-            if (eacc) {
-                fx_add(eacc + i, hlf, bad);
-                fx_add(eacc + j, hlf, bad);
-            }
-            if (wacc) {
-                for (int q = 0; q < 9; ++q) {
-                    fx_add(wacc + 9 * (size_t)i + q, wij[q] / 2, bad);
-                    fx_add(wacc + 9 * (size_t)j + q, wij[q] / 2, bad);
-                }
+            for (int x = 0; x < 3; ++x) wij[x + 3 * y] = wij[x + 3 * y] + rij[x] * df[y];
+        fx_add3(facc, i, fi, bad);
+        fx_add3(facc, j, fj, bad);
+        if (eacc) {
+            fx_add(eacc + i, hlf, bad);
+            fx_add(eacc + j, hlf, bad);
+        }
+        if (wacc) {
+            for (int q = 0; q < 9; ++q) {
+                fx_add(wacc + 9 * (size_t)i + q, wij[q] / 2, bad);
+                fx_add(wacc + 9 * (size_t)j + q, wij[q] / 2, bad);
             }
         }
     }
@@ -1184,10 +1220,33 @@ int rebo2_compute(agpu_potential *pot, const agpu_eval_args *a)
     int nblk = (ntot + 255) / 256;
     if (nblk < 1) nblk = 1;
     AGPU_CHECK(st->partial.reserve(sizeof(double) * 10 * nblk));
-    AGPU_LAUNCH("rebo2_pair", k_rb_pair, dim3(nblk), dim3(256), 0, s, ntot, D, st->tab.as<double>(), el,
-                nl->offsets.as<int>(), st->bcnt.as<int>(), st->bown.as<int>(), st->bond.as<RbBond>(),
-                st->bij.as<int2>(), st->nn.as<double2>(), st->nconj.as<double>(),
-                st->facc.as<unsigned long long>(), eacc, wacc, st->partial.as<double>(), dstats);
+    AGPU_CHECK(st->bo.reserve(sizeof(double4) * (total + 1)));
+    AGPU_CHECK(st->rev.reserve(sizeof(int) * (total + 1)));
+    AGPU_CHECK(st->half.reserve(sizeof(int) * (total + 1)));
+    AGPU_CHECK(st->hpos.reserve(sizeof(int) * (total + 1)));
+    AGPU_CHECK(st->list.reserve(sizeof(int) * (total + 2)));
+    int *count = st->list.as<int>() + total + 1;
+    if (ntot > 0) {
+        AGPU_LAUNCH("rebo2_bo", k_rb_bo, dim3(nblk), dim3(256), 0, s, ntot, D, st->tab.as<double>(), el,
+                    nl->offsets.as<int>(), st->bcnt.as<int>(), st->bown.as<int>(), st->bond.as<RbBond>(),
+                    st->bij.as<int2>(), st->nn.as<double2>(), st->bo.as<double4>(), st->rev.as<int>(),
+                    st->half.as<int>(), dstats);
+        size_t tmp = 0;
+        AGPU_HIP(rocprim::exclusive_scan(nullptr, tmp, st->half.as<int>(), st->hpos.as<int>(), 0, (size_t)ntot,
+                                         rocprim::plus<int>(), s));
+        AGPU_CHECK(st->scan_tmp.reserve(tmp + 16));
+        AGPU_HIP(rocprim::exclusive_scan(st->scan_tmp.ptr, tmp, st->half.as<int>(), st->hpos.as<int>(), 0,
+                                         (size_t)ntot, rocprim::plus<int>(), s));
+        AGPU_LAUNCH("rebo2_list", k_rb_list, dim3(nblk), dim3(256), 0, s, ntot, st->half.as<int>(),
+                    st->hpos.as<int>(), st->list.as<int>(), count);
+    } else {
+        AGPU_HIP(hipMemsetAsync(count, 0, sizeof(int), s));
+    }
+    AGPU_LAUNCH("rebo2_pair", k_rb_pair, dim3(nblk), dim3(256), 0, s, ntot, st->list.as<int>(), count, D,
+                st->tab.as<double>(), el, nl->offsets.as<int>(), st->bcnt.as<int>(), st->bown.as<int>(),
+                st->bond.as<RbBond>(), st->bij.as<int2>(), st->nn.as<double2>(), st->nconj.as<double>(),
+                st->bo.as<double4>(), st->rev.as<int>(), st->facc.as<unsigned long long>(), eacc, wacc,
+                st->partial.as<double>(), dstats);
     AGPU_LAUNCH("rebo2_finish", k_rb_finish, dim3(nb), dim3(256), 0, s, nat,
                 st->facc.as<unsigned long long>(), eacc, wacc, a->f, a->epot_per_at, a->wpot_per_at);
     AGPU_CHECK(reduce_ev(st->partial.as<double>(), nblk, 10, a->epot, a->wpot, nullptr));
