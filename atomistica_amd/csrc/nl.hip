@@ -432,6 +432,82 @@ k_bin_pairs(int B, BinGeom g, CellArg C, double cutoff_sq, const SortedAtom *__r
     if (lane == 0 && any_over) atomicOr(flags, DERR_STAGE_OVERFLOW);
 }
 
+// Sparse bins (a few atoms each): one thread per home atom in bin order,
+// walking its own (2 nd + 1)^3 stencil in x-major bin order and each bin's
+// atoms in ascending order -- the reference's candidate order
+// (python_neighbors.f90:664-735) -- with the same non-contracted distance
+// test, so rows equal those of the wavefront kernel above.
+template <bool ORTHO>
+__global__ void __launch_bounds__(256)
+k_atom_pairs(int nat, BinGeom g, CellArg C, double cutoff_sq, const SortedAtom *__restrict__ atoms,
+             const int *__restrict__ cell_start, int *__restrict__ counts, int2 *__restrict__ stage, int S,
+             int *__restrict__ flags)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    bool over = false, bad = false;
+    if (p < nat) {
+        const SortedAtom A = atoms[p];
+        const double ri[3] = {A.x, A.y, A.z};
+        int hc[3], hw[3];
+        bin_of(g, ri, hc, hw);
+        int w0, w1, w2;
+        dc_unpack(A.w, w0, w1, w2);
+        const size_t srow = (size_t)p * S;
+        int cnt = 0;
+        for (int ox = -g.nd[0]; ox <= g.nd[0]; ++ox) {
+            int cx = hc[0] + ox, ux = 0;
+            if (g.pbc[0]) {
+                const int t = floordiv(cx - 1, g.nc[0]);
+                cx -= t * g.nc[0];
+                ux = -t;
+            } else if (cx < 1 || cx > g.nc[0]) {
+                continue;
+            }
+            for (int oy = -g.nd[1]; oy <= g.nd[1]; ++oy) {
+                int cy = hc[1] + oy, uy = 0;
+                if (g.pbc[1]) {
+                    const int t = floordiv(cy - 1, g.nc[1]);
+                    cy -= t * g.nc[1];
+                    uy = -t;
+                } else if (cy < 1 || cy > g.nc[1]) {
+                    continue;
+                }
+                for (int oz = -g.nd[2]; oz <= g.nd[2]; ++oz) {
+                    int cz = hc[2] + oz, uz = 0;
+                    if (g.pbc[2]) {
+                        const int t = floordiv(cz - 1, g.nc[2]);
+                        cz -= t * g.nc[2];
+                        uz = -t;
+                    } else if (cz < 1 || cz > g.nc[2]) {
+                        continue;
+                    }
+                    const int b = ((cx - 1) * g.nc[1] + (cy - 1)) * g.nc[2] + (cz - 1);
+                    const int q0 = cell_start[b], q1 = cell_start[b + 1];
+                    for (int q = q0; q < q1; ++q) {
+                        const SortedAtom Bq = atoms[q];
+                        int v0, v1, v2;
+                        dc_unpack(Bq.w, v0, v1, v2);
+                        const int s0 = w0 + ux - v0, s1 = w1 + uy - v1, s2 = w2 + uz - v2;
+                        if (Bq.j == A.j && s0 == 0 && s1 == 0 && s2 == 0) continue;
+                        if (pair_d2<ORTHO>(C, A.x, A.y, A.z, Bq.x, Bq.y, Bq.z, s0, s1, s2) < cutoff_sq) {
+                            if (!dc_pack_ok(s0) || !dc_pack_ok(s1) || !dc_pack_ok(s2)) bad = true;
+                            if (cnt < S) stage[srow + cnt] = make_int2(Bq.j, dc_pack(s0, s1, s2));
+                            ++cnt;
+                        }
+                    }
+                }
+            }
+        }
+        counts[A.j] = cnt;
+        over = cnt > S;
+    }
+    const unsigned long long mo = __ballot(over), mb = __ballot(bad);
+    if ((threadIdx.x & 63) == 0) {
+        if (mo) atomicOr(flags, DERR_STAGE_OVERFLOW);
+        if (mb) atomicOr(flags, DERR_IMAGE_RANGE);
+    }
+}
+
 // Copy staged rows (sorted slot p, stride S) into the CSR rows of the caller
 // atoms; applies the reference's capacity rule (python_neighbors.f90:713-718).
 __global__ void __launch_bounds__(256)
@@ -625,7 +701,17 @@ int nl_update(agpu_neighbors *nl, agpu_particles *p, int force)
         }
         const int S = nl->stage_w;
         AGPU_CHECK(nl->stage.reserve(sizeof(int2) * (size_t)nat * S));
-        if (ortho)
+        if (per_bin < 4.0) {
+            const int na = (nat + TB - 1) / TB;
+            if (ortho)
+                AGPU_LAUNCH("nl_pairs", k_atom_pairs<true>, dim3(na), dim3(TB), 0, s, nat, g, C, cutoff_sq,
+                            nl->atoms_sorted.as<SortedAtom>(), nl->cell_start.as<int>(), nl->counts.as<int>(),
+                            nl->stage.as<int2>(), S, flags + 3);
+            else
+                AGPU_LAUNCH("nl_pairs", k_atom_pairs<false>, dim3(na), dim3(TB), 0, s, nat, g, C, cutoff_sq,
+                            nl->atoms_sorted.as<SortedAtom>(), nl->cell_start.as<int>(), nl->counts.as<int>(),
+                            nl->stage.as<int2>(), S, flags + 3);
+        } else if (ortho)
             AGPU_LAUNCH("nl_pairs", k_bin_pairs<true>, dim3(nbin_blk), dim3(TB), 0, s, B, g, C, cutoff_sq,
                         nl->atoms_sorted.as<SortedAtom>(), nl->cell_start.as<int>(), nl->counts.as<int>(),
                         nl->stage.as<int2>(), S, flags + 3, flags);
