@@ -5,20 +5,23 @@
 // binning_init/binning_update (:765-959):
 //   1. k_bin:      bin of every atom (floor(rec_cell_size.r) + 1, wrapped on
 //                  periodic axes, clamped otherwise) and its wrap count.
-//   2. radix sort of (bin, atom) pairs: atoms of a bin become contiguous and
-//                  stay in ascending atom order -- the reference's linked
-//                  lists, as arrays (Morton-free row-major bin order).
-//   3. k_cell_start: first sorted slot of each bin.
-//   4. k_gather:   bin-sorted copy of (r, atom, wrap) so that candidate scans
+//   2. counting sort by bin (k_cell_count, exclusive scan, k_cell_fill), then
+//                  k_cell_order puts each bin back into ascending atom order:
+//                  the reference's linked lists, as arrays, in row-major bin
+//                  order.  A bin too full for k_cell_order flags a rebuild
+//                  with a radix sort of (bin, atom) pairs + k_cell_start.
+//   3. k_gather:   bin-sorted copy of (r, atom, wrap) so that candidate scans
 //                  read contiguous 32-byte records.
-//   5. k_bin_pairs<false>: one wavefront per home bin counts, for every atom
-//                  of the bin, the accepted (j, image) pairs over the stencil
-//                  (x-major, then y, then z) -> exclusive scan -> row offsets.
-//   6. k_bin_pairs<true>:  same walk, writes (j, packed image) in row order.
-// A home bin's atoms share one candidate list staged in LDS, so candidate
-// loads are shared by the whole wavefront and the pair tests run without
-// divergence.  The distance test is the reference's FP64 expression without
-// contraction, so the accepted set is bit-identical.
+//   4. nl_pairs:   the accepted (j, image) pairs of every atom over its
+//                  stencil (x-major, then y, then z; atoms of a bin ascending)
+//                  into a padded staging row of stride S, and the row counts.
+//                  k_bin_pairs (a wavefront per block of home bins, candidates
+//                  shared by the wavefront) when bins are dense, k_atom_pairs
+//                  (a thread per home atom) when they hold < 4 atoms.
+//   5. exclusive scan of the counts -> row offsets; k_nl_compact copies the
+//                  staging rows into the CSR list.
+// The distance test is the reference's FP64 expression without contraction,
+// so the accepted set is bit-identical.
 #include <cmath>
 #include <cstring>
 
