@@ -180,6 +180,97 @@ def cpu_baseline(wl, budget_s=12.0):
                       "OpenMP %d threads, C restatement of the reference loops" % (n, sample_atoms, threads)}
 
 
+def _kernel_times(N, fn, steps):
+    """Per-kernel HIP-event averages over `steps` calls of fn()."""
+    import torch
+    N.profile_reset()
+    N.profile_enable(True)
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    N.profile_enable(False)
+    kern = {}
+    for k in N.profile_list():
+        ms, cnt = N.profile_query(k)
+        if cnt:
+            kern[k] = {"avg_us": 1e3 * ms / cnt, "launches": cnt}
+    return kern
+
+
+def bench_batch(args):
+    """Batched Tersoff for fitting: 4096 periodic Si structures of 8..512
+    atoms (log-uniform), random triclinic cells at 2.33 g/cm^3 +-10 %, random
+    positions at minimum distance 2.0 A.  A step is one fitting step: forward
+    E+F in FP32-arith/FP64-acc, then the backward pass for d(loss)/dtheta of
+    a force-matching loss (the neighbour list is built once; positions do not
+    move during fitting)."""
+    import numpy as np
+    import torch
+    from atomistica_amd import native as N
+    from atomistica_amd import parameters as P
+    from atomistica_amd import workloads as W
+    from atomistica_amd.batch import TersoffBatch, theta_from_params
+    t0 = time.perf_counter()
+    S = W.random_si_structures(4096, seed=4)
+    t_gen = time.perf_counter() - t0
+    b = TersoffBatch(S)
+    th = theta_from_params(P.Tersoff_Si)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e64, f64 = b.forward(th, "fp64")              # builds the list
+    torch.cuda.synchronize()
+    t_first = time.perf_counter() - t0
+    ntot = b.ntot
+    gf = torch.randn_like(f64)
+    ge = torch.ones_like(e64)
+
+    def timed(fn, steps):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / steps
+
+    fwd64 = timed(lambda: b.forward(th, "fp64"), args.steps)
+    fwd32 = timed(lambda: b.forward(th, "mixed"), args.steps)
+    bwd = timed(lambda: b.backward(th, ge, gf), args.steps)
+    e32, f32 = b.forward(th, "mixed")
+    e64, f64 = b.forward(th, "fp64")
+    df = (f32 - f64).cpu().numpy()
+    de = (e32 - e64).cpu().numpy() / np.diff(b.offsets)
+    step = fwd32 + bwd
+    kern = _kernel_times(N, lambda: (b.forward(th, "fp64"), b.forward(th, "mixed"), b.backward(th, ge, gf)),
+                         args.steps)
+    out = {
+        "metric": METRIC,
+        "value": ntot / step,
+        "unit": "atom-evals/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32-arith/f64-acc", "data": "synthetic",
+        "config": {"workload": "batched Tersoff Si fitting: 4096 periodic structures, %d atoms, %d pairs "
+                               "(log-uniform 8..512 atoms, triclinic, 2.33 g/cm^3 +-10%%, dmin 2.0 A, rng 4)"
+                               % (ntot, b.pairs()),
+                   "step": "forward E+F (FP32-arith/FP64-acc) + backward d(loss)/dtheta, 11 parameters",
+                   "inputs": "positions and neighbour list resident in HBM"},
+        "forward_fp64": {"ms": 1e3 * fwd64, "value": ntot / fwd64},
+        "forward_mixed": {"ms": 1e3 * fwd32, "value": ntot / fwd32, "speedup_vs_fp64": fwd64 / fwd32,
+                          "force_rms_err": float(np.sqrt(np.mean(df ** 2))),
+                          "force_max_err": float(np.abs(df).max()),
+                          "energy_err_per_atom_max": float(np.abs(de).max())},
+        "backward": {"ms": 1e3 * bwd},
+        "first_forward_ms_incl_list": 1e3 * t_first,
+        "structure_generation_s": t_gen,
+        "kernels": kern,
+        "roofline": None,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -197,6 +288,10 @@ def main():
         print("bench.py: this build measures one GPU only (single-GPU boxes; no collective path "
               "is built); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "tersoff_batch_4096":
+        bench_batch(args)
+        return
 
     import numpy as np
     import torch

@@ -57,6 +57,33 @@ def test_batch_forward_fp64_parity():
         assert np.abs(f[a0:a1] - rf).max() <= REL * scale, s
 
 
+def test_batch_forward_long_rows():
+    """Rows longer than one 256-entry chunk (R = 10.5, S = 11 A: ~280
+    neighbours) take the unfused per-entry kernels; same parity bar."""
+    from atomistica_amd.batch import TersoffBatch, params_from_theta
+    th = _theta0()
+    th[9], th[10] = 10.5, 11.0
+    S = _structures(3, seed=8, nmax=24)
+    b = TersoffBatch(S)
+    e, f = b.forward(th, "fp64")
+    e = e.cpu().numpy()
+    f = f.cpu().numpy()
+    print("batch long rows: %d atoms, %d pairs" % (b.ntot, b.pairs()), flush=True)
+    assert b.pairs() > 256 * b.ntot * 0.9
+    prm = params_from_theta(th)
+    for s, (pos, cell) in enumerate(S):
+        nl = O.nl_build(pos, cell, [True] * 3, th[10], avgn=600)
+        re_, rf, _ = O.tersoff(prm, pos, np.zeros(len(pos), np.int32), cell, nl)
+        a0, a1 = b.offsets[s], b.offsets[s + 1]
+        assert abs(e[s] - re_) <= REL * max(1.0, abs(re_)), s
+        assert np.abs(f[a0:a1] - rf).max() <= REL * max(1.0, np.abs(rf).max()), s
+    # the unfused backward: dE/dtheta against central differences
+    import torch
+    g = b.backward(th, torch.ones(b.nstruct, dtype=torch.float64, device="cuda"), None)
+    fd = _fd_grad(lambda t: float(b.forward(t, "fp64")[0].sum().item()), th, range(11))
+    assert np.allclose(g, fd, rtol=1e-5, atol=1e-6 * np.abs(fd).max())
+
+
 def test_batch_forward_mixed_error():
     from atomistica_amd.batch import TersoffBatch
     th = _theta0()
