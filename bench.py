@@ -26,6 +26,7 @@ sys.path.insert(0, ROOT)
 METRIC = "atom-evals/s (E+F+virial), FP64 plain vs FP32/FP64-acc batched at force err≤1e-5"
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
 
 
 def build_workload(name):
@@ -180,6 +181,30 @@ def cpu_baseline(wl, budget_s=12.0):
                       "OpenMP %d threads, C restatement of the reference loops" % (n, sample_atoms, threads)}
 
 
+def valu_rates(workload, kern):
+    """FP64/FP32 VALU FLOP rates of the profiled kernels: FLOPs per launch
+    from the committed rocprofv3 counter summary (profiles/valu.json) over
+    this run's HIP-event average duration."""
+    fn = os.path.join(ROOT, "profiles", "valu.json")
+    if not os.path.exists(fn):
+        return None
+    try:
+        tab = json.load(open(fn)).get(workload, {})
+    except ValueError:
+        return None
+    out = {}
+    for k, rec in tab.items():
+        if k not in kern:
+            continue
+        s = kern[k]["avg_us"] * 1e-6
+        f64 = rec["flops_fp64"] / s / 1e12
+        f32 = rec["flops_fp32"] / s / 1e12
+        out[k] = {"tflops_fp64": f64, "frac_fp64_peak": f64 / FP64_PEAK_TFLOPS,
+                  "tflops_fp32": f32, "frac_fp32_peak": f32 / FP32_PEAK_TFLOPS,
+                  "valu_winst_per_s": rec["valu_insts"] / s}
+    return out or None
+
+
 def _kernel_times(N, fn, steps):
     """Per-kernel HIP-event averages over `steps` calls of fn()."""
     import torch
@@ -267,7 +292,27 @@ def bench_batch(args):
         "structure_generation_s": t_gen,
         "kernels": kern,
         "roofline": None,
+        "valu": valu_rates("tersoff_batch_4096", kern),
     }
+    if not args.no_cpu_baseline:
+        # the oracle (FP64 forward: list build + E + F), structure by
+        # structure on one core, over as many structures as ~12 s allow
+        from oracle import oracle as O
+        from atomistica_amd.batch import params_from_theta
+        prm = params_from_theta(th)
+        t0 = time.perf_counter()
+        na = ns = 0
+        for pos, cell in S:
+            nl = O.nl_build(pos, cell, [True] * 3, th[10])
+            O.tersoff(prm, pos, np.zeros(len(pos), np.int32), cell, nl)
+            na += len(pos)
+            ns += 1
+            if time.perf_counter() - t0 > 12.0:
+                break
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": na / dt, "unit": "atom-evals/s", "cores": 1, "kind": "port",
+                               "sample": "FP64 forward (list build + E + F) of the first %d of the 4096 structures "
+                                         "(%d atoms), one core, C restatement of the reference loops" % (ns, na)}
     print(json.dumps(out))
 
 
@@ -364,6 +409,7 @@ def main():
         "force_only": {"value": nat * args.steps / dt_force, "ms_per_step": 1e3 * dt_force / args.steps},
         "kernels": kern,
         "roofline": roof,
+        "valu": valu_rates(args.workload, kern),
     }
 
     if not args.no_mixed and args.precision == "fp64" and wl["kind"] in ("eam", "tersoff", "ewald"):
