@@ -112,6 +112,7 @@ PROTOTYPES = {
     "agpu_batch_get_size": [vp, ip, ip, lp],
     "agpu_tersoff_batch_forward": [vp, vp, i, vp, vp, i],
     "agpu_tersoff_batch_backward": [vp, vp, vp, vp, vp, i],
+    "agpu_ewald_batch_compute": [vp, vp, d, d, d, i, vp, vp, vp, vp, i],
     "agpu_random_positions": [i, vp, d, ctypes.c_uint64, i, vp],
     "agpu_lammps_compute": [vp, vp, vp, i, i, vp, vp, vp, vp, vp, vp],
 }

@@ -12,8 +12,9 @@ omega = 1 and no h-term).  :class:`TersoffBatchFunction` exposes this as a
 ``torch.autograd.Function`` so that a force-matching loss written in torch
 differentiates into the parameters.
 
-There is no counterpart of batching in the reference, which evaluates one
-structure per call.
+:func:`ewald_batch` evaluates the Ewald Coulomb sum over the same kind of
+batch.  There is no counterpart of batching in the reference, which
+evaluates one structure per call.
 """
 import ctypes
 
@@ -43,8 +44,10 @@ def params_from_theta(theta, el="Si"):
             "xi": [1.0], "omega": [1.0], "mubo": [0.0], "m": [1]}
 
 
-class TersoffBatch:
-    """A set of periodic structures [(positions (n,3), cell (3,3)), ...]."""
+class Batch:
+    """A set of periodic structures [(positions (n,3), cell (3,3)), ...],
+    concatenated on the GPU (atoms of structure s are offsets[s] ..
+    offsets[s+1]-1)."""
 
     def __init__(self, structures):
         import torch
@@ -76,6 +79,10 @@ class TersoffBatch:
         n = ctypes.c_int64()
         call("agpu_batch_get_size", self._h, None, None, ctypes.byref(n))
         return n.value
+
+
+class TersoffBatch(Batch):
+    """Batched single-element Tersoff over a :class:`Batch` of structures."""
 
     def forward(self, theta, precision="fp64"):
         """(E per structure (nstruct,), F (ntot, 3)) as torch device tensors."""
@@ -135,3 +142,24 @@ def tersoff_batch(theta, batch, precision="mixed"):
     if _FN is None:
         _FN = _function()
     return _FN.apply(theta, batch, precision)
+
+
+def ewald_batch(batch, q, cutoff, beta, kmax, precision="fp64", forces=True, virial=True, phi=False):
+    """Ewald sum of every structure of ``batch`` with charges q (ntot,), in
+    e^2/A: real space + self term (pme.f90:215-321, 386-437, 467-472) and the
+    direct FP64 S(k) sum over |k| <= kmax.  Returns torch device tensors
+    (E (nstruct,), F (ntot, 3) or None, W (nstruct, 3, 3) or None, phi (ntot,)
+    or None); phi = dE/dq, the backward with respect to the charges."""
+    import torch
+    qd = torch.as_tensor(np.asarray(q, dtype=np.float64), device="cuda").contiguous()
+    e = torch.zeros(batch.nstruct, dtype=torch.float64, device="cuda")
+    f = torch.zeros((batch.ntot, 3), dtype=torch.float64, device="cuda") if forces else None
+    w = torch.zeros((batch.nstruct, 3, 3), dtype=torch.float64, device="cuda") if virial else None
+    p = torch.zeros(batch.ntot, dtype=torch.float64, device="cuda") if phi else None
+    prec = {"fp64": _lib.PREC_FP64, "mixed": _lib.PREC_MIXED}[precision]
+
+    def ptr(t):
+        return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+    call("agpu_ewald_batch_compute", batch._h, ptr(qd), float(cutoff), float(beta), float(kmax), prec,
+         ptr(e), ptr(f), ptr(w), ptr(p), 0)
+    return e, f, w, p

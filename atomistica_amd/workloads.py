@@ -56,6 +56,20 @@ def rocksalt(n=30, a=5.64, sigma=0.1, seed=5):
     return pos, cell, np.array([True] * 3), Z, q
 
 
+def rocksalt_cells(ncells=256, nmin=2, nmax=8, a=5.64, sigma=0.1, seed=5):
+    """Batched Ewald workload: ncells rocksalt cells of n^3 conventional cells
+    each (8 n^3 ions, n uniform in nmin..nmax), Gaussian displacements sigma.
+    Returns ([(pos, cell), ...], q concatenated)."""
+    rng = np.random.default_rng(seed)
+    out, qs = [], []
+    for _ in range(ncells):
+        n = int(rng.integers(nmin, nmax + 1))
+        pos, cell, _, _, q = rocksalt(n, a=a, sigma=sigma, seed=int(rng.integers(0, 2 ** 31)))
+        out.append((pos, cell))
+        qs.append(q)
+    return out, np.concatenate(qs)
+
+
 # --------------------------------------------------------------------------
 # Finnis-Sinclair-form Cu tabulated as a setfl file
 # --------------------------------------------------------------------------

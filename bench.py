@@ -316,6 +316,56 @@ def bench_batch(args):
     print(json.dumps(out))
 
 
+def bench_ewald_batch(args):
+    """Batched Ewald: 256 rocksalt cells of n^3 conventional cells (n uniform
+    in 2..8, 64..4096 ions), sigma = 0.1, rc = 10 A, erfc(beta rc) = 1e-8,
+    exp(-kmax^2/4beta^2) = 1e-8.  A step is E + F + virial of every cell."""
+    import numpy as np
+    import torch
+    from atomistica_amd import native as N
+    from atomistica_amd import workloads as W
+    from atomistica_amd.batch import Batch, ewald_batch
+    S, q = W.rocksalt_cells(256, seed=5)
+    b = Batch(S)
+    rc = 10.0
+    beta = W.ewald_beta(rc, 1e-8)
+    kmax = W.ewald_kmax(beta, 1e-8)
+    N.use_torch_stream()
+
+    def timed(prec, steps):
+        for _ in range(args.warmup):
+            ewald_batch(b, q, rc, beta, kmax, prec)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            ewald_batch(b, q, rc, beta, kmax, prec)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / steps
+
+    t64 = timed("fp64", args.steps)
+    t32 = timed("mixed", args.steps)
+    e64, f64, w64, _ = ewald_batch(b, q, rc, beta, kmax, "fp64")
+    e32, f32, w32, _ = ewald_batch(b, q, rc, beta, kmax, "mixed")
+    df = (f32 - f64).cpu().numpy()
+    de = np.abs((e32 - e64).cpu().numpy()) / np.diff(b.offsets)
+    kern = _kernel_times(N, lambda: ewald_batch(b, q, rc, beta, kmax, "fp64"), args.steps)
+    out = {
+        "metric": METRIC, "value": b.ntot / t64, "unit": "atom-evals/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * t64,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "batched Ewald: 256 rocksalt cells, n^3 conventional cells each, n uniform in 2..8 "
+                               "(%d ions), a=5.64, sigma=0.1 (rng 5), rc=10, beta=%.6f, kmax=%.4f" % (b.ntot, beta, kmax),
+                   "step": "E + F + virial of every cell (real space + self + direct FP64 S(k) sum)",
+                   "inputs": "positions and charges resident in HBM"},
+        "mixed": {"ms_per_step": 1e3 * t32, "value": b.ntot / t32, "speedup_vs_fp64": t64 / t32,
+                  "force_rms_err": float(np.sqrt(np.mean(df ** 2))), "force_max_err": float(np.abs(df).max()),
+                  "energy_err_per_atom_max": float(de.max()), "units": "e^2/A"},
+        "kernels": kern,
+        "roofline": None,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -336,6 +386,9 @@ def main():
 
     if args.workload == "tersoff_batch_4096":
         bench_batch(args)
+        return
+    if args.workload == "ewald_batch_256":
+        bench_ewald_batch(args)
         return
 
     import numpy as np

@@ -247,6 +247,20 @@ int agpu_tersoff_batch_forward(agpu_batch *b, const double *theta, int precision
 int agpu_tersoff_batch_backward(agpu_batch *b, const double *theta, const double *grad_e,
                                 const double *grad_f, double *grad_theta, int flags);
 
+/* Batched Ewald Coulomb sum over the structures of b (e^2/Angstrom): real
+ * space + self term as pme_energy_and_forces / pme_potential
+ * (src/potentials/coulomb/pme.f90:215-321, 386-437, 467-472), reciprocal space
+ * as the direct S(k) sum over the half space |k| <= kmax in FP64 (the
+ * reference uses smooth PME, pme_kernel.f90:251-752).  Per-ion charges q[ntot]
+ * (|q| <= 1e-10 skipped as in the reference).  e_struct[nstruct] and
+ * wpot_struct[9*nstruct] (optional) are set; f[3*ntot] and phi[ntot] (both
+ * optional; phi = dE/dq) are accumulated.  precision AGPU_PREC_MIXED runs the
+ * real-space erfc/exp in FP32 with FP64 distances and accumulation.  No
+ * counterpart in the reference, which evaluates one structure per call. */
+int agpu_ewald_batch_compute(agpu_batch *b, const double *q, double cutoff, double beta, double kmax,
+                             int precision, double *e_struct, double *f, double *wpot_struct,
+                             double *phi, int flags);
+
 /* LAMMPS pair-style step (PairAtomistica::FAtomistica,
  * src/lammps/pair_style/pair_atomistica.cpp:499-561): x[3*nall] holds the
  * nlocal local atoms followed by the ghosts (host memory, LAMMPS order), Z
