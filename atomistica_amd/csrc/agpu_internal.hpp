@@ -108,6 +108,8 @@ struct agpu_neighbors {
         counts, offsets, nbr, sort_tmp, rowmax, rank, stage;
     int stage_w = 0;              // staging stride (entries per atom)
     bool radix = false;           // bin sort by radix sort (after a very full bin)
+    // the list was imported from the caller (LAMMPS) and is not rebuilt
+    bool external = false;
     agpu::DevBuf cell_cnt;
     agpu::HostStats stats;
     int64_t n_updates = 0;
@@ -116,6 +118,7 @@ struct agpu_neighbors {
 namespace agpu {
 
 int nl_update(agpu_neighbors *nl, agpu_particles *p, int force);
+int nl_import(agpu_neighbors *nl, agpu_particles *p, const int64_t *seed, const int *numneigh, const int *neigh);
 int nl_export(agpu_neighbors *nl, int64_t *offsets, int *nbr, int *dc, int on_device);
 int compute_inverse(const double *A, double *B);
 

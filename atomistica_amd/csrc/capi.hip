@@ -740,11 +740,16 @@ int agpu_coulomb_potential(agpu_potential *pot, agpu_particles *p, agpu_neighbor
 // local atoms so that only terms centred on them carry energy, forces on all
 // nall atoms (LAMMPS returns the ghost forces to their owners), energy added
 // to *eng and the virial subtracted in Voigt order as the reference does.
-int agpu_lammps_compute(agpu_potential *pot, agpu_particles *p, agpu_neighbors *nl, int nlocal, int nall,
-                        const double *x, const int *Z, double *f, double *eng, double *virial, double *eatom)
+// One LAMMPS step; the list is built here (list == false) or was imported.
+static int lammps_step(agpu_potential *pot, agpu_particles *p, agpu_neighbors *nl, int nlocal, int nall,
+                       const double *x, const int *Z, const int64_t *seed, const int *numneigh, const int *neigh,
+                       double *f, double *eng, double *virial, double *eatom)
 {
     if (!pot || !p || !nl || nlocal < 0 || nall < nlocal || (nall > 0 && (!x || !Z || !f)) || !eng || !virial)
         return set_error(AGPU_ERR_ARG, "lammps_compute: bad arguments");
+    const bool list = seed != nullptr;
+    if (list && nall > 0 && (!numneigh || !neigh))
+        return set_error(AGPU_ERR_ARG, "lammps_compute_list: bad neighbour arrays");
     if (pot->kind == AGPU_KIND_REBO2)
         return set_error(AGPU_ERR_UNSUPPORTED, "lammps_compute: REBO2 takes no mask (needs the tag half rule)");
     if (pot->kind == AGPU_KIND_EWALD)
@@ -769,7 +774,9 @@ int agpu_lammps_compute(agpu_potential *pot, agpu_particles *p, agpu_neighbors *
     if (p->h_Z.size() != (size_t)nall || !std::equal(Z, Z + nall, p->h_Z.begin()))
         AGPU_CHECK(agpu_particles_set_atomic_numbers(p, Z, AGPU_HOST));
     AGPU_CHECK(agpu_particles_set_positions(p, xs.data(), AGPU_HOST));
+    nl->external = false;
     if (!pot->bound || pot->p != p || pot->nl != nl) AGPU_CHECK(agpu_potential_bind_to(pot, p, nl));
+    if (list) AGPU_CHECK(nl_import(nl, p, seed, numneigh, neigh));
     std::vector<int> mask(nall);
     for (int i = 0; i < nall; ++i) mask[i] = i < nlocal;
     double epot = 0.0, wpot[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -783,6 +790,20 @@ int agpu_lammps_compute(agpu_potential *pot, agpu_particles *p, agpu_neighbors *
     virial[4] -= 0.5 * (wpot[2] + wpot[6]);
     virial[5] -= 0.5 * (wpot[5] + wpot[7]);
     return AGPU_OK;
+}
+
+int agpu_lammps_compute(agpu_potential *pot, agpu_particles *p, agpu_neighbors *nl, int nlocal, int nall,
+                        const double *x, const int *Z, double *f, double *eng, double *virial, double *eatom)
+{
+    return lammps_step(pot, p, nl, nlocal, nall, x, Z, nullptr, nullptr, nullptr, f, eng, virial, eatom);
+}
+
+int agpu_lammps_compute_list(agpu_potential *pot, agpu_particles *p, agpu_neighbors *nl, int nlocal, int nall,
+                             const double *x, const int *Z, const int64_t *seed, const int *numneigh,
+                             const int *neigh, double *f, double *eng, double *virial, double *eatom)
+{
+    if (!seed) return set_error(AGPU_ERR_ARG, "lammps_compute_list: seed is NULL");
+    return lammps_step(pot, p, nl, nlocal, nall, x, Z, seed, numneigh, neigh, f, eng, virial, eatom);
 }
 
 }  // extern "C"

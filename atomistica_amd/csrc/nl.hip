@@ -22,8 +22,10 @@
 //                  staging rows into the CSR list.
 // The distance test is the reference's FP64 expression without contraction,
 // so the accepted set is bit-identical.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <vector>
 
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
@@ -605,6 +607,12 @@ static void geometry(const agpu_particles *p, agpu_neighbors *nl, BinGeom &g)
 
 int nl_update(agpu_neighbors *nl, agpu_particles *p, int force)
 {
+    if (nl->external) {
+        // an imported list stays as the caller gave it
+        if (nl->p != p || nl->nat != p->nat)
+            return set_error(AGPU_ERR_STATE, "neighbour list: imported for other particles");
+        return AGPU_OK;
+    }
     if (!p->cell_set) return set_error(AGPU_ERR_STATE, "particles: cell not set");
     if (p->nat > 0 && !p->d_r) return set_error(AGPU_ERR_STATE, "particles: positions not set");
     if (!force && nl->p == p && nl->pos_rev == p->pos_rev && nl->cell_rev == p->cell_rev &&
@@ -773,6 +781,63 @@ int nl_update(agpu_neighbors *nl, agpu_particles *p, int force)
                              std::to_string(nat) + " separators exceed nat*avgn = " +
                              std::to_string(nl->cap) + ". Increase avgn.");
     }
+    return AGPU_OK;
+}
+
+__global__ void k_iota(int n, int *__restrict__ a)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = i;
+}
+
+// The caller's own full list (LAMMPS: firstneigh/numneigh of local and ghost
+// atoms, as PairAtomistica::Atomistica_neigh hands it to the reference,
+// pair_atomistica.cpp:385-470): row i is neigh[seed[i] .. seed[i] +
+// numneigh[i]), indices into the particles, no periodic images (the caller
+// supplies ghost atoms instead).  LAMMPS keeps special-bond flags in the top
+// two bits of an entry; they are masked off.  The rows are kept in the
+// caller's order and are used until the next import or rebuild request.
+int nl_import(agpu_neighbors *nl, agpu_particles *p, const int64_t *seed, const int *numneigh, const int *neigh)
+{
+    const int nat = p->nat;
+    std::vector<int> off(nat + 1, 0);
+    int64_t total = 0;
+    int maxrow = 0;
+    for (int i = 0; i < nat; ++i) {
+        if (numneigh[i] < 0) return set_error(AGPU_ERR_ARG, "neighbour import: negative row length");
+        off[i] = (int)total;
+        total += numneigh[i];
+        maxrow = std::max(maxrow, numneigh[i]);
+        if (total > 0x7fffffffll) return set_error(AGPU_ERR_ARG, "neighbour import: more than 2^31 entries");
+    }
+    off[nat] = (int)total;
+    std::vector<int2> rows((size_t)total + 1);
+    const int zero = (512 << 20) | (512 << 10) | 512;     // image (0, 0, 0)
+    for (int i = 0; i < nat; ++i) {
+        const int *src = neigh + seed[i];
+        for (int k = 0; k < numneigh[i]; ++k) {
+            const int j = src[k] & 0x3fffffff;
+            if (j >= nat) return set_error(AGPU_ERR_ARG, "neighbour import: index beyond the atoms");
+            rows[off[i] + k] = make_int2(j, zero);
+        }
+    }
+    AGPU_CHECK(nl->offsets.reserve(sizeof(int) * (nat + 1)));
+    AGPU_CHECK(nl->nbr.reserve(sizeof(int2) * ((size_t)total + 1)));
+    AGPU_CHECK(nl->idx_sorted.reserve(sizeof(int) * (nat + 1)));
+    hipStream_t s = stream();
+    AGPU_HIP(hipMemcpyAsync(nl->offsets.ptr, off.data(), sizeof(int) * (nat + 1), hipMemcpyHostToDevice, s));
+    if (total) AGPU_HIP(hipMemcpyAsync(nl->nbr.ptr, rows.data(), sizeof(int2) * total, hipMemcpyHostToDevice, s));
+    if (nat) AGPU_LAUNCH("nl_iota", k_iota, dim3((nat + 255) / 256), dim3(256), 0, s, nat, nl->idx_sorted.as<int>());
+    AGPU_HIP(hipStreamSynchronize(s));
+    nl->external = true;
+    nl->p = p;
+    nl->nat = nat;
+    nl->pos_rev = p->pos_rev;
+    nl->cell_rev = p->cell_rev;
+    nl->total = total;
+    nl->maxrow = maxrow;
+    nl->cap = total + nat;
+    nl->n_updates++;
     return AGPU_OK;
 }
 

@@ -375,6 +375,28 @@ def lammps_compute(pot, p, nl, nlocal, x, Z, f=None, eatom=False):
     return eng.value, f, vir, ea
 
 
+def lammps_compute_list(pot, p, nl, nlocal, x, Z, seed, numneigh, neigh, f=None, eatom=False):
+    """The LAMMPS step over LAMMPS' own full list with ghosts: row i is
+    neigh[seed[i] : seed[i] + numneigh[i]] (indices into x, special-bond bits
+    in the top two bits are ignored).  Returns as :func:`lammps_compute`."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    Z = np.ascontiguousarray(Z, dtype=np.int32)
+    seed = np.ascontiguousarray(seed, dtype=np.int64)
+    numneigh = np.ascontiguousarray(numneigh, dtype=np.int32)
+    neigh = np.ascontiguousarray(neigh, dtype=np.int32)
+    nall = len(x)
+    if len(seed) != nall or len(numneigh) != nall:
+        raise ValueError("seed and numneigh need one entry per local and ghost atom")
+    if f is None:
+        f = np.zeros((nall, 3))
+    eng = ctypes.c_double(0.0)
+    vir = np.zeros(6)
+    ea = np.zeros(nall) if eatom else None
+    call("agpu_lammps_compute_list", pot._h, p._h, nl._h, int(nlocal), int(nall), _ptr(x), _ptr(Z), _ptr(seed),
+         _ptr(numneigh), _ptr(neigh), _ptr(f), ctypes.byref(eng), _ptr(vir), _ptr(ea))
+    return eng.value, f, vir, ea
+
+
 def use_torch_stream():
     """Launch every kernel on torch's current stream of the current device,
     so library work is ordered with torch ops on the same tensors."""

@@ -275,6 +275,18 @@ int agpu_lammps_compute(agpu_potential *pot, agpu_particles *p, agpu_neighbors *
                         const double *x, const int *Z, double *f, double *eng, double *virial,
                         double *eatom);
 
+/* The same LAMMPS step over LAMMPS' own full neighbour list with ghosts
+ * instead of a list built here: PairAtomistica::Atomistica_neigh
+ * (src/lammps/pair_style/pair_atomistica.cpp:385-470) hands the reference
+ * seed/last offsets into the LAMMPS neighbour memory; here row i (of all nall
+ * local and ghost atoms) is neigh[seed[i] .. seed[i] + numneigh[i]), entries
+ * are indices into x with LAMMPS' special-bond bits (the top two) masked
+ * off, and no periodic images.  The list is imported into nl and kept until
+ * the next call on nl. */
+int agpu_lammps_compute_list(agpu_potential *pot, agpu_particles *p, agpu_neighbors *nl, int nlocal, int nall,
+                             const double *x, const int *Z, const int64_t *seed, const int *numneigh,
+                             const int *neigh, double *f, double *eng, double *virial, double *eatom);
+
 /* Synthetic-workload utility (no counterpart in the reference): n random
  * points in the periodic cell (cell[3k+x] = component x of lattice vector k)
  * by random sequential addition with minimum distance dmin over every

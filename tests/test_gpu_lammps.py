@@ -93,6 +93,83 @@ def test_lammps_eam(tmp_path):
     assert close(f1, f0) and close(v1, v0)
 
 
+def lammps_full_list(x, rc, seed=0):
+    """A LAMMPS-style full list with ghosts: every atom (local and ghost) gets
+    the atoms within rc (cut-off + skin) in a shuffled order, the rows stored
+    in reverse order in one array (seed = offset of each row), and some
+    entries carry special-bond bits in the top two bits."""
+    from scipy.spatial import cKDTree
+    rng = np.random.default_rng(seed)
+    pairs = cKDTree(x).query_ball_point(x, rc)
+    rows = []
+    for i, js in enumerate(pairs):
+        js = np.array([j for j in js if j != i], dtype=np.int64)
+        rng.shuffle(js)
+        special = rng.random(len(js)) < 0.1
+        js = js | (special.astype(np.int64) << 30)
+        rows.append(js.astype(np.uint32).view(np.int32))
+    neigh, seed_, pos = [], np.zeros(len(x), np.int64), 0
+    for i in reversed(range(len(x))):
+        seed_[i] = pos
+        neigh.append(rows[i])
+        pos += len(rows[i])
+    return seed_, np.array([len(r) for r in rows], np.int32), np.concatenate(neigh)
+
+
+def run_lammps_list(pot, pos, cell, Z, border, rc_list):
+    gx, gown = ghosts(pos, cell, border)
+    nlocal = len(pos)
+    x = np.concatenate([pos, gx])
+    Zall = np.concatenate([Z, Z[gown]]).astype(np.int32)
+    seed, numneigh, neigh = lammps_full_list(x, rc_list)
+    p = N.Particles()
+    nl = N.Neighbors(200)
+    eng, f, vir, _ = N.lammps_compute_list(pot, p, nl, nlocal, x, Zall, seed, numneigh, neigh)
+    floc = f[:nlocal].copy()
+    np.add.at(floc, gown, f[nlocal:])
+    return eng, floc, vir
+
+
+def test_lammps_list_tersoff_sic():
+    """Tersoff over LAMMPS' own full list with ghosts (skin 1 A, shuffled
+    rows, special-bond bits set) equals the periodic evaluation."""
+    pos, cell, pbc, Z = W.diamond_si(3, a=4.36, sigma=0.06, seed=11)
+    Z = np.where(np.arange(len(pos)) % 8 >= 4, 14, 6).astype(np.int32)
+    pos = _wrap(pos, cell)
+    e0, f0, v0 = periodic(N.Tersoff(), pos, cell, Z)
+    e1, f1, v1 = run_lammps_list(N.Tersoff(), pos, cell, Z, 2 * 3.0 + 1.5, 3.0 + 1.0)
+    print("lammps list tersoff: dE %.3e" % (e1 - e0), flush=True)
+    assert abs(e1 - e0) <= REL * abs(e0)
+    assert close(f1, f0) and close(v1, v0)
+
+
+def test_lammps_list_eam():
+    import tempfile
+    import os
+    d = tempfile.mkdtemp()
+    fn = os.path.join(d, "fs_cu.eam.alloy")
+    with open(fn, "w") as fh:
+        fh.write(W.fs_cu_setfl_text(nr=4000, nrho=4000))
+    pos, cell, pbc, Z = W.fcc_cu(4, sigma=0.08, seed=3)
+    pos = _wrap(pos, cell)
+    e0, f0, v0 = periodic(N.TabulatedAlloyEAM(fn=fn), pos, cell, Z)
+    e1, f1, v1 = run_lammps_list(N.TabulatedAlloyEAM(fn=fn), pos, cell, Z, 5.5 + 1.5, 5.5 + 1.0)
+    print("lammps list eam: dE %.3e" % (e1 - e0), flush=True)
+    assert abs(e1 - e0) <= REL * abs(e0)
+    assert close(f1, f0) and close(v1, v0)
+
+
+def test_lammps_list_bad_index():
+    pos, cell, pbc, Z = W.diamond_c(2, sigma=0.02, seed=1)
+    n = len(pos)
+    seed = np.zeros(n, np.int64)
+    numneigh = np.zeros(n, np.int32)
+    numneigh[0] = 1
+    neigh = np.array([n + 5], np.int32)
+    with pytest.raises(AgpuError):
+        N.lammps_compute_list(N.Tersoff(), N.Particles(), N.Neighbors(100), n, pos, Z, seed, numneigh, neigh)
+
+
 def test_lammps_unsupported_classes():
     pos, cell, pbc, Z = W.diamond_c(2, sigma=0.02, seed=1)
     with pytest.raises(AgpuError):
