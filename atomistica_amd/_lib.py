@@ -30,6 +30,7 @@ KIND_TERSOFF = 1
 KIND_EAM_ALLOY = 2
 KIND_EWALD = 3
 KIND_REBO2 = 4
+KIND_KUMAGAI = 5
 REBO2_ZERO_TABLES = 1
 REBO2_DIHEDRAL = 2
 
@@ -59,6 +60,13 @@ class TersoffParams(ctypes.Structure):
         [("m", ctypes.c_int * 6)] + \
         [(k, ctypes.c_double * 3) for k in ("beta", "n", "c", "d", "h")] + \
         [(k, ctypes.c_double * 6) for k in ("r1", "r2")]
+
+
+class KumagaiParams(ctypes.Structure):
+    _fields_ = [("Z", ctypes.c_int)] + \
+        [(k, ctypes.c_double) for k in ("A", "B", "lambda1", "lambda2", "eta", "delta", "alpha")] + \
+        [("beta", ctypes.c_int)] + \
+        [(k, ctypes.c_double) for k in ("c1", "c2", "c3", "c4", "c5", "h", "r1", "r2")]
 
 
 # name -> argtypes (every function returns int unless listed in _RESTYPE)
@@ -95,6 +103,7 @@ PROTOTYPES = {
     "agpu_neighbors_get": [vp, vp, vp, vp, i],
     "agpu_tersoff_create": [ctypes.POINTER(vp), ctypes.POINTER(TersoffParams)],
     "agpu_tersoff_set_params": [vp, ctypes.POINTER(TersoffParams)],
+    "agpu_kumagai_create": [ctypes.POINTER(vp), ctypes.POINTER(KumagaiParams)],
     "agpu_eam_alloy_create": [ctypes.POINTER(vp), ctypes.c_char_p, ctypes.c_char_p],
     "agpu_ewald_create": [ctypes.POINTER(vp), d, d, d],
     "agpu_rebo2_create": [ctypes.POINTER(vp), i],

@@ -257,3 +257,4 @@ Tersoff = _make(native.Tersoff, "Tersoff")
 TabulatedAlloyEAM = _make(native.TabulatedAlloyEAM, "TabulatedAlloyEAM")
 Ewald = _make(native.Ewald, "Ewald")
 Rebo2 = _make(native.Rebo2, "Rebo2")
+Kumagai = _make(native.Kumagai, "Kumagai")

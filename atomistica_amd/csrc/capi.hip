@@ -610,7 +610,8 @@ int agpu_potential_bind_to(agpu_potential *pot, agpu_particles *p, agpu_neighbor
     pot->bound = 0;
     int rc = AGPU_ERR_STATE;
     switch (pot->kind) {
-    case AGPU_KIND_TERSOFF: rc = tersoff_bind(pot); break;
+    case AGPU_KIND_TERSOFF:
+    case AGPU_KIND_KUMAGAI: rc = tersoff_bind(pot); break;
     case AGPU_KIND_EAM_ALLOY: rc = eam_bind(pot); break;
     case AGPU_KIND_EWALD: rc = ewald_bind(pot); break;
     case AGPU_KIND_REBO2: rc = rebo2_bind(pot); break;
@@ -629,7 +630,8 @@ static int stage_and_call(agpu_potential *pot, agpu_particles *p, agpu_neighbors
     int rc = AGPU_OK;
     if (!(a.flags & AGPU_HOST)) {
         switch (pot->kind) {
-        case AGPU_KIND_TERSOFF: rc = tersoff_compute(pot, &a); break;
+        case AGPU_KIND_TERSOFF:
+        case AGPU_KIND_KUMAGAI: rc = tersoff_compute(pot, &a); break;
         case AGPU_KIND_EAM_ALLOY: rc = eam_compute(pot, &a); break;
         case AGPU_KIND_EWALD: rc = ewald_compute(pot, &a); break;
         case AGPU_KIND_REBO2: rc = rebo2_compute(pot, &a); break;
@@ -673,7 +675,8 @@ static int stage_and_call(agpu_potential *pot, agpu_particles *p, agpu_neighbors
     if (wpa) AGPU_HIP(hipMemcpyAsync(wpa, a.wpot_per_at, sizeof(double) * 9 * nat, hipMemcpyHostToDevice, s));
     if (md) AGPU_HIP(hipMemcpyAsync(md, a.mask, sizeof(int) * nat, hipMemcpyHostToDevice, s));
     switch (pot->kind) {
-    case AGPU_KIND_TERSOFF: rc = tersoff_compute(pot, &d); break;
+    case AGPU_KIND_TERSOFF:
+    case AGPU_KIND_KUMAGAI: rc = tersoff_compute(pot, &d); break;
     case AGPU_KIND_EAM_ALLOY: rc = eam_compute(pot, &d); break;
     case AGPU_KIND_EWALD: rc = ewald_compute(pot, &d); break;
     case AGPU_KIND_REBO2: rc = rebo2_compute(pot, &d); break;

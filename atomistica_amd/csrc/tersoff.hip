@@ -37,6 +37,12 @@ struct TersoffDev {
     double r1[6], r2[6];
     // derived
     double c2[3], d2[3], bn[3], e[3], fac[6];
+    // Kumagai family (kumagai_func.f90): g, bond order and h forms; A, B,
+    // lambda (= lambda1), mu (= lambda2), h, r1, r2 are shared with the above
+    int fam;                      // 0 Tersoff, 1 Kumagai
+    double kc1[3], kc2[3], kc3[3], kc4[3], kc5[3], keta[3], kdelta[3];
+    double kalpha[6];
+    int kbeta[6];
 };
 
 struct TersoffState {
@@ -57,6 +63,7 @@ __host__ __device__ __forceinline__ int pair_index(int i, int j, int nel)
 
 static void derive(TersoffDev &D, const agpu_tersoff_params &p)
 {
+    std::memset(&D, 0, sizeof D);
     D.nel = p.nel;
     for (int k = 0; k < 6; ++k) {
         D.A[k] = p.A[k];
@@ -160,9 +167,32 @@ template <typename R> __device__ __forceinline__ void rsincos(R x, R *s, R *c);
 template <> __device__ __forceinline__ void rsincos<double>(double x, double *s, double *c) { sincos(x, s, c); }
 template <> __device__ __forceinline__ void rsincos<float>(float x, float *s, float *c) { sincosf(x, s, c); }
 
-template <typename R>
+// h(r_ij - r_ik): Tersoff (tersoff_func.f90:168-211) or Kumagai
+// (kumagai_func.f90:179-221, including its beta = 3 branch, which leaves
+// alpha out of the exponent as the reference does)
+template <typename R, int FAM>
 __device__ __forceinline__ void h_fn(const TersoffDev &P, int ikpot, R dr, R &val, R &dval)
 {
+    if (FAM == 1) {
+        const R alpha = (R)P.kalpha[ikpot];
+        if (alpha == (R)0) {
+            val = (R)1;
+            dval = (R)0;
+            return;
+        }
+        const int beta = P.kbeta[ikpot];
+        if (beta == 1) {
+            val = rexp<R>(alpha * dr);
+            dval = alpha * val;
+        } else if (beta == 3) {
+            val = rexp<R>(dr * dr * dr);
+            dval = (R)3 * alpha * dr * dr * val;
+        } else {
+            val = rexp<R>(alpha * rpow<R>(dr, (R)beta));
+            dval = (R)beta * alpha * rpow<R>(dr, (R)(beta - 1)) * val;
+        }
+        return;
+    }
     R mu = (R)P.mubo[ikpot];
     if (mu == (R)0) {
         val = (R)1;
@@ -183,7 +213,60 @@ __device__ __forceinline__ void h_fn(const TersoffDev &P, int ikpot, R dr, R &va
     }
 }
 
-template <typename R, int G>
+// g(cos theta) and dg/dcos.  Tersoff (tersoff_func.f90:86-114) in the
+// cancellation-free form omega (1 + c^2 t^2 / (d^2 (d^2 + t^2))), t = h - cos;
+// Kumagai (kumagai_func.f90:86-131): c1 + t go (1 + ga1), go = c2 t / (c3 +
+// t^2), ga1 = c4 exp(-c5 t^2), all of the centre element.
+template <typename R, int FAM>
+__device__ __forceinline__ void g_fn(const TersoffDev &P, int eli, int ikpot, R costh, R &g, R &dg)
+{
+    if (FAM == 1) {
+        const R t = (R)P.h[eli] - costh, tt = t * t;
+        const R tmp = t / ((R)P.kc3[eli] + tt);
+        const R go = (R)P.kc2[eli] * tmp;
+        const R ga1 = (R)P.kc4[eli] * rexp<R>(-(R)P.kc5[eli] * tt);
+        const R v = go * ((R)1 + ga1);
+        dg = (R)-2 * ((R)1 - t * tmp) * v + (R)2 * (R)P.kc5[eli] * tt * go * ga1;
+        g = (R)P.kc1[eli] + t * v;
+        return;
+    }
+    const R c2 = (R)P.c2[eli], d2 = (R)P.d2[eli];
+    const R h_c = (R)P.h[eli] - costh;
+    const R hh = d2 + h_c * h_c;
+    const R om = (R)P.omega[ikpot];
+    g = om * ((R)1 + c2 * h_c * h_c / (d2 * hh));
+    dg = (R)-2 * om * c2 * h_c / (hh * hh);
+}
+
+// bond order b_ij and Pz = 1/2 fc V_A db/dzeta (V_A already carries the mask
+// factor): Tersoff xi (1 + (beta zeta)^n)^(-1/2n) (tersoff_func.f90:122-160),
+// Kumagai (1 + zeta^eta)^(-delta) (kumagai_func.f90:139-171)
+template <typename R, int FAM>
+__device__ __forceinline__ void bo_fn(const TersoffDev &P, int eli, int typ, R zij, R fc, R VA, R &bij, R &Pz)
+{
+    bij = (R)1;
+    Pz = (R)0;
+    if (!(zij > (R)0)) return;
+    if (FAM == 1) {
+        const R eta = (R)P.keta[eli], delta = (R)P.kdelta[eli];
+        const R zn1 = rpow<R>(zij, eta - (R)1);
+        const R arg = (R)1 + zn1 * zij;
+        const R ad1 = rpow<R>(arg, -delta - (R)1);
+        bij = ad1 * arg;
+        Pz = (R)0.5 * fc * VA * eta * zn1 * (-delta) * ad1;
+        return;
+    }
+    R nn = (R)P.n[eli];
+    R e = (R)P.e[eli];
+    R b = (R)P.bn[eli];
+    R zn1 = rpow<R>(zij, nn - (R)1);
+    R arg = (R)1 + b * zn1 * zij;
+    R ae1 = rpow<R>(arg, e - (R)1);
+    bij = (R)P.xi[typ] * ae1 * arg;
+    Pz = (R)-0.25 * fc * VA * (R)P.xi[typ] * b * zn1 * ae1;
+}
+
+template <typename R, int G, int FAM>
 __global__ void __launch_bounds__(256)
 k_tersoff(TersoffDev P, int nat, const int *__restrict__ order, const double *__restrict__ r,
           const int *__restrict__ el, CellArg C, const int *__restrict__ offsets,
@@ -260,9 +343,7 @@ k_tersoff(TersoffDev P, int nat, const int *__restrict__ order, const double *__
     const bool on = act && ok && maskfac > 0;
 
     // ---- pass 1: zeta_ij ----
-    const R hi = (R)P.h[eli < 0 ? 0 : eli];
-    const R c2 = (R)P.c2[eli < 0 ? 0 : eli];
-    const R d2 = (R)P.d2[eli < 0 ? 0 : eli];
+    const int elc = eli < 0 ? 0 : eli;
     R zij = (R)0;
     // the k loops stop at the wavefront's largest bond count (uniform, so the
     // group shuffles below always see their whole group)
@@ -277,12 +358,10 @@ k_tersoff(TersoffDev P, int nat, const int *__restrict__ order, const double *__
         if (on && k != lane && k < nb && tpk >= 0) {
             R rnk[3] = {s_rn[0][tk], s_rn[1][tk], s_rn[2][tk]};
             R costh = rnk[0] * rn[0] + rnk[1] * rn[1] + rnk[2] * rn[2];
-            R h_c = hi - costh;
-            R hh = d2 + h_c * h_c;
-            // c^2/d^2 - c^2/(d^2 + t^2) = c^2 t^2 / (d^2 (d^2 + t^2)): no cancellation
-            R g = (R)P.omega[tpk] * ((R)1 + c2 * h_c * h_c / (d2 * hh));
+            R g, dg;
+            g_fn<R, FAM>(P, elc, tpk, costh, g, dg);
             R hv, dh;
-            h_fn<R>(P, tpk, rl - s_rl[tk], hv, dh);
+            h_fn<R, FAM>(P, tpk, rl - s_rl[tk], hv, dh);
             zij += s_fc[tk] * g * hv;
         }
     }
@@ -301,16 +380,7 @@ k_tersoff(TersoffDev P, int nat, const int *__restrict__ order, const double *__
         dVA *= mf;
         VR *= mf;
         dVR *= mf;
-        if (zij > (R)0) {
-            R nn = (R)P.n[eli];
-            R e = (R)P.e[eli];
-            R b = (R)P.bn[eli];
-            R zn1 = rpow<R>(zij, nn - (R)1);
-            R arg = (R)1 + b * zn1 * zij;
-            R ae1 = rpow<R>(arg, e - (R)1);
-            bij = (R)P.xi[typ] * ae1 * arg;
-            Pz = (R)-0.25 * fc * VA * (R)P.xi[typ] * b * zn1 * ae1;
-        }
+        bo_fn<R, FAM>(P, eli, typ, zij, fc, VA, bij, Pz);
     }
 
     // ---- pass 2: derivatives; forces on k reduced into lane k ----
@@ -331,13 +401,10 @@ k_tersoff(TersoffDev P, int nat, const int *__restrict__ order, const double *__
             R rnk[3] = {s_rn[0][tk], s_rn[1][tk], s_rn[2][tk]};
             R rlik = s_rl[tk], fcik = s_fc[tk], dfcik = s_dfc[tk];
             R hv, dh;
-            h_fn<R>(P, tpk, rl - rlik, hv, dh);
+            h_fn<R, FAM>(P, tpk, rl - rlik, hv, dh);
             R costh = rnk[0] * rn[0] + rnk[1] * rn[1] + rnk[2] * rn[2];
-            R h_c = hi - costh;
-            R hh = d2 + h_c * h_c;
-            R om = (R)P.omega[tpk];
-            R g = om * ((R)1 + c2 * h_c * h_c / (d2 * hh));
-            R dg = (R)-2 * om * c2 * h_c / (hh * hh);
+            R g, dg;
+            g_fn<R, FAM>(P, elc, tpk, costh, g, dg);
             R dkc[3];
             for (int x = 0; x < 3; ++x) dkc[x] = rnk[x] * rlik - rn[x] * rl;
             R disjk = rsqrt_<R>(dkc[0] * dkc[0] + dkc[1] * dkc[1] + dkc[2] * dkc[2]);
@@ -550,10 +617,16 @@ static int launch_tersoff(TersoffState *st, int nat, const int *order, const dou
                           const CellArg &C, agpu_neighbors *nl, const int *mask, double *eslot,
                           double *wslot, int nblk, hipStream_t s)
 {
-    AGPU_LAUNCH(sizeof(R) == 8 ? "tersoff_fp64" : "tersoff_mixed", (k_tersoff<R, G>), dim3(nblk), dim3(256), 0,
-                s, st->dev, nat, order, r, el, C, nl->offsets.as<int>(), nl->nbr.as<int2>(),
-                st->bslot.as<int>(), st->bcnt.as<int>(), mask, st->fslot.as<double>(),
-                st->fown.as<double>(), eslot, wslot, st->partial.as<double>());
+    if (st->dev.fam == 1)
+        AGPU_LAUNCH(sizeof(R) == 8 ? "kumagai_fp64" : "kumagai_mixed", (k_tersoff<R, G, 1>), dim3(nblk), dim3(256),
+                    0, s, st->dev, nat, order, r, el, C, nl->offsets.as<int>(), nl->nbr.as<int2>(),
+                    st->bslot.as<int>(), st->bcnt.as<int>(), mask, st->fslot.as<double>(),
+                    st->fown.as<double>(), eslot, wslot, st->partial.as<double>());
+    else
+        AGPU_LAUNCH(sizeof(R) == 8 ? "tersoff_fp64" : "tersoff_mixed", (k_tersoff<R, G, 0>), dim3(nblk), dim3(256),
+                    0, s, st->dev, nat, order, r, el, C, nl->offsets.as<int>(), nl->nbr.as<int2>(),
+                    st->bslot.as<int>(), st->bcnt.as<int>(), mask, st->fslot.as<double>(),
+                    st->fown.as<double>(), eslot, wslot, st->partial.as<double>());
     return AGPU_OK;
 }
 
@@ -703,5 +776,83 @@ extern "C" int agpu_tersoff_set_params(agpu_potential *pot, const agpu_tersoff_p
     for (int k = 0; k < st->prm.nel * (st->prm.nel + 1) / 2; ++k)
         if (st->prm.r2[k] > pot->cutoff) pot->cutoff = st->prm.r2[k];
     if (pot->bound) return tersoff_bind(pot);
+    return AGPU_OK;
+}
+
+// Kumagai_CompMaterSci_39_457_Si (kumagai_params.f90:97-132)
+static agpu_kumagai_params kumagai_default()
+{
+    agpu_kumagai_params k;
+    k.Z = 14;
+    k.A = 3281.5905;
+    k.B = 121.00047;
+    k.lambda1 = 3.2300135;
+    k.lambda2 = 1.3457970;
+    k.eta = 1.0;
+    k.delta = 0.53298909;
+    k.alpha = 2.3890327;
+    k.beta = 1;
+    k.c1 = 0.20173476;
+    k.c2 = 730418.72;
+    k.c3 = 1000000.0;
+    k.c4 = 1.0;
+    k.c5 = 26.0;
+    k.h = -0.36500000;
+    k.r1 = 2.70;
+    k.r2 = 3.30;
+    return k;
+}
+
+// Kumagai (INIT_FUNC, kumagai_module.f90:30-78): one element, the generic bond-
+// order kernel with the Kumagai g, h and bond-order forms.
+extern "C" int agpu_kumagai_create(agpu_potential **pot, const agpu_kumagai_params *params)
+{
+    if (!pot) return set_error(AGPU_ERR_ARG, "pot is NULL");
+    const agpu_kumagai_params k = params ? *params : kumagai_default();
+    if (!(k.r2 > k.r1 && k.r1 > 0.0) || k.Z <= 0 || k.Z >= 128)
+        return set_error(AGPU_ERR_ARG, "Kumagai: need 0 < r1 < r2 and a valid element");
+    TersoffState *st = new TersoffState();
+    agpu_tersoff_params &p = st->prm;
+    std::memset(&p, 0, sizeof p);
+    p.nel = 1;
+    p.Z[0] = k.Z;
+    for (int q = 0; q < 6; ++q) {
+        p.A[q] = k.A;
+        p.B[q] = k.B;
+        p.lambda[q] = k.lambda1;
+        p.mu[q] = k.lambda2;
+        p.xi[q] = 1.0;
+        p.omega[q] = 1.0;
+        p.mubo[q] = 0.0;
+        p.m[q] = 1;
+        p.r1[q] = k.r1;
+        p.r2[q] = k.r2;
+    }
+    for (int q = 0; q < 3; ++q) {
+        p.beta[q] = p.n[q] = p.c[q] = p.d[q] = 1.0;
+        p.h[q] = k.h;
+    }
+    derive(st->dev, p);
+    TersoffDev &D = st->dev;
+    D.fam = 1;
+    for (int q = 0; q < 3; ++q) {
+        D.kc1[q] = k.c1;
+        D.kc2[q] = k.c2;
+        D.kc3[q] = k.c3;
+        D.kc4[q] = k.c4;
+        D.kc5[q] = k.c5;
+        D.keta[q] = k.eta;
+        D.kdelta[q] = k.delta;
+    }
+    for (int q = 0; q < 6; ++q) {
+        D.kalpha[q] = k.alpha;
+        D.kbeta[q] = k.beta;
+    }
+    agpu_potential *P = new agpu_potential();
+    P->kind = AGPU_KIND_KUMAGAI;
+    P->cutoff = k.r2;
+    P->impl = st;
+    P->free_impl = tersoff_free;
+    *pot = P;
     return AGPU_OK;
 }

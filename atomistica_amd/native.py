@@ -297,6 +297,31 @@ class Tersoff(_Potential):
         call("agpu_tersoff_set_params", self._h, ctypes.byref(s))
 
 
+class Kumagai(_Potential):
+    """Kumagai (kumagai.f90, classname Kumagai): the generic bond-order
+    kernel with the Kumagai forms.  Keyword arguments are the names of
+    kumagai_params.f90 (A, B, lambda1, lambda2, eta, delta, alpha, beta, c1
+    .. c5, h, r1, r2, el); without arguments Kumagai_CompMaterSci_39_457_Si."""
+    kind = _lib.KIND_KUMAGAI
+
+    def __init__(self, **kwargs):
+        super().__init__()
+        from .parameters import ATOMIC_NUMBERS, Kumagai_CompMaterSci_39_457_Si
+        params = dict(Kumagai_CompMaterSci_39_457_Si)
+        params.update(kwargs)
+        self._params = params
+        s = _lib.KumagaiParams()
+        el = params["el"][0] if isinstance(params["el"], (list, tuple)) else params["el"]
+        s.Z = ATOMIC_NUMBERS[el]
+        for k in ("A", "B", "lambda1", "lambda2", "eta", "delta", "alpha", "c1", "c2", "c3", "c4", "c5", "h",
+                  "r1", "r2"):
+            setattr(s, k, float(params[k]))
+        s.beta = int(params["beta"])
+        h = ctypes.c_void_p()
+        call("agpu_kumagai_create", ctypes.byref(h), ctypes.byref(s))
+        self._h = h
+
+
 class TabulatedAlloyEAM(_Potential):
     """TabulatedAlloyEAM (tabulated_alloy_eam.f90): fn = setfl file,
     elements = filter string."""

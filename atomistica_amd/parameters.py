@@ -58,6 +58,30 @@ Tersoff_Si = {
     "r2": [3.00],
 }
 
+# Kumagai, Izumi, Hara, Sakai, Comp. Mater. Sci. 39, 457 (2007), Si
+# (kumagai_params.f90:97-132; the names there, one element).
+Kumagai_CompMaterSci_39_457_Si = {
+    "__ref__": "Kumagai, Izumi, Hara, Sakai, Comp. Mater. Sci. 39, 457 (2007)",
+    "family": "kumagai",
+    "el": ["Si"],
+    "A": 3281.5905,
+    "B": 121.00047,
+    "lambda1": 3.2300135,
+    "lambda2": 1.3457970,
+    "eta": 1.0000000,
+    "delta": 0.53298909,
+    "alpha": 2.3890327,
+    "beta": 1,
+    "c1": 0.20173476,
+    "c2": 730418.72,
+    "c3": 1000000.0,
+    "c4": 1.0000000,
+    "c5": 26.000000,
+    "h": -0.36500000,
+    "r1": 2.70,
+    "r2": 3.30,
+}
+
 # The eleven scalar Tersoff parameters trained by the parameter-gradient path,
 # in the order used throughout (theta vector of a single-element set).
 TERSOFF_THETA_NAMES = ("A", "B", "lambda", "mu", "beta", "n", "c", "d", "h", "r1", "r2")

@@ -61,6 +61,7 @@ extern "C" {
 #define AGPU_KIND_EAM_ALLOY 2
 #define AGPU_KIND_EWALD 3
 #define AGPU_KIND_REBO2 4
+#define AGPU_KIND_KUMAGAI 5
 
 /* agpu_rebo2_create flags */
 #define AGPU_REBO2_ZERO_TABLES 1  /* rebo2 zero_tables = .true. (rebo2_module.f90:30-42) */
@@ -173,6 +174,23 @@ typedef struct agpu_tersoff_params {
  * Tersoff_PRB_39_5566_SiC (tersoff_params.f90:99-138). */
 int agpu_tersoff_create(agpu_potential **pot, const agpu_tersoff_params *params);
 int agpu_tersoff_set_params(agpu_potential *pot, const agpu_tersoff_params *params);
+
+/* Kumagai-Izumi-Hara-Sakai bond-order potential (one element), the
+ * parameters of kumagai_params.f90:34-94 with the pair/element arrays reduced
+ * to one entry.  INIT_FUNC (kumagai_module.f90:30-78) over the generic bond-order
+ * kernel (bop_kernel.f90:563-1529) with the forms of kumagai_func.f90:32-221:
+ * V_R = A exp(-lambda1 r), V_A = -B exp(-lambda2 r),
+ * g = c1 + t c2 t/(c3 + t^2) (1 + c4 exp(-c5 t^2)), t = h - cos,
+ * b = (1 + zeta^eta)^(-delta), h(dr) = exp(alpha dr^beta), trig_off cut-off
+ * on [r1, r2].  params == NULL selects Kumagai_CompMaterSci_39_457_Si
+ * (kumagai_params.f90:97-132). */
+typedef struct agpu_kumagai_params {
+    int Z;
+    double A, B, lambda1, lambda2, eta, delta, alpha;
+    int beta;
+    double c1, c2, c3, c4, c5, h, r1, r2;
+} agpu_kumagai_params;
+int agpu_kumagai_create(agpu_potential **pot, const agpu_kumagai_params *params);
 
 /* TabulatedAlloyEAM: tabulated_alloy_eam_init (tabulated_alloy_eam.f90:147-259)
  * reading a setfl file; elements = filter string ("*" or NULL = all). */

@@ -133,7 +133,10 @@ class _TersoffParams(ctypes.Structure):
         [(k, ctypes.c_double * 6) for k in ("A", "B", "xi", "lambda_", "mu", "omega", "mubo")] + \
         [("m", ctypes.c_int * 6)] + \
         [(k, ctypes.c_double * 3) for k in ("beta", "n", "c", "d", "h")] + \
-        [(k, ctypes.c_double * 6) for k in ("r1", "r2")]
+        [(k, ctypes.c_double * 6) for k in ("r1", "r2")] + \
+        [("fam", ctypes.c_int)] + \
+        [(k, ctypes.c_double * 3) for k in ("kc1", "kc2", "kc3", "kc4", "kc5", "keta", "kdelta")] + \
+        [("kalpha", ctypes.c_double * 6), ("kbeta", ctypes.c_int * 6)]
 
 
 def _tersoff_struct(params, Zs):
@@ -161,8 +164,33 @@ def _tersoff_struct(params, Zs):
     return s
 
 
+def _kumagai_struct(k):
+    """Kumagai parameters (one element; names of kumagai_params.f90) in the
+    shared bond-order parameter block with fam = 1."""
+    s = _TersoffParams()
+    s.nel = 1
+    s.Z[0] = 14
+    for i in range(6):
+        s.A[i], s.B[i] = k["A"], k["B"]
+        s.lambda_[i], s.mu[i] = k["lambda1"], k["lambda2"]
+        s.xi[i] = s.omega[i] = 1.0
+        s.mubo[i] = 0.0
+        s.m[i] = 1
+        s.r1[i], s.r2[i] = k["r1"], k["r2"]
+        s.kalpha[i], s.kbeta[i] = k["alpha"], int(k["beta"])
+    for i in range(3):
+        s.beta[i] = s.n[i] = s.c[i] = s.d[i] = 1.0
+        s.h[i] = k["h"]
+        s.kc1[i], s.kc2[i], s.kc3[i], s.kc4[i], s.kc5[i] = k["c1"], k["c2"], k["c3"], k["c4"], k["c5"]
+        s.keta[i], s.kdelta[i] = k["eta"], k["delta"]
+    s.fam = 1
+    return s
+
+
 def tersoff(params, r, el, cell, nl, mask=None, per_at=False, nthreads=1):
-    """Energy, forces and virial; ``el`` holds database element indices (-1 = none)."""
+    """Energy, forces and virial; ``el`` holds database element indices (-1 = none).
+    ``params`` is a Tersoff parameter dict, or a Kumagai one (``"family":
+    "kumagai"``) for the Kumagai forms of the same kernel."""
     r = _c(r, np.float64)
     nat = len(r)
     el = _c(el, np.int32)
@@ -171,7 +199,7 @@ def tersoff(params, r, el, cell, nl, mask=None, per_at=False, nthreads=1):
     offsets = _c(offsets, np.int64)
     nbr = _c(nbr, np.int32)
     dc = _c(dc, np.int32)
-    s = _tersoff_struct(params, [0, 0, 0])
+    s = _kumagai_struct(params) if params.get("family") == "kumagai" else _tersoff_struct(params, [0, 0, 0])
     epot = ctypes.c_double(0.0)
     f = np.zeros((nat, 3))
     w = np.zeros(9)

@@ -50,6 +50,12 @@ typedef struct {
     int m[6];
     double beta[3], n[3], c[3], d[3], h[3];
     double r1[6], r2[6];
+    /* fam = 1: the Kumagai forms (kumagai_func.f90) with A, B, lambda
+     * (= lambda1), mu (= lambda2), h, r1, r2 from above */
+    int fam;
+    double kc1[3], kc2[3], kc3[3], kc4[3], kc5[3], keta[3], kdelta[3];
+    double kalpha[6];
+    int kbeta[6];
 } orc_tersoff_params;
 
 int orc_tersoff(const orc_tersoff_params *p, int nat, const double *r,

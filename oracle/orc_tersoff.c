@@ -11,6 +11,8 @@
  *   - reduction epot += 1/2 sum(pe), epot_per_at = pe/2
  *       src/potentials/bop/bop_kernel.f90:1613-1628
  *   - VA, VR, g, bo, h    src/potentials/bop/tersoff/tersoff_func.f90:32-211
+ *   - fam = 1: g, bo, h of src/potentials/bop/kumagai/kumagai_func.f90:86-221
+ *     (VA, VR have the Tersoff form, :32-78)
  *   - PAIR_INDEX          src/macros.inc:123
  *   - element map / nebmax src/potentials/bop/default_compute_func.f90:62-73
  * The OpenMP split (per-thread force/pe buffers summed in thread order) is the
@@ -56,6 +58,17 @@ static void trig_off(double r1, double r2, double r, double *val, double *dval)
 
 static void h_fn(const orc_tersoff_params *p, int ikpot, double dr, double *val, double *dval)
 {
+    if (p->fam == 1) {
+        /* kumagai_func.f90:179-221 (the beta = 3 branch has no alpha in the
+         * exponent, as written there) */
+        double alpha = p->kalpha[ikpot];
+        if (alpha == 0.0) { *val = 1.0; *dval = 0.0; return; }
+        int beta = p->kbeta[ikpot];
+        if (beta == 1) { *val = exp(alpha * dr); *dval = alpha * *val; }
+        else if (beta == 3) { *val = exp(dr * dr * dr); *dval = 3 * alpha * dr * dr * *val; }
+        else { *val = exp(alpha * pow(dr, beta)); *dval = beta * alpha * pow(dr, beta - 1) * *val; }
+        return;
+    }
     double mu = p->mubo[ikpot];
     if (mu == 0.0) { *val = 1.0; *dval = 0.0; return; }
     int m = p->m[ikpot];
@@ -186,14 +199,27 @@ int orc_tersoff(const orc_tersoff_params *p, int nat, const double *r,
                     double h_Dr, dh_dDr;
                     h_fn(p, ikpot, rlij - rlik, &h_Dr, &dh_dDr);
                     double costh = rnik[0] * rnij[0] + rnik[1] * rnij[1] + rnik[2] * rnij[2];
-                    /* g(theta): c, d, h of the centre atom, omega of ik */
-                    double omega = p->omega[ikpot];
-                    double h_c = p->h[eli] - costh;
-                    double c_sq = p->c[eli] * p->c[eli];
-                    double d_sq = p->d[eli] * p->d[eli];
-                    double hh = d_sq + h_c * h_c;
-                    double g_costh = omega * (1.0 + c_sq / d_sq - c_sq / hh);
-                    double dg_dcosth = -2 * omega * c_sq * h_c / (hh * hh);
+                    double g_costh, dg_dcosth;
+                    if (p->fam == 1) {
+                        /* kumagai_func.f90:86-131, parameters of the centre atom */
+                        double h_cos = p->h[eli] - costh;
+                        double h_cos_sq = h_cos * h_cos;
+                        double tmp = h_cos / (p->kc3[eli] + h_cos_sq);
+                        double go = p->kc2[eli] * tmp;
+                        double ga1 = p->kc4[eli] * exp(-p->kc5[eli] * h_cos_sq);
+                        double v = go * (1.0 + ga1);
+                        dg_dcosth = -2 * (1.0 - h_cos * tmp) * v + 2 * p->kc5[eli] * h_cos_sq * go * ga1;
+                        g_costh = p->kc1[eli] + h_cos * v;
+                    } else {
+                        /* g(theta): c, d, h of the centre atom, omega of ik */
+                        double omega = p->omega[ikpot];
+                        double h_c = p->h[eli] - costh;
+                        double c_sq = p->c[eli] * p->c[eli];
+                        double d_sq = p->d[eli] * p->d[eli];
+                        double hh = d_sq + h_c * h_c;
+                        g_costh = omega * (1.0 + c_sq / d_sq - c_sq / hh);
+                        dg_dcosth = -2 * omega * c_sq * h_c / (hh * hh);
+                    }
                     double dkc[3];
                     for (int x = 0; x < 3; ++x) dkc[x] = rnik[x] * rlik - rnij[x] * rlij;
                     double disjk = sqrt(dkc[0] * dkc[0] + dkc[1] * dkc[1] + dkc[2] * dkc[2]);
@@ -223,7 +249,13 @@ int orc_tersoff(const orc_tersoff_params *p, int nat, const double *r,
                 int numnbi = ikc;
                 /* bo: b_ij and dE/dzeta */
                 double bij = 1.0, dbij_dzij = 0.0;
-                if (zij > 0.0) {
+                if (zij > 0.0 && p->fam == 1) {
+                    /* kumagai_func.f90:139-171 */
+                    double eta = p->keta[eli], delta = -p->kdelta[eli];
+                    double arg = 1.0 + pow(zij, eta);
+                    bij = pow(arg, delta);
+                    dbij_dzij = 0.5 * fcarij * VAij * eta * pow(zij, eta - 1.0) * delta * pow(arg, delta - 1.0);
+                } else if (zij > 0.0) {
                     double nn = p->n[eli];
                     double e = -0.5 / nn;
                     double b = pow(p->beta[eli], nn);
