@@ -31,6 +31,7 @@ KIND_EAM_ALLOY = 2
 KIND_EWALD = 3
 KIND_REBO2 = 4
 KIND_KUMAGAI = 5
+KIND_BRENNER = 6
 REBO2_ZERO_TABLES = 1
 REBO2_DIHEDRAL = 2
 
@@ -69,6 +70,13 @@ class KumagaiParams(ctypes.Structure):
         [(k, ctypes.c_double) for k in ("c1", "c2", "c3", "c4", "c5", "h", "r1", "r2")]
 
 
+class BrennerParams(ctypes.Structure):
+    _fields_ = [("nel", ctypes.c_int), ("Z", ctypes.c_int * 3)] + \
+        [(k, ctypes.c_double * 6) for k in ("D0", "r0", "S", "beta", "gamma", "c", "d", "h", "mu", "n")] + \
+        [("m", ctypes.c_int * 6)] + \
+        [(k, ctypes.c_double * 6) for k in ("r1", "r2")]
+
+
 # name -> argtypes (every function returns int unless listed in _RESTYPE)
 PROTOTYPES = {
     "agpu_version": [],
@@ -104,6 +112,7 @@ PROTOTYPES = {
     "agpu_tersoff_create": [ctypes.POINTER(vp), ctypes.POINTER(TersoffParams)],
     "agpu_tersoff_set_params": [vp, ctypes.POINTER(TersoffParams)],
     "agpu_kumagai_create": [ctypes.POINTER(vp), ctypes.POINTER(KumagaiParams)],
+    "agpu_brenner_create": [ctypes.POINTER(vp), ctypes.POINTER(BrennerParams)],
     "agpu_eam_alloy_create": [ctypes.POINTER(vp), ctypes.c_char_p, ctypes.c_char_p],
     "agpu_ewald_create": [ctypes.POINTER(vp), d, d, d],
     "agpu_rebo2_create": [ctypes.POINTER(vp), i],

@@ -258,3 +258,4 @@ TabulatedAlloyEAM = _make(native.TabulatedAlloyEAM, "TabulatedAlloyEAM")
 Ewald = _make(native.Ewald, "Ewald")
 Rebo2 = _make(native.Rebo2, "Rebo2")
 Kumagai = _make(native.Kumagai, "Kumagai")
+Brenner = _make(native.Brenner, "Brenner")

@@ -611,7 +611,8 @@ int agpu_potential_bind_to(agpu_potential *pot, agpu_particles *p, agpu_neighbor
     int rc = AGPU_ERR_STATE;
     switch (pot->kind) {
     case AGPU_KIND_TERSOFF:
-    case AGPU_KIND_KUMAGAI: rc = tersoff_bind(pot); break;
+    case AGPU_KIND_KUMAGAI:
+    case AGPU_KIND_BRENNER: rc = tersoff_bind(pot); break;
     case AGPU_KIND_EAM_ALLOY: rc = eam_bind(pot); break;
     case AGPU_KIND_EWALD: rc = ewald_bind(pot); break;
     case AGPU_KIND_REBO2: rc = rebo2_bind(pot); break;
@@ -631,7 +632,8 @@ static int stage_and_call(agpu_potential *pot, agpu_particles *p, agpu_neighbors
     if (!(a.flags & AGPU_HOST)) {
         switch (pot->kind) {
         case AGPU_KIND_TERSOFF:
-        case AGPU_KIND_KUMAGAI: rc = tersoff_compute(pot, &a); break;
+        case AGPU_KIND_KUMAGAI:
+        case AGPU_KIND_BRENNER: rc = tersoff_compute(pot, &a); break;
         case AGPU_KIND_EAM_ALLOY: rc = eam_compute(pot, &a); break;
         case AGPU_KIND_EWALD: rc = ewald_compute(pot, &a); break;
         case AGPU_KIND_REBO2: rc = rebo2_compute(pot, &a); break;
@@ -676,7 +678,8 @@ static int stage_and_call(agpu_potential *pot, agpu_particles *p, agpu_neighbors
     if (md) AGPU_HIP(hipMemcpyAsync(md, a.mask, sizeof(int) * nat, hipMemcpyHostToDevice, s));
     switch (pot->kind) {
     case AGPU_KIND_TERSOFF:
-    case AGPU_KIND_KUMAGAI: rc = tersoff_compute(pot, &d); break;
+    case AGPU_KIND_KUMAGAI:
+    case AGPU_KIND_BRENNER: rc = tersoff_compute(pot, &d); break;
     case AGPU_KIND_EAM_ALLOY: rc = eam_compute(pot, &d); break;
     case AGPU_KIND_EWALD: rc = ewald_compute(pot, &d); break;
     case AGPU_KIND_REBO2: rc = rebo2_compute(pot, &d); break;

@@ -39,10 +39,15 @@ struct TersoffDev {
     double c2[3], d2[3], bn[3], e[3], fac[6];
     // Kumagai family (kumagai_func.f90): g, bond order and h forms; A, B,
     // lambda (= lambda1), mu (= lambda2), h, r1, r2 are shared with the above
-    int fam;                      // 0 Tersoff, 1 Kumagai
+    int fam;                      // 0 Tersoff, 1 Kumagai, 2 Brenner
     double kc1[3], kc2[3], kc3[3], kc4[3], kc5[3], keta[3], kdelta[3];
     double kalpha[6];
     int kbeta[6];
+    // Brenner family (brenner_func.f90, per element pair; h-term mu, m in
+    // mubo, m above): V_R = VR_f exp(-expR (r - r0)), V_A = -VA_f exp(-expA
+    // (r - r0)), g = gamma (1 + c^2/d^2 - c^2/(d^2 + (h + cos)^2)),
+    // b = (1 + zeta^n)^(-1/2n)
+    double br0[6], bVRf[6], bVAf[6], bexpR[6], bexpA[6], bgam[6], bc2[6], bd2[6], bh[6], bnn[6], bboe[6];
 };
 
 struct TersoffState {
@@ -220,6 +225,16 @@ __device__ __forceinline__ void h_fn(const TersoffDev &P, int ikpot, R dr, R &va
 template <typename R, int FAM>
 __device__ __forceinline__ void g_fn(const TersoffDev &P, int eli, int ikpot, R costh, R &g, R &dg)
 {
+    if (FAM == 2) {
+        // brenner_func.f90:77-101 in the cancellation-free form
+        // gamma (1 + c^2 t^2 / (d^2 (d^2 + t^2))), t = h + cos, of the pair ik
+        const R c2 = (R)P.bc2[ikpot], d2 = (R)P.bd2[ikpot], gam = (R)P.bgam[ikpot];
+        const R t = (R)P.bh[ikpot] + costh;
+        const R hh = d2 + t * t;
+        g = gam * ((R)1 + c2 * t * t / (d2 * hh));
+        dg = (R)2 * gam * c2 * t / (hh * hh);
+        return;
+    }
     if (FAM == 1) {
         const R t = (R)P.h[eli] - costh, tt = t * t;
         const R tmp = t / ((R)P.kc3[eli] + tt);
@@ -246,6 +261,23 @@ __device__ __forceinline__ void bo_fn(const TersoffDev &P, int eli, int typ, R z
 {
     bij = (R)1;
     Pz = (R)0;
+    if (FAM == 2) {
+        // brenner_func.f90:107-147: n = 1 needs no zeta > 0 guard
+        const R n = (R)P.bnn[typ], boe = (R)P.bboe[typ];
+        if (n == (R)1) {
+            const R arg = (R)1 + zij;
+            const R a1 = rpow<R>(arg, boe - (R)1);
+            bij = a1 * arg;
+            Pz = (R)0.5 * boe * n * fc * VA * a1;
+        } else if (zij > (R)0) {
+            const R zn1 = rpow<R>(zij, n - (R)1);
+            const R arg = (R)1 + zn1 * zij;
+            const R a1 = rpow<R>(arg, boe - (R)1);
+            bij = a1 * arg;
+            Pz = (R)0.5 * boe * n * fc * VA * zn1 * a1;
+        }
+        return;
+    }
     if (!(zij > (R)0)) return;
     if (FAM == 1) {
         const R eta = (R)P.keta[eli], delta = (R)P.kdelta[eli];
@@ -369,12 +401,23 @@ k_tersoff(TersoffDev P, int nat, const int *__restrict__ order, const double *__
     // ---- bond order ----
     R VA = (R)0, dVA = (R)0, VR = (R)0, dVR = (R)0, bij = (R)1, Pz = (R)0;
     if (on) {
-        R ea = rexp<R>(-(R)P.mu[typ] * rl);
-        VA = -(R)P.B[typ] * ea;
-        dVA = (R)P.B[typ] * (R)P.mu[typ] * ea;
-        R er = rexp<R>(-(R)P.lambda[typ] * rl);
-        VR = (R)P.A[typ] * er;
-        dVR = -(R)P.A[typ] * (R)P.lambda[typ] * er;
+        if (FAM == 2) {
+            // brenner_func.f90:27-71
+            const R dr0 = rl - (R)P.br0[typ];
+            const R ea = rexp<R>(-(R)P.bexpA[typ] * dr0);
+            VA = -(R)P.bVAf[typ] * ea;
+            dVA = (R)P.bVAf[typ] * (R)P.bexpA[typ] * ea;
+            const R er = rexp<R>(-(R)P.bexpR[typ] * dr0);
+            VR = (R)P.bVRf[typ] * er;
+            dVR = -(R)P.bVRf[typ] * (R)P.bexpR[typ] * er;
+        } else {
+            R ea = rexp<R>(-(R)P.mu[typ] * rl);
+            VA = -(R)P.B[typ] * ea;
+            dVA = (R)P.B[typ] * (R)P.mu[typ] * ea;
+            R er = rexp<R>(-(R)P.lambda[typ] * rl);
+            VR = (R)P.A[typ] * er;
+            dVR = -(R)P.A[typ] * (R)P.lambda[typ] * er;
+        }
         R mf = (R)0.5 * (R)maskfac;
         VA *= mf;
         dVA *= mf;
@@ -622,6 +665,11 @@ static int launch_tersoff(TersoffState *st, int nat, const int *order, const dou
                     0, s, st->dev, nat, order, r, el, C, nl->offsets.as<int>(), nl->nbr.as<int2>(),
                     st->bslot.as<int>(), st->bcnt.as<int>(), mask, st->fslot.as<double>(),
                     st->fown.as<double>(), eslot, wslot, st->partial.as<double>());
+    else if (st->dev.fam == 2)
+        AGPU_LAUNCH(sizeof(R) == 8 ? "brenner_fp64" : "brenner_mixed", (k_tersoff<R, G, 2>), dim3(nblk), dim3(256),
+                    0, s, st->dev, nat, order, r, el, C, nl->offsets.as<int>(), nl->nbr.as<int2>(),
+                    st->bslot.as<int>(), st->bcnt.as<int>(), mask, st->fslot.as<double>(),
+                    st->fown.as<double>(), eslot, wslot, st->partial.as<double>());
     else
         AGPU_LAUNCH(sizeof(R) == 8 ? "tersoff_fp64" : "tersoff_mixed", (k_tersoff<R, G, 0>), dim3(nblk), dim3(256),
                     0, s, st->dev, nat, order, r, el, C, nl->offsets.as<int>(), nl->nbr.as<int2>(),
@@ -851,6 +899,96 @@ extern "C" int agpu_kumagai_create(agpu_potential **pot, const agpu_kumagai_para
     agpu_potential *P = new agpu_potential();
     P->kind = AGPU_KIND_KUMAGAI;
     P->cutoff = k.r2;
+    P->impl = st;
+    P->free_impl = tersoff_free;
+    *pot = P;
+    return AGPU_OK;
+}
+
+// Erhart_PRB_71_035211_SiC without screening (brenner_params.f90:78-113)
+static agpu_brenner_params brenner_default()
+{
+    agpu_brenner_params b;
+    std::memset(&b, 0, sizeof b);
+    b.nel = 2;
+    b.Z[0] = 6;
+    b.Z[1] = 14;
+    const double D0[3] = {6.00, 4.36, 3.24}, r0[3] = {1.4276, 1.79, 2.232}, S[3] = {2.167, 1.847, 1.842},
+                 beta[3] = {2.0099, 1.6991, 1.4761}, gam[3] = {0.11233, 0.011877, 0.114354},
+                 c[3] = {181.910, 273987.0, 2.00494}, d[3] = {6.28433, 180.314, 0.81472}, h[3] = {0.5556, 0.68, 0.259},
+                 r1[3] = {1.85, 2.20, 2.68}, r2[3] = {2.15, 2.60, 2.96};
+    for (int q = 0; q < 3; ++q) {
+        b.D0[q] = D0[q];
+        b.r0[q] = r0[q];
+        b.S[q] = S[q];
+        b.beta[q] = beta[q];
+        b.gamma[q] = gam[q];
+        b.c[q] = c[q];
+        b.d[q] = d[q];
+        b.h[q] = h[q];
+        b.mu[q] = 0.0;
+        b.n[q] = 1.0;
+        b.m[q] = 1;
+        b.r1[q] = r1[q];
+        b.r2[q] = r2[q];
+    }
+    return b;
+}
+
+// Brenner (INIT_FUNC and the constants BIND_TO_FUNC derives,
+// brenner_module.f90:27-152, 267-296): up to three elements, every
+// parameter per element pair, the generic bond-order kernel with the Brenner
+// forms.
+extern "C" int agpu_brenner_create(agpu_potential **pot, const agpu_brenner_params *params)
+{
+    if (!pot) return set_error(AGPU_ERR_ARG, "pot is NULL");
+    const agpu_brenner_params b = params ? *params : brenner_default();
+    if (b.nel < 1 || b.nel > 3) return set_error(AGPU_ERR_ARG, "Brenner: 1 to 3 elements supported");
+    const int npairs = b.nel * (b.nel + 1) / 2;
+    for (int q = 0; q < npairs; ++q) {
+        if (b.d[q] == 0.0)
+            return set_error(AGPU_ERR_ARG, "d = 0! This leads to problems computing c**2/d**2. Please specify d != 0.");
+        if (b.S[q] <= 1.0)
+            return set_error(AGPU_ERR_ARG, "S <= 1! This leads to problems computing (S-1)**(-1). Please specify S > 1.");
+        if (!(b.r2[q] > b.r1[q])) return set_error(AGPU_ERR_ARG, "Brenner: need r1 < r2 for every pair");
+    }
+    TersoffState *st = new TersoffState();
+    agpu_tersoff_params &p = st->prm;
+    std::memset(&p, 0, sizeof p);
+    p.nel = b.nel;
+    for (int e = 0; e < 3; ++e) p.Z[e] = b.Z[e];
+    for (int q = 0; q < 6; ++q) {
+        const bool in = q < npairs;
+        p.A[q] = p.B[q] = p.lambda[q] = p.mu[q] = 1.0;
+        p.xi[q] = p.omega[q] = 1.0;
+        p.mubo[q] = in ? b.mu[q] : 0.0;
+        p.m[q] = in ? b.m[q] : 1;
+        p.r1[q] = in ? b.r1[q] : 1.0;
+        p.r2[q] = in ? b.r2[q] : 2.0;
+    }
+    for (int q = 0; q < 3; ++q) p.beta[q] = p.n[q] = p.c[q] = p.d[q] = p.h[q] = 1.0;
+    derive(st->dev, p);
+    TersoffDev &D = st->dev;
+    D.fam = 2;
+    for (int q = 0; q < 6; ++q) {
+        const bool in = q < npairs;
+        const double S = in ? b.S[q] : 2.0, n = in ? b.n[q] : 1.0, beta = in ? b.beta[q] : 1.0;
+        D.br0[q] = in ? b.r0[q] : 0.0;
+        D.bVRf[q] = in ? b.D0[q] / (S - 1.0) : 0.0;
+        D.bVAf[q] = in ? S * b.D0[q] / (S - 1.0) : 0.0;
+        D.bexpR[q] = beta * std::sqrt(2.0 * S);
+        D.bexpA[q] = beta * std::sqrt(2.0 / S);
+        D.bgam[q] = in ? b.gamma[q] : 0.0;
+        D.bc2[q] = in ? b.c[q] * b.c[q] : 0.0;
+        D.bd2[q] = in ? b.d[q] * b.d[q] : 1.0;
+        D.bh[q] = in ? b.h[q] : 0.0;
+        D.bnn[q] = n;
+        D.bboe[q] = -0.5 / n;
+    }
+    agpu_potential *P = new agpu_potential();
+    P->kind = AGPU_KIND_BRENNER;
+    for (int q = 0; q < npairs; ++q)
+        if (b.r2[q] > P->cutoff) P->cutoff = b.r2[q];
     P->impl = st;
     P->free_impl = tersoff_free;
     *pot = P;

@@ -322,6 +322,40 @@ class Kumagai(_Potential):
         self._h = h
 
 
+class Brenner(_Potential):
+    """Brenner (brenner.f90, classname Brenner): the generic bond-order
+    kernel with the Brenner (Albe/Erhart) forms.  Keyword arguments are the
+    names of brenner_params.f90 (el, D0, r0, S, beta, gamma, c, d, h, mu, n,
+    m, r1, r2; one value per element pair); without arguments
+    Erhart_PRB_71_035211_SiC."""
+    kind = _lib.KIND_BRENNER
+
+    def __init__(self, **kwargs):
+        super().__init__()
+        from .parameters import ATOMIC_NUMBERS, Erhart_PRB_71_035211_SiC
+        params = dict(Erhart_PRB_71_035211_SiC)
+        params.update(kwargs)
+        self._params = params
+        nel = len(params["el"])
+        npairs = nel * (nel + 1) // 2
+        s = _lib.BrennerParams()
+        s.nel = nel
+        for i, e in enumerate(params["el"]):
+            s.Z[i] = ATOMIC_NUMBERS[e]
+        for k in ("D0", "r0", "S", "beta", "gamma", "c", "d", "h", "mu", "n", "r1", "r2"):
+            vals = params[k]
+            if len(vals) < npairs:
+                raise ValueError("Brenner: %s needs one value per element pair (%d)" % (k, npairs))
+            arr = getattr(s, k)
+            for i in range(npairs):
+                arr[i] = float(vals[i])
+        for i in range(npairs):
+            s.m[i] = int(params["m"][i])
+        h = ctypes.c_void_p()
+        call("agpu_brenner_create", ctypes.byref(h), ctypes.byref(s))
+        self._h = h
+
+
 class TabulatedAlloyEAM(_Potential):
     """TabulatedAlloyEAM (tabulated_alloy_eam.f90): fn = setfl file,
     elements = filter string."""

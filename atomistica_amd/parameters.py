@@ -82,6 +82,84 @@ Kumagai_CompMaterSci_39_457_Si = {
     "r2": 3.30,
 }
 
+# Brenner-form (Albe/Erhart) sets without screening, every parameter per
+# element pair in PAIR_INDEX order (brenner_params.f90:78-185 and the
+# reference's Python parameters.py for Brenner's 1990 carbon sets).
+Erhart_PRB_71_035211_SiC = {
+    "__ref__": "Erhart P., Albe K., Phys. Rev. B 71, 035211 (2005)",
+    "family": "brenner",
+    "el": ["C", "Si"],
+    "D0": [6.00, 4.36, 3.24],
+    "r0": [1.4276, 1.79, 2.232],
+    "S": [2.167, 1.847, 1.842],
+    "beta": [2.0099, 1.6991, 1.4761],
+    "gamma": [0.11233, 0.011877, 0.114354],
+    "c": [181.910, 273987.0, 2.00494],
+    "d": [6.28433, 180.314, 0.81472],
+    "h": [0.5556, 0.68, 0.259],
+    "mu": [0.0, 0.0, 0.0],
+    "n": [1.0, 1.0, 1.0],
+    "m": [1, 1, 1],
+    "r1": [1.85, 2.20, 2.68],
+    "r2": [2.15, 2.60, 2.96],
+}
+
+Albe_PRB_65_195124_PtC = {
+    "__ref__": "Albe K., Nordlund K., Averback R. S., Phys. Rev. B 65, 195124 (2002)",
+    "family": "brenner",
+    "el": ["Pt", "C"],
+    "D0": [3.683, 5.3, 6.0],
+    "r0": [2.384, 1.84, 1.39],
+    "S": [2.24297, 1.1965, 1.22],
+    "beta": [1.64249, 1.836, 2.1],
+    "gamma": [0.0008542, 0.0097, 0.00020813],
+    "c": [34.0, 1.23, 330.0],
+    "d": [1.1, 0.36, 3.5],
+    "h": [1.0, 1.0, 1.0],
+    "mu": [1.335, 0.0, 0.0],
+    "n": [1.0, 1.0, 1.0],
+    "m": [1, 1, 1],
+    "r1": [2.9, 2.5, 1.7],
+    "r2": [3.3, 2.8, 2.0],
+}
+
+Henriksson_PRB_79_144107_FeC = {
+    "__ref__": "Henriksson K.O.E., Nordlund K., Phys. Rev. B 79, 144107 (2009)",
+    "family": "brenner",
+    "el": ["Fe", "C"],
+    "D0": [1.5, 4.82645134, 6.0],
+    "r0": [2.29, 1.47736510, 1.39],
+    "S": [2.0693109, 1.43134755, 1.22],
+    "beta": [1.4, 1.63208170, 2.1],
+    "gamma": [0.0115751, 0.00205862, 0.00020813],
+    "c": [1.2898716, 8.95583221, 330.0],
+    "d": [0.3413219, 0.72062047, 3.5],
+    "h": [-0.26, 0.87099874, 1.0],
+    "mu": [0.0, 0.0, 0.0],
+    "n": [1.0, 1.0, 1.0],
+    "m": [1, 1, 1],
+    "r1": [2.95, 2.3, 1.70],
+    "r2": [3.35, 2.7, 2.00],
+}
+
+Brenner_PRB_42_9458_C_I = {
+    "__ref__": "Brenner D., Phys. Rev. B 42, 9458 (1990) [potential I]",
+    "family": "brenner",
+    "el": ["C"],
+    "D0": [6.325], "r0": [1.315], "S": [1.29], "beta": [1.5], "gamma": [0.011304],
+    "c": [19.0], "d": [2.5], "h": [1.0], "mu": [0.0], "n": [1.0 / (2 * 0.80469)], "m": [1],
+    "r1": [1.70], "r2": [2.00],
+}
+
+Brenner_PRB_42_9458_C_II = {
+    "__ref__": "Brenner D., Phys. Rev. B 42, 9458 (1990) [potential II]",
+    "family": "brenner",
+    "el": ["C"],
+    "D0": [6.0], "r0": [1.39], "S": [1.22], "beta": [2.1], "gamma": [0.00020813],
+    "c": [330.0], "d": [3.5], "h": [1.0], "mu": [0.0], "n": [1.0 / (2 * 0.5)], "m": [1],
+    "r1": [1.70], "r2": [2.00],
+}
+
 # The eleven scalar Tersoff parameters trained by the parameter-gradient path,
 # in the order used throughout (theta vector of a single-element set).
 TERSOFF_THETA_NAMES = ("A", "B", "lambda", "mu", "beta", "n", "c", "d", "h", "r1", "r2")

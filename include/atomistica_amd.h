@@ -62,6 +62,7 @@ extern "C" {
 #define AGPU_KIND_EWALD 3
 #define AGPU_KIND_REBO2 4
 #define AGPU_KIND_KUMAGAI 5
+#define AGPU_KIND_BRENNER 6
 
 /* agpu_rebo2_create flags */
 #define AGPU_REBO2_ZERO_TABLES 1  /* rebo2 zero_tables = .true. (rebo2_module.f90:30-42) */
@@ -191,6 +192,25 @@ typedef struct agpu_kumagai_params {
     double c1, c2, c3, c4, c5, h, r1, r2;
 } agpu_kumagai_params;
 int agpu_kumagai_create(agpu_potential **pot, const agpu_kumagai_params *params);
+
+/* Brenner-form (Albe/Erhart) bond-order potential, up to three elements,
+ * every parameter per element pair (PAIR_INDEX order as Tersoff):
+ * brenner_params.f90:34-71 without screening.  INIT_FUNC / BIND_TO_FUNC
+ * (brenner_module.f90:27-152, 267-296) over the generic bond-order kernel
+ * with the forms of brenner_func.f90:27-196:
+ * V_R = D0/(S-1) exp(-beta sqrt(2S) (r - r0)),
+ * V_A = -S D0/(S-1) exp(-beta sqrt(2/S) (r - r0)),
+ * g = gamma (1 + c^2/d^2 - c^2/(d^2 + (h + cos)^2)), b = (1 + zeta^n)^(-1/2n),
+ * h-term exp((2 mu dr)^m), trig_off cut-off on [r1, r2].
+ * params == NULL selects Erhart_PRB_71_035211_SiC. */
+typedef struct agpu_brenner_params {
+    int nel;
+    int Z[3];
+    double D0[6], r0[6], S[6], beta[6], gamma[6], c[6], d[6], h[6], mu[6], n[6];
+    int m[6];
+    double r1[6], r2[6];
+} agpu_brenner_params;
+int agpu_brenner_create(agpu_potential **pot, const agpu_brenner_params *params);
 
 /* TabulatedAlloyEAM: tabulated_alloy_eam_init (tabulated_alloy_eam.f90:147-259)
  * reading a setfl file; elements = filter string ("*" or NULL = all). */

@@ -136,7 +136,8 @@ class _TersoffParams(ctypes.Structure):
         [(k, ctypes.c_double * 6) for k in ("r1", "r2")] + \
         [("fam", ctypes.c_int)] + \
         [(k, ctypes.c_double * 3) for k in ("kc1", "kc2", "kc3", "kc4", "kc5", "keta", "kdelta")] + \
-        [("kalpha", ctypes.c_double * 6), ("kbeta", ctypes.c_int * 6)]
+        [("kalpha", ctypes.c_double * 6), ("kbeta", ctypes.c_int * 6)] + \
+        [(k, ctypes.c_double * 6) for k in ("bD0", "br0", "bS", "bbeta", "bgam", "bc", "bd", "bh", "bn")]
 
 
 def _tersoff_struct(params, Zs):
@@ -187,10 +188,35 @@ def _kumagai_struct(k):
     return s
 
 
+def _brenner_struct(b):
+    """Brenner parameters (brenner_params.f90 names, per element pair) in the
+    shared bond-order parameter block with fam = 2."""
+    s = _TersoffParams()
+    nel = len(b["el"])
+    s.nel = nel
+    npairs = nel * (nel + 1) // 2
+    for i in range(6):
+        inside = i < npairs
+        s.A[i] = s.B[i] = s.lambda_[i] = s.mu[i] = s.xi[i] = s.omega[i] = 1.0
+        s.mubo[i] = float(b["mu"][i]) if inside else 0.0
+        s.m[i] = int(b["m"][i]) if inside else 1
+        s.r1[i] = float(b["r1"][i]) if inside else 1.0
+        s.r2[i] = float(b["r2"][i]) if inside else 2.0
+        for dst, src, dflt in (("bD0", "D0", 0.0), ("br0", "r0", 0.0), ("bS", "S", 2.0), ("bbeta", "beta", 1.0),
+                               ("bgam", "gamma", 0.0), ("bc", "c", 0.0), ("bd", "d", 1.0), ("bh", "h", 0.0),
+                               ("bn", "n", 1.0)):
+            getattr(s, dst)[i] = float(b[src][i]) if inside else dflt
+    for i in range(3):
+        s.beta[i] = s.n[i] = s.c[i] = s.d[i] = s.h[i] = 1.0
+    s.fam = 2
+    return s
+
+
 def tersoff(params, r, el, cell, nl, mask=None, per_at=False, nthreads=1):
     """Energy, forces and virial; ``el`` holds database element indices (-1 = none).
-    ``params`` is a Tersoff parameter dict, or a Kumagai one (``"family":
-    "kumagai"``) for the Kumagai forms of the same kernel."""
+    ``params`` is a Tersoff parameter dict, or a Kumagai or Brenner one
+    (``"family": "kumagai"`` / ``"brenner"``) for those forms of the same
+    kernel."""
     r = _c(r, np.float64)
     nat = len(r)
     el = _c(el, np.int32)
@@ -199,7 +225,9 @@ def tersoff(params, r, el, cell, nl, mask=None, per_at=False, nthreads=1):
     offsets = _c(offsets, np.int64)
     nbr = _c(nbr, np.int32)
     dc = _c(dc, np.int32)
-    s = _kumagai_struct(params) if params.get("family") == "kumagai" else _tersoff_struct(params, [0, 0, 0])
+    fam = params.get("family")
+    s = (_kumagai_struct(params) if fam == "kumagai" else
+         _brenner_struct(params) if fam == "brenner" else _tersoff_struct(params, [0, 0, 0]))
     epot = ctypes.c_double(0.0)
     f = np.zeros((nat, 3))
     w = np.zeros(9)

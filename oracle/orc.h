@@ -56,6 +56,9 @@ typedef struct {
     double kc1[3], kc2[3], kc3[3], kc4[3], kc5[3], keta[3], kdelta[3];
     double kalpha[6];
     int kbeta[6];
+    /* fam = 2: the Brenner forms (brenner_func.f90), every parameter per
+     * element pair; the h-term takes mu, m from mubo, m above */
+    double bD0[6], br0[6], bS[6], bbeta[6], bgam[6], bc[6], bd[6], bh[6], bn[6];
 } orc_tersoff_params;
 
 int orc_tersoff(const orc_tersoff_params *p, int nat, const double *r,

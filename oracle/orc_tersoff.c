@@ -13,6 +13,9 @@
  *   - VA, VR, g, bo, h    src/potentials/bop/tersoff/tersoff_func.f90:32-211
  *   - fam = 1: g, bo, h of src/potentials/bop/kumagai/kumagai_func.f90:86-221
  *     (VA, VR have the Tersoff form, :32-78)
+ *   - fam = 2: VA, VR, g, bo of src/potentials/bop/brenner/brenner_func.f90:27-147
+ *     with the constants of BIND_TO_FUNC (brenner_module.f90:267-289); its h
+ *     (:153-196) is the Tersoff form
  *   - PAIR_INDEX          src/macros.inc:123
  *   - element map / nebmax src/potentials/bop/default_compute_func.f90:62-73
  * The OpenMP split (per-thread force/pe buffers summed in thread order) is the
@@ -173,12 +176,25 @@ int orc_tersoff(const orc_tersoff_params *p, int nat, const double *r,
                 const double *rnij = bnd_ij->nm;
                 double rij[3] = {rlij * rnij[0], rlij * rnij[1], rlij * rnij[2]};
                 double fcarij = bnd_ij->fc, dfcarijr = bnd_ij->dfc;
-                double ex = exp(-p->mu[el2ij] * rlij);
-                double VAij = -p->B[el2ij] * ex;
-                double dVAij = p->B[el2ij] * p->mu[el2ij] * ex;
-                ex = exp(-p->lambda[el2ij] * rlij);
-                double VRij = p->A[el2ij] * ex;
-                double dVRij = -p->A[el2ij] * p->lambda[el2ij] * ex;
+                double VAij, dVAij, VRij, dVRij;
+                if (p->fam == 2) {
+                    double S = p->bS[el2ij];
+                    double expR = p->bbeta[el2ij] * sqrt(2 * S), expA = p->bbeta[el2ij] * sqrt(2 / S);
+                    double VR_f = p->bD0[el2ij] / (S - 1), VA_f = S * p->bD0[el2ij] / (S - 1);
+                    double ex = exp(-expA * (rlij - p->br0[el2ij]));
+                    VAij = -VA_f * ex;
+                    dVAij = VA_f * expA * ex;
+                    ex = exp(-expR * (rlij - p->br0[el2ij]));
+                    VRij = VR_f * ex;
+                    dVRij = -VR_f * expR * ex;
+                } else {
+                    double ex = exp(-p->mu[el2ij] * rlij);
+                    VAij = -p->B[el2ij] * ex;
+                    dVAij = p->B[el2ij] * p->mu[el2ij] * ex;
+                    ex = exp(-p->lambda[el2ij] * rlij);
+                    VRij = p->A[el2ij] * ex;
+                    dVRij = -p->A[el2ij] * p->lambda[el2ij] * ex;
+                }
                 VAij = 0.5 * maskfac * VAij;
                 dVAij = 0.5 * maskfac * dVAij;
                 VRij = 0.5 * maskfac * VRij;
@@ -200,7 +216,13 @@ int orc_tersoff(const orc_tersoff_params *p, int nat, const double *r,
                     h_fn(p, ikpot, rlij - rlik, &h_Dr, &dh_dDr);
                     double costh = rnik[0] * rnij[0] + rnik[1] * rnij[1] + rnik[2] * rnij[2];
                     double g_costh, dg_dcosth;
-                    if (p->fam == 1) {
+                    if (p->fam == 2) {
+                        /* brenner_func.f90:77-101, parameters of the pair ik */
+                        double c_sq = p->bc[ikpot] * p->bc[ikpot], d_sq = p->bd[ikpot] * p->bd[ikpot];
+                        double hh = d_sq + (p->bh[ikpot] + costh) * (p->bh[ikpot] + costh);
+                        g_costh = p->bgam[ikpot] * (1 + c_sq / d_sq - c_sq / hh);
+                        dg_dcosth = 2 * p->bgam[ikpot] * c_sq * (p->bh[ikpot] + costh) / (hh * hh);
+                    } else if (p->fam == 1) {
                         /* kumagai_func.f90:86-131, parameters of the centre atom */
                         double h_cos = p->h[eli] - costh;
                         double h_cos_sq = h_cos * h_cos;
@@ -249,7 +271,20 @@ int orc_tersoff(const orc_tersoff_params *p, int nat, const double *r,
                 int numnbi = ikc;
                 /* bo: b_ij and dE/dzeta */
                 double bij = 1.0, dbij_dzij = 0.0;
-                if (zij > 0.0 && p->fam == 1) {
+                if (p->fam == 2) {
+                    /* brenner_func.f90:107-147 */
+                    double n = p->bn[el2ij];
+                    double bo_exp = -0.5 / n, bo_fac = 0.5 * bo_exp * n, bo_exp1 = bo_exp - 1.0;
+                    if (n == 1.0) {
+                        double arg = 1.0 + zij;
+                        bij = pow(arg, bo_exp);
+                        dbij_dzij = bo_fac * fcarij * VAij * pow(arg, bo_exp1);
+                    } else if (zij > 0.0) {
+                        double arg = 1.0 + pow(zij, n);
+                        bij = pow(arg, bo_exp);
+                        dbij_dzij = bo_fac * fcarij * VAij * pow(zij, n - 1.0) * pow(arg, bo_exp1);
+                    }
+                } else if (zij > 0.0 && p->fam == 1) {
                     /* kumagai_func.f90:139-171 */
                     double eta = p->keta[eli], delta = -p->kdelta[eli];
                     double arg = 1.0 + pow(zij, eta);
