@@ -40,8 +40,26 @@ module atomistica_gpu
      real(c_double) :: r1(6), r2(6)
   end type tersoff_params_t
 
+  !> Kumagai parameters (BOP_DB_TYPE of kumagai_params.f90:34-94, one element)
+  type, bind(C), public :: kumagai_params_t
+     integer(c_int) :: Z
+     real(c_double) :: A, B, lambda1, lambda2, eta, delta, alpha
+     integer(c_int) :: beta
+     real(c_double) :: c1, c2, c3, c4, c5, h, r1, r2
+  end type kumagai_params_t
+
+  !> Brenner parameters (BOP_DB_TYPE of brenner_params.f90:34-71, no screening)
+  type, bind(C), public :: brenner_params_t
+     integer(c_int) :: nel
+     integer(c_int) :: Z(3)
+     real(c_double) :: D0(6), r0(6), S(6), beta(6), gamma(6), c(6), d(6), h(6), mu(6), n(6)
+     integer(c_int) :: m(6)
+     real(c_double) :: r1(6), r2(6)
+  end type brenner_params_t
+
   public :: particles_init, particles_set_cell, particles_set_Z, particles_set_positions
   public :: neighbors_init, tersoff_init, eam_alloy_init, ewald_init, rebo2_init
+  public :: kumagai_init, brenner_init
   public :: set_precision, bind_to, energy_and_forces, coulomb_energy_and_forces
   public :: last_error, del
 
@@ -98,6 +116,16 @@ module atomistica_gpu
        type(c_ptr), value :: nl
      end function
      integer(c_int) function agpu_tersoff_create(pot, params) bind(C, name="agpu_tersoff_create")
+       import :: c_int, c_ptr
+       type(c_ptr) :: pot
+       type(c_ptr), value :: params
+     end function
+     integer(c_int) function agpu_kumagai_create(pot, params) bind(C, name="agpu_kumagai_create")
+       import :: c_int, c_ptr
+       type(c_ptr) :: pot
+       type(c_ptr), value :: params
+     end function
+     integer(c_int) function agpu_brenner_create(pot, params) bind(C, name="agpu_brenner_create")
        import :: c_int, c_ptr
        type(c_ptr) :: pot
        type(c_ptr), value :: params
@@ -213,6 +241,32 @@ contains
        ierror = agpu_tersoff_create(this%h, c_null_ptr)
     end if
   end subroutine tersoff_init
+
+  !> kumagai_init (kumagai_module.f90:30-78): db absent selects
+  !> Kumagai_CompMaterSci_39_457_Si
+  subroutine kumagai_init(this, ierror, db)
+    type(potential_gpu_t), intent(inout) :: this
+    integer, intent(out) :: ierror
+    type(kumagai_params_t), optional, target, intent(in) :: db
+    if (present(db)) then
+       ierror = agpu_kumagai_create(this%h, c_loc(db))
+    else
+       ierror = agpu_kumagai_create(this%h, c_null_ptr)
+    end if
+  end subroutine kumagai_init
+
+  !> brenner_init (brenner_module.f90:27-152): db absent selects
+  !> Erhart_PRB_71_035211_SiC
+  subroutine brenner_init(this, ierror, db)
+    type(potential_gpu_t), intent(inout) :: this
+    integer, intent(out) :: ierror
+    type(brenner_params_t), optional, target, intent(in) :: db
+    if (present(db)) then
+       ierror = agpu_brenner_create(this%h, c_loc(db))
+    else
+       ierror = agpu_brenner_create(this%h, c_null_ptr)
+    end if
+  end subroutine brenner_init
 
   subroutine eam_alloy_init(this, fn, elements, ierror)
     type(potential_gpu_t), intent(inout) :: this
