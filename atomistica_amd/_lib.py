@@ -131,6 +131,10 @@ PROTOTYPES = {
     "agpu_tersoff_batch_forward": [vp, vp, i, vp, vp, i],
     "agpu_tersoff_batch_backward": [vp, vp, vp, vp, vp, i],
     "agpu_ewald_batch_compute": [vp, vp, d, d, d, i, vp, vp, vp, vp, i],
+    "agpu_md_create": [ctypes.POINTER(vp), vp, vp, vp, vp, vp, d, d],
+    "agpu_md_destroy": [vp],
+    "agpu_md_run": [vp, i, i, vp],
+    "agpu_md_get": [vp, vp, vp, vp, vp, vp, i],
     "agpu_random_positions": [i, vp, d, ctypes.c_uint64, i, vp],
     "agpu_lammps_compute": [vp, vp, vp, i, i, vp, vp, vp, vp, vp, vp],
     "agpu_lammps_compute_list": [vp, vp, vp, i, i, vp, vp, vp, vp, vp, vp, vp, vp, vp],

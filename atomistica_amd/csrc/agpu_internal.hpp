@@ -110,6 +110,11 @@ struct agpu_neighbors {
     bool radix = false;           // bin sort by radix sort (after a very full bin)
     // the list was imported from the caller (LAMMPS) and is not rebuilt
     bool external = false;
+    // Verlet shell added to the interaction range, and a hold flag: while
+    // held, the list is kept although positions move (the MD driver tracks
+    // the displacements against the shell and rebuilds)
+    double skin = 0.0;
+    bool held = false;
     agpu::DevBuf cell_cnt;
     agpu::HostStats stats;
     int64_t n_updates = 0;
@@ -118,6 +123,10 @@ struct agpu_neighbors {
 namespace agpu {
 
 int nl_update(agpu_neighbors *nl, agpu_particles *p, int force);
+// graph-capture mode (no host synchronisation, no memset nodes)
+bool capturing();
+void set_capturing(bool on);
+int zero_dev(void *ptr, size_t nbytes);
 int nl_import(agpu_neighbors *nl, agpu_particles *p, const int64_t *seed, const int *numneigh, const int *neigh);
 int nl_export(agpu_neighbors *nl, int64_t *offsets, int *nbr, int *dc, int on_device);
 int compute_inverse(const double *A, double *B);

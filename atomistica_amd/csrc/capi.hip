@@ -93,7 +93,30 @@ static bool g_prof = false;
 static std::vector<ProfRec> g_prof_recs;
 static std::map<std::string, std::pair<double, int>> g_prof_done;
 
-bool profiling() { return g_prof; }
+// while a step is being captured into a graph: no host synchronisation, no
+// memset nodes, no profiling events (set by the MD driver)
+static bool g_capture = false;
+bool capturing() { return g_capture; }
+void set_capturing(bool on) { g_capture = on; }
+
+bool profiling() { return g_prof && !g_capture; }
+
+__global__ void k_zero_words(unsigned *__restrict__ p, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+
+// zero a device range with a kernel (a fill kernel records as a kernel node
+// when captured; nbytes a multiple of 4)
+int zero_dev(void *ptr, size_t nbytes)
+{
+    const size_t n = nbytes / 4;
+    if (n == 0) return AGPU_OK;
+    const unsigned nb = (unsigned)std::min<size_t>(1024, (n + 255) / 256);
+    AGPU_LAUNCH("zero", k_zero_words, dim3(nb), dim3(256), 0, stream(), (unsigned *)ptr, n);
+    return AGPU_OK;
+}
 
 int prof_begin(const char *name, hipStream_t s)
 {

@@ -651,7 +651,7 @@ int eam_compute(agpu_potential *pot, const agpu_eval_args *a)
     AGPU_CHECK(nl_update(pot->nl, p, 0));
     const int nat = p->nat;
     hipStream_t s = stream();
-    AGPU_HIP(hipMemsetAsync(a->epot, 0, sizeof(double), s));
+    AGPU_CHECK(zero_dev(a->epot, sizeof(double)));
     if (nat == 0) return AGPU_OK;
     const int nblk = (nat + 256 / EAM_G - 1) / (256 / EAM_G);
     AGPU_CHECK(st->dF.reserve(sizeof(double) * (nat + 1)));
@@ -659,7 +659,7 @@ int eam_compute(agpu_potential *pot, const agpu_eval_args *a)
     AGPU_CHECK(st->partial.reserve(sizeof(double) * 11 * (size_t)nblk));
     AGPU_CHECK(st->stats.reserve(sizeof(int) * 4));
     AGPU_CHECK(pot->stats.init());
-    AGPU_HIP(hipMemsetAsync(st->stats.ptr, 0, sizeof(int) * 4, s));
+    AGPU_CHECK(zero_dev(st->stats.ptr, sizeof(int) * 4));
     AGPU_CHECK(st->pk.reserve(sizeof(double4) * 2 * (size_t)(nat + 1)));
     if (pot->precision == AGPU_PREC_MIXED)
         AGPU_CHECK(eam_dispatch<float>(st, pot, a, st->d_rec32.as<float>(), nblk));
@@ -667,6 +667,7 @@ int eam_compute(agpu_potential *pot, const agpu_eval_args *a)
         AGPU_CHECK(eam_dispatch<double>(st, pot, a, st->d_rec64.as<double>(), nblk));
     AGPU_CHECK(reduce_ev(st->partial.as<double>(), nblk, 1, a->epot, nullptr, nullptr));
     AGPU_CHECK(reduce_ev(st->partial.as<double>() + nblk, nblk, 10, a->epot, a->wpot, nullptr));
+    if (capturing()) return AGPU_OK;     // the driver reads the flags after the replay
     AGPU_HIP(hipMemcpyAsync(pot->stats.h, st->stats.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
     AGPU_HIP(hipStreamSynchronize(s));
     if (pot->stats.h[0] & DERR_SPLINE_RANGE)

@@ -613,12 +613,14 @@ int nl_update(agpu_neighbors *nl, agpu_particles *p, int force)
             return set_error(AGPU_ERR_STATE, "neighbour list: imported for other particles");
         return AGPU_OK;
     }
+    if (nl->held && force == 0 && nl->p == p && nl->nat == p->nat) return AGPU_OK;
     if (!p->cell_set) return set_error(AGPU_ERR_STATE, "particles: cell not set");
     if (p->nat > 0 && !p->d_r) return set_error(AGPU_ERR_STATE, "particles: positions not set");
     if (!force && nl->p == p && nl->pos_rev == p->pos_rev && nl->cell_rev == p->cell_rev &&
         nl->nat == p->nat)
         return AGPU_OK;
-    nl->cutoff = nl->interaction_range;   // verlet shell 0 (aseinterface.py:255)
+    // verlet shell 0 (aseinterface.py:255) unless a driver set one
+    nl->cutoff = nl->interaction_range + nl->skin;
     if (nl->cutoff <= 0.0)
         return set_error(AGPU_ERR_ARG, "Cutoff needs to be larger than zero.");
     const int nat = p->nat;

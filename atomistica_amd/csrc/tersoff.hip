@@ -731,7 +731,7 @@ int tersoff_compute(agpu_potential *pot, const agpu_eval_args *a)
     AGPU_CHECK(nl_update(nl, p, 0));
     const int nat = p->nat;
     hipStream_t s = stream();
-    AGPU_HIP(hipMemsetAsync(a->epot, 0, sizeof(double), s));
+    AGPU_CHECK(zero_dev(a->epot, sizeof(double)));
     if (nat == 0) return AGPU_OK;
     const int64_t total = nl->total;
     AGPU_CHECK(st->bslot.reserve(sizeof(int) * (total + 1)));
@@ -751,7 +751,7 @@ int tersoff_compute(agpu_potential *pot, const agpu_eval_args *a)
         wslot = st->wslot.as<double>();
     }
     int *dstats = st->stats.as<int>();
-    AGPU_HIP(hipMemsetAsync(dstats, 0, sizeof(int) * 4, s));
+    AGPU_CHECK(zero_dev(dstats, sizeof(int) * 4));
     CellArg C;
     std::memcpy(C.A, p->cell.A, sizeof C.A);
     const int *el = pot->d_el.as<int>();
@@ -761,6 +761,8 @@ int tersoff_compute(agpu_potential *pot, const agpu_eval_args *a)
     // group width: the row maximum bounds the bond count without a host
     // round trip when it is small; otherwise read the exact bond maximum
     int maxb = nl->maxrow;
+    if (maxb > 64 && capturing())
+        return set_error(AGPU_ERR_UNSUPPORTED, "bond-order step capture needs at most 64 neighbours per atom");
     if (maxb > 64) {
         AGPU_CHECK(int_max(st->bcnt.as<int>(), nat, dstats));
         AGPU_HIP(hipMemcpyAsync(pot->stats.h, dstats, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
@@ -783,6 +785,7 @@ int tersoff_compute(agpu_potential *pot, const agpu_eval_args *a)
     AGPU_CHECK(launch_gather(G, nat, nl, st, eslot, wslot, a, s));
     // per-block records [e, W(9)] -> epot, wpot
     AGPU_CHECK(reduce_ev(st->partial.as<double>(), nblk, 10, a->epot, a->wpot, nullptr));
+    if (capturing()) return AGPU_OK;     // the driver reads the flags after the replay
     AGPU_HIP(hipMemcpyAsync(pot->stats.h, dstats, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
     AGPU_HIP(hipStreamSynchronize(s));
     if (pot->stats.h[1] & DERR_BOND_OVERFLOW)

@@ -456,6 +456,49 @@ def lammps_compute_list(pot, p, nl, nlocal, x, Z, seed, numneigh, neigh, f=None,
     return eng.value, f, vir, ea
 
 
+class MD:
+    """NVE velocity-Verlet dynamics on the GPU around one potential (Tersoff
+    family or EAM), with a Verlet shell ``skin`` on the list and the steps
+    replayed as HIP graphs of ``graph_steps`` steps.  Units are the
+    potential's (eV, Angstrom, amu: time unit 10.18 fs)."""
+
+    def __init__(self, pot, p, nl, masses, dt, skin=0.3, v0=None):
+        pot.bind_to(p, nl)
+        self.nat = p.nat
+        masses = np.ascontiguousarray(np.broadcast_to(np.asarray(masses, dtype=np.float64), (self.nat,)))
+        v0 = None if v0 is None else np.ascontiguousarray(v0, dtype=np.float64)
+        h = ctypes.c_void_p()
+        call("agpu_md_create", ctypes.byref(h), pot._h, p._h, nl._h, _ptr(masses), _ptr(v0), float(dt), float(skin))
+        self._h = h
+        self._keep = (pot, p, nl)
+        self.masses = masses
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib._lib is not None:
+            _lib.load().agpu_md_destroy(h)
+            self._h = None
+
+    def run(self, nsteps, graph_steps=10):
+        """nsteps steps; returns the potential energy after the last one."""
+        e = ctypes.c_double(0.0)
+        call("agpu_md_run", self._h, int(nsteps), int(graph_steps), ctypes.byref(e))
+        return e.value
+
+    def state(self):
+        """(r, v, f, counters {steps, builds, replays, rollbacks}) on the host."""
+        r = np.zeros((self.nat, 3))
+        v = np.zeros((self.nat, 3))
+        f = np.zeros((self.nat, 3))
+        c = np.zeros(4, np.int64)
+        call("agpu_md_get", self._h, _ptr(r), _ptr(v), _ptr(f), None, _ptr(c), _lib.HOST)
+        return r, v, f, dict(zip(("steps", "builds", "replays", "rollbacks"), c.tolist()))
+
+    def kinetic_energy(self):
+        _, v, _, _ = self.state()
+        return 0.5 * float(np.sum(self.masses[:, None] * v * v))
+
+
 def use_torch_stream():
     """Launch every kernel on torch's current stream of the current device,
     so library work is ordered with torch ops on the same tensors."""

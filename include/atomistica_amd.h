@@ -71,6 +71,7 @@ extern "C" {
 typedef struct agpu_particles agpu_particles;
 typedef struct agpu_neighbors agpu_neighbors;
 typedef struct agpu_potential agpu_potential;
+typedef struct agpu_md agpu_md;
 
 /* Internal argument pack (not used across the ABI). */
 typedef struct agpu_eval_args {
@@ -298,6 +299,30 @@ int agpu_tersoff_batch_backward(agpu_batch *b, const double *theta, const double
 int agpu_ewald_batch_compute(agpu_batch *b, const double *q, double cutoff, double beta, double kmax,
                              int precision, double *e_struct, double *f, double *wpot_struct,
                              double *phi, int flags);
+
+/* ======================================================================
+ * NVE molecular dynamics around one potential, with the steps replayed as
+ * a HIP graph -- the velocity-Verlet loop of the reference's MDCORE driver
+ * (src/standalone/main.f90:429-531) with a list kept inside a Verlet shell
+ * (the reference's Python path never rebuilds with a shell,
+ * python_particles.f90:125-128, so the shell logic is this library's own).
+ * Units are the potential's: with eV, Angstrom and amu the time unit is
+ * Angstrom sqrt(amu/eV) = 10.18 fs.  No counterpart in the reference's
+ * potential interface.
+ * ====================================================================== */
+/* masses[nat] and v0[3*nat] (optional, zero if NULL) on the host; the
+ * potential must be a Tersoff-family or EAM class bound to p and nl; skin is
+ * the Verlet shell added to the list cut-off (0: rebuild every step). */
+int agpu_md_create(agpu_md **md, agpu_potential *pot, agpu_particles *p, agpu_neighbors *nl,
+                   const double *masses, const double *v0, double dt, double skin);
+int agpu_md_destroy(agpu_md *md);
+/* nsteps steps; graph_steps > 0 captures that many steps into one graph and
+ * replays it (same trajectory as graph_steps = 0, bitwise); *epot (host,
+ * optional) = potential energy after the last step. */
+int agpu_md_run(agpu_md *md, int nsteps, int graph_steps, double *epot);
+/* r, v, f (3*nat), wpot (9) copies, host or device per flags; counters[4] =
+ * steps, list builds, graph replays, replays redone step by step. */
+int agpu_md_get(agpu_md *md, double *r, double *v, double *f, double *wpot, int64_t *counters, int flags);
 
 /* LAMMPS pair-style step (PairAtomistica::FAtomistica,
  * src/lammps/pair_style/pair_atomistica.cpp:499-561): x[3*nall] holds the
