@@ -14,8 +14,9 @@
 // accum_max_dr is never advanced, python_particles.f90:125-128), so the skin
 // logic is this driver's own.
 //
-// Graph mode: K steps are captured once per list build and replayed as one
-// graph launch.  Before a replay r, v, f and the displacement maximum are
+// Graph mode: runs of K steps (K = 32, 16, 8 or 4, the longest predicted to
+// stay inside the shell) are captured once per list build and replayed as
+// one graph launch.  Before a replay r, v, f and the displacement maximum are
 // copied aside; after it the maximum is read.  If it passed skin/2 inside the
 // replay, the state is restored and the K steps are redone one at a time,
 // each checking the displacement after the drift and rebuilding the list
@@ -37,14 +38,18 @@ struct agpu_md {
     agpu::DevBuf v, minv, rref, f, ck, ev, dmax;
     agpu::HostStats hs;
     bool ready = false;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    int gsteps = 0;
+    // graphs of 32, 16, 8 and 4 steps, captured when first needed after a
+    // list build
+    static constexpr int NG = 4;
+    hipGraph_t graph[NG] = {nullptr, nullptr, nullptr, nullptr};
+    hipGraphExec_t exec[NG] = {nullptr, nullptr, nullptr, nullptr};
     int64_t builds = 0, replays = 0, rollbacks = 0, steps = 0;
     ~agpu_md()
     {
-        if (exec) (void)hipGraphExecDestroy(exec);
-        if (graph) (void)hipGraphDestroy(graph);
+        for (int g = 0; g < NG; ++g) {
+            if (exec[g]) (void)hipGraphExecDestroy(exec[g]);
+            if (graph[g]) (void)hipGraphDestroy(graph[g]);
+        }
         if (nl) {
             nl->held = false;
             nl->skin = 0.0;
@@ -82,14 +87,26 @@ k_md_drift(int n, double hdt, double dt, const double *__restrict__ minv, const 
     if ((threadIdx.x & 63) == 0 && d2 > 0.0f) atomicMax(dmax, __float_as_uint(d2));
 }
 
+// also the largest squared speed since the last list build (rounded up)
 __global__ void __launch_bounds__(256)
-k_md_kick(int n, double hdt, const double *__restrict__ minv, const double *__restrict__ f, double *__restrict__ v)
+k_md_kick(int n, double hdt, const double *__restrict__ minv, const double *__restrict__ f, double *__restrict__ v,
+          unsigned *__restrict__ vmax)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double mi = minv[i];
+    float v2 = 0.0f;
+    if (i < n) {
+        const double mi = minv[i];
+        double s = 0.0;
 #pragma unroll
-    for (int x = 0; x < 3; ++x) v[3 * (size_t)i + x] += hdt * f[3 * (size_t)i + x] * mi;
+        for (int x = 0; x < 3; ++x) {
+            const double vk = v[3 * (size_t)i + x] + hdt * f[3 * (size_t)i + x] * mi;
+            v[3 * (size_t)i + x] = vk;
+            s += vk * vk;
+        }
+        v2 = __double2float_ru(s);
+    }
+    for (int o = 32; o >= 1; o >>= 1) v2 = fmaxf(v2, __shfl_xor(v2, o, 64));
+    if ((threadIdx.x & 63) == 0 && v2 > 0.0f) atomicMax(vmax, __float_as_uint(v2));
 }
 
 static int md_force(agpu_md *md)
@@ -117,15 +134,17 @@ static int md_rebuild(agpu_md *md)
     AGPU_CHECK(nl_update(md->nl, md->p, 1));
     md->nl->held = true;
     AGPU_HIP(hipMemcpyAsync(md->rref.ptr, md->p->d_r, sizeof(double) * 3 * md->nat, hipMemcpyDeviceToDevice, s));
-    AGPU_CHECK(zero_dev(md->dmax.ptr, sizeof(unsigned)));
+    AGPU_CHECK(zero_dev(md->dmax.ptr, sizeof(unsigned) * 3));   // displacement, its copy, speed
     md->builds++;
-    if (md->exec) {
-        AGPU_HIP(hipGraphExecDestroy(md->exec));
-        md->exec = nullptr;
-    }
-    if (md->graph) {
-        AGPU_HIP(hipGraphDestroy(md->graph));
-        md->graph = nullptr;
+    for (int g = 0; g < agpu_md::NG; ++g) {
+        if (md->exec[g]) {
+            AGPU_HIP(hipGraphExecDestroy(md->exec[g]));
+            md->exec[g] = nullptr;
+        }
+        if (md->graph[g]) {
+            AGPU_HIP(hipGraphDestroy(md->graph[g]));
+            md->graph[g] = nullptr;
+        }
     }
     return AGPU_OK;
 }
@@ -143,20 +162,22 @@ static int md_kick(agpu_md *md)
 {
     const int nb = (md->nat + 255) / 256;
     AGPU_LAUNCH("md_kick", k_md_kick, dim3(nb), dim3(256), 0, stream(), md->nat, 0.5 * md->dt,
-                md->minv.as<double>(), md->f.as<double>(), md->v.as<double>());
+                md->minv.as<double>(), md->f.as<double>(), md->v.as<double>(), md->dmax.as<unsigned>() + 2);
     return AGPU_OK;
 }
 
-static double md_read_dmax(agpu_md *md)
+// largest squared displacement since the list build; *v2 (optional) the
+// largest squared speed seen since then
+static double md_read_dmax(agpu_md *md, double *v2 = nullptr)
 {
     hipStream_t s = stream();
-    unsigned u = 0;
-    if (hipMemcpyAsync(md->hs.h, md->dmax.ptr, sizeof(unsigned), hipMemcpyDeviceToHost, s) != hipSuccess ||
+    if (hipMemcpyAsync(md->hs.h, md->dmax.ptr, sizeof(unsigned) * 3, hipMemcpyDeviceToHost, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess)
         return INFINITY;
-    std::memcpy(&u, md->hs.h, sizeof u);
-    float d2;
-    std::memcpy(&d2, &u, sizeof d2);
+    float d2, w2;
+    std::memcpy(&d2, md->hs.h, sizeof d2);
+    std::memcpy(&w2, md->hs.h + 2, sizeof w2);
+    if (v2) *v2 = (double)w2;
     return (double)d2;
 }
 
@@ -172,7 +193,7 @@ static int md_step_checked(agpu_md *md)
     return AGPU_OK;
 }
 
-static int md_capture(agpu_md *md, int K)
+static int md_capture(agpu_md *md, int gi, int K)
 {
     hipStream_t s = stream();
     AGPU_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -191,9 +212,8 @@ static int md_capture(agpu_md *md, int K)
         return rc;
     }
     if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture", __FILE__, __LINE__);
-    md->graph = g;
-    AGPU_HIP(hipGraphInstantiate(&md->exec, g, nullptr, nullptr, 0));
-    md->gsteps = K;
+    md->graph[gi] = g;
+    AGPU_HIP(hipGraphInstantiate(&md->exec[gi], g, nullptr, nullptr, 0));
     return AGPU_OK;
 }
 
@@ -253,9 +273,11 @@ extern "C" int agpu_md_destroy(agpu_md *md)
     return AGPU_OK;
 }
 
-// nsteps velocity-Verlet steps; graph_steps > 0 replays graphs of that many
-// steps (the remainder runs step by step).  epot (host, optional) receives
-// the potential energy of the last step.
+// nsteps velocity-Verlet steps.  graph_steps > 0 replays captured graphs of
+// up to that many steps (32, 16, 8 or 4: the longest that the displacement so
+// far and the largest speed predict to stay inside the shell); other steps
+// run one by one.  epot (host, optional) receives the potential energy of
+// the last step.
 extern "C" int agpu_md_run(agpu_md *md, int nsteps, int graph_steps, double *epot)
 {
     if (!md || nsteps < 0 || graph_steps < 0) return set_error(AGPU_ERR_ARG, "md_run: bad arguments");
@@ -270,31 +292,40 @@ extern "C" int agpu_md_run(agpu_md *md, int nsteps, int graph_steps, double *epo
         md->ready = true;
     }
     const double lim = 0.25 * md->skin * md->skin;
+    static const int KS[agpu_md::NG] = {32, 16, 8, 4};
     int done = 0;
+    double v2 = 0.0;
+    double d2 = md_read_dmax(md, &v2);
     while (done < nsteps) {
-        const int K = graph_steps;
-        if (K > 0 && nsteps - done >= K) {
-            if (!md->exec || md->gsteps != K) {
-                if (md->exec) {
-                    AGPU_HIP(hipGraphExecDestroy(md->exec));
-                    md->exec = nullptr;
+        // steps that surely stay inside the shell: the largest speed so far,
+        // with a quarter of margin for acceleration, from the displacement
+        // reached so far (the rollback below keeps this a prediction only)
+        int K = 0;
+        if (graph_steps > 0 && md->skin > 0.0) {
+            const double room = 0.5 * md->skin - std::sqrt(d2);
+            const double per = 1.25 * std::sqrt(v2) * md->dt;
+            const double safe = per > 0.0 ? room / per : 1e9;
+            for (int g = 0; g < agpu_md::NG; ++g)
+                if (KS[g] <= graph_steps && KS[g] <= nsteps - done && KS[g] <= safe) {
+                    K = KS[g];
+                    break;
                 }
-                if (md->graph) {
-                    AGPU_HIP(hipGraphDestroy(md->graph));
-                    md->graph = nullptr;
-                }
-                AGPU_CHECK(md_capture(md, K));
-            }
+        }
+        if (K > 0) {
+            int gi = 0;
+            while (KS[gi] != K) ++gi;
+            if (!md->exec[gi]) AGPU_CHECK(md_capture(md, gi, K));
             const size_t n3 = 3 * (size_t)md->nat;
             double *ck = md->ck.as<double>();
+            unsigned *dm = md->dmax.as<unsigned>();
             AGPU_HIP(hipMemcpyAsync(ck, md->p->d_r, sizeof(double) * n3, hipMemcpyDeviceToDevice, s));
             AGPU_HIP(hipMemcpyAsync(ck + n3, md->v.ptr, sizeof(double) * n3, hipMemcpyDeviceToDevice, s));
             AGPU_HIP(hipMemcpyAsync(ck + 2 * n3, md->f.ptr, sizeof(double) * n3, hipMemcpyDeviceToDevice, s));
-            unsigned *dm = md->dmax.as<unsigned>();
             AGPU_HIP(hipMemcpyAsync(dm + 1, dm, sizeof(unsigned), hipMemcpyDeviceToDevice, s));
-            AGPU_HIP(hipGraphLaunch(md->exec, s));
+            AGPU_HIP(hipGraphLaunch(md->exec[gi], s));
             md->replays++;
-            if (md_read_dmax(md) >= lim) {
+            d2 = md_read_dmax(md, &v2);
+            if (d2 >= lim) {
                 // the list went stale inside the replay: restore and redo step by step
                 AGPU_HIP(hipMemcpyAsync(md->p->d_r, ck, sizeof(double) * n3, hipMemcpyDeviceToDevice, s));
                 AGPU_HIP(hipMemcpyAsync(md->v.ptr, ck + n3, sizeof(double) * n3, hipMemcpyDeviceToDevice, s));
@@ -302,12 +333,14 @@ extern "C" int agpu_md_run(agpu_md *md, int nsteps, int graph_steps, double *epo
                 AGPU_HIP(hipMemcpyAsync(dm, dm + 1, sizeof(unsigned), hipMemcpyDeviceToDevice, s));
                 md->rollbacks++;
                 for (int k = 0; k < K; ++k) AGPU_CHECK(md_step_checked(md));
+                d2 = md_read_dmax(md, &v2);
             } else {
                 md->steps += K;
             }
             done += K;
         } else {
             AGPU_CHECK(md_step_checked(md));
+            d2 = md_read_dmax(md, &v2);
             ++done;
         }
     }

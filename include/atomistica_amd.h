@@ -316,9 +316,12 @@ int agpu_ewald_batch_compute(agpu_batch *b, const double *q, double cutoff, doub
 int agpu_md_create(agpu_md **md, agpu_potential *pot, agpu_particles *p, agpu_neighbors *nl,
                    const double *masses, const double *v0, double dt, double skin);
 int agpu_md_destroy(agpu_md *md);
-/* nsteps steps; graph_steps > 0 captures that many steps into one graph and
- * replays it (same trajectory as graph_steps = 0, bitwise); *epot (host,
- * optional) = potential energy after the last step. */
+/* nsteps steps.  graph_steps > 0 replays captured graphs of up to that many
+ * steps (32, 16, 8 or 4, the longest that the displacement so far and the
+ * largest speed predict to keep the list valid; a replay that still goes
+ * stale is rolled back and redone step by step), giving the same trajectory
+ * as graph_steps = 0, bitwise.  *epot (host, optional) = potential energy
+ * after the last step. */
 int agpu_md_run(agpu_md *md, int nsteps, int graph_steps, double *epot);
 /* r, v, f (3*nat), wpot (9) copies, host or device per flags; counters[4] =
  * steps, list builds, graph replays, replays redone step by step. */
