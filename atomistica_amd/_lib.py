@@ -123,6 +123,8 @@ PROTOTYPES = {
     "agpu_potential_bind_to": [vp, vp, vp],
     "agpu_potential_energy_and_forces": [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, i],
     "agpu_coulomb_potential": [vp, vp, vp, vp, vp, i],
+    "agpu_potential_autotune": [vp, vp, vp, i, ctypes.c_char_p, i],
+    "agpu_potential_set_launch": [vp, vp, i, i, ip],
     "agpu_batch_create": [ctypes.POINTER(vp)],
     "agpu_batch_destroy": [vp],
     "agpu_batch_set_structures": [vp, i, vp, vp],

@@ -115,6 +115,9 @@ struct agpu_neighbors {
     // the displacements against the shell and rebuilds)
     double skin = 0.0;
     bool held = false;
+    // pair search kernel: 0 chosen from the bin occupancy, 1 thread per
+    // atom, 2 wavefront per bin block (both give the same rows)
+    int pair_mode = 0;
     agpu::DevBuf cell_cnt;
     agpu::HostStats stats;
     int64_t n_updates = 0;
@@ -213,6 +216,9 @@ int tersoff_bind(agpu_potential *pot);
 int tersoff_compute(agpu_potential *pot, const agpu_eval_args *a);
 int eam_bind(agpu_potential *pot);
 int eam_compute(agpu_potential *pot, const agpu_eval_args *a);
+// selects launch variant k (lanes per atom g, entries per lane u) when k is
+// in range and returns the number of variants (0: this binding has none)
+int eam_variant(agpu_potential *pot, int k, int *g, int *u);
 int ewald_bind(agpu_potential *pot);
 int ewald_compute(agpu_potential *pot, const agpu_eval_args *a);
 int ewald_potential(agpu_potential *pot, const double *q, double *phi, int flags);

@@ -120,6 +120,9 @@ struct EamState {
     uint64_t Z_rev = ~0ull;
     DevBuf d_hdr, d_rec64, d_rec32, dF, partial, stats, pk;
     bool all_known = false;       // every atom is of a tabulated element
+    // lanes per atom and entries per lane of the single-element kernels
+    // (chosen by eam_autotune; the defaults otherwise)
+    int tg = 16, tu = 4;
 };
 
 static void eam_free(void *p) { delete static_cast<EamState *>(p); }
@@ -324,6 +327,8 @@ __device__ __forceinline__ void spl_fdf_x(const SplHdr &h, const R *__restrict__
 
 // Lanes per atom: rows of ~50 neighbours are split over G lanes that each
 // walk a strided share, so several independent gathers are in flight per lane.
+// This is the default; the launch autotuner may pick another G (and U below)
+// for the single-element force path.
 #ifndef AGPU_EAM_G
 #define AGPU_EAM_G 16
 #endif
@@ -359,7 +364,7 @@ __global__ void k_eam_pack(int nat, const double *__restrict__ r, const int *__r
 }
 
 // KN: one element and every atom tabulated, so no element is read per pair.
-template <typename R, int G, bool ONE, bool KN>
+template <typename R, int G, int U, bool ONE, bool KN>
 __global__ void __launch_bounds__(256)
 k_eam_density(EamDev P, const SplHdr *__restrict__ hdr, const R *__restrict__ rec, int nat,
               const int *__restrict__ order, const double4 *__restrict__ q1, CellArg C,
@@ -382,18 +387,18 @@ k_eam_density(EamDev P, const SplHdr *__restrict__ hdr, const R *__restrict__ re
         if (on) {
             const double ri[3] = {qi.x, qi.y, qi.z};
             const int beg = offsets[i], end = offsets[i + 1];
-            for (int s0 = beg + lane; s0 < end; s0 += EAM_U * G) {
-                int2 e[EAM_U];
+            for (int s0 = beg + lane; s0 < end; s0 += U * G) {
+                int2 e[U];
 #pragma unroll
-                for (int u = 0; u < EAM_U; ++u) {
+                for (int u = 0; u < U; ++u) {
                     const int s = s0 + u * G;
                     e[u] = s < end ? nbr[s] : make_int2(-1, 0);
                 }
-                double4 qj[EAM_U];
+                double4 qj[U];
 #pragma unroll
-                for (int u = 0; u < EAM_U; ++u) qj[u] = q1[e[u].x < 0 ? i : e[u].x];
+                for (int u = 0; u < U; ++u) qj[u] = q1[e[u].x < 0 ? i : e[u].x];
 #pragma unroll
-                for (int u = 0; u < EAM_U; ++u) {
+                for (int u = 0; u < U; ++u) {
                     const int elj = e[u].x < 0 ? -1 : (KN ? 0 : (int)qj[u].w);
                     if (elj < 0) continue;
                     int a, b, c;
@@ -434,7 +439,7 @@ k_eam_density(EamDev P, const SplHdr *__restrict__ hdr, const R *__restrict__ re
 
 // GEN = false: no mask and no per-atom virial requested (the common call),
 // so neither is loaded nor accumulated.
-template <typename R, int G, bool ONE, bool GEN, bool KN>
+template <typename R, int G, int U, bool ONE, bool GEN, bool KN>
 __global__ void __launch_bounds__(256)
 k_eam_force(EamDev P, const SplHdr *__restrict__ hdr, const R *__restrict__ rec, int nat,
             const int *__restrict__ order, const double4 *__restrict__ q1, const double4 *__restrict__ q2,
@@ -461,25 +466,25 @@ k_eam_force(EamDev P, const SplHdr *__restrict__ hdr, const R *__restrict__ rec,
             const double ri[3] = {qi.x, qi.y, qi.z};
             const int beg = offsets[i], end = offsets[i + 1];
             const SplHdr hri = hdr_of<ONE>(P, hdr, 1, eli, 0);
-            for (int s0 = beg + lane; s0 < end; s0 += EAM_U * G) {
-                int2 en[EAM_U];
+            for (int s0 = beg + lane; s0 < end; s0 += U * G) {
+                int2 en[U];
 #pragma unroll
-                for (int u = 0; u < EAM_U; ++u) {
+                for (int u = 0; u < U; ++u) {
                     const int s = s0 + u * G;
                     en[u] = s < end ? nbr[s] : make_int2(-1, 0);
                 }
-                double4 qj[EAM_U];
-                int elj[EAM_U];
-                double mj[EAM_U];
+                double4 qj[U];
+                int elj[U];
+                double mj[U];
 #pragma unroll
-                for (int u = 0; u < EAM_U; ++u) {
+                for (int u = 0; u < U; ++u) {
                     const int j = en[u].x < 0 ? i : en[u].x;
                     qj[u] = q2[j];
                     elj[u] = en[u].x < 0 ? -1 : (KN ? 0 : (int)q1[j].w);
                     mj[u] = (GEN && mask && mask[j] == 0) ? 0.0 : 1.0;
                 }
 #pragma unroll
-                for (int u = 0; u < EAM_U; ++u) {
+                for (int u = 0; u < U; ++u) {
                     if (elj[u] < 0) continue;
                     int a, b, c;
                     dc_unpack(en[u].y, a, b, c);
@@ -602,45 +607,96 @@ int eam_bind(agpu_potential *pot)
     return AGPU_OK;
 }
 
-template <typename R, bool ONE, bool KN>
-static int eam_launch(EamState *st, agpu_potential *pot, const agpu_eval_args *a, const R *rec, int nblk)
+template <typename R, int G, int U, bool ONE, bool GEN, bool KN>
+static int eam_passes(EamState *st, agpu_potential *pot, const agpu_eval_args *a, const R *rec, int nblk)
 {
     agpu_particles *p = pot->p;
     agpu_neighbors *nl = pot->nl;
     hipStream_t s = stream();
     CellArg C;
     std::memcpy(C.A, p->cell.A, sizeof C.A);
-    const int *el = pot->d_el.as<int>();
     const int *order = nl->idx_sorted.as<int>();
     double *part = st->partial.as<double>();
     int *flags = st->stats.as<int>();
     const bool mixed = sizeof(R) == 4;
     const int nat = p->nat;
     double4 *q1 = st->pk.as<double4>(), *q2 = q1 + nat;
-    AGPU_LAUNCH("eam_pack", k_eam_pack, dim3((nat + 255) / 256), dim3(256), 0, s, nat, p->d_r, el, q1);
-    AGPU_LAUNCH(mixed ? "eam_density_mixed" : "eam_density_fp64", (k_eam_density<R, EAM_G, ONE, KN>), dim3(nblk),
+    AGPU_LAUNCH(mixed ? "eam_density_mixed" : "eam_density_fp64", (k_eam_density<R, G, U, ONE, KN>), dim3(nblk),
                 dim3(256), 0, s, st->dev, st->d_hdr.as<SplHdr>(), rec, nat, order, q1, C,
                 nl->offsets.as<int>(), nl->nbr.as<int2>(), a->mask, st->dF.as<double>(), q2, a->epot_per_at, part,
                 flags);
-    if (a->mask || a->wpot_per_at)
-        AGPU_LAUNCH(mixed ? "eam_force_mixed" : "eam_force_fp64", (k_eam_force<R, EAM_G, ONE, true, KN>),
-                    dim3(nblk), dim3(256), 0, s, st->dev, st->d_hdr.as<SplHdr>(), rec, nat, order, q1, q2, C,
-                    nl->offsets.as<int>(), nl->nbr.as<int2>(), a->mask, a->f, a->epot_per_at, a->wpot_per_at,
-                    part + nblk, flags);
-    else
-        AGPU_LAUNCH(mixed ? "eam_force_mixed" : "eam_force_fp64", (k_eam_force<R, EAM_G, ONE, false, KN>),
-                    dim3(nblk), dim3(256), 0, s, st->dev, st->d_hdr.as<SplHdr>(), rec, nat, order, q1, q2, C,
-                    nl->offsets.as<int>(), nl->nbr.as<int2>(), a->mask, a->f, a->epot_per_at, a->wpot_per_at,
-                    part + nblk, flags);
+    AGPU_LAUNCH(mixed ? "eam_force_mixed" : "eam_force_fp64", (k_eam_force<R, G, U, ONE, GEN, KN>), dim3(nblk),
+                dim3(256), 0, s, st->dev, st->d_hdr.as<SplHdr>(), rec, nat, order, q1, q2, C,
+                nl->offsets.as<int>(), nl->nbr.as<int2>(), a->mask, a->f, a->epot_per_at, a->wpot_per_at,
+                part + nblk, flags);
     return AGPU_OK;
 }
 
+// the (lanes per atom, entries per lane) variants the autotuner may pick;
+// EAM_GMAX (fewest atoms per block, so the most blocks) sizes the per-block
+// partial records
+constexpr int EAM_TUNE[][2] = {{8, 4}, {16, 2}, {16, 4}, {16, 8}, {32, 2}, {32, 4}};
+constexpr int EAM_GMAX = 32;
+constexpr bool eam_tune_fits()
+{
+    for (const auto &v : EAM_TUNE)
+        if (v[0] > EAM_GMAX) return false;
+    return EAM_G <= EAM_GMAX;
+}
+static_assert(eam_tune_fits(), "a lane group is wider than EAM_GMAX");
+
+template <typename R, bool ONE, bool KN>
+static int eam_launch(EamState *st, agpu_potential *pot, const agpu_eval_args *a, const R *rec, int *nblk_out)
+{
+    const int nat = pot->p->nat;
+    const bool gen = a->mask || a->wpot_per_at;
+    int G = EAM_G, U = EAM_U;
+    if (ONE && KN && !gen) {
+        G = st->tg;
+        U = st->tu;
+    }
+    const int nblk = (nat + 256 / G - 1) / (256 / G);
+    if (G > EAM_GMAX) return set_error(AGPU_ERR_STATE, "EAM: lane group wider than the partial records allow");
+    *nblk_out = nblk;
+    const int *el = pot->d_el.as<int>();
+    AGPU_LAUNCH("eam_pack", k_eam_pack, dim3((nat + 255) / 256), dim3(256), 0, stream(), nat, pot->p->d_r, el,
+                st->pk.as<double4>());
+    if constexpr (ONE && KN) {
+        if (!gen) {
+            switch (G * 100 + U) {
+            case 804: return eam_passes<R, 8, 4, ONE, false, KN>(st, pot, a, rec, nblk);
+            case 1602: return eam_passes<R, 16, 2, ONE, false, KN>(st, pot, a, rec, nblk);
+            case 1608: return eam_passes<R, 16, 8, ONE, false, KN>(st, pot, a, rec, nblk);
+            case 3202: return eam_passes<R, 32, 2, ONE, false, KN>(st, pot, a, rec, nblk);
+            case 3204: return eam_passes<R, 32, 4, ONE, false, KN>(st, pot, a, rec, nblk);
+            default: break;
+            }
+        }
+    }
+    if (gen) return eam_passes<R, EAM_G, EAM_U, ONE, true, KN>(st, pot, a, rec, nblk);
+    return eam_passes<R, EAM_G, EAM_U, ONE, false, KN>(st, pot, a, rec, nblk);
+}
+
 template <typename R>
-static int eam_dispatch(EamState *st, agpu_potential *pot, const agpu_eval_args *a, const R *rec, int nblk)
+static int eam_dispatch(EamState *st, agpu_potential *pot, const agpu_eval_args *a, const R *rec, int *nblk)
 {
     if (st->nel != 1) return eam_launch<R, false, false>(st, pot, a, rec, nblk);
     if (st->all_known) return eam_launch<R, true, true>(st, pot, a, rec, nblk);
     return eam_launch<R, true, false>(st, pot, a, rec, nblk);
+}
+
+int eam_variant(agpu_potential *pot, int k, int *g, int *u)
+{
+    constexpr int n = (int)(sizeof EAM_TUNE / sizeof EAM_TUNE[0]);
+    EamState *st = static_cast<EamState *>(pot->impl);
+    if (k >= 0 && k < n) {
+        st->tg = EAM_TUNE[k][0];
+        st->tu = EAM_TUNE[k][1];
+    }
+    if (g) *g = st->tg;
+    if (u) *u = st->tu;
+    // only the single-element, all-tabulated force path has variants
+    return st->nel == 1 && st->all_known ? n : 0;
 }
 
 int eam_compute(agpu_potential *pot, const agpu_eval_args *a)
@@ -653,18 +709,21 @@ int eam_compute(agpu_potential *pot, const agpu_eval_args *a)
     hipStream_t s = stream();
     AGPU_CHECK(zero_dev(a->epot, sizeof(double)));
     if (nat == 0) return AGPU_OK;
-    const int nblk = (nat + 256 / EAM_G - 1) / (256 / EAM_G);
+    // blocks of the widest lane group (EAM_GMAX lanes per atom): the most
+    // blocks any variant launches
+    const int nblk_max = (nat + 256 / EAM_GMAX - 1) / (256 / EAM_GMAX);
     AGPU_CHECK(st->dF.reserve(sizeof(double) * (nat + 1)));
     // layout: nblk energy scalars of the density pass, then nblk 10-records
-    AGPU_CHECK(st->partial.reserve(sizeof(double) * 11 * (size_t)nblk));
+    AGPU_CHECK(st->partial.reserve(sizeof(double) * 11 * (size_t)nblk_max));
     AGPU_CHECK(st->stats.reserve(sizeof(int) * 4));
     AGPU_CHECK(pot->stats.init());
     AGPU_CHECK(zero_dev(st->stats.ptr, sizeof(int) * 4));
     AGPU_CHECK(st->pk.reserve(sizeof(double4) * 2 * (size_t)(nat + 1)));
+    int nblk = 0;
     if (pot->precision == AGPU_PREC_MIXED)
-        AGPU_CHECK(eam_dispatch<float>(st, pot, a, st->d_rec32.as<float>(), nblk));
+        AGPU_CHECK(eam_dispatch<float>(st, pot, a, st->d_rec32.as<float>(), &nblk));
     else
-        AGPU_CHECK(eam_dispatch<double>(st, pot, a, st->d_rec64.as<double>(), nblk));
+        AGPU_CHECK(eam_dispatch<double>(st, pot, a, st->d_rec64.as<double>(), &nblk));
     AGPU_CHECK(reduce_ev(st->partial.as<double>(), nblk, 1, a->epot, nullptr, nullptr));
     AGPU_CHECK(reduce_ev(st->partial.as<double>() + nblk, nblk, 10, a->epot, a->wpot, nullptr));
     if (capturing()) return AGPU_OK;     // the driver reads the flags after the replay

@@ -714,7 +714,8 @@ int nl_update(agpu_neighbors *nl, agpu_particles *p, int force)
         }
         const int S = nl->stage_w;
         AGPU_CHECK(nl->stage.reserve(sizeof(int2) * (size_t)nat * S));
-        if (per_bin < 4.0) {
+        const bool per_atom = nl->pair_mode == 1 || (nl->pair_mode == 0 && per_bin < 4.0);
+        if (per_atom) {
             const int na = (nat + TB - 1) / TB;
             if (ortho)
                 AGPU_LAUNCH("nl_pairs", k_atom_pairs<true>, dim3(na), dim3(TB), 0, s, nat, g, C, cutoff_sq,

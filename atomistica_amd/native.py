@@ -192,6 +192,22 @@ class _Potential:
     def bind_to(self, p, nl):
         call("agpu_potential_bind_to", self._h, p._h, nl._h)
 
+    def autotune(self, p, nl, reps=5):
+        """Times the kernel launch variants on this configuration and keeps
+        the fastest (neighbour-list pair search; EAM lanes per atom and
+        entries per lane).  Returns the timing report as text."""
+        buf = ctypes.create_string_buffer(4096)
+        call("agpu_potential_autotune", self._h, p._h, nl._h, int(reps), buf, len(buf))
+        return buf.value.decode()
+
+    def set_launch(self, nl, pair_mode=0, variant=-1):
+        """Pins the launch variants (pair_mode 0 auto, 1 thread per atom,
+        2 wavefront per bin block; variant as enumerated by the autotuner,
+        -1 keeps it).  Returns the number of kernel variants offered."""
+        n = ctypes.c_int(0)
+        call("agpu_potential_set_launch", self._h, nl._h, int(pair_mode), int(variant), ctypes.byref(n))
+        return n.value
+
     def set_Coulomb(self, coul):
         raise AgpuError(_lib.ERR_UNSUPPORTED, "set_Coulomb: this class has no charge coupling")
 

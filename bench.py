@@ -426,6 +426,8 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch shapes instead of timing the variants in warm-up")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="end with one 512 MiB device-to-device copy (reference kernel for "
                          "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -454,6 +456,14 @@ def main():
     nat = run.nat
     for _ in range(args.warmup):
         run.step()
+    tune = None
+    if not args.no_autotune:
+        # launch-shape autotuner (warm-up, outside the timed region)
+        rep = run.pot.autotune(run.p, run.nl, reps=5)
+        tune = {"report": rep.strip().splitlines(),
+                "chosen": [ln.strip() for ln in rep.splitlines() if "->" in ln]}
+        for _ in range(2):
+            run.step()
     dt = run.timed(args.steps)
     value = nat * args.steps / dt
     total, maxrow = run.nl.get_size()
@@ -517,6 +527,7 @@ def main():
         "kernels": kern,
         "roofline": roof,
         "valu": valu_rates(args.workload, kern),
+        "autotune": tune,
     }
 
     if not args.no_mixed and args.precision == "fp64" and wl["kind"] in ("eam", "tersoff", "ewald"):

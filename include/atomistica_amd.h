@@ -251,6 +251,25 @@ int agpu_potential_energy_and_forces(agpu_potential *pot, agpu_particles *p,
                                      double *f_per_bond, double *wpot_per_at,
                                      double *wpot_per_bond, int flags);
 
+/* Launch-shape autotuner (no counterpart in the reference, whose kernels are
+ * CPU loops): times the interchangeable kernel variants on the bound
+ * configuration with HIP events, reps calls each, and keeps the fastest for
+ * later calls on pot and nl -- the neighbour-list pair search (thread per
+ * atom or wavefront per bin block; identical rows) and, for single-element
+ * tabulated EAM, lanes per atom x row entries per lane.  A text report of
+ * the timings and choices is written to report (len bytes, may be NULL). */
+int agpu_potential_autotune(agpu_potential *pot, agpu_particles *p, agpu_neighbors *nl, int reps,
+                            char *report, int len);
+
+/* Pins the launch variants instead of timing them (to repeat an earlier
+ * autotuner choice): pair_mode 0 = from the bin occupancy, 1 = thread per
+ * atom, 2 = wavefront per bin block; variant -1 keeps the current kernel
+ * variant, else selects 0 <= variant < *nvariants, the number of variants
+ * this binding offers (0 for all but single-element EAM; variant 0 is then
+ * the only accepted value). */
+int agpu_potential_set_launch(agpu_potential *pot, agpu_neighbors *nl, int pair_mode, int variant,
+                              int *nvariants);
+
 /* Coulomb potential phi_i = dE/dq_i (pme_potential, pme.f90:215-321);
  * phi is accumulated (+=). */
 int agpu_coulomb_potential(agpu_potential *pot, agpu_particles *p, agpu_neighbors *nl,
