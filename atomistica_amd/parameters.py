@@ -9,7 +9,8 @@ default ``Tersoff_PRB_39_5566_SiC``).
 Pair-indexed entries follow the symmetric pair index of ``src/macros.inc:123``
 (0-based here): for elements (C, Si) the order is C-C, C-Si, Si-Si.
 """
-from math import sqrt
+import copy
+from math import log, sqrt
 
 
 def pair_index(i, j, nel):
@@ -162,6 +163,56 @@ Brenner_PRB_42_9458_C_II = {
 
 # The eleven scalar Tersoff parameters trained by the parameter-gradient path,
 # in the order used throughout (theta vector of a single-element set).
+# ---------------------------------------------------------------------------
+# Screened variants (TersoffScr, KumagaiScr, BrennerScr): inner cutoff
+# [r1, r2], pair cutoff [or1, or2], bond-order cutoff [bor1, bor2], all of the
+# exp_cutoff form, and the screening parameters Cmin, Cmax (reference:
+# src/python/atomistica/parameters.py:79-107, 196-218, 409-420).
+# ---------------------------------------------------------------------------
+SCREENING_KEYS = ("or1", "or2", "bor1", "bor2", "Cmin", "Cmax")
+
+Tersoff_PRB_39_5566_Si_C__Scr = copy.deepcopy(Tersoff_PRB_39_5566_Si_C)
+Tersoff_PRB_39_5566_Si_C__Scr.update({
+    "m": [3, 3, 3],
+    "r1": [2.00, sqrt(2.00 * 2.50), 2.50],
+    "r2": [2.00 * 1.2, sqrt(2.00 * 2.50) * 1.2, 2.50 * 1.2],
+    "or1": [2.00, sqrt(2.00 * 3.00), 3.00],
+    "or2": [2.00 * 2.0, sqrt(2.00 * 3.00) * 2.0, 3.00 * 2.0],
+    "bor1": [2.00, sqrt(2.00 * 3.00), 3.00],
+    "bor2": [2.00 * 2.0, sqrt(2.00 * 3.00) * 2.0, 3.00 * 2.0],
+    "Cmin": [1.00, 1.00, 1.00],
+    "Cmax": [3.00, 3.00, 3.00],
+})
+# mubo is the inverse dimer length of each pair
+_p = Tersoff_PRB_39_5566_Si_C__Scr
+_p["mubo"] = [(_p["lambda"][i] - _p["mu"][i]) /
+              log((_p["lambda"][i] * _p["A"][i]) / (_p["mu"][i] * _p["B"][i])) for i in range(3)]
+del _p
+
+Kumagai_CompMaterSci_39_457_Si__Scr = copy.deepcopy(Kumagai_CompMaterSci_39_457_Si)
+Kumagai_CompMaterSci_39_457_Si__Scr.update({
+    "r1": 2.50, "r2": 2.50 * 1.2,
+    "or1": 3.00, "or2": 3.00 * 2.0,
+    "bor1": 3.00, "bor2": 3.00 * 2.0,
+    "Cmin": 1.00, "Cmax": 3.00,
+})
+
+Erhart_PRB_71_035211_SiC__Scr = copy.deepcopy(Erhart_PRB_71_035211_SiC)
+Erhart_PRB_71_035211_SiC__Scr.update({
+    "mu": [1.0 / 1.4276, 1.0 / 1.79, 1.0 / 1.842],
+    "n": [1.0, 1.0, 1.0],
+    "m": [3, 3, 3],
+    "r1": [2.00, 2.40, 2.50],
+    "r2": [2.00 * 1.2, 2.40 * 1.2, 2.50 * 1.2],
+    "or1": [2.00, 2.40, 3.00],
+    "or2": [2.00 * 2.0, 2.40 * 2.0, 3.00 * 2.0],
+    "bor1": [2.00, 2.40, 3.00],
+    "bor2": [2.00 * 2.0, 2.40 * 2.0, 3.00 * 2.0],
+    "Cmin": [1.00, 1.00, 1.00],
+    "Cmax": [3.00, 3.00, 3.00],
+})
+
+
 TERSOFF_THETA_NAMES = ("A", "B", "lambda", "mu", "beta", "n", "c", "d", "h", "r1", "r2")
 
 ATOMIC_NUMBERS = {"H": 1, "He": 2, "Li": 3, "Be": 4, "B": 5, "C": 6, "N": 7, "O": 8,

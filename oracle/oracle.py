@@ -137,7 +137,27 @@ class _TersoffParams(ctypes.Structure):
         [("fam", ctypes.c_int)] + \
         [(k, ctypes.c_double * 3) for k in ("kc1", "kc2", "kc3", "kc4", "kc5", "keta", "kdelta")] + \
         [("kalpha", ctypes.c_double * 6), ("kbeta", ctypes.c_int * 6)] + \
-        [(k, ctypes.c_double * 6) for k in ("bD0", "br0", "bS", "bbeta", "bgam", "bc", "bd", "bh", "bn")]
+        [(k, ctypes.c_double * 6) for k in ("bD0", "br0", "bS", "bbeta", "bgam", "bc", "bd", "bh", "bn")] + \
+        [("scr", ctypes.c_int)] + \
+        [(k, ctypes.c_double * 6) for k in ("or1", "or2", "bor1", "bor2", "Cmin", "Cmax")]
+
+
+_SCR_KEYS = ("or1", "or2", "bor1", "bor2", "Cmin", "Cmax")
+
+
+def _screening(s, params):
+    """The screened variants' parameters (or1, or2, bor1, bor2, Cmin, Cmax per
+    element pair, tersoff_params.f90:82-89) when the dict has them."""
+    if "or1" not in params:
+        return s
+    s.scr = 1
+    for k in _SCR_KEYS:
+        arr, vals = getattr(s, k), params[k]
+        if not isinstance(vals, (list, tuple)):
+            vals = [vals] * 6            # one-element sets give scalars
+        for i in range(6):
+            arr[i] = float(vals[i]) if i < len(vals) else 0.0
+    return s
 
 
 def _tersoff_struct(params, Zs):
@@ -216,7 +236,8 @@ def tersoff(params, r, el, cell, nl, mask=None, per_at=False, nthreads=1):
     """Energy, forces and virial; ``el`` holds database element indices (-1 = none).
     ``params`` is a Tersoff parameter dict, or a Kumagai or Brenner one
     (``"family": "kumagai"`` / ``"brenner"``) for those forms of the same
-    kernel."""
+    kernel; a dict with the screening keys (or1, or2, bor1, bor2, Cmin, Cmax)
+    selects the screened variant of its family."""
     r = _c(r, np.float64)
     nat = len(r)
     el = _c(el, np.int32)
@@ -228,6 +249,7 @@ def tersoff(params, r, el, cell, nl, mask=None, per_at=False, nthreads=1):
     fam = params.get("family")
     s = (_kumagai_struct(params) if fam == "kumagai" else
          _brenner_struct(params) if fam == "brenner" else _tersoff_struct(params, [0, 0, 0]))
+    _screening(s, params)
     epot = ctypes.c_double(0.0)
     f = np.zeros((nat, 3))
     w = np.zeros(9)

@@ -59,6 +59,11 @@ typedef struct {
     /* fam = 2: the Brenner forms (brenner_func.f90), every parameter per
      * element pair; the h-term takes mu, m from mubo, m above */
     double bD0[6], br0[6], bS[6], bbeta[6], bgam[6], bc[6], bd[6], bh[6], bn[6];
+    /* scr = 1: the screened variants (TersoffScr, KumagaiScr, BrennerScr):
+     * exp_cutoff on [r1, r2] (inner), [or1, or2] (pair), [bor1, bor2] (bond
+     * order) and the screening of bonds by third atoms with Cmin, Cmax */
+    int scr;
+    double or1[6], or2[6], bor1[6], bor2[6], Cmin[6], Cmax[6];
 } orc_tersoff_params;
 
 int orc_tersoff(const orc_tersoff_params *p, int nat, const double *r,

@@ -18,6 +18,12 @@
  *     (:153-196) is the Tersoff form
  *   - PAIR_INDEX          src/macros.inc:123
  *   - element map / nebmax src/potentials/bop/default_compute_func.f90:62-73
+ *   - scr = 1 (TersoffScr, KumagaiScr, BrennerScr): loop1's screening function
+ *     and its cutoff regions (bop_kernel.f90:563-1066, exp_cutoff of
+ *     src/support/cutoff.f90:232-293, C_dr_cut and the requested range of
+ *     default_bind_to_func.f90:44-150), the forces of the pair-term screening
+ *     in loop2 (:1448-1492) and of the bond-order screening in the last loop
+ *     (:1531-1609)
  * The OpenMP split (per-thread force/pe buffers summed in thread order) is the
  * model of src/support/tls.f90:58-269; with nthreads = 1 it is the serial loop.
  *
@@ -44,8 +50,166 @@ static int pair_index(int i, int j, int nel)
 typedef struct {
     int j, typ;
     int dc[3];
-    double len, nm[3], fc, dfc;
+    double len, nm[3], fc, dfc;     /* fc: the pair (ar) cutoff */
+    double fcbo, dfcbo;             /* bond-order cutoff (= fc unscreened) */
+    int64_t s0, s1;                 /* screening records [s0, s1) */
 } bond_t;
+
+/* one third atom k screening bond ij: its list entry and the derivative
+ * factors of the pair (ar) and bond-order (bo) cutoffs along r_ik and r_jk */
+typedef struct {
+    int64_t kn;
+    double arik, arjk, boik, bojk;
+} scr_t;
+
+/* exp_cutoff_f (cutoff.f90:245-293): exp(-8 x^3) corrected so that value,
+ * first and second derivative vanish at x = 1 */
+static void exp_cut(double r1, double r2, double r, double *val, double *dval)
+{
+    if (r <= r1) { *val = 1.0; *dval = 0.0; return; }
+    if (r >= r2) { *val = 0.0; *dval = 0.0; return; }
+    const double v1 = exp(-8.0), dv1 = -24 * v1, ddv1 = -48 * v1 - 24 * dv1;
+    const double c = (-3 * dv1 + ddv1) / 3, d = (2 * dv1 - ddv1) / 4;
+    const double fac1 = 1.0 / (r2 - r1), fac2 = 1.0 / (1 - v1 - c - d), off = v1 + c + d;
+    double x = fac1 * (r - r1), x2 = x * x;
+    double v = exp(-8 * x * x2);
+    double dv = -24 * x2 * v;
+    *dval = fac1 * fac2 * (dv + 3 * c * x2 + 4 * d * x * x2);
+    *val = fac2 * (v + c * x * x2 + d * x2 * x2 - off);
+}
+
+static void entry_vec(const double *r, const double *cell, const int *dc, int i, int j, int64_t jn,
+                      double *rij)
+{
+    for (int x = 0; x < 3; ++x) {
+        double as = 0.0;
+        as = as + cell[3 * 0 + x] * (double)dc[3 * jn + 0];
+        as = as + cell[3 * 1 + x] * (double)dc[3 * jn + 1];
+        as = as + cell[3 * 2 + x] * (double)dc[3 * jn + 2];
+        rij[x] = (r[3 * j + x] - r[3 * i + x]) - as;
+    }
+}
+
+/* loop1 of the screened kernels for the list entry jn of atom i (r^2 = rl):
+ * returns 1 and fills *b (and appends records) when the bond is kept */
+static int screened_bond(const orc_tersoff_params *p, int i, int64_t jn, const double *rij, double rl,
+                         int t, const double *r, const double *cell, const int64_t *offsets,
+                         const int *nbr, const int *dc, bond_t *b, scr_t **rec, int64_t *nrec,
+                         int64_t *cap)
+{
+    const double cin_l = p->r1[t], cin_h = p->r2[t];
+    double mx = cin_h;
+    if (p->or2[t] > mx) mx = p->or2[t];
+    if (p->bor2[t] > mx) mx = p->bor2[t];
+    const double max_cut_sq = mx * mx;
+    const double Cmin = p->Cmin[t], Cmax = p->Cmax[t], dC = Cmax - Cmin;
+    const double C_dr_cut = Cmax > 2.0 ? Cmax * Cmax / (4 * (Cmax - 1)) : 1.0;
+    const double thr = log(1e-6), dot_thr = 1e-10;
+    b->s0 = b->s1 = *nrec;
+    if (rl < cin_l * cin_l) {
+        /* region (a): no screening */
+        b->fc = 1.0; b->dfc = 0.0; b->fcbo = 1.0; b->dfcbo = 0.0;
+        b->len = sqrt(rl);
+        return 1;
+    }
+    if (rl < max_cut_sq && p->or1[t] < p->or2[t]) {
+        int screened = 0, need = 0;
+        double sij = 0.0, dsijdrij = 0.0;
+        const int64_t ineb = *nrec;
+        for (int64_t kn = offsets[i]; !(screened || sij < thr) && kn < offsets[i + 1]; ++kn) {
+            const int k = nbr[kn];
+            double rik[3];
+            entry_vec(r, cell, dc, i, k, kn, rik);
+            if (!(rik[0] * rik[0] + rik[1] * rik[1] + rik[2] * rik[2] < C_dr_cut * rl)) continue;
+            if (k == nbr[jn] && dc[3 * kn] == dc[3 * jn] && dc[3 * kn + 1] == dc[3 * jn + 1] &&
+                dc[3 * kn + 2] == dc[3 * jn + 2])
+                continue;
+            const double dot_ij_ik = rij[0] * rik[0] + rij[1] * rik[1] + rij[2] * rik[2];
+            const double rlik = rik[0] * rik[0] + rik[1] * rik[1] + rik[2] * rik[2];
+            double rjk[3];
+            for (int x = 0; x < 3; ++x) rjk[x] = -rij[x] + rik[x];
+            const double dot_ij_jk = rij[0] * rjk[0] + rij[1] * rjk[1] + rij[2] * rjk[2];
+            const double rljk = rjk[0] * rjk[0] + rjk[1] * rjk[1] + rjk[2] * rjk[2];
+            if (!(dot_ij_ik > dot_thr && dot_ij_jk < -dot_thr)) continue;
+            const double xik = rlik / rl, xjk = rljk / rl;
+            const double xm = xik - xjk, xp = xik + xjk;
+            double fac = 1.0 / (1 - xm * xm);
+            const double C = (2 * xp - xm * xm - 1) * fac;
+            if (C <= Cmin) {
+                screened = 1;
+            } else if (C < Cmax) {
+                need = 1;
+                const double Cmax_C = Cmax - C, C_Cmin = C - Cmin;
+                sij = sij - (Cmax_C / C_Cmin) * (Cmax_C / C_Cmin);
+                const double dCdrik = 4 * xik * fac * (1 + (C - 1) * xm);
+                const double dCdrjk = 4 * xjk * fac * (1 - (C - 1) * xm);
+                const double dCdrij = -(dCdrik + dCdrjk);
+                fac = 2 * Cmax_C * dC / (C_Cmin * C_Cmin * C_Cmin);
+                dsijdrij = dsijdrij + fac * dCdrij;
+                if (*nrec >= *cap) {
+                    *cap = *cap * 2 + 64;
+                    *rec = realloc(*rec, sizeof(scr_t) * (size_t)*cap);
+                }
+                scr_t *q = &(*rec)[(*nrec)++];
+                q->kn = kn;
+                q->arik = fac * dCdrik / rlik;
+                q->arjk = fac * dCdrjk / rljk;
+            }
+        }
+        if ((screened || sij < thr) && rl > cin_h * cin_h) {
+            *nrec = ineb;           /* fully screened: dropped */
+            return 0;
+        }
+        const double len = sqrt(rl);
+        b->len = len;
+        double fcin, dfcin;
+        exp_cut(cin_l, cin_h, len, &fcin, &dfcin);
+        if (screened) {
+            b->fc = fcin; b->dfc = dfcin; b->fcbo = fcin; b->dfcbo = dfcin;
+            *nrec = ineb;
+        } else if (need) {
+            sij = exp(sij);
+            double fcar, dfcar, fcbo, dfcbo;
+            exp_cut(p->or1[t], p->or2[t], len, &fcar, &dfcar);
+            exp_cut(p->bor1[t], p->bor2[t], len, &fcbo, &dfcbo);
+            b->fc = (1.0 - fcin) * sij * fcar + fcin;
+            b->dfc = (1.0 - fcin) * sij * (dfcar + fcar * dsijdrij / len) - dfcin * sij * fcar + dfcin;
+            b->fcbo = (1.0 - fcin) * sij * fcbo + fcin;
+            b->dfcbo = (1.0 - fcin) * sij * (dfcbo + fcbo * dsijdrij / len) - dfcin * sij * fcbo + dfcin;
+            for (int64_t q = ineb; q < *nrec; ++q) {
+                scr_t *e = &(*rec)[q];
+                e->boik = e->arik * sij * fcbo * (1.0 - fcin);
+                e->bojk = e->arjk * sij * fcbo * (1.0 - fcin);
+                e->arik = e->arik * sij * fcar * (1.0 - fcin);
+                e->arjk = e->arjk * sij * fcar * (1.0 - fcin);
+            }
+        } else {
+            double fcar, dfcar, fcbo, dfcbo;
+            exp_cut(p->or1[t], p->or2[t], len, &fcar, &dfcar);
+            exp_cut(p->bor1[t], p->bor2[t], len, &fcbo, &dfcbo);
+            if (len < cin_h) {
+                b->fc = (1.0 - fcin) * fcar + fcin;
+                b->dfc = (1.0 - fcin) * dfcar - dfcin * fcar + dfcin;
+                b->fcbo = (1.0 - fcin) * fcbo + fcin;
+                b->dfcbo = (1.0 - fcin) * dfcbo - dfcin * fcbo + dfcin;
+            } else {
+                b->fc = fcar; b->dfc = dfcar; b->fcbo = fcbo; b->dfcbo = dfcbo;
+            }
+        }
+        b->s0 = ineb;
+        b->s1 = *nrec;
+        return 1;
+    }
+    if (rl < cin_h * cin_h) {
+        /* beyond the screening range: inner cutoff only */
+        b->len = sqrt(rl);
+        exp_cut(cin_l, cin_h, b->len, &b->fc, &b->dfc);
+        b->fcbo = b->fc;
+        b->dfcbo = b->dfc;
+        return 1;
+    }
+    return 0;
+}
 
 static void trig_off(double r1, double r2, double r, double *val, double *dval)
 {
@@ -98,6 +262,8 @@ int orc_tersoff(const orc_tersoff_params *p, int nat, const double *r,
     int64_t *bseed = malloc(sizeof(int64_t) * (nat + 1));
     int64_t *blast = malloc(sizeof(int64_t) * (nat + 1));
     int nebmax = 0;
+    scr_t *rec = NULL;
+    int64_t nrec = 0, rcap = 0;
     for (int i = 0; i < nat; ++i) {
         bseed[i] = offsets[i];
         int64_t nb = offsets[i];
@@ -106,16 +272,19 @@ int orc_tersoff(const orc_tersoff_params *p, int nat, const double *r,
                 int j = nbr[jn];
                 if (el[j] < 0) continue;
                 double rij[3];
-                for (int x = 0; x < 3; ++x) {
-                    double as = 0.0;
-                    as = as + cell[3 * 0 + x] * (double)dc[3 * jn + 0];
-                    as = as + cell[3 * 1 + x] * (double)dc[3 * jn + 1];
-                    as = as + cell[3 * 2 + x] * (double)dc[3 * jn + 2];
-                    rij[x] = (r[3 * j + x] - r[3 * i + x]) - as;
-                }
+                entry_vec(r, cell, dc, i, j, jn, rij);
                 double rl = rij[0] * rij[0] + rij[1] * rij[1] + rij[2] * rij[2];
                 int t = pair_index(el[i], el[j], nel);
-                if (rl < p->r2[t] * p->r2[t]) {
+                if (p->scr) {
+                    bond_t *b = &bonds[nb];
+                    if (screened_bond(p, i, jn, rij, rl, t, r, cell, offsets, nbr, dc, b, &rec, &nrec,
+                                      &rcap)) {
+                        b->j = j; b->typ = t;
+                        b->dc[0] = dc[3 * jn]; b->dc[1] = dc[3 * jn + 1]; b->dc[2] = dc[3 * jn + 2];
+                        for (int x = 0; x < 3; ++x) b->nm[x] = rij[x] / b->len;
+                        ++nb;
+                    }
+                } else if (rl < p->r2[t] * p->r2[t]) {
                     bond_t *b = &bonds[nb];
                     b->j = j; b->typ = t;
                     b->dc[0] = dc[3 * jn]; b->dc[1] = dc[3 * jn + 1]; b->dc[2] = dc[3 * jn + 2];
@@ -123,6 +292,9 @@ int orc_tersoff(const orc_tersoff_params *p, int nat, const double *r,
                     for (int x = 0; x < 3; ++x) b->nm[x] = rij[x] / b->len;
                     if (rl < p->r1[t] * p->r1[t]) { b->fc = 1.0; b->dfc = 0.0; }
                     else trig_off(p->r1[t], p->r2[t], b->len, &b->fc, &b->dfc);
+                    b->fcbo = b->fc;
+                    b->dfcbo = b->dfc;
+                    b->s0 = b->s1 = 0;
                     ++nb;
                 }
             }
@@ -137,6 +309,8 @@ int orc_tersoff(const orc_tersoff_params *p, int nat, const double *r,
     int nt = 1;
     (void)nthreads;
 #endif
+    /* bond-order screening factor per bond (sfacbo of bop_kernel.f90:1456) */
+    double *sfac = calloc((size_t)(offsets[nat] > 0 ? offsets[nat] : 1), sizeof(double));
     double *fbuf = calloc((size_t)nt * nat * 3 + 1, sizeof(double));
     double *pebuf = calloc((size_t)nt * nat + 1, sizeof(double));
     double *wbuf = calloc((size_t)nt * 9, sizeof(double));
@@ -153,6 +327,7 @@ int orc_tersoff(const orc_tersoff_params *p, int nat, const double *r,
         double *pe = pebuf + (size_t)tid * nat;
         double *w = wbuf + 9 * tid;
         double (*dbidk)[3] = malloc(sizeof(double[3]) * (nebmax + 1));
+        double *zfac = malloc(sizeof(double) * (nebmax + 1));
 #pragma omp for schedule(static)
         for (int i = 0; i < nat; ++i) {
             int eli = el[i];
@@ -168,7 +343,8 @@ int orc_tersoff(const orc_tersoff_params *p, int nat, const double *r,
                 }
                 int el2ij = bnd_ij->typ;
                 double rlij = bnd_ij->len;
-                if (!(maskfac > 0 && rlij < p->r2[el2ij])) continue;
+                /* cut_ar_h is the outer cutoff or2 when screened */
+                if (!(maskfac > 0 && rlij < (p->scr ? p->or2[el2ij] : p->r2[el2ij]))) continue;
                 double fj[3] = {0, 0, 0};
                 int elj = el[j];
                 (void)elj;
@@ -206,12 +382,13 @@ int orc_tersoff(const orc_tersoff_params *p, int nat, const double *r,
                     const bond_t *bik = &bonds[ik];
                     dbidk[ikc][0] = dbidk[ikc][1] = dbidk[ikc][2] = 0.0;
                     if (ik == ij) continue;
+                    zfac[ikc] = 0.0;
                     int ikpot = bik->typ;
                     double rlik = bik->len;
-                    if (!(rlik < p->r2[ikpot])) continue;
+                    if (!(rlik < (p->scr ? p->bor2[ikpot] : p->r2[ikpot]))) continue;
                     const double *rnik = bik->nm;
                     double rik[3] = {rlik * rnik[0], rlik * rnik[1], rlik * rnik[2]};
-                    double fcik = bik->fc, dfcikr = bik->dfc;
+                    double fcik = bik->fcbo, dfcikr = bik->dfcbo;
                     double h_Dr, dh_dDr;
                     h_fn(p, ikpot, rlij - rlik, &h_Dr, &dh_dDr);
                     double costh = rnik[0] * rnij[0] + rnik[1] * rnij[1] + rnik[2] * rnij[2];
@@ -250,6 +427,7 @@ int orc_tersoff(const orc_tersoff_params *p, int nat, const double *r,
                     double dcsdik = rlijr - costh / rlik;
                     double dcsdjk = -disjk * rlijr / rlik;
                     double dzfac = fcik * dg_dcosth * h_Dr;
+                    zfac[ikc] = g_costh * h_Dr;
                     zij = zij + fcik * g_costh * h_Dr;
                     double dzdrij = g_costh * fcik * dh_dDr;
                     double dzdrik = g_costh * (dfcikr * h_Dr - fcik * dh_dDr);
@@ -322,8 +500,35 @@ int orc_tersoff(const orc_tersoff_params *p, int nat, const double *r,
                     const bond_t *bik = &bonds[ik];
                     int k = bik->j;
                     if (k != j || bik->dc[0] != bnd_ij->dc[0] || bik->dc[1] != bnd_ij->dc[1] ||
-                        bik->dc[2] != bnd_ij->dc[2])
+                        bik->dc[2] != bnd_ij->dc[2]) {
                         for (int x = 0; x < 3; ++x) ft[3 * k + x] += -dbij_dzij * dbidk[ikc][x];
+                        if (p->scr) sfac[ik] += zfac[ikc] * dbij_dzij;
+                    }
+                }
+                if (p->scr) {
+                    /* forces of the pair-term screening (bop_kernel.f90:1469-1490) */
+                    double dff = 0.5 * (VRij + bij * VAij);
+                    for (int64_t q = bnd_ij->s0; q < bnd_ij->s1; ++q) {
+                        const scr_t *e = &rec[q];
+                        int k = nbr[e->kn];
+                        double rik[3], rjk[3], d1[3], d2[3];
+                        entry_vec(r, cell, dc, i, k, e->kn, rik);
+                        for (int x = 0; x < 3; ++x) rjk[x] = -rij[x] + rik[x];
+                        for (int x = 0; x < 3; ++x) {
+                            d1[x] = dff * e->arik * rik[x];
+                            fi[x] += d1[x];
+                            ft[3 * k + x] += -d1[x];
+                        }
+                        for (int a = 0; a < 3; ++a)
+                            for (int b = 0; b < 3; ++b) wij[a + 3 * b] += rik[a] * d1[b];
+                        for (int x = 0; x < 3; ++x) {
+                            d2[x] = dff * e->arjk * rjk[x];
+                            fj[x] += d2[x];
+                            ft[3 * k + x] += -d2[x];
+                        }
+                        for (int a = 0; a < 3; ++a)
+                            for (int b = 0; b < 3; ++b) wij[a + 3 * b] += rjk[a] * d2[b];
+                    }
                 }
                 for (int a = 0; a < 9; ++a) w[a] += wij[a];
                 if (wpa) {
@@ -337,7 +542,55 @@ int orc_tersoff(const orc_tersoff_params *p, int nat, const double *r,
             }
             for (int x = 0; x < 3; ++x) ft[3 * i + x] += fi[x];
         }
+        if (p->scr) {
+            /* forces of the bond-order screening (bop_kernel.f90:1531-1609);
+             * the omp for above ends with a barrier, so sfac is complete */
+#pragma omp for schedule(static)
+            for (int i = 0; i < nat; ++i) {
+                if (el[i] < 0) continue;
+                double fi[3] = {0, 0, 0};
+                for (int64_t ij = bseed[i]; ij < blast[i]; ++ij) {
+                    const bond_t *b = &bonds[ij];
+                    int j = b->j;
+                    double fj[3] = {0, 0, 0}, wij[9] = {0};
+                    double rij[3] = {b->len * b->nm[0], b->len * b->nm[1], b->len * b->nm[2]};
+                    for (int64_t q = b->s0; q < b->s1; ++q) {
+                        const scr_t *e = &rec[q];
+                        double cik = sfac[ij] * e->boik, cjk = sfac[ij] * e->bojk;
+                        int k = nbr[e->kn];
+                        double rik[3], rjk[3], d1[3], d2[3];
+                        entry_vec(r, cell, dc, i, k, e->kn, rik);
+                        for (int x = 0; x < 3; ++x) rjk[x] = -rij[x] + rik[x];
+                        for (int x = 0; x < 3; ++x) {
+                            d1[x] = cik * rik[x];
+                            fi[x] += d1[x];
+                            ft[3 * k + x] += -d1[x];
+                        }
+                        for (int a = 0; a < 3; ++a)
+                            for (int c = 0; c < 3; ++c) wij[a + 3 * c] += rik[a] * d1[c];
+                        for (int x = 0; x < 3; ++x) {
+                            d2[x] = cjk * rjk[x];
+                            fj[x] += d2[x];
+                            ft[3 * k + x] += -d2[x];
+                        }
+                        for (int a = 0; a < 3; ++a)
+                            for (int c = 0; c < 3; ++c) wij[a + 3 * c] += rjk[a] * d2[c];
+                    }
+                    for (int a = 0; a < 9; ++a) w[a] += wij[a];
+                    if (wpa) {
+#pragma omp critical
+                        for (int a = 0; a < 9; ++a) {
+                            wpa[9 * i + a] += wij[a] / 2;
+                            wpa[9 * j + a] += wij[a] / 2;
+                        }
+                    }
+                    for (int x = 0; x < 3; ++x) ft[3 * j + x] += fj[x];
+                }
+                for (int x = 0; x < 3; ++x) ft[3 * i + x] += fi[x];
+            }
+        }
         free(dbidk);
+        free(zfac);
     }
     /* tls_reduce model: thread buffers summed in thread order */
     double esum = 0.0;
@@ -352,5 +605,6 @@ int orc_tersoff(const orc_tersoff_params *p, int nat, const double *r,
     }
     free(fbuf); free(pebuf); free(wbuf);
     free(bonds); free(bseed); free(blast);
+    free(sfac); free(rec);
     return ORC_OK;
 }

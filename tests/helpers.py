@@ -135,3 +135,17 @@ def in_vacuum_box(pos, vacuum=5.0):
     on every side."""
     pos = pos - pos.min(0) + vacuum
     return pos, np.diag(pos.max(0) + vacuum), np.array([False] * 3)
+
+
+def bop_cutoff(params):
+    """Neighbour-list range of a bond-order parameter dict: r2 unscreened; for
+    the screened variants sqrt(C_dr_cut) times the largest of r2, or2, bor2
+    over the pairs, C_dr_cut = Cmax^2 / (4 (Cmax - 1)) for Cmax > 2, else 1
+    (default_bind_to_func.f90:44-150)."""
+    r2 = np.atleast_1d(params["r2"]).astype(float)
+    if "or1" not in params:
+        return float(r2.max())
+    cm = np.atleast_1d(params["Cmax"]).astype(float)
+    x = np.where(cm > 2.0, cm ** 2 / (4 * (cm - 1)), 1.0)
+    mx = max(r2.max(), np.atleast_1d(params["or2"]).max(), np.atleast_1d(params["bor2"]).max())
+    return float(np.sqrt(x.max()) * mx)
