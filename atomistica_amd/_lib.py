@@ -32,6 +32,9 @@ KIND_EWALD = 3
 KIND_REBO2 = 4
 KIND_KUMAGAI = 5
 KIND_BRENNER = 6
+KIND_TERSOFF_SCR = 7
+KIND_KUMAGAI_SCR = 8
+KIND_BRENNER_SCR = 9
 REBO2_ZERO_TABLES = 1
 REBO2_DIHEDRAL = 2
 
@@ -77,6 +80,10 @@ class BrennerParams(ctypes.Structure):
         [(k, ctypes.c_double * 6) for k in ("r1", "r2")]
 
 
+class ScreeningParams(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_double * 6) for k in ("or1", "or2", "bor1", "bor2", "Cmin", "Cmax")]
+
+
 # name -> argtypes (every function returns int unless listed in _RESTYPE)
 PROTOTYPES = {
     "agpu_version": [],
@@ -113,6 +120,9 @@ PROTOTYPES = {
     "agpu_tersoff_set_params": [vp, ctypes.POINTER(TersoffParams)],
     "agpu_kumagai_create": [ctypes.POINTER(vp), ctypes.POINTER(KumagaiParams)],
     "agpu_brenner_create": [ctypes.POINTER(vp), ctypes.POINTER(BrennerParams)],
+    "agpu_tersoff_scr_create": [ctypes.POINTER(vp), vp, vp],
+    "agpu_kumagai_scr_create": [ctypes.POINTER(vp), vp, vp],
+    "agpu_brenner_scr_create": [ctypes.POINTER(vp), vp, vp],
     "agpu_eam_alloy_create": [ctypes.POINTER(vp), ctypes.c_char_p, ctypes.c_char_p],
     "agpu_ewald_create": [ctypes.POINTER(vp), d, d, d],
     "agpu_rebo2_create": [ctypes.POINTER(vp), i],

@@ -40,6 +40,7 @@ enum DevErr : int {
     DERR_STAGE_OVERFLOW = 16,
     DERR_FIXED_RANGE = 32,
     DERR_SORT_FALLBACK = 64,
+    DERR_SCR_ROW = 128,           // screened bond order: row beyond the staged length
 };
 
 hipStream_t stream();

@@ -635,7 +635,10 @@ int agpu_potential_bind_to(agpu_potential *pot, agpu_particles *p, agpu_neighbor
     switch (pot->kind) {
     case AGPU_KIND_TERSOFF:
     case AGPU_KIND_KUMAGAI:
-    case AGPU_KIND_BRENNER: rc = tersoff_bind(pot); break;
+    case AGPU_KIND_BRENNER:
+    case AGPU_KIND_TERSOFF_SCR:
+    case AGPU_KIND_KUMAGAI_SCR:
+    case AGPU_KIND_BRENNER_SCR: rc = tersoff_bind(pot); break;
     case AGPU_KIND_EAM_ALLOY: rc = eam_bind(pot); break;
     case AGPU_KIND_EWALD: rc = ewald_bind(pot); break;
     case AGPU_KIND_REBO2: rc = rebo2_bind(pot); break;
@@ -656,7 +659,10 @@ static int stage_and_call(agpu_potential *pot, agpu_particles *p, agpu_neighbors
         switch (pot->kind) {
         case AGPU_KIND_TERSOFF:
         case AGPU_KIND_KUMAGAI:
-        case AGPU_KIND_BRENNER: rc = tersoff_compute(pot, &a); break;
+        case AGPU_KIND_BRENNER:
+        case AGPU_KIND_TERSOFF_SCR:
+        case AGPU_KIND_KUMAGAI_SCR:
+        case AGPU_KIND_BRENNER_SCR: rc = tersoff_compute(pot, &a); break;
         case AGPU_KIND_EAM_ALLOY: rc = eam_compute(pot, &a); break;
         case AGPU_KIND_EWALD: rc = ewald_compute(pot, &a); break;
         case AGPU_KIND_REBO2: rc = rebo2_compute(pot, &a); break;
@@ -702,7 +708,10 @@ static int stage_and_call(agpu_potential *pot, agpu_particles *p, agpu_neighbors
     switch (pot->kind) {
     case AGPU_KIND_TERSOFF:
     case AGPU_KIND_KUMAGAI:
-    case AGPU_KIND_BRENNER: rc = tersoff_compute(pot, &d); break;
+    case AGPU_KIND_BRENNER:
+    case AGPU_KIND_TERSOFF_SCR:
+    case AGPU_KIND_KUMAGAI_SCR:
+    case AGPU_KIND_BRENNER_SCR: rc = tersoff_compute(pot, &d); break;
     case AGPU_KIND_EAM_ALLOY: rc = eam_compute(pot, &d); break;
     case AGPU_KIND_EWALD: rc = ewald_compute(pot, &d); break;
     case AGPU_KIND_REBO2: rc = rebo2_compute(pot, &d); break;

@@ -20,9 +20,12 @@
 //
 // Precision: R = double (FP64) or float (mixed: FP32 pair/triplet arithmetic,
 // FP64 position differences and FP64 accumulation of forces, energy, virial).
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
+
+#include <rocprim/device/device_scan.hpp>
 
 #include "agpu_internal.hpp"
 #include "profile.hpp"
@@ -50,6 +53,8 @@ struct TersoffDev {
     double br0[6], bVRf[6], bVAf[6], bexpR[6], bexpA[6], bgam[6], bc2[6], bd2[6], bh[6], bnn[6], bboe[6];
 };
 
+struct ScrDev;
+
 struct TersoffState {
     agpu_tersoff_params prm;
     TersoffDev dev;
@@ -57,6 +62,13 @@ struct TersoffState {
     int z2db[128];
     uint64_t Z_rev = ~0ull;
     DevBuf bslot, bcnt, rev, fslot, fown, eslot, wslot, partial, stats;
+    // screened variants: parameters (host copy of ScrDev) and per-entry /
+    // per-record buffers
+    bool scr = false;
+    agpu_screening_params sp;
+    DevBuf keep, rcnt, roff, fcv, rkn, rco, frec, fsl, nlrev, scan_tmp;
+    int64_t nlrev_upd = -1;
+    const agpu_neighbors *nlrev_nl = nullptr;
 };
 
 __host__ __device__ __forceinline__ int pair_index(int i, int j, int nel)
@@ -582,6 +594,586 @@ k_bop_gather(int nat, const int *__restrict__ offsets, const int *__restrict__ b
 }
 
 // -------------------------------------------------------------------------
+// screened variants (TersoffScr, KumagaiScr, BrennerScr: SCREENING in
+// bop_kernel.f90).  Every list entry i->j (image n) inside the screening
+// range gets the screening function of the bond from the third atoms k of
+// i's row (loop1, bop_kernel.f90:563-1066), in the reference's row order and
+// with its early exit:
+//   k_scr_screen<false>  keep flag and number of screening records per entry
+//   (exclusive scan of the record counts)
+//   k_scr_screen<true>   pair and bond-order cutoffs (value, d/dr) per entry
+//                        and the records (k's entry, d fc_ar / d r_ik,
+//                        d fc_ar / d r_jk, and the same for fc_bo)
+//   k_scr_bonds          kept entries of every row -> the bond list
+//   k_bop_scr            the bond-order kernel of k_tersoff with these
+//                        cutoffs, plus the screening forces: for the pair
+//                        term (:1448-1492) and, with the bond's factor
+//                        sum_j g h dE/dzeta_ij, for the bond order
+//                        (:1531-1609); the force a record puts on its k is
+//                        written per record
+//   k_scr_slots          per entry of i's row: the total force i's terms put
+//                        on that neighbour (bond slot + records naming it)
+//   k_scr_gather         f_a = own + sum over a's row of the slot the
+//                        neighbour wrote for a (through the reverse entry)
+// No atomics: every sum has one owner and a fixed order.
+// -------------------------------------------------------------------------
+struct ScrDev {
+    double r1[6], r2[6], or1[6], or2[6], bor1[6], bor2[6];
+    double fin[6], fout[6], fbo[6];            // 1 / (r2 - r1) of the three cutoffs
+    double Cmin[6], Cmax[6], dC[6], C_dr_cut[6], max_cut_sq[6];
+    double ec, ed, efac2, eoff;                // exp_cutoff constants
+};
+
+constexpr int SCR_ROW = 256;       // row entries staged per atom
+constexpr int SCR_WAVES = 4;       // atoms (wavefronts) per block
+
+// exp_cutoff_f (cutoff.f90:271-293)
+__device__ __forceinline__ void exp_cut(const ScrDev &S, double r1, double r2, double fac1, double r, double &val,
+                                        double &dval)
+{
+    if (r <= r1) {
+        val = 1.0;
+        dval = 0.0;
+    } else if (r >= r2) {
+        val = 0.0;
+        dval = 0.0;
+    } else {
+        const double x = fac1 * (r - r1), x2 = x * x;
+        const double v = exp(-8 * x * x2);
+        const double dv = -24 * x2 * v;
+        dval = fac1 * S.efac2 * (dv + 3 * S.ec * x2 + 4 * S.ed * x * x2);
+        val = S.efac2 * (v + S.ec * x * x2 + S.ed * x2 * x2 - S.eoff);
+    }
+}
+
+struct ScrShared {
+    double x[SCR_ROW], y[SCR_ROW], z[SCR_ROW];
+    int2 e[SCR_ROW];
+};
+
+// One wavefront per atom, lanes over the row's entries; the row's vectors
+// r_j - r_i - A.n are staged in LDS.  FILL = false: keep flag and record
+// count; FILL = true: cutoffs and records at roff.
+template <bool FILL>
+__global__ void __launch_bounds__(256)
+k_scr_screen(ScrDev S, int nat_tot, const double *__restrict__ r, const int *__restrict__ el, int nel, CellArg C,
+             const int *__restrict__ offsets, const int2 *__restrict__ nbr, int *__restrict__ keep,
+             int *__restrict__ rcnt, const int *__restrict__ roff, double4 *__restrict__ fcv,
+             int *__restrict__ rkn, double4 *__restrict__ rco, int *__restrict__ flags)
+{
+    __shared__ ScrShared sh[SCR_WAVES];
+    ScrShared &L = sh[threadIdx.x >> 6];
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * SCR_WAVES + (threadIdx.x >> 6);
+    if (i >= nat_tot) return;                       // uniform per wave
+    const int beg = offsets[i], n = offsets[i + 1] - beg;
+    if (n > SCR_ROW) {
+        if (lane == 0) atomicOr(flags, DERR_SCR_ROW);
+        return;
+    }
+    const int eli = el[i];
+    for (int t = lane; t < n; t += 64) {
+        const int2 e = nbr[beg + t];
+        int a, b, c;
+        dc_unpack(e.y, a, b, c);
+        L.x[t] = (r[3 * e.x + 0] - r[3 * i + 0]) - image_x(C, 0, a, b, c);
+        L.y[t] = (r[3 * e.x + 1] - r[3 * i + 1]) - image_x(C, 1, a, b, c);
+        L.z[t] = (r[3 * e.x + 2] - r[3 * i + 2]) - image_x(C, 2, a, b, c);
+        L.e[t] = e;
+    }
+    __builtin_amdgcn_wave_barrier();
+    const double thr = -13.815510557964274;        // log(1e-6)
+    const double dot_thr = 1e-10;
+    for (int t = lane; t < n; t += 64) {
+        const int s = beg + t;
+        const int2 ej = L.e[t];
+        const int elj = el[ej.x];
+        int kept = 0, cnt = 0;
+        double fcar = 0.0, dfcar = 0.0, fcbo = 0.0, dfcbo = 0.0;
+        if (eli >= 0 && elj >= 0) {
+            const int typ = pair_index(eli, elj, nel);
+            const double rij[3] = {L.x[t], L.y[t], L.z[t]};
+            const double rl = rij[0] * rij[0] + rij[1] * rij[1] + rij[2] * rij[2];
+            const double cin_l = S.r1[typ], cin_h = S.r2[typ];
+            // records go to [roff[s], roff[s] + rcnt[s]) as counted by pass 1:
+            // an entry that ends up dropped or fully screened counted none
+            const int q0 = FILL ? roff[s] : 0;
+            const int qcap = FILL ? rcnt[s] : 0;
+            if (rl < cin_l * cin_l) {
+                // region (a): not screened
+                kept = 1;
+                fcar = fcbo = 1.0;
+            } else if (rl < S.max_cut_sq[typ] && S.or1[typ] < S.or2[typ]) {
+                bool screened = false, need = false;
+                double sij = 0.0, dsijdrij = 0.0;
+                const double Cmin = S.Cmin[typ], Cmax = S.Cmax[typ], dC = S.dC[typ];
+                const double lim = S.C_dr_cut[typ] * rl;
+                for (int u = 0; !(screened || sij < thr) && u < n; ++u) {
+                    const double rik[3] = {L.x[u], L.y[u], L.z[u]};
+                    const double rlik = rik[0] * rik[0] + rik[1] * rik[1] + rik[2] * rik[2];
+                    if (!(rlik < lim)) continue;
+                    const int2 ek = L.e[u];
+                    if (ek.x == ej.x && ek.y == ej.y) continue;
+                    const double dot_ij_ik = rij[0] * rik[0] + rij[1] * rik[1] + rij[2] * rik[2];
+                    const double rjk[3] = {-rij[0] + rik[0], -rij[1] + rik[1], -rij[2] + rik[2]};
+                    const double dot_ij_jk = rij[0] * rjk[0] + rij[1] * rjk[1] + rij[2] * rjk[2];
+                    const double rljk = rjk[0] * rjk[0] + rjk[1] * rjk[1] + rjk[2] * rjk[2];
+                    if (!(dot_ij_ik > dot_thr && dot_ij_jk < -dot_thr)) continue;
+                    const double xik = rlik / rl, xjk = rljk / rl;
+                    const double xm = xik - xjk, xp = xik + xjk;
+                    double fac = 1.0 / (1 - xm * xm);
+                    const double Cv = (2 * xp - xm * xm - 1) * fac;
+                    if (Cv <= Cmin) {
+                        screened = true;
+                    } else if (Cv < Cmax) {
+                        need = true;
+                        const double Cmax_C = Cmax - Cv, C_Cmin = Cv - Cmin;
+                        sij = sij - (Cmax_C / C_Cmin) * (Cmax_C / C_Cmin);
+                        const double dCdrik = 4 * xik * fac * (1 + (Cv - 1) * xm);
+                        const double dCdrjk = 4 * xjk * fac * (1 - (Cv - 1) * xm);
+                        fac = 2 * Cmax_C * dC / (C_Cmin * C_Cmin * C_Cmin);
+                        dsijdrij = dsijdrij + fac * -(dCdrik + dCdrjk);
+                        if (FILL && cnt < qcap) {
+                            rkn[q0 + cnt] = beg + u;
+                            rco[q0 + cnt] = make_double4(fac * dCdrik / rlik, fac * dCdrjk / rljk, 0.0, 0.0);
+                        }
+                        ++cnt;
+                    }
+                }
+                if ((screened || sij < thr) && rl > cin_h * cin_h) {
+                    cnt = 0;              // fully screened: dropped
+                } else {
+                    kept = 1;
+                    const double len = sqrt(rl);
+                    double fcin, dfcin;
+                    exp_cut(S, cin_l, cin_h, S.fin[typ], len, fcin, dfcin);
+                    double fo, dfo, fb, dfb;
+                    exp_cut(S, S.or1[typ], S.or2[typ], S.fout[typ], len, fo, dfo);
+                    exp_cut(S, S.bor1[typ], S.bor2[typ], S.fbo[typ], len, fb, dfb);
+                    if (screened) {
+                        fcar = fcbo = fcin;
+                        dfcar = dfcbo = dfcin;
+                        cnt = 0;
+                    } else if (need) {
+                        const double sv = exp(sij);
+                        fcar = (1.0 - fcin) * sv * fo + fcin;
+                        dfcar = (1.0 - fcin) * sv * (dfo + fo * dsijdrij / len) - dfcin * sv * fo + dfcin;
+                        fcbo = (1.0 - fcin) * sv * fb + fcin;
+                        dfcbo = (1.0 - fcin) * sv * (dfb + fb * dsijdrij / len) - dfcin * sv * fb + dfcin;
+                        if (FILL) {
+                            const double car = sv * fo * (1.0 - fcin), cbo = sv * fb * (1.0 - fcin);
+                            for (int q = 0; q < min(cnt, qcap); ++q) {
+                                const double4 v = rco[q0 + q];
+                                rco[q0 + q] = make_double4(v.x * car, v.y * car, v.x * cbo, v.y * cbo);
+                            }
+                        }
+                    } else if (len < cin_h) {
+                        fcar = (1.0 - fcin) * fo + fcin;
+                        dfcar = (1.0 - fcin) * dfo - dfcin * fo + dfcin;
+                        fcbo = (1.0 - fcin) * fb + fcin;
+                        dfcbo = (1.0 - fcin) * dfb - dfcin * fb + dfcin;
+                    } else {
+                        fcar = fo;
+                        dfcar = dfo;
+                        fcbo = fb;
+                        dfcbo = dfb;
+                    }
+                }
+            } else if (rl < cin_h * cin_h) {
+                // beyond the screening range: the inner cutoff only
+                kept = 1;
+                exp_cut(S, cin_l, cin_h, S.fin[typ], sqrt(rl), fcar, dfcar);
+                fcbo = fcar;
+                dfcbo = dfcar;
+            }
+        }
+        if (FILL) {
+            if (kept) fcv[s] = make_double4(fcar, dfcar, fcbo, dfcbo);
+        } else {
+            keep[s] = kept;
+            rcnt[s] = kept ? cnt : 0;
+        }
+    }
+}
+
+// thread per atom: the kept entries of its row, in row order
+__global__ void k_scr_bonds(int nat, const int *__restrict__ offsets, const int *__restrict__ keep,
+                            int *__restrict__ bslot, int *__restrict__ bcnt)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nat) return;
+    const int beg = offsets[i], end = offsets[i + 1];
+    int nb = 0;
+    for (int s = beg; s < end; ++s)
+        if (keep[s]) bslot[beg + nb++] = s;
+    bcnt[i] = nb;
+}
+
+// reverse entry of every list entry: i->j (image n) -> j->i (image -n)
+__global__ void k_nl_rev_entries(int nat, const int *__restrict__ offsets, const int2 *__restrict__ nbr,
+                                 int *__restrict__ rev, int *__restrict__ flags)
+{
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= nat) return;
+    const int beg = offsets[i], end = offsets[i + 1];
+    for (int s = beg + lane; s < end; s += 64) {
+        const int2 e = nbr[s];
+        int a, b, c;
+        dc_unpack(e.y, a, b, c);
+        const int want = dc_pack(-a, -b, -c);
+        int found = -1;
+        for (int q = offsets[e.x]; q < offsets[e.x + 1]; ++q) {
+            const int2 f = nbr[q];
+            if (f.x == i && f.y == want) {
+                found = q;
+                break;
+            }
+        }
+        rev[s] = found;
+        if (found < 0) atomicOr(flags, DERR_BOND_OVERFLOW);
+    }
+}
+
+template <typename R, int G, int FAM>
+__global__ void __launch_bounds__(256)
+k_bop_scr(TersoffDev P, ScrDev S, int nat, const int *__restrict__ order, const double *__restrict__ r,
+          const int *__restrict__ el, CellArg C, const int *__restrict__ offsets, const int2 *__restrict__ nbr,
+          const int *__restrict__ bslot, const int *__restrict__ bcnt, const int *__restrict__ mask,
+          const double4 *__restrict__ fcv, const int *__restrict__ roff, const int *__restrict__ rkn,
+          const double4 *__restrict__ rco, double *__restrict__ fslot, double *__restrict__ frec,
+          double *__restrict__ fown, double *__restrict__ eslot, double *__restrict__ wslot,
+          double *__restrict__ partial)
+{
+    constexpr int GPB = 256 / G;
+    __shared__ R s_rn[3][256];
+    __shared__ R s_rl[256], s_fc[256], s_dfc[256];
+    __shared__ int s_typ[256];
+    __shared__ double s_red[4][10];
+
+    const int tid = threadIdx.x;
+    const int lane = tid % G;
+    const int grp = tid / G;
+    const int gbase = grp * G;
+    const int ai = blockIdx.x * GPB + grp;
+    const bool valid = ai < nat;
+    const int i = valid ? order[ai] : 0;
+    const int eli = valid ? el[i] : -1;
+    const int nb = (eli >= 0) ? min(bcnt[i], G) : 0;
+    const bool act = lane < nb;
+
+    int s = -1, j = 0, typ = 0;
+    double rijd[3] = {0.0, 0.0, 0.0};
+    R rn[3] = {(R)0, (R)0, (R)0}, rl = (R)1, fc = (R)0, dfc = (R)0;
+    bool ok = false, okbo = false;
+    R fbo = (R)0, dfbo = (R)0;
+    if (act) {
+        s = bslot[offsets[i] + lane];
+        const int2 e = nbr[s];
+        j = e.x;
+        int a, b, c;
+        dc_unpack(e.y, a, b, c);
+        for (int x = 0; x < 3; ++x) rijd[x] = (r[3 * j + x] - r[3 * i + x]) - image_x(C, x, a, b, c);
+        const double rld = sqrt(rijd[0] * rijd[0] + rijd[1] * rijd[1] + rijd[2] * rijd[2]);
+        typ = pair_index(eli, el[j], P.nel);
+        for (int x = 0; x < 3; ++x) rn[x] = (R)(rijd[x] / rld);
+        rl = (R)rld;
+        const double4 cv = fcv[s];
+        fc = (R)cv.x;
+        dfc = (R)cv.y;
+        fbo = (R)cv.z;
+        dfbo = (R)cv.w;
+        ok = rld < S.or2[typ];           // cut_ar_h
+        okbo = rld < S.bor2[typ];        // cut_bo_h
+    }
+    s_rn[0][tid] = rn[0];
+    s_rn[1][tid] = rn[1];
+    s_rn[2][tid] = rn[2];
+    s_rl[tid] = rl;
+    s_fc[tid] = fbo;
+    s_dfc[tid] = dfbo;
+    s_typ[tid] = okbo ? typ : -1;
+    __syncthreads();
+
+    int maskfac = 2;
+    if (mask && act) {
+        int mi = mask[i], mj = mask[j];
+        if (mi == 0 && mj == 0) maskfac = 0;
+        else if (mi == 0 || mj == 0) maskfac = 1;
+    }
+    const bool on = act && ok && maskfac > 0;
+
+    const int elc = eli < 0 ? 0 : eli;
+    R zij = (R)0;
+    int nbw = nb;
+    for (int o = 32; o >= 1; o >>= 1) nbw = max(nbw, __shfl_xor(nbw, o, 64));
+    for (int k = 0; k < G; ++k) {
+        if (k >= nbw) break;
+        const int tk = gbase + k;
+        const int tpk = s_typ[tk];
+        if (on && k != lane && k < nb && tpk >= 0) {
+            R rnk[3] = {s_rn[0][tk], s_rn[1][tk], s_rn[2][tk]};
+            R costh = rnk[0] * rn[0] + rnk[1] * rn[1] + rnk[2] * rn[2];
+            R g, dg;
+            g_fn<R, FAM>(P, elc, tpk, costh, g, dg);
+            R hv, dh;
+            h_fn<R, FAM>(P, tpk, rl - s_rl[tk], hv, dh);
+            zij += s_fc[tk] * g * hv;
+        }
+    }
+
+    R VA = (R)0, dVA = (R)0, VR = (R)0, dVR = (R)0, bij = (R)1, Pz = (R)0;
+    if (on) {
+        if (FAM == 2) {
+            const R dr0 = rl - (R)P.br0[typ];
+            const R ea = rexp<R>(-(R)P.bexpA[typ] * dr0);
+            VA = -(R)P.bVAf[typ] * ea;
+            dVA = (R)P.bVAf[typ] * (R)P.bexpA[typ] * ea;
+            const R er = rexp<R>(-(R)P.bexpR[typ] * dr0);
+            VR = (R)P.bVRf[typ] * er;
+            dVR = -(R)P.bVRf[typ] * (R)P.bexpR[typ] * er;
+        } else {
+            R ea = rexp<R>(-(R)P.mu[typ] * rl);
+            VA = -(R)P.B[typ] * ea;
+            dVA = (R)P.B[typ] * (R)P.mu[typ] * ea;
+            R er = rexp<R>(-(R)P.lambda[typ] * rl);
+            VR = (R)P.A[typ] * er;
+            dVR = -(R)P.A[typ] * (R)P.lambda[typ] * er;
+        }
+        R mf = (R)0.5 * (R)maskfac;
+        VA *= mf;
+        dVA *= mf;
+        VR *= mf;
+        dVR *= mf;
+        bo_fn<R, FAM>(P, eli, typ, zij, fc, VA, bij, Pz);
+    }
+
+    // derivatives; forces on k and the bond-order screening factor of bond
+    // ik (sum_j g h dE/dzeta_ij, bop_kernel.f90:1448-1462) reduced into lane k
+    R dbidi[3] = {(R)0, (R)0, (R)0}, dbidj[3] = {(R)0, (R)0, (R)0};
+    R wijb[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) wijb[q] = (R)0;
+    double recv[3] = {0.0, 0.0, 0.0}, sfac = 0.0;
+    const R rlijr = (R)1 / rl;
+    const R rij[3] = {rl * rn[0], rl * rn[1], rl * rn[2]};
+    for (int k = 0; k < G; ++k) {
+        if (k >= nbw) break;
+        const int tk = gbase + k;
+        const int tpk = s_typ[tk];
+        R dk[3] = {(R)0, (R)0, (R)0};
+        R zf = (R)0;
+        if (on && k != lane && k < nb && tpk >= 0) {
+            R rnk[3] = {s_rn[0][tk], s_rn[1][tk], s_rn[2][tk]};
+            R rlik = s_rl[tk], fcik = s_fc[tk], dfcik = s_dfc[tk];
+            R hv, dh;
+            h_fn<R, FAM>(P, tpk, rl - rlik, hv, dh);
+            R costh = rnk[0] * rn[0] + rnk[1] * rn[1] + rnk[2] * rn[2];
+            R g, dg;
+            g_fn<R, FAM>(P, elc, tpk, costh, g, dg);
+            zf = g * hv * Pz;
+            R dkc[3];
+            for (int x = 0; x < 3; ++x) dkc[x] = rnk[x] * rlik - rn[x] * rl;
+            R disjk = rsqrt_<R>(dkc[0] * dkc[0] + dkc[1] * dkc[1] + dkc[2] * dkc[2]);
+            for (int x = 0; x < 3; ++x) dkc[x] = dkc[x] / disjk;
+            R rlikr = (R)1 / rlik;
+            R dcsdij = rlikr - costh * rlijr;
+            R dcsdik = rlijr - costh * rlikr;
+            R dcsdjk = -disjk * rlijr * rlikr;
+            R dzfac = fcik * dg * hv;
+            R dzdrij = g * fcik * dh;
+            R dzdrik = g * (dfcik * hv - fcik * dh);
+            R df[3];
+            for (int x = 0; x < 3; ++x) {
+                R dcsdi = -dcsdij * rn[x] - dcsdik * rnk[x];
+                R dcsdj = dcsdij * rn[x] - dcsdjk * dkc[x];
+                R dcsdk = dcsdik * rnk[x] + dcsdjk * dkc[x];
+                dbidi[x] += -dzdrij * rn[x] - dzdrik * rnk[x] + dzfac * dcsdi;
+                df[x] = dzdrij * rn[x] + dzfac * dcsdj;
+                dbidj[x] += df[x];
+                dk[x] = dzdrik * rnk[x] + dzfac * dcsdk;
+            }
+            R rik[3] = {rlik * rnk[0], rlik * rnk[1], rlik * rnk[2]};
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) wijb[a + 3 * b] -= rij[a] * df[b] + rik[a] * dk[b];
+        }
+        double c0 = -(double)Pz * (double)dk[0];
+        double c1 = -(double)Pz * (double)dk[1];
+        double c2v = -(double)Pz * (double)dk[2];
+        double c3 = (double)zf;
+        c0 = group_sum<G>(c0);
+        c1 = group_sum<G>(c1);
+        c2v = group_sum<G>(c2v);
+        c3 = group_sum<G>(c3);
+        if (lane == k) {
+            recv[0] = c0;
+            recv[1] = c1;
+            recv[2] = c2v;
+            sfac = c3;
+        }
+    }
+
+    double fi[3] = {0.0, 0.0, 0.0}, fj[3] = {0.0, 0.0, 0.0};
+    double e_ij = 0.0;
+    double w[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) w[q] = 0.0;
+    if (on) {
+        e_ij = (double)((R)0.5 * fc * (VR + bij * VA));
+        R dffac = (R)0.5 * (dVR * fc + bij * dVA * fc + VR * dfc + bij * VA * dfc);
+        for (int x = 0; x < 3; ++x) {
+            double df = (double)(dffac * rn[x]);
+            fi[x] = df - (double)Pz * (double)dbidi[x];
+            fj[x] = -df - (double)Pz * (double)dbidj[x];
+        }
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b)
+                w[a + 3 * b] = (double)rij[a] * (double)(dffac * rn[b]) - (double)Pz * (double)wijb[a + 3 * b];
+    }
+    if (act) {
+        // screening forces of this bond's records: the pair term (when on)
+        // and the bond order (factor sfac, zero unless the bond was a k-bond)
+        const double dff = on ? 0.5 * ((double)VR + (double)bij * (double)VA) : 0.0;
+        for (int q = roff[s]; q < roff[s + 1]; ++q) {
+            const int kn = rkn[q];
+            const int2 ek = nbr[kn];
+            int a, b, c;
+            dc_unpack(ek.y, a, b, c);
+            double rik[3], rjk[3];
+            for (int x = 0; x < 3; ++x) {
+                rik[x] = (r[3 * ek.x + x] - r[3 * i + x]) - image_x(C, x, a, b, c);
+                rjk[x] = -rijd[x] + rik[x];
+            }
+            const double4 co = rco[q];
+            const double cik = dff * co.x + sfac * co.z, cjk = dff * co.y + sfac * co.w;
+            double fk[3];
+            for (int x = 0; x < 3; ++x) {
+                const double d1 = cik * rik[x], d2 = cjk * rjk[x];
+                fi[x] += d1;
+                fj[x] += d2;
+                fk[x] = -d1 - d2;
+            }
+            for (int a2 = 0; a2 < 3; ++a2)
+                for (int b2 = 0; b2 < 3; ++b2) w[a2 + 3 * b2] += rik[a2] * (cik * rik[b2]) + rjk[a2] * (cjk * rjk[b2]);
+            frec[3 * (size_t)q + 0] = fk[0];
+            frec[3 * (size_t)q + 1] = fk[1];
+            frec[3 * (size_t)q + 2] = fk[2];
+        }
+        fslot[3 * (size_t)s + 0] = fj[0] + recv[0];
+        fslot[3 * (size_t)s + 1] = fj[1] + recv[1];
+        fslot[3 * (size_t)s + 2] = fj[2] + recv[2];
+        eslot[s] = e_ij;
+        for (int q = 0; q < 9; ++q) wslot[9 * (size_t)s + q] = w[q];
+    }
+    double f0 = group_sum<G>(fi[0]);
+    double f1 = group_sum<G>(fi[1]);
+    double f2 = group_sum<G>(fi[2]);
+    if (valid && lane == 0) {
+        fown[3 * (size_t)i + 0] = f0;
+        fown[3 * (size_t)i + 1] = f1;
+        fown[3 * (size_t)i + 2] = f2;
+    }
+    double red[10];
+    red[0] = e_ij;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) red[1 + q] = w[q];
+#pragma unroll
+    for (int q = 0; q < 10; ++q) red[q] = group_sum<64>(red[q]);
+    const int wave = tid / 64;
+    if ((tid & 63) == 0)
+        for (int q = 0; q < 10; ++q) s_red[wave][q] = red[q];
+    __syncthreads();
+    if (tid < 10)
+        partial[(size_t)blockIdx.x * 10 + tid] =
+            ((s_red[0][tid] + s_red[1][tid]) + s_red[2][tid]) + s_red[3][tid];
+}
+
+// wavefront per atom, lanes over its row: total force the atom's terms put
+// on each neighbour entry (bond slot + records naming the entry, in record
+// order)
+__global__ void __launch_bounds__(256)
+k_scr_slots(int nat, const int *__restrict__ offsets, const int *__restrict__ keep,
+            const double *__restrict__ fslot, const int *__restrict__ roff, const int *__restrict__ rkn,
+            const double *__restrict__ frec, double *__restrict__ fsl)
+{
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= nat) return;
+    const int beg = offsets[i], end = offsets[i + 1];
+    const int q0 = roff[beg], q1 = roff[end];
+    for (int s = beg + lane; s < end; s += 64) {
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+        if (keep[s]) {
+            a0 = fslot[3 * (size_t)s];
+            a1 = fslot[3 * (size_t)s + 1];
+            a2 = fslot[3 * (size_t)s + 2];
+        }
+        for (int q = q0; q < q1; ++q)
+            if (rkn[q] == s) {
+                a0 += frec[3 * (size_t)q];
+                a1 += frec[3 * (size_t)q + 1];
+                a2 += frec[3 * (size_t)q + 2];
+            }
+        fsl[3 * (size_t)s] = a0;
+        fsl[3 * (size_t)s + 1] = a1;
+        fsl[3 * (size_t)s + 2] = a2;
+    }
+}
+
+// 8 lanes per atom over its row: f_a += own + sum of the slots the
+// neighbours wrote for a; per-atom energy and virial halves of a's bonds and
+// of the bonds pointing at a (pe(i), pe(j) of bop_kernel.f90:1366-1368)
+constexpr int SCR_GL = 8;
+__global__ void __launch_bounds__(256)
+k_scr_gather(int nat, const int *__restrict__ offsets, const int *__restrict__ rev, const int *__restrict__ keep,
+             const double *__restrict__ fsl, const double *__restrict__ fown, const double *__restrict__ eslot,
+             const double *__restrict__ wslot, double *__restrict__ f, double *__restrict__ epa,
+             double *__restrict__ wpa)
+{
+    const int lane = threadIdx.x % SCR_GL;
+    const int a = blockIdx.x * (256 / SCR_GL) + threadIdx.x / SCR_GL;
+    const bool ok = a < nat;
+    double acc[3] = {0.0, 0.0, 0.0}, e = 0.0, w[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) w[q] = 0.0;
+    if (ok) {
+        for (int s = offsets[a] + lane; s < offsets[a + 1]; s += SCR_GL) {
+            const int t = rev[s];
+            if (t >= 0) {
+                acc[0] += fsl[3 * (size_t)t];
+                acc[1] += fsl[3 * (size_t)t + 1];
+                acc[2] += fsl[3 * (size_t)t + 2];
+            }
+            if (epa || wpa) {
+                if (keep[s]) {
+                    if (epa) e += eslot[s];
+                    if (wpa)
+                        for (int q = 0; q < 9; ++q) w[q] += wslot[9 * (size_t)s + q];
+                }
+                if (t >= 0 && keep[t]) {
+                    if (epa) e += eslot[t];
+                    if (wpa)
+                        for (int q = 0; q < 9; ++q) w[q] += wslot[9 * (size_t)t + q];
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int x = 0; x < 3; ++x) acc[x] = group_sum<SCR_GL>(acc[x]);
+    if (epa) e = group_sum<SCR_GL>(e);
+    if (wpa)
+        for (int q = 0; q < 9; ++q) w[q] = group_sum<SCR_GL>(w[q]);
+    if (ok && lane == 0) {
+        f[3 * (size_t)a] += fown[3 * (size_t)a] + acc[0];
+        f[3 * (size_t)a + 1] += fown[3 * (size_t)a + 1] + acc[1];
+        f[3 * (size_t)a + 2] += fown[3 * (size_t)a + 2] + acc[2];
+        if (epa) epa[a] += 0.5 * e;
+        if (wpa)
+            for (int q = 0; q < 9; ++q) wpa[9 * (size_t)a + q] += 0.5 * w[q];
+    }
+}
+
+// -------------------------------------------------------------------------
 // host side
 // -------------------------------------------------------------------------
 static void tersoff_free(void *p) { delete static_cast<TersoffState *>(p); }
@@ -641,11 +1233,23 @@ int tersoff_bind(agpu_potential *pot)
     bool present[3] = {false, false, false};
     for (int z : p->h_Z)
         if (z > 0 && z < 128 && st->z2db[z] >= 0) present[st->z2db[z]] = true;
+    // screened: x(pair) * sqrt(maxval(max_cut_sq)) with x = sqrt(C_dr_cut)
+    // (default_bind_to_func.f90:113-150)
+    double mc_all = 0.0;
+    if (st->scr)
+        for (int q = 0; q < prm.nel * (prm.nel + 1) / 2; ++q)
+            mc_all = std::max(mc_all, std::max(prm.r2[q], std::max(st->sp.or2[q], st->sp.bor2[q])));
     double cut = 0.0;
     for (int a = 0; a < prm.nel; ++a)
         for (int b = 0; b < prm.nel; ++b)
             if (present[a] && present[b]) {
-                double r2 = prm.r2[pair_index(a, b, prm.nel)];
+                const int q = pair_index(a, b, prm.nel);
+                double r2 = prm.r2[q];
+                if (st->scr) {
+                    const double cm = st->sp.Cmax[q];
+                    const double cdr = cm > 2.0 ? cm * cm / (4 * (cm - 1)) : 1.0;
+                    r2 = std::sqrt(cdr) * mc_all;
+                }
                 if (r2 > cut) cut = r2;
                 AGPU_CHECK(agpu_neighbors_request_interaction_range(pot->nl, r2));
             }
@@ -722,6 +1326,180 @@ static int launch_gather(int G, int nat, agpu_neighbors *nl, TersoffState *st, d
     }
 }
 
+static void derive_scr(ScrDev &S, const agpu_tersoff_params &prm, const agpu_screening_params &sp)
+{
+    std::memset(&S, 0, sizeof S);
+    for (int q = 0; q < 6; ++q) {
+        S.r1[q] = prm.r1[q];
+        S.r2[q] = prm.r2[q];
+        S.or1[q] = sp.or1[q];
+        S.or2[q] = sp.or2[q];
+        S.bor1[q] = sp.bor1[q];
+        S.bor2[q] = sp.bor2[q];
+        S.fin[q] = S.r2[q] > S.r1[q] ? 1.0 / (S.r2[q] - S.r1[q]) : 0.0;
+        S.fout[q] = S.or2[q] > S.or1[q] ? 1.0 / (S.or2[q] - S.or1[q]) : 0.0;
+        S.fbo[q] = S.bor2[q] > S.bor1[q] ? 1.0 / (S.bor2[q] - S.bor1[q]) : 0.0;
+        S.Cmin[q] = sp.Cmin[q];
+        S.Cmax[q] = sp.Cmax[q];
+        S.dC[q] = sp.Cmax[q] - sp.Cmin[q];
+        // default_bind_to_func.f90:44-70
+        S.C_dr_cut[q] = sp.Cmax[q] > 2.0 ? sp.Cmax[q] * sp.Cmax[q] / (4 * (sp.Cmax[q] - 1)) : 1.0;
+        double mx = std::max(S.r2[q], std::max(S.or2[q], S.bor2[q]));
+        S.max_cut_sq[q] = mx * mx;
+    }
+    // exp_cutoff_init (cutoff.f90:245-258)
+    const double v1 = std::exp(-8.0), dv1 = -24 * v1, ddv1 = -48 * v1 - 24 * dv1;
+    S.ec = (-3 * dv1 + ddv1) / 3;
+    S.ed = (2 * dv1 - ddv1) / 4;
+    S.efac2 = 1.0 / (1 - v1 - S.ec - S.ed);
+    S.eoff = v1 + S.ec + S.ed;
+}
+
+template <typename R, int G>
+static int launch_bop_scr(TersoffState *st, const ScrDev &S, int nat, const int *order, const double *r,
+                          const int *el, const CellArg &C, agpu_neighbors *nl, const int *mask, int nblk,
+                          hipStream_t s)
+{
+    const TersoffDev &P = st->dev;
+    const int *offs = nl->offsets.as<int>();
+    const int2 *nb = nl->nbr.as<int2>();
+    const int *bs = st->bslot.as<int>(), *bc = st->bcnt.as<int>();
+    const double4 *fcv = st->fcv.as<double4>();
+    const int *roff = st->roff.as<int>(), *rkn = st->rkn.as<int>();
+    const double4 *rco = st->rco.as<double4>();
+    double *fslot = st->fslot.as<double>(), *frec = st->frec.as<double>(), *fown = st->fown.as<double>();
+    double *es = st->eslot.as<double>(), *ws = st->wslot.as<double>(), *part = st->partial.as<double>();
+    const bool f64 = sizeof(R) == 8;
+    if (P.fam == 1)
+        AGPU_LAUNCH(f64 ? "kumagai_scr_fp64" : "kumagai_scr_mixed", (k_bop_scr<R, G, 1>), dim3(nblk), dim3(256), 0,
+                    s, P, S, nat, order, r, el, C, offs, nb, bs, bc, mask, fcv, roff, rkn, rco, fslot, frec, fown,
+                    es, ws, part);
+    else if (P.fam == 2)
+        AGPU_LAUNCH(f64 ? "brenner_scr_fp64" : "brenner_scr_mixed", (k_bop_scr<R, G, 2>), dim3(nblk), dim3(256), 0,
+                    s, P, S, nat, order, r, el, C, offs, nb, bs, bc, mask, fcv, roff, rkn, rco, fslot, frec, fown,
+                    es, ws, part);
+    else
+        AGPU_LAUNCH(f64 ? "tersoff_scr_fp64" : "tersoff_scr_mixed", (k_bop_scr<R, G, 0>), dim3(nblk), dim3(256), 0,
+                    s, P, S, nat, order, r, el, C, offs, nb, bs, bc, mask, fcv, roff, rkn, rco, fslot, frec, fown,
+                    es, ws, part);
+    return AGPU_OK;
+}
+
+template <typename R>
+static int dispatch_bop_scr(int G, TersoffState *st, const ScrDev &S, int nat, const int *order, const double *r,
+                            const int *el, const CellArg &C, agpu_neighbors *nl, const int *mask, hipStream_t s,
+                            int *nblk_out)
+{
+    const int nblk = std::max(1, (nat + 256 / G - 1) / (256 / G));
+    *nblk_out = nblk;
+    AGPU_CHECK(st->partial.reserve(sizeof(double) * 10 * nblk));
+    switch (G) {
+    case 2: return launch_bop_scr<R, 2>(st, S, nat, order, r, el, C, nl, mask, nblk, s);
+    case 4: return launch_bop_scr<R, 4>(st, S, nat, order, r, el, C, nl, mask, nblk, s);
+    case 8: return launch_bop_scr<R, 8>(st, S, nat, order, r, el, C, nl, mask, nblk, s);
+    case 16: return launch_bop_scr<R, 16>(st, S, nat, order, r, el, C, nl, mask, nblk, s);
+    case 32: return launch_bop_scr<R, 32>(st, S, nat, order, r, el, C, nl, mask, nblk, s);
+    default: return launch_bop_scr<R, 64>(st, S, nat, order, r, el, C, nl, mask, nblk, s);
+    }
+}
+
+// the screened variants (bop_kernel.f90 with SCREENING); bound and list
+// updated by the caller
+static int tersoff_scr_compute(agpu_potential *pot, const agpu_eval_args *a)
+{
+    TersoffState *st = static_cast<TersoffState *>(pot->impl);
+    agpu_particles *p = pot->p;
+    agpu_neighbors *nl = pot->nl;
+    if (capturing())
+        return set_error(AGPU_ERR_UNSUPPORTED, "screened bond-order potentials are not graph-capturable");
+    const int nat = p->nat;
+    hipStream_t s = stream();
+    const int64_t E = nl->total;
+    if (E > 0x7ffffff0ll) return set_error(AGPU_ERR_ARG, "screened bond order: list too long");
+    ScrDev S;
+    derive_scr(S, st->prm, st->sp);
+    AGPU_CHECK(st->keep.reserve(sizeof(int) * (E + 1)));
+    AGPU_CHECK(st->rcnt.reserve(sizeof(int) * (E + 1)));
+    AGPU_CHECK(st->roff.reserve(sizeof(int) * (E + 1)));
+    AGPU_CHECK(st->fcv.reserve(sizeof(double4) * (E + 1)));
+    AGPU_CHECK(st->bslot.reserve(sizeof(int) * (E + 1)));
+    AGPU_CHECK(st->fslot.reserve(sizeof(double) * 3 * (E + 1)));
+    AGPU_CHECK(st->fsl.reserve(sizeof(double) * 3 * (E + 1)));
+    AGPU_CHECK(st->eslot.reserve(sizeof(double) * (E + 1)));
+    AGPU_CHECK(st->wslot.reserve(sizeof(double) * 9 * (E + 1)));
+    AGPU_CHECK(st->nlrev.reserve(sizeof(int) * (E + 1)));
+    AGPU_CHECK(st->bcnt.reserve(sizeof(int) * (nat + 1)));
+    AGPU_CHECK(st->fown.reserve(sizeof(double) * 3 * (nat + 1)));
+    AGPU_CHECK(st->stats.reserve(sizeof(int) * 4));
+    AGPU_CHECK(pot->stats.init());
+    int *dstats = st->stats.as<int>();
+    AGPU_CHECK(zero_dev(dstats, sizeof(int) * 4));
+    CellArg C;
+    std::memcpy(C.A, p->cell.A, sizeof C.A);
+    const int *el = pot->d_el.as<int>();
+    const int *offs = nl->offsets.as<int>();
+    const int2 *nbr = nl->nbr.as<int2>();
+    const dim3 gw((unsigned)((nat + SCR_WAVES - 1) / SCR_WAVES));
+    // pass 1: keep flags and record counts; exclusive scan to record offsets
+    AGPU_CHECK(zero_dev(st->rcnt.as<int>() + E, sizeof(int)));
+    AGPU_LAUNCH("scr_count", k_scr_screen<false>, gw, dim3(256), 0, s, S, nat, p->d_r, el, st->prm.nel, C, offs,
+                nbr, st->keep.as<int>(), st->rcnt.as<int>(), nullptr, nullptr, nullptr, nullptr, dstats);
+    size_t tmp = 0;
+    AGPU_HIP(rocprim::exclusive_scan(nullptr, tmp, st->rcnt.as<int>(), st->roff.as<int>(), 0, (size_t)E + 1,
+                                     rocprim::plus<int>(), s));
+    AGPU_CHECK(st->scan_tmp.reserve(tmp + 16));
+    AGPU_HIP(rocprim::exclusive_scan(st->scan_tmp.ptr, tmp, st->rcnt.as<int>(), st->roff.as<int>(), 0,
+                                     (size_t)E + 1, rocprim::plus<int>(), s));
+    AGPU_HIP(hipMemcpyAsync(pot->stats.h, st->roff.as<int>() + E, sizeof(int), hipMemcpyDeviceToHost, s));
+    AGPU_HIP(hipMemcpyAsync(pot->stats.h + 1, dstats, sizeof(int), hipMemcpyDeviceToHost, s));
+    AGPU_HIP(hipStreamSynchronize(s));
+    if (pot->stats.h[1] & DERR_SCR_ROW)
+        return set_error(AGPU_ERR_NEB, "screened bond order: more than 256 neighbours within the screening range");
+    const int64_t nrec = pot->stats.h[0];
+    AGPU_CHECK(st->rkn.reserve(sizeof(int) * (nrec + 1)));
+    AGPU_CHECK(st->rco.reserve(sizeof(double4) * (nrec + 1)));
+    AGPU_CHECK(st->frec.reserve(sizeof(double) * 3 * (nrec + 1)));
+    // pass 2: cutoffs and records
+    AGPU_LAUNCH("scr_fill", k_scr_screen<true>, gw, dim3(256), 0, s, S, nat, p->d_r, el, st->prm.nel, C, offs, nbr,
+                st->keep.as<int>(), st->rcnt.as<int>(), st->roff.as<int>(), st->fcv.as<double4>(),
+                st->rkn.as<int>(), st->rco.as<double4>(), dstats);
+    AGPU_LAUNCH("scr_bonds", k_scr_bonds, dim3((nat + 255) / 256), dim3(256), 0, s, nat, offs, st->keep.as<int>(),
+                st->bslot.as<int>(), st->bcnt.as<int>());
+    // reverse entries of the list, once per list build
+    if (st->nlrev_nl != nl || st->nlrev_upd != nl->n_updates) {
+        AGPU_LAUNCH("scr_rev", k_nl_rev_entries, gw, dim3(256), 0, s, nat, offs, nbr, st->nlrev.as<int>(), dstats);
+        st->nlrev_nl = nl;
+        st->nlrev_upd = nl->n_updates;
+    }
+    AGPU_CHECK(int_max(st->bcnt.as<int>(), nat, dstats + 2));
+    AGPU_HIP(hipMemcpyAsync(pot->stats.h, dstats, sizeof(int) * 4, hipMemcpyDeviceToHost, s));
+    AGPU_HIP(hipStreamSynchronize(s));
+    if (pot->stats.h[1] & DERR_BOND_OVERFLOW) {
+        st->nlrev_nl = nullptr;
+        return set_error(AGPU_ERR_NEB, "screened bond order: list entry without its reverse entry");
+    }
+    const int maxb = pot->stats.h[2];
+    if (maxb > 64)
+        return set_error(AGPU_ERR_NEB, "screened bond order: more than 64 bonds on one atom (nebmax too small)");
+    int G = 2;
+    while (G < maxb) G <<= 1;
+    int nblk = 0;
+    const int *order = nl->idx_sorted.as<int>();
+    if (pot->precision == AGPU_PREC_MIXED)
+        AGPU_CHECK(dispatch_bop_scr<float>(G, st, S, nat, order, p->d_r, el, C, nl, a->mask, s, &nblk));
+    else
+        AGPU_CHECK(dispatch_bop_scr<double>(G, st, S, nat, order, p->d_r, el, C, nl, a->mask, s, &nblk));
+    AGPU_LAUNCH("scr_slots", k_scr_slots, gw, dim3(256), 0, s, nat, offs, st->keep.as<int>(),
+                st->fslot.as<double>(), st->roff.as<int>(), st->rkn.as<int>(), st->frec.as<double>(),
+                st->fsl.as<double>());
+    AGPU_LAUNCH("scr_gather", k_scr_gather, dim3((unsigned)((nat + 256 / SCR_GL - 1) / (256 / SCR_GL))), dim3(256), 0,
+                s, nat, offs, st->nlrev.as<int>(), st->keep.as<int>(), st->fsl.as<double>(), st->fown.as<double>(),
+                st->eslot.as<double>(), st->wslot.as<double>(), a->f, a->epot_per_at, a->wpot_per_at);
+    AGPU_CHECK(reduce_ev(st->partial.as<double>(), nblk, 10, a->epot, a->wpot, nullptr));
+    AGPU_HIP(hipStreamSynchronize(s));
+    return AGPU_OK;
+}
+
 int tersoff_compute(agpu_potential *pot, const agpu_eval_args *a)
 {
     TersoffState *st = static_cast<TersoffState *>(pot->impl);
@@ -733,6 +1511,7 @@ int tersoff_compute(agpu_potential *pot, const agpu_eval_args *a)
     hipStream_t s = stream();
     AGPU_CHECK(zero_dev(a->epot, sizeof(double)));
     if (nat == 0) return AGPU_OK;
+    if (st->scr) return tersoff_scr_compute(pot, a);
     const int64_t total = nl->total;
     AGPU_CHECK(st->bslot.reserve(sizeof(int) * (total + 1)));
     AGPU_CHECK(st->rev.reserve(sizeof(int) * (total + 1)));
@@ -996,4 +1775,121 @@ extern "C" int agpu_brenner_create(agpu_potential **pot, const agpu_brenner_para
     P->free_impl = tersoff_free;
     *pot = P;
     return AGPU_OK;
+}
+
+// -------------------------------------------------------------------------
+// screened variants: TersoffScr, KumagaiScr, BrennerScr
+// -------------------------------------------------------------------------
+static agpu_screening_params scr_uniform(int npairs, const double *or1, const double *or2)
+{
+    agpu_screening_params sp;
+    std::memset(&sp, 0, sizeof sp);
+    for (int q = 0; q < npairs; ++q) {
+        sp.or1[q] = sp.bor1[q] = or1[q];
+        sp.or2[q] = sp.bor2[q] = or2[q];
+        sp.Cmin[q] = 1.0;
+        sp.Cmax[q] = 3.0;
+    }
+    return sp;
+}
+
+static int attach_screening(agpu_potential *P, const agpu_screening_params &sp, int kind)
+{
+    TersoffState *st = static_cast<TersoffState *>(P->impl);
+    const int npairs = st->prm.nel * (st->prm.nel + 1) / 2;
+    double mc = 0.0, x = 1.0;
+    for (int q = 0; q < npairs; ++q) {
+        if (!(sp.Cmax[q] > sp.Cmin[q])) {
+            agpu_potential_destroy(P);
+            return set_error(AGPU_ERR_ARG, "screening: need Cmin < Cmax for every element pair");
+        }
+        mc = std::max(mc, std::max(st->prm.r2[q], std::max(sp.or2[q], sp.bor2[q])));
+        const double cm = sp.Cmax[q];
+        x = std::max(x, cm > 2.0 ? std::sqrt(cm * cm / (4 * (cm - 1))) : 1.0);
+    }
+    st->scr = true;
+    st->sp = sp;
+    P->kind = kind;
+    P->cutoff = x * mc;
+    return AGPU_OK;
+}
+
+extern "C" int agpu_tersoff_scr_create(agpu_potential **pot, const agpu_tersoff_params *params,
+                                       const agpu_screening_params *scr)
+{
+    if (!pot) return set_error(AGPU_ERR_ARG, "pot is NULL");
+    agpu_tersoff_params p = params ? *params : default_params();
+    agpu_screening_params sp;
+    if (!params) {
+        // Tersoff_PRB_39_5566_SiC with SCREENING (tersoff_params.f90:111-133)
+        const double mubo[3] = {0.69103023078057590, 0.56580821386164815, 0.43569872294774004};
+        const double r1[3] = {2.00, std::sqrt(2.00 * 2.50), 2.50};
+        const double r2[3] = {2.00 * 1.2, std::sqrt(2.00 * 2.50) * 1.2, 2.50 * 1.2};
+        for (int q = 0; q < 3; ++q) {
+            p.mubo[q] = mubo[q];
+            p.m[q] = 3;
+            p.r1[q] = r1[q];
+            p.r2[q] = r2[q];
+        }
+    }
+    if (scr) {
+        sp = *scr;
+    } else {
+        const double o1[3] = {2.00, std::sqrt(2.00 * 3.00), 3.00};
+        const double o2[3] = {2.00 * 2.0, std::sqrt(2.00 * 3.00) * 2.0, 3.00 * 2.0};
+        sp = scr_uniform(3, o1, o2);
+    }
+    AGPU_CHECK(agpu_tersoff_create(pot, &p));
+    return attach_screening(*pot, sp, AGPU_KIND_TERSOFF_SCR);
+}
+
+extern "C" int agpu_kumagai_scr_create(agpu_potential **pot, const agpu_kumagai_params *params,
+                                       const agpu_screening_params *scr)
+{
+    if (!pot) return set_error(AGPU_ERR_ARG, "pot is NULL");
+    agpu_kumagai_params k = params ? *params : kumagai_default();
+    if (!params) {
+        // Kumagai_CompMaterSci_39_457_Si with SCREENING (kumagai_params.f90:119-127)
+        k.r1 = 2.50;
+        k.r2 = 2.50 * 1.2;
+    }
+    agpu_screening_params sp;
+    if (scr) {
+        sp = *scr;
+    } else {
+        const double o1[1] = {3.00}, o2[1] = {3.00 * 2.0};
+        sp = scr_uniform(1, o1, o2);
+    }
+    AGPU_CHECK(agpu_kumagai_create(pot, &k));
+    return attach_screening(*pot, sp, AGPU_KIND_KUMAGAI_SCR);
+}
+
+extern "C" int agpu_brenner_scr_create(agpu_potential **pot, const agpu_brenner_params *params,
+                                       const agpu_screening_params *scr)
+{
+    if (!pot) return set_error(AGPU_ERR_ARG, "pot is NULL");
+    agpu_brenner_params b = params ? *params : brenner_default();
+    if (!params) {
+        // Erhart_PRB_71_035211_SiC with SCREENING (brenner_params.f90:94-105)
+        const double mu[3] = {1.0 / 1.4276, 1.0 / 1.79, 1.0 / 1.842};
+        const double r1[3] = {2.00, std::sqrt(2.00 * 2.50), 2.50};
+        const double r2[3] = {2.00 * 1.2, std::sqrt(2.00 * 2.50) * 1.2, 2.50 * 1.2};
+        for (int q = 0; q < 3; ++q) {
+            b.mu[q] = mu[q];
+            b.n[q] = 1.0;
+            b.m[q] = 3;
+            b.r1[q] = r1[q];
+            b.r2[q] = r2[q];
+        }
+    }
+    agpu_screening_params sp;
+    if (scr) {
+        sp = *scr;
+    } else {
+        const double o1[3] = {2.00, std::sqrt(2.00 * 3.00), 3.00};
+        const double o2[3] = {2.00 * 2.0, std::sqrt(2.00 * 3.00) * 2.0, 3.00 * 2.0};
+        sp = scr_uniform(3, o1, o2);
+    }
+    AGPU_CHECK(agpu_brenner_create(pot, &b));
+    return attach_screening(*pot, sp, AGPU_KIND_BRENNER_SCR);
 }

@@ -322,17 +322,11 @@ class Kumagai(_Potential):
 
     def __init__(self, **kwargs):
         super().__init__()
-        from .parameters import ATOMIC_NUMBERS, Kumagai_CompMaterSci_39_457_Si
+        from .parameters import Kumagai_CompMaterSci_39_457_Si
         params = dict(Kumagai_CompMaterSci_39_457_Si)
         params.update(kwargs)
         self._params = params
-        s = _lib.KumagaiParams()
-        el = params["el"][0] if isinstance(params["el"], (list, tuple)) else params["el"]
-        s.Z = ATOMIC_NUMBERS[el]
-        for k in ("A", "B", "lambda1", "lambda2", "eta", "delta", "alpha", "c1", "c2", "c3", "c4", "c5", "h",
-                  "r1", "r2"):
-            setattr(s, k, float(params[k]))
-        s.beta = int(params["beta"])
+        s = _kumagai_struct(params)
         h = ctypes.c_void_p()
         call("agpu_kumagai_create", ctypes.byref(h), ctypes.byref(s))
         self._h = h
@@ -348,28 +342,131 @@ class Brenner(_Potential):
 
     def __init__(self, **kwargs):
         super().__init__()
-        from .parameters import ATOMIC_NUMBERS, Erhart_PRB_71_035211_SiC
+        from .parameters import Erhart_PRB_71_035211_SiC
         params = dict(Erhart_PRB_71_035211_SiC)
         params.update(kwargs)
         self._params = params
-        nel = len(params["el"])
-        npairs = nel * (nel + 1) // 2
-        s = _lib.BrennerParams()
-        s.nel = nel
-        for i, e in enumerate(params["el"]):
-            s.Z[i] = ATOMIC_NUMBERS[e]
-        for k in ("D0", "r0", "S", "beta", "gamma", "c", "d", "h", "mu", "n", "r1", "r2"):
-            vals = params[k]
-            if len(vals) < npairs:
-                raise ValueError("Brenner: %s needs one value per element pair (%d)" % (k, npairs))
-            arr = getattr(s, k)
-            for i in range(npairs):
-                arr[i] = float(vals[i])
-        for i in range(npairs):
-            s.m[i] = int(params["m"][i])
+        s = _brenner_struct(params)
         h = ctypes.c_void_p()
         call("agpu_brenner_create", ctypes.byref(h), ctypes.byref(s))
         self._h = h
+
+
+def _kumagai_struct(params):
+    from .parameters import ATOMIC_NUMBERS
+    s = _lib.KumagaiParams()
+    el = params["el"][0] if isinstance(params["el"], (list, tuple)) else params["el"]
+    s.Z = ATOMIC_NUMBERS[el]
+    for k in ("A", "B", "lambda1", "lambda2", "eta", "delta", "alpha", "c1", "c2", "c3", "c4", "c5", "h",
+              "r1", "r2"):
+        v = params[k]
+        setattr(s, k, float(v[0] if isinstance(v, (list, tuple)) else v))
+    b = params["beta"]
+    s.beta = int(b[0] if isinstance(b, (list, tuple)) else b)
+    return s
+
+
+def _brenner_struct(params):
+    from .parameters import ATOMIC_NUMBERS
+    nel = len(params["el"])
+    npairs = nel * (nel + 1) // 2
+    s = _lib.BrennerParams()
+    s.nel = nel
+    for i, e in enumerate(params["el"]):
+        s.Z[i] = ATOMIC_NUMBERS[e]
+    for k in ("D0", "r0", "S", "beta", "gamma", "c", "d", "h", "mu", "n", "r1", "r2"):
+        vals = params[k]
+        if len(vals) < npairs:
+            raise ValueError("Brenner: %s needs one value per element pair (%d)" % (k, npairs))
+        arr = getattr(s, k)
+        for i in range(npairs):
+            arr[i] = float(vals[i])
+    for i in range(npairs):
+        s.m[i] = int(params["m"][i])
+    return s
+
+
+def _screening_struct(params, npairs):
+    """or1, or2, bor1, bor2, Cmin, Cmax per element pair (scalars for
+    one-element sets)."""
+    from .parameters import SCREENING_KEYS
+    s = _lib.ScreeningParams()
+    for k in SCREENING_KEYS:
+        vals = params[k]
+        if not isinstance(vals, (list, tuple)):
+            vals = [vals]
+        if len(vals) < npairs:
+            raise ValueError("screening: %s needs one value per element pair (%d)" % (k, npairs))
+        arr = getattr(s, k)
+        for i in range(npairs):
+            arr[i] = float(vals[i])
+    return s
+
+
+class _Screened(_Potential):
+    """Common constructor of the screened variants: keyword arguments
+    update the default screened set; without arguments the library's
+    default (the reference's Fortran database entry) is used."""
+    _create = None
+    _default = None
+
+    def __init__(self, **kwargs):
+        super().__init__()
+        from . import parameters as _P
+        h = ctypes.c_void_p()
+        if kwargs:
+            params = dict(getattr(_P, self._default))
+            params.update(kwargs)
+            self._params = params
+            s, npairs = self._struct(params)
+            sc = _screening_struct(params, npairs)
+            call(self._create, ctypes.byref(h), ctypes.byref(s), ctypes.byref(sc))
+        else:
+            self._params = None
+            call(self._create, ctypes.byref(h), None, None)
+        self._h = h
+
+
+class TersoffScr(_Screened):
+    """TersoffScr (tersoff_scr.f90): Tersoff with the screening of
+    bop_kernel.f90 (SCREENING) and exp cutoffs.  Keyword arguments are those
+    of Tersoff plus or1, or2, bor1, bor2, Cmin, Cmax per element pair;
+    without arguments the screened Tersoff_PRB_39_5566_SiC."""
+    kind = _lib.KIND_TERSOFF_SCR
+    _create = "agpu_tersoff_scr_create"
+    _default = "Tersoff_PRB_39_5566_Si_C__Scr"
+
+    @staticmethod
+    def _struct(params):
+        nel = len(params["el"])
+        return _tersoff_struct(params), nel * (nel + 1) // 2
+
+
+class KumagaiScr(_Screened):
+    """KumagaiScr (kumagai_scr.f90); keyword arguments of Kumagai plus the
+    screening parameters; without arguments the screened
+    Kumagai_CompMaterSci_39_457_Si."""
+    kind = _lib.KIND_KUMAGAI_SCR
+    _create = "agpu_kumagai_scr_create"
+    _default = "Kumagai_CompMaterSci_39_457_Si__Scr"
+
+    @staticmethod
+    def _struct(params):
+        return _kumagai_struct(params), 1
+
+
+class BrennerScr(_Screened):
+    """BrennerScr (brenner_scr.f90); keyword arguments of Brenner plus the
+    screening parameters; without arguments the screened
+    Erhart_PRB_71_035211_SiC."""
+    kind = _lib.KIND_BRENNER_SCR
+    _create = "agpu_brenner_scr_create"
+    _default = "Erhart_PRB_71_035211_SiC__Scr"
+
+    @staticmethod
+    def _struct(params):
+        nel = len(params["el"])
+        return _brenner_struct(params), nel * (nel + 1) // 2
 
 
 class TabulatedAlloyEAM(_Potential):

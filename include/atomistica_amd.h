@@ -63,6 +63,9 @@ extern "C" {
 #define AGPU_KIND_REBO2 4
 #define AGPU_KIND_KUMAGAI 5
 #define AGPU_KIND_BRENNER 6
+#define AGPU_KIND_TERSOFF_SCR 7   /* tersoff_scr.f90, classname TersoffScr */
+#define AGPU_KIND_KUMAGAI_SCR 8   /* kumagai_scr.f90, classname KumagaiScr */
+#define AGPU_KIND_BRENNER_SCR 9   /* brenner_scr.f90, classname BrennerScr */
 
 /* agpu_rebo2_create flags */
 #define AGPU_REBO2_ZERO_TABLES 1  /* rebo2 zero_tables = .true. (rebo2_module.f90:30-42) */
@@ -212,6 +215,30 @@ typedef struct agpu_brenner_params {
     double r1[6], r2[6];
 } agpu_brenner_params;
 int agpu_brenner_create(agpu_potential **pot, const agpu_brenner_params *params);
+
+/* Screened variants: the same bond-order kernels with SCREENING defined
+ * (bop_kernel.f90:563-1066 loop1 screening, :1448-1492 and :1531-1609 its
+ * forces).  r1, r2 of the family's parameter block become the inner cutoff;
+ * every cutoff is the exp_cutoff form (cutoff.f90:232-293).  Per element
+ * pair (PAIR_INDEX order): outer (pair-term) cutoff [or1, or2], bond-order
+ * cutoff [bor1, bor2], screening parameters Cmin < Cmax
+ * (tersoff_params.f90:82-89).  The neighbour range requested is
+ * sqrt(C_dr_cut) * max(r2, or2, bor2), C_dr_cut = Cmax^2 / (4 (Cmax - 1))
+ * for Cmax > 2 (default_bind_to_func.f90:44-150).  NULL parameter blocks
+ * select the reference's screened defaults: Tersoff_PRB_39_5566_SiC,
+ * Kumagai_CompMaterSci_39_457_Si and Erhart_PRB_71_035211_SiC with
+ * SCREENING (tersoff_params.f90:99-138, kumagai_params.f90:97-132,
+ * brenner_params.f90:78-113).  Rows of more than 256 neighbours within the
+ * screening range return AGPU_ERR_NEB. */
+typedef struct {
+    double or1[6], or2[6], bor1[6], bor2[6], Cmin[6], Cmax[6];
+} agpu_screening_params;
+int agpu_tersoff_scr_create(agpu_potential **pot, const agpu_tersoff_params *params,
+                            const agpu_screening_params *scr);
+int agpu_kumagai_scr_create(agpu_potential **pot, const agpu_kumagai_params *params,
+                            const agpu_screening_params *scr);
+int agpu_brenner_scr_create(agpu_potential **pot, const agpu_brenner_params *params,
+                            const agpu_screening_params *scr);
 
 /* TabulatedAlloyEAM: tabulated_alloy_eam_init (tabulated_alloy_eam.f90:147-259)
  * reading a setfl file; elements = filter string ("*" or NULL = all). */
