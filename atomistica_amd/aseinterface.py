@@ -259,3 +259,6 @@ Ewald = _make(native.Ewald, "Ewald")
 Rebo2 = _make(native.Rebo2, "Rebo2")
 Kumagai = _make(native.Kumagai, "Kumagai")
 Brenner = _make(native.Brenner, "Brenner")
+TersoffScr = _make(native.TersoffScr, "TersoffScr")
+KumagaiScr = _make(native.KumagaiScr, "KumagaiScr")
+BrennerScr = _make(native.BrennerScr, "BrennerScr")

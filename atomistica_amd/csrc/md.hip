@@ -305,7 +305,11 @@ extern "C" int agpu_md_run(agpu_md *md, int nsteps, int graph_steps, double *epo
         // with a quarter of margin for acceleration, from the displacement
         // reached so far (the rollback below keeps this a prediction only)
         int K = 0;
-        if (graph_steps > 0 && md->skin > 0.0) {
+        // the screened bond-order kernels size their records on the host
+        // every step, so they always run step by step
+        const bool graphable = md->pot->kind != AGPU_KIND_TERSOFF_SCR && md->pot->kind != AGPU_KIND_KUMAGAI_SCR &&
+                               md->pot->kind != AGPU_KIND_BRENNER_SCR;
+        if (graphable && graph_steps > 0 && md->skin > 0.0) {
             const double room = 0.5 * md->skin - std::sqrt(d2);
             const double per = 1.25 * std::sqrt(v2) * md->dt;
             const double safe = per > 0.0 ? room / per : 1e9;

@@ -57,9 +57,16 @@ module atomistica_gpu
      real(c_double) :: r1(6), r2(6)
   end type brenner_params_t
 
+  !> Screening parameters of the screened variants (tersoff_params.f90:82-89):
+  !> outer and bond-order cutoffs and Cmin, Cmax per element pair
+  type, bind(C), public :: screening_params_t
+     real(c_double) :: or1(6), or2(6), bor1(6), bor2(6), Cmin(6), Cmax(6)
+  end type screening_params_t
+
   public :: particles_init, particles_set_cell, particles_set_Z, particles_set_positions
   public :: neighbors_init, tersoff_init, eam_alloy_init, ewald_init, rebo2_init
   public :: kumagai_init, brenner_init
+  public :: tersoff_scr_init, kumagai_scr_init, brenner_scr_init
   public :: set_precision, bind_to, energy_and_forces, coulomb_energy_and_forces
   public :: last_error, del
 
@@ -129,6 +136,21 @@ module atomistica_gpu
        import :: c_int, c_ptr
        type(c_ptr) :: pot
        type(c_ptr), value :: params
+     end function
+     integer(c_int) function agpu_tersoff_scr_create(pot, params, scr) bind(C, name="agpu_tersoff_scr_create")
+       import :: c_int, c_ptr
+       type(c_ptr) :: pot
+       type(c_ptr), value :: params, scr
+     end function
+     integer(c_int) function agpu_kumagai_scr_create(pot, params, scr) bind(C, name="agpu_kumagai_scr_create")
+       import :: c_int, c_ptr
+       type(c_ptr) :: pot
+       type(c_ptr), value :: params, scr
+     end function
+     integer(c_int) function agpu_brenner_scr_create(pot, params, scr) bind(C, name="agpu_brenner_scr_create")
+       import :: c_int, c_ptr
+       type(c_ptr) :: pot
+       type(c_ptr), value :: params, scr
      end function
      integer(c_int) function agpu_eam_alloy_create(pot, fn, elements) bind(C, name="agpu_eam_alloy_create")
        import :: c_int, c_ptr, c_char
@@ -267,6 +289,48 @@ contains
        ierror = agpu_brenner_create(this%h, c_null_ptr)
     end if
   end subroutine brenner_init
+
+  !> tersoff_scr_init (tersoff_scr.f90 with tersoff_module.f90:30-83): db and
+  !> scr absent select the screened Tersoff_PRB_39_5566_SiC; both or neither
+  subroutine tersoff_scr_init(this, ierror, db, scr)
+    type(potential_gpu_t), intent(inout) :: this
+    integer, intent(out) :: ierror
+    type(tersoff_params_t), optional, target, intent(in) :: db
+    type(screening_params_t), optional, target, intent(in) :: scr
+    if (present(db) .and. present(scr)) then
+       ierror = agpu_tersoff_scr_create(this%h, c_loc(db), c_loc(scr))
+    else
+       ierror = agpu_tersoff_scr_create(this%h, c_null_ptr, c_null_ptr)
+    end if
+  end subroutine tersoff_scr_init
+
+  !> kumagai_scr_init (kumagai_scr.f90): db and scr absent select the
+  !> screened Kumagai_CompMaterSci_39_457_Si
+  subroutine kumagai_scr_init(this, ierror, db, scr)
+    type(potential_gpu_t), intent(inout) :: this
+    integer, intent(out) :: ierror
+    type(kumagai_params_t), optional, target, intent(in) :: db
+    type(screening_params_t), optional, target, intent(in) :: scr
+    if (present(db) .and. present(scr)) then
+       ierror = agpu_kumagai_scr_create(this%h, c_loc(db), c_loc(scr))
+    else
+       ierror = agpu_kumagai_scr_create(this%h, c_null_ptr, c_null_ptr)
+    end if
+  end subroutine kumagai_scr_init
+
+  !> brenner_scr_init (brenner_scr.f90): db and scr absent select the
+  !> screened Erhart_PRB_71_035211_SiC
+  subroutine brenner_scr_init(this, ierror, db, scr)
+    type(potential_gpu_t), intent(inout) :: this
+    integer, intent(out) :: ierror
+    type(brenner_params_t), optional, target, intent(in) :: db
+    type(screening_params_t), optional, target, intent(in) :: scr
+    if (present(db) .and. present(scr)) then
+       ierror = agpu_brenner_scr_create(this%h, c_loc(db), c_loc(scr))
+    else
+       ierror = agpu_brenner_scr_create(this%h, c_null_ptr, c_null_ptr)
+    end if
+  end subroutine brenner_scr_init
 
   subroutine eam_alloy_init(this, fn, elements, ierror)
     type(potential_gpu_t), intent(inout) :: this

@@ -1,6 +1,7 @@
 !! Fortran host driver: a perfect 3x3x3 diamond Si crystal through
 !! init / bind_to / energy_and_forces of the atomistica_gpu module.
-!! Prints "E/atom <value>" and the largest force component.
+!! Prints "E/atom <value>" and the largest force component, then the same
+!! for the screened variant (TersoffScr).
 program tersoff_check
   use, intrinsic :: iso_c_binding
   use atomistica_gpu
@@ -12,7 +13,8 @@ program tersoff_check
   integer :: Z(nat), ix, iy, iz, b, k, ierr
   type(particles_gpu_t) :: p
   type(neighbors_gpu_t) :: nl
-  type(potential_gpu_t) :: pot
+  type(potential_gpu_t) :: pot, pscr
+  type(neighbors_gpu_t) :: nl2
 
   basis = reshape([0d0, 0d0, 0d0, 0d0, .5d0, .5d0, .5d0, 0d0, .5d0, .5d0, .5d0, 0d0, &
                    .25d0, .25d0, .25d0, .25d0, .75d0, .75d0, .75d0, .25d0, .75d0, .75d0, .75d0, .25d0], [3, 8])
@@ -47,6 +49,18 @@ program tersoff_check
   write (*, '(A,F20.12)') 'E/atom ', epot / nat
   write (*, '(A,ES12.4)') 'max|f| ', maxval(abs(f))
   write (*, '(A,ES12.4)') 'sum(epot_per_at)-epot ', sum(epa) - epot
+  ! the screened variant: in a perfect crystal every second neighbour is
+  ! screened and the bonds are equal, so the energy is the same
+  call neighbors_init(nl2, 200, ierr); call check(ierr)
+  call tersoff_scr_init(pscr, ierr); call check(ierr)
+  call bind_to(pscr, p, nl2, ierr); call check(ierr)
+  f = 0d0
+  wpot = 0d0
+  call energy_and_forces(pscr, p, nl2, epot, f, wpot, ierr); call check(ierr)
+  write (*, '(A,F20.12)') 'E/atom TersoffScr ', epot / nat
+  write (*, '(A,ES12.4)') 'max|f| TersoffScr ', maxval(abs(f))
+  call del(pscr)
+  call del(nl2)
   call del(pot)
   call del(nl)
   call del(p)

@@ -46,6 +46,11 @@ def build_workload(name):
         desc = ("Tersoff Si (PRB 39, 5566) diamond %dx%dx%d cells = %d atoms, a=5.431, "
                 "sigma=0.05 (rng 1), periodic" % (n, n, n, len(pos)))
         return dict(pos=pos, cell=cell, pbc=pbc, Z=Z, kind="tersoff", avgn=16, desc=desc)
+    if name == "tersoff_scr_si_64k":
+        pos, cell, pbc, Z = W.diamond_si(20, sigma=0.1, seed=1)
+        desc = ("TersoffScr Si (screened PRB 39, 5566 set) diamond 20x20x20 cells = %d atoms, a=5.431, "
+                "sigma=0.1 (rng 1), periodic; screening range 6.36 A" % len(pos))
+        return dict(pos=pos, cell=cell, pbc=pbc, Z=Z, kind="tersoff_scr", avgn=80, desc=desc)
     if name == "ewald_nacl_216k":
         pos, cell, pbc, Z, q = W.rocksalt(30, sigma=0.1, seed=5)
         beta = W.ewald_beta(10.0, 1e-8)
@@ -67,6 +72,8 @@ def make_potential(N, wl, precision):
         pot = N.TabulatedAlloyEAM(fn=wl["fn"])
     elif wl["kind"] == "tersoff":
         pot = N.Tersoff()
+    elif wl["kind"] == "tersoff_scr":
+        pot = N.TersoffScr()
     elif wl["kind"] == "rebo2":
         pot = N.Rebo2(dihedral=wl.get("dihedral", False))
     else:
@@ -90,6 +97,10 @@ def algorithmic_bytes(kernel, nn):
         return 36
     if kernel.startswith("ewald_real"):
         return 64 + 8 * nn
+    if kernel in ("scr_count", "scr_fill"):
+        return 36 + 16 * nn           # the row (8 per entry) and per-entry flags / cutoffs
+    if kernel.startswith("tersoff_scr"):
+        return 56 + 8 * nn
     if kernel == "rebo2_pair":
         return 64 + 8 * nn            # bond records stream once per bond-pair owner
     return None
@@ -157,6 +168,15 @@ def cpu_baseline(wl, budget_s=12.0):
         def one():
             nl = O.nl_build(pos, cell, pbc, 3.0, nthreads=threads)
             O.tersoff(P.Tersoff_PRB_39_5566_Si_C, pos, el, cell, nl, nthreads=threads)
+        sample_atoms = nat
+    elif wl["kind"] == "tersoff_scr":
+        el = np.ones(nat, np.int32)
+        prm = P.Tersoff_PRB_39_5566_Si_C__Scr
+        rc = 6.0 * np.sqrt(9.0 / 8.0)
+
+        def one():
+            nl = O.nl_build(pos, cell, pbc, rc, avgn=80, nthreads=threads)
+            O.tersoff(prm, pos, el, cell, nl, nthreads=threads)
         sample_atoms = nat
     elif wl["kind"] == "rebo2":
         R = O.Rebo2(dihedral=wl.get("dihedral", False))
@@ -530,7 +550,7 @@ def main():
         "autotune": tune,
     }
 
-    if not args.no_mixed and args.precision == "fp64" and wl["kind"] in ("eam", "tersoff", "ewald"):
+    if not args.no_mixed and args.precision == "fp64" and wl["kind"] in ("eam", "tersoff", "tersoff_scr", "ewald"):
         run.pot.set_precision("mixed")
         for _ in range(args.warmup):
             run.step()

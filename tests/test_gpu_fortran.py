@@ -25,3 +25,5 @@ def test_fortran_host_tersoff_diamond():
     assert abs(vals["E/atom"] - (-4.629593127667)) < 1e-10
     assert vals["max|f|"] < 1e-9
     assert abs(vals["sum(epot_per_at)-epot"]) < 1e-9
+    assert abs(vals["E/atom TersoffScr"] - (-4.629593127667)) < 1e-10
+    assert vals["max|f| TersoffScr"] < 1e-9
