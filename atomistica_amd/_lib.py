@@ -35,6 +35,7 @@ KIND_BRENNER = 6
 KIND_TERSOFF_SCR = 7
 KIND_KUMAGAI_SCR = 8
 KIND_BRENNER_SCR = 9
+KIND_PME = 10
 REBO2_ZERO_TABLES = 1
 REBO2_DIHEDRAL = 2
 
@@ -125,6 +126,7 @@ PROTOTYPES = {
     "agpu_brenner_scr_create": [ctypes.POINTER(vp), vp, vp],
     "agpu_eam_alloy_create": [ctypes.POINTER(vp), ctypes.c_char_p, ctypes.c_char_p],
     "agpu_ewald_create": [ctypes.POINTER(vp), d, d, d],
+    "agpu_pme_create": [ctypes.POINTER(vp), d, i, ip],
     "agpu_rebo2_create": [ctypes.POINTER(vp), i],
     "agpu_potential_destroy": [vp],
     "agpu_potential_kind": [vp, ip],

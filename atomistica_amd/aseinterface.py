@@ -262,3 +262,4 @@ Brenner = _make(native.Brenner, "Brenner")
 TersoffScr = _make(native.TersoffScr, "TersoffScr")
 KumagaiScr = _make(native.KumagaiScr, "KumagaiScr")
 BrennerScr = _make(native.BrennerScr, "BrennerScr")
+PME = _make(native.PME, "PME")

@@ -223,6 +223,13 @@ int eam_variant(agpu_potential *pot, int k, int *g, int *u);
 int ewald_bind(agpu_potential *pot);
 int ewald_compute(agpu_potential *pot, const agpu_eval_args *a);
 int ewald_potential(agpu_potential *pot, const double *q, double *phi, int flags);
+// smooth PME reciprocal sum (pme.hip): rec10 := [E, W column-major], f +=
+// q E, phi += phi_rec (f, phi may be NULL)
+struct PmeState;
+PmeState *pme_state_new(int order, const int *grid);
+void pme_state_free(PmeState *pm);
+int pme_recip(PmeState *pm, const agpu_particles *p, const double *q, double beta, double *rec10, double *f,
+              double *phi);
 int rebo2_bind(agpu_potential *pot);
 int rebo2_compute(agpu_potential *pot, const agpu_eval_args *a);
 

@@ -640,7 +640,8 @@ int agpu_potential_bind_to(agpu_potential *pot, agpu_particles *p, agpu_neighbor
     case AGPU_KIND_KUMAGAI_SCR:
     case AGPU_KIND_BRENNER_SCR: rc = tersoff_bind(pot); break;
     case AGPU_KIND_EAM_ALLOY: rc = eam_bind(pot); break;
-    case AGPU_KIND_EWALD: rc = ewald_bind(pot); break;
+    case AGPU_KIND_EWALD:
+    case AGPU_KIND_PME: rc = ewald_bind(pot); break;
     case AGPU_KIND_REBO2: rc = rebo2_bind(pot); break;
     default: return set_error(AGPU_ERR_STATE, "bind_to: unknown potential kind");
     }
@@ -664,7 +665,8 @@ static int stage_and_call(agpu_potential *pot, agpu_particles *p, agpu_neighbors
         case AGPU_KIND_KUMAGAI_SCR:
         case AGPU_KIND_BRENNER_SCR: rc = tersoff_compute(pot, &a); break;
         case AGPU_KIND_EAM_ALLOY: rc = eam_compute(pot, &a); break;
-        case AGPU_KIND_EWALD: rc = ewald_compute(pot, &a); break;
+        case AGPU_KIND_EWALD:
+        case AGPU_KIND_PME: rc = ewald_compute(pot, &a); break;
         case AGPU_KIND_REBO2: rc = rebo2_compute(pot, &a); break;
         default: rc = set_error(AGPU_ERR_STATE, "unknown potential kind");
         }
@@ -713,7 +715,8 @@ static int stage_and_call(agpu_potential *pot, agpu_particles *p, agpu_neighbors
     case AGPU_KIND_KUMAGAI_SCR:
     case AGPU_KIND_BRENNER_SCR: rc = tersoff_compute(pot, &d); break;
     case AGPU_KIND_EAM_ALLOY: rc = eam_compute(pot, &d); break;
-    case AGPU_KIND_EWALD: rc = ewald_compute(pot, &d); break;
+    case AGPU_KIND_EWALD:
+    case AGPU_KIND_PME: rc = ewald_compute(pot, &d); break;
     case AGPU_KIND_REBO2: rc = rebo2_compute(pot, &d); break;
     default: rc = set_error(AGPU_ERR_STATE, "unknown potential kind");
     }
@@ -737,7 +740,7 @@ int agpu_potential_energy_and_forces(agpu_potential *pot, agpu_particles *p, agp
     if (epot_per_bond || f_per_bond || wpot_per_bond)
         return set_error(AGPU_ERR_UNSUPPORTED,
                          "energy_and_forces: per-bond properties are not supported by this class");
-    if (pot->kind == AGPU_KIND_EWALD && !q)
+    if ((pot->kind == AGPU_KIND_EWALD || pot->kind == AGPU_KIND_PME) && !q)
         return set_error(AGPU_ERR_ARG, "energy_and_forces: Coulomb class needs charges");
     if (!epot || !f || !wpot) return set_error(AGPU_ERR_ARG, "energy_and_forces: epot, f and wpot are required");
     agpu_eval_args a;
@@ -756,7 +759,8 @@ int agpu_coulomb_potential(agpu_potential *pot, agpu_particles *p, agpu_neighbor
                            const double *q, double *phi, int flags)
 {
     if (!pot || !p || !nl || !q || !phi) return set_error(AGPU_ERR_ARG, "potential: bad arguments");
-    if (pot->kind != AGPU_KIND_EWALD) return set_error(AGPU_ERR_UNSUPPORTED, "potential: not a Coulomb class");
+    if (pot->kind != AGPU_KIND_EWALD && pot->kind != AGPU_KIND_PME)
+        return set_error(AGPU_ERR_UNSUPPORTED, "potential: not a Coulomb class");
     if (!pot->bound || pot->p != p || pot->nl != nl)
         return set_error(AGPU_ERR_STATE, "potential: not bound to these objects");
     if (!(flags & AGPU_HOST)) return ewald_potential(pot, q, phi, 0);
@@ -790,7 +794,7 @@ static int lammps_step(agpu_potential *pot, agpu_particles *p, agpu_neighbors *n
         return set_error(AGPU_ERR_ARG, "lammps_compute_list: bad neighbour arrays");
     if (pot->kind == AGPU_KIND_REBO2)
         return set_error(AGPU_ERR_UNSUPPORTED, "lammps_compute: REBO2 takes no mask (needs the tag half rule)");
-    if (pot->kind == AGPU_KIND_EWALD)
+    if (pot->kind == AGPU_KIND_EWALD || pot->kind == AGPU_KIND_PME)
         return set_error(AGPU_ERR_UNSUPPORTED, "lammps_compute: Ewald needs a periodic cell");
     if (nall == 0) return AGPU_OK;
     double lo[3], hi[3];

@@ -7,7 +7,7 @@
 // row (one wavefront per atom), so the reference's i<j scatter becomes a
 // gather with the 1/2 of the directed double count -- no atomics.
 //
-// Reciprocal space: direct structure-factor sum over the half space of
+// Reciprocal space (Ewald class): direct structure-factor sum over the half space of
 // k = 2 pi Bbox^T n with |k| <= kmax (the reference uses smooth PME instead,
 // pme_kernel.f90:251-752).  S(k) is accumulated in FP64:
 //   k_ewald_sk     thread = run of up to L consecutive n3 in one (n1, n2)
@@ -18,6 +18,8 @@
 //                  virial records, and T(k) = c(k) S(k) for the gather.
 //   k_ewald_kforce thread = ion: gathers sum_k T(k) exp(-i k.r) over k staged
 //                  through LDS; force and electrostatic potential per ion.
+// The PME class (agpu_pme_create) shares the real space and self term and
+// takes its reciprocal sum from pme.hip.
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -44,6 +46,12 @@ struct EwaldState {
     std::vector<EwItem> items;
     std::vector<int> kpack;
     DevBuf d_items, d_kpack, d_S, d_T, d_frac, partial, phi_tmp;
+    // smooth PME reciprocal sum instead of the direct S(k) sum (PME class)
+    PmeState *pme = nullptr;
+    ~EwaldState()
+    {
+        if (pme) pme_state_free(pme);
+    }
 };
 
 static void ewald_free(void *p) { delete static_cast<EwaldState *>(p); }
@@ -442,7 +450,7 @@ static int ewald_run(agpu_potential *pot, const double *q, double *epot, double 
     agpu_particles *p = pot->p;
     agpu_neighbors *nl = pot->nl;
     AGPU_CHECK(nl_update(nl, p, 0));
-    if (!st->have_k || std::memcmp(st->cell_key, p->cell.A, sizeof st->cell_key) != 0)
+    if (!st->pme && (!st->have_k || std::memcmp(st->cell_key, p->cell.A, sizeof st->cell_key) != 0))
         AGPU_CHECK(build_kvectors(*st, p));
     const int nat = p->nat;
     hipStream_t s = stream();
@@ -465,6 +473,13 @@ static int ewald_run(agpu_potential *pot, const double *q, double *epot, double 
         AGPU_LAUNCH("ewald_real_fp64", k_ewald_real<double>, dim3(nblk_r), dim3(256), 0, s, nat, order, p->d_r, q,
                     C, nl->offsets.as<int>(), nl->nbr.as<int2>(), st->cutoff * st->cutoff, st->beta, sap, f,
                     phi, part);
+    if (st->pme) {
+        // reciprocal record after the real-space block records
+        AGPU_CHECK(pme_recip(st->pme, p, q, st->beta, part + 10 * (size_t)nblk_r, f, phi));
+        AGPU_CHECK(reduce_ev(part, nblk_r + 1, 10, epot, wpot, nullptr));
+        AGPU_HIP(hipStreamSynchronize(s));
+        return AGPU_OK;
+    }
     AGPU_CHECK(st->d_frac.reserve(sizeof(double) * 4 * (size_t)(nat + 1)));
     AGPU_LAUNCH("ewald_frac", k_frac, dim3((nat + 255) / 256), dim3(256), 0, s, nat, p->d_r, Bt,
                 st->d_frac.as<double>());
@@ -514,6 +529,32 @@ extern "C" int agpu_ewald_create(agpu_potential **pot, double cutoff, double bet
     st->kmax_req = kmax;
     agpu_potential *p = new agpu_potential();
     p->kind = AGPU_KIND_EWALD;
+    p->cutoff = cutoff;
+    p->impl = st;
+    p->free_impl = ewald_free;
+    *pot = p;
+    return AGPU_OK;
+}
+
+extern "C" int agpu_pme_create(agpu_potential **pot, double cutoff, int order, const int *grid)
+{
+    if (!pot) return set_error(AGPU_ERR_ARG, "pme_create: pot is NULL");
+    // pme_t defaults (pme.f90:36-46): cutoff 10, grid 64^3, order 8
+    if (!(cutoff > 0.0)) cutoff = 10.0;
+    if (order <= 0) order = 8;
+    int K[3] = {64, 64, 64};
+    if (grid)
+        for (int d = 0; d < 3; ++d) K[d] = grid[d];
+    if (order < 3 || order > 12) return set_error(AGPU_ERR_ARG, "PME: order must be 3 .. 12");
+    for (int d = 0; d < 3; ++d)
+        if (K[d] < order || K[d] > 1000)
+            return set_error(AGPU_ERR_ARG, "PME: every grid dimension must be >= order and <= 1000");
+    if ((long long)K[0] * K[1] * K[2] > (1ll << 30)) return set_error(AGPU_ERR_ARG, "PME: grid too large");
+    EwaldState *st = new EwaldState();
+    st->cutoff = cutoff;
+    st->pme = pme_state_new(order, K);
+    agpu_potential *p = new agpu_potential();
+    p->kind = AGPU_KIND_PME;
     p->cutoff = cutoff;
     p->impl = st;
     p->free_impl = ewald_free;

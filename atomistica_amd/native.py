@@ -531,6 +531,24 @@ class Ewald(_Potential):
         return phi
 
 
+class PME(Ewald):
+    """PME (pme.f90, classname PME): pme_init(cutoff, order, grid); real
+    space and self term as Ewald with beta^2 = ln(10)*12/cutoff^2, smooth-PME
+    reciprocal sum with B-splines of the given order on a K1 x K2 x K3
+    FFT grid."""
+    kind = _lib.KIND_PME
+
+    def __init__(self, cutoff=10.0, order=8, grid=(64, 64, 64)):
+        _Potential.__init__(self)
+        h = ctypes.c_void_p()
+        g = (ctypes.c_int * 3)(*[int(x) for x in grid])
+        call("agpu_pme_create", ctypes.byref(h), float(cutoff), int(order), g)
+        self._h = h
+        self.cutoff = cutoff
+        self.order = order
+        self.grid = tuple(int(x) for x in grid)
+
+
 def lammps_compute(pot, p, nl, nlocal, x, Z, f=None, eatom=False):
     """One LAMMPS pair-style step: local atoms then ghosts in x (nall, 3).
     Returns (eng, f (nall, 3), virial[6] in LAMMPS order and sign, eatom)."""

@@ -58,6 +58,12 @@ def build_workload(name):
                 "rc=10, erfc(beta rc)=1e-8, exp(-kmax^2/4beta^2)=1e-8")
         return dict(pos=pos, cell=cell, pbc=pbc, Z=Z, q=q, kind="ewald", beta=beta,
                     kmax=W.ewald_kmax(beta, 1e-8), avgn=220, desc=desc)
+    if name == "pme_nacl_216k":
+        pos, cell, pbc, Z, q = W.rocksalt(30, sigma=0.1, seed=5)
+        desc = ("PME rocksalt +-1e 30^3 cells = 216000 ions, a=5.64, sigma=0.1 (rng 5), rc=10 "
+                "(beta^2 = ln(10) 12 / rc^2), B-spline order 8 on a 160^3 FFT grid")
+        return dict(pos=pos, cell=cell, pbc=pbc, Z=Z, q=q, kind="pme", grid=(160, 160, 160), order=8,
+                    avgn=220, desc=desc)
     if name == "rebo2_ch_100k":
         pos, cell, pbc, Z = W.amorphous_ch(100000, x_h=0.2, density=2.0, seed=10)
         desc = ("REBO2 amorphous hydrocarbon 100000 atoms (80% C, 20% H), cubic periodic box at "
@@ -76,6 +82,8 @@ def make_potential(N, wl, precision):
         pot = N.TersoffScr()
     elif wl["kind"] == "rebo2":
         pot = N.Rebo2(dihedral=wl.get("dihedral", False))
+    elif wl["kind"] == "pme":
+        pot = N.PME(cutoff=10.0, order=wl["order"], grid=wl["grid"])
     else:
         pot = N.Ewald(cutoff=10.0, beta=wl["beta"], kmax=wl["kmax"])
     pot.set_precision(precision)
@@ -101,6 +109,8 @@ def algorithmic_bytes(kernel, nn):
         return 36 + 16 * nn           # the row (8 per entry) and per-entry flags / cutoffs
     if kernel.startswith("tersoff_scr"):
         return 56 + 8 * nn
+    if kernel == "pme_spread":
+        return None                   # grid-point gather; traffic is per grid point
     if kernel == "rebo2_pair":
         return 64 + 8 * nn            # bond records stream once per bond-pair owner
     return None

@@ -66,6 +66,7 @@ extern "C" {
 #define AGPU_KIND_TERSOFF_SCR 7   /* tersoff_scr.f90, classname TersoffScr */
 #define AGPU_KIND_KUMAGAI_SCR 8   /* kumagai_scr.f90, classname KumagaiScr */
 #define AGPU_KIND_BRENNER_SCR 9   /* brenner_scr.f90, classname BrennerScr */
+#define AGPU_KIND_PME 10          /* pme.f90, classname PME */
 
 /* agpu_rebo2_create flags */
 #define AGPU_REBO2_ZERO_TABLES 1  /* rebo2 zero_tables = .true. (rebo2_module.f90:30-42) */
@@ -250,6 +251,15 @@ int agpu_eam_alloy_create(agpu_potential **pot, const char *fn, const char *elem
  * beta^2 = ln(10)*12/cutoff^2 (pme.f90:192); kmax <= 0 selects
  * exp(-kmax^2/(4 beta^2)) = 1e-13. */
 int agpu_ewald_create(agpu_potential **pot, double cutoff, double beta, double kmax);
+
+/* PME: pme_init(cutoff, order, grid) (pme.f90:129-147) -- the same real space
+ * and self term with beta^2 = ln(10)*12/cutoff^2, and the smooth PME
+ * reciprocal sum of pme_kernel.f90:251-752 (cardinal B-splines of the given
+ * order, FFT grid K1 x K2 x K3).  cutoff <= 0, order <= 0 and grid NULL
+ * select the defaults 10, 8, 64^3.  order 3 .. 12, every K >= order.  As in
+ * the reference, scaled fractional coordinates are taken with the columns of
+ * Bbox, exact for orthorhombic cells. */
+int agpu_pme_create(agpu_potential **pot, double cutoff, int order, const int *grid);
 
 /* REBO2 (Brenner et al. 2002) for C and H without screening: rebo2_init and
  * rebo2_db_init_with_parameters (rebo2_module.f90:30-42, rebo2_db.f90:81-303)

@@ -373,6 +373,123 @@ def ewald_kspace(r, q, cell, beta, kmax, nthreads=1):
 
 
 # --------------------------------------------------------------------------
+# smooth PME reciprocal sum (numpy restatement of pme_kernel.f90)
+# --------------------------------------------------------------------------
+
+def _bsp_one_pass(c, x, k):
+    """bsp_one_pass (pme_kernel.f90), 0-based, vectorised over atoms (c has
+    shape (order, n))."""
+    div = 1.0 / (k - 1)
+    c[k - 1] = div * x * c[k - 2]
+    for j in range(1, k - 1):
+        c[k - j - 1] = div * ((x + j) * c[k - j - 2] + (k - j - x) * c[k - j - 1])
+    c[0] = div * (1 - x) * c[0]
+
+
+def pme_bspline(w, order):
+    """fill_bspline (pme_kernel.f90): cardinal B-spline weights and their
+    derivatives at offset w for one dimension; returns (order, n) arrays."""
+    w = np.atleast_1d(np.asarray(w, float))
+    c = np.zeros((order, len(w)))
+    c[1] = w
+    c[0] = 1.0 - w
+    for k in range(3, order):
+        _bsp_one_pass(c, w, k)
+    d = np.empty_like(c)
+    d[0] = -c[0]
+    d[1:] = c[:-1] - c[1:]
+    _bsp_one_pass(c, w, order)
+    return c, d
+
+
+def pme_bsp_moduli(K, order):
+    """load_bsp_moduli / dftmod (pme_kernel.f90): |b(m)|^-2 denominators."""
+    arr, _ = pme_bspline([0.0], order)
+    bsp = np.zeros(K)
+    bsp[1:order + 1] = arr[:min(order, K - 1), 0]
+    j = np.arange(K)
+    mod = np.empty(K)
+    for k in range(K):
+        a = 2 * np.pi * k * j / K
+        mod[k] = np.sum(bsp * np.cos(a)) ** 2 + np.sum(bsp * np.sin(a)) ** 2
+    for k in range(K):
+        if mod[k] < 1e-7:
+            mod[k] = 0.5 * (mod[k - 1] + mod[(k + 1) % K])
+    return mod
+
+
+def pme_recip(r, q, cell, beta, grid, order=8):
+    """Reciprocal part of smooth PME as pme_grid_potential_and_field does it
+    (pme_kernel.f90:251-752): scaled fractionals with the columns of Bbox,
+    B-spline charge spreading, backward FFT, energy and virial with the
+    influence function exp(-pi^2 m^2/beta^2)/(B(m) m^2), forward FFT, field
+    gather.  Returns (epot, f, wpot, phi) in e^2/A units; f = q E."""
+    r = np.asarray(r, float)
+    q = np.asarray(q, float)
+    nat = len(r)
+    K = [int(g) for g in grid]
+    recip = np.linalg.inv(np.asarray(cell, float).T)      # Bbox(i, j)
+    V = abs(np.linalg.det(cell))
+    fr, th, dth, base = [], [], [], []
+    for d in range(3):
+        w = r[:, 0] * recip[0, d] + r[:, 1] * recip[1, d] + r[:, 2] * recip[2, d]
+        an = np.where(w >= 0, np.floor(w + 0.5), np.ceil(w - 0.5))     # Fortran anint
+        f = K[d] * (w - an + 0.5)
+        ib = f.astype(int)
+        t, dt = pme_bspline(f - ib, order)
+        fr.append(f)
+        th.append(t)
+        dth.append(dt)
+        base.append(ib)
+    # grid indices (b - order + 1 + t) mod K, weights theta[t]
+    idx = [np.mod(base[d][None, :] - order + 1 + np.arange(order)[:, None], K[d]) for d in range(3)]
+    Q = np.zeros(K, complex)
+    I = idx[0][:, None, None, :]
+    J = idx[1][None, :, None, :]
+    Kk = idx[2][None, None, :, :]
+    W = th[0][:, None, None, :] * th[1][None, :, None, :] * th[2][None, None, :, :] * q[None, None, None, :]
+    np.add.at(Q, (np.broadcast_to(I, W.shape), np.broadcast_to(J, W.shape), np.broadcast_to(Kk, W.shape)), W)
+    Q = np.fft.ifftn(Q) * Q.size                              # FFTW backward, unnormalised
+    mods = [pme_bsp_moduli(K[d], order) for d in range(3)]
+    m = []
+    for d in range(3):
+        k = np.arange(K[d])
+        nf = (K[d] + 1) // 2
+        m.append(np.where(k < nf, k, k - K[d]).astype(float))
+    M1, M2, M3 = np.meshgrid(m[0], m[1], m[2], indexing="ij")
+    mh = [recip[i, 0] * M1 + recip[i, 1] * M2 + recip[i, 2] * M3 for i in range(3)]
+    msq = mh[0] ** 2 + mh[1] ** 2 + mh[2] ** 2
+    msq[0, 0, 0] = 1.0
+    fac = np.pi ** 2 / beta ** 2
+    denom = mods[0][:, None, None] * mods[1][None, :, None] * mods[2][None, None, :] * msq
+    eterm = np.exp(-fac * msq) / denom
+    eterm[0, 0, 0] = 0.0
+    vterm = 2.0 * (fac * msq + 1.0) / msq
+    struc2 = (Q * np.conj(Q)).real
+    es = eterm * struc2
+    vir = np.zeros((3, 3))
+    for a in range(3):
+        for b in range(3):
+            vir[a, b] = np.sum(es * (vterm * mh[a] * mh[b] - (1.0 if a == b else 0.0)))
+    energy = np.sum(es)
+    Q = Q * eterm
+    Q = np.fft.fftn(Q)                                         # FFTW forward
+    pre = 1.0 / (2 * np.pi * V)
+    T = Q.real
+    G = T[np.broadcast_to(I, W.shape), np.broadcast_to(J, W.shape), np.broadcast_to(Kk, W.shape)]
+    t1, t2, t3 = th[0][:, None, None, :], th[1][None, :, None, :], th[2][None, None, :, :]
+    d1, d2, d3 = dth[0][:, None, None, :], dth[1][None, :, None, :], dth[2][None, None, :, :]
+    f0 = np.sum(G * t1 * t2 * t3, axis=(0, 1, 2))
+    g1 = -K[0] * np.sum(G * d1 * t2 * t3, axis=(0, 1, 2))
+    g2 = -K[1] * np.sum(G * t1 * d2 * t3, axis=(0, 1, 2))
+    g3 = -K[2] * np.sum(G * t1 * t2 * d3, axis=(0, 1, 2))
+    fac2 = 1.0 / (np.pi * V)
+    phi = fac2 * f0
+    E = np.stack([fac2 * (recip[i, 0] * g1 + recip[i, 1] * g2 + recip[i, 2] * g3) for i in range(3)], axis=1)
+    return pre * energy, q[:, None] * E, pre * vir, phi
+
+
+# --------------------------------------------------------------------------
 # setfl tables for eam()
 # --------------------------------------------------------------------------
 
