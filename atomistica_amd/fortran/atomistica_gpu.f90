@@ -66,7 +66,7 @@ module atomistica_gpu
   public :: particles_init, particles_set_cell, particles_set_Z, particles_set_positions
   public :: neighbors_init, tersoff_init, eam_alloy_init, ewald_init, rebo2_init
   public :: kumagai_init, brenner_init
-  public :: tersoff_scr_init, kumagai_scr_init, brenner_scr_init
+  public :: tersoff_scr_init, kumagai_scr_init, brenner_scr_init, pme_init
   public :: set_precision, bind_to, energy_and_forces, coulomb_energy_and_forces
   public :: last_error, del
 
@@ -161,6 +161,13 @@ module atomistica_gpu
        import :: c_int, c_ptr, c_double
        type(c_ptr) :: pot
        real(c_double), value :: cutoff, beta, kmax
+     end function
+     integer(c_int) function agpu_pme_create(pot, cutoff, order, grid) bind(C, name="agpu_pme_create")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr) :: pot
+       real(c_double), value :: cutoff
+       integer(c_int), value :: order
+       integer(c_int), intent(in) :: grid(3)
      end function
      integer(c_int) function agpu_rebo2_create(pot, flags) bind(C, name="agpu_rebo2_create")
        import :: c_int, c_ptr
@@ -345,6 +352,25 @@ contains
     integer, intent(out) :: ierror
     ierror = agpu_ewald_create(this%h, cutoff, beta, kmax)
   end subroutine ewald_init
+
+  !> pme_init (pme.f90:129-147): cutoff, order and grid as there; absent
+  !> arguments keep the defaults 10, 8, 64^3
+  subroutine pme_init(this, ierror, cutoff, order, grid)
+    type(potential_gpu_t), intent(inout) :: this
+    integer, intent(out) :: ierror
+    real(c_double), intent(in), optional :: cutoff
+    integer, intent(in), optional :: order
+    integer, intent(in), optional :: grid(3)
+    real(c_double) :: c
+    integer(c_int) :: o, g(3)
+    c = 10.0d0
+    o = 8
+    g = 64
+    if (present(cutoff)) c = cutoff
+    if (present(order)) o = order
+    if (present(grid)) g = grid
+    ierror = agpu_pme_create(this%h, c, o, g)
+  end subroutine pme_init
 
   !> REBO2 (rebo2_init, rebo2_module.f90:30-42): dihedral = with_dihedral,
   !> zero_tables as the reference's switch of the same name
