@@ -86,6 +86,11 @@ struct agpu_particles {
     uint64_t cell_rev = 0;
     uint64_t Z_rev = 0;
     std::vector<int> h_Z;         // host copy of atomic numbers
+    // LAMMPS step: global atom IDs of the local atoms and ghosts and the
+    // number of local atoms (the REBO2 half rule tag(j) >= tag(i) over
+    // local i, bop_kernel_rebo2.f90:1328-1333); natloc < 0 when unset
+    agpu::DevBuf d_tag;
+    int natloc = -1;
 };
 
 struct agpu_neighbors {

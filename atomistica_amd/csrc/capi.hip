@@ -784,16 +784,18 @@ int agpu_coulomb_potential(agpu_potential *pot, agpu_particles *p, agpu_neighbor
 // to *eng and the virial subtracted in Voigt order as the reference does.
 // One LAMMPS step; the list is built here (list == false) or was imported.
 static int lammps_step(agpu_potential *pot, agpu_particles *p, agpu_neighbors *nl, int nlocal, int nall,
-                       const double *x, const int *Z, const int64_t *seed, const int *numneigh, const int *neigh,
-                       double *f, double *eng, double *virial, double *eatom)
+                       const double *x, const int *Z, const int *tag, const int64_t *seed, const int *numneigh,
+                       const int *neigh, double *f, double *eng, double *virial, double *eatom)
 {
     if (!pot || !p || !nl || nlocal < 0 || nall < nlocal || (nall > 0 && (!x || !Z || !f)) || !eng || !virial)
         return set_error(AGPU_ERR_ARG, "lammps_compute: bad arguments");
     const bool list = seed != nullptr;
     if (list && nall > 0 && (!numneigh || !neigh))
         return set_error(AGPU_ERR_ARG, "lammps_compute_list: bad neighbour arrays");
-    if (pot->kind == AGPU_KIND_REBO2)
-        return set_error(AGPU_ERR_UNSUPPORTED, "lammps_compute: REBO2 takes no mask (needs the tag half rule)");
+    const bool rebo2 = pot->kind == AGPU_KIND_REBO2;
+    if (rebo2 && !tag)
+        return set_error(AGPU_ERR_UNSUPPORTED,
+                         "lammps_compute: REBO2 takes no mask and needs the atom tags (agpu_lammps_compute_tagged)");
     if (pot->kind == AGPU_KIND_EWALD || pot->kind == AGPU_KIND_PME)
         return set_error(AGPU_ERR_UNSUPPORTED, "lammps_compute: Ewald needs a periodic cell");
     if (nall == 0) return AGPU_OK;
@@ -819,11 +821,22 @@ static int lammps_step(agpu_potential *pot, agpu_particles *p, agpu_neighbors *n
     nl->external = false;
     if (!pot->bound || pot->p != p || pot->nl != nl) AGPU_CHECK(agpu_potential_bind_to(pot, p, nl));
     if (list) AGPU_CHECK(nl_import(nl, p, seed, numneigh, neigh));
-    std::vector<int> mask(nall);
-    for (int i = 0; i < nall; ++i) mask[i] = i < nlocal;
     double epot = 0.0, wpot[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    AGPU_CHECK(agpu_potential_energy_and_forces(pot, p, nl, nullptr, &epot, f, wpot, mask.data(), eatom,
-                                                nullptr, nullptr, nullptr, nullptr, AGPU_HOST));
+    if (rebo2) {
+        // half rule by tags over the local centres instead of a mask
+        AGPU_CHECK(p->d_tag.reserve(sizeof(int) * (size_t)nall));
+        AGPU_HIP(hipMemcpyAsync(p->d_tag.ptr, tag, sizeof(int) * (size_t)nall, hipMemcpyHostToDevice, stream()));
+        p->natloc = nlocal;
+        const int rc = agpu_potential_energy_and_forces(pot, p, nl, nullptr, &epot, f, wpot, nullptr, eatom,
+                                                        nullptr, nullptr, nullptr, nullptr, AGPU_HOST);
+        p->natloc = -1;
+        AGPU_CHECK(rc);
+    } else {
+        std::vector<int> mask(nall);
+        for (int i = 0; i < nall; ++i) mask[i] = i < nlocal;
+        AGPU_CHECK(agpu_potential_energy_and_forces(pot, p, nl, nullptr, &epot, f, wpot, mask.data(), eatom,
+                                                    nullptr, nullptr, nullptr, nullptr, AGPU_HOST));
+    }
     *eng += epot;
     virial[0] -= wpot[0];
     virial[1] -= wpot[4];
@@ -837,7 +850,15 @@ static int lammps_step(agpu_potential *pot, agpu_particles *p, agpu_neighbors *n
 int agpu_lammps_compute(agpu_potential *pot, agpu_particles *p, agpu_neighbors *nl, int nlocal, int nall,
                         const double *x, const int *Z, double *f, double *eng, double *virial, double *eatom)
 {
-    return lammps_step(pot, p, nl, nlocal, nall, x, Z, nullptr, nullptr, nullptr, f, eng, virial, eatom);
+    return lammps_step(pot, p, nl, nlocal, nall, x, Z, nullptr, nullptr, nullptr, nullptr, f, eng, virial, eatom);
+}
+
+int agpu_lammps_compute_tagged(agpu_potential *pot, agpu_particles *p, agpu_neighbors *nl, int nlocal, int nall,
+                               const double *x, const int *Z, const int *tag, const int64_t *seed,
+                               const int *numneigh, const int *neigh, double *f, double *eng, double *virial,
+                               double *eatom)
+{
+    return lammps_step(pot, p, nl, nlocal, nall, x, Z, tag, seed, numneigh, neigh, f, eng, virial, eatom);
 }
 
 int agpu_lammps_compute_list(agpu_potential *pot, agpu_particles *p, agpu_neighbors *nl, int nlocal, int nall,
@@ -845,7 +866,7 @@ int agpu_lammps_compute_list(agpu_potential *pot, agpu_particles *p, agpu_neighb
                              const int *neigh, double *f, double *eng, double *virial, double *eatom)
 {
     if (!seed) return set_error(AGPU_ERR_ARG, "lammps_compute_list: seed is NULL");
-    return lammps_step(pot, p, nl, nlocal, nall, x, Z, seed, numneigh, neigh, f, eng, virial, eatom);
+    return lammps_step(pot, p, nl, nlocal, nall, x, Z, nullptr, seed, numneigh, neigh, f, eng, virial, eatom);
 }
 
 }  // extern "C"

@@ -898,12 +898,14 @@ __device__ double rb_torsion(int ib, int inb, int sij, int jb, int jnb, int sji,
 // Bond order of every directed bond c->o (thread per bond slot): zeta with
 // P(N_H, N_C) for a carbon centre, b = (1 + zeta)^-1/2 and dV_A b'/dzeta,
 // dP/dN + dzeta/dN; the slot of the reverse bond o->c; and whether the slot
-// carries a half pair ((dc = 0 and o > c) or dc > 0).
+// carries a half pair ((dc = 0 and o > c) or dc > 0; for a LAMMPS step, o of
+// tag >= tag(c) with c local).
 __global__ void __launch_bounds__(256)
 k_rb_bo(int ntot, const RbDev *__restrict__ Dp, const double *__restrict__ tab, const int *__restrict__ el,
         const int *__restrict__ offsets, const int *__restrict__ bcnt, const int *__restrict__ bown,
         const RbBond *__restrict__ bond, const int2 *__restrict__ bij, const double2 *__restrict__ nn,
-        double4 *__restrict__ bo, int *__restrict__ rev, int *__restrict__ half, int *__restrict__ flags)
+        double4 *__restrict__ bo, int *__restrict__ rev, int *__restrict__ half, int natloc,
+        const int *__restrict__ tag, int *__restrict__ flags)
 {
     const RbDev &D = *Dp;
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -923,7 +925,11 @@ k_rb_bo(int ntot, const RbDev *__restrict__ Dp, const double *__restrict__ tab, 
     const bool pos = a != 0 ? a > 0 : (b != 0 ? b > 0 : c > 0);
     const bool zero = a == 0 && b == 0 && c == 0;
     const bool act = B.len < D.cut_h[ijpot];
-    half[s] = (act && ((zero && j > i) || pos)) ? 1 : 0;
+    // LAMMPS: local centres only, tag(j) >= tag(i) (bop_kernel_rebo2.f90:1328-1329)
+    if (tag)
+        half[s] = (act && i < natloc && tag[j] >= tag[i]) ? 1 : 0;
+    else
+        half[s] = (act && ((zero && j > i) || pos)) ? 1 : 0;
     int sr = -1;
     const int jb = offsets[j], jn = bcnt[j];
     for (int q = jb; q < jb + jn; ++q) {
@@ -1230,7 +1236,7 @@ int rebo2_compute(agpu_potential *pot, const agpu_eval_args *a)
         AGPU_LAUNCH("rebo2_bo", k_rb_bo, dim3(nblk), dim3(256), 0, s, ntot, D, st->tab.as<double>(), el,
                     nl->offsets.as<int>(), st->bcnt.as<int>(), st->bown.as<int>(), st->bond.as<RbBond>(),
                     st->bij.as<int2>(), st->nn.as<double2>(), st->bo.as<double4>(), st->rev.as<int>(),
-                    st->half.as<int>(), dstats);
+                    st->half.as<int>(), p->natloc, p->natloc >= 0 ? p->d_tag.as<int>() : nullptr, dstats);
         size_t tmp = 0;
         AGPU_HIP(rocprim::exclusive_scan(nullptr, tmp, st->half.as<int>(), st->hpos.as<int>(), 0, (size_t)ntot,
                                          rocprim::plus<int>(), s));

@@ -587,6 +587,31 @@ def lammps_compute_list(pot, p, nl, nlocal, x, Z, seed, numneigh, neigh, f=None,
     return eng.value, f, vir, ea
 
 
+def lammps_compute_tagged(pot, p, nl, nlocal, x, Z, tag, seed=None, numneigh=None, neigh=None, f=None,
+                          eatom=False):
+    """The LAMMPS step with the atom tags (required by REBO2, whose bonds are
+    evaluated once from the local atom with tag(j) >= tag(i)); with seed,
+    numneigh, neigh over LAMMPS' own list, else over a list built here."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    Z = np.ascontiguousarray(Z, dtype=np.int32)
+    tag = np.ascontiguousarray(tag, dtype=np.int32)
+    nall = len(x)
+    if len(tag) != nall:
+        raise ValueError("tag needs one entry per local and ghost atom")
+    if seed is not None:
+        seed = np.ascontiguousarray(seed, dtype=np.int64)
+        numneigh = np.ascontiguousarray(numneigh, dtype=np.int32)
+        neigh = np.ascontiguousarray(neigh, dtype=np.int32)
+    if f is None:
+        f = np.zeros((nall, 3))
+    eng = ctypes.c_double(0.0)
+    vir = np.zeros(6)
+    ea = np.zeros(nall) if eatom else None
+    call("agpu_lammps_compute_tagged", pot._h, p._h, nl._h, int(nlocal), int(nall), _ptr(x), _ptr(Z), _ptr(tag),
+         _ptr(seed), _ptr(numneigh), _ptr(neigh), _ptr(f), ctypes.byref(eng), _ptr(vir), _ptr(ea))
+    return eng.value, f, vir, ea
+
+
 class MD:
     """NVE velocity-Verlet dynamics on the GPU around one potential (Tersoff
     family or EAM), with a Verlet shell ``skin`` on the list and the steps

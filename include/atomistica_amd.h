@@ -391,7 +391,8 @@ int agpu_md_get(agpu_md *md, double *r, double *v, double *f, double *wpot, int6
  * f[3*nall] is accumulated on local atoms and ghosts (LAMMPS' reverse
  * communication returns the ghost forces), *eng += energy, virial[6] -=
  * wpot in LAMMPS Voigt order (xx yy zz xy xz yz) as the reference does.
- * eatom (nall) is optional.  Tersoff and EAM; REBO2 and Ewald return
+ * eatom (nall) is optional.  The bond-order classes and EAM; REBO2 needs the
+ * atom tags (agpu_lammps_compute_tagged); Ewald and PME return
  * AGPU_ERR_UNSUPPORTED. */
 int agpu_lammps_compute(agpu_potential *pot, agpu_particles *p, agpu_neighbors *nl, int nlocal, int nall,
                         const double *x, const int *Z, double *f, double *eng, double *virial,
@@ -405,6 +406,17 @@ int agpu_lammps_compute(agpu_potential *pot, agpu_particles *p, agpu_neighbors *
  * are indices into x with LAMMPS' special-bond bits (the top two) masked
  * off, and no periodic images.  The list is imported into nl and kept until
  * the next call on nl. */
+/* The LAMMPS step with the atom tags (atom->tag of local atoms and ghosts),
+ * which REBO2 needs: it has no mask (bop_kernel_rebo2.f90 takes none) and
+ * evaluates each bond once, from the local atom i for tag(j) >= tag(i)
+ * (bop_kernel_rebo2.f90:1328-1329).  seed == NULL builds the list here (as
+ * agpu_lammps_compute), else imports LAMMPS' list (as
+ * agpu_lammps_compute_list); tag may be NULL for the other classes. */
+int agpu_lammps_compute_tagged(agpu_potential *pot, agpu_particles *p, agpu_neighbors *nl, int nlocal, int nall,
+                               const double *x, const int *Z, const int *tag, const int64_t *seed,
+                               const int *numneigh, const int *neigh, double *f, double *eng, double *virial,
+                               double *eatom);
+
 int agpu_lammps_compute_list(agpu_potential *pot, agpu_particles *p, agpu_neighbors *nl, int nlocal, int nall,
                              const double *x, const int *Z, const int64_t *seed, const int *numneigh,
                              const int *neigh, double *f, double *eng, double *virial, double *eatom);

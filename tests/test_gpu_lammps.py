@@ -174,3 +174,47 @@ def test_lammps_unsupported_classes():
     pos, cell, pbc, Z = W.diamond_c(2, sigma=0.02, seed=1)
     with pytest.raises(AgpuError):
         N.lammps_compute(N.Rebo2(), N.Particles(), N.Neighbors(100), len(pos), pos, Z)
+
+
+def _ach(n=400, seed=21):
+    pos, cell, pbc, Z = W.amorphous_ch(n, x_h=0.2, density=2.0, seed=seed)
+    return _wrap(pos, cell), cell, Z
+
+
+def _periodic_rebo2(pos, cell, Z):
+    p = N.from_atoms(pos, cell, np.array([True] * 3), Z)
+    nl = N.Neighbors(50)
+    pot = N.Rebo2(dihedral=True)
+    pot.bind_to(p, nl)
+    e, f, w, epa, *_ = pot.energy_and_forces(p, nl, epot_per_at=True)
+    vir = -np.array([w[0, 0], w[1, 1], w[2, 2], 0.5 * (w[0, 1] + w[1, 0]), 0.5 * (w[0, 2] + w[2, 0]),
+                     0.5 * (w[1, 2] + w[2, 1])])
+    return e, f, vir, epa
+
+
+@pytest.mark.parametrize("own_list", [False, True], ids=["list_here", "lammps_list"])
+def test_lammps_rebo2_tag_half_rule(own_list):
+    """REBO2 (torsion on) through the tagged step: each bond once from the
+    local atom with tag(j) >= tag(i), ghosts 8 A deep (neighbours of
+    neighbours of neighbours), then reverse communication of forces and
+    per-atom energies; equals the periodic evaluation."""
+    pos, cell, Z = _ach()
+    e0, f0, v0, epa0 = _periodic_rebo2(pos, cell, Z)
+    gx, gown = ghosts(pos, cell, 8.0)
+    nlocal = len(pos)
+    x = np.concatenate([pos, gx])
+    Zall = np.concatenate([Z, Z[gown]]).astype(np.int32)
+    tag = np.concatenate([np.arange(nlocal), gown]).astype(np.int32) + 1
+    kw = {}
+    if own_list:
+        kw = dict(zip(("seed", "numneigh", "neigh"), lammps_full_list(x, 2.0 + 1.0, seed=5)))
+    eng, f, vir, ea = N.lammps_compute_tagged(N.Rebo2(dihedral=True), N.Particles(), N.Neighbors(50), nlocal, x,
+                                              Zall, tag, eatom=True, **kw)
+    floc = f[:nlocal].copy()
+    np.add.at(floc, gown, f[nlocal:])
+    eloc = ea[:nlocal].copy()
+    np.add.at(eloc, gown, ea[nlocal:])
+    print("lammps rebo2 (%s): %d local, %d ghosts, dE %.3e" % ("lammps list" if own_list else "list here",
+                                                               nlocal, len(gx), eng - e0), flush=True)
+    assert abs(eng - e0) <= REL * abs(e0)
+    assert close(floc, f0) and close(vir, v0) and close(eloc, epa0)
