@@ -152,7 +152,7 @@ PROTOTYPES = {
     "agpu_random_positions": [i, vp, d, ctypes.c_uint64, i, vp],
     "agpu_lammps_compute": [vp, vp, vp, i, i, vp, vp, vp, vp, vp, vp],
     "agpu_lammps_compute_list": [vp, vp, vp, i, i, vp, vp, vp, vp, vp, vp, vp, vp, vp],
-    "agpu_lammps_compute_tagged": [vp, vp, vp, i, i, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp],
+    "agpu_lammps_compute_tagged": [vp, vp, vp, i, i, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp],
 }
 _RESTYPE = {"agpu_version": ctypes.c_char_p}
 

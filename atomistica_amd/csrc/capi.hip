@@ -785,7 +785,8 @@ int agpu_coulomb_potential(agpu_potential *pot, agpu_particles *p, agpu_neighbor
 // One LAMMPS step; the list is built here (list == false) or was imported.
 static int lammps_step(agpu_potential *pot, agpu_particles *p, agpu_neighbors *nl, int nlocal, int nall,
                        const double *x, const int *Z, const int *tag, const int64_t *seed, const int *numneigh,
-                       const int *neigh, double *f, double *eng, double *virial, double *eatom)
+                       const int *neigh, double *f, double *eng, double *virial, double *eatom,
+                       double *vatom = nullptr)
 {
     if (!pot || !p || !nl || nlocal < 0 || nall < nlocal || (nall > 0 && (!x || !Z || !f)) || !eng || !virial)
         return set_error(AGPU_ERR_ARG, "lammps_compute: bad arguments");
@@ -822,21 +823,36 @@ static int lammps_step(agpu_potential *pot, agpu_particles *p, agpu_neighbors *n
     if (!pot->bound || pot->p != p || pot->nl != nl) AGPU_CHECK(agpu_potential_bind_to(pot, p, nl));
     if (list) AGPU_CHECK(nl_import(nl, p, seed, numneigh, neigh));
     double epot = 0.0, wpot[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    // per-atom virial: the 3x3 per atom, turned into LAMMPS' Voigt vatom with
+    // the sign of SUM_VIRIAL in LAMMPS mode (macros.inc:202)
+    std::vector<double> wpa(vatom ? 9 * (size_t)nall : 0, 0.0);
+    double *wpa_p = vatom ? wpa.data() : nullptr;
     if (rebo2) {
         // half rule by tags over the local centres instead of a mask
         AGPU_CHECK(p->d_tag.reserve(sizeof(int) * (size_t)nall));
         AGPU_HIP(hipMemcpyAsync(p->d_tag.ptr, tag, sizeof(int) * (size_t)nall, hipMemcpyHostToDevice, stream()));
         p->natloc = nlocal;
         const int rc = agpu_potential_energy_and_forces(pot, p, nl, nullptr, &epot, f, wpot, nullptr, eatom,
-                                                        nullptr, nullptr, nullptr, nullptr, AGPU_HOST);
+                                                        nullptr, nullptr, wpa_p, nullptr, AGPU_HOST);
         p->natloc = -1;
         AGPU_CHECK(rc);
     } else {
         std::vector<int> mask(nall);
         for (int i = 0; i < nall; ++i) mask[i] = i < nlocal;
         AGPU_CHECK(agpu_potential_energy_and_forces(pot, p, nl, nullptr, &epot, f, wpot, mask.data(), eatom,
-                                                    nullptr, nullptr, nullptr, nullptr, AGPU_HOST));
+                                                    nullptr, nullptr, wpa_p, nullptr, AGPU_HOST));
     }
+    if (vatom)
+        for (int i = 0; i < nall; ++i) {
+            const double *w = wpa_p + 9 * (size_t)i;    // W(a, b) at a + 3 b
+            double *v = vatom + 6 * (size_t)i;
+            v[0] -= w[0];
+            v[1] -= w[4];
+            v[2] -= w[8];
+            v[3] -= w[1];
+            v[4] -= w[2];
+            v[5] -= w[5];
+        }
     *eng += epot;
     virial[0] -= wpot[0];
     virial[1] -= wpot[4];
@@ -856,9 +872,9 @@ int agpu_lammps_compute(agpu_potential *pot, agpu_particles *p, agpu_neighbors *
 int agpu_lammps_compute_tagged(agpu_potential *pot, agpu_particles *p, agpu_neighbors *nl, int nlocal, int nall,
                                const double *x, const int *Z, const int *tag, const int64_t *seed,
                                const int *numneigh, const int *neigh, double *f, double *eng, double *virial,
-                               double *eatom)
+                               double *eatom, double *vatom)
 {
-    return lammps_step(pot, p, nl, nlocal, nall, x, Z, tag, seed, numneigh, neigh, f, eng, virial, eatom);
+    return lammps_step(pot, p, nl, nlocal, nall, x, Z, tag, seed, numneigh, neigh, f, eng, virial, eatom, vatom);
 }
 
 int agpu_lammps_compute_list(agpu_potential *pot, agpu_particles *p, agpu_neighbors *nl, int nlocal, int nall,

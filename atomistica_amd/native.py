@@ -588,7 +588,7 @@ def lammps_compute_list(pot, p, nl, nlocal, x, Z, seed, numneigh, neigh, f=None,
 
 
 def lammps_compute_tagged(pot, p, nl, nlocal, x, Z, tag, seed=None, numneigh=None, neigh=None, f=None,
-                          eatom=False):
+                          eatom=False, vatom=False):
     """The LAMMPS step with the atom tags (required by REBO2, whose bonds are
     evaluated once from the local atom with tag(j) >= tag(i)); with seed,
     numneigh, neigh over LAMMPS' own list, else over a list built here."""
@@ -607,8 +607,11 @@ def lammps_compute_tagged(pot, p, nl, nlocal, x, Z, tag, seed=None, numneigh=Non
     eng = ctypes.c_double(0.0)
     vir = np.zeros(6)
     ea = np.zeros(nall) if eatom else None
+    va = np.zeros((nall, 6)) if vatom else None
     call("agpu_lammps_compute_tagged", pot._h, p._h, nl._h, int(nlocal), int(nall), _ptr(x), _ptr(Z), _ptr(tag),
-         _ptr(seed), _ptr(numneigh), _ptr(neigh), _ptr(f), ctypes.byref(eng), _ptr(vir), _ptr(ea))
+         _ptr(seed), _ptr(numneigh), _ptr(neigh), _ptr(f), ctypes.byref(eng), _ptr(vir), _ptr(ea), _ptr(va))
+    if vatom:
+        return eng.value, f, vir, ea, va
     return eng.value, f, vir, ea
 
 

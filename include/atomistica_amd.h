@@ -406,20 +406,23 @@ int agpu_lammps_compute(agpu_potential *pot, agpu_particles *p, agpu_neighbors *
  * are indices into x with LAMMPS' special-bond bits (the top two) masked
  * off, and no periodic images.  The list is imported into nl and kept until
  * the next call on nl. */
+int agpu_lammps_compute_list(agpu_potential *pot, agpu_particles *p, agpu_neighbors *nl, int nlocal, int nall,
+                             const double *x, const int *Z, const int64_t *seed, const int *numneigh,
+                             const int *neigh, double *f, double *eng, double *virial, double *eatom);
+
 /* The LAMMPS step with the atom tags (atom->tag of local atoms and ghosts),
  * which REBO2 needs: it has no mask (bop_kernel_rebo2.f90 takes none) and
  * evaluates each bond once, from the local atom i for tag(j) >= tag(i)
  * (bop_kernel_rebo2.f90:1328-1329).  seed == NULL builds the list here (as
  * agpu_lammps_compute), else imports LAMMPS' list (as
- * agpu_lammps_compute_list); tag may be NULL for the other classes. */
+ * agpu_lammps_compute_list); tag may be NULL for the other classes.  vatom
+ * (6 nall, optional) receives LAMMPS' per-atom virial: -= (W_xx, W_yy, W_zz,
+ * W_yx, W_zx, W_zy) of each atom's half of its terms, as SUM_VIRIAL in
+ * LAMMPS mode (macros.inc:202, pair_atomistica.cpp:523-524). */
 int agpu_lammps_compute_tagged(agpu_potential *pot, agpu_particles *p, agpu_neighbors *nl, int nlocal, int nall,
                                const double *x, const int *Z, const int *tag, const int64_t *seed,
                                const int *numneigh, const int *neigh, double *f, double *eng, double *virial,
-                               double *eatom);
-
-int agpu_lammps_compute_list(agpu_potential *pot, agpu_particles *p, agpu_neighbors *nl, int nlocal, int nall,
-                             const double *x, const int *Z, const int64_t *seed, const int *numneigh,
-                             const int *neigh, double *f, double *eng, double *virial, double *eatom);
+                               double *eatom, double *vatom);
 
 /* Synthetic-workload utility (no counterpart in the reference): n random
  * points in the periodic cell (cell[3k+x] = component x of lattice vector k)

@@ -186,10 +186,12 @@ def _periodic_rebo2(pos, cell, Z):
     nl = N.Neighbors(50)
     pot = N.Rebo2(dihedral=True)
     pot.bind_to(p, nl)
-    e, f, w, epa, *_ = pot.energy_and_forces(p, nl, epot_per_at=True)
+    e, f, w, epa, _, _, wpa, _ = pot.energy_and_forces(p, nl, epot_per_at=True, wpot_per_at=True)
     vir = -np.array([w[0, 0], w[1, 1], w[2, 2], 0.5 * (w[0, 1] + w[1, 0]), 0.5 * (w[0, 2] + w[2, 0]),
                      0.5 * (w[1, 2] + w[2, 1])])
-    return e, f, vir, epa
+    # LAMMPS vatom of each atom: -(W_xx, W_yy, W_zz, W_yx, W_zx, W_zy)
+    vat = -np.stack([wpa[:, 0, 0], wpa[:, 1, 1], wpa[:, 2, 2], wpa[:, 1, 0], wpa[:, 2, 0], wpa[:, 2, 1]], axis=1)
+    return e, f, vir, epa, vat
 
 
 @pytest.mark.parametrize("own_list", [False, True], ids=["list_here", "lammps_list"])
@@ -199,7 +201,7 @@ def test_lammps_rebo2_tag_half_rule(own_list):
     neighbours of neighbours), then reverse communication of forces and
     per-atom energies; equals the periodic evaluation."""
     pos, cell, Z = _ach()
-    e0, f0, v0, epa0 = _periodic_rebo2(pos, cell, Z)
+    e0, f0, v0, epa0, vat0 = _periodic_rebo2(pos, cell, Z)
     gx, gown = ghosts(pos, cell, 8.0)
     nlocal = len(pos)
     x = np.concatenate([pos, gx])
@@ -208,13 +210,39 @@ def test_lammps_rebo2_tag_half_rule(own_list):
     kw = {}
     if own_list:
         kw = dict(zip(("seed", "numneigh", "neigh"), lammps_full_list(x, 2.0 + 1.0, seed=5)))
-    eng, f, vir, ea = N.lammps_compute_tagged(N.Rebo2(dihedral=True), N.Particles(), N.Neighbors(50), nlocal, x,
-                                              Zall, tag, eatom=True, **kw)
+    eng, f, vir, ea, va = N.lammps_compute_tagged(N.Rebo2(dihedral=True), N.Particles(), N.Neighbors(50), nlocal,
+                                                  x, Zall, tag, eatom=True, vatom=True, **kw)
     floc = f[:nlocal].copy()
     np.add.at(floc, gown, f[nlocal:])
     eloc = ea[:nlocal].copy()
     np.add.at(eloc, gown, ea[nlocal:])
     print("lammps rebo2 (%s): %d local, %d ghosts, dE %.3e" % ("lammps list" if own_list else "list here",
                                                                nlocal, len(gx), eng - e0), flush=True)
+    vloc = va[:nlocal].copy()
+    np.add.at(vloc, gown, va[nlocal:])
     assert abs(eng - e0) <= REL * abs(e0)
-    assert close(floc, f0) and close(vir, v0) and close(eloc, epa0)
+    assert close(floc, f0) and close(vir, v0) and close(eloc, epa0) and close(vloc, vat0)
+
+
+def test_lammps_tagged_tersoff_vatom():
+    """The tagged step for a masked class (Tersoff SiC): per-atom virials
+    after reverse communication equal the periodic per-atom virials."""
+    pos, cell, pbc, Z = W.diamond_si(3, a=4.36, sigma=0.06, seed=11)
+    Z = np.where(np.arange(len(pos)) % 8 >= 4, 14, 6).astype(np.int32)
+    pos = _wrap(pos, cell)
+    p0 = N.from_atoms(pos, cell, np.array([True] * 3), Z)
+    nl0 = N.Neighbors(100)
+    pot0 = N.Tersoff()
+    pot0.bind_to(p0, nl0)
+    e0, f0, w0, _, _, _, wpa0, _ = pot0.energy_and_forces(p0, nl0, wpot_per_at=True)
+    vat0 = -np.stack([wpa0[:, 0, 0], wpa0[:, 1, 1], wpa0[:, 2, 2], wpa0[:, 1, 0], wpa0[:, 2, 0], wpa0[:, 2, 1]], axis=1)
+    gx, gown = ghosts(pos, cell, 2 * 3.0 + 0.5)
+    nlocal = len(pos)
+    x = np.concatenate([pos, gx])
+    Zall = np.concatenate([Z, Z[gown]]).astype(np.int32)
+    tag = np.concatenate([np.arange(nlocal), gown]).astype(np.int32) + 1
+    eng, f, vir, ea, va = N.lammps_compute_tagged(N.Tersoff(), N.Particles(), N.Neighbors(200), nlocal, x, Zall,
+                                                  tag, vatom=True)
+    vloc = va[:nlocal].copy()
+    np.add.at(vloc, gown, va[nlocal:])
+    assert abs(eng - e0) <= REL * abs(e0) and close(vloc, vat0)
