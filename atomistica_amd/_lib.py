@@ -36,6 +36,7 @@ KIND_TERSOFF_SCR = 7
 KIND_KUMAGAI_SCR = 8
 KIND_BRENNER_SCR = 9
 KIND_PME = 10
+KIND_JUSLIN = 11
 REBO2_ZERO_TABLES = 1
 REBO2_DIHEDRAL = 2
 
@@ -81,6 +82,13 @@ class BrennerParams(ctypes.Structure):
         [(k, ctypes.c_double * 6) for k in ("r1", "r2")]
 
 
+class JuslinParams(ctypes.Structure):
+    _fields_ = [("nel", ctypes.c_int), ("Z", ctypes.c_int * 3)] + \
+        [(k, ctypes.c_double * 9) for k in ("D0", "r0", "S", "beta", "gamma", "c", "d", "h", "n")] + \
+        [("alpha", ctypes.c_double * 27), ("omega", ctypes.c_double * 27), ("m", ctypes.c_int * 27)] + \
+        [(k, ctypes.c_double * 9) for k in ("r1", "r2")]
+
+
 class ScreeningParams(ctypes.Structure):
     _fields_ = [(k, ctypes.c_double * 6) for k in ("or1", "or2", "bor1", "bor2", "Cmin", "Cmax")]
 
@@ -121,6 +129,7 @@ PROTOTYPES = {
     "agpu_tersoff_set_params": [vp, ctypes.POINTER(TersoffParams)],
     "agpu_kumagai_create": [ctypes.POINTER(vp), ctypes.POINTER(KumagaiParams)],
     "agpu_brenner_create": [ctypes.POINTER(vp), ctypes.POINTER(BrennerParams)],
+    "agpu_juslin_create": [ctypes.POINTER(vp), ctypes.POINTER(JuslinParams)],
     "agpu_tersoff_scr_create": [ctypes.POINTER(vp), vp, vp],
     "agpu_kumagai_scr_create": [ctypes.POINTER(vp), vp, vp],
     "agpu_brenner_scr_create": [ctypes.POINTER(vp), vp, vp],

@@ -263,3 +263,4 @@ TersoffScr = _make(native.TersoffScr, "TersoffScr")
 KumagaiScr = _make(native.KumagaiScr, "KumagaiScr")
 BrennerScr = _make(native.BrennerScr, "BrennerScr")
 PME = _make(native.PME, "PME")
+Juslin = _make(native.Juslin, "Juslin")

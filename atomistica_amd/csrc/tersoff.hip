@@ -37,9 +37,9 @@ struct TersoffDev {
     double A[6], B[6], xi[6], lambda[6], mu[6], omega[6], mubo[6];
     int m[6];
     double beta[3], n[3], c[3], d[3], h[3];
-    double r1[6], r2[6];
+    double r1[9], r2[9];          // 9: the ordered pairs of the Juslin family
     // derived
-    double c2[3], d2[3], bn[3], e[3], fac[6];
+    double c2[3], d2[3], bn[3], e[3], fac[9];
     // Kumagai family (kumagai_func.f90): g, bond order and h forms; A, B,
     // lambda (= lambda1), mu (= lambda2), h, r1, r2 are shared with the above
     int fam;                      // 0 Tersoff, 1 Kumagai, 2 Brenner
@@ -50,7 +50,12 @@ struct TersoffDev {
     // mubo, m above): V_R = VR_f exp(-expR (r - r0)), V_A = -VA_f exp(-expA
     // (r - r0)), g = gamma (1 + c^2/d^2 - c^2/(d^2 + (h + cos)^2)),
     // b = (1 + zeta^n)^(-1/2n)
-    double br0[6], bVRf[6], bVAf[6], bexpR[6], bexpA[6], bgam[6], bc2[6], bd2[6], bh[6], bnn[6], bboe[6];
+    double br0[9], bVRf[9], bVAf[9], bexpR[9], bexpA[9], bgam[9], bc2[9], bd2[9], bh[9], bnn[9], bboe[9];
+    // Juslin family (juslin_func.f90): the Brenner forms above over ordered
+    // element pairs j + i nel, and h = omega exp((alpha dr)^m) per ordered
+    // triplet k + nel (j + nel i) of centre i
+    double jal[27], jom[27];
+    int jm[27];
 };
 
 struct ScrDev;
@@ -76,6 +81,13 @@ __host__ __device__ __forceinline__ int pair_index(int i, int j, int nel)
     int a = i + j * nel, b = j + i * nel;
     int c = i * (i + 1) / 2, d = j * (j + 1) / 2;
     return (a < b ? a : b) - (c < d ? c : d);
+}
+
+// element pair of bond i-j: ordered j + i nel for the Juslin family
+// (PAIR_INDEX_NS, macros.inc:139), else the symmetric PAIR_INDEX
+__host__ __device__ __forceinline__ int bop_pair(const TersoffDev &P, int i, int j)
+{
+    return P.fam == 3 ? j + i * P.nel : pair_index(i, j, P.nel);
 }
 
 static void derive(TersoffDev &D, const agpu_tersoff_params &p)
@@ -132,7 +144,7 @@ __global__ void k_bop_bonds(int nat, const double *__restrict__ r, const int *__
             double d[3];
             delta_r(C, ri, rj, a, b, c, d);
             double rl = dot3_nc(d);
-            int t = pair_index(eli, elj, P.nel);
+            int t = bop_pair(P, eli, elj);
             if (rl < P.r2[t] * P.r2[t]) bslot[beg + nb++] = s;
         }
     }
@@ -188,8 +200,28 @@ template <> __device__ __forceinline__ void rsincos<float>(float x, float *s, fl
 // (kumagai_func.f90:179-221, including its beta = 3 branch, which leaves
 // alpha out of the exponent as the reference does)
 template <typename R, int FAM>
-__device__ __forceinline__ void h_fn(const TersoffDev &P, int ikpot, R dr, R &val, R &dval)
+__device__ __forceinline__ void h_fn(const TersoffDev &P, int ijpot, int ikpot, R dr, R &val, R &dval)
 {
+    if (FAM == 3) {
+        // juslin_func.f90:272-298: centre i = ikpot / nel, j = ijpot % nel,
+        // k = ikpot % nel
+        const int t = ikpot % P.nel + P.nel * (ijpot % P.nel + P.nel * (ikpot / P.nel));
+        const R alpha = (R)P.jal[t], omega = (R)P.jom[t];
+        const int m = P.jm[t];
+        if (m == 1) {
+            val = omega * rexp<R>(alpha * dr);
+            dval = alpha * val;
+        } else if (m == 3) {
+            const R arg = alpha * dr;
+            val = omega * rexp<R>(arg * arg * arg);
+            dval = (R)3 * alpha * arg * arg * val;
+        } else {
+            const R arg = alpha * dr;
+            val = omega * rexp<R>(rpow<R>(arg, (R)m));
+            dval = (R)m * rpow<R>(arg, (R)(m - 1)) * alpha * val;
+        }
+        return;
+    }
     if (FAM == 1) {
         const R alpha = (R)P.kalpha[ikpot];
         if (alpha == (R)0) {
@@ -237,7 +269,7 @@ __device__ __forceinline__ void h_fn(const TersoffDev &P, int ikpot, R dr, R &va
 template <typename R, int FAM>
 __device__ __forceinline__ void g_fn(const TersoffDev &P, int eli, int ikpot, R costh, R &g, R &dg)
 {
-    if (FAM == 2) {
+    if (FAM >= 2) {
         // brenner_func.f90:77-101 in the cancellation-free form
         // gamma (1 + c^2 t^2 / (d^2 (d^2 + t^2))), t = h + cos, of the pair ik
         const R c2 = (R)P.bc2[ikpot], d2 = (R)P.bd2[ikpot], gam = (R)P.bgam[ikpot];
@@ -273,7 +305,7 @@ __device__ __forceinline__ void bo_fn(const TersoffDev &P, int eli, int typ, R z
 {
     bij = (R)1;
     Pz = (R)0;
-    if (FAM == 2) {
+    if (FAM >= 2) {
         // brenner_func.f90:107-147: n = 1 needs no zeta > 0 guard
         const R n = (R)P.bnn[typ], boe = (R)P.bboe[typ];
         if (n == (R)1) {
@@ -351,7 +383,7 @@ k_tersoff(TersoffDev P, int nat, const int *__restrict__ order, const double *__
             rij[x] = (r[3 * j + x] - r[3 * i + x]) - image_x(C, x, a, b, c);
         double rl2 = rij[0] * rij[0] + rij[1] * rij[1] + rij[2] * rij[2];
         double rld = sqrt(rl2);
-        typ = pair_index(eli, el[j], P.nel);
+        typ = FAM == 3 ? el[j] + eli * P.nel : pair_index(eli, el[j], P.nel);
         for (int x = 0; x < 3; ++x) rn[x] = (R)(rij[x] / rld);
         rl = (R)rld;
         ok = rld < P.r2[typ];
@@ -405,7 +437,7 @@ k_tersoff(TersoffDev P, int nat, const int *__restrict__ order, const double *__
             R g, dg;
             g_fn<R, FAM>(P, elc, tpk, costh, g, dg);
             R hv, dh;
-            h_fn<R, FAM>(P, tpk, rl - s_rl[tk], hv, dh);
+            h_fn<R, FAM>(P, typ, tpk, rl - s_rl[tk], hv, dh);
             zij += s_fc[tk] * g * hv;
         }
     }
@@ -413,8 +445,8 @@ k_tersoff(TersoffDev P, int nat, const int *__restrict__ order, const double *__
     // ---- bond order ----
     R VA = (R)0, dVA = (R)0, VR = (R)0, dVR = (R)0, bij = (R)1, Pz = (R)0;
     if (on) {
-        if (FAM == 2) {
-            // brenner_func.f90:27-71
+        if (FAM >= 2) {
+            // brenner_func.f90:27-71 (juslin_func.f90:69-99 the same)
             const R dr0 = rl - (R)P.br0[typ];
             const R ea = rexp<R>(-(R)P.bexpA[typ] * dr0);
             VA = -(R)P.bVAf[typ] * ea;
@@ -456,7 +488,7 @@ k_tersoff(TersoffDev P, int nat, const int *__restrict__ order, const double *__
             R rnk[3] = {s_rn[0][tk], s_rn[1][tk], s_rn[2][tk]};
             R rlik = s_rl[tk], fcik = s_fc[tk], dfcik = s_dfc[tk];
             R hv, dh;
-            h_fn<R, FAM>(P, tpk, rl - rlik, hv, dh);
+            h_fn<R, FAM>(P, typ, tpk, rl - rlik, hv, dh);
             R costh = rnk[0] * rn[0] + rnk[1] * rn[1] + rnk[2] * rn[2];
             R g, dg;
             g_fn<R, FAM>(P, elc, tpk, costh, g, dg);
@@ -875,7 +907,7 @@ k_bop_scr(TersoffDev P, ScrDev S, int nat, const int *__restrict__ order, const 
         dc_unpack(e.y, a, b, c);
         for (int x = 0; x < 3; ++x) rijd[x] = (r[3 * j + x] - r[3 * i + x]) - image_x(C, x, a, b, c);
         const double rld = sqrt(rijd[0] * rijd[0] + rijd[1] * rijd[1] + rijd[2] * rijd[2]);
-        typ = pair_index(eli, el[j], P.nel);
+        typ = FAM == 3 ? el[j] + eli * P.nel : pair_index(eli, el[j], P.nel);
         for (int x = 0; x < 3; ++x) rn[x] = (R)(rijd[x] / rld);
         rl = (R)rld;
         const double4 cv = fcv[s];
@@ -917,14 +949,14 @@ k_bop_scr(TersoffDev P, ScrDev S, int nat, const int *__restrict__ order, const 
             R g, dg;
             g_fn<R, FAM>(P, elc, tpk, costh, g, dg);
             R hv, dh;
-            h_fn<R, FAM>(P, tpk, rl - s_rl[tk], hv, dh);
+            h_fn<R, FAM>(P, typ, tpk, rl - s_rl[tk], hv, dh);
             zij += s_fc[tk] * g * hv;
         }
     }
 
     R VA = (R)0, dVA = (R)0, VR = (R)0, dVR = (R)0, bij = (R)1, Pz = (R)0;
     if (on) {
-        if (FAM == 2) {
+        if (FAM >= 2) {
             const R dr0 = rl - (R)P.br0[typ];
             const R ea = rexp<R>(-(R)P.bexpA[typ] * dr0);
             VA = -(R)P.bVAf[typ] * ea;
@@ -967,7 +999,7 @@ k_bop_scr(TersoffDev P, ScrDev S, int nat, const int *__restrict__ order, const 
             R rnk[3] = {s_rn[0][tk], s_rn[1][tk], s_rn[2][tk]};
             R rlik = s_rl[tk], fcik = s_fc[tk], dfcik = s_dfc[tk];
             R hv, dh;
-            h_fn<R, FAM>(P, tpk, rl - rlik, hv, dh);
+            h_fn<R, FAM>(P, typ, tpk, rl - rlik, hv, dh);
             R costh = rnk[0] * rn[0] + rnk[1] * rn[1] + rnk[2] * rn[2];
             R g, dg;
             g_fn<R, FAM>(P, elc, tpk, costh, g, dg);
@@ -1244,7 +1276,7 @@ int tersoff_bind(agpu_potential *pot)
         for (int b = 0; b < prm.nel; ++b)
             if (present[a] && present[b]) {
                 const int q = pair_index(a, b, prm.nel);
-                double r2 = prm.r2[q];
+                double r2 = st->dev.fam == 3 ? st->dev.r2[b + a * prm.nel] : prm.r2[q];
                 if (st->scr) {
                     const double cm = st->sp.Cmax[q];
                     const double cdr = cm > 2.0 ? cm * cm / (4 * (cm - 1)) : 1.0;
@@ -1264,7 +1296,12 @@ static int launch_tersoff(TersoffState *st, int nat, const int *order, const dou
                           const CellArg &C, agpu_neighbors *nl, const int *mask, double *eslot,
                           double *wslot, int nblk, hipStream_t s)
 {
-    if (st->dev.fam == 1)
+    if (st->dev.fam == 3)
+        AGPU_LAUNCH(sizeof(R) == 8 ? "juslin_fp64" : "juslin_mixed", (k_tersoff<R, G, 3>), dim3(nblk), dim3(256),
+                    0, s, st->dev, nat, order, r, el, C, nl->offsets.as<int>(), nl->nbr.as<int2>(),
+                    st->bslot.as<int>(), st->bcnt.as<int>(), mask, st->fslot.as<double>(),
+                    st->fown.as<double>(), eslot, wslot, st->partial.as<double>());
+    else if (st->dev.fam == 1)
         AGPU_LAUNCH(sizeof(R) == 8 ? "kumagai_fp64" : "kumagai_mixed", (k_tersoff<R, G, 1>), dim3(nblk), dim3(256),
                     0, s, st->dev, nat, order, r, el, C, nl->offsets.as<int>(), nl->nbr.as<int2>(),
                     st->bslot.as<int>(), st->bcnt.as<int>(), mask, st->fslot.as<double>(),
@@ -1892,4 +1929,136 @@ extern "C" int agpu_brenner_scr_create(agpu_potential **pot, const agpu_brenner_
     }
     AGPU_CHECK(agpu_brenner_create(pot, &b));
     return attach_screening(*pot, sp, AGPU_KIND_BRENNER_SCR);
+}
+
+// -------------------------------------------------------------------------
+// Juslin (juslin.f90): ordered element pairs and per-triplet h-terms
+// -------------------------------------------------------------------------
+static agpu_juslin_params juslin_default()
+{
+    // Juslin_J_Appl_Phys_98_123520_WCH (juslin_params.f90:74-112, without
+    // SCREENING)
+    agpu_juslin_params j;
+    std::memset(&j, 0, sizeof j);
+    j.nel = 3;
+    j.Z[0] = 74;
+    j.Z[1] = 6;
+    j.Z[2] = 1;
+    const double D0[9] = {5.41861, 6.64, 2.748, 0.0, 6.0, 3.6422, 0.0, 3.642, 4.7509};
+    const double r0[9] = {2.34095, 1.90547, 1.727, -1.0, 1.39, 1.1199, -1.0, 1.1199, 0.74144};
+    const double S[9] = {1.92708, 2.96149, 1.2489, 0.0, 1.22, 1.69077, 0.0, 1.69077, 2.3432};
+    const double beta[9] = {1.38528, 1.80370, 1.52328, 0.0, 2.1, 1.9583, 0.0, 1.9583, 1.9436};
+    const double gam[9] = {0.00188227, 0.072855, 0.0054, 0.0, 0.00020813, 0.00020813, 0.0, 12.33, 12.33};
+    const double c[9] = {2.14969, 1.10304, 1.788, 0.0, 330.0, 330.0, 0.0, 0.0, 0.0};
+    const double d[9] = {0.17126, 0.33018, 0.8255, 0.0, 3.5, 3.5, 0.0, 1.0, 1.0};
+    const double h[9] = {-0.27780, 0.75107, 0.38912, 0.0, 1.0, 1.0, 0.0, 1.0, 1.0};
+    const double n[9] = {1.0, 1.0, 1.0, 0.0, 1.0, 1.0, 0.0, 1.0, 1.0};
+    const double r1[9] = {3.20, 2.60, 1.95, 0.0, 1.70, 1.30, 0.0, 1.30, 1.10};
+    const double r2[9] = {3.80, 3.00, 2.35, 0.0, 2.00, 1.80, 0.0, 1.80, 1.70};
+    for (int q = 0; q < 9; ++q) {
+        j.D0[q] = D0[q];
+        j.r0[q] = r0[q];
+        j.S[q] = S[q];
+        j.beta[q] = beta[q];
+        j.gamma[q] = gam[q];
+        j.c[q] = c[q];
+        j.d[q] = d[q];
+        j.h[q] = h[q];
+        j.n[q] = n[q];
+        j.r1[q] = r1[q];
+        j.r2[q] = r2[q];
+    }
+    for (int t = 0; t < 27; ++t) {
+        j.alpha[t] = 0.0;
+        j.omega[t] = 1.0;
+        j.m[t] = 1;
+    }
+    j.alpha[0] = j.alpha[3] = j.alpha[6] = 0.45876;
+    const int a4[] = {14, 16, 17, 22, 23, 25, 26};
+    for (int t : a4) j.alpha[t] = 4.0;
+    j.omega[16] = 2.94586;
+    j.omega[17] = 4.54415;
+    j.omega[22] = 0.33946;
+    j.omega[23] = 0.22006;
+    return j;
+}
+
+extern "C" int agpu_juslin_create(agpu_potential **pot, const agpu_juslin_params *params)
+{
+    if (!pot) return set_error(AGPU_ERR_ARG, "pot is NULL");
+    agpu_juslin_params j = params ? *params : juslin_default();
+    const int nel = j.nel;
+    if (nel < 1 || nel > 3) return set_error(AGPU_ERR_ARG, "Juslin: 1 to 3 elements supported");
+    // entries with r0 < 0 take the reversed pair (juslin_module.f90:273-305)
+    for (int a = 0; a < nel; ++a)
+        for (int b = 0; b < nel; ++b) {
+            const int q = b + a * nel, t = a + b * nel;
+            if (j.r0[q] < 0.0) {
+                j.D0[q] = j.D0[t];
+                j.r0[q] = j.r0[t];
+                j.S[q] = j.S[t];
+                j.beta[q] = j.beta[t];
+                j.gamma[q] = j.gamma[t];
+                j.c[q] = j.c[t];
+                j.d[q] = j.d[t];
+                j.h[q] = j.h[t];
+                j.n[q] = j.n[t];
+                j.r1[q] = j.r1[t];
+                j.r2[q] = j.r2[t];
+            }
+        }
+    for (int q = 0; q < nel * nel; ++q) {
+        if (j.d[q] == 0.0)
+            return set_error(AGPU_ERR_ARG, "d = 0! This leads to problems computing c**2/d**2. Please specify d != 0.");
+        if (j.S[q] <= 1.0)
+            return set_error(AGPU_ERR_ARG, "S <= 1! This leads to problems computing (S-1)**(-1). Please specify S > 1.");
+        if (!(j.r2[q] > j.r1[q])) return set_error(AGPU_ERR_ARG, "Juslin: need r1 < r2 for every pair");
+    }
+    TersoffState *st = new TersoffState();
+    agpu_tersoff_params &p = st->prm;
+    std::memset(&p, 0, sizeof p);
+    p.nel = nel;
+    for (int e = 0; e < 3; ++e) p.Z[e] = j.Z[e];
+    for (int q = 0; q < 6; ++q) {
+        p.A[q] = p.B[q] = p.lambda[q] = p.mu[q] = p.xi[q] = p.omega[q] = 1.0;
+        p.m[q] = 1;
+        p.r1[q] = 1.0;
+        p.r2[q] = 2.0;
+    }
+    for (int q = 0; q < 3; ++q) p.beta[q] = p.n[q] = p.c[q] = p.d[q] = p.h[q] = 1.0;
+    derive(st->dev, p);
+    TersoffDev &D = st->dev;
+    D.fam = 3;
+    for (int q = 0; q < 9; ++q) {
+        const bool in = q < nel * nel;
+        const double S = in ? j.S[q] : 2.0, nn = in ? j.n[q] : 1.0, beta = in ? j.beta[q] : 1.0;
+        D.r1[q] = in ? j.r1[q] : 1.0;
+        D.r2[q] = in ? j.r2[q] : 2.0;
+        D.fac[q] = M_PI / (D.r2[q] - D.r1[q]);
+        D.br0[q] = in ? j.r0[q] : 0.0;
+        D.bVRf[q] = in ? j.D0[q] / (S - 1.0) : 0.0;
+        D.bVAf[q] = in ? S * j.D0[q] / (S - 1.0) : 0.0;
+        D.bexpR[q] = beta * std::sqrt(2.0 * S);
+        D.bexpA[q] = beta * std::sqrt(2.0 / S);
+        D.bgam[q] = in ? j.gamma[q] : 0.0;
+        D.bc2[q] = in ? j.c[q] * j.c[q] : 0.0;
+        D.bd2[q] = in ? j.d[q] * j.d[q] : 1.0;
+        D.bh[q] = in ? j.h[q] : 0.0;
+        D.bnn[q] = nn;
+        D.bboe[q] = -0.5 / nn;
+    }
+    for (int t = 0; t < 27; ++t) {
+        const bool in = t < nel * nel * nel;
+        D.jal[t] = in ? j.alpha[t] : 0.0;
+        D.jom[t] = in ? j.omega[t] : 1.0;
+        D.jm[t] = in ? j.m[t] : 1;
+    }
+    agpu_potential *P = new agpu_potential();
+    P->kind = AGPU_KIND_JUSLIN;
+    for (int q = 0; q < nel * nel; ++q)
+        if (D.r2[q] > P->cutoff) P->cutoff = D.r2[q];
+    P->impl = st;
+    P->free_impl = tersoff_free;
+    *pot = P;
+    return AGPU_OK;
 }

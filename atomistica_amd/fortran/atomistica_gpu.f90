@@ -57,6 +57,17 @@ module atomistica_gpu
      real(c_double) :: r1(6), r2(6)
   end type brenner_params_t
 
+  !> Juslin parameters (BOP_DB_TYPE of juslin_params.f90:26-72): ordered
+  !> element pairs j + i nel, ordered triplets k + nel (j + nel i)
+  type, bind(C), public :: juslin_params_t
+     integer(c_int) :: nel
+     integer(c_int) :: Z(3)
+     real(c_double) :: D0(9), r0(9), S(9), beta(9), gamma(9), c(9), d(9), h(9), n(9)
+     real(c_double) :: alpha(27), omega(27)
+     integer(c_int) :: m(27)
+     real(c_double) :: r1(9), r2(9)
+  end type juslin_params_t
+
   !> Screening parameters of the screened variants (tersoff_params.f90:82-89):
   !> outer and bond-order cutoffs and Cmin, Cmax per element pair
   type, bind(C), public :: screening_params_t
@@ -66,7 +77,7 @@ module atomistica_gpu
   public :: particles_init, particles_set_cell, particles_set_Z, particles_set_positions
   public :: neighbors_init, tersoff_init, eam_alloy_init, ewald_init, rebo2_init
   public :: kumagai_init, brenner_init
-  public :: tersoff_scr_init, kumagai_scr_init, brenner_scr_init, pme_init
+  public :: tersoff_scr_init, kumagai_scr_init, brenner_scr_init, pme_init, juslin_init
   public :: set_precision, bind_to, energy_and_forces, coulomb_energy_and_forces
   public :: last_error, del
 
@@ -133,6 +144,11 @@ module atomistica_gpu
        type(c_ptr), value :: params
      end function
      integer(c_int) function agpu_brenner_create(pot, params) bind(C, name="agpu_brenner_create")
+       import :: c_int, c_ptr
+       type(c_ptr) :: pot
+       type(c_ptr), value :: params
+     end function
+     integer(c_int) function agpu_juslin_create(pot, params) bind(C, name="agpu_juslin_create")
        import :: c_int, c_ptr
        type(c_ptr) :: pot
        type(c_ptr), value :: params
@@ -296,6 +312,19 @@ contains
        ierror = agpu_brenner_create(this%h, c_null_ptr)
     end if
   end subroutine brenner_init
+
+  !> juslin_init (juslin_module.f90:27-150): db absent selects
+  !> Juslin_J_Appl_Phys_98_123520_WCH
+  subroutine juslin_init(this, ierror, db)
+    type(potential_gpu_t), intent(inout) :: this
+    integer, intent(out) :: ierror
+    type(juslin_params_t), optional, target, intent(in) :: db
+    if (present(db)) then
+       ierror = agpu_juslin_create(this%h, c_loc(db))
+    else
+       ierror = agpu_juslin_create(this%h, c_null_ptr)
+    end if
+  end subroutine juslin_init
 
   !> tersoff_scr_init (tersoff_scr.f90 with tersoff_module.f90:30-83): db and
   !> scr absent select the screened Tersoff_PRB_39_5566_SiC; both or neither

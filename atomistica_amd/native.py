@@ -352,6 +352,56 @@ class Brenner(_Potential):
         self._h = h
 
 
+class Juslin(_Potential):
+    """Juslin (juslin.f90, classname Juslin): the bond-order kernel with the
+    Brenner forms over ordered element pairs and a per-triplet h-term.
+    Keyword arguments are the names of juslin_params.f90 (el, D0, r0, S,
+    beta, gamma, c, d, h, n over nel^2 ordered pairs j + i nel; alpha,
+    omega, m over nel^3 ordered triplets; r1, r2); without arguments
+    Juslin_J_Appl_Phys_98_123520_WCH."""
+    kind = _lib.KIND_JUSLIN
+
+    def __init__(self, **kwargs):
+        super().__init__()
+        h = ctypes.c_void_p()
+        if kwargs:
+            from .parameters import Juslin_JAP_98_123520_WCH
+            params = dict(Juslin_JAP_98_123520_WCH)
+            params.update(kwargs)
+            self._params = params
+            s = _juslin_struct(params)
+            call("agpu_juslin_create", ctypes.byref(h), ctypes.byref(s))
+        else:
+            self._params = None
+            call("agpu_juslin_create", ctypes.byref(h), None)
+        self._h = h
+
+
+def _juslin_struct(params):
+    from .parameters import ATOMIC_NUMBERS
+    nel = len(params["el"])
+    npairs, ntrip = nel * nel, nel ** 3
+    s = _lib.JuslinParams()
+    s.nel = nel
+    for i, e in enumerate(params["el"]):
+        s.Z[i] = ATOMIC_NUMBERS[e]
+    for k in ("D0", "r0", "S", "beta", "gamma", "c", "d", "h", "n", "r1", "r2"):
+        vals = params[k]
+        if len(vals) < npairs:
+            raise ValueError("Juslin: %s needs one value per ordered element pair (%d)" % (k, npairs))
+        arr = getattr(s, k)
+        for i in range(npairs):
+            arr[i] = float(vals[i])
+    for k in ("alpha", "omega", "m"):
+        vals = params.get(k, [1] * ntrip if k == "m" else None)
+        if vals is None or len(vals) < ntrip:
+            raise ValueError("Juslin: %s needs one value per ordered element triplet (%d)" % (k, ntrip))
+        arr = getattr(s, k)
+        for t in range(ntrip):
+            arr[t] = int(vals[t]) if k == "m" else float(vals[t])
+    return s
+
+
 def _kumagai_struct(params):
     from .parameters import ATOMIC_NUMBERS
     s = _lib.KumagaiParams()

@@ -163,6 +163,57 @@ Brenner_PRB_42_9458_C_II = {
 
 # The eleven scalar Tersoff parameters trained by the parameter-gradient path,
 # in the order used throughout (theta vector of a single-element set).
+# Juslin-form (W-C-H, Fe-C-H) sets: every pair parameter over ordered element
+# pairs j + i nel (PAIR_INDEX_NS), alpha, omega, m over ordered triplets
+# k + nel (j + nel i) of centre i (TRIPLET_INDEX_NS); an r0 < 0 entry takes
+# the reversed pair's values (reference: src/python/atomistica/parameters.py:
+# 279-295, 312-328).
+Juslin_JAP_98_123520_WCH = {
+    "__ref__": "Juslin N., Erhart P., Traskelin P., Nord J., Henriksson K.O.E, Nordlund K., Salonen E., "
+               "Albe K., J. Appl. Phys. 98, 123520 (2005)",
+    "family": "juslin",
+    "el": ["W", "C", "H"],
+    "D0": [5.41861, 6.64, 2.748, 0.0, 6.0, 3.6422, 0.0, 3.642, 4.7509],
+    "r0": [2.34095, 1.90547, 1.727, -1.0, 1.39, 1.1199, -1.0, 1.1199, 0.74144],
+    "S": [1.92708, 2.96149, 1.2489, 0.0, 1.22, 1.69077, 0.0, 1.69077, 2.3432],
+    "beta": [1.38528, 1.80370, 1.52328, 0.0, 2.1, 1.9583, 0.0, 1.9583, 1.9436],
+    "gamma": [0.00188227, 0.072855, 0.0054, 0.0, 0.00020813, 0.00020813, 0.0, 12.33, 12.33],
+    "c": [2.14969, 1.10304, 1.788, 0.0, 330.0, 330.0, 0.0, 0.0, 0.0],
+    "d": [0.17126, 0.33018, 0.8255, 0.0, 3.5, 3.5, 0.0, 1.0, 1.0],
+    "h": [-0.27780, 0.75107, 0.38912, 0.0, 1.0, 1.0, 0.0, 1.0, 1.0],
+    "n": [1.0, 1.0, 1.0, 0.0, 1.0, 1.0, 0.0, 1.0, 1.0],
+    "alpha": [0.45876, 0.0, 0.0, 0.45876, 0.0, 0.0, 0.45876, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 4.0, 0.0, 4.0,
+              4.0, 0.0, 0.0, 0.0, 0.0, 4.0, 4.0, 0.0, 4.0, 4.0],
+    "omega": [1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 2.94586, 4.54415,
+              1.0, 1.0, 1.0, 1.0, 0.33946, 0.22006, 1.0, 1.0, 1.0],
+    "m": [1] * 27,
+    "r1": [3.20, 2.60, 2.68, 0.0, 1.70, 1.30, 0.0, 1.30, 1.10],
+    "r2": [3.80, 3.00, 2.96, 0.0, 2.00, 1.80, 0.0, 1.80, 1.70],
+}
+
+Kuopanportti_CMS_111_525_FeCH = {
+    "__ref__": "Kuopanportti P., Hayward, N., Fu C., Kuronen A., Nordlund K., Comp. Mat. Sci. 111, 525 (2016)",
+    "family": "juslin",
+    "el": ["Fe", "C", "H"],
+    "D0": [1.5, 4.82645134, 1.630, 0.0, 6.0, 3.6422, 0.0, 3.642, 4.7509],
+    "r0": [2.29, 1.47736510, 1.589, -1.0, 1.39, 1.1199, -1.0, 1.1199, 0.74144],
+    "S": [2.0693, 1.43134755, 4.000, 0.0, 1.22, 1.69077, 0.0, 1.69077, 2.3432],
+    "beta": [1.4, 1.63208170, 1.875, 0.0, 2.1, 1.9583, 0.0, 1.9583, 1.9436],
+    "gamma": [0.01158, 0.00205862, 0.01332, 0.0, 0.00020813, 0.00020813, 0.0, 12.33, 12.33],
+    "c": [1.2899, 8.95583221, 424.5, 0.0, 330.0, 330.0, 0.0, 0.0, 0.0],
+    "d": [0.3413, 0.72062047, 7.282, 0.0, 3.5, 3.5, 0.0, 1.0, 1.0],
+    "h": [-0.26, 0.87099874, -0.1091, 0.0, 1.0, 1.0, 0.0, 1.0, 1.0],
+    "n": [1.0, 1.0, 1.0, 0.0, 1.0, 1.0, 0.0, 1.0, 1.0],
+    "alpha": [0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 4.0, 0.0, 4.0, 4.0, 0.0, 0.0,
+              0.0, 0.0, 4.0, 4.0, 0.0, 4.0, 4.0],
+    "omega": [1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 2.94586, 4.54415,
+              1.0, 1.0, 1.0, 1.0, 0.33946, 0.22006, 1.0, 1.0, 1.0],
+    "m": [1] * 27,
+    "r1": [2.95, 2.30, 2.2974, 0.0, 1.70, 1.30, 0.0, 1.30, 1.10],
+    "r2": [3.35, 2.70, 2.6966, 0.0, 2.00, 1.80, 0.0, 1.80, 1.70],
+}
+
+
 # ---------------------------------------------------------------------------
 # Screened variants (TersoffScr, KumagaiScr, BrennerScr): inner cutoff
 # [r1, r2], pair cutoff [or1, or2], bond-order cutoff [bor1, bor2], all of the
@@ -215,9 +266,11 @@ Erhart_PRB_71_035211_SiC__Scr.update({
 
 TERSOFF_THETA_NAMES = ("A", "B", "lambda", "mu", "beta", "n", "c", "d", "h", "r1", "r2")
 
-ATOMIC_NUMBERS = {"H": 1, "He": 2, "Li": 3, "Be": 4, "B": 5, "C": 6, "N": 7, "O": 8,
-                  "F": 9, "Ne": 10, "Na": 11, "Mg": 12, "Al": 13, "Si": 14, "P": 15,
-                  "S": 16, "Cl": 17, "Ar": 18, "K": 19, "Ca": 20, "Ti": 22, "Fe": 26,
-                  "Co": 27, "Ni": 28, "Cu": 29, "Zn": 30, "Ga": 31, "Ge": 32, "As": 33,
-                  "Ag": 47, "Sn": 50, "Pt": 78, "Au": 79, "Pb": 82}
+_SYMBOLS = ("H He Li Be B C N O F Ne Na Mg Al Si P S Cl Ar K Ca "
+            "Sc Ti V Cr Mn Fe Co Ni Cu Zn Ga Ge As Se Br Kr Rb Sr Y Zr "
+            "Nb Mo Tc Ru Rh Pd Ag Cd In Sn Sb Te I Xe Cs Ba La Ce Pr Nd "
+            "Pm Sm Eu Gd Tb Dy Ho Er Tm Yb Lu Hf Ta W Re Os Ir Pt Au Hg "
+            "Tl Pb Bi Po At Rn Fr Ra Ac Th Pa U Np Pu Am Cm Bk Cf Es Fm "
+            "Md No Lr").split()
+ATOMIC_NUMBERS = {sym: z + 1 for z, sym in enumerate(_SYMBOLS)}
 ELEMENT_NAMES = {v: k for k, v in ATOMIC_NUMBERS.items()}

@@ -67,6 +67,7 @@ extern "C" {
 #define AGPU_KIND_KUMAGAI_SCR 8   /* kumagai_scr.f90, classname KumagaiScr */
 #define AGPU_KIND_BRENNER_SCR 9   /* brenner_scr.f90, classname BrennerScr */
 #define AGPU_KIND_PME 10          /* pme.f90, classname PME */
+#define AGPU_KIND_JUSLIN 11       /* juslin.f90, classname Juslin */
 
 /* agpu_rebo2_create flags */
 #define AGPU_REBO2_ZERO_TABLES 1  /* rebo2 zero_tables = .true. (rebo2_module.f90:30-42) */
@@ -216,6 +217,23 @@ typedef struct agpu_brenner_params {
     double r1[6], r2[6];
 } agpu_brenner_params;
 int agpu_brenner_create(agpu_potential **pot, const agpu_brenner_params *params);
+
+/* Juslin (juslin.f90, juslin_func.f90): the Brenner forms of V_R, V_A, g and
+ * b over ORDERED element pairs q = j + i nel (PAIR_INDEX_NS,
+ * macros.inc:139; 9 entries for 3 elements), and the h-term
+ * omega exp((alpha dr)^m) per ordered triplet t = k + nel (j + nel i) of the
+ * centre i (TRIPLET_INDEX_NS, macros.inc:146).  A pair with r0 < 0 takes the
+ * reversed pair's parameters (juslin_module.f90:273-305).  NULL selects
+ * Juslin_J_Appl_Phys_98_123520_WCH (juslin_params.f90:74-112). */
+typedef struct {
+    int nel;
+    int Z[3];
+    double D0[9], r0[9], S[9], beta[9], gamma[9], c[9], d[9], h[9], n[9];
+    double alpha[27], omega[27];
+    int m[27];
+    double r1[9], r2[9];
+} agpu_juslin_params;
+int agpu_juslin_create(agpu_potential **pot, const agpu_juslin_params *params);
 
 /* Screened variants: the same bond-order kernels with SCREENING defined
  * (bop_kernel.f90:563-1066 loop1 screening, :1448-1492 and :1531-1609 its

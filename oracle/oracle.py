@@ -133,13 +133,14 @@ class _TersoffParams(ctypes.Structure):
         [(k, ctypes.c_double * 6) for k in ("A", "B", "xi", "lambda_", "mu", "omega", "mubo")] + \
         [("m", ctypes.c_int * 6)] + \
         [(k, ctypes.c_double * 3) for k in ("beta", "n", "c", "d", "h")] + \
-        [(k, ctypes.c_double * 6) for k in ("r1", "r2")] + \
+        [(k, ctypes.c_double * 9) for k in ("r1", "r2")] + \
         [("fam", ctypes.c_int)] + \
         [(k, ctypes.c_double * 3) for k in ("kc1", "kc2", "kc3", "kc4", "kc5", "keta", "kdelta")] + \
         [("kalpha", ctypes.c_double * 6), ("kbeta", ctypes.c_int * 6)] + \
-        [(k, ctypes.c_double * 6) for k in ("bD0", "br0", "bS", "bbeta", "bgam", "bc", "bd", "bh", "bn")] + \
+        [(k, ctypes.c_double * 9) for k in ("bD0", "br0", "bS", "bbeta", "bgam", "bc", "bd", "bh", "bn")] + \
         [("scr", ctypes.c_int)] + \
-        [(k, ctypes.c_double * 6) for k in ("or1", "or2", "bor1", "bor2", "Cmin", "Cmax")]
+        [(k, ctypes.c_double * 6) for k in ("or1", "or2", "bor1", "bor2", "Cmin", "Cmax")] + \
+        [("jal", ctypes.c_double * 27), ("jom", ctypes.c_double * 27), ("jm", ctypes.c_int * 27)]
 
 
 _SCR_KEYS = ("or1", "or2", "bor1", "bor2", "Cmin", "Cmax")
@@ -232,6 +233,47 @@ def _brenner_struct(b):
     return s
 
 
+def juslin_symmetrized(j):
+    """Juslin parameters over ordered pairs with the entries of r0 < 0 taken
+    from the reversed pair, as BIND_TO_FUNC does (juslin_module.f90:273-305)."""
+    nel = len(j["el"])
+    out = {k: list(v) if isinstance(v, (list, tuple)) else v for k, v in j.items()}
+    keys = ("D0", "r0", "S", "beta", "gamma", "c", "d", "h", "n", "r1", "r2")
+    for a in range(nel):
+        for b in range(nel):
+            if out["r0"][b + a * nel] < 0.0:
+                for k in keys:
+                    out[k][b + a * nel] = out[k][a + b * nel]
+    return out
+
+
+def _juslin_struct(j):
+    """Juslin parameters (juslin_params.f90 names) in the shared bond-order
+    block with fam = 3: ordered pairs j + i nel, triplets k + nel (j + nel i)."""
+    j = juslin_symmetrized(j)
+    s = _TersoffParams()
+    nel = len(j["el"])
+    s.nel = nel
+    for i in range(9):
+        inside = i < nel * nel
+        s.A[i % 6] = s.B[i % 6] = s.lambda_[i % 6] = s.mu[i % 6] = s.xi[i % 6] = s.omega[i % 6] = 1.0
+        s.r1[i] = float(j["r1"][i]) if inside else 1.0
+        s.r2[i] = float(j["r2"][i]) if inside else 2.0
+        for dst, src, dflt in (("bD0", "D0", 0.0), ("br0", "r0", 0.0), ("bS", "S", 2.0), ("bbeta", "beta", 1.0),
+                               ("bgam", "gamma", 0.0), ("bc", "c", 0.0), ("bd", "d", 1.0), ("bh", "h", 0.0),
+                               ("bn", "n", 1.0)):
+            getattr(s, dst)[i] = float(j[src][i]) if inside else dflt
+    for t in range(27):
+        inside = t < nel ** 3
+        s.jal[t] = float(j["alpha"][t]) if inside else 0.0
+        s.jom[t] = float(j["omega"][t]) if inside else 1.0
+        s.jm[t] = int(j["m"][t]) if inside and "m" in j else 1
+    for i in range(3):
+        s.beta[i] = s.n[i] = s.c[i] = s.d[i] = s.h[i] = 1.0
+    s.fam = 3
+    return s
+
+
 def tersoff(params, r, el, cell, nl, mask=None, per_at=False, nthreads=1):
     """Energy, forces and virial; ``el`` holds database element indices (-1 = none).
     ``params`` is a Tersoff parameter dict, or a Kumagai or Brenner one
@@ -248,7 +290,8 @@ def tersoff(params, r, el, cell, nl, mask=None, per_at=False, nthreads=1):
     dc = _c(dc, np.int32)
     fam = params.get("family")
     s = (_kumagai_struct(params) if fam == "kumagai" else
-         _brenner_struct(params) if fam == "brenner" else _tersoff_struct(params, [0, 0, 0]))
+         _brenner_struct(params) if fam == "brenner" else
+         _juslin_struct(params) if fam == "juslin" else _tersoff_struct(params, [0, 0, 0]))
     _screening(s, params)
     epot = ctypes.c_double(0.0)
     f = np.zeros((nat, 3))

@@ -49,7 +49,7 @@ typedef struct {
     double A[6], B[6], xi[6], lambda[6], mu[6], omega[6], mubo[6];
     int m[6];
     double beta[3], n[3], c[3], d[3], h[3];
-    double r1[6], r2[6];
+    double r1[9], r2[9];
     /* fam = 1: the Kumagai forms (kumagai_func.f90) with A, B, lambda
      * (= lambda1), mu (= lambda2), h, r1, r2 from above */
     int fam;
@@ -58,12 +58,18 @@ typedef struct {
     int kbeta[6];
     /* fam = 2: the Brenner forms (brenner_func.f90), every parameter per
      * element pair; the h-term takes mu, m from mubo, m above */
-    double bD0[6], br0[6], bS[6], bbeta[6], bgam[6], bc[6], bd[6], bh[6], bn[6];
+    double bD0[9], br0[9], bS[9], bbeta[9], bgam[9], bc[9], bd[9], bh[9], bn[9];
     /* scr = 1: the screened variants (TersoffScr, KumagaiScr, BrennerScr):
      * exp_cutoff on [r1, r2] (inner), [or1, or2] (pair), [bor1, bor2] (bond
      * order) and the screening of bonds by third atoms with Cmin, Cmax */
     int scr;
     double or1[6], or2[6], bor1[6], bor2[6], Cmin[6], Cmax[6];
+    /* fam = 3: the Juslin forms (juslin_func.f90): the Brenner V_A, V_R, g,
+     * b above over ordered element pairs j + i nel (PAIR_INDEX_NS, 9 pairs,
+     * r1 r2 too), and h = omega exp((alpha dr)^m) per ordered triplet
+     * k + nel (j + nel i) (TRIPLET_INDEX_NS) of centre i */
+    double jal[27], jom[27];
+    int jm[27];
 } orc_tersoff_params;
 
 int orc_tersoff(const orc_tersoff_params *p, int nat, const double *r,

@@ -16,6 +16,10 @@
  *   - fam = 2: VA, VR, g, bo of src/potentials/bop/brenner/brenner_func.f90:27-147
  *     with the constants of BIND_TO_FUNC (brenner_module.f90:267-289); its h
  *     (:153-196) is the Tersoff form
+ *   - fam = 3: VA, VR, g, bo, h of src/potentials/bop/juslin/juslin_func.f90:69-313
+ *     with the constants of BIND_TO_FUNC (juslin_module.f90:263-330):
+ *     ordered pair index PAIR_INDEX_NS, triplet index TRIPLET_INDEX_NS
+ *     (src/macros.inc:139-146)
  *   - PAIR_INDEX          src/macros.inc:123
  *   - element map / nebmax src/potentials/bop/default_compute_func.f90:62-73
  *   - scr = 1 (TersoffScr, KumagaiScr, BrennerScr): loop1's screening function
@@ -223,8 +227,34 @@ static void trig_off(double r1, double r2, double r, double *val, double *dval)
     }
 }
 
-static void h_fn(const orc_tersoff_params *p, int ikpot, double dr, double *val, double *dval)
+/* element pair of the family: ordered (Juslin) or symmetric */
+static int bop_pair(const orc_tersoff_params *p, int i, int j)
 {
+    return p->fam == 3 ? j + i * p->nel : pair_index(i, j, p->nel);
+}
+
+static void h_fn(const orc_tersoff_params *p, int ikpot, double dr, double *val, double *dval, int eli, int elj,
+                 int elk)
+{
+    if (p->fam == 3) {
+        /* juslin_func.f90:272-298, h(ktypj, ktypi, ktypk) of centre i */
+        const int t = elk + p->nel * (elj + p->nel * eli);
+        const double alpha = p->jal[t], omega = p->jom[t];
+        const int m = p->jm[t];
+        if (m == 1) {
+            *val = omega * exp(alpha * dr);
+            *dval = alpha * *val;
+        } else if (m == 3) {
+            const double arg = alpha * dr;
+            *val = omega * exp(arg * arg * arg);
+            *dval = 3 * alpha * arg * arg * *val;
+        } else {
+            const double arg = alpha * dr;
+            *val = omega * exp(pow(arg, m));
+            *dval = m * pow(arg, m - 1) * alpha * *val;
+        }
+        return;
+    }
     if (p->fam == 1) {
         /* kumagai_func.f90:179-221 (the beta = 3 branch has no alpha in the
          * exponent, as written there) */
@@ -274,7 +304,7 @@ int orc_tersoff(const orc_tersoff_params *p, int nat, const double *r,
                 double rij[3];
                 entry_vec(r, cell, dc, i, j, jn, rij);
                 double rl = rij[0] * rij[0] + rij[1] * rij[1] + rij[2] * rij[2];
-                int t = pair_index(el[i], el[j], nel);
+                int t = bop_pair(p, el[i], el[j]);
                 if (p->scr) {
                     bond_t *b = &bonds[nb];
                     if (screened_bond(p, i, jn, rij, rl, t, r, cell, offsets, nbr, dc, b, &rec, &nrec,
@@ -353,7 +383,7 @@ int orc_tersoff(const orc_tersoff_params *p, int nat, const double *r,
                 double rij[3] = {rlij * rnij[0], rlij * rnij[1], rlij * rnij[2]};
                 double fcarij = bnd_ij->fc, dfcarijr = bnd_ij->dfc;
                 double VAij, dVAij, VRij, dVRij;
-                if (p->fam == 2) {
+                if (p->fam >= 2) {
                     double S = p->bS[el2ij];
                     double expR = p->bbeta[el2ij] * sqrt(2 * S), expA = p->bbeta[el2ij] * sqrt(2 / S);
                     double VR_f = p->bD0[el2ij] / (S - 1), VA_f = S * p->bD0[el2ij] / (S - 1);
@@ -390,10 +420,10 @@ int orc_tersoff(const orc_tersoff_params *p, int nat, const double *r,
                     double rik[3] = {rlik * rnik[0], rlik * rnik[1], rlik * rnik[2]};
                     double fcik = bik->fcbo, dfcikr = bik->dfcbo;
                     double h_Dr, dh_dDr;
-                    h_fn(p, ikpot, rlij - rlik, &h_Dr, &dh_dDr);
+                    h_fn(p, ikpot, rlij - rlik, &h_Dr, &dh_dDr, eli, el[j], el[bik->j]);
                     double costh = rnik[0] * rnij[0] + rnik[1] * rnij[1] + rnik[2] * rnij[2];
                     double g_costh, dg_dcosth;
-                    if (p->fam == 2) {
+                    if (p->fam >= 2) {
                         /* brenner_func.f90:77-101, parameters of the pair ik */
                         double c_sq = p->bc[ikpot] * p->bc[ikpot], d_sq = p->bd[ikpot] * p->bd[ikpot];
                         double hh = d_sq + (p->bh[ikpot] + costh) * (p->bh[ikpot] + costh);
@@ -449,7 +479,7 @@ int orc_tersoff(const orc_tersoff_params *p, int nat, const double *r,
                 int numnbi = ikc;
                 /* bo: b_ij and dE/dzeta */
                 double bij = 1.0, dbij_dzij = 0.0;
-                if (p->fam == 2) {
+                if (p->fam >= 2) {
                     /* brenner_func.f90:107-147 */
                     double n = p->bn[el2ij];
                     double bo_exp = -0.5 / n, bo_fac = 0.5 * bo_exp * n, bo_exp1 = bo_exp - 1.0;
