@@ -631,11 +631,12 @@ k_bop_gather(int nat, const int *__restrict__ offsets, const int *__restrict__ b
 // range gets the screening function of the bond from the third atoms k of
 // i's row (loop1, bop_kernel.f90:563-1066), in the reference's row order and
 // with its early exit:
-//   k_scr_screen<false>  keep flag and number of screening records per entry
+//   k_scr_screen<false>  keep flag, pair and bond-order cutoffs (value, d/dr)
+//                        and the number of screening records per entry
 //   (exclusive scan of the record counts)
-//   k_scr_screen<true>   pair and bond-order cutoffs (value, d/dr) per entry
-//                        and the records (k's entry, d fc_ar / d r_ik,
-//                        d fc_ar / d r_jk, and the same for fc_bo)
+//   k_scr_screen<true>   the records (k's entry, d fc_ar / d r_ik,
+//                        d fc_ar / d r_jk, and the same for fc_bo), only in
+//                        rows that have any
 //   k_scr_bonds          kept entries of every row -> the bond list
 //   k_bop_scr            the bond-order kernel of k_tersoff with these
 //                        cutoffs, plus the screening forces: for the pair
@@ -703,6 +704,8 @@ k_scr_screen(ScrDev S, int nat_tot, const double *__restrict__ r, const int *__r
         if (lane == 0) atomicOr(flags, DERR_SCR_ROW);
         return;
     }
+    // pass 2 only writes records: rows without any have nothing to do
+    if (FILL && roff[beg + n] == roff[beg]) return;
     const int eli = el[i];
     for (int t = lane; t < n; t += 64) {
         const int2 e = nbr[beg + t];
@@ -819,9 +822,8 @@ k_scr_screen(ScrDev S, int nat_tot, const double *__restrict__ r, const int *__r
                 dfcbo = dfcar;
             }
         }
-        if (FILL) {
+        if (!FILL) {
             if (kept) fcv[s] = make_double4(fcar, dfcar, fcbo, dfcbo);
-        } else {
             keep[s] = kept;
             rcnt[s] = kept ? cnt : 0;
         }
@@ -1477,10 +1479,12 @@ static int tersoff_scr_compute(agpu_potential *pot, const agpu_eval_args *a)
     const int *offs = nl->offsets.as<int>();
     const int2 *nbr = nl->nbr.as<int2>();
     const dim3 gw((unsigned)((nat + SCR_WAVES - 1) / SCR_WAVES));
-    // pass 1: keep flags and record counts; exclusive scan to record offsets
+    // pass 1: keep flags, cutoffs and record counts; exclusive scan to record
+    // offsets
     AGPU_CHECK(zero_dev(st->rcnt.as<int>() + E, sizeof(int)));
     AGPU_LAUNCH("scr_count", k_scr_screen<false>, gw, dim3(256), 0, s, S, nat, p->d_r, el, st->prm.nel, C, offs,
-                nbr, st->keep.as<int>(), st->rcnt.as<int>(), nullptr, nullptr, nullptr, nullptr, dstats);
+                nbr, st->keep.as<int>(), st->rcnt.as<int>(), nullptr, st->fcv.as<double4>(), nullptr, nullptr,
+                dstats);
     size_t tmp = 0;
     AGPU_HIP(rocprim::exclusive_scan(nullptr, tmp, st->rcnt.as<int>(), st->roff.as<int>(), 0, (size_t)E + 1,
                                      rocprim::plus<int>(), s));
@@ -1496,7 +1500,7 @@ static int tersoff_scr_compute(agpu_potential *pot, const agpu_eval_args *a)
     AGPU_CHECK(st->rkn.reserve(sizeof(int) * (nrec + 1)));
     AGPU_CHECK(st->rco.reserve(sizeof(double4) * (nrec + 1)));
     AGPU_CHECK(st->frec.reserve(sizeof(double) * 3 * (nrec + 1)));
-    // pass 2: cutoffs and records
+    // pass 2: the records (rows with none return at once)
     AGPU_LAUNCH("scr_fill", k_scr_screen<true>, gw, dim3(256), 0, s, S, nat, p->d_r, el, st->prm.nel, C, offs, nbr,
                 st->keep.as<int>(), st->rcnt.as<int>(), st->roff.as<int>(), st->fcv.as<double4>(),
                 st->rkn.as<int>(), st->rco.as<double4>(), dstats);
