@@ -726,7 +726,7 @@ int eam_compute(agpu_potential *pot, const agpu_eval_args *a)
         AGPU_CHECK(eam_dispatch<double>(st, pot, a, st->d_rec64.as<double>(), &nblk));
     AGPU_CHECK(reduce_ev(st->partial.as<double>(), nblk, 1, a->epot, nullptr, nullptr));
     AGPU_CHECK(reduce_ev(st->partial.as<double>() + nblk, nblk, 10, a->epot, a->wpot, nullptr));
-    if (capturing()) return AGPU_OK;     // the driver reads the flags after the replay
+    if (capturing()) return AGPU_OK;     // the MD driver reads the flags after the replay
     AGPU_HIP(hipMemcpyAsync(pot->stats.h, st->stats.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
     AGPU_HIP(hipStreamSynchronize(s));
     if (pot->stats.h[0] & DERR_SPLINE_RANGE)

@@ -8,8 +8,8 @@
 // gather with the 1/2 of the directed double count -- no atomics.
 //
 // Reciprocal space (Ewald class): direct structure-factor sum over the half space of
-// k = 2 pi Bbox^T n with |k| <= kmax (the reference uses smooth PME instead,
-// pme_kernel.f90:251-752).  S(k) is accumulated in FP64:
+// k = 2 pi Bbox^T n with |k| <= kmax (the reference's PME class uses smooth PME,
+// pme_kernel.f90:251-752, built in pme.hip).  S(k) is accumulated in FP64:
 //   k_ewald_sk     thread = run of up to L consecutive n3 in one (n1, n2)
 //                  column; ions staged through LDS in chunks; the phase of
 //                  the first k comes from sincospi, the rest of the run by

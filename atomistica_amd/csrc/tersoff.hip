@@ -1605,7 +1605,7 @@ int tersoff_compute(agpu_potential *pot, const agpu_eval_args *a)
     AGPU_CHECK(launch_gather(G, nat, nl, st, eslot, wslot, a, s));
     // per-block records [e, W(9)] -> epot, wpot
     AGPU_CHECK(reduce_ev(st->partial.as<double>(), nblk, 10, a->epot, a->wpot, nullptr));
-    if (capturing()) return AGPU_OK;     // the driver reads the flags after the replay
+    if (capturing()) return AGPU_OK;     // the MD driver reads the flags after the replay
     AGPU_HIP(hipMemcpyAsync(pot->stats.h, dstats, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
     AGPU_HIP(hipStreamSynchronize(s));
     if (pot->stats.h[1] & DERR_BOND_OVERFLOW)
