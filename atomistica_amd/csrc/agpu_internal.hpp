@@ -124,6 +124,10 @@ struct agpu_neighbors {
     // pair search kernel: 0 chosen from the bin occupancy, 1 thread per
     // atom, 2 wavefront per bin block (both give the same rows)
     int pair_mode = 0;
+    // the CSR rows (offsets, nbr) hold the last build; false while its rows
+    // are only staged (sorted slot p at stage + p*stage_w, counts[atom]
+    // entries) because every caller since the build read them there
+    bool csr = true;
     agpu::DevBuf cell_cnt;
     agpu::HostStats stats;
     int64_t n_updates = 0;
@@ -131,7 +135,10 @@ struct agpu_neighbors {
 
 namespace agpu {
 
-int nl_update(agpu_neighbors *nl, agpu_particles *p, int force);
+// Rebuilds the list when positions, cell or atoms changed (force > 0:
+// always).  csr = false: the caller reads the rows through NlRows only, so
+// the copy of staged rows into the CSR rows is left until a caller asks.
+int nl_update(agpu_neighbors *nl, agpu_particles *p, int force, bool csr = true);
 // graph-capture mode (no host synchronisation, no memset nodes)
 bool capturing();
 void set_capturing(bool on);
@@ -144,6 +151,28 @@ int compute_inverse(const double *A, double *B);
 struct CellArg {
     double A[9];
 };
+
+// The rows of the current list for kernels that visit atoms in bin order
+// (atom i = idx_sorted[ai]): staged rows when S > 0 (row of slot ai at
+// rows + ai*S, len[i] entries), else CSR rows (row i at rows + len[i],
+// len[i + 1] - len[i] entries).  Entry order is the same in both.
+struct NlRows {
+    const int2 *rows;
+    const int *len;
+    int S;
+};
+NlRows nl_rows(const agpu_neighbors *nl);
+
+__device__ __forceinline__ const int2 *nl_row(const NlRows &R, int ai, int i, int &n)
+{
+    if (R.S > 0) {
+        n = R.len[i];
+        return R.rows + (size_t)ai * R.S;
+    }
+    const int b = R.len[i];
+    n = R.len[i + 1] - b;
+    return R.rows + b;
+}
 
 __device__ __forceinline__ int dc_pack_ok(int a) { return a >= -512 && a <= 511; }
 __device__ __forceinline__ int dc_pack(int a, int b, int c)

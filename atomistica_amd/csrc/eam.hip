@@ -368,8 +368,7 @@ template <typename R, int G, int U, bool ONE, bool KN>
 __global__ void __launch_bounds__(256)
 k_eam_density(EamDev P, const SplHdr *__restrict__ hdr, const R *__restrict__ rec, int nat,
               const int *__restrict__ order, const double4 *__restrict__ q1, CellArg C,
-              const int *__restrict__ offsets, const int2 *__restrict__ nbr, const int *__restrict__ mask,
-              double *__restrict__ dF, double4 *__restrict__ q2, double *__restrict__ epa,
+              NlRows L, const int *__restrict__ mask, double *__restrict__ dF, double4 *__restrict__ q2, double *__restrict__ epa,
               double *__restrict__ partial, int *__restrict__ flags)
 {
     constexpr int APB = 256 / G;
@@ -386,13 +385,14 @@ k_eam_density(EamDev P, const SplHdr *__restrict__ hdr, const R *__restrict__ re
         double rho = 0.0;
         if (on) {
             const double ri[3] = {qi.x, qi.y, qi.z};
-            const int beg = offsets[i], end = offsets[i + 1];
-            for (int s0 = beg + lane; s0 < end; s0 += U * G) {
+            int n;
+            const int2 *row = nl_row(L, ai, i, n);
+            for (int s0 = lane; s0 < n; s0 += U * G) {
                 int2 e[U];
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
                     const int s = s0 + u * G;
-                    e[u] = s < end ? nbr[s] : make_int2(-1, 0);
+                    e[u] = s < n ? row[s] : make_int2(-1, 0);
                 }
                 double4 qj[U];
 #pragma unroll
@@ -443,8 +443,7 @@ template <typename R, int G, int U, bool ONE, bool GEN, bool KN>
 __global__ void __launch_bounds__(256)
 k_eam_force(EamDev P, const SplHdr *__restrict__ hdr, const R *__restrict__ rec, int nat,
             const int *__restrict__ order, const double4 *__restrict__ q1, const double4 *__restrict__ q2,
-            CellArg C, const int *__restrict__ offsets, const int2 *__restrict__ nbr,
-            const int *__restrict__ mask, double *__restrict__ f, double *__restrict__ epa,
+            CellArg C, NlRows L, const int *__restrict__ mask, double *__restrict__ f, double *__restrict__ epa,
             double *__restrict__ wpa, double *__restrict__ partial, int *__restrict__ flags)
 {
     constexpr int APB = 256 / G;
@@ -464,14 +463,15 @@ k_eam_force(EamDev P, const SplHdr *__restrict__ hdr, const R *__restrict__ rec,
             const double4 qi = q2[i];
             const R dFi = (R)qi.w;
             const double ri[3] = {qi.x, qi.y, qi.z};
-            const int beg = offsets[i], end = offsets[i + 1];
+            int n;
+            const int2 *row = nl_row(L, ai, i, n);
             const SplHdr hri = hdr_of<ONE>(P, hdr, 1, eli, 0);
-            for (int s0 = beg + lane; s0 < end; s0 += U * G) {
+            for (int s0 = lane; s0 < n; s0 += U * G) {
                 int2 en[U];
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
                     const int s = s0 + u * G;
-                    en[u] = s < end ? nbr[s] : make_int2(-1, 0);
+                    en[u] = s < n ? row[s] : make_int2(-1, 0);
                 }
                 double4 qj[U];
                 int elj[U];
@@ -621,14 +621,13 @@ static int eam_passes(EamState *st, agpu_potential *pot, const agpu_eval_args *a
     const bool mixed = sizeof(R) == 4;
     const int nat = p->nat;
     double4 *q1 = st->pk.as<double4>(), *q2 = q1 + nat;
+    const NlRows L = nl_rows(nl);
     AGPU_LAUNCH(mixed ? "eam_density_mixed" : "eam_density_fp64", (k_eam_density<R, G, U, ONE, KN>), dim3(nblk),
                 dim3(256), 0, s, st->dev, st->d_hdr.as<SplHdr>(), rec, nat, order, q1, C,
-                nl->offsets.as<int>(), nl->nbr.as<int2>(), a->mask, st->dF.as<double>(), q2, a->epot_per_at, part,
-                flags);
+                L, a->mask, st->dF.as<double>(), q2, a->epot_per_at, part, flags);
     AGPU_LAUNCH(mixed ? "eam_force_mixed" : "eam_force_fp64", (k_eam_force<R, G, U, ONE, GEN, KN>), dim3(nblk),
                 dim3(256), 0, s, st->dev, st->d_hdr.as<SplHdr>(), rec, nat, order, q1, q2, C,
-                nl->offsets.as<int>(), nl->nbr.as<int2>(), a->mask, a->f, a->epot_per_at, a->wpot_per_at,
-                part + nblk, flags);
+                L, a->mask, a->f, a->epot_per_at, a->wpot_per_at, part + nblk, flags);
     return AGPU_OK;
 }
 
@@ -704,7 +703,7 @@ int eam_compute(agpu_potential *pot, const agpu_eval_args *a)
     EamState *st = static_cast<EamState *>(pot->impl);
     agpu_particles *p = pot->p;
     if (st->Z_rev != p->Z_rev) AGPU_CHECK(eam_bind(pot));
-    AGPU_CHECK(nl_update(pot->nl, p, 0));
+    AGPU_CHECK(nl_update(pot->nl, p, 0, false));
     const int nat = p->nat;
     hipStream_t s = stream();
     AGPU_CHECK(zero_dev(a->epot, sizeof(double)));

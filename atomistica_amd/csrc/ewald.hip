@@ -78,8 +78,7 @@ template <typename T> __device__ __forceinline__ T wsum(T v)
 template <typename R>
 __global__ void __launch_bounds__(256)
 k_ewald_real(int nat, const int *__restrict__ order, const double *__restrict__ r,
-             const double *__restrict__ q, CellArg C, const int *__restrict__ offsets,
-             const int2 *__restrict__ nbr, double cutoff_sq, double beta, double sqrt_alpha_pi,
+             const double *__restrict__ q, CellArg C, NlRows L, double cutoff_sq, double beta, double sqrt_alpha_pi,
              double *__restrict__ f, double *__restrict__ phi, double *__restrict__ partial)
 {
     __shared__ double s_red[4][10];
@@ -94,10 +93,11 @@ k_ewald_real(int nat, const int *__restrict__ order, const double *__restrict__ 
         qi = q[i];
         if (fabs(qi) > 1e-10) {
             const double ri[3] = {r[3 * i], r[3 * i + 1], r[3 * i + 2]};
-            const int beg = offsets[i], end = offsets[i + 1];
+            int n;
+            const int2 *row = nl_row(L, ai, i, n);
             const R b = (R)beta, sap = (R)sqrt_alpha_pi, al = (R)(beta * beta);
-            for (int s = beg + lane; s < end; s += 64) {
-                int2 en = nbr[s];
+            for (int s = lane; s < n; s += 64) {
+                int2 en = row[s];
                 const int j = en.x;
                 const double qj = q[j];
                 if (!(fabs(qj) > 1e-10)) continue;
@@ -449,7 +449,7 @@ static int ewald_run(agpu_potential *pot, const double *q, double *epot, double 
     EwaldState *st = static_cast<EwaldState *>(pot->impl);
     agpu_particles *p = pot->p;
     agpu_neighbors *nl = pot->nl;
-    AGPU_CHECK(nl_update(nl, p, 0));
+    AGPU_CHECK(nl_update(nl, p, 0, false));
     if (!st->pme && (!st->have_k || std::memcmp(st->cell_key, p->cell.A, sizeof st->cell_key) != 0))
         AGPU_CHECK(build_kvectors(*st, p));
     const int nat = p->nat;
@@ -467,12 +467,10 @@ static int ewald_run(agpu_potential *pot, const double *q, double *epot, double 
     const int *order = nl->idx_sorted.as<int>();
     if (pot->precision == AGPU_PREC_MIXED)
         AGPU_LAUNCH("ewald_real_mixed", k_ewald_real<float>, dim3(nblk_r), dim3(256), 0, s, nat, order, p->d_r, q,
-                    C, nl->offsets.as<int>(), nl->nbr.as<int2>(), st->cutoff * st->cutoff, st->beta, sap, f,
-                    phi, part);
+                    C, nl_rows(nl), st->cutoff * st->cutoff, st->beta, sap, f, phi, part);
     else
         AGPU_LAUNCH("ewald_real_fp64", k_ewald_real<double>, dim3(nblk_r), dim3(256), 0, s, nat, order, p->d_r, q,
-                    C, nl->offsets.as<int>(), nl->nbr.as<int2>(), st->cutoff * st->cutoff, st->beta, sap, f,
-                    phi, part);
+                    C, nl_rows(nl), st->cutoff * st->cutoff, st->beta, sap, f, phi, part);
     if (st->pme) {
         // reciprocal record after the real-space block records
         AGPU_CHECK(pme_recip(st->pme, p, q, st->beta, part + 10 * (size_t)nblk_r, f, phi));

@@ -135,7 +135,8 @@ static int md_rebuild(agpu_md *md)
 {
     hipStream_t s = stream();
     md->nl->held = false;
-    AGPU_CHECK(nl_update(md->nl, md->p, 1));
+    // EAM reads the staged rows; the bond-order kernels need the CSR rows
+    AGPU_CHECK(nl_update(md->nl, md->p, 1, md->pot->kind != AGPU_KIND_EAM_ALLOY));
     md->nl->held = true;
     AGPU_HIP(hipMemcpyAsync(md->rref.ptr, md->p->d_r, sizeof(double) * 3 * md->nat, hipMemcpyDeviceToDevice, s));
     AGPU_CHECK(zero_dev(md->dmax.ptr, sizeof(unsigned) * 3));   // displacement, its copy, speed

@@ -513,12 +513,28 @@ k_atom_pairs(int nat, BinGeom g, CellArg C, double cutoff_sq, const SortedAtom *
     }
 }
 
+// The reference's capacity rule (python_neighbors.f90:713-718): row i ends at
+// 1-based slot offsets[i] + cnt + i (one separator per earlier atom), and a
+// non-empty row that ends at or beyond nat*avgn overflows the list.
+__global__ void __launch_bounds__(256)
+k_nl_capacity(int nat, const int *__restrict__ counts, const int *__restrict__ offsets, long long cap,
+              int *__restrict__ flags)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool over = false;
+    if (i < nat) {
+        const int cnt = counts[i];
+        over = cnt > 0 && (long long)offsets[i] + cnt + i >= cap;
+    }
+    if (__ballot(over) && (threadIdx.x & 63) == 0) atomicOr(flags, DERR_NL_OVERFLOW);
+}
+
 // Copy staged rows (sorted slot p, stride S) into the CSR rows of the caller
-// atoms; applies the reference's capacity rule (python_neighbors.f90:713-718).
+// atoms (nothing when the build overflowed).
 __global__ void __launch_bounds__(256)
 k_nl_compact(int nat, const SortedAtom *__restrict__ atoms, const int *__restrict__ counts,
              const int *__restrict__ offsets, const int2 *__restrict__ stage, int S, int2 *__restrict__ nbr,
-             long long cap, int *__restrict__ flags)
+             const int *__restrict__ flags)
 {
     constexpr int G = 16;
     const int lane = threadIdx.x % G;
@@ -527,14 +543,28 @@ k_nl_compact(int nat, const SortedAtom *__restrict__ atoms, const int *__restric
     if (*flags & (DERR_NL_OVERFLOW | DERR_STAGE_OVERFLOW)) return;
     const int i = atoms[p].j;
     const int cnt = counts[i];
-    const long long base = offsets[i];
-    // the last 1-based slot written, counting one separator per earlier atom
-    if (cnt > 0 && base + cnt + (long long)i >= cap) {
-        if (lane == 0) atomicOr(flags, DERR_NL_OVERFLOW);
-        return;
-    }
     if (cnt > S) return;
-    for (int o = lane; o < cnt; o += G) nbr[base + o] = stage[(size_t)p * S + o];
+    const int2 *src = stage + (size_t)p * S;
+    int2 *dst = nbr + offsets[i];
+    for (int o = lane; o < cnt; o += G) dst[o] = src[o];
+}
+
+// the CSR rows of the current build, copied from the staged rows if needed
+static int nl_compact(agpu_neighbors *nl)
+{
+    if (nl->csr) return AGPU_OK;
+    if (nl->nat > 0)
+        AGPU_LAUNCH("nl_compact", k_nl_compact, dim3((nl->nat + 15) / 16), dim3(256), 0, stream(), nl->nat,
+                    nl->atoms_sorted.as<SortedAtom>(), nl->counts.as<int>(), nl->offsets.as<int>(),
+                    nl->stage.as<int2>(), nl->stage_w, nl->nbr.as<int2>(), nl->rowmax.as<int>() + 3);
+    nl->csr = true;
+    return AGPU_OK;
+}
+
+NlRows nl_rows(const agpu_neighbors *nl)
+{
+    if (nl->csr) return NlRows{nl->nbr.as<int2>(), nl->offsets.as<int>(), 0};
+    return NlRows{nl->stage.as<int2>(), nl->counts.as<int>(), nl->stage_w};
 }
 
 __global__ void k_offsets_tail(int nat, const int *__restrict__ counts, int *__restrict__ offsets,
@@ -605,7 +635,7 @@ static void geometry(const agpu_particles *p, agpu_neighbors *nl, BinGeom &g)
     }
 }
 
-int nl_update(agpu_neighbors *nl, agpu_particles *p, int force)
+int nl_update(agpu_neighbors *nl, agpu_particles *p, int force, bool csr)
 {
     if (nl->external) {
         // an imported list stays as the caller gave it
@@ -613,12 +643,15 @@ int nl_update(agpu_neighbors *nl, agpu_particles *p, int force)
             return set_error(AGPU_ERR_STATE, "neighbour list: imported for other particles");
         return AGPU_OK;
     }
-    if (nl->held && force == 0 && nl->p == p && nl->nat == p->nat) return AGPU_OK;
+    if (nl->held && force == 0 && nl->p == p && nl->nat == p->nat) return csr ? nl_compact(nl) : AGPU_OK;
     if (!p->cell_set) return set_error(AGPU_ERR_STATE, "particles: cell not set");
     if (p->nat > 0 && !p->d_r) return set_error(AGPU_ERR_STATE, "particles: positions not set");
     if (!force && nl->p == p && nl->pos_rev == p->pos_rev && nl->cell_rev == p->cell_rev &&
         nl->nat == p->nat)
-        return AGPU_OK;
+        return csr ? nl_compact(nl) : AGPU_OK;
+    // invalid until this build completes
+    nl->p = nullptr;
+    nl->csr = true;
     // verlet shell 0 (aseinterface.py:255) unless a driver set one
     nl->cutoff = nl->interaction_range + nl->skin;
     if (nl->cutoff <= 0.0)
@@ -742,9 +775,12 @@ int nl_update(agpu_neighbors *nl, agpu_particles *p, int force)
         AGPU_CHECK(int_max(nl->counts.as<int>(), nat, flags));
         AGPU_LAUNCH("nl_offsets_tail", k_offsets_tail, dim3(1), dim3(64), 0, s, nat,
                     nl->counts.as<int>(), nl->offsets.as<int>(), flags);
-        AGPU_LAUNCH("nl_compact", k_nl_compact, dim3((nat + 15) / 16), dim3(TB), 0, s, nat,
-                    nl->atoms_sorted.as<SortedAtom>(), nl->counts.as<int>(), nl->offsets.as<int>(),
-                    nl->stage.as<int2>(), S, nl->nbr.as<int2>(), (long long)nl->cap, flags + 3);
+        AGPU_LAUNCH("nl_capacity", k_nl_capacity, dim3(nb), dim3(TB), 0, s, nat, nl->counts.as<int>(),
+                    nl->offsets.as<int>(), (long long)nl->cap, flags + 3);
+        if (csr)
+            AGPU_LAUNCH("nl_compact", k_nl_compact, dim3((nat + 15) / 16), dim3(TB), 0, s, nat,
+                        nl->atoms_sorted.as<SortedAtom>(), nl->counts.as<int>(), nl->offsets.as<int>(),
+                        nl->stage.as<int2>(), S, nl->nbr.as<int2>(), flags + 3);
     } else {
         AGPU_HIP(hipMemsetAsync(nl->offsets.ptr, 0, sizeof(int), s));
     }
@@ -757,16 +793,17 @@ int nl_update(agpu_neighbors *nl, agpu_particles *p, int force)
         // binning with the radix sort from now on
         nl->radix = true;
         nl->p = nullptr;
-        return nl_update(nl, p, force > 0 ? force : 1);
+        return nl_update(nl, p, force > 0 ? force : 1, csr);
     }
     if (fl & DERR_STAGE_OVERFLOW) {
         // a row outgrew the staging stride: widen it and rebuild once
         nl->stage_w = ((int)(1.25 * nl->maxrow) + 16 + 7) & ~7;
         nl->p = nullptr;
         if (force > 1) return set_error(AGPU_ERR_NL_OVERFLOW, "neighbour list: staging stride overflow");
-        return nl_update(nl, p, 2);
+        return nl_update(nl, p, 2, csr);
     }
     nl->total = (int64_t)nl->stats.h[1] | ((int64_t)nl->stats.h[2] << 31);
+    nl->csr = csr || nat == 0;
     nl->p = p;
     nl->nat = nat;
     nl->pos_rev = p->pos_rev;
@@ -833,6 +870,7 @@ int nl_import(agpu_neighbors *nl, agpu_particles *p, const int64_t *seed, const 
     if (nat) AGPU_LAUNCH("nl_iota", k_iota, dim3((nat + 255) / 256), dim3(256), 0, s, nat, nl->idx_sorted.as<int>());
     AGPU_HIP(hipStreamSynchronize(s));
     nl->external = true;
+    nl->csr = true;
     nl->p = p;
     nl->nat = nat;
     nl->pos_rev = p->pos_rev;
@@ -864,6 +902,7 @@ __global__ void k_export(int nat, const int *__restrict__ offsets, const int2 *_
 
 int nl_export(agpu_neighbors *nl, int64_t *offsets, int *nbr, int *dc, int on_device)
 {
+    AGPU_CHECK(nl_compact(nl));
     hipStream_t s = stream();
     const int nat = nl->nat;
     const int64_t n = nl->total;

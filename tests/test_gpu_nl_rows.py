@@ -1,0 +1,103 @@
+"""Staged and CSR neighbour rows.
+
+EAM and the Ewald real-space sum read the list in bin order straight from the
+staging rows of the build; the copy into CSR rows (offsets + entries in atom
+order, the layout python_neighbors.f90:690-760 fills) runs only when a caller
+needs it: an export, or a bond-order potential on the same list.  Both layouts
+hold the same entries in the same order, so every result here is compared
+bit for bit, and the exported list against the oracle's.
+"""
+
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from atomistica_amd import native as N
+from atomistica_amd import workloads as W
+from atomistica_amd._lib import AgpuError
+from helpers import GOLDEN, read_setfl_tables
+
+pytestmark = pytest.mark.gpu
+
+MISHIN = os.path.join(GOLDEN, "Cu_mishin1.eam.alloy")
+
+
+def _eam(p, nl, prebuild):
+    pot = N.TabulatedAlloyEAM(fn=MISHIN)
+    pot.bind_to(p, nl)
+    if prebuild:
+        nl.update(p)          # CSR rows first: the potential reads those
+    return pot.energy_and_forces(p, nl, epot_per_at=True, wpot_per_at=True)
+
+
+def test_eam_staged_rows_equal_csr_rows_and_export():
+    pos, cell, pbc, Z = W.fcc_cu(6, sigma=0.08, seed=4)
+    print("eam staged vs csr", len(pos), flush=True)
+    p = N.from_atoms(pos, cell, pbc, Z)
+    nl = N.Neighbors(100)
+    staged = _eam(p, nl, prebuild=False)
+    p2 = N.from_atoms(pos, cell, pbc, Z)
+    csr = _eam(p2, N.Neighbors(100), prebuild=True)
+    for a, b in zip(staged, csr):
+        if a is not None:
+            assert np.array_equal(np.asarray(a), np.asarray(b))
+    # the export after a staged-only build copies the rows on demand
+    off, j, dc = nl.get_list(p)
+    ro, rj, rdc = O.nl_build(pos, cell, pbc, read_setfl_tables(MISHIN)["cutoff"])
+    assert np.array_equal(off, ro) and np.array_equal(j, rj) and np.array_equal(dc, rdc)
+    # moved atoms: a second staged build, then the export again
+    p.coordinates[7] += 2.5 * cell[2]
+    p.coordinates[11] += np.array([0.3, -0.2, 0.1])
+    p.I_changed_positions()
+    e2 = _eam(p, nl, prebuild=False)
+    assert e2[0] != staged[0]
+    off, j, dc = nl.get_list(p)
+    ro, rj, rdc = O.nl_build(p.coordinates, cell, pbc, read_setfl_tables(MISHIN)["cutoff"])
+    assert np.array_equal(off, ro) and np.array_equal(j, rj) and np.array_equal(dc, rdc)
+
+
+def test_bond_order_after_ewald_on_one_list():
+    # Ewald leaves the rows staged; Tersoff on the same list asks for CSR rows
+    pos, cell, pbc, Z = W.diamond_si(4, sigma=0.05, seed=3)
+    q = np.where(np.arange(len(pos)) % 2 == 0, 0.4, -0.4)
+    rc = 6.0
+    beta = W.ewald_beta(rc, 1e-8)
+    kmax = W.ewald_kmax(beta, 1e-8)
+    print("ewald + tersoff shared list", len(pos), flush=True)
+    p = N.from_atoms(pos, cell, pbc, Z)
+    nl = N.Neighbors(200)
+    ew = N.Ewald(cutoff=rc, beta=beta, kmax=kmax)
+    ters = N.Tersoff()
+    ew.bind_to(p, nl)
+    ters.bind_to(p, nl)
+    e_ew = ew.energy_and_forces(p, nl, q)
+    e_t = ters.energy_and_forces(p, nl)
+    # references on lists of their own, built with CSR rows first
+    pr = N.from_atoms(pos, cell, pbc, Z)
+    nr = N.Neighbors(200)
+    nr.request_interaction_range(rc)
+    ew_r = N.Ewald(cutoff=rc, beta=beta, kmax=kmax)
+    ew_r.bind_to(pr, nr)
+    nr.update(pr)
+    r_ew = ew_r.energy_and_forces(pr, nr, q)
+    t_r = N.Tersoff()
+    t_r.bind_to(pr, nr)
+    r_t = t_r.energy_and_forces(pr, nr)
+    for a, b in zip(e_ew, r_ew):
+        assert np.array_equal(np.asarray(a), np.asarray(b))
+    for a, b in zip(e_t[:3], r_t[:3]):
+        assert np.array_equal(np.asarray(a), np.asarray(b))
+
+
+def test_capacity_overflow_without_csr_copy():
+    # the reference's nat*avgn rule is applied to builds whose rows stay staged
+    pos, cell, pbc, Z = W.fcc_cu(3, sigma=0.0)
+    p = N.from_atoms(pos, cell, pbc, Z)
+    nl = N.Neighbors(8)
+    pot = N.TabulatedAlloyEAM(fn=MISHIN)
+    pot.bind_to(p, nl)
+    with pytest.raises(AgpuError) as ei:
+        pot.energy_and_forces(p, nl)
+    assert ei.value.code == 3
