@@ -129,6 +129,7 @@ struct agpu_neighbors {
     // entries) because every caller since the build read them there
     bool csr = true;
     agpu::DevBuf cell_cnt;
+    long long cnt_zero = 0;       // leading cell_cnt entries known to be zero
     agpu::HostStats stats;
     int64_t n_updates = 0;
 };
@@ -268,9 +269,11 @@ int rebo2_bind(agpu_potential *pot);
 int rebo2_compute(agpu_potential *pot, const agpu_eval_args *a);
 
 // Deterministic fixed-order reduction of nblk per-block records of width w
-// (double): column 0 -> *epot (+=), columns 1..9 -> wpot[0..8] (+=),
-// columns 10.. -> extra[0..] (+=).  Runs on the library stream.
-int reduce_ev(const double *partials, int nblk, int w, double *epot, double *wpot, double *extra);
+// (double): column 0 -> *epot (+=, or = with set_epot), columns 1..9 ->
+// wpot[0..8] (+=), columns 10.. -> extra[0..] (+=).  Runs on the library
+// stream.
+int reduce_ev(const double *partials, int nblk, int w, double *epot, double *wpot, double *extra,
+              bool set_epot = false);
 
 // *out = max(*out, max(v[0..n))) on the library stream (device pointers)
 int int_max(const int *v, int n, int *out);

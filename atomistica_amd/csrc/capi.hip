@@ -166,7 +166,7 @@ constexpr int RED_W_MAX = 16;
 __global__ void __launch_bounds__(256)
 k_reduce_rec(const double *__restrict__ part, int nblk, int w, int chunk,
              double *__restrict__ out, double *__restrict__ epot, double *__restrict__ wpot,
-             double *__restrict__ extra)
+             double *__restrict__ extra, int set_epot)
 {
     __shared__ double sh[256];
     const int lanes = 256 / w;         // records read per sweep
@@ -187,7 +187,7 @@ k_reduce_rec(const double *__restrict__ part, int nblk, int w, int chunk,
         } else {
             int col = threadIdx.x;
             if (col == 0) {
-                if (epot) *epot += t;
+                if (epot) *epot = set_epot ? t : *epot + t;
             } else if (col < 10) {
                 if (wpot) wpot[col - 1] += t;
             } else if (extra) {
@@ -200,7 +200,7 @@ k_reduce_rec(const double *__restrict__ part, int nblk, int w, int chunk,
 // process-lifetime scratch (never freed: no device frees from static destructors)
 static DevBuf *g_red_tmp = new DevBuf();
 
-int reduce_ev(const double *partials, int nblk, int w, double *epot, double *wpot, double *extra)
+int reduce_ev(const double *partials, int nblk, int w, double *epot, double *wpot, double *extra, bool set_epot)
 {
     if (nblk <= 0 || w <= 0) return AGPU_OK;
     if (w > RED_W_MAX) return set_error(AGPU_ERR_ARG, "reduce_ev: record too wide");
@@ -209,14 +209,14 @@ int reduce_ev(const double *partials, int nblk, int w, double *epot, double *wpo
     int n1 = (nblk + chunk - 1) / chunk;
     if (n1 <= 1) {
         AGPU_LAUNCH("reduce_ev", k_reduce_rec, dim3(1), dim3(256), 0, s, partials, nblk, w, nblk,
-                    (double *)nullptr, epot, wpot, extra);
+                    (double *)nullptr, epot, wpot, extra, (int)set_epot);
         return AGPU_OK;
     }
     AGPU_CHECK(g_red_tmp->reserve(sizeof(double) * (size_t)n1 * w));
     AGPU_LAUNCH("reduce_ev", k_reduce_rec, dim3(n1), dim3(256), 0, s, partials, nblk, w, chunk,
-                g_red_tmp->as<double>(), (double *)nullptr, (double *)nullptr, (double *)nullptr);
+                g_red_tmp->as<double>(), (double *)nullptr, (double *)nullptr, (double *)nullptr, 0);
     AGPU_LAUNCH("reduce_ev", k_reduce_rec, dim3(1), dim3(256), 0, s, g_red_tmp->as<double>(), n1, w,
-                n1, (double *)nullptr, epot, wpot, extra);
+                n1, (double *)nullptr, epot, wpot, extra, (int)set_epot);
     return AGPU_OK;
 }
 

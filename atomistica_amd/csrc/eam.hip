@@ -355,10 +355,12 @@ constexpr int EAM_U = AGPU_EAM_U;
 // Packed per-atom records of 32 bytes, so that one neighbour gather reads
 // one 32-byte sector: q1 = {x, y, z, element index} for the density pass,
 // q2 = {x, y, z, F'(rho)} written by the density pass for the force pass.
+// Also clears the call's four error/statistics words.
 __global__ void k_eam_pack(int nat, const double *__restrict__ r, const int *__restrict__ el,
-                           double4 *__restrict__ q1)
+                           double4 *__restrict__ q1, int *__restrict__ flags)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 4) flags[i] = 0;
     if (i >= nat) return;
     q1[i] = make_double4(r[3 * i], r[3 * i + 1], r[3 * i + 2], (double)el[i]);
 }
@@ -444,7 +446,8 @@ __global__ void __launch_bounds__(256)
 k_eam_force(EamDev P, const SplHdr *__restrict__ hdr, const R *__restrict__ rec, int nat,
             const int *__restrict__ order, const double4 *__restrict__ q1, const double4 *__restrict__ q2,
             CellArg C, NlRows L, const int *__restrict__ mask, double *__restrict__ f, double *__restrict__ epa,
-            double *__restrict__ wpa, double *__restrict__ partial, int *__restrict__ flags)
+            double *__restrict__ wpa, const double *__restrict__ dens_e, double *__restrict__ partial,
+            int *__restrict__ flags)
 {
     constexpr int APB = 256 / G;
     __shared__ double s_red[APB][7];
@@ -556,9 +559,10 @@ k_eam_force(EamDev P, const SplHdr *__restrict__ hdr, const R *__restrict__ rec,
         const int q = threadIdx.x;
         double v = 0.0;
         for (int k = 0; k < APB; ++k) v += s_red[k][q];
-        // record layout [e, W column-major 3x3]
+        // record layout [e, W column-major 3x3]; e includes the embedding
+        // energy of the same block's atoms from the density pass
         double *rec10 = partial + (size_t)blockIdx.x * 10;
-        if (q == 0) rec10[0] = v;
+        if (q == 0) rec10[0] = v + dens_e[blockIdx.x];
         else if (q == 1) rec10[1 + 0] = v;
         else if (q == 2) rec10[1 + 4] = v;
         else if (q == 3) rec10[1 + 8] = v;
@@ -627,7 +631,7 @@ static int eam_passes(EamState *st, agpu_potential *pot, const agpu_eval_args *a
                 L, a->mask, st->dF.as<double>(), q2, a->epot_per_at, part, flags);
     AGPU_LAUNCH(mixed ? "eam_force_mixed" : "eam_force_fp64", (k_eam_force<R, G, U, ONE, GEN, KN>), dim3(nblk),
                 dim3(256), 0, s, st->dev, st->d_hdr.as<SplHdr>(), rec, nat, order, q1, q2, C,
-                L, a->mask, a->f, a->epot_per_at, a->wpot_per_at, part + nblk, flags);
+                L, a->mask, a->f, a->epot_per_at, a->wpot_per_at, part, part + nblk, flags);
     return AGPU_OK;
 }
 
@@ -659,7 +663,7 @@ static int eam_launch(EamState *st, agpu_potential *pot, const agpu_eval_args *a
     *nblk_out = nblk;
     const int *el = pot->d_el.as<int>();
     AGPU_LAUNCH("eam_pack", k_eam_pack, dim3((nat + 255) / 256), dim3(256), 0, stream(), nat, pot->p->d_r, el,
-                st->pk.as<double4>());
+                st->pk.as<double4>(), st->stats.as<int>());
     if constexpr (ONE && KN) {
         if (!gen) {
             switch (G * 100 + U) {
@@ -706,8 +710,7 @@ int eam_compute(agpu_potential *pot, const agpu_eval_args *a)
     AGPU_CHECK(nl_update(pot->nl, p, 0, false));
     const int nat = p->nat;
     hipStream_t s = stream();
-    AGPU_CHECK(zero_dev(a->epot, sizeof(double)));
-    if (nat == 0) return AGPU_OK;
+    if (nat == 0) return zero_dev(a->epot, sizeof(double));
     // blocks of the widest lane group (EAM_GMAX lanes per atom): the most
     // blocks any variant launches
     const int nblk_max = (nat + 256 / EAM_GMAX - 1) / (256 / EAM_GMAX);
@@ -716,15 +719,14 @@ int eam_compute(agpu_potential *pot, const agpu_eval_args *a)
     AGPU_CHECK(st->partial.reserve(sizeof(double) * 11 * (size_t)nblk_max));
     AGPU_CHECK(st->stats.reserve(sizeof(int) * 4));
     AGPU_CHECK(pot->stats.init());
-    AGPU_CHECK(zero_dev(st->stats.ptr, sizeof(int) * 4));
     AGPU_CHECK(st->pk.reserve(sizeof(double4) * 2 * (size_t)(nat + 1)));
     int nblk = 0;
     if (pot->precision == AGPU_PREC_MIXED)
         AGPU_CHECK(eam_dispatch<float>(st, pot, a, st->d_rec32.as<float>(), &nblk));
     else
         AGPU_CHECK(eam_dispatch<double>(st, pot, a, st->d_rec64.as<double>(), &nblk));
-    AGPU_CHECK(reduce_ev(st->partial.as<double>(), nblk, 1, a->epot, nullptr, nullptr));
-    AGPU_CHECK(reduce_ev(st->partial.as<double>() + nblk, nblk, 10, a->epot, a->wpot, nullptr));
+    // epot := sum of the block records (embedding + pair energy), wpot +=
+    AGPU_CHECK(reduce_ev(st->partial.as<double>() + nblk, nblk, 10, a->epot, a->wpot, nullptr, true));
     if (capturing()) return AGPU_OK;     // the MD driver reads the flags after the replay
     AGPU_HIP(hipMemcpyAsync(pot->stats.h, st->stats.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
     AGPU_HIP(hipStreamSynchronize(s));

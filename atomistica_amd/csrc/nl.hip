@@ -5,7 +5,7 @@
 // binning_init/binning_update (:765-959):
 //   1. k_bin:      bin of every atom (floor(rec_cell_size.r) + 1, wrapped on
 //                  periodic axes, clamped otherwise) and its wrap count.
-//   2. counting sort by bin (k_cell_count, exclusive scan, k_cell_fill), then
+//   2. counting sort by bin (counts from k_bin, exclusive scan, k_cell_fill), then
 //                  k_cell_order puts each bin back into ascending atom order:
 //                  the reference's linked lists, as arrays, in row-major bin
 //                  order.  A bin too full for k_cell_order flags a rebuild
@@ -79,8 +79,9 @@ __device__ __forceinline__ void bin_of(const BinGeom &g, const double *r, int *c
     }
 }
 
+// cnt (counting sort): the bin populations are counted here as well
 __global__ void k_bin(int nat, const double *__restrict__ r, BinGeom g, int *__restrict__ key,
-                      int *__restrict__ idx, int *__restrict__ flags)
+                      int *__restrict__ idx, int *__restrict__ cnt, int *__restrict__ flags)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nat) return;
@@ -88,8 +89,10 @@ __global__ void k_bin(int nat, const double *__restrict__ r, BinGeom g, int *__r
     int c[3], w[3];
     bin_of(g, ri, c, w);
     if (!dc_pack_ok(w[0]) || !dc_pack_ok(w[1]) || !dc_pack_ok(w[2])) atomicOr(flags, DERR_IMAGE_RANGE);
-    key[i] = ((c[0] - 1) * g.nc[1] + (c[1] - 1)) * g.nc[2] + (c[2] - 1);
+    const int b = ((c[0] - 1) * g.nc[1] + (c[1] - 1)) * g.nc[2] + (c[2] - 1);
+    key[i] = b;
     idx[i] = i;
+    if (cnt) atomicAdd(&cnt[b], 1);
 }
 
 __global__ void k_cell_start(int nat, int ncell, const int *__restrict__ key_sorted,
@@ -113,12 +116,6 @@ __global__ void k_zero_i(int *__restrict__ a, long long n)
 {
     for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long long)gridDim.x * blockDim.x)
         a[q] = 0;
-}
-
-__global__ void k_cell_count(int nat, const int *__restrict__ key, int *__restrict__ cnt)
-{
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nat) atomicAdd(&cnt[key[i]], 1);
 }
 
 __global__ void k_cell_fill(int nat, int ncell, const int *__restrict__ key, int *__restrict__ cell_start,
@@ -206,6 +203,18 @@ __device__ __forceinline__ int find_bin(const int *pre, int e)
     return lo;
 }
 
+constexpr int DC_ZERO = (512 << 20) | (512 << 10) | 512;   // dc_pack(0, 0, 0)
+
+// a wave-uniform value moved to scalar registers
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ double uniform(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffffll));
+    const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 __device__ __forceinline__ int off_pack(int a, int b, int c) { return ((a + 128) << 16) | ((b + 128) << 8) | (c + 128); }
 
 struct NlShared {
@@ -234,6 +243,20 @@ __device__ __forceinline__ double pair_d2(const CellArg &C, double xi, double yi
     return dot3_nc(d);
 }
 
+// The same distance when the home atom has no wrap (s = the candidate's own
+// shift ct) and A.s was formed per candidate: (r_i - r_j) + A.s, then the
+// squared norm, rounded step by step as pair_d2<true> does
+__device__ __forceinline__ double pair_d2_pre(double xi, double yi, double zi, double xj, double yj, double zj,
+                                              double p0, double p1, double p2)
+{
+#pragma clang fp contract(off)
+    double d[3];
+    d[0] = (xi - xj) + p0;
+    d[1] = (yi - yj) + p1;
+    d[2] = (zi - zj) + p2;
+    return dot3_nc(d);
+}
+
 // One wavefront per block of B x B x B home bins (B = 1 for dense bins).
 // The block's candidates -- the atoms of its (B + 2 nd)^3 stencil bins in
 // x-major bin order, ascending atom order inside each bin -- are held in
@@ -245,7 +268,9 @@ __device__ __forceinline__ double pair_d2(const CellArg &C, double xi, double yi
 // (python_neighbors.f90:664-735).  The ballot of accepted lanes ranks each
 // accepted pair inside the row; rows go to a padded staging area (stride S
 // per sorted slot) and are compacted afterwards.
-template <bool ORTHO>
+// MULTI: blocks of B > 1 bins, whose candidates are checked against each home
+// atom's own stencil
+template <bool ORTHO, bool MULTI>
 __global__ void __launch_bounds__(256)
 k_bin_pairs(int B, BinGeom g, CellArg C, double cutoff_sq, const SortedAtom *__restrict__ atoms,
             const int *__restrict__ cell_start, int *__restrict__ counts, int2 *__restrict__ stage,
@@ -356,17 +381,18 @@ k_bin_pairs(int B, BinGeom g, CellArg C, double cutoff_sq, const SortedAtom *__r
             const int T = L.pre[63];
             for (int c0 = 0; c0 < T; c0 += 64 * NL_R) {
                 // ---- load this pass's candidates into registers ----
-                double cx[NL_R], cy[NL_R], cz[NL_R];
-                int cj[NL_R], ct0[NL_R], ct1[NL_R], ct2[NL_R], co[NL_R];
+                double cx[NL_R], cy[NL_R], cz[NL_R], cp0[NL_R], cp1[NL_R], cp2[NL_R];
+                int cj[NL_R], co[NL_R], cpk[NL_R];
                 bool cv[NL_R];
 #pragma unroll
                 for (int r = 0; r < NL_R; ++r) {
                     const int e = c0 + 64 * r + lane;
                     cv[r] = e < T;
                     cx[r] = cy[r] = cz[r] = 0.0;
+                    cp0[r] = cp1[r] = cp2[r] = 0.0;
                     cj[r] = -1;
-                    ct0[r] = ct1[r] = ct2[r] = 0;
                     co[r] = 0;
+                    cpk[r] = -1;
                     if (cv[r]) {
                         const int b = find_bin(L.pre, e);
                         const int src = L.start[b] + e - (b ? L.pre[b - 1] : 0);
@@ -378,41 +404,75 @@ k_bin_pairs(int B, BinGeom g, CellArg C, double cutoff_sq, const SortedAtom *__r
                         cy[r] = A.y;
                         cz[r] = A.z;
                         cj[r] = A.j;
-                        ct0[r] = u0 - w0;
-                        ct1[r] = u1 - w1;
-                        ct2[r] = u2 - w2;
-                        co[r] = L.boff[b];
+                        const int t0 = u0 - w0, t1 = u1 - w1, t2 = u2 - w2;
+                        if (MULTI) co[r] = L.boff[b];
+                        if (ORTHO) {
+                            cp0[r] = C.A[0] * (double)t0;
+                            cp1[r] = C.A[4] * (double)t1;
+                            cp2[r] = C.A[8] * (double)t2;
+                        }
+                        // -1: beyond the packed range (read again below)
+                        if (dc_pack_ok(t0) && dc_pack_ok(t1) && dc_pack_ok(t2)) cpk[r] = dc_pack(t0, t1, t2);
                     }
                 }
                 // ---- home atoms one at a time (wave-uniform) ----
                 for (int a = 0; a < ma; ++a) {
-                    const double xi = L.hx[a], yi = L.hy[a], zi = L.hz[a];
-                    const int ia = L.hi[a], w0 = L.hw0[a], w1 = L.hw1[a], w2 = L.hw2[a];
-                    const int ho = L.ho[a];
-                    const size_t srow = (size_t)L.hp[a] * S;
-                    int cnt = L.hcnt[a];
+                    // wave-uniform home data, held in scalar registers
+                    const double xi = uniform(L.hx[a]), yi = uniform(L.hy[a]), zi = uniform(L.hz[a]);
+                    const int ia = uniform(L.hi[a]), w0 = uniform(L.hw0[a]), w1 = uniform(L.hw1[a]),
+                              w2 = uniform(L.hw2[a]);
+                    const int ho = uniform(L.ho[a]);
+                    const size_t srow = (size_t)uniform(L.hp[a]) * S;
+                    int cnt = uniform(L.hcnt[a]);
+                    // a home atom inside the cell (no wrap) pairs with shift
+                    // s = ct, whose A.s and packed form were made at the load
+                    const bool hz0 = ORTHO && (w0 | w1 | w2) == 0;
 #pragma unroll
                     for (int r = 0; r < NL_R; ++r) {
+                        if (c0 + 64 * r >= T) break;     // slot row past the last candidate
                         bool acc = false;
-                        const int s0 = w0 + ct0[r], s1 = w1 + ct1[r], s2 = w2 + ct2[r];
-                        if (cv[r]) {
-                            bool member = true;
-                            if (B > 1) {
-                                const int d0 = ((co[r] >> 16) & 255) - ((ho >> 16) & 255);
-                                const int d1 = ((co[r] >> 8) & 255) - ((ho >> 8) & 255);
-                                const int d2 = (co[r] & 255) - (ho & 255);
-                                member = d0 >= -g.nd[0] && d0 <= g.nd[0] && d1 >= -g.nd[1] && d1 <= g.nd[1] &&
-                                         d2 >= -g.nd[2] && d2 <= g.nd[2];
+                        bool member = cv[r];
+                        if (MULTI && member) {
+                            const int d0 = ((co[r] >> 16) & 255) - ((ho >> 16) & 255);
+                            const int d1 = ((co[r] >> 8) & 255) - ((ho >> 8) & 255);
+                            const int d2 = (co[r] & 255) - (ho & 255);
+                            member = d0 >= -g.nd[0] && d0 <= g.nd[0] && d1 >= -g.nd[1] && d1 <= g.nd[1] &&
+                                     d2 >= -g.nd[2] && d2 <= g.nd[2];
+                        }
+                        int pk;
+                        if (hz0) {
+                            pk = cpk[r];
+                            if (member && !(cj[r] == ia && pk == DC_ZERO))
+                                acc = pair_d2_pre(xi, yi, zi, cx[r], cy[r], cz[r], cp0[r], cp1[r], cp2[r]) < cutoff_sq;
+                        } else {
+                            int t0, t1, t2;
+                            if (cpk[r] >= 0) {
+                                dc_unpack(cpk[r], t0, t1, t2);
+                            } else {
+                                // shift beyond the packed range, or no candidate
+                                t0 = t1 = t2 = 0;
+                                if (cv[r]) {
+                                    const int e = c0 + 64 * r + lane;
+                                    const int b = find_bin(L.pre, e);
+                                    const int src = L.start[b] + e - (b ? L.pre[b - 1] : 0);
+                                    int u0, u1, u2, v0, v1, v2;
+                                    dc_unpack(L.bu[b], u0, u1, u2);
+                                    dc_unpack(atoms[src].w, v0, v1, v2);
+                                    t0 = u0 - v0;
+                                    t1 = u1 - v1;
+                                    t2 = u2 - v2;
+                                }
                             }
+                            const int s0 = w0 + t0, s1 = w1 + t1, s2 = w2 + t2;
                             if (member && !(cj[r] == ia && s0 == 0 && s1 == 0 && s2 == 0))
                                 acc = pair_d2<ORTHO>(C, xi, yi, zi, cx[r], cy[r], cz[r], s0, s1, s2) < cutoff_sq;
+                            pk = dc_pack_ok(s0) && dc_pack_ok(s1) && dc_pack_ok(s2) ? dc_pack(s0, s1, s2) : -1;
                         }
                         const unsigned long long m = __ballot(acc);
                         if (acc) {
                             const int pos = cnt + __popcll(m & lanes_below(lane));
-                            if (!dc_pack_ok(s0) || !dc_pack_ok(s1) || !dc_pack_ok(s2))
-                                atomicOr(flags, DERR_IMAGE_RANGE);
-                            if (pos < S) stage[srow + pos] = make_int2(cj[r], dc_pack(s0, s1, s2));
+                            if (pk < 0) atomicOr(flags, DERR_IMAGE_RANGE);
+                            if (pos < S) stage[srow + pos] = make_int2(cj[r], pk);
                         }
                         cnt += __popcll(m);
                     }
@@ -513,20 +573,35 @@ k_atom_pairs(int nat, BinGeom g, CellArg C, double cutoff_sq, const SortedAtom *
     }
 }
 
-// The reference's capacity rule (python_neighbors.f90:713-718): row i ends at
-// 1-based slot offsets[i] + cnt + i (one separator per earlier atom), and a
-// non-empty row that ends at or beyond nat*avgn overflows the list.
+// One pass over the row lengths of a build: the longest row (flags[0],
+// atomicMax once per block of 4096 rows) and the reference's capacity rule
+// (python_neighbors.f90:713-718): row i ends at 1-based slot offsets[i] +
+// cnt + i (one separator per earlier atom), and a non-empty row that ends at
+// or beyond nat*avgn overflows the list (flags[3]).
 __global__ void __launch_bounds__(256)
-k_nl_capacity(int nat, const int *__restrict__ counts, const int *__restrict__ offsets, long long cap,
+k_nl_rowstats(int nat, const int *__restrict__ counts, const int *__restrict__ offsets, long long cap,
               int *__restrict__ flags)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ int sm[4];
+    int m = 0;
     bool over = false;
-    if (i < nat) {
-        const int cnt = counts[i];
-        over = cnt > 0 && (long long)offsets[i] + cnt + i >= cap;
+    const int base = blockIdx.x * 4096;
+    for (int k = threadIdx.x; k < 4096; k += 256) {
+        const int i = base + k;
+        if (i < nat) {
+            const int cnt = counts[i];
+            m = max(m, cnt);
+            over = over || (cnt > 0 && (long long)offsets[i] + cnt + i >= cap);
+        }
     }
-    if (__ballot(over) && (threadIdx.x & 63) == 0) atomicOr(flags, DERR_NL_OVERFLOW);
+    for (int o = 32; o >= 1; o >>= 1) m = max(m, __shfl_xor(m, o, 64));
+    const bool any_over = __ballot(over) != 0ull;
+    if ((threadIdx.x & 63) == 0) {
+        sm[threadIdx.x >> 6] = m;
+        if (any_over) atomicOr(flags + 3, DERR_NL_OVERFLOW);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) atomicMax(flags, max(max(sm[0], sm[1]), max(sm[2], sm[3])));
 }
 
 // Copy staged rows (sorted slot p, stride S) into the CSR rows of the caller
@@ -684,14 +759,24 @@ int nl_update(agpu_neighbors *nl, agpu_particles *p, int force, bool csr)
     const int TB = 256;
     if (nat > 0) {
         int nb = (nat + TB - 1) / TB;
-        AGPU_LAUNCH("nl_bin", k_bin, dim3(nb), dim3(TB), 0, s, nat, p->d_r, g,
-                    nl->key.as<int>(), nl->idx.as<int>(), flags + 3);
+        int *cnt = nullptr;
+        long long cnt_zero = 0;
         if (!nl->radix) {
+            // the bin counters are zero again after k_cell_fill; they are
+            // cleared only when new, grown or left by an unfinished build
+            void *old = nl->cell_cnt.ptr;
             AGPU_CHECK(nl->cell_cnt.reserve(sizeof(int) * (ncell + 1)));
-            int *cnt = nl->cell_cnt.as<int>();
-            const int nzb = (int)std::min<long long>(1024, (ncell + TB - 1) / TB);
-            AGPU_LAUNCH("nl_sort_zero", k_zero_i, dim3(nzb), dim3(TB), 0, s, cnt, ncell);
-            AGPU_LAUNCH("nl_sort_count", k_cell_count, dim3(nb), dim3(TB), 0, s, nat, nl->key.as<int>(), cnt);
+            cnt = nl->cell_cnt.as<int>();
+            cnt_zero = nl->cell_cnt.ptr == old ? nl->cnt_zero : 0;
+            nl->cnt_zero = 0;
+            if (cnt_zero < ncell) {
+                const int nzb = (int)std::min<long long>(1024, (ncell + TB - 1) / TB);
+                AGPU_LAUNCH("nl_sort_zero", k_zero_i, dim3(nzb), dim3(TB), 0, s, cnt, ncell);
+            }
+        }
+        AGPU_LAUNCH("nl_bin", k_bin, dim3(nb), dim3(TB), 0, s, nat, p->d_r, g,
+                    nl->key.as<int>(), nl->idx.as<int>(), cnt, flags + 3);
+        if (!nl->radix) {
             size_t tmp = 0;
             AGPU_HIP(rocprim::exclusive_scan(nullptr, tmp, cnt, nl->cell_start.as<int>(), 0, (size_t)ncell,
                                              rocprim::plus<int>(), s));
@@ -700,6 +785,7 @@ int nl_update(agpu_neighbors *nl, agpu_particles *p, int force, bool csr)
                                              (size_t)ncell, rocprim::plus<int>(), s));
             AGPU_LAUNCH("nl_sort_fill", k_cell_fill, dim3(nb), dim3(TB), 0, s, nat, (int)ncell, nl->key.as<int>(),
                         nl->cell_start.as<int>(), cnt, nl->idx_sorted.as<int>());
+            nl->cnt_zero = std::max(cnt_zero, ncell);
             // lanes per bin from the mean occupancy (capacity 4 atoms per lane)
             const double occ = (double)nat / (double)ncell;
             if (occ > 8.0)
@@ -758,25 +844,23 @@ int nl_update(agpu_neighbors *nl, agpu_particles *p, int force, bool csr)
                 AGPU_LAUNCH("nl_pairs", k_atom_pairs<false>, dim3(na), dim3(TB), 0, s, nat, g, C, cutoff_sq,
                             nl->atoms_sorted.as<SortedAtom>(), nl->cell_start.as<int>(), nl->counts.as<int>(),
                             nl->stage.as<int2>(), S, flags + 3);
-        } else if (ortho)
-            AGPU_LAUNCH("nl_pairs", k_bin_pairs<true>, dim3(nbin_blk), dim3(TB), 0, s, B, g, C, cutoff_sq,
+        } else {
+            auto kern = ortho ? (B > 1 ? k_bin_pairs<true, true> : k_bin_pairs<true, false>)
+                              : (B > 1 ? k_bin_pairs<false, true> : k_bin_pairs<false, false>);
+            AGPU_LAUNCH("nl_pairs", kern, dim3(nbin_blk), dim3(TB), 0, s, B, g, C, cutoff_sq,
                         nl->atoms_sorted.as<SortedAtom>(), nl->cell_start.as<int>(), nl->counts.as<int>(),
                         nl->stage.as<int2>(), S, flags + 3, flags);
-        else
-            AGPU_LAUNCH("nl_pairs", k_bin_pairs<false>, dim3(nbin_blk), dim3(TB), 0, s, B, g, C, cutoff_sq,
-                        nl->atoms_sorted.as<SortedAtom>(), nl->cell_start.as<int>(), nl->counts.as<int>(),
-                        nl->stage.as<int2>(), S, flags + 3, flags);
+        }
         size_t tmp2 = 0;
         AGPU_HIP(rocprim::exclusive_scan(nullptr, tmp2, nl->counts.as<int>(), nl->offsets.as<int>(), 0,
                                          (size_t)nat, rocprim::plus<int>(), s));
         AGPU_CHECK(nl->sort_tmp.reserve(tmp2 + 16));
         AGPU_HIP(rocprim::exclusive_scan(nl->sort_tmp.ptr, tmp2, nl->counts.as<int>(),
                                          nl->offsets.as<int>(), 0, (size_t)nat, rocprim::plus<int>(), s));
-        AGPU_CHECK(int_max(nl->counts.as<int>(), nat, flags));
+        AGPU_LAUNCH("nl_rowstats", k_nl_rowstats, dim3((nat + 4095) / 4096), dim3(TB), 0, s, nat,
+                    nl->counts.as<int>(), nl->offsets.as<int>(), (long long)nl->cap, flags);
         AGPU_LAUNCH("nl_offsets_tail", k_offsets_tail, dim3(1), dim3(64), 0, s, nat,
                     nl->counts.as<int>(), nl->offsets.as<int>(), flags);
-        AGPU_LAUNCH("nl_capacity", k_nl_capacity, dim3(nb), dim3(TB), 0, s, nat, nl->counts.as<int>(),
-                    nl->offsets.as<int>(), (long long)nl->cap, flags + 3);
         if (csr)
             AGPU_LAUNCH("nl_compact", k_nl_compact, dim3((nat + 15) / 16), dim3(TB), 0, s, nat,
                         nl->atoms_sorted.as<SortedAtom>(), nl->counts.as<int>(), nl->offsets.as<int>(),

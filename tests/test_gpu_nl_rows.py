@@ -101,3 +101,30 @@ def test_capacity_overflow_without_csr_copy():
     with pytest.raises(AgpuError) as ei:
         pot.energy_and_forces(p, nl)
     assert ei.value.code == 3
+
+
+@pytest.mark.parametrize("shape", ["ortho", "triclinic"])
+def test_pair_modes_with_wrapped_atoms(shape):
+    # dense bins (a wavefront per bin) and sparse ones (a thread per atom),
+    # with a third of the atoms moved by whole cell vectors so that both home
+    # atoms inside the cell and wrapped ones are paired
+    pos, cell, pbc, Z = W.fcc_cu(6, sigma=0.08, seed=8)
+    rng = np.random.default_rng(8)
+    if shape == "triclinic":
+        cell = cell + np.array([[0.0, 0.0, 0.0], [2.1, 0.0, 0.0], [-1.3, 1.7, 0.0]])
+        pos = (pos @ np.linalg.inv(W.fcc_cu(6, sigma=0.0)[1])) @ cell
+    sel = rng.random(len(pos)) < 0.33
+    pos[sel] += rng.integers(-3, 4, (int(sel.sum()), 3)) @ cell
+    rc = 5.5
+    ref = O.nl_build(pos, cell, pbc, rc)
+    p = N.from_atoms(pos, cell, pbc, Z)
+    nl = N.Neighbors(100)
+    nl.request_interaction_range(rc)
+    pot = N.Tersoff()            # handle for set_launch; its range is below rc
+    pot.bind_to(p, nl)
+    for mode in (1, 2):
+        print("wrapped", shape, "pair mode", mode, flush=True)
+        pot.set_launch(nl, pair_mode=mode)
+        p.changed_positions()
+        off, j, dc = nl.get_list(p)
+        assert np.array_equal(off, ref[0]) and np.array_equal(j, ref[1]) and np.array_equal(dc, ref[2])
