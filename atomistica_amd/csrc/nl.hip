@@ -188,11 +188,6 @@ constexpr int NL_R = 4;         // candidate slots per lane (registers): 256 per
 constexpr int NL_WAVES = 4;     // wavefronts per block
 constexpr int NL_HMAX = 64;     // home atoms per pass
 
-__device__ __forceinline__ unsigned long long lanes_below(int l)
-{
-    return l >= 64 ? ~0ull : ((1ull << l) - 1ull);
-}
-
 // smallest b in [0, 64) with pre[b] > e (pre = inclusive prefix, pre[63] > e)
 __device__ __forceinline__ int find_bin(const int *pre, int e)
 {
@@ -218,8 +213,7 @@ __device__ __forceinline__ double uniform(double v)
 __device__ __forceinline__ int off_pack(int a, int b, int c) { return ((a + 128) << 16) | ((b + 128) << 8) | (c + 128); }
 
 struct NlShared {
-    double hx[NL_HMAX], hy[NL_HMAX], hz[NL_HMAX];
-    int hi[NL_HMAX], hw0[NL_HMAX], hw1[NL_HMAX], hw2[NL_HMAX], ho[NL_HMAX], hp[NL_HMAX], hcnt[NL_HMAX];
+    int hi[NL_HMAX], ho[NL_HMAX], hp[NL_HMAX], hcnt[NL_HMAX];
     int pre[64], start[64], bu[64], boff[64];
 };
 
@@ -309,7 +303,7 @@ k_bin_pairs(int B, BinGeom g, CellArg C, double cutoff_sq, const SortedAtom *__r
     const int sx = B + 2 * g.nd[0], sy = B + 2 * g.nd[1], sz = B + 2 * g.nd[2];
     const int nst = sx * sy * sz;
     int wmax = 0;
-    bool stage_over = false;
+    bool stage_over = false, img_bad = false;
 
     for (int a0 = 0; a0 < MH; a0 += NL_HMAX) {
         const int ma = min(NL_HMAX, MH - a0);
@@ -322,16 +316,7 @@ k_bin_pairs(int B, BinGeom g, CellArg C, double cutoff_sq, const SortedAtom *__r
             const int e = a0 + lane;
             const int hb = find_bin(L.pre, e);
             const int p = L.start[hb] + e - (hb ? L.pre[hb - 1] : 0);
-            const SortedAtom A = atoms[p];
-            int w0, w1, w2;
-            dc_unpack(A.w, w0, w1, w2);
-            L.hx[lane] = A.x;
-            L.hy[lane] = A.y;
-            L.hz[lane] = A.z;
-            L.hi[lane] = A.j;
-            L.hw0[lane] = w0;
-            L.hw1[lane] = w1;
-            L.hw2[lane] = w2;
+            L.hi[lane] = atoms[p].j;
             L.ho[lane] = L.boff[hb];
             L.hp[lane] = p;
             L.hcnt[lane] = 0;
@@ -417,64 +402,72 @@ k_bin_pairs(int B, BinGeom g, CellArg C, double cutoff_sq, const SortedAtom *__r
                 }
                 // ---- home atoms one at a time (wave-uniform) ----
                 for (int a = 0; a < ma; ++a) {
-                    // wave-uniform home data, held in scalar registers
-                    const double xi = uniform(L.hx[a]), yi = uniform(L.hy[a]), zi = uniform(L.hz[a]);
-                    const int ia = uniform(L.hi[a]), w0 = uniform(L.hw0[a]), w1 = uniform(L.hw1[a]),
-                              w2 = uniform(L.hw2[a]);
-                    const int ho = uniform(L.ho[a]);
-                    const size_t srow = (size_t)uniform(L.hp[a]) * S;
+                    // the home slot from LDS, its record by a uniform
+                    // (scalar) load
+                    const int hp = uniform(L.hp[a]);
+                    const SortedAtom H = atoms[hp];
+                    const int ia = H.j;
+                    int w0, w1, w2;
+                    dc_unpack(H.w, w0, w1, w2);
+                    const int ho = MULTI ? uniform(L.ho[a]) : 0;
+                    int2 *row = stage + (size_t)hp * S;
                     int cnt = uniform(L.hcnt[a]);
-                    // a home atom inside the cell (no wrap) pairs with shift
-                    // s = ct, whose A.s and packed form were made at the load
-                    const bool hz0 = ORTHO && (w0 | w1 | w2) == 0;
-#pragma unroll
-                    for (int r = 0; r < NL_R; ++r) {
-                        if (c0 + 64 * r >= T) break;     // slot row past the last candidate
-                        bool acc = false;
-                        bool member = cv[r];
-                        if (MULTI && member) {
-                            const int d0 = ((co[r] >> 16) & 255) - ((ho >> 16) & 255);
-                            const int d1 = ((co[r] >> 8) & 255) - ((ho >> 8) & 255);
-                            const int d2 = (co[r] & 255) - (ho & 255);
-                            member = d0 >= -g.nd[0] && d0 <= g.nd[0] && d1 >= -g.nd[1] && d1 <= g.nd[1] &&
-                                     d2 >= -g.nd[2] && d2 <= g.nd[2];
-                        }
-                        int pk;
-                        if (hz0) {
-                            pk = cpk[r];
-                            if (member && !(cj[r] == ia && pk == DC_ZERO))
-                                acc = pair_d2_pre(xi, yi, zi, cx[r], cy[r], cz[r], cp0[r], cp1[r], cp2[r]) < cutoff_sq;
-                        } else {
-                            int t0, t1, t2;
-                            if (cpk[r] >= 0) {
-                                dc_unpack(cpk[r], t0, t1, t2);
-                            } else {
-                                // shift beyond the packed range, or no candidate
-                                t0 = t1 = t2 = 0;
-                                if (cv[r]) {
-                                    const int e = c0 + 64 * r + lane;
-                                    const int b = find_bin(L.pre, e);
-                                    const int src = L.start[b] + e - (b ? L.pre[b - 1] : 0);
-                                    int u0, u1, u2, v0, v1, v2;
-                                    dc_unpack(L.bu[b], u0, u1, u2);
-                                    dc_unpack(atoms[src].w, v0, v1, v2);
-                                    t0 = u0 - v0;
-                                    t1 = u1 - v1;
-                                    t2 = u2 - v2;
-                                }
-                            }
-                            const int s0 = w0 + t0, s1 = w1 + t1, s2 = w2 + t2;
-                            if (member && !(cj[r] == ia && s0 == 0 && s1 == 0 && s2 == 0))
-                                acc = pair_d2<ORTHO>(C, xi, yi, zi, cx[r], cy[r], cz[r], s0, s1, s2) < cutoff_sq;
-                            pk = dc_pack_ok(s0) && dc_pack_ok(s1) && dc_pack_ok(s2) ? dc_pack(s0, s1, s2) : -1;
-                        }
+                    // accepted lanes of one slot row, ranked after the row's
+                    // entries so far
+                    auto emit = [&](bool acc, int j, int pk) {
                         const unsigned long long m = __ballot(acc);
                         if (acc) {
-                            const int pos = cnt + __popcll(m & lanes_below(lane));
-                            if (pk < 0) atomicOr(flags, DERR_IMAGE_RANGE);
-                            if (pos < S) stage[srow + pos] = make_int2(cj[r], pk);
+                            const unsigned pos = __builtin_amdgcn_mbcnt_hi(
+                                (unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, (unsigned)cnt));
+                            img_bad = img_bad || pk < 0;
+                            if (pos < (unsigned)S) row[pos] = make_int2(j, pk);
                         }
                         cnt += __popcll(m);
+                    };
+                    // a candidate of a multi-bin block counts only inside the
+                    // home atom's own stencil
+                    auto member = [&](int r) {
+                        if (!MULTI) return cv[r];
+                        const int d0 = ((co[r] >> 16) & 255) - ((ho >> 16) & 255);
+                        const int d1 = ((co[r] >> 8) & 255) - ((ho >> 8) & 255);
+                        const int d2 = (co[r] & 255) - (ho & 255);
+                        return cv[r] && d0 >= -g.nd[0] && d0 <= g.nd[0] && d1 >= -g.nd[1] && d1 <= g.nd[1] &&
+                               d2 >= -g.nd[2] && d2 <= g.nd[2];
+                    };
+                    if (ORTHO && (w0 | w1 | w2) == 0) {
+                        // home atom inside the cell: s = ct, whose A.s and
+                        // packed form were made at the candidate load
+#pragma unroll
+                        for (int r = 0; r < NL_R; ++r) {
+                            if (c0 + 64 * r >= T) break;     // slot row past the last candidate
+                            const double d2 = pair_d2_pre(H.x, H.y, H.z, cx[r], cy[r], cz[r], cp0[r], cp1[r], cp2[r]);
+                            emit(member(r) && !(cj[r] == ia && cpk[r] == DC_ZERO) && d2 < cutoff_sq, cj[r], cpk[r]);
+                        }
+                    } else {
+#pragma unroll
+                        for (int r = 0; r < NL_R; ++r) {
+                            if (c0 + 64 * r >= T) break;
+                            int t0 = 0, t1 = 0, t2 = 0;
+                            if (cpk[r] >= 0) {
+                                dc_unpack(cpk[r], t0, t1, t2);
+                            } else if (cv[r]) {
+                                // shift beyond the packed range: read it again
+                                const int e = c0 + 64 * r + lane;
+                                const int b = find_bin(L.pre, e);
+                                const int src = L.start[b] + e - (b ? L.pre[b - 1] : 0);
+                                int u0, u1, u2, v0, v1, v2;
+                                dc_unpack(L.bu[b], u0, u1, u2);
+                                dc_unpack(atoms[src].w, v0, v1, v2);
+                                t0 = u0 - v0;
+                                t1 = u1 - v1;
+                                t2 = u2 - v2;
+                            }
+                            const int s0 = w0 + t0, s1 = w1 + t1, s2 = w2 + t2;
+                            const double d2 = pair_d2<ORTHO>(C, H.x, H.y, H.z, cx[r], cy[r], cz[r], s0, s1, s2);
+                            const int pk = dc_pack_ok(s0) && dc_pack_ok(s1) && dc_pack_ok(s2) ? dc_pack(s0, s1, s2) : -1;
+                            emit(member(r) && !(cj[r] == ia && s0 == 0 && s1 == 0 && s2 == 0) && d2 < cutoff_sq,
+                                 cj[r], pk);
+                        }
                     }
                     if (lane == 0) L.hcnt[a] = cnt;
                     __builtin_amdgcn_wave_barrier();
@@ -494,7 +487,9 @@ k_bin_pairs(int B, BinGeom g, CellArg C, double cutoff_sq, const SortedAtom *__r
     (void)wmax;
     (void)rowmax;
     const bool any_over = __ballot(stage_over) != 0ull;
+    const bool any_bad = __ballot(img_bad) != 0ull;
     if (lane == 0 && any_over) atomicOr(flags, DERR_STAGE_OVERFLOW);
+    if (lane == 0 && any_bad) atomicOr(flags, DERR_IMAGE_RANGE);
 }
 
 // Sparse bins (a few atoms each): one thread per home atom in bin order,
