@@ -512,6 +512,9 @@ k_atom_pairs(int nat, BinGeom g, CellArg C, double cutoff_sq, const SortedAtom *
         bin_of(g, ri, hc, hw);
         int w0, w1, w2;
         dc_unpack(A.w, w0, w1, w2);
+        // home and candidate without wrap (the common case): s = u, the
+        // bin's own shift, whose A.u and packed form are made per bin
+        const bool hz0 = ORTHO && A.w == DC_ZERO;
         const size_t srow = (size_t)p * S;
         int cnt = 0;
         for (int ox = -g.nd[0]; ox <= g.nd[0]; ++ox) {
@@ -543,8 +546,25 @@ k_atom_pairs(int nat, BinGeom g, CellArg C, double cutoff_sq, const SortedAtom *
                     }
                     const int b = ((cx - 1) * g.nc[1] + (cy - 1)) * g.nc[2] + (cz - 1);
                     const int q0 = cell_start[b], q1 = cell_start[b + 1];
+                    double au0 = 0.0, au1 = 0.0, au2 = 0.0;
+                    if (ORTHO) {
+                        au0 = C.A[0] * (double)ux;
+                        au1 = C.A[4] * (double)uy;
+                        au2 = C.A[8] * (double)uz;
+                    }
+                    const bool u_ok = dc_pack_ok(ux) && dc_pack_ok(uy) && dc_pack_ok(uz);
+                    const int upk = u_ok ? dc_pack(ux, uy, uz) : -1;
                     for (int q = q0; q < q1; ++q) {
                         const SortedAtom Bq = atoms[q];
+                        if (hz0 && Bq.w == DC_ZERO) {
+                            if (Bq.j == A.j && upk == DC_ZERO) continue;
+                            if (pair_d2_pre(A.x, A.y, A.z, Bq.x, Bq.y, Bq.z, au0, au1, au2) < cutoff_sq) {
+                                if (!u_ok) bad = true;
+                                if (cnt < S) stage[srow + cnt] = make_int2(Bq.j, upk);
+                                ++cnt;
+                            }
+                            continue;
+                        }
                         int v0, v1, v2;
                         dc_unpack(Bq.w, v0, v1, v2);
                         const int s0 = w0 + ux - v0, s1 = w1 + uy - v1, s2 = w2 + uz - v2;
