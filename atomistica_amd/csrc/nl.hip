@@ -589,7 +589,7 @@ k_atom_pairs(int nat, BinGeom g, CellArg C, double cutoff_sq, const SortedAtom *
 }
 
 // One pass over the row lengths of a build: the longest row (flags[0],
-// atomicMax once per block of 4096 rows) and the reference's capacity rule
+// atomicMax once per block of 1024 rows) and the reference's capacity rule
 // (python_neighbors.f90:713-718): row i ends at 1-based slot offsets[i] +
 // cnt + i (one separator per earlier atom), and a non-empty row that ends at
 // or beyond nat*avgn overflows the list (flags[3]).
@@ -600,8 +600,8 @@ k_nl_rowstats(int nat, const int *__restrict__ counts, const int *__restrict__ o
     __shared__ int sm[4];
     int m = 0;
     bool over = false;
-    const int base = blockIdx.x * 4096;
-    for (int k = threadIdx.x; k < 4096; k += 256) {
+    const int base = blockIdx.x * 1024;
+    for (int k = threadIdx.x; k < 1024; k += 256) {
         const int i = base + k;
         if (i < nat) {
             const int cnt = counts[i];
@@ -872,7 +872,7 @@ int nl_update(agpu_neighbors *nl, agpu_particles *p, int force, bool csr)
         AGPU_CHECK(nl->sort_tmp.reserve(tmp2 + 16));
         AGPU_HIP(rocprim::exclusive_scan(nl->sort_tmp.ptr, tmp2, nl->counts.as<int>(),
                                          nl->offsets.as<int>(), 0, (size_t)nat, rocprim::plus<int>(), s));
-        AGPU_LAUNCH("nl_rowstats", k_nl_rowstats, dim3((nat + 4095) / 4096), dim3(TB), 0, s, nat,
+        AGPU_LAUNCH("nl_rowstats", k_nl_rowstats, dim3((nat + 1023) / 1024), dim3(TB), 0, s, nat,
                     nl->counts.as<int>(), nl->offsets.as<int>(), (long long)nl->cap, flags);
         AGPU_LAUNCH("nl_offsets_tail", k_offsets_tail, dim3(1), dim3(64), 0, s, nat,
                     nl->counts.as<int>(), nl->offsets.as<int>(), flags);
