@@ -37,6 +37,7 @@ KIND_KUMAGAI_SCR = 8
 KIND_BRENNER_SCR = 9
 KIND_PME = 10
 KIND_JUSLIN = 11
+KIND_JUSLIN_SCR = 12
 REBO2_ZERO_TABLES = 1
 REBO2_DIHEDRAL = 2
 
@@ -90,7 +91,8 @@ class JuslinParams(ctypes.Structure):
 
 
 class ScreeningParams(ctypes.Structure):
-    _fields_ = [(k, ctypes.c_double * 6) for k in ("or1", "or2", "bor1", "bor2", "Cmin", "Cmax")]
+    # per element pair: symmetric index (6 used) or, for JuslinScr, ordered (9)
+    _fields_ = [(k, ctypes.c_double * 9) for k in ("or1", "or2", "bor1", "bor2", "Cmin", "Cmax")]
 
 
 # name -> argtypes (every function returns int unless listed in _RESTYPE)
@@ -133,6 +135,7 @@ PROTOTYPES = {
     "agpu_tersoff_scr_create": [ctypes.POINTER(vp), vp, vp],
     "agpu_kumagai_scr_create": [ctypes.POINTER(vp), vp, vp],
     "agpu_brenner_scr_create": [ctypes.POINTER(vp), vp, vp],
+    "agpu_juslin_scr_create": [ctypes.POINTER(vp), vp, vp],
     "agpu_eam_alloy_create": [ctypes.POINTER(vp), ctypes.c_char_p, ctypes.c_char_p],
     "agpu_ewald_create": [ctypes.POINTER(vp), d, d, d],
     "agpu_pme_create": [ctypes.POINTER(vp), d, i, ip],

@@ -262,5 +262,6 @@ Brenner = _make(native.Brenner, "Brenner")
 TersoffScr = _make(native.TersoffScr, "TersoffScr")
 KumagaiScr = _make(native.KumagaiScr, "KumagaiScr")
 BrennerScr = _make(native.BrennerScr, "BrennerScr")
+JuslinScr = _make(native.JuslinScr, "JuslinScr")
 PME = _make(native.PME, "PME")
 Juslin = _make(native.Juslin, "Juslin")

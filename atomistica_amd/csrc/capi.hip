@@ -639,7 +639,8 @@ int agpu_potential_bind_to(agpu_potential *pot, agpu_particles *p, agpu_neighbor
     case AGPU_KIND_TERSOFF_SCR:
     case AGPU_KIND_KUMAGAI_SCR:
     case AGPU_KIND_BRENNER_SCR:
-    case AGPU_KIND_JUSLIN: rc = tersoff_bind(pot); break;
+    case AGPU_KIND_JUSLIN:
+    case AGPU_KIND_JUSLIN_SCR: rc = tersoff_bind(pot); break;
     case AGPU_KIND_EAM_ALLOY: rc = eam_bind(pot); break;
     case AGPU_KIND_EWALD:
     case AGPU_KIND_PME: rc = ewald_bind(pot); break;
@@ -665,7 +666,8 @@ static int stage_and_call(agpu_potential *pot, agpu_particles *p, agpu_neighbors
         case AGPU_KIND_TERSOFF_SCR:
         case AGPU_KIND_KUMAGAI_SCR:
         case AGPU_KIND_BRENNER_SCR:
-        case AGPU_KIND_JUSLIN: rc = tersoff_compute(pot, &a); break;
+        case AGPU_KIND_JUSLIN:
+        case AGPU_KIND_JUSLIN_SCR: rc = tersoff_compute(pot, &a); break;
         case AGPU_KIND_EAM_ALLOY: rc = eam_compute(pot, &a); break;
         case AGPU_KIND_EWALD:
         case AGPU_KIND_PME: rc = ewald_compute(pot, &a); break;
@@ -716,7 +718,8 @@ static int stage_and_call(agpu_potential *pot, agpu_particles *p, agpu_neighbors
     case AGPU_KIND_TERSOFF_SCR:
     case AGPU_KIND_KUMAGAI_SCR:
     case AGPU_KIND_BRENNER_SCR:
-    case AGPU_KIND_JUSLIN: rc = tersoff_compute(pot, &d); break;
+    case AGPU_KIND_JUSLIN:
+    case AGPU_KIND_JUSLIN_SCR: rc = tersoff_compute(pot, &d); break;
     case AGPU_KIND_EAM_ALLOY: rc = eam_compute(pot, &d); break;
     case AGPU_KIND_EWALD:
     case AGPU_KIND_PME: rc = ewald_compute(pot, &d); break;

@@ -125,7 +125,8 @@ static int md_force(agpu_md *md)
     case AGPU_KIND_TERSOFF_SCR:
     case AGPU_KIND_KUMAGAI_SCR:
     case AGPU_KIND_BRENNER_SCR:
-    case AGPU_KIND_JUSLIN: return tersoff_compute(md->pot, &a);
+    case AGPU_KIND_JUSLIN:
+    case AGPU_KIND_JUSLIN_SCR: return tersoff_compute(md->pot, &a);
     case AGPU_KIND_EAM_ALLOY: return eam_compute(md->pot, &a);
     default: return set_error(AGPU_ERR_UNSUPPORTED, "MD driver: Tersoff-family and EAM potentials only");
     }
@@ -233,7 +234,8 @@ extern "C" int agpu_md_create(agpu_md **out, agpu_potential *pot, agpu_particles
         return set_error(AGPU_ERR_ARG, "md_create: bad arguments");
     if (!(pot->kind == AGPU_KIND_TERSOFF || pot->kind == AGPU_KIND_KUMAGAI || pot->kind == AGPU_KIND_BRENNER ||
           pot->kind == AGPU_KIND_TERSOFF_SCR || pot->kind == AGPU_KIND_KUMAGAI_SCR ||
-          pot->kind == AGPU_KIND_BRENNER_SCR || pot->kind == AGPU_KIND_JUSLIN || pot->kind == AGPU_KIND_EAM_ALLOY))
+          pot->kind == AGPU_KIND_BRENNER_SCR || pot->kind == AGPU_KIND_JUSLIN || pot->kind == AGPU_KIND_JUSLIN_SCR ||
+          pot->kind == AGPU_KIND_EAM_ALLOY))
         return set_error(AGPU_ERR_UNSUPPORTED, "MD driver: Tersoff-family and EAM potentials only");
     const int n = p->nat;
     for (int i = 0; i < n; ++i)

@@ -650,10 +650,12 @@ k_bop_gather(int nat, const int *__restrict__ offsets, const int *__restrict__ b
 //                        neighbour wrote for a (through the reverse entry)
 // No atomics: every sum has one owner and a fixed order.
 // -------------------------------------------------------------------------
+// per element pair: the symmetric pair index (6 used), or the ordered one
+// j + i nel of the Juslin family (9)
 struct ScrDev {
-    double r1[6], r2[6], or1[6], or2[6], bor1[6], bor2[6];
-    double fin[6], fout[6], fbo[6];            // 1 / (r2 - r1) of the three cutoffs
-    double Cmin[6], Cmax[6], dC[6], C_dr_cut[6], max_cut_sq[6];
+    double r1[9], r2[9], or1[9], or2[9], bor1[9], bor2[9];
+    double fin[9], fout[9], fbo[9];            // 1 / (r2 - r1) of the three cutoffs
+    double Cmin[9], Cmax[9], dC[9], C_dr_cut[9], max_cut_sq[9];
     double ec, ed, efac2, eoff;                // exp_cutoff constants
 };
 
@@ -689,7 +691,8 @@ struct ScrShared {
 // count; FILL = true: cutoffs and records at roff.
 template <bool FILL>
 __global__ void __launch_bounds__(256)
-k_scr_screen(ScrDev S, int nat_tot, const double *__restrict__ r, const int *__restrict__ el, int nel, CellArg C,
+k_scr_screen(ScrDev S, int nat_tot, const double *__restrict__ r, const int *__restrict__ el, int nel, int ordered,
+             CellArg C,
              const int *__restrict__ offsets, const int2 *__restrict__ nbr, int *__restrict__ keep,
              int *__restrict__ rcnt, const int *__restrict__ roff, double4 *__restrict__ fcv,
              int *__restrict__ rkn, double4 *__restrict__ rco, int *__restrict__ flags)
@@ -726,7 +729,7 @@ k_scr_screen(ScrDev S, int nat_tot, const double *__restrict__ r, const int *__r
         int kept = 0, cnt = 0;
         double fcar = 0.0, dfcar = 0.0, fcbo = 0.0, dfcbo = 0.0;
         if (eli >= 0 && elj >= 0) {
-            const int typ = pair_index(eli, elj, nel);
+            const int typ = ordered ? elj + eli * nel : pair_index(eli, elj, nel);
             const double rij[3] = {L.x[t], L.y[t], L.z[t]};
             const double rl = rij[0] * rij[0] + rij[1] * rij[1] + rij[2] * rij[2];
             const double cin_l = S.r1[typ], cin_h = S.r2[typ];
@@ -1269,16 +1272,17 @@ int tersoff_bind(agpu_potential *pot)
         if (z > 0 && z < 128 && st->z2db[z] >= 0) present[st->z2db[z]] = true;
     // screened: x(pair) * sqrt(maxval(max_cut_sq)) with x = sqrt(C_dr_cut)
     // (default_bind_to_func.f90:113-150)
+    const bool ordered = st->dev.fam == 3;
     double mc_all = 0.0;
     if (st->scr)
-        for (int q = 0; q < prm.nel * (prm.nel + 1) / 2; ++q)
-            mc_all = std::max(mc_all, std::max(prm.r2[q], std::max(st->sp.or2[q], st->sp.bor2[q])));
+        for (int q = 0; q < (ordered ? prm.nel * prm.nel : prm.nel * (prm.nel + 1) / 2); ++q)
+            mc_all = std::max(mc_all, std::max(st->dev.r2[q], std::max(st->sp.or2[q], st->sp.bor2[q])));
     double cut = 0.0;
     for (int a = 0; a < prm.nel; ++a)
         for (int b = 0; b < prm.nel; ++b)
             if (present[a] && present[b]) {
-                const int q = pair_index(a, b, prm.nel);
-                double r2 = st->dev.fam == 3 ? st->dev.r2[b + a * prm.nel] : prm.r2[q];
+                const int q = ordered ? b + a * prm.nel : pair_index(a, b, prm.nel);
+                double r2 = ordered ? st->dev.r2[q] : prm.r2[q];
                 if (st->scr) {
                     const double cm = st->sp.Cmax[q];
                     const double cdr = cm > 2.0 ? cm * cm / (4 * (cm - 1)) : 1.0;
@@ -1365,12 +1369,14 @@ static int launch_gather(int G, int nat, agpu_neighbors *nl, TersoffState *st, d
     }
 }
 
-static void derive_scr(ScrDev &S, const agpu_tersoff_params &prm, const agpu_screening_params &sp)
+// inner cutoffs from the family's device parameters (the ordered pairs of
+// the Juslin family included)
+static void derive_scr(ScrDev &S, const TersoffDev &D, const agpu_screening_params &sp)
 {
     std::memset(&S, 0, sizeof S);
-    for (int q = 0; q < 6; ++q) {
-        S.r1[q] = prm.r1[q];
-        S.r2[q] = prm.r2[q];
+    for (int q = 0; q < 9; ++q) {
+        S.r1[q] = D.r1[q];
+        S.r2[q] = D.r2[q];
         S.or1[q] = sp.or1[q];
         S.or2[q] = sp.or2[q];
         S.bor1[q] = sp.bor1[q];
@@ -1417,6 +1423,10 @@ static int launch_bop_scr(TersoffState *st, const ScrDev &S, int nat, const int 
         AGPU_LAUNCH(f64 ? "brenner_scr_fp64" : "brenner_scr_mixed", (k_bop_scr<R, G, 2>), dim3(nblk), dim3(256), 0,
                     s, P, S, nat, order, r, el, C, offs, nb, bs, bc, mask, fcv, roff, rkn, rco, fslot, frec, fown,
                     es, ws, part);
+    else if (P.fam == 3)
+        AGPU_LAUNCH(f64 ? "juslin_scr_fp64" : "juslin_scr_mixed", (k_bop_scr<R, G, 3>), dim3(nblk), dim3(256), 0,
+                    s, P, S, nat, order, r, el, C, offs, nb, bs, bc, mask, fcv, roff, rkn, rco, fslot, frec, fown,
+                    es, ws, part);
     else
         AGPU_LAUNCH(f64 ? "tersoff_scr_fp64" : "tersoff_scr_mixed", (k_bop_scr<R, G, 0>), dim3(nblk), dim3(256), 0,
                     s, P, S, nat, order, r, el, C, offs, nb, bs, bc, mask, fcv, roff, rkn, rco, fslot, frec, fown,
@@ -1456,7 +1466,7 @@ static int tersoff_scr_compute(agpu_potential *pot, const agpu_eval_args *a)
     const int64_t E = nl->total;
     if (E > 0x7ffffff0ll) return set_error(AGPU_ERR_ARG, "screened bond order: list too long");
     ScrDev S;
-    derive_scr(S, st->prm, st->sp);
+    derive_scr(S, st->dev, st->sp);
     AGPU_CHECK(st->keep.reserve(sizeof(int) * (E + 1)));
     AGPU_CHECK(st->rcnt.reserve(sizeof(int) * (E + 1)));
     AGPU_CHECK(st->roff.reserve(sizeof(int) * (E + 1)));
@@ -1482,7 +1492,8 @@ static int tersoff_scr_compute(agpu_potential *pot, const agpu_eval_args *a)
     // pass 1: keep flags, cutoffs and record counts; exclusive scan to record
     // offsets
     AGPU_CHECK(zero_dev(st->rcnt.as<int>() + E, sizeof(int)));
-    AGPU_LAUNCH("scr_count", k_scr_screen<false>, gw, dim3(256), 0, s, S, nat, p->d_r, el, st->prm.nel, C, offs,
+    AGPU_LAUNCH("scr_count", k_scr_screen<false>, gw, dim3(256), 0, s, S, nat, p->d_r, el, st->prm.nel,
+                (int)(st->dev.fam == 3), C, offs,
                 nbr, st->keep.as<int>(), st->rcnt.as<int>(), nullptr, st->fcv.as<double4>(), nullptr, nullptr,
                 dstats);
     size_t tmp = 0;
@@ -1501,7 +1512,8 @@ static int tersoff_scr_compute(agpu_potential *pot, const agpu_eval_args *a)
     AGPU_CHECK(st->rco.reserve(sizeof(double4) * (nrec + 1)));
     AGPU_CHECK(st->frec.reserve(sizeof(double) * 3 * (nrec + 1)));
     // pass 2: the records (rows with none return at once)
-    AGPU_LAUNCH("scr_fill", k_scr_screen<true>, gw, dim3(256), 0, s, S, nat, p->d_r, el, st->prm.nel, C, offs, nbr,
+    AGPU_LAUNCH("scr_fill", k_scr_screen<true>, gw, dim3(256), 0, s, S, nat, p->d_r, el, st->prm.nel,
+                (int)(st->dev.fam == 3), C, offs, nbr,
                 st->keep.as<int>(), st->rcnt.as<int>(), st->roff.as<int>(), st->fcv.as<double4>(),
                 st->rkn.as<int>(), st->rco.as<double4>(), dstats);
     AGPU_LAUNCH("scr_bonds", k_scr_bonds, dim3((nat + 255) / 256), dim3(256), 0, s, nat, offs, st->keep.as<int>(),
@@ -1837,14 +1849,19 @@ static agpu_screening_params scr_uniform(int npairs, const double *or1, const do
 static int attach_screening(agpu_potential *P, const agpu_screening_params &sp, int kind)
 {
     TersoffState *st = static_cast<TersoffState *>(P->impl);
-    const int npairs = st->prm.nel * (st->prm.nel + 1) / 2;
+    const int nel = st->prm.nel;
+    const bool ordered = st->dev.fam == 3;
+    const int npairs = ordered ? nel * nel : nel * (nel + 1) / 2;
     double mc = 0.0, x = 1.0;
     for (int q = 0; q < npairs; ++q) {
+        // ordered pairs without parameters (r0 < 0 in the table) take their
+        // transposed pair, as the unscreened Juslin family does
+        if (ordered && !(st->dev.r2[q] > 0.0)) continue;
         if (!(sp.Cmax[q] > sp.Cmin[q])) {
             agpu_potential_destroy(P);
             return set_error(AGPU_ERR_ARG, "screening: need Cmin < Cmax for every element pair");
         }
-        mc = std::max(mc, std::max(st->prm.r2[q], std::max(sp.or2[q], sp.bor2[q])));
+        mc = std::max(mc, std::max(st->dev.r2[q], std::max(sp.or2[q], sp.bor2[q])));
         const double cm = sp.Cmax[q];
         x = std::max(x, cm > 2.0 ? std::sqrt(cm * cm / (4 * (cm - 1))) : 1.0);
     }
@@ -2065,4 +2082,50 @@ extern "C" int agpu_juslin_create(agpu_potential **pot, const agpu_juslin_params
     P->free_impl = tersoff_free;
     *pot = P;
     return AGPU_OK;
+}
+
+// JuslinScr (juslin_scr.f90): the Juslin family with the screening of the
+// shared kernel (SCREENING with EXP_BOP).  Screening parameters are per
+// ordered pair; entries whose r0 < 0 take those of the transposed pair, as
+// the other parameters do (juslin_module.f90:273-305).  Defaults: the
+// screened W-C-H set of juslin_params.f90 (W-H cutoffs 2.68-2.96 A; outer and
+// bond-order cutoffs equal to the inner ones; Cmin 1, Cmax 3).
+extern "C" int agpu_juslin_scr_create(agpu_potential **pot, const agpu_juslin_params *params,
+                                      const agpu_screening_params *scr)
+{
+    if (!pot) return set_error(AGPU_ERR_ARG, "pot is NULL");
+    agpu_juslin_params j = params ? *params : juslin_default();
+    if (!params) {
+        j.r1[2] = 2.68;
+        j.r2[2] = 2.96;
+    }
+    const int nel = j.nel;
+    if (nel < 1 || nel > 3) return set_error(AGPU_ERR_ARG, "Juslin: 1 to 3 elements supported");
+    agpu_screening_params sp;
+    if (scr) {
+        sp = *scr;
+    } else {
+        std::memset(&sp, 0, sizeof sp);
+        for (int q = 0; q < nel * nel; ++q) {
+            if (j.r0[q] < 0.0) continue;
+            sp.or1[q] = sp.bor1[q] = j.r1[q];
+            sp.or2[q] = sp.bor2[q] = j.r2[q];
+            sp.Cmin[q] = 1.0;
+            sp.Cmax[q] = 3.0;
+        }
+    }
+    for (int a = 0; a < nel; ++a)
+        for (int b = 0; b < nel; ++b) {
+            const int q = b + a * nel, t = a + b * nel;
+            if (j.r0[q] < 0.0) {
+                sp.or1[q] = sp.or1[t];
+                sp.or2[q] = sp.or2[t];
+                sp.bor1[q] = sp.bor1[t];
+                sp.bor2[q] = sp.bor2[t];
+                sp.Cmin[q] = sp.Cmin[t];
+                sp.Cmax[q] = sp.Cmax[t];
+            }
+        }
+    AGPU_CHECK(agpu_juslin_create(pot, &j));
+    return attach_screening(*pot, sp, AGPU_KIND_JUSLIN_SCR);
 }

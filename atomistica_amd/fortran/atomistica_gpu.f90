@@ -71,13 +71,14 @@ module atomistica_gpu
   !> Screening parameters of the screened variants (tersoff_params.f90:82-89):
   !> outer and bond-order cutoffs and Cmin, Cmax per element pair
   type, bind(C), public :: screening_params_t
-     real(c_double) :: or1(6), or2(6), bor1(6), bor2(6), Cmin(6), Cmax(6)
+     ! per element pair: symmetric index (6 used), ordered for Juslin (9)
+     real(c_double) :: or1(9), or2(9), bor1(9), bor2(9), Cmin(9), Cmax(9)
   end type screening_params_t
 
   public :: particles_init, particles_set_cell, particles_set_Z, particles_set_positions
   public :: neighbors_init, tersoff_init, eam_alloy_init, ewald_init, rebo2_init
   public :: kumagai_init, brenner_init
-  public :: tersoff_scr_init, kumagai_scr_init, brenner_scr_init, pme_init, juslin_init
+  public :: tersoff_scr_init, kumagai_scr_init, brenner_scr_init, pme_init, juslin_init, juslin_scr_init
   public :: set_precision, bind_to, energy_and_forces, coulomb_energy_and_forces
   public :: last_error, del
 
@@ -164,6 +165,11 @@ module atomistica_gpu
        type(c_ptr), value :: params, scr
      end function
      integer(c_int) function agpu_brenner_scr_create(pot, params, scr) bind(C, name="agpu_brenner_scr_create")
+       import :: c_int, c_ptr
+       type(c_ptr) :: pot
+       type(c_ptr), value :: params, scr
+     end function
+     integer(c_int) function agpu_juslin_scr_create(pot, params, scr) bind(C, name="agpu_juslin_scr_create")
        import :: c_int, c_ptr
        type(c_ptr) :: pot
        type(c_ptr), value :: params, scr
@@ -367,6 +373,20 @@ contains
        ierror = agpu_brenner_scr_create(this%h, c_null_ptr, c_null_ptr)
     end if
   end subroutine brenner_scr_init
+
+  !> juslin_scr_init (juslin_scr.f90): db and scr absent select the screened
+  !> Juslin W-C-H set; scr holds one entry per ordered element pair
+  subroutine juslin_scr_init(this, ierror, db, scr)
+    type(potential_gpu_t), intent(inout) :: this
+    integer, intent(out) :: ierror
+    type(juslin_params_t), optional, target, intent(in) :: db
+    type(screening_params_t), optional, target, intent(in) :: scr
+    if (present(db) .and. present(scr)) then
+       ierror = agpu_juslin_scr_create(this%h, c_loc(db), c_loc(scr))
+    else
+       ierror = agpu_juslin_scr_create(this%h, c_null_ptr, c_null_ptr)
+    end if
+  end subroutine juslin_scr_init
 
   subroutine eam_alloy_init(this, fn, elements, ierror)
     type(potential_gpu_t), intent(inout) :: this

@@ -519,6 +519,20 @@ class BrennerScr(_Screened):
         return _brenner_struct(params), nel * (nel + 1) // 2
 
 
+class JuslinScr(_Screened):
+    """JuslinScr (juslin_scr.f90); keyword arguments of Juslin plus the
+    screening parameters (or1, or2, bor1, bor2, Cmin, Cmax per ordered pair
+    j + i nel); without arguments the screened Juslin W-C-H set."""
+    kind = _lib.KIND_JUSLIN_SCR
+    _create = "agpu_juslin_scr_create"
+    _default = "Juslin_JAP_98_123520_WCH__Scr"
+
+    @staticmethod
+    def _struct(params):
+        nel = len(params["el"])
+        return _juslin_struct(params), nel * nel
+
+
 class TabulatedAlloyEAM(_Potential):
     """TabulatedAlloyEAM (tabulated_alloy_eam.f90): fn = setfl file,
     elements = filter string."""

@@ -222,6 +222,18 @@ Kuopanportti_CMS_111_525_FeCH = {
 # ---------------------------------------------------------------------------
 SCREENING_KEYS = ("or1", "or2", "bor1", "bor2", "Cmin", "Cmax")
 
+# JuslinScr: the screened W-C-H set (parameters.py of the reference and
+# juslin_params.f90 with SCREENING), per ordered pair j + i nel
+Juslin_JAP_98_123520_WCH__Scr = copy.deepcopy(Juslin_JAP_98_123520_WCH)
+Juslin_JAP_98_123520_WCH__Scr.update({
+    "or1": [3.20, 2.60, 2.68, 0.0, 1.70, 1.30, 0.0, 1.30, 1.10],
+    "or2": [3.80, 3.00, 2.96, 0.0, 2.00, 1.80, 0.0, 1.80, 1.70],
+    "bor1": [3.20, 2.60, 2.68, 0.0, 1.70, 1.30, 0.0, 1.30, 1.10],
+    "bor2": [3.80, 3.00, 2.96, 0.0, 2.00, 1.80, 0.0, 1.80, 1.70],
+    "Cmin": [1.00, 1.00, 1.00, 0.0, 1.00, 1.00, 0.0, 1.00, 1.00],
+    "Cmax": [3.00, 3.00, 3.00, 0.0, 3.00, 3.00, 0.0, 3.00, 3.00],
+})
+
 Tersoff_PRB_39_5566_Si_C__Scr = copy.deepcopy(Tersoff_PRB_39_5566_Si_C)
 Tersoff_PRB_39_5566_Si_C__Scr.update({
     "m": [3, 3, 3],

@@ -68,6 +68,7 @@ extern "C" {
 #define AGPU_KIND_BRENNER_SCR 9   /* brenner_scr.f90, classname BrennerScr */
 #define AGPU_KIND_PME 10          /* pme.f90, classname PME */
 #define AGPU_KIND_JUSLIN 11       /* juslin.f90, classname Juslin */
+#define AGPU_KIND_JUSLIN_SCR 12   /* juslin_scr.f90, classname JuslinScr */
 
 /* agpu_rebo2_create flags */
 #define AGPU_REBO2_ZERO_TABLES 1  /* rebo2 zero_tables = .true. (rebo2_module.f90:30-42) */
@@ -249,8 +250,11 @@ int agpu_juslin_create(agpu_potential **pot, const agpu_juslin_params *params);
  * SCREENING (tersoff_params.f90:99-138, kumagai_params.f90:97-132,
  * brenner_params.f90:78-113).  Rows of more than 256 neighbours within the
  * screening range return AGPU_ERR_NEB. */
+/* Indexed by the family's element pair: the symmetric pair index for
+ * Tersoff, Kumagai and Brenner (6 entries used), the ordered pair j + i*nel
+ * for Juslin (9). */
 typedef struct {
-    double or1[6], or2[6], bor1[6], bor2[6], Cmin[6], Cmax[6];
+    double or1[9], or2[9], bor1[9], bor2[9], Cmin[9], Cmax[9];
 } agpu_screening_params;
 int agpu_tersoff_scr_create(agpu_potential **pot, const agpu_tersoff_params *params,
                             const agpu_screening_params *scr);
@@ -258,6 +262,11 @@ int agpu_kumagai_scr_create(agpu_potential **pot, const agpu_kumagai_params *par
                             const agpu_screening_params *scr);
 int agpu_brenner_scr_create(agpu_potential **pot, const agpu_brenner_params *params,
                             const agpu_screening_params *scr);
+/* JuslinScr: juslin_scr_init (juslin_scr.f90, the shared bop_kernel.f90 with
+ * SCREENING); params/scr NULL = the screened Juslin W-C-H set of
+ * juslin_params.f90. */
+int agpu_juslin_scr_create(agpu_potential **pot, const agpu_juslin_params *params,
+                           const agpu_screening_params *scr);
 
 /* TabulatedAlloyEAM: tabulated_alloy_eam_init (tabulated_alloy_eam.f90:147-259)
  * reading a setfl file; elements = filter string ("*" or NULL = all). */

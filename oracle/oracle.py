@@ -139,7 +139,7 @@ class _TersoffParams(ctypes.Structure):
         [("kalpha", ctypes.c_double * 6), ("kbeta", ctypes.c_int * 6)] + \
         [(k, ctypes.c_double * 9) for k in ("bD0", "br0", "bS", "bbeta", "bgam", "bc", "bd", "bh", "bn")] + \
         [("scr", ctypes.c_int)] + \
-        [(k, ctypes.c_double * 6) for k in ("or1", "or2", "bor1", "bor2", "Cmin", "Cmax")] + \
+        [(k, ctypes.c_double * 9) for k in ("or1", "or2", "bor1", "bor2", "Cmin", "Cmax")] + \
         [("jal", ctypes.c_double * 27), ("jom", ctypes.c_double * 27), ("jm", ctypes.c_int * 27)]
 
 
@@ -156,7 +156,7 @@ def _screening(s, params):
         arr, vals = getattr(s, k), params[k]
         if not isinstance(vals, (list, tuple)):
             vals = [vals] * 6            # one-element sets give scalars
-        for i in range(6):
+        for i in range(9):               # 9: the ordered pairs of JuslinScr
             arr[i] = float(vals[i]) if i < len(vals) else 0.0
     return s
 
@@ -238,7 +238,8 @@ def juslin_symmetrized(j):
     from the reversed pair, as BIND_TO_FUNC does (juslin_module.f90:273-305)."""
     nel = len(j["el"])
     out = {k: list(v) if isinstance(v, (list, tuple)) else v for k, v in j.items()}
-    keys = ("D0", "r0", "S", "beta", "gamma", "c", "d", "h", "n", "r1", "r2")
+    keys = ("D0", "r0", "S", "beta", "gamma", "c", "d", "h", "n", "r1", "r2") + \
+        tuple(k for k in _SCR_KEYS if k in out)      # screened: juslin_module.f90:286-293
     for a in range(nel):
         for b in range(nel):
             if out["r0"][b + a * nel] < 0.0:
@@ -292,7 +293,7 @@ def tersoff(params, r, el, cell, nl, mask=None, per_at=False, nthreads=1):
     s = (_kumagai_struct(params) if fam == "kumagai" else
          _brenner_struct(params) if fam == "brenner" else
          _juslin_struct(params) if fam == "juslin" else _tersoff_struct(params, [0, 0, 0]))
-    _screening(s, params)
+    _screening(s, juslin_symmetrized(params) if fam == "juslin" else params)
     epot = ctypes.c_double(0.0)
     f = np.zeros((nat, 3))
     w = np.zeros(9)

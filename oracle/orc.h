@@ -63,7 +63,7 @@ typedef struct {
      * exp_cutoff on [r1, r2] (inner), [or1, or2] (pair), [bor1, bor2] (bond
      * order) and the screening of bonds by third atoms with Cmin, Cmax */
     int scr;
-    double or1[6], or2[6], bor1[6], bor2[6], Cmin[6], Cmax[6];
+    double or1[9], or2[9], bor1[9], bor2[9], Cmin[9], Cmax[9];   /* 9: ordered pairs (Juslin) */
     /* fam = 3: the Juslin forms (juslin_func.f90): the Brenner V_A, V_R, g,
      * b above over ordered element pairs j + i nel (PAIR_INDEX_NS, 9 pairs,
      * r1 r2 too), and h = omega exp((alpha dr)^m) per ordered triplet

@@ -90,3 +90,27 @@ def test_juslin_mixed_error():
     rms = float(np.sqrt(np.mean((f32 - f64) ** 2)))
     print("juslin mixed: F rms %.3e" % rms, flush=True)
     assert rms <= 1e-5
+
+
+JS = P.Juslin_JAP_98_123520_WCH__Scr
+
+
+def gpu_scr(prm, pos, cell, pbc, el, default=False):
+    Z = np.array([P.ATOMIC_NUMBERS[prm["el"][e]] for e in el], np.int32)
+    p = N.from_atoms(pos, cell, pbc, Z)
+    nl = N.Neighbors(300)
+    pot = N.JuslinScr() if default else N.JuslinScr(**{k: v for k, v in prm.items() if k not in ("__ref__", "family")})
+    pot.bind_to(p, nl)
+    return pot.energy_and_forces(p, nl)
+
+
+@pytest.mark.parametrize("case", ["WCH_b1", "W_bcc"])
+def test_juslin_scr_parity(case):
+    _, pos, cell, pbc, el = dict(CASES)[case]()
+    print("juslin_scr", case, len(pos), flush=True)
+    nl = O.nl_build(pos, cell, pbc, bop_cutoff(JS), avgn=300)
+    re_, rf, rw = O.tersoff(JS, pos, np.asarray(el, np.int32), cell, nl)
+    e, f, w = gpu_scr(JS, pos, cell, pbc, el)[:3]
+    assert abs(e - re_) <= REL * abs(re_) and close(f, rf) and close(w, rw)
+    e0, f0, w0 = gpu_scr(JS, pos, cell, pbc, el, default=True)[:3]
+    assert abs(e0 - e) <= REL * abs(e) and close(f0, f)

@@ -115,3 +115,25 @@ def test_fd_kuopanportti_fech():
     el[(el == 1) & (rng.random(len(pos)) < 0.3)] = 2
     pos = pos + rng.normal(0, 0.12, pos.shape)
     _fd_check(prm, pos, cell, np.array([True] * 3), el, [1, 8, 19, 44])
+
+
+# JuslinScr (juslin_scr.f90): the same kernel with SCREENING
+JS = P.Juslin_JAP_98_123520_WCH__Scr
+
+
+def test_scr_dimer_equals_unscreened():
+    # an isolated W-W dimer inside r1 = or1 = bor1: no third atom screens it and
+    # every cutoff is 1, so the screened form reduces to Juslin exactly
+    cell = np.eye(3) * 30.0
+    pos = np.array([[10.0, 10.0, 10.0], [12.6, 10.0, 10.0]])
+    pbc = np.array([False] * 3)
+    es = energy(JS, pos, cell, pbc, [0, 0])[0]
+    eu = energy(J, pos, cell, pbc, [0, 0])[0]
+    assert eu < 0 and abs(es - eu) <= 1e-12 * abs(eu)
+
+
+def test_scr_fd_wch():
+    pos, cell, pbc, el = _wch(seed=7)
+    _fd_check(JS, pos, cell, pbc, el, [0, 3, 9, 22, 37, 60])
+    # second neighbours inside the cutoffs are partly screened here
+    assert abs(energy(JS, pos, cell, pbc, el)[0] - energy(J, pos, cell, pbc, el)[0]) > 1e-3
