@@ -109,8 +109,8 @@ class Atomistica:
     # -- state -------------------------------------------------------------
     def initialize(self, atoms):
         if self.mask is not None and len(self.mask) != len(atoms):
-            raise RuntimeError("Length of mask array (= {0}) does not equal number of atoms "
-                               "(= {1}).".format(len(self.mask), len(atoms)))
+            raise RuntimeError("mask has {0} entries but the Atoms object holds {1} atoms"
+                               .format(len(self.mask), len(atoms)))
         self.particles = native.Particles()
         self.particles.allocate(len(atoms))
         self.particles.set_cell(atoms.get_cell(), atoms.get_pbc())

@@ -250,6 +250,9 @@ namespace agpu {
 // class hooks
 int tersoff_bind(agpu_potential *pot);
 int tersoff_compute(agpu_potential *pot, const agpu_eval_args *a);
+// true for a bond-order potential with screening (its records are sized on
+// the host every call, so it cannot run under stream capture)
+bool tersoff_screened(const agpu_potential *pot);
 int eam_bind(agpu_potential *pot);
 int eam_compute(agpu_potential *pot, const agpu_eval_args *a);
 // selects launch variant k (lanes per atom g, entries per lane u) when k is

@@ -311,8 +311,7 @@ extern "C" int agpu_md_run(agpu_md *md, int nsteps, int graph_steps, double *epo
         int K = 0;
         // the screened bond-order kernels size their records on the host
         // every step, so they always run step by step
-        const bool graphable = md->pot->kind != AGPU_KIND_TERSOFF_SCR && md->pot->kind != AGPU_KIND_KUMAGAI_SCR &&
-                               md->pot->kind != AGPU_KIND_BRENNER_SCR;
+        const bool graphable = md->pot->kind == AGPU_KIND_EAM_ALLOY || !tersoff_screened(md->pot);
         if (graphable && graph_steps > 0 && md->skin > 0.0) {
             const double room = 0.5 * md->skin - std::sqrt(d2);
             const double per = 1.25 * std::sqrt(v2) * md->dt;

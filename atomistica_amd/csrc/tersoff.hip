@@ -1553,6 +1553,11 @@ static int tersoff_scr_compute(agpu_potential *pot, const agpu_eval_args *a)
     return AGPU_OK;
 }
 
+bool tersoff_screened(const agpu_potential *pot)
+{
+    return pot && pot->impl && static_cast<const TersoffState *>(pot->impl)->scr;
+}
+
 int tersoff_compute(agpu_potential *pot, const agpu_eval_args *a)
 {
     TersoffState *st = static_cast<TersoffState *>(pot->impl);

@@ -227,8 +227,7 @@ class _Potential:
         if mask is not None:
             mask = np.ascontiguousarray(mask, dtype=np.int32)
             if len(mask) != nat:
-                raise RuntimeError("Length of mask array (= %d) does not equal number of atoms (= %d)."
-                                   % (len(mask), nat))
+                raise RuntimeError("mask has %d entries but the particles hold %d atoms" % (len(mask), nat))
         if epot_per_bond or f_per_bond or wpot_per_bond:
             raise AgpuError(_lib.ERR_UNSUPPORTED, "per-bond properties are not supported")
         epot = ctypes.c_double(0.0)

@@ -222,14 +222,31 @@ Kuopanportti_CMS_111_525_FeCH = {
 # ---------------------------------------------------------------------------
 SCREENING_KEYS = ("or1", "or2", "bor1", "bor2", "Cmin", "Cmax")
 
-# JuslinScr: the screened W-C-H set (parameters.py of the reference and
-# juslin_params.f90 with SCREENING), per ordered pair j + i nel
+# JuslinScr: the screened W-C-H set of the Fortran database
+# (juslin_params.f90:113-122, the SCREENING branch), per ordered pair
+# j + i nel.  The reference's Python dict of the same name
+# (src/python/atomistica/parameters.py:297-308) sets Cmin and Cmax to the
+# r1 and r2 lists instead; this set keeps the database's Cmin = 1, Cmax = 3,
+# which is what the compiled JuslinScr class uses by default.
 Juslin_JAP_98_123520_WCH__Scr = copy.deepcopy(Juslin_JAP_98_123520_WCH)
 Juslin_JAP_98_123520_WCH__Scr.update({
     "or1": [3.20, 2.60, 2.68, 0.0, 1.70, 1.30, 0.0, 1.30, 1.10],
     "or2": [3.80, 3.00, 2.96, 0.0, 2.00, 1.80, 0.0, 1.80, 1.70],
     "bor1": [3.20, 2.60, 2.68, 0.0, 1.70, 1.30, 0.0, 1.30, 1.10],
     "bor2": [3.80, 3.00, 2.96, 0.0, 2.00, 1.80, 0.0, 1.80, 1.70],
+    "Cmin": [1.00, 1.00, 1.00, 0.0, 1.00, 1.00, 0.0, 1.00, 1.00],
+    "Cmax": [3.00, 3.00, 3.00, 0.0, 3.00, 3.00, 0.0, 3.00, 3.00],
+})
+
+# screened Fe-C-H set of the Fortran database (juslin_params.f90:131-139):
+# the Fe-Fe, Fe-C and Fe-H pair and bond-order cutoffs start at 100 A, past
+# their upper end, so those pairs keep only the inner cutoff [r1, r2]
+Kuopanportti_CMS_111_525_FeCH__Scr = copy.deepcopy(Kuopanportti_CMS_111_525_FeCH)
+Kuopanportti_CMS_111_525_FeCH__Scr.update({
+    "or1": [100.0, 100.0, 100.0, 0.0, 1.70, 1.30, 0.0, 1.30, 1.10],
+    "or2": [3.35, 2.70, 2.6966, 0.0, 2.00, 1.80, 0.0, 1.80, 1.70],
+    "bor1": [100.0, 100.0, 100.0, 0.0, 1.70, 1.30, 0.0, 1.30, 1.10],
+    "bor2": [3.35, 2.70, 2.6966, 0.0, 2.00, 1.80, 0.0, 1.80, 1.70],
     "Cmin": [1.00, 1.00, 1.00, 0.0, 1.00, 1.00, 0.0, 1.00, 1.00],
     "Cmax": [3.00, 3.00, 3.00, 0.0, 3.00, 3.00, 0.0, 3.00, 3.00],
 })

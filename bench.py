@@ -16,12 +16,19 @@ import argparse
 import json
 import math
 import os
+import subprocess
 import sys
 import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# the CPUs this process may use, taken before an OpenMP runtime starts: with
+# OMP_PROC_BIND the runtime pins the initial thread to its first place
+ALLOWED_CPUS = sorted(os.sched_getaffinity(0))
+# OpenMP placement of the CPU-baseline threads (read when the runtime starts)
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
 
 METRIC = "atom-evals/s (E+F+virial), FP64 plain vs FP32/FP64-acc batched at force err≤1e-5"
 HBM_PEAK_GBS = 8000.0
@@ -155,48 +162,58 @@ class Runner:
         return time.perf_counter() - t0
 
 
-def cpu_baseline(wl, budget_s=12.0):
-    """The CPU oracle (a restatement of the reference algorithm) on the host
-    cores: neighbour-list rebuild + energy/forces/virial per step."""
-    import numpy as np
-    from oracle import oracle as O
-    from atomistica_amd import parameters as P
-    threads = max(1, min(16, os.cpu_count() or 1))
-    pos, cell, pbc = wl["pos"], wl["cell"], wl["pbc"]
-    nat = len(pos)
-    if wl["kind"] == "eam":
-        tables = O.read_setfl_tables(wl["fn"])
-        el = np.zeros(nat, np.int32)
+def host_info():
+    """lscpu model, sockets, physical cores and the CPUs this process may use."""
+    info = {"nproc": os.cpu_count(), "affinity_cpus": len(ALLOWED_CPUS)}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+    except Exception:
+        out = ""
+    kv = {}
+    for line in out.splitlines():
+        if ":" in line:
+            k, v = line.split(":", 1)
+            kv[k.strip()] = v.strip()
+    info["model"] = kv.get("Model name")
+    try:
+        sockets = int(kv.get("Socket(s)", "1"))
+        cps = int(kv.get("Core(s) per socket", "0"))
+        info["sockets"] = sockets
+        info["physical_cores"] = sockets * cps if cps else None
+        info["threads_per_core"] = int(kv.get("Thread(s) per core", "1"))
+    except ValueError:
+        info["physical_cores"] = None
+    return info
 
-        def one():
-            nl = O.nl_build(pos, cell, pbc, tables["cutoff"], nthreads=threads)
-            O.eam(tables, pos, el, cell, nl, nthreads=threads)
-        sample_atoms = nat
-    elif wl["kind"] == "tersoff":
-        el = np.ones(nat, np.int32)
 
-        def one():
-            nl = O.nl_build(pos, cell, pbc, 3.0, nthreads=threads)
-            O.tersoff(P.Tersoff_PRB_39_5566_Si_C, pos, el, cell, nl, nthreads=threads)
-        sample_atoms = nat
-    elif wl["kind"] == "tersoff_scr":
-        el = np.ones(nat, np.int32)
-        prm = P.Tersoff_PRB_39_5566_Si_C__Scr
-        rc = 6.0 * np.sqrt(9.0 / 8.0)
+def baseline_threads(info):
+    """All physical cores this process may run on; OMP_NUM_THREADS, when the
+    environment sets it (the CPU share of the GPU's host), caps it."""
+    n = min(x for x in (info.get("physical_cores"), info["affinity_cpus"]) if x)
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(1, n)
 
-        def one():
-            nl = O.nl_build(pos, cell, pbc, rc, avgn=80, nthreads=threads)
-            O.tersoff(prm, pos, el, cell, nl, nthreads=threads)
-        sample_atoms = nat
-    elif wl["kind"] == "rebo2":
-        R = O.Rebo2(dihedral=wl.get("dihedral", False))
 
-        def one():
-            nl = O.nl_build(pos, cell, pbc, 2.0, nthreads=threads)
-            R(pos, wl["Z"], cell, nl, nthreads=threads)
-        sample_atoms = nat
-    else:
-        return None
+_ORACLE_BUILD = None
+
+
+def native_oracle():
+    """The oracle compiled on this host with -O3 -march=native -fopenmp
+    (-ffp-contract=off kept), loaded before its first use."""
+    global _ORACLE_BUILD
+    if _ORACLE_BUILD is None:
+        from oracle import oracle as O
+        try:
+            O.use_library(O.build_native(tempfile.mkdtemp()))
+            _ORACLE_BUILD = "gcc -O3 -march=native -fopenmp -ffp-contract=off"
+        except Exception as e:           # keep the prebuilt -O2 oracle
+            _ORACLE_BUILD = "prebuilt oracle/_build (-O2); native build failed: %s" % type(e).__name__
+    return _ORACLE_BUILD
+
+
+def _time_calls(one, budget_s):
     t0 = time.perf_counter()
     one()                      # warm-up (page faults, thread pool)
     warm = time.perf_counter() - t0
@@ -204,11 +221,58 @@ def cpu_baseline(wl, budget_s=12.0):
     t0 = time.perf_counter()
     for _ in range(n):
         one()
-    dt = time.perf_counter() - t0
-    return {"value": sample_atoms * n / dt, "unit": "atom-evals/s", "cores": threads,
-            "kind": "port",
-            "sample": "%d full steps (NL rebuild + E+F+virial) of the same %d-atom system, "
-                      "OpenMP %d threads, C restatement of the reference loops" % (n, sample_atoms, threads)}
+    return n, time.perf_counter() - t0
+
+
+def cpu_baseline(wl, budget_s=8.0):
+    """The CPU oracle (a C restatement of the reference loops) on the host
+    cores, on all physical cores available and on one thread: neighbour-list
+    rebuild + energy/forces/virial per call, the same system as the GPU."""
+    import numpy as np
+    build = native_oracle()
+    from oracle import oracle as O
+    from atomistica_amd import parameters as P
+    info = host_info()
+    pos, cell, pbc = wl["pos"], wl["cell"], wl["pbc"]
+    nat = len(pos)
+    if wl["kind"] == "eam":
+        tables = O.read_setfl_tables(wl["fn"])
+        el = np.zeros(nat, np.int32)
+
+        def one(t):
+            nl = O.nl_build(pos, cell, pbc, tables["cutoff"], nthreads=t)
+            O.eam(tables, pos, el, cell, nl, nthreads=t)
+    elif wl["kind"] == "tersoff":
+        el = np.ones(nat, np.int32)
+
+        def one(t):
+            nl = O.nl_build(pos, cell, pbc, 3.0, nthreads=t)
+            O.tersoff(P.Tersoff_PRB_39_5566_Si_C, pos, el, cell, nl, nthreads=t)
+    elif wl["kind"] == "tersoff_scr":
+        el = np.ones(nat, np.int32)
+        prm = P.Tersoff_PRB_39_5566_Si_C__Scr
+        rc = 6.0 * np.sqrt(9.0 / 8.0)
+
+        def one(t):
+            nl = O.nl_build(pos, cell, pbc, rc, avgn=80, nthreads=t)
+            O.tersoff(prm, pos, el, cell, nl, nthreads=t)
+    elif wl["kind"] == "rebo2":
+        R = O.Rebo2(dihedral=wl.get("dihedral", False))
+
+        def one(t):
+            nl = O.nl_build(pos, cell, pbc, 2.0, nthreads=t)
+            R(pos, wl["Z"], cell, nl, nthreads=t)
+    else:
+        return None
+    threads = baseline_threads(info)
+    n, dt = _time_calls(lambda: one(threads), budget_s)
+    n1, dt1 = _time_calls(lambda: one(1), budget_s)
+    what = "full steps (NL rebuild + E+F+virial) of the same %d-atom system" % nat
+    return {"value": nat * n / dt, "unit": "atom-evals/s", "cores": threads, "kind": "port",
+            "sample": "%d %s, OpenMP %d threads (OMP_PROC_BIND=close, OMP_PLACES=cores); C restatement of "
+                      "the reference loops, not the reference binary" % (n, what, threads),
+            "one_thread": {"value": nat * n1 / dt1, "cores": 1, "sample": "%d %s, 1 thread" % (n1, what)},
+            "build": build, "host": info}
 
 
 def valu_rates(workload, kern):
@@ -252,7 +316,7 @@ def _kernel_times(N, fn, steps):
     return kern
 
 
-def bench_batch(args):
+def batch_block(args, cpu=True):
     """Batched Tersoff for fitting: 4096 periodic Si structures of 8..512
     atoms (log-uniform), random triclinic cells at 2.33 g/cm^3 +-10 %, random
     positions at minimum distance 2.0 A.  A step is one fitting step: forward
@@ -299,6 +363,19 @@ def bench_batch(args):
     step = fwd32 + bwd
     kern = _kernel_times(N, lambda: (b.forward(th, "fp64"), b.forward(th, "mixed"), b.backward(th, ge, gf)),
                          args.steps)
+    npairs = b.pairs()
+    nn = npairs / ntot
+    roof = None
+    dom = "tersoff_batch_fp64"
+    if dom in kern:
+        bpa = 56 + 8 * nn
+        achieved = bpa * ntot / (kern[dom]["avg_us"] * 1e-6) / 1e9
+        roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
+                "algorithmic_bytes_per_atom": bpa, "atoms_per_launch": ntot}
+    valu = valu_rates("tersoff_batch_4096", kern)
+    if roof and valu and dom in valu:
+        roof["valu_frac_fp64"] = valu[dom]["frac_fp64_peak"]
     out = {
         "metric": METRIC,
         "value": ntot / step,
@@ -309,7 +386,7 @@ def bench_batch(args):
         "dtype": "f32-arith/f64-acc", "data": "synthetic",
         "config": {"workload": "batched Tersoff Si fitting: 4096 periodic structures, %d atoms, %d pairs "
                                "(log-uniform 8..512 atoms, triclinic, 2.33 g/cm^3 +-10%%, dmin 2.0 A, rng 4)"
-                               % (ntot, b.pairs()),
+                               % (ntot, npairs),
                    "step": "forward E+F (FP32-arith/FP64-acc) + backward d(loss)/dtheta, 11 parameters",
                    "inputs": "positions and neighbour list resident in HBM"},
         "forward_fp64": {"ms": 1e3 * fwd64, "value": ntot / fwd64},
@@ -321,29 +398,60 @@ def bench_batch(args):
         "first_forward_ms_incl_list": 1e3 * t_first,
         "structure_generation_s": t_gen,
         "kernels": kern,
-        "roofline": None,
-        "valu": valu_rates("tersoff_batch_4096", kern),
+        "roofline": roof,
+        "valu": valu,
     }
-    if not args.no_cpu_baseline:
-        # the oracle (FP64 forward: list build + E + F), structure by
-        # structure on one core, over as many structures as ~12 s allow
-        from oracle import oracle as O
-        from atomistica_amd.batch import params_from_theta
-        prm = params_from_theta(th)
+    if cpu:
+        out["cpu_baseline"] = cpu_baseline_batch(S, th)
+    del b
+    torch.cuda.empty_cache()
+    return out
+
+
+def cpu_baseline_batch(S, th, budget_s=8.0):
+    """The oracle's FP64 forward (list build + E + F) structure by structure,
+    on all available physical cores (one structure per worker) and on one
+    thread, over as many structures as the budget allows."""
+    import numpy as np
+    from concurrent.futures import ThreadPoolExecutor
+    build = native_oracle()
+    from oracle import oracle as O
+    from atomistica_amd.batch import params_from_theta
+    prm = params_from_theta(th)
+    info = host_info()
+    threads = baseline_threads(info)
+
+    def one(k):
+        # a pool thread inherits the pinned affinity of the initial thread
+        os.sched_setaffinity(0, ALLOWED_CPUS)
+        pos, cell = S[k]
+        nl = O.nl_build(pos, cell, [True] * 3, th[10])
+        O.tersoff(prm, pos, np.zeros(len(pos), np.int32), cell, nl)
+        return len(pos)
+
+    def run(workers):
         t0 = time.perf_counter()
         na = ns = 0
-        for pos, cell in S:
-            nl = O.nl_build(pos, cell, [True] * 3, th[10])
-            O.tersoff(prm, pos, np.zeros(len(pos), np.int32), cell, nl)
-            na += len(pos)
-            ns += 1
-            if time.perf_counter() - t0 > 12.0:
-                break
-        dt = time.perf_counter() - t0
-        out["cpu_baseline"] = {"value": na / dt, "unit": "atom-evals/s", "cores": 1, "kind": "port",
-                               "sample": "FP64 forward (list build + E + F) of the first %d of the 4096 structures "
-                                         "(%d atoms), one core, C restatement of the reference loops" % (ns, na)}
-    print(json.dumps(out))
+        with ThreadPoolExecutor(max_workers=workers) as ex:
+            while ns < len(S) and time.perf_counter() - t0 < budget_s:
+                chunk = list(range(ns, min(len(S), ns + 8 * workers)))
+                na += sum(ex.map(one, chunk))
+                ns += len(chunk)
+        return na, ns, time.perf_counter() - t0
+
+    na, ns, dt = run(threads)
+    na1, ns1, dt1 = run(1)
+    return {"value": na / dt, "unit": "atom-evals/s", "cores": threads, "kind": "port",
+            "sample": "FP64 forward (list build + E + F) of the first %d of the 4096 structures (%d atoms), "
+                      "%d worker threads, one structure each (the ctypes calls release the GIL); "
+                      "C restatement of the reference loops" % (ns, na, threads),
+            "one_thread": {"value": na1 / dt1, "cores": 1,
+                           "sample": "the first %d structures (%d atoms), one thread" % (ns1, na1)},
+            "build": build, "host": info}
+
+
+def bench_batch(args):
+    print(json.dumps(batch_block(args, cpu=not args.no_cpu_baseline)))
 
 
 def bench_ewald_batch(args):
@@ -447,6 +555,141 @@ def bench_md(args):
     print(json.dumps(out))
 
 
+def roofline(workload, kern, nn, nat):
+    """HBM roofline of the dominant kernel (largest total time): algorithmic
+    bytes per launch over its measured average duration, the counter traffic
+    of the committed rocprofv3 summary, and its FP64 VALU fraction."""
+    if not kern:
+        return None
+    dom = max(kern, key=lambda k: kern[k]["avg_us"] * kern[k]["launches"])
+    bpa = algorithmic_bytes(dom, nn)
+    if bpa is None:
+        return None
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tf):
+        try:
+            ent = json.load(open(tf)).get(workload, {}).get(dom)
+            if ent:
+                traffic = ent["bytes_per_launch"]
+        except ValueError:
+            traffic = None
+    achieved = bpa * nat / (kern[dom]["avg_us"] * 1e-6) / 1e9
+    roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
+            "algorithmic_bytes_per_atom": bpa, "atoms_per_launch": nat}
+    if traffic:
+        roof["traffic_over_algorithmic"] = traffic / (bpa * nat)
+    valu = valu_rates(workload, kern)
+    if valu and dom in valu:
+        roof["valu_frac_fp64"] = valu[dom]["frac_fp64_peak"]
+        roof["valu_frac_fp32"] = valu[dom]["frac_fp32_peak"]
+    return roof
+
+
+# algorithmic bytes of one atom-evaluation of the whole E+F+virial pass
+# (positions, species, row offset, the list, forces)
+def step_bytes(kind, nn):
+    return (72 if kind == "eam" else 56) + 8 * nn
+
+
+def potential_block(workload, args, precision="fp64", autotune=True, cpu=True, mixed=True):
+    """One workload through the potential class: full steps (list rebuilt),
+    force-only steps, per-kernel HIP-event times, roofline, the mixed form
+    against FP64 and the CPU baseline."""
+    import numpy as np
+    import torch
+    from atomistica_amd import native as N
+    wl = build_workload(workload)
+    if args.shuffle:
+        perm = np.random.default_rng(12345).permutation(len(wl["pos"]))
+        wl["pos"] = wl["pos"][perm].copy()
+        wl["Z"] = wl["Z"][perm].copy()
+        if "q" in wl:
+            wl["q"] = wl["q"][perm].copy()
+        wl["desc"] += "; atoms randomly permuted (rng 12345)"
+    run = Runner(wl, precision)
+    nat = run.nat
+    for _ in range(args.warmup):
+        run.step()
+    tune = None
+    if autotune:
+        # launch-shape autotuner (warm-up, outside the timed region)
+        rep = run.pot.autotune(run.p, run.nl, reps=5)
+        tune = {"report": rep.strip().splitlines(),
+                "chosen": [ln.strip() for ln in rep.splitlines() if "->" in ln]}
+        for _ in range(2):
+            run.step()
+    dt = run.timed(args.steps)
+    value = nat * args.steps / dt
+    total, maxrow = run.nl.get_size()
+    nn = total / nat
+    # force pass with the list reused
+    dt_force = run.timed(args.steps, rebuild=False)
+    f64 = run.f.cpu().numpy().copy()
+    e64 = float(run.epot.cpu()[0])
+    # profiled pass: per-kernel HIP-event durations on the launch stream
+    kern = _kernel_times(N, run.step, args.steps)
+    B = step_bytes(wl["kind"], nn)
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "atom-evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * dt / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if precision == "fp64" else "f32-arith/f64-acc",
+        "data": "synthetic",
+        "config": {"workload": wl["desc"], "atoms": nat, "neighbors_per_atom": nn,
+                   "step": "neighbour-list rebuild + energy, forces, virial",
+                   "inputs": "positions resident in HBM"},
+        "force_only": {"value": nat * args.steps / dt_force, "ms_per_step": 1e3 * dt_force / args.steps},
+        "step_hbm": {"algorithmic_bytes_per_atom": B, "roofline_rate": HBM_PEAK_GBS * 1e9 / B,
+                     "frac_full_step": value * B / (HBM_PEAK_GBS * 1e9),
+                     "frac_force_only": nat * args.steps / dt_force * B / (HBM_PEAK_GBS * 1e9)},
+        "kernels": kern,
+        "roofline": roofline(workload, kern, nn, nat),
+        "valu": valu_rates(workload, kern),
+        "autotune": tune,
+    }
+    if mixed and precision == "fp64" and wl["kind"] in ("eam", "tersoff", "tersoff_scr", "ewald", "rebo2"):
+        try:
+            run.pot.set_precision("mixed")
+        except Exception as e:        # a class without the mixed form
+            out["mixed"] = {"unsupported": str(e)}
+        else:
+            for _ in range(args.warmup):
+                run.step()
+            dtm = run.timed(args.steps)
+            dtmf = run.timed(args.steps, rebuild=False)
+            fm = run.f.cpu().numpy()
+            em = float(run.epot.cpu()[0])
+            out["mixed"] = {
+                "value": nat * args.steps / dtm, "ms_per_step": 1e3 * dtm / args.steps,
+                "force_only_ms": 1e3 * dtmf / args.steps,
+                "dtype": "f32-arith/f64-acc",
+                "speedup_vs_fp64": (nat * args.steps / dtm) / value,
+                "force_only_speedup_vs_fp64": dt_force / dtmf,
+                "force_rms_err": float(np.sqrt(np.mean((fm - f64) ** 2))),
+                "force_max_err": float(np.abs(fm - f64).max()),
+                "energy_err_per_atom": abs(em - e64) / nat,
+            }
+            run.pot.set_precision("fp64")
+    if cpu:
+        out["cpu_baseline"] = cpu_baseline(wl)
+    del run
+    torch.cuda.empty_cache()
+    return out
+
+
+# the north-star cases carried in the default line next to the headline
+EXTRA_BLOCKS = ("tersoff_si_1m", "rebo2_ch_100k", "tersoff_batch_4096")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -456,6 +699,12 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="only the named workload (the default line also carries the Tersoff 10^6, "
+                         "REBO2 10^5 and batched-Tersoff blocks)")
+    ap.add_argument("--shuffle", action="store_true",
+                    help="randomly permute the input atoms (the library's own bin ordering must "
+                         "make the step independent of the caller's order)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch shapes instead of timing the variants in warm-up")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -477,108 +726,20 @@ def main():
         bench_md(args)
         return
 
-    import numpy as np
     import torch
-    from atomistica_amd import native as N
-
-    wl = build_workload(args.workload)
-    run = Runner(wl, args.precision)
-    nat = run.nat
-    for _ in range(args.warmup):
-        run.step()
-    tune = None
-    if not args.no_autotune:
-        # launch-shape autotuner (warm-up, outside the timed region)
-        rep = run.pot.autotune(run.p, run.nl, reps=5)
-        tune = {"report": rep.strip().splitlines(),
-                "chosen": [ln.strip() for ln in rep.splitlines() if "->" in ln]}
-        for _ in range(2):
-            run.step()
-    dt = run.timed(args.steps)
-    value = nat * args.steps / dt
-    total, maxrow = run.nl.get_size()
-    nn = total / nat
-
-    # force pass with the list reused
-    dt_force = run.timed(args.steps, rebuild=False)
-    f64 = run.f.cpu().numpy().copy()
-    e64 = float(run.epot.cpu()[0])
-    w64 = run.wpot.cpu().numpy().copy()
-
-    # profiled pass: per-kernel HIP-event durations on the launch stream
-    N.profile_reset()
-    N.profile_enable(True)
-    for _ in range(args.steps):
-        run.step()
-    torch.cuda.synchronize()
-    N.profile_enable(False)
-    kern = {}
-    for k in N.profile_list():
-        ms, cnt = N.profile_query(k)
-        if cnt:
-            kern[k] = {"avg_us": 1e3 * ms / cnt, "launches": cnt}
-    dom = max(kern, key=lambda k: kern[k]["avg_us"] * kern[k]["launches"]) if kern else None
-    roof = None
-    if dom:
-        bpa = algorithmic_bytes(dom, nn)
-        traffic = None
-        tf = os.path.join(ROOT, "profiles", "traffic.json")
-        if os.path.exists(tf):
-            try:
-                tj = json.load(open(tf))
-                ent = tj.get(args.workload, {}).get(dom)
-                if ent:
-                    traffic = ent["bytes_per_launch"]
-            except Exception:
-                traffic = None
-        if bpa is not None:
-            achieved = bpa * nat / (kern[dom]["avg_us"] * 1e-6) / 1e9
-            roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
-                    "algorithmic_bytes_per_atom": bpa, "atoms_per_launch": nat}
-
-    out = {
-        "metric": METRIC,
-        "value": value,
-        "unit": "atom-evals/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": 1e3 * dt / args.steps,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f32-arith/f64-acc",
-        "data": "synthetic",
-        "config": {"workload": wl["desc"], "atoms": nat, "neighbors_per_atom": nn,
-                   "step": "neighbour-list rebuild + energy, forces, virial (FP64)",
-                   "inputs": "positions resident in HBM"},
-        "force_only": {"value": nat * args.steps / dt_force, "ms_per_step": 1e3 * dt_force / args.steps},
-        "kernels": kern,
-        "roofline": roof,
-        "valu": valu_rates(args.workload, kern),
-        "autotune": tune,
-    }
-
-    if not args.no_mixed and args.precision == "fp64" and wl["kind"] in ("eam", "tersoff", "tersoff_scr", "ewald"):
-        run.pot.set_precision("mixed")
-        for _ in range(args.warmup):
-            run.step()
-        dtm = run.timed(args.steps)
-        fm = run.f.cpu().numpy()
-        em = float(run.epot.cpu()[0])
-        out["mixed"] = {
-            "value": nat * args.steps / dtm, "ms_per_step": 1e3 * dtm / args.steps,
-            "dtype": "f32-arith/f64-acc",
-            "speedup_vs_fp64": (nat * args.steps / dtm) / value,
-            "force_rms_err": float(np.sqrt(np.mean((fm - f64) ** 2))),
-            "force_max_err": float(np.abs(fm - f64).max()),
-            "energy_err_per_atom": abs(em - e64) / nat,
-        }
-        run.pot.set_precision("fp64")
-
-    if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(wl)
+    out = potential_block(args.workload, args, precision=args.precision, autotune=not args.no_autotune,
+                          cpu=not args.no_cpu_baseline, mixed=not args.no_mixed)
+    if args.workload == "eam_cu_256k" and not args.no_extra:
+        for name in EXTRA_BLOCKS:
+            print("bench: %s" % name, file=sys.stderr, flush=True)
+            if name == "tersoff_batch_4096":
+                blk = batch_block(args, cpu=not args.no_cpu_baseline)
+            else:
+                blk = potential_block(name, args, autotune=not args.no_autotune, cpu=not args.no_cpu_baseline,
+                                      mixed=not args.no_mixed)
+            for k in ("metric", "n_gpus", "higher_is_better", "scaling", "vs_baseline", "data", "warmup"):
+                blk.pop(k, None)
+            out[name] = blk
     if args.copy_calibration:
         a = torch.ones(64 * 1024 * 1024, dtype=torch.float64, device="cuda")
         b = torch.empty_like(a)

@@ -38,6 +38,27 @@ def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
+def use_library(path):
+    """Load the oracle from another build of the same sources (the CPU
+    baseline's -O3 -march=native build); must come before the first call."""
+    global _lib, _LIBPATH
+    if _lib is not None and _LIBPATH != path:
+        raise OracleError("oracle library already loaded from " + _LIBPATH)
+    _LIBPATH = path
+
+
+def build_native(outdir):
+    """Compile the oracle sources with -O3 -march=native (keeping
+    -ffp-contract=off) into outdir; returns the library path."""
+    import glob
+    import subprocess
+    out = os.path.join(outdir, "liboracle_native.so")
+    srcs = sorted(glob.glob(os.path.join(_HERE, "orc_*.c")))
+    subprocess.run(["gcc", "-O3", "-march=native", "-fPIC", "-fopenmp", "-ffp-contract=off", "-std=c11",
+                    "-shared", "-o", out] + srcs + ["-lm"], check=True, capture_output=True)
+    return out
+
+
 def lib():
     global _lib
     if _lib is None:

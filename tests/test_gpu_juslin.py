@@ -114,3 +114,36 @@ def test_juslin_scr_parity(case):
     assert abs(e - re_) <= REL * abs(re_) and close(f, rf) and close(w, rw)
     e0, f0, w0 = gpu_scr(JS, pos, cell, pbc, el, default=True)[:3]
     assert abs(e0 - e) <= REL * abs(e) and close(f0, f)
+
+
+def test_juslin_scr_screening_changes_result_and_mixed_error():
+    """On rattled W-C-H the screened class differs from the unscreened one
+    (the partially screened branch, the one that writes screening records,
+    is taken), and its FP32-arithmetic / FP64-accumulate form keeps the
+    force RMS within 1e-5 eV/A of its FP64 form."""
+    _, pos, cell, pbc, el = _wch(seed=5)
+    e_j = gpu(J, pos, cell, pbc, el)[0]
+    Z = np.array([P.ATOMIC_NUMBERS[JS["el"][e]] for e in el], np.int32)
+    p = N.from_atoms(pos, cell, pbc, Z)
+    nl = N.Neighbors(300)
+    pot = N.JuslinScr()
+    pot.bind_to(p, nl)
+    e64, f64 = pot.energy_and_forces(p, nl)[:2]
+    pot.set_precision("mixed")
+    e32, f32 = pot.energy_and_forces(p, nl)[:2]
+    rms = float(np.sqrt(np.mean((f32 - f64) ** 2)))
+    print("juslin_scr: E %.8f vs Juslin %.8f; mixed F rms %.3e" % (e64, e_j, rms), flush=True)
+    assert abs(e64 - e_j) > 1e-3
+    assert rms <= 1e-5 and abs(e32 - e64) / len(pos) <= 1e-5
+
+
+def test_juslin_scr_kuopanportti_parity():
+    """The screened Fe-C-H set, whose Fe pairs have or1 = bor1 = 100 A past
+    or2: those pairs keep the inner cutoff only."""
+    KS = P.Kuopanportti_CMS_111_525_FeCH__Scr
+    _, pos, cell, pbc, el = _wch(KS, a=4.0, seed=6)
+    nl = O.nl_build(pos, cell, pbc, bop_cutoff(KS), avgn=300)
+    re_, rf, rw = O.tersoff(KS, pos, np.asarray(el, np.int32), cell, nl)
+    e, f, w = gpu_scr(KS, pos, cell, pbc, el)[:3]
+    print("juslin_scr kuopanportti: E %.12f (oracle %.12f)" % (e, re_), flush=True)
+    assert abs(e - re_) <= REL * abs(re_) and close(f, rf) and close(w, rw)

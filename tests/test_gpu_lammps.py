@@ -2,11 +2,26 @@
 
 A periodic system is handed over the way LAMMPS does it
 (pair_atomistica.cpp:172-561): the local atoms followed by ghost copies of
-every atom within the border of the box, no periodicity.  After the call the
-ghost forces are returned to their owners (LAMMPS' reverse communication,
-newton on).  Energy, forces and virial must then equal the periodic
-evaluation of the same system within 1e-10 relative.
+every atom within the border of the box, no periodicity.  Every case is
+anchored to the oracle twice:
+
+* the raw step (forces on local and ghost atoms, energy, per-atom energies
+  and virials before any communication) equals the oracle's masked
+  evaluation of the same open local+ghost set, the mask marking the local
+  atoms as pair_atomistica.cpp:520-545 does (Tersoff, EAM);
+* after LAMMPS' reverse communication (ghost forces, energies and virials
+  returned to their owners, newton on) the result equals the oracle's
+  periodic evaluation of the system, in LAMMPS' Voigt order and sign
+  (pair_atomistica.cpp:554-559, macros.inc:202).
+
+REBO2 runs with the tag(j) >= tag(i) half rule (bop_kernel_rebo2.f90:
+1328-1329), which the oracle has no open-box counterpart for; it is anchored
+to the periodic oracle after reverse communication.  The periodic GPU path
+is kept as an extra check.
 """
+import os
+import tempfile
+
 import numpy as np
 import pytest
 
@@ -14,6 +29,7 @@ from atomistica_amd import native as N
 from atomistica_amd import parameters as P
 from atomistica_amd import workloads as W
 from atomistica_amd._lib import AgpuError
+from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 
@@ -42,27 +58,76 @@ def ghosts(pos, cell, border):
     return np.concatenate(xs), np.concatenate(owner)
 
 
-def run_lammps(pot, pos, cell, Z, border):
-    gx, gown = ghosts(pos, cell, border)
-    nlocal = len(pos)
-    x = np.concatenate([pos, gx])
-    Zall = np.concatenate([Z, Z[gown]]).astype(np.int32)
-    p = N.Particles()
-    nl = N.Neighbors(200)
-    eng, f, vir, _ = N.lammps_compute(pot, p, nl, nlocal, x, Zall)
-    floc = f[:nlocal].copy()
-    np.add.at(floc, gown, f[nlocal:])           # reverse communication
-    return eng, floc, vir, len(gx)
+def voigt(w):
+    """LAMMPS global virial of a 3x3 wpot: -(xx, yy, zz, xy, xz, yz), the
+    off-diagonals symmetrised (pair_atomistica.cpp:554-559)."""
+    return -np.array([w[0, 0], w[1, 1], w[2, 2], 0.5 * (w[0, 1] + w[1, 0]), 0.5 * (w[0, 2] + w[2, 0]),
+                      0.5 * (w[1, 2] + w[2, 1])])
 
 
-def periodic(pot, pos, cell, Z):
-    p = N.from_atoms(pos, cell, np.array([True] * 3), Z)
-    nl = N.Neighbors(200)
-    pot.bind_to(p, nl)
-    e, f, w, *_ = pot.energy_and_forces(p, nl)
-    vir = -np.array([w[0, 0], w[1, 1], w[2, 2], 0.5 * (w[0, 1] + w[1, 0]), 0.5 * (w[0, 2] + w[2, 0]),
-                     0.5 * (w[1, 2] + w[2, 1])])
-    return e, f, vir
+def vatom_of(wpa):
+    """LAMMPS vatom of per-atom 3x3 virials: -(W_xx, W_yy, W_zz, W_yx, W_zx,
+    W_zy), SUM_VIRIAL in LAMMPS mode (macros.inc:202)."""
+    return -np.stack([wpa[:, 0, 0], wpa[:, 1, 1], wpa[:, 2, 2], wpa[:, 1, 0], wpa[:, 2, 0], wpa[:, 2, 1]], axis=1)
+
+
+def reverse(arr, nlocal, gown):
+    out = arr[:nlocal].copy()
+    np.add.at(out, gown, arr[nlocal:])
+    return out
+
+
+class Case:
+    """One potential as both sides see it: a GPU class factory and the
+    oracle call over a neighbour list of the matching cut-off."""
+
+    def __init__(self, kind, fn=None):
+        self.kind = kind
+        self.fn = fn
+        if kind == "tersoff":
+            self.params = P.Tersoff_PRB_39_5566_Si_C
+            self.rc = max(self.params["r2"])
+        else:
+            self.tables = O.read_setfl_tables(fn)
+            self.rc = self.tables["cutoff"]
+
+    def gpu(self):
+        return N.Tersoff() if self.kind == "tersoff" else N.TabulatedAlloyEAM(fn=self.fn)
+
+    def el(self, Z):
+        if self.kind == "tersoff":
+            m = {P.ATOMIC_NUMBERS[s]: k for k, s in enumerate(self.params["el"])}
+        else:
+            m = {z: k for k, z in enumerate(self.tables["Z"])}
+        return np.array([m.get(int(z), -1) for z in Z], np.int32)
+
+    def oracle(self, pos, cell, pbc, Z, mask=None):
+        nl = O.nl_build(pos, cell, pbc, self.rc, avgn=200)
+        if self.kind == "tersoff":
+            return O.tersoff(self.params, pos, self.el(Z), cell, nl, mask=mask, per_at=True)
+        return O.eam(self.tables, pos, self.el(Z), cell, nl, mask=mask, per_at=True)
+
+    def oracle_periodic(self, pos, cell, Z):
+        e, f, w, epa, wpa = self.oracle(pos, cell, [True] * 3, Z)
+        return e, f, voigt(w), epa, vatom_of(wpa)
+
+    def oracle_open(self, x, Z, nlocal):
+        """The open local+ghost set in a box with a margin (the non-periodic
+        list does not depend on where the box sits), local atoms masked."""
+        lo, hi = x.min(axis=0), x.max(axis=0)
+        margin = 1.0 + self.rc
+        cell = np.diag(hi - lo + 2 * margin)
+        mask = (np.arange(len(x)) < nlocal).astype(np.int32)
+        e, f, w, epa, wpa = self.oracle(x - lo + margin, cell, [False] * 3, Z, mask=mask)
+        return e, f, voigt(w), epa, vatom_of(wpa)
+
+
+def fs_case(nr=4000):
+    d = tempfile.mkdtemp()
+    fn = os.path.join(d, "fs_cu.eam.alloy")
+    with open(fn, "w") as fh:
+        fh.write(W.fs_cu_setfl_text(nr=nr, nrho=nr))
+    return Case("eam", fn)
 
 
 def _wrap(pos, cell):
@@ -70,27 +135,23 @@ def _wrap(pos, cell):
     return pos - np.floor(pos / L) * L
 
 
-def test_lammps_tersoff_sic():
+def sic_system():
     pos, cell, pbc, Z = W.diamond_si(3, a=4.36, sigma=0.06, seed=11)
     Z = np.where(np.arange(len(pos)) % 8 >= 4, 14, 6).astype(np.int32)
-    pos = _wrap(pos, cell)
-    e0, f0, v0 = periodic(N.Tersoff(), pos, cell, Z)
-    e1, f1, v1, ng = run_lammps(N.Tersoff(), pos, cell, Z, 2 * 3.0 + 0.5)
-    print("lammps tersoff: %d local, %d ghosts, dE %.3e" % (len(pos), ng, e1 - e0), flush=True)
-    assert abs(e1 - e0) <= REL * abs(e0)
-    assert close(f1, f0) and close(v1, v0)
+    return _wrap(pos, cell), cell, Z
 
 
-def test_lammps_eam(tmp_path):
-    fn = tmp_path / "fs_cu.eam.alloy"
-    fn.write_text(W.fs_cu_setfl_text(nr=4000, nrho=4000))
+def cu_system():
     pos, cell, pbc, Z = W.fcc_cu(4, sigma=0.08, seed=3)
-    pos = _wrap(pos, cell)
-    e0, f0, v0 = periodic(N.TabulatedAlloyEAM(fn=str(fn)), pos, cell, Z)
-    e1, f1, v1, ng = run_lammps(N.TabulatedAlloyEAM(fn=str(fn)), pos, cell, Z, 5.5 + 0.5)
-    print("lammps eam: %d local, %d ghosts, dE %.3e" % (len(pos), ng, e1 - e0), flush=True)
-    assert abs(e1 - e0) <= REL * abs(e0)
-    assert close(f1, f0) and close(v1, v0)
+    return _wrap(pos, cell), cell, Z
+
+
+def periodic_gpu(pot, pos, cell, Z):
+    p = N.from_atoms(pos, cell, np.array([True] * 3), Z)
+    nl = N.Neighbors(200)
+    pot.bind_to(p, nl)
+    e, f, w, *_ = pot.energy_and_forces(p, nl)
+    return e, f, voigt(w)
 
 
 def lammps_full_list(x, rc, seed=0):
@@ -116,47 +177,92 @@ def lammps_full_list(x, rc, seed=0):
     return seed_, np.array([len(r) for r in rows], np.int32), np.concatenate(neigh)
 
 
-def run_lammps_list(pot, pos, cell, Z, border, rc_list):
+def run_step(case, pos, cell, Z, border, skin=None):
+    """One LAMMPS step over the local atoms plus ghosts, over a list built
+    here (skin None) or LAMMPS' own full list with that skin; per-atom
+    energies and virials requested.  Returns the raw arrays on all atoms."""
     gx, gown = ghosts(pos, cell, border)
     nlocal = len(pos)
     x = np.concatenate([pos, gx])
     Zall = np.concatenate([Z, Z[gown]]).astype(np.int32)
-    seed, numneigh, neigh = lammps_full_list(x, rc_list)
-    p = N.Particles()
-    nl = N.Neighbors(200)
-    eng, f, vir, _ = N.lammps_compute_list(pot, p, nl, nlocal, x, Zall, seed, numneigh, neigh)
-    floc = f[:nlocal].copy()
-    np.add.at(floc, gown, f[nlocal:])
-    return eng, floc, vir
+    tag = np.concatenate([np.arange(nlocal), gown]).astype(np.int32) + 1
+    kw = {}
+    if skin is not None:
+        kw = dict(zip(("seed", "numneigh", "neigh"), lammps_full_list(x, case.rc + skin)))
+    eng, f, vir, ea, va = N.lammps_compute_tagged(case.gpu(), N.Particles(), N.Neighbors(200), nlocal, x, Zall,
+                                                  tag, eatom=True, vatom=True, **kw)
+    return dict(x=x, Z=Zall, gown=gown, nlocal=nlocal, e=eng, f=f, vir=vir, ea=ea, va=va)
+
+
+def check_against_oracle(case, pos, cell, Z, out, label):
+    nlocal, gown = out["nlocal"], out["gown"]
+    # raw step == oracle on the open local+ghost set with the local mask
+    eo, fo, vo, eao, vao = case.oracle_open(out["x"], out["Z"], nlocal)
+    print("%s: %d local, %d ghosts, raw dE vs open oracle %.3e" % (label, nlocal, len(gown), out["e"] - eo),
+          flush=True)
+    assert abs(out["e"] - eo) <= REL * abs(eo)
+    assert close(out["f"], fo) and close(out["vir"], vo) and close(out["ea"], eao) and close(out["va"], vao)
+    # after reverse communication == periodic oracle
+    ep, fp, vp, eap, vap = case.oracle_periodic(pos, cell, Z)
+    assert abs(out["e"] - ep) <= REL * abs(ep)
+    assert close(reverse(out["f"], nlocal, gown), fp)
+    assert close(out["vir"], vp)
+    assert close(reverse(out["ea"], nlocal, gown), eap)
+    assert close(reverse(out["va"], nlocal, gown), vap)
+    return ep, fp, vp
+
+
+def test_lammps_tersoff_sic():
+    case = Case("tersoff")
+    pos, cell, Z = sic_system()
+    out = run_step(case, pos, cell, Z, 2 * 3.0 + 0.5)
+    ep, fp, vp = check_against_oracle(case, pos, cell, Z, out, "lammps tersoff")
+    e0, f0, v0 = periodic_gpu(N.Tersoff(), pos, cell, Z)
+    assert abs(e0 - ep) <= REL * abs(ep) and close(f0, fp) and close(v0, vp)
+
+
+def test_lammps_eam():
+    case = fs_case()
+    pos, cell, Z = cu_system()
+    out = run_step(case, pos, cell, Z, 5.5 + 0.5)
+    ep, fp, vp = check_against_oracle(case, pos, cell, Z, out, "lammps eam")
+    e0, f0, v0 = periodic_gpu(case.gpu(), pos, cell, Z)
+    assert abs(e0 - ep) <= REL * abs(ep) and close(f0, fp) and close(v0, vp)
+
+
+def test_lammps_untagged_entry_points():
+    """agpu_lammps_compute and agpu_lammps_compute_list (no tags) give the
+    same raw step as the oracle on the open set."""
+    case = Case("tersoff")
+    pos, cell, Z = sic_system()
+    gx, gown = ghosts(pos, cell, 2 * 3.0 + 1.5)
+    nlocal = len(pos)
+    x = np.concatenate([pos, gx])
+    Zall = np.concatenate([Z, Z[gown]]).astype(np.int32)
+    eo, fo, vo, eao, _ = case.oracle_open(x, Zall, nlocal)
+    e1, f1, v1, ea1 = N.lammps_compute(N.Tersoff(), N.Particles(), N.Neighbors(200), nlocal, x, Zall, eatom=True)
+    seed, numneigh, neigh = lammps_full_list(x, 4.0)
+    e2, f2, v2, ea2 = N.lammps_compute_list(N.Tersoff(), N.Particles(), N.Neighbors(200), nlocal, x, Zall,
+                                            seed, numneigh, neigh, eatom=True)
+    for e, f, v, ea in ((e1, f1, v1, ea1), (e2, f2, v2, ea2)):
+        assert abs(e - eo) <= REL * abs(eo)
+        assert close(f, fo) and close(v, vo) and close(ea, eao)
 
 
 def test_lammps_list_tersoff_sic():
     """Tersoff over LAMMPS' own full list with ghosts (skin 1 A, shuffled
-    rows, special-bond bits set) equals the periodic evaluation."""
-    pos, cell, pbc, Z = W.diamond_si(3, a=4.36, sigma=0.06, seed=11)
-    Z = np.where(np.arange(len(pos)) % 8 >= 4, 14, 6).astype(np.int32)
-    pos = _wrap(pos, cell)
-    e0, f0, v0 = periodic(N.Tersoff(), pos, cell, Z)
-    e1, f1, v1 = run_lammps_list(N.Tersoff(), pos, cell, Z, 2 * 3.0 + 1.5, 3.0 + 1.0)
-    print("lammps list tersoff: dE %.3e" % (e1 - e0), flush=True)
-    assert abs(e1 - e0) <= REL * abs(e0)
-    assert close(f1, f0) and close(v1, v0)
+    rows, special-bond bits set)."""
+    case = Case("tersoff")
+    pos, cell, Z = sic_system()
+    out = run_step(case, pos, cell, Z, 2 * 3.0 + 1.5, skin=1.0)
+    check_against_oracle(case, pos, cell, Z, out, "lammps list tersoff")
 
 
 def test_lammps_list_eam():
-    import tempfile
-    import os
-    d = tempfile.mkdtemp()
-    fn = os.path.join(d, "fs_cu.eam.alloy")
-    with open(fn, "w") as fh:
-        fh.write(W.fs_cu_setfl_text(nr=4000, nrho=4000))
-    pos, cell, pbc, Z = W.fcc_cu(4, sigma=0.08, seed=3)
-    pos = _wrap(pos, cell)
-    e0, f0, v0 = periodic(N.TabulatedAlloyEAM(fn=fn), pos, cell, Z)
-    e1, f1, v1 = run_lammps_list(N.TabulatedAlloyEAM(fn=fn), pos, cell, Z, 5.5 + 1.5, 5.5 + 1.0)
-    print("lammps list eam: dE %.3e" % (e1 - e0), flush=True)
-    assert abs(e1 - e0) <= REL * abs(e0)
-    assert close(f1, f0) and close(v1, v0)
+    case = fs_case()
+    pos, cell, Z = cu_system()
+    out = run_step(case, pos, cell, Z, 5.5 + 1.5, skin=1.0)
+    check_against_oracle(case, pos, cell, Z, out, "lammps list eam")
 
 
 def test_lammps_list_bad_index():
@@ -181,27 +287,18 @@ def _ach(n=400, seed=21):
     return _wrap(pos, cell), cell, Z
 
 
-def _periodic_rebo2(pos, cell, Z):
-    p = N.from_atoms(pos, cell, np.array([True] * 3), Z)
-    nl = N.Neighbors(50)
-    pot = N.Rebo2(dihedral=True)
-    pot.bind_to(p, nl)
-    e, f, w, epa, _, _, wpa, _ = pot.energy_and_forces(p, nl, epot_per_at=True, wpot_per_at=True)
-    vir = -np.array([w[0, 0], w[1, 1], w[2, 2], 0.5 * (w[0, 1] + w[1, 0]), 0.5 * (w[0, 2] + w[2, 0]),
-                     0.5 * (w[1, 2] + w[2, 1])])
-    # LAMMPS vatom of each atom: -(W_xx, W_yy, W_zz, W_yx, W_zx, W_zy)
-    vat = -np.stack([wpa[:, 0, 0], wpa[:, 1, 1], wpa[:, 2, 2], wpa[:, 1, 0], wpa[:, 2, 0], wpa[:, 2, 1]], axis=1)
-    return e, f, vir, epa, vat
-
-
 @pytest.mark.parametrize("own_list", [False, True], ids=["list_here", "lammps_list"])
 def test_lammps_rebo2_tag_half_rule(own_list):
     """REBO2 (torsion on) through the tagged step: each bond once from the
     local atom with tag(j) >= tag(i), ghosts 8 A deep (neighbours of
-    neighbours of neighbours), then reverse communication of forces and
-    per-atom energies; equals the periodic evaluation."""
+    neighbours of neighbours), then reverse communication of forces,
+    per-atom energies and per-atom virials; equals the oracle's periodic
+    evaluation."""
     pos, cell, Z = _ach()
-    e0, f0, v0, epa0, vat0 = _periodic_rebo2(pos, cell, Z)
+    R = O.Rebo2(dihedral=True)
+    onl = O.nl_build(pos, cell, [True] * 3, 2.0)
+    ep, fp, wp, eap, wpap = R(pos, Z, cell, onl, per_at=True)
+    vp, vap = voigt(wp), vatom_of(wpap)
     gx, gown = ghosts(pos, cell, 8.0)
     nlocal = len(pos)
     x = np.concatenate([pos, gx])
@@ -212,37 +309,37 @@ def test_lammps_rebo2_tag_half_rule(own_list):
         kw = dict(zip(("seed", "numneigh", "neigh"), lammps_full_list(x, 2.0 + 1.0, seed=5)))
     eng, f, vir, ea, va = N.lammps_compute_tagged(N.Rebo2(dihedral=True), N.Particles(), N.Neighbors(50), nlocal,
                                                   x, Zall, tag, eatom=True, vatom=True, **kw)
-    floc = f[:nlocal].copy()
-    np.add.at(floc, gown, f[nlocal:])
-    eloc = ea[:nlocal].copy()
-    np.add.at(eloc, gown, ea[nlocal:])
-    print("lammps rebo2 (%s): %d local, %d ghosts, dE %.3e" % ("lammps list" if own_list else "list here",
-                                                               nlocal, len(gx), eng - e0), flush=True)
-    vloc = va[:nlocal].copy()
-    np.add.at(vloc, gown, va[nlocal:])
-    assert abs(eng - e0) <= REL * abs(e0)
-    assert close(floc, f0) and close(vir, v0) and close(eloc, epa0) and close(vloc, vat0)
+    print("lammps rebo2 (%s): %d local, %d ghosts, dE vs oracle %.3e" % (
+        "lammps list" if own_list else "list here", nlocal, len(gx), eng - ep), flush=True)
+    assert abs(eng - ep) <= REL * abs(ep)
+    assert close(reverse(f, nlocal, gown), fp) and close(vir, vp)
+    assert close(reverse(ea, nlocal, gown), eap) and close(reverse(va, nlocal, gown), vap)
+    # the periodic GPU path agrees too
+    p = N.from_atoms(pos, cell, np.array([True] * 3), Z)
+    nl = N.Neighbors(50)
+    pot = N.Rebo2(dihedral=True)
+    pot.bind_to(p, nl)
+    e0, f0, w0, *_ = pot.energy_and_forces(p, nl)
+    assert abs(e0 - ep) <= REL * abs(ep) and close(f0, fp)
 
 
-def test_lammps_tagged_tersoff_vatom():
-    """The tagged step for a masked class (Tersoff SiC): per-atom virials
-    after reverse communication equal the periodic per-atom virials."""
-    pos, cell, pbc, Z = W.diamond_si(3, a=4.36, sigma=0.06, seed=11)
-    Z = np.where(np.arange(len(pos)) % 8 >= 4, 14, 6).astype(np.int32)
-    pos = _wrap(pos, cell)
-    p0 = N.from_atoms(pos, cell, np.array([True] * 3), Z)
-    nl0 = N.Neighbors(100)
-    pot0 = N.Tersoff()
-    pot0.bind_to(p0, nl0)
-    e0, f0, w0, _, _, _, wpa0, _ = pot0.energy_and_forces(p0, nl0, wpot_per_at=True)
-    vat0 = -np.stack([wpa0[:, 0, 0], wpa0[:, 1, 1], wpa0[:, 2, 2], wpa0[:, 1, 0], wpa0[:, 2, 0], wpa0[:, 2, 1]], axis=1)
-    gx, gown = ghosts(pos, cell, 2 * 3.0 + 0.5)
+def test_lammps_rebo2_local_energy_split():
+    """The tag half rule puts each bond's energy on the local atom of lower
+    tag and on its partner as in the periodic evaluation: the raw per-atom
+    energies of the local atoms plus those of their ghost copies sum to the
+    oracle's per-atom energies, and no bond is counted twice (total energy
+    equal)."""
+    pos, cell, Z = _ach(n=200, seed=33)
+    R = O.Rebo2(dihedral=False)
+    onl = O.nl_build(pos, cell, [True] * 3, 2.0)
+    ep, fp, wp, eap, _ = R(pos, Z, cell, onl, per_at=True)
+    gx, gown = ghosts(pos, cell, 8.0)
     nlocal = len(pos)
     x = np.concatenate([pos, gx])
     Zall = np.concatenate([Z, Z[gown]]).astype(np.int32)
     tag = np.concatenate([np.arange(nlocal), gown]).astype(np.int32) + 1
-    eng, f, vir, ea, va = N.lammps_compute_tagged(N.Tersoff(), N.Particles(), N.Neighbors(200), nlocal, x, Zall,
-                                                  tag, vatom=True)
-    vloc = va[:nlocal].copy()
-    np.add.at(vloc, gown, va[nlocal:])
-    assert abs(eng - e0) <= REL * abs(e0) and close(vloc, vat0)
+    eng, f, vir, ea = N.lammps_compute_tagged(N.Rebo2(dihedral=False), N.Particles(), N.Neighbors(50), nlocal,
+                                              x, Zall, tag, eatom=True)
+    assert abs(eng - ep) <= REL * abs(ep)
+    assert abs(ea.sum() - ep) <= REL * abs(ep)
+    assert close(reverse(ea, nlocal, gown), eap) and close(reverse(f, nlocal, gown), fp)

@@ -106,3 +106,36 @@ def test_md_eam_graph_equals_stepwise(tmp_path):
     rb, vb, _, cb = b.state()
     assert np.array_equal(ra, rb) and np.array_equal(va, vb)
     assert ca["replays"] > 0 and ca["builds"] == cb["builds"]
+
+
+def _bcc_w(n=3, T=900.0, seed=6):
+    pos, cell = W.lattice(np.array([[0, 0, 0], [.5, .5, .5]]), n, 3.165)
+    rng = np.random.default_rng(seed)
+    pos = pos + rng.normal(0, 0.03, pos.shape)
+    Z = np.full(len(pos), 74, np.int32)
+    v = rng.normal(0.0, np.sqrt(KB * T / 183.84), pos.shape)
+    v -= v.mean(axis=0)
+    return pos, cell, np.array([True] * 3), Z, v
+
+
+@pytest.mark.parametrize("kind", ["tersoff_scr", "juslin_scr"])
+def test_md_screened_graph_request_equals_stepwise(kind):
+    """The screened bond-order classes size their screening records on the
+    host every call, so a run that asks for graph replays runs them step by
+    step instead (no error), with the same trajectory bit for bit."""
+    if kind == "tersoff_scr":
+        pos, cell, pbc, Z, v = _si(3, T=900.0, seed=4)
+        make, mass = N.TersoffScr, M_SI
+    else:
+        pos, cell, pbc, Z, v = _bcc_w()
+        make, mass = N.JuslinScr, 183.84
+    a = _md(make(), pos, cell, pbc, Z, v, skin=0.5, masses=mass)
+    b = _md(make(), pos, cell, pbc, Z, v, skin=0.5, masses=mass)
+    ea = a.run(40, graph_steps=32)
+    eb = b.run(40, graph_steps=0)
+    ra, va, fa, ca = a.state()
+    rb, vb, fb, cb = b.state()
+    print("md %s: %s / %s" % (kind, ca, cb), flush=True)
+    assert ea == eb
+    assert np.array_equal(ra, rb) and np.array_equal(va, vb) and np.array_equal(fa, fb)
+    assert ca["steps"] == 40 and ca["replays"] == 0
