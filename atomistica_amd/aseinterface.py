@@ -105,6 +105,13 @@ class Atomistica:
         self.results = {}
         self.properties_changed = list(all_changes)
         self._last = None
+        # per-bond properties (aseinterface.py:209-216 of the reference)
+        self.compute_epot_per_bond = False
+        self.compute_f_per_bond = False
+        self.compute_wpot_per_bond = False
+        self.epot_per_bond = None
+        self.f_per_bond = None
+        self.wpot_per_bond = None
 
     # -- state -------------------------------------------------------------
     def initialize(self, atoms):
@@ -139,6 +146,19 @@ class Atomistica:
             self.mask = None if mask is None else np.asarray(mask)
             self.properties_changed += ["mask"]
 
+    def set_per_bond(self, epot=None, f=None, wpot=None):
+        """Request per-bond energies, pair forces and virials on the next
+        calculation (aseinterface.py:290-297); they are kept in
+        epot_per_bond, f_per_bond and wpot_per_bond in the order of
+        get_neighbors()."""
+        if epot is not None:
+            self.compute_epot_per_bond = epot
+        if f is not None:
+            self.compute_f_per_bond = f
+        if wpot is not None:
+            self.compute_wpot_per_bond = wpot
+        self.properties_changed += ["per_bond"]
+
     def update(self, atoms):
         if atoms is None:
             return
@@ -172,13 +192,18 @@ class Atomistica:
         self.results = {}
         compute_epot_per_at = "energies" in properties
         compute_wpot_per_at = "stresses" in properties
-        kw = dict(epot_per_at=compute_epot_per_at, wpot_per_at=compute_wpot_per_at)
+        kw = dict(epot_per_at=compute_epot_per_at, wpot_per_at=compute_wpot_per_at,
+                  epot_per_bond=self.compute_epot_per_bond, f_per_bond=self.compute_f_per_bond,
+                  wpot_per_bond=self.compute_wpot_per_bond)
         if self.mask is not None:
             kw["mask"] = np.asarray(self.mask, dtype=np.int32)
         epot_per_at = np.zeros(nat) if compute_epot_per_at else None
         wpot_per_at = np.zeros((nat, 3, 3)) if compute_wpot_per_at else None
         for pot in self.pots:
-            _e, _f, _w, epa, _, _, wpa, _ = pot.energy_and_forces(self.particles, self.nl, forces=forces, **kw)
+            # the per-bond arrays are those of the last potential, as in the
+            # reference (aseinterface.py:380-392)
+            _e, _f, _w, epa, self.epot_per_bond, self.f_per_bond, wpa, self.wpot_per_bond = \
+                pot.energy_and_forces(self.particles, self.nl, forces=forces, **kw)
             epot += _e
             wpot += _w
             if compute_epot_per_at:

@@ -676,10 +676,17 @@ static int stage_and_call(agpu_potential *pot, agpu_particles *p, agpu_neighbors
         }
         return rc;
     }
-    // host arrays: stage through one device block
+    // host arrays: stage through one device block; per-bond arrays hold one
+    // entry per list entry of the updated list
+    int64_t nbond = 0;
+    if (a.epot_per_bond || a.f_per_bond || a.wpot_per_bond) {
+        AGPU_CHECK(nl_update(nl, p, 0));
+        nbond = nl->total;
+    }
     DevBuf st;
     size_t n = 1 + 3 * (size_t)nat + 9 + (a.q ? nat : 0) + (a.epot_per_at ? nat : 0) +
-               (a.wpot_per_at ? 9 * (size_t)nat : 0);
+               (a.wpot_per_at ? 9 * (size_t)nat : 0) + (a.epot_per_bond ? (size_t)nbond : 0) +
+               (a.f_per_bond ? 3 * (size_t)nbond : 0) + (a.wpot_per_bond ? 9 * (size_t)nbond : 0);
     size_t nmask = a.mask ? (size_t)nat : 0;
     AGPU_CHECK(st.reserve(sizeof(double) * n + sizeof(int) * (nmask + 1)));
     double *b = st.as<double>();
@@ -696,6 +703,13 @@ static int stage_and_call(agpu_potential *pot, agpu_particles *p, agpu_neighbors
     if (a.q) { qd = cur; cur += nat; }
     if (a.epot_per_at) { epa = cur; cur += nat; }
     if (a.wpot_per_at) { wpa = cur; cur += 9 * (size_t)nat; }
+    double *epb = nullptr, *fpb = nullptr, *wpb = nullptr;
+    if (a.epot_per_bond) { epb = cur; cur += nbond; }
+    if (a.f_per_bond) { fpb = cur; cur += 3 * (size_t)nbond; }
+    if (a.wpot_per_bond) { wpb = cur; cur += 9 * (size_t)nbond; }
+    d.epot_per_bond = epb;
+    d.f_per_bond = fpb;
+    d.wpot_per_bond = wpb;
     int *md = a.mask ? reinterpret_cast<int *>(cur) : nullptr;
     d.q = qd;
     d.epot_per_at = epa;
@@ -711,6 +725,12 @@ static int stage_and_call(agpu_potential *pot, agpu_particles *p, agpu_neighbors
     if (epa) AGPU_HIP(hipMemcpyAsync(epa, a.epot_per_at, sizeof(double) * nat, hipMemcpyHostToDevice, s));
     if (wpa) AGPU_HIP(hipMemcpyAsync(wpa, a.wpot_per_at, sizeof(double) * 9 * nat, hipMemcpyHostToDevice, s));
     if (md) AGPU_HIP(hipMemcpyAsync(md, a.mask, sizeof(int) * nat, hipMemcpyHostToDevice, s));
+    if (epb && nbond)
+        AGPU_HIP(hipMemcpyAsync(epb, a.epot_per_bond, sizeof(double) * nbond, hipMemcpyHostToDevice, s));
+    if (fpb && nbond)
+        AGPU_HIP(hipMemcpyAsync(fpb, a.f_per_bond, sizeof(double) * 3 * nbond, hipMemcpyHostToDevice, s));
+    if (wpb && nbond)
+        AGPU_HIP(hipMemcpyAsync(wpb, a.wpot_per_bond, sizeof(double) * 9 * nbond, hipMemcpyHostToDevice, s));
     switch (pot->kind) {
     case AGPU_KIND_TERSOFF:
     case AGPU_KIND_KUMAGAI:
@@ -732,6 +752,14 @@ static int stage_and_call(agpu_potential *pot, agpu_particles *p, agpu_neighbors
     if (a.wpot) AGPU_HIP(hipMemcpyAsync(a.wpot, d.wpot, sizeof(double) * 9, hipMemcpyDeviceToHost, s));
     if (epa) AGPU_HIP(hipMemcpyAsync(a.epot_per_at, epa, sizeof(double) * nat, hipMemcpyDeviceToHost, s));
     if (wpa) AGPU_HIP(hipMemcpyAsync(a.wpot_per_at, wpa, sizeof(double) * 9 * nat, hipMemcpyDeviceToHost, s));
+    if (nl->total != nbond && (epb || fpb || wpb))
+        return set_error(AGPU_ERR_STATE, "energy_and_forces: the list changed size during the call");
+    if (epb && nbond)
+        AGPU_HIP(hipMemcpyAsync(a.epot_per_bond, epb, sizeof(double) * nbond, hipMemcpyDeviceToHost, s));
+    if (fpb && nbond)
+        AGPU_HIP(hipMemcpyAsync(a.f_per_bond, fpb, sizeof(double) * 3 * nbond, hipMemcpyDeviceToHost, s));
+    if (wpb && nbond)
+        AGPU_HIP(hipMemcpyAsync(a.wpot_per_bond, wpb, sizeof(double) * 9 * nbond, hipMemcpyDeviceToHost, s));
     AGPU_HIP(hipStreamSynchronize(s));
     return AGPU_OK;
 }
@@ -743,7 +771,11 @@ int agpu_potential_energy_and_forces(agpu_potential *pot, agpu_particles *p, agp
                                      int flags)
 {
     if (!pot || !p || !nl) return set_error(AGPU_ERR_ARG, "energy_and_forces: NULL handle");
-    if (epot_per_bond || f_per_bond || wpot_per_bond)
+    // per-bond properties: the classes whose reference kernels fill them
+    // (features per_bond: the bond-order families and REBO2)
+    const bool per_bond = pot->kind != AGPU_KIND_EAM_ALLOY && pot->kind != AGPU_KIND_EWALD &&
+                          pot->kind != AGPU_KIND_PME;
+    if ((epot_per_bond || f_per_bond || wpot_per_bond) && !per_bond)
         return set_error(AGPU_ERR_UNSUPPORTED,
                          "energy_and_forces: per-bond properties are not supported by this class");
     if ((pot->kind == AGPU_KIND_EWALD || pot->kind == AGPU_KIND_PME) && !q)
@@ -757,6 +789,9 @@ int agpu_potential_energy_and_forces(agpu_potential *pot, agpu_particles *p, agp
     a.mask = mask;
     a.epot_per_at = epot_per_at;
     a.wpot_per_at = wpot_per_at;
+    a.epot_per_bond = epot_per_bond;
+    a.f_per_bond = f_per_bond;
+    a.wpot_per_bond = wpot_per_bond;
     a.flags = flags;
     return stage_and_call(pot, p, nl, a);
 }

@@ -83,7 +83,7 @@ struct Rebo2State {
     RbDev hdev{};
     std::vector<double> htab;
     DevBuf dev, tab, bond, bij, bown, bcnt, nn, nconj, facc, eacc, wacc, partial, stats, bo, rev, half, hpos,
-        list, scan_tmp;
+        list, scan_tmp, bsrc;
     int z2db[128];
     uint64_t Z_rev = ~0ull;
     double cutoff = 0.0;
@@ -565,7 +565,7 @@ __global__ void __launch_bounds__(256)
 k_rb_bonds(int nat, const double *__restrict__ r, const int *__restrict__ el, CellArg C,
            const RbDev *__restrict__ Dp, const int *__restrict__ offsets,
            const int2 *__restrict__ nbr, RbBond *__restrict__ bond, int2 *__restrict__ bij,
-           int *__restrict__ bown, int *__restrict__ bcnt, double2 *__restrict__ nn)
+           int *__restrict__ bown, int *__restrict__ bcnt, double2 *__restrict__ nn, int *__restrict__ bsrc)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nat) return;
@@ -609,6 +609,7 @@ k_rb_bonds(int nat, const double *__restrict__ r, const int *__restrict__ el, Ce
             bond[beg + nb] = B;
             bij[beg + nb] = e;
             bown[beg + nb] = i;
+            if (bsrc) bsrc[beg + nb] = s;      // list entry of the bond (nbb of the reference)
             ++nb;
         }
     }
@@ -978,7 +979,9 @@ k_rb_pair(int ntot, const int *__restrict__ list, const int *__restrict__ count,
           const int2 *__restrict__ bij, const double2 *__restrict__ nn, const double *__restrict__ nconjit,
           const double4 *__restrict__ bo, const int *__restrict__ rev,
           unsigned long long *__restrict__ facc, unsigned long long *__restrict__ eacc,
-          unsigned long long *__restrict__ wacc, double *__restrict__ partial, int *__restrict__ flags)
+          unsigned long long *__restrict__ wacc, double *__restrict__ partial, int *__restrict__ flags,
+          const int *__restrict__ bsrc, double *__restrict__ epb, double *__restrict__ fpb,
+          double *__restrict__ wpb)
 {
     __shared__ double s_red[4][10];
     const RbDev &D = *Dp;
@@ -1078,6 +1081,17 @@ k_rb_pair(int ntot, const int *__restrict__ list, const int *__restrict__ count,
             for (int x = 0; x < 3; ++x) wij[x + 3 * y] = wij[x + 3 * y] + rij[x] * df[y];
         fx_add3(facc, i, fi, bad);
         fx_add3(facc, j, fj, bad);
+        // per-bond properties on the list entry of the half pair's bond i->j
+        // (bop_kernel_rebo2.f90:2546-2547, 2571-2572, 2762-2763); one thread
+        // owns each half pair, so the updates do not collide
+        if (bsrc) {
+            const size_t e = (size_t)bsrc[s];
+            if (epb) epb[e] += epair;
+            if (fpb)
+                for (int x = 0; x < 3; ++x) fpb[3 * e + x] += df[x];
+            if (wpb)
+                for (int q = 0; q < 9; ++q) wpb[9 * e + q] += wij[q];
+        }
         if (eacc) {
             fx_add(eacc + i, hlf, bad);
             fx_add(eacc + j, hlf, bad);
@@ -1216,9 +1230,12 @@ int rebo2_compute(agpu_potential *pot, const agpu_eval_args *a)
     const int *el = pot->d_el.as<int>();
     const RbDev *D = st->dev.as<RbDev>();
     const int nb = (nat + 255) / 256;
+    const bool per_bond = a->epot_per_bond || a->f_per_bond || a->wpot_per_bond;
+    if (per_bond) AGPU_CHECK(st->bsrc.reserve(sizeof(int) * (total + 1)));
+    int *bsrc = per_bond ? st->bsrc.as<int>() : nullptr;
     AGPU_LAUNCH("rebo2_bonds", k_rb_bonds, dim3(nb), dim3(256), 0, s, nat, p->d_r, el, C, D,
                 nl->offsets.as<int>(), nl->nbr.as<int2>(), st->bond.as<RbBond>(), st->bij.as<int2>(),
-                st->bown.as<int>(), st->bcnt.as<int>(), st->nn.as<double2>());
+                st->bown.as<int>(), st->bcnt.as<int>(), st->nn.as<double2>(), bsrc);
     AGPU_LAUNCH("rebo2_conj", k_rb_conj, dim3(nb), dim3(256), 0, s, nat, el, nl->offsets.as<int>(),
                 st->bcnt.as<int>(), st->bij.as<int2>(), st->nn.as<double2>(), st->bond.as<RbBond>(),
                 st->nconj.as<double>());
@@ -1252,7 +1269,7 @@ int rebo2_compute(agpu_potential *pot, const agpu_eval_args *a)
                 st->tab.as<double>(), el, nl->offsets.as<int>(), st->bcnt.as<int>(), st->bown.as<int>(),
                 st->bond.as<RbBond>(), st->bij.as<int2>(), st->nn.as<double2>(), st->nconj.as<double>(),
                 st->bo.as<double4>(), st->rev.as<int>(), st->facc.as<unsigned long long>(), eacc, wacc,
-                st->partial.as<double>(), dstats);
+                st->partial.as<double>(), dstats, bsrc, a->epot_per_bond, a->f_per_bond, a->wpot_per_bond);
     AGPU_LAUNCH("rebo2_finish", k_rb_finish, dim3(nb), dim3(256), 0, s, nat,
                 st->facc.as<unsigned long long>(), eacc, wacc, a->f, a->epot_per_at, a->wpot_per_at);
     AGPU_CHECK(reduce_ev(st->partial.as<double>(), nblk, 10, a->epot, a->wpot, nullptr));

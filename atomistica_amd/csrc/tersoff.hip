@@ -348,7 +348,8 @@ k_tersoff(TersoffDev P, int nat, const int *__restrict__ order, const double *__
           const int *__restrict__ el, CellArg C, const int *__restrict__ offsets,
           const int2 *__restrict__ nbr, const int *__restrict__ bslot, const int *__restrict__ bcnt,
           const int *__restrict__ mask, double *__restrict__ fslot, double *__restrict__ fown,
-          double *__restrict__ eslot, double *__restrict__ wslot, double *__restrict__ partial)
+          double *__restrict__ eslot, double *__restrict__ wslot, double *__restrict__ partial,
+          double *__restrict__ epb, double *__restrict__ fpb, double *__restrict__ wpb)
 {
     constexpr int GPB = 256 / G;
     __shared__ R s_rn[3][256];
@@ -531,7 +532,7 @@ k_tersoff(TersoffDev P, int nat, const int *__restrict__ order, const double *__
         }
     }
 
-    double fi[3] = {0.0, 0.0, 0.0}, fj[3] = {0.0, 0.0, 0.0};
+    double fi[3] = {0.0, 0.0, 0.0}, fj[3] = {0.0, 0.0, 0.0}, dfp[3] = {0.0, 0.0, 0.0};
     double e_ij = 0.0;
     double w[9];
 #pragma unroll
@@ -541,6 +542,7 @@ k_tersoff(TersoffDev P, int nat, const int *__restrict__ order, const double *__
         R dffac = (R)0.5 * (dVR * fc + bij * dVA * fc + VR * dfc + bij * VA * dfc);
         for (int x = 0; x < 3; ++x) {
             double df = (double)(dffac * rn[x]);
+            dfp[x] = df;
             fi[x] = df - (double)Pz * (double)dbidi[x];
             fj[x] = -df - (double)Pz * (double)dbidj[x];
         }
@@ -555,6 +557,14 @@ k_tersoff(TersoffDev P, int nat, const int *__restrict__ order, const double *__
         if (eslot) eslot[s] = e_ij;
         if (wslot)
             for (int q = 0; q < 9; ++q) wslot[9 * (size_t)s + q] = w[q];
+        // per-bond properties of bond s (bop_kernel.f90:1370-1373,1406-1408,1511-1512)
+        if (on) {
+            if (epb) epb[s] += e_ij;
+            if (fpb)
+                for (int x = 0; x < 3; ++x) fpb[3 * (size_t)s + x] += dfp[x];
+            if (wpb)
+                for (int q = 0; q < 9; ++q) wpb[9 * (size_t)s + q] += w[q];
+        }
     }
     // own force of i: sum over the group
     double f0 = group_sum<G>(fi[0]);
@@ -880,7 +890,8 @@ k_bop_scr(TersoffDev P, ScrDev S, int nat, const int *__restrict__ order, const 
           const double4 *__restrict__ fcv, const int *__restrict__ roff, const int *__restrict__ rkn,
           const double4 *__restrict__ rco, double *__restrict__ fslot, double *__restrict__ frec,
           double *__restrict__ fown, double *__restrict__ eslot, double *__restrict__ wslot,
-          double *__restrict__ partial)
+          double *__restrict__ partial, double *__restrict__ epb, double *__restrict__ fpb,
+          double *__restrict__ wpb)
 {
     constexpr int GPB = 256 / G;
     __shared__ R s_rn[3][256];
@@ -1050,7 +1061,7 @@ k_bop_scr(TersoffDev P, ScrDev S, int nat, const int *__restrict__ order, const 
         }
     }
 
-    double fi[3] = {0.0, 0.0, 0.0}, fj[3] = {0.0, 0.0, 0.0};
+    double fi[3] = {0.0, 0.0, 0.0}, fj[3] = {0.0, 0.0, 0.0}, dfp[3] = {0.0, 0.0, 0.0};
     double e_ij = 0.0;
     double w[9];
 #pragma unroll
@@ -1060,6 +1071,7 @@ k_bop_scr(TersoffDev P, ScrDev S, int nat, const int *__restrict__ order, const 
         R dffac = (R)0.5 * (dVR * fc + bij * dVA * fc + VR * dfc + bij * VA * dfc);
         for (int x = 0; x < 3; ++x) {
             double df = (double)(dffac * rn[x]);
+            dfp[x] = df;
             fi[x] = df - (double)Pz * (double)dbidi[x];
             fj[x] = -df - (double)Pz * (double)dbidj[x];
         }
@@ -1101,6 +1113,14 @@ k_bop_scr(TersoffDev P, ScrDev S, int nat, const int *__restrict__ order, const 
         fslot[3 * (size_t)s + 2] = fj[2] + recv[2];
         eslot[s] = e_ij;
         for (int q = 0; q < 9; ++q) wslot[9 * (size_t)s + q] = w[q];
+        // per-bond properties: the pair term of the bond (when on) and the
+        // virial of loop 2 plus both screening loops (bop_kernel.f90:1370-1373,
+        // 1406-1408, 1511-1512, 1593-1594)
+        if (epb) epb[s] += e_ij;
+        if (fpb)
+            for (int x = 0; x < 3; ++x) fpb[3 * (size_t)s + x] += dfp[x];
+        if (wpb)
+            for (int q = 0; q < 9; ++q) wpb[9 * (size_t)s + q] += w[q];
     }
     double f0 = group_sum<G>(fi[0]);
     double f1 = group_sum<G>(fi[1]);
@@ -1300,35 +1320,40 @@ int tersoff_bind(agpu_potential *pot)
 template <typename R, int G>
 static int launch_tersoff(TersoffState *st, int nat, const int *order, const double *r, const int *el,
                           const CellArg &C, agpu_neighbors *nl, const int *mask, double *eslot,
-                          double *wslot, int nblk, hipStream_t s)
+                          double *wslot, const agpu_eval_args *a, int nblk, hipStream_t s)
 {
     if (st->dev.fam == 3)
         AGPU_LAUNCH(sizeof(R) == 8 ? "juslin_fp64" : "juslin_mixed", (k_tersoff<R, G, 3>), dim3(nblk), dim3(256),
                     0, s, st->dev, nat, order, r, el, C, nl->offsets.as<int>(), nl->nbr.as<int2>(),
                     st->bslot.as<int>(), st->bcnt.as<int>(), mask, st->fslot.as<double>(),
-                    st->fown.as<double>(), eslot, wslot, st->partial.as<double>());
+                    st->fown.as<double>(), eslot, wslot, st->partial.as<double>(), a->epot_per_bond,
+                    a->f_per_bond, a->wpot_per_bond);
     else if (st->dev.fam == 1)
         AGPU_LAUNCH(sizeof(R) == 8 ? "kumagai_fp64" : "kumagai_mixed", (k_tersoff<R, G, 1>), dim3(nblk), dim3(256),
                     0, s, st->dev, nat, order, r, el, C, nl->offsets.as<int>(), nl->nbr.as<int2>(),
                     st->bslot.as<int>(), st->bcnt.as<int>(), mask, st->fslot.as<double>(),
-                    st->fown.as<double>(), eslot, wslot, st->partial.as<double>());
+                    st->fown.as<double>(), eslot, wslot, st->partial.as<double>(), a->epot_per_bond,
+                    a->f_per_bond, a->wpot_per_bond);
     else if (st->dev.fam == 2)
         AGPU_LAUNCH(sizeof(R) == 8 ? "brenner_fp64" : "brenner_mixed", (k_tersoff<R, G, 2>), dim3(nblk), dim3(256),
                     0, s, st->dev, nat, order, r, el, C, nl->offsets.as<int>(), nl->nbr.as<int2>(),
                     st->bslot.as<int>(), st->bcnt.as<int>(), mask, st->fslot.as<double>(),
-                    st->fown.as<double>(), eslot, wslot, st->partial.as<double>());
+                    st->fown.as<double>(), eslot, wslot, st->partial.as<double>(), a->epot_per_bond,
+                    a->f_per_bond, a->wpot_per_bond);
     else
         AGPU_LAUNCH(sizeof(R) == 8 ? "tersoff_fp64" : "tersoff_mixed", (k_tersoff<R, G, 0>), dim3(nblk), dim3(256),
                     0, s, st->dev, nat, order, r, el, C, nl->offsets.as<int>(), nl->nbr.as<int2>(),
                     st->bslot.as<int>(), st->bcnt.as<int>(), mask, st->fslot.as<double>(),
-                    st->fown.as<double>(), eslot, wslot, st->partial.as<double>());
+                    st->fown.as<double>(), eslot, wslot, st->partial.as<double>(), a->epot_per_bond,
+                    a->f_per_bond, a->wpot_per_bond);
     return AGPU_OK;
 }
 
 template <typename R>
 static int dispatch_tersoff(int G, TersoffState *st, int nat, const int *order, const double *r,
                             const int *el, const CellArg &C, agpu_neighbors *nl, const int *mask,
-                            double *eslot, double *wslot, hipStream_t s, int *nblk_out)
+                            double *eslot, double *wslot, const agpu_eval_args *a, hipStream_t s,
+                            int *nblk_out)
 {
     int gpb = 256 / G;
     int nblk = (nat + gpb - 1) / gpb;
@@ -1336,12 +1361,12 @@ static int dispatch_tersoff(int G, TersoffState *st, int nat, const int *order, 
     *nblk_out = nblk;
     AGPU_CHECK(st->partial.reserve(sizeof(double) * 10 * nblk));
     switch (G) {
-    case 2: return launch_tersoff<R, 2>(st, nat, order, r, el, C, nl, mask, eslot, wslot, nblk, s);
-    case 4: return launch_tersoff<R, 4>(st, nat, order, r, el, C, nl, mask, eslot, wslot, nblk, s);
-    case 8: return launch_tersoff<R, 8>(st, nat, order, r, el, C, nl, mask, eslot, wslot, nblk, s);
-    case 16: return launch_tersoff<R, 16>(st, nat, order, r, el, C, nl, mask, eslot, wslot, nblk, s);
-    case 32: return launch_tersoff<R, 32>(st, nat, order, r, el, C, nl, mask, eslot, wslot, nblk, s);
-    default: return launch_tersoff<R, 64>(st, nat, order, r, el, C, nl, mask, eslot, wslot, nblk, s);
+    case 2: return launch_tersoff<R, 2>(st, nat, order, r, el, C, nl, mask, eslot, wslot, a, nblk, s);
+    case 4: return launch_tersoff<R, 4>(st, nat, order, r, el, C, nl, mask, eslot, wslot, a, nblk, s);
+    case 8: return launch_tersoff<R, 8>(st, nat, order, r, el, C, nl, mask, eslot, wslot, a, nblk, s);
+    case 16: return launch_tersoff<R, 16>(st, nat, order, r, el, C, nl, mask, eslot, wslot, a, nblk, s);
+    case 32: return launch_tersoff<R, 32>(st, nat, order, r, el, C, nl, mask, eslot, wslot, a, nblk, s);
+    default: return launch_tersoff<R, 64>(st, nat, order, r, el, C, nl, mask, eslot, wslot, a, nblk, s);
     }
 }
 
@@ -1402,8 +1427,8 @@ static void derive_scr(ScrDev &S, const TersoffDev &D, const agpu_screening_para
 
 template <typename R, int G>
 static int launch_bop_scr(TersoffState *st, const ScrDev &S, int nat, const int *order, const double *r,
-                          const int *el, const CellArg &C, agpu_neighbors *nl, const int *mask, int nblk,
-                          hipStream_t s)
+                          const int *el, const CellArg &C, agpu_neighbors *nl, const int *mask,
+                          const agpu_eval_args *a, int nblk, hipStream_t s)
 {
     const TersoffDev &P = st->dev;
     const int *offs = nl->offsets.as<int>();
@@ -1418,37 +1443,37 @@ static int launch_bop_scr(TersoffState *st, const ScrDev &S, int nat, const int 
     if (P.fam == 1)
         AGPU_LAUNCH(f64 ? "kumagai_scr_fp64" : "kumagai_scr_mixed", (k_bop_scr<R, G, 1>), dim3(nblk), dim3(256), 0,
                     s, P, S, nat, order, r, el, C, offs, nb, bs, bc, mask, fcv, roff, rkn, rco, fslot, frec, fown,
-                    es, ws, part);
+                    es, ws, part, a->epot_per_bond, a->f_per_bond, a->wpot_per_bond);
     else if (P.fam == 2)
         AGPU_LAUNCH(f64 ? "brenner_scr_fp64" : "brenner_scr_mixed", (k_bop_scr<R, G, 2>), dim3(nblk), dim3(256), 0,
                     s, P, S, nat, order, r, el, C, offs, nb, bs, bc, mask, fcv, roff, rkn, rco, fslot, frec, fown,
-                    es, ws, part);
+                    es, ws, part, a->epot_per_bond, a->f_per_bond, a->wpot_per_bond);
     else if (P.fam == 3)
         AGPU_LAUNCH(f64 ? "juslin_scr_fp64" : "juslin_scr_mixed", (k_bop_scr<R, G, 3>), dim3(nblk), dim3(256), 0,
                     s, P, S, nat, order, r, el, C, offs, nb, bs, bc, mask, fcv, roff, rkn, rco, fslot, frec, fown,
-                    es, ws, part);
+                    es, ws, part, a->epot_per_bond, a->f_per_bond, a->wpot_per_bond);
     else
         AGPU_LAUNCH(f64 ? "tersoff_scr_fp64" : "tersoff_scr_mixed", (k_bop_scr<R, G, 0>), dim3(nblk), dim3(256), 0,
                     s, P, S, nat, order, r, el, C, offs, nb, bs, bc, mask, fcv, roff, rkn, rco, fslot, frec, fown,
-                    es, ws, part);
+                    es, ws, part, a->epot_per_bond, a->f_per_bond, a->wpot_per_bond);
     return AGPU_OK;
 }
 
 template <typename R>
 static int dispatch_bop_scr(int G, TersoffState *st, const ScrDev &S, int nat, const int *order, const double *r,
-                            const int *el, const CellArg &C, agpu_neighbors *nl, const int *mask, hipStream_t s,
-                            int *nblk_out)
+                            const int *el, const CellArg &C, agpu_neighbors *nl, const int *mask,
+                            const agpu_eval_args *a, hipStream_t s, int *nblk_out)
 {
     const int nblk = std::max(1, (nat + 256 / G - 1) / (256 / G));
     *nblk_out = nblk;
     AGPU_CHECK(st->partial.reserve(sizeof(double) * 10 * nblk));
     switch (G) {
-    case 2: return launch_bop_scr<R, 2>(st, S, nat, order, r, el, C, nl, mask, nblk, s);
-    case 4: return launch_bop_scr<R, 4>(st, S, nat, order, r, el, C, nl, mask, nblk, s);
-    case 8: return launch_bop_scr<R, 8>(st, S, nat, order, r, el, C, nl, mask, nblk, s);
-    case 16: return launch_bop_scr<R, 16>(st, S, nat, order, r, el, C, nl, mask, nblk, s);
-    case 32: return launch_bop_scr<R, 32>(st, S, nat, order, r, el, C, nl, mask, nblk, s);
-    default: return launch_bop_scr<R, 64>(st, S, nat, order, r, el, C, nl, mask, nblk, s);
+    case 2: return launch_bop_scr<R, 2>(st, S, nat, order, r, el, C, nl, mask, a, nblk, s);
+    case 4: return launch_bop_scr<R, 4>(st, S, nat, order, r, el, C, nl, mask, a, nblk, s);
+    case 8: return launch_bop_scr<R, 8>(st, S, nat, order, r, el, C, nl, mask, a, nblk, s);
+    case 16: return launch_bop_scr<R, 16>(st, S, nat, order, r, el, C, nl, mask, a, nblk, s);
+    case 32: return launch_bop_scr<R, 32>(st, S, nat, order, r, el, C, nl, mask, a, nblk, s);
+    default: return launch_bop_scr<R, 64>(st, S, nat, order, r, el, C, nl, mask, a, nblk, s);
     }
 }
 
@@ -1539,9 +1564,9 @@ static int tersoff_scr_compute(agpu_potential *pot, const agpu_eval_args *a)
     int nblk = 0;
     const int *order = nl->idx_sorted.as<int>();
     if (pot->precision == AGPU_PREC_MIXED)
-        AGPU_CHECK(dispatch_bop_scr<float>(G, st, S, nat, order, p->d_r, el, C, nl, a->mask, s, &nblk));
+        AGPU_CHECK(dispatch_bop_scr<float>(G, st, S, nat, order, p->d_r, el, C, nl, a->mask, a, s, &nblk));
     else
-        AGPU_CHECK(dispatch_bop_scr<double>(G, st, S, nat, order, p->d_r, el, C, nl, a->mask, s, &nblk));
+        AGPU_CHECK(dispatch_bop_scr<double>(G, st, S, nat, order, p->d_r, el, C, nl, a->mask, a, s, &nblk));
     AGPU_LAUNCH("scr_slots", k_scr_slots, gw, dim3(256), 0, s, nat, offs, st->keep.as<int>(),
                 st->fslot.as<double>(), st->roff.as<int>(), st->rkn.as<int>(), st->frec.as<double>(),
                 st->fsl.as<double>());
@@ -1616,9 +1641,9 @@ int tersoff_compute(agpu_potential *pot, const agpu_eval_args *a)
     int nblk = 0;
     const int *order = nl->idx_sorted.as<int>();
     if (pot->precision == AGPU_PREC_MIXED)
-        AGPU_CHECK(dispatch_tersoff<float>(G, st, nat, order, p->d_r, el, C, nl, a->mask, eslot, wslot, s, &nblk));
+        AGPU_CHECK(dispatch_tersoff<float>(G, st, nat, order, p->d_r, el, C, nl, a->mask, eslot, wslot, a, s, &nblk));
     else
-        AGPU_CHECK(dispatch_tersoff<double>(G, st, nat, order, p->d_r, el, C, nl, a->mask, eslot, wslot, s, &nblk));
+        AGPU_CHECK(dispatch_tersoff<double>(G, st, nat, order, p->d_r, el, C, nl, a->mask, eslot, wslot, a, s, &nblk));
     AGPU_CHECK(launch_gather(G, nat, nl, st, eslot, wslot, a, s));
     // per-block records [e, W(9)] -> epot, wpot
     AGPU_CHECK(reduce_ev(st->partial.as<double>(), nblk, 10, a->epot, a->wpot, nullptr));

@@ -215,7 +215,10 @@ class _Potential:
                           epot_per_bond=False, f_per_bond=False, wpot_per_at=False,
                           wpot_per_bond=False, mask=None):
         """Returns (epot, forces, wpot, epot_per_at, epot_per_bond, f_per_bond,
-        wpot_per_at, wpot_per_bond) as potential.c:622-657 does."""
+        wpot_per_at, wpot_per_bond) as potential.c:622-657 does.  Per-bond
+        arrays have one entry per neighbour-list entry in the order of
+        Neighbors.get_list (the packed order of potential.c:587-611):
+        epot_per_bond (nb,), f_per_bond (nb, 3), wpot_per_bond (nb, 3, 3)."""
         nat = p.nat
         if forces is None:
             forces = np.zeros((nat, 3))
@@ -228,27 +231,37 @@ class _Potential:
             mask = np.ascontiguousarray(mask, dtype=np.int32)
             if len(mask) != nat:
                 raise RuntimeError("mask has %d entries but the particles hold %d atoms" % (len(mask), nat))
+        nb = 0
         if epot_per_bond or f_per_bond or wpot_per_bond:
-            raise AgpuError(_lib.ERR_UNSUPPORTED, "per-bond properties are not supported")
+            # sized by the list the call will use (neighbors_size of the
+            # reference bounds it)
+            nl.update(p)
+            nb = nl.get_size()[0]
         epot = ctypes.c_double(0.0)
         wpot = np.zeros(9)
         epa = np.zeros(nat) if epot_per_at else None
         wpa = np.zeros(9 * nat) if wpot_per_at else None
+        epb = np.zeros(nb) if epot_per_bond else None
+        fpb = np.zeros((nb, 3)) if f_per_bond else None
+        wpb = np.zeros(9 * nb) if wpot_per_bond else None
         q = np.ascontiguousarray(charges, dtype=np.float64) if charges is not None else None
         call("agpu_potential_energy_and_forces", self._h, p._h, nl._h, _ptr(q),
-             ctypes.byref(epot), _ptr(forces), _ptr(wpot), _ptr(mask), _ptr(epa), None, None,
-             _ptr(wpa), None, _lib.HOST)
+             ctypes.byref(epot), _ptr(forces), _ptr(wpot), _ptr(mask), _ptr(epa), _ptr(epb), _ptr(fpb),
+             _ptr(wpa), _ptr(wpb), _lib.HOST)
         w = wpot.reshape(3, 3, order="F").copy()
         wpa_out = wpa.reshape(nat, 3, 3).transpose(0, 2, 1).copy() if wpot_per_at else None
-        return epot.value, forces, w, epa, None, None, wpa_out, None
+        wpb_out = wpb.reshape(nb, 3, 3).transpose(0, 2, 1).copy() if wpot_per_bond else None
+        return epot.value, forces, w, epa, epb, fpb, wpa_out, wpb_out
 
     def energy_and_forces_device(self, p, nl, f, epot, wpot, charges=None, mask=None,
-                                 epot_per_at=None, wpot_per_at=None):
+                                 epot_per_at=None, wpot_per_at=None, epot_per_bond=None,
+                                 f_per_bond=None, wpot_per_bond=None):
         """Device variant: every array is a torch CUDA tensor (f (nat,3) and
-        wpot (9,) float64 accumulated, epot (1,) float64 set)."""
+        wpot (9,) float64 accumulated, epot (1,) float64 set; per-bond arrays
+        sized by the updated list)."""
         call("agpu_potential_energy_and_forces", self._h, p._h, nl._h, _dev(charges), _dev(epot),
-             _dev(f), _dev(wpot), _dev(mask), _dev(epot_per_at), None, None, _dev(wpot_per_at),
-             None, 0)
+             _dev(f), _dev(wpot), _dev(mask), _dev(epot_per_at), _dev(epot_per_bond), _dev(f_per_bond),
+             _dev(wpot_per_at), _dev(wpot_per_bond), 0)
 
 
 def _tersoff_struct(params):

@@ -657,10 +657,13 @@ def potential_block(workload, args, precision="fp64", autotune=True, cpu=True, m
         "autotune": tune,
     }
     if mixed and precision == "fp64" and wl["kind"] in ("eam", "tersoff", "tersoff_scr", "ewald", "rebo2"):
+        from atomistica_amd._lib import AgpuError
         try:
             run.pot.set_precision("mixed")
-        except Exception as e:        # a class without the mixed form
+            run.step()
+        except AgpuError as e:        # a class without the mixed form
             out["mixed"] = {"unsupported": str(e)}
+            run.pot.set_precision("fp64")
         else:
             for _ in range(args.warmup):
                 run.step()

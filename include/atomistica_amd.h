@@ -88,6 +88,9 @@ typedef struct agpu_eval_args {
     const int *mask;
     double *epot_per_at;
     double *wpot_per_at;
+    double *epot_per_bond;       /* one value per list entry (CSR order) */
+    double *f_per_bond;          /* 3 per list entry */
+    double *wpot_per_bond;       /* 3x3 per list entry, W(a, b) at a + 3 b */
     int flags;
 } agpu_eval_args;
 
@@ -306,8 +309,28 @@ int agpu_potential_get_cutoff(agpu_potential *pot, double *cutoff);
 int agpu_potential_bind_to(agpu_potential *pot, agpu_particles *p, agpu_neighbors *nl);
 
 /* energy_and_forces: the argument list of potential_class_t.energy_and_forces
- * (factory.template.h:92-94).  q is used by Coulomb classes only; per-bond
- * arrays must be NULL (AGPU_ERR_UNSUPPORTED otherwise). */
+ * (factory.template.h:92-94).  q is used by Coulomb classes only.
+ *
+ * Per-bond arrays (the bond-order classes and REBO2, whose reference kernels
+ * fill them: bop_kernel.f90:1370-1373,1406-1408,1511-1512,1593-1594 and
+ * bop_kernel_rebo2.f90:2546-2547,2571-2572,2762-2763,2855-2856; the other
+ * classes return AGPU_ERR_UNSUPPORTED for a non-NULL one, as
+ * gen_factory.py:303-304 does) hold one entry per neighbour-list entry, in
+ * the CSR order agpu_neighbors_get exports (row i = offsets[i] ..
+ * offsets[i+1]-1).  That is the packed order in which the reference's Python
+ * layer returns epot_per_bond and wpot_per_bond (potential.c:587-611,
+ * neighbors_wrap.f90:498-547); f_per_bond is packed the same way here.  The
+ * arrays need room for the list total after the update
+ * (agpu_neighbors_get_size; the reference sizes them nat*avgn,
+ * default_compute_func.f90:37-44, which bounds it) and are accumulated (+=):
+ *   epot_per_bond[e]      += 1/2 fc (V_R + b V_A) of bond e (bond-order
+ *                            classes) or the whole bond energy fc (V_R +
+ *                            bbar V_A) on the half-pair entry (REBO2);
+ *   f_per_bond[3e + x]    += the pair-term force dE/dr r_hat (no bond-order
+ *                            part), as the reference;
+ *   wpot_per_bond[9e + q] += the virial of every term evaluated for that
+ *                            bond (pair, bond order, screening, REBO2
+ *                            coordination, conjugation and torsion). */
 int agpu_potential_energy_and_forces(agpu_potential *pot, agpu_particles *p,
                                      agpu_neighbors *nl, const double *q, double *epot,
                                      double *f, double *wpot, const int *mask,

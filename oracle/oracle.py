@@ -296,12 +296,24 @@ def _juslin_struct(j):
     return s
 
 
-def tersoff(params, r, el, cell, nl, mask=None, per_at=False, nthreads=1):
+def _per_bond_arrays(nbond):
+    return np.zeros(nbond), np.zeros((nbond, 3)), np.zeros(9 * nbond)
+
+
+def _per_bond_out(epb, fpb, wpb):
+    """(epot_per_bond, f_per_bond, wpot_per_bond) in list-entry order, the
+    virial as W[e, a, b] (stored W(a, b) at 9 e + a + 3 b)."""
+    return [epb, fpb, wpb.reshape(-1, 3, 3).transpose(0, 2, 1).copy()]
+
+
+def tersoff(params, r, el, cell, nl, mask=None, per_at=False, nthreads=1, per_bond=False):
     """Energy, forces and virial; ``el`` holds database element indices (-1 = none).
     ``params`` is a Tersoff parameter dict, or a Kumagai or Brenner one
     (``"family": "kumagai"`` / ``"brenner"``) for those forms of the same
     kernel; a dict with the screening keys (or1, or2, bor1, bor2, Cmin, Cmax)
-    selects the screened variant of its family."""
+    selects the screened variant of its family.  ``per_bond`` appends the
+    per-bond energy, pair force and virial, one entry per list entry
+    (bop_kernel.f90:1370-1373,1406-1408,1511-1512,1593-1594)."""
     r = _c(r, np.float64)
     nat = len(r)
     el = _c(el, np.int32)
@@ -321,16 +333,20 @@ def tersoff(params, r, el, cell, nl, mask=None, per_at=False, nthreads=1):
     epa = np.zeros(nat) if per_at else None
     wpa = np.zeros(9 * nat) if per_at else None
     m = _c(mask, np.int32) if mask is not None else None
+    pb = _per_bond_arrays(len(nbr)) if per_bond else (None, None, None)
     err = lib().orc_tersoff(ctypes.byref(s), nat, _d(r), _i(el), _d(cell), _l(offsets),
                             _i(nbr), _i(dc), _i(m) if m is not None else None,
                             ctypes.byref(epot), _d(f), _d(w),
                             _d(epa) if per_at else None, _d(wpa) if per_at else None,
+                            *[_d(a) if a is not None else None for a in pb],
                             int(nthreads))
     _check(err, "tersoff")
     wpot = w.reshape(3, 3).T.copy()
     out = [epot.value, f, wpot]
     if per_at:
         out += [epa, wpa.reshape(nat, 3, 3).transpose(0, 2, 1).copy()]
+    if per_bond:
+        out += _per_bond_out(*pb)
     return tuple(out)
 
 
@@ -656,11 +672,13 @@ class Rebo2:
         _check(lib().orc_rebo2_default_table(w, _d(out)), "rebo2 default table")
         return out.reshape((5, 5, 10) if tri else (6, 6))
 
-    def __call__(self, r, Z, cell, nl, per_at=False, nthreads=1):
-        """Energy, forces and virial (plus per-atom energies/virials)."""
+    def __call__(self, r, Z, cell, nl, per_at=False, nthreads=1, per_bond=False):
+        """Energy, forces and virial (plus per-atom energies/virials, and the
+        per-bond energy, pair force and virial on the half-pair entries,
+        bop_kernel_rebo2.f90:2546-2547,2571-2572,2762-2763)."""
         L = lib()
         L.orc_rebo2.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _ip, _dp, _lp, _ip, _ip,
-                                _dp, _dp, _dp, _dp, _dp, ctypes.c_int]
+                                _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_int]
         r = _c(r, np.float64)
         nat = len(r)
         Z = _c(Z, np.int32)
@@ -674,12 +692,16 @@ class Rebo2:
         w = np.zeros(9)
         epa = np.zeros(nat) if per_at else None
         wpa = np.zeros(9 * nat) if per_at else None
+        pb = _per_bond_arrays(len(nbr)) if per_bond else (None, None, None)
         err = L.orc_rebo2(self._h, nat, _d(r), _i(Z), _d(cell), _l(offsets), _i(nbr), _i(dc),
                           ctypes.byref(epot), _d(f), _d(w), _d(epa) if per_at else None,
-                          _d(wpa) if per_at else None, int(nthreads))
+                          _d(wpa) if per_at else None, *[_d(a) if a is not None else None for a in pb],
+                          int(nthreads))
         _check(err, "rebo2")
         wpot = w.reshape(3, 3).T.copy()
         out = [epot.value, f, wpot]
         if per_at:
             out += [epa, wpa.reshape(nat, 3, 3).transpose(0, 2, 1).copy()]
+        if per_bond:
+            out += _per_bond_out(*pb)
         return tuple(out)

@@ -76,7 +76,8 @@ int orc_tersoff(const orc_tersoff_params *p, int nat, const double *r,
                 const int *el, const double *cell, const int64_t *offsets,
                 const int *nbr, const int *dc, const int *mask,
                 double *epot, double *f, double *wpot, double *epot_per_at,
-                double *wpot_per_at, int nthreads);
+                double *wpot_per_at, double *epot_per_bond, double *f_per_bond,
+                double *wpot_per_bond, int nthreads);
 
 /* ---- splines / EAM ---- */
 typedef struct {
@@ -114,7 +115,8 @@ void orc_rebo2_free(orc_rebo2_t *p);
 int orc_rebo2(const orc_rebo2_t *p, int nat, const double *r, const int *Z,
               const double *cell, const int64_t *offsets, const int *nbr, const int *dc,
               double *epot, double *f, double *wpot, double *epot_per_at,
-              double *wpot_per_at, int nthreads);
+              double *wpot_per_at, double *epot_per_bond, double *f_per_bond,
+              double *wpot_per_bond, int nthreads);
 int orc_rebo2_eval_g(const orc_rebo2_t *p, int ktyp, double costh, double n, double *out3);
 int orc_rebo2_eval_table(const orc_rebo2_t *p, int which, double x, double y, double z,
                          double *out4);

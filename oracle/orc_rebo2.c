@@ -687,6 +687,7 @@ typedef struct {
     int dc[3];
     double len, nm[3], fc, dfc;
     double fx, dfx;     /* conjugation F(x_ik) and dF/dx for carbon k, else 0 */
+    int64_t jn;          /* list entry of the bond (nbb, bop_kernel_rebo2.f90) */
 } rbond;
 
 static int dc_is_pos(const int *dc)
@@ -709,7 +710,8 @@ static inline void outer_add(double *w, double s, const double *a, const double 
 int orc_rebo2(const orc_rebo2_t *p, int nat, const double *r, const int *Z,
               const double *cell, const int64_t *offsets, const int *nbr, const int *dc,
               double *epot, double *f, double *wpot, double *epot_per_at,
-              double *wpot_per_at, int nthreads)
+              double *wpot_per_at, double *epot_per_bond, double *f_per_bond,
+              double *wpot_per_bond, int nthreads)
 {
     int *ktyp = malloc(sizeof(int) * (nat > 0 ? nat : 1));
     int64_t *bseed = malloc(sizeof(int64_t) * (nat + 1));
@@ -758,7 +760,7 @@ int orc_rebo2(const orc_rebo2_t *p, int nat, const double *r, const int *Z,
                 continue;
             }
             for (int x = 0; x < 3; ++x) b.nm[x] = rij[x] / b.len;
-            b.j = j; b.typ = t;
+            b.j = j; b.typ = t; b.jn = s;
             b.dc[0] = d[0]; b.dc[1] = d[1]; b.dc[2] = d[2];
             b.fx = 0.0; b.dfx = 0.0;
             bd[nb++] = b;
@@ -1192,6 +1194,10 @@ int orc_rebo2(const orc_rebo2_t *p, int nat, const double *r, const int *Z,
                 double hlfvij = fcarij * (frij + baveij * faij) / 2;
                 pe[i] += hlfvij;
                 pe[j] += hlfvij;
+                /* per-bond properties on the half pair's entry i->j (one
+                 * thread per centre i, so no two threads share an entry):
+                 * bop_kernel_rebo2.f90:2546-2547 */
+                if (epot_per_bond) epot_per_bond[bij_->jn] += 2 * hlfvij;
                 double dffac = dfrijr * fcarij + baveij * dfaijr * fcarij + frij * dfcarijr +
                                baveij * faij * dfcarijr;
                 double df[3];
@@ -1200,6 +1206,9 @@ int orc_rebo2(const orc_rebo2_t *p, int nat, const double *r, const int *Z,
                     fi[x] += df[x];
                     fj[x] -= df[x];
                 }
+                /* :2571-2572, the pair term only */
+                if (f_per_bond)
+                    for (int x = 0; x < 3; ++x) f_per_bond[3 * bij_->jn + x] += df[x];
                 for (int y = 0; y < 3; ++y)
                     for (int x = 0; x < 3; ++x)
                         wij[x + 3 * y] = wij[x + 3 * y] + rij[x] * df[y] - dfbij * wijb[x + 3 * y] -
@@ -1224,6 +1233,9 @@ int orc_rebo2(const orc_rebo2_t *p, int nat, const double *r, const int *Z,
                     for (int x = 0; x < 3; ++x) fl[3 * l + x] += -dfbji * dbjdl[jlc][x];
                 }
                 for (int k = 0; k < 9; ++k) wl[k] += wij[k];
+                /* :2762-2763 */
+                if (wpot_per_bond)
+                    for (int k = 0; k < 9; ++k) wpot_per_bond[9 * bij_->jn + k] += wij[k];
                 if (wal) {
                     for (int k = 0; k < 9; ++k) {
                         wal[9 * i + k] += wij[k] / 2;

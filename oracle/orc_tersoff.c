@@ -57,6 +57,7 @@ typedef struct {
     double len, nm[3], fc, dfc;     /* fc: the pair (ar) cutoff */
     double fcbo, dfcbo;             /* bond-order cutoff (= fc unscreened) */
     int64_t s0, s1;                 /* screening records [s0, s1) */
+    int64_t jn;                     /* list entry of the bond (nbb, bop_kernel.f90:1030) */
 } bond_t;
 
 /* one third atom k screening bond ij: its list entry and the derivative
@@ -284,9 +285,13 @@ int orc_tersoff(const orc_tersoff_params *p, int nat, const double *r,
                 const int *el, const double *cell, const int64_t *offsets,
                 const int *nbr, const int *dc, const int *mask,
                 double *epot, double *f, double *wpot, double *epot_per_at,
-                double *wpot_per_at, int nthreads)
+                double *wpot_per_at, double *epot_per_bond, double *f_per_bond,
+                double *wpot_per_bond, int nthreads)
 {
     const int nel = p->nel;
+    /* per-bond arrays are indexed by the bond's list entry; every directed
+     * bond belongs to one centre, hence to one thread */
+    double *epb = epot_per_bond, *fpb = f_per_bond, *wpb = wpot_per_bond;
     /* loop1: internal bond lists */
     bond_t *bonds = malloc(sizeof(bond_t) * (offsets[nat] > 0 ? offsets[nat] : 1));
     int64_t *bseed = malloc(sizeof(int64_t) * (nat + 1));
@@ -309,14 +314,14 @@ int orc_tersoff(const orc_tersoff_params *p, int nat, const double *r,
                     bond_t *b = &bonds[nb];
                     if (screened_bond(p, i, jn, rij, rl, t, r, cell, offsets, nbr, dc, b, &rec, &nrec,
                                       &rcap)) {
-                        b->j = j; b->typ = t;
+                        b->j = j; b->typ = t; b->jn = jn;
                         b->dc[0] = dc[3 * jn]; b->dc[1] = dc[3 * jn + 1]; b->dc[2] = dc[3 * jn + 2];
                         for (int x = 0; x < 3; ++x) b->nm[x] = rij[x] / b->len;
                         ++nb;
                     }
                 } else if (rl < p->r2[t] * p->r2[t]) {
                     bond_t *b = &bonds[nb];
-                    b->j = j; b->typ = t;
+                    b->j = j; b->typ = t; b->jn = jn;
                     b->dc[0] = dc[3 * jn]; b->dc[1] = dc[3 * jn + 1]; b->dc[2] = dc[3 * jn + 2];
                     b->len = sqrt(rl);
                     for (int x = 0; x < 3; ++x) b->nm[x] = rij[x] / b->len;
@@ -510,6 +515,8 @@ int orc_tersoff(const orc_tersoff_params *p, int nat, const double *r,
                 double dffac = 0.5 * fcarij * (VRij + bij * VAij);
                 pe[i] += dffac;
                 pe[j] += dffac;
+                /* bop_kernel.f90:1370-1373 */
+                if (epb) epb[bnd_ij->jn] += dffac;
                 dffac = 0.5 * (dVRij * fcarij + bij * dVAij * fcarij + VRij * dfcarijr +
                                bij * VAij * dfcarijr);
                 double df[3];
@@ -518,6 +525,9 @@ int orc_tersoff(const orc_tersoff_params *p, int nat, const double *r,
                     fi[x] += df[x];
                     fj[x] -= df[x];
                 }
+                /* bop_kernel.f90:1406-1408: the pair term only */
+                if (fpb)
+                    for (int x = 0; x < 3; ++x) fpb[3 * bnd_ij->jn + x] += df[x];
                 for (int a = 0; a < 3; ++a)
                     for (int b = 0; b < 3; ++b)
                         wij[a + 3 * b] = wij[a + 3 * b] + rij[a] * df[b] - dbij_dzij * wijb[a + 3 * b];
@@ -561,6 +571,9 @@ int orc_tersoff(const orc_tersoff_params *p, int nat, const double *r,
                     }
                 }
                 for (int a = 0; a < 9; ++a) w[a] += wij[a];
+                /* bop_kernel.f90:1511-1512 */
+                if (wpb)
+                    for (int a = 0; a < 9; ++a) wpb[9 * bnd_ij->jn + a] += wij[a];
                 if (wpa) {
 #pragma omp critical
                     for (int a = 0; a < 9; ++a) {
@@ -607,6 +620,9 @@ int orc_tersoff(const orc_tersoff_params *p, int nat, const double *r,
                             for (int c = 0; c < 3; ++c) wij[a + 3 * c] += rjk[a] * d2[c];
                     }
                     for (int a = 0; a < 9; ++a) w[a] += wij[a];
+                    /* bop_kernel.f90:1593-1594 */
+                    if (wpb)
+                        for (int a = 0; a < 9; ++a) wpb[9 * b->jn + a] += wij[a];
                     if (wpa) {
 #pragma omp critical
                         for (int a = 0; a < 9; ++a) {
