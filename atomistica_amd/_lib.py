@@ -38,6 +38,7 @@ KIND_BRENNER_SCR = 9
 KIND_PME = 10
 KIND_JUSLIN = 11
 KIND_JUSLIN_SCR = 12
+KIND_EAM = 13
 REBO2_ZERO_TABLES = 1
 REBO2_DIHEDRAL = 2
 
@@ -137,6 +138,7 @@ PROTOTYPES = {
     "agpu_brenner_scr_create": [ctypes.POINTER(vp), vp, vp],
     "agpu_juslin_scr_create": [ctypes.POINTER(vp), vp, vp],
     "agpu_eam_alloy_create": [ctypes.POINTER(vp), ctypes.c_char_p, ctypes.c_char_p],
+    "agpu_eam_create": [ctypes.POINTER(vp), ctypes.c_char_p, ctypes.c_char_p],
     "agpu_ewald_create": [ctypes.POINTER(vp), d, d, d],
     "agpu_pme_create": [ctypes.POINTER(vp), d, i, ip],
     "agpu_rebo2_create": [ctypes.POINTER(vp), i],

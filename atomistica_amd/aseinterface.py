@@ -280,6 +280,7 @@ def _make(cls, name):
 # one calculator class per GPU potential class (aseinterface.py:493-505)
 Tersoff = _make(native.Tersoff, "Tersoff")
 TabulatedAlloyEAM = _make(native.TabulatedAlloyEAM, "TabulatedAlloyEAM")
+TabulatedEAM = _make(native.TabulatedEAM, "TabulatedEAM")
 Ewald = _make(native.Ewald, "Ewald")
 Rebo2 = _make(native.Rebo2, "Rebo2")
 Kumagai = _make(native.Kumagai, "Kumagai")

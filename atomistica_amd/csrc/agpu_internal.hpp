@@ -258,6 +258,11 @@ int eam_compute(agpu_potential *pot, const agpu_eval_args *a);
 // selects launch variant k (lanes per atom g, entries per lane u) when k is
 // in range and returns the number of variants (0: this binding has none)
 int eam_variant(agpu_potential *pot, int k, int *g, int *u);
+// unscreened bond-order classes: k = -1 returns the number of variants (2;
+// 0 for other classes), k = 0 selects the row path (a lane per row entry,
+// reverse entries found in the neighbour's row), k = 1 the bond-compacting
+// path (bond slots, reverse-slot pass, slot gather)
+int bop_variant(agpu_potential *pot, int k);
 int ewald_bind(agpu_potential *pot);
 int ewald_compute(agpu_potential *pot, const agpu_eval_args *a);
 int ewald_potential(agpu_potential *pot, const double *q, double *phi, int flags);
@@ -275,8 +280,10 @@ int rebo2_compute(agpu_potential *pot, const agpu_eval_args *a);
 // (double): column 0 -> *epot (+=, or = with set_epot), columns 1..9 ->
 // wpot[0..8] (+=), columns 10.. -> extra[0..] (+=).  Runs on the library
 // stream.
+// tmp: room for the second-level records, ceil(nblk / 256) * w doubles
+// (NULL: the library's own buffer, grown on demand).
 int reduce_ev(const double *partials, int nblk, int w, double *epot, double *wpot, double *extra,
-              bool set_epot = false);
+              bool set_epot = false, double *tmp = nullptr);
 
 // *out = max(*out, max(v[0..n))) on the library stream (device pointers)
 int int_max(const int *v, int n, int *out);

@@ -9,6 +9,10 @@
 //     wavefront) x row entries per lane and batch U (loads in flight).  The
 //     per-atom sums change their association order with G, so results move
 //     at the level of double rounding only.
+//   - unscreened bond-order classes: the row path (a lane per row entry,
+//     reverse entries found in the neighbour's row) vs the bond-compacting
+//     path (bond slots, reverse-slot pass).  The per-atom sums keep their
+//     order; the virial sums differ at the level of double rounding.
 //
 // The reference has no counterpart: its kernels are CPU loops with no
 // launch shape to choose.
@@ -86,7 +90,8 @@ static int autotune(agpu_potential *pot, agpu_particles *p, agpu_neighbors *nl, 
         AGPU_CHECK(nl_update(nl, p, 1));
         rep += pick == 1 ? "nl_pairs -> per_atom\n" : "nl_pairs -> bin_block\n";
     }
-    if (pot->kind == AGPU_KIND_EAM_ALLOY && p->nat > 0) {
+    const bool eam = pot->kind == AGPU_KIND_EAM_ALLOY || pot->kind == AGPU_KIND_EAM;
+    if (eam && p->nat > 0) {
         const int nv = eam_variant(pot, -1, nullptr, nullptr);
         if (nv > 0) {
             DevBuf out;
@@ -114,6 +119,33 @@ static int autotune(agpu_potential *pot, agpu_particles *p, agpu_neighbors *nl, 
             eam_variant(pot, pick, &g, &u);
             append(rep, "eam -> lanes_per_atom %.0f entries_per_lane %.0f\n", g, u);
         }
+    }
+    const int nb = bop_variant(pot, -1);
+    if (nb > 0 && p->nat > 0) {
+        DevBuf out;
+        const size_t nat = (size_t)p->nat;
+        AGPU_CHECK(out.reserve(sizeof(double) * (3 * nat + 10)));
+        agpu_eval_args a{};
+        a.epot = out.as<double>();
+        a.wpot = a.epot + 1;
+        a.f = a.epot + 10;
+        double best = 0.0;
+        int pick = -1;
+        for (int k = 0; k < nb; ++k) {
+            bop_variant(pot, k);
+            double ms = 0.0;
+            AGPU_CHECK(time_calls(ev, reps, [&] {
+                AGPU_CHECK(zero_dev(out.ptr, sizeof(double) * (3 * nat + 10)));
+                return tersoff_compute(pot, &a);
+            }, &ms));
+            append(rep, k == 0 ? "bond order rows %.4f ms\n" : "bond order compacted %.4f ms\n", ms);
+            if (pick < 0 || ms < best) {
+                best = ms;
+                pick = k;
+            }
+        }
+        bop_variant(pot, pick);
+        rep += pick == 0 ? "bond order -> rows\n" : "bond order -> compacted\n";
     }
     return AGPU_OK;
 }
@@ -147,11 +179,15 @@ extern "C" int agpu_potential_set_launch(agpu_potential *pot, agpu_neighbors *nl
     if (!pot->bound || pot->nl != nl)
         return set_error(AGPU_ERR_STATE, "set_launch: potential not bound to this neighbour list");
     if (pair_mode < 0 || pair_mode > 2) return set_error(AGPU_ERR_ARG, "set_launch: pair_mode must be 0, 1 or 2");
-    const int nv = pot->kind == AGPU_KIND_EAM_ALLOY ? eam_variant(pot, -1, nullptr, nullptr) : 0;
+    const bool eam = pot->kind == AGPU_KIND_EAM_ALLOY || pot->kind == AGPU_KIND_EAM;
+    const int nv = eam ? eam_variant(pot, -1, nullptr, nullptr) : bop_variant(pot, -1);
     if (variant < -1 || variant >= (nv > 0 ? nv : 1))
         return set_error(AGPU_ERR_ARG, "set_launch: no such kernel variant for this potential");
     nl->pair_mode = pair_mode;
-    if (variant >= 0 && nv > 0) eam_variant(pot, variant, nullptr, nullptr);
+    if (variant >= 0 && nv > 0) {
+        if (eam) eam_variant(pot, variant, nullptr, nullptr);
+        else bop_variant(pot, variant);
+    }
     if (nvariants) *nvariants = nv;
     return AGPU_OK;
 }

@@ -200,7 +200,8 @@ k_reduce_rec(const double *__restrict__ part, int nblk, int w, int chunk,
 // process-lifetime scratch (never freed: no device frees from static destructors)
 static DevBuf *g_red_tmp = new DevBuf();
 
-int reduce_ev(const double *partials, int nblk, int w, double *epot, double *wpot, double *extra, bool set_epot)
+int reduce_ev(const double *partials, int nblk, int w, double *epot, double *wpot, double *extra, bool set_epot,
+              double *tmp)
 {
     if (nblk <= 0 || w <= 0) return AGPU_OK;
     if (w > RED_W_MAX) return set_error(AGPU_ERR_ARG, "reduce_ev: record too wide");
@@ -212,10 +213,13 @@ int reduce_ev(const double *partials, int nblk, int w, double *epot, double *wpo
                     (double *)nullptr, epot, wpot, extra, (int)set_epot);
         return AGPU_OK;
     }
-    AGPU_CHECK(g_red_tmp->reserve(sizeof(double) * (size_t)n1 * w));
+    if (!tmp) {
+        AGPU_CHECK(g_red_tmp->reserve(sizeof(double) * (size_t)n1 * w));
+        tmp = g_red_tmp->as<double>();
+    }
     AGPU_LAUNCH("reduce_ev", k_reduce_rec, dim3(n1), dim3(256), 0, s, partials, nblk, w, chunk,
-                g_red_tmp->as<double>(), (double *)nullptr, (double *)nullptr, (double *)nullptr, 0);
-    AGPU_LAUNCH("reduce_ev", k_reduce_rec, dim3(1), dim3(256), 0, s, g_red_tmp->as<double>(), n1, w,
+                tmp, (double *)nullptr, (double *)nullptr, (double *)nullptr, 0);
+    AGPU_LAUNCH("reduce_ev", k_reduce_rec, dim3(1), dim3(256), 0, s, (const double *)tmp, n1, w,
                 n1, (double *)nullptr, epot, wpot, extra, (int)set_epot);
     return AGPU_OK;
 }
@@ -641,7 +645,8 @@ int agpu_potential_bind_to(agpu_potential *pot, agpu_particles *p, agpu_neighbor
     case AGPU_KIND_BRENNER_SCR:
     case AGPU_KIND_JUSLIN:
     case AGPU_KIND_JUSLIN_SCR: rc = tersoff_bind(pot); break;
-    case AGPU_KIND_EAM_ALLOY: rc = eam_bind(pot); break;
+    case AGPU_KIND_EAM_ALLOY:
+    case AGPU_KIND_EAM: rc = eam_bind(pot); break;
     case AGPU_KIND_EWALD:
     case AGPU_KIND_PME: rc = ewald_bind(pot); break;
     case AGPU_KIND_REBO2: rc = rebo2_bind(pot); break;
@@ -668,7 +673,8 @@ static int stage_and_call(agpu_potential *pot, agpu_particles *p, agpu_neighbors
         case AGPU_KIND_BRENNER_SCR:
         case AGPU_KIND_JUSLIN:
         case AGPU_KIND_JUSLIN_SCR: rc = tersoff_compute(pot, &a); break;
-        case AGPU_KIND_EAM_ALLOY: rc = eam_compute(pot, &a); break;
+        case AGPU_KIND_EAM_ALLOY:
+        case AGPU_KIND_EAM: rc = eam_compute(pot, &a); break;
         case AGPU_KIND_EWALD:
         case AGPU_KIND_PME: rc = ewald_compute(pot, &a); break;
         case AGPU_KIND_REBO2: rc = rebo2_compute(pot, &a); break;
@@ -740,7 +746,8 @@ static int stage_and_call(agpu_potential *pot, agpu_particles *p, agpu_neighbors
     case AGPU_KIND_BRENNER_SCR:
     case AGPU_KIND_JUSLIN:
     case AGPU_KIND_JUSLIN_SCR: rc = tersoff_compute(pot, &d); break;
-    case AGPU_KIND_EAM_ALLOY: rc = eam_compute(pot, &d); break;
+    case AGPU_KIND_EAM_ALLOY:
+    case AGPU_KIND_EAM: rc = eam_compute(pot, &d); break;
     case AGPU_KIND_EWALD:
     case AGPU_KIND_PME: rc = ewald_compute(pot, &d); break;
     case AGPU_KIND_REBO2: rc = rebo2_compute(pot, &d); break;
@@ -773,11 +780,14 @@ int agpu_potential_energy_and_forces(agpu_potential *pot, agpu_particles *p, agp
     if (!pot || !p || !nl) return set_error(AGPU_ERR_ARG, "energy_and_forces: NULL handle");
     // per-bond properties: the classes whose reference kernels fill them
     // (features per_bond: the bond-order families and REBO2)
-    const bool per_bond = pot->kind != AGPU_KIND_EAM_ALLOY && pot->kind != AGPU_KIND_EWALD &&
-                          pot->kind != AGPU_KIND_PME;
+    const bool per_bond = pot->kind != AGPU_KIND_EAM_ALLOY && pot->kind != AGPU_KIND_EAM &&
+                          pot->kind != AGPU_KIND_EWALD && pot->kind != AGPU_KIND_PME;
     if ((epot_per_bond || f_per_bond || wpot_per_bond) && !per_bond)
         return set_error(AGPU_ERR_UNSUPPORTED,
                          "energy_and_forces: per-bond properties are not supported by this class");
+    // TabulatedEAM has no mask feature (tabulated_eam.f90:24)
+    if (mask && pot->kind == AGPU_KIND_EAM)
+        return set_error(AGPU_ERR_UNSUPPORTED, "energy_and_forces: TabulatedEAM takes no mask");
     if ((pot->kind == AGPU_KIND_EWALD || pot->kind == AGPU_KIND_PME) && !q)
         return set_error(AGPU_ERR_ARG, "energy_and_forces: Coulomb class needs charges");
     if (!epot || !f || !wpot) return set_error(AGPU_ERR_ARG, "energy_and_forces: epot, f and wpot are required");

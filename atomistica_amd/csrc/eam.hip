@@ -1,8 +1,12 @@
-// Tabulated alloy EAM (setfl) on gfx950.
+// Tabulated alloy EAM (setfl) and tabulated single-element EAM (funcfl) on
+// gfx950.
 //
 // Semantics: tabulated_alloy_eam.f90 -- reader :147-259, bind_to :297-350,
 // kernel :423-627 -- with the natural cubic spline tables of
-// simple_spline.f90:127-615.  Work decomposition (this library's own):
+// simple_spline.f90:127-615; TabulatedEAM (tabulated_eam.f90: reader
+// :141-214, kernel :334-511) is the same decomposition with one element and
+// the pair term Z(r)^2 / r of the effective-charge table.  Work
+// decomposition (this library's own):
 //
 //   k_eam_density  one wavefront per atom, lanes stride over its neighbour
 //                  row: rho_i = sum rho_{el j}(r), wave reduction, rho clamped
@@ -102,6 +106,7 @@ constexpr int EAM_MAX_EL = 8;
 
 struct EamDev {
     int nel;
+    int funcfl;                   // pair table is Z(r): r phi = Z^2 (TabulatedEAM)
     int tF[EAM_MAX_EL], tR[EAM_MAX_EL], tP[EAM_MAX_EL * EAM_MAX_EL];
     double cutoff_sq;
     SplHdr h1F, h1R, h1P;     // the three tables of a single-element potential
@@ -109,6 +114,7 @@ struct EamDev {
 
 struct EamState {
     std::string fn, elements;
+    bool funcfl = false;          // TabulatedEAM (funcfl file)
     int nel = 0;
     std::vector<std::string> names;
     std::vector<int> Zs;
@@ -228,6 +234,63 @@ static int read_setfl(EamState &st)
     return AGPU_OK;
 }
 
+// Units of the reference (src/support/Units.f90:74,76)
+constexpr double HARTREE_REF = 27.2113961;
+constexpr double BOHR_REF = 0.529177249;
+
+// funcfl reader, tabulated_eam_init (tabulated_eam.f90:141-214): a comment
+// line, "Z mass a0 lattice", "nF dF nr dr cutoff", then list-directed reads
+// of F(rho) (nF from 0), Z(r) and rho(r) (nr each, from 0).  Z is scaled by
+// sqrt(Hartree Bohr / 2) (:178), so the pair energy of a directed pair is
+// Z^2 / r.
+static int read_funcfl(EamState &st)
+{
+    std::ifstream in(st.fn);
+    if (!in) return set_error(AGPU_ERR_IO, "TabulatedEAM: cannot open '" + st.fn + "'");
+    Tok tk;
+    std::string l;
+    while (std::getline(in, l)) tk.lines.push_back(l);
+    if (tk.lines.size() < 4) return set_error(AGPU_ERR_IO, "TabulatedEAM: file too short");
+    std::string dummy;
+    tk.next_line(dummy);
+    tk.next_line(l);
+    int Z = 0;
+    {
+        std::istringstream is(l);
+        is >> Z;
+        if (!is || Z <= 0 || Z >= kNumElements) return set_error(AGPU_ERR_IO, "TabulatedEAM: bad element line");
+    }
+    tk.next_line(l);
+    int nF = 0, nr = 0;
+    double dF = 0, dr = 0, cutoff = 0;
+    {
+        for (char &c : l)
+            if (c == ',') c = ' ';
+        std::istringstream is(l);
+        is >> nF >> dF >> nr >> dr >> cutoff;
+        if (!is || nF < 3 || nr < 3) return set_error(AGPU_ERR_IO, "TabulatedEAM: bad grid line");
+    }
+    st.funcfl = true;
+    st.nel = 1;
+    st.names = {kElementNames[Z]};
+    st.Zs = {Z};
+    st.cutoff = cutoff;
+    st.F.resize(1);
+    st.rho.resize(1);
+    st.phi.resize(1);
+    std::vector<double> y(nF), z(nr), rho(nr);
+    for (int k = 0; k < nF; ++k)
+        if (!tk.number(y[k])) return set_error(AGPU_ERR_IO, "TabulatedEAM: truncated F table");
+    for (int k = 0; k < nr; ++k)
+        if (!tk.number(z[k])) return set_error(AGPU_ERR_IO, "TabulatedEAM: truncated Z table");
+    for (int k = 0; k < nr; ++k)
+        if (!tk.number(rho[k])) return set_error(AGPU_ERR_IO, "TabulatedEAM: truncated rho table");
+    spline_build(st.F[0], y, dF, 1.0);
+    spline_build(st.phi[0], z, dr, std::sqrt(0.5 * HARTREE_REF * BOHR_REF));
+    spline_build(st.rho[0], rho, dr, 1.0);
+    return AGPU_OK;
+}
+
 static int upload_tables(EamState &st)
 {
     std::vector<double> rec64;
@@ -248,6 +311,7 @@ static int upload_tables(EamState &st)
     for (int a = 0; a < st.nel; ++a)
         for (int b = 0; b < st.nel; ++b) st.dev.tP[a * EAM_MAX_EL + b] = add(st.phi[a * st.nel + b]);
     st.dev.nel = st.nel;
+    st.dev.funcfl = st.funcfl ? 1 : 0;
     st.dev.cutoff_sq = st.cutoff * st.cutoff;
     st.dev.h1F = st.hdr[st.dev.tF[0]];
     st.dev.h1R = st.hdr[st.dev.tR[0]];
@@ -500,6 +564,11 @@ k_eam_force(EamDev P, const SplHdr *__restrict__ hdr, const R *__restrict__ rec,
                     const double ad = d2 * rrd;
                     R phi, dphi;
                     spl_fdf_x<R>(hdr_of<ONE>(P, hdr, 2, eli, elj[u]), rec, ad, phi, dphi);
+                    if (P.funcfl) {
+                        // effective charge Z(r): r phi = Z^2 (tabulated_eam.f90:446-450)
+                        dphi = (R)2 * phi * dphi;
+                        phi = phi * phi;
+                    }
                     const R rr = (R)rrd;
                     const R pp = (dphi - phi * rr) * rr;
                     const R drho_j = spl_df<R>(hdr_of<ONE>(P, hdr, 1, elj[u], 0), rec, ad, err);
@@ -731,13 +800,35 @@ int eam_compute(agpu_potential *pot, const agpu_eval_args *a)
     AGPU_HIP(hipMemcpyAsync(pot->stats.h, st->stats.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
     AGPU_HIP(hipStreamSynchronize(s));
     if (pot->stats.h[0] & DERR_SPLINE_RANGE)
-        return set_error(AGPU_ERR_SPLINE, "TabulatedAlloyEAM: spline argument outside the defined interval");
+        return set_error(AGPU_ERR_SPLINE, std::string(st->funcfl ? "TabulatedEAM" : "TabulatedAlloyEAM") +
+                                              ": spline argument outside the defined interval");
     return AGPU_OK;
 }
 
 }  // namespace agpu
 
 using namespace agpu;
+
+extern "C" int agpu_eam_create(agpu_potential **pot, const char *fn, const char *elements)
+{
+    if (!pot || !fn) return set_error(AGPU_ERR_ARG, "eam_create: NULL argument");
+    EamState *st = new EamState();
+    st->fn = fn;
+    st->elements = elements ? elements : "*";
+    int rc = read_funcfl(*st);
+    if (rc == AGPU_OK) rc = upload_tables(*st);
+    if (rc != AGPU_OK) {
+        delete st;
+        return rc;
+    }
+    agpu_potential *p = new agpu_potential();
+    p->kind = AGPU_KIND_EAM;
+    p->cutoff = st->cutoff;
+    p->impl = st;
+    p->free_impl = eam_free;
+    *pot = p;
+    return AGPU_OK;
+}
 
 extern "C" int agpu_eam_alloy_create(agpu_potential **pot, const char *fn, const char *elements)
 {

@@ -127,7 +127,8 @@ static int md_force(agpu_md *md)
     case AGPU_KIND_BRENNER_SCR:
     case AGPU_KIND_JUSLIN:
     case AGPU_KIND_JUSLIN_SCR: return tersoff_compute(md->pot, &a);
-    case AGPU_KIND_EAM_ALLOY: return eam_compute(md->pot, &a);
+    case AGPU_KIND_EAM_ALLOY:
+    case AGPU_KIND_EAM: return eam_compute(md->pot, &a);
     default: return set_error(AGPU_ERR_UNSUPPORTED, "MD driver: Tersoff-family and EAM potentials only");
     }
 }
@@ -137,7 +138,8 @@ static int md_rebuild(agpu_md *md)
     hipStream_t s = stream();
     md->nl->held = false;
     // EAM reads the staged rows; the bond-order kernels need the CSR rows
-    AGPU_CHECK(nl_update(md->nl, md->p, 1, md->pot->kind != AGPU_KIND_EAM_ALLOY));
+    const bool eam = md->pot->kind == AGPU_KIND_EAM_ALLOY || md->pot->kind == AGPU_KIND_EAM;
+    AGPU_CHECK(nl_update(md->nl, md->p, 1, !eam));
     md->nl->held = true;
     AGPU_HIP(hipMemcpyAsync(md->rref.ptr, md->p->d_r, sizeof(double) * 3 * md->nat, hipMemcpyDeviceToDevice, s));
     AGPU_CHECK(zero_dev(md->dmax.ptr, sizeof(unsigned) * 3));   // displacement, its copy, speed
@@ -235,7 +237,7 @@ extern "C" int agpu_md_create(agpu_md **out, agpu_potential *pot, agpu_particles
     if (!(pot->kind == AGPU_KIND_TERSOFF || pot->kind == AGPU_KIND_KUMAGAI || pot->kind == AGPU_KIND_BRENNER ||
           pot->kind == AGPU_KIND_TERSOFF_SCR || pot->kind == AGPU_KIND_KUMAGAI_SCR ||
           pot->kind == AGPU_KIND_BRENNER_SCR || pot->kind == AGPU_KIND_JUSLIN || pot->kind == AGPU_KIND_JUSLIN_SCR ||
-          pot->kind == AGPU_KIND_EAM_ALLOY))
+          pot->kind == AGPU_KIND_EAM_ALLOY || pot->kind == AGPU_KIND_EAM))
         return set_error(AGPU_ERR_UNSUPPORTED, "MD driver: Tersoff-family and EAM potentials only");
     const int n = p->nat;
     for (int i = 0; i < n; ++i)
@@ -311,7 +313,8 @@ extern "C" int agpu_md_run(agpu_md *md, int nsteps, int graph_steps, double *epo
         int K = 0;
         // the screened bond-order kernels size their records on the host
         // every step, so they always run step by step
-        const bool graphable = md->pot->kind == AGPU_KIND_EAM_ALLOY || !tersoff_screened(md->pot);
+        const bool graphable = md->pot->kind == AGPU_KIND_EAM_ALLOY || md->pot->kind == AGPU_KIND_EAM ||
+                               !tersoff_screened(md->pot);
         if (graphable && graph_steps > 0 && md->skin > 0.0) {
             const double room = 0.5 * md->skin - std::sqrt(d2);
             const double per = 1.25 * std::sqrt(v2) * md->dt;

@@ -517,6 +517,44 @@ k_atom_pairs(int nat, BinGeom g, CellArg C, double cutoff_sq, const SortedAtom *
         const bool hz0 = ORTHO && A.w == DC_ZERO;
         const size_t srow = (size_t)p * S;
         int cnt = 0;
+        // the candidates q0 .. q1 - 1 (atoms of consecutive bins of one
+        // shift u), in order
+        auto scan = [&](int q0, int q1, int ux, int uy, int uz) {
+            double au0 = 0.0, au1 = 0.0, au2 = 0.0;
+            if (ORTHO) {
+                au0 = C.A[0] * (double)ux;
+                au1 = C.A[4] * (double)uy;
+                au2 = C.A[8] * (double)uz;
+            }
+            const bool u_ok = dc_pack_ok(ux) && dc_pack_ok(uy) && dc_pack_ok(uz);
+            const int upk = u_ok ? dc_pack(ux, uy, uz) : -1;
+            for (int q = q0; q < q1; ++q) {
+                const SortedAtom Bq = atoms[q];
+                if (hz0 && Bq.w == DC_ZERO) {
+                    if (Bq.j == A.j && upk == DC_ZERO) continue;
+                    if (pair_d2_pre(A.x, A.y, A.z, Bq.x, Bq.y, Bq.z, au0, au1, au2) < cutoff_sq) {
+                        if (!u_ok) bad = true;
+                        if (cnt < S) stage[srow + cnt] = make_int2(Bq.j, upk);
+                        ++cnt;
+                    }
+                    continue;
+                }
+                int v0, v1, v2;
+                dc_unpack(Bq.w, v0, v1, v2);
+                const int s0 = w0 + ux - v0, s1 = w1 + uy - v1, s2 = w2 + uz - v2;
+                if (Bq.j == A.j && s0 == 0 && s1 == 0 && s2 == 0) continue;
+                if (pair_d2<ORTHO>(C, A.x, A.y, A.z, Bq.x, Bq.y, Bq.z, s0, s1, s2) < cutoff_sq) {
+                    if (!dc_pack_ok(s0) || !dc_pack_ok(s1) || !dc_pack_ok(s2)) bad = true;
+                    if (cnt < S) stage[srow + cnt] = make_int2(Bq.j, dc_pack(s0, s1, s2));
+                    ++cnt;
+                }
+            }
+        };
+        // a z column of the stencil inside the box is one range of the
+        // sorted atoms (bins z-major in memory, no wrap): same candidates,
+        // same order as bin by bin
+        const int z0 = hc[2] - g.nd[2], z1 = hc[2] + g.nd[2];
+        const bool zcol = z0 >= 1 && z1 <= g.nc[2];
         for (int ox = -g.nd[0]; ox <= g.nd[0]; ++ox) {
             int cx = hc[0] + ox, ux = 0;
             if (g.pbc[0]) {
@@ -535,6 +573,11 @@ k_atom_pairs(int nat, BinGeom g, CellArg C, double cutoff_sq, const SortedAtom *
                 } else if (cy < 1 || cy > g.nc[1]) {
                     continue;
                 }
+                const int bcol = ((cx - 1) * g.nc[1] + (cy - 1)) * g.nc[2];
+                if (zcol) {
+                    scan(cell_start[bcol + z0 - 1], cell_start[bcol + z1], ux, uy, 0);
+                    continue;
+                }
                 for (int oz = -g.nd[2]; oz <= g.nd[2]; ++oz) {
                     int cz = hc[2] + oz, uz = 0;
                     if (g.pbc[2]) {
@@ -544,37 +587,8 @@ k_atom_pairs(int nat, BinGeom g, CellArg C, double cutoff_sq, const SortedAtom *
                     } else if (cz < 1 || cz > g.nc[2]) {
                         continue;
                     }
-                    const int b = ((cx - 1) * g.nc[1] + (cy - 1)) * g.nc[2] + (cz - 1);
-                    const int q0 = cell_start[b], q1 = cell_start[b + 1];
-                    double au0 = 0.0, au1 = 0.0, au2 = 0.0;
-                    if (ORTHO) {
-                        au0 = C.A[0] * (double)ux;
-                        au1 = C.A[4] * (double)uy;
-                        au2 = C.A[8] * (double)uz;
-                    }
-                    const bool u_ok = dc_pack_ok(ux) && dc_pack_ok(uy) && dc_pack_ok(uz);
-                    const int upk = u_ok ? dc_pack(ux, uy, uz) : -1;
-                    for (int q = q0; q < q1; ++q) {
-                        const SortedAtom Bq = atoms[q];
-                        if (hz0 && Bq.w == DC_ZERO) {
-                            if (Bq.j == A.j && upk == DC_ZERO) continue;
-                            if (pair_d2_pre(A.x, A.y, A.z, Bq.x, Bq.y, Bq.z, au0, au1, au2) < cutoff_sq) {
-                                if (!u_ok) bad = true;
-                                if (cnt < S) stage[srow + cnt] = make_int2(Bq.j, upk);
-                                ++cnt;
-                            }
-                            continue;
-                        }
-                        int v0, v1, v2;
-                        dc_unpack(Bq.w, v0, v1, v2);
-                        const int s0 = w0 + ux - v0, s1 = w1 + uy - v1, s2 = w2 + uz - v2;
-                        if (Bq.j == A.j && s0 == 0 && s1 == 0 && s2 == 0) continue;
-                        if (pair_d2<ORTHO>(C, A.x, A.y, A.z, Bq.x, Bq.y, Bq.z, s0, s1, s2) < cutoff_sq) {
-                            if (!dc_pack_ok(s0) || !dc_pack_ok(s1) || !dc_pack_ok(s2)) bad = true;
-                            if (cnt < S) stage[srow + cnt] = make_int2(Bq.j, dc_pack(s0, s1, s2));
-                            ++cnt;
-                        }
-                    }
+                    const int b = bcol + (cz - 1);
+                    scan(cell_start[b], cell_start[b + 1], ux, uy, uz);
                 }
             }
         }
@@ -963,10 +977,15 @@ int nl_import(agpu_neighbors *nl, agpu_particles *p, const int64_t *seed, const 
     AGPU_CHECK(nl->offsets.reserve(sizeof(int) * (nat + 1)));
     AGPU_CHECK(nl->nbr.reserve(sizeof(int2) * ((size_t)total + 1)));
     AGPU_CHECK(nl->idx_sorted.reserve(sizeof(int) * (nat + 1)));
+    AGPU_CHECK(nl->rank.reserve(sizeof(int) * (nat + 1)));
     hipStream_t s = stream();
     AGPU_HIP(hipMemcpyAsync(nl->offsets.ptr, off.data(), sizeof(int) * (nat + 1), hipMemcpyHostToDevice, s));
     if (total) AGPU_HIP(hipMemcpyAsync(nl->nbr.ptr, rows.data(), sizeof(int2) * total, hipMemcpyHostToDevice, s));
-    if (nat) AGPU_LAUNCH("nl_iota", k_iota, dim3((nat + 255) / 256), dim3(256), 0, s, nat, nl->idx_sorted.as<int>());
+    // atoms in the caller's order: slot = atom
+    if (nat) {
+        AGPU_LAUNCH("nl_iota", k_iota, dim3((nat + 255) / 256), dim3(256), 0, s, nat, nl->idx_sorted.as<int>());
+        AGPU_LAUNCH("nl_iota", k_iota, dim3((nat + 255) / 256), dim3(256), 0, s, nat, nl->rank.as<int>());
+    }
     AGPU_HIP(hipStreamSynchronize(s));
     nl->external = true;
     nl->csr = true;

@@ -2,9 +2,16 @@
 // non-screened rebo2 class of the reference (src/potentials/bop/rebo2/),
 // with the optional dihedral (torsion) term of with_dihedral = .true.
 //
+// The kernels work on slots, the bin order of the neighbour-list builder
+// (slot p holds atom order[p]): bond records, coordinations, accumulators
+// and the half-pair list are indexed by slot, so the atoms a half pair
+// touches lie close together in memory whatever the caller's atom order.
+// Forces and per-atom properties go back to the caller's order at the end.
+//
 // Kernels (one stream, no host round trip inside a call):
 //   k_rb_zero     clear the 64-bit fixed-point accumulators
-//   k_rb_bonds    thread per atom: compact the neighbour row to its bonds
+//   k_rb_slots    row lengths, elements (and LAMMPS tags) in slot order
+//   k_rb_bonds    thread per slot: compact the neighbour row to its bonds
 //                 (r^2 < r2^2 of the pair type), bond record = unit vector,
 //                 length, cut-off and derivative; coordination N_C, N_H
 //                 (loop1 and the nn sum of bop_kernel_rebo2.f90:700-1205)
@@ -22,7 +29,8 @@
 //                 integer atomics (exact and order independent, so results
 //                 are bitwise reproducible); total energy and virial are
 //                 reduced per block and then in a fixed order.
-//   k_rb_finish   fixed point -> FP64, added to the caller's arrays.
+//   k_rb_finish   fixed point -> FP64, added to the caller's arrays (slot p
+//                 -> atom order[p]).
 //
 // Tables (F_CC, F_CH, F_HH, T_CC tricubic, P_CC, P_CH bicubic) are built on the
 // host from their node data in closed form: the tensor product of cubic
@@ -83,7 +91,7 @@ struct Rebo2State {
     RbDev hdev{};
     std::vector<double> htab;
     DevBuf dev, tab, bond, bij, bown, bcnt, nn, nconj, facc, eacc, wacc, partial, stats, bo, rev, half, hpos,
-        list, scan_tmp, bsrc;
+        list, scan_tmp, bsrc, cnt_s, offs_s, el_s, tag_s;
     int z2db[128];
     uint64_t Z_rev = ~0ull;
     double cutoff = 0.0;
@@ -561,23 +569,46 @@ __global__ void k_rb_zero(unsigned long long *__restrict__ a, size_t n)
         a[q] = 0ull;
 }
 
-__global__ void __launch_bounds__(256)
-k_rb_bonds(int nat, const double *__restrict__ r, const int *__restrict__ el, CellArg C,
-           const RbDev *__restrict__ Dp, const int *__restrict__ offsets,
-           const int2 *__restrict__ nbr, RbBond *__restrict__ bond, int2 *__restrict__ bij,
-           int *__restrict__ bown, int *__restrict__ bcnt, double2 *__restrict__ nn, int *__restrict__ bsrc)
+// row lengths (the bond slots reserved per slot), elements and LAMMPS tags
+// in slot order; cnt_s[nat] = 0 closes the scan
+__global__ void k_rb_slots(int nat, const int *__restrict__ order, NlRows L, const int *__restrict__ el,
+                           const int *__restrict__ tag, int *__restrict__ cnt_s, int *__restrict__ el_s,
+                           int *__restrict__ tag_s)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nat) return;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p == nat) cnt_s[nat] = 0;
+    if (p >= nat) return;
+    const int i = order[p];
+    int n;
+    nl_row(L, p, i, n);
+    cnt_s[p] = n;
+    el_s[p] = el[i];
+    if (tag) tag_s[p] = tag[i];
+}
+
+// thread per slot p (atom order[p]): its bonds at offs_s[p] .., neighbours
+// by slot; bsrc (optional) = the bond's entry in the caller-order list
+__global__ void __launch_bounds__(256)
+k_rb_bonds(int nat, const int *__restrict__ order, const int *__restrict__ rank, const double *__restrict__ r,
+           const int *__restrict__ el, CellArg C, const RbDev *__restrict__ Dp, NlRows L,
+           const int *__restrict__ offs_s, const int *__restrict__ offsets, RbBond *__restrict__ bond,
+           int2 *__restrict__ bij, int *__restrict__ bown, int *__restrict__ bcnt, double2 *__restrict__ nn,
+           int *__restrict__ bsrc)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nat) return;
     const RbDev &D = *Dp;
+    const int i = order[p];
     const int ki = el[i];
-    const int beg = offsets[i], end = offsets[i + 1];
+    const int beg = offs_s[p], end = offs_s[p + 1];
+    int nrow;
+    const int2 *row = nl_row(L, p, i, nrow);
     int nb = 0;
     double nC = 0.0, nH = 0.0;
     if (ki > 0) {
         const double ri[3] = {r[3 * i], r[3 * i + 1], r[3 * i + 2]};
-        for (int s = beg; s < end; ++s) {
-            const int2 e = nbr[s];
+        for (int k = 0; k < nrow; ++k) {
+            const int2 e = row[k];
             const int kj = el[e.x];
             if (kj <= 0) continue;
             int a, b, c;
@@ -607,15 +638,15 @@ k_rb_bonds(int nat, const double *__restrict__ r, const int *__restrict__ el, Ce
             if (kj == RB_C) nC += B.fc;
             else nH += B.fc;
             bond[beg + nb] = B;
-            bij[beg + nb] = e;
-            bown[beg + nb] = i;
-            if (bsrc) bsrc[beg + nb] = s;      // list entry of the bond (nbb of the reference)
+            bij[beg + nb] = make_int2(rank[e.x], e.y);
+            bown[beg + nb] = p;
+            if (bsrc) bsrc[beg + nb] = offsets[i] + k;   // list entry of the bond (nbb of the reference)
             ++nb;
         }
     }
     for (int s = beg + nb; s < end; ++s) bown[s] = -1;
-    bcnt[i] = nb;
-    nn[i] = make_double2(nC, nH);
+    bcnt[p] = nb;
+    nn[p] = make_double2(nC, nH);
 }
 
 __global__ void __launch_bounds__(256)
@@ -906,7 +937,7 @@ k_rb_bo(int ntot, const RbDev *__restrict__ Dp, const double *__restrict__ tab, 
         const int *__restrict__ offsets, const int *__restrict__ bcnt, const int *__restrict__ bown,
         const RbBond *__restrict__ bond, const int2 *__restrict__ bij, const double2 *__restrict__ nn,
         double4 *__restrict__ bo, int *__restrict__ rev, int *__restrict__ half, int natloc,
-        const int *__restrict__ tag, int *__restrict__ flags)
+        const int *__restrict__ tag, const int *__restrict__ order, int *__restrict__ flags)
 {
     const RbDev &D = *Dp;
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -926,11 +957,12 @@ k_rb_bo(int ntot, const RbDev *__restrict__ Dp, const double *__restrict__ tab, 
     const bool pos = a != 0 ? a > 0 : (b != 0 ? b > 0 : c > 0);
     const bool zero = a == 0 && b == 0 && c == 0;
     const bool act = B.len < D.cut_h[ijpot];
-    // LAMMPS: local centres only, tag(j) >= tag(i) (bop_kernel_rebo2.f90:1328-1329)
+    // i and j are slots; the rules compare caller indices order[.].  LAMMPS:
+    // local centres only, tag(j) >= tag(i) (bop_kernel_rebo2.f90:1328-1329)
     if (tag)
-        half[s] = (act && i < natloc && tag[j] >= tag[i]) ? 1 : 0;
+        half[s] = (act && order[i] < natloc && tag[j] >= tag[i]) ? 1 : 0;
     else
-        half[s] = (act && ((zero && j > i) || pos)) ? 1 : 0;
+        half[s] = (act && ((zero && order[j] > order[i]) || pos)) ? 1 : 0;
     int sr = -1;
     const int jb = offsets[j], jn = bcnt[j];
     for (int q = jb; q < jb + jn; ++q) {
@@ -1120,24 +1152,25 @@ k_rb_pair(int ntot, const int *__restrict__ list, const int *__restrict__ count,
             ((s_red[0][tid] + s_red[1][tid]) + s_red[2][tid]) + s_red[3][tid];
 }
 
-__global__ void k_rb_finish(int nat, unsigned long long *__restrict__ facc,
+__global__ void k_rb_finish(int nat, const int *__restrict__ order, unsigned long long *__restrict__ facc,
                             unsigned long long *__restrict__ eacc, unsigned long long *__restrict__ wacc,
                             double *__restrict__ f, double *__restrict__ epa, double *__restrict__ wpa)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nat) return;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nat) return;
+    const size_t i = (size_t)order[p];
     for (int x = 0; x < 3; ++x) {
-        f[3 * i + x] += (double)(long long)facc[3 * (size_t)i + x] * RB_FX_INV;
-        facc[3 * (size_t)i + x] = 0ull;
+        f[3 * i + x] += (double)(long long)facc[3 * (size_t)p + x] * RB_FX_INV;
+        facc[3 * (size_t)p + x] = 0ull;
     }
     if (epa) {
-        epa[i] += (double)(long long)eacc[i] * RB_FX_INV;
-        eacc[i] = 0ull;
+        epa[i] += (double)(long long)eacc[p] * RB_FX_INV;
+        eacc[p] = 0ull;
     }
     if (wpa)
         for (int q = 0; q < 9; ++q) {
-            wpa[9 * (size_t)i + q] += (double)(long long)wacc[9 * (size_t)i + q] * RB_FX_INV;
-            wacc[9 * (size_t)i + q] = 0ull;
+            wpa[9 * i + q] += (double)(long long)wacc[9 * (size_t)p + q] * RB_FX_INV;
+            wacc[9 * (size_t)p + q] = 0ull;
         }
 }
 
@@ -1194,7 +1227,8 @@ int rebo2_compute(agpu_potential *pot, const agpu_eval_args *a)
     if (a->mask)
         return set_error(AGPU_ERR_UNSUPPORTED, "REBO2: the reference kernel takes no mask");
     if (st->Z_rev != p->Z_rev) AGPU_CHECK(rebo2_bind(pot));
-    AGPU_CHECK(nl_update(nl, p, 0));
+    // the rows are read where the builder left them (slot order)
+    AGPU_CHECK(nl_update(nl, p, 0, false));
     const int nat = p->nat;
     hipStream_t s = stream();
     AGPU_HIP(hipMemsetAsync(a->epot, 0, sizeof(double), s));
@@ -1233,12 +1267,32 @@ int rebo2_compute(agpu_potential *pot, const agpu_eval_args *a)
     const bool per_bond = a->epot_per_bond || a->f_per_bond || a->wpot_per_bond;
     if (per_bond) AGPU_CHECK(st->bsrc.reserve(sizeof(int) * (total + 1)));
     int *bsrc = per_bond ? st->bsrc.as<int>() : nullptr;
-    AGPU_LAUNCH("rebo2_bonds", k_rb_bonds, dim3(nb), dim3(256), 0, s, nat, p->d_r, el, C, D,
-                nl->offsets.as<int>(), nl->nbr.as<int2>(), st->bond.as<RbBond>(), st->bij.as<int2>(),
+    // slot order: row lengths -> bond offsets per slot, elements, tags
+    const int *order = nl->idx_sorted.as<int>();
+    const bool lammps = p->natloc >= 0;
+    AGPU_CHECK(st->cnt_s.reserve(sizeof(int) * (nat + 1)));
+    AGPU_CHECK(st->offs_s.reserve(sizeof(int) * (nat + 1)));
+    AGPU_CHECK(st->el_s.reserve(sizeof(int) * (nat + 1)));
+    if (lammps) AGPU_CHECK(st->tag_s.reserve(sizeof(int) * (nat + 1)));
+    const NlRows L = nl_rows(nl);
+    AGPU_LAUNCH("rebo2_slots", k_rb_slots, dim3((nat + 256) / 256), dim3(256), 0, s, nat, order, L, el,
+                lammps ? p->d_tag.as<int>() : nullptr, st->cnt_s.as<int>(), st->el_s.as<int>(),
+                lammps ? st->tag_s.as<int>() : nullptr);
+    {
+        size_t tmp = 0;
+        AGPU_HIP(rocprim::exclusive_scan(nullptr, tmp, st->cnt_s.as<int>(), st->offs_s.as<int>(), 0,
+                                         (size_t)nat + 1, rocprim::plus<int>(), s));
+        AGPU_CHECK(st->scan_tmp.reserve(tmp + 16));
+        AGPU_HIP(rocprim::exclusive_scan(st->scan_tmp.ptr, tmp, st->cnt_s.as<int>(), st->offs_s.as<int>(), 0,
+                                         (size_t)nat + 1, rocprim::plus<int>(), s));
+    }
+    const int *offs = st->offs_s.as<int>();
+    const int *els = st->el_s.as<int>();
+    AGPU_LAUNCH("rebo2_bonds", k_rb_bonds, dim3(nb), dim3(256), 0, s, nat, order, nl->rank.as<int>(), p->d_r, el,
+                C, D, L, offs, nl->offsets.as<int>(), st->bond.as<RbBond>(), st->bij.as<int2>(),
                 st->bown.as<int>(), st->bcnt.as<int>(), st->nn.as<double2>(), bsrc);
-    AGPU_LAUNCH("rebo2_conj", k_rb_conj, dim3(nb), dim3(256), 0, s, nat, el, nl->offsets.as<int>(),
-                st->bcnt.as<int>(), st->bij.as<int2>(), st->nn.as<double2>(), st->bond.as<RbBond>(),
-                st->nconj.as<double>());
+    AGPU_LAUNCH("rebo2_conj", k_rb_conj, dim3(nb), dim3(256), 0, s, nat, els, offs, st->bcnt.as<int>(),
+                st->bij.as<int2>(), st->nn.as<double2>(), st->bond.as<RbBond>(), st->nconj.as<double>());
     const int ntot = (int)total;
     int nblk = (ntot + 255) / 256;
     if (nblk < 1) nblk = 1;
@@ -1250,10 +1304,10 @@ int rebo2_compute(agpu_potential *pot, const agpu_eval_args *a)
     AGPU_CHECK(st->list.reserve(sizeof(int) * (total + 2)));
     int *count = st->list.as<int>() + total + 1;
     if (ntot > 0) {
-        AGPU_LAUNCH("rebo2_bo", k_rb_bo, dim3(nblk), dim3(256), 0, s, ntot, D, st->tab.as<double>(), el,
-                    nl->offsets.as<int>(), st->bcnt.as<int>(), st->bown.as<int>(), st->bond.as<RbBond>(),
-                    st->bij.as<int2>(), st->nn.as<double2>(), st->bo.as<double4>(), st->rev.as<int>(),
-                    st->half.as<int>(), p->natloc, p->natloc >= 0 ? p->d_tag.as<int>() : nullptr, dstats);
+        AGPU_LAUNCH("rebo2_bo", k_rb_bo, dim3(nblk), dim3(256), 0, s, ntot, D, st->tab.as<double>(), els, offs,
+                    st->bcnt.as<int>(), st->bown.as<int>(), st->bond.as<RbBond>(), st->bij.as<int2>(),
+                    st->nn.as<double2>(), st->bo.as<double4>(), st->rev.as<int>(), st->half.as<int>(), p->natloc,
+                    lammps ? st->tag_s.as<int>() : nullptr, order, dstats);
         size_t tmp = 0;
         AGPU_HIP(rocprim::exclusive_scan(nullptr, tmp, st->half.as<int>(), st->hpos.as<int>(), 0, (size_t)ntot,
                                          rocprim::plus<int>(), s));
@@ -1266,11 +1320,11 @@ int rebo2_compute(agpu_potential *pot, const agpu_eval_args *a)
         AGPU_HIP(hipMemsetAsync(count, 0, sizeof(int), s));
     }
     AGPU_LAUNCH("rebo2_pair", k_rb_pair, dim3(nblk), dim3(256), 0, s, ntot, st->list.as<int>(), count, D,
-                st->tab.as<double>(), el, nl->offsets.as<int>(), st->bcnt.as<int>(), st->bown.as<int>(),
+                st->tab.as<double>(), els, offs, st->bcnt.as<int>(), st->bown.as<int>(),
                 st->bond.as<RbBond>(), st->bij.as<int2>(), st->nn.as<double2>(), st->nconj.as<double>(),
                 st->bo.as<double4>(), st->rev.as<int>(), st->facc.as<unsigned long long>(), eacc, wacc,
                 st->partial.as<double>(), dstats, bsrc, a->epot_per_bond, a->f_per_bond, a->wpot_per_bond);
-    AGPU_LAUNCH("rebo2_finish", k_rb_finish, dim3(nb), dim3(256), 0, s, nat,
+    AGPU_LAUNCH("rebo2_finish", k_rb_finish, dim3(nb), dim3(256), 0, s, nat, order,
                 st->facc.as<unsigned long long>(), eacc, wacc, a->f, a->epot_per_at, a->wpot_per_at);
     AGPU_CHECK(reduce_ev(st->partial.as<double>(), nblk, 10, a->epot, a->wpot, nullptr));
     AGPU_HIP(hipMemcpyAsync(pot->stats.h, dstats, sizeof(int), hipMemcpyDeviceToHost, s));

@@ -74,6 +74,18 @@ struct TersoffState {
     DevBuf keep, rcnt, roff, fcv, rkn, rco, frec, fsl, nlrev, scan_tmp;
     int64_t nlrev_upd = -1;
     const agpu_neighbors *nlrev_nl = nullptr;
+    // take the bond-compacting path even for rows of at most 64 entries
+    // (agpu_potential_set_launch variant 1 of a bond-order class)
+    bool compact = false;
+};
+
+// Device pointers of the per-call scratch of the unscreened bond-order path:
+// bond slots and counts, reverse slots, slot forces, own forces, per-block
+// records (10 doubles per block), second-level reduction records (NULL: the
+// library's own) and 4 flag words.
+struct BopBufs {
+    int *bslot, *bcnt, *rev, *stats;
+    double *fslot, *fown, *partial, *red;
 };
 
 __host__ __device__ __forceinline__ int pair_index(int i, int j, int nel)
@@ -195,6 +207,37 @@ template <> __device__ __forceinline__ float rsqrt_<float>(float x) { return sqr
 template <typename R> __device__ __forceinline__ void rsincos(R x, R *s, R *c);
 template <> __device__ __forceinline__ void rsincos<double>(double x, double *s, double *c) { sincos(x, s, c); }
 template <> __device__ __forceinline__ void rsincos<float>(float x, float *s, float *c) { sincosf(x, s, c); }
+template <typename R> __device__ __forceinline__ R rlog(R x);
+template <> __device__ __forceinline__ double rlog<double>(double x) { return log(x); }
+template <> __device__ __forceinline__ float rlog<float>(float x) { return logf(x); }
+// x^y for x > 0 as exp(y log x): one log and one exp instead of the
+// general pow; the relative error stays ~(1 + |y log x|) ulp
+template <typename R> __device__ __forceinline__ R rpow_pos(R x, R y) { return rexp<R>(y * rlog<R>(x)); }
+
+// DPP lane moves of both halves of a double (quad_perm and row mirrors stay
+// inside a 16-lane row; no LDS traffic)
+template <int CTRL> __device__ __forceinline__ double dpp_d(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Sum over aligned groups of G lanes; every lane of a group ends with the
+// same value.  Pairings: lane ^ 1 and lane ^ 2 (quad_perm), lane <-> 7 - lane
+// and lane <-> 15 - lane inside the row (half mirror, mirror), then lane ^ 16
+// and lane ^ 32 across rows.  A fixed pairing, so sums are reproducible.
+template <int G> __device__ __forceinline__ double gsum(double v)
+{
+    if (G >= 2) v += dpp_d<0xB1>(v);
+    if (G >= 4) v += dpp_d<0x4E>(v);
+    if (G >= 8) v += dpp_d<0x141>(v);
+    if (G >= 16) v += dpp_d<0x140>(v);
+    if (G >= 32) v += __shfl_xor(v, 16, 64);
+    if (G >= 64) v += __shfl_xor(v, 32, 64);
+    return v;
+}
 
 // h(r_ij - r_ik): Tersoff (tersoff_func.f90:168-211) or Kumagai
 // (kumagai_func.f90:179-221, including its beta = 3 branch, which leaves
@@ -310,13 +353,13 @@ __device__ __forceinline__ void bo_fn(const TersoffDev &P, int eli, int typ, R z
         const R n = (R)P.bnn[typ], boe = (R)P.bboe[typ];
         if (n == (R)1) {
             const R arg = (R)1 + zij;
-            const R a1 = rpow<R>(arg, boe - (R)1);
+            const R a1 = rpow_pos<R>(arg, boe - (R)1);
             bij = a1 * arg;
             Pz = (R)0.5 * boe * n * fc * VA * a1;
         } else if (zij > (R)0) {
-            const R zn1 = rpow<R>(zij, n - (R)1);
+            const R zn1 = rpow_pos<R>(zij, n - (R)1);
             const R arg = (R)1 + zn1 * zij;
-            const R a1 = rpow<R>(arg, boe - (R)1);
+            const R a1 = rpow_pos<R>(arg, boe - (R)1);
             bij = a1 * arg;
             Pz = (R)0.5 * boe * n * fc * VA * zn1 * a1;
         }
@@ -325,9 +368,9 @@ __device__ __forceinline__ void bo_fn(const TersoffDev &P, int eli, int typ, R z
     if (!(zij > (R)0)) return;
     if (FAM == 1) {
         const R eta = (R)P.keta[eli], delta = (R)P.kdelta[eli];
-        const R zn1 = rpow<R>(zij, eta - (R)1);
+        const R zn1 = rpow_pos<R>(zij, eta - (R)1);
         const R arg = (R)1 + zn1 * zij;
-        const R ad1 = rpow<R>(arg, -delta - (R)1);
+        const R ad1 = rpow_pos<R>(arg, -delta - (R)1);
         bij = ad1 * arg;
         Pz = (R)0.5 * fc * VA * eta * zn1 * (-delta) * ad1;
         return;
@@ -335,9 +378,9 @@ __device__ __forceinline__ void bo_fn(const TersoffDev &P, int eli, int typ, R z
     R nn = (R)P.n[eli];
     R e = (R)P.e[eli];
     R b = (R)P.bn[eli];
-    R zn1 = rpow<R>(zij, nn - (R)1);
+    R zn1 = rpow_pos<R>(zij, nn - (R)1);
     R arg = (R)1 + b * zn1 * zij;
-    R ae1 = rpow<R>(arg, e - (R)1);
+    R ae1 = rpow_pos<R>(arg, e - (R)1);
     bij = (R)P.xi[typ] * ae1 * arg;
     Pz = (R)-0.25 * fc * VA * (R)P.xi[typ] * b * zn1 * ae1;
 }
@@ -633,6 +676,350 @@ k_bop_gather(int nat, const int *__restrict__ offsets, const int *__restrict__ b
         if (wpa)
             for (int q = 0; q < 9; ++q) wpa[9 * (size_t)a + q] += 0.5 * w[q];
     }
+}
+
+// -------------------------------------------------------------------------
+// Bond-order kernel over the neighbour rows as the list builder left them
+// (staged in bin order, or CSR), with no bond compaction and no reverse-slot
+// pass: a group of G lanes per atom (G >= the longest row), lane = row
+// entry; entries past r2 of their pair, or towards atoms of no element, take
+// part as zero-force lanes.  Each lane finds its reverse entry j -> i (image
+// -n) in j's row and writes there the total force the terms centred at i put
+// on j, so the assembly only sums every atom's own row (k_bop_rows_gather).
+// g(cos), dg and h of pass 1 are kept for pass 2 when G <= 8; the virial
+// of the terms centred at i is -sum_j r_ij (x) F_j unless per-bond virials
+// are needed (WB: the reference's per-bond wij, bop_kernel.f90:1325-1512).
+// -------------------------------------------------------------------------
+constexpr int DERR_REV_MISSING = DERR_BOND_OVERFLOW;
+
+template <typename R, int G, int FAM, bool WB>
+__global__ void __launch_bounds__(256)
+k_bop_rows(TersoffDev P, int nat, const int *__restrict__ order, const double *__restrict__ r,
+           const int *__restrict__ el, CellArg C, NlRows L, const int *__restrict__ slot,
+           const int *__restrict__ offsets, const int *__restrict__ mask, double *__restrict__ frev,
+           double *__restrict__ fown, double *__restrict__ eslot, double *__restrict__ erev,
+           double *__restrict__ wslot, double *__restrict__ wrev, double *__restrict__ partial,
+           double *__restrict__ epb, double *__restrict__ fpb, double *__restrict__ wpb, int *__restrict__ flags)
+{
+    constexpr int GPB = 256 / G;
+    constexpr bool CACHE = G <= 8;
+    constexpr int NC = CACHE ? G : 1;
+    __shared__ R s_rn[3][256];
+    __shared__ R s_rl[256], s_ri[256], s_fc[256], s_dfc[256];
+    __shared__ int s_typ[256];
+    __shared__ double s_acc[10][256 / 16];
+
+    const int tid = threadIdx.x;
+    const int lane = tid % G;
+    const int grp = tid / G;
+    const int gbase = grp * G;
+    const int ai = blockIdx.x * GPB + grp;
+    const bool valid = ai < nat;
+    const int i = valid ? order[ai] : 0;
+    const int eli = valid ? el[i] : -1;
+    int n = 0;
+    const int2 *row = valid ? nl_row(L, ai, i, n) : nullptr;
+    n = min(n, G);                      // G >= every row length (host side)
+    const bool act = lane < n;
+
+    int j = 0, code = 0, typ = 0;
+    R rn[3] = {(R)0, (R)0, (R)0}, rl = (R)1, fc = (R)0, dfc = (R)0;
+    bool ok = false;
+    if (act) {
+        const int2 e = row[lane];
+        j = e.x;
+        code = e.y;
+        const int elj = el[j];
+        if (eli >= 0 && elj >= 0) {
+            int a, b, c;
+            dc_unpack(code, a, b, c);
+            // rij = r_j - r_i - A.dc   (bop_kernel.f90:611)
+            double rij[3];
+            for (int x = 0; x < 3; ++x) rij[x] = (r[3 * j + x] - r[3 * i + x]) - image_x(C, x, a, b, c);
+            const double rl2 = rij[0] * rij[0] + rij[1] * rij[1] + rij[2] * rij[2];
+            const double rld = sqrt(rl2);
+            typ = FAM == 3 ? elj + eli * P.nel : pair_index(eli, elj, P.nel);
+            ok = rld < P.r2[typ];
+            if (ok) {
+                const double inv = 1.0 / rld;
+                for (int x = 0; x < 3; ++x) rn[x] = (R)(rij[x] * inv);
+                rl = (R)rld;
+                if (rl2 < P.r1[typ] * P.r1[typ]) {
+                    fc = (R)1;
+                    dfc = (R)0;
+                } else {
+                    // trig_off (cutoff.f90:171-196)
+                    const R fac = (R)P.fac[typ];
+                    R sn, cs;
+                    rsincos<R>(fac * (rl - (R)P.r1[typ]), &sn, &cs);
+                    fc = (R)0.5 * ((R)1 + cs);
+                    dfc = (R)-0.5 * fac * sn;
+                    if (!(rld > P.r1[typ])) { fc = (R)1; dfc = (R)0; }
+                }
+            }
+        }
+    }
+    s_rn[0][tid] = rn[0];
+    s_rn[1][tid] = rn[1];
+    s_rn[2][tid] = rn[2];
+    s_rl[tid] = rl;
+    s_ri[tid] = (R)1 / rl;
+    s_fc[tid] = fc;
+    s_dfc[tid] = dfc;
+    s_typ[tid] = ok ? typ : -1;
+    __syncthreads();
+
+    int maskfac = 2;
+    if (mask && act) {
+        const int mi = mask[i], mj = mask[j];
+        if (mi == 0 && mj == 0) maskfac = 0;
+        else if (mi == 0 || mj == 0) maskfac = 1;
+    }
+    const bool on = ok && maskfac > 0;
+
+    // ---- pass 1: zeta_ij (g, dg, h, dh kept for pass 2) ----
+    const int elc = eli < 0 ? 0 : eli;
+    R zij = (R)0;
+    R cg[NC], cdg[NC], chv[NC], cdh[NC];
+    int nbw = n;
+    for (int o = 32; o >= 1; o >>= 1) nbw = max(nbw, __shfl_xor(nbw, o, 64));
+#pragma unroll
+    for (int k = 0; k < G; ++k) {
+        if (!CACHE && k >= nbw) break;
+        const int tk = gbase + k;
+        const int tpk = s_typ[tk];
+        R g = (R)0, dg = (R)0, hv = (R)0, dh = (R)0;
+        if (on && k != lane && tpk >= 0) {
+            const R costh = s_rn[0][tk] * rn[0] + s_rn[1][tk] * rn[1] + s_rn[2][tk] * rn[2];
+            g_fn<R, FAM>(P, elc, tpk, costh, g, dg);
+            h_fn<R, FAM>(P, typ, tpk, rl - s_rl[tk], hv, dh);
+            zij += s_fc[tk] * g * hv;
+        }
+        if (CACHE) {
+            cg[CACHE ? k : 0] = g;
+            cdg[CACHE ? k : 0] = dg;
+            chv[CACHE ? k : 0] = hv;
+            cdh[CACHE ? k : 0] = dh;
+        }
+    }
+
+    // ---- bond order ----
+    R VA = (R)0, dVA = (R)0, VR = (R)0, dVR = (R)0, bij = (R)1, Pz = (R)0;
+    if (on) {
+        if (FAM >= 2) {
+            // brenner_func.f90:27-71 (juslin_func.f90:69-99 the same)
+            const R dr0 = rl - (R)P.br0[typ];
+            const R ea = rexp<R>(-(R)P.bexpA[typ] * dr0);
+            VA = -(R)P.bVAf[typ] * ea;
+            dVA = (R)P.bVAf[typ] * (R)P.bexpA[typ] * ea;
+            const R er = rexp<R>(-(R)P.bexpR[typ] * dr0);
+            VR = (R)P.bVRf[typ] * er;
+            dVR = -(R)P.bVRf[typ] * (R)P.bexpR[typ] * er;
+        } else {
+            const R ea = rexp<R>(-(R)P.mu[typ] * rl);
+            VA = -(R)P.B[typ] * ea;
+            dVA = (R)P.B[typ] * (R)P.mu[typ] * ea;
+            const R er = rexp<R>(-(R)P.lambda[typ] * rl);
+            VR = (R)P.A[typ] * er;
+            dVR = -(R)P.A[typ] * (R)P.lambda[typ] * er;
+        }
+        const R mf = (R)0.5 * (R)maskfac;
+        VA *= mf;
+        dVA *= mf;
+        VR *= mf;
+        dVR *= mf;
+        bo_fn<R, FAM>(P, eli, typ, zij, fc, VA, bij, Pz);
+    }
+
+    // ---- pass 2: derivatives; forces on k summed over the group into lane k ----
+    R dbidi[3] = {(R)0, (R)0, (R)0}, dbidj[3] = {(R)0, (R)0, (R)0};
+    R wijb[WB ? 9 : 1];
+#pragma unroll
+    for (int q = 0; q < (WB ? 9 : 1); ++q) wijb[q] = (R)0;
+    double recv[3] = {0.0, 0.0, 0.0};
+    const R rlijr = (R)1 / rl;
+#pragma unroll
+    for (int k = 0; k < G; ++k) {
+        if (!CACHE && k >= nbw) break;
+        const int tk = gbase + k;
+        const int tpk = s_typ[tk];
+        R dk[3] = {(R)0, (R)0, (R)0};
+        if (on && k != lane && tpk >= 0) {
+            const R rnk[3] = {s_rn[0][tk], s_rn[1][tk], s_rn[2][tk]};
+            const R rlik = s_rl[tk], rlikr = s_ri[tk], fcik = s_fc[tk], dfcik = s_dfc[tk];
+            R g, dg, hv, dh;
+            if (CACHE) {
+                g = cg[CACHE ? k : 0];
+                dg = cdg[CACHE ? k : 0];
+                hv = chv[CACHE ? k : 0];
+                dh = cdh[CACHE ? k : 0];
+            } else {
+                const R cs0 = rnk[0] * rn[0] + rnk[1] * rn[1] + rnk[2] * rn[2];
+                g_fn<R, FAM>(P, elc, tpk, cs0, g, dg);
+                h_fn<R, FAM>(P, typ, tpk, rl - rlik, hv, dh);
+            }
+            const R costh = rnk[0] * rn[0] + rnk[1] * rn[1] + rnk[2] * rn[2];
+            R dkc[3];
+            for (int x = 0; x < 3; ++x) dkc[x] = rnk[x] * rlik - rn[x] * rl;
+            const R disjk = rsqrt_<R>(dkc[0] * dkc[0] + dkc[1] * dkc[1] + dkc[2] * dkc[2]);
+            const R dinv = (R)1 / disjk;
+            for (int x = 0; x < 3; ++x) dkc[x] = dkc[x] * dinv;
+            const R dcsdij = rlikr - costh * rlijr;
+            const R dcsdik = rlijr - costh * rlikr;
+            const R dcsdjk = -disjk * rlijr * rlikr;
+            const R dzfac = fcik * dg * hv;
+            const R dzdrij = g * fcik * dh;
+            const R dzdrik = g * (dfcik * hv - fcik * dh);
+            R df[3];
+            for (int x = 0; x < 3; ++x) {
+                const R dcsdi = -dcsdij * rn[x] - dcsdik * rnk[x];
+                const R dcsdj = dcsdij * rn[x] - dcsdjk * dkc[x];
+                const R dcsdk = dcsdik * rnk[x] + dcsdjk * dkc[x];
+                dbidi[x] += -dzdrij * rn[x] - dzdrik * rnk[x] + dzfac * dcsdi;
+                df[x] = dzdrij * rn[x] + dzfac * dcsdj;
+                dbidj[x] += df[x];
+                dk[x] = dzdrik * rnk[x] + dzfac * dcsdk;
+            }
+            if (WB) {
+                const R rij[3] = {rl * rn[0], rl * rn[1], rl * rn[2]};
+                const R rik[3] = {rlik * rnk[0], rlik * rnk[1], rlik * rnk[2]};
+                for (int a = 0; a < 3; ++a)
+                    for (int b = 0; b < 3; ++b) wijb[WB ? a + 3 * b : 0] -= rij[a] * df[b] + rik[a] * dk[b];
+            }
+        }
+        // force on k from bond ij: -P_ij dzeta_ij/dr_k, summed over the group
+        const double c0 = gsum<G>(-(double)Pz * (double)dk[0]);
+        const double c1 = gsum<G>(-(double)Pz * (double)dk[1]);
+        const double c2v = gsum<G>(-(double)Pz * (double)dk[2]);
+        if (lane == k) {
+            recv[0] = c0;
+            recv[1] = c1;
+            recv[2] = c2v;
+        }
+    }
+
+    double fi[3] = {0.0, 0.0, 0.0}, F[3] = {0.0, 0.0, 0.0}, dfp[3] = {0.0, 0.0, 0.0};
+    double e_ij = 0.0;
+    double w[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) w[q] = 0.0;
+    const double rijd[3] = {(double)(rl * rn[0]), (double)(rl * rn[1]), (double)(rl * rn[2])};
+    if (on) {
+        e_ij = (double)((R)0.5 * fc * (VR + bij * VA));
+        const R dffac = (R)0.5 * (dVR * fc + bij * dVA * fc + VR * dfc + bij * VA * dfc);
+        for (int x = 0; x < 3; ++x) {
+            dfp[x] = (double)(dffac * rn[x]);
+            fi[x] = dfp[x] - (double)Pz * (double)dbidi[x];
+            F[x] = -dfp[x] - (double)Pz * (double)dbidj[x];
+        }
+        if (WB)
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b)
+                    w[a + 3 * b] = rijd[a] * dfp[b] - (double)Pz * (double)wijb[WB ? a + 3 * b : 0];
+    }
+    for (int x = 0; x < 3; ++x) F[x] += recv[x];
+    if (!WB)
+        // virial of the terms centred at i: -sum_j r_ij (x) F_j (their
+        // energy depends on the r_ij only)
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) w[a + 3 * b] = -rijd[a] * F[b];
+    if (act) {
+        // reverse entry j -> i (image -n) in j's row
+        int a0, b0, c0;
+        dc_unpack(code, a0, b0, c0);
+        const int want = dc_pack(-a0, -b0, -c0);
+        int nj = 0;
+        const int2 *rowj = nl_row(L, L.S > 0 ? slot[j] : 0, j, nj);
+        int t = -1;
+        for (int q = 0; q < nj; ++q) {
+            const int2 ej = rowj[q];
+            if (ej.x == i && ej.y == want) {
+                t = q;
+                break;
+            }
+        }
+        const size_t sown = (size_t)offsets[i] + lane;
+        if (t < 0) {
+            atomicOr(flags, DERR_REV_MISSING);
+        } else {
+            const size_t rt = (size_t)offsets[j] + t;
+            frev[3 * rt + 0] = F[0];
+            frev[3 * rt + 1] = F[1];
+            frev[3 * rt + 2] = F[2];
+            if (erev) erev[rt] = e_ij;
+            if (wrev)
+                for (int q = 0; q < 9; ++q) wrev[9 * rt + q] = w[q];
+        }
+        if (eslot) eslot[sown] = e_ij;
+        if (wslot)
+            for (int q = 0; q < 9; ++q) wslot[9 * sown + q] = w[q];
+        // per-bond properties (bop_kernel.f90:1370-1373,1406-1408,1511-1512)
+        if (on) {
+            if (epb) epb[sown] += e_ij;
+            if (fpb)
+                for (int x = 0; x < 3; ++x) fpb[3 * sown + x] += dfp[x];
+            if (wpb)
+                for (int q = 0; q < 9; ++q) wpb[9 * sown + q] += w[q];
+        }
+    }
+    // own force of i: sum over the group
+    const double f0 = gsum<G>(fi[0]);
+    const double f1 = gsum<G>(fi[1]);
+    const double f2 = gsum<G>(fi[2]);
+    if (valid && lane == 0) {
+        fown[3 * (size_t)i + 0] = f0;
+        fown[3 * (size_t)i + 1] = f1;
+        fown[3 * (size_t)i + 2] = f2;
+    }
+    // energy + virial of the block: 16-lane sums (DPP), then 16 partial
+    // records per value summed in order
+    double red[10];
+    red[0] = e_ij;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) red[1 + q] = w[q];
+#pragma unroll
+    for (int q = 0; q < 10; ++q) red[q] = gsum<16>(red[q]);
+    if ((tid & 15) == 0)
+        for (int q = 0; q < 10; ++q) s_acc[q][tid >> 4] = red[q];
+    __syncthreads();
+    if (tid < 10) {
+        double t = 0.0;
+        for (int c = 0; c < 16; ++c) t += s_acc[tid][c];
+        partial[(size_t)blockIdx.x * 10 + tid] = t;
+    }
+}
+
+// Assembly for k_bop_rows: f_a += own force + the forces a's neighbours'
+// terms put on a (stored in a's own row), in row order; per-atom energy and
+// virial halves of a's bonds in both directions (pe(i), pe(j) of
+// bop_kernel.f90:1366-1368).  A thread per atom.
+__global__ void __launch_bounds__(256)
+k_bop_rows_gather(int nat, const int *__restrict__ offsets, const double *__restrict__ frev,
+                  const double *__restrict__ fown, const double *__restrict__ eslot, const double *__restrict__ erev,
+                  const double *__restrict__ wslot, const double *__restrict__ wrev, double *__restrict__ f,
+                  double *__restrict__ epa, double *__restrict__ wpa)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= nat) return;
+    const int s0 = offsets[a], s1 = offsets[a + 1];
+    double acc[3] = {fown[3 * (size_t)a], fown[3 * (size_t)a + 1], fown[3 * (size_t)a + 2]};
+    double e = 0.0;
+    for (int s = s0; s < s1; ++s) {
+        acc[0] += frev[3 * (size_t)s];
+        acc[1] += frev[3 * (size_t)s + 1];
+        acc[2] += frev[3 * (size_t)s + 2];
+        if (epa) e += eslot[s] + erev[s];
+    }
+    f[3 * (size_t)a] += acc[0];
+    f[3 * (size_t)a + 1] += acc[1];
+    f[3 * (size_t)a + 2] += acc[2];
+    if (epa) epa[a] += 0.5 * e;
+    if (wpa)
+        for (int q = 0; q < 9; ++q) {
+            double w = 0.0;
+            for (int s = s0; s < s1; ++s) w += wslot[9 * (size_t)s + q] + wrev[9 * (size_t)s + q];
+            wpa[9 * (size_t)a + q] += 0.5 * w;
+        }
 }
 
 // -------------------------------------------------------------------------
@@ -1317,80 +1704,79 @@ int tersoff_bind(agpu_potential *pot)
     return AGPU_OK;
 }
 
-template <typename R, int G>
-static int launch_tersoff(TersoffState *st, int nat, const int *order, const double *r, const int *el,
-                          const CellArg &C, agpu_neighbors *nl, const int *mask, double *eslot,
-                          double *wslot, const agpu_eval_args *a, int nblk, hipStream_t s)
+template <typename R, int G, int FAM>
+static int launch_tersoff_fam(const char *name, TersoffState *st, const BopBufs &B, int nat, const int *order,
+                              const double *r, const int *el, const CellArg &C, agpu_neighbors *nl,
+                              const int *mask, double *eslot, double *wslot, const agpu_eval_args *a, int nblk,
+                              hipStream_t s)
 {
-    if (st->dev.fam == 3)
-        AGPU_LAUNCH(sizeof(R) == 8 ? "juslin_fp64" : "juslin_mixed", (k_tersoff<R, G, 3>), dim3(nblk), dim3(256),
-                    0, s, st->dev, nat, order, r, el, C, nl->offsets.as<int>(), nl->nbr.as<int2>(),
-                    st->bslot.as<int>(), st->bcnt.as<int>(), mask, st->fslot.as<double>(),
-                    st->fown.as<double>(), eslot, wslot, st->partial.as<double>(), a->epot_per_bond,
-                    a->f_per_bond, a->wpot_per_bond);
-    else if (st->dev.fam == 1)
-        AGPU_LAUNCH(sizeof(R) == 8 ? "kumagai_fp64" : "kumagai_mixed", (k_tersoff<R, G, 1>), dim3(nblk), dim3(256),
-                    0, s, st->dev, nat, order, r, el, C, nl->offsets.as<int>(), nl->nbr.as<int2>(),
-                    st->bslot.as<int>(), st->bcnt.as<int>(), mask, st->fslot.as<double>(),
-                    st->fown.as<double>(), eslot, wslot, st->partial.as<double>(), a->epot_per_bond,
-                    a->f_per_bond, a->wpot_per_bond);
-    else if (st->dev.fam == 2)
-        AGPU_LAUNCH(sizeof(R) == 8 ? "brenner_fp64" : "brenner_mixed", (k_tersoff<R, G, 2>), dim3(nblk), dim3(256),
-                    0, s, st->dev, nat, order, r, el, C, nl->offsets.as<int>(), nl->nbr.as<int2>(),
-                    st->bslot.as<int>(), st->bcnt.as<int>(), mask, st->fslot.as<double>(),
-                    st->fown.as<double>(), eslot, wslot, st->partial.as<double>(), a->epot_per_bond,
-                    a->f_per_bond, a->wpot_per_bond);
-    else
-        AGPU_LAUNCH(sizeof(R) == 8 ? "tersoff_fp64" : "tersoff_mixed", (k_tersoff<R, G, 0>), dim3(nblk), dim3(256),
-                    0, s, st->dev, nat, order, r, el, C, nl->offsets.as<int>(), nl->nbr.as<int2>(),
-                    st->bslot.as<int>(), st->bcnt.as<int>(), mask, st->fslot.as<double>(),
-                    st->fown.as<double>(), eslot, wslot, st->partial.as<double>(), a->epot_per_bond,
-                    a->f_per_bond, a->wpot_per_bond);
+    AGPU_LAUNCH(name, (k_tersoff<R, G, FAM>), dim3(nblk), dim3(256), 0, s, st->dev, nat, order, r, el, C,
+                nl->offsets.as<int>(), nl->nbr.as<int2>(), B.bslot, B.bcnt, mask, B.fslot, B.fown, eslot, wslot,
+                B.partial, a->epot_per_bond, a->f_per_bond, a->wpot_per_bond);
     return AGPU_OK;
 }
 
-template <typename R>
-static int dispatch_tersoff(int G, TersoffState *st, int nat, const int *order, const double *r,
-                            const int *el, const CellArg &C, agpu_neighbors *nl, const int *mask,
-                            double *eslot, double *wslot, const agpu_eval_args *a, hipStream_t s,
-                            int *nblk_out)
+template <typename R, int G>
+static int launch_tersoff(TersoffState *st, const BopBufs &B, int nat, const int *order, const double *r,
+                          const int *el, const CellArg &C, agpu_neighbors *nl, const int *mask, double *eslot,
+                          double *wslot, const agpu_eval_args *a, int nblk, hipStream_t s)
 {
-    int gpb = 256 / G;
-    int nblk = (nat + gpb - 1) / gpb;
-    if (nblk < 1) nblk = 1;
-    *nblk_out = nblk;
-    AGPU_CHECK(st->partial.reserve(sizeof(double) * 10 * nblk));
+    const bool f64 = sizeof(R) == 8;
+    switch (st->dev.fam) {
+    case 3:
+        return launch_tersoff_fam<R, G, 3>(f64 ? "juslin_fp64" : "juslin_mixed", st, B, nat, order, r, el, C, nl,
+                                           mask, eslot, wslot, a, nblk, s);
+    case 1:
+        return launch_tersoff_fam<R, G, 1>(f64 ? "kumagai_fp64" : "kumagai_mixed", st, B, nat, order, r, el, C, nl,
+                                           mask, eslot, wslot, a, nblk, s);
+    case 2:
+        return launch_tersoff_fam<R, G, 2>(f64 ? "brenner_fp64" : "brenner_mixed", st, B, nat, order, r, el, C, nl,
+                                           mask, eslot, wslot, a, nblk, s);
+    default:
+        return launch_tersoff_fam<R, G, 0>(f64 ? "tersoff_fp64" : "tersoff_mixed", st, B, nat, order, r, el, C, nl,
+                                           mask, eslot, wslot, a, nblk, s);
+    }
+}
+
+// groups of G lanes, 256 / G atoms per block: the block count grows with G
+static int bop_blocks(int nat, int G) { return std::max(1, (nat + 256 / G - 1) / (256 / G)); }
+
+template <typename R>
+static int dispatch_tersoff(int G, TersoffState *st, const BopBufs &B, int nat, const int *order, const double *r,
+                            const int *el, const CellArg &C, agpu_neighbors *nl, const int *mask,
+                            double *eslot, double *wslot, const agpu_eval_args *a, hipStream_t s)
+{
+    const int nblk = bop_blocks(nat, G);
     switch (G) {
-    case 2: return launch_tersoff<R, 2>(st, nat, order, r, el, C, nl, mask, eslot, wslot, a, nblk, s);
-    case 4: return launch_tersoff<R, 4>(st, nat, order, r, el, C, nl, mask, eslot, wslot, a, nblk, s);
-    case 8: return launch_tersoff<R, 8>(st, nat, order, r, el, C, nl, mask, eslot, wslot, a, nblk, s);
-    case 16: return launch_tersoff<R, 16>(st, nat, order, r, el, C, nl, mask, eslot, wslot, a, nblk, s);
-    case 32: return launch_tersoff<R, 32>(st, nat, order, r, el, C, nl, mask, eslot, wslot, a, nblk, s);
-    default: return launch_tersoff<R, 64>(st, nat, order, r, el, C, nl, mask, eslot, wslot, a, nblk, s);
+    case 2: return launch_tersoff<R, 2>(st, B, nat, order, r, el, C, nl, mask, eslot, wslot, a, nblk, s);
+    case 4: return launch_tersoff<R, 4>(st, B, nat, order, r, el, C, nl, mask, eslot, wslot, a, nblk, s);
+    case 8: return launch_tersoff<R, 8>(st, B, nat, order, r, el, C, nl, mask, eslot, wslot, a, nblk, s);
+    case 16: return launch_tersoff<R, 16>(st, B, nat, order, r, el, C, nl, mask, eslot, wslot, a, nblk, s);
+    case 32: return launch_tersoff<R, 32>(st, B, nat, order, r, el, C, nl, mask, eslot, wslot, a, nblk, s);
+    default: return launch_tersoff<R, 64>(st, B, nat, order, r, el, C, nl, mask, eslot, wslot, a, nblk, s);
     }
 }
 
 template <int G>
-static int launch_gather_g(int nat, agpu_neighbors *nl, TersoffState *st, double *eslot, double *wslot,
+static int launch_gather_g(int nat, agpu_neighbors *nl, const BopBufs &B, double *eslot, double *wslot,
                            const agpu_eval_args *a, hipStream_t s)
 {
     const int nblk = (nat + 256 / G - 1) / (256 / G);
-    AGPU_LAUNCH("bop_gather", k_bop_gather<G>, dim3(nblk), dim3(256), 0, s, nat, nl->offsets.as<int>(),
-                st->bslot.as<int>(), st->bcnt.as<int>(), st->rev.as<int>(), st->fslot.as<double>(),
-                st->fown.as<double>(), eslot, wslot, a->f, a->epot_per_at, a->wpot_per_at);
+    AGPU_LAUNCH("bop_gather", k_bop_gather<G>, dim3(nblk), dim3(256), 0, s, nat, nl->offsets.as<int>(), B.bslot,
+                B.bcnt, B.rev, B.fslot, B.fown, eslot, wslot, a->f, a->epot_per_at, a->wpot_per_at);
     return AGPU_OK;
 }
 
-static int launch_gather(int G, int nat, agpu_neighbors *nl, TersoffState *st, double *eslot, double *wslot,
+static int launch_gather(int G, int nat, agpu_neighbors *nl, const BopBufs &B, double *eslot, double *wslot,
                          const agpu_eval_args *a, hipStream_t s)
 {
     switch (G) {
-    case 2: return launch_gather_g<2>(nat, nl, st, eslot, wslot, a, s);
-    case 4: return launch_gather_g<4>(nat, nl, st, eslot, wslot, a, s);
-    case 8: return launch_gather_g<8>(nat, nl, st, eslot, wslot, a, s);
-    case 16: return launch_gather_g<16>(nat, nl, st, eslot, wslot, a, s);
-    case 32: return launch_gather_g<32>(nat, nl, st, eslot, wslot, a, s);
-    default: return launch_gather_g<64>(nat, nl, st, eslot, wslot, a, s);
+    case 2: return launch_gather_g<2>(nat, nl, B, eslot, wslot, a, s);
+    case 4: return launch_gather_g<4>(nat, nl, B, eslot, wslot, a, s);
+    case 8: return launch_gather_g<8>(nat, nl, B, eslot, wslot, a, s);
+    case 16: return launch_gather_g<16>(nat, nl, B, eslot, wslot, a, s);
+    case 32: return launch_gather_g<32>(nat, nl, B, eslot, wslot, a, s);
+    default: return launch_gather_g<64>(nat, nl, B, eslot, wslot, a, s);
     }
 }
 
@@ -1578,9 +1964,197 @@ static int tersoff_scr_compute(agpu_potential *pot, const agpu_eval_args *a)
     return AGPU_OK;
 }
 
+int bop_variant(agpu_potential *pot, int k)
+{
+    if (!pot || !pot->impl) return 0;
+    switch (pot->kind) {
+    case AGPU_KIND_TERSOFF:
+    case AGPU_KIND_KUMAGAI:
+    case AGPU_KIND_BRENNER:
+    case AGPU_KIND_JUSLIN: break;
+    default: return 0;
+    }
+    TersoffState *st = static_cast<TersoffState *>(pot->impl);
+    if (st->scr) return 0;
+    if (k >= 0) st->compact = k == 1;
+    return 2;
+}
+
 bool tersoff_screened(const agpu_potential *pot)
 {
     return pot && pot->impl && static_cast<const TersoffState *>(pot->impl)->scr;
+}
+
+// Per-call scratch of the row path: slot forces by list entry (3 per
+// entry), own forces (3 per atom), per-block records (10 per block),
+// second-level reduction records (NULL: the library's own) and 4 flag words.
+struct RowBufs {
+    double *frev, *fown, *partial, *red;
+    int *stats;
+};
+
+template <typename R, int G, int FAM, bool WB>
+static int launch_rows_k(const char *name, TersoffState *st, const RowBufs &B, agpu_particles *p,
+                         agpu_neighbors *nl, const int *el, const agpu_eval_args *a, double *eslot, double *erev,
+                         double *wslot, double *wrev, int nblk, hipStream_t s)
+{
+    CellArg C;
+    std::memcpy(C.A, p->cell.A, sizeof C.A);
+    AGPU_LAUNCH(name, (k_bop_rows<R, G, FAM, WB>), dim3(nblk), dim3(256), 0, s, st->dev, p->nat,
+                nl->idx_sorted.as<int>(), p->d_r, el, C, nl_rows(nl), nl->rank.as<int>(), nl->offsets.as<int>(),
+                a->mask, B.frev, B.fown, eslot, erev, wslot, wrev, B.partial, a->epot_per_bond, a->f_per_bond,
+                a->wpot_per_bond, B.stats);
+    return AGPU_OK;
+}
+
+template <typename R, int G, int FAM>
+static int launch_rows_wb(const char *name, bool wb, TersoffState *st, const RowBufs &B, agpu_particles *p,
+                          agpu_neighbors *nl, const int *el, const agpu_eval_args *a, double *eslot, double *erev,
+                          double *wslot, double *wrev, int nblk, hipStream_t s)
+{
+    if (wb) return launch_rows_k<R, G, FAM, true>(name, st, B, p, nl, el, a, eslot, erev, wslot, wrev, nblk, s);
+    return launch_rows_k<R, G, FAM, false>(name, st, B, p, nl, el, a, eslot, erev, wslot, wrev, nblk, s);
+}
+
+template <typename R, int G>
+static int launch_rows_g(bool wb, TersoffState *st, const RowBufs &B, agpu_particles *p, agpu_neighbors *nl,
+                         const int *el, const agpu_eval_args *a, double *eslot, double *erev, double *wslot,
+                         double *wrev, int nblk, hipStream_t s)
+{
+    const bool f64 = sizeof(R) == 8;
+    switch (st->dev.fam) {
+    case 3:
+        return launch_rows_wb<R, G, 3>(f64 ? "juslin_fp64" : "juslin_mixed", wb, st, B, p, nl, el, a, eslot, erev,
+                                       wslot, wrev, nblk, s);
+    case 1:
+        return launch_rows_wb<R, G, 1>(f64 ? "kumagai_fp64" : "kumagai_mixed", wb, st, B, p, nl, el, a, eslot,
+                                       erev, wslot, wrev, nblk, s);
+    case 2:
+        return launch_rows_wb<R, G, 2>(f64 ? "brenner_fp64" : "brenner_mixed", wb, st, B, p, nl, el, a, eslot,
+                                       erev, wslot, wrev, nblk, s);
+    default:
+        return launch_rows_wb<R, G, 0>(f64 ? "tersoff_fp64" : "tersoff_mixed", wb, st, B, p, nl, el, a, eslot,
+                                       erev, wslot, wrev, nblk, s);
+    }
+}
+
+template <typename R>
+static int dispatch_rows(int G, bool wb, TersoffState *st, const RowBufs &B, agpu_particles *p, agpu_neighbors *nl,
+                         const int *el, const agpu_eval_args *a, double *eslot, double *erev, double *wslot,
+                         double *wrev, int nblk, hipStream_t s)
+{
+    switch (G) {
+    case 4: return launch_rows_g<R, 4>(wb, st, B, p, nl, el, a, eslot, erev, wslot, wrev, nblk, s);
+    case 8: return launch_rows_g<R, 8>(wb, st, B, p, nl, el, a, eslot, erev, wslot, wrev, nblk, s);
+    case 16: return launch_rows_g<R, 16>(wb, st, B, p, nl, el, a, eslot, erev, wslot, wrev, nblk, s);
+    case 32: return launch_rows_g<R, 32>(wb, st, B, p, nl, el, a, eslot, erev, wslot, wrev, nblk, s);
+    default: return launch_rows_g<R, 64>(wb, st, B, p, nl, el, a, eslot, erev, wslot, wrev, nblk, s);
+    }
+}
+
+// group width of the row path: the longest row, at least 4 lanes
+static int rows_width(const agpu_neighbors *nl)
+{
+    int G = 4;
+    while (G < nl->maxrow) G <<= 1;
+    return G;
+}
+
+// The row path of the unscreened bond-order classes over the current list
+// (rows of at most 64 entries): the bond-order kernel, the gather and the
+// fixed-order energy / virial sum.  Flags are read back unless capturing.
+static int bop_rows_run(agpu_potential *pot, const agpu_eval_args *a, const RowBufs &B, double *eslot,
+                        double *erev, double *wslot, double *wrev, int *h_flags)
+{
+    TersoffState *st = static_cast<TersoffState *>(pot->impl);
+    agpu_particles *p = pot->p;
+    agpu_neighbors *nl = pot->nl;
+    const int nat = p->nat;
+    hipStream_t s = stream();
+    const int G = rows_width(nl);
+    const int nblk = bop_blocks(nat, G);
+    const bool wb = wslot || a->wpot_per_bond;
+    AGPU_CHECK(zero_dev(B.stats, sizeof(int) * 4));
+    const int *el = pot->d_el.as<int>();
+    if (pot->precision == AGPU_PREC_MIXED)
+        AGPU_CHECK(dispatch_rows<float>(G, wb, st, B, p, nl, el, a, eslot, erev, wslot, wrev, nblk, s));
+    else
+        AGPU_CHECK(dispatch_rows<double>(G, wb, st, B, p, nl, el, a, eslot, erev, wslot, wrev, nblk, s));
+    AGPU_LAUNCH("bop_gather", k_bop_rows_gather, dim3((nat + 255) / 256), dim3(256), 0, s, nat,
+                nl->offsets.as<int>(), B.frev, B.fown, eslot, erev, wslot, wrev, a->f, a->epot_per_at,
+                a->wpot_per_at);
+    AGPU_CHECK(reduce_ev(B.partial, nblk, 10, a->epot, a->wpot, nullptr, false, B.red));
+    if (capturing()) return AGPU_OK;     // the MD driver reads the flags after the replay
+    AGPU_HIP(hipMemcpyAsync(h_flags, B.stats, sizeof(int) * 4, hipMemcpyDeviceToHost, s));
+    AGPU_HIP(hipStreamSynchronize(s));
+    if (h_flags[0] & DERR_REV_MISSING)
+        return set_error(AGPU_ERR_NEB, "Tersoff: bond without its reverse bond (inconsistent list)");
+    return AGPU_OK;
+}
+
+static int bop_rows_compute(agpu_potential *pot, const agpu_eval_args *a)
+{
+    TersoffState *st = static_cast<TersoffState *>(pot->impl);
+    const int nat = pot->p->nat;
+    const int64_t total = pot->nl->total;
+    AGPU_CHECK(st->fslot.reserve(sizeof(double) * 3 * (total + 1)));
+    AGPU_CHECK(st->fown.reserve(sizeof(double) * 3 * (nat + 1)));
+    AGPU_CHECK(st->stats.reserve(sizeof(int) * 4));
+    AGPU_CHECK(st->partial.reserve(sizeof(double) * 10 * bop_blocks(nat, rows_width(pot->nl))));
+    AGPU_CHECK(pot->stats.init());
+    double *eslot = nullptr, *erev = nullptr, *wslot = nullptr, *wrev = nullptr;
+    if (a->epot_per_at) {
+        AGPU_CHECK(st->eslot.reserve(sizeof(double) * 2 * (total + 1)));
+        eslot = st->eslot.as<double>();
+        erev = eslot + total + 1;
+    }
+    if (a->wpot_per_at) {
+        AGPU_CHECK(st->wslot.reserve(sizeof(double) * 18 * (total + 1)));
+        wslot = st->wslot.as<double>();
+        wrev = wslot + 9 * (total + 1);
+    }
+    RowBufs B{st->fslot.as<double>(), st->fown.as<double>(), st->partial.as<double>(), nullptr, st->stats.as<int>()};
+    return bop_rows_run(pot, a, B, eslot, erev, wslot, wrev, pot->stats.h);
+}
+
+// bonds of every row (k_bop_bonds) into B.bslot / B.bcnt
+static int bop_bonds_launch(agpu_potential *pot, const BopBufs &B)
+{
+    TersoffState *st = static_cast<TersoffState *>(pot->impl);
+    agpu_particles *p = pot->p;
+    agpu_neighbors *nl = pot->nl;
+    CellArg C;
+    std::memcpy(C.A, p->cell.A, sizeof C.A);
+    const int nat = p->nat;
+    AGPU_LAUNCH("bop_bonds", k_bop_bonds, dim3((nat + 255) / 256), dim3(256), 0, stream(), nat, p->d_r,
+                pot->d_el.as<int>(), C, st->dev, nl->offsets.as<int>(), nl->nbr.as<int2>(), B.bslot, B.bcnt,
+                B.stats);
+    return AGPU_OK;
+}
+
+// reverse slots, the bond-order kernel (G lanes per atom, G >= every bond
+// count), the owner-computes gather and the fixed-order energy / virial sum
+static int bop_plain_tail(agpu_potential *pot, const agpu_eval_args *a, const BopBufs &B, int G, double *eslot,
+                          double *wslot)
+{
+    TersoffState *st = static_cast<TersoffState *>(pot->impl);
+    agpu_particles *p = pot->p;
+    agpu_neighbors *nl = pot->nl;
+    const int nat = p->nat;
+    hipStream_t s = stream();
+    CellArg C;
+    std::memcpy(C.A, p->cell.A, sizeof C.A);
+    const int *el = pot->d_el.as<int>();
+    AGPU_LAUNCH("bop_rev", k_bop_rev, dim3((unsigned)(((long long)nat * G + 255) / 256)), dim3(256), 0, s, nat, G,
+                nl->offsets.as<int>(), nl->nbr.as<int2>(), B.bslot, B.bcnt, B.rev, B.stats);
+    const int *order = nl->idx_sorted.as<int>();
+    if (pot->precision == AGPU_PREC_MIXED)
+        AGPU_CHECK(dispatch_tersoff<float>(G, st, B, nat, order, p->d_r, el, C, nl, a->mask, eslot, wslot, a, s));
+    else
+        AGPU_CHECK(dispatch_tersoff<double>(G, st, B, nat, order, p->d_r, el, C, nl, a->mask, eslot, wslot, a, s));
+    AGPU_CHECK(launch_gather(G, nat, nl, B, eslot, wslot, a, s));
+    // per-block records [e, W(9)] -> epot, wpot
+    return reduce_ev(B.partial, bop_blocks(nat, G), 10, a->epot, a->wpot, nullptr, false, B.red);
 }
 
 int tersoff_compute(agpu_potential *pot, const agpu_eval_args *a)
@@ -1589,12 +2163,19 @@ int tersoff_compute(agpu_potential *pot, const agpu_eval_args *a)
     agpu_particles *p = pot->p;
     agpu_neighbors *nl = pot->nl;
     if (st->Z_rev != p->Z_rev) AGPU_CHECK(tersoff_bind(pot));
-    AGPU_CHECK(nl_update(nl, p, 0));
+    // the row path reads the rows where the builder left them
+    AGPU_CHECK(nl_update(nl, p, 0, false));
     const int nat = p->nat;
     hipStream_t s = stream();
     AGPU_CHECK(zero_dev(a->epot, sizeof(double)));
     if (nat == 0) return AGPU_OK;
-    if (st->scr) return tersoff_scr_compute(pot, a);
+    if (st->scr) {
+        AGPU_CHECK(nl_update(nl, p, 0));
+        return tersoff_scr_compute(pot, a);
+    }
+    if (nl->maxrow <= 64 && !st->compact) return bop_rows_compute(pot, a);
+    // longer rows: compact them to their bonds (at most 64) over CSR rows
+    AGPU_CHECK(nl_update(nl, p, 0));
     const int64_t total = nl->total;
     AGPU_CHECK(st->bslot.reserve(sizeof(int) * (total + 1)));
     AGPU_CHECK(st->rev.reserve(sizeof(int) * (total + 1)));
@@ -1602,6 +2183,8 @@ int tersoff_compute(agpu_potential *pot, const agpu_eval_args *a)
     AGPU_CHECK(st->bcnt.reserve(sizeof(int) * (nat + 1)));
     AGPU_CHECK(st->fown.reserve(sizeof(double) * 3 * (nat + 1)));
     AGPU_CHECK(st->stats.reserve(sizeof(int) * 4));
+    // records for the widest group (64 lanes per atom: the most blocks)
+    AGPU_CHECK(st->partial.reserve(sizeof(double) * 10 * bop_blocks(nat, 64)));
     AGPU_CHECK(pot->stats.init());
     double *eslot = nullptr, *wslot = nullptr;
     if (a->epot_per_at) {
@@ -1612,21 +2195,18 @@ int tersoff_compute(agpu_potential *pot, const agpu_eval_args *a)
         AGPU_CHECK(st->wslot.reserve(sizeof(double) * 9 * (total + 1)));
         wslot = st->wslot.as<double>();
     }
-    int *dstats = st->stats.as<int>();
+    BopBufs B{st->bslot.as<int>(), st->bcnt.as<int>(), st->rev.as<int>(), st->stats.as<int>(),
+              st->fslot.as<double>(), st->fown.as<double>(), st->partial.as<double>(), nullptr};
+    int *dstats = B.stats;
     AGPU_CHECK(zero_dev(dstats, sizeof(int) * 4));
-    CellArg C;
-    std::memcpy(C.A, p->cell.A, sizeof C.A);
-    const int *el = pot->d_el.as<int>();
-    const int nb = (nat + 255) / 256;
-    AGPU_LAUNCH("bop_bonds", k_bop_bonds, dim3(nb), dim3(256), 0, s, nat, p->d_r, el, C, st->dev,
-                nl->offsets.as<int>(), nl->nbr.as<int2>(), st->bslot.as<int>(), st->bcnt.as<int>(), dstats);
+    AGPU_CHECK(bop_bonds_launch(pot, B));
     // group width: the row maximum bounds the bond count without a host
     // round trip when it is small; otherwise read the exact bond maximum
     int maxb = nl->maxrow;
     if (maxb > 64 && capturing())
         return set_error(AGPU_ERR_UNSUPPORTED, "bond-order step capture needs at most 64 neighbours per atom");
     if (maxb > 64) {
-        AGPU_CHECK(int_max(st->bcnt.as<int>(), nat, dstats));
+        AGPU_CHECK(int_max(B.bcnt, nat, dstats));
         AGPU_HIP(hipMemcpyAsync(pot->stats.h, dstats, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
         AGPU_HIP(hipStreamSynchronize(s));
         maxb = pot->stats.h[0];
@@ -1635,18 +2215,7 @@ int tersoff_compute(agpu_potential *pot, const agpu_eval_args *a)
     }
     int G = 2;
     while (G < maxb) G <<= 1;
-    AGPU_LAUNCH("bop_rev", k_bop_rev, dim3((unsigned)(((long long)nat * G + 255) / 256)), dim3(256), 0, s, nat, G,
-                nl->offsets.as<int>(), nl->nbr.as<int2>(), st->bslot.as<int>(), st->bcnt.as<int>(),
-                st->rev.as<int>(), dstats);
-    int nblk = 0;
-    const int *order = nl->idx_sorted.as<int>();
-    if (pot->precision == AGPU_PREC_MIXED)
-        AGPU_CHECK(dispatch_tersoff<float>(G, st, nat, order, p->d_r, el, C, nl, a->mask, eslot, wslot, a, s, &nblk));
-    else
-        AGPU_CHECK(dispatch_tersoff<double>(G, st, nat, order, p->d_r, el, C, nl, a->mask, eslot, wslot, a, s, &nblk));
-    AGPU_CHECK(launch_gather(G, nat, nl, st, eslot, wslot, a, s));
-    // per-block records [e, W(9)] -> epot, wpot
-    AGPU_CHECK(reduce_ev(st->partial.as<double>(), nblk, 10, a->epot, a->wpot, nullptr));
+    AGPU_CHECK(bop_plain_tail(pot, a, B, G, eslot, wslot));
     if (capturing()) return AGPU_OK;     // the MD driver reads the flags after the replay
     AGPU_HIP(hipMemcpyAsync(pot->stats.h, dstats, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
     AGPU_HIP(hipStreamSynchronize(s));

@@ -557,6 +557,19 @@ class TabulatedAlloyEAM(_Potential):
         self._h = h
 
 
+class TabulatedEAM(_Potential):
+    """TabulatedEAM (tabulated_eam.f90): fn = single-element funcfl file
+    (F(rho), effective charge Z(r), rho(r)), elements = filter string.  No
+    mask, as the reference class."""
+    kind = _lib.KIND_EAM
+
+    def __init__(self, fn, elements="*"):
+        super().__init__()
+        h = ctypes.c_void_p()
+        call("agpu_eam_create", ctypes.byref(h), str(fn).encode(), str(elements).encode())
+        self._h = h
+
+
 class Rebo2(_Potential):
     """Second-generation REBO for C and H (rebo2.f90, classname Rebo2) with
     the default tables of rebo2_default_tables.f90.  Keyword arguments follow

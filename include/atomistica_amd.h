@@ -69,6 +69,7 @@ extern "C" {
 #define AGPU_KIND_PME 10          /* pme.f90, classname PME */
 #define AGPU_KIND_JUSLIN 11       /* juslin.f90, classname Juslin */
 #define AGPU_KIND_JUSLIN_SCR 12   /* juslin_scr.f90, classname JuslinScr */
+#define AGPU_KIND_EAM 13          /* tabulated_eam.f90, classname TabulatedEAM */
 
 /* agpu_rebo2_create flags */
 #define AGPU_REBO2_ZERO_TABLES 1  /* rebo2 zero_tables = .true. (rebo2_module.f90:30-42) */
@@ -275,6 +276,15 @@ int agpu_juslin_scr_create(agpu_potential **pot, const agpu_juslin_params *param
  * reading a setfl file; elements = filter string ("*" or NULL = all). */
 int agpu_eam_alloy_create(agpu_potential **pot, const char *fn, const char *elements);
 
+/* TabulatedEAM: tabulated_eam_init (tabulated_eam.f90:141-214) reading a
+ * single-element funcfl file (F(rho), effective charge Z(r), rho(r)); the
+ * kernel is tabulated_eam.f90:334-511: pair energy Z(r)^2 / r per directed
+ * pair with Z scaled by sqrt(Hartree Bohr / 2) (Units.f90 values).  Like the
+ * reference class (features: per_at) it takes no mask and no per-bond
+ * arrays.  Arguments beyond a table extend its last interval (the
+ * reference's inline spline macros do not check their range there). */
+int agpu_eam_create(agpu_potential **pot, const char *fn, const char *elements);
+
 /* Ewald Coulomb: real-space erfc sum and self term as pme.f90:332-484, with
  * the reciprocal space as a direct FP64 structure-factor sum over
  * |k| <= kmax.  beta <= 0 selects the reference rule
@@ -342,9 +352,11 @@ int agpu_potential_energy_and_forces(agpu_potential *pot, agpu_particles *p,
  * CPU loops): times the interchangeable kernel variants on the bound
  * configuration with HIP events, reps calls each, and keeps the fastest for
  * later calls on pot and nl -- the neighbour-list pair search (thread per
- * atom or wavefront per bin block; identical rows) and, for single-element
- * tabulated EAM, lanes per atom x row entries per lane.  A text report of
- * the timings and choices is written to report (len bytes, may be NULL). */
+ * atom or wavefront per bin block; identical rows), for single-element
+ * tabulated EAM, lanes per atom x row entries per lane, and for the
+ * unscreened bond-order classes, the row path (a lane per row entry) or the
+ * bond-compacting path.  A text report of the timings and choices is written
+ * to report (len bytes, may be NULL). */
 int agpu_potential_autotune(agpu_potential *pot, agpu_particles *p, agpu_neighbors *nl, int reps,
                             char *report, int len);
 
@@ -352,8 +364,9 @@ int agpu_potential_autotune(agpu_potential *pot, agpu_particles *p, agpu_neighbo
  * autotuner choice): pair_mode 0 = from the bin occupancy, 1 = thread per
  * atom, 2 = wavefront per bin block; variant -1 keeps the current kernel
  * variant, else selects 0 <= variant < *nvariants, the number of variants
- * this binding offers (0 for all but single-element EAM; variant 0 is then
- * the only accepted value). */
+ * this binding offers (single-element EAM: the lanes x entries shapes;
+ * unscreened bond-order classes: 0 = row path, 1 = bond-compacting path;
+ * 0 for the other classes, variant 0 is then the only accepted value). */
 int agpu_potential_set_launch(agpu_potential *pot, agpu_neighbors *nl, int pair_mode, int variant,
                               int *nvariants);
 

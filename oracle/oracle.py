@@ -406,7 +406,7 @@ def eam(tables, r, el, cell, nl, mask=None, per_at=False, nthreads=1):
                         _i(nF), _d(dF), cF, _i(nR), _d(dR), cR, _i(nP), _d(dP), cP,
                         ctypes.byref(epot), _d(f), _d(w),
                         _d(epa) if per_at else None, _d(wpa) if per_at else None,
-                        int(nthreads))
+                        int(bool(tables.get("funcfl"))), int(nthreads))
     _check(err, "eam (spline argument outside its table?)")
     out = [epot.value, f, w.reshape(3, 3).T.copy()]
     if per_at:
@@ -573,6 +573,36 @@ def pme_recip(r, q, cell, beta, grid, order=8):
 # --------------------------------------------------------------------------
 # setfl tables for eam()
 # --------------------------------------------------------------------------
+
+# Units of the reference (src/support/Units.f90:74,76)
+HARTREE_REF = 27.2113961
+BOHR_REF = 0.529177249
+
+
+def read_funcfl_tables(path):
+    """Tables of a single-element funcfl file read as tabulated_eam_init does
+    (src/potentials/eam/tabulated_eam.f90:141-214): a comment line, Z mass a0
+    lattice, nF dF nr dr cutoff, then F(rho) (nF values from 0), Z(r) and
+    rho(r) (nr values each, from 0), read list-directed; Z is scaled by
+    sqrt(Hartree Bohr / 2) (:178).  The pair entry holds the scaled Z table
+    and "funcfl" is set, so eam() takes r*phi = Z^2."""
+    with open(path) as f:
+        lines = f.read().splitlines()
+    head = lines[1].split()
+    Z = int(head[0])
+    nF, dF, nr, dr, cutoff = lines[2].replace(",", " ").split()[:5]
+    nF, nr = int(nF), int(nr)
+    dF, dr, cutoff = float(dF), float(dr), float(cutoff)
+    vals = " ".join(lines[3:]).replace(",", " ").replace("D", "e").replace("d", "e").split()
+    need = nF + 2 * nr
+    if len(vals) < need:
+        raise OracleError("funcfl file truncated: %d of %d values" % (len(vals), need))
+    v = np.array([float(x) for x in vals[:need]])
+    Fy, Zy, Ry = v[:nF], v[nF:nF + nr], v[nF + nr:need]
+    zs = np.sqrt(0.5 * HARTREE_REF * BOHR_REF)
+    return {"nel": 1, "cutoff": cutoff, "F": [(Fy, dF)], "rho": [(Ry, dr)],
+            "phi": [[(zs * Zy, dr)]], "names": None, "Z": [Z], "funcfl": True}
+
 
 def read_setfl_tables(path):
     """Tables of a setfl file in the form the oracle's eam() takes, read as

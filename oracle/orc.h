@@ -96,7 +96,7 @@ int orc_eam(int nel, int nat, const double *r, const int *el,
             const double *const *coef_rho, const int *n_phi,
             const double *dphi, const double *const *coef_phi,
             double *epot, double *f, double *wpot, double *epot_per_at,
-            double *wpot_per_at, int nthreads);
+            double *wpot_per_at, int funcfl, int nthreads);
 
 /* ---- Ewald ---- */
 int orc_ewald_real(int nat, const double *r, const double *q,

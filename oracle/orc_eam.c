@@ -12,6 +12,12 @@
  * coefficient block per table; pair tables hold r*phi(r) already scaled by 1/2
  * (scale_y_axis, simple_spline.f90:756-773).  el[] holds database indices
  * (el2db), -1 for atoms the potential ignores.
+ *
+ * funcfl != 0: the single-element TabulatedEAM kernel
+ * (src/potentials/eam/tabulated_eam.f90:334-511) instead: the pair table is
+ * the effective charge Z(r) scaled by sqrt(Hartree Bohr / 2) (:178), and the
+ * pair energy per directed pair is Z^2 / r with derivative 2 Z Z' / r -
+ * Z^2 / r^2 (:446-450), i.e. the r*phi form above with r*phi = Z^2.
  */
 #include <math.h>
 #include <stdlib.h>
@@ -109,7 +115,7 @@ int orc_eam(int nel, int nat, const double *r, const int *el,
             const double *const *coef_rho, const int *n_phi,
             const double *dphi, const double *const *coef_phi,
             double *epot, double *f, double *wpot, double *epot_per_at,
-            double *wpot_per_at, int nthreads)
+            double *wpot_per_at, int funcfl, int nthreads)
 {
     double cutoff_sq = cutoff * cutoff;
 #ifdef _OPENMP
@@ -186,6 +192,11 @@ int orc_eam(int nel, int nat, const double *r, const int *el,
                 int pidx = dbi * nel + dbj;
                 double phi, dphi_;
                 spl_f_df_extrap(n_phi[pidx], 0.0, dphi[pidx], coef_phi[pidx], ad, &phi, &dphi_);
+                if (funcfl) {
+                    /* Z(r): r phi = Z^2, (r phi)' = 2 Z Z' */
+                    dphi_ = 2 * phi * dphi_;
+                    phi = phi * phi;
+                }
                 double rad = 1.0 / ad;
                 et[i] += phi * rad;
                 double fac;

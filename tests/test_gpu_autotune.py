@@ -97,3 +97,64 @@ def test_autotune_keeps_parity():
         assert "nl_pairs ->" in rep and "eam ->" in rep
         e, f, w, *_ = pot.energy_and_forces(p, nl)
         assert abs(e - e0) <= REL * abs(e0) and close(f, f0) and close(w, w0)
+
+
+@pytest.mark.parametrize("case", ["si_rattled", "sic_mask_mixed", "juslin_wch", "long_rows"])
+def test_bond_order_paths_parity(case):
+    """Both bond-order paths (row path, bond-compacting path) against the
+    oracle, with per-atom energies and virials; rows longer than 64 entries
+    (a list requested out to 8 A) fall back to the compacting path."""
+    from atomistica_amd import parameters as P
+    rng = np.random.default_rng(4)
+    prm, mask, avgn = P.Tersoff_PRB_39_5566_Si_C, None, 100
+    pos, cell, pbc, _ = W.diamond_si(4, sigma=0.12, seed=3)
+    el = np.ones(len(pos), int)
+    if case == "sic_mask_mixed":
+        el = rng.integers(0, 2, len(pos))
+        mask = (rng.random(len(pos)) < 0.6).astype(np.int32)
+    elif case == "juslin_wch":
+        prm = P.Juslin_JAP_98_123520_WCH
+        pos, cell, pbc, _ = W.diamond_si(3, a=4.4, sigma=0.1, seed=4)
+        el = rng.integers(0, 3, len(pos))
+    elif case == "long_rows":
+        avgn = 400
+    names = prm["el"]
+    Z = np.array([P.ATOMIC_NUMBERS[names[e]] for e in el], np.int32)
+    cls = N.Juslin if prm.get("family") == "juslin" else N.Tersoff
+    kw = {k: v for k, v in prm.items() if k not in ("__ref__", "family")}
+    nlo = O.nl_build(pos, cell, pbc, max(np.max(prm["r2"]), 8.0 if case == "long_rows" else 0.0), avgn=avgn)
+    e0, f0, w0, epa0, wpa0 = O.tersoff(prm, pos, np.asarray(el, np.int32), cell, nlo, mask=mask, per_at=True)
+    out = []
+    for variant in (0, 1):
+        p = N.from_atoms(pos, cell, pbc, Z)
+        nl = N.Neighbors(avgn)
+        if case == "long_rows":
+            nl.request_interaction_range(8.0)
+        pot = cls(**kw)
+        pot.bind_to(p, nl)
+        assert pot.set_launch(nl, 0, variant) == 2
+        r = pot.energy_and_forces(p, nl, mask=mask, epot_per_at=True, wpot_per_at=True)
+        e, f, w, epa, _, _, wpa, _ = r
+        print("bond order", case, "variant", variant, "E", e, "oracle", e0, flush=True)
+        assert abs(e - e0) <= REL * abs(e0)
+        assert close(f, f0) and close(w, w0) and close(epa, epa0) and close(wpa, wpa0)
+        if case == "long_rows":
+            assert nl.get_size()[1] > 64
+        out.append(f)
+    assert close(out[0], out[1], 1e-12)
+
+
+def test_autotune_bond_order_report():
+    pos, cell, pbc, Z = W.diamond_si(6, sigma=0.05, seed=1)
+    p = N.from_atoms(pos, cell, pbc, Z)
+    nl = N.Neighbors(100)
+    pot = N.Tersoff()
+    pot.bind_to(p, nl)
+    rep = pot.autotune(p, nl, reps=2)
+    print(rep, flush=True)
+    assert "bond order rows" in rep and "bond order compacted" in rep and "bond order ->" in rep
+    e, f, w, *_ = pot.energy_and_forces(p, nl)
+    nlo = O.nl_build(pos, cell, pbc, 3.0)
+    from atomistica_amd import parameters as P
+    e0, f0, w0 = O.tersoff(P.Tersoff_PRB_39_5566_Si_C, pos, np.ones(len(pos), np.int32), cell, nlo)
+    assert abs(e - e0) <= REL * abs(e0) and close(f, f0) and close(w, w0)

@@ -314,12 +314,15 @@ def test_ewald_requires_full_pbc():
     assert ei.value.code == 8
 
 
-def test_per_bond_request_is_unsupported():
-    pos, cell, pbc, Z = W.diamond_si(2, sigma=0.0)
+def test_per_bond_request_is_unsupported_for_ewald():
+    """Classes without per-bond output in the reference (no per_bond
+    feature) refuse the arrays as gen_factory.py:303-304 does."""
+    pos, cell, pbc, Z, q = W.rocksalt(2, sigma=0.0, seed=5)
     p = particles(pos, cell, pbc, Z)
-    nl = N.Neighbors(100)
-    pot = N.Tersoff()
+    nl = N.Neighbors(300)
+    pot = N.Ewald(cutoff=6.0)
     pot.bind_to(p, nl)
     with pytest.raises(AgpuError) as ei:
-        pot.energy_and_forces(p, nl, epot_per_bond=True)
+        # the potential-class argument list (Coulomb classes have their own)
+        N._Potential.energy_and_forces(pot, p, nl, charges=q, epot_per_bond=True)
     assert ei.value.code == 6
