@@ -148,6 +148,7 @@ PROTOTYPES = {
     "agpu_potential_get_cutoff": [vp, dp],
     "agpu_potential_bind_to": [vp, vp, vp],
     "agpu_potential_energy_and_forces": [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, i],
+    "agpu_potential_energy_and_forces_noalloc": [vp, vp, vp, vp, vp, vp, vp],
     "agpu_coulomb_potential": [vp, vp, vp, vp, vp, i],
     "agpu_potential_autotune": [vp, vp, vp, i, ctypes.c_char_p, i],
     "agpu_potential_set_launch": [vp, vp, i, i, ip],

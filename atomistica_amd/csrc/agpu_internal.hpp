@@ -140,6 +140,9 @@ namespace agpu {
 // always).  csr = false: the caller reads the rows through NlRows only, so
 // the copy of staged rows into the CSR rows is left until a caller asks.
 int nl_update(agpu_neighbors *nl, agpu_particles *p, int force, bool csr = true);
+// true when nl_update would keep the list as it is (no rebuild, nothing
+// allocated): imported, held by a driver, or built from these positions
+bool nl_current(const agpu_neighbors *nl, const agpu_particles *p);
 // graph-capture mode (no host synchronisation, no memset nodes)
 bool capturing();
 void set_capturing(bool on);

@@ -739,6 +739,12 @@ static void geometry(const agpu_particles *p, agpu_neighbors *nl, BinGeom &g)
     }
 }
 
+bool nl_current(const agpu_neighbors *nl, const agpu_particles *p)
+{
+    if (nl->external || nl->held) return nl->p == p && nl->nat == p->nat;
+    return nl->p == p && nl->pos_rev == p->pos_rev && nl->cell_rev == p->cell_rev && nl->nat == p->nat;
+}
+
 int nl_update(agpu_neighbors *nl, agpu_particles *p, int force, bool csr)
 {
     if (nl->external) {

@@ -23,6 +23,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include <rocprim/device/device_scan.hpp>
@@ -2224,9 +2225,120 @@ int tersoff_compute(agpu_potential *pot, const agpu_eval_args *a)
     return AGPU_OK;
 }
 
+// -------------------------------------------------------------------------
+// Allocation-free energy_and_forces of the unscreened bond-order classes.
+// The per-call scratch (bond slots, reverse slots, slot and own forces,
+// per-block records, flags) lives in a file-scope device array of fixed
+// size instead of library buffers, for callers that cannot allocate device
+// memory.  Every device holds its own copy of the array, at its own
+// address, until the process ends; host code uses the address
+// hipGetSymbolAddress returns on that device, never the array's name.
+// All calls on one device share its array: calls that overlap on one
+// device (other streams, other host threads) race.
+// -------------------------------------------------------------------------
+constexpr size_t NOALLOC_BYTES = 64ull << 20;
+constexpr int NOALLOC_DEVICES = 64;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+static std::mutex g_scratch_mu;
+static unsigned char *g_scratch_ptr[NOALLOC_DEVICES] = {};
+
+// the address of this device's copy of scratch, looked up once per device
+static int scratch_base(unsigned char **out)
+{
+    int dev = 0;
+    AGPU_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_DEVICES) return set_error(AGPU_ERR_ARG, "noalloc: device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return set_error(AGPU_ERR_HIP, std::string("noalloc: hipGetSymbolAddress(scratch): ") +
+                                               hipGetErrorString(e));
+        g_scratch_ptr[dev] = static_cast<unsigned char *>(ptr);
+    }
+    *out = g_scratch_ptr[dev];
+    return AGPU_OK;
+}
+
+// 256-byte aligned sub-buffers of the scratch array, in order; returns the
+// bytes the layout needs (compared with NOALLOC_BYTES before any use)
+static size_t noalloc_layout(unsigned char *base, int64_t total, int nat, int G, RowBufs *B)
+{
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        unsigned char *p = base ? base + off : nullptr;
+        off += (bytes + 255) & ~size_t(255);
+        return p;
+    };
+    const int nblk = bop_blocks(nat, G);
+    const int nred = (nblk + 255) / 256;
+    B->frev = reinterpret_cast<double *>(take(sizeof(double) * 3 * (size_t)(total + 1)));
+    B->fown = reinterpret_cast<double *>(take(sizeof(double) * 3 * (size_t)(nat + 1)));
+    B->partial = reinterpret_cast<double *>(take(sizeof(double) * 10 * (size_t)nblk));
+    B->red = reinterpret_cast<double *>(take(sizeof(double) * 10 * (size_t)nred));
+    B->stats = reinterpret_cast<int *>(take(sizeof(int) * 4));
+    return off;
+}
+
+// The row path with its scratch in the array above
+static int tersoff_compute_noalloc(agpu_potential *pot, agpu_particles *p, agpu_neighbors *nl, double *epot,
+                                   double *f, double *wpot, const int *mask)
+{
+    TersoffState *st = static_cast<TersoffState *>(pot->impl);
+    if (st->scr) return set_error(AGPU_ERR_UNSUPPORTED, "noalloc: the screened variants are not covered");
+    if (st->Z_rev != p->Z_rev)
+        return set_error(AGPU_ERR_STATE, "noalloc: atomic numbers changed since bind_to");
+    if (!nl_current(nl, p))
+        return set_error(AGPU_ERR_STATE, "noalloc: the neighbour list is not current (agpu_neighbors_update first)");
+    const int nat = p->nat;
+    if (nl->maxrow > 64)
+        return set_error(AGPU_ERR_UNSUPPORTED, "noalloc: rows of more than 64 neighbours");
+    RowBufs B;
+    const int G = rows_width(nl);
+    const size_t need = noalloc_layout(nullptr, nl->total, nat, G, &B);
+    if (need > NOALLOC_BYTES)
+        return set_error(AGPU_ERR_ARG, "noalloc: " + std::to_string(nat) + " atoms and " +
+                                           std::to_string(nl->total) + " list entries need " +
+                                           std::to_string(need) + " bytes of scratch; the maximum is " +
+                                           std::to_string(NOALLOC_BYTES) +
+                                           " (24 per list entry + about 25 per atom)");
+    unsigned char *base = nullptr;
+    AGPU_CHECK(scratch_base(&base));
+    noalloc_layout(base, nl->total, nat, G, &B);
+    AGPU_CHECK(zero_dev(epot, sizeof(double)));
+    if (nat == 0) return AGPU_OK;
+    agpu_eval_args a{};
+    a.epot = epot;
+    a.f = f;
+    a.wpot = wpot;
+    a.mask = mask;
+    int h[4] = {0, 0, 0, 0};
+    // the flags start at zero (in bop_rows_run); every other buffer is
+    // written before it is read
+    return bop_rows_run(pot, &a, B, nullptr, nullptr, nullptr, nullptr, h);
+}
+
 }  // namespace agpu
 
 using namespace agpu;
+
+extern "C" int agpu_potential_energy_and_forces_noalloc(agpu_potential *pot, agpu_particles *p,
+                                                        agpu_neighbors *nl, double *epot, double *f, double *wpot,
+                                                        const int *mask)
+{
+    if (!pot || !p || !nl || !epot || !f || !wpot) return set_error(AGPU_ERR_ARG, "noalloc: bad arguments");
+    if (!pot->bound || pot->p != p || pot->nl != nl)
+        return set_error(AGPU_ERR_STATE, "noalloc: potential not bound to these objects");
+    switch (pot->kind) {
+    case AGPU_KIND_TERSOFF:
+    case AGPU_KIND_KUMAGAI:
+    case AGPU_KIND_BRENNER:
+    case AGPU_KIND_JUSLIN: return tersoff_compute_noalloc(pot, p, nl, epot, f, wpot, mask);
+    default: return set_error(AGPU_ERR_UNSUPPORTED, "noalloc: only the unscreened bond-order classes");
+    }
+}
 
 extern "C" int agpu_tersoff_create(agpu_potential **pot, const agpu_tersoff_params *params)
 {

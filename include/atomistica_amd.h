@@ -348,6 +348,24 @@ int agpu_potential_energy_and_forces(agpu_potential *pot, agpu_particles *p,
                                      double *f_per_bond, double *wpot_per_at,
                                      double *wpot_per_bond, int flags);
 
+/* Allocation-free energy_and_forces.  It has no counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * unscreened bond-order classes (Tersoff, Kumagai, Brenner, Juslin) only;
+ * device pointers only (epot set, f and wpot accumulated, mask nullable).
+ * Nothing is allocated: the potential must be bound to p and nl with the
+ * current atomic numbers, and the list current for the positions
+ * (agpu_neighbors_update called beforehand), else AGPU_ERR_STATE.  The
+ * per-call scratch is a fixed 64 MiB device array per GPU (24 bytes per list
+ * entry plus about 25 per atom; larger systems return AGPU_ERR_ARG), held
+ * until the process ends and shared by every call on that GPU: calls that
+ * overlap on one GPU race.  Rows of more than 64 neighbours return
+ * AGPU_ERR_UNSUPPORTED.  Results equal those of
+ * agpu_potential_energy_and_forces on its row path (set_launch variant 0,
+ * the default). */
+int agpu_potential_energy_and_forces_noalloc(agpu_potential *pot, agpu_particles *p, agpu_neighbors *nl,
+                                             double *epot, double *f, double *wpot, const int *mask);
+
 /* Launch-shape autotuner (no counterpart in the reference, whose kernels are
  * CPU loops): times the interchangeable kernel variants on the bound
  * configuration with HIP events, reps calls each, and keeps the fastest for
