@@ -690,10 +690,13 @@ k_bop_gather(int nat, const int *__restrict__ offsets, const int *__restrict__ b
 // g(cos), dg and h of pass 1 are kept for pass 2 when G <= 8; the virial
 // of the terms centred at i is -sum_j r_ij (x) F_j unless per-bond virials
 // are needed (WB: the reference's per-bond wij, bop_kernel.f90:1325-1512).
+// HT = false: every h-term of the parameter set is 1 (mubo = 0), so h and
+// dh are constants.  The reverse entry is looked up first, so its loads
+// overlap the arithmetic.
 // -------------------------------------------------------------------------
 constexpr int DERR_REV_MISSING = DERR_BOND_OVERFLOW;
 
-template <typename R, int G, int FAM, bool WB>
+template <typename R, int G, int FAM, bool WB, bool HT = true>
 __global__ void __launch_bounds__(256)
 k_bop_rows(TersoffDev P, int nat, const int *__restrict__ order, const double *__restrict__ r,
            const int *__restrict__ el, CellArg C, NlRows L, const int *__restrict__ slot,
@@ -726,10 +729,39 @@ k_bop_rows(TersoffDev P, int nat, const int *__restrict__ order, const double *_
     int j = 0, code = 0, typ = 0;
     R rn[3] = {(R)0, (R)0, (R)0}, rl = (R)1, fc = (R)0, dfc = (R)0;
     bool ok = false;
+    int rt = -1;                        // list index of the reverse entry j -> i
     if (act) {
         const int2 e = row[lane];
         j = e.x;
         code = e.y;
+        // reverse entry j -> i (image -n) in j's row
+        {
+            int a0, b0, c0;
+            dc_unpack(code, a0, b0, c0);
+            const int want = dc_pack(-a0, -b0, -c0);
+            int nj = 0;
+            const int2 *rowj = nl_row(L, L.S > 0 ? slot[j] : 0, j, nj);
+            int t = -1;
+            if (G <= 8) {
+                // every candidate load issued before the first compare
+                int2 ej[G];
+#pragma unroll
+                for (int q = 0; q < G; ++q) ej[q] = q < nj ? rowj[q] : make_int2(-1, 0);
+#pragma unroll
+                for (int q = G - 1; q >= 0; --q)
+                    if (ej[q].x == i && ej[q].y == want) t = q;
+                if (nj > G) t = -1;     // longer rows only on hosts that chose a wider G
+            } else {
+                for (int q = 0; q < nj; ++q) {
+                    const int2 eq = rowj[q];
+                    if (eq.x == i && eq.y == want) {
+                        t = q;
+                        break;
+                    }
+                }
+            }
+            rt = t < 0 ? -1 : offsets[j] + t;
+        }
         const int elj = el[j];
         if (eli >= 0 && elj >= 0) {
             int a, b, c;
@@ -793,7 +825,12 @@ k_bop_rows(TersoffDev P, int nat, const int *__restrict__ order, const double *_
         if (on && k != lane && tpk >= 0) {
             const R costh = s_rn[0][tk] * rn[0] + s_rn[1][tk] * rn[1] + s_rn[2][tk] * rn[2];
             g_fn<R, FAM>(P, elc, tpk, costh, g, dg);
-            h_fn<R, FAM>(P, typ, tpk, rl - s_rl[tk], hv, dh);
+            if (HT) {
+                h_fn<R, FAM>(P, typ, tpk, rl - s_rl[tk], hv, dh);
+            } else {
+                hv = (R)1;
+                dh = (R)0;
+            }
             zij += s_fc[tk] * g * hv;
         }
         if (CACHE) {
@@ -857,7 +894,16 @@ k_bop_rows(TersoffDev P, int nat, const int *__restrict__ order, const double *_
             } else {
                 const R cs0 = rnk[0] * rn[0] + rnk[1] * rn[1] + rnk[2] * rn[2];
                 g_fn<R, FAM>(P, elc, tpk, cs0, g, dg);
-                h_fn<R, FAM>(P, typ, tpk, rl - rlik, hv, dh);
+                if (HT) {
+                    h_fn<R, FAM>(P, typ, tpk, rl - rlik, hv, dh);
+                } else {
+                    hv = (R)1;
+                    dh = (R)0;
+                }
+            }
+            if (!HT) {
+                hv = (R)1;
+                dh = (R)0;
             }
             const R costh = rnk[0] * rn[0] + rnk[1] * rn[1] + rnk[2] * rn[2];
             R dkc[3];
@@ -925,31 +971,16 @@ k_bop_rows(TersoffDev P, int nat, const int *__restrict__ order, const double *_
         for (int a = 0; a < 3; ++a)
             for (int b = 0; b < 3; ++b) w[a + 3 * b] = -rijd[a] * F[b];
     if (act) {
-        // reverse entry j -> i (image -n) in j's row
-        int a0, b0, c0;
-        dc_unpack(code, a0, b0, c0);
-        const int want = dc_pack(-a0, -b0, -c0);
-        int nj = 0;
-        const int2 *rowj = nl_row(L, L.S > 0 ? slot[j] : 0, j, nj);
-        int t = -1;
-        for (int q = 0; q < nj; ++q) {
-            const int2 ej = rowj[q];
-            if (ej.x == i && ej.y == want) {
-                t = q;
-                break;
-            }
-        }
         const size_t sown = (size_t)offsets[i] + lane;
-        if (t < 0) {
+        if (rt < 0) {
             atomicOr(flags, DERR_REV_MISSING);
         } else {
-            const size_t rt = (size_t)offsets[j] + t;
-            frev[3 * rt + 0] = F[0];
-            frev[3 * rt + 1] = F[1];
-            frev[3 * rt + 2] = F[2];
+            frev[3 * (size_t)rt + 0] = F[0];
+            frev[3 * (size_t)rt + 1] = F[1];
+            frev[3 * (size_t)rt + 2] = F[2];
             if (erev) erev[rt] = e_ij;
             if (wrev)
-                for (int q = 0; q < 9; ++q) wrev[9 * rt + q] = w[q];
+                for (int q = 0; q < 9; ++q) wrev[9 * (size_t)rt + q] = w[q];
         }
         if (eslot) eslot[sown] = e_ij;
         if (wslot)
@@ -1994,18 +2025,28 @@ struct RowBufs {
     int *stats;
 };
 
-template <typename R, int G, int FAM, bool WB>
+template <typename R, int G, int FAM, bool WB, bool HT = true>
 static int launch_rows_k(const char *name, TersoffState *st, const RowBufs &B, agpu_particles *p,
                          agpu_neighbors *nl, const int *el, const agpu_eval_args *a, double *eslot, double *erev,
                          double *wslot, double *wrev, int nblk, hipStream_t s)
 {
     CellArg C;
     std::memcpy(C.A, p->cell.A, sizeof C.A);
-    AGPU_LAUNCH(name, (k_bop_rows<R, G, FAM, WB>), dim3(nblk), dim3(256), 0, s, st->dev, p->nat,
+    AGPU_LAUNCH(name, (k_bop_rows<R, G, FAM, WB, HT>), dim3(nblk), dim3(256), 0, s, st->dev, p->nat,
                 nl->idx_sorted.as<int>(), p->d_r, el, C, nl_rows(nl), nl->rank.as<int>(), nl->offsets.as<int>(),
                 a->mask, B.frev, B.fown, eslot, erev, wslot, wrev, B.partial, a->epot_per_bond, a->f_per_bond,
                 a->wpot_per_bond, B.stats);
     return AGPU_OK;
+}
+
+// true when some h-term of the parameter set differs from 1: mubo of the
+// Tersoff and Brenner forms, alpha of Kumagai; Juslin's omega exp(...) always
+static bool has_hterm(const TersoffDev &D)
+{
+    if (D.fam == 3) return true;
+    for (int k = 0; k < 6; ++k)
+        if ((D.fam == 1 ? D.kalpha[k] : D.mubo[k]) != 0.0) return true;
+    return false;
 }
 
 template <typename R, int G, int FAM>
@@ -2013,6 +2054,11 @@ static int launch_rows_wb(const char *name, bool wb, TersoffState *st, const Row
                           agpu_neighbors *nl, const int *el, const agpu_eval_args *a, double *eslot, double *erev,
                           double *wslot, double *wrev, int nblk, hipStream_t s)
 {
+    // the h-term-free forms of the Tersoff family (FAM 0) without per-bond virials
+    if constexpr (FAM == 0)
+        if (!wb && !has_hterm(st->dev))
+            return launch_rows_k<R, G, FAM, false, false>(name, st, B, p, nl, el, a, eslot, erev, wslot, wrev, nblk,
+                                                          s);
     if (wb) return launch_rows_k<R, G, FAM, true>(name, st, B, p, nl, el, a, eslot, erev, wslot, wrev, nblk, s);
     return launch_rows_k<R, G, FAM, false>(name, st, B, p, nl, el, a, eslot, erev, wslot, wrev, nblk, s);
 }
