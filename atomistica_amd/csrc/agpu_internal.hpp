@@ -111,7 +111,7 @@ struct agpu_neighbors {
     int maxrow = 0;
     // device storage
     agpu::DevBuf key, key_sorted, idx, idx_sorted, cell_start, atoms_sorted,
-        counts, offsets, nbr, sort_tmp, rowmax, rank, stage;
+        counts, offsets, nbr, sort_tmp, rowmax, rank, stage, pos32, part;
     int stage_w = 0;              // staging stride (entries per atom)
     bool radix = false;           // bin sort by radix sort (after a very full bin)
     // the list was imported from the caller (LAMMPS) and is not rebuilt

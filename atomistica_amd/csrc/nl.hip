@@ -164,9 +164,11 @@ k_cell_order(int ncell, const int *__restrict__ cell_start, int *__restrict__ id
     }
 }
 
-__global__ void k_gather(int nat, const double *__restrict__ r, BinGeom g,
+// pos32 (optional): the wrapped position r + A.w rounded to FP32, with the
+// wrap flag (w != 0) in bit 31 of .w
+__global__ void k_gather(int nat, const double *__restrict__ r, BinGeom g, CellArg C,
                          const int *__restrict__ idx_sorted, SortedAtom *__restrict__ out,
-                         int *__restrict__ rank)
+                         int *__restrict__ rank, float4 *__restrict__ pos32)
 {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= nat) return;
@@ -182,6 +184,12 @@ __global__ void k_gather(int nat, const double *__restrict__ r, BinGeom g,
     a.w = dc_pack(w[0], w[1], w[2]);
     out[p] = a;
     rank[j] = p;
+    if (pos32) {
+        float q[3];
+        for (int x = 0; x < 3; ++x) q[x] = (float)(rj[x] + image_x(C, x, w[0], w[1], w[2]));
+        const int flag = (w[0] | w[1] | w[2]) != 0 ? (int)0x80000000u : 0;
+        pos32[p] = make_float4(q[0], q[1], q[2], __int_as_float(j | flag));
+    }
 }
 
 constexpr int NL_R = 4;         // candidate slots per lane (registers): 256 per pass
@@ -251,6 +259,21 @@ __device__ __forceinline__ double pair_d2_pre(double xi, double yi, double zi, d
     return dot3_nc(d);
 }
 
+// Row statistics of a build, one record per wavefront (reduced by
+// k_offsets_tail; atomics on one address from every wavefront serialise):
+// part[2w] = the longest row, part[2w + 1] = 1 + the largest caller index
+// with a non-empty row (the last row of the list, on which the capacity rule
+// is decided)
+__device__ __forceinline__ void row_stats_wave(int longest, int last, int2 *__restrict__ part, int w)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        longest = max(longest, __shfl_xor(longest, o, 64));
+        last = max(last, __shfl_xor(last, o, 64));
+    }
+    if ((threadIdx.x & 63) == 0) part[w] = make_int2(longest, last);
+}
+
 // One wavefront per block of B x B x B home bins (B = 1 for dense bins).
 // The block's candidates -- the atoms of its (B + 2 nd)^3 stencil bins in
 // x-major bin order, ascending atom order inside each bin -- are held in
@@ -268,14 +291,18 @@ template <bool ORTHO, bool MULTI>
 __global__ void __launch_bounds__(256)
 k_bin_pairs(int B, BinGeom g, CellArg C, double cutoff_sq, const SortedAtom *__restrict__ atoms,
             const int *__restrict__ cell_start, int *__restrict__ counts, int2 *__restrict__ stage,
-            int S, int *__restrict__ flags, int *__restrict__ rowmax)
+            int S, int *__restrict__ flags, int2 *__restrict__ part)
 {
     __shared__ NlShared sh[NL_WAVES];
     NlShared &L = sh[threadIdx.x >> 6];
     const int lane = threadIdx.x & 63;
     const int nbx = (g.nc[0] + B - 1) / B, nby = (g.nc[1] + B - 1) / B, nbz = (g.nc[2] + B - 1) / B;
     const int blk = blockIdx.x * NL_WAVES + (threadIdx.x >> 6);
-    if (blk >= nbx * nby * nbz) return;
+    // every wavefront of the grid leaves its row-statistics record
+    if (blk >= nbx * nby * nbz) {
+        if (lane == 0) part[blk] = make_int2(0, 0);
+        return;
+    }
     const int bz = blk % nbz, by = (blk / nbz) % nby, bx = blk / (nbz * nby);
     const int h0[3] = {bx * B + 1, by * B + 1, bz * B + 1};
     const int nh = B * B * B;
@@ -299,10 +326,13 @@ k_bin_pairs(int B, BinGeom g, CellArg C, double cutoff_sq, const SortedAtom *__r
         if (lane >= o) hpre += v;
     }
     const int MH = __shfl(hpre, 63, 64);
-    if (MH == 0) return;
+    if (MH == 0) {
+        if (lane == 0) part[blk] = make_int2(0, 0);
+        return;
+    }
     const int sx = B + 2 * g.nd[0], sy = B + 2 * g.nd[1], sz = B + 2 * g.nd[2];
     const int nst = sx * sy * sz;
-    int wmax = 0;
+    int wmax = 0, wlast = 0;
     bool stage_over = false, img_bad = false;
 
     for (int a0 = 0; a0 < MH; a0 += NL_HMAX) {
@@ -479,13 +509,11 @@ k_bin_pairs(int B, BinGeom g, CellArg C, double cutoff_sq, const SortedAtom *__r
             const int c = L.hcnt[lane];
             counts[L.hi[lane]] = c;
             wmax = max(wmax, c);
+            if (c > 0) wlast = max(wlast, L.hi[lane] + 1);
             if (c > S) stage_over = true;
         }
     }
-    // the row maximum is reduced afterwards from counts (k_int_max); only the
-    // rare staging overflow raises a flag here
-    (void)wmax;
-    (void)rowmax;
+    row_stats_wave(wmax, wlast, part, blk);
     const bool any_over = __ballot(stage_over) != 0ull;
     const bool any_bad = __ballot(img_bad) != 0ull;
     if (lane == 0 && any_over) atomicOr(flags, DERR_STAGE_OVERFLOW);
@@ -497,64 +525,89 @@ k_bin_pairs(int B, BinGeom g, CellArg C, double cutoff_sq, const SortedAtom *__r
 // atoms in ascending order -- the reference's candidate order
 // (python_neighbors.f90:664-735) -- with the same non-contracted distance
 // test, so rows equal those of the wavefront kernel above.
+//
+// A z column of the stencil is at most three runs of consecutive bins (bins
+// are z-major in memory, so each run is one range of the sorted atoms): the
+// bins wrapped from the top of the cell (shift u_z = +1), the unwrapped
+// ones, the bins wrapped from the bottom (u_z = -1), in that order -- the
+// order of the stencil offsets.  Every thread runs the same three scans
+// (most of them empty), so wavefronts do not split at the z faces.  Cells
+// with fewer than 2 nd + 1 bins along z take the bin-by-bin walk.
+//
+// Candidates are screened in FP32 on the wrapped positions q = r + A.w
+// (r_i - r_j + A.s = q_i - q_j + A.u for the bin shift u).  Per pair, the
+// error of each FP32 difference component is below
+// E = 2^-24 (8 max(|q_i|, |q_j|) + 4 |A.u|_1 + 8 r_c) (the last term
+// covers r_c rounded to FP32 as well): a squared distance
+// below (r_c - 2E)^2 (1 - 1e-6) is inside the cut-off in exact arithmetic
+// and in the FP64 test, one at or above (r_c + 2E)^2 (1 + 1e-6) outside
+// (E / 4 of slack either side after the rounding of the FP32 sum of
+// squares); the band between takes the FP64 test on the caller's
+// positions.  A band wider than 1e-3 r_c (or a NaN / infinite coordinate)
+// sends the pair to the FP64 test as well.
 template <bool ORTHO>
 __global__ void __launch_bounds__(256)
 k_atom_pairs(int nat, BinGeom g, CellArg C, double cutoff_sq, const SortedAtom *__restrict__ atoms,
-             const int *__restrict__ cell_start, int *__restrict__ counts, int2 *__restrict__ stage, int S,
-             int *__restrict__ flags)
+             const float4 *__restrict__ pos32, const int *__restrict__ cell_start, int *__restrict__ counts,
+             int2 *__restrict__ stage, int S, int *__restrict__ flags, int2 *__restrict__ part)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     bool over = false, bad = false;
+    int longest = 0, last = 0;
     if (p < nat) {
-        const SortedAtom A = atoms[p];
-        const double ri[3] = {A.x, A.y, A.z};
-        int hc[3], hw[3];
-        bin_of(g, ri, hc, hw);
-        int w0, w1, w2;
-        dc_unpack(A.w, w0, w1, w2);
-        // home and candidate without wrap (the common case): s = u, the
-        // bin's own shift, whose A.u and packed form are made per bin
-        const bool hz0 = ORTHO && A.w == DC_ZERO;
+        const float4 H = pos32[p];
+        int w0, w1, w2, hc[3];
+        {
+            const SortedAtom A = atoms[p];
+            const double ri[3] = {A.x, A.y, A.z};
+            int hw[3];
+            bin_of(g, ri, hc, hw);
+            dc_unpack(A.w, w0, w1, w2);
+        }
+        const float rc = (float)sqrt(cutoff_sq);
+        const float hm = fmaxf(fmaxf(fabsf(H.x), fabsf(H.y)), fabsf(H.z));
         const size_t srow = (size_t)p * S;
         int cnt = 0;
         // the candidates q0 .. q1 - 1 (atoms of consecutive bins of one
         // shift u), in order
         auto scan = [&](int q0, int q1, int ux, int uy, int uz) {
-            double au0 = 0.0, au1 = 0.0, au2 = 0.0;
-            if (ORTHO) {
-                au0 = C.A[0] * (double)ux;
-                au1 = C.A[4] * (double)uy;
-                au2 = C.A[8] * (double)uz;
-            }
-            const bool u_ok = dc_pack_ok(ux) && dc_pack_ok(uy) && dc_pack_ok(uz);
-            const int upk = u_ok ? dc_pack(ux, uy, uz) : -1;
+            double au[3];
+            for (int x = 0; x < 3; ++x) au[x] = image_x(C, x, ux, uy, uz);
+            const float hx = H.x + (float)au[0], hy = H.y + (float)au[1], hz = H.z + (float)au[2];
+            const float e0 = 5.9604645e-8f * (4.0f * (float)(fabs(au[0]) + fabs(au[1]) + fabs(au[2])) + 8.0f * rc);
+            const int s0u = w0 + ux, s1u = w1 + uy, s2u = w2 + uz;
             for (int q = q0; q < q1; ++q) {
-                const SortedAtom Bq = atoms[q];
-                if (hz0 && Bq.w == DC_ZERO) {
-                    if (Bq.j == A.j && upk == DC_ZERO) continue;
-                    if (pair_d2_pre(A.x, A.y, A.z, Bq.x, Bq.y, Bq.z, au0, au1, au2) < cutoff_sq) {
-                        if (!u_ok) bad = true;
-                        if (cnt < S) stage[srow + cnt] = make_int2(Bq.j, upk);
-                        ++cnt;
-                    }
-                    continue;
+                const float4 B = pos32[q];
+                const float dx = hx - B.x, dy = hy - B.y, dz = hz - B.z;
+                const float d2f = dx * dx + dy * dy + dz * dz;
+                const float bm = fmaxf(fmaxf(fabsf(B.x), fabsf(B.y)), fabsf(B.z));
+                const float E2 = 2.0f * (e0 + 4.76837158e-7f * fmaxf(hm, bm));
+                const float th = rc + E2, tl = rc - E2;
+                const bool exact = !(E2 < 2e-3f * rc);
+                if (!exact && !(d2f < th * th * 1.000001f)) continue;
+                const int jb = __float_as_int(B.w);
+                int s0 = s0u, s1 = s1u, s2 = s2u;
+                if (jb < 0) {
+                    int v0, v1, v2;
+                    dc_unpack(atoms[q].w, v0, v1, v2);
+                    s0 -= v0;
+                    s1 -= v1;
+                    s2 -= v2;
                 }
-                int v0, v1, v2;
-                dc_unpack(Bq.w, v0, v1, v2);
-                const int s0 = w0 + ux - v0, s1 = w1 + uy - v1, s2 = w2 + uz - v2;
-                if (Bq.j == A.j && s0 == 0 && s1 == 0 && s2 == 0) continue;
-                if (pair_d2<ORTHO>(C, A.x, A.y, A.z, Bq.x, Bq.y, Bq.z, s0, s1, s2) < cutoff_sq) {
-                    if (!dc_pack_ok(s0) || !dc_pack_ok(s1) || !dc_pack_ok(s2)) bad = true;
-                    if (cnt < S) stage[srow + cnt] = make_int2(Bq.j, dc_pack(s0, s1, s2));
-                    ++cnt;
+                if (q == p && s0 == 0 && s1 == 0 && s2 == 0) continue;
+                if (exact || !(d2f < tl * tl * 0.999999f)) {
+                    const SortedAtom Ap = atoms[p], Bq = atoms[q];
+                    if (!(pair_d2<ORTHO>(C, Ap.x, Ap.y, Ap.z, Bq.x, Bq.y, Bq.z, s0, s1, s2) < cutoff_sq))
+                        continue;
                 }
+                if (!dc_pack_ok(s0) || !dc_pack_ok(s1) || !dc_pack_ok(s2)) bad = true;
+                if (cnt < S) stage[srow + cnt] = make_int2(jb & 0x7fffffff, dc_pack(s0, s1, s2));
+                ++cnt;
             }
         };
-        // a z column of the stencil inside the box is one range of the
-        // sorted atoms (bins z-major in memory, no wrap): same candidates,
-        // same order as bin by bin
-        const int z0 = hc[2] - g.nd[2], z1 = hc[2] + g.nd[2];
-        const bool zcol = z0 >= 1 && z1 <= g.nc[2];
+        const int nz = g.nc[2], ndz = g.nd[2];
+        const bool runs = !g.pbc[2] || 2 * ndz + 1 <= nz;
+        const int zlo = hc[2] - ndz, zhi = hc[2] + ndz;
         for (int ox = -g.nd[0]; ox <= g.nd[0]; ++ox) {
             int cx = hc[0] + ox, ux = 0;
             if (g.pbc[0]) {
@@ -573,64 +626,57 @@ k_atom_pairs(int nat, BinGeom g, CellArg C, double cutoff_sq, const SortedAtom *
                 } else if (cy < 1 || cy > g.nc[1]) {
                     continue;
                 }
-                const int bcol = ((cx - 1) * g.nc[1] + (cy - 1)) * g.nc[2];
-                if (zcol) {
-                    scan(cell_start[bcol + z0 - 1], cell_start[bcol + z1], ux, uy, 0);
+                const int bcol = ((cx - 1) * g.nc[1] + (cy - 1)) * nz;
+                if (runs) {
+                    // bins (1-based) of the three runs; an empty run has b1 < b0
+                    const int m0 = max(zlo, 1), m1 = min(zhi, nz);
+                    int t0 = 1, t1 = 0, u0 = 1, u1 = 0;
+                    if (g.pbc[2]) {
+                        t0 = zlo + nz;            // wrapped from the top: zlo .. 0
+                        t1 = nz;
+                        u0 = 1;                   // wrapped from the bottom: nz+1 .. zhi
+                        u1 = zhi - nz;
+                    }
+                    if (t0 <= t1) scan(cell_start[bcol + t0 - 1], cell_start[bcol + t1], ux, uy, 1);
+                    scan(cell_start[bcol + m0 - 1], cell_start[bcol + m1], ux, uy, 0);
+                    if (u0 <= u1) scan(cell_start[bcol + u0 - 1], cell_start[bcol + u1], ux, uy, -1);
                     continue;
                 }
-                for (int oz = -g.nd[2]; oz <= g.nd[2]; ++oz) {
+                for (int oz = -ndz; oz <= ndz; ++oz) {
                     int cz = hc[2] + oz, uz = 0;
-                    if (g.pbc[2]) {
-                        const int t = floordiv(cz - 1, g.nc[2]);
-                        cz -= t * g.nc[2];
-                        uz = -t;
-                    } else if (cz < 1 || cz > g.nc[2]) {
-                        continue;
-                    }
+                    const int t = floordiv(cz - 1, nz);
+                    cz -= t * nz;
+                    uz = -t;
                     const int b = bcol + (cz - 1);
                     scan(cell_start[b], cell_start[b + 1], ux, uy, uz);
                 }
             }
         }
-        counts[A.j] = cnt;
+        const int ih = __float_as_int(H.w) & 0x7fffffff;
+        counts[ih] = cnt;
         over = cnt > S;
+        longest = cnt;
+        last = cnt > 0 ? ih + 1 : 0;
+    }
+    // one row-statistics record per block
+    {
+        __shared__ int2 sm[4];
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) {
+            longest = max(longest, __shfl_xor(longest, o, 64));
+            last = max(last, __shfl_xor(last, o, 64));
+        }
+        if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = make_int2(longest, last);
+        __syncthreads();
+        if (threadIdx.x == 0)
+            part[blockIdx.x] = make_int2(max(max(sm[0].x, sm[1].x), max(sm[2].x, sm[3].x)),
+                                         max(max(sm[0].y, sm[1].y), max(sm[2].y, sm[3].y)));
     }
     const unsigned long long mo = __ballot(over), mb = __ballot(bad);
     if ((threadIdx.x & 63) == 0) {
         if (mo) atomicOr(flags, DERR_STAGE_OVERFLOW);
         if (mb) atomicOr(flags, DERR_IMAGE_RANGE);
     }
-}
-
-// One pass over the row lengths of a build: the longest row (flags[0],
-// atomicMax once per block of 1024 rows) and the reference's capacity rule
-// (python_neighbors.f90:713-718): row i ends at 1-based slot offsets[i] +
-// cnt + i (one separator per earlier atom), and a non-empty row that ends at
-// or beyond nat*avgn overflows the list (flags[3]).
-__global__ void __launch_bounds__(256)
-k_nl_rowstats(int nat, const int *__restrict__ counts, const int *__restrict__ offsets, long long cap,
-              int *__restrict__ flags)
-{
-    __shared__ int sm[4];
-    int m = 0;
-    bool over = false;
-    const int base = blockIdx.x * 1024;
-    for (int k = threadIdx.x; k < 1024; k += 256) {
-        const int i = base + k;
-        if (i < nat) {
-            const int cnt = counts[i];
-            m = max(m, cnt);
-            over = over || (cnt > 0 && (long long)offsets[i] + cnt + i >= cap);
-        }
-    }
-    for (int o = 32; o >= 1; o >>= 1) m = max(m, __shfl_xor(m, o, 64));
-    const bool any_over = __ballot(over) != 0ull;
-    if ((threadIdx.x & 63) == 0) {
-        sm[threadIdx.x >> 6] = m;
-        if (any_over) atomicOr(flags + 3, DERR_NL_OVERFLOW);
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) atomicMax(flags, max(max(sm[0], sm[1]), max(sm[2], sm[3])));
 }
 
 // Copy staged rows (sorted slot p, stride S) into the CSR rows of the caller
@@ -671,9 +717,39 @@ NlRows nl_rows(const agpu_neighbors *nl)
     return NlRows{nl->stage.as<int2>(), nl->counts.as<int>(), nl->stage_w};
 }
 
-__global__ void k_offsets_tail(int nat, const int *__restrict__ counts, int *__restrict__ offsets,
-                               int *__restrict__ stats)
+// The row statistics of the pair search (npart records {longest row,
+// 1 + last non-empty caller row}) reduced into stats[0] and the last row;
+// the total (stats[1..2]); and the reference's capacity rule
+// (python_neighbors.f90:713-718): row i ends at 1-based slot offsets[i] +
+// cnt + i (one separator per earlier atom), increasing with i, so the list
+// overflows exactly when the last non-empty row ends at or beyond nat*avgn.
+// One block of 1024 threads.
+__global__ void __launch_bounds__(1024)
+k_offsets_tail(int nat, const int *__restrict__ counts, int *__restrict__ offsets, int *__restrict__ stats,
+               long long cap, const int2 *__restrict__ part, int npart)
 {
+    __shared__ int2 sm[16];
+    int longest = 0, last = 0;
+    for (int k = threadIdx.x; k < npart; k += 1024) {
+        const int2 v = part[k];
+        longest = max(longest, v.x);
+        last = max(last, v.y);
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        longest = max(longest, __shfl_xor(longest, o, 64));
+        last = max(last, __shfl_xor(last, o, 64));
+    }
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = make_int2(longest, last);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 16; ++w) {
+            longest = max(longest, sm[w].x);
+            last = max(last, sm[w].y);
+        }
+        stats[0] = longest;
+        stats[5] = last;
+    }
     // offsets[nat] = total.  The 32-bit scan cannot wrap while
     // nat * rowmax < 2^31; larger lists are refused as an overflow (the
     // capacity nat*avgn is below 2^31 anyway).
@@ -689,6 +765,8 @@ __global__ void k_offsets_tail(int nat, const int *__restrict__ counts, int *__r
         offsets[nat] = (int)t;
         stats[1] = (int)(t & 0x7fffffff);
         stats[2] = (int)(t >> 31);
+        const int il = stats[5] - 1;
+        if (il >= 0 && (long long)offsets[il] + counts[il] + il >= cap) stats[3] |= DERR_NL_OVERFLOW;
     }
 }
 
@@ -785,10 +863,12 @@ int nl_update(agpu_neighbors *nl, agpu_particles *p, int force, bool csr)
     AGPU_CHECK(nl->atoms_sorted.reserve(sizeof(SortedAtom) * (nat + 1)));
     AGPU_CHECK(nl->counts.reserve(sizeof(int) * (nat + 1)));
     AGPU_CHECK(nl->offsets.reserve(sizeof(int) * (nat + 1)));
-    AGPU_CHECK(nl->rowmax.reserve(sizeof(int) * 4));
+    AGPU_CHECK(nl->rowmax.reserve(sizeof(int) * 8));
     AGPU_CHECK(nl->nbr.reserve(sizeof(int2) * (nl->cap > 0 ? nl->cap : 1)));
-    int *flags = nl->rowmax.as<int>();   // [0] rowmax, [1..2] total, [3] flags
-    AGPU_HIP(hipMemsetAsync(flags, 0, sizeof(int) * 4, s));
+    // [0] rowmax, [1..2] total, [3] flags, [5] 1 + the last caller index
+    // with a non-empty row
+    int *flags = nl->rowmax.as<int>();
+    AGPU_HIP(hipMemsetAsync(flags, 0, sizeof(int) * 8, s));
     CellArg C;
     std::memcpy(C.A, p->cell.A, sizeof C.A);
     const int TB = 256;
@@ -846,12 +926,15 @@ int nl_update(agpu_neighbors *nl, agpu_particles *p, int force, bool csr)
             AGPU_LAUNCH("nl_cell_start", k_cell_start, dim3((nat + 1 + TB - 1) / TB), dim3(TB), 0, s,
                         nat, (int)ncell, nl->key_sorted.as<int>(), nl->cell_start.as<int>());
         }
-        AGPU_LAUNCH("nl_gather", k_gather, dim3(nb), dim3(TB), 0, s, nat, p->d_r, g,
-                    nl->idx_sorted.as<int>(), nl->atoms_sorted.as<SortedAtom>(), nl->rank.as<int>());
+        const double per_bin = (double)nat / (double)ncell;
+        const bool per_atom = nl->pair_mode == 1 || (nl->pair_mode == 0 && per_bin < 4.0);
+        if (per_atom) AGPU_CHECK(nl->pos32.reserve(sizeof(float4) * (nat + 1)));
+        AGPU_LAUNCH("nl_gather", k_gather, dim3(nb), dim3(TB), 0, s, nat, p->d_r, g, C,
+                    nl->idx_sorted.as<int>(), nl->atoms_sorted.as<SortedAtom>(), nl->rank.as<int>(),
+                    per_atom ? nl->pos32.as<float4>() : nullptr);
         double cutoff_sq = nl->cutoff * nl->cutoff;
         // bins per wavefront edge: one bin when bins are dense, a 2^3 or 3^3
         // block of bins when they hold few atoms (stencil of +-1 bin only)
-        const double per_bin = (double)nat / (double)ncell;
         int B = 1;
         if (g.nd[0] == 1 && g.nd[1] == 1 && g.nd[2] == 1 && per_bin < 4.0) B = 2;
         const bool ortho = C.A[1] == 0.0 && C.A[2] == 0.0 && C.A[3] == 0.0 && C.A[5] == 0.0 &&
@@ -868,23 +951,27 @@ int nl_update(agpu_neighbors *nl, agpu_particles *p, int force, bool csr)
         }
         const int S = nl->stage_w;
         AGPU_CHECK(nl->stage.reserve(sizeof(int2) * (size_t)nat * S));
-        const bool per_atom = nl->pair_mode == 1 || (nl->pair_mode == 0 && per_bin < 4.0);
+        // row-statistics records: one per block (thread per atom) or per
+        // wavefront (wavefront per bin block)
+        const int na = (nat + TB - 1) / TB;
+        const int npart = per_atom ? na : nbin_blk * NL_WAVES;
+        AGPU_CHECK(nl->part.reserve(sizeof(int2) * (size_t)npart));
+        int2 *part = nl->part.as<int2>();
         if (per_atom) {
-            const int na = (nat + TB - 1) / TB;
             if (ortho)
                 AGPU_LAUNCH("nl_pairs", k_atom_pairs<true>, dim3(na), dim3(TB), 0, s, nat, g, C, cutoff_sq,
-                            nl->atoms_sorted.as<SortedAtom>(), nl->cell_start.as<int>(), nl->counts.as<int>(),
-                            nl->stage.as<int2>(), S, flags + 3);
+                            nl->atoms_sorted.as<SortedAtom>(), nl->pos32.as<float4>(), nl->cell_start.as<int>(),
+                            nl->counts.as<int>(), nl->stage.as<int2>(), S, flags + 3, part);
             else
                 AGPU_LAUNCH("nl_pairs", k_atom_pairs<false>, dim3(na), dim3(TB), 0, s, nat, g, C, cutoff_sq,
-                            nl->atoms_sorted.as<SortedAtom>(), nl->cell_start.as<int>(), nl->counts.as<int>(),
-                            nl->stage.as<int2>(), S, flags + 3);
+                            nl->atoms_sorted.as<SortedAtom>(), nl->pos32.as<float4>(), nl->cell_start.as<int>(),
+                            nl->counts.as<int>(), nl->stage.as<int2>(), S, flags + 3, part);
         } else {
             auto kern = ortho ? (B > 1 ? k_bin_pairs<true, true> : k_bin_pairs<true, false>)
                               : (B > 1 ? k_bin_pairs<false, true> : k_bin_pairs<false, false>);
             AGPU_LAUNCH("nl_pairs", kern, dim3(nbin_blk), dim3(TB), 0, s, B, g, C, cutoff_sq,
                         nl->atoms_sorted.as<SortedAtom>(), nl->cell_start.as<int>(), nl->counts.as<int>(),
-                        nl->stage.as<int2>(), S, flags + 3, flags);
+                        nl->stage.as<int2>(), S, flags + 3, part);
         }
         size_t tmp2 = 0;
         AGPU_HIP(rocprim::exclusive_scan(nullptr, tmp2, nl->counts.as<int>(), nl->offsets.as<int>(), 0,
@@ -892,10 +979,8 @@ int nl_update(agpu_neighbors *nl, agpu_particles *p, int force, bool csr)
         AGPU_CHECK(nl->sort_tmp.reserve(tmp2 + 16));
         AGPU_HIP(rocprim::exclusive_scan(nl->sort_tmp.ptr, tmp2, nl->counts.as<int>(),
                                          nl->offsets.as<int>(), 0, (size_t)nat, rocprim::plus<int>(), s));
-        AGPU_LAUNCH("nl_rowstats", k_nl_rowstats, dim3((nat + 1023) / 1024), dim3(TB), 0, s, nat,
-                    nl->counts.as<int>(), nl->offsets.as<int>(), (long long)nl->cap, flags);
-        AGPU_LAUNCH("nl_offsets_tail", k_offsets_tail, dim3(1), dim3(64), 0, s, nat,
-                    nl->counts.as<int>(), nl->offsets.as<int>(), flags);
+        AGPU_LAUNCH("nl_offsets_tail", k_offsets_tail, dim3(1), dim3(1024), 0, s, nat,
+                    nl->counts.as<int>(), nl->offsets.as<int>(), flags, (long long)nl->cap, part, npart);
         if (csr)
             AGPU_LAUNCH("nl_compact", k_nl_compact, dim3((nat + 15) / 16), dim3(TB), 0, s, nat,
                         nl->atoms_sorted.as<SortedAtom>(), nl->counts.as<int>(), nl->offsets.as<int>(),

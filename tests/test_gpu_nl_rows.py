@@ -128,3 +128,48 @@ def test_pair_modes_with_wrapped_atoms(shape):
         p.changed_positions()
         off, j, dc = nl.get_list(p)
         assert np.array_equal(off, ref[0]) and np.array_equal(j, ref[1]) and np.array_equal(dc, ref[2])
+
+
+def _sc_lattice(n, a, jitter, seed):
+    g = np.arange(n, dtype=np.float64) * a
+    pos = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    pos = pos + np.random.default_rng(seed).normal(0.0, jitter, pos.shape)
+    return pos, np.eye(3) * (n * a)
+
+
+@pytest.mark.parametrize("case", ["at_cutoff", "at_cutoff_triclinic", "far_cluster", "very_far_cluster"])
+def test_sparse_search_screening_band(case):
+    """The thread-per-atom search screens candidates in FP32 and takes the
+    FP64 test only near the cut-off: lattice distances equal to the cut-off
+    up to 1e-13 A (every such pair decided by the FP64 test), a triclinic
+    cell, and clusters far from the origin in an open box (a wide band, and
+    one so wide that every candidate takes the FP64 test).  Rows equal the
+    oracle's entry for entry in both pair modes."""
+    a = 3.3                      # above the Tersoff range, so the list cut-off is rc
+    if case.startswith("at_cutoff"):
+        pos, cell = _sc_lattice(12, a, 1e-13, 3)
+        pbc = np.array([True, True, True])
+        if case == "at_cutoff_triclinic":
+            shear = np.array([[1.0, 0.0, 0.0], [0.25, 1.0, 0.0], [0.0, -0.5, 1.0]])
+            pos, cell = pos @ shear, cell @ shear
+        rc = a
+    else:
+        off = 1.5e3 if case == "far_cluster" else 2.0e5
+        pos, _ = _sc_lattice(8, a, 0.05, 4)
+        pos = pos + off
+        cell = np.eye(3) * (8 * a)
+        pbc = np.array([False, False, False])
+        rc = 1.05 * a
+    Z = np.full(len(pos), 14, np.int32)
+    ref = O.nl_build(pos, cell, pbc, rc)
+    p = N.from_atoms(pos, cell, pbc, Z)
+    nl = N.Neighbors(100)
+    nl.request_interaction_range(rc)
+    pot = N.Tersoff()
+    pot.bind_to(p, nl)
+    for mode in (1, 2):
+        print("screening band", case, "pair mode", mode, flush=True)
+        pot.set_launch(nl, pair_mode=mode)
+        p.changed_positions()
+        off_, j, dc = nl.get_list(p)
+        assert np.array_equal(off_, ref[0]) and np.array_equal(j, ref[1]) and np.array_equal(dc, ref[2])
