@@ -51,19 +51,45 @@ namespace agpu {
 // pair codes of the reference (Z2pair): C = 1, H = 3; CC = 1, CH = 3, HH = 6
 enum { RB_C = 1, RB_H = 3, RB_CC = 1, RB_CH = 3, RB_HH = 6 };
 
-struct RbDev {
-    double ccB[3], ccBeta[3];        // VA of C-C: three exponentials
-    double B1[7], beta1[7];          // VA of C-H and H-H
-    double A[7], Q[7], alpha[7];     // VR
-    double cut_l[7], cut_h[7], cut_h2[7], cut_fac[7];
-    double conear[7][7];
-    double conalp, conpe, conan, conpf;
-    double th1, th2, n_hi;           // g interval boundaries; upper N switch
-    double g1c[3][6], g2c[3][6];     // carbon g(cos): high / low coordination
-    double spgh[3][6];               // hydrogen g(cos) polynomials
+template <typename R> struct RbDevT {
+    R ccB[3], ccBeta[3];        // VA of C-C: three exponentials
+    R B1[7], beta1[7];          // VA of C-H and H-H
+    R A[7], Q[7], alpha[7];     // VR
+    R cut_l[7], cut_h[7], cut_h2[7], cut_fac[7];
+    R conear[7][7];
+    R conalp, conpe, conan, conpf;
+    R th1, th2, n_hi;           // g interval boundaries; upper N switch
+    R g1c[3][6], g2c[3][6];     // carbon g(cos): high / low coordination
+    R spgh[3][6];               // hydrogen g(cos) polynomials
     int igh[25];
     int with_dihedral;
 };
+using RbDev = RbDevT<double>;
+
+// the same constants rounded to FP32 (the mixed form)
+static RbDevT<float> rb_dev_float(const RbDev &D)
+{
+    RbDevT<float> F;
+    for (int k = 0; k < 3; ++k) { F.ccB[k] = (float)D.ccB[k]; F.ccBeta[k] = (float)D.ccBeta[k]; }
+    for (int k = 0; k < 7; ++k) {
+        F.B1[k] = (float)D.B1[k]; F.beta1[k] = (float)D.beta1[k];
+        F.A[k] = (float)D.A[k]; F.Q[k] = (float)D.Q[k]; F.alpha[k] = (float)D.alpha[k];
+        F.cut_l[k] = (float)D.cut_l[k]; F.cut_h[k] = (float)D.cut_h[k];
+        F.cut_h2[k] = (float)D.cut_h2[k]; F.cut_fac[k] = (float)D.cut_fac[k];
+        for (int q = 0; q < 7; ++q) F.conear[k][q] = (float)D.conear[k][q];
+    }
+    F.conalp = (float)D.conalp; F.conpe = (float)D.conpe; F.conan = (float)D.conan; F.conpf = (float)D.conpf;
+    F.th1 = (float)D.th1; F.th2 = (float)D.th2; F.n_hi = (float)D.n_hi;
+    for (int a = 0; a < 3; ++a)
+        for (int q = 0; q < 6; ++q) {
+            F.g1c[a][q] = (float)D.g1c[a][q];
+            F.g2c[a][q] = (float)D.g2c[a][q];
+            F.spgh[a][q] = (float)D.spgh[a][q];
+        }
+    for (int q = 0; q < 25; ++q) F.igh[q] = D.igh[q];
+    F.with_dihedral = D.with_dihedral;
+    return F;
+}
 
 // table layout in one device block: 3 tricubic tables of 4*4*9 boxes x 64
 // coefficients, then 2 bicubic tables of 5*5 boxes x 16 coefficients
@@ -79,10 +105,18 @@ constexpr double RB_FX = 1099511627776.0;
 constexpr double RB_FX_INV = 1.0 / 1099511627776.0;
 constexpr double RB_FX_MAX = 2097152.0;   // 2^21: contributions beyond flag an error
 
-struct RbBond {
-    double n[3];
-    double len;
-    double fc, dfc, fx, dfx;
+// bond record: unit vector r_ij / r_ij, length, cut-off and derivative, the
+// conjugation F(x_ik) and dF/dx (64 bytes in FP64, 32 in FP32)
+template <typename R> struct RbBondT {
+    R n[3];
+    R len;
+    R fc, dfc, fx, dfx;
+};
+using RbBond = RbBondT<double>;
+
+// per directed bond: b, dV_A b', dP/dN_C + dzeta/dN, dP/dN_H + dzeta/dN
+template <typename R> struct alignas(4 * sizeof(R)) Rb4 {
+    R x, y, z, w;
 };
 
 struct Rebo2State {
@@ -91,7 +125,7 @@ struct Rebo2State {
     RbDev hdev{};
     std::vector<double> htab;
     DevBuf dev, tab, bond, bij, bown, bcnt, nn, nconj, facc, eacc, wacc, partial, stats, bo, rev, half, hpos,
-        list, scan_tmp, bsrc, cnt_s, offs_s, el_s, tag_s;
+        list, scan_tmp, bsrc, cnt_s, offs_s, el_s, tag_s, dev32, tab32, bond32;
     int z2db[128];
     uint64_t Z_rev = ~0ull;
     double cutoff = 0.0;
@@ -376,6 +410,13 @@ static int rebo2_setup(Rebo2State *st)
     AGPU_CHECK(st->tab.reserve(sizeof(double) * RB_TAB_N));
     AGPU_HIP(hipMemcpy(st->dev.ptr, &D, sizeof(RbDev), hipMemcpyHostToDevice));
     AGPU_HIP(hipMemcpy(st->tab.ptr, st->htab.data(), sizeof(double) * RB_TAB_N, hipMemcpyHostToDevice));
+    // FP32 copies for the mixed form
+    const RbDevT<float> D32 = rb_dev_float(D);
+    const std::vector<float> tab32(st->htab.begin(), st->htab.end());
+    AGPU_CHECK(st->dev32.reserve(sizeof(RbDevT<float>)));
+    AGPU_CHECK(st->tab32.reserve(sizeof(float) * RB_TAB_N));
+    AGPU_HIP(hipMemcpy(st->dev32.ptr, &D32, sizeof(RbDevT<float>), hipMemcpyHostToDevice));
+    AGPU_HIP(hipMemcpy(st->tab32.ptr, tab32.data(), sizeof(float) * RB_TAB_N, hipMemcpyHostToDevice));
     return AGPU_OK;
 }
 
@@ -387,120 +428,132 @@ __device__ __forceinline__ int rb_pair(int ki, int kj)
     return ki == RB_C ? kj : (kj == RB_C ? ki : ki + kj);
 }
 
-__device__ __forceinline__ void rb_fconj(double x, double &fx, double &dfx)
+__device__ __forceinline__ void rb_sincos(double x, double &s, double &c) { sincos(x, &s, &c); }
+__device__ __forceinline__ void rb_sincos(float x, float &s, float &c) { sincosf(x, &s, &c); }
+
+template <typename R>
+__device__ __forceinline__ void rb_fconj(R x, R &fx, R &dfx)
 {
-    if (x <= 2.0) { fx = 1.0; dfx = 0.0; }
-    else if (x >= 3.0) { fx = 0.0; dfx = 0.0; }
+    if (x <= R(2.0)) { fx = R(1.0); dfx = R(0.0); }
+    else if (x >= R(3.0)) { fx = R(0.0); dfx = R(0.0); }
     else {
-        double s, c;
-        sincos(M_PI * (x - 2.0), &s, &c);
-        fx = 0.5 * (1.0 + c);
-        dfx = -0.5 * M_PI * s;
+        R s, c;
+        rb_sincos(R(M_PI) * (x - R(2.0)), s, c);
+        fx = R(0.5) * (R(1.0) + c);
+        dfx = -R(0.5) * R(M_PI) * s;
     }
 }
 
-__device__ __forceinline__ void rb_fcin(const RbDev &D, int t, double r, double &v, double &dv)
+template <typename R>
+__device__ __forceinline__ void rb_fcin(const RbDevT<R> &D, int t, R r, R &v, R &dv)
 {
-    const double l = D.cut_l[t], h = D.cut_h[t];
-    if (r >= h) { v = 0.0; dv = 0.0; }
-    else if (r <= l) { v = 1.0; dv = 0.0; }
+    const R l = D.cut_l[t], h = D.cut_h[t];
+    if (r >= h) { v = R(0.0); dv = R(0.0); }
+    else if (r <= l) { v = R(1.0); dv = R(0.0); }
     else {
-        const double fac = D.cut_fac[t];
-        double s, c;
-        sincos(fac * (r - l), &s, &c);
-        v = 0.5 * (1.0 + c);
-        dv = -0.5 * fac * s;
+        const R fac = D.cut_fac[t];
+        R s, c;
+        rb_sincos(fac * (r - l), s, c);
+        v = R(0.5) * (R(1.0) + c);
+        dv = -R(0.5) * fac * s;
     }
 }
 
-__device__ __forceinline__ void rb_va(const RbDev &D, int t, double r, double &v, double &dv)
+template <typename R>
+__device__ __forceinline__ void rb_va(const RbDevT<R> &D, int t, R r, R &v, R &dv)
 {
     if (t == RB_CC) {
-        const double e1 = D.ccB[0] * exp(-D.ccBeta[0] * r);
-        const double e2 = D.ccB[1] * exp(-D.ccBeta[1] * r);
-        const double e3 = D.ccB[2] * exp(-D.ccBeta[2] * r);
+        const R e1 = D.ccB[0] * exp(-D.ccBeta[0] * r);
+        const R e2 = D.ccB[1] * exp(-D.ccBeta[1] * r);
+        const R e3 = D.ccB[2] * exp(-D.ccBeta[2] * r);
         v = -(e1 + e2 + e3);
         dv = -(-D.ccBeta[0] * e1 - D.ccBeta[1] * e2 - D.ccBeta[2] * e3);
     } else {
-        const double e1 = D.B1[t] * exp(-D.beta1[t] * r);
+        const R e1 = D.B1[t] * exp(-D.beta1[t] * r);
         v = -e1;
         dv = D.beta1[t] * e1;
     }
 }
 
-__device__ __forceinline__ void rb_vr(const RbDev &D, int t, double r, double &v, double &dv)
+template <typename R>
+__device__ __forceinline__ void rb_vr(const RbDevT<R> &D, int t, R r, R &v, R &dv)
 {
-    const double e1 = D.A[t] * exp(-D.alpha[t] * r);
-    const double hl = 1 + D.Q[t] / r;
+    const R e1 = D.A[t] * exp(-D.alpha[t] * r);
+    const R hl = 1 + D.Q[t] / r;
     v = hl * e1;
     dv = (-D.Q[t] / (r * r) - hl * D.alpha[t]) * e1;
 }
 
-__device__ __forceinline__ void rb_poly5(const double *c, double x, double &v, double &dv)
+template <typename R>
+__device__ __forceinline__ void rb_poly5(const R *c, R x, R &v, R &dv)
 {
-    const double x2 = x * x, x3 = x2 * x, x4 = x3 * x, x5 = x4 * x;
+    const R x2 = x * x, x3 = x2 * x, x4 = x3 * x, x5 = x4 * x;
     v = c[0] + c[1] * x + c[2] * x2 + c[3] * x3 + c[4] * x4 + c[5] * x5;
     dv = c[1] + 2 * c[2] * x + 3 * c[3] * x2 + 4 * c[4] * x3 + 5 * c[5] * x4;
 }
 
-__device__ __forceinline__ void rb_g(const RbDev &D, int ktyp, double costh, double n, double &val,
-                                     double &dcos, double &dn)
+template <typename R>
+__device__ __forceinline__ void rb_g(const RbDevT<R> &D, int ktyp, R costh, R n, R &val,
+                                     R &dcos, R &dn)
 {
     if (ktyp == RB_C) {
         const int j = costh < D.th1 ? 0 : (costh < D.th2 ? 1 : 2);
-        if (n < 3.2) {
+        if (n < R(3.2)) {
             rb_poly5(D.g2c[j], costh, val, dcos);
-            dn = 0.0;
+            dn = R(0.0);
         } else if (n > D.n_hi) {
             rb_poly5(D.g1c[j], costh, val, dcos);
-            dn = 0.0;
+            dn = R(0.0);
         } else {
-            double v1, d1, v2, d2, s, c;
+            R v1, d1, v2, d2, s, c;
             rb_poly5(D.g1c[j], costh, v1, d1);
             rb_poly5(D.g2c[j], costh, v2, d2);
-            sincos(2 * M_PI * (n - 3.2), &s, &c);
-            const double sw = (1 + c) / 2;
+            rb_sincos(2 * R(M_PI) * (n - R(3.2)), s, c);
+            const R sw = (1 + c) / 2;
             val = v1 * (1 - sw) + v2 * sw;
             dcos = d1 * (1 - sw) + d2 * sw;
-            dn = (v2 - v1) * (-M_PI * s);
+            dn = (v2 - v1) * (-R(M_PI) * s);
         }
     } else {
-        int q = (int)(-costh * 12.0) + 12;
+        int q = (int)(-costh * R(12.0)) + 12;
         q = q < 0 ? 0 : (q > 24 ? 24 : q);
         rb_poly5(D.spgh[D.igh[q] - 1], costh, val, dcos);
-        dn = 0.0;
+        dn = R(0.0);
     }
 }
 
-__device__ __forceinline__ void rb_h(const RbDev &D, int ijpot, int ikpot, double dr, double &v,
-                                     double &dv)
+template <typename R>
+__device__ __forceinline__ void rb_h(const RbDevT<R> &D, int ijpot, int ikpot, R dr, R &v,
+                                     R &dv)
 {
-    if (ijpot + ikpot <= 4) { v = 1.0; dv = 0.0; }
+    if (ijpot + ikpot <= 4) { v = R(1.0); dv = R(0.0); }
     else {
         v = D.conear[ijpot][ikpot] * exp(D.conalp * dr);
         dv = D.conalp * v;
     }
 }
 
-__device__ __forceinline__ int rb_box(double v, int n)
+template <typename R>
+__device__ __forceinline__ int rb_box(R v, int n)
 {
     int b = (int)v;
     return b < 0 ? 0 : (b >= n ? n - 1 : b);
 }
 
-__device__ void rb_table3(const double *__restrict__ T, double x, double y, double z, double &f,
-                          double &fx, double &fy, double &fz)
+template <typename R>
+__device__ void rb_table3(const R *__restrict__ T, R x, R y, R z, R &f,
+                          R &fx, R &fy, R &fz)
 {
     const int bx = rb_box(x, 4), by = rb_box(y, 4), bz = rb_box(z, 9);
-    const double *c = T + (size_t)((bx * 4 + by) * 9 + bz) * 64;
-    const double x1 = x - bx, x2 = y - by, x3 = z - bz;
-    double F = 0.0, Fi = 0.0, Fj = 0.0, Fk = 0.0;
+    const R *c = T + (size_t)((bx * 4 + by) * 9 + bz) * 64;
+    const R x1 = x - bx, x2 = y - by, x3 = z - bz;
+    R F = R(0.0), Fi = R(0.0), Fj = R(0.0), Fk = R(0.0);
     for (int i = 3; i >= 0; --i) {
-        double s = 0.0, sj = 0.0, sk = 0.0;
+        R s = R(0.0), sj = R(0.0), sk = R(0.0);
         for (int j = 3; j >= 0; --j) {
-            double t = 0.0, tk = 0.0;
+            R t = R(0.0), tk = R(0.0);
             for (int k = 3; k >= 0; --k) {
-                const double cijk = c[16 * i + 4 * j + k];
+                const R cijk = c[16 * i + 4 * j + k];
                 t = t * x3 + cijk;
                 if (k > 0) tk = tk * x3 + k * cijk;
             }
@@ -516,17 +569,18 @@ __device__ void rb_table3(const double *__restrict__ T, double x, double y, doub
     f = F; fx = Fi; fy = Fj; fz = Fk;
 }
 
-__device__ void rb_table2(const double *__restrict__ T, double x, double y, double &f, double &fx,
-                          double &fy)
+template <typename R>
+__device__ void rb_table2(const R *__restrict__ T, R x, R y, R &f, R &fx,
+                          R &fy)
 {
     const int bx = rb_box(x, 5), by = rb_box(y, 5);
-    const double *c = T + (size_t)(bx * 5 + by) * 16;
-    const double x1 = x - bx, x2 = y - by;
-    double h = 0.0, dh = 0.0, dc = 0.0;
+    const R *c = T + (size_t)(bx * 5 + by) * 16;
+    const R x1 = x - bx, x2 = y - by;
+    R h = R(0.0), dh = R(0.0), dc = R(0.0);
     for (int i = 3; i >= 0; --i) {
-        double s = 0.0, sd = 0.0;
+        R s = R(0.0), sd = R(0.0);
         for (int j = 3; j >= 0; --j) {
-            const double cij = c[4 * i + j];
+            const R cij = c[4 * i + j];
             s = s * x2 + cij;
             if (j > 0) sd = sd * x2 + j * cij;
         }
@@ -545,7 +599,8 @@ __device__ __forceinline__ void fx_add(unsigned long long *a, double v, int &bad
     atomicAdd(a, (unsigned long long)q);
 }
 
-__device__ __forceinline__ void fx_add3(unsigned long long *a, int i, const double *v, int &bad)
+template <typename T>
+__device__ __forceinline__ void fx_add3(unsigned long long *a, int i, const T *v, int &bad)
 {
     fx_add(a + 3 * (size_t)i + 0, v[0], bad);
     fx_add(a + 3 * (size_t)i + 1, v[1], bad);
@@ -588,12 +643,13 @@ __global__ void k_rb_slots(int nat, const int *__restrict__ order, NlRows L, con
 
 // thread per slot p (atom order[p]): its bonds at offs_s[p] .., neighbours
 // by slot; bsrc (optional) = the bond's entry in the caller-order list
+// bond32 (optional, the mixed form): the same records rounded to FP32
 __global__ void __launch_bounds__(256)
 k_rb_bonds(int nat, const int *__restrict__ order, const int *__restrict__ rank, const double *__restrict__ r,
            const int *__restrict__ el, CellArg C, const RbDev *__restrict__ Dp, NlRows L,
            const int *__restrict__ offs_s, const int *__restrict__ offsets, RbBond *__restrict__ bond,
-           int2 *__restrict__ bij, int *__restrict__ bown, int *__restrict__ bcnt, double2 *__restrict__ nn,
-           int *__restrict__ bsrc)
+           RbBondT<float> *__restrict__ bond32, int2 *__restrict__ bij, int *__restrict__ bown,
+           int *__restrict__ bcnt, double2 *__restrict__ nn, int *__restrict__ bsrc)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= nat) return;
@@ -618,26 +674,33 @@ k_rb_bonds(int nat, const int *__restrict__ order, const int *__restrict__ rank,
             delta_r(C, ri, rj, a, b, c, d);
             const double rl = dot3_nc(d);
             const int t = rb_pair(ki, kj);
-            RbBond B;
+            double len, fc, dfc;
             if (rl < D.cut_l[t] * D.cut_l[t]) {
-                B.len = sqrt(rl);
-                B.fc = 1.0;
-                B.dfc = 0.0;
+                len = sqrt(rl);
+                fc = 1.0;
+                dfc = 0.0;
             } else if (rl < D.cut_h2[t]) {
-                B.len = sqrt(rl);
-                rb_fcin(D, t, B.len, B.fc, B.dfc);
+                len = sqrt(rl);
+                rb_fcin<double>(D, t, len, fc, dfc);
             } else {
                 continue;
             }
             // r_ij = r_j - r_i - A.dc = -d, normalised by division
-            B.n[0] = -d[0] / B.len;
-            B.n[1] = -d[1] / B.len;
-            B.n[2] = -d[2] / B.len;
+            RbBond B;
+            B.len = len;
+            B.fc = fc;
+            B.dfc = dfc;
+            B.n[0] = -d[0] / len;
+            B.n[1] = -d[1] / len;
+            B.n[2] = -d[2] / len;
             B.fx = 0.0;
             B.dfx = 0.0;
-            if (kj == RB_C) nC += B.fc;
-            else nH += B.fc;
+            if (kj == RB_C) nC += fc;
+            else nH += fc;
             bond[beg + nb] = B;
+            if (bond32)
+                bond32[beg + nb] = RbBondT<float>{{(float)B.n[0], (float)B.n[1], (float)B.n[2]}, (float)len,
+                                                  (float)fc, (float)dfc, 0.0f, 0.0f};
             bij[beg + nb] = make_int2(rank[e.x], e.y);
             bown[beg + nb] = p;
             if (bsrc) bsrc[beg + nb] = offsets[i] + k;   // list entry of the bond (nbb of the reference)
@@ -652,7 +715,7 @@ k_rb_bonds(int nat, const int *__restrict__ order, const int *__restrict__ rank,
 __global__ void __launch_bounds__(256)
 k_rb_conj(int nat, const int *__restrict__ el, const int *__restrict__ offsets,
           const int *__restrict__ bcnt, const int2 *__restrict__ bij, const double2 *__restrict__ nn,
-          RbBond *__restrict__ bond, double *__restrict__ nconjit)
+          RbBond *__restrict__ bond, RbBondT<float> *__restrict__ bond32, double *__restrict__ nconjit)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nat) return;
@@ -668,37 +731,44 @@ k_rb_conj(int nat, const int *__restrict__ el, const int *__restrict__ offsets,
         rb_fconj(x, fx, dfx);
         bond[b].fx = fx;
         bond[b].dfx = dfx;
+        if (bond32) {
+            bond32[b].fx = (float)fx;
+            bond32[b].dfx = (float)dfx;
+        }
         s = s + fc * fx;
     }
     nconjit[i] = s;
 }
 
-__device__ __forceinline__ void outer_sub(double *w, const double *a, const double *b)
+// virial and force sums are held in FP64 in both forms (products in R)
+template <typename R>
+__device__ __forceinline__ void outer_sub(double *w, const R *a, const R *b)
 {
 #pragma unroll
     for (int y = 0; y < 3; ++y)
 #pragma unroll
-        for (int x = 0; x < 3; ++x) w[x + 3 * y] = w[x + 3 * y] - a[x] * b[y];
+        for (int x = 0; x < 3; ++x) w[x + 3 * y] = w[x + 3 * y] - (double)(a[x] * b[y]);
 }
 
 // zeta of the bond c->o over the other bonds of centre c, and dzeta/dN
 // (ik_loop2 of bop_kernel_rebo2.f90:1407-1594 without the derivatives)
-__device__ __forceinline__ void rb_zeta(const RbDev &D, int kc, int ijpot, double rlij, const double *rn,
-                                        int beg, int nb, int skip, const RbBond *__restrict__ bond,
-                                        const int2 *__restrict__ bij, const int *__restrict__ el, double nt,
-                                        double &z, double &dzdn)
+template <typename R>
+__device__ __forceinline__ void rb_zeta(const RbDevT<R> &D, int kc, int ijpot, R rlij, const R *rn,
+                                        int beg, int nb, int skip, const RbBondT<R> *__restrict__ bond,
+                                        const int2 *__restrict__ bij, const int *__restrict__ el, R nt,
+                                        R &z, R &dzdn)
 {
-    z = 0.0;
-    dzdn = 0.0;
+    z = R(0.0);
+    dzdn = R(0.0);
     for (int b = beg; b < beg + nb; ++b) {
         if (b == skip) continue;
-        const RbBond B = bond[b];
+        const RbBondT<R> B = bond[b];
         const int ikpot = rb_pair(kc, el[bij[b].x]);
-        const double rlik = B.len;
+        const R rlik = B.len;
         if (!(rlik < D.cut_h[ikpot])) continue;
-        double qf, qd, gf, gd, dgdn;
+        R qf, qd, gf, gd, dgdn;
         rb_h(D, ijpot, ikpot, rlij - rlik, qf, qd);
-        double costh = 0.0;
+        R costh = R(0.0);
         costh = costh + B.n[0] * rn[0];
         costh = costh + B.n[1] * rn[1];
         costh = costh + B.n[2] * rn[2];
@@ -715,44 +785,45 @@ __device__ __forceinline__ void rb_zeta(const RbDev &D, int kc, int ijpot, doubl
 // sgn = +1 for the centre i (bond towards j), -1 for the centre j; the
 // bond-order forces on the centre and on the other atom of the bond go to
 // fcen and foth, the virial to w, already scaled by -dV_A b'.
-__device__ __forceinline__ void rb_neighbours(const RbDev &D, int centre, int kc, int ijpot, double rlij,
-                                              double rlijr, const double *rnij, double sgn, int beg, int nb,
-                                              int skip, const RbBond *__restrict__ bond,
+template <typename R>
+__device__ __forceinline__ void rb_neighbours(const RbDevT<R> &D, int centre, int kc, int ijpot, R rlij,
+                                              R rlijr, const R *rnij, R sgn, int beg, int nb,
+                                              int skip, const RbBondT<R> *__restrict__ bond,
                                               const int2 *__restrict__ bij, const int *__restrict__ offsets,
-                                              const int *__restrict__ bcnt, const int *__restrict__ el, double nt,
-                                              double dfdn, double dfdncn, double dfb, double dpdnc, double dpdnh,
+                                              const int *__restrict__ bcnt, const int *__restrict__ el, R nt,
+                                              R dfdn, R dfdncn, R dfb, R dpdnc, R dpdnh,
                                               double *fcen, double *foth, double *w,
                                               unsigned long long *__restrict__ facc, int &bad)
 {
-    const double rco[3] = {sgn * rlij * rnij[0], sgn * rlij * rnij[1], sgn * rlij * rnij[2]};
+    const R rco[3] = {sgn * rlij * rnij[0], sgn * rlij * rnij[1], sgn * rlij * rnij[2]};
     for (int b = beg; b < beg + nb; ++b) {
         if (b == skip) continue;
-        const RbBond B = bond[b];
+        const RbBondT<R> B = bond[b];
         const int2 e = bij[b];
         const int k = e.x;
         const int ko = el[k];
         const bool kC = ko == RB_C;
-        double fk[3], dr[3];
+        R fk[3], dr[3];
         // coordination / conjugation
-        const double dpn = kC ? dpdnc : dpdnh;
+        const R dpn = kC ? dpdnc : dpdnh;
 #pragma unroll
         for (int x = 0; x < 3; ++x) {
-            const double dn = B.n[x] * B.dfc;
-            const double dncn = kC ? (B.fx * B.dfc) * B.n[x] : 0.0;
+            const R dn = B.n[x] * B.dfc;
+            const R dncn = kC ? (B.fx * B.dfc) * B.n[x] : R(0.0);
             fk[x] = -(dfdn * dn + dfdncn * dncn) - dfb * (dpn * dn);
             fcen[x] -= fk[x];
             dr[x] = B.len * B.n[x];
         }
         outer_sub(w, dr, fk);
-        if (kC && dfdncn != 0.0) {
-            const double nconjdx = B.fc * B.dfx;
+        if (kC && dfdncn != R(0.0)) {
+            const R nconjdx = B.fc * B.dfx;
             const int kb = offsets[k], kn = bcnt[k];
             for (int m = kb; m < kb + kn; ++m) {
                 const int2 em = bij[m];
                 // the centre itself (m == centre at the centre's image) is skipped
                 if (em.x == centre && dc_zero_sum(e.y, em.y)) continue;
-                const RbBond M = bond[m];
-                double d2[3], drm[3];
+                const RbBondT<R> M = bond[m];
+                R d2[3], drm[3];
 #pragma unroll
                 for (int x = 0; x < 3; ++x) {
                     d2[x] = -dfdncn * (nconjdx * (M.dfc * M.n[x]));
@@ -765,40 +836,40 @@ __device__ __forceinline__ void rb_neighbours(const RbDev &D, int centre, int kc
         }
         // bond order: angular terms of zeta_co with the final dV_A b'
         const int ikpot = rb_pair(kc, ko);
-        const double rlik = B.len;
+        const R rlik = B.len;
         if (rlik < D.cut_h[ikpot]) {
-            double qf, qd, gf, gd, dgdn;
+            R qf, qd, gf, gd, dgdn;
             rb_h(D, ijpot, ikpot, rlij - rlik, qf, qd);
-            double costh = 0.0;
+            R costh = R(0.0);
             costh = costh + B.n[0] * rnij[0];
             costh = costh + B.n[1] * rnij[1];
             costh = costh + B.n[2] * rnij[2];
             if (sgn < 0) costh = -costh;
             rb_g(D, kc, costh, nt, gf, gd, dgdn);
-            const double fcik = B.fc;
-            double dk[3], dd = 0.0;
+            const R fcik = B.fc;
+            R dk[3], dd = R(0.0);
 #pragma unroll
             for (int x = 0; x < 3; ++x) dk[x] = B.n[x] * rlik - sgn * rnij[x] * rlij;
             dd = dd + dk[0] * dk[0];
             dd = dd + dk[1] * dk[1];
             dd = dd + dk[2] * dk[2];
-            const double dis = sqrt(dd);
+            const R dis = sqrt(dd);
 #pragma unroll
             for (int x = 0; x < 3; ++x) dk[x] = dk[x] / dis;
-            const double dcs_io = 1.0 / rlik - costh * rlijr;
-            const double dcs_ik = rlijr - costh / rlik;
-            const double dcs_ok = -dis * rlijr / rlik;
-            const double dzfac = fcik * gd * qf;
-            const double dzdro = gf * fcik * qd;
-            const double dzdrk = gf * (B.dfc * qf - fcik * qd);
-            double dfo[3], dbk[3];
-            const double rik[3] = {rlik * B.n[0], rlik * B.n[1], rlik * B.n[2]};
+            const R dcs_io = R(1.0) / rlik - costh * rlijr;
+            const R dcs_ik = rlijr - costh / rlik;
+            const R dcs_ok = -dis * rlijr / rlik;
+            const R dzfac = fcik * gd * qf;
+            const R dzdro = gf * fcik * qd;
+            const R dzdrk = gf * (B.dfc * qf - fcik * qd);
+            R dfo[3], dbk[3];
+            const R rik[3] = {rlik * B.n[0], rlik * B.n[1], rlik * B.n[2]};
 #pragma unroll
             for (int x = 0; x < 3; ++x) {
-                const double u = sgn * rnij[x];      // unit vector centre -> other
-                const double dcs_c = -dcs_io * u - dcs_ik * B.n[x];
-                const double dcs_o = dcs_io * u - dcs_ok * dk[x];
-                const double dcs_k = dcs_ik * B.n[x] + dcs_ok * dk[x];
+                const R u = sgn * rnij[x];      // unit vector centre -> other
+                const R dcs_c = -dcs_io * u - dcs_ik * B.n[x];
+                const R dcs_o = dcs_io * u - dcs_ok * dk[x];
+                const R dcs_k = dcs_ik * B.n[x] + dcs_ok * dk[x];
                 fcen[x] += -dfb * (-dzdro * u - dzdrk * B.n[x] + dzfac * dcs_c);
                 dfo[x] = dzdro * u + dzfac * dcs_o;
                 foth[x] += -dfb * dfo[x];
@@ -816,42 +887,43 @@ __device__ __forceinline__ void rb_neighbours(const RbDev &D, int centre, int kc
     }
 }
 
-__device__ __forceinline__ void outer_add(double *w, const double *a, const double *b)
+template <typename R>
+__device__ __forceinline__ void outer_add(double *w, const R *a, const R *b)
 {
 #pragma unroll
     for (int y = 0; y < 3; ++y)
 #pragma unroll
-        for (int x = 0; x < 3; ++x) w[x + 3 * y] = w[x + 3 * y] + a[x] * b[y];
+        for (int x = 0; x < 3; ++x) w[x + 3 * y] = w[x + 3 * y] + (double)(a[x] * b[y]);
 }
 
 // torsion sum over k (neighbours of i but j) and l (neighbours of j but i,
 // and not k): returns bdh = sum (1 - cos^2 w) fc_ik fc_jl; forces on i, j
 // into fi, fj, on k and l through the accumulators; virial into w
 // (bop_kernel_rebo2.f90:1939-2120)
-__device__ double rb_torsion(int ib, int inb, int sij, int jb, int jnb, int sji, int dcij,
-                             const RbBond *__restrict__ bond, const int2 *__restrict__ bij,
-                             const double *rnij, double rlij, double tij, double fa, double fcar,
-                             double *fi, double *fj, double *w, unsigned long long *__restrict__ facc,
-                             int &bad)
+// bond_at(b, 0) / bond_at(b, 1): the record of bond b of i / of j
+template <typename R, typename F>
+__device__ R rb_torsion(int ib, int inb, int sij, int jb, int jnb, int sji, int dcij, F bond_at,
+                        const int2 *__restrict__ bij, const R *rnij, R rlij, R tij, R fa, R fcar, double *fi,
+                        double *fj, double *w, unsigned long long *__restrict__ facc, int &bad)
 {
-    const double tije = tij * fa * fcar;
-    double bdh = 0.0;
+    const R tije = tij * fa * fcar;
+    R bdh = R(0.0);
     int a0, b0, c0;
     dc_unpack(dcij, a0, b0, c0);
     for (int kb = ib; kb < ib + inb; ++kb) {
         if (kb == sij) continue;
-        const RbBond K = bond[kb];
+        const RbBondT<R> K = bond_at(kb, 0);
         const int2 ek = bij[kb];
         int ak, bk, ck;
         dc_unpack(ek.y, ak, bk, ck);
-        const double *rnik = K.n;
-        const double rlik = K.len;
-        double dot_ij_ik = 0.0;
+        const R *rnik = K.n;
+        const R rlik = K.len;
+        R dot_ij_ik = R(0.0);
         dot_ij_ik = dot_ij_ik + rnij[0] * rnik[0];
         dot_ij_ik = dot_ij_ik + rnij[1] * rnik[1];
         dot_ij_ik = dot_ij_ik + rnij[2] * rnik[2];
-        const double dcik = 1.0 - dot_ij_ik * dot_ij_ik;
-        double fk[3] = {0, 0, 0};
+        const R dcik = R(1.0) - dot_ij_ik * dot_ij_ik;
+        R fk[3] = {0, 0, 0};
         for (int lb = jb; lb < jb + jnb; ++lb) {
             if (lb == sji) continue;
             const int2 e = bij[lb];
@@ -860,30 +932,30 @@ __device__ double rb_torsion(int ib, int inb, int sij, int jb, int jnb, int sji,
                 dc_unpack(e.y, al, bl, cl);
                 if (a0 + al == ak && b0 + bl == bk && c0 + cl == ck) continue;
             }
-            const RbBond L = bond[lb];
-            const double *rnjl = L.n;
-            const double rljl = L.len;
-            double dot_ij_jl = 0.0, dot_ik_jl = 0.0;
+            const RbBondT<R> L = bond_at(lb, 1);
+            const R *rnjl = L.n;
+            const R rljl = L.len;
+            R dot_ij_jl = R(0.0), dot_ik_jl = R(0.0);
             dot_ij_jl = dot_ij_jl + rnij[0] * rnjl[0];
             dot_ij_jl = dot_ij_jl + rnij[1] * rnjl[1];
             dot_ij_jl = dot_ij_jl + rnij[2] * rnjl[2];
             dot_ik_jl = dot_ik_jl + rnik[0] * rnjl[0];
             dot_ik_jl = dot_ik_jl + rnik[1] * rnjl[1];
             dot_ik_jl = dot_ik_jl + rnik[2] * rnjl[2];
-            const double dcjl = 1.0 - dot_ij_jl * dot_ij_jl;
-            const double abs_dc = sqrt(dcik * dcjl);
-            const double cost = (dot_ij_ik * dot_ij_jl - dot_ik_jl) / abs_dc;
-            double bdhij = 1 - cost * cost;
+            const R dcjl = R(1.0) - dot_ij_jl * dot_ij_jl;
+            const R abs_dc = sqrt(dcik * dcjl);
+            const R cost = (dot_ij_ik * dot_ij_jl - dot_ik_jl) / abs_dc;
+            R bdhij = 1 - cost * cost;
             bdh = bdh + bdhij * K.fc * L.fc;
             bdhij = bdhij * tij * fa * fcar / 2;
-            const double dbdhij = -2 * cost * tije * K.fc * L.fc / 2;
-            double df[3], v[3], fl[3];
+            const R dbdhij = -2 * cost * tije * K.fc * L.fc / 2;
+            R df[3], v[3], fl[3];
 #pragma unroll
             for (int x = 0; x < 3; ++x)
                 df[x] = dbdhij *
                         ((dot_ij_jl / abs_dc + cost * dot_ij_ik / dcik) * rnik[x] +
                          (dot_ij_ik / abs_dc + cost * dot_ij_jl / dcjl) * rnjl[x] -
-                         (2 * dot_ik_jl / abs_dc + cost * (1.0 / dcik + 1.0 / dcjl)) * rnij[x]) /
+                         (2 * dot_ik_jl / abs_dc + cost * (R(1.0) / dcik + R(1.0) / dcjl)) * rnij[x]) /
                         rlij;
 #pragma unroll
             for (int x = 0; x < 3; ++x) {
@@ -895,7 +967,7 @@ __device__ double rb_torsion(int ib, int inb, int sij, int jb, int jnb, int sji,
 #pragma unroll
             for (int x = 0; x < 3; ++x)
                 df[x] = dbdhij *
-                            (-1.0 / dcik * cost * rnik[x] - 1.0 / abs_dc * rnjl[x] +
+                            (-R(1.0) / dcik * cost * rnik[x] - R(1.0) / abs_dc * rnjl[x] +
                              (dot_ij_jl / abs_dc + cost * dot_ij_ik / dcik) * rnij[x]) /
                             rlik +
                         bdhij * K.dfc * L.fc * rnik[x];
@@ -909,7 +981,7 @@ __device__ double rb_torsion(int ib, int inb, int sij, int jb, int jnb, int sji,
 #pragma unroll
             for (int x = 0; x < 3; ++x)
                 df[x] = dbdhij *
-                            (-1.0 / dcjl * cost * rnjl[x] - 1.0 / abs_dc * rnik[x] +
+                            (-R(1.0) / dcjl * cost * rnjl[x] - R(1.0) / abs_dc * rnik[x] +
                              (dot_ij_ik / abs_dc + cost * dot_ij_jl / dcjl) * rnij[x]) /
                             rljl +
                         bdhij * K.fc * L.dfc * rnjl[x];
@@ -932,14 +1004,15 @@ __device__ double rb_torsion(int ib, int inb, int sij, int jb, int jnb, int sji,
 // dP/dN + dzeta/dN; the slot of the reverse bond o->c; and whether the slot
 // carries a half pair ((dc = 0 and o > c) or dc > 0; for a LAMMPS step, o of
 // tag >= tag(c) with c local).
+template <typename R>
 __global__ void __launch_bounds__(256)
-k_rb_bo(int ntot, const RbDev *__restrict__ Dp, const double *__restrict__ tab, const int *__restrict__ el,
+k_rb_bo(int ntot, const RbDevT<R> *__restrict__ Dp, const R *__restrict__ tab, const int *__restrict__ el,
         const int *__restrict__ offsets, const int *__restrict__ bcnt, const int *__restrict__ bown,
-        const RbBond *__restrict__ bond, const int2 *__restrict__ bij, const double2 *__restrict__ nn,
-        double4 *__restrict__ bo, int *__restrict__ rev, int *__restrict__ half, int natloc,
+        const RbBondT<R> *__restrict__ bond, const int2 *__restrict__ bij, const double2 *__restrict__ nn,
+        Rb4<R> *__restrict__ bo, int *__restrict__ rev, int *__restrict__ half, int natloc,
         const int *__restrict__ tag, const int *__restrict__ order, int *__restrict__ flags)
 {
-    const RbDev &D = *Dp;
+    const RbDevT<R> &D = *Dp;
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= ntot) return;
     const int i = bown[s];
@@ -950,7 +1023,7 @@ k_rb_bo(int ntot, const RbDev *__restrict__ Dp, const double *__restrict__ tab, 
     const int2 e = bij[s];
     const int j = e.x;
     const int ki = el[i], kj = el[j];
-    const RbBond B = bond[s];
+    const RbBondT<R> B = bond[s];
     const int ijpot = rb_pair(ki, kj);
     int a, b, c;
     dc_unpack(e.y, a, b, c);
@@ -973,25 +1046,25 @@ k_rb_bo(int ntot, const RbDev *__restrict__ Dp, const double *__restrict__ tab, 
     if (sr < 0) atomicOr(flags, DERR_BOND_OVERFLOW);
     if (!act) return;
     const double2 nni = nn[i];
-    double niC = nni.x, niH = nni.y;
+    R niC = nni.x, niH = nni.y;
     if (kj == RB_C) niC = niC - B.fc; else niH = niH - B.fc;
-    niC = niC > 4.0 ? 4.0 : niC;
-    niH = niH > 4.0 ? 4.0 : niH;
-    double z, dzdn;
+    niC = niC > R(4.0) ? R(4.0) : niC;
+    niH = niH > R(4.0) ? R(4.0) : niH;
+    R z, dzdn;
     rb_zeta(D, ki, ijpot, B.len, B.n, offsets[i], bcnt[i], s, bond, bij, el, niC + niH, z, dzdn);
-    double dpc = 0.0, dph = 0.0;
+    R dpc = R(0.0), dph = R(0.0);
     if (ki == RB_C) {
-        double pij;
+        R pij;
         rb_table2(tab + (ijpot == RB_CC ? RB_TAB_PCC : RB_TAB_PCH), niH, niC, pij, dph, dpc);
         z = z + pij;
         dpc = dpc + dzdn;
         dph = dph + dzdn;
     }
-    double fa, dfa;
+    R fa, dfa;
     rb_va(D, ijpot, B.len, fa, dfa);
-    const double bb = pow(1.0 + z, D.conpe);
-    const double dfb = D.conan * B.fc * fa * pow(1.0 + z, D.conpf);
-    bo[s] = make_double4(bb, dfb, dpc, dph);
+    const R bb = pow(R(1.0) + z, D.conpe);
+    const R dfb = D.conan * B.fc * fa * pow(R(1.0) + z, D.conpf);
+    bo[s] = Rb4<R>{bb, dfb, dpc, dph};
 }
 
 // ordered list of the half-pair slots (exclusive scan of the flags)
@@ -1004,19 +1077,21 @@ __global__ void k_rb_list(int ntot, const int *__restrict__ half, const int *__r
     if (s == ntot - 1) *count = pos[s] + half[s];
 }
 
+template <typename R>
 __global__ void __launch_bounds__(256)
-k_rb_pair(int ntot, const int *__restrict__ list, const int *__restrict__ count, const RbDev *__restrict__ Dp,
-          const double *__restrict__ tab, const int *__restrict__ el, const int *__restrict__ offsets,
-          const int *__restrict__ bcnt, const int *__restrict__ bown, const RbBond *__restrict__ bond,
+k_rb_pair(int ntot, const int *__restrict__ list, const int *__restrict__ count, const RbDevT<R> *__restrict__ Dp,
+          const R *__restrict__ tab, const int *__restrict__ el, const int *__restrict__ offsets,
+          const int *__restrict__ bcnt, const int *__restrict__ bown, const RbBondT<R> *__restrict__ bond,
           const int2 *__restrict__ bij, const double2 *__restrict__ nn, const double *__restrict__ nconjit,
-          const double4 *__restrict__ bo, const int *__restrict__ rev,
+          const Rb4<double> *__restrict__ bo, const int *__restrict__ rev,
           unsigned long long *__restrict__ facc, unsigned long long *__restrict__ eacc,
           unsigned long long *__restrict__ wacc, double *__restrict__ partial, int *__restrict__ flags,
           const int *__restrict__ bsrc, double *__restrict__ epb, double *__restrict__ fpb,
-          double *__restrict__ wpb)
+          double *__restrict__ wpb, const RbDev *__restrict__ D64p, const RbBond *__restrict__ bond64)
 {
     __shared__ double s_red[4][10];
-    const RbDev &D = *Dp;
+    const RbDevT<R> &D = *Dp;
+    constexpr bool F32 = sizeof(R) == 4;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     int bad = 0;
     double epair = 0.0;
@@ -1030,50 +1105,71 @@ k_rb_pair(int ntot, const int *__restrict__ list, const int *__restrict__ count,
         const int j = eij.x;
         const int sji = rev[s];
         const int ki = el[i], kj = el[j];
-        const RbBond Bij = bond[s];
+        const RbBondT<R> Bij = bond[s];
         const int ijpot = rb_pair(ki, kj);
-        const double rlij = Bij.len;
-        const double rlijr = 1.0 / rlij;
-        const double *rnij = Bij.n;
-        const double rij[3] = {rlij * rnij[0], rlij * rnij[1], rlij * rnij[2]};
-        const double fcar = Bij.fc, dfcar = Bij.dfc;
+        const R rlij = Bij.len;
+        const R rlijr = R(1.0) / rlij;
+        const R *rnij = Bij.n;
+        const R fcar = Bij.fc, dfcar = Bij.dfc;
         const double2 nni = nn[i], nnj = nn[j];
-        double niC = nni.x, niH = nni.y, njC = nnj.x, njH = nnj.y;
+        R niC = nni.x, niH = nni.y, njC = nnj.x, njH = nnj.y;
         if (kj == RB_C) niC = niC - Bij.fc; else niH = niH - Bij.fc;
         if (ki == RB_C) njC = njC - Bij.fc; else njH = njH - Bij.fc;
-        niC = niC > 4.0 ? 4.0 : niC;
-        niH = niH > 4.0 ? 4.0 : niH;
-        njC = njC > 4.0 ? 4.0 : njC;
-        njH = njH > 4.0 ? 4.0 : njH;
+        niC = niC > R(4.0) ? R(4.0) : niC;
+        niH = niH > R(4.0) ? R(4.0) : niH;
+        njC = njC > R(4.0) ? R(4.0) : njC;
+        njH = njH > R(4.0) ? R(4.0) : njH;
         // N_i, N_j as g sees them; F sees them clamped to 3 below
-        const double nti_g = niC + niH, ntj_g = njC + njH;
-        double nti = nti_g, ntj = ntj_g;
-        double fa, dfa, fr, dfr;
+        const R nti_g = niC + niH, ntj_g = njC + njH;
+        R nti = nti_g, ntj = ntj_g;
+        R fa, dfa, fr, dfr;
         rb_va(D, ijpot, rlij, fa, dfa);
         rb_vr(D, ijpot, rlij, fr, dfr);
         const int ib = offsets[i], inb = bcnt[i];
         const int jb = offsets[j], jnb = bcnt[j];
-        const double4 boi = bo[s], boj = bo[sji];
-        const double bi = boi.x, dfbi = boi.y, dpdnci = boi.z, dpdnhi = boi.w;
-        const double bj = boj.x, dfbj = boj.y, dpdncj = boj.z, dpdnhj = boj.w;
-        const double nconji = nconjit[i] - Bij.fc * Bij.fx;
-        double nconjj = 0.0;
+        // bond orders in FP64 in both forms
+        const Rb4<double> boi = bo[s], boj = bo[sji];
+        const double bi = boi.x, bj = boj.x;
+        const R dfbi = (R)boi.y, dpdnci = (R)boi.z, dpdnhi = (R)boi.w;
+        const R dfbj = (R)boj.y, dpdncj = (R)boj.z, dpdnhj = (R)boj.w;
+        const R nconji = nconjit[i] - Bij.fc * Bij.fx;
+        R nconjj = R(0.0);
         for (int b = jb; b < jb + jnb; ++b)
             if (b != sji && el[bij[b].x] == RB_C) nconjj = nconjj + bond[b].fc * bond[b].fx;
         // F(N_i, N_j, N_conj) and the torsion sum
-        double nconj = nconji * nconji + nconjj * nconjj;
-        nconj = nconj > 8.0 ? 8.0 : nconj;
-        nti = nti > 3.0 ? 3.0 : nti;
-        ntj = ntj > 3.0 ? 3.0 : ntj;
+        R nconj = nconji * nconji + nconjj * nconjj;
+        nconj = nconj > R(8.0) ? R(8.0) : nconj;
+        nti = nti > R(3.0) ? R(3.0) : nti;
+        ntj = ntj > R(3.0) ? R(3.0) : ntj;
         double fi[3] = {0, 0, 0}, fj[3] = {0, 0, 0};
-        double bdh = 0.0, tij = 0.0, dtdni = 0.0, dtdnj = 0.0, dtdncn = 0.0;
+        R bdh = R(0.0), tij = R(0.0), dtdni = R(0.0), dtdnj = R(0.0), dtdncn = R(0.0);
+        // the mixed form evaluates the pair's radial terms and the torsion
+        // on the FP64 records (B64): the torsion's 1 / sin^2 factors and the
+        // large V_R' + b V_A' cancellation do not survive FP32 bond vectors
+        RbBond B64;
+        if constexpr (F32) {
+            B64 = bond64[s];
+        } else {
+            B64 = Bij;
+        }
         if (D.with_dihedral && ijpot == RB_CC) {
             rb_table3(tab + RB_TAB_TCC, nti, ntj, nconj, tij, dtdni, dtdnj, dtdncn);
-            if (tij != 0)
-                bdh = rb_torsion(ib, inb, s, jb, jnb, sji, eij.y, bond, bij, rnij, rlij, tij, fa, fcar, fi, fj,
-                                 wij, facc, bad);
+            if (tij != 0) {
+                if constexpr (F32) {
+                    const RbDev &D64 = *D64p;
+                    double fa64, dfa64;
+                    rb_va<double>(D64, ijpot, B64.len, fa64, dfa64);
+                    auto at64 = [&](int b, int) { return bond64[b]; };
+                    bdh = (R)rb_torsion<double>(ib, inb, s, jb, jnb, sji, eij.y, at64, bij, B64.n, B64.len,
+                                                (double)tij, fa64, B64.fc, fi, fj, wij, facc, bad);
+                } else {
+                    auto at = [&](int b, int) { return bond[b]; };
+                    bdh = rb_torsion<R>(ib, inb, s, jb, jnb, sji, eij.y, at, bij, rnij, rlij, tij, fa, fcar, fi,
+                                        fj, wij, facc, bad);
+                }
+            }
         }
-        double fij = 0.0, dfdni = 0.0, dfdnj = 0.0, dfdncn = 0.0;
+        R fij = R(0.0), dfdni = R(0.0), dfdnj = R(0.0), dfdncn = R(0.0);
         if (ijpot == RB_CC)
             rb_table3(tab + RB_TAB_FCC, nti, ntj, nconj, fij, dfdni, dfdnj, dfdncn);
         else if (ijpot == RB_HH)
@@ -1085,32 +1181,38 @@ k_rb_pair(int ntot, const int *__restrict__ list, const int *__restrict__ count,
         dfdni = dfdni + dtdni * bdh;
         dfdnj = dfdnj + dtdnj * bdh;
         dfdncn = dfdncn + dtdncn * bdh;
-        dfdni = 0.5 * fcar * fa * dfdni;
-        dfdnj = 0.5 * fcar * fa * dfdnj;
-        dfdncn = 0.5 * fcar * fa * dfdncn;
-        const double dfdncni = 2 * dfdncn * nconji;
-        const double dfdncnj = 2 * dfdncn * nconjj;
+        dfdni = R(0.5) * fcar * fa * dfdni;
+        dfdnj = R(0.5) * fcar * fa * dfdnj;
+        dfdncn = R(0.5) * fcar * fa * dfdncn;
+        const R dfdncni = 2 * dfdncn * nconji;
+        const R dfdncnj = 2 * dfdncn * nconjj;
         // neighbours of i and of j: coordination, conjugation and bond-order forces
-        rb_neighbours(D, i, ki, ijpot, rlij, rlijr, rnij, 1.0, ib, inb, s, bond, bij, offsets, bcnt, el, nti_g,
+        rb_neighbours(D, i, ki, ijpot, rlij, rlijr, rnij, R(1.0), ib, inb, s, bond, bij, offsets, bcnt, el, nti_g,
                       dfdni, dfdncni, dfbi, dpdnci, dpdnhi, fi, fj, wij, facc, bad);
-        rb_neighbours(D, j, kj, ijpot, rlij, rlijr, rnij, -1.0, jb, jnb, sji, bond, bij, offsets, bcnt, el, ntj_g,
+        rb_neighbours(D, j, kj, ijpot, rlij, rlijr, rnij, -R(1.0), jb, jnb, sji, bond, bij, offsets, bcnt, el, ntj_g,
                       dfdnj, dfdncnj, dfbj, dpdncj, dpdnhj, fj, fi, wij, facc, bad);
-        // pair energy and forces
-        const double bave = 0.5 * (bi + bj + fij + tij * bdh);
-        const double hlf = fcar * (fr + bave * fa) / 2;
+        // pair energy and forces (FP64 radial terms in the mixed form)
+        double fa2 = fa, dfa2 = dfa, fr2 = fr, dfr2 = dfr;
+        if constexpr (F32) {
+            rb_va<double>(*D64p, ijpot, B64.len, fa2, dfa2);
+            rb_vr<double>(*D64p, ijpot, B64.len, fr2, dfr2);
+        }
+        const double fcar2 = B64.fc, dfcar2 = B64.dfc;
+        const double bave = 0.5 * ((bi + bj + (double)fij) + (double)tij * (double)bdh);
+        const double hlf = fcar2 * (fr2 + bave * fa2) / 2;
         epair = hlf + hlf;
-        const double dffac = dfr * fcar + bave * dfa * fcar + fr * dfcar + bave * fa * dfcar;
+        const double dffac = dfr2 * fcar2 + bave * dfa2 * fcar2 + fr2 * dfcar2 + bave * fa2 * dfcar2;
         double df[3];
 #pragma unroll
         for (int x = 0; x < 3; ++x) {
-            df[x] = dffac * rnij[x];
+            df[x] = dffac * B64.n[x];
             fi[x] += df[x];
             fj[x] -= df[x];
         }
 #pragma unroll
         for (int y = 0; y < 3; ++y)
 #pragma unroll
-            for (int x = 0; x < 3; ++x) wij[x + 3 * y] = wij[x + 3 * y] + rij[x] * df[y];
+            for (int x = 0; x < 3; ++x) wij[x + 3 * y] = wij[x + 3 * y] + (B64.len * B64.n[x]) * df[y];
         fx_add3(facc, i, fi, bad);
         fx_add3(facc, j, fj, bad);
         // per-bond properties on the list entry of the half pair's bond i->j
@@ -1217,13 +1319,86 @@ int rebo2_bind(agpu_potential *pot)
     return AGPU_OK;
 }
 
+// bonds, conjugation, bond orders (FP64 in both forms), the half-pair list
+// and the pair kernel in precision R: FP64, or the mixed form (FP32 copies of
+// the bond records for the per-neighbour loops, FP64 records for the pair's
+// radial terms and the torsion, 64-bit fixed-point accumulation); the slot
+// arrays and accumulators are ready
+template <typename R>
+static int rebo2_kernels(Rebo2State *st, agpu_potential *pot, const agpu_eval_args *a, const int *bsrc,
+                         unsigned long long *eacc, unsigned long long *wacc, int *nblk_out)
+{
+    agpu_particles *p = pot->p;
+    agpu_neighbors *nl = pot->nl;
+    hipStream_t s = stream();
+    const int nat = p->nat;
+    const int64_t total = nl->total;
+    const bool f64 = sizeof(R) == 8;
+    AGPU_CHECK(st->bond.reserve(sizeof(RbBond) * (total + 1)));
+    if (!f64) AGPU_CHECK(st->bond32.reserve(sizeof(RbBondT<float>) * (total + 1)));
+    AGPU_CHECK(st->bo.reserve(sizeof(Rb4<double>) * (total + 1)));
+    CellArg C;
+    std::memcpy(C.A, p->cell.A, sizeof C.A);
+    const int *el = pot->d_el.as<int>();
+    const RbDev *D64 = st->dev.as<RbDev>();
+    const RbDevT<R> *D = f64 ? reinterpret_cast<const RbDevT<R> *>(st->dev.ptr)
+                             : reinterpret_cast<const RbDevT<R> *>(st->dev32.ptr);
+    const R *tab = f64 ? reinterpret_cast<const R *>(st->tab.ptr) : reinterpret_cast<const R *>(st->tab32.ptr);
+    const int nb = (nat + 255) / 256;
+    const int *order = nl->idx_sorted.as<int>();
+    const bool lammps = p->natloc >= 0;
+    const int *offs = st->offs_s.as<int>();
+    const int *els = st->el_s.as<int>();
+    RbBond *bond = st->bond.as<RbBond>();
+    RbBondT<float> *bond32 = f64 ? nullptr : st->bond32.as<RbBondT<float>>();
+    const RbBondT<R> *bondR = f64 ? reinterpret_cast<const RbBondT<R> *>(bond)
+                                  : reinterpret_cast<const RbBondT<R> *>(bond32);
+    Rb4<double> *bo = st->bo.as<Rb4<double>>();
+    int *dstats = st->stats.as<int>();
+    AGPU_LAUNCH("rebo2_bonds", k_rb_bonds, dim3(nb), dim3(256), 0, s, nat, order, nl->rank.as<int>(), p->d_r, el,
+                C, D64, nl_rows(nl), offs, nl->offsets.as<int>(), bond, bond32, st->bij.as<int2>(),
+                st->bown.as<int>(), st->bcnt.as<int>(), st->nn.as<double2>(), const_cast<int *>(bsrc));
+    AGPU_LAUNCH("rebo2_conj", k_rb_conj, dim3(nb), dim3(256), 0, s, nat, els, offs, st->bcnt.as<int>(),
+                st->bij.as<int2>(), st->nn.as<double2>(), bond, bond32, st->nconj.as<double>());
+    const int ntot = (int)total;
+    int nblk = (ntot + 255) / 256;
+    if (nblk < 1) nblk = 1;
+    *nblk_out = nblk;
+    AGPU_CHECK(st->partial.reserve(sizeof(double) * 10 * nblk));
+    AGPU_CHECK(st->rev.reserve(sizeof(int) * (total + 1)));
+    AGPU_CHECK(st->half.reserve(sizeof(int) * (total + 1)));
+    AGPU_CHECK(st->hpos.reserve(sizeof(int) * (total + 1)));
+    AGPU_CHECK(st->list.reserve(sizeof(int) * (total + 2)));
+    int *count = st->list.as<int>() + total + 1;
+    if (ntot > 0) {
+        AGPU_LAUNCH("rebo2_bo", k_rb_bo<double>, dim3(nblk), dim3(256), 0, s, ntot, D64, st->tab.as<double>(),
+                    els, offs, st->bcnt.as<int>(), st->bown.as<int>(), bond, st->bij.as<int2>(),
+                    st->nn.as<double2>(), bo, st->rev.as<int>(), st->half.as<int>(), p->natloc,
+                    lammps ? st->tag_s.as<int>() : nullptr, order, dstats);
+        size_t tmp = 0;
+        AGPU_HIP(rocprim::exclusive_scan(nullptr, tmp, st->half.as<int>(), st->hpos.as<int>(), 0, (size_t)ntot,
+                                         rocprim::plus<int>(), s));
+        AGPU_CHECK(st->scan_tmp.reserve(tmp + 16));
+        AGPU_HIP(rocprim::exclusive_scan(st->scan_tmp.ptr, tmp, st->half.as<int>(), st->hpos.as<int>(), 0,
+                                         (size_t)ntot, rocprim::plus<int>(), s));
+        AGPU_LAUNCH("rebo2_list", k_rb_list, dim3(nblk), dim3(256), 0, s, ntot, st->half.as<int>(),
+                    st->hpos.as<int>(), st->list.as<int>(), count);
+    } else {
+        AGPU_CHECK(zero_dev(count, sizeof(int)));
+    }
+    AGPU_LAUNCH(f64 ? "rebo2_pair" : "rebo2_pair_mixed", k_rb_pair<R>, dim3(nblk), dim3(256), 0, s, ntot,
+                st->list.as<int>(), count, D, tab, els, offs, st->bcnt.as<int>(), st->bown.as<int>(), bondR,
+                st->bij.as<int2>(), st->nn.as<double2>(), st->nconj.as<double>(), bo, st->rev.as<int>(),
+                st->facc.as<unsigned long long>(), eacc, wacc, st->partial.as<double>(), dstats, bsrc,
+                a->epot_per_bond, a->f_per_bond, a->wpot_per_bond, D64, bond);
+    return AGPU_OK;
+}
+
 int rebo2_compute(agpu_potential *pot, const agpu_eval_args *a)
 {
     Rebo2State *st = static_cast<Rebo2State *>(pot->impl);
     agpu_particles *p = pot->p;
     agpu_neighbors *nl = pot->nl;
-    if (pot->precision != AGPU_PREC_FP64)
-        return set_error(AGPU_ERR_UNSUPPORTED, "REBO2: only the FP64 path is implemented");
     if (a->mask)
         return set_error(AGPU_ERR_UNSUPPORTED, "REBO2: the reference kernel takes no mask");
     if (st->Z_rev != p->Z_rev) AGPU_CHECK(rebo2_bind(pot));
@@ -1231,10 +1406,9 @@ int rebo2_compute(agpu_potential *pot, const agpu_eval_args *a)
     AGPU_CHECK(nl_update(nl, p, 0, false));
     const int nat = p->nat;
     hipStream_t s = stream();
-    AGPU_HIP(hipMemsetAsync(a->epot, 0, sizeof(double), s));
+    AGPU_CHECK(zero_dev(a->epot, sizeof(double)));
     if (nat == 0) return AGPU_OK;
     const int64_t total = nl->total;
-    AGPU_CHECK(st->bond.reserve(sizeof(RbBond) * (total + 1)));
     AGPU_CHECK(st->bij.reserve(sizeof(int2) * (total + 1)));
     AGPU_CHECK(st->bown.reserve(sizeof(int) * (total + 1)));
     AGPU_CHECK(st->bcnt.reserve(sizeof(int) * (nat + 1)));
@@ -1258,11 +1432,8 @@ int rebo2_compute(agpu_potential *pot, const agpu_eval_args *a)
     if (eacc) AGPU_LAUNCH("rebo2_zero", k_rb_zero, dim3(256), dim3(256), 0, s, eacc, (size_t)nat);
     if (wacc) AGPU_LAUNCH("rebo2_zero", k_rb_zero, dim3(512), dim3(256), 0, s, wacc, (size_t)9 * nat);
     int *dstats = st->stats.as<int>();
-    AGPU_HIP(hipMemsetAsync(dstats, 0, sizeof(int) * 4, s));
-    CellArg C;
-    std::memcpy(C.A, p->cell.A, sizeof C.A);
+    AGPU_CHECK(zero_dev(dstats, sizeof(int) * 4));
     const int *el = pot->d_el.as<int>();
-    const RbDev *D = st->dev.as<RbDev>();
     const int nb = (nat + 255) / 256;
     const bool per_bond = a->epot_per_bond || a->f_per_bond || a->wpot_per_bond;
     if (per_bond) AGPU_CHECK(st->bsrc.reserve(sizeof(int) * (total + 1)));
@@ -1274,8 +1445,7 @@ int rebo2_compute(agpu_potential *pot, const agpu_eval_args *a)
     AGPU_CHECK(st->offs_s.reserve(sizeof(int) * (nat + 1)));
     AGPU_CHECK(st->el_s.reserve(sizeof(int) * (nat + 1)));
     if (lammps) AGPU_CHECK(st->tag_s.reserve(sizeof(int) * (nat + 1)));
-    const NlRows L = nl_rows(nl);
-    AGPU_LAUNCH("rebo2_slots", k_rb_slots, dim3((nat + 256) / 256), dim3(256), 0, s, nat, order, L, el,
+    AGPU_LAUNCH("rebo2_slots", k_rb_slots, dim3((nat + 256) / 256), dim3(256), 0, s, nat, order, nl_rows(nl), el,
                 lammps ? p->d_tag.as<int>() : nullptr, st->cnt_s.as<int>(), st->el_s.as<int>(),
                 lammps ? st->tag_s.as<int>() : nullptr);
     {
@@ -1286,44 +1456,11 @@ int rebo2_compute(agpu_potential *pot, const agpu_eval_args *a)
         AGPU_HIP(rocprim::exclusive_scan(st->scan_tmp.ptr, tmp, st->cnt_s.as<int>(), st->offs_s.as<int>(), 0,
                                          (size_t)nat + 1, rocprim::plus<int>(), s));
     }
-    const int *offs = st->offs_s.as<int>();
-    const int *els = st->el_s.as<int>();
-    AGPU_LAUNCH("rebo2_bonds", k_rb_bonds, dim3(nb), dim3(256), 0, s, nat, order, nl->rank.as<int>(), p->d_r, el,
-                C, D, L, offs, nl->offsets.as<int>(), st->bond.as<RbBond>(), st->bij.as<int2>(),
-                st->bown.as<int>(), st->bcnt.as<int>(), st->nn.as<double2>(), bsrc);
-    AGPU_LAUNCH("rebo2_conj", k_rb_conj, dim3(nb), dim3(256), 0, s, nat, els, offs, st->bcnt.as<int>(),
-                st->bij.as<int2>(), st->nn.as<double2>(), st->bond.as<RbBond>(), st->nconj.as<double>());
-    const int ntot = (int)total;
-    int nblk = (ntot + 255) / 256;
-    if (nblk < 1) nblk = 1;
-    AGPU_CHECK(st->partial.reserve(sizeof(double) * 10 * nblk));
-    AGPU_CHECK(st->bo.reserve(sizeof(double4) * (total + 1)));
-    AGPU_CHECK(st->rev.reserve(sizeof(int) * (total + 1)));
-    AGPU_CHECK(st->half.reserve(sizeof(int) * (total + 1)));
-    AGPU_CHECK(st->hpos.reserve(sizeof(int) * (total + 1)));
-    AGPU_CHECK(st->list.reserve(sizeof(int) * (total + 2)));
-    int *count = st->list.as<int>() + total + 1;
-    if (ntot > 0) {
-        AGPU_LAUNCH("rebo2_bo", k_rb_bo, dim3(nblk), dim3(256), 0, s, ntot, D, st->tab.as<double>(), els, offs,
-                    st->bcnt.as<int>(), st->bown.as<int>(), st->bond.as<RbBond>(), st->bij.as<int2>(),
-                    st->nn.as<double2>(), st->bo.as<double4>(), st->rev.as<int>(), st->half.as<int>(), p->natloc,
-                    lammps ? st->tag_s.as<int>() : nullptr, order, dstats);
-        size_t tmp = 0;
-        AGPU_HIP(rocprim::exclusive_scan(nullptr, tmp, st->half.as<int>(), st->hpos.as<int>(), 0, (size_t)ntot,
-                                         rocprim::plus<int>(), s));
-        AGPU_CHECK(st->scan_tmp.reserve(tmp + 16));
-        AGPU_HIP(rocprim::exclusive_scan(st->scan_tmp.ptr, tmp, st->half.as<int>(), st->hpos.as<int>(), 0,
-                                         (size_t)ntot, rocprim::plus<int>(), s));
-        AGPU_LAUNCH("rebo2_list", k_rb_list, dim3(nblk), dim3(256), 0, s, ntot, st->half.as<int>(),
-                    st->hpos.as<int>(), st->list.as<int>(), count);
-    } else {
-        AGPU_HIP(hipMemsetAsync(count, 0, sizeof(int), s));
-    }
-    AGPU_LAUNCH("rebo2_pair", k_rb_pair, dim3(nblk), dim3(256), 0, s, ntot, st->list.as<int>(), count, D,
-                st->tab.as<double>(), els, offs, st->bcnt.as<int>(), st->bown.as<int>(),
-                st->bond.as<RbBond>(), st->bij.as<int2>(), st->nn.as<double2>(), st->nconj.as<double>(),
-                st->bo.as<double4>(), st->rev.as<int>(), st->facc.as<unsigned long long>(), eacc, wacc,
-                st->partial.as<double>(), dstats, bsrc, a->epot_per_bond, a->f_per_bond, a->wpot_per_bond);
+    int nblk = 0;
+    if (pot->precision == AGPU_PREC_MIXED)
+        AGPU_CHECK(rebo2_kernels<float>(st, pot, a, bsrc, eacc, wacc, &nblk));
+    else
+        AGPU_CHECK(rebo2_kernels<double>(st, pot, a, bsrc, eacc, wacc, &nblk));
     AGPU_LAUNCH("rebo2_finish", k_rb_finish, dim3(nb), dim3(256), 0, s, nat, order,
                 st->facc.as<unsigned long long>(), eacc, wacc, a->f, a->epot_per_at, a->wpot_per_at);
     AGPU_CHECK(reduce_ev(st->partial.as<double>(), nblk, 10, a->epot, a->wpot, nullptr));

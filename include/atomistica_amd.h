@@ -305,8 +305,10 @@ int agpu_pme_create(agpu_potential **pot, double cutoff, int order, const int *g
  * rebo2_db_init_with_parameters (rebo2_module.f90:30-42, rebo2_db.f90:81-303)
  * with the default tables of rebo2_default_tables.f90; energy_and_forces is
  * rebo2_kernel (bop_kernel_rebo2.f90), including the torsion term when
- * AGPU_REBO2_DIHEDRAL is set.  FP64 only; no mask (the reference kernel
- * takes none). */
+ * AGPU_REBO2_DIHEDRAL is set.  No mask (the reference kernel takes none).
+ * AGPU_PREC_MIXED keeps the bond records, bond orders, radial pair terms and
+ * the torsion in FP64 and runs the per-neighbour angular and conjugation
+ * loops in FP32 (force RMS within 1e-5 eV/A of FP64 on a-C:H). */
 int agpu_rebo2_create(agpu_potential **pot, int flags);
 
 int agpu_potential_destroy(agpu_potential *pot);

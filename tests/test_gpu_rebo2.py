@@ -96,9 +96,41 @@ def test_rebo2_bitwise_reproducible():
 def test_rebo2_unsupported_modes():
     pos, cell, pbc, Z = W.diamond_c(2, sigma=0.02, seed=1)
     with pytest.raises(AgpuError):
-        run_gpu(pos, cell, pbc, Z, precision="mixed")
-    with pytest.raises(AgpuError):
         run_gpu(pos, cell, pbc, Z, mask=np.ones(len(pos), np.int32))
+
+
+# (name, generator, force bar): a-C:H with 1e-5 eV/A absolute; the random
+# C/H packing, whose forces are ~100 eV/A, with 2e-7 of its force RMS
+MIXED_CASES = [
+    ("aCH_5000", lambda: W.amorphous_ch(5000, x_h=0.2, density=2.0, seed=10), None),
+    ("aC", lambda: read_cfg("aC.cfg"), None),
+    ("random_CH_large", lambda: W.random_ch(1500, 500, 2.2, seed=8, strain=0.15), 2e-7),
+]
+
+
+@pytest.mark.parametrize("name,make,rel", MIXED_CASES, ids=[c[0] for c in MIXED_CASES])
+def test_rebo2_mixed_error(name, make, rel):
+    """The mixed form (FP32 bond-record copies and FP32 arithmetic in the
+    per-neighbour angular and conjugation loops; FP64 bond records, bond
+    orders, radial pair terms and torsion; fixed-point accumulation) against
+    the oracle with torsion: force RMS <= 1e-5 eV/A on a-C:H (the bench
+    system's generator and the reference's a-C fixture), energy <= 1e-5
+    eV/atom, virial <= 1e-5 eV/atom per component, bitwise reproducible."""
+    pos, cell, pbc, Z = make()
+    n = len(pos)
+    print("rebo2 mixed case", name, n, flush=True)
+    e, f, w, epa, _, _, wpa, _ = run_gpu(pos, cell, pbc, Z, per_at=True, precision="mixed", dihedral=True)
+    re_, rf, rw, repa, rwpa = run_oracle(pos, cell, pbc, Z, per_at=True, dihedral=True)
+    rms = float(np.sqrt(np.mean((f - rf) ** 2)))
+    print("rebo2 mixed %s: F rms %.3e max %.3e (|F| rms %.3e)  |dE|/N %.3e  |dW|/N %.3e  |dEpa| max %.3e" %
+          (name, rms, np.abs(f - rf).max(), float(np.sqrt(np.mean(rf ** 2))), abs(e - re_) / n,
+           np.abs(w - rw).max() / n, np.abs(epa - repa).max()), flush=True)
+    assert rms <= (1e-5 if rel is None else rel * float(np.sqrt(np.mean(rf ** 2))))
+    assert abs(e - re_) / n <= 1e-5
+    assert np.abs(w - rw).max() / n <= 1e-5
+    assert np.abs(epa - repa).max() <= 1e-4
+    e2, f2, w2, *_ = run_gpu(pos, cell, pbc, Z, precision="mixed", dihedral=True)
+    assert e2 == e and np.array_equal(f2, f) and np.array_equal(w2, w)
 
 
 def test_rebo2_full_size_diamond():
