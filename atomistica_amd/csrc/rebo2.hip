@@ -125,7 +125,8 @@ struct Rebo2State {
     RbDev hdev{};
     std::vector<double> htab;
     DevBuf dev, tab, bond, bij, bown, bcnt, nn, nconj, facc, eacc, wacc, partial, stats, bo, rev, half, hpos,
-        list, scan_tmp, bsrc, cnt_s, offs_s, el_s, tag_s, dev32, tab32, bond32;
+        list, scan_tmp, bsrc, cnt_s, offs_s, el_s, tag_s, dev32, tab32, bond32, coef, tor, wdir, fnb, fcj,
+        bconj, own;
     int z2db[128];
     uint64_t Z_rev = ~0ull;
     double cutoff = 0.0;
@@ -1010,11 +1011,14 @@ k_rb_bo(int ntot, const RbDevT<R> *__restrict__ Dp, const R *__restrict__ tab, c
         const int *__restrict__ offsets, const int *__restrict__ bcnt, const int *__restrict__ bown,
         const RbBondT<R> *__restrict__ bond, const int2 *__restrict__ bij, const double2 *__restrict__ nn,
         Rb4<R> *__restrict__ bo, int *__restrict__ rev, int *__restrict__ half, int natloc,
-        const int *__restrict__ tag, const int *__restrict__ order, int *__restrict__ flags)
+        const int *__restrict__ tag, const int *__restrict__ order, int *__restrict__ flags,
+        double4 *__restrict__ coef)
 {
     const RbDevT<R> &D = *Dp;
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= ntot) return;
+    // neighbour-pass coefficients: zero (.w = 0) until a half pair sets them
+    coef[s] = make_double4(0.0, 0.0, 0.0, 0.0);
     const int i = bown[s];
     if (i < 0) {
         half[s] = 0;
@@ -1077,21 +1081,56 @@ __global__ void k_rb_list(int ntot, const int *__restrict__ half, const int *__r
     if (s == ntot - 1) *count = pos[s] + half[s];
 }
 
+// [e, W(9)] of a thread reduced over the block into partial[blk]
+__device__ __forceinline__ void rb_block_record(double e, const double *w, double *__restrict__ partial, int blk)
+{
+    __shared__ double s_red[4][10];
+    double red[10];
+    red[0] = e;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) red[1 + q] = w[q];
+#pragma unroll
+    for (int q = 0; q < 10; ++q) red[q] = group_sum<64>(red[q]);
+    const int tid = threadIdx.x, wave = tid / 64;
+    if ((tid & 63) == 0)
+        for (int q = 0; q < 10; ++q) s_red[wave][q] = red[q];
+    __syncthreads();
+    if (tid < 10)
+        partial[(size_t)blk * 10 + tid] = ((s_red[0][tid] + s_red[1][tid]) + s_red[2][tid]) + s_red[3][tid];
+}
+
+// the FP64 radial pair terms of a half pair for a bond-order average bave:
+// energy 2 hlf, force df on i (-df on j), virial r_ij (x) df
+__device__ __forceinline__ void rb_radial(const RbBond &B, double fa, double dfa, double fr, double dfr,
+                                          double bave, double &hlf, double *df)
+{
+    hlf = B.fc * (fr + bave * fa) / 2;
+    const double dffac = dfr * B.fc + bave * dfa * B.fc + fr * B.dfc + bave * fa * B.dfc;
+#pragma unroll
+    for (int x = 0; x < 3; ++x) df[x] = dffac * B.n[x];
+}
+
+// Pass 1, thread per half pair i->j (ij_loop of bop_kernel_rebo2.f90:1326-
+// 2779 without the neighbour loops): N^conj of j, F(N_i, N_j, N^conj) and
+// T(N_i, N_j, N^conj); the pair energy and radial force with
+// b = (b_ij + b_ji + F) / 2 (the torsion part T sum follows in pass 2); the
+// neighbour-loop coefficients of both directed bonds into coef[slot] =
+// {dE/dN_c, dE/dN^conj_c, N^conj_c}; the torsion table values into tor[t].
+// The mixed form takes R = float for the tables and coefficients and the
+// FP64 records (bond64) and constants (D64) for the radial terms.
 template <typename R>
 __global__ void __launch_bounds__(256)
-k_rb_pair(int ntot, const int *__restrict__ list, const int *__restrict__ count, const RbDevT<R> *__restrict__ Dp,
+k_rb_pair(const int *__restrict__ list, const int *__restrict__ count, const RbDevT<R> *__restrict__ Dp,
           const R *__restrict__ tab, const int *__restrict__ el, const int *__restrict__ offsets,
           const int *__restrict__ bcnt, const int *__restrict__ bown, const RbBondT<R> *__restrict__ bond,
           const int2 *__restrict__ bij, const double2 *__restrict__ nn, const double *__restrict__ nconjit,
-          const Rb4<double> *__restrict__ bo, const int *__restrict__ rev,
-          unsigned long long *__restrict__ facc, unsigned long long *__restrict__ eacc,
+          const Rb4<double> *__restrict__ bo, const int *__restrict__ rev, double4 *__restrict__ coef,
+          Rb4<R> *__restrict__ tor, unsigned long long *__restrict__ facc, unsigned long long *__restrict__ eacc,
           unsigned long long *__restrict__ wacc, double *__restrict__ partial, int *__restrict__ flags,
           const int *__restrict__ bsrc, double *__restrict__ epb, double *__restrict__ fpb,
           double *__restrict__ wpb, const RbDev *__restrict__ D64p, const RbBond *__restrict__ bond64)
 {
-    __shared__ double s_red[4][10];
     const RbDevT<R> &D = *Dp;
-    constexpr bool F32 = sizeof(R) == 4;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     int bad = 0;
     double epair = 0.0;
@@ -1101,16 +1140,12 @@ k_rb_pair(int ntot, const int *__restrict__ list, const int *__restrict__ count,
     if (t < *count) {
         const int s = list[t];
         const int i = bown[s];
-        const int2 eij = bij[s];
-        const int j = eij.x;
+        const int j = bij[s].x;
         const int sji = rev[s];
         const int ki = el[i], kj = el[j];
         const RbBondT<R> Bij = bond[s];
         const int ijpot = rb_pair(ki, kj);
-        const R rlij = Bij.len;
-        const R rlijr = R(1.0) / rlij;
-        const R *rnij = Bij.n;
-        const R fcar = Bij.fc, dfcar = Bij.dfc;
+        const R fcar = Bij.fc;
         const double2 nni = nn[i], nnj = nn[j];
         R niC = nni.x, niH = nni.y, njC = nnj.x, njH = nnj.y;
         if (kj == RB_C) niC = niC - Bij.fc; else niH = niH - Bij.fc;
@@ -1119,56 +1154,23 @@ k_rb_pair(int ntot, const int *__restrict__ list, const int *__restrict__ count,
         niH = niH > R(4.0) ? R(4.0) : niH;
         njC = njC > R(4.0) ? R(4.0) : njC;
         njH = njH > R(4.0) ? R(4.0) : njH;
-        // N_i, N_j as g sees them; F sees them clamped to 3 below
-        const R nti_g = niC + niH, ntj_g = njC + njH;
-        R nti = nti_g, ntj = ntj_g;
-        R fa, dfa, fr, dfr;
-        rb_va(D, ijpot, rlij, fa, dfa);
-        rb_vr(D, ijpot, rlij, fr, dfr);
-        const int ib = offsets[i], inb = bcnt[i];
+        // F sees N_i, N_j clamped to 3
+        R nti = niC + niH, ntj = njC + njH;
+        R fa, dfa;
+        rb_va(D, ijpot, Bij.len, fa, dfa);
         const int jb = offsets[j], jnb = bcnt[j];
-        // bond orders in FP64 in both forms
-        const Rb4<double> boi = bo[s], boj = bo[sji];
-        const double bi = boi.x, bj = boj.x;
-        const R dfbi = (R)boi.y, dpdnci = (R)boi.z, dpdnhi = (R)boi.w;
-        const R dfbj = (R)boj.y, dpdncj = (R)boj.z, dpdnhj = (R)boj.w;
+        const double bi = bo[s].x, bj = bo[sji].x;
         const R nconji = nconjit[i] - Bij.fc * Bij.fx;
         R nconjj = R(0.0);
         for (int b = jb; b < jb + jnb; ++b)
             if (b != sji && el[bij[b].x] == RB_C) nconjj = nconjj + bond[b].fc * bond[b].fx;
-        // F(N_i, N_j, N_conj) and the torsion sum
         R nconj = nconji * nconji + nconjj * nconjj;
         nconj = nconj > R(8.0) ? R(8.0) : nconj;
         nti = nti > R(3.0) ? R(3.0) : nti;
         ntj = ntj > R(3.0) ? R(3.0) : ntj;
-        double fi[3] = {0, 0, 0}, fj[3] = {0, 0, 0};
-        R bdh = R(0.0), tij = R(0.0), dtdni = R(0.0), dtdnj = R(0.0), dtdncn = R(0.0);
-        // the mixed form evaluates the pair's radial terms and the torsion
-        // on the FP64 records (B64): the torsion's 1 / sin^2 factors and the
-        // large V_R' + b V_A' cancellation do not survive FP32 bond vectors
-        RbBond B64;
-        if constexpr (F32) {
-            B64 = bond64[s];
-        } else {
-            B64 = Bij;
-        }
-        if (D.with_dihedral && ijpot == RB_CC) {
-            rb_table3(tab + RB_TAB_TCC, nti, ntj, nconj, tij, dtdni, dtdnj, dtdncn);
-            if (tij != 0) {
-                if constexpr (F32) {
-                    const RbDev &D64 = *D64p;
-                    double fa64, dfa64;
-                    rb_va<double>(D64, ijpot, B64.len, fa64, dfa64);
-                    auto at64 = [&](int b, int) { return bond64[b]; };
-                    bdh = (R)rb_torsion<double>(ib, inb, s, jb, jnb, sji, eij.y, at64, bij, B64.n, B64.len,
-                                                (double)tij, fa64, B64.fc, fi, fj, wij, facc, bad);
-                } else {
-                    auto at = [&](int b, int) { return bond[b]; };
-                    bdh = rb_torsion<R>(ib, inb, s, jb, jnb, sji, eij.y, at, bij, rnij, rlij, tij, fa, fcar, fi,
-                                        fj, wij, facc, bad);
-                }
-            }
-        }
+        Rb4<R> tv{R(0.0), R(0.0), R(0.0), R(0.0)};
+        if (D.with_dihedral && ijpot == RB_CC) rb_table3(tab + RB_TAB_TCC, nti, ntj, nconj, tv.x, tv.y, tv.z, tv.w);
+        if (tor) tor[t] = tv;
         R fij = R(0.0), dfdni = R(0.0), dfdnj = R(0.0), dfdncn = R(0.0);
         if (ijpot == RB_CC)
             rb_table3(tab + RB_TAB_FCC, nti, ntj, nconj, fij, dfdni, dfdnj, dfdncn);
@@ -1178,46 +1180,27 @@ k_rb_pair(int ntot, const int *__restrict__ list, const int *__restrict__ count,
             rb_table3(tab + RB_TAB_FCH, ntj, nti, nconj, fij, dfdnj, dfdni, dfdncn);
         else
             rb_table3(tab + RB_TAB_FCH, nti, ntj, nconj, fij, dfdni, dfdnj, dfdncn);
-        dfdni = dfdni + dtdni * bdh;
-        dfdnj = dfdnj + dtdnj * bdh;
-        dfdncn = dfdncn + dtdncn * bdh;
-        dfdni = R(0.5) * fcar * fa * dfdni;
-        dfdnj = R(0.5) * fcar * fa * dfdnj;
-        dfdncn = R(0.5) * fcar * fa * dfdncn;
-        const R dfdncni = 2 * dfdncn * nconji;
-        const R dfdncnj = 2 * dfdncn * nconjj;
-        // neighbours of i and of j: coordination, conjugation and bond-order forces
-        rb_neighbours(D, i, ki, ijpot, rlij, rlijr, rnij, R(1.0), ib, inb, s, bond, bij, offsets, bcnt, el, nti_g,
-                      dfdni, dfdncni, dfbi, dpdnci, dpdnhi, fi, fj, wij, facc, bad);
-        rb_neighbours(D, j, kj, ijpot, rlij, rlijr, rnij, -R(1.0), jb, jnb, sji, bond, bij, offsets, bcnt, el, ntj_g,
-                      dfdnj, dfdncnj, dfbj, dpdncj, dpdnhj, fj, fi, wij, facc, bad);
-        // pair energy and forces (FP64 radial terms in the mixed form)
-        double fa2 = fa, dfa2 = dfa, fr2 = fr, dfr2 = dfr;
-        if constexpr (F32) {
-            rb_va<double>(*D64p, ijpot, B64.len, fa2, dfa2);
-            rb_vr<double>(*D64p, ijpot, B64.len, fr2, dfr2);
-        }
-        const double fcar2 = B64.fc, dfcar2 = B64.dfc;
-        const double bave = 0.5 * ((bi + bj + (double)fij) + (double)tij * (double)bdh);
-        const double hlf = fcar2 * (fr2 + bave * fa2) / 2;
+        const R h = R(0.5) * fcar * fa;
+        dfdncn = h * dfdncn;
+        coef[s] = make_double4(h * dfdni, 2 * dfdncn * nconji, nconji, 1.0);
+        coef[sji] = make_double4(h * dfdnj, 2 * dfdncn * nconjj, nconjj, 1.0);
+        // pair energy and radial force (FP64 terms in the mixed form)
+        const RbBond B64 = bond64[s];
+        double fa2, dfa2, fr2, dfr2, hlf, df[3];
+        rb_va<double>(*D64p, ijpot, B64.len, fa2, dfa2);
+        rb_vr<double>(*D64p, ijpot, B64.len, fr2, dfr2);
+        rb_radial(B64, fa2, dfa2, fr2, dfr2, 0.5 * (bi + bj + (double)fij), hlf, df);
         epair = hlf + hlf;
-        const double dffac = dfr2 * fcar2 + bave * dfa2 * fcar2 + fr2 * dfcar2 + bave * fa2 * dfcar2;
-        double df[3];
-#pragma unroll
-        for (int x = 0; x < 3; ++x) {
-            df[x] = dffac * B64.n[x];
-            fi[x] += df[x];
-            fj[x] -= df[x];
-        }
 #pragma unroll
         for (int y = 0; y < 3; ++y)
 #pragma unroll
-            for (int x = 0; x < 3; ++x) wij[x + 3 * y] = wij[x + 3 * y] + (B64.len * B64.n[x]) * df[y];
-        fx_add3(facc, i, fi, bad);
-        fx_add3(facc, j, fj, bad);
+            for (int x = 0; x < 3; ++x) wij[x + 3 * y] = (B64.len * B64.n[x]) * df[y];
+        const double mdf[3] = {-df[0], -df[1], -df[2]};
+        fx_add3(facc, i, df, bad);
+        fx_add3(facc, j, mdf, bad);
         // per-bond properties on the list entry of the half pair's bond i->j
         // (bop_kernel_rebo2.f90:2546-2547, 2571-2572, 2762-2763); one thread
-        // owns each half pair, so the updates do not collide
+        // owns each half pair in each pass
         if (bsrc) {
             const size_t e = (size_t)bsrc[s];
             if (epb) epb[e] += epair;
@@ -1238,20 +1221,388 @@ k_rb_pair(int ntot, const int *__restrict__ list, const int *__restrict__ count,
         }
     }
     if (bad) atomicOr(flags, ((bad & 1) ? DERR_FIXED_RANGE : 0) | ((bad & 2) ? DERR_BOND_OVERFLOW : 0));
-    // block reduction of [e, W(9)]
-    double red[10];
-    red[0] = epair;
+    rb_block_record(epair, wij, partial, blockIdx.x);
+}
+
+// Pass 2, thread per half pair with T != 0 (C-C with torsion): the torsion
+// sum over k, l with its forces (bop_kernel_rebo2.f90:1939-2120, always on
+// the FP64 records), its share T sum / 2 of the bond order in the pair
+// energy and radial force, and its terms dT/dN sum, dT/dN^conj sum added to
+// both directed bonds' coefficients before the neighbour pass.
+template <typename R>
+__global__ void __launch_bounds__(256)
+k_rb_torsion(const int *__restrict__ list, const int *__restrict__ count, const int *__restrict__ offsets,
+             const int *__restrict__ bcnt, const int *__restrict__ bown, const int2 *__restrict__ bij,
+             const int *__restrict__ rev, const Rb4<R> *__restrict__ tor, double4 *__restrict__ coef,
+             unsigned long long *__restrict__ facc, unsigned long long *__restrict__ eacc,
+             unsigned long long *__restrict__ wacc, double *__restrict__ partial, int *__restrict__ flags,
+             const int *__restrict__ bsrc, double *__restrict__ epb, double *__restrict__ fpb,
+             double *__restrict__ wpb, const RbDev *__restrict__ D64p, const RbBond *__restrict__ bond64)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int bad = 0;
+    double epair = 0.0;
+    double wij[9];
 #pragma unroll
-    for (int q = 0; q < 9; ++q) red[1 + q] = wij[q];
+    for (int q = 0; q < 9; ++q) wij[q] = 0.0;
+    Rb4<R> tv{R(0.0), R(0.0), R(0.0), R(0.0)};
+    if (tor && t < *count) tv = tor[t];
+    if (tv.x != R(0.0)) {
+        const int s = list[t];
+        const int i = bown[s];
+        const int2 eij = bij[s];
+        const int j = eij.x;
+        const int sji = rev[s];
+        const RbBond B = bond64[s];
+        double fa, dfa, fr, dfr;
+        rb_va<double>(*D64p, RB_CC, B.len, fa, dfa);
+        auto at = [&](int b, int) { return bond64[b]; };
+        double fi[3] = {0, 0, 0}, fj[3] = {0, 0, 0};
+        const double tij = (double)tv.x;
+        const double bdh = rb_torsion<double>(offsets[i], bcnt[i], s, offsets[j], bcnt[j], sji, eij.y, at, bij, B.n,
+                                              B.len, tij, fa, B.fc, fi, fj, wij, facc, bad);
+        // coefficients of the neighbour pass
+        const double h = 0.5 * B.fc * fa * bdh;
+        const double4 ci = coef[s], cj = coef[sji];
+        const double dn = h * (double)tv.w;
+        coef[s] = make_double4(ci.x + h * (double)tv.y, ci.y + 2 * dn * ci.z, ci.z, 1.0);
+        coef[sji] = make_double4(cj.x + h * (double)tv.z, cj.y + 2 * dn * cj.z, cj.z, 1.0);
+        // the pair energy's torsion share: b += T sum / 2, no repulsion
+        double hlf, df[3];
+        rb_radial(B, fa, dfa, 0.0, 0.0, 0.5 * tij * bdh, hlf, df);
+        epair = hlf + hlf;
 #pragma unroll
-    for (int q = 0; q < 10; ++q) red[q] = group_sum<64>(red[q]);
-    const int tid = threadIdx.x, wave = tid / 64;
-    if ((tid & 63) == 0)
-        for (int q = 0; q < 10; ++q) s_red[wave][q] = red[q];
-    __syncthreads();
-    if (tid < 10)
-        partial[(size_t)blockIdx.x * 10 + tid] =
-            ((s_red[0][tid] + s_red[1][tid]) + s_red[2][tid]) + s_red[3][tid];
+        for (int x = 0; x < 3; ++x) {
+            fi[x] += df[x];
+            fj[x] -= df[x];
+        }
+        double wt[9];
+#pragma unroll
+        for (int y = 0; y < 3; ++y)
+#pragma unroll
+            for (int x = 0; x < 3; ++x) {
+                wt[x + 3 * y] = (B.len * B.n[x]) * df[y];
+                wij[x + 3 * y] += wt[x + 3 * y];
+            }
+        fx_add3(facc, i, fi, bad);
+        fx_add3(facc, j, fj, bad);
+        if (bsrc) {
+            const size_t e = (size_t)bsrc[s];
+            if (epb) epb[e] += epair;
+            if (fpb)
+                for (int x = 0; x < 3; ++x) fpb[3 * e + x] += df[x];
+            if (wpb)
+                for (int q = 0; q < 9; ++q) wpb[9 * e + q] += wij[q];
+        }
+        if (eacc) {
+            fx_add(eacc + i, hlf, bad);
+            fx_add(eacc + j, hlf, bad);
+        }
+        if (wacc) {
+            for (int q = 0; q < 9; ++q) {
+                fx_add(wacc + 9 * (size_t)i + q, wij[q] / 2, bad);
+                fx_add(wacc + 9 * (size_t)j + q, wij[q] / 2, bad);
+            }
+        }
+    }
+    if (bad) atomicOr(flags, ((bad & 1) ? DERR_FIXED_RANGE : 0) | ((bad & 2) ? DERR_BOND_OVERFLOW : 0));
+    rb_block_record(epair, wij, partial, blockIdx.x);
+}
+
+// Pass 3, thread per directed bond c->o of a half pair (two threads per
+// half pair: i->j, then j->i): the coordination, conjugation and bond-order
+// forces of c's other neighbours k and their neighbours m
+// (bop_kernel_rebo2.f90:2405-2520 and the dbidk/dbjdl parts of 1407-1920),
+// with the coefficients of passes 1 and 2.  wdir (optional, for the per-bond
+// virial) receives the virial of each directed bond.
+template <typename R>
+__global__ void __launch_bounds__(256)
+k_rb_nbrs(const int *__restrict__ list, const int *__restrict__ count, const RbDevT<R> *__restrict__ Dp,
+          const int *__restrict__ el, const int *__restrict__ offsets, const int *__restrict__ bcnt,
+          const int *__restrict__ bown, const RbBondT<R> *__restrict__ bond, const int2 *__restrict__ bij,
+          const double2 *__restrict__ nn, const Rb4<double> *__restrict__ bo, const int *__restrict__ rev,
+          const double4 *__restrict__ coef, unsigned long long *__restrict__ facc,
+          unsigned long long *__restrict__ wacc, double *__restrict__ partial, int *__restrict__ flags,
+          double *__restrict__ wdir)
+{
+    const RbDevT<R> &D = *Dp;
+    const int t2 = blockIdx.x * blockDim.x + threadIdx.x;
+    int bad = 0;
+    double w[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) w[q] = 0.0;
+    if (t2 < 2 * *count) {
+        const int s0 = list[t2 >> 1];
+        const int s = (t2 & 1) ? rev[s0] : s0;
+        const int c = bown[s];
+        const int o = bij[s].x;
+        const int kc = el[c], ko = el[o];
+        const RbBondT<R> B = bond[s];
+        const int ijpot = rb_pair(kc, ko);
+        const double2 nnc = nn[c];
+        R nC = nnc.x, nH = nnc.y;
+        if (ko == RB_C) nC = nC - B.fc; else nH = nH - B.fc;
+        nC = nC > R(4.0) ? R(4.0) : nC;
+        nH = nH > R(4.0) ? R(4.0) : nH;
+        const double4 cf = coef[s];
+        const Rb4<double> bs = bo[s];
+        double fc3[3] = {0, 0, 0}, fo3[3] = {0, 0, 0};
+        rb_neighbours(D, c, kc, ijpot, B.len, R(1.0) / B.len, B.n, R(1.0), offsets[c], bcnt[c], s, bond, bij,
+                      offsets, bcnt, el, nC + nH, (R)cf.x, (R)cf.y, (R)bs.y, (R)bs.z, (R)bs.w, fc3, fo3, w, facc,
+                      bad);
+        fx_add3(facc, c, fc3, bad);
+        fx_add3(facc, o, fo3, bad);
+        if (wacc) {
+            for (int q = 0; q < 9; ++q) {
+                fx_add(wacc + 9 * (size_t)c + q, w[q] / 2, bad);
+                fx_add(wacc + 9 * (size_t)o + q, w[q] / 2, bad);
+            }
+        }
+        if (wdir)
+            for (int q = 0; q < 9; ++q) wdir[9 * (size_t)t2 + q] = w[q];
+    }
+    if (bad) atomicOr(flags, ((bad & 1) ? DERR_FIXED_RANGE : 0) | ((bad & 2) ? DERR_BOND_OVERFLOW : 0));
+    rb_block_record(0.0, w, partial, blockIdx.x);
+}
+
+// Pass 3 without atomics (no per-atom or per-bond virial requested): a
+// group of G lanes per centre slot c, lane = bond c->x.  The coordination
+// and conjugation terms of every directed bond s = c->o act on the other
+// neighbours k through scalars of s and the bond c->k only, so they are sums
+// over the centre's bonds:
+//   F_k = -dfc_ck n_ck [sum_{s != ck} (dE/dN_s + dfb_s dP/dN_type(k),s)
+//                       + [k = C] F(x_ck) sum_{s != ck} dE/dN^conj_s]
+// and the second-neighbour conjugation forces through k reduce to the bond
+// scalar B_ck = [k = C] fc_ck dF(x_ck) sum_{s != ck} dE/dN^conj_s, applied in
+// k_rb_conj_force.  The bond-order angular terms of each pair (s, k) are
+// evaluated per lane s with the centre's bonds k walked in turn, the force on
+// k summed over the group into lane k.  Per bond c->x the force on x goes to
+// fnb[slot]; the centre's own force to own[c]; forces are gathered by the
+// atoms afterwards (k_rb_gather), so the pass is free of atomics and its
+// result does not depend on scheduling.
+template <typename R, int G>
+__global__ void __launch_bounds__(256)
+k_rb_centre(int nat, const RbDevT<R> *__restrict__ Dp, const int *__restrict__ el, const int *__restrict__ offsets,
+            const int *__restrict__ bcnt, const RbBondT<R> *__restrict__ bond, const int2 *__restrict__ bij,
+            const double2 *__restrict__ nn, const Rb4<double> *__restrict__ bo, const double4 *__restrict__ coef,
+            double *__restrict__ fnb, double *__restrict__ own, double *__restrict__ bconj,
+            double *__restrict__ partial)
+{
+    const RbDevT<R> &D = *Dp;
+    const int l = threadIdx.x % G;
+    const int c = blockIdx.x * (256 / G) + threadIdx.x / G;
+    const bool cv = c < nat;
+    const int nb = cv ? bcnt[c] : 0;
+    const int beg = cv ? offsets[c] : 0;
+    const int kc = cv ? el[c] : 0;
+    const bool has = l < nb;
+    const int b = beg + l;
+    RbBondT<R> B{};
+    int kx = 0;
+    double dfdn = 0.0, dfdncn = 0.0, dfb = 0.0, pc = 0.0, ph = 0.0;
+    bool sv = false;      // the bond is a directed bond of a half pair (coef set)
+    if (has) {
+        B = bond[b];
+        kx = el[bij[b].x];
+        const double4 cf = coef[b];
+        sv = cf.w != 0.0;
+        if (sv) {
+            const Rb4<double> bs = bo[b];
+            dfdn = cf.x;
+            dfdncn = cf.y;
+            dfb = bs.y;
+            pc = bs.y * bs.z;
+            ph = bs.y * bs.w;
+        }
+    }
+    const double S1 = group_sum<G>(dfdn), S2c = group_sum<G>(pc), S2h = group_sum<G>(ph);
+    const double S3 = group_sum<G>(dfdncn);
+    double fo[3] = {0, 0, 0}, fcen[3] = {0, 0, 0}, w[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) w[q] = 0.0;
+    // coordination / conjugation / P on this lane's neighbour x
+    if (has) {
+        const bool xC = kx == RB_C;
+        const double A = (S1 - dfdn) + (xC ? (S2c - pc) + (double)B.fx * (S3 - dfdncn) : (S2h - ph));
+#pragma unroll
+        for (int x = 0; x < 3; ++x) {
+            const double fk = -A * (double)(B.dfc * B.n[x]);
+            fo[x] = fk;
+            fcen[x] = -fk;
+        }
+#pragma unroll
+        for (int y = 0; y < 3; ++y)
+#pragma unroll
+            for (int x = 0; x < 3; ++x) w[x + 3 * y] -= (double)(B.len * B.n[x]) * fo[y];
+        bconj[b] = xC ? (double)(B.fc * B.dfx) * (S3 - dfdncn) : 0.0;
+    }
+    // bond-order angular terms: this lane is s = c->o, k walks the bonds
+    int xmax = nb;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) xmax = max(xmax, __shfl_xor(xmax, o, 64));
+    R nt = R(0.0);
+    int ijpot = 0;
+    if (has) {
+        const double2 nnc = nn[c];
+        R nC = nnc.x, nH = nnc.y;
+        if (kx == RB_C) nC = nC - B.fc; else nH = nH - B.fc;
+        nC = nC > R(4.0) ? R(4.0) : nC;
+        nH = nH > R(4.0) ? R(4.0) : nH;
+        nt = nC + nH;
+        ijpot = rb_pair(kc, kx);
+    }
+    const R rlij = B.len, rlijr = has ? R(1.0) / B.len : R(0.0);
+    const R dfbR = (R)dfb;
+    double fkx[3] = {0, 0, 0};
+    for (int k = 0; k < xmax; ++k) {
+        double dbk3[3] = {0, 0, 0};
+        if (sv && k < nb && k != l) {
+            const RbBondT<R> K = bond[beg + k];
+            const int ikpot = rb_pair(kc, el[bij[beg + k].x]);
+            const R rlik = K.len;
+            if (rlik < D.cut_h[ikpot]) {
+                R qf, qd, gf, gd, dgdn;
+                rb_h(D, ijpot, ikpot, rlij - rlik, qf, qd);
+                R costh = R(0.0);
+                costh = costh + K.n[0] * B.n[0];
+                costh = costh + K.n[1] * B.n[1];
+                costh = costh + K.n[2] * B.n[2];
+                rb_g(D, kc, costh, nt, gf, gd, dgdn);
+                R dk[3], dd = R(0.0);
+#pragma unroll
+                for (int x = 0; x < 3; ++x) dk[x] = K.n[x] * rlik - B.n[x] * rlij;
+                dd = dd + dk[0] * dk[0];
+                dd = dd + dk[1] * dk[1];
+                dd = dd + dk[2] * dk[2];
+                const R dis = sqrt(dd);
+#pragma unroll
+                for (int x = 0; x < 3; ++x) dk[x] = dk[x] / dis;
+                const R dcs_io = R(1.0) / rlik - costh * rlijr;
+                const R dcs_ik = rlijr - costh / rlik;
+                const R dcs_ok = -dis * rlijr / rlik;
+                const R dzfac = K.fc * gd * qf;
+                const R dzdro = gf * K.fc * qd;
+                const R dzdrk = gf * (K.dfc * qf - K.fc * qd);
+                R dfo[3], dbk[3];
+#pragma unroll
+                for (int x = 0; x < 3; ++x) {
+                    const R u = B.n[x];
+                    const R dcs_c = -dcs_io * u - dcs_ik * K.n[x];
+                    const R dcs_o = dcs_io * u - dcs_ok * dk[x];
+                    const R dcs_k = dcs_ik * K.n[x] + dcs_ok * dk[x];
+                    fcen[x] += (double)(-dfbR * (-dzdro * u - dzdrk * K.n[x] + dzfac * dcs_c));
+                    dfo[x] = dzdro * u + dzfac * dcs_o;
+                    fo[x] += (double)(-dfbR * dfo[x]);
+                    dbk[x] = dzdrk * K.n[x] + dzfac * dcs_k;
+                    dbk3[x] = (double)(-dfbR * dbk[x]);
+                }
+                const R rco[3] = {rlij * B.n[0], rlij * B.n[1], rlij * B.n[2]};
+                const R rik[3] = {rlik * K.n[0], rlik * K.n[1], rlik * K.n[2]};
+#pragma unroll
+                for (int y = 0; y < 3; ++y)
+#pragma unroll
+                    for (int x = 0; x < 3; ++x)
+                        w[x + 3 * y] += dfb * (double)(rco[x] * dfo[y] + rik[x] * dbk[y]);
+            }
+        }
+        // the force on k, summed over the lanes s of the group
+#pragma unroll
+        for (int x = 0; x < 3; ++x) {
+            const double v = group_sum<G>(dbk3[x]);
+            if (l == k) fkx[x] = v;
+        }
+    }
+    if (has)
+        for (int x = 0; x < 3; ++x) fnb[3 * (size_t)b + x] = fo[x] + fkx[x];
+#pragma unroll
+    for (int x = 0; x < 3; ++x) fcen[x] = group_sum<G>(fcen[x]);
+    if (cv && l == 0)
+        for (int x = 0; x < 3; ++x) own[3 * (size_t)c + x] = fcen[x];
+    rb_block_record(0.0, w, partial, blockIdx.x);
+}
+
+// The second-neighbour conjugation forces: a group of G lanes per atom k,
+// lane = bond k->m; with B_ck from k_rb_centre on the bonds c->k (the
+// reverse slots), the force on m is -(sum_c B_ck - B_mk) dfc_km n_km (the
+// centre c = m itself excluded, bop_kernel_rebo2.f90:2405-2520), its
+// opposite on k.
+template <typename R, int G>
+__global__ void __launch_bounds__(256)
+k_rb_conj_force(int nat, const int *__restrict__ offsets, const int *__restrict__ bcnt,
+                const RbBondT<R> *__restrict__ bond, const int *__restrict__ rev, const double *__restrict__ bconj,
+                double *__restrict__ fcj, double *__restrict__ own2, double *__restrict__ partial)
+{
+    const int l = threadIdx.x % G;
+    const int k = blockIdx.x * (256 / G) + threadIdx.x / G;
+    const bool cv = k < nat;
+    const int nb = cv ? bcnt[k] : 0;
+    const bool has = l < nb;
+    const int b = (cv ? offsets[k] : 0) + l;
+    const double Bv = has ? bconj[rev[b]] : 0.0;
+    const double Bs = group_sum<G>(Bv);
+    double w[9], fk[3] = {0, 0, 0};
+#pragma unroll
+    for (int q = 0; q < 9; ++q) w[q] = 0.0;
+    if (has) {
+        const RbBondT<R> M = bond[b];
+        const double a = Bs - Bv;
+        double d2[3];
+#pragma unroll
+        for (int x = 0; x < 3; ++x) {
+            d2[x] = -a * (double)(M.dfc * M.n[x]);
+            fk[x] = -d2[x];
+            fcj[3 * (size_t)b + x] = d2[x];
+        }
+#pragma unroll
+        for (int y = 0; y < 3; ++y)
+#pragma unroll
+            for (int x = 0; x < 3; ++x) w[x + 3 * y] -= (double)(M.len * M.n[x]) * d2[y];
+    }
+#pragma unroll
+    for (int x = 0; x < 3; ++x) fk[x] = group_sum<G>(fk[x]);
+    if (cv && l == 0)
+        for (int x = 0; x < 3; ++x) own2[3 * (size_t)k + x] = fk[x];
+    rb_block_record(0.0, w, partial, blockIdx.x);
+}
+
+// f(atom order[p]) += own + own2 + the forces its neighbours' groups left on
+// the bonds towards p (reverse slots) + the fixed-point pair and torsion sums
+__global__ void k_rb_gather(int nat, const int *__restrict__ order, const int *__restrict__ offsets,
+                            const int *__restrict__ bcnt, const int *__restrict__ rev, const double *__restrict__ own,
+                            const double *__restrict__ own2, const double *__restrict__ fnb,
+                            const double *__restrict__ fcj, unsigned long long *__restrict__ facc,
+                            unsigned long long *__restrict__ eacc, double *__restrict__ f,
+                            double *__restrict__ epa)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nat) return;
+    double a[3];
+    for (int x = 0; x < 3; ++x) a[x] = own[3 * (size_t)p + x] + own2[3 * (size_t)p + x];
+    const int beg = offsets[p], nb = bcnt[p];
+    for (int b = beg; b < beg + nb; ++b) {
+        const size_t r = (size_t)rev[b];
+        for (int x = 0; x < 3; ++x) a[x] += fnb[3 * r + x] + fcj[3 * r + x];
+    }
+    const size_t i = (size_t)order[p];
+    for (int x = 0; x < 3; ++x) {
+        f[3 * i + x] += a[x] + (double)(long long)facc[3 * (size_t)p + x] * RB_FX_INV;
+        facc[3 * (size_t)p + x] = 0ull;
+    }
+    if (epa) {
+        epa[i] += (double)(long long)eacc[p] * RB_FX_INV;
+        eacc[p] = 0ull;
+    }
+}
+
+// per-bond virial: the two directed bonds' neighbour-pass virials of each
+// half pair added to its list entry, in a fixed order
+__global__ void k_rb_wpb(const int *__restrict__ list, const int *__restrict__ count, const int *__restrict__ bsrc,
+                         const double *__restrict__ wdir, double *__restrict__ wpb)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= *count) return;
+    const size_t e = (size_t)bsrc[list[t]];
+    for (int q = 0; q < 9; ++q) wpb[9 * e + q] += wdir[9 * (size_t)(2 * t) + q] + wdir[9 * (size_t)(2 * t + 1) + q];
 }
 
 __global__ void k_rb_finish(int nat, const int *__restrict__ order, unsigned long long *__restrict__ facc,
@@ -1326,7 +1677,7 @@ int rebo2_bind(agpu_potential *pot)
 // arrays and accumulators are ready
 template <typename R>
 static int rebo2_kernels(Rebo2State *st, agpu_potential *pot, const agpu_eval_args *a, const int *bsrc,
-                         unsigned long long *eacc, unsigned long long *wacc, int *nblk_out)
+                         unsigned long long *eacc, unsigned long long *wacc, int *nblk_out, bool *gathered)
 {
     agpu_particles *p = pot->p;
     agpu_neighbors *nl = pot->nl;
@@ -1364,7 +1715,19 @@ static int rebo2_kernels(Rebo2State *st, agpu_potential *pot, const agpu_eval_ar
     int nblk = (ntot + 255) / 256;
     if (nblk < 1) nblk = 1;
     *nblk_out = nblk;
-    AGPU_CHECK(st->partial.reserve(sizeof(double) * 10 * nblk));
+    // neighbour pass: per centre groups and an owner gather when no per-atom
+    // or per-bond virial is asked for (those split each pair's virial over
+    // its atoms and bond, which the centre sums do not keep); otherwise a
+    // thread per directed bond with fixed-point accumulation
+    const int mr = nl->maxrow;
+    const int G = mr <= 4 ? 4 : (mr <= 8 ? 8 : (mr <= 16 ? 16 : 0));
+    const bool by_centre = !wacc && !a->wpot_per_bond && G > 0;
+    const int nbc = G > 0 ? (nat + 256 / G - 1) / (256 / G) : 0;
+    // block records: pair, torsion, then the neighbour pass (2 nblk, or
+    // centre and conjugation passes)
+    const size_t nrec = by_centre ? 2 * (size_t)nblk + 2 * (size_t)nbc : 4 * (size_t)nblk;
+    AGPU_CHECK(st->partial.reserve(sizeof(double) * 10 * nrec));
+    AGPU_CHECK(st->coef.reserve(sizeof(double4) * (total + 1)));
     AGPU_CHECK(st->rev.reserve(sizeof(int) * (total + 1)));
     AGPU_CHECK(st->half.reserve(sizeof(int) * (total + 1)));
     AGPU_CHECK(st->hpos.reserve(sizeof(int) * (total + 1)));
@@ -1374,7 +1737,7 @@ static int rebo2_kernels(Rebo2State *st, agpu_potential *pot, const agpu_eval_ar
         AGPU_LAUNCH("rebo2_bo", k_rb_bo<double>, dim3(nblk), dim3(256), 0, s, ntot, D64, st->tab.as<double>(),
                     els, offs, st->bcnt.as<int>(), st->bown.as<int>(), bond, st->bij.as<int2>(),
                     st->nn.as<double2>(), bo, st->rev.as<int>(), st->half.as<int>(), p->natloc,
-                    lammps ? st->tag_s.as<int>() : nullptr, order, dstats);
+                    lammps ? st->tag_s.as<int>() : nullptr, order, dstats, st->coef.as<double4>());
         size_t tmp = 0;
         AGPU_HIP(rocprim::exclusive_scan(nullptr, tmp, st->half.as<int>(), st->hpos.as<int>(), 0, (size_t)ntot,
                                          rocprim::plus<int>(), s));
@@ -1386,11 +1749,59 @@ static int rebo2_kernels(Rebo2State *st, agpu_potential *pot, const agpu_eval_ar
     } else {
         AGPU_CHECK(zero_dev(count, sizeof(int)));
     }
-    AGPU_LAUNCH(f64 ? "rebo2_pair" : "rebo2_pair_mixed", k_rb_pair<R>, dim3(nblk), dim3(256), 0, s, ntot,
-                st->list.as<int>(), count, D, tab, els, offs, st->bcnt.as<int>(), st->bown.as<int>(), bondR,
-                st->bij.as<int2>(), st->nn.as<double2>(), st->nconj.as<double>(), bo, st->rev.as<int>(),
-                st->facc.as<unsigned long long>(), eacc, wacc, st->partial.as<double>(), dstats, bsrc,
+    // pass records: pair (nblk blocks), torsion (nblk), neighbours (2 nblk)
+    const bool tors = st->hdev.with_dihedral != 0;
+    if (tors) AGPU_CHECK(st->tor.reserve(sizeof(Rb4<R>) * (total + 1)));
+    Rb4<R> *tor = tors ? st->tor.as<Rb4<R>>() : nullptr;
+    double4 *coef = st->coef.as<double4>();
+    double *part = st->partial.as<double>();
+    double *wdir = nullptr;
+    if (a->wpot_per_bond) {
+        AGPU_CHECK(st->wdir.reserve(sizeof(double) * 9 * (2 * (size_t)total + 2)));
+        wdir = st->wdir.as<double>();
+    }
+    const int *list = st->list.as<int>();
+    AGPU_LAUNCH(f64 ? "rebo2_pair" : "rebo2_pair_mixed", k_rb_pair<R>, dim3(nblk), dim3(256), 0, s, list, count,
+                D, tab, els, offs, st->bcnt.as<int>(), st->bown.as<int>(), bondR, st->bij.as<int2>(),
+                st->nn.as<double2>(), st->nconj.as<double>(), bo, st->rev.as<int>(), coef, tor,
+                st->facc.as<unsigned long long>(), eacc, wacc, part, dstats, bsrc, a->epot_per_bond,
+                a->f_per_bond, a->wpot_per_bond, D64, bond);
+    // without torsion (tor == nullptr) the pass only writes zero records
+    AGPU_LAUNCH("rebo2_torsion", k_rb_torsion<R>, dim3(nblk), dim3(256), 0, s, list, count, offs,
+                st->bcnt.as<int>(), st->bown.as<int>(), st->bij.as<int2>(), st->rev.as<int>(), tor, coef,
+                st->facc.as<unsigned long long>(), eacc, wacc, part + 10 * (size_t)nblk, dstats, bsrc,
                 a->epot_per_bond, a->f_per_bond, a->wpot_per_bond, D64, bond);
+    if (by_centre) {
+        AGPU_CHECK(st->fnb.reserve(sizeof(double) * 3 * (total + 1)));
+        AGPU_CHECK(st->fcj.reserve(sizeof(double) * 3 * (total + 1)));
+        AGPU_CHECK(st->bconj.reserve(sizeof(double) * (total + 1)));
+        AGPU_CHECK(st->own.reserve(sizeof(double) * 6 * (nat + 1)));
+        double *own = st->own.as<double>(), *own2 = own + 3 * (size_t)nat;
+        double *pc = part + 20 * (size_t)nblk, *pj = pc + 10 * (size_t)nbc;
+        auto centre = G == 4 ? k_rb_centre<R, 4> : (G == 8 ? k_rb_centre<R, 8> : k_rb_centre<R, 16>);
+        auto conjf = G == 4 ? k_rb_conj_force<R, 4> : (G == 8 ? k_rb_conj_force<R, 8> : k_rb_conj_force<R, 16>);
+        AGPU_LAUNCH(f64 ? "rebo2_centre" : "rebo2_centre_mixed", centre, dim3(nbc), dim3(256), 0, s, nat, D, els,
+                    offs, st->bcnt.as<int>(), bondR, st->bij.as<int2>(), st->nn.as<double2>(), bo, coef,
+                    st->fnb.as<double>(), own, st->bconj.as<double>(), pc);
+        AGPU_LAUNCH(f64 ? "rebo2_conj_force" : "rebo2_conj_force_mixed", conjf, dim3(nbc), dim3(256), 0, s, nat,
+                    offs, st->bcnt.as<int>(), bondR, st->rev.as<int>(), st->bconj.as<double>(),
+                    st->fcj.as<double>(), own2, pj);
+        AGPU_LAUNCH("rebo2_gather", k_rb_gather, dim3(nb), dim3(256), 0, s, nat, order, offs, st->bcnt.as<int>(),
+                    st->rev.as<int>(), own, own2, st->fnb.as<double>(), st->fcj.as<double>(),
+                    st->facc.as<unsigned long long>(), eacc, a->f, a->epot_per_at);
+        *nblk_out = 2 * nblk + 2 * nbc;
+        *gathered = true;
+        return AGPU_OK;
+    }
+    AGPU_LAUNCH(f64 ? "rebo2_nbrs" : "rebo2_nbrs_mixed", k_rb_nbrs<R>, dim3(2 * nblk), dim3(256), 0, s, list,
+                count, D, els, offs, st->bcnt.as<int>(), st->bown.as<int>(), bondR, st->bij.as<int2>(),
+                st->nn.as<double2>(), bo, st->rev.as<int>(), coef, st->facc.as<unsigned long long>(), wacc,
+                part + 20 * (size_t)nblk, dstats, wdir);
+    if (wdir)
+        AGPU_LAUNCH("rebo2_wpb", k_rb_wpb, dim3(nblk), dim3(256), 0, s, list, count, bsrc, wdir,
+                    a->wpot_per_bond);
+    *nblk_out = 4 * nblk;
+    *gathered = false;
     return AGPU_OK;
 }
 
@@ -1457,12 +1868,14 @@ int rebo2_compute(agpu_potential *pot, const agpu_eval_args *a)
                                          (size_t)nat + 1, rocprim::plus<int>(), s));
     }
     int nblk = 0;
+    bool gathered = false;
     if (pot->precision == AGPU_PREC_MIXED)
-        AGPU_CHECK(rebo2_kernels<float>(st, pot, a, bsrc, eacc, wacc, &nblk));
+        AGPU_CHECK(rebo2_kernels<float>(st, pot, a, bsrc, eacc, wacc, &nblk, &gathered));
     else
-        AGPU_CHECK(rebo2_kernels<double>(st, pot, a, bsrc, eacc, wacc, &nblk));
-    AGPU_LAUNCH("rebo2_finish", k_rb_finish, dim3(nb), dim3(256), 0, s, nat, order,
-                st->facc.as<unsigned long long>(), eacc, wacc, a->f, a->epot_per_at, a->wpot_per_at);
+        AGPU_CHECK(rebo2_kernels<double>(st, pot, a, bsrc, eacc, wacc, &nblk, &gathered));
+    if (!gathered)
+        AGPU_LAUNCH("rebo2_finish", k_rb_finish, dim3(nb), dim3(256), 0, s, nat, order,
+                    st->facc.as<unsigned long long>(), eacc, wacc, a->f, a->epot_per_at, a->wpot_per_at);
     AGPU_CHECK(reduce_ev(st->partial.as<double>(), nblk, 10, a->epot, a->wpot, nullptr));
     AGPU_HIP(hipMemcpyAsync(pot->stats.h, dstats, sizeof(int), hipMemcpyDeviceToHost, s));
     AGPU_HIP(hipStreamSynchronize(s));

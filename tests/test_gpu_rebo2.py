@@ -65,16 +65,20 @@ CASES = [
 ]
 
 
+@pytest.mark.parametrize("per_at", [True, False], ids=["per_atom", "totals"])
 @pytest.mark.parametrize("dihedral", [False, True], ids=["plain", "torsion"])
 @pytest.mark.parametrize("name,make", CASES, ids=[c[0] for c in CASES])
-def test_rebo2_fp64_parity(name, make, dihedral):
+def test_rebo2_fp64_parity(name, make, dihedral, per_at):
+    # per-atom virials take the fixed-point neighbour pass; totals only, the
+    # per-centre pass with its owner gather
     pos, cell, pbc, Z = make()
-    print("rebo2 case", name, len(pos), "dihedral", dihedral, flush=True)
-    e, f, w, epa, _, _, wpa, _ = run_gpu(pos, cell, pbc, Z, per_at=True, dihedral=dihedral)
+    print("rebo2 case", name, len(pos), "dihedral", dihedral, "per_at", per_at, flush=True)
+    e, f, w, epa, _, _, wpa, _ = run_gpu(pos, cell, pbc, Z, per_at=per_at, dihedral=dihedral)
     re_, rf, rw, repa, rwpa = run_oracle(pos, cell, pbc, Z, per_at=True, dihedral=dihedral)
     assert abs(e - re_) <= REL * max(1.0, abs(re_))
     assert close(f, rf) and close(w, rw)
-    assert close(epa, repa) and close(wpa, rwpa)
+    if per_at:
+        assert close(epa, repa) and close(wpa, rwpa)
 
 
 def test_rebo2_ignores_other_elements():
@@ -129,8 +133,14 @@ def test_rebo2_mixed_error(name, make, rel):
     assert abs(e - re_) / n <= 1e-5
     assert np.abs(w - rw).max() / n <= 1e-5
     assert np.abs(epa - repa).max() <= 1e-4
+    # totals only: the per-centre neighbour pass, bitwise reproducible
     e2, f2, w2, *_ = run_gpu(pos, cell, pbc, Z, precision="mixed", dihedral=True)
-    assert e2 == e and np.array_equal(f2, f) and np.array_equal(w2, w)
+    rms2 = float(np.sqrt(np.mean((f2 - rf) ** 2)))
+    print("rebo2 mixed %s (per-centre pass): F rms %.3e  |dE|/N %.3e" % (name, rms2, abs(e2 - re_) / n), flush=True)
+    assert rms2 <= (1e-5 if rel is None else rel * float(np.sqrt(np.mean(rf ** 2))))
+    assert abs(e2 - re_) / n <= 1e-5 and np.abs(w2 - rw).max() / n <= 1e-5
+    e3, f3, w3, *_ = run_gpu(pos, cell, pbc, Z, precision="mixed", dihedral=True)
+    assert e3 == e2 and np.array_equal(f3, f2) and np.array_equal(w3, w2)
 
 
 def test_rebo2_full_size_diamond():
