@@ -5,13 +5,16 @@
 // binning_init/binning_update (:765-959):
 //   1. k_bin:      bin of every atom (floor(rec_cell_size.r) + 1, wrapped on
 //                  periodic axes, clamped otherwise) and its wrap count.
-//   2. counting sort by bin (counts from k_bin, exclusive scan, k_cell_fill), then
-//                  k_cell_order puts each bin back into ascending atom order:
-//                  the reference's linked lists, as arrays, in row-major bin
-//                  order.  A bin too full for k_cell_order flags a rebuild
-//                  with a radix sort of (bin, atom) pairs + k_cell_start.
-//   3. k_gather:   bin-sorted copy of (r, atom, wrap) so that candidate scans
-//                  read contiguous 32-byte records.
+//   2. counting sort by bin (counts and in-bin places from k_bin, exclusive
+//                  scan, k_cell_fill), then k_cell_order puts each bin back
+//                  into ascending atom order: the reference's linked lists,
+//                  as arrays, in row-major bin order.  A bin too full for
+//                  k_cell_order flags a rebuild with a radix sort of
+//                  (bin, atom) pairs + k_cell_start.
+//   3. the bin-sorted copy of (r, atom, wrap) (and the FP32 wrapped position
+//                  for the thread-per-atom search), written by k_cell_order
+//                  (k_gather after the radix sort), so that candidate scans
+//                  read contiguous records.
 //   4. nl_pairs:   the accepted (j, image) pairs of every atom over its
 //                  stencil (x-major, then y, then z; atoms of a bin ascending)
 //                  into a padded staging row of stride S, and the row counts.
@@ -91,8 +94,9 @@ __global__ void k_bin(int nat, const double *__restrict__ r, BinGeom g, int *__r
     if (!dc_pack_ok(w[0]) || !dc_pack_ok(w[1]) || !dc_pack_ok(w[2])) atomicOr(flags, DERR_IMAGE_RANGE);
     const int b = ((c[0] - 1) * g.nc[1] + (c[1] - 1)) * g.nc[2] + (c[2] - 1);
     key[i] = b;
-    idx[i] = i;
-    if (cnt) atomicAdd(&cnt[b], 1);
+    // counting sort: the atom's provisional place in its bin; the radix
+    // sort takes (key, index) pairs
+    idx[i] = cnt ? atomicAdd(&cnt[b], 1) : i;
 }
 
 __global__ void k_cell_start(int nat, int ncell, const int *__restrict__ key_sorted,
@@ -105,75 +109,36 @@ __global__ void k_cell_start(int nat, int ncell, const int *__restrict__ key_sor
     for (int c = kprev + 1; c <= kcur; ++c) cell_start[c] = p;
 }
 
-// Counting sort of the atoms by bin: counts, exclusive scan, scatter with
-// atomic slots, then each bin's atoms are put back into ascending index
-// order, which is the order a stable sort gives.  The reordering takes one
-// group of lanes per bin: every element's rank is the number of smaller
-// indices in the bin (indices are distinct), counted through lane
-// broadcasts.  A bin too full for its group flags a fallback to the radix
-// sort.
+// Counting sort of the atoms by bin: k_bin counts and takes each atom's
+// provisional place in its bin with the same atomic, an exclusive scan gives
+// the bin starts, k_cell_fill scatters, then each bin's atoms are put back
+// into ascending index order, which is the order a stable sort gives.  The
+// reordering takes one group of lanes per bin: every element's rank is the
+// number of smaller indices in the bin (indices are distinct), counted
+// through lane broadcasts; the same pass writes the sorted records.  A bin
+// too full for its group flags a fallback to the radix sort.
 __global__ void k_zero_i(int *__restrict__ a, long long n)
 {
     for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long long)gridDim.x * blockDim.x)
         a[q] = 0;
 }
 
-__global__ void k_cell_fill(int nat, int ncell, const int *__restrict__ key, int *__restrict__ cell_start,
-                            int *__restrict__ cnt, int *__restrict__ idx_sorted)
+__global__ void k_cell_fill(int nat, int ncell, const int *__restrict__ key, const int *__restrict__ sub,
+                            int *__restrict__ cell_start, int *__restrict__ idx_sorted)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) cell_start[ncell] = nat;
     if (i >= nat) return;
-    const int c = key[i];
-    const int slot = atomicSub(&cnt[c], 1) - 1;     // leaves cnt at zero
-    idx_sorted[cell_start[c] + slot] = i;
+    idx_sorted[cell_start[key[i]] + sub[i]] = i;
 }
 
-// G lanes per bin, up to 4 atoms per lane: bins of up to 4G atoms are
-// ordered in registers; a fuller bin flags the radix-sort fallback.
-template <int G>
-__global__ void __launch_bounds__(256)
-k_cell_order(int ncell, const int *__restrict__ cell_start, int *__restrict__ idx_sorted, int *__restrict__ flags)
+// the bin-sorted record of atom j at slot p (and the optional FP32 wrapped
+// position r + A.w, with the wrap flag (w != 0) in bit 31 of .w)
+__device__ __forceinline__ void gather_atom(int p, int j, const double *__restrict__ r, const BinGeom &g,
+                                            const CellArg &C, SortedAtom *__restrict__ out, int *__restrict__ rank,
+                                            float4 *__restrict__ pos32)
 {
-    const int lane = threadIdx.x % G;
-    const int c = blockIdx.x * (256 / G) + threadIdx.x / G;
-    if (c >= ncell) return;                      // uniform per group
-    const int b = cell_start[c], n = cell_start[c + 1] - b;
-    if (n <= 1) return;
-    if (n > 4 * G) {
-        if (lane == 0) atomicOr(flags, DERR_SORT_FALLBACK);
-        return;
-    }
-    int v[4], r[4] = {0, 0, 0, 0};
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int k = lane + G * q;
-        v[q] = k < n ? idx_sorted[b + k] : 0x7fffffff;
-    }
-    for (int k = 0; k < n; ++k) {
-        const int src = k / G;
-        const int w = src == 0 ? v[0] : (src == 1 ? v[1] : (src == 2 ? v[2] : v[3]));
-        const int u = __shfl(w, k % G, G);
-#pragma unroll
-        for (int q = 0; q < 4; ++q) r[q] += u < v[q];
-    }
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int k = lane + G * q;
-        if (k < n) idx_sorted[b + r[q]] = v[q];
-    }
-}
-
-// pos32 (optional): the wrapped position r + A.w rounded to FP32, with the
-// wrap flag (w != 0) in bit 31 of .w
-__global__ void k_gather(int nat, const double *__restrict__ r, BinGeom g, CellArg C,
-                         const int *__restrict__ idx_sorted, SortedAtom *__restrict__ out,
-                         int *__restrict__ rank, float4 *__restrict__ pos32)
-{
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= nat) return;
-    int j = idx_sorted[p];
-    double rj[3] = {r[3 * j], r[3 * j + 1], r[3 * j + 2]};
+    double rj[3] = {r[3 * (size_t)j], r[3 * (size_t)j + 1], r[3 * (size_t)j + 2]};
     int c[3], w[3];
     bin_of(g, rj, c, w);
     SortedAtom a;
@@ -190,6 +155,63 @@ __global__ void k_gather(int nat, const double *__restrict__ r, BinGeom g, CellA
         const int flag = (w[0] | w[1] | w[2]) != 0 ? (int)0x80000000u : 0;
         pos32[p] = make_float4(q[0], q[1], q[2], __int_as_float(j | flag));
     }
+}
+
+// G lanes per bin, up to 4 atoms per lane: bins of up to 4G atoms are put
+// into ascending index order in registers, the bin counter is cleared for
+// the next build, and each atom's sorted record is written at its slot; a
+// fuller bin flags the radix-sort fallback.
+template <int G>
+__global__ void __launch_bounds__(256)
+k_cell_order(int ncell, const int *__restrict__ cell_start, int *__restrict__ cnt, int *__restrict__ idx_sorted,
+             int *__restrict__ flags, const double *__restrict__ r, BinGeom g, CellArg C,
+             SortedAtom *__restrict__ out, int *__restrict__ rank, float4 *__restrict__ pos32)
+{
+    const int lane = threadIdx.x % G;
+    const int c = blockIdx.x * (256 / G) + threadIdx.x / G;
+    if (c >= ncell) return;                      // uniform per group
+    const int b = cell_start[c], n = cell_start[c + 1] - b;
+    if (lane == 0 && n > 0) cnt[c] = 0;
+    if (n == 0) return;
+    if (n > 4 * G) {
+        // the build is redone with the radix sort; until then every slot
+        // still holds one of the bin's atoms (in scatter order), so the
+        // kernels after this one read valid records
+        if (lane == 0) atomicOr(flags, DERR_SORT_FALLBACK);
+        for (int k = lane; k < n; k += G) gather_atom(b + k, idx_sorted[b + k], r, g, C, out, rank, pos32);
+        return;
+    }
+    int v[4], rk[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int k = lane + G * q;
+        v[q] = k < n ? idx_sorted[b + k] : 0x7fffffff;
+    }
+    for (int k = 0; k < n; ++k) {
+        const int src = k / G;
+        const int w = src == 0 ? v[0] : (src == 1 ? v[1] : (src == 2 ? v[2] : v[3]));
+        const int u = __shfl(w, k % G, G);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) rk[q] += u < v[q];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int k = lane + G * q;
+        if (k < n) {
+            idx_sorted[b + rk[q]] = v[q];
+            gather_atom(b + rk[q], v[q], r, g, C, out, rank, pos32);
+        }
+    }
+}
+
+// the sorted records after the radix sort (slot p -> atom idx_sorted[p])
+__global__ void k_gather(int nat, const double *__restrict__ r, BinGeom g, CellArg C,
+                         const int *__restrict__ idx_sorted, SortedAtom *__restrict__ out,
+                         int *__restrict__ rank, float4 *__restrict__ pos32)
+{
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nat) return;
+    gather_atom(p, idx_sorted[p], r, g, C, out, rank, pos32);
 }
 
 constexpr int NL_R = 4;         // candidate slots per lane (registers): 256 per pass
@@ -466,12 +488,21 @@ k_bin_pairs(int B, BinGeom g, CellArg C, double cutoff_sq, const SortedAtom *__r
                     };
                     if (ORTHO && (w0 | w1 | w2) == 0) {
                         // home atom inside the cell: s = ct, whose A.s and
-                        // packed form were made at the candidate load
+                        // packed form were made at the candidate load; the
+                        // tests combine without short-circuit branches and
+                        // one exec region guards the store
 #pragma unroll
                         for (int r = 0; r < NL_R; ++r) {
                             if (c0 + 64 * r >= T) break;     // slot row past the last candidate
                             const double d2 = pair_d2_pre(H.x, H.y, H.z, cx[r], cy[r], cz[r], cp0[r], cp1[r], cp2[r]);
-                            emit(member(r) && !(cj[r] == ia && cpk[r] == DC_ZERO) && d2 < cutoff_sq, cj[r], cpk[r]);
+                            const bool acc = (bool)((int)member(r) & (int)(d2 < cutoff_sq) &
+                                                    (int)((cj[r] != ia) | (cpk[r] != DC_ZERO)));
+                            const unsigned long long m = __ballot(acc);
+                            const unsigned pos = __builtin_amdgcn_mbcnt_hi(
+                                (unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, (unsigned)cnt));
+                            img_bad = (bool)((int)img_bad | ((int)acc & (int)(cpk[r] < 0)));
+                            if ((int)acc & (int)(pos < (unsigned)S)) row[pos] = make_int2(cj[r], cpk[r]);
+                            cnt += __popcll(m);
                         }
                     } else {
 #pragma unroll
@@ -653,7 +684,7 @@ k_atom_pairs(int nat, BinGeom g, CellArg C, double cutoff_sq, const SortedAtom *
             }
         }
         const int ih = __float_as_int(H.w) & 0x7fffffff;
-        counts[ih] = cnt;
+        if (ih < nat) counts[ih] = cnt;
         over = cnt > S;
         longest = cnt;
         last = cnt > 0 ? ih + 1 : 0;
@@ -889,6 +920,10 @@ int nl_update(agpu_neighbors *nl, agpu_particles *p, int force, bool csr)
                 AGPU_LAUNCH("nl_sort_zero", k_zero_i, dim3(nzb), dim3(TB), 0, s, cnt, ncell);
             }
         }
+        const double per_bin = (double)nat / (double)ncell;
+        const bool per_atom = nl->pair_mode == 1 || (nl->pair_mode == 0 && per_bin < 4.0);
+        if (per_atom) AGPU_CHECK(nl->pos32.reserve(sizeof(float4) * (nat + 1)));
+        float4 *pos32 = per_atom ? nl->pos32.as<float4>() : nullptr;
         AGPU_LAUNCH("nl_bin", k_bin, dim3(nb), dim3(TB), 0, s, nat, p->d_r, g,
                     nl->key.as<int>(), nl->idx.as<int>(), cnt, flags + 3);
         if (!nl->radix) {
@@ -899,19 +934,16 @@ int nl_update(agpu_neighbors *nl, agpu_particles *p, int force, bool csr)
             AGPU_HIP(rocprim::exclusive_scan(nl->sort_tmp.ptr, tmp, cnt, nl->cell_start.as<int>(), 0,
                                              (size_t)ncell, rocprim::plus<int>(), s));
             AGPU_LAUNCH("nl_sort_fill", k_cell_fill, dim3(nb), dim3(TB), 0, s, nat, (int)ncell, nl->key.as<int>(),
-                        nl->cell_start.as<int>(), cnt, nl->idx_sorted.as<int>());
-            nl->cnt_zero = std::max(cnt_zero, ncell);
-            // lanes per bin from the mean occupancy (capacity 4 atoms per lane)
+                        nl->idx.as<int>(), nl->cell_start.as<int>(), nl->idx_sorted.as<int>());
+            // lanes per bin from the mean occupancy (capacity 4 atoms per
+            // lane); the same pass writes the sorted records
             const double occ = (double)nat / (double)ncell;
-            if (occ > 8.0)
-                AGPU_LAUNCH("nl_sort_order", k_cell_order<64>, dim3((unsigned)((ncell + 3) / 4)), dim3(TB), 0, s,
-                            (int)ncell, nl->cell_start.as<int>(), nl->idx_sorted.as<int>(), flags + 3);
-            else if (occ > 2.0)
-                AGPU_LAUNCH("nl_sort_order", k_cell_order<16>, dim3((unsigned)((ncell + 15) / 16)), dim3(TB), 0, s,
-                            (int)ncell, nl->cell_start.as<int>(), nl->idx_sorted.as<int>(), flags + 3);
-            else
-                AGPU_LAUNCH("nl_sort_order", k_cell_order<4>, dim3((unsigned)((ncell + 63) / 64)), dim3(TB), 0, s,
-                            (int)ncell, nl->cell_start.as<int>(), nl->idx_sorted.as<int>(), flags + 3);
+            auto ord = occ > 8.0 ? k_cell_order<64> : (occ > 2.0 ? k_cell_order<16> : k_cell_order<4>);
+            const int bpb = occ > 8.0 ? 4 : (occ > 2.0 ? 16 : 64);
+            AGPU_LAUNCH("nl_sort_order", ord, dim3((unsigned)((ncell + bpb - 1) / bpb)), dim3(TB), 0, s, (int)ncell,
+                        nl->cell_start.as<int>(), cnt, nl->idx_sorted.as<int>(), flags + 3, p->d_r, g, C,
+                        nl->atoms_sorted.as<SortedAtom>(), nl->rank.as<int>(), pos32);
+            nl->cnt_zero = std::max(cnt_zero, ncell);
         } else {
             size_t tmp = 0;
             int end_bit = 1;
@@ -926,12 +958,9 @@ int nl_update(agpu_neighbors *nl, agpu_particles *p, int force, bool csr)
             AGPU_LAUNCH("nl_cell_start", k_cell_start, dim3((nat + 1 + TB - 1) / TB), dim3(TB), 0, s,
                         nat, (int)ncell, nl->key_sorted.as<int>(), nl->cell_start.as<int>());
         }
-        const double per_bin = (double)nat / (double)ncell;
-        const bool per_atom = nl->pair_mode == 1 || (nl->pair_mode == 0 && per_bin < 4.0);
-        if (per_atom) AGPU_CHECK(nl->pos32.reserve(sizeof(float4) * (nat + 1)));
-        AGPU_LAUNCH("nl_gather", k_gather, dim3(nb), dim3(TB), 0, s, nat, p->d_r, g, C,
-                    nl->idx_sorted.as<int>(), nl->atoms_sorted.as<SortedAtom>(), nl->rank.as<int>(),
-                    per_atom ? nl->pos32.as<float4>() : nullptr);
+        if (nl->radix)
+            AGPU_LAUNCH("nl_gather", k_gather, dim3(nb), dim3(TB), 0, s, nat, p->d_r, g, C,
+                        nl->idx_sorted.as<int>(), nl->atoms_sorted.as<SortedAtom>(), nl->rank.as<int>(), pos32);
         double cutoff_sq = nl->cutoff * nl->cutoff;
         // bins per wavefront edge: one bin when bins are dense, a 2^3 or 3^3
         // block of bins when they hold few atoms (stencil of +-1 bin only)

@@ -126,7 +126,7 @@ struct Rebo2State {
     std::vector<double> htab;
     DevBuf dev, tab, bond, bij, bown, bcnt, nn, nconj, facc, eacc, wacc, partial, stats, bo, rev, half, hpos,
         list, scan_tmp, bsrc, cnt_s, offs_s, el_s, tag_s, dev32, tab32, bond32, coef, tor, wdir, fnb, fcj,
-        bconj, own;
+        bconj, own, tflag, tpos, tlist;
     int z2db[128];
     uint64_t Z_rev = ~0ull;
     double cutoff = 0.0;
@@ -1125,7 +1125,8 @@ k_rb_pair(const int *__restrict__ list, const int *__restrict__ count, const RbD
           const int *__restrict__ bcnt, const int *__restrict__ bown, const RbBondT<R> *__restrict__ bond,
           const int2 *__restrict__ bij, const double2 *__restrict__ nn, const double *__restrict__ nconjit,
           const Rb4<double> *__restrict__ bo, const int *__restrict__ rev, double4 *__restrict__ coef,
-          Rb4<R> *__restrict__ tor, unsigned long long *__restrict__ facc, unsigned long long *__restrict__ eacc,
+          Rb4<R> *__restrict__ tor, int *__restrict__ tflag, int ntot, unsigned long long *__restrict__ facc,
+          unsigned long long *__restrict__ eacc,
           unsigned long long *__restrict__ wacc, double *__restrict__ partial, int *__restrict__ flags,
           const int *__restrict__ bsrc, double *__restrict__ epb, double *__restrict__ fpb,
           double *__restrict__ wpb, const RbDev *__restrict__ D64p, const RbBond *__restrict__ bond64)
@@ -1137,7 +1138,9 @@ k_rb_pair(const int *__restrict__ list, const int *__restrict__ count, const RbD
     double wij[9];
 #pragma unroll
     for (int q = 0; q < 9; ++q) wij[q] = 0.0;
-    if (t < *count) {
+    const int nhalf = *count;
+    if (tflag && t >= nhalf && t < ntot) tflag[t] = 0;
+    if (t < nhalf) {
         const int s = list[t];
         const int i = bown[s];
         const int j = bij[s].x;
@@ -1170,7 +1173,10 @@ k_rb_pair(const int *__restrict__ list, const int *__restrict__ count, const RbD
         ntj = ntj > R(3.0) ? R(3.0) : ntj;
         Rb4<R> tv{R(0.0), R(0.0), R(0.0), R(0.0)};
         if (D.with_dihedral && ijpot == RB_CC) rb_table3(tab + RB_TAB_TCC, nti, ntj, nconj, tv.x, tv.y, tv.z, tv.w);
-        if (tor) tor[t] = tv;
+        if (tor) {
+            tor[t] = tv;
+            tflag[t] = tv.x != R(0.0) ? 1 : 0;
+        }
         R fij = R(0.0), dfdni = R(0.0), dfdnj = R(0.0), dfdncn = R(0.0);
         if (ijpot == RB_CC)
             rb_table3(tab + RB_TAB_FCC, nti, ntj, nconj, fij, dfdni, dfdnj, dfdncn);
@@ -1229,84 +1235,210 @@ k_rb_pair(const int *__restrict__ list, const int *__restrict__ count, const RbD
 // the FP64 records), its share T sum / 2 of the bond order in the pair
 // energy and radial force, and its terms dT/dN sum, dT/dN^conj sum added to
 // both directed bonds' coefficients before the neighbour pass.
+// one (k, l) term of the torsion sum of pair i-j (the body of the double loop
+// of bop_kernel_rebo2.f90:1939-2120): returns (1 - cos^2 w) fc_ik fc_jl;
+// forces on i, j into fi, fj and on k, l into fk, fl; virial into w
+__device__ __forceinline__ double rb_torsion_term(const RbBond &K, const RbBond &L, const double *rnij, double rlij,
+                                                  double tij, double fa, double fcar, double *fi, double *fj,
+                                                  double *fk, double *fl, double *w)
+{
+    const double tije = tij * fa * fcar;
+    const double *rnik = K.n, *rnjl = L.n;
+    const double rlik = K.len, rljl = L.len;
+    double dot_ij_ik = 0.0, dot_ij_jl = 0.0, dot_ik_jl = 0.0;
+    dot_ij_ik = dot_ij_ik + rnij[0] * rnik[0];
+    dot_ij_ik = dot_ij_ik + rnij[1] * rnik[1];
+    dot_ij_ik = dot_ij_ik + rnij[2] * rnik[2];
+    dot_ij_jl = dot_ij_jl + rnij[0] * rnjl[0];
+    dot_ij_jl = dot_ij_jl + rnij[1] * rnjl[1];
+    dot_ij_jl = dot_ij_jl + rnij[2] * rnjl[2];
+    dot_ik_jl = dot_ik_jl + rnik[0] * rnjl[0];
+    dot_ik_jl = dot_ik_jl + rnik[1] * rnjl[1];
+    dot_ik_jl = dot_ik_jl + rnik[2] * rnjl[2];
+    const double dcik = 1.0 - dot_ij_ik * dot_ij_ik;
+    const double dcjl = 1.0 - dot_ij_jl * dot_ij_jl;
+    const double abs_dc = sqrt(dcik * dcjl);
+    const double cost = (dot_ij_ik * dot_ij_jl - dot_ik_jl) / abs_dc;
+    double bdhij = 1 - cost * cost;
+    const double term = bdhij * K.fc * L.fc;
+    bdhij = bdhij * tij * fa * fcar / 2;
+    const double dbdhij = -2 * cost * tije * K.fc * L.fc / 2;
+    double df[3], v[3];
+#pragma unroll
+    for (int x = 0; x < 3; ++x)
+        df[x] = dbdhij *
+                ((dot_ij_jl / abs_dc + cost * dot_ij_ik / dcik) * rnik[x] +
+                 (dot_ij_ik / abs_dc + cost * dot_ij_jl / dcjl) * rnjl[x] -
+                 (2 * dot_ik_jl / abs_dc + cost * (1.0 / dcik + 1.0 / dcjl)) * rnij[x]) /
+                rlij;
+#pragma unroll
+    for (int x = 0; x < 3; ++x) {
+        fi[x] += df[x];
+        fj[x] -= df[x];
+        v[x] = rlij * rnij[x];
+    }
+    outer_add(w, v, df);
+#pragma unroll
+    for (int x = 0; x < 3; ++x)
+        df[x] = dbdhij *
+                    (-1.0 / dcik * cost * rnik[x] - 1.0 / abs_dc * rnjl[x] +
+                     (dot_ij_jl / abs_dc + cost * dot_ij_ik / dcik) * rnij[x]) /
+                    rlik +
+                bdhij * K.dfc * L.fc * rnik[x];
+#pragma unroll
+    for (int x = 0; x < 3; ++x) {
+        fi[x] += df[x];
+        fk[x] = -df[x];
+        v[x] = rlik * rnik[x];
+    }
+    outer_add(w, v, df);
+#pragma unroll
+    for (int x = 0; x < 3; ++x)
+        df[x] = dbdhij *
+                    (-1.0 / dcjl * cost * rnjl[x] - 1.0 / abs_dc * rnik[x] +
+                     (dot_ij_ik / abs_dc + cost * dot_ij_jl / dcjl) * rnij[x]) /
+                    rljl +
+                bdhij * K.fc * L.dfc * rnjl[x];
+#pragma unroll
+    for (int x = 0; x < 3; ++x) {
+        fj[x] += df[x];
+        fl[x] = -df[x];
+        v[x] = rljl * rnjl[x];
+    }
+    outer_add(w, v, df);
+    return term;
+}
+
+// Pass 2, a group of 4 lanes per half pair with T != 0 (C-C with torsion,
+// from the scanned list; the grid strides over the list): lane q takes the
+// terms q, q + 4, ... of the torsion sum over k (bonds of i but j) and l
+// (bonds of j but i, and not k at the same image) (bop_kernel_rebo2.f90:
+// 1939-2120, always on the FP64 records), with their forces on k and l; the
+// group sums give T sum, the forces on i, j and the virial.  Then its share
+// T sum / 2 of the bond order in the pair energy and radial force, and its
+// dT/dN terms added to both directed bonds' coefficients before the
+// neighbour pass.  Pairs are dealt to groups in a fixed order, so the block
+// records do not depend on scheduling.
 template <typename R>
 __global__ void __launch_bounds__(256)
-k_rb_torsion(const int *__restrict__ list, const int *__restrict__ count, const int *__restrict__ offsets,
-             const int *__restrict__ bcnt, const int *__restrict__ bown, const int2 *__restrict__ bij,
-             const int *__restrict__ rev, const Rb4<R> *__restrict__ tor, double4 *__restrict__ coef,
+k_rb_torsion(const int *__restrict__ list, const int *__restrict__ tlist, const int *__restrict__ tcount,
+             const int *__restrict__ offsets, const int *__restrict__ bcnt, const int *__restrict__ bown,
+             const int2 *__restrict__ bij, const int *__restrict__ rev, const Rb4<R> *__restrict__ tor,
+             double4 *__restrict__ coef,
              unsigned long long *__restrict__ facc, unsigned long long *__restrict__ eacc,
              unsigned long long *__restrict__ wacc, double *__restrict__ partial, int *__restrict__ flags,
              const int *__restrict__ bsrc, double *__restrict__ epb, double *__restrict__ fpb,
              double *__restrict__ wpb, const RbDev *__restrict__ D64p, const RbBond *__restrict__ bond64)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    constexpr int G = 4;
+    const int lane = threadIdx.x % G;
+    const int ngroups = gridDim.x * (256 / G);
     int bad = 0;
-    double epair = 0.0;
-    double wij[9];
+    double e_run = 0.0, w_run[9];
 #pragma unroll
-    for (int q = 0; q < 9; ++q) wij[q] = 0.0;
-    Rb4<R> tv{R(0.0), R(0.0), R(0.0), R(0.0)};
-    if (tor && t < *count) tv = tor[t];
-    if (tv.x != R(0.0)) {
-        const int s = list[t];
-        const int i = bown[s];
-        const int2 eij = bij[s];
-        const int j = eij.x;
-        const int sji = rev[s];
-        const RbBond B = bond64[s];
-        double fa, dfa, fr, dfr;
-        rb_va<double>(*D64p, RB_CC, B.len, fa, dfa);
-        auto at = [&](int b, int) { return bond64[b]; };
-        double fi[3] = {0, 0, 0}, fj[3] = {0, 0, 0};
+    for (int q = 0; q < 9; ++q) w_run[q] = 0.0;
+    const int ntor = tor ? *tcount : 0;
+    // wavefront-uniform trip count (the group sums need every lane)
+    for (int tt0 = blockIdx.x * (256 / G) + (threadIdx.x & ~63) / G; tt0 < ntor; tt0 += ngroups) {
+        const int tt = tt0 + (threadIdx.x & 63) / G;
+        Rb4<R> tv{R(0.0), R(0.0), R(0.0), R(0.0)};
+        int t = 0;
+        if (tt < ntor) {
+            t = tlist[tt];
+            tv = tor[t];
+        }
+        const bool on = tv.x != R(0.0);
+        int s = 0, i = 0, j = 0, sji = 0;
+        RbBond B{};
+        double fa = 0.0, dfa = 0.0;
         const double tij = (double)tv.x;
-        const double bdh = rb_torsion<double>(offsets[i], bcnt[i], s, offsets[j], bcnt[j], sji, eij.y, at, bij, B.n,
-                                              B.len, tij, fa, B.fc, fi, fj, wij, facc, bad);
-        // coefficients of the neighbour pass
-        const double h = 0.5 * B.fc * fa * bdh;
-        const double4 ci = coef[s], cj = coef[sji];
-        const double dn = h * (double)tv.w;
-        coef[s] = make_double4(ci.x + h * (double)tv.y, ci.y + 2 * dn * ci.z, ci.z, 1.0);
-        coef[sji] = make_double4(cj.x + h * (double)tv.z, cj.y + 2 * dn * cj.z, cj.z, 1.0);
-        // the pair energy's torsion share: b += T sum / 2, no repulsion
-        double hlf, df[3];
-        rb_radial(B, fa, dfa, 0.0, 0.0, 0.5 * tij * bdh, hlf, df);
-        epair = hlf + hlf;
+        double fi[3] = {0, 0, 0}, fj[3] = {0, 0, 0}, bdh = 0.0, wij[9];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) wij[q] = 0.0;
+        if (on) {
+            s = list[t];
+            i = bown[s];
+            const int2 eij = bij[s];
+            j = eij.x;
+            sji = rev[s];
+            B = bond64[s];
+            rb_va<double>(*D64p, RB_CC, B.len, fa, dfa);
+            const int ib = offsets[i], ni = bcnt[i] - 1, jb = offsets[j], nj = bcnt[j] - 1;
+            const int xs = s - ib, xr = sji - jb;      // the bond i->j in i's row, j->i in j's row
+            int a0, b0, c0;
+            dc_unpack(eij.y, a0, b0, c0);
+            for (int q = lane; q < ni * nj; q += G) {
+                const int qk = q / nj, ql = q % nj;
+                const int kb = ib + qk + (qk >= xs ? 1 : 0), lb = jb + ql + (ql >= xr ? 1 : 0);
+                const int2 ek = bij[kb], e = bij[lb];
+                if (e.x == ek.x) {
+                    // l == k at the same image
+                    int ak, bk, ck, al, bl, cl;
+                    dc_unpack(ek.y, ak, bk, ck);
+                    dc_unpack(e.y, al, bl, cl);
+                    if (a0 + al == ak && b0 + bl == bk && c0 + cl == ck) continue;
+                }
+                double fk[3], fl[3];
+                bdh += rb_torsion_term(bond64[kb], bond64[lb], B.n, B.len, tij, fa, B.fc, fi, fj, fk, fl, wij);
+                fx_add3(facc, ek.x, fk, bad);
+                fx_add3(facc, e.x, fl, bad);
+            }
+        }
+        bdh = group_sum<G>(bdh);
 #pragma unroll
         for (int x = 0; x < 3; ++x) {
-            fi[x] += df[x];
-            fj[x] -= df[x];
+            fi[x] = group_sum<G>(fi[x]);
+            fj[x] = group_sum<G>(fj[x]);
         }
-        double wt[9];
 #pragma unroll
-        for (int y = 0; y < 3; ++y)
+        for (int q = 0; q < 9; ++q) wij[q] = group_sum<G>(wij[q]);
+        if (on && lane == 0) {
+            // coefficients of the neighbour pass
+            const double h = 0.5 * B.fc * fa * bdh;
+            const double4 ci = coef[s], cj = coef[sji];
+            const double dn = h * (double)tv.w;
+            coef[s] = make_double4(ci.x + h * (double)tv.y, ci.y + 2 * dn * ci.z, ci.z, 1.0);
+            coef[sji] = make_double4(cj.x + h * (double)tv.z, cj.y + 2 * dn * cj.z, cj.z, 1.0);
+            // the pair energy's torsion share: b += T sum / 2, no repulsion
+            double hlf, df[3];
+            rb_radial(B, fa, dfa, 0.0, 0.0, 0.5 * tij * bdh, hlf, df);
+            const double epair = hlf + hlf;
 #pragma unroll
             for (int x = 0; x < 3; ++x) {
-                wt[x + 3 * y] = (B.len * B.n[x]) * df[y];
-                wij[x + 3 * y] += wt[x + 3 * y];
+                fi[x] += df[x];
+                fj[x] -= df[x];
             }
-        fx_add3(facc, i, fi, bad);
-        fx_add3(facc, j, fj, bad);
-        if (bsrc) {
-            const size_t e = (size_t)bsrc[s];
-            if (epb) epb[e] += epair;
-            if (fpb)
-                for (int x = 0; x < 3; ++x) fpb[3 * e + x] += df[x];
-            if (wpb)
-                for (int q = 0; q < 9; ++q) wpb[9 * e + q] += wij[q];
-        }
-        if (eacc) {
-            fx_add(eacc + i, hlf, bad);
-            fx_add(eacc + j, hlf, bad);
-        }
-        if (wacc) {
-            for (int q = 0; q < 9; ++q) {
-                fx_add(wacc + 9 * (size_t)i + q, wij[q] / 2, bad);
-                fx_add(wacc + 9 * (size_t)j + q, wij[q] / 2, bad);
+#pragma unroll
+            for (int y = 0; y < 3; ++y)
+#pragma unroll
+                for (int x = 0; x < 3; ++x) wij[x + 3 * y] += (B.len * B.n[x]) * df[y];
+            fx_add3(facc, i, fi, bad);
+            fx_add3(facc, j, fj, bad);
+            if (bsrc) {
+                const size_t e = (size_t)bsrc[s];
+                if (epb) epb[e] += epair;
+                if (fpb)
+                    for (int x = 0; x < 3; ++x) fpb[3 * e + x] += df[x];
+                if (wpb)
+                    for (int q = 0; q < 9; ++q) wpb[9 * e + q] += wij[q];
             }
+            if (eacc) {
+                fx_add(eacc + i, hlf, bad);
+                fx_add(eacc + j, hlf, bad);
+            }
+            if (wacc) {
+                for (int q = 0; q < 9; ++q) {
+                    fx_add(wacc + 9 * (size_t)i + q, wij[q] / 2, bad);
+                    fx_add(wacc + 9 * (size_t)j + q, wij[q] / 2, bad);
+                }
+            }
+            e_run += epair;
+#pragma unroll
+            for (int q = 0; q < 9; ++q) w_run[q] += wij[q];
         }
     }
     if (bad) atomicOr(flags, ((bad & 1) ? DERR_FIXED_RANGE : 0) | ((bad & 2) ? DERR_BOND_OVERFLOW : 0));
-    rb_block_record(epair, wij, partial, blockIdx.x);
+    rb_block_record(e_run, w_run, partial, blockIdx.x);
 }
 
 // Pass 3, thread per directed bond c->o of a half pair (two threads per
@@ -1386,13 +1518,16 @@ k_rb_centre(int nat, const RbDevT<R> *__restrict__ Dp, const int *__restrict__ e
             const int *__restrict__ bcnt, const RbBondT<R> *__restrict__ bond, const int2 *__restrict__ bij,
             const double2 *__restrict__ nn, const Rb4<double> *__restrict__ bo, const double4 *__restrict__ coef,
             double *__restrict__ fnb, double *__restrict__ own, double *__restrict__ bconj,
-            double *__restrict__ partial)
+            double *__restrict__ partial, int nlo)
 {
     const RbDevT<R> &D = *Dp;
     const int l = threadIdx.x % G;
     const int c = blockIdx.x * (256 / G) + threadIdx.x / G;
-    const bool cv = c < nat;
-    const int nb = cv ? bcnt[c] : 0;
+    // this launch takes the centres of nlo < bonds <= G (the others belong
+    // to the launch of their width)
+    const int nb0 = c < nat ? bcnt[c] : -1;
+    const bool cv = nb0 > nlo && nb0 <= G;
+    const int nb = cv ? nb0 : 0;
     const int beg = cv ? offsets[c] : 0;
     const int kc = cv ? el[c] : 0;
     const bool has = l < nb;
@@ -1530,12 +1665,13 @@ template <typename R, int G>
 __global__ void __launch_bounds__(256)
 k_rb_conj_force(int nat, const int *__restrict__ offsets, const int *__restrict__ bcnt,
                 const RbBondT<R> *__restrict__ bond, const int *__restrict__ rev, const double *__restrict__ bconj,
-                double *__restrict__ fcj, double *__restrict__ own2, double *__restrict__ partial)
+                double *__restrict__ fcj, double *__restrict__ own2, double *__restrict__ partial, int nlo)
 {
     const int l = threadIdx.x % G;
     const int k = blockIdx.x * (256 / G) + threadIdx.x / G;
-    const bool cv = k < nat;
-    const int nb = cv ? bcnt[k] : 0;
+    const int nb0 = k < nat ? bcnt[k] : -1;
+    const bool cv = nb0 > nlo && nb0 <= G;
+    const int nb = cv ? nb0 : 0;
     const bool has = l < nb;
     const int b = (cv ? offsets[k] : 0) + l;
     const double Bv = has ? bconj[rev[b]] : 0.0;
@@ -1719,13 +1855,17 @@ static int rebo2_kernels(Rebo2State *st, agpu_potential *pot, const agpu_eval_ar
     // or per-bond virial is asked for (those split each pair's virial over
     // its atoms and bond, which the centre sums do not keep); otherwise a
     // thread per directed bond with fixed-point accumulation
+    // centres are taken by groups of 4 lanes (up to 4 bonds), then 8 and 16
+    // lanes for the wider rows, one launch per width present
     const int mr = nl->maxrow;
-    const int G = mr <= 4 ? 4 : (mr <= 8 ? 8 : (mr <= 16 ? 16 : 0));
-    const bool by_centre = !wacc && !a->wpot_per_bond && G > 0;
-    const int nbc = G > 0 ? (nat + 256 / G - 1) / (256 / G) : 0;
+    const bool by_centre = !wacc && !a->wpot_per_bond && mr <= 16;
+    const int nb4 = (nat + 63) / 64, nb8 = mr > 4 ? (nat + 31) / 32 : 0, nb16 = mr > 8 ? (nat + 15) / 16 : 0;
+    const int nbc = nb4 + nb8 + nb16;
     // block records: pair, torsion, then the neighbour pass (2 nblk, or
     // centre and conjugation passes)
-    const size_t nrec = by_centre ? 2 * (size_t)nblk + 2 * (size_t)nbc : 4 * (size_t)nblk;
+    // torsion groups: 64 per block, the grid striding over the pairs
+    const int ntb = std::max(1, std::min((ntot + 63) / 64, 2048));
+    const size_t nrec = (size_t)nblk + (size_t)ntb + (by_centre ? 2 * (size_t)nbc : 2 * (size_t)nblk);
     AGPU_CHECK(st->partial.reserve(sizeof(double) * 10 * nrec));
     AGPU_CHECK(st->coef.reserve(sizeof(double4) * (total + 1)));
     AGPU_CHECK(st->rev.reserve(sizeof(int) * (total + 1)));
@@ -1751,8 +1891,16 @@ static int rebo2_kernels(Rebo2State *st, agpu_potential *pot, const agpu_eval_ar
     }
     // pass records: pair (nblk blocks), torsion (nblk), neighbours (2 nblk)
     const bool tors = st->hdev.with_dihedral != 0;
-    if (tors) AGPU_CHECK(st->tor.reserve(sizeof(Rb4<R>) * (total + 1)));
+    if (tors) {
+        AGPU_CHECK(st->tor.reserve(sizeof(Rb4<R>) * (total + 1)));
+        AGPU_CHECK(st->tflag.reserve(sizeof(int) * (total + 1)));
+        AGPU_CHECK(st->tpos.reserve(sizeof(int) * (total + 1)));
+        AGPU_CHECK(st->tlist.reserve(sizeof(int) * (total + 2)));
+    }
     Rb4<R> *tor = tors ? st->tor.as<Rb4<R>>() : nullptr;
+    int *tflag = tors ? st->tflag.as<int>() : nullptr;
+    int *tlist = tors ? st->tlist.as<int>() : nullptr;
+    int *tcount = tors ? tlist + total + 1 : count;
     double4 *coef = st->coef.as<double4>();
     double *part = st->partial.as<double>();
     double *wdir = nullptr;
@@ -1763,11 +1911,22 @@ static int rebo2_kernels(Rebo2State *st, agpu_potential *pot, const agpu_eval_ar
     const int *list = st->list.as<int>();
     AGPU_LAUNCH(f64 ? "rebo2_pair" : "rebo2_pair_mixed", k_rb_pair<R>, dim3(nblk), dim3(256), 0, s, list, count,
                 D, tab, els, offs, st->bcnt.as<int>(), st->bown.as<int>(), bondR, st->bij.as<int2>(),
-                st->nn.as<double2>(), st->nconj.as<double>(), bo, st->rev.as<int>(), coef, tor,
+                st->nn.as<double2>(), st->nconj.as<double>(), bo, st->rev.as<int>(), coef, tor, tflag, ntot,
                 st->facc.as<unsigned long long>(), eacc, wacc, part, dstats, bsrc, a->epot_per_bond,
                 a->f_per_bond, a->wpot_per_bond, D64, bond);
+    if (tors && ntot > 0) {
+        // the half pairs with T != 0, in list order (flags + scan)
+        size_t tmp = 0;
+        AGPU_HIP(rocprim::exclusive_scan(nullptr, tmp, tflag, st->tpos.as<int>(), 0, (size_t)ntot,
+                                         rocprim::plus<int>(), s));
+        AGPU_CHECK(st->scan_tmp.reserve(tmp + 16));
+        AGPU_HIP(rocprim::exclusive_scan(st->scan_tmp.ptr, tmp, tflag, st->tpos.as<int>(), 0, (size_t)ntot,
+                                         rocprim::plus<int>(), s));
+        AGPU_LAUNCH("rebo2_tlist", k_rb_list, dim3(nblk), dim3(256), 0, s, ntot, tflag, st->tpos.as<int>(),
+                    tlist, tcount);
+    }
     // without torsion (tor == nullptr) the pass only writes zero records
-    AGPU_LAUNCH("rebo2_torsion", k_rb_torsion<R>, dim3(nblk), dim3(256), 0, s, list, count, offs,
+    AGPU_LAUNCH("rebo2_torsion", k_rb_torsion<R>, dim3(ntb), dim3(256), 0, s, list, tlist, tcount, offs,
                 st->bcnt.as<int>(), st->bown.as<int>(), st->bij.as<int2>(), st->rev.as<int>(), tor, coef,
                 st->facc.as<unsigned long long>(), eacc, wacc, part + 10 * (size_t)nblk, dstats, bsrc,
                 a->epot_per_bond, a->f_per_bond, a->wpot_per_bond, D64, bond);
@@ -1777,30 +1936,47 @@ static int rebo2_kernels(Rebo2State *st, agpu_potential *pot, const agpu_eval_ar
         AGPU_CHECK(st->bconj.reserve(sizeof(double) * (total + 1)));
         AGPU_CHECK(st->own.reserve(sizeof(double) * 6 * (nat + 1)));
         double *own = st->own.as<double>(), *own2 = own + 3 * (size_t)nat;
-        double *pc = part + 20 * (size_t)nblk, *pj = pc + 10 * (size_t)nbc;
-        auto centre = G == 4 ? k_rb_centre<R, 4> : (G == 8 ? k_rb_centre<R, 8> : k_rb_centre<R, 16>);
-        auto conjf = G == 4 ? k_rb_conj_force<R, 4> : (G == 8 ? k_rb_conj_force<R, 8> : k_rb_conj_force<R, 16>);
-        AGPU_LAUNCH(f64 ? "rebo2_centre" : "rebo2_centre_mixed", centre, dim3(nbc), dim3(256), 0, s, nat, D, els,
-                    offs, st->bcnt.as<int>(), bondR, st->bij.as<int2>(), st->nn.as<double2>(), bo, coef,
-                    st->fnb.as<double>(), own, st->bconj.as<double>(), pc);
-        AGPU_LAUNCH(f64 ? "rebo2_conj_force" : "rebo2_conj_force_mixed", conjf, dim3(nbc), dim3(256), 0, s, nat,
-                    offs, st->bcnt.as<int>(), bondR, st->rev.as<int>(), st->bconj.as<double>(),
-                    st->fcj.as<double>(), own2, pj);
+        double *pc = part + 10 * ((size_t)nblk + ntb), *pj = pc + 10 * (size_t)nbc;
+        const int nbw[3] = {nb4, nb8, nb16};
+        const int lo[3] = {-1, 4, 8};
+        auto cen = [&](int w) {
+            return w == 0 ? k_rb_centre<R, 4> : (w == 1 ? k_rb_centre<R, 8> : k_rb_centre<R, 16>);
+        };
+        auto cjf = [&](int w) {
+            return w == 0 ? k_rb_conj_force<R, 4> : (w == 1 ? k_rb_conj_force<R, 8> : k_rb_conj_force<R, 16>);
+        };
+        size_t off = 0;
+        for (int w = 0; w < 3; ++w) {
+            if (nbw[w] == 0) continue;
+            AGPU_LAUNCH(f64 ? "rebo2_centre" : "rebo2_centre_mixed", cen(w), dim3(nbw[w]), dim3(256), 0, s, nat, D,
+                        els, offs, st->bcnt.as<int>(), bondR, st->bij.as<int2>(), st->nn.as<double2>(), bo, coef,
+                        st->fnb.as<double>(), own, st->bconj.as<double>(), pc + 10 * off, lo[w]);
+            off += nbw[w];
+        }
+        // the bond scalars of every width are in place before any conjugation force
+        off = 0;
+        for (int w = 0; w < 3; ++w) {
+            if (nbw[w] == 0) continue;
+            AGPU_LAUNCH(f64 ? "rebo2_conj_force" : "rebo2_conj_force_mixed", cjf(w), dim3(nbw[w]), dim3(256), 0, s,
+                        nat, offs, st->bcnt.as<int>(), bondR, st->rev.as<int>(), st->bconj.as<double>(),
+                        st->fcj.as<double>(), own2, pj + 10 * off, lo[w]);
+            off += nbw[w];
+        }
         AGPU_LAUNCH("rebo2_gather", k_rb_gather, dim3(nb), dim3(256), 0, s, nat, order, offs, st->bcnt.as<int>(),
                     st->rev.as<int>(), own, own2, st->fnb.as<double>(), st->fcj.as<double>(),
                     st->facc.as<unsigned long long>(), eacc, a->f, a->epot_per_at);
-        *nblk_out = 2 * nblk + 2 * nbc;
+        *nblk_out = nblk + ntb + 2 * nbc;
         *gathered = true;
         return AGPU_OK;
     }
     AGPU_LAUNCH(f64 ? "rebo2_nbrs" : "rebo2_nbrs_mixed", k_rb_nbrs<R>, dim3(2 * nblk), dim3(256), 0, s, list,
                 count, D, els, offs, st->bcnt.as<int>(), st->bown.as<int>(), bondR, st->bij.as<int2>(),
                 st->nn.as<double2>(), bo, st->rev.as<int>(), coef, st->facc.as<unsigned long long>(), wacc,
-                part + 20 * (size_t)nblk, dstats, wdir);
+                part + 10 * ((size_t)nblk + ntb), dstats, wdir);
     if (wdir)
         AGPU_LAUNCH("rebo2_wpb", k_rb_wpb, dim3(nblk), dim3(256), 0, s, list, count, bsrc, wdir,
                     a->wpot_per_bond);
-    *nblk_out = 4 * nblk;
+    *nblk_out = 3 * nblk + ntb;
     *gathered = false;
     return AGPU_OK;
 }

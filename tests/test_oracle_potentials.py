@@ -46,6 +46,42 @@ def test_tersoff_diamond_closed_form():
     assert np.abs(f).max() < 1e-10
 
 
+@pytest.mark.parametrize("r", [2.55, 2.71, 2.8, 2.9, 2.99, 3.01])
+def test_tersoff_cutoff_shell_closed_form(r):
+    """The cut-off shell [R, S] = [2.7, 3.0] of Si (trig_off, cutoff.f90:
+    171-196: fc = (1 + cos(pi (r - R) / (S - R))) / 2): a dimer (zeta = 0,
+    b = 1) has E = fc (A exp(-lambda r) - B exp(-mu r)) and the bond force
+    -dE/dr; an equilateral trimer of side r has zeta = fc g(cos 60 deg) for
+    every bond, E = 3 fc (A exp(-lambda r) - b B exp(-mu r))."""
+    A, B, lam, mu = 1830.8, 471.18, 2.4799, 1.7322
+    beta, n, c, d, h = 1.1e-6, 0.78734, 1.0039e5, 16.217, -0.59825
+    R, S = 2.7, 3.0
+    if r <= R:
+        fc, dfc = 1.0, 0.0
+    elif r >= S:
+        fc, dfc = 0.0, 0.0
+    else:
+        x = math.pi * (r - R) / (S - R)
+        fc, dfc = 0.5 * (1 + math.cos(x)), -0.5 * math.pi / (S - R) * math.sin(x)
+    er, ea = A * math.exp(-lam * r), B * math.exp(-mu * r)
+    cell = np.eye(3) * 30.0
+    pbc = np.array([False, False, False])
+    # dimer along a general direction
+    u = np.array([0.36, -0.48, 0.8])
+    pos = np.array([[10.0, 11.0, 12.0], [10.0, 11.0, 12.0]]) + np.outer([0.0, 1.0], u * r)
+    e, f, w = tersoff_E(pos, cell, pbc)
+    assert abs(e - fc * (er - ea)) < 1e-12
+    dedr = dfc * (er - ea) + fc * (-lam * er + mu * ea)
+    assert np.abs(f[1] + dedr * u).max() < 1e-10 and np.abs(f[0] - dedr * u).max() < 1e-10
+    # equilateral trimer in the xy plane
+    pos3 = np.array([[10.0, 10.0, 10.0], [10.0 + r, 10.0, 10.0], [10.0 + r / 2, 10.0 + r * math.sqrt(3) / 2, 10.0]])
+    g = 1 + c * c / (d * d) - c * c / (d * d + (h - 0.5) ** 2)
+    zeta = fc * g
+    b = (1 + beta ** n * zeta ** n) ** (-1 / (2 * n)) if zeta > 0 else 1.0
+    e3 = tersoff_E(pos3, cell, pbc)[0]
+    assert abs(e3 - 3 * fc * (er - b * ea)) < 1e-11
+
+
 def test_tersoff_si100_surface_energy():
     # tests/test_pbc.py:41-59 (Diamond Si a=5.432, 2x2x2, surface along z)
     pos, cell, pbc, Z = W.diamond_si(2, a=5.432, sigma=0.0)
