@@ -855,11 +855,12 @@ __device__ __forceinline__ void rb_neighbours(const RbDevT<R> &D, int centre, in
             dd = dd + dk[1] * dk[1];
             dd = dd + dk[2] * dk[2];
             const R dis = sqrt(dd);
+            const R rdis = R(1.0) / dis, rlikr = R(1.0) / rlik;
 #pragma unroll
-            for (int x = 0; x < 3; ++x) dk[x] = dk[x] / dis;
-            const R dcs_io = R(1.0) / rlik - costh * rlijr;
-            const R dcs_ik = rlijr - costh / rlik;
-            const R dcs_ok = -dis * rlijr / rlik;
+            for (int x = 0; x < 3; ++x) dk[x] = dk[x] * rdis;
+            const R dcs_io = rlikr - costh * rlijr;
+            const R dcs_ik = rlijr - costh * rlikr;
+            const R dcs_ok = -dis * rlijr * rlikr;
             const R dzfac = fcik * gd * qf;
             const R dzdro = gf * fcik * qd;
             const R dzdrk = gf * (B.dfc * qf - fcik * qd);
@@ -895,109 +896,6 @@ __device__ __forceinline__ void outer_add(double *w, const R *a, const R *b)
     for (int y = 0; y < 3; ++y)
 #pragma unroll
         for (int x = 0; x < 3; ++x) w[x + 3 * y] = w[x + 3 * y] + (double)(a[x] * b[y]);
-}
-
-// torsion sum over k (neighbours of i but j) and l (neighbours of j but i,
-// and not k): returns bdh = sum (1 - cos^2 w) fc_ik fc_jl; forces on i, j
-// into fi, fj, on k and l through the accumulators; virial into w
-// (bop_kernel_rebo2.f90:1939-2120)
-// bond_at(b, 0) / bond_at(b, 1): the record of bond b of i / of j
-template <typename R, typename F>
-__device__ R rb_torsion(int ib, int inb, int sij, int jb, int jnb, int sji, int dcij, F bond_at,
-                        const int2 *__restrict__ bij, const R *rnij, R rlij, R tij, R fa, R fcar, double *fi,
-                        double *fj, double *w, unsigned long long *__restrict__ facc, int &bad)
-{
-    const R tije = tij * fa * fcar;
-    R bdh = R(0.0);
-    int a0, b0, c0;
-    dc_unpack(dcij, a0, b0, c0);
-    for (int kb = ib; kb < ib + inb; ++kb) {
-        if (kb == sij) continue;
-        const RbBondT<R> K = bond_at(kb, 0);
-        const int2 ek = bij[kb];
-        int ak, bk, ck;
-        dc_unpack(ek.y, ak, bk, ck);
-        const R *rnik = K.n;
-        const R rlik = K.len;
-        R dot_ij_ik = R(0.0);
-        dot_ij_ik = dot_ij_ik + rnij[0] * rnik[0];
-        dot_ij_ik = dot_ij_ik + rnij[1] * rnik[1];
-        dot_ij_ik = dot_ij_ik + rnij[2] * rnik[2];
-        const R dcik = R(1.0) - dot_ij_ik * dot_ij_ik;
-        R fk[3] = {0, 0, 0};
-        for (int lb = jb; lb < jb + jnb; ++lb) {
-            if (lb == sji) continue;
-            const int2 e = bij[lb];
-            if (e.x == ek.x) {
-                int al, bl, cl;
-                dc_unpack(e.y, al, bl, cl);
-                if (a0 + al == ak && b0 + bl == bk && c0 + cl == ck) continue;
-            }
-            const RbBondT<R> L = bond_at(lb, 1);
-            const R *rnjl = L.n;
-            const R rljl = L.len;
-            R dot_ij_jl = R(0.0), dot_ik_jl = R(0.0);
-            dot_ij_jl = dot_ij_jl + rnij[0] * rnjl[0];
-            dot_ij_jl = dot_ij_jl + rnij[1] * rnjl[1];
-            dot_ij_jl = dot_ij_jl + rnij[2] * rnjl[2];
-            dot_ik_jl = dot_ik_jl + rnik[0] * rnjl[0];
-            dot_ik_jl = dot_ik_jl + rnik[1] * rnjl[1];
-            dot_ik_jl = dot_ik_jl + rnik[2] * rnjl[2];
-            const R dcjl = R(1.0) - dot_ij_jl * dot_ij_jl;
-            const R abs_dc = sqrt(dcik * dcjl);
-            const R cost = (dot_ij_ik * dot_ij_jl - dot_ik_jl) / abs_dc;
-            R bdhij = 1 - cost * cost;
-            bdh = bdh + bdhij * K.fc * L.fc;
-            bdhij = bdhij * tij * fa * fcar / 2;
-            const R dbdhij = -2 * cost * tije * K.fc * L.fc / 2;
-            R df[3], v[3], fl[3];
-#pragma unroll
-            for (int x = 0; x < 3; ++x)
-                df[x] = dbdhij *
-                        ((dot_ij_jl / abs_dc + cost * dot_ij_ik / dcik) * rnik[x] +
-                         (dot_ij_ik / abs_dc + cost * dot_ij_jl / dcjl) * rnjl[x] -
-                         (2 * dot_ik_jl / abs_dc + cost * (R(1.0) / dcik + R(1.0) / dcjl)) * rnij[x]) /
-                        rlij;
-#pragma unroll
-            for (int x = 0; x < 3; ++x) {
-                fi[x] += df[x];
-                fj[x] -= df[x];
-                v[x] = rlij * rnij[x];
-            }
-            outer_add(w, v, df);
-#pragma unroll
-            for (int x = 0; x < 3; ++x)
-                df[x] = dbdhij *
-                            (-R(1.0) / dcik * cost * rnik[x] - R(1.0) / abs_dc * rnjl[x] +
-                             (dot_ij_jl / abs_dc + cost * dot_ij_ik / dcik) * rnij[x]) /
-                            rlik +
-                        bdhij * K.dfc * L.fc * rnik[x];
-#pragma unroll
-            for (int x = 0; x < 3; ++x) {
-                fi[x] += df[x];
-                fk[x] -= df[x];
-                v[x] = rlik * rnik[x];
-            }
-            outer_add(w, v, df);
-#pragma unroll
-            for (int x = 0; x < 3; ++x)
-                df[x] = dbdhij *
-                            (-R(1.0) / dcjl * cost * rnjl[x] - R(1.0) / abs_dc * rnik[x] +
-                             (dot_ij_ik / abs_dc + cost * dot_ij_jl / dcjl) * rnij[x]) /
-                            rljl +
-                        bdhij * K.fc * L.dfc * rnjl[x];
-#pragma unroll
-            for (int x = 0; x < 3; ++x) {
-                fj[x] += df[x];
-                fl[x] = -df[x];
-                v[x] = rljl * rnjl[x];
-            }
-            outer_add(w, v, df);
-            fx_add3(facc, e.x, fl, bad);
-        }
-        fx_add3(facc, ek.x, fk, bad);
-    }
-    return bdh;
 }
 
 // Bond order of every directed bond c->o (thread per bond slot): zeta with
@@ -1258,19 +1156,20 @@ __device__ __forceinline__ double rb_torsion_term(const RbBond &K, const RbBond 
     const double dcik = 1.0 - dot_ij_ik * dot_ij_ik;
     const double dcjl = 1.0 - dot_ij_jl * dot_ij_jl;
     const double abs_dc = sqrt(dcik * dcjl);
-    const double cost = (dot_ij_ik * dot_ij_jl - dot_ik_jl) / abs_dc;
+    // the reciprocals once (the terms below are products with them)
+    const double ra = 1.0 / abs_dc, rik_ = 1.0 / dcik, rjl_ = 1.0 / dcjl;
+    const double cost = (dot_ij_ik * dot_ij_jl - dot_ik_jl) * ra;
     double bdhij = 1 - cost * cost;
     const double term = bdhij * K.fc * L.fc;
     bdhij = bdhij * tij * fa * fcar / 2;
     const double dbdhij = -2 * cost * tije * K.fc * L.fc / 2;
+    const double cij = dbdhij / rlij, cik = dbdhij / rlik, cjl = dbdhij / rljl;
+    const double a_ik = dot_ij_jl * ra + cost * dot_ij_ik * rik_;     // coefficient of n_ik in dcos/dn
+    const double a_jl = dot_ij_ik * ra + cost * dot_ij_jl * rjl_;     // coefficient of n_jl
     double df[3], v[3];
 #pragma unroll
     for (int x = 0; x < 3; ++x)
-        df[x] = dbdhij *
-                ((dot_ij_jl / abs_dc + cost * dot_ij_ik / dcik) * rnik[x] +
-                 (dot_ij_ik / abs_dc + cost * dot_ij_jl / dcjl) * rnjl[x] -
-                 (2 * dot_ik_jl / abs_dc + cost * (1.0 / dcik + 1.0 / dcjl)) * rnij[x]) /
-                rlij;
+        df[x] = cij * (a_ik * rnik[x] + a_jl * rnjl[x] - (2 * dot_ik_jl * ra + cost * (rik_ + rjl_)) * rnij[x]);
 #pragma unroll
     for (int x = 0; x < 3; ++x) {
         fi[x] += df[x];
@@ -1280,11 +1179,7 @@ __device__ __forceinline__ double rb_torsion_term(const RbBond &K, const RbBond 
     outer_add(w, v, df);
 #pragma unroll
     for (int x = 0; x < 3; ++x)
-        df[x] = dbdhij *
-                    (-1.0 / dcik * cost * rnik[x] - 1.0 / abs_dc * rnjl[x] +
-                     (dot_ij_jl / abs_dc + cost * dot_ij_ik / dcik) * rnij[x]) /
-                    rlik +
-                bdhij * K.dfc * L.fc * rnik[x];
+        df[x] = cik * (-rik_ * cost * rnik[x] - ra * rnjl[x] + a_ik * rnij[x]) + bdhij * K.dfc * L.fc * rnik[x];
 #pragma unroll
     for (int x = 0; x < 3; ++x) {
         fi[x] += df[x];
@@ -1294,11 +1189,7 @@ __device__ __forceinline__ double rb_torsion_term(const RbBond &K, const RbBond 
     outer_add(w, v, df);
 #pragma unroll
     for (int x = 0; x < 3; ++x)
-        df[x] = dbdhij *
-                    (-1.0 / dcjl * cost * rnjl[x] - 1.0 / abs_dc * rnik[x] +
-                     (dot_ij_ik / abs_dc + cost * dot_ij_jl / dcjl) * rnij[x]) /
-                    rljl +
-                bdhij * K.fc * L.dfc * rnjl[x];
+        df[x] = cjl * (-rjl_ * cost * rnjl[x] - ra * rnik[x] + a_jl * rnij[x]) + bdhij * K.fc * L.dfc * rnjl[x];
 #pragma unroll
     for (int x = 0; x < 3; ++x) {
         fj[x] += df[x];
@@ -1610,11 +1501,12 @@ k_rb_centre(int nat, const RbDevT<R> *__restrict__ Dp, const int *__restrict__ e
                 dd = dd + dk[1] * dk[1];
                 dd = dd + dk[2] * dk[2];
                 const R dis = sqrt(dd);
+                const R rdis = R(1.0) / dis, rlikr = R(1.0) / rlik;
 #pragma unroll
-                for (int x = 0; x < 3; ++x) dk[x] = dk[x] / dis;
-                const R dcs_io = R(1.0) / rlik - costh * rlijr;
-                const R dcs_ik = rlijr - costh / rlik;
-                const R dcs_ok = -dis * rlijr / rlik;
+                for (int x = 0; x < 3; ++x) dk[x] = dk[x] * rdis;
+                const R dcs_io = rlikr - costh * rlijr;
+                const R dcs_ik = rlijr - costh * rlikr;
+                const R dcs_ok = -dis * rlijr * rlikr;
                 const R dzfac = K.fc * gd * qf;
                 const R dzdro = gf * K.fc * qd;
                 const R dzdrk = gf * (K.dfc * qf - K.fc * qd);

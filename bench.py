@@ -118,8 +118,8 @@ def algorithmic_bytes(kernel, nn):
         return 56 + 8 * nn
     if kernel == "pme_spread":
         return None                   # grid-point gather; traffic is per grid point
-    if kernel == "rebo2_pair":
-        return 64 + 8 * nn            # bond records stream once per bond-pair owner
+    if kernel.startswith("rebo2_"):
+        return 64 + 8 * nn            # the whole pass's minimum per atom: r, species, row, list, forces
     return None
 
 
