@@ -220,6 +220,19 @@ __device__ __forceinline__ double dot3_nc(const double *d)
     return s;
 }
 
+// The work block of this workgroup when consecutive work blocks share data
+// (neighbouring atoms in bin order): workgroups go to the 8 XCDs in turn and
+// each XCD has its own L2, so the first 8 floor(n / 8) workgroups are dealt
+// out so that XCD x takes one contiguous eighth of them; the rest keep their
+// index.  A bijection on [0, n).
+__device__ __forceinline__ int xcd_block()
+{
+    const int n = gridDim.x, b = blockIdx.x;
+    const int per = n / 8;
+    if (b >= 8 * per) return b;
+    return (b % 8) * per + b / 8;
+}
+
 // warp-level (wave64) helpers
 template <typename T> __device__ __forceinline__ T shfl_xor(T v, int m)
 {

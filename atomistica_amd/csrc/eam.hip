@@ -440,7 +440,7 @@ k_eam_density(EamDev P, const SplHdr *__restrict__ hdr, const R *__restrict__ re
     constexpr int APB = 256 / G;
     __shared__ double s_e[APB];
     const int lane = threadIdx.x % G, grp = threadIdx.x / G;
-    const int ai = blockIdx.x * APB + grp;
+    const int ai = xcd_block() * APB + grp;
     double Fi = 0.0;
     int err = 0;
     if (ai < nat) {
@@ -516,7 +516,7 @@ k_eam_force(EamDev P, const SplHdr *__restrict__ hdr, const R *__restrict__ rec,
     constexpr int APB = 256 / G;
     __shared__ double s_red[APB][7];
     const int lane = threadIdx.x % G, grp = threadIdx.x / G;
-    const int ai = blockIdx.x * APB + grp;
+    const int ai = xcd_block() * APB + grp;
     double acc[3] = {0.0, 0.0, 0.0}, e = 0.0;
     double w[6] = {0, 0, 0, 0, 0, 0};      // own-loop virial (xx yy zz yz xz xy)
     double wp[6] = {0, 0, 0, 0, 0, 0};     // per-atom virial

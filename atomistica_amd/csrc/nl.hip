@@ -319,7 +319,7 @@ k_bin_pairs(int B, BinGeom g, CellArg C, double cutoff_sq, const SortedAtom *__r
     NlShared &L = sh[threadIdx.x >> 6];
     const int lane = threadIdx.x & 63;
     const int nbx = (g.nc[0] + B - 1) / B, nby = (g.nc[1] + B - 1) / B, nbz = (g.nc[2] + B - 1) / B;
-    const int blk = blockIdx.x * NL_WAVES + (threadIdx.x >> 6);
+    const int blk = xcd_block() * NL_WAVES + (threadIdx.x >> 6);
     // every wavefront of the grid leaves its row-statistics record
     if (blk >= nbx * nby * nbz) {
         if (lane == 0) part[blk] = make_int2(0, 0);
@@ -582,7 +582,7 @@ k_atom_pairs(int nat, BinGeom g, CellArg C, double cutoff_sq, const SortedAtom *
              const float4 *__restrict__ pos32, const int *__restrict__ cell_start, int *__restrict__ counts,
              int2 *__restrict__ stage, int S, int *__restrict__ flags, int2 *__restrict__ part)
 {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int p = xcd_block() * blockDim.x + threadIdx.x;
     bool over = false, bad = false;
     int longest = 0, last = 0;
     if (p < nat) {
@@ -639,6 +639,53 @@ k_atom_pairs(int nat, BinGeom g, CellArg C, double cutoff_sq, const SortedAtom *
         const int nz = g.nc[2], ndz = g.nd[2];
         const bool runs = !g.pbc[2] || 2 * ndz + 1 <= nz;
         const int zlo = hc[2] - ndz, zhi = hc[2] + ndz;
+        if (runs && g.nd[0] == 1 && g.nd[1] == 1 && ndz == 1) {
+            // a 3 x 3 x 3 stencil (bins of at least the cut-off): the nine
+            // column ranges are read first, all loads in flight together,
+            // then scanned in stencil order
+            const int m0 = max(zlo, 1), m1 = min(zhi, nz);
+            // column k = 3 (ox + 1) + (oy + 1): its first bin (-1 outside an
+            // open cell) and image shifts
+            auto column = [&](int k, int &ux, int &uy) {
+                int cx = hc[0] + k / 3 - 1, cy = hc[1] + k % 3 - 1;
+                ux = 0;
+                uy = 0;
+                if (g.pbc[0]) {
+                    const int t = floordiv(cx - 1, g.nc[0]);
+                    cx -= t * g.nc[0];
+                    ux = -t;
+                } else if (cx < 1 || cx > g.nc[0]) {
+                    return -1;
+                }
+                if (g.pbc[1]) {
+                    const int t = floordiv(cy - 1, g.nc[1]);
+                    cy -= t * g.nc[1];
+                    uy = -t;
+                } else if (cy < 1 || cy > g.nc[1]) {
+                    return -1;
+                }
+                return ((cx - 1) * g.nc[1] + (cy - 1)) * nz;
+            };
+            int q0[9], q1[9];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) {
+                int ux, uy;
+                const int bc = column(k, ux, uy);
+                q0[k] = bc >= 0 ? cell_start[bc + m0 - 1] : 0;
+                q1[k] = bc >= 0 ? cell_start[bc + m1] : 0;
+            }
+            const bool tw = g.pbc[2] && zlo < 1, uw = g.pbc[2] && zhi > nz;
+#pragma unroll
+            for (int k = 0; k < 9; ++k) {
+                int ux, uy;
+                const int bc = column(k, ux, uy);
+                if (bc < 0) continue;
+                // wrapped from the top (u_z = +1), unwrapped, wrapped from the bottom (u_z = -1)
+                if (tw) scan(cell_start[bc + zlo + nz - 1], cell_start[bc + nz], ux, uy, 1);
+                scan(q0[k], q1[k], ux, uy, 0);
+                if (uw) scan(cell_start[bc], cell_start[bc + zhi - nz], ux, uy, -1);
+            }
+        } else
         for (int ox = -g.nd[0]; ox <= g.nd[0]; ++ox) {
             int cx = hc[0] + ox, ux = 0;
             if (g.pbc[0]) {

@@ -405,7 +405,7 @@ k_tersoff(TersoffDev P, int nat, const int *__restrict__ order, const double *__
     const int lane = tid % G;
     const int grp = tid / G;
     const int gbase = grp * G;
-    const int ai = blockIdx.x * GPB + grp;
+    const int ai = xcd_block() * GPB + grp;
     const bool valid = ai < nat;
     const int i = valid ? order[ai] : 0;
     const int eli = valid ? el[i] : -1;
@@ -717,7 +717,7 @@ k_bop_rows(TersoffDev P, int nat, const int *__restrict__ order, const double *_
     const int lane = tid % G;
     const int grp = tid / G;
     const int gbase = grp * G;
-    const int ai = blockIdx.x * GPB + grp;
+    const int ai = xcd_block() * GPB + grp;
     const bool valid = ai < nat;
     const int i = valid ? order[ai] : 0;
     const int eli = valid ? el[i] : -1;
@@ -1322,7 +1322,7 @@ k_bop_scr(TersoffDev P, ScrDev S, int nat, const int *__restrict__ order, const 
     const int lane = tid % G;
     const int grp = tid / G;
     const int gbase = grp * G;
-    const int ai = blockIdx.x * GPB + grp;
+    const int ai = xcd_block() * GPB + grp;
     const bool valid = ai < nat;
     const int i = valid ? order[ai] : 0;
     const int eli = valid ? el[i] : -1;
