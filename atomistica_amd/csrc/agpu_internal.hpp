@@ -111,8 +111,12 @@ struct agpu_neighbors {
     int maxrow = 0;
     // device storage
     agpu::DevBuf key, key_sorted, idx, idx_sorted, cell_start, atoms_sorted,
-        counts, offsets, nbr, sort_tmp, rowmax, rank, stage, pos32, part;
+        counts, offsets, nbr, sort_tmp, rowmax, stage, pos32, part, scnt, soffs;
     int stage_w = 0;              // staging stride (entries per atom)
+    // particles' pos_rev when the slot positions (atoms_sorted) were written
+    uint64_t slot_rev = ~0ull;
+    // offsets holds the caller-order row offsets of the last build
+    bool caller_off = false;
     bool radix = false;           // bin sort by radix sort (after a very full bin)
     // the list was imported from the caller (LAMMPS) and is not rebuilt
     bool external = false;
@@ -124,9 +128,10 @@ struct agpu_neighbors {
     // pair search kernel: 0 chosen from the bin occupancy, 1 thread per
     // atom, 2 wavefront per bin block (both give the same rows)
     int pair_mode = 0;
-    // the CSR rows (offsets, nbr) hold the last build; false while its rows
-    // are only staged (sorted slot p at stage + p*stage_w, counts[atom]
-    // entries) because every caller since the build read them there
+    // the CSR rows (offsets, nbr: caller indices) hold the last build; false
+    // while its rows are only staged (slot p's row at stage + p*stage_w,
+    // scnt[p] entries naming slots) because every caller since the build
+    // read them there
     bool csr = true;
     agpu::DevBuf cell_cnt;
     long long cnt_zero = 0;       // leading cell_cnt entries known to be zero
@@ -156,27 +161,53 @@ struct CellArg {
     double A[9];
 };
 
-// The rows of the current list for kernels that visit atoms in bin order
-// (atom i = idx_sorted[ai]): staged rows when S > 0 (row of slot ai at
-// rows + ai*S, len[i] entries), else CSR rows (row i at rows + len[i],
-// len[i + 1] - len[i] entries).  Entry order is the same in both.
+// An atom of the list's bin order ("slot"): its position as the caller gave
+// it, its caller index and the packed wrap counts of its bin.
+struct __align__(16) SortedAtom {
+    double x, y, z;
+    int j;       // caller index
+    int w;       // packed wrap counts of its bin (dc_pack layout)
+};
+
+// The rows of the current list in bin-order storage.  Atoms are named by
+// slot (caller atom order[p] at slot p), in the rows as well: an entry is
+// {slot of the neighbour, packed image}.  Row x has its entries at
+// rows + x*S (cnt[x] of them) when S > 0 (the builder's staging rows), else
+// at rows + off[x] (off[x + 1] - off[x] of them: CSR by slot, an imported
+// list, whose slots are the caller's atoms).  off is the exclusive scan of
+// the row lengths in slot order in both cases: entry k of row x is entry
+// off[x] + k of the per-entry arrays, which are dense and ordered by slot.
+// atoms[x] holds slot x's position, read where a kernel gathers neighbours.
 struct NlRows {
     const int2 *rows;
-    const int *len;
+    const int *cnt;
+    const int *off;
     int S;
+    const SortedAtom *atoms;
 };
+// the rows of the current list (nl_update brings the slot positions up to
+// date before it returns)
 NlRows nl_rows(const agpu_neighbors *nl);
+// the slot positions brought up to date without a rebuild (for callers that
+// do not go through nl_update)
+int nl_slots_current(agpu_neighbors *nl, const agpu_particles *p);
+// offsets of the caller-order rows (the reference's list layout: per-bond
+// arrays are indexed by them); computed on the first request after a build
+int nl_caller_offsets(agpu_neighbors *nl);
 
-__device__ __forceinline__ const int2 *nl_row(const NlRows &R, int ai, int i, int &n)
+__device__ __forceinline__ const int2 *nl_row(const NlRows &R, int x, int &n)
 {
+    const int b = R.off[x];
     if (R.S > 0) {
-        n = R.len[i];
-        return R.rows + (size_t)ai * R.S;
+        n = R.cnt[x];
+        return R.rows + (size_t)x * R.S;
     }
-    const int b = R.len[i];
-    n = R.len[i + 1] - b;
+    n = R.off[x + 1] - b;
     return R.rows + b;
 }
+
+// index of entry k of row x in the per-entry arrays
+__device__ __forceinline__ size_t nl_at(const NlRows &R, int x, int k) { return (size_t)R.off[x] + k; }
 
 __device__ __forceinline__ int dc_pack_ok(int a) { return a >= -512 && a <= 511; }
 __device__ __forceinline__ int dc_pack(int a, int b, int c)

@@ -124,7 +124,7 @@ struct EamState {
     EamDev dev{};
     int z2db[128];
     uint64_t Z_rev = ~0ull;
-    DevBuf d_hdr, d_rec64, d_rec32, dF, partial, stats, pk;
+    DevBuf d_hdr, d_rec64, d_rec32, partial, stats, pk;
     bool all_known = false;       // every atom is of a tabulated element
     // lanes per atom and entries per lane of the single-element kernels
     // (chosen by eam_autotune; the defaults otherwise)
@@ -416,17 +416,19 @@ __device__ __forceinline__ SplHdr hdr_of(const EamDev &P, const SplHdr *__restri
 #endif
 constexpr int EAM_U = AGPU_EAM_U;
 
-// Packed per-atom records of 32 bytes, so that one neighbour gather reads
-// one 32-byte sector: q1 = {x, y, z, element index} for the density pass,
-// q2 = {x, y, z, F'(rho)} written by the density pass for the force pass.
-// Also clears the call's four error/statistics words.
-__global__ void k_eam_pack(int nat, const double *__restrict__ r, const int *__restrict__ el,
-                           double4 *__restrict__ q1, int *__restrict__ flags)
+// Packed per-slot records of 32 bytes in the list's bin order (slot p holds
+// caller atom order[p]), so that one neighbour gather reads one 32-byte
+// sector near the home atom's own: q1 = {x, y, z, element index} for the
+// density pass, q2 = {x, y, z, F'(rho)} written by the density pass for the
+// force pass.  Also clears the call's four error/statistics words.
+__global__ void k_eam_pack(int nat, const SortedAtom *__restrict__ atoms, const int *__restrict__ order,
+                           const int *__restrict__ el, double4 *__restrict__ q1, int *__restrict__ flags)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < 4) flags[i] = 0;
-    if (i >= nat) return;
-    q1[i] = make_double4(r[3 * i], r[3 * i + 1], r[3 * i + 2], (double)el[i]);
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < 4) flags[p] = 0;
+    if (p >= nat) return;
+    const SortedAtom a = atoms[p];
+    q1[p] = make_double4(a.x, a.y, a.z, (double)el[order[p]]);
 }
 
 // KN: one element and every atom tabulated, so no element is read per pair.
@@ -434,7 +436,7 @@ template <typename R, int G, int U, bool ONE, bool KN>
 __global__ void __launch_bounds__(256)
 k_eam_density(EamDev P, const SplHdr *__restrict__ hdr, const R *__restrict__ rec, int nat,
               const int *__restrict__ order, const double4 *__restrict__ q1, CellArg C,
-              NlRows L, const int *__restrict__ mask, double *__restrict__ dF, double4 *__restrict__ q2, double *__restrict__ epa,
+              NlRows L, const int *__restrict__ mask, double4 *__restrict__ q2, double *__restrict__ epa,
               double *__restrict__ partial, int *__restrict__ flags)
 {
     constexpr int APB = 256 / G;
@@ -445,14 +447,14 @@ k_eam_density(EamDev P, const SplHdr *__restrict__ hdr, const R *__restrict__ re
     int err = 0;
     if (ai < nat) {
         const int i = order[ai];
-        const double4 qi = q1[i];
+        const double4 qi = q1[ai];
         const int eli = KN ? 0 : (int)qi.w;
         const bool on = eli >= 0 && !(mask && mask[i] == 0);
         double rho = 0.0;
         if (on) {
             const double ri[3] = {qi.x, qi.y, qi.z};
             int n;
-            const int2 *row = nl_row(L, ai, i, n);
+            const int2 *row = nl_row(L, ai, n);
             for (int s0 = lane; s0 < n; s0 += U * G) {
                 int2 e[U];
 #pragma unroll
@@ -462,7 +464,7 @@ k_eam_density(EamDev P, const SplHdr *__restrict__ hdr, const R *__restrict__ re
                 }
                 double4 qj[U];
 #pragma unroll
-                for (int u = 0; u < U; ++u) qj[u] = q1[e[u].x < 0 ? i : e[u].x];
+                for (int u = 0; u < U; ++u) qj[u] = q1[e[u].x < 0 ? ai : e[u].x];
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
                     const int elj = e[u].x < 0 ? -1 : (KN ? 0 : (int)qj[u].w);
@@ -488,8 +490,7 @@ k_eam_density(EamDev P, const SplHdr *__restrict__ hdr, const R *__restrict__ re
             dFi = (double)df;
         }
         if (lane == 0) {
-            dF[i] = dFi;
-            q2[i] = make_double4(qi.x, qi.y, qi.z, dFi);
+            q2[ai] = make_double4(qi.x, qi.y, qi.z, dFi);
             if (epa && on) epa[i] += Fi;
         }
     }
@@ -524,14 +525,14 @@ k_eam_force(EamDev P, const SplHdr *__restrict__ hdr, const R *__restrict__ rec,
     int i = -1;
     if (ai < nat) {
         i = order[ai];
-        const int eli = KN ? 0 : (int)q1[i].w;
+        const int eli = KN ? 0 : (int)q1[ai].w;
         if (eli >= 0) {
             const double mi = (GEN && mask && mask[i] == 0) ? 0.0 : 1.0;
-            const double4 qi = q2[i];
+            const double4 qi = q2[ai];
             const R dFi = (R)qi.w;
             const double ri[3] = {qi.x, qi.y, qi.z};
             int n;
-            const int2 *row = nl_row(L, ai, i, n);
+            const int2 *row = nl_row(L, ai, n);
             const SplHdr hri = hdr_of<ONE>(P, hdr, 1, eli, 0);
             for (int s0 = lane; s0 < n; s0 += U * G) {
                 int2 en[U];
@@ -545,10 +546,10 @@ k_eam_force(EamDev P, const SplHdr *__restrict__ hdr, const R *__restrict__ rec,
                 double mj[U];
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
-                    const int j = en[u].x < 0 ? i : en[u].x;
+                    const int j = en[u].x < 0 ? ai : en[u].x;
                     qj[u] = q2[j];
                     elj[u] = en[u].x < 0 ? -1 : (KN ? 0 : (int)q1[j].w);
-                    mj[u] = (GEN && mask && mask[j] == 0) ? 0.0 : 1.0;
+                    mj[u] = (GEN && mask && mask[order[j]] == 0) ? 0.0 : 1.0;
                 }
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
@@ -697,7 +698,7 @@ static int eam_passes(EamState *st, agpu_potential *pot, const agpu_eval_args *a
     const NlRows L = nl_rows(nl);
     AGPU_LAUNCH(mixed ? "eam_density_mixed" : "eam_density_fp64", (k_eam_density<R, G, U, ONE, KN>), dim3(nblk),
                 dim3(256), 0, s, st->dev, st->d_hdr.as<SplHdr>(), rec, nat, order, q1, C,
-                L, a->mask, st->dF.as<double>(), q2, a->epot_per_at, part, flags);
+                L, a->mask, q2, a->epot_per_at, part, flags);
     AGPU_LAUNCH(mixed ? "eam_force_mixed" : "eam_force_fp64", (k_eam_force<R, G, U, ONE, GEN, KN>), dim3(nblk),
                 dim3(256), 0, s, st->dev, st->d_hdr.as<SplHdr>(), rec, nat, order, q1, q2, C,
                 L, a->mask, a->f, a->epot_per_at, a->wpot_per_at, part, part + nblk, flags);
@@ -731,8 +732,9 @@ static int eam_launch(EamState *st, agpu_potential *pot, const agpu_eval_args *a
     if (G > EAM_GMAX) return set_error(AGPU_ERR_STATE, "EAM: lane group wider than the partial records allow");
     *nblk_out = nblk;
     const int *el = pot->d_el.as<int>();
-    AGPU_LAUNCH("eam_pack", k_eam_pack, dim3((nat + 255) / 256), dim3(256), 0, stream(), nat, pot->p->d_r, el,
-                st->pk.as<double4>(), st->stats.as<int>());
+    const agpu_neighbors *nl = pot->nl;
+    AGPU_LAUNCH("eam_pack", k_eam_pack, dim3((nat + 255) / 256), dim3(256), 0, stream(), nat,
+                nl_rows(nl).atoms, nl->idx_sorted.as<int>(), el, st->pk.as<double4>(), st->stats.as<int>());
     if constexpr (ONE && KN) {
         if (!gen) {
             switch (G * 100 + U) {
@@ -783,7 +785,6 @@ int eam_compute(agpu_potential *pot, const agpu_eval_args *a)
     // blocks of the widest lane group (EAM_GMAX lanes per atom): the most
     // blocks any variant launches
     const int nblk_max = (nat + 256 / EAM_GMAX - 1) / (256 / EAM_GMAX);
-    AGPU_CHECK(st->dF.reserve(sizeof(double) * (nat + 1)));
     // layout: nblk energy scalars of the density pass, then nblk 10-records
     AGPU_CHECK(st->partial.reserve(sizeof(double) * 11 * (size_t)nblk_max));
     AGPU_CHECK(st->stats.reserve(sizeof(int) * 4));

@@ -45,7 +45,7 @@ struct EwaldState {
     int nitems = 0;
     std::vector<EwItem> items;
     std::vector<int> kpack;
-    DevBuf d_items, d_kpack, d_S, d_T, d_frac, partial, phi_tmp;
+    DevBuf d_items, d_kpack, d_S, d_T, d_frac, partial, phi_tmp, q4;
     // smooth PME reciprocal sum instead of the direct S(k) sum (PME class)
     PmeState *pme = nullptr;
     ~EwaldState()
@@ -75,11 +75,21 @@ template <typename T> __device__ __forceinline__ T wsum(T v)
     return v;
 }
 
+// {x, y, z, q} per slot of the list's bin order (caller atom order[p])
+__global__ void k_ewald_pack(int nat, const SortedAtom *__restrict__ atoms, const int *__restrict__ order,
+                             const double *__restrict__ q, double4 *__restrict__ q4)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nat) return;
+    const SortedAtom a = atoms[p];
+    q4[p] = make_double4(a.x, a.y, a.z, q[order[p]]);
+}
+
 template <typename R>
 __global__ void __launch_bounds__(256)
-k_ewald_real(int nat, const int *__restrict__ order, const double *__restrict__ r,
-             const double *__restrict__ q, CellArg C, NlRows L, double cutoff_sq, double beta, double sqrt_alpha_pi,
-             double *__restrict__ f, double *__restrict__ phi, double *__restrict__ partial)
+k_ewald_real(int nat, const int *__restrict__ order, const double4 *__restrict__ q4, CellArg C, NlRows L,
+             double cutoff_sq, double beta, double sqrt_alpha_pi, double *__restrict__ f, double *__restrict__ phi,
+             double *__restrict__ partial)
 {
     __shared__ double s_red[4][10];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -90,20 +100,21 @@ k_ewald_real(int nat, const int *__restrict__ order, const double *__restrict__ 
     double qi = 0.0;
     if (ai < nat) {
         i = order[ai];
-        qi = q[i];
+        const double4 hi = q4[ai];
+        qi = hi.w;
         if (fabs(qi) > 1e-10) {
-            const double ri[3] = {r[3 * i], r[3 * i + 1], r[3 * i + 2]};
+            const double ri[3] = {hi.x, hi.y, hi.z};
             int n;
-            const int2 *row = nl_row(L, ai, i, n);
+            const int2 *row = nl_row(L, ai, n);
             const R b = (R)beta, sap = (R)sqrt_alpha_pi, al = (R)(beta * beta);
             for (int s = lane; s < n; s += 64) {
                 int2 en = row[s];
-                const int j = en.x;
-                const double qj = q[j];
+                const double4 qa = q4[en.x];
+                const double qj = qa.w;
                 if (!(fabs(qj) > 1e-10)) continue;
                 int a, bb, c;
                 dc_unpack(en.y, a, bb, c);
-                const double rj[3] = {r[3 * j], r[3 * j + 1], r[3 * j + 2]};
+                const double rj[3] = {qa.x, qa.y, qa.z};
                 double d[3];
                 delta_r(C, ri, rj, a, bb, c, d);
                 double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
@@ -465,12 +476,16 @@ static int ewald_run(agpu_potential *pot, const double *q, double *epot, double 
     AGPU_CHECK(st->partial.reserve(sizeof(double) * 10 * (size_t)(nblk_r + nblk_k + 1)));
     double *part = st->partial.as<double>();
     const int *order = nl->idx_sorted.as<int>();
+    const NlRows L = nl_rows(nl);
+    AGPU_CHECK(st->q4.reserve(sizeof(double4) * (size_t)(nat + 1)));
+    double4 *q4 = st->q4.as<double4>();
+    AGPU_LAUNCH("ewald_pack", k_ewald_pack, dim3((nat + 255) / 256), dim3(256), 0, s, nat, L.atoms, order, q, q4);
     if (pot->precision == AGPU_PREC_MIXED)
-        AGPU_LAUNCH("ewald_real_mixed", k_ewald_real<float>, dim3(nblk_r), dim3(256), 0, s, nat, order, p->d_r, q,
-                    C, nl_rows(nl), st->cutoff * st->cutoff, st->beta, sap, f, phi, part);
+        AGPU_LAUNCH("ewald_real_mixed", k_ewald_real<float>, dim3(nblk_r), dim3(256), 0, s, nat, order, q4, C, L,
+                    st->cutoff * st->cutoff, st->beta, sap, f, phi, part);
     else
-        AGPU_LAUNCH("ewald_real_fp64", k_ewald_real<double>, dim3(nblk_r), dim3(256), 0, s, nat, order, p->d_r, q,
-                    C, nl_rows(nl), st->cutoff * st->cutoff, st->beta, sap, f, phi, part);
+        AGPU_LAUNCH("ewald_real_fp64", k_ewald_real<double>, dim3(nblk_r), dim3(256), 0, s, nat, order, q4, C, L,
+                    st->cutoff * st->cutoff, st->beta, sap, f, phi, part);
     if (st->pme) {
         // reciprocal record after the real-space block records
         AGPU_CHECK(pme_recip(st->pme, p, q, st->beta, part + 10 * (size_t)nblk_r, f, phi));

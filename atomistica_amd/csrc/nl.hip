@@ -17,12 +17,17 @@
 //                  read contiguous records.
 //   4. nl_pairs:   the accepted (j, image) pairs of every atom over its
 //                  stencil (x-major, then y, then z; atoms of a bin ascending)
-//                  into a padded staging row of stride S, and the row counts.
-//                  k_bin_pairs (a wavefront per block of home bins, candidates
-//                  shared by the wavefront) when bins are dense, k_atom_pairs
-//                  (a thread per home atom) when they hold < 4 atoms.
-//   5. exclusive scan of the counts -> row offsets; k_nl_compact copies the
-//                  staging rows into the CSR list.
+//                  into a padded staging row of stride S per bin-order slot,
+//                  neighbours named by slot, and the row counts (by caller
+//                  atom and by slot).  k_bin_pairs (a wavefront per block of
+//                  home bins, candidates shared by the wavefront) when bins
+//                  are dense, k_atom_pairs (a thread per home atom) when they
+//                  hold < 4 atoms.  The force kernels read these rows and the
+//                  slot records directly (bin-order storage: a neighbour's
+//                  position is a slot record near the home atom's own).
+//   5. exclusive scan of the counts -> caller row offsets; k_nl_compact
+//                  copies the staging rows into the caller-order CSR list
+//                  (slots turned back into caller indices) when asked.
 // The distance test is the reference's FP64 expression without contraction,
 // so the accepted set is bit-identical.
 #include <algorithm>
@@ -37,12 +42,6 @@
 #include "profile.hpp"
 
 namespace agpu {
-
-struct __align__(16) SortedAtom {
-    double x, y, z;
-    int j;       // caller index
-    int w;       // packed wrap counts of its bin (dc_pack layout)
-};
 
 struct BinGeom {
     double rec[9];   // rec_cell_size, column-major image: rec[3*k + x]
@@ -135,7 +134,7 @@ __global__ void k_cell_fill(int nat, int ncell, const int *__restrict__ key, con
 // the bin-sorted record of atom j at slot p (and the optional FP32 wrapped
 // position r + A.w, with the wrap flag (w != 0) in bit 31 of .w)
 __device__ __forceinline__ void gather_atom(int p, int j, const double *__restrict__ r, const BinGeom &g,
-                                            const CellArg &C, SortedAtom *__restrict__ out, int *__restrict__ rank,
+                                            const CellArg &C, SortedAtom *__restrict__ out,
                                             float4 *__restrict__ pos32)
 {
     double rj[3] = {r[3 * (size_t)j], r[3 * (size_t)j + 1], r[3 * (size_t)j + 2]};
@@ -148,7 +147,6 @@ __device__ __forceinline__ void gather_atom(int p, int j, const double *__restri
     a.j = j;
     a.w = dc_pack(w[0], w[1], w[2]);
     out[p] = a;
-    rank[j] = p;
     if (pos32) {
         float q[3];
         for (int x = 0; x < 3; ++x) q[x] = (float)(rj[x] + image_x(C, x, w[0], w[1], w[2]));
@@ -165,7 +163,7 @@ template <int G>
 __global__ void __launch_bounds__(256)
 k_cell_order(int ncell, const int *__restrict__ cell_start, int *__restrict__ cnt, int *__restrict__ idx_sorted,
              int *__restrict__ flags, const double *__restrict__ r, BinGeom g, CellArg C,
-             SortedAtom *__restrict__ out, int *__restrict__ rank, float4 *__restrict__ pos32)
+             SortedAtom *__restrict__ out, float4 *__restrict__ pos32)
 {
     const int lane = threadIdx.x % G;
     const int c = blockIdx.x * (256 / G) + threadIdx.x / G;
@@ -178,7 +176,7 @@ k_cell_order(int ncell, const int *__restrict__ cell_start, int *__restrict__ cn
         // still holds one of the bin's atoms (in scatter order), so the
         // kernels after this one read valid records
         if (lane == 0) atomicOr(flags, DERR_SORT_FALLBACK);
-        for (int k = lane; k < n; k += G) gather_atom(b + k, idx_sorted[b + k], r, g, C, out, rank, pos32);
+        for (int k = lane; k < n; k += G) gather_atom(b + k, idx_sorted[b + k], r, g, C, out, pos32);
         return;
     }
     int v[4], rk[4] = {0, 0, 0, 0};
@@ -199,7 +197,7 @@ k_cell_order(int ncell, const int *__restrict__ cell_start, int *__restrict__ cn
         const int k = lane + G * q;
         if (k < n) {
             idx_sorted[b + rk[q]] = v[q];
-            gather_atom(b + rk[q], v[q], r, g, C, out, rank, pos32);
+            gather_atom(b + rk[q], v[q], r, g, C, out, pos32);
         }
     }
 }
@@ -207,11 +205,11 @@ k_cell_order(int ncell, const int *__restrict__ cell_start, int *__restrict__ cn
 // the sorted records after the radix sort (slot p -> atom idx_sorted[p])
 __global__ void k_gather(int nat, const double *__restrict__ r, BinGeom g, CellArg C,
                          const int *__restrict__ idx_sorted, SortedAtom *__restrict__ out,
-                         int *__restrict__ rank, float4 *__restrict__ pos32)
+                         float4 *__restrict__ pos32)
 {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= nat) return;
-    gather_atom(p, idx_sorted[p], r, g, C, out, rank, pos32);
+    gather_atom(p, idx_sorted[p], r, g, C, out, pos32);
 }
 
 constexpr int NL_R = 4;         // candidate slots per lane (registers): 256 per pass
@@ -306,14 +304,14 @@ __device__ __forceinline__ void row_stats_wave(int longest, int last, int2 *__re
 // sees exactly the reference's candidates in the reference's order
 // (python_neighbors.f90:664-735).  The ballot of accepted lanes ranks each
 // accepted pair inside the row; rows go to a padded staging area (stride S
-// per sorted slot) and are compacted afterwards.
+// per sorted slot, neighbours by slot).
 // MULTI: blocks of B > 1 bins, whose candidates are checked against each home
 // atom's own stencil
 template <bool ORTHO, bool MULTI>
 __global__ void __launch_bounds__(256)
 k_bin_pairs(int B, BinGeom g, CellArg C, double cutoff_sq, const SortedAtom *__restrict__ atoms,
-            const int *__restrict__ cell_start, int *__restrict__ counts, int2 *__restrict__ stage,
-            int S, int *__restrict__ flags, int2 *__restrict__ part)
+            const int *__restrict__ cell_start, int *__restrict__ counts, int *__restrict__ scnt,
+            int2 *__restrict__ stage, int S, int *__restrict__ flags, int2 *__restrict__ part)
 {
     __shared__ NlShared sh[NL_WAVES];
     NlShared &L = sh[threadIdx.x >> 6];
@@ -440,7 +438,7 @@ k_bin_pairs(int B, BinGeom g, CellArg C, double cutoff_sq, const SortedAtom *__r
                         cx[r] = A.x;
                         cy[r] = A.y;
                         cz[r] = A.z;
-                        cj[r] = A.j;
+                        cj[r] = src;
                         const int t0 = u0 - w0, t1 = u1 - w1, t2 = u2 - w2;
                         if (MULTI) co[r] = L.boff[b];
                         if (ORTHO) {
@@ -458,7 +456,6 @@ k_bin_pairs(int B, BinGeom g, CellArg C, double cutoff_sq, const SortedAtom *__r
                     // (scalar) load
                     const int hp = uniform(L.hp[a]);
                     const SortedAtom H = atoms[hp];
-                    const int ia = H.j;
                     int w0, w1, w2;
                     dc_unpack(H.w, w0, w1, w2);
                     const int ho = MULTI ? uniform(L.ho[a]) : 0;
@@ -496,7 +493,7 @@ k_bin_pairs(int B, BinGeom g, CellArg C, double cutoff_sq, const SortedAtom *__r
                             if (c0 + 64 * r >= T) break;     // slot row past the last candidate
                             const double d2 = pair_d2_pre(H.x, H.y, H.z, cx[r], cy[r], cz[r], cp0[r], cp1[r], cp2[r]);
                             const bool acc = (bool)((int)member(r) & (int)(d2 < cutoff_sq) &
-                                                    (int)((cj[r] != ia) | (cpk[r] != DC_ZERO)));
+                                                    (int)((cj[r] != hp) | (cpk[r] != DC_ZERO)));
                             const unsigned long long m = __ballot(acc);
                             const unsigned pos = __builtin_amdgcn_mbcnt_hi(
                                 (unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, (unsigned)cnt));
@@ -526,7 +523,7 @@ k_bin_pairs(int B, BinGeom g, CellArg C, double cutoff_sq, const SortedAtom *__r
                             const int s0 = w0 + t0, s1 = w1 + t1, s2 = w2 + t2;
                             const double d2 = pair_d2<ORTHO>(C, H.x, H.y, H.z, cx[r], cy[r], cz[r], s0, s1, s2);
                             const int pk = dc_pack_ok(s0) && dc_pack_ok(s1) && dc_pack_ok(s2) ? dc_pack(s0, s1, s2) : -1;
-                            emit(member(r) && !(cj[r] == ia && s0 == 0 && s1 == 0 && s2 == 0) && d2 < cutoff_sq,
+                            emit(member(r) && !(cj[r] == hp && s0 == 0 && s1 == 0 && s2 == 0) && d2 < cutoff_sq,
                                  cj[r], pk);
                         }
                     }
@@ -539,6 +536,7 @@ k_bin_pairs(int B, BinGeom g, CellArg C, double cutoff_sq, const SortedAtom *__r
         if (lane < ma) {
             const int c = L.hcnt[lane];
             counts[L.hi[lane]] = c;
+            scnt[L.hp[lane]] = c;
             wmax = max(wmax, c);
             if (c > 0) wlast = max(wlast, L.hi[lane] + 1);
             if (c > S) stage_over = true;
@@ -580,7 +578,8 @@ template <bool ORTHO>
 __global__ void __launch_bounds__(256)
 k_atom_pairs(int nat, BinGeom g, CellArg C, double cutoff_sq, const SortedAtom *__restrict__ atoms,
              const float4 *__restrict__ pos32, const int *__restrict__ cell_start, int *__restrict__ counts,
-             int2 *__restrict__ stage, int S, int *__restrict__ flags, int2 *__restrict__ part)
+             int *__restrict__ scnt, int2 *__restrict__ stage, int S, int *__restrict__ flags,
+             int2 *__restrict__ part)
 {
     const int p = xcd_block() * blockDim.x + threadIdx.x;
     bool over = false, bad = false;
@@ -632,7 +631,7 @@ k_atom_pairs(int nat, BinGeom g, CellArg C, double cutoff_sq, const SortedAtom *
                         continue;
                 }
                 if (!dc_pack_ok(s0) || !dc_pack_ok(s1) || !dc_pack_ok(s2)) bad = true;
-                if (cnt < S) stage[srow + cnt] = make_int2(jb & 0x7fffffff, dc_pack(s0, s1, s2));
+                if (cnt < S) stage[srow + cnt] = make_int2(q, dc_pack(s0, s1, s2));
                 ++cnt;
             }
         };
@@ -732,6 +731,7 @@ k_atom_pairs(int nat, BinGeom g, CellArg C, double cutoff_sq, const SortedAtom *
         }
         const int ih = __float_as_int(H.w) & 0x7fffffff;
         if (ih < nat) counts[ih] = cnt;
+        scnt[p] = cnt;
         over = cnt > S;
         longest = cnt;
         last = cnt > 0 ? ih + 1 : 0;
@@ -757,10 +757,11 @@ k_atom_pairs(int nat, BinGeom g, CellArg C, double cutoff_sq, const SortedAtom *
     }
 }
 
-// Copy staged rows (sorted slot p, stride S) into the CSR rows of the caller
-// atoms (nothing when the build overflowed).
+// Copy staged rows (sorted slot p, stride S, neighbours by slot) into the
+// CSR rows of the caller atoms, neighbours by caller index (nothing when the
+// build overflowed).
 __global__ void __launch_bounds__(256)
-k_nl_compact(int nat, const SortedAtom *__restrict__ atoms, const int *__restrict__ counts,
+k_nl_compact(int nat, const SortedAtom *__restrict__ atoms, const int *__restrict__ scnt,
              const int *__restrict__ offsets, const int2 *__restrict__ stage, int S, int2 *__restrict__ nbr,
              const int *__restrict__ flags)
 {
@@ -770,38 +771,104 @@ k_nl_compact(int nat, const SortedAtom *__restrict__ atoms, const int *__restric
     if (p >= nat) return;
     if (*flags & (DERR_NL_OVERFLOW | DERR_STAGE_OVERFLOW)) return;
     const int i = atoms[p].j;
-    const int cnt = counts[i];
+    const int cnt = scnt[p];
     if (cnt > S) return;
     const int2 *src = stage + (size_t)p * S;
     int2 *dst = nbr + offsets[i];
-    for (int o = lane; o < cnt; o += G) dst[o] = src[o];
+    for (int o = lane; o < cnt; o += G) {
+        const int2 e = src[o];
+        dst[o] = make_int2(atoms[e.x].j, e.y);
+    }
 }
 
 // the CSR rows of the current build, copied from the staged rows if needed
 static int nl_compact(agpu_neighbors *nl)
 {
     if (nl->csr) return AGPU_OK;
+    AGPU_CHECK(nl_caller_offsets(nl));
     if (nl->nat > 0)
         AGPU_LAUNCH("nl_compact", k_nl_compact, dim3((nl->nat + 15) / 16), dim3(256), 0, stream(), nl->nat,
-                    nl->atoms_sorted.as<SortedAtom>(), nl->counts.as<int>(), nl->offsets.as<int>(),
+                    nl->atoms_sorted.as<SortedAtom>(), nl->scnt.as<int>(), nl->offsets.as<int>(),
                     nl->stage.as<int2>(), nl->stage_w, nl->nbr.as<int2>(), nl->rowmax.as<int>() + 3);
     nl->csr = true;
     return AGPU_OK;
 }
 
+// the positions of the slots from the caller's positions (caller atom
+// order[p] at slot p); caller index and wrap counts stay
+__global__ void __launch_bounds__(256)
+k_slot_positions(int nat, const int *__restrict__ order, const double *__restrict__ r,
+                 SortedAtom *__restrict__ atoms)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nat) return;
+    const int i = order[p];
+    SortedAtom a = atoms[p];
+    a.x = r[3 * (size_t)i];
+    a.y = r[3 * (size_t)i + 1];
+    a.z = r[3 * (size_t)i + 2];
+    atoms[p] = a;
+}
+
+// The slot positions from the particles where they may lag them: a list
+// held over moving atoms (positions change without a revision), or an
+// imported one whose particles moved since.  Built lists write them.
+static int nl_slots_refresh(agpu_neighbors *nl, const agpu_particles *p)
+{
+    const int nat = nl->nat;
+    if (nat == 0 || (!nl->held && nl->slot_rev == p->pos_rev)) return AGPU_OK;
+    AGPU_LAUNCH("nl_slot_positions", k_slot_positions, dim3((nat + 255) / 256), dim3(256), 0, stream(), nat,
+                nl->idx_sorted.as<int>(), p->d_r, nl->atoms_sorted.as<SortedAtom>());
+    nl->slot_rev = p->pos_rev;
+    return AGPU_OK;
+}
+
+int nl_slots_current(agpu_neighbors *nl, const agpu_particles *p) { return nl_slots_refresh(nl, p); }
+
 NlRows nl_rows(const agpu_neighbors *nl)
 {
-    if (nl->csr) return NlRows{nl->nbr.as<int2>(), nl->offsets.as<int>(), 0};
-    return NlRows{nl->stage.as<int2>(), nl->counts.as<int>(), nl->stage_w};
+    if (nl->external)
+        return NlRows{nl->nbr.as<int2>(), nullptr, nl->offsets.as<int>(), 0, nl->atoms_sorted.as<SortedAtom>()};
+    return NlRows{nl->stage.as<int2>(), nl->scnt.as<int>(), nl->soffs.as<int>(), nl->stage_w,
+                  nl->atoms_sorted.as<SortedAtom>()};
+}
+
+// offsets[nat] = offsets[nat - 1] + counts[nat - 1] after the exclusive
+// scan of nat entries
+__global__ void k_offsets_end(int nat, const int *__restrict__ counts, int *__restrict__ offsets)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) offsets[nat] = nat > 0 ? offsets[nat - 1] + counts[nat - 1] : 0;
+}
+
+int nl_caller_offsets(agpu_neighbors *nl)
+{
+    if (nl->caller_off || nl->external) return AGPU_OK;
+    const int nat = nl->nat;
+    hipStream_t s = stream();
+    if (nat > 0) {
+        size_t tmp = 0;
+        AGPU_HIP(rocprim::exclusive_scan(nullptr, tmp, nl->counts.as<int>(), nl->offsets.as<int>(), 0, (size_t)nat,
+                                         rocprim::plus<int>(), s));
+        // reserved by the build for both scans
+        if (tmp + 16 > nl->sort_tmp.bytes) return set_error(AGPU_ERR_STATE, "neighbour list: scan scratch too small");
+        AGPU_HIP(rocprim::exclusive_scan(nl->sort_tmp.ptr, tmp, nl->counts.as<int>(), nl->offsets.as<int>(), 0,
+                                         (size_t)nat, rocprim::plus<int>(), s));
+        AGPU_LAUNCH("nl_offsets_end", k_offsets_end, dim3(1), dim3(64), 0, s, nat, nl->counts.as<int>(),
+                    nl->offsets.as<int>());
+    }
+    nl->caller_off = true;
+    return AGPU_OK;
 }
 
 // The row statistics of the pair search (npart records {longest row,
 // 1 + last non-empty caller row}) reduced into stats[0] and the last row;
-// the total (stats[1..2]); and the reference's capacity rule
-// (python_neighbors.f90:713-718): row i ends at 1-based slot offsets[i] +
-// cnt + i (one separator per earlier atom), increasing with i, so the list
-// overflows exactly when the last non-empty row ends at or beyond nat*avgn.
-// One block of 1024 threads.
+// the total (stats[1..2], from the scan of the row lengths in slot order,
+// offsets[nat] = total); and the reference's capacity rule
+// (python_neighbors.f90:713-718): caller row i ends at 1-based slot
+// (entries of rows before i) + cnt + i (one separator per earlier atom),
+// increasing with i, so the list overflows exactly when the last non-empty
+// row il ends at or beyond nat*avgn -- and that row ends at total + il,
+// every later row being empty.  One block of 1024 threads.
 __global__ void __launch_bounds__(1024)
 k_offsets_tail(int nat, const int *__restrict__ counts, int *__restrict__ offsets, int *__restrict__ stats,
                long long cap, const int2 *__restrict__ part, int npart)
@@ -844,7 +911,7 @@ k_offsets_tail(int nat, const int *__restrict__ counts, int *__restrict__ offset
         stats[1] = (int)(t & 0x7fffffff);
         stats[2] = (int)(t >> 31);
         const int il = stats[5] - 1;
-        if (il >= 0 && (long long)offsets[il] + counts[il] + il >= cap) stats[3] |= DERR_NL_OVERFLOW;
+        if (il >= 0 && t + il >= cap) stats[3] |= DERR_NL_OVERFLOW;
     }
 }
 
@@ -907,9 +974,12 @@ int nl_update(agpu_neighbors *nl, agpu_particles *p, int force, bool csr)
         // an imported list stays as the caller gave it
         if (nl->p != p || nl->nat != p->nat)
             return set_error(AGPU_ERR_STATE, "neighbour list: imported for other particles");
-        return AGPU_OK;
+        return nl_slots_refresh(nl, p);
     }
-    if (nl->held && force == 0 && nl->p == p && nl->nat == p->nat) return csr ? nl_compact(nl) : AGPU_OK;
+    if (nl->held && force == 0 && nl->p == p && nl->nat == p->nat) {
+        AGPU_CHECK(nl_slots_refresh(nl, p));
+        return csr ? nl_compact(nl) : AGPU_OK;
+    }
     if (!p->cell_set) return set_error(AGPU_ERR_STATE, "particles: cell not set");
     if (p->nat > 0 && !p->d_r) return set_error(AGPU_ERR_STATE, "particles: positions not set");
     if (!force && nl->p == p && nl->pos_rev == p->pos_rev && nl->cell_rev == p->cell_rev &&
@@ -936,10 +1006,11 @@ int nl_update(agpu_neighbors *nl, agpu_particles *p, int force, bool csr)
     AGPU_CHECK(nl->key_sorted.reserve(sizeof(int) * (nat + 1)));
     AGPU_CHECK(nl->idx.reserve(sizeof(int) * (nat + 1)));
     AGPU_CHECK(nl->idx_sorted.reserve(sizeof(int) * (nat + 1)));
-    AGPU_CHECK(nl->rank.reserve(sizeof(int) * (nat + 1)));
     AGPU_CHECK(nl->cell_start.reserve(sizeof(int) * (ncell + 1)));
     AGPU_CHECK(nl->atoms_sorted.reserve(sizeof(SortedAtom) * (nat + 1)));
     AGPU_CHECK(nl->counts.reserve(sizeof(int) * (nat + 1)));
+    AGPU_CHECK(nl->scnt.reserve(sizeof(int) * (nat + 1)));
+    AGPU_CHECK(nl->soffs.reserve(sizeof(int) * (nat + 1)));
     AGPU_CHECK(nl->offsets.reserve(sizeof(int) * (nat + 1)));
     AGPU_CHECK(nl->rowmax.reserve(sizeof(int) * 8));
     AGPU_CHECK(nl->nbr.reserve(sizeof(int2) * (nl->cap > 0 ? nl->cap : 1)));
@@ -989,7 +1060,7 @@ int nl_update(agpu_neighbors *nl, agpu_particles *p, int force, bool csr)
             const int bpb = occ > 8.0 ? 4 : (occ > 2.0 ? 16 : 64);
             AGPU_LAUNCH("nl_sort_order", ord, dim3((unsigned)((ncell + bpb - 1) / bpb)), dim3(TB), 0, s, (int)ncell,
                         nl->cell_start.as<int>(), cnt, nl->idx_sorted.as<int>(), flags + 3, p->d_r, g, C,
-                        nl->atoms_sorted.as<SortedAtom>(), nl->rank.as<int>(), pos32);
+                        nl->atoms_sorted.as<SortedAtom>(), pos32);
             nl->cnt_zero = std::max(cnt_zero, ncell);
         } else {
             size_t tmp = 0;
@@ -1007,7 +1078,7 @@ int nl_update(agpu_neighbors *nl, agpu_particles *p, int force, bool csr)
         }
         if (nl->radix)
             AGPU_LAUNCH("nl_gather", k_gather, dim3(nb), dim3(TB), 0, s, nat, p->d_r, g, C,
-                        nl->idx_sorted.as<int>(), nl->atoms_sorted.as<SortedAtom>(), nl->rank.as<int>(), pos32);
+                        nl->idx_sorted.as<int>(), nl->atoms_sorted.as<SortedAtom>(), pos32);
         double cutoff_sq = nl->cutoff * nl->cutoff;
         // bins per wavefront edge: one bin when bins are dense, a 2^3 or 3^3
         // block of bins when they hold few atoms (stencil of +-1 bin only)
@@ -1037,31 +1108,30 @@ int nl_update(agpu_neighbors *nl, agpu_particles *p, int force, bool csr)
             if (ortho)
                 AGPU_LAUNCH("nl_pairs", k_atom_pairs<true>, dim3(na), dim3(TB), 0, s, nat, g, C, cutoff_sq,
                             nl->atoms_sorted.as<SortedAtom>(), nl->pos32.as<float4>(), nl->cell_start.as<int>(),
-                            nl->counts.as<int>(), nl->stage.as<int2>(), S, flags + 3, part);
+                            nl->counts.as<int>(), nl->scnt.as<int>(), nl->stage.as<int2>(), S, flags + 3, part);
             else
                 AGPU_LAUNCH("nl_pairs", k_atom_pairs<false>, dim3(na), dim3(TB), 0, s, nat, g, C, cutoff_sq,
                             nl->atoms_sorted.as<SortedAtom>(), nl->pos32.as<float4>(), nl->cell_start.as<int>(),
-                            nl->counts.as<int>(), nl->stage.as<int2>(), S, flags + 3, part);
+                            nl->counts.as<int>(), nl->scnt.as<int>(), nl->stage.as<int2>(), S, flags + 3, part);
         } else {
             auto kern = ortho ? (B > 1 ? k_bin_pairs<true, true> : k_bin_pairs<true, false>)
                               : (B > 1 ? k_bin_pairs<false, true> : k_bin_pairs<false, false>);
             AGPU_LAUNCH("nl_pairs", kern, dim3(nbin_blk), dim3(TB), 0, s, B, g, C, cutoff_sq,
                         nl->atoms_sorted.as<SortedAtom>(), nl->cell_start.as<int>(), nl->counts.as<int>(),
-                        nl->stage.as<int2>(), S, flags + 3, part);
+                        nl->scnt.as<int>(), nl->stage.as<int2>(), S, flags + 3, part);
         }
+        // row offsets in slot order (the per-entry arrays of the force
+        // kernels); the caller-order offsets follow when a caller asks
         size_t tmp2 = 0;
-        AGPU_HIP(rocprim::exclusive_scan(nullptr, tmp2, nl->counts.as<int>(), nl->offsets.as<int>(), 0,
+        AGPU_HIP(rocprim::exclusive_scan(nullptr, tmp2, nl->scnt.as<int>(), nl->soffs.as<int>(), 0,
                                          (size_t)nat, rocprim::plus<int>(), s));
         AGPU_CHECK(nl->sort_tmp.reserve(tmp2 + 16));
-        AGPU_HIP(rocprim::exclusive_scan(nl->sort_tmp.ptr, tmp2, nl->counts.as<int>(),
-                                         nl->offsets.as<int>(), 0, (size_t)nat, rocprim::plus<int>(), s));
+        AGPU_HIP(rocprim::exclusive_scan(nl->sort_tmp.ptr, tmp2, nl->scnt.as<int>(),
+                                         nl->soffs.as<int>(), 0, (size_t)nat, rocprim::plus<int>(), s));
         AGPU_LAUNCH("nl_offsets_tail", k_offsets_tail, dim3(1), dim3(1024), 0, s, nat,
-                    nl->counts.as<int>(), nl->offsets.as<int>(), flags, (long long)nl->cap, part, npart);
-        if (csr)
-            AGPU_LAUNCH("nl_compact", k_nl_compact, dim3((nat + 15) / 16), dim3(TB), 0, s, nat,
-                        nl->atoms_sorted.as<SortedAtom>(), nl->counts.as<int>(), nl->offsets.as<int>(),
-                        nl->stage.as<int2>(), S, nl->nbr.as<int2>(), flags + 3);
+                    nl->scnt.as<int>(), nl->soffs.as<int>(), flags, (long long)nl->cap, part, npart);
     } else {
+        AGPU_HIP(hipMemsetAsync(nl->soffs.ptr, 0, sizeof(int), s));
         AGPU_HIP(hipMemsetAsync(nl->offsets.ptr, 0, sizeof(int), s));
     }
     AGPU_HIP(hipMemcpyAsync(nl->stats.h, flags, sizeof(int) * 4, hipMemcpyDeviceToHost, s));
@@ -1083,11 +1153,13 @@ int nl_update(agpu_neighbors *nl, agpu_particles *p, int force, bool csr)
         return nl_update(nl, p, 2, csr);
     }
     nl->total = (int64_t)nl->stats.h[1] | ((int64_t)nl->stats.h[2] << 31);
-    nl->csr = csr || nat == 0;
+    nl->csr = nat == 0;
+    nl->caller_off = nat == 0;
     nl->p = p;
     nl->nat = nat;
     nl->pos_rev = p->pos_rev;
     nl->cell_rev = p->cell_rev;
+    nl->slot_rev = p->pos_rev;
     nl->n_updates++;
     if (fl & DERR_IMAGE_RANGE) {
         nl->p = nullptr;
@@ -1101,13 +1173,27 @@ int nl_update(agpu_neighbors *nl, agpu_particles *p, int force, bool csr)
                              std::to_string(nat) + " separators exceed nat*avgn = " +
                              std::to_string(nl->cap) + ". Increase avgn.");
     }
-    return AGPU_OK;
+    return csr ? nl_compact(nl) : AGPU_OK;
 }
 
 __global__ void k_iota(int n, int *__restrict__ a)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) a[i] = i;
+}
+
+// slots of an imported list: the caller's atoms in their own order
+__global__ void k_slots_identity(int nat, const double *__restrict__ r, SortedAtom *__restrict__ atoms)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nat) return;
+    SortedAtom a;
+    a.x = r[3 * (size_t)p];
+    a.y = r[3 * (size_t)p + 1];
+    a.z = r[3 * (size_t)p + 2];
+    a.j = p;
+    a.w = DC_ZERO;
+    atoms[p] = a;
 }
 
 // The caller's own full list (LAMMPS: firstneigh/numneigh of local and ghost
@@ -1144,14 +1230,16 @@ int nl_import(agpu_neighbors *nl, agpu_particles *p, const int64_t *seed, const 
     AGPU_CHECK(nl->offsets.reserve(sizeof(int) * (nat + 1)));
     AGPU_CHECK(nl->nbr.reserve(sizeof(int2) * ((size_t)total + 1)));
     AGPU_CHECK(nl->idx_sorted.reserve(sizeof(int) * (nat + 1)));
-    AGPU_CHECK(nl->rank.reserve(sizeof(int) * (nat + 1)));
+    AGPU_CHECK(nl->atoms_sorted.reserve(sizeof(SortedAtom) * (nat + 1)));
+    if (nat > 0 && !p->d_r) return set_error(AGPU_ERR_STATE, "particles: positions not set");
     hipStream_t s = stream();
     AGPU_HIP(hipMemcpyAsync(nl->offsets.ptr, off.data(), sizeof(int) * (nat + 1), hipMemcpyHostToDevice, s));
     if (total) AGPU_HIP(hipMemcpyAsync(nl->nbr.ptr, rows.data(), sizeof(int2) * total, hipMemcpyHostToDevice, s));
     // atoms in the caller's order: slot = atom
     if (nat) {
         AGPU_LAUNCH("nl_iota", k_iota, dim3((nat + 255) / 256), dim3(256), 0, s, nat, nl->idx_sorted.as<int>());
-        AGPU_LAUNCH("nl_iota", k_iota, dim3((nat + 255) / 256), dim3(256), 0, s, nat, nl->rank.as<int>());
+        AGPU_LAUNCH("nl_slots", k_slots_identity, dim3((nat + 255) / 256), dim3(256), 0, s, nat, p->d_r,
+                    nl->atoms_sorted.as<SortedAtom>());
     }
     AGPU_HIP(hipStreamSynchronize(s));
     nl->external = true;
@@ -1160,6 +1248,7 @@ int nl_import(agpu_neighbors *nl, agpu_particles *p, const int64_t *seed, const 
     nl->nat = nat;
     nl->pos_rev = p->pos_rev;
     nl->cell_rev = p->cell_rev;
+    nl->slot_rev = p->pos_rev;
     nl->total = total;
     nl->maxrow = maxrow;
     nl->cap = total + nat;

@@ -636,41 +636,43 @@ __global__ void k_rb_slots(int nat, const int *__restrict__ order, NlRows L, con
     if (p >= nat) return;
     const int i = order[p];
     int n;
-    nl_row(L, p, i, n);
+    nl_row(L, p, n);
     cnt_s[p] = n;
     el_s[p] = el[i];
     if (tag) tag_s[p] = tag[i];
 }
 
 // thread per slot p (atom order[p]): its bonds at offs_s[p] .., neighbours
-// by slot; bsrc (optional) = the bond's entry in the caller-order list
-// bond32 (optional, the mixed form): the same records rounded to FP32
+// by slot (the list's rows name slots, el_s and the slot records hold their
+// elements and positions); bsrc (optional) = the bond's entry in the
+// caller-order list.  bond32 (optional, the mixed form): the same records
+// rounded to FP32
 __global__ void __launch_bounds__(256)
-k_rb_bonds(int nat, const int *__restrict__ order, const int *__restrict__ rank, const double *__restrict__ r,
-           const int *__restrict__ el, CellArg C, const RbDev *__restrict__ Dp, NlRows L,
-           const int *__restrict__ offs_s, const int *__restrict__ offsets, RbBond *__restrict__ bond,
-           RbBondT<float> *__restrict__ bond32, int2 *__restrict__ bij, int *__restrict__ bown,
-           int *__restrict__ bcnt, double2 *__restrict__ nn, int *__restrict__ bsrc)
+k_rb_bonds(int nat, const int *__restrict__ order, const int *__restrict__ el_s, CellArg C,
+           const RbDev *__restrict__ Dp, NlRows L, const int *__restrict__ offs_s, const int *__restrict__ offsets,
+           RbBond *__restrict__ bond, RbBondT<float> *__restrict__ bond32, int2 *__restrict__ bij,
+           int *__restrict__ bown, int *__restrict__ bcnt, double2 *__restrict__ nn, int *__restrict__ bsrc)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= nat) return;
     const RbDev &D = *Dp;
-    const int i = order[p];
-    const int ki = el[i];
+    const int ki = el_s[p];
     const int beg = offs_s[p], end = offs_s[p + 1];
     int nrow;
-    const int2 *row = nl_row(L, p, i, nrow);
+    const int2 *row = nl_row(L, p, nrow);
     int nb = 0;
     double nC = 0.0, nH = 0.0;
     if (ki > 0) {
-        const double ri[3] = {r[3 * i], r[3 * i + 1], r[3 * i + 2]};
+        const SortedAtom A = L.atoms[p];
+        const double ri[3] = {A.x, A.y, A.z};
         for (int k = 0; k < nrow; ++k) {
             const int2 e = row[k];
-            const int kj = el[e.x];
+            const int kj = el_s[e.x];
             if (kj <= 0) continue;
             int a, b, c;
             dc_unpack(e.y, a, b, c);
-            const double rj[3] = {r[3 * e.x], r[3 * e.x + 1], r[3 * e.x + 2]};
+            const SortedAtom Aj = L.atoms[e.x];
+            const double rj[3] = {Aj.x, Aj.y, Aj.z};
             double d[3];
             delta_r(C, ri, rj, a, b, c, d);
             const double rl = dot3_nc(d);
@@ -702,9 +704,9 @@ k_rb_bonds(int nat, const int *__restrict__ order, const int *__restrict__ rank,
             if (bond32)
                 bond32[beg + nb] = RbBondT<float>{{(float)B.n[0], (float)B.n[1], (float)B.n[2]}, (float)len,
                                                   (float)fc, (float)dfc, 0.0f, 0.0f};
-            bij[beg + nb] = make_int2(rank[e.x], e.y);
+            bij[beg + nb] = make_int2(e.x, e.y);
             bown[beg + nb] = p;
-            if (bsrc) bsrc[beg + nb] = offsets[i] + k;   // list entry of the bond (nbb of the reference)
+            if (bsrc) bsrc[beg + nb] = offsets[order[p]] + k;   // list entry of the bond (nbb of the reference)
             ++nb;
         }
     }
@@ -1718,7 +1720,6 @@ static int rebo2_kernels(Rebo2State *st, agpu_potential *pot, const agpu_eval_ar
     AGPU_CHECK(st->bo.reserve(sizeof(Rb4<double>) * (total + 1)));
     CellArg C;
     std::memcpy(C.A, p->cell.A, sizeof C.A);
-    const int *el = pot->d_el.as<int>();
     const RbDev *D64 = st->dev.as<RbDev>();
     const RbDevT<R> *D = f64 ? reinterpret_cast<const RbDevT<R> *>(st->dev.ptr)
                              : reinterpret_cast<const RbDevT<R> *>(st->dev32.ptr);
@@ -1734,9 +1735,9 @@ static int rebo2_kernels(Rebo2State *st, agpu_potential *pot, const agpu_eval_ar
                                   : reinterpret_cast<const RbBondT<R> *>(bond32);
     Rb4<double> *bo = st->bo.as<Rb4<double>>();
     int *dstats = st->stats.as<int>();
-    AGPU_LAUNCH("rebo2_bonds", k_rb_bonds, dim3(nb), dim3(256), 0, s, nat, order, nl->rank.as<int>(), p->d_r, el,
-                C, D64, nl_rows(nl), offs, nl->offsets.as<int>(), bond, bond32, st->bij.as<int2>(),
-                st->bown.as<int>(), st->bcnt.as<int>(), st->nn.as<double2>(), const_cast<int *>(bsrc));
+    AGPU_LAUNCH("rebo2_bonds", k_rb_bonds, dim3(nb), dim3(256), 0, s, nat, order, els, C, D64, nl_rows(nl), offs,
+                nl->offsets.as<int>(), bond, bond32, st->bij.as<int2>(), st->bown.as<int>(), st->bcnt.as<int>(),
+                st->nn.as<double2>(), const_cast<int *>(bsrc));
     AGPU_LAUNCH("rebo2_conj", k_rb_conj, dim3(nb), dim3(256), 0, s, nat, els, offs, st->bcnt.as<int>(),
                 st->bij.as<int2>(), st->nn.as<double2>(), bond, bond32, st->nconj.as<double>());
     const int ntot = (int)total;
@@ -1916,6 +1917,8 @@ int rebo2_compute(agpu_potential *pot, const agpu_eval_args *a)
     const int nb = (nat + 255) / 256;
     const bool per_bond = a->epot_per_bond || a->f_per_bond || a->wpot_per_bond;
     if (per_bond) AGPU_CHECK(st->bsrc.reserve(sizeof(int) * (total + 1)));
+    // the bonds' entries in the caller-order list (the reference's indexing)
+    if (per_bond) AGPU_CHECK(nl_caller_offsets(nl));
     int *bsrc = per_bond ? st->bsrc.as<int>() : nullptr;
     // slot order: row lengths -> bond offsets per slot, elements, tags
     const int *order = nl->idx_sorted.as<int>();

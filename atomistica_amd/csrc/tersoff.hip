@@ -67,7 +67,7 @@ struct TersoffState {
     double cutoff = 0.0;
     int z2db[128];
     uint64_t Z_rev = ~0ull;
-    DevBuf bslot, bcnt, rev, fslot, fown, eslot, wslot, partial, stats;
+    DevBuf bslot, bcnt, rev, fslot, fown, eslot, wslot, partial, stats, el_s;
     // screened variants: parameters (host copy of ScrDev) and per-entry /
     // per-record buffers
     bool scr = false;
@@ -698,9 +698,8 @@ constexpr int DERR_REV_MISSING = DERR_BOND_OVERFLOW;
 
 template <typename R, int G, int FAM, bool WB, bool HT = true>
 __global__ void __launch_bounds__(256)
-k_bop_rows(TersoffDev P, int nat, const int *__restrict__ order, const double *__restrict__ r,
-           const int *__restrict__ el, CellArg C, NlRows L, const int *__restrict__ slot,
-           const int *__restrict__ offsets, const int *__restrict__ mask, double *__restrict__ frev,
+k_bop_rows(TersoffDev P, int nat, const int *__restrict__ order, const int *__restrict__ el_s, CellArg C,
+           NlRows L, const int *__restrict__ offsets, const int *__restrict__ mask, double *__restrict__ frev,
            double *__restrict__ fown, double *__restrict__ eslot, double *__restrict__ erev,
            double *__restrict__ wslot, double *__restrict__ wrev, double *__restrict__ partial,
            double *__restrict__ epb, double *__restrict__ fpb, double *__restrict__ wpb, int *__restrict__ flags)
@@ -719,17 +718,17 @@ k_bop_rows(TersoffDev P, int nat, const int *__restrict__ order, const double *_
     const int gbase = grp * G;
     const int ai = xcd_block() * GPB + grp;
     const bool valid = ai < nat;
-    const int i = valid ? order[ai] : 0;
-    const int eli = valid ? el[i] : -1;
+    const int i = valid ? order[ai] : 0;           // caller index (mask, per-bond arrays)
+    const int eli = valid ? el_s[ai] : -1;
     int n = 0;
-    const int2 *row = valid ? nl_row(L, ai, i, n) : nullptr;
+    const int2 *row = valid ? nl_row(L, ai, n) : nullptr;
     n = min(n, G);                      // G >= every row length (host side)
     const bool act = lane < n;
 
-    int j = 0, code = 0, typ = 0;
+    int j = 0, code = 0, typ = 0;       // j: the neighbour's slot
     R rn[3] = {(R)0, (R)0, (R)0}, rl = (R)1, fc = (R)0, dfc = (R)0;
     bool ok = false;
-    int rt = -1;                        // list index of the reverse entry j -> i
+    long long rt = -1;                  // entry index of the reverse entry j -> i
     if (act) {
         const int2 e = row[lane];
         j = e.x;
@@ -740,7 +739,7 @@ k_bop_rows(TersoffDev P, int nat, const int *__restrict__ order, const double *_
             dc_unpack(code, a0, b0, c0);
             const int want = dc_pack(-a0, -b0, -c0);
             int nj = 0;
-            const int2 *rowj = nl_row(L, L.S > 0 ? slot[j] : 0, j, nj);
+            const int2 *rowj = nl_row(L, j, nj);
             int t = -1;
             if (G <= 8) {
                 // every candidate load issued before the first compare
@@ -749,26 +748,28 @@ k_bop_rows(TersoffDev P, int nat, const int *__restrict__ order, const double *_
                 for (int q = 0; q < G; ++q) ej[q] = q < nj ? rowj[q] : make_int2(-1, 0);
 #pragma unroll
                 for (int q = G - 1; q >= 0; --q)
-                    if (ej[q].x == i && ej[q].y == want) t = q;
+                    if (ej[q].x == ai && ej[q].y == want) t = q;
                 if (nj > G) t = -1;     // longer rows only on hosts that chose a wider G
             } else {
                 for (int q = 0; q < nj; ++q) {
                     const int2 eq = rowj[q];
-                    if (eq.x == i && eq.y == want) {
+                    if (eq.x == ai && eq.y == want) {
                         t = q;
                         break;
                     }
                 }
             }
-            rt = t < 0 ? -1 : offsets[j] + t;
+            rt = t < 0 ? -1 : (long long)nl_at(L, j, t);
         }
-        const int elj = el[j];
+        const int elj = el_s[j];
         if (eli >= 0 && elj >= 0) {
             int a, b, c;
             dc_unpack(code, a, b, c);
+            const SortedAtom Ai = L.atoms[ai], Aj = L.atoms[j];
+            const double ri[3] = {Ai.x, Ai.y, Ai.z}, rj[3] = {Aj.x, Aj.y, Aj.z};
             // rij = r_j - r_i - A.dc   (bop_kernel.f90:611)
             double rij[3];
-            for (int x = 0; x < 3; ++x) rij[x] = (r[3 * j + x] - r[3 * i + x]) - image_x(C, x, a, b, c);
+            for (int x = 0; x < 3; ++x) rij[x] = (rj[x] - ri[x]) - image_x(C, x, a, b, c);
             const double rl2 = rij[0] * rij[0] + rij[1] * rij[1] + rij[2] * rij[2];
             const double rld = sqrt(rl2);
             typ = FAM == 3 ? elj + eli * P.nel : pair_index(eli, elj, P.nel);
@@ -804,7 +805,7 @@ k_bop_rows(TersoffDev P, int nat, const int *__restrict__ order, const double *_
 
     int maskfac = 2;
     if (mask && act) {
-        const int mi = mask[i], mj = mask[j];
+        const int mi = mask[i], mj = mask[order[j]];
         if (mi == 0 && mj == 0) maskfac = 0;
         else if (mi == 0 || mj == 0) maskfac = 1;
     }
@@ -971,7 +972,9 @@ k_bop_rows(TersoffDev P, int nat, const int *__restrict__ order, const double *_
         for (int a = 0; a < 3; ++a)
             for (int b = 0; b < 3; ++b) w[a + 3 * b] = -rijd[a] * F[b];
     if (act) {
-        const size_t sown = (size_t)offsets[i] + lane;
+        const size_t sown = nl_at(L, ai, lane);
+        // the caller's list entry (per-bond arrays only)
+        const size_t sbond = (epb || fpb || wpb) ? (size_t)offsets[i] + lane : 0;
         if (rt < 0) {
             atomicOr(flags, DERR_REV_MISSING);
         } else {
@@ -987,11 +990,11 @@ k_bop_rows(TersoffDev P, int nat, const int *__restrict__ order, const double *_
             for (int q = 0; q < 9; ++q) wslot[9 * sown + q] = w[q];
         // per-bond properties (bop_kernel.f90:1370-1373,1406-1408,1511-1512)
         if (on) {
-            if (epb) epb[sown] += e_ij;
+            if (epb) epb[sbond] += e_ij;
             if (fpb)
-                for (int x = 0; x < 3; ++x) fpb[3 * sown + x] += dfp[x];
+                for (int x = 0; x < 3; ++x) fpb[3 * sbond + x] += dfp[x];
             if (wpb)
-                for (int q = 0; q < 9; ++q) wpb[9 * sown + q] += w[q];
+                for (int q = 0; q < 9; ++q) wpb[9 * sbond + q] += w[q];
         }
     }
     // own force of i: sum over the group
@@ -999,9 +1002,9 @@ k_bop_rows(TersoffDev P, int nat, const int *__restrict__ order, const double *_
     const double f1 = gsum<G>(fi[1]);
     const double f2 = gsum<G>(fi[2]);
     if (valid && lane == 0) {
-        fown[3 * (size_t)i + 0] = f0;
-        fown[3 * (size_t)i + 1] = f1;
-        fown[3 * (size_t)i + 2] = f2;
+        fown[3 * (size_t)ai + 0] = f0;
+        fown[3 * (size_t)ai + 1] = f1;
+        fown[3 * (size_t)ai + 2] = f2;
     }
     // energy + virial of the block: 16-lane sums (DPP), then 16 partial
     // records per value summed in order
@@ -1024,22 +1027,25 @@ k_bop_rows(TersoffDev P, int nat, const int *__restrict__ order, const double *_
 // Assembly for k_bop_rows: f_a += own force + the forces a's neighbours'
 // terms put on a (stored in a's own row), in row order; per-atom energy and
 // virial halves of a's bonds in both directions (pe(i), pe(j) of
-// bop_kernel.f90:1366-1368).  A thread per atom.
+// bop_kernel.f90:1366-1368).  A thread per slot x (caller atom order[x]).
 __global__ void __launch_bounds__(256)
-k_bop_rows_gather(int nat, const int *__restrict__ offsets, const double *__restrict__ frev,
+k_bop_rows_gather(int nat, const int *__restrict__ order, NlRows L, const double *__restrict__ frev,
                   const double *__restrict__ fown, const double *__restrict__ eslot, const double *__restrict__ erev,
                   const double *__restrict__ wslot, const double *__restrict__ wrev, double *__restrict__ f,
                   double *__restrict__ epa, double *__restrict__ wpa)
 {
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= nat) return;
-    const int s0 = offsets[a], s1 = offsets[a + 1];
-    double acc[3] = {fown[3 * (size_t)a], fown[3 * (size_t)a + 1], fown[3 * (size_t)a + 2]};
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= nat) return;
+    const int a = order[x];
+    int n;
+    nl_row(L, x, n);
+    const size_t s0 = nl_at(L, x, 0);
+    double acc[3] = {fown[3 * (size_t)x], fown[3 * (size_t)x + 1], fown[3 * (size_t)x + 2]};
     double e = 0.0;
-    for (int s = s0; s < s1; ++s) {
-        acc[0] += frev[3 * (size_t)s];
-        acc[1] += frev[3 * (size_t)s + 1];
-        acc[2] += frev[3 * (size_t)s + 2];
+    for (size_t s = s0; s < s0 + n; ++s) {
+        acc[0] += frev[3 * s];
+        acc[1] += frev[3 * s + 1];
+        acc[2] += frev[3 * s + 2];
         if (epa) e += eslot[s] + erev[s];
     }
     f[3 * (size_t)a] += acc[0];
@@ -1049,7 +1055,7 @@ k_bop_rows_gather(int nat, const int *__restrict__ offsets, const double *__rest
     if (wpa)
         for (int q = 0; q < 9; ++q) {
             double w = 0.0;
-            for (int s = s0; s < s1; ++s) w += wslot[9 * (size_t)s + q] + wrev[9 * (size_t)s + q];
+            for (size_t s = s0; s < s0 + n; ++s) w += wslot[9 * s + q] + wrev[9 * s + q];
             wpa[9 * (size_t)a + q] += 0.5 * w;
         }
 }
@@ -2017,13 +2023,24 @@ bool tersoff_screened(const agpu_potential *pot)
     return pot && pot->impl && static_cast<const TersoffState *>(pot->impl)->scr;
 }
 
-// Per-call scratch of the row path: slot forces by list entry (3 per
-// entry), own forces (3 per atom), per-block records (10 per block),
-// second-level reduction records (NULL: the library's own) and 4 flag words.
+// Per-call scratch of the row path: slot forces by list entry (3 per entry
+// of the list's entry space), own forces (3 per atom), per-block records
+// (10 per block), second-level reduction records (NULL: the library's own),
+// 4 flag words and the elements by slot (1 per atom).
 struct RowBufs {
     double *frev, *fown, *partial, *red;
-    int *stats;
+    int *stats, *el_s;
 };
+
+// the row path's flags cleared and its elements put in slot order (slot p
+// holds caller atom order[p])
+__global__ void k_bop_prep(int nat, const int *__restrict__ order, const int *__restrict__ el,
+                           int *__restrict__ el_s, int *__restrict__ stats)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < 4) stats[p] = 0;
+    if (p < nat) el_s[p] = el[order[p]];
+}
 
 template <typename R, int G, int FAM, bool WB, bool HT = true>
 static int launch_rows_k(const char *name, TersoffState *st, const RowBufs &B, agpu_particles *p,
@@ -2033,9 +2050,8 @@ static int launch_rows_k(const char *name, TersoffState *st, const RowBufs &B, a
     CellArg C;
     std::memcpy(C.A, p->cell.A, sizeof C.A);
     AGPU_LAUNCH(name, (k_bop_rows<R, G, FAM, WB, HT>), dim3(nblk), dim3(256), 0, s, st->dev, p->nat,
-                nl->idx_sorted.as<int>(), p->d_r, el, C, nl_rows(nl), nl->rank.as<int>(), nl->offsets.as<int>(),
-                a->mask, B.frev, B.fown, eslot, erev, wslot, wrev, B.partial, a->epot_per_bond, a->f_per_bond,
-                a->wpot_per_bond, B.stats);
+                nl->idx_sorted.as<int>(), B.el_s, C, nl_rows(nl), nl->offsets.as<int>(), a->mask, B.frev, B.fown,
+                eslot, erev, wslot, wrev, B.partial, a->epot_per_bond, a->f_per_bond, a->wpot_per_bond, B.stats);
     return AGPU_OK;
 }
 
@@ -2121,15 +2137,18 @@ static int bop_rows_run(agpu_potential *pot, const agpu_eval_args *a, const RowB
     const int G = rows_width(nl);
     const int nblk = bop_blocks(nat, G);
     const bool wb = wslot || a->wpot_per_bond;
-    AGPU_CHECK(zero_dev(B.stats, sizeof(int) * 4));
     const int *el = pot->d_el.as<int>();
+    const int *order = nl->idx_sorted.as<int>();
+    // per-bond arrays are indexed like the reference's list: caller order
+    if (a->epot_per_bond || a->f_per_bond || a->wpot_per_bond) AGPU_CHECK(nl_caller_offsets(nl));
+    AGPU_LAUNCH("bop_prep", k_bop_prep, dim3((std::max(nat, 4) + 255) / 256), dim3(256), 0, s, nat, order, el,
+                B.el_s, B.stats);
     if (pot->precision == AGPU_PREC_MIXED)
         AGPU_CHECK(dispatch_rows<float>(G, wb, st, B, p, nl, el, a, eslot, erev, wslot, wrev, nblk, s));
     else
         AGPU_CHECK(dispatch_rows<double>(G, wb, st, B, p, nl, el, a, eslot, erev, wslot, wrev, nblk, s));
-    AGPU_LAUNCH("bop_gather", k_bop_rows_gather, dim3((nat + 255) / 256), dim3(256), 0, s, nat,
-                nl->offsets.as<int>(), B.frev, B.fown, eslot, erev, wslot, wrev, a->f, a->epot_per_at,
-                a->wpot_per_at);
+    AGPU_LAUNCH("bop_gather", k_bop_rows_gather, dim3((nat + 255) / 256), dim3(256), 0, s, nat, order,
+                nl_rows(nl), B.frev, B.fown, eslot, erev, wslot, wrev, a->f, a->epot_per_at, a->wpot_per_at);
     AGPU_CHECK(reduce_ev(B.partial, nblk, 10, a->epot, a->wpot, nullptr, false, B.red));
     if (capturing()) return AGPU_OK;     // the MD driver reads the flags after the replay
     AGPU_HIP(hipMemcpyAsync(h_flags, B.stats, sizeof(int) * 4, hipMemcpyDeviceToHost, s));
@@ -2146,6 +2165,7 @@ static int bop_rows_compute(agpu_potential *pot, const agpu_eval_args *a)
     const int64_t total = pot->nl->total;
     AGPU_CHECK(st->fslot.reserve(sizeof(double) * 3 * (total + 1)));
     AGPU_CHECK(st->fown.reserve(sizeof(double) * 3 * (nat + 1)));
+    AGPU_CHECK(st->el_s.reserve(sizeof(int) * (nat + 1)));
     AGPU_CHECK(st->stats.reserve(sizeof(int) * 4));
     AGPU_CHECK(st->partial.reserve(sizeof(double) * 10 * bop_blocks(nat, rows_width(pot->nl))));
     AGPU_CHECK(pot->stats.init());
@@ -2160,7 +2180,8 @@ static int bop_rows_compute(agpu_potential *pot, const agpu_eval_args *a)
         wslot = st->wslot.as<double>();
         wrev = wslot + 9 * (total + 1);
     }
-    RowBufs B{st->fslot.as<double>(), st->fown.as<double>(), st->partial.as<double>(), nullptr, st->stats.as<int>()};
+    RowBufs B{st->fslot.as<double>(), st->fown.as<double>(), st->partial.as<double>(), nullptr, st->stats.as<int>(),
+              st->el_s.as<int>()};
     return bop_rows_run(pot, a, B, eslot, erev, wslot, wrev, pot->stats.h);
 }
 
@@ -2325,6 +2346,7 @@ static size_t noalloc_layout(unsigned char *base, int64_t total, int nat, int G,
     B->partial = reinterpret_cast<double *>(take(sizeof(double) * 10 * (size_t)nblk));
     B->red = reinterpret_cast<double *>(take(sizeof(double) * 10 * (size_t)nred));
     B->stats = reinterpret_cast<int *>(take(sizeof(int) * 4));
+    B->el_s = reinterpret_cast<int *>(take(sizeof(int) * (size_t)(nat + 1)));
     return off;
 }
 
@@ -2349,10 +2371,12 @@ static int tersoff_compute_noalloc(agpu_potential *pot, agpu_particles *p, agpu_
                                            std::to_string(nl->total) + " list entries need " +
                                            std::to_string(need) + " bytes of scratch; the maximum is " +
                                            std::to_string(NOALLOC_BYTES) +
-                                           " (24 per list entry + about 25 per atom)");
+                                           " (24 per list entry + about 30 per atom)");
     unsigned char *base = nullptr;
     AGPU_CHECK(scratch_base(&base));
     noalloc_layout(base, nl->total, nat, G, &B);
+    // slot positions of a held list follow the atoms (no allocation)
+    AGPU_CHECK(nl_slots_current(nl, p));
     AGPU_CHECK(zero_dev(epot, sizeof(double)));
     if (nat == 0) return AGPU_OK;
     agpu_eval_args a{};
