@@ -322,6 +322,10 @@ int pme_recip(PmeState *pm, const agpu_particles *p, const double *q, double bet
               double *phi);
 int rebo2_bind(agpu_potential *pot);
 int rebo2_compute(agpu_potential *pot, const agpu_eval_args *a);
+// REBO2 launch variants: k = 0, 1, 2 select 4, 8 or 16 lanes per half pair
+// in the torsion pass (the per-pair sums change their association order);
+// returns the number of variants (3), *g (optional) the current width
+int rebo2_variant(agpu_potential *pot, int k, int *g);
 
 // Deterministic fixed-order reduction of nblk per-block records of width w
 // (double): column 0 -> *epot (+=, or = with set_epot), columns 1..9 ->

@@ -13,6 +13,8 @@
 //     reverse entries found in the neighbour's row) vs the bond-compacting
 //     path (bond slots, reverse-slot pass).  The per-atom sums keep their
 //     order; the virial sums differ at the level of double rounding.
+//   - REBO2: lanes per half pair of the torsion pass (4, 8, 16).  The
+//     torsion sums of a pair change their association order with the width.
 //
 // The reference has no counterpart: its kernels are CPU loops with no
 // launch shape to choose.
@@ -147,6 +149,33 @@ static int autotune(agpu_potential *pot, agpu_particles *p, agpu_neighbors *nl, 
         bop_variant(pot, pick);
         rep += pick == 0 ? "bond order -> rows\n" : "bond order -> compacted\n";
     }
+    if (pot->kind == AGPU_KIND_REBO2 && p->nat > 0) {
+        DevBuf out;
+        const size_t nat = (size_t)p->nat;
+        AGPU_CHECK(out.reserve(sizeof(double) * (3 * nat + 10)));
+        agpu_eval_args a{};
+        a.epot = out.as<double>();
+        a.wpot = a.epot + 1;
+        a.f = a.epot + 10;
+        const int nv = rebo2_variant(pot, -1, nullptr);
+        double best = 0.0;
+        int pick = -1, g = 0;
+        for (int k = 0; k < nv; ++k) {
+            rebo2_variant(pot, k, &g);
+            double ms = 0.0;
+            AGPU_CHECK(time_calls(ev, reps, [&] {
+                AGPU_CHECK(zero_dev(out.ptr, sizeof(double) * (3 * nat + 10)));
+                return rebo2_compute(pot, &a);
+            }, &ms));
+            append(rep, "rebo2 torsion lanes_per_pair %.0f: %.4f ms\n", g, ms);
+            if (pick < 0 || ms < best) {
+                best = ms;
+                pick = k;
+            }
+        }
+        rebo2_variant(pot, pick, &g);
+        append(rep, "rebo2 -> torsion lanes_per_pair %.0f\n", g);
+    }
     return AGPU_OK;
 }
 
@@ -180,12 +209,15 @@ extern "C" int agpu_potential_set_launch(agpu_potential *pot, agpu_neighbors *nl
         return set_error(AGPU_ERR_STATE, "set_launch: potential not bound to this neighbour list");
     if (pair_mode < 0 || pair_mode > 2) return set_error(AGPU_ERR_ARG, "set_launch: pair_mode must be 0, 1 or 2");
     const bool eam = pot->kind == AGPU_KIND_EAM_ALLOY || pot->kind == AGPU_KIND_EAM;
-    const int nv = eam ? eam_variant(pot, -1, nullptr, nullptr) : bop_variant(pot, -1);
+    const bool rebo2 = pot->kind == AGPU_KIND_REBO2;
+    const int nv = eam ? eam_variant(pot, -1, nullptr, nullptr)
+                       : (rebo2 ? rebo2_variant(pot, -1, nullptr) : bop_variant(pot, -1));
     if (variant < -1 || variant >= (nv > 0 ? nv : 1))
         return set_error(AGPU_ERR_ARG, "set_launch: no such kernel variant for this potential");
     nl->pair_mode = pair_mode;
     if (variant >= 0 && nv > 0) {
         if (eam) eam_variant(pot, variant, nullptr, nullptr);
+        else if (rebo2) rebo2_variant(pot, variant, nullptr);
         else bop_variant(pot, variant);
     }
     if (nvariants) *nvariants = nv;

@@ -126,10 +126,11 @@ struct Rebo2State {
     std::vector<double> htab;
     DevBuf dev, tab, bond, bij, bown, bcnt, nn, nconj, facc, eacc, wacc, partial, stats, bo, rev, half, hpos,
         list, scan_tmp, bsrc, cnt_s, offs_s, el_s, tag_s, dev32, tab32, bond32, coef, tor, wdir, fnb, fcj,
-        bconj, own, tflag, tpos, tlist;
+        bconj, own, tflag, tpos, tlist, cls, fpr;
     int z2db[128];
     uint64_t Z_rev = ~0ull;
     double cutoff = 0.0;
+    int tor_g = 4;      // lanes per half pair of the torsion pass (4, 8 or 16)
 };
 
 // ---------------------------------------------------------------------------
@@ -625,6 +626,12 @@ __global__ void k_rb_zero(unsigned long long *__restrict__ a, size_t n)
         a[q] = 0ull;
 }
 
+// Width classes of the per-centre neighbour pass: up to 4, 8 and 16 bonds
+// (groups of 4, 8 and 16 lanes).  Class counts are packed into one 64-bit
+// word, RB_CLS_BITS bits each, for a single scan (nat < 2^RB_CLS_BITS).
+constexpr int RB_CLS_BITS = 21;
+__device__ __forceinline__ int rb_class(int nb) { return nb <= 4 ? 0 : (nb <= 8 ? 1 : 2); }
+
 // row lengths (the bond slots reserved per slot), elements and LAMMPS tags
 // in slot order; cnt_s[nat] = 0 closes the scan
 __global__ void k_rb_slots(int nat, const int *__restrict__ order, NlRows L, const int *__restrict__ el,
@@ -651,7 +658,8 @@ __global__ void __launch_bounds__(256)
 k_rb_bonds(int nat, const int *__restrict__ order, const int *__restrict__ el_s, CellArg C,
            const RbDev *__restrict__ Dp, NlRows L, const int *__restrict__ offs_s, const int *__restrict__ offsets,
            RbBond *__restrict__ bond, RbBondT<float> *__restrict__ bond32, int2 *__restrict__ bij,
-           int *__restrict__ bown, int *__restrict__ bcnt, double2 *__restrict__ nn, int *__restrict__ bsrc)
+           int *__restrict__ bown, int *__restrict__ bcnt, double2 *__restrict__ nn, int *__restrict__ bsrc,
+           unsigned long long *__restrict__ cflag)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= nat) return;
@@ -713,6 +721,25 @@ k_rb_bonds(int nat, const int *__restrict__ order, const int *__restrict__ el_s,
     for (int s = beg + nb; s < end; ++s) bown[s] = -1;
     bcnt[p] = nb;
     nn[p] = make_double2(nC, nH);
+    if (cflag) {
+        cflag[p] = 1ull << (RB_CLS_BITS * rb_class(nb));
+        if (p == nat - 1) cflag[nat] = 0ull;
+    }
+}
+
+// The centres of each width class (rb_class) in ascending slot order, from
+// the exclusive scan of the packed class flags: list k at clist + k*nat,
+// its length in ccount[k]
+__global__ void k_rb_class_list(int nat, const int *__restrict__ bcnt, const unsigned long long *__restrict__ cpos,
+                                int *__restrict__ clist, int *__restrict__ ccount)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned long long m = (1ull << RB_CLS_BITS) - 1;
+    if (p == nat)
+        for (int k = 0; k < 3; ++k) ccount[k] = (int)((cpos[nat] >> (RB_CLS_BITS * k)) & m);
+    if (p >= nat) return;
+    const int k = rb_class(bcnt[p]);
+    clist[(size_t)k * nat + (int)((cpos[p] >> (RB_CLS_BITS * k)) & m)] = p;
 }
 
 __global__ void __launch_bounds__(256)
@@ -912,13 +939,16 @@ k_rb_bo(int ntot, const RbDevT<R> *__restrict__ Dp, const R *__restrict__ tab, c
         const RbBondT<R> *__restrict__ bond, const int2 *__restrict__ bij, const double2 *__restrict__ nn,
         Rb4<R> *__restrict__ bo, int *__restrict__ rev, int *__restrict__ half, int natloc,
         const int *__restrict__ tag, const int *__restrict__ order, int *__restrict__ flags,
-        double4 *__restrict__ coef)
+        double4 *__restrict__ coef, double *__restrict__ fpr)
 {
     const RbDevT<R> &D = *Dp;
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= ntot) return;
-    // neighbour-pass coefficients: zero (.w = 0) until a half pair sets them
+    // neighbour-pass coefficients: zero (.w = 0) until a half pair sets them;
+    // the pair forces by directed bond likewise
     coef[s] = make_double4(0.0, 0.0, 0.0, 0.0);
+    if (fpr)
+        for (int x = 0; x < 3; ++x) fpr[3 * (size_t)s + x] = 0.0;
     const int i = bown[s];
     if (i < 0) {
         half[s] = 0;
@@ -1029,7 +1059,8 @@ k_rb_pair(const int *__restrict__ list, const int *__restrict__ count, const RbD
           unsigned long long *__restrict__ eacc,
           unsigned long long *__restrict__ wacc, double *__restrict__ partial, int *__restrict__ flags,
           const int *__restrict__ bsrc, double *__restrict__ epb, double *__restrict__ fpb,
-          double *__restrict__ wpb, const RbDev *__restrict__ D64p, const RbBond *__restrict__ bond64)
+          double *__restrict__ wpb, const RbDev *__restrict__ D64p, const RbBond *__restrict__ bond64,
+          double *__restrict__ fpr)
 {
     const RbDevT<R> &D = *Dp;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1102,8 +1133,18 @@ k_rb_pair(const int *__restrict__ list, const int *__restrict__ count, const RbD
 #pragma unroll
             for (int x = 0; x < 3; ++x) wij[x + 3 * y] = (B64.len * B64.n[x]) * df[y];
         const double mdf[3] = {-df[0], -df[1], -df[2]};
-        fx_add3(facc, i, df, bad);
-        fx_add3(facc, j, mdf, bad);
+        if (fpr) {
+            // by directed bond (the half pair owns both slots): i's on i->j,
+            // j's on j->i, summed by the owner gather
+#pragma unroll
+            for (int x = 0; x < 3; ++x) {
+                fpr[3 * (size_t)s + x] = df[x];
+                fpr[3 * (size_t)sji + x] = mdf[x];
+            }
+        } else {
+            fx_add3(facc, i, df, bad);
+            fx_add3(facc, j, mdf, bad);
+        }
         // per-bond properties on the list entry of the half pair's bond i->j
         // (bop_kernel_rebo2.f90:2546-2547, 2571-2572, 2762-2763); one thread
         // owns each half pair in each pass
@@ -1202,17 +1243,18 @@ __device__ __forceinline__ double rb_torsion_term(const RbBond &K, const RbBond 
     return term;
 }
 
-// Pass 2, a group of 4 lanes per half pair with T != 0 (C-C with torsion,
-// from the scanned list; the grid strides over the list): lane q takes the
-// terms q, q + 4, ... of the torsion sum over k (bonds of i but j) and l
-// (bonds of j but i, and not k at the same image) (bop_kernel_rebo2.f90:
-// 1939-2120, always on the FP64 records), with their forces on k and l; the
-// group sums give T sum, the forces on i, j and the virial.  Then its share
+// Pass 2, a group of G lanes (4, 8 or 16: a launch variant) per half pair
+// with T != 0 (C-C with torsion, from the scanned list; the grid strides
+// over the list): the torsion sum over k (bonds of i but j) and l (bonds of
+// j but i, and not k at the same image) (bop_kernel_rebo2.f90:1939-2120,
+// always on the FP64 records), lane q taking the bonds k = q, q + G, ...
+// while the group walks the bonds l in order, with their forces on k and l;
+// the group sums give T sum, the forces on i, j and the virial.  Then its share
 // T sum / 2 of the bond order in the pair energy and radial force, and its
 // dT/dN terms added to both directed bonds' coefficients before the
 // neighbour pass.  Pairs are dealt to groups in a fixed order, so the block
 // records do not depend on scheduling.
-template <typename R>
+template <typename R, int G>
 __global__ void __launch_bounds__(256)
 k_rb_torsion(const int *__restrict__ list, const int *__restrict__ tlist, const int *__restrict__ tcount,
              const int *__restrict__ offsets, const int *__restrict__ bcnt, const int *__restrict__ bown,
@@ -1221,9 +1263,9 @@ k_rb_torsion(const int *__restrict__ list, const int *__restrict__ tlist, const 
              unsigned long long *__restrict__ facc, unsigned long long *__restrict__ eacc,
              unsigned long long *__restrict__ wacc, double *__restrict__ partial, int *__restrict__ flags,
              const int *__restrict__ bsrc, double *__restrict__ epb, double *__restrict__ fpb,
-             double *__restrict__ wpb, const RbDev *__restrict__ D64p, const RbBond *__restrict__ bond64)
+             double *__restrict__ wpb, const RbDev *__restrict__ D64p, const RbBond *__restrict__ bond64,
+             double *__restrict__ fpr)
 {
-    constexpr int G = 4;
     const int lane = threadIdx.x % G;
     const int ngroups = gridDim.x * (256 / G);
     int bad = 0;
@@ -1241,7 +1283,7 @@ k_rb_torsion(const int *__restrict__ list, const int *__restrict__ tlist, const 
             tv = tor[t];
         }
         const bool on = tv.x != R(0.0);
-        int s = 0, i = 0, j = 0, sji = 0;
+        int s = 0, i = 0, j = 0, sji = 0, ib = 0, ni = 0, jb = 0, nj = 0, xs = 0, xr = 0, a0 = 0, b0 = 0, c0 = 0;
         RbBond B{};
         double fa = 0.0, dfa = 0.0;
         const double tij = (double)tv.x;
@@ -1256,26 +1298,48 @@ k_rb_torsion(const int *__restrict__ list, const int *__restrict__ tlist, const 
             sji = rev[s];
             B = bond64[s];
             rb_va<double>(*D64p, RB_CC, B.len, fa, dfa);
-            const int ib = offsets[i], ni = bcnt[i] - 1, jb = offsets[j], nj = bcnt[j] - 1;
-            const int xs = s - ib, xr = sji - jb;      // the bond i->j in i's row, j->i in j's row
-            int a0, b0, c0;
+            ib = offsets[i];
+            ni = bcnt[i] - 1;
+            jb = offsets[j];
+            nj = bcnt[j] - 1;
+            xs = s - ib;                 // the bond i->j in i's row
+            xr = sji - jb;               // j->i in j's row
             dc_unpack(eij.y, a0, b0, c0);
-            for (int q = lane; q < ni * nj; q += G) {
-                const int qk = q / nj, ql = q % nj;
-                const int kb = ib + qk + (qk >= xs ? 1 : 0), lb = jb + ql + (ql >= xr ? 1 : 0);
-                const int2 ek = bij[kb], e = bij[lb];
-                if (e.x == ek.x) {
-                    // l == k at the same image
-                    int ak, bk, ck, al, bl, cl;
-                    dc_unpack(ek.y, ak, bk, ck);
-                    dc_unpack(e.y, al, bl, cl);
-                    if (a0 + al == ak && b0 + bl == bk && c0 + cl == ck) continue;
+        }
+        // lane q of the group owns the bonds k = q, q + G, ... of i (but j)
+        // and walks the bonds l of j (but i) in order: its force on k is
+        // summed in registers, the forces on l over the group, so each
+        // atom k or l takes one fixed-point add per pair (per chunk of G
+        // bonds k) in a fixed order
+        const int nkc = on ? (ni + G - 1) / G : 0;      // the same for the whole group
+        for (int kc = 0; kc < nkc; ++kc) {
+            const int qk = kc * G + lane;
+            const bool kv = qk < ni;
+            const int kb = ib + qk + (qk >= xs ? 1 : 0);
+            const int2 ek = kv ? bij[kb] : make_int2(-1, 0);
+            int ak, bk, ck;
+            dc_unpack(ek.y, ak, bk, ck);
+            double fks[3] = {0, 0, 0};
+            for (int ql = 0; ql < nj; ++ql) {
+                const int lb = jb + ql + (ql >= xr ? 1 : 0);
+                const int2 e = bij[lb];
+                double fl[3] = {0, 0, 0};
+                int al, bl, cl;
+                dc_unpack(e.y, al, bl, cl);
+                // l == k at the same image takes no term
+                const bool same = e.x == ek.x && a0 + al == ak && b0 + bl == bk && c0 + cl == ck;
+                if (kv && !same) {
+                    double fk[3];
+                    bdh += rb_torsion_term(bond64[kb], bond64[lb], B.n, B.len, tij, fa, B.fc, fi, fj, fk, fl,
+                                           wij);
+#pragma unroll
+                    for (int x = 0; x < 3; ++x) fks[x] += fk[x];
                 }
-                double fk[3], fl[3];
-                bdh += rb_torsion_term(bond64[kb], bond64[lb], B.n, B.len, tij, fa, B.fc, fi, fj, fk, fl, wij);
-                fx_add3(facc, ek.x, fk, bad);
-                fx_add3(facc, e.x, fl, bad);
+#pragma unroll
+                for (int x = 0; x < 3; ++x) fl[x] = group_sum<G>(fl[x]);
+                if (lane == 0) fx_add3(facc, e.x, fl, bad);
             }
+            if (kv) fx_add3(facc, ek.x, fks, bad);
         }
         bdh = group_sum<G>(bdh);
 #pragma unroll
@@ -1305,8 +1369,17 @@ k_rb_torsion(const int *__restrict__ list, const int *__restrict__ tlist, const 
             for (int y = 0; y < 3; ++y)
 #pragma unroll
                 for (int x = 0; x < 3; ++x) wij[x + 3 * y] += (B.len * B.n[x]) * df[y];
-            fx_add3(facc, i, fi, bad);
-            fx_add3(facc, j, fj, bad);
+            if (fpr) {
+                // the half pair's slots, written by the pair pass
+#pragma unroll
+                for (int x = 0; x < 3; ++x) {
+                    fpr[3 * (size_t)s + x] += fi[x];
+                    fpr[3 * (size_t)sji + x] += fj[x];
+                }
+            } else {
+                fx_add3(facc, i, fi, bad);
+                fx_add3(facc, j, fj, bad);
+            }
             if (bsrc) {
                 const size_t e = (size_t)bsrc[s];
                 if (epb) epb[e] += epair;
@@ -1407,20 +1480,27 @@ k_rb_nbrs(const int *__restrict__ list, const int *__restrict__ count, const RbD
 // result does not depend on scheduling.
 template <typename R, int G>
 __global__ void __launch_bounds__(256)
-k_rb_centre(int nat, const RbDevT<R> *__restrict__ Dp, const int *__restrict__ el, const int *__restrict__ offsets,
+k_rb_centre(const int *__restrict__ clist, const int *__restrict__ ccount, const RbDevT<R> *__restrict__ Dp,
+            const int *__restrict__ el, const int *__restrict__ offsets,
             const int *__restrict__ bcnt, const RbBondT<R> *__restrict__ bond, const int2 *__restrict__ bij,
             const double2 *__restrict__ nn, const Rb4<double> *__restrict__ bo, const double4 *__restrict__ coef,
             double *__restrict__ fnb, double *__restrict__ own, double *__restrict__ bconj,
-            double *__restrict__ partial, int nlo)
+            double *__restrict__ partial)
 {
     const RbDevT<R> &D = *Dp;
     const int l = threadIdx.x % G;
-    const int c = blockIdx.x * (256 / G) + threadIdx.x / G;
-    // this launch takes the centres of nlo < bonds <= G (the others belong
-    // to the launch of their width)
-    const int nb0 = c < nat ? bcnt[c] : -1;
-    const bool cv = nb0 > nlo && nb0 <= G;
-    const int nb = cv ? nb0 : 0;
+    // this launch takes the centres of its width class (list clist, ccount
+    // of them); the grid covers every atom, blocks past the list only leave
+    // a zero record
+    const int ncls = *ccount;
+    if ((int)blockIdx.x * (256 / G) >= ncls) {
+        if (threadIdx.x < 10) partial[(size_t)blockIdx.x * 10 + threadIdx.x] = 0.0;
+        return;
+    }
+    const int gi = blockIdx.x * (256 / G) + threadIdx.x / G;
+    const bool cv = gi < ncls;
+    const int c = cv ? clist[gi] : 0;
+    const int nb = cv ? bcnt[c] : 0;
     const int beg = cv ? offsets[c] : 0;
     const int kc = cv ? el[c] : 0;
     const bool has = l < nb;
@@ -1557,15 +1637,21 @@ k_rb_centre(int nat, const RbDevT<R> *__restrict__ Dp, const int *__restrict__ e
 // opposite on k.
 template <typename R, int G>
 __global__ void __launch_bounds__(256)
-k_rb_conj_force(int nat, const int *__restrict__ offsets, const int *__restrict__ bcnt,
-                const RbBondT<R> *__restrict__ bond, const int *__restrict__ rev, const double *__restrict__ bconj,
-                double *__restrict__ fcj, double *__restrict__ own2, double *__restrict__ partial, int nlo)
+k_rb_conj_force(const int *__restrict__ clist, const int *__restrict__ ccount, const int *__restrict__ offsets,
+                const int *__restrict__ bcnt, const RbBondT<R> *__restrict__ bond, const int *__restrict__ rev,
+                const double *__restrict__ bconj, double *__restrict__ fcj, double *__restrict__ own2,
+                double *__restrict__ partial)
 {
     const int l = threadIdx.x % G;
-    const int k = blockIdx.x * (256 / G) + threadIdx.x / G;
-    const int nb0 = k < nat ? bcnt[k] : -1;
-    const bool cv = nb0 > nlo && nb0 <= G;
-    const int nb = cv ? nb0 : 0;
+    const int ncls = *ccount;
+    if ((int)blockIdx.x * (256 / G) >= ncls) {
+        if (threadIdx.x < 10) partial[(size_t)blockIdx.x * 10 + threadIdx.x] = 0.0;
+        return;
+    }
+    const int gi = blockIdx.x * (256 / G) + threadIdx.x / G;
+    const bool cv = gi < ncls;
+    const int k = cv ? clist[gi] : 0;
+    const int nb = cv ? bcnt[k] : 0;
     const bool has = l < nb;
     const int b = (cv ? offsets[k] : 0) + l;
     const double Bv = has ? bconj[rev[b]] : 0.0;
@@ -1596,13 +1682,14 @@ k_rb_conj_force(int nat, const int *__restrict__ offsets, const int *__restrict_
 }
 
 // f(atom order[p]) += own + own2 + the forces its neighbours' groups left on
-// the bonds towards p (reverse slots) + the fixed-point pair and torsion sums
+// the bonds towards p (reverse slots) + the pair and torsion forces on p by
+// its own bonds (fpr) + the fixed-point sums (torsion forces on k and l)
 __global__ void k_rb_gather(int nat, const int *__restrict__ order, const int *__restrict__ offsets,
                             const int *__restrict__ bcnt, const int *__restrict__ rev, const double *__restrict__ own,
                             const double *__restrict__ own2, const double *__restrict__ fnb,
-                            const double *__restrict__ fcj, unsigned long long *__restrict__ facc,
-                            unsigned long long *__restrict__ eacc, double *__restrict__ f,
-                            double *__restrict__ epa)
+                            const double *__restrict__ fcj, const double *__restrict__ fpr,
+                            unsigned long long *__restrict__ facc, unsigned long long *__restrict__ eacc,
+                            double *__restrict__ f, double *__restrict__ epa)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= nat) return;
@@ -1611,7 +1698,7 @@ __global__ void k_rb_gather(int nat, const int *__restrict__ order, const int *_
     const int beg = offsets[p], nb = bcnt[p];
     for (int b = beg; b < beg + nb; ++b) {
         const size_t r = (size_t)rev[b];
-        for (int x = 0; x < 3; ++x) a[x] += fnb[3 * r + x] + fcj[3 * r + x];
+        for (int x = 0; x < 3; ++x) a[x] += (fnb[3 * r + x] + fcj[3 * r + x]) + fpr[3 * (size_t)b + x];
     }
     const size_t i = (size_t)order[p];
     for (int x = 0; x < 3; ++x) {
@@ -1735,29 +1822,56 @@ static int rebo2_kernels(Rebo2State *st, agpu_potential *pot, const agpu_eval_ar
                                   : reinterpret_cast<const RbBondT<R> *>(bond32);
     Rb4<double> *bo = st->bo.as<Rb4<double>>();
     int *dstats = st->stats.as<int>();
+    // neighbour pass: per centre groups and an owner gather when no per-atom
+    // or per-bond virial is asked for (those split each pair's virial over
+    // its atoms and bond, which the centre sums do not keep); otherwise a
+    // thread per directed bond with fixed-point accumulation.  Centres are
+    // taken by groups of 4 lanes (up to 4 bonds), then 8 and 16 lanes for
+    // the wider rows, one launch per width class over that class's list
+    const int mr = nl->maxrow;
+    const bool by_centre = !wacc && !a->wpot_per_bond && mr <= 16 && nat < (1 << RB_CLS_BITS);
+    unsigned long long *cflag = nullptr, *cpos = nullptr;
+    // pair and torsion forces on i, j by directed bond (the owner gather sums
+    // them), instead of fixed-point adds, on the per-centre pass
+    double *fpr = nullptr;
+    if (by_centre) {
+        AGPU_CHECK(st->fpr.reserve(sizeof(double) * 3 * (size_t)(total + 1)));
+        fpr = st->fpr.as<double>();
+        AGPU_CHECK(st->cls.reserve(sizeof(unsigned long long) * 2 * (size_t)(nat + 1) + sizeof(int) * (3 * (size_t)nat + 4)));
+        cflag = st->cls.as<unsigned long long>();
+        cpos = cflag + nat + 1;
+    }
     AGPU_LAUNCH("rebo2_bonds", k_rb_bonds, dim3(nb), dim3(256), 0, s, nat, order, els, C, D64, nl_rows(nl), offs,
                 nl->offsets.as<int>(), bond, bond32, st->bij.as<int2>(), st->bown.as<int>(), st->bcnt.as<int>(),
-                st->nn.as<double2>(), const_cast<int *>(bsrc));
+                st->nn.as<double2>(), const_cast<int *>(bsrc), cflag);
+    int *clist = nullptr, *ccount = nullptr;
+    if (by_centre) {
+        clist = reinterpret_cast<int *>(cpos + nat + 1);
+        ccount = clist + 3 * (size_t)nat;
+        size_t tmp = 0;
+        AGPU_HIP(rocprim::exclusive_scan(nullptr, tmp, cflag, cpos, 0ull, (size_t)nat + 1,
+                                         rocprim::plus<unsigned long long>(), s));
+        AGPU_CHECK(st->scan_tmp.reserve(tmp + 16));
+        AGPU_HIP(rocprim::exclusive_scan(st->scan_tmp.ptr, tmp, cflag, cpos, 0ull, (size_t)nat + 1,
+                                         rocprim::plus<unsigned long long>(), s));
+        AGPU_LAUNCH("rebo2_classes", k_rb_class_list, dim3((nat + 256) / 256), dim3(256), 0, s, nat,
+                    st->bcnt.as<int>(), cpos, clist, ccount);
+    }
     AGPU_LAUNCH("rebo2_conj", k_rb_conj, dim3(nb), dim3(256), 0, s, nat, els, offs, st->bcnt.as<int>(),
                 st->bij.as<int2>(), st->nn.as<double2>(), bond, bond32, st->nconj.as<double>());
     const int ntot = (int)total;
     int nblk = (ntot + 255) / 256;
     if (nblk < 1) nblk = 1;
     *nblk_out = nblk;
-    // neighbour pass: per centre groups and an owner gather when no per-atom
-    // or per-bond virial is asked for (those split each pair's virial over
-    // its atoms and bond, which the centre sums do not keep); otherwise a
-    // thread per directed bond with fixed-point accumulation
-    // centres are taken by groups of 4 lanes (up to 4 bonds), then 8 and 16
-    // lanes for the wider rows, one launch per width present
-    const int mr = nl->maxrow;
-    const bool by_centre = !wacc && !a->wpot_per_bond && mr <= 16;
     const int nb4 = (nat + 63) / 64, nb8 = mr > 4 ? (nat + 31) / 32 : 0, nb16 = mr > 8 ? (nat + 15) / 16 : 0;
     const int nbc = nb4 + nb8 + nb16;
     // block records: pair, torsion, then the neighbour pass (2 nblk, or
     // centre and conjugation passes)
     // torsion groups: 64 per block, the grid striding over the pairs
-    const int ntb = std::max(1, std::min((ntot + 63) / 64, 2048));
+    // torsion groups: 256 / G per block, at most 2^17 groups, the grid
+    // striding over the pairs
+    const int tg = st->tor_g, tgpb = 256 / tg;
+    const int ntb = std::max(1, std::min((ntot + tgpb - 1) / tgpb, (1 << 17) / tgpb));
     const size_t nrec = (size_t)nblk + (size_t)ntb + (by_centre ? 2 * (size_t)nbc : 2 * (size_t)nblk);
     AGPU_CHECK(st->partial.reserve(sizeof(double) * 10 * nrec));
     AGPU_CHECK(st->coef.reserve(sizeof(double4) * (total + 1)));
@@ -1770,7 +1884,7 @@ static int rebo2_kernels(Rebo2State *st, agpu_potential *pot, const agpu_eval_ar
         AGPU_LAUNCH("rebo2_bo", k_rb_bo<double>, dim3(nblk), dim3(256), 0, s, ntot, D64, st->tab.as<double>(),
                     els, offs, st->bcnt.as<int>(), st->bown.as<int>(), bond, st->bij.as<int2>(),
                     st->nn.as<double2>(), bo, st->rev.as<int>(), st->half.as<int>(), p->natloc,
-                    lammps ? st->tag_s.as<int>() : nullptr, order, dstats, st->coef.as<double4>());
+                    lammps ? st->tag_s.as<int>() : nullptr, order, dstats, st->coef.as<double4>(), fpr);
         size_t tmp = 0;
         AGPU_HIP(rocprim::exclusive_scan(nullptr, tmp, st->half.as<int>(), st->hpos.as<int>(), 0, (size_t)ntot,
                                          rocprim::plus<int>(), s));
@@ -1806,7 +1920,7 @@ static int rebo2_kernels(Rebo2State *st, agpu_potential *pot, const agpu_eval_ar
                 D, tab, els, offs, st->bcnt.as<int>(), st->bown.as<int>(), bondR, st->bij.as<int2>(),
                 st->nn.as<double2>(), st->nconj.as<double>(), bo, st->rev.as<int>(), coef, tor, tflag, ntot,
                 st->facc.as<unsigned long long>(), eacc, wacc, part, dstats, bsrc, a->epot_per_bond,
-                a->f_per_bond, a->wpot_per_bond, D64, bond);
+                a->f_per_bond, a->wpot_per_bond, D64, bond, fpr);
     if (tors && ntot > 0) {
         // the half pairs with T != 0, in list order (flags + scan)
         size_t tmp = 0;
@@ -1819,10 +1933,11 @@ static int rebo2_kernels(Rebo2State *st, agpu_potential *pot, const agpu_eval_ar
                     tlist, tcount);
     }
     // without torsion (tor == nullptr) the pass only writes zero records
-    AGPU_LAUNCH("rebo2_torsion", k_rb_torsion<R>, dim3(ntb), dim3(256), 0, s, list, tlist, tcount, offs,
+    auto ktor = tg == 16 ? k_rb_torsion<R, 16> : (tg == 8 ? k_rb_torsion<R, 8> : k_rb_torsion<R, 4>);
+    AGPU_LAUNCH("rebo2_torsion", ktor, dim3(ntb), dim3(256), 0, s, list, tlist, tcount, offs,
                 st->bcnt.as<int>(), st->bown.as<int>(), st->bij.as<int2>(), st->rev.as<int>(), tor, coef,
                 st->facc.as<unsigned long long>(), eacc, wacc, part + 10 * (size_t)nblk, dstats, bsrc,
-                a->epot_per_bond, a->f_per_bond, a->wpot_per_bond, D64, bond);
+                a->epot_per_bond, a->f_per_bond, a->wpot_per_bond, D64, bond, fpr);
     if (by_centre) {
         AGPU_CHECK(st->fnb.reserve(sizeof(double) * 3 * (total + 1)));
         AGPU_CHECK(st->fcj.reserve(sizeof(double) * 3 * (total + 1)));
@@ -1831,7 +1946,6 @@ static int rebo2_kernels(Rebo2State *st, agpu_potential *pot, const agpu_eval_ar
         double *own = st->own.as<double>(), *own2 = own + 3 * (size_t)nat;
         double *pc = part + 10 * ((size_t)nblk + ntb), *pj = pc + 10 * (size_t)nbc;
         const int nbw[3] = {nb4, nb8, nb16};
-        const int lo[3] = {-1, 4, 8};
         auto cen = [&](int w) {
             return w == 0 ? k_rb_centre<R, 4> : (w == 1 ? k_rb_centre<R, 8> : k_rb_centre<R, 16>);
         };
@@ -1841,9 +1955,10 @@ static int rebo2_kernels(Rebo2State *st, agpu_potential *pot, const agpu_eval_ar
         size_t off = 0;
         for (int w = 0; w < 3; ++w) {
             if (nbw[w] == 0) continue;
-            AGPU_LAUNCH(f64 ? "rebo2_centre" : "rebo2_centre_mixed", cen(w), dim3(nbw[w]), dim3(256), 0, s, nat, D,
-                        els, offs, st->bcnt.as<int>(), bondR, st->bij.as<int2>(), st->nn.as<double2>(), bo, coef,
-                        st->fnb.as<double>(), own, st->bconj.as<double>(), pc + 10 * off, lo[w]);
+            AGPU_LAUNCH(f64 ? "rebo2_centre" : "rebo2_centre_mixed", cen(w), dim3(nbw[w]), dim3(256), 0, s,
+                        clist + (size_t)w * nat, ccount + w, D, els, offs, st->bcnt.as<int>(), bondR,
+                        st->bij.as<int2>(), st->nn.as<double2>(), bo, coef, st->fnb.as<double>(), own,
+                        st->bconj.as<double>(), pc + 10 * off);
             off += nbw[w];
         }
         // the bond scalars of every width are in place before any conjugation force
@@ -1851,12 +1966,12 @@ static int rebo2_kernels(Rebo2State *st, agpu_potential *pot, const agpu_eval_ar
         for (int w = 0; w < 3; ++w) {
             if (nbw[w] == 0) continue;
             AGPU_LAUNCH(f64 ? "rebo2_conj_force" : "rebo2_conj_force_mixed", cjf(w), dim3(nbw[w]), dim3(256), 0, s,
-                        nat, offs, st->bcnt.as<int>(), bondR, st->rev.as<int>(), st->bconj.as<double>(),
-                        st->fcj.as<double>(), own2, pj + 10 * off, lo[w]);
+                        clist + (size_t)w * nat, ccount + w, offs, st->bcnt.as<int>(), bondR, st->rev.as<int>(),
+                        st->bconj.as<double>(), st->fcj.as<double>(), own2, pj + 10 * off);
             off += nbw[w];
         }
         AGPU_LAUNCH("rebo2_gather", k_rb_gather, dim3(nb), dim3(256), 0, s, nat, order, offs, st->bcnt.as<int>(),
-                    st->rev.as<int>(), own, own2, st->fnb.as<double>(), st->fcj.as<double>(),
+                    st->rev.as<int>(), own, own2, st->fnb.as<double>(), st->fcj.as<double>(), fpr,
                     st->facc.as<unsigned long long>(), eacc, a->f, a->epot_per_at);
         *nblk_out = nblk + ntb + 2 * nbc;
         *gathered = true;
@@ -1872,6 +1987,15 @@ static int rebo2_kernels(Rebo2State *st, agpu_potential *pot, const agpu_eval_ar
     *nblk_out = 3 * nblk + ntb;
     *gathered = false;
     return AGPU_OK;
+}
+
+int rebo2_variant(agpu_potential *pot, int k, int *g)
+{
+    constexpr int TG[3] = {4, 8, 16};
+    Rebo2State *st = static_cast<Rebo2State *>(pot->impl);
+    if (k >= 0 && k < 3) st->tor_g = TG[k];
+    if (g) *g = st->tor_g;
+    return 3;
 }
 
 int rebo2_compute(agpu_potential *pot, const agpu_eval_args *a)

@@ -555,16 +555,22 @@ def bench_md(args):
     print(json.dumps(out))
 
 
-def roofline(workload, kern, nn, nat):
+def roofline(workload, kern, nn, nat, steps=None):
     """HBM roofline of the dominant kernel (largest total time): algorithmic
     bytes per launch over its measured average duration, the counter traffic
-    of the committed rocprofv3 summary, and its FP64 VALU fraction."""
+    of the committed rocprofv3 summary, and its FP64 VALU fraction.  A kernel
+    launched k > 1 times per step over disjoint shares of the atoms (one
+    launch per row-width class) is taken per step: all atoms over the k
+    launches' summed duration."""
     if not kern:
         return None
     dom = max(kern, key=lambda k: kern[k]["avg_us"] * kern[k]["launches"])
     bpa = algorithmic_bytes(dom, nn)
     if bpa is None:
         return None
+    per_step = 1
+    if steps and kern[dom]["launches"] > steps and dom.startswith("rebo2_"):
+        per_step = kern[dom]["launches"] // steps
     traffic = None
     tf = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tf):
@@ -574,10 +580,14 @@ def roofline(workload, kern, nn, nat):
                 traffic = ent["bytes_per_launch"]
         except ValueError:
             traffic = None
-    achieved = bpa * nat / (kern[dom]["avg_us"] * 1e-6) / 1e9
+    achieved = bpa * nat / (per_step * kern[dom]["avg_us"] * 1e-6) / 1e9
     roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
-            "algorithmic_bytes_per_atom": bpa, "atoms_per_launch": nat}
+            "algorithmic_bytes_per_atom": bpa, "atoms_per_launch": nat if per_step == 1 else None,
+            "atoms_per_step": nat, "launches_per_step": per_step}
+    if per_step > 1 and traffic:
+        traffic = traffic * per_step          # the summary's bytes are per launch
+        roof["traffic"] = traffic
     if traffic:
         roof["traffic_over_algorithmic"] = traffic / (bpa * nat)
     valu = valu_rates(workload, kern)
@@ -652,7 +662,7 @@ def potential_block(workload, args, precision="fp64", autotune=True, cpu=True, m
                      "frac_full_step": value * B / (HBM_PEAK_GBS * 1e9),
                      "frac_force_only": nat * args.steps / dt_force * B / (HBM_PEAK_GBS * 1e9)},
         "kernels": kern,
-        "roofline": roofline(workload, kern, nn, nat),
+        "roofline": roofline(workload, kern, nn, nat, args.steps),
         "valu": valu_rates(workload, kern),
         "autotune": tune,
     }

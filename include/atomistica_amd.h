@@ -373,10 +373,11 @@ int agpu_potential_energy_and_forces_noalloc(agpu_potential *pot, agpu_particles
  * configuration with HIP events, reps calls each, and keeps the fastest for
  * later calls on pot and nl -- the neighbour-list pair search (thread per
  * atom or wavefront per bin block; identical rows), for single-element
- * tabulated EAM, lanes per atom x row entries per lane, and for the
+ * tabulated EAM, lanes per atom x row entries per lane, for the
  * unscreened bond-order classes, the row path (a lane per row entry) or the
- * bond-compacting path.  A text report of the timings and choices is written
- * to report (len bytes, may be NULL). */
+ * bond-compacting path, and for REBO2, the lanes per half pair of the
+ * torsion pass (4, 8 or 16).  A text report of the timings and choices is
+ * written to report (len bytes, may be NULL). */
 int agpu_potential_autotune(agpu_potential *pot, agpu_particles *p, agpu_neighbors *nl, int reps,
                             char *report, int len);
 
@@ -386,6 +387,7 @@ int agpu_potential_autotune(agpu_potential *pot, agpu_particles *p, agpu_neighbo
  * variant, else selects 0 <= variant < *nvariants, the number of variants
  * this binding offers (single-element EAM: the lanes x entries shapes;
  * unscreened bond-order classes: 0 = row path, 1 = bond-compacting path;
+ * REBO2: 0, 1, 2 = 4, 8, 16 lanes per half pair in the torsion pass;
  * 0 for the other classes, variant 0 is then the only accepted value). */
 int agpu_potential_set_launch(agpu_potential *pot, agpu_neighbors *nl, int pair_mode, int variant,
                               int *nvariants);

@@ -7,7 +7,10 @@ Bars:
     lane) gives the oracle's energy, forces and virial within 1e-10
     relative, and the variants agree with each other to 1e-12;
   - autotune() reports its timings, keeps a variant, and the results after
-    it still meet the oracle bar; set_launch() rejects unknown variants.
+    it still meet the oracle bar; set_launch() rejects unknown variants;
+  - every REBO2 torsion width (4, 8, 16 lanes per half pair) gives the
+    oracle's energy, forces and virial with torsion on within 1e-10, and
+    the widths agree with each other to 1e-12.
 """
 import os
 import tempfile
@@ -158,3 +161,31 @@ def test_autotune_bond_order_report():
     from atomistica_amd import parameters as P
     e0, f0, w0 = O.tersoff(P.Tersoff_PRB_39_5566_Si_C, pos, np.ones(len(pos), np.int32), cell, nlo)
     assert abs(e - e0) <= REL * abs(e0) and close(f, f0) and close(w, w0)
+
+
+def test_rebo2_torsion_widths_parity():
+    from helpers import read_cfg
+    pos, cell, pbc, Z = read_cfg("aC_small.cfg")
+    rng = np.random.default_rng(11)
+    Z = np.where(rng.random(len(pos)) < 0.15, 1, 6).astype(np.int32)
+    e0, f0, w0 = O.Rebo2(dihedral=True)(pos, Z, cell, O.nl_build(pos, cell, pbc, 2.0))[:3]
+    out = []
+    for variant in (0, 1, 2):
+        p = N.from_atoms(pos, cell, pbc, Z)
+        nl = N.Neighbors(100)
+        pot = N.Rebo2(dihedral=True)
+        pot.bind_to(p, nl)
+        assert pot.set_launch(nl, 0, variant) == 3
+        e, f, w, *_ = pot.energy_and_forces(p, nl)
+        print("rebo2 torsion variant", variant, "E", e, "oracle", e0, flush=True)
+        assert abs(e - e0) <= REL * max(1.0, abs(e0))
+        assert close(f, f0) and close(w, w0)
+        out.append(f)
+    assert close(out[0], out[1], 1e-12) and close(out[0], out[2], 1e-12)
+    with pytest.raises(AgpuError):
+        pot.set_launch(nl, 0, 3)
+    rep = pot.autotune(p, nl, reps=2)
+    print(rep, flush=True)
+    assert "rebo2 torsion lanes_per_pair 16" in rep and "rebo2 -> torsion lanes_per_pair" in rep
+    e, f, w, *_ = pot.energy_and_forces(p, nl)
+    assert abs(e - e0) <= REL * max(1.0, abs(e0)) and close(f, f0) and close(w, w0)
