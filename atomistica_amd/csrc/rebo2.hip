@@ -9,28 +9,38 @@
 // Forces and per-atom properties go back to the caller's order at the end.
 //
 // Kernels (one stream, no host round trip inside a call):
-//   k_rb_zero     clear the 64-bit fixed-point accumulators
-//   k_rb_slots    row lengths, elements (and LAMMPS tags) in slot order
+//   k_rb_slots    elements (and LAMMPS tags) in slot order; clears the
+//                 fixed-point accumulators, flags and energy.  The bond
+//                 slots of slot p are its list row's entries (the list's
+//                 slot-order offsets)
 //   k_rb_bonds    thread per slot: compact the neighbour row to its bonds
 //                 (r^2 < r2^2 of the pair type), bond record = unit vector,
 //                 length, cut-off and derivative; coordination N_C, N_H
-//                 (loop1 and the nn sum of bop_kernel_rebo2.f90:700-1205)
+//                 (loop1 and the nn sum of bop_kernel_rebo2.f90:700-1205);
+//                 the centre's width class (4, 8, 16 bonds), listed per
+//                 class by k_rb_class_list after a scan
 //   k_rb_conj     thread per atom: F(x_ik) of every carbon neighbour k and
 //                 the sum N^conj_i (ik_loop1, bop_kernel_rebo2.f90:1231-1325)
 //   k_rb_bo       thread per directed bond: zeta with P, b and dV_A b', the
 //                 reverse bond's slot, the half-pair flag
 //   k_rb_list     ordered list of half pairs (exclusive scan of the flags)
 //   k_rb_pair     thread per half pair i->j ((dc = 0 and j > i) or dc > 0):
-//                 F_ij(N_i, N_j, N_conj), the torsion sum
-//                 T_ij(N_i, N_j, N_conj) sum_kl (1 - cos^2 w_kijl), the pair energy
-//                 and every force on i, j, k, l, m, n (ij_loop,
-//                 bop_kernel_rebo2.f90:1326-2779).  Forces, per-atom energies
-//                 and virials go to 64-bit fixed-point accumulators with
-//                 integer atomics (exact and order independent, so results
-//                 are bitwise reproducible); total energy and virial are
-//                 reduced per block and then in a fixed order.
-//   k_rb_finish   fixed point -> FP64, added to the caller's arrays (slot p
-//                 -> atom order[p]).
+//                 F_ij(N_i, N_j, N_conj) and the torsion factor
+//                 T_ij(N_i, N_j, N_conj), the pair energy and radial force,
+//                 the neighbour-pass coefficients of both directed bonds
+//                 (ij_loop, bop_kernel_rebo2.f90:1326-2779)
+//   k_rb_torsion  group per half pair with T != 0: the torsion sum over the
+//                 bonds k of i and l of j with its forces
+//   neighbour pass, per centre (k_rb_centre, k_rb_conj_force per width
+//                 class, then the owner gather k_rb_gather), or, when
+//                 per-atom or per-bond virials are asked for, a thread per
+//                 directed bond (k_rb_nbrs) and k_rb_finish.
+// Forces on an atom from its own bonds' passes are kept per directed bond
+// and summed by the owner; forces that many half pairs put on one atom
+// (torsion k and l, the per-bond-thread pass) go to 64-bit fixed-point
+// accumulators with integer atomics (exact and order independent).  Every
+// sum therefore has a fixed order, and results are bitwise reproducible;
+// total energy and virial are reduced per block, then in a fixed order.
 //
 // Tables (F_CC, F_CH, F_HH, T_CC tricubic, P_CC, P_CH bicubic) are built on the
 // host from their node data in closed form: the tensor product of cubic
@@ -125,7 +135,7 @@ struct Rebo2State {
     RbDev hdev{};
     std::vector<double> htab;
     DevBuf dev, tab, bond, bij, bown, bcnt, nn, nconj, facc, eacc, wacc, partial, stats, bo, rev, half, hpos,
-        list, scan_tmp, bsrc, cnt_s, offs_s, el_s, tag_s, dev32, tab32, bond32, coef, tor, wdir, fnb, fcj,
+        list, scan_tmp, bsrc, el_s, tag_s, dev32, tab32, bond32, coef, tor, wdir, fnb, fcj,
         bconj, own, tflag, tpos, tlist, cls, fpr;
     int z2db[128];
     uint64_t Z_rev = ~0ull;
@@ -620,11 +630,6 @@ __device__ __forceinline__ bool dc_zero_sum(int c1, int c2)
 // ---------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------
-__global__ void k_rb_zero(unsigned long long *__restrict__ a, size_t n)
-{
-    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
-        a[q] = 0ull;
-}
 
 // Width classes of the per-centre neighbour pass: up to 4, 8 and 16 bonds
 // (groups of 4, 8 and 16 lanes).  Class counts are packed into one 64-bit
@@ -632,21 +637,25 @@ __global__ void k_rb_zero(unsigned long long *__restrict__ a, size_t n)
 constexpr int RB_CLS_BITS = 21;
 __device__ __forceinline__ int rb_class(int nb) { return nb <= 4 ? 0 : (nb <= 8 ? 1 : 2); }
 
-// row lengths (the bond slots reserved per slot), elements and LAMMPS tags
-// in slot order; cnt_s[nat] = 0 closes the scan
-__global__ void k_rb_slots(int nat, const int *__restrict__ order, NlRows L, const int *__restrict__ el,
-                           const int *__restrict__ tag, int *__restrict__ cnt_s, int *__restrict__ el_s,
-                           int *__restrict__ tag_s)
+// Elements and LAMMPS tags in slot order (the bond slots of slot p are its
+// row's entries: the list's slot-order offsets); also clears the call's
+// fixed-point accumulators (eacc, wacc optional), flags and energy
+__global__ void k_rb_slots(int nat, const int *__restrict__ order, const int *__restrict__ el,
+                           const int *__restrict__ tag, int *__restrict__ el_s, int *__restrict__ tag_s,
+                           unsigned long long *__restrict__ facc, unsigned long long *__restrict__ eacc,
+                           unsigned long long *__restrict__ wacc, int *__restrict__ flags, double *__restrict__ epot)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p == nat) cnt_s[nat] = 0;
+    if (p < 4) flags[p] = 0;
+    if (p == 0) *epot = 0.0;
     if (p >= nat) return;
     const int i = order[p];
-    int n;
-    nl_row(L, p, n);
-    cnt_s[p] = n;
     el_s[p] = el[i];
     if (tag) tag_s[p] = tag[i];
+    for (int x = 0; x < 3; ++x) facc[3 * (size_t)p + x] = 0ull;
+    if (eacc) eacc[p] = 0ull;
+    if (wacc)
+        for (int q = 0; q < 9; ++q) wacc[9 * (size_t)p + q] = 0ull;
 }
 
 // thread per slot p (atom order[p]): its bonds at offs_s[p] .., neighbours
@@ -1703,11 +1712,9 @@ __global__ void k_rb_gather(int nat, const int *__restrict__ order, const int *_
     const size_t i = (size_t)order[p];
     for (int x = 0; x < 3; ++x) {
         f[3 * i + x] += a[x] + (double)(long long)facc[3 * (size_t)p + x] * RB_FX_INV;
-        facc[3 * (size_t)p + x] = 0ull;
     }
     if (epa) {
         epa[i] += (double)(long long)eacc[p] * RB_FX_INV;
-        eacc[p] = 0ull;
     }
 }
 
@@ -1731,16 +1738,13 @@ __global__ void k_rb_finish(int nat, const int *__restrict__ order, unsigned lon
     const size_t i = (size_t)order[p];
     for (int x = 0; x < 3; ++x) {
         f[3 * i + x] += (double)(long long)facc[3 * (size_t)p + x] * RB_FX_INV;
-        facc[3 * (size_t)p + x] = 0ull;
     }
     if (epa) {
         epa[i] += (double)(long long)eacc[p] * RB_FX_INV;
-        eacc[p] = 0ull;
     }
     if (wpa)
         for (int q = 0; q < 9; ++q) {
             wpa[9 * i + q] += (double)(long long)wacc[9 * (size_t)p + q] * RB_FX_INV;
-            wacc[9 * (size_t)p + q] = 0ull;
         }
 }
 
@@ -1814,7 +1818,7 @@ static int rebo2_kernels(Rebo2State *st, agpu_potential *pot, const agpu_eval_ar
     const int nb = (nat + 255) / 256;
     const int *order = nl->idx_sorted.as<int>();
     const bool lammps = p->natloc >= 0;
-    const int *offs = st->offs_s.as<int>();
+    const int *offs = nl_rows(nl).off;
     const int *els = st->el_s.as<int>();
     RbBond *bond = st->bond.as<RbBond>();
     RbBondT<float> *bond32 = f64 ? nullptr : st->bond32.as<RbBondT<float>>();
@@ -2010,8 +2014,7 @@ int rebo2_compute(agpu_potential *pot, const agpu_eval_args *a)
     AGPU_CHECK(nl_update(nl, p, 0, false));
     const int nat = p->nat;
     hipStream_t s = stream();
-    AGPU_CHECK(zero_dev(a->epot, sizeof(double)));
-    if (nat == 0) return AGPU_OK;
+    if (nat == 0) return zero_dev(a->epot, sizeof(double));
     const int64_t total = nl->total;
     AGPU_CHECK(st->bij.reserve(sizeof(int2) * (total + 1)));
     AGPU_CHECK(st->bown.reserve(sizeof(int) * (total + 1)));
@@ -2030,13 +2033,7 @@ int rebo2_compute(agpu_potential *pot, const agpu_eval_args *a)
         AGPU_CHECK(st->wacc.reserve(sizeof(unsigned long long) * 9 * (nat + 1)));
         wacc = st->wacc.as<unsigned long long>();
     }
-    // accumulators start at zero (k_rb_finish leaves them zero for the next call)
-    AGPU_LAUNCH("rebo2_zero", k_rb_zero, dim3(512), dim3(256), 0, s, st->facc.as<unsigned long long>(),
-                (size_t)3 * nat);
-    if (eacc) AGPU_LAUNCH("rebo2_zero", k_rb_zero, dim3(256), dim3(256), 0, s, eacc, (size_t)nat);
-    if (wacc) AGPU_LAUNCH("rebo2_zero", k_rb_zero, dim3(512), dim3(256), 0, s, wacc, (size_t)9 * nat);
     int *dstats = st->stats.as<int>();
-    AGPU_CHECK(zero_dev(dstats, sizeof(int) * 4));
     const int *el = pot->d_el.as<int>();
     const int nb = (nat + 255) / 256;
     const bool per_bond = a->epot_per_bond || a->f_per_bond || a->wpot_per_bond;
@@ -2044,24 +2041,16 @@ int rebo2_compute(agpu_potential *pot, const agpu_eval_args *a)
     // the bonds' entries in the caller-order list (the reference's indexing)
     if (per_bond) AGPU_CHECK(nl_caller_offsets(nl));
     int *bsrc = per_bond ? st->bsrc.as<int>() : nullptr;
-    // slot order: row lengths -> bond offsets per slot, elements, tags
+    // slot order: elements, tags; the bond slots of slot p are its row's
+    // entries (the list's slot-order offsets); accumulators, flags and
+    // energy cleared in the same pass
     const int *order = nl->idx_sorted.as<int>();
     const bool lammps = p->natloc >= 0;
-    AGPU_CHECK(st->cnt_s.reserve(sizeof(int) * (nat + 1)));
-    AGPU_CHECK(st->offs_s.reserve(sizeof(int) * (nat + 1)));
     AGPU_CHECK(st->el_s.reserve(sizeof(int) * (nat + 1)));
     if (lammps) AGPU_CHECK(st->tag_s.reserve(sizeof(int) * (nat + 1)));
-    AGPU_LAUNCH("rebo2_slots", k_rb_slots, dim3((nat + 256) / 256), dim3(256), 0, s, nat, order, nl_rows(nl), el,
-                lammps ? p->d_tag.as<int>() : nullptr, st->cnt_s.as<int>(), st->el_s.as<int>(),
-                lammps ? st->tag_s.as<int>() : nullptr);
-    {
-        size_t tmp = 0;
-        AGPU_HIP(rocprim::exclusive_scan(nullptr, tmp, st->cnt_s.as<int>(), st->offs_s.as<int>(), 0,
-                                         (size_t)nat + 1, rocprim::plus<int>(), s));
-        AGPU_CHECK(st->scan_tmp.reserve(tmp + 16));
-        AGPU_HIP(rocprim::exclusive_scan(st->scan_tmp.ptr, tmp, st->cnt_s.as<int>(), st->offs_s.as<int>(), 0,
-                                         (size_t)nat + 1, rocprim::plus<int>(), s));
-    }
+    AGPU_LAUNCH("rebo2_slots", k_rb_slots, dim3((std::max(nat, 4) + 255) / 256), dim3(256), 0, s, nat, order, el,
+                lammps ? p->d_tag.as<int>() : nullptr, st->el_s.as<int>(), lammps ? st->tag_s.as<int>() : nullptr,
+                st->facc.as<unsigned long long>(), eacc, wacc, dstats, a->epot);
     int nblk = 0;
     bool gathered = false;
     if (pot->precision == AGPU_PREC_MIXED)
