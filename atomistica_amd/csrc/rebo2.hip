@@ -141,6 +141,9 @@ struct Rebo2State {
     uint64_t Z_rev = ~0ull;
     double cutoff = 0.0;
     int tor_g = 4;      // lanes per half pair of the torsion pass (4, 8 or 16)
+    // centres per width class on the last per-centre pass (-1: none yet)
+    int chint[3] = {-1, -1, -1};
+    bool chint_pending = false;
 };
 
 // ---------------------------------------------------------------------------
@@ -1488,25 +1491,16 @@ k_rb_nbrs(const int *__restrict__ list, const int *__restrict__ count, const RbD
 // atoms afterwards (k_rb_gather), so the pass is free of atomics and its
 // result does not depend on scheduling.
 template <typename R, int G>
-__global__ void __launch_bounds__(256)
-k_rb_centre(const int *__restrict__ clist, const int *__restrict__ ccount, const RbDevT<R> *__restrict__ Dp,
-            const int *__restrict__ el, const int *__restrict__ offsets,
-            const int *__restrict__ bcnt, const RbBondT<R> *__restrict__ bond, const int2 *__restrict__ bij,
-            const double2 *__restrict__ nn, const Rb4<double> *__restrict__ bo, const double4 *__restrict__ coef,
-            double *__restrict__ fnb, double *__restrict__ own, double *__restrict__ bconj,
-            double *__restrict__ partial)
+__device__ __forceinline__ void
+rb_centre_chunk(int chunk, const int *__restrict__ clist, int ncls, const RbDevT<R> &D,
+                const int *__restrict__ el, const int *__restrict__ offsets,
+                const int *__restrict__ bcnt, const RbBondT<R> *__restrict__ bond, const int2 *__restrict__ bij,
+                const double2 *__restrict__ nn, const Rb4<double> *__restrict__ bo, const double4 *__restrict__ coef,
+                double *__restrict__ fnb, double *__restrict__ own, double *__restrict__ bconj, double *w)
 {
-    const RbDevT<R> &D = *Dp;
     const int l = threadIdx.x % G;
-    // this launch takes the centres of its width class (list clist, ccount
-    // of them); the grid covers every atom, blocks past the list only leave
-    // a zero record
-    const int ncls = *ccount;
-    if ((int)blockIdx.x * (256 / G) >= ncls) {
-        if (threadIdx.x < 10) partial[(size_t)blockIdx.x * 10 + threadIdx.x] = 0.0;
-        return;
-    }
-    const int gi = blockIdx.x * (256 / G) + threadIdx.x / G;
+    // centres chunk * 256 / G .. of the class list (ncls of them)
+    const int gi = chunk * (256 / G) + threadIdx.x / G;
     const bool cv = gi < ncls;
     const int c = cv ? clist[gi] : 0;
     const int nb = cv ? bcnt[c] : 0;
@@ -1534,9 +1528,7 @@ k_rb_centre(const int *__restrict__ clist, const int *__restrict__ ccount, const
     }
     const double S1 = group_sum<G>(dfdn), S2c = group_sum<G>(pc), S2h = group_sum<G>(ph);
     const double S3 = group_sum<G>(dfdncn);
-    double fo[3] = {0, 0, 0}, fcen[3] = {0, 0, 0}, w[9];
-#pragma unroll
-    for (int q = 0; q < 9; ++q) w[q] = 0.0;
+    double fo[3] = {0, 0, 0}, fcen[3] = {0, 0, 0};
     // coordination / conjugation / P on this lane's neighbour x
     if (has) {
         const bool xC = kx == RB_C;
@@ -1573,9 +1565,17 @@ k_rb_centre(const int *__restrict__ clist, const int *__restrict__ ccount, const
     double fkx[3] = {0, 0, 0};
     for (int k = 0; k < xmax; ++k) {
         double dbk3[3] = {0, 0, 0};
+        // bond k of the centre from the lane of the group that holds it
+        // (cross-lane moves instead of a global load per step)
+        RbBondT<R> K{};
+#pragma unroll
+        for (int x = 0; x < 3; ++x) K.n[x] = __shfl(B.n[x], k, G);
+        K.len = __shfl(B.len, k, G);
+        K.fc = __shfl(B.fc, k, G);
+        K.dfc = __shfl(B.dfc, k, G);
+        const int kxk = __shfl(kx, k, G);
         if (sv && k < nb && k != l) {
-            const RbBondT<R> K = bond[beg + k];
-            const int ikpot = rb_pair(kc, el[bij[beg + k].x]);
+            const int ikpot = rb_pair(kc, kxk);
             const R rlik = K.len;
             if (rlik < D.cut_h[ikpot]) {
                 R qf, qd, gf, gd, dgdn;
@@ -1636,7 +1636,6 @@ k_rb_centre(const int *__restrict__ clist, const int *__restrict__ ccount, const
     for (int x = 0; x < 3; ++x) fcen[x] = group_sum<G>(fcen[x]);
     if (cv && l == 0)
         for (int x = 0; x < 3; ++x) own[3 * (size_t)c + x] = fcen[x];
-    rb_block_record(0.0, w, partial, blockIdx.x);
 }
 
 // The second-neighbour conjugation forces: a group of G lanes per atom k,
@@ -1645,19 +1644,13 @@ k_rb_centre(const int *__restrict__ clist, const int *__restrict__ ccount, const
 // centre c = m itself excluded, bop_kernel_rebo2.f90:2405-2520), its
 // opposite on k.
 template <typename R, int G>
-__global__ void __launch_bounds__(256)
-k_rb_conj_force(const int *__restrict__ clist, const int *__restrict__ ccount, const int *__restrict__ offsets,
-                const int *__restrict__ bcnt, const RbBondT<R> *__restrict__ bond, const int *__restrict__ rev,
-                const double *__restrict__ bconj, double *__restrict__ fcj, double *__restrict__ own2,
-                double *__restrict__ partial)
+__device__ __forceinline__ void
+rb_conj_force_chunk(int chunk, const int *__restrict__ clist, int ncls, const int *__restrict__ offsets,
+                    const int *__restrict__ bcnt, const RbBondT<R> *__restrict__ bond, const int *__restrict__ rev,
+                    const double *__restrict__ bconj, double *__restrict__ fcj, double *__restrict__ own2, double *w)
 {
     const int l = threadIdx.x % G;
-    const int ncls = *ccount;
-    if ((int)blockIdx.x * (256 / G) >= ncls) {
-        if (threadIdx.x < 10) partial[(size_t)blockIdx.x * 10 + threadIdx.x] = 0.0;
-        return;
-    }
-    const int gi = blockIdx.x * (256 / G) + threadIdx.x / G;
+    const int gi = chunk * (256 / G) + threadIdx.x / G;
     const bool cv = gi < ncls;
     const int k = cv ? clist[gi] : 0;
     const int nb = cv ? bcnt[k] : 0;
@@ -1665,9 +1658,7 @@ k_rb_conj_force(const int *__restrict__ clist, const int *__restrict__ ccount, c
     const int b = (cv ? offsets[k] : 0) + l;
     const double Bv = has ? bconj[rev[b]] : 0.0;
     const double Bs = group_sum<G>(Bv);
-    double w[9], fk[3] = {0, 0, 0};
-#pragma unroll
-    for (int q = 0; q < 9; ++q) w[q] = 0.0;
+    double fk[3] = {0, 0, 0};
     if (has) {
         const RbBondT<R> M = bond[b];
         const double a = Bs - Bv;
@@ -1687,6 +1678,74 @@ k_rb_conj_force(const int *__restrict__ clist, const int *__restrict__ ccount, c
     for (int x = 0; x < 3; ++x) fk[x] = group_sum<G>(fk[x]);
     if (cv && l == 0)
         for (int x = 0; x < 3; ++x) own2[3 * (size_t)k + x] = fk[x];
+}
+
+// The per-centre passes of all three width classes in one launch: block
+// chunks of 16 centres of 9-16 bonds, then of 32 centres of 5-8 bonds,
+// then of 64 centres of up to 4 bonds (the class lists of
+// k_rb_class_list), dealt to the blocks in turn, so the few long chunks of
+// the wide centres run beside the others.  A block's virial sums its chunks
+// in order, so the records do not depend on scheduling.  k_rb_conj_forces:
+// the second-neighbour conjugation forces (needs every bond scalar of the
+// centre pass).
+__device__ __forceinline__ void rb_chunks(const int *__restrict__ ccount, int &c4, int &c8, int &c16)
+{
+    c4 = (ccount[0] + 63) / 64;
+    c8 = (ccount[1] + 31) / 32;
+    c16 = (ccount[2] + 15) / 16;
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256)
+k_rb_centres(int nat, const int *__restrict__ clist, const int *__restrict__ ccount,
+             const RbDevT<R> *__restrict__ Dp, const int *__restrict__ el, const int *__restrict__ offsets,
+             const int *__restrict__ bcnt, const RbBondT<R> *__restrict__ bond, const int2 *__restrict__ bij,
+             const double2 *__restrict__ nn, const Rb4<double> *__restrict__ bo, const double4 *__restrict__ coef,
+             double *__restrict__ fnb, double *__restrict__ own, double *__restrict__ bconj,
+             double *__restrict__ partial)
+{
+    int c4, c8, c16;
+    rb_chunks(ccount, c4, c8, c16);
+    double w[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) w[q] = 0.0;
+    for (int ch = blockIdx.x; ch < c4 + c8 + c16; ch += gridDim.x) {
+        if (ch < c16)
+            rb_centre_chunk<R, 16>(ch, clist + 2 * (size_t)nat, ccount[2], *Dp, el, offsets, bcnt, bond, bij, nn,
+                                   bo, coef, fnb, own, bconj, w);
+        else if (ch < c16 + c8)
+            rb_centre_chunk<R, 8>(ch - c16, clist + nat, ccount[1], *Dp, el, offsets, bcnt, bond, bij, nn, bo,
+                                  coef, fnb, own, bconj, w);
+        else
+            rb_centre_chunk<R, 4>(ch - c16 - c8, clist, ccount[0], *Dp, el, offsets, bcnt, bond, bij, nn, bo, coef,
+                                  fnb, own, bconj, w);
+    }
+    rb_block_record(0.0, w, partial, blockIdx.x);
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256)
+k_rb_conj_forces(int nat, const int *__restrict__ clist, const int *__restrict__ ccount,
+                 const int *__restrict__ offsets, const int *__restrict__ bcnt, const RbBondT<R> *__restrict__ bond,
+                 const int *__restrict__ rev, const double *__restrict__ bconj, double *__restrict__ fcj,
+                 double *__restrict__ own2, double *__restrict__ partial)
+{
+    int c4, c8, c16;
+    rb_chunks(ccount, c4, c8, c16);
+    double w[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) w[q] = 0.0;
+    for (int ch = blockIdx.x; ch < c4 + c8 + c16; ch += gridDim.x) {
+        if (ch < c16)
+            rb_conj_force_chunk<R, 16>(ch, clist + 2 * (size_t)nat, ccount[2], offsets, bcnt, bond, rev, bconj, fcj,
+                                       own2, w);
+        else if (ch < c16 + c8)
+            rb_conj_force_chunk<R, 8>(ch - c16, clist + nat, ccount[1], offsets, bcnt, bond, rev, bconj, fcj, own2,
+                                      w);
+        else
+            rb_conj_force_chunk<R, 4>(ch - c16 - c8, clist, ccount[0], offsets, bcnt, bond, rev, bconj, fcj, own2,
+                                      w);
+    }
     rb_block_record(0.0, w, partial, blockIdx.x);
 }
 
@@ -1867,8 +1926,15 @@ static int rebo2_kernels(Rebo2State *st, agpu_potential *pot, const agpu_eval_ar
     int nblk = (ntot + 255) / 256;
     if (nblk < 1) nblk = 1;
     *nblk_out = nblk;
-    const int nb4 = (nat + 63) / 64, nb8 = mr > 4 ? (nat + 31) / 32 : 0, nb16 = mr > 8 ? (nat + 15) / 16 : 0;
-    const int nbc = nb4 + nb8 + nb16;
+    // per-centre pass grid: the chunks of the class sizes seen on the
+    // previous call (+25 %), else of every atom in each class the longest
+    // row allows, at most 2048 blocks striding over the chunks
+    int nbw[3] = {(nat + 63) / 64, mr > 4 ? (nat + 31) / 32 : 0, mr > 8 ? (nat + 15) / 16 : 0};
+    for (int w = 0; w < 3; ++w) {
+        const int cpb = 64 >> w;
+        if (st->chint[w] >= 0) nbw[w] = std::min(nbw[w], (st->chint[w] + st->chint[w] / 4 + cpb - 1) / cpb + 8);
+    }
+    const int nbc = std::min(nbw[0] + nbw[1] + nbw[2], 2048);
     // block records: pair, torsion, then the neighbour pass (2 nblk, or
     // centre and conjugation passes)
     // torsion groups: 64 per block, the grid striding over the pairs
@@ -1949,31 +2015,16 @@ static int rebo2_kernels(Rebo2State *st, agpu_potential *pot, const agpu_eval_ar
         AGPU_CHECK(st->own.reserve(sizeof(double) * 6 * (nat + 1)));
         double *own = st->own.as<double>(), *own2 = own + 3 * (size_t)nat;
         double *pc = part + 10 * ((size_t)nblk + ntb), *pj = pc + 10 * (size_t)nbc;
-        const int nbw[3] = {nb4, nb8, nb16};
-        auto cen = [&](int w) {
-            return w == 0 ? k_rb_centre<R, 4> : (w == 1 ? k_rb_centre<R, 8> : k_rb_centre<R, 16>);
-        };
-        auto cjf = [&](int w) {
-            return w == 0 ? k_rb_conj_force<R, 4> : (w == 1 ? k_rb_conj_force<R, 8> : k_rb_conj_force<R, 16>);
-        };
-        size_t off = 0;
-        for (int w = 0; w < 3; ++w) {
-            if (nbw[w] == 0) continue;
-            AGPU_LAUNCH(f64 ? "rebo2_centre" : "rebo2_centre_mixed", cen(w), dim3(nbw[w]), dim3(256), 0, s,
-                        clist + (size_t)w * nat, ccount + w, D, els, offs, st->bcnt.as<int>(), bondR,
-                        st->bij.as<int2>(), st->nn.as<double2>(), bo, coef, st->fnb.as<double>(), own,
-                        st->bconj.as<double>(), pc + 10 * off);
-            off += nbw[w];
-        }
-        // the bond scalars of every width are in place before any conjugation force
-        off = 0;
-        for (int w = 0; w < 3; ++w) {
-            if (nbw[w] == 0) continue;
-            AGPU_LAUNCH(f64 ? "rebo2_conj_force" : "rebo2_conj_force_mixed", cjf(w), dim3(nbw[w]), dim3(256), 0, s,
-                        clist + (size_t)w * nat, ccount + w, offs, st->bcnt.as<int>(), bondR, st->rev.as<int>(),
-                        st->bconj.as<double>(), st->fcj.as<double>(), own2, pj + 10 * off);
-            off += nbw[w];
-        }
+        AGPU_LAUNCH(f64 ? "rebo2_centre" : "rebo2_centre_mixed", k_rb_centres<R>, dim3(nbc), dim3(256), 0, s, nat,
+                    clist, ccount, D, els, offs, st->bcnt.as<int>(), bondR, st->bij.as<int2>(), st->nn.as<double2>(),
+                    bo, coef, st->fnb.as<double>(), own, st->bconj.as<double>(), pc);
+        // the bond scalars of every centre are in place before any conjugation force
+        AGPU_LAUNCH(f64 ? "rebo2_conj_force" : "rebo2_conj_force_mixed", k_rb_conj_forces<R>, dim3(nbc), dim3(256), 0,
+                    s, nat, clist, ccount, offs, st->bcnt.as<int>(), bondR, st->rev.as<int>(),
+                    st->bconj.as<double>(), st->fcj.as<double>(), own2, pj);
+        // the class sizes, read back with the flags: the next call's grids
+        AGPU_HIP(hipMemcpyAsync(pot->stats.h + 8, ccount, sizeof(int) * 3, hipMemcpyDeviceToHost, s));
+        st->chint_pending = true;
         AGPU_LAUNCH("rebo2_gather", k_rb_gather, dim3(nb), dim3(256), 0, s, nat, order, offs, st->bcnt.as<int>(),
                     st->rev.as<int>(), own, own2, st->fnb.as<double>(), st->fcj.as<double>(), fpr,
                     st->facc.as<unsigned long long>(), eacc, a->f, a->epot_per_at);
@@ -2063,6 +2114,10 @@ int rebo2_compute(agpu_potential *pot, const agpu_eval_args *a)
     AGPU_CHECK(reduce_ev(st->partial.as<double>(), nblk, 10, a->epot, a->wpot, nullptr));
     AGPU_HIP(hipMemcpyAsync(pot->stats.h, dstats, sizeof(int), hipMemcpyDeviceToHost, s));
     AGPU_HIP(hipStreamSynchronize(s));
+    if (st->chint_pending) {
+        for (int w = 0; w < 3; ++w) st->chint[w] = pot->stats.h[8 + w];
+        st->chint_pending = false;
+    }
     if (pot->stats.h[0] & DERR_FIXED_RANGE)
         return set_error(AGPU_ERR_DOMAIN, "REBO2: a force contribution exceeds the fixed-point range (2^21 eV/A)");
     if (pot->stats.h[0] & DERR_BOND_OVERFLOW)
