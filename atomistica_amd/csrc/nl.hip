@@ -18,16 +18,16 @@
 //   4. nl_pairs:   the accepted (j, image) pairs of every atom over its
 //                  stencil (x-major, then y, then z; atoms of a bin ascending)
 //                  into a padded staging row of stride S per bin-order slot,
-//                  neighbours named by slot, and the row counts (by caller
-//                  atom and by slot).  k_bin_pairs (a wavefront per block of
-//                  home bins, candidates shared by the wavefront) when bins
-//                  are dense, k_atom_pairs (a thread per home atom) when they
-//                  hold < 4 atoms.  The force kernels read these rows and the
-//                  slot records directly (bin-order storage: a neighbour's
+//                  neighbours named by slot, and the row counts by slot.
+//                  k_bin_pairs (a wavefront per block of home bins,
+//                  candidates shared by the wavefront) when bins are dense,
+//                  k_atom_pairs (a thread per home atom) when they hold < 4
+//                  atoms.  The force kernels read these rows and the slot
+//                  records directly (bin-order storage: a neighbour's
 //                  position is a slot record near the home atom's own).
-//   5. exclusive scan of the counts -> caller row offsets; k_nl_compact
-//                  copies the staging rows into the caller-order CSR list
-//                  (slots turned back into caller indices) when asked.
+//   5. exclusive scan of the counts -> slot row offsets; the caller-order
+//                  offsets and k_nl_compact's caller-order CSR list (slots
+//                  turned back into caller indices) only when asked.
 // The distance test is the reference's FP64 expression without contraction,
 // so the accepted set is bit-identical.
 #include <algorithm>
@@ -132,7 +132,8 @@ __global__ void k_cell_fill(int nat, int ncell, const int *__restrict__ key, con
 }
 
 // the bin-sorted record of atom j at slot p (and the optional FP32 wrapped
-// position r + A.w, with the wrap flag (w != 0) in bit 31 of .w)
+// position q = r + A.w with max(|q_x|, |q_y|, |q_z|) in .w, its sign bit
+// the wrap flag (w != 0))
 __device__ __forceinline__ void gather_atom(int p, int j, const double *__restrict__ r, const BinGeom &g,
                                             const CellArg &C, SortedAtom *__restrict__ out,
                                             float4 *__restrict__ pos32)
@@ -151,7 +152,8 @@ __device__ __forceinline__ void gather_atom(int p, int j, const double *__restri
         float q[3];
         for (int x = 0; x < 3; ++x) q[x] = (float)(rj[x] + image_x(C, x, w[0], w[1], w[2]));
         const int flag = (w[0] | w[1] | w[2]) != 0 ? (int)0x80000000u : 0;
-        pos32[p] = make_float4(q[0], q[1], q[2], __int_as_float(j | flag));
+        const float qm = fmaxf(fmaxf(fabsf(q[0]), fabsf(q[1])), fabsf(q[2]));
+        pos32[p] = make_float4(q[0], q[1], q[2], __int_as_float(__float_as_int(qm) | flag));
     }
 }
 
@@ -310,7 +312,7 @@ __device__ __forceinline__ void row_stats_wave(int longest, int last, int2 *__re
 template <bool ORTHO, bool MULTI>
 __global__ void __launch_bounds__(256)
 k_bin_pairs(int B, BinGeom g, CellArg C, double cutoff_sq, const SortedAtom *__restrict__ atoms,
-            const int *__restrict__ cell_start, int *__restrict__ counts, int *__restrict__ scnt,
+            const int *__restrict__ cell_start, int *__restrict__ scnt,
             int2 *__restrict__ stage, int S, int *__restrict__ flags, int2 *__restrict__ part)
 {
     __shared__ NlShared sh[NL_WAVES];
@@ -535,7 +537,6 @@ k_bin_pairs(int B, BinGeom g, CellArg C, double cutoff_sq, const SortedAtom *__r
         __builtin_amdgcn_wave_barrier();
         if (lane < ma) {
             const int c = L.hcnt[lane];
-            counts[L.hi[lane]] = c;
             scnt[L.hp[lane]] = c;
             wmax = max(wmax, c);
             if (c > 0) wlast = max(wlast, L.hi[lane] + 1);
@@ -577,8 +578,7 @@ k_bin_pairs(int B, BinGeom g, CellArg C, double cutoff_sq, const SortedAtom *__r
 template <bool ORTHO>
 __global__ void __launch_bounds__(256)
 k_atom_pairs(int nat, BinGeom g, CellArg C, double cutoff_sq, const SortedAtom *__restrict__ atoms,
-             const float4 *__restrict__ pos32, const int *__restrict__ cell_start, int *__restrict__ counts,
-             int *__restrict__ scnt, int2 *__restrict__ stage, int S, int *__restrict__ flags,
+             const float4 *__restrict__ pos32, const int *__restrict__ cell_start, int *__restrict__ scnt, int2 *__restrict__ stage, int S, int *__restrict__ flags,
              int2 *__restrict__ part)
 {
     const int p = xcd_block() * blockDim.x + threadIdx.x;
@@ -586,23 +586,32 @@ k_atom_pairs(int nat, BinGeom g, CellArg C, double cutoff_sq, const SortedAtom *
     int longest = 0, last = 0;
     if (p < nat) {
         const float4 H = pos32[p];
-        int w0, w1, w2, hc[3];
+        int w0, w1, w2, hc[3], ih;
         {
             const SortedAtom A = atoms[p];
             const double ri[3] = {A.x, A.y, A.z};
             int hw[3];
             bin_of(g, ri, hc, hw);
             dc_unpack(A.w, w0, w1, w2);
+            ih = A.j;
         }
         const float rc = (float)sqrt(cutoff_sq);
-        const float hm = fmaxf(fmaxf(fabsf(H.x), fabsf(H.y)), fabsf(H.z));
+        const float hm = fabsf(H.w);
         const size_t srow = (size_t)p * S;
         int cnt = 0;
         // the candidates q0 .. q1 - 1 (atoms of consecutive bins of one
         // shift u), in order
         auto scan = [&](int q0, int q1, int ux, int uy, int uz) {
             double au[3];
-            for (int x = 0; x < 3; ++x) au[x] = image_x(C, x, ux, uy, uz);
+            if (ORTHO) {
+                // off-diagonal cell entries are zero: A.u is the diagonal
+                // products, bit for bit
+                au[0] = C.A[0] * (double)ux;
+                au[1] = C.A[4] * (double)uy;
+                au[2] = C.A[8] * (double)uz;
+            } else {
+                for (int x = 0; x < 3; ++x) au[x] = image_x(C, x, ux, uy, uz);
+            }
             const float hx = H.x + (float)au[0], hy = H.y + (float)au[1], hz = H.z + (float)au[2];
             const float e0 = 5.9604645e-8f * (4.0f * (float)(fabs(au[0]) + fabs(au[1]) + fabs(au[2])) + 8.0f * rc);
             const int s0u = w0 + ux, s1u = w1 + uy, s2u = w2 + uz;
@@ -610,7 +619,7 @@ k_atom_pairs(int nat, BinGeom g, CellArg C, double cutoff_sq, const SortedAtom *
                 const float4 B = pos32[q];
                 const float dx = hx - B.x, dy = hy - B.y, dz = hz - B.z;
                 const float d2f = dx * dx + dy * dy + dz * dz;
-                const float bm = fmaxf(fmaxf(fabsf(B.x), fabsf(B.y)), fabsf(B.z));
+                const float bm = fabsf(B.w);
                 const float E2 = 2.0f * (e0 + 4.76837158e-7f * fmaxf(hm, bm));
                 const float th = rc + E2, tl = rc - E2;
                 const bool exact = !(E2 < 2e-3f * rc);
@@ -641,48 +650,42 @@ k_atom_pairs(int nat, BinGeom g, CellArg C, double cutoff_sq, const SortedAtom *
         if (runs && g.nd[0] == 1 && g.nd[1] == 1 && ndz == 1) {
             // a 3 x 3 x 3 stencil (bins of at least the cut-off): the nine
             // column ranges are read first, all loads in flight together,
-            // then scanned in stencil order
+            // then scanned in stencil order.  Column k = 3 (ox + 1) + (oy + 1)
+            // lies one bin at most outside the cell, so its wrap is a compare
             const int m0 = max(zlo, 1), m1 = min(zhi, nz);
-            // column k = 3 (ox + 1) + (oy + 1): its first bin (-1 outside an
-            // open cell) and image shifts
-            auto column = [&](int k, int &ux, int &uy) {
-                int cx = hc[0] + k / 3 - 1, cy = hc[1] + k % 3 - 1;
-                ux = 0;
-                uy = 0;
-                if (g.pbc[0]) {
-                    const int t = floordiv(cx - 1, g.nc[0]);
-                    cx -= t * g.nc[0];
-                    ux = -t;
-                } else if (cx < 1 || cx > g.nc[0]) {
-                    return -1;
-                }
-                if (g.pbc[1]) {
-                    const int t = floordiv(cy - 1, g.nc[1]);
-                    cy -= t * g.nc[1];
-                    uy = -t;
-                } else if (cy < 1 || cy > g.nc[1]) {
-                    return -1;
-                }
-                return ((cx - 1) * g.nc[1] + (cy - 1)) * nz;
-            };
-            int q0[9], q1[9];
+            int bcs[9], uxs[9], uys[9], q0[9], q1[9];
 #pragma unroll
             for (int k = 0; k < 9; ++k) {
-                int ux, uy;
-                const int bc = column(k, ux, uy);
+                int cx = hc[0] + k / 3 - 1, cy = hc[1] + k % 3 - 1, ux = 0, uy = 0;
+                bool ok = true;
+                if (g.pbc[0]) {
+                    if (cx < 1) { cx += g.nc[0]; ux = 1; }
+                    else if (cx > g.nc[0]) { cx -= g.nc[0]; ux = -1; }
+                } else {
+                    ok = cx >= 1 && cx <= g.nc[0];
+                }
+                if (g.pbc[1]) {
+                    if (cy < 1) { cy += g.nc[1]; uy = 1; }
+                    else if (cy > g.nc[1]) { cy -= g.nc[1]; uy = -1; }
+                } else {
+                    ok = ok && cy >= 1 && cy <= g.nc[1];
+                }
+                const int bc = ok ? ((cx - 1) * g.nc[1] + (cy - 1)) * nz : -1;
+                bcs[k] = bc;
+                uxs[k] = ux;
+                uys[k] = uy;
                 q0[k] = bc >= 0 ? cell_start[bc + m0 - 1] : 0;
                 q1[k] = bc >= 0 ? cell_start[bc + m1] : 0;
             }
             const bool tw = g.pbc[2] && zlo < 1, uw = g.pbc[2] && zhi > nz;
 #pragma unroll
             for (int k = 0; k < 9; ++k) {
-                int ux, uy;
-                const int bc = column(k, ux, uy);
+                const int bc = bcs[k];
                 if (bc < 0) continue;
                 // wrapped from the top (u_z = +1), unwrapped, wrapped from the bottom (u_z = -1)
-                if (tw) scan(cell_start[bc + zlo + nz - 1], cell_start[bc + nz], ux, uy, 1);
-                scan(q0[k], q1[k], ux, uy, 0);
-                if (uw) scan(cell_start[bc], cell_start[bc + zhi - nz], ux, uy, -1);
+                if (tw) scan(cell_start[bc + zlo + nz - 1], cell_start[bc + nz], uxs[k], uys[k], 1);
+                scan(q0[k], q1[k], uxs[k], uys[k], 0);
+                if (uw) scan(cell_start[bc], cell_start[bc + zhi - nz], uxs[k], uys[k], -1);
             }
         } else
         for (int ox = -g.nd[0]; ox <= g.nd[0]; ++ox) {
@@ -729,8 +732,6 @@ k_atom_pairs(int nat, BinGeom g, CellArg C, double cutoff_sq, const SortedAtom *
                 }
             }
         }
-        const int ih = __float_as_int(H.w) & 0x7fffffff;
-        if (ih < nat) counts[ih] = cnt;
         scnt[p] = cnt;
         over = cnt > S;
         longest = cnt;
@@ -833,6 +834,14 @@ NlRows nl_rows(const agpu_neighbors *nl)
                   nl->atoms_sorted.as<SortedAtom>()};
 }
 
+// row lengths by caller atom (caller atom order[p] at slot p)
+__global__ void k_counts_caller(int nat, const int *__restrict__ order, const int *__restrict__ scnt,
+                                int *__restrict__ counts)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < nat) counts[order[p]] = scnt[p];
+}
+
 // offsets[nat] = offsets[nat - 1] + counts[nat - 1] after the exclusive
 // scan of nat entries
 __global__ void k_offsets_end(int nat, const int *__restrict__ counts, int *__restrict__ offsets)
@@ -846,6 +855,8 @@ int nl_caller_offsets(agpu_neighbors *nl)
     const int nat = nl->nat;
     hipStream_t s = stream();
     if (nat > 0) {
+        AGPU_LAUNCH("nl_counts", k_counts_caller, dim3((nat + 255) / 256), dim3(256), 0, s, nat,
+                    nl->idx_sorted.as<int>(), nl->scnt.as<int>(), nl->counts.as<int>());
         size_t tmp = 0;
         AGPU_HIP(rocprim::exclusive_scan(nullptr, tmp, nl->counts.as<int>(), nl->offsets.as<int>(), 0, (size_t)nat,
                                          rocprim::plus<int>(), s));
@@ -1108,17 +1119,17 @@ int nl_update(agpu_neighbors *nl, agpu_particles *p, int force, bool csr)
             if (ortho)
                 AGPU_LAUNCH("nl_pairs", k_atom_pairs<true>, dim3(na), dim3(TB), 0, s, nat, g, C, cutoff_sq,
                             nl->atoms_sorted.as<SortedAtom>(), nl->pos32.as<float4>(), nl->cell_start.as<int>(),
-                            nl->counts.as<int>(), nl->scnt.as<int>(), nl->stage.as<int2>(), S, flags + 3, part);
+                            nl->scnt.as<int>(), nl->stage.as<int2>(), S, flags + 3, part);
             else
                 AGPU_LAUNCH("nl_pairs", k_atom_pairs<false>, dim3(na), dim3(TB), 0, s, nat, g, C, cutoff_sq,
                             nl->atoms_sorted.as<SortedAtom>(), nl->pos32.as<float4>(), nl->cell_start.as<int>(),
-                            nl->counts.as<int>(), nl->scnt.as<int>(), nl->stage.as<int2>(), S, flags + 3, part);
+                            nl->scnt.as<int>(), nl->stage.as<int2>(), S, flags + 3, part);
         } else {
             auto kern = ortho ? (B > 1 ? k_bin_pairs<true, true> : k_bin_pairs<true, false>)
                               : (B > 1 ? k_bin_pairs<false, true> : k_bin_pairs<false, false>);
             AGPU_LAUNCH("nl_pairs", kern, dim3(nbin_blk), dim3(TB), 0, s, B, g, C, cutoff_sq,
-                        nl->atoms_sorted.as<SortedAtom>(), nl->cell_start.as<int>(), nl->counts.as<int>(),
-                        nl->scnt.as<int>(), nl->stage.as<int2>(), S, flags + 3, part);
+                        nl->atoms_sorted.as<SortedAtom>(), nl->cell_start.as<int>(), nl->scnt.as<int>(),
+                        nl->stage.as<int2>(), S, flags + 3, part);
         }
         // row offsets in slot order (the per-entry arrays of the force
         // kernels); the caller-order offsets follow when a caller asks
