@@ -112,7 +112,8 @@ struct agpu_neighbors {
     // device storage
     agpu::DevBuf key, key_sorted, idx, idx_sorted, cell_start, atoms_sorted,
         counts, offsets, nbr, sort_tmp, rowmax, stage, pos32, part, scnt, soffs;
-    int stage_w = 0;              // staging stride (entries per atom)
+    int stage_w = 0;              // staging stride of the next build (entries per atom)
+    int stage_s = 0;              // staging stride of the current rows
     // particles' pos_rev when the slot positions (atoms_sorted) were written
     uint64_t slot_rev = ~0ull;
     // offsets holds the caller-order row offsets of the last build
@@ -129,7 +130,7 @@ struct agpu_neighbors {
     // atom, 2 wavefront per bin block (both give the same rows)
     int pair_mode = 0;
     // the CSR rows (offsets, nbr: caller indices) hold the last build; false
-    // while its rows are only staged (slot p's row at stage + p*stage_w,
+    // while its rows are only staged (slot p's row at stage + p*stage_s,
     // scnt[p] entries naming slots) because every caller since the build
     // read them there
     bool csr = true;

@@ -790,7 +790,7 @@ static int nl_compact(agpu_neighbors *nl)
     if (nl->nat > 0)
         AGPU_LAUNCH("nl_compact", k_nl_compact, dim3((nl->nat + 15) / 16), dim3(256), 0, stream(), nl->nat,
                     nl->atoms_sorted.as<SortedAtom>(), nl->scnt.as<int>(), nl->offsets.as<int>(),
-                    nl->stage.as<int2>(), nl->stage_w, nl->nbr.as<int2>(), nl->rowmax.as<int>() + 3);
+                    nl->stage.as<int2>(), nl->stage_s, nl->nbr.as<int2>(), nl->rowmax.as<int>() + 3);
     nl->csr = true;
     return AGPU_OK;
 }
@@ -830,7 +830,7 @@ NlRows nl_rows(const agpu_neighbors *nl)
 {
     if (nl->external)
         return NlRows{nl->nbr.as<int2>(), nullptr, nl->offsets.as<int>(), 0, nl->atoms_sorted.as<SortedAtom>()};
-    return NlRows{nl->stage.as<int2>(), nl->scnt.as<int>(), nl->soffs.as<int>(), nl->stage_w,
+    return NlRows{nl->stage.as<int2>(), nl->scnt.as<int>(), nl->soffs.as<int>(), nl->stage_s,
                   nl->atoms_sorted.as<SortedAtom>()};
 }
 
@@ -1164,6 +1164,15 @@ int nl_update(agpu_neighbors *nl, agpu_particles *p, int force, bool csr)
         return nl_update(nl, p, 2, csr);
     }
     nl->total = (int64_t)nl->stats.h[1] | ((int64_t)nl->stats.h[2] << 31);
+    nl->stage_s = nl->stage_w;
+    // the staging stride follows the rows: the next build narrows it when
+    // the longest row leaves a quarter of it unused (a quarter of the row,
+    // at least 2 entries, kept as headroom; a row that outgrows it later
+    // widens it again through the overflow rebuild)
+    {
+        const int want = (nl->maxrow + std::max(2, nl->maxrow / 4) + 3) & ~3;
+        if (want >= 4 && 4 * want <= 3 * nl->stage_w) nl->stage_w = want;
+    }
     nl->csr = nat == 0;
     nl->caller_off = nat == 0;
     nl->p = p;
