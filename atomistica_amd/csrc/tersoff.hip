@@ -2149,7 +2149,7 @@ static int bop_rows_run(agpu_potential *pot, const agpu_eval_args *a, const RowB
         AGPU_CHECK(dispatch_rows<double>(G, wb, st, B, p, nl, el, a, eslot, erev, wslot, wrev, nblk, s));
     AGPU_LAUNCH("bop_gather", k_bop_rows_gather, dim3((nat + 255) / 256), dim3(256), 0, s, nat, order,
                 nl_rows(nl), B.frev, B.fown, eslot, erev, wslot, wrev, a->f, a->epot_per_at, a->wpot_per_at);
-    AGPU_CHECK(reduce_ev(B.partial, nblk, 10, a->epot, a->wpot, nullptr, false, B.red));
+    AGPU_CHECK(reduce_ev(B.partial, nblk, 10, a->epot, a->wpot, nullptr, true, B.red));
     if (capturing()) return AGPU_OK;     // the MD driver reads the flags after the replay
     AGPU_HIP(hipMemcpyAsync(h_flags, B.stats, sizeof(int) * 4, hipMemcpyDeviceToHost, s));
     AGPU_HIP(hipStreamSynchronize(s));
@@ -2235,13 +2235,14 @@ int tersoff_compute(agpu_potential *pot, const agpu_eval_args *a)
     AGPU_CHECK(nl_update(nl, p, 0, false));
     const int nat = p->nat;
     hipStream_t s = stream();
+    if (nat == 0) return zero_dev(a->epot, sizeof(double));
+    // the row path sets epot from its block records; the others add to it
+    if (nl->maxrow <= 64 && !st->compact && !st->scr) return bop_rows_compute(pot, a);
     AGPU_CHECK(zero_dev(a->epot, sizeof(double)));
-    if (nat == 0) return AGPU_OK;
     if (st->scr) {
         AGPU_CHECK(nl_update(nl, p, 0));
         return tersoff_scr_compute(pot, a);
     }
-    if (nl->maxrow <= 64 && !st->compact) return bop_rows_compute(pot, a);
     // longer rows: compact them to their bonds (at most 64) over CSR rows
     AGPU_CHECK(nl_update(nl, p, 0));
     const int64_t total = nl->total;
