@@ -301,7 +301,7 @@ k_ewald_kforce(int nat, int nitems, const double *__restrict__ s4, const double 
                double *__restrict__ f, double *__restrict__ phi)
 {
     __shared__ int sh_i[EW_ICHUNK][5];
-    __shared__ double sh_t[EW_ICHUNK * EW_L][2];
+    __shared__ __align__(16) double sh_t[EW_ICHUNK * EW_L][2];
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     double s1 = 0, s2 = 0, s3 = 0, ec = 1, es = 0;
     if (j < nat) {
@@ -335,17 +335,24 @@ k_ewald_kforce(int nat, int nitems, const double *__restrict__ s4, const double 
             const double ph = frac_turn((double)n1 * s1 + (double)n2 * s2) + frac_turn((double)n3s * s3);
             double c, s;
             sincospi(2.0 * frac_turn(ph), &s, &c);
-            double sim = 0.0, s3im = 0.0;
-            for (int l = 0; l < cnt; ++l) {
-                const double tc = sh_t[ko + l][0], ts = sh_t[ko + l][1];
-                const double im = s * tc - c * ts;
-                ap += c * tc + s * ts;
-                sim += im;
-                s3im += (double)(n3s + l) * im;
-                const double c2 = c * ec - s * es;
-                s = c * es + s * ec;
-                c = c2;
+            // the run's terms, unrolled over the run length (EW_L, shorter
+            // runs predicated): sim = sum im, lim = sum l im, so that the
+            // n3 moment sum_l (n3s + l) im is n3s sim + lim
+            double sim = 0.0, lim = 0.0;
+#pragma unroll
+            for (int l = 0; l < EW_L; ++l) {
+                if (l < cnt) {
+                    const double2 tv = *reinterpret_cast<const double2 *>(sh_t[ko + l]);
+                    const double im = s * tv.x - c * tv.y;
+                    ap += c * tv.x + s * tv.y;
+                    sim += im;
+                    lim += (double)l * im;
+                    const double c2 = c * ec - s * es;
+                    s = c * es + s * ec;
+                    c = c2;
+                }
             }
+            const double s3im = (double)n3s * sim + lim;
             a1 += (double)n1 * sim;
             a2 += (double)n2 * sim;
             a3 += s3im;

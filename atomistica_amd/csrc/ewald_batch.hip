@@ -295,7 +295,7 @@ k_ewb_kforce(const EwbTile *__restrict__ itiles, const int *__restrict__ rfirst,
              double *__restrict__ phi)
 {
     __shared__ int sh_i[EWB_RCHUNK][5];
-    __shared__ double sh_t[EWB_RCHUNK * EWB_L][2];
+    __shared__ __align__(16) double sh_t[EWB_RCHUNK * EWB_L][2];
     const EwbTile Ti = itiles[blockIdx.x];
     const int lane = threadIdx.x;
     const int j = Ti.b + lane;
@@ -335,17 +335,23 @@ k_ewb_kforce(const EwbTile *__restrict__ itiles, const int *__restrict__ rfirst,
             const double ph = ewb_turn((double)n1 * s1 + (double)n2 * s2) + ewb_turn((double)n3s * s3);
             double c, s;
             sincospi(2.0 * ewb_turn(ph), &s, &c);
-            double sim = 0.0, s3im = 0.0;
-            for (int l = 0; l < cnt; ++l) {
-                const double tc = sh_t[ko + l][0], ts = sh_t[ko + l][1];
-                const double im = s * tc - c * ts;
-                ap += c * tc + s * ts;
-                sim += im;
-                s3im += (double)(n3s + l) * im;
-                const double c2 = c * ec - s * es;
-                s = c * es + s * ec;
-                c = c2;
+            // unrolled over the run length (shorter runs predicated):
+            // sum_l (n3s + l) im = n3s sim + lim
+            double sim = 0.0, lim = 0.0;
+#pragma unroll
+            for (int l = 0; l < EWB_L; ++l) {
+                if (l < cnt) {
+                    const double2 tv = *reinterpret_cast<const double2 *>(sh_t[ko + l]);
+                    const double im = s * tv.x - c * tv.y;
+                    ap += c * tv.x + s * tv.y;
+                    sim += im;
+                    lim += (double)l * im;
+                    const double c2 = c * ec - s * es;
+                    s = c * es + s * ec;
+                    c = c2;
+                }
             }
+            const double s3im = (double)n3s * sim + lim;
             a1 += (double)n1 * sim;
             a2 += (double)n2 * sim;
             a3 += s3im;
