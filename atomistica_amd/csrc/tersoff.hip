@@ -1118,6 +1118,7 @@ __device__ __forceinline__ void exp_cut(const ScrDev &S, double r1, double r2, d
 
 struct ScrShared {
     double x[SCR_ROW], y[SCR_ROW], z[SCR_ROW];
+    double r2[SCR_ROW];           // |r_ik|^2 of each entry, tested first
     int2 e[SCR_ROW];
 };
 
@@ -1149,9 +1150,13 @@ k_scr_screen(ScrDev S, int nat_tot, const double *__restrict__ r, const int *__r
         const int2 e = nbr[beg + t];
         int a, b, c;
         dc_unpack(e.y, a, b, c);
-        L.x[t] = (r[3 * e.x + 0] - r[3 * i + 0]) - image_x(C, 0, a, b, c);
-        L.y[t] = (r[3 * e.x + 1] - r[3 * i + 1]) - image_x(C, 1, a, b, c);
-        L.z[t] = (r[3 * e.x + 2] - r[3 * i + 2]) - image_x(C, 2, a, b, c);
+        const double dx = (r[3 * e.x + 0] - r[3 * i + 0]) - image_x(C, 0, a, b, c);
+        const double dy = (r[3 * e.x + 1] - r[3 * i + 1]) - image_x(C, 1, a, b, c);
+        const double dz = (r[3 * e.x + 2] - r[3 * i + 2]) - image_x(C, 2, a, b, c);
+        L.x[t] = dx;
+        L.y[t] = dy;
+        L.z[t] = dz;
+        L.r2[t] = dx * dx + dy * dy + dz * dz;
         L.e[t] = e;
     }
     __builtin_amdgcn_wave_barrier();
@@ -1182,9 +1187,9 @@ k_scr_screen(ScrDev S, int nat_tot, const double *__restrict__ r, const int *__r
                 const double Cmin = S.Cmin[typ], Cmax = S.Cmax[typ], dC = S.dC[typ];
                 const double lim = S.C_dr_cut[typ] * rl;
                 for (int u = 0; !(screened || sij < thr) && u < n; ++u) {
-                    const double rik[3] = {L.x[u], L.y[u], L.z[u]};
-                    const double rlik = rik[0] * rik[0] + rik[1] * rik[1] + rik[2] * rik[2];
+                    const double rlik = L.r2[u];
                     if (!(rlik < lim)) continue;
+                    const double rik[3] = {L.x[u], L.y[u], L.z[u]};
                     const int2 ek = L.e[u];
                     if (ek.x == ej.x && ek.y == ej.y) continue;
                     const double dot_ij_ik = rij[0] * rik[0] + rij[1] * rik[1] + rij[2] * rik[2];
@@ -1294,13 +1299,17 @@ __global__ void k_nl_rev_entries(int nat, const int *__restrict__ offsets, const
         int a, b, c;
         dc_unpack(e.y, a, b, c);
         const int want = dc_pack(-a, -b, -c);
+        // j's row in chunks of 8 entries, all loads of a chunk in flight
+        // together; the first match in row order
         int found = -1;
-        for (int q = offsets[e.x]; q < offsets[e.x + 1]; ++q) {
-            const int2 f = nbr[q];
-            if (f.x == i && f.y == want) {
-                found = q;
-                break;
-            }
+        const int qe = offsets[e.x + 1];
+        for (int q0 = offsets[e.x]; q0 < qe && found < 0; q0 += 8) {
+            int2 f[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) f[u] = q0 + u < qe ? nbr[q0 + u] : make_int2(-1, 0);
+#pragma unroll
+            for (int u = 7; u >= 0; --u)
+                if (f[u].x == i && f[u].y == want) found = q0 + u;
         }
         rev[s] = found;
         if (found < 0) atomicOr(flags, DERR_BOND_OVERFLOW);
