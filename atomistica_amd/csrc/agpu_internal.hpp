@@ -41,6 +41,8 @@ enum DevErr : int {
     DERR_FIXED_RANGE = 32,
     DERR_SORT_FALLBACK = 64,
     DERR_SCR_ROW = 128,           // screened bond order: row beyond the staged length
+    // a list build whose rows must not be used (nl_settle redoes or refuses it)
+    DERR_NL_UNUSABLE = DERR_NL_OVERFLOW | DERR_IMAGE_RANGE | DERR_STAGE_OVERFLOW | DERR_SORT_FALLBACK,
 };
 
 hipStream_t stream();
@@ -118,6 +120,9 @@ struct agpu_neighbors {
     uint64_t slot_rev = ~0ull;
     // offsets holds the caller-order row offsets of the last build
     bool caller_off = false;
+    // a build whose flags are on their way to the host and not yet checked
+    // (nl_update with defer): nl_settle checks them after the caller's sync
+    bool pending = false;
     bool radix = false;           // bin sort by radix sort (after a very full bin)
     // the list was imported from the caller (LAMMPS) and is not rebuilt
     bool external = false;
@@ -145,7 +150,15 @@ namespace agpu {
 // Rebuilds the list when positions, cell or atoms changed (force > 0:
 // always).  csr = false: the caller reads the rows through NlRows only, so
 // the copy of staged rows into the CSR rows is left until a caller asks.
-int nl_update(agpu_neighbors *nl, agpu_particles *p, int force, bool csr = true);
+int nl_update(agpu_neighbors *nl, agpu_particles *p, int force, bool csr = true, bool defer = false);
+// defer = true (staged rows only, not under capture): a build returns
+// without waiting for its flags; the caller's kernels test the device flag
+// word (nl_flags) and write nothing when it marks the build unusable, and
+// after its own stream synchronisation the caller calls nl_settle, which
+// checks the flags, records the list size and sets *redo when the build
+// has to be redone (the caller then rebuilds and runs again)
+int nl_settle(agpu_neighbors *nl, bool *redo);
+const int *nl_flags(const agpu_neighbors *nl);
 // true when nl_update would keep the list as it is (no rebuild, nothing
 // allocated): imported, held by a driver, or built from these positions
 bool nl_current(const agpu_neighbors *nl, const agpu_particles *p);

@@ -437,10 +437,16 @@ __global__ void __launch_bounds__(256)
 k_eam_density(EamDev P, const SplHdr *__restrict__ hdr, const R *__restrict__ rec, int nat,
               const int *__restrict__ order, const double4 *__restrict__ q1, CellArg C,
               NlRows L, const int *__restrict__ mask, double4 *__restrict__ q2, double *__restrict__ epa,
-              double *__restrict__ partial, int *__restrict__ flags)
+              double *__restrict__ partial, int *__restrict__ flags, const int *__restrict__ nlflags)
 {
     constexpr int APB = 256 / G;
     __shared__ double s_e[APB];
+    // a list build marked unusable: nothing is read from its rows and
+    // nothing written but a zero record (the call is redone or refused)
+    if (*nlflags & DERR_NL_UNUSABLE) {
+        if (threadIdx.x == 0) partial[blockIdx.x] = 0.0;
+        return;
+    }
     const int lane = threadIdx.x % G, grp = threadIdx.x / G;
     const int ai = xcd_block() * APB + grp;
     double Fi = 0.0;
@@ -512,10 +518,14 @@ k_eam_force(EamDev P, const SplHdr *__restrict__ hdr, const R *__restrict__ rec,
             const int *__restrict__ order, const double4 *__restrict__ q1, const double4 *__restrict__ q2,
             CellArg C, NlRows L, const int *__restrict__ mask, double *__restrict__ f, double *__restrict__ epa,
             double *__restrict__ wpa, const double *__restrict__ dens_e, double *__restrict__ partial,
-            int *__restrict__ flags)
+            int *__restrict__ flags, const int *__restrict__ nlflags)
 {
     constexpr int APB = 256 / G;
     __shared__ double s_red[APB][7];
+    if (*nlflags & DERR_NL_UNUSABLE) {
+        if (threadIdx.x < 10) partial[(size_t)blockIdx.x * 10 + threadIdx.x] = 0.0;
+        return;
+    }
     const int lane = threadIdx.x % G, grp = threadIdx.x / G;
     const int ai = xcd_block() * APB + grp;
     double acc[3] = {0.0, 0.0, 0.0}, e = 0.0;
@@ -698,10 +708,10 @@ static int eam_passes(EamState *st, agpu_potential *pot, const agpu_eval_args *a
     const NlRows L = nl_rows(nl);
     AGPU_LAUNCH(mixed ? "eam_density_mixed" : "eam_density_fp64", (k_eam_density<R, G, U, ONE, KN>), dim3(nblk),
                 dim3(256), 0, s, st->dev, st->d_hdr.as<SplHdr>(), rec, nat, order, q1, C,
-                L, a->mask, q2, a->epot_per_at, part, flags);
+                L, a->mask, q2, a->epot_per_at, part, flags, nl_flags(nl));
     AGPU_LAUNCH(mixed ? "eam_force_mixed" : "eam_force_fp64", (k_eam_force<R, G, U, ONE, GEN, KN>), dim3(nblk),
                 dim3(256), 0, s, st->dev, st->d_hdr.as<SplHdr>(), rec, nat, order, q1, q2, C,
-                L, a->mask, a->f, a->epot_per_at, a->wpot_per_at, part, part + nblk, flags);
+                L, a->mask, a->f, a->epot_per_at, a->wpot_per_at, part, part + nblk, flags, nl_flags(nl));
     return AGPU_OK;
 }
 
@@ -778,7 +788,8 @@ int eam_compute(agpu_potential *pot, const agpu_eval_args *a)
     EamState *st = static_cast<EamState *>(pot->impl);
     agpu_particles *p = pot->p;
     if (st->Z_rev != p->Z_rev) AGPU_CHECK(eam_bind(pot));
-    AGPU_CHECK(nl_update(pot->nl, p, 0, false));
+    // the list's flags are checked with this call's own (one host round trip)
+    AGPU_CHECK(nl_update(pot->nl, p, 0, false, true));
     const int nat = p->nat;
     hipStream_t s = stream();
     if (nat == 0) return zero_dev(a->epot, sizeof(double));
@@ -800,6 +811,14 @@ int eam_compute(agpu_potential *pot, const agpu_eval_args *a)
     if (capturing()) return AGPU_OK;     // the MD driver reads the flags after the replay
     AGPU_HIP(hipMemcpyAsync(pot->stats.h, st->stats.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
     AGPU_HIP(hipStreamSynchronize(s));
+    bool redo = false;
+    AGPU_CHECK(nl_settle(pot->nl, &redo));
+    if (redo) {
+        // the build is redone synchronously (radix sort or a wider
+        // staging stride), then the passes again
+        AGPU_CHECK(nl_update(pot->nl, p, 1, false));
+        return eam_compute(pot, a);
+    }
     if (pot->stats.h[0] & DERR_SPLINE_RANGE)
         return set_error(AGPU_ERR_SPLINE, std::string(st->funcfl ? "TabulatedEAM" : "TabulatedAlloyEAM") +
                                               ": spline argument outside the defined interval");

@@ -975,12 +975,59 @@ static void geometry(const agpu_particles *p, agpu_neighbors *nl, BinGeom &g)
 
 bool nl_current(const agpu_neighbors *nl, const agpu_particles *p)
 {
+    if (nl->pending) return false;      // a build whose flags nobody checked
     if (nl->external || nl->held) return nl->p == p && nl->nat == p->nat;
     return nl->p == p && nl->pos_rev == p->pos_rev && nl->cell_rev == p->cell_rev && nl->nat == p->nat;
 }
 
-int nl_update(agpu_neighbors *nl, agpu_particles *p, int force, bool csr)
+const int *nl_flags(const agpu_neighbors *nl) { return nl->rowmax.as<int>() + 3; }
+
+int nl_settle(agpu_neighbors *nl, bool *redo)
 {
+    *redo = false;
+    if (!nl->pending) return AGPU_OK;
+    nl->pending = false;
+    const int fl = nl->stats.h[3];
+    nl->maxrow = nl->stats.h[0];
+    if (fl & DERR_SORT_FALLBACK) {
+        nl->radix = true;
+        nl->p = nullptr;
+        *redo = true;
+        return AGPU_OK;
+    }
+    if (fl & DERR_STAGE_OVERFLOW) {
+        nl->stage_w = ((int)(1.25 * nl->maxrow) + 16 + 7) & ~7;
+        nl->p = nullptr;
+        *redo = true;
+        return AGPU_OK;
+    }
+    nl->total = (int64_t)nl->stats.h[1] | ((int64_t)nl->stats.h[2] << 31);
+    {
+        const int want = (nl->maxrow + std::max(2, nl->maxrow / 4) + 3) & ~3;
+        if (want >= 4 && 4 * want <= 3 * nl->stage_w) nl->stage_w = want;
+    }
+    if (fl & DERR_IMAGE_RANGE) {
+        nl->p = nullptr;
+        return set_error(AGPU_ERR_DOMAIN, "Particle outside simulation domain: image shift beyond +-511 cells.");
+    }
+    if (fl & DERR_NL_OVERFLOW) {
+        nl->p = nullptr;
+        return set_error(AGPU_ERR_NL_OVERFLOW,
+                         "Neighbor list overflow: " + std::to_string(nl->total) + " neighbours + " +
+                             std::to_string(nl->nat) + " separators exceed nat*avgn = " +
+                             std::to_string(nl->cap) + ". Increase avgn.");
+    }
+    return AGPU_OK;
+}
+
+int nl_update(agpu_neighbors *nl, agpu_particles *p, int force, bool csr, bool defer)
+{
+    if (nl->pending) {
+        // an earlier deferred build that nobody checked
+        AGPU_HIP(hipStreamSynchronize(stream()));
+        bool redo = false;
+        AGPU_CHECK(nl_settle(nl, &redo));
+    }
     if (nl->external) {
         // an imported list stays as the caller gave it
         if (nl->p != p || nl->nat != p->nat)
@@ -1146,6 +1193,21 @@ int nl_update(agpu_neighbors *nl, agpu_particles *p, int force, bool csr)
         AGPU_HIP(hipMemsetAsync(nl->offsets.ptr, 0, sizeof(int), s));
     }
     AGPU_HIP(hipMemcpyAsync(nl->stats.h, flags, sizeof(int) * 4, hipMemcpyDeviceToHost, s));
+    if (defer && !csr && nat > 0 && !capturing()) {
+        // the rows are taken as built; nl_settle checks the flags after the
+        // caller's synchronisation
+        nl->pending = true;
+        nl->stage_s = nl->stage_w;
+        nl->csr = false;
+        nl->caller_off = false;
+        nl->p = p;
+        nl->nat = nat;
+        nl->pos_rev = p->pos_rev;
+        nl->cell_rev = p->cell_rev;
+        nl->slot_rev = p->pos_rev;
+        nl->n_updates++;
+        return AGPU_OK;
+    }
     AGPU_HIP(hipStreamSynchronize(s));
     int fl = nl->stats.h[3];
     nl->maxrow = nl->stats.h[0];
@@ -1251,8 +1313,11 @@ int nl_import(agpu_neighbors *nl, agpu_particles *p, const int64_t *seed, const 
     AGPU_CHECK(nl->nbr.reserve(sizeof(int2) * ((size_t)total + 1)));
     AGPU_CHECK(nl->idx_sorted.reserve(sizeof(int) * (nat + 1)));
     AGPU_CHECK(nl->atoms_sorted.reserve(sizeof(SortedAtom) * (nat + 1)));
+    AGPU_CHECK(nl->rowmax.reserve(sizeof(int) * 8));
     if (nat > 0 && !p->d_r) return set_error(AGPU_ERR_STATE, "particles: positions not set");
     hipStream_t s = stream();
+    // no build flags: the imported rows are usable as they are
+    AGPU_HIP(hipMemsetAsync(nl->rowmax.ptr, 0, sizeof(int) * 8, s));
     AGPU_HIP(hipMemcpyAsync(nl->offsets.ptr, off.data(), sizeof(int) * (nat + 1), hipMemcpyHostToDevice, s));
     if (total) AGPU_HIP(hipMemcpyAsync(nl->nbr.ptr, rows.data(), sizeof(int2) * total, hipMemcpyHostToDevice, s));
     // atoms in the caller's order: slot = atom

@@ -204,3 +204,54 @@ def test_staging_stride_follows_rows():
         assert abs(e - e0) <= 1e-10 * abs(e0)
         scale_f = max(1.0, np.abs(f0).max())
         assert np.abs(f - f0).max() <= 1e-10 * scale_f
+
+
+def _eam_oracle(pos, cell, pbc, Z, avgn=100):
+    t = read_setfl_tables(MISHIN)
+    el = np.zeros(len(pos), np.int32)
+    return O.eam(t, pos, el, cell, O.nl_build(pos, cell, pbc, t["cutoff"], avgn=avgn))
+
+
+def _close(a, b, rel=1e-10):
+    a, b = np.asarray(a), np.asarray(b)
+    return np.abs(a - b).max() <= rel * max(1.0, np.abs(b).max())
+
+
+def test_eam_deferred_list_check_redo():
+    """EAM checks its list build's flags with its own at the end of the call.
+    A build that outgrew the staging stride, and one whose crowded bin hands
+    over to the radix sort, are marked unusable on the device: the passes
+    write nothing, the call rebuilds and runs again, and the results are the
+    oracle's."""
+    # rows outgrow the stride the previous build narrowed to
+    pos, cell, pbc, Z = W.fcc_cu(4, sigma=0.05, seed=12)
+    p = N.from_atoms(pos, cell, pbc, Z)
+    nl = N.Neighbors(200)
+    pot = N.TabulatedAlloyEAM(fn=MISHIN)
+    pot.bind_to(p, nl)
+    for scale in (1.0, 1.0, 0.85, 0.85):
+        print("deferred check, stride case scale", scale, flush=True)
+        ps, cs = pos * scale, cell * scale
+        p.set_cell(cs, pbc)
+        p.coordinates[:] = ps
+        p.I_changed_positions()
+        e, f, w, *_ = pot.energy_and_forces(p, nl)
+        e0, f0, w0 = _eam_oracle(ps, cs, pbc, Z, avgn=200)
+        assert abs(e - e0) <= 1e-10 * abs(e0) and _close(f, f0) and _close(w, w0)
+    # a crowded bin: the counting sort hands over to the radix sort
+    rng = np.random.default_rng(22)
+    pos = np.concatenate([40.0 + rng.random((400, 3)) * 3.0, rng.random((200, 3)) * 100.0])
+    cell = np.diag([100.0, 100.0, 100.0])
+    pbc = np.array([True, True, True])
+    Z = np.full(len(pos), 29, np.int32)
+    print("deferred check, crowded bin", len(pos), flush=True)
+    p = N.from_atoms(pos, cell, pbc, Z)
+    nl = N.Neighbors(500)
+    pot = N.TabulatedAlloyEAM(fn=MISHIN)
+    pot.bind_to(p, nl)
+    e, f, w, *_ = pot.energy_and_forces(p, nl)
+    e0, f0, w0 = _eam_oracle(pos, cell, pbc, Z, avgn=500)
+    assert abs(e - e0) <= 1e-10 * abs(e0) and _close(f, f0) and _close(w, w0)
+    off, j, dc = nl.get_list(p)
+    ro, rj, rdc = O.nl_build(pos, cell, pbc, read_setfl_tables(MISHIN)["cutoff"], avgn=500)
+    assert np.array_equal(off, ro) and np.array_equal(j, rj) and np.array_equal(dc, rdc)
