@@ -43,6 +43,10 @@ enum DevErr : int {
     DERR_SCR_ROW = 128,           // screened bond order: row beyond the staged length
     // a list build whose rows must not be used (nl_settle redoes or refuses it)
     DERR_NL_UNUSABLE = DERR_NL_OVERFLOW | DERR_IMAGE_RANGE | DERR_STAGE_OVERFLOW | DERR_SORT_FALLBACK,
+    // a pass run ahead of its checks found it must be redone (an unusable
+    // list build, a row longer than its lane group): later passes write
+    // nothing to the caller's outputs
+    DERR_REDO = 256,
 };
 
 hipStream_t stream();
@@ -348,7 +352,7 @@ int rebo2_variant(agpu_potential *pot, int k, int *g);
 // tmp: room for the second-level records, ceil(nblk / 256) * w doubles
 // (NULL: the library's own buffer, grown on demand).
 int reduce_ev(const double *partials, int nblk, int w, double *epot, double *wpot, double *extra,
-              bool set_epot = false, double *tmp = nullptr);
+              bool set_epot = false, double *tmp = nullptr, const int *skip = nullptr);
 
 // *out = max(*out, max(v[0..n))) on the library stream (device pointers)
 int int_max(const int *v, int n, int *out);

@@ -166,8 +166,10 @@ constexpr int RED_W_MAX = 16;
 __global__ void __launch_bounds__(256)
 k_reduce_rec(const double *__restrict__ part, int nblk, int w, int chunk,
              double *__restrict__ out, double *__restrict__ epot, double *__restrict__ wpot,
-             double *__restrict__ extra, int set_epot)
+             double *__restrict__ extra, int set_epot, const int *__restrict__ skip)
 {
+    // a call that will be redone leaves the outputs as they are
+    if (skip && (*skip & DERR_REDO)) return;
     __shared__ double sh[256];
     const int lanes = 256 / w;         // records read per sweep
     const int c = threadIdx.x % w;
@@ -201,7 +203,7 @@ k_reduce_rec(const double *__restrict__ part, int nblk, int w, int chunk,
 static DevBuf *g_red_tmp = new DevBuf();
 
 int reduce_ev(const double *partials, int nblk, int w, double *epot, double *wpot, double *extra, bool set_epot,
-              double *tmp)
+              double *tmp, const int *skip)
 {
     if (nblk <= 0 || w <= 0) return AGPU_OK;
     if (w > RED_W_MAX) return set_error(AGPU_ERR_ARG, "reduce_ev: record too wide");
@@ -210,7 +212,7 @@ int reduce_ev(const double *partials, int nblk, int w, double *epot, double *wpo
     int n1 = (nblk + chunk - 1) / chunk;
     if (n1 <= 1) {
         AGPU_LAUNCH("reduce_ev", k_reduce_rec, dim3(1), dim3(256), 0, s, partials, nblk, w, nblk,
-                    (double *)nullptr, epot, wpot, extra, (int)set_epot);
+                    (double *)nullptr, epot, wpot, extra, (int)set_epot, skip);
         return AGPU_OK;
     }
     if (!tmp) {
@@ -218,9 +220,9 @@ int reduce_ev(const double *partials, int nblk, int w, double *epot, double *wpo
         tmp = g_red_tmp->as<double>();
     }
     AGPU_LAUNCH("reduce_ev", k_reduce_rec, dim3(n1), dim3(256), 0, s, partials, nblk, w, chunk,
-                tmp, (double *)nullptr, (double *)nullptr, (double *)nullptr, 0);
+                tmp, (double *)nullptr, (double *)nullptr, (double *)nullptr, 0, (const int *)nullptr);
     AGPU_LAUNCH("reduce_ev", k_reduce_rec, dim3(1), dim3(256), 0, s, (const double *)tmp, n1, w,
-                n1, (double *)nullptr, epot, wpot, extra, (int)set_epot);
+                n1, (double *)nullptr, epot, wpot, extra, (int)set_epot, skip);
     return AGPU_OK;
 }
 

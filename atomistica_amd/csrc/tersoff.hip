@@ -78,6 +78,9 @@ struct TersoffState {
     // take the bond-compacting path even for rows of at most 64 entries
     // (agpu_potential_set_launch variant 1 of a bond-order class)
     bool compact = false;
+    // lane-group width that a row of an unchecked build outgrew: calls over
+    // unchecked builds use at least this width from then on
+    int row_g = 0;
 };
 
 // Device pointers of the per-call scratch of the unscreened bond-order path:
@@ -692,7 +695,10 @@ k_bop_gather(int nat, const int *__restrict__ offsets, const int *__restrict__ b
 // are needed (WB: the reference's per-bond wij, bop_kernel.f90:1325-1512).
 // HT = false: every h-term of the parameter set is 1 (mubo = 0), so h and
 // dh are constants.  The reverse entry is looked up first, so its loads
-// overlap the arithmetic.
+// overlap the arithmetic.  Over a list build whose flags the host has not
+// read (nlflags non-NULL), a build that proved unusable, or a row longer
+// than G, sets DERR_REDO in flags[0]: the gather and the reduction then
+// leave the caller's outputs alone and the host redoes the call.
 // -------------------------------------------------------------------------
 constexpr int DERR_REV_MISSING = DERR_BOND_OVERFLOW;
 
@@ -702,8 +708,13 @@ k_bop_rows(TersoffDev P, int nat, const int *__restrict__ order, const int *__re
            NlRows L, const int *__restrict__ offsets, const int *__restrict__ mask, double *__restrict__ frev,
            double *__restrict__ fown, double *__restrict__ eslot, double *__restrict__ erev,
            double *__restrict__ wslot, double *__restrict__ wrev, double *__restrict__ partial,
-           double *__restrict__ epb, double *__restrict__ fpb, double *__restrict__ wpb, int *__restrict__ flags)
+           double *__restrict__ epb, double *__restrict__ fpb, double *__restrict__ wpb, int *__restrict__ flags,
+           const int *__restrict__ nlflags)
 {
+    if (nlflags && (*nlflags & DERR_NL_UNUSABLE)) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(flags, DERR_REDO);
+        return;
+    }
     constexpr int GPB = 256 / G;
     constexpr bool CACHE = G <= 8;
     constexpr int NC = CACHE ? G : 1;
@@ -722,7 +733,8 @@ k_bop_rows(TersoffDev P, int nat, const int *__restrict__ order, const int *__re
     const int eli = valid ? el_s[ai] : -1;
     int n = 0;
     const int2 *row = valid ? nl_row(L, ai, n) : nullptr;
-    n = min(n, G);                      // G >= every row length (host side)
+    if (n > G) atomicOr(flags, DERR_REDO);  // only over an unchecked build
+    n = min(n, G);
     const bool act = lane < n;
 
     int j = 0, code = 0, typ = 0;       // j: the neighbour's slot
@@ -1032,10 +1044,10 @@ __global__ void __launch_bounds__(256)
 k_bop_rows_gather(int nat, const int *__restrict__ order, NlRows L, const double *__restrict__ frev,
                   const double *__restrict__ fown, const double *__restrict__ eslot, const double *__restrict__ erev,
                   const double *__restrict__ wslot, const double *__restrict__ wrev, double *__restrict__ f,
-                  double *__restrict__ epa, double *__restrict__ wpa)
+                  double *__restrict__ epa, double *__restrict__ wpa, const int *__restrict__ flags)
 {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    if (x >= nat) return;
+    if (x >= nat || (*flags & DERR_REDO)) return;
     const int a = order[x];
     int n;
     nl_row(L, x, n);
@@ -2060,7 +2072,8 @@ static int launch_rows_k(const char *name, TersoffState *st, const RowBufs &B, a
     std::memcpy(C.A, p->cell.A, sizeof C.A);
     AGPU_LAUNCH(name, (k_bop_rows<R, G, FAM, WB, HT>), dim3(nblk), dim3(256), 0, s, st->dev, p->nat,
                 nl->idx_sorted.as<int>(), B.el_s, C, nl_rows(nl), nl->offsets.as<int>(), a->mask, B.frev, B.fown,
-                eslot, erev, wslot, wrev, B.partial, a->epot_per_bond, a->f_per_bond, a->wpot_per_bond, B.stats);
+                eslot, erev, wslot, wrev, B.partial, a->epot_per_bond, a->f_per_bond, a->wpot_per_bond, B.stats,
+                nl->pending ? nl_flags(nl) : nullptr);
     return AGPU_OK;
 }
 
@@ -2124,26 +2137,40 @@ static int dispatch_rows(int G, bool wb, TersoffState *st, const RowBufs &B, agp
     }
 }
 
-// group width of the row path: the longest row, at least 4 lanes
-static int rows_width(const agpu_neighbors *nl)
+// group width of the row path: the longest row, at least 4 lanes; over an
+// unchecked build (whose longest row is not known yet) the longest row of
+// the build before, and at least the width a row of an unchecked build
+// once outgrew
+static int rows_width(const agpu_neighbors *nl, int floor_g = 0)
 {
     int G = 4;
     while (G < nl->maxrow) G <<= 1;
-    return G;
+    return nl->pending ? std::min(64, std::max(G, floor_g)) : G;
+}
+
+// entries the per-entry buffers of the row path must hold: the list's
+// total, or over an unchecked build the staging rows' bound (the kernels
+// write no entry of a build that overflowed them)
+static int64_t rows_entries(const agpu_neighbors *nl)
+{
+    return nl->pending ? (int64_t)nl->nat * nl->stage_s : nl->total;
 }
 
 // The row path of the unscreened bond-order classes over the current list
 // (rows of at most 64 entries): the bond-order kernel, the gather and the
-// fixed-order energy / virial sum.  Flags are read back unless capturing.
+// fixed-order energy / virial sum.  Flags are read back unless capturing,
+// and with them a deferred list build's (nl_settle): *again = true when the
+// call has to be made again (the build was unusable, or a row was longer
+// than the lanes the call took).
 static int bop_rows_run(agpu_potential *pot, const agpu_eval_args *a, const RowBufs &B, double *eslot,
-                        double *erev, double *wslot, double *wrev, int *h_flags)
+                        double *erev, double *wslot, double *wrev, int *h_flags, bool *again = nullptr)
 {
     TersoffState *st = static_cast<TersoffState *>(pot->impl);
     agpu_particles *p = pot->p;
     agpu_neighbors *nl = pot->nl;
     const int nat = p->nat;
     hipStream_t s = stream();
-    const int G = rows_width(nl);
+    const int G = rows_width(nl, st->row_g);
     const int nblk = bop_blocks(nat, G);
     const bool wb = wslot || a->wpot_per_bond;
     const int *el = pot->d_el.as<int>();
@@ -2157,26 +2184,37 @@ static int bop_rows_run(agpu_potential *pot, const agpu_eval_args *a, const RowB
     else
         AGPU_CHECK(dispatch_rows<double>(G, wb, st, B, p, nl, el, a, eslot, erev, wslot, wrev, nblk, s));
     AGPU_LAUNCH("bop_gather", k_bop_rows_gather, dim3((nat + 255) / 256), dim3(256), 0, s, nat, order,
-                nl_rows(nl), B.frev, B.fown, eslot, erev, wslot, wrev, a->f, a->epot_per_at, a->wpot_per_at);
-    AGPU_CHECK(reduce_ev(B.partial, nblk, 10, a->epot, a->wpot, nullptr, true, B.red));
+                nl_rows(nl), B.frev, B.fown, eslot, erev, wslot, wrev, a->f, a->epot_per_at, a->wpot_per_at,
+                (const int *)B.stats);
+    AGPU_CHECK(reduce_ev(B.partial, nblk, 10, a->epot, a->wpot, nullptr, true, B.red, B.stats));
     if (capturing()) return AGPU_OK;     // the MD driver reads the flags after the replay
     AGPU_HIP(hipMemcpyAsync(h_flags, B.stats, sizeof(int) * 4, hipMemcpyDeviceToHost, s));
     AGPU_HIP(hipStreamSynchronize(s));
+    const bool deferred = nl->pending;
+    bool redo = false;
+    AGPU_CHECK(nl_settle(nl, &redo));
+    if (redo || (h_flags[0] & DERR_REDO)) {
+        if (!deferred || !again)
+            return set_error(AGPU_ERR_STATE, "Tersoff: a row longer than the lanes of its group");
+        if (!redo) st->row_g = std::max(st->row_g, rows_width(nl));
+        *again = true;
+        return AGPU_OK;
+    }
     if (h_flags[0] & DERR_REV_MISSING)
         return set_error(AGPU_ERR_NEB, "Tersoff: bond without its reverse bond (inconsistent list)");
     return AGPU_OK;
 }
 
-static int bop_rows_compute(agpu_potential *pot, const agpu_eval_args *a)
+static int bop_rows_compute(agpu_potential *pot, const agpu_eval_args *a, bool *again)
 {
     TersoffState *st = static_cast<TersoffState *>(pot->impl);
     const int nat = pot->p->nat;
-    const int64_t total = pot->nl->total;
+    const int64_t total = rows_entries(pot->nl);
     AGPU_CHECK(st->fslot.reserve(sizeof(double) * 3 * (total + 1)));
     AGPU_CHECK(st->fown.reserve(sizeof(double) * 3 * (nat + 1)));
     AGPU_CHECK(st->el_s.reserve(sizeof(int) * (nat + 1)));
     AGPU_CHECK(st->stats.reserve(sizeof(int) * 4));
-    AGPU_CHECK(st->partial.reserve(sizeof(double) * 10 * bop_blocks(nat, rows_width(pot->nl))));
+    AGPU_CHECK(st->partial.reserve(sizeof(double) * 10 * bop_blocks(nat, rows_width(pot->nl, st->row_g))));
     AGPU_CHECK(pot->stats.init());
     double *eslot = nullptr, *erev = nullptr, *wslot = nullptr, *wrev = nullptr;
     if (a->epot_per_at) {
@@ -2191,7 +2229,7 @@ static int bop_rows_compute(agpu_potential *pot, const agpu_eval_args *a)
     }
     RowBufs B{st->fslot.as<double>(), st->fown.as<double>(), st->partial.as<double>(), nullptr, st->stats.as<int>(),
               st->el_s.as<int>()};
-    return bop_rows_run(pot, a, B, eslot, erev, wslot, wrev, pot->stats.h);
+    return bop_rows_run(pot, a, B, eslot, erev, wslot, wrev, pot->stats.h, again);
 }
 
 // bonds of every row (k_bop_bonds) into B.bslot / B.bcnt
@@ -2240,13 +2278,27 @@ int tersoff_compute(agpu_potential *pot, const agpu_eval_args *a)
     agpu_particles *p = pot->p;
     agpu_neighbors *nl = pot->nl;
     if (st->Z_rev != p->Z_rev) AGPU_CHECK(tersoff_bind(pot));
-    // the row path reads the rows where the builder left them
-    AGPU_CHECK(nl_update(nl, p, 0, false));
+    // the row path reads the rows where the builder left them.  When the
+    // last build's rows fit it (and no per-bond array is asked for, which
+    // the bond-order kernel writes itself), a new build's flags are checked
+    // with this call's own: one host round trip per call
+    const bool rows = !st->compact && !st->scr;
+    const bool defer = rows && nl->maxrow > 0 && nl->maxrow <= 64 && !a->epot_per_bond && !a->f_per_bond &&
+                       !a->wpot_per_bond;
+    AGPU_CHECK(nl_update(nl, p, 0, false, defer));
     const int nat = p->nat;
     hipStream_t s = stream();
     if (nat == 0) return zero_dev(a->epot, sizeof(double));
     // the row path sets epot from its block records; the others add to it
-    if (nl->maxrow <= 64 && !st->compact && !st->scr) return bop_rows_compute(pot, a);
+    if (rows && (nl->pending || nl->maxrow <= 64)) {
+        bool again = false;
+        AGPU_CHECK(bop_rows_compute(pot, a, &again));
+        if (!again) return AGPU_OK;
+        // the build is redone synchronously when it was unusable (radix
+        // sort, a wider staging stride), then the call with its rows known
+        if (!nl_current(nl, p)) AGPU_CHECK(nl_update(nl, p, 1, false));
+        return tersoff_compute(pot, a);
+    }
     AGPU_CHECK(zero_dev(a->epot, sizeof(double)));
     if (st->scr) {
         AGPU_CHECK(nl_update(nl, p, 0));

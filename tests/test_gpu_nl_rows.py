@@ -255,3 +255,35 @@ def test_eam_deferred_list_check_redo():
     off, j, dc = nl.get_list(p)
     ro, rj, rdc = O.nl_build(pos, cell, pbc, read_setfl_tables(MISHIN)["cutoff"], avgn=500)
     assert np.array_equal(off, ro) and np.array_equal(j, rj) and np.array_equal(dc, rdc)
+
+
+def test_tersoff_deferred_list_check_redo():
+    """The Tersoff row path checks a new build's flags with its own when the
+    last build's rows fit it.  A row longer than the lane group the last
+    build gave, and a row longer than the staging stride, each make the call
+    redo itself (the first writes nothing); every call matches the oracle,
+    the per-atom arrays too."""
+    from atomistica_amd import parameters as P
+    prm = P.Tersoff_PRB_39_5566_Si_C
+    pos, cell, pbc, Z = W.diamond_si(3, sigma=0.0, seed=1)
+    a0 = 5.431
+    el = np.ones(len(pos), np.int32)             # Si of the (C, Si) set
+    p = N.from_atoms(pos, cell, pbc, Z)
+    nl = N.Neighbors(100)
+    pot = N.Tersoff()                            # the (C, Si) set above
+    pot.bind_to(p, nl)
+    # atom 0 towards the tetrahedral site: its neighbours' rows reach 5
+    # entries (frac 0.3; lanes of 4 so far) and its own 10 (frac 1.0; a
+    # stride of 8 so far), then back to the lattice
+    for frac, per_at in ((0.0, False), (0.3, True), (1.0, False), (0.0, True), (0.8, False)):
+        ps = pos.copy()
+        ps[0] = pos[0] + frac * np.full(3, a0 / 2)
+        print("tersoff deferred check, frac", frac, flush=True)
+        p.coordinates[:] = ps
+        p.I_changed_positions()
+        e, f, w, epa, _, _, wpa, _ = pot.energy_and_forces(p, nl, epot_per_at=per_at, wpot_per_at=per_at)
+        ref = O.tersoff(prm, ps, el, cell, O.nl_build(ps, cell, pbc, max(prm["r2"])), per_at=True)
+        assert abs(e - ref[0]) <= 1e-10 * abs(ref[0])
+        assert _close(f, ref[1]) and _close(w, ref[2])
+        if per_at:
+            assert _close(epa, ref[3]) and _close(wpa, ref[4])
