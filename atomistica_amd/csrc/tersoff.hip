@@ -40,7 +40,7 @@ struct TersoffDev {
     double beta[3], n[3], c[3], d[3], h[3];
     double r1[9], r2[9];          // 9: the ordered pairs of the Juslin family
     // derived
-    double c2[3], d2[3], bn[3], e[3], fac[9];
+    double c2[3], d2[3], cd2[3], bn[3], e[3], fac[9];   // cd2 = c^2 / d^2
     // Kumagai family (kumagai_func.f90): g, bond order and h forms; A, B,
     // lambda (= lambda1), mu (= lambda2), h, r1, r2 are shared with the above
     int fam;                      // 0 Tersoff, 1 Kumagai, 2 Brenner
@@ -52,6 +52,7 @@ struct TersoffDev {
     // (r - r0)), g = gamma (1 + c^2/d^2 - c^2/(d^2 + (h + cos)^2)),
     // b = (1 + zeta^n)^(-1/2n)
     double br0[9], bVRf[9], bVAf[9], bexpR[9], bexpA[9], bgam[9], bc2[9], bd2[9], bh[9], bnn[9], bboe[9];
+    double bcd2[9];               // c^2 / d^2
     // Juslin family (juslin_func.f90): the Brenner forms above over ordered
     // element pairs j + i nel, and h = omega exp((alpha dr)^m) per ordered
     // triplet k + nel (j + nel i) of centre i
@@ -131,6 +132,7 @@ static void derive(TersoffDev &D, const agpu_tersoff_params &p)
         D.h[k] = p.h[k];
         D.c2[k] = p.c[k] * p.c[k];
         D.d2[k] = p.d[k] * p.d[k];
+        D.cd2[k] = D.d2[k] != 0.0 ? D.c2[k] / D.d2[k] : 0.0;
         D.bn[k] = std::pow(p.beta[k], p.n[k]);
         D.e[k] = -0.5 / p.n[k];
     }
@@ -319,11 +321,12 @@ __device__ __forceinline__ void g_fn(const TersoffDev &P, int eli, int ikpot, R 
     if (FAM >= 2) {
         // brenner_func.f90:77-101 in the cancellation-free form
         // gamma (1 + c^2 t^2 / (d^2 (d^2 + t^2))), t = h + cos, of the pair ik
+        // (one reciprocal of hh for both)
         const R c2 = (R)P.bc2[ikpot], d2 = (R)P.bd2[ikpot], gam = (R)P.bgam[ikpot];
         const R t = (R)P.bh[ikpot] + costh;
-        const R hh = d2 + t * t;
-        g = gam * ((R)1 + c2 * t * t / (d2 * hh));
-        dg = (R)2 * gam * c2 * t / (hh * hh);
+        const R rh = (R)1 / (d2 + t * t);
+        g = gam * ((R)1 + (R)P.bcd2[ikpot] * t * t * rh);
+        dg = (R)2 * gam * c2 * t * rh * rh;
         return;
     }
     if (FAM == 1) {
@@ -340,8 +343,10 @@ __device__ __forceinline__ void g_fn(const TersoffDev &P, int eli, int ikpot, R 
     const R h_c = (R)P.h[eli] - costh;
     const R hh = d2 + h_c * h_c;
     const R om = (R)P.omega[ikpot];
-    g = om * ((R)1 + c2 * h_c * h_c / (d2 * hh));
-    dg = (R)-2 * om * c2 * h_c / (hh * hh);
+    // one reciprocal of hh for both
+    const R rh = (R)1 / hh;
+    g = om * ((R)1 + (R)P.cd2[eli] * h_c * h_c * rh);
+    dg = (R)-2 * om * c2 * h_c * rh * rh;
 }
 
 // bond order b_ij and Pz = 1/2 fc V_A db/dzeta (V_A already carries the mask
@@ -540,14 +545,13 @@ k_tersoff(TersoffDev P, int nat, const int *__restrict__ order, const double *__
             R costh = rnk[0] * rn[0] + rnk[1] * rn[1] + rnk[2] * rn[2];
             R g, dg;
             g_fn<R, FAM>(P, elc, tpk, costh, g, dg);
+            // r_jk = r_k - r_j; its length cancels in d cos / d r_k and d r_j
             R dkc[3];
             for (int x = 0; x < 3; ++x) dkc[x] = rnk[x] * rlik - rn[x] * rl;
-            R disjk = rsqrt_<R>(dkc[0] * dkc[0] + dkc[1] * dkc[1] + dkc[2] * dkc[2]);
-            for (int x = 0; x < 3; ++x) dkc[x] = dkc[x] / disjk;
             R rlikr = (R)1 / rlik;
             R dcsdij = rlikr - costh * rlijr;
             R dcsdik = rlijr - costh * rlikr;
-            R dcsdjk = -disjk * rlijr * rlikr;
+            R dcsdjk = -rlijr * rlikr;
             R dzfac = fcik * dg * hv;
             R dzdrij = g * fcik * dh;
             R dzdrik = g * (dfcik * hv - fcik * dh);
@@ -919,14 +923,14 @@ k_bop_rows(TersoffDev P, int nat, const int *__restrict__ order, const int *__re
                 dh = (R)0;
             }
             const R costh = rnk[0] * rn[0] + rnk[1] * rn[1] + rnk[2] * rn[2];
+            // r_jk = r_k - r_j; d cos / d r_jk along the unit vector r_jk / |r_jk|
+            // is -|r_jk| / (r_ij r_ik), so with the vector itself the length
+            // cancels (no square root or division per triplet)
             R dkc[3];
             for (int x = 0; x < 3; ++x) dkc[x] = rnk[x] * rlik - rn[x] * rl;
-            const R disjk = rsqrt_<R>(dkc[0] * dkc[0] + dkc[1] * dkc[1] + dkc[2] * dkc[2]);
-            const R dinv = (R)1 / disjk;
-            for (int x = 0; x < 3; ++x) dkc[x] = dkc[x] * dinv;
             const R dcsdij = rlikr - costh * rlijr;
             const R dcsdik = rlijr - costh * rlikr;
-            const R dcsdjk = -disjk * rlijr * rlikr;
+            const R dcsdjk = -rlijr * rlikr;
             const R dzfac = fcik * dg * hv;
             const R dzdrij = g * fcik * dh;
             const R dzdrik = g * (dfcik * hv - fcik * dh);
@@ -1466,14 +1470,13 @@ k_bop_scr(TersoffDev P, ScrDev S, int nat, const int *__restrict__ order, const 
             R g, dg;
             g_fn<R, FAM>(P, elc, tpk, costh, g, dg);
             zf = g * hv * Pz;
+            // r_jk = r_k - r_j; its length cancels in d cos / d r_k and d r_j
             R dkc[3];
             for (int x = 0; x < 3; ++x) dkc[x] = rnk[x] * rlik - rn[x] * rl;
-            R disjk = rsqrt_<R>(dkc[0] * dkc[0] + dkc[1] * dkc[1] + dkc[2] * dkc[2]);
-            for (int x = 0; x < 3; ++x) dkc[x] = dkc[x] / disjk;
             R rlikr = (R)1 / rlik;
             R dcsdij = rlikr - costh * rlijr;
             R dcsdik = rlijr - costh * rlikr;
-            R dcsdjk = -disjk * rlijr * rlikr;
+            R dcsdjk = -rlijr * rlikr;
             R dzfac = fcik * dg * hv;
             R dzdrij = g * fcik * dh;
             R dzdrik = g * (dfcik * hv - fcik * dh);
@@ -2659,6 +2662,7 @@ extern "C" int agpu_brenner_create(agpu_potential **pot, const agpu_brenner_para
         D.bgam[q] = in ? b.gamma[q] : 0.0;
         D.bc2[q] = in ? b.c[q] * b.c[q] : 0.0;
         D.bd2[q] = in ? b.d[q] * b.d[q] : 1.0;
+        D.bcd2[q] = D.bd2[q] != 0.0 ? D.bc2[q] / D.bd2[q] : 0.0;
         D.bh[q] = in ? b.h[q] : 0.0;
         D.bnn[q] = n;
         D.bboe[q] = -0.5 / n;
@@ -2907,6 +2911,7 @@ extern "C" int agpu_juslin_create(agpu_potential **pot, const agpu_juslin_params
         D.bgam[q] = in ? j.gamma[q] : 0.0;
         D.bc2[q] = in ? j.c[q] * j.c[q] : 0.0;
         D.bd2[q] = in ? j.d[q] * j.d[q] : 1.0;
+        D.bcd2[q] = D.bd2[q] != 0.0 ? D.bc2[q] / D.bd2[q] : 0.0;
         D.bh[q] = in ? j.h[q] : 0.0;
         D.bnn[q] = nn;
         D.bboe[q] = -0.5 / nn;
