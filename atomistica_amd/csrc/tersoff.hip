@@ -221,12 +221,14 @@ template <> __device__ __forceinline__ float rlog<float>(float x) { return logf(
 template <typename R> __device__ __forceinline__ R rpow_pos(R x, R y) { return rexp<R>(y * rlog<R>(x)); }
 
 // DPP lane moves of both halves of a double (quad_perm and row mirrors stay
-// inside a 16-lane row; no LDS traffic)
+// inside a 16-lane row; no LDS traffic).  These patterns read no lane
+// outside the row, so no lane keeps an old value: a plain DPP move, without
+// the initialisation of its destination
 template <int CTRL> __device__ __forceinline__ double dpp_d(double v)
 {
     const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL, 0xf, 0xf, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
+    const int lo = __builtin_amdgcn_mov_dpp((int)(b & 0xffffffffll), CTRL, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xf, 0xf, true);
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
@@ -887,7 +889,9 @@ k_bop_rows(TersoffDev P, int nat, const int *__restrict__ order, const int *__re
     }
 
     // ---- pass 2: derivatives; forces on k summed over the group into lane k ----
-    R dbidi[3] = {(R)0, (R)0, (R)0}, dbidj[3] = {(R)0, (R)0, (R)0};
+    // zeta_ij depends on r_ij and the r_ik only, so its gradient in r_i is
+    // minus the sum of those in r_j and the r_k (dksum: the latter)
+    R dksum[3] = {(R)0, (R)0, (R)0}, dbidj[3] = {(R)0, (R)0, (R)0};
     R wijb[WB ? 9 : 1];
 #pragma unroll
     for (int q = 0; q < (WB ? 9 : 1); ++q) wijb[q] = (R)0;
@@ -936,13 +940,12 @@ k_bop_rows(TersoffDev P, int nat, const int *__restrict__ order, const int *__re
             const R dzdrik = g * (dfcik * hv - fcik * dh);
             R df[3];
             for (int x = 0; x < 3; ++x) {
-                const R dcsdi = -dcsdij * rn[x] - dcsdik * rnk[x];
                 const R dcsdj = dcsdij * rn[x] - dcsdjk * dkc[x];
                 const R dcsdk = dcsdik * rnk[x] + dcsdjk * dkc[x];
-                dbidi[x] += -dzdrij * rn[x] - dzdrik * rnk[x] + dzfac * dcsdi;
                 df[x] = dzdrij * rn[x] + dzfac * dcsdj;
                 dbidj[x] += df[x];
                 dk[x] = dzdrik * rnk[x] + dzfac * dcsdk;
+                dksum[x] += dk[x];
             }
             if (WB) {
                 const R rij[3] = {rl * rn[0], rl * rn[1], rl * rn[2]};
@@ -973,7 +976,8 @@ k_bop_rows(TersoffDev P, int nat, const int *__restrict__ order, const int *__re
         const R dffac = (R)0.5 * (dVR * fc + bij * dVA * fc + VR * dfc + bij * VA * dfc);
         for (int x = 0; x < 3; ++x) {
             dfp[x] = (double)(dffac * rn[x]);
-            fi[x] = dfp[x] - (double)Pz * (double)dbidi[x];
+            const R dbidi = -(dbidj[x] + dksum[x]);
+            fi[x] = dfp[x] - (double)Pz * (double)dbidi;
             F[x] = -dfp[x] - (double)Pz * (double)dbidj[x];
         }
         if (WB)
