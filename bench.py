@@ -699,7 +699,8 @@ def potential_block(workload, args, precision="fp64", autotune=True, cpu=True, m
     return out
 
 
-# the north-star cases carried in the default line next to the headline
+# the Tersoff 1M, REBO2 100k and batched Tersoff cases, carried in the default
+# line next to the headline EAM workload
 EXTRA_BLOCKS = ("tersoff_si_1m", "rebo2_ch_100k", "tersoff_batch_4096")
 
 
