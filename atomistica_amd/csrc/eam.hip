@@ -17,9 +17,10 @@
 //                  with p = (phi' - phi/r)/r of the half-scaled r*phi table;
 //                  the reference's scatter to j becomes the second term, so no
 //                  atomics are needed and every sum has a fixed order.
-// Spline records are 8-wide AoS per interval {y, c1, c2, c3, dc1, dc2, dc3, 0}
-// so one f-and-derivative evaluation reads one 64-byte (FP64) or 32-byte
-// (FP32) record.
+// Spline records are 4-wide AoS per interval {y, c1, c2, c3}: one value or
+// derivative evaluation reads one 32-byte (FP64) or 16-byte (FP32) record;
+// the derivative (c1 + 2 c2 B + 3 c3 B^2) / dx is formed from the same
+// coefficients, so a value-and-derivative gather is two 16-byte loads.
 #include <cctype>
 #include <cmath>
 #include <cstdio>
@@ -56,8 +57,10 @@ static int z_of_symbol(const std::string &s)
 struct HostSpline {
     int n = 0;
     double dx = 0.0;
-    std::vector<double> rec;   // 8 doubles per interval
+    std::vector<double> rec;   // SPL_W doubles per interval
 };
+
+constexpr int SPL_W = 4;       // coefficients per interval record
 
 static void spline_build(HostSpline &s, const std::vector<double> &y, double dx, double scale)
 {
@@ -76,21 +79,17 @@ static void spline_build(HostSpline &s, const std::vector<double> &y, double dx,
     const double dx2 = dx * dx;
     s.n = n;
     s.dx = dx;
-    s.rec.assign(8 * (size_t)(n - 1), 0.0);
+    s.rec.assign(SPL_W * (size_t)(n - 1), 0.0);
     for (int k = 0; k < n - 1; ++k) {
         double c1 = y[k + 1] - y[k] - (2 * d2y[k] + d2y[k + 1]) * dx2 / 6;
         double c2 = d2y[k] * dx2 / 2;
         double c3 = (d2y[k + 1] - d2y[k]) * dx2 / 6;
-        double *r = &s.rec[8 * (size_t)k];
+        double *r = &s.rec[SPL_W * (size_t)k];
         // scale_y_axis multiplies every coefficient (simple_spline.f90:756-773)
         r[0] = scale * y[k];
         r[1] = scale * c1;
         r[2] = scale * c2;
         r[3] = scale * c3;
-        r[4] = scale * (c1 / dx);
-        r[5] = scale * (2 * c2 / dx);
-        r[6] = scale * (3 * c3 / dx);
-        r[7] = 0.0;
     }
 }
 
@@ -298,7 +297,7 @@ static int upload_tables(EamState &st)
     auto add = [&](const HostSpline &s) {
         SplHdr h;
         h.n = s.n;
-        h.off = (int)(rec64.size() / 8);
+        h.off = (int)(rec64.size() / SPL_W);
         h.dx = s.dx;
         h.rdx = 1.0 / s.dx;
         h.cut = s.dx * (s.n - 1);
@@ -357,7 +356,7 @@ __device__ __forceinline__ R spl_f(const SplHdr &h, const R *__restrict__ rec, d
         return (R)1;
     }
     R B = (R)(xf - i);
-    const R *c = rec + 8 * (size_t)(h.off + i - 1);
+    const R *c = rec + SPL_W * (size_t)(h.off + i - 1);
     return c[0] + B * (c[1] + B * (c[2] + B * c[3]));
 }
 
@@ -371,8 +370,8 @@ __device__ __forceinline__ R spl_df(const SplHdr &h, const R *__restrict__ rec, 
         return (R)1;
     }
     R B = (R)(xf - i);
-    const R *c = rec + 8 * (size_t)(h.off + i - 1);
-    return c[4] + B * (c[5] + B * c[6]);
+    const R *c = rec + SPL_W * (size_t)(h.off + i - 1);
+    return (c[1] + B * ((R)2 * c[2] + B * ((R)3 * c[3]))) * (R)h.rdx;
 }
 
 // f_and_df(extrapolate=.true.)
@@ -384,9 +383,9 @@ __device__ __forceinline__ void spl_fdf_x(const SplHdr &h, const R *__restrict__
     // clamp to the first / last interval; NaN goes to the first
     int i = !(fl >= 1.0) ? 1 : (!(fl < (double)h.n) ? h.n - 1 : (int)fl);
     R B = (R)(xf - i);
-    const R *c = rec + 8 * (size_t)(h.off + i - 1);
+    const R *c = rec + SPL_W * (size_t)(h.off + i - 1);
     f = c[0] + B * (c[1] + B * (c[2] + B * c[3]));
-    df = c[4] + B * (c[5] + B * c[6]);
+    df = (c[1] + B * ((R)2 * c[2] + B * ((R)3 * c[3]))) * (R)h.rdx;
 }
 
 // Lanes per atom: rows of ~50 neighbours are split over G lanes that each
