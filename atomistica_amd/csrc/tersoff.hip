@@ -313,6 +313,18 @@ __device__ __forceinline__ void h_fn(const TersoffDev &P, int ijpot, int ikpot, 
     }
 }
 
+// Tersoff's g and dg/dcos from its centre parameters c^2, c^2/d^2, d^2, h
+// and the pair's omega (one reciprocal of d^2 + (h - cos)^2 for both)
+template <typename R>
+__device__ __forceinline__ void g_tersoff(R om, R c2, R cd2, R d2, R h, R costh, R &g, R &dg)
+{
+    const R h_c = h - costh;
+    const R hh = d2 + h_c * h_c;
+    const R rh = (R)1 / hh;
+    g = om * ((R)1 + cd2 * h_c * h_c * rh);
+    dg = (R)-2 * om * c2 * h_c * rh * rh;
+}
+
 // g(cos theta) and dg/dcos.  Tersoff (tersoff_func.f90:86-114) in the
 // cancellation-free form omega (1 + c^2 t^2 / (d^2 (d^2 + t^2))), t = h - cos;
 // Kumagai (kumagai_func.f90:86-131): c1 + t go (1 + ga1), go = c2 t / (c3 +
@@ -341,14 +353,7 @@ __device__ __forceinline__ void g_fn(const TersoffDev &P, int eli, int ikpot, R 
         g = (R)P.kc1[eli] + t * v;
         return;
     }
-    const R c2 = (R)P.c2[eli], d2 = (R)P.d2[eli];
-    const R h_c = (R)P.h[eli] - costh;
-    const R hh = d2 + h_c * h_c;
-    const R om = (R)P.omega[ikpot];
-    // one reciprocal of hh for both
-    const R rh = (R)1 / hh;
-    g = om * ((R)1 + (R)P.cd2[eli] * h_c * h_c * rh);
-    dg = (R)-2 * om * c2 * h_c * rh * rh;
+    g_tersoff<R>((R)P.omega[ikpot], (R)P.c2[eli], (R)P.cd2[eli], (R)P.d2[eli], (R)P.h[eli], costh, g, dg);
 }
 
 // bond order b_ij and Pz = 1/2 fc V_A db/dzeta (V_A already carries the mask
@@ -831,6 +836,9 @@ k_bop_rows(TersoffDev P, int nat, const int *__restrict__ order, const int *__re
 
     // ---- pass 1: zeta_ij (g, dg, h, dh kept for pass 2) ----
     const int elc = eli < 0 ? 0 : eli;
+    // the centre's parameters of g, read once rather than per triplet
+    const R gc2 = FAM == 0 ? (R)P.c2[elc] : (R)0, gcd2 = FAM == 0 ? (R)P.cd2[elc] : (R)0;
+    const R gd2 = FAM == 0 ? (R)P.d2[elc] : (R)1, gh = FAM == 0 ? (R)P.h[elc] : (R)0;
     R zij = (R)0;
     R cg[NC], cdg[NC], chv[NC], cdh[NC];
     int nbw = n;
@@ -843,7 +851,10 @@ k_bop_rows(TersoffDev P, int nat, const int *__restrict__ order, const int *__re
         R g = (R)0, dg = (R)0, hv = (R)0, dh = (R)0;
         if (on && k != lane && tpk >= 0) {
             const R costh = s_rn[0][tk] * rn[0] + s_rn[1][tk] * rn[1] + s_rn[2][tk] * rn[2];
-            g_fn<R, FAM>(P, elc, tpk, costh, g, dg);
+            if (FAM == 0)
+                g_tersoff<R>((R)P.omega[tpk], gc2, gcd2, gd2, gh, costh, g, dg);
+            else
+                g_fn<R, FAM>(P, elc, tpk, costh, g, dg);
             if (HT) {
                 h_fn<R, FAM>(P, typ, tpk, rl - s_rl[tk], hv, dh);
             } else {
