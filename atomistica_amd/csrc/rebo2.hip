@@ -1120,15 +1120,14 @@ k_rb_pair(const int *__restrict__ list, const int *__restrict__ count, const RbD
             tor[t] = tv;
             tflag[t] = tv.x != R(0.0) ? 1 : 0;
         }
-        R fij = R(0.0), dfdni = R(0.0), dfdnj = R(0.0), dfdncn = R(0.0);
-        if (ijpot == RB_CC)
-            rb_table3(tab + RB_TAB_FCC, nti, ntj, nconj, fij, dfdni, dfdnj, dfdncn);
-        else if (ijpot == RB_HH)
-            rb_table3(tab + RB_TAB_FHH, nti, ntj, nconj, fij, dfdni, dfdnj, dfdncn);
-        else if (ki == RB_C)
-            rb_table3(tab + RB_TAB_FCH, ntj, nti, nconj, fij, dfdnj, dfdni, dfdncn);
-        else
-            rb_table3(tab + RB_TAB_FCH, nti, ntj, nconj, fij, dfdni, dfdnj, dfdncn);
+        // F of the pair's table, one evaluation per lane whatever the pair
+        // types in the wavefront: F_CH is read with (N_j, N_i) when i is
+        // the carbon, and its two N derivatives swap back
+        const R *ftab = tab + (ijpot == RB_CC ? RB_TAB_FCC : (ijpot == RB_HH ? RB_TAB_FHH : RB_TAB_FCH));
+        const bool fsw = ijpot != RB_CC && ijpot != RB_HH && ki == RB_C;
+        R fij = R(0.0), df1 = R(0.0), df2 = R(0.0), dfdncn = R(0.0);
+        rb_table3(ftab, fsw ? ntj : nti, fsw ? nti : ntj, nconj, fij, df1, df2, dfdncn);
+        const R dfdni = fsw ? df2 : df1, dfdnj = fsw ? df1 : df2;
         const R h = R(0.5) * fcar * fa;
         dfdncn = h * dfdncn;
         coef[s] = make_double4(h * dfdni, 2 * dfdncn * nconji, nconji, 1.0);
