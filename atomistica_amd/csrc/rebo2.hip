@@ -511,28 +511,28 @@ template <typename R>
 __device__ __forceinline__ void rb_g(const RbDevT<R> &D, int ktyp, R costh, R n, R &val,
                                      R &dcos, R &dn)
 {
-    if (ktyp == RB_C) {
-        const int j = costh < D.th1 ? 0 : (costh < D.th2 ? 1 : 2);
-        if (n < R(3.2)) {
-            rb_poly5(D.g2c[j], costh, val, dcos);
-            dn = R(0.0);
-        } else if (n > D.n_hi) {
-            rb_poly5(D.g1c[j], costh, val, dcos);
-            dn = R(0.0);
-        } else {
-            R v1, d1, v2, d2, s, c;
-            rb_poly5(D.g1c[j], costh, v1, d1);
-            rb_poly5(D.g2c[j], costh, v2, d2);
-            rb_sincos(2 * R(M_PI) * (n - R(3.2)), s, c);
-            const R sw = (1 + c) / 2;
-            val = v1 * (1 - sw) + v2 * sw;
-            dcos = d1 * (1 - sw) + d2 * sw;
-            dn = (v2 - v1) * (-R(M_PI) * s);
-        }
+    // one polynomial per lane from a per-lane coefficient set (carbon: g_2
+    // below N = 3.2, else g_1; hydrogen: its interval's set), so lanes of
+    // different branches share the loads; carbon between 3.2 and n_hi also
+    // takes g_2 and the switch between them
+    const bool kc = ktyp == RB_C;
+    const int j = costh < D.th1 ? 0 : (costh < D.th2 ? 1 : 2);
+    int q = (int)(-costh * R(12.0)) + 12;
+    q = q < 0 ? 0 : (q > 24 ? 24 : q);
+    const R *c1 = kc ? (n < R(3.2) ? D.g2c[j] : D.g1c[j]) : D.spgh[D.igh[q] - 1];
+    R v1, d1;
+    rb_poly5(c1, costh, v1, d1);
+    if (kc && !(n < R(3.2)) && !(n > D.n_hi)) {
+        R v2, d2, s, c;
+        rb_poly5(D.g2c[j], costh, v2, d2);
+        rb_sincos(2 * R(M_PI) * (n - R(3.2)), s, c);
+        const R sw = (1 + c) / 2;
+        val = v1 * (1 - sw) + v2 * sw;
+        dcos = d1 * (1 - sw) + d2 * sw;
+        dn = (v2 - v1) * (-R(M_PI) * s);
     } else {
-        int q = (int)(-costh * R(12.0)) + 12;
-        q = q < 0 ? 0 : (q > 24 ? 24 : q);
-        rb_poly5(D.spgh[D.igh[q] - 1], costh, val, dcos);
+        val = v1;
+        dcos = d1;
         dn = R(0.0);
     }
 }
