@@ -243,7 +243,7 @@ __device__ __forceinline__ double uniform(double v)
 __device__ __forceinline__ int off_pack(int a, int b, int c) { return ((a + 128) << 16) | ((b + 128) << 8) | (c + 128); }
 
 struct NlShared {
-    int hi[NL_HMAX], ho[NL_HMAX], hp[NL_HMAX], hcnt[NL_HMAX];
+    int hi[NL_HMAX], ho[NL_HMAX], hp[NL_HMAX];
     int pre[64], start[64], bu[64], boff[64];
 };
 
@@ -371,9 +371,14 @@ k_bin_pairs(int B, BinGeom g, CellArg C, double cutoff_sq, const SortedAtom *__r
             L.hi[lane] = atoms[p].j;
             L.ho[lane] = L.boff[hb];
             L.hp[lane] = p;
-            L.hcnt[lane] = 0;
         }
         __builtin_amdgcn_wave_barrier();
+        // home slots, stencil offsets and row counts held lane-wise (lane a:
+        // home atom a), read with lane moves and written by lane a, so the home loop
+        // below has no LDS access and its record loads can run ahead
+        const int hp_lane = L.hp[lane];
+        const int ho_lane = MULTI ? L.ho[lane] : 0;
+        int hcnt_lane = 0;
 
         for (int st0 = 0; st0 < nst; st0 += 64) {
             // lane L owns stencil bin st0 + L of the block (x-major order)
@@ -453,16 +458,20 @@ k_bin_pairs(int B, BinGeom g, CellArg C, double cutoff_sq, const SortedAtom *__r
                     }
                 }
                 // ---- home atoms one at a time (wave-uniform) ----
+                // each home record is loaded (scalar) one atom ahead: the
+                // index is clamped, so the load is unconditional
+                int hpn = __builtin_amdgcn_readlane(hp_lane, 0);
+                SortedAtom Hn = atoms[hpn];
                 for (int a = 0; a < ma; ++a) {
-                    // the home slot from LDS, its record by a uniform
-                    // (scalar) load
-                    const int hp = uniform(L.hp[a]);
-                    const SortedAtom H = atoms[hp];
+                    const int hp = hpn;
+                    const SortedAtom H = Hn;
+                    hpn = __builtin_amdgcn_readlane(hp_lane, min(a + 1, ma - 1));
+                    Hn = atoms[hpn];
                     int w0, w1, w2;
                     dc_unpack(H.w, w0, w1, w2);
-                    const int ho = MULTI ? uniform(L.ho[a]) : 0;
+                    const int ho = MULTI ? __builtin_amdgcn_readlane(ho_lane, a) : 0;
                     int2 *row = stage + (size_t)hp * S;
-                    int cnt = uniform(L.hcnt[a]);
+                    int cnt = __builtin_amdgcn_readlane(hcnt_lane, a);
                     // accepted lanes of one slot row, ranked after the row's
                     // entries so far
                     auto emit = [&](bool acc, int j, int pk) {
@@ -529,15 +538,14 @@ k_bin_pairs(int B, BinGeom g, CellArg C, double cutoff_sq, const SortedAtom *__r
                                  cj[r], pk);
                         }
                     }
-                    if (lane == 0) L.hcnt[a] = cnt;
-                    __builtin_amdgcn_wave_barrier();
+                    if (lane == a) hcnt_lane = cnt;
                 }
             }
         }
         __builtin_amdgcn_wave_barrier();
         if (lane < ma) {
-            const int c = L.hcnt[lane];
-            scnt[L.hp[lane]] = c;
+            const int c = hcnt_lane;
+            scnt[hp_lane] = c;
             wmax = max(wmax, c);
             if (c > 0) wlast = max(wlast, L.hi[lane] + 1);
             if (c > S) stage_over = true;
