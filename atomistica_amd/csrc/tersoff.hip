@@ -720,7 +720,7 @@ k_bop_rows(TersoffDev P, int nat, const int *__restrict__ order, const int *__re
            double *__restrict__ fown, double *__restrict__ eslot, double *__restrict__ erev,
            double *__restrict__ wslot, double *__restrict__ wrev, double *__restrict__ partial,
            double *__restrict__ epb, double *__restrict__ fpb, double *__restrict__ wpb, int *__restrict__ flags,
-           const int *__restrict__ nlflags)
+           const int *__restrict__ nlflags, int vrev)
 {
     if (nlflags && (*nlflags & DERR_NL_UNUSABLE)) {
         if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(flags, DERR_REDO);
@@ -765,10 +765,22 @@ k_bop_rows(TersoffDev P, int nat, const int *__restrict__ order, const int *__re
             const int2 *rowj = nl_row(L, j, nj);
             int t = -1;
             if (G <= 8) {
-                // every candidate load issued before the first compare
+                // every candidate load issued before the first compare; staged
+                // rows of an even stride of at least G (vrev) are read as G/2
+                // aligned 16-byte vectors, entries past the row masked
                 int2 ej[G];
+                if (vrev) {
+                    const int4 *r4 = reinterpret_cast<const int4 *>(rowj);
 #pragma unroll
-                for (int q = 0; q < G; ++q) ej[q] = q < nj ? rowj[q] : make_int2(-1, 0);
+                    for (int q = 0; q < G / 2; ++q) {
+                        const int4 v = r4[q];
+                        ej[2 * q] = make_int2(2 * q < nj ? v.x : -1, v.y);
+                        ej[2 * q + 1] = make_int2(2 * q + 1 < nj ? v.z : -1, v.w);
+                    }
+                } else {
+#pragma unroll
+                    for (int q = 0; q < G; ++q) ej[q] = q < nj ? rowj[q] : make_int2(-1, 0);
+                }
 #pragma unroll
                 for (int q = G - 1; q >= 0; --q)
                     if (ej[q].x == ai && ej[q].y == want) t = q;
@@ -2081,6 +2093,14 @@ __global__ void k_bop_prep(int nat, const int *__restrict__ order, const int *__
     if (p < nat) el_s[p] = el[order[p]];
 }
 
+// the reverse-entry search may read G entries of a row as aligned vectors:
+// staged rows (every row has room for S entries) of an even stride S >= G
+static int rows_vector_ok(const agpu_neighbors *nl, int G)
+{
+    const NlRows L = nl_rows(nl);
+    return L.S >= G && L.S % 2 == 0 ? 1 : 0;
+}
+
 template <typename R, int G, int FAM, bool WB, bool HT = true>
 static int launch_rows_k(const char *name, TersoffState *st, const RowBufs &B, agpu_particles *p,
                          agpu_neighbors *nl, const int *el, const agpu_eval_args *a, double *eslot, double *erev,
@@ -2091,7 +2111,7 @@ static int launch_rows_k(const char *name, TersoffState *st, const RowBufs &B, a
     AGPU_LAUNCH(name, (k_bop_rows<R, G, FAM, WB, HT>), dim3(nblk), dim3(256), 0, s, st->dev, p->nat,
                 nl->idx_sorted.as<int>(), B.el_s, C, nl_rows(nl), nl->offsets.as<int>(), a->mask, B.frev, B.fown,
                 eslot, erev, wslot, wrev, B.partial, a->epot_per_bond, a->f_per_bond, a->wpot_per_bond, B.stats,
-                nl->pending ? nl_flags(nl) : nullptr);
+                nl->pending ? nl_flags(nl) : nullptr, rows_vector_ok(nl, G));
     return AGPU_OK;
 }
 
