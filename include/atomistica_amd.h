@@ -359,7 +359,7 @@ int agpu_potential_energy_and_forces(agpu_potential *pot, agpu_particles *p,
  * current atomic numbers, and the list current for the positions
  * (agpu_neighbors_update called beforehand), else AGPU_ERR_STATE.  The
  * per-call scratch is a fixed 64 MiB device array per GPU (24 bytes per list
- * entry plus about 25 per atom; larger systems return AGPU_ERR_ARG), held
+ * entry plus about 30 per atom; larger systems return AGPU_ERR_ARG), held
  * until the process ends and shared by every call on that GPU: calls that
  * overlap on one GPU race.  Rows of more than 64 neighbours return
  * AGPU_ERR_UNSUPPORTED.  Results equal those of
