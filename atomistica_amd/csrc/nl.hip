@@ -623,15 +623,14 @@ k_atom_pairs(int nat, BinGeom g, CellArg C, double cutoff_sq, const SortedAtom *
             const float hx = H.x + (float)au[0], hy = H.y + (float)au[1], hz = H.z + (float)au[2];
             const float e0 = 5.9604645e-8f * (4.0f * (float)(fabs(au[0]) + fabs(au[1]) + fabs(au[2])) + 8.0f * rc);
             const int s0u = w0 + ux, s1u = w1 + uy, s2u = w2 + uz;
-            for (int q = q0; q < q1; ++q) {
-                const float4 B = pos32[q];
+            auto test = [&](const float4 B, int q) {
                 const float dx = hx - B.x, dy = hy - B.y, dz = hz - B.z;
                 const float d2f = dx * dx + dy * dy + dz * dz;
                 const float bm = fabsf(B.w);
                 const float E2 = 2.0f * (e0 + 4.76837158e-7f * fmaxf(hm, bm));
                 const float th = rc + E2, tl = rc - E2;
                 const bool exact = !(E2 < 2e-3f * rc);
-                if (!exact && !(d2f < th * th * 1.000001f)) continue;
+                if (!exact && !(d2f < th * th * 1.000001f)) return;
                 const int jb = __float_as_int(B.w);
                 int s0 = s0u, s1 = s1u, s2 = s2u;
                 if (jb < 0) {
@@ -641,15 +640,24 @@ k_atom_pairs(int nat, BinGeom g, CellArg C, double cutoff_sq, const SortedAtom *
                     s1 -= v1;
                     s2 -= v2;
                 }
-                if (q == p && s0 == 0 && s1 == 0 && s2 == 0) continue;
+                if (q == p && s0 == 0 && s1 == 0 && s2 == 0) return;
                 if (exact || !(d2f < tl * tl * 0.999999f)) {
                     const SortedAtom Ap = atoms[p], Bq = atoms[q];
                     if (!(pair_d2<ORTHO>(C, Ap.x, Ap.y, Ap.z, Bq.x, Bq.y, Bq.z, s0, s1, s2) < cutoff_sq))
-                        continue;
+                        return;
                 }
                 if (!dc_pack_ok(s0) || !dc_pack_ok(s1) || !dc_pack_ok(s2)) bad = true;
                 if (cnt < S) stage[srow + cnt] = make_int2(q, dc_pack(s0, s1, s2));
                 ++cnt;
+            };
+            // two candidates per trip, both records loaded before either test
+            // (the second index clamped into the run, so both loads are in
+            // bounds and unconditional)
+            for (int q = q0; q < q1; q += 2) {
+                const float4 B0 = pos32[q];
+                const float4 B1 = pos32[min(q + 1, q1 - 1)];
+                test(B0, q);
+                if (q + 1 < q1) test(B1, q + 1);
             }
         };
         const int nz = g.nc[2], ndz = g.nd[2];
